@@ -1,0 +1,155 @@
+"""Benchmark: converged glint walks/s of unbiased glint SMS on one MI355X.
+
+Workload (one step = the SMS estimate of one frame, queries resident in HBM):
+  1x1 mirror plane with a seed-generated 2048^2 RGB8 normal map (4096 line scratches of
+  width 1-3 texels + Gaussian bumps, slope std 0.3) under a point light, 512x512 pixels at
+  4 spp (1,048,576 footprint queries), pixel-footprint UV seeding, unbiased Bernoulli
+  estimator with at most 1e5 trials, Newton max 20 iterations, threshold 4e-5, uniqueness
+  2e-4 (the glint settings of results/Figure_19_GlintsComparison/shoes_sms.xml:9-11 and
+  render_shoes.py:84).
+
+Prints ONE JSON line.  `value` = converged walks/s (walks that returned success) over the
+whole timed region; `roofline` refers to the dominant kernel (k_glint_estimate) timed with
+HIP events on the stream it runs on; `cpu_baseline` times the CPU oracle on a bounded sample
+of the same queries on this host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "specular-manifold-sampling_amd"))
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+FP32_PEAK_TFLOPS = 157.3       # MI355X FP32 vector
+BYTES_PER_ITER = 32            # 4 texels x (2 x fp32 slope) per Newton iteration
+FLOPS_PER_ITER = 120           # FP32 ops per glint Newton iteration (fetch, solve, footprint test)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--width", type=int, default=512)
+    ap.add_argument("--height", type=int, default=512)
+    ap.add_argument("--spp", type=int, default=4)
+    ap.add_argument("--nm-res", type=int, default=2048)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample length")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import numpy as np
+    from smsamd import abi, gpu, scenes
+
+    t0 = time.time()
+    sc = scenes.glint_plane(args.width, args.height, nm_res=args.nm_res, n_scratches=4096, seed=1)
+    cfg = abi.SmsConfig.default(max_trials=100000, solver_threshold=4e-5, uniqueness_threshold=2e-4)
+    ctx = gpu.Context(sc, cfg)
+    dev, ncu = ctx.device_info()
+    qbuf, n = ctx.glint_queries_device(args.spp, seed=1)
+    out = gpu.DeviceBuffer(ctx, n * 12)
+    print(f"# setup {time.time() - t0:.1f}s device={dev} cus={ncu} queries={n}", file=sys.stderr, flush=True)
+
+    for w in range(args.warmup):
+        ctx.glint_estimate_device(qbuf, n, 1000 + w, out)
+    ctx.sync()
+    tot = {}
+    k1_ms = k2_ms = 0.0
+    iters_pass1 = 0
+    t_start = time.perf_counter()
+    for s in range(args.steps):
+        st = ctx.glint_estimate_device(qbuf, n, 1 + s, out)
+        a, b = ctx.last_kernel_ms()
+        k1_ms += a
+        k2_ms += b
+        iters_pass1 += st["newton_iterations"] - st["tail_iterations"]
+        for k, v in st.items():
+            tot[k] = tot.get(k, 0) + v
+    ctx.sync()
+    elapsed = time.perf_counter() - t_start
+    ms_per_step = 1e3 * elapsed / args.steps
+    conv_per_s = tot["walks_converged"] / elapsed
+    walks_per_s = tot["walks"] / elapsed
+    print(f"# steps={args.steps} {ms_per_step:.2f} ms/step walks/s={walks_per_s:.4g} conv/s={conv_per_s:.4g} "
+          f"pass1={k1_ms / args.steps:.2f}ms tail={k2_ms / args.steps:.2f}ms stats={tot}", file=sys.stderr, flush=True)
+
+    k1_s = k1_ms / 1e3 / args.steps
+    achieved_gbs = BYTES_PER_ITER * (iters_pass1 / args.steps) / k1_s / 1e9
+    achieved_tflops = FLOPS_PER_ITER * (iters_pass1 / args.steps) / k1_s / 1e12
+    roofline = {"bound": "hbm", "achieved": round(achieved_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved_gbs / HBM_PEAK_GBS, 5), "traffic": None,
+                "kernel": "k_glint_estimate", "kernel_ms": round(k1_s * 1e3, 3),
+                "valu": {"achieved": round(achieved_tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 5),
+                         "flops_per_iter": FLOPS_PER_ITER}}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(sc, cfg, ctx, n, args)
+    result = {
+        "metric": "converged walks/s at 1 GPU",
+        "value": conv_per_s,
+        "unit": "walks/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"glint SMS, {args.width}x{args.height}x{args.spp}spp footprint queries, "
+                               f"{args.nm_res}^2 RGB8 scratch+bump normal map, unbiased (max_trials 1e5)",
+                   "queries_per_step": n, "walks_per_step": tot["walks"] / args.steps,
+                   "walks_per_s": walks_per_s, "device": dev},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if cpu and cpu.get("value"):
+        result["gpu_over_cpu"] = conv_per_s / cpu["value"]
+    print(json.dumps(result), flush=True)
+    out.free()
+    qbuf.free()
+    ctx.close()
+
+
+def cpu_baseline(sc, cfg, ctx, n, args):
+    """CPU oracle on a strided sample of the same queries (same seeds per query)."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    from smsamd import abi
+    q = ctx.glint_queries(args.spp, seed=1)
+    threads = max(1, min(16, os.cpu_count() or 1))
+    orc = oracle_lib.Oracle(sc, cfg, threads=threads)
+    # calibrate on a small sample, then size the timed sample to ~cpu_seconds
+    stride = 4096
+    t = time.perf_counter()
+    _, st = orc.glint_estimate(np.ascontiguousarray(q[::stride]), 1)
+    dt = max(time.perf_counter() - t, 1e-3)
+    per_query = dt / len(q[::stride])
+    m = int(min(len(q), max(1000, args.cpu_seconds / per_query)))
+    stride = max(1, len(q) // m)
+    sample = np.ascontiguousarray(q[::stride])
+    t = time.perf_counter()
+    _, st = orc.glint_estimate(sample, 1)
+    dt = time.perf_counter() - t
+    orc.close()
+    return {"value": st["walks_converged"] / dt, "unit": "walks/s", "cores": threads, "kind": "port",
+            "sample": f"every {stride}th query of the frame ({len(sample)} queries, {st['walks']} walks, "
+                      f"{dt:.1f}s on {threads} threads)", "walks_per_s": st["walks"] / dt}
+
+
+if __name__ == "__main__":
+    main()

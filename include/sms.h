@@ -1,0 +1,318 @@
+/*
+ * sms.h -- C ABI of the MI355X (gfx950) Specular Manifold Sampling solver library.
+ *
+ * Plain C types only: pointers, sizes and POD structs.  No C++ exception crosses this
+ * boundary; every entry point returns an sms_status.  Per-walk failure is data (a flag in
+ * the output record), never an error code.
+ *
+ * The library owns a context (sms_ctx) holding device copies of the scene (analytic shapes,
+ * triangle meshes with their BVH, normal maps), the solver configuration and its streams.
+ * One context is used by one host thread at a time.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   SMSConfig                                   include/mitsuba/render/manifold.h:11-28
+ *   SpecularManifoldSingleScatter::sample_path  src/librender/manifold_ss.cpp:203-279
+ *   SpecularManifoldSingleScatter::specular_manifold_sampling
+ *                                               src/librender/manifold_ss.cpp:28-201
+ *   SpecularManifoldGlints::sample_glint        src/librender/manifold_glints.cpp:225-265
+ *   SpecularManifoldGlints::specular_manifold_sampling
+ *                                               src/librender/manifold_glints.cpp:28-223
+ *   SpecularManifoldMultiScatter::sample_path   src/librender/manifold_ms.cpp:203-227
+ *   SpecularManifoldMultiScatter::specular_manifold_sampling
+ *                                               src/librender/manifold_ms.cpp:27-201
+ *   Scene::ray_intersect (whole-scene closest hit used by every reprojection)
+ *                                               src/librender/scene_native.inl:17-34
+ */
+#ifndef SMS_H
+#define SMS_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status codes */
+typedef int32_t sms_status;
+#define SMS_OK              0
+#define SMS_E_INVALID      -1   /* bad argument / unsupported scene description      */
+#define SMS_E_HIP          -2   /* a HIP runtime call failed                         */
+#define SMS_E_OOM          -3   /* device allocation failed                          */
+#define SMS_E_QUEUE_FULL   -4   /* a device work queue overflowed its capacity       */
+#define SMS_E_UNSUPPORTED  -5   /* requested mode not built                          */
+#define SMS_E_NO_DEVICE    -6   /* no HIP device visible                             */
+
+/* ---------------------------------------------------------------- configuration */
+/* Field-for-field mirror of SMSConfig (include/mitsuba/render/manifold.h:11-28). */
+typedef struct sms_config {
+    int32_t biased;                  /* biased (trial set of size max_trials) vs unbiased SMS  */
+    int32_t twostage;                /* two-stage solver for normal maps (not built: must be 0) */
+    int32_t halfvector_constraints;  /* half-vector constraint instead of angle difference     */
+    int32_t mnee_init;               /* deterministic MNEE seed instead of a random one        */
+    float   step_scale;              /* Newton step scale                                      */
+    int32_t max_iterations;          /* Newton iteration cap                                   */
+    float   solver_threshold;        /* |C| stopping criterion                                 */
+    float   uniqueness_threshold;    /* duplicate-solution test                                */
+    int32_t max_trials;              /* biased: trial-set size M; unbiased: Bernoulli cap      */
+    int32_t bounces;                 /* multi-scatter chain length                             */
+    int32_t bsdf_strategy_only;      /* glints: disable MIS, BSDF strategy only                */
+    int32_t sms_strategy_only;       /* glints: disable MIS, SMS strategy only                 */
+} sms_config;
+
+/* Defaults of SMSConfig() plus the integrator property defaults
+   (src/integrators/path_sms_ss.cpp:49-57). */
+void sms_config_default(sms_config *cfg);
+
+/* ---------------------------------------------------------------- scene description */
+enum sms_shape_type { SMS_SHAPE_SPHERE = 0, SMS_SHAPE_RECTANGLE = 1, SMS_SHAPE_MESH = 2 };
+
+enum sms_bsdf_type {
+    SMS_BSDF_DIFFUSE    = 0,   /* src/bsdfs/diffuse.cpp                                   */
+    SMS_BSDF_DIELECTRIC = 1,   /* smooth dielectric, ior() = (eta, 0)  dielectric.cpp:332  */
+    SMS_BSDF_CONDUCTOR  = 2,   /* smooth conductor, ior() = (eta, k)   conductor.cpp:292   */
+    SMS_BSDF_NORMALMAP  = 3    /* normal-mapped smooth conductor  src/bsdfs/normalmap.cpp  */
+};
+
+/* Shape flags (include/mitsuba/render/shape.h:330-343). */
+#define SMS_FLAG_CAUSTIC_CASTER_SS   0x01
+#define SMS_FLAG_CAUSTIC_CASTER_MS   0x02
+#define SMS_FLAG_CAUSTIC_RECEIVER    0x04
+#define SMS_FLAG_CAUSTIC_BOUNCER     0x08
+#define SMS_FLAG_GLINTY              0x10
+
+typedef struct sms_shape_desc {
+    int32_t type;              /* sms_shape_type                                          */
+    int32_t bsdf;              /* sms_bsdf_type                                           */
+    int32_t flags;             /* SMS_FLAG_*                                              */
+    int32_t emitter;           /* index into sms_scene_desc.emitters, or -1              */
+    /* sphere */
+    float center[3];
+    float radius;
+    /* rectangle: row-major 3x4 affine to_world (local [-1,1]^2 x {0}) */
+    float to_world[12];
+    /* bsdf parameters */
+    float eta;                 /* dielectric: int_ior/ext_ior; conductor: real part       */
+    float k;                   /* conductor: imaginary part                                */
+    float reflectance[3];      /* diffuse albedo                                          */
+    int32_t normalmap;         /* index into sms_scene_desc.normalmaps (NORMALMAP bsdf)   */
+    float tiles;               /* normal-map tiling factor (bsdfs/normalmap.cpp:43)       */
+    /* triangle mesh (SMS_SHAPE_MESH); host pointers, copied at sms_ctx_create */
+    int32_t n_vertices;
+    int32_t n_faces;
+    const float    *positions;  /* n_vertices x 3                                         */
+    const float    *normals;    /* n_vertices x 3, or NULL                                */
+    const float    *texcoords;  /* n_vertices x 2, or NULL                                */
+    const uint32_t *faces;      /* n_faces x 3                                            */
+} sms_shape_desc;
+
+enum sms_emitter_type { SMS_EMITTER_AREA = 0, SMS_EMITTER_POINT = 1 };
+
+typedef struct sms_emitter_desc {
+    int32_t type;              /* sms_emitter_type                                         */
+    int32_t shape;             /* AREA: index of the emitting shape (a sphere)             */
+    float   radiance[3];       /* AREA: radiance (src/emitters/area.cpp:71-78)             */
+    float   position[3];       /* POINT: position                                          */
+    float   intensity[3];      /* POINT: intensity (src/emitters/point.cpp:74-105)         */
+    int32_t caustic_ss;        /* emitter is an SMS single-scatter target                  */
+    int32_t caustic_ms;        /* emitter is an SMS multi-scatter target                   */
+} sms_emitter_desc;
+
+typedef struct sms_normalmap_desc {
+    int32_t res;               /* square power-of-two resolution                           */
+    const uint8_t *rgb;        /* res*res*3 bytes, row-major, byte/255 (no linearisation)  */
+} sms_normalmap_desc;
+
+typedef struct sms_camera_desc {
+    float origin[3];
+    float target[3];
+    float up[3];
+    float fov_deg;             /* horizontal field of view (perspective.cpp)               */
+    int32_t width, height;
+} sms_camera_desc;
+
+typedef struct sms_scene_desc {
+    int32_t n_shapes;
+    const sms_shape_desc *shapes;
+    int32_t n_emitters;
+    const sms_emitter_desc *emitters;
+    int32_t n_normalmaps;
+    const sms_normalmap_desc *normalmaps;
+    sms_camera_desc camera;
+} sms_scene_desc;
+
+/* ---------------------------------------------------------------- emitter interaction */
+/* EmitterInteraction (include/mitsuba/render/manifold.h:145-190), flattened. */
+enum sms_ei_kind { SMS_EI_AREA = 0, SMS_EI_POINT = 1, SMS_EI_DIRECTIONAL = 2 };
+
+typedef struct sms_emitter_interaction {
+    float p[3];
+    float n[3];
+    float d[3];
+    float weight[3];
+    float pdf;
+    int32_t kind;              /* sms_ei_kind                                              */
+    int32_t emitter;           /* emitter index                                            */
+} sms_emitter_interaction;
+
+/* ---------------------------------------------------------------- walk records (parity API) */
+/* One single-scatter walk = SpecularManifoldSingleScatter::sample_path
+   (src/librender/manifold_ss.cpp:203-279): seed on `shape`, seed-visibility ray cast,
+   Newton solve with reprojection.  A zero p_start means "no override" (:224-226). */
+typedef struct sms_ss_walk_in {
+    float si_p[3];             /* shading point x                                          */
+    int32_t shape;             /* target specular shape                                    */
+    sms_emitter_interaction ei;
+    float n_offset[3];         /* (0,0,1) for smooth casters                               */
+    float sample[2];           /* 2D sample for Shape::sample_position                     */
+    float p_start[3];          /* seed override                                            */
+} sms_ss_walk_in;
+
+typedef struct sms_ss_walk_out {
+    int32_t success;           /* Mask returned by sample_path                             */
+    int32_t iterations;        /* Newton iterations consumed (0 if the seed ray failed)    */
+    int32_t final_shape;       /* shape of si_final, -1 if invalid                         */
+    int32_t init_valid;        /* seed ray hit the target shape                            */
+    float p[3];                /* si_final.p                                               */
+    float n[3];                /* si_final.sh_frame.n                                      */
+    float p_init[3];           /* si_init.p                                                */
+} sms_ss_walk_out;
+
+/* One glint walk = SpecularManifoldGlints::sample_glint (manifold_glints.cpp:225-265).
+   xi_start == (1,1) means "no override" (:253-255). */
+typedef struct sms_glint_walk_in {
+    float sensor_p[3];
+    float ei_p[3];
+    float si_p[3];
+    float frame_s[3], frame_t[3], frame_n[3];   /* si.sh_frame                          */
+    float uv[2];
+    float duv_dx[2], duv_dy[2];
+    int32_t shape;             /* normal-mapped shape                                      */
+    float n_offset[3];
+    float sample[2];
+    float xi_start[2];
+} sms_glint_walk_in;
+
+typedef struct sms_glint_walk_out {
+    int32_t success;
+    int32_t iterations;
+    float uv[2];
+    float uv_init[2];
+} sms_glint_walk_out;
+
+/* ---------------------------------------------------------------- estimator queries */
+/* A shading point handed to the single-scatter estimator (the SurfaceInteraction
+   passed to specular_manifold_sampling, manifold_ss.cpp:28-30). */
+typedef struct sms_ss_query {
+    float p[3];
+    float n[3];                /* geometric normal                                         */
+    float sh_n[3], sh_s[3], sh_t[3];
+    float wi[3];               /* local incident direction (si.wi)                         */
+    int32_t shape;             /* receiver shape                                           */
+    int32_t pad;
+} sms_ss_query;
+
+/* A glinty first hit handed to the glint estimator (manifold_glints.cpp:28-31). */
+typedef struct sms_glint_query {
+    float sensor_p[3];
+    float p[3];
+    float n[3];
+    float sh_n[3], sh_s[3], sh_t[3];
+    float dp_du[3], dp_dv[3];
+    float wi[3];
+    float uv[2];
+    float duv_dx[2], duv_dy[2];
+    int32_t shape;
+    int32_t pixel;             /* film pixel index (for We * film area)                    */
+} sms_glint_query;
+
+/* Counters returned per call (the reference keeps static atomics,
+   manifold_ss.cpp:14-18; here they are per call). */
+typedef struct sms_stats {
+    uint64_t walks;             /* sample_path / sample_glint calls                        */
+    uint64_t walks_converged;   /* calls that returned success                             */
+    uint64_t newton_iterations; /* Newton iterations over all walks                        */
+    uint64_t bernoulli_calls;   /* unbiased: estimates that started a trial loop           */
+    uint64_t bernoulli_trials;  /* unbiased: trials over all loops (incl. the match)        */
+    uint64_t bernoulli_max;     /* unbiased: longest trial loop                            */
+    uint64_t overflow_queries;  /* queries finished by the cooperative tail pass           */
+    uint64_t tail_iterations;   /* Newton iterations run by the tail pass (subset of the   */
+                                /* newton_iterations total)                                */
+} sms_stats;
+
+/* ---------------------------------------------------------------- context */
+typedef struct sms_ctx sms_ctx;
+
+/* Copy the scene to the current HIP device, build the BVH, decode normal maps. */
+sms_status sms_ctx_create(const sms_scene_desc *scene, const sms_config *cfg, sms_ctx **out);
+sms_status sms_ctx_destroy(sms_ctx *ctx);
+sms_status sms_ctx_set_config(sms_ctx *ctx, const sms_config *cfg);
+/* Scheduling knobs of the persistent estimator kernels (chosen by the autotuner).
+   lane_trials: Bernoulli trials one lane runs for a query before the query is handed to the
+   one-wavefront-per-query tail pass; blocks_per_cu: persistent blocks per compute unit
+   (0 = the occupancy limit).  No counterpart in the reference (its trial loops are serial). */
+typedef struct sms_schedule {
+    int32_t lane_trials;
+    int32_t blocks_per_cu;
+    int32_t pad[2];
+} sms_schedule;
+sms_status sms_ctx_set_schedule(sms_ctx *ctx, const sms_schedule *s);
+sms_status sms_ctx_get_schedule(const sms_ctx *ctx, sms_schedule *s);
+
+/* Last error message for this context (static storage owned by the context). */
+const char *sms_ctx_last_error(const sms_ctx *ctx);
+/* Device name and compute-unit count of the device the context lives on. */
+sms_status sms_ctx_device_info(const sms_ctx *ctx, char *name, int32_t name_len, int32_t *n_cu);
+
+/* ---------------------------------------------------------------- device-pointer entry points
+   All pointers below are DEVICE pointers (already resident in HBM); the calls are
+   asynchronous on the context stream unless noted.  sms_ctx_sync waits for completion. */
+sms_status sms_ctx_sync(sms_ctx *ctx);
+
+/* Parity entry points: one walk per record. */
+sms_status sms_ss_walk_batch(sms_ctx *ctx, const sms_ss_walk_in *in, int64_t n,
+                             sms_ss_walk_out *out);
+sms_status sms_glint_walk_batch(sms_ctx *ctx, const sms_glint_walk_in *in, int64_t n,
+                                sms_glint_walk_out *out);
+
+/* Throughput entry points: the whole estimator per query, run by a persistent kernel that
+   refills idle lanes from a device work queue.  Random numbers are drawn from per-(query,
+   walk) PCG32 streams derived from `seed`, so results do not depend on scheduling.
+   out_rgb: n x 3 floats.  stats: host pointer, filled after completion (may be NULL). */
+sms_status sms_ss_estimate_batch(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed,
+                                 float *out_rgb, sms_stats *stats);
+sms_status sms_glint_estimate_batch(sms_ctx *ctx, const sms_glint_query *q, int64_t n,
+                                    uint64_t seed, float *out_rgb, sms_stats *stats);
+
+/* Camera-ray generation for glint frames: for every pixel sample of a width x height x spp
+   frame, trace the primary ray differential, intersect the glinty shape and write the
+   glint query (shape = -1 when the first hit is not glinty).  q: device, n = w*h*spp. */
+sms_status sms_glint_make_queries(sms_ctx *ctx, int32_t spp, uint64_t seed, sms_glint_query *q);
+
+/* Kernel timing of the most recent estimator call, measured with HIP events on the context
+   stream the kernels ran on: ms[0] = first (persistent lane-per-query) pass, ms[1] = the
+   tail / finalize passes.  ms must hold 2 floats. */
+sms_status sms_ctx_last_kernel_ms(const sms_ctx *ctx, float *ms);
+
+/* ---------------------------------------------------------------- host-buffer convenience
+   Same as above but with host buffers; copies in/out over PCIe and synchronises. */
+sms_status sms_ss_walk_batch_host(sms_ctx *ctx, const sms_ss_walk_in *in, int64_t n,
+                                  sms_ss_walk_out *out);
+sms_status sms_glint_walk_batch_host(sms_ctx *ctx, const sms_glint_walk_in *in, int64_t n,
+                                     sms_glint_walk_out *out);
+sms_status sms_ss_estimate_batch_host(sms_ctx *ctx, const sms_ss_query *q, int64_t n,
+                                      uint64_t seed, float *out_rgb, sms_stats *stats);
+sms_status sms_glint_estimate_batch_host(sms_ctx *ctx, const sms_glint_query *q, int64_t n,
+                                         uint64_t seed, float *out_rgb, sms_stats *stats);
+
+/* ---------------------------------------------------------------- device memory helpers */
+sms_status sms_device_alloc(sms_ctx *ctx, size_t bytes, void **ptr);
+sms_status sms_device_free(sms_ctx *ctx, void *ptr);
+sms_status sms_copy_h2d(sms_ctx *ctx, void *dst, const void *src, size_t bytes);
+sms_status sms_copy_d2h(sms_ctx *ctx, void *dst, const void *src, size_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SMS_H */

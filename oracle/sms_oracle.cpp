@@ -1,0 +1,1742 @@
+/*
+ * sms_oracle.cpp -- TEST INFRASTRUCTURE ONLY: scalar CPU restatement of the reference's
+ * Specular Manifold Sampling solver path.  Only tests/, __graft_entry__.smoke() and the
+ * CPU-baseline leg of bench.py load it, as the checker.  The product never links it.
+ *
+ * Each routine cites the reference file:line it restates (paths relative to the reference
+ * repository).  Parity status of every part is recorded in the test-suite; parts that no
+ * closed form, brute force or invariant pins say so where they are tested.
+ *
+ * Scene model: analytic spheres and rectangles are tested one by one; triangle meshes go
+ * through a median-split BVH of this file (boxes padded so that traversal is conservative
+ * and the closest hit equals brute force; ties broken by (t, shape, prim)).
+ */
+#include "oracle_math.h"
+#include "../include/sms.h"
+
+#include <vector>
+#include <algorithm>
+#include <thread>
+#include <atomic>
+#include <stdlib.h>
+#include <stdio.h>
+
+using namespace orc;
+
+namespace {
+
+/* ---------------------------------------------------------------- scene */
+struct Shape {
+    int type, bsdf, flags, emitter;
+    V3 center; float radius, inv_radius;
+    float M[12], Minv[12];          // rectangle to_world / to_object (row-major 3x4)
+    V3 fr_s, fr_t, fr_n;            // rectangle m_frame (dp_du, dp_dv, normal)
+    float eta_r, k, eta_vtx;        // ior() = (eta_r, k) and the ManifoldVertex eta encoding
+    V3 bbox_center;
+    V3 refl;
+    int normalmap; float tiles;
+    // mesh
+    int vbase, fbase, nv, nf; bool has_n, has_uv;
+    std::vector<float> cdf, pmf; float area_sum, area_norm;
+    float inv_surface_area;
+};
+
+struct Emitter { int type, shape; V3 radiance, position, intensity; bool ss, ms; };
+
+struct Normalmap {
+    int res;
+    std::vector<float> slope;       // level-0 slopes (res*res*2), normalmap.cpp:46-66
+};
+
+struct Camera {
+    V3 o, left, up, dir;            // look-at frame (columns of camera-to-world)
+    float tan_half, aspect;
+    V3 dx, dy;                      // near-plane differentials (perspective.cpp:136-140)
+    float rect_x, rect_y, normalization;
+    int w, h;
+};
+
+struct Node { float bmin[3], bmax[3]; int left, right, first, count; };
+
+struct Scene {
+    std::vector<Shape> shapes;
+    std::vector<Emitter> emitters;
+    std::vector<Normalmap> nmaps;
+    std::vector<float> pos, nrm, uvs;    // all mesh vertices
+    std::vector<uint32_t> faces;         // all mesh faces (global vertex indices)
+    std::vector<int> tri_shape, tri_prim;
+    std::vector<int> tri_order;
+    std::vector<Node> nodes;
+    std::vector<int> casters_ss, emitters_ss, emitters_all;
+    Camera cam;
+    bool has_cam;
+    sms_config cfg;
+};
+
+struct Ray { V3 o, d, d_rcp; float mint, maxt; };
+static inline Ray make_ray(V3 o, V3 d, float mint = RAY_EPS_F, float maxt = INF_F) {
+    Ray r; r.o = o; r.d = d; r.d_rcp = v3(1.f / d.x, 1.f / d.y, 1.f / d.z); r.mint = mint; r.maxt = maxt;
+    return r;
+}
+static inline V3 ray_at(const Ray &r, float t) { return fmadd3(r.d, t, r.o); }
+
+struct SI {
+    bool valid; float t;
+    V3 p, n, sh_n, sh_s, sh_t, dp_du, dp_dv, wi;
+    V2 uv, duv_dx, duv_dy;
+    int shape, prim;
+    float b1, b2;                    // triangle barycentrics (mesh cache)
+    float lx, ly;                    // rectangle local coords (rectangle cache)
+};
+static SI si_invalid() { SI s; memset(&s, 0, sizeof(s)); s.valid = false; s.t = INF_F; s.shape = -1; s.prim = -1; return s; }
+
+static inline V3 xform_p(const float *M, V3 p) {
+    return v3(((M[0] * p.x + M[1] * p.y) + M[2] * p.z) + M[3],
+              ((M[4] * p.x + M[5] * p.y) + M[6] * p.z) + M[7],
+              ((M[8] * p.x + M[9] * p.y) + M[10] * p.z) + M[11]);
+}
+static inline V3 xform_v(const float *M, V3 v) {
+    return v3((M[0] * v.x + M[1] * v.y) + M[2] * v.z,
+              (M[4] * v.x + M[5] * v.y) + M[6] * v.z,
+              (M[8] * v.x + M[9] * v.y) + M[10] * v.z);
+}
+
+/* coordinate_system, include/mitsuba/core/vector.h:116-136 (Duff et al. 2017) */
+static inline void coordinate_system(V3 n, V3 *s, V3 *t) {
+    float sign = signbit(n.z) ? -1.f : 1.f;
+    float a = -(1.f / (sign + n.z));
+    float b = n.x * n.y * a;
+    *s = v3(mulsign(n.x * n.x * a, n.z) + 1.f, mulsign(b, n.z), -mulsign(n.x, n.z));
+    *t = v3(b, sign + n.y * n.y * a, -n.y);
+}
+
+/* hmean of an RGB spectrum whose three channels hold the same value */
+static inline float hmean_same(float e) { return ((e + e) + e) / 3.f; }
+
+/* ---------------------------------------------------------------- scene setup */
+static void invert_affine(const float *M, float *Minv) {
+    double a[3][3], inv[3][3];
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) a[i][j] = M[i * 4 + j];
+    double det = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) - a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0])
+               + a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+    double id = 1.0 / det;
+    inv[0][0] = (a[1][1] * a[2][2] - a[1][2] * a[2][1]) * id;
+    inv[0][1] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) * id;
+    inv[0][2] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) * id;
+    inv[1][0] = (a[1][2] * a[2][0] - a[1][0] * a[2][2]) * id;
+    inv[1][1] = (a[0][0] * a[2][2] - a[0][2] * a[2][0]) * id;
+    inv[1][2] = (a[0][2] * a[1][0] - a[0][0] * a[1][2]) * id;
+    inv[2][0] = (a[1][0] * a[2][1] - a[1][1] * a[2][0]) * id;
+    inv[2][1] = (a[0][1] * a[2][0] - a[0][0] * a[2][1]) * id;
+    inv[2][2] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) * id;
+    double t[3] = {M[3], M[7], M[11]};
+    for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) Minv[i * 4 + j] = (float) inv[i][j];
+        Minv[i * 4 + 3] = (float) (-(inv[i][0] * t[0] + inv[i][1] * t[1] + inv[i][2] * t[2]));
+    }
+}
+
+struct BuildItem { float c[3]; float bmin[3], bmax[3]; int tri; };
+
+static void bvh_build(Scene &S) {
+    int nt = (int) S.tri_shape.size();
+    S.nodes.clear(); S.tri_order.clear();
+    if (nt == 0) return;
+    std::vector<BuildItem> items(nt);
+    for (int i = 0; i < nt; ++i) {
+        BuildItem &it = items[i]; it.tri = i;
+        for (int a = 0; a < 3; ++a) { it.bmin[a] = INF_F; it.bmax[a] = -INF_F; }
+        for (int k = 0; k < 3; ++k) {
+            const float *p = &S.pos[3 * S.faces[3 * i + k]];
+            for (int a = 0; a < 3; ++a) { it.bmin[a] = std::min(it.bmin[a], p[a]); it.bmax[a] = std::max(it.bmax[a], p[a]); }
+        }
+        for (int a = 0; a < 3; ++a) it.c[a] = 0.5f * (it.bmin[a] + it.bmax[a]);
+    }
+    struct Task { int node, lo, hi; };
+    std::vector<Task> stack;
+    S.nodes.push_back(Node());
+    stack.push_back({0, 0, nt});
+    while (!stack.empty()) {
+        Task tk = stack.back(); stack.pop_back();
+        Node nd; nd.left = nd.right = -1; nd.first = tk.lo; nd.count = tk.hi - tk.lo;
+        float cmin[3], cmax[3];
+        for (int a = 0; a < 3; ++a) { nd.bmin[a] = INF_F; nd.bmax[a] = -INF_F; cmin[a] = INF_F; cmax[a] = -INF_F; }
+        for (int i = tk.lo; i < tk.hi; ++i)
+            for (int a = 0; a < 3; ++a) {
+                nd.bmin[a] = std::min(nd.bmin[a], items[i].bmin[a]); nd.bmax[a] = std::max(nd.bmax[a], items[i].bmax[a]);
+                cmin[a] = std::min(cmin[a], items[i].c[a]); cmax[a] = std::max(cmax[a], items[i].c[a]);
+            }
+        // pad so that the slab test can never reject a triangle the triangle test accepts
+        for (int a = 0; a < 3; ++a) {
+            float ext = nd.bmax[a] - nd.bmin[a];
+            float pad = 1e-5f * ext + 1e-5f * std::max(fabsf(nd.bmin[a]), fabsf(nd.bmax[a])) + 1e-7f;
+            nd.bmin[a] -= pad; nd.bmax[a] += pad;
+        }
+        if (nd.count <= 4) { S.nodes[tk.node] = nd; continue; }
+        int axis = 0; float best = cmax[0] - cmin[0];
+        for (int a = 1; a < 3; ++a) if (cmax[a] - cmin[a] > best) { best = cmax[a] - cmin[a]; axis = a; }
+        int mid = (tk.lo + tk.hi) / 2;
+        std::nth_element(items.begin() + tk.lo, items.begin() + mid, items.begin() + tk.hi,
+                         [axis](const BuildItem &x, const BuildItem &y) { return x.c[axis] < y.c[axis]; });
+        nd.left = (int) S.nodes.size(); S.nodes.push_back(Node());
+        nd.right = (int) S.nodes.size(); S.nodes.push_back(Node());
+        nd.count = 0;
+        S.nodes[tk.node] = nd;
+        stack.push_back({nd.left, tk.lo, mid});
+        stack.push_back({nd.right, mid, tk.hi});
+    }
+    S.tri_order.resize(nt);
+    for (int i = 0; i < nt; ++i) S.tri_order[i] = items[i].tri;
+}
+
+static void camera_setup(Camera &c, const sms_camera_desc &d) {
+    double o[3] = {d.origin[0], d.origin[1], d.origin[2]};
+    double tg[3] = {d.target[0], d.target[1], d.target[2]};
+    double up[3] = {d.up[0], d.up[1], d.up[2]};
+    double dir[3] = {tg[0] - o[0], tg[1] - o[1], tg[2] - o[2]};
+    double l = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+    for (int i = 0; i < 3; ++i) dir[i] /= l;
+    double left[3] = {up[1] * dir[2] - up[2] * dir[1], up[2] * dir[0] - up[0] * dir[2], up[0] * dir[1] - up[1] * dir[0]};
+    l = sqrt(left[0] * left[0] + left[1] * left[1] + left[2] * left[2]);
+    for (int i = 0; i < 3; ++i) left[i] /= l;
+    double nup[3] = {dir[1] * left[2] - dir[2] * left[1], dir[2] * left[0] - dir[0] * left[2], dir[0] * left[1] - dir[1] * left[0]};
+    c.o = v3((float) o[0], (float) o[1], (float) o[2]);
+    c.left = v3((float) left[0], (float) left[1], (float) left[2]);
+    c.up = v3((float) nup[0], (float) nup[1], (float) nup[2]);
+    c.dir = v3((float) dir[0], (float) dir[1], (float) dir[2]);
+    double th = tan(0.5 * d.fov_deg * M_PI / 180.0);
+    double aspect = (double) d.width / (double) d.height;
+    c.tan_half = (float) th; c.aspect = (float) aspect;
+    c.dx = v3((float) (-2.0 * th / d.width), 0.f, 0.f);
+    c.dy = v3(0.f, (float) (-2.0 * th / aspect / d.height), 0.f);
+    c.rect_x = (float) th; c.rect_y = (float) (th / aspect);
+    c.normalization = (float) (1.0 / ((2.0 * th) * (2.0 * th / aspect)));
+    c.w = d.width; c.h = d.height;
+}
+
+static Scene *scene_create(const sms_scene_desc *desc, const sms_config *cfg) {
+    Scene *S = new Scene();
+    S->cfg = *cfg;
+    for (int i = 0; i < desc->n_normalmaps; ++i) {
+        const sms_normalmap_desc &nd = desc->normalmaps[i];
+        Normalmap nm; nm.res = nd.res;
+        size_t N = (size_t) nd.res * nd.res;
+        nm.slope.resize(2 * N);
+        /* normalmap.cpp:46-66: rgb = byte/255, n = normalize(2 rgb - 1), slope = -n.xy/n.z */
+        for (size_t s = 0; s < N; ++s) {
+            float r = (float) nd.rgb[3 * s + 0] / 255.f, g = (float) nd.rgb[3 * s + 1] / 255.f,
+                  b = (float) nd.rgb[3 * s + 2] / 255.f;
+            V3 n = normalize(v3(2.f * r - 1.f, 2.f * g - 1.f, 2.f * b - 1.f));
+            nm.slope[2 * s + 0] = -n.x / n.z;
+            nm.slope[2 * s + 1] = -n.y / n.z;
+        }
+        S->nmaps.push_back(nm);
+    }
+    for (int i = 0; i < desc->n_emitters; ++i) {
+        const sms_emitter_desc &e = desc->emitters[i];
+        Emitter em; em.type = e.type; em.shape = e.shape;
+        em.radiance = v3(e.radiance[0], e.radiance[1], e.radiance[2]);
+        em.position = v3(e.position[0], e.position[1], e.position[2]);
+        em.intensity = v3(e.intensity[0], e.intensity[1], e.intensity[2]);
+        em.ss = e.caustic_ss != 0; em.ms = e.caustic_ms != 0;
+        S->emitters.push_back(em);
+        S->emitters_all.push_back(i);
+        if (em.ss) S->emitters_ss.push_back(i);
+    }
+    for (int i = 0; i < desc->n_shapes; ++i) {
+        const sms_shape_desc &d = desc->shapes[i];
+        Shape sh; memset(&sh.center, 0, sizeof(V3));
+        sh.type = d.type; sh.bsdf = d.bsdf; sh.flags = d.flags; sh.emitter = d.emitter;
+        sh.center = v3(d.center[0], d.center[1], d.center[2]);
+        sh.radius = d.radius; sh.inv_radius = 1.f / d.radius;
+        memcpy(sh.M, d.to_world, sizeof(sh.M));
+        /* BSDF::ior(): diffuse (0,1) (bsdf.cpp:59-62), dielectric (eta,0), conductor (eta,k) */
+        if (d.bsdf == SMS_BSDF_DIFFUSE) { sh.eta_r = 0.f; sh.k = 1.f; }
+        else if (d.bsdf == SMS_BSDF_DIELECTRIC) { sh.eta_r = d.eta; sh.k = 0.f; }
+        else { sh.eta_r = d.eta; sh.k = d.k; }
+        sh.refl = v3(d.reflectance[0], d.reflectance[1], d.reflectance[2]);
+        sh.normalmap = d.normalmap; sh.tiles = d.tiles;
+        sh.vbase = sh.fbase = sh.nv = sh.nf = 0; sh.has_n = sh.has_uv = false;
+        sh.area_sum = sh.area_norm = 0.f;
+        /* ManifoldVertex eta: dielectric real IOR, conductors (k != 0) and others 1
+           (manifold.h:93-95; bsdf.cpp:59-62 default ior (0,1)) */
+        sh.eta_vtx = sh.k == 0.f ? hmean_same(sh.eta_r) : 1.f;
+        sh.bbox_center = sh.center;
+        if (d.type == SMS_SHAPE_SPHERE) {
+            sh.inv_surface_area = 1.f / (4.f * PI_F * sh.radius * sh.radius);
+        } else if (d.type == SMS_SHAPE_RECTANGLE) {
+            invert_affine(sh.M, sh.Minv);
+            /* rectangle.cpp:86-90: dp_du = M (2,0,0), dp_dv = M (0,2,0), n = normalize(M^-T (0,0,1)) */
+            sh.fr_s = v3(2.f * sh.M[0], 2.f * sh.M[4], 2.f * sh.M[8]);
+            sh.fr_t = v3(2.f * sh.M[1], 2.f * sh.M[5], 2.f * sh.M[9]);
+            double nx = sh.Minv[8], ny = sh.Minv[9], nz = sh.Minv[10];
+            double l = sqrt(nx * nx + ny * ny + nz * nz);
+            sh.fr_n = v3((float) (nx / l), (float) (ny / l), (float) (nz / l));
+            sh.inv_surface_area = 1.f / norm(cross(sh.fr_s, sh.fr_t));
+            sh.bbox_center = v3(sh.M[3], sh.M[7], sh.M[11]);
+        } else if (d.type == SMS_SHAPE_MESH) {
+            sh.vbase = (int) (S->pos.size() / 3); sh.fbase = (int) (S->faces.size() / 3);
+            sh.nv = d.n_vertices; sh.nf = d.n_faces;
+            sh.has_n = d.normals != nullptr; sh.has_uv = d.texcoords != nullptr;
+            for (int v = 0; v < d.n_vertices; ++v) {
+                for (int a = 0; a < 3; ++a) S->pos.push_back(d.positions[3 * v + a]);
+                for (int a = 0; a < 3; ++a) S->nrm.push_back(d.normals ? d.normals[3 * v + a] : 0.f);
+                for (int a = 0; a < 2; ++a) S->uvs.push_back(d.texcoords ? d.texcoords[2 * v + a] : 0.f);
+            }
+            /* area distribution: mesh.cpp area_distr_ensure + DiscreteDistribution::update
+               (include/mitsuba/core/distr_1d.h: double running sum, float cdf) */
+            sh.pmf.resize(d.n_faces); sh.cdf.resize(d.n_faces);
+            double sum = 0.0;
+            for (int f = 0; f < d.n_faces; ++f) {
+                uint32_t i0 = d.faces[3 * f], i1 = d.faces[3 * f + 1], i2 = d.faces[3 * f + 2];
+                for (int k = 0; k < 3; ++k) S->faces.push_back((uint32_t) sh.vbase + d.faces[3 * f + k]);
+                S->tri_shape.push_back(i); S->tri_prim.push_back(f);
+                V3 p0 = v3(d.positions[3 * i0], d.positions[3 * i0 + 1], d.positions[3 * i0 + 2]);
+                V3 p1 = v3(d.positions[3 * i1], d.positions[3 * i1 + 1], d.positions[3 * i1 + 2]);
+                V3 p2 = v3(d.positions[3 * i2], d.positions[3 * i2 + 1], d.positions[3 * i2 + 2]);
+                float a = .5f * norm(cross(p1 - p0, p2 - p0));
+                sh.pmf[f] = a;
+                sum += (double) a;
+                sh.cdf[f] = (float) sum;
+            }
+            sh.area_sum = (float) sum; sh.area_norm = (float) (1.0 / sum);
+            float lo[3] = {INF_F, INF_F, INF_F}, hi[3] = {-INF_F, -INF_F, -INF_F};
+            for (int v = 0; v < d.n_vertices; ++v)
+                for (int a = 0; a < 3; ++a) { lo[a] = std::min(lo[a], d.positions[3 * v + a]); hi[a] = std::max(hi[a], d.positions[3 * v + a]); }
+            sh.bbox_center = v3(0.5f * (lo[0] + hi[0]), 0.5f * (lo[1] + hi[1]), 0.5f * (lo[2] + hi[2]));
+            sh.inv_surface_area = sh.area_norm;
+        }
+        S->shapes.push_back(sh);
+        if (d.flags & SMS_FLAG_CAUSTIC_CASTER_SS) S->casters_ss.push_back(i);
+    }
+    bvh_build(*S);
+    S->has_cam = desc->camera.width > 0;
+    if (S->has_cam) camera_setup(S->cam, desc->camera);
+    return S;
+}
+
+/* ---------------------------------------------------------------- intersection */
+/* Sphere::ray_intersect, FP64 (src/shapes/sphere.cpp:276-303) with math::solve_quadratic
+   (include/mitsuba/core/math.h:371-411). */
+static bool sphere_intersect(const Shape &sh, const Ray &ray, float *t_out) {
+    double mint = (double) ray.mint, maxt = (double) ray.maxt;
+    double ox = (double) ray.o.x - (double) sh.center.x, oy = (double) ray.o.y - (double) sh.center.y,
+           oz = (double) ray.o.z - (double) sh.center.z;
+    double dx = ray.d.x, dy = ray.d.y, dz = ray.d.z;
+    double A = (dx * dx + dy * dy) + dz * dz;
+    double B = 2.0 * ((ox * dx + oy * dy) + oz * dz);
+    double C = ((ox * ox + oy * oy) + oz * oz) - (double) sh.radius * (double) sh.radius;
+    bool linear = A == 0.0;
+    bool active = !linear || B > 0.0;
+    double x0 = -C / B, x1 = x0;
+    double discrim = fma(B, B, -(4.0 * A * C));
+    active = active && (linear || discrim >= 0.0);
+    if (!active) return false;
+    if (!linear) {
+        double sq = sqrt(discrim);
+        double temp = -0.5 * (B + copysign(sq, B));
+        double x0p = temp / A, x1p = C / temp;
+        x0 = std::min(x0p, x1p); x1 = std::max(x0p, x1p);
+    }
+    bool out_bounds = !(x0 <= maxt && x1 >= mint);
+    bool in_bounds = x0 < mint && x1 > maxt;
+    if (out_bounds || in_bounds) return false;
+    *t_out = (float) (x0 < mint ? x1 : x0);
+    return true;
+}
+
+/* Rectangle::ray_intersect (src/shapes/rectangle.cpp:141-163) */
+static bool rect_intersect(const Shape &sh, const Ray &ray, float *t_out, float *lx, float *ly) {
+    V3 o = xform_p(sh.Minv, ray.o), d = xform_v(sh.Minv, ray.d);
+    float t = -o.z * (1.f / d.z);
+    V3 local = fmadd3(d, t, o);
+    if (!(t >= ray.mint && t <= ray.maxt && fabsf(local.x) <= 1.f && fabsf(local.y) <= 1.f)) return false;
+    *t_out = t; *lx = local.x; *ly = local.y;
+    return true;
+}
+
+/* Mesh::ray_intersect_triangle (include/mitsuba/render/mesh.h:202-230) */
+static bool tri_intersect(const Scene &S, int tri, const Ray &ray, float *t_out, float *u_out, float *v_out) {
+    const uint32_t *f = &S.faces[3 * tri];
+    V3 p0 = v3(S.pos[3 * f[0]], S.pos[3 * f[0] + 1], S.pos[3 * f[0] + 2]);
+    V3 p1 = v3(S.pos[3 * f[1]], S.pos[3 * f[1] + 1], S.pos[3 * f[1] + 2]);
+    V3 p2 = v3(S.pos[3 * f[2]], S.pos[3 * f[2] + 1], S.pos[3 * f[2] + 2]);
+    V3 e1 = p1 - p0, e2 = p2 - p0;
+    V3 pvec = cross(ray.d, e2);
+    float inv_det = 1.f / dot(e1, pvec);
+    V3 tvec = ray.o - p0;
+    float u = dot(tvec, pvec) * inv_det;
+    if (!(u >= 0.f && u <= 1.f)) return false;
+    V3 qvec = cross(tvec, e1);
+    float v = dot(ray.d, qvec) * inv_det;
+    if (!(v >= 0.f && u + v <= 1.f)) return false;
+    float t = dot(e2, qvec) * inv_det;
+    if (!(t >= ray.mint && t <= ray.maxt)) return false;
+    *t_out = t; *u_out = u; *v_out = v;
+    return true;
+}
+
+static inline bool box_hit(const Node &n, const Ray &r, float tmax) {
+    float t0 = r.mint, t1 = tmax;
+    const float o[3] = {r.o.x, r.o.y, r.o.z}, ir[3] = {r.d_rcp.x, r.d_rcp.y, r.d_rcp.z};
+    for (int a = 0; a < 3; ++a) {
+        float ta = (n.bmin[a] - o[a]) * ir[a], tb = (n.bmax[a] - o[a]) * ir[a];
+        if (ta != ta) ta = -INF_F;           // 0 * inf inside the slab
+        if (tb != tb) tb = INF_F;
+        float lo = std::min(ta, tb), hi = std::max(ta, tb);
+        t0 = std::max(t0, lo); t1 = std::min(t1, hi);
+    }
+    return t0 <= t1 * (1.f + 4.f * EPS_F);
+}
+
+static inline bool better(float t, int shape, int prim, float bt, int bshape, int bprim) {
+    if (t < bt) return true;
+    if (t > bt) return false;
+    if (shape != bshape) return shape < bshape;
+    return prim < bprim;
+}
+
+struct Hit { bool hit; float t; int shape, prim; float b1, b2, lx, ly; };
+
+/* Scene::ray_intersect closest hit over every shape (scene_native.inl:17-34) */
+static Hit scene_hit(const Scene &S, const Ray &ray, bool any_hit) {
+    Hit h; h.hit = false; h.t = INF_F; h.shape = 1 << 30; h.prim = 1 << 30; h.b1 = h.b2 = h.lx = h.ly = 0.f;
+    for (int i = 0; i < (int) S.shapes.size(); ++i) {
+        const Shape &sh = S.shapes[i];
+        float t, lx = 0.f, ly = 0.f;
+        bool ok = false;
+        Ray r = ray; r.maxt = std::min(ray.maxt, h.t);
+        if (sh.type == SMS_SHAPE_SPHERE) ok = sphere_intersect(sh, r, &t);
+        else if (sh.type == SMS_SHAPE_RECTANGLE) ok = rect_intersect(sh, r, &t, &lx, &ly);
+        if (ok && better(t, i, 0, h.t, h.shape, h.prim)) {
+            h.hit = true; h.t = t; h.shape = i; h.prim = 0; h.lx = lx; h.ly = ly;
+            if (any_hit) return h;
+        }
+    }
+    if (!S.nodes.empty()) {
+        int stack[128]; int sp = 0; stack[sp++] = 0;
+        while (sp > 0) {
+            const Node &n = S.nodes[stack[--sp]];
+            if (!box_hit(n, ray, std::min(ray.maxt, h.t))) continue;
+            if (n.left < 0) {
+                for (int k = n.first; k < n.first + n.count; ++k) {
+                    int tri = S.tri_order[k];
+                    float t, u, v;
+                    Ray r = ray; r.maxt = std::min(ray.maxt, h.t);
+                    if (tri_intersect(S, tri, r, &t, &u, &v) &&
+                        better(t, S.tri_shape[tri], S.tri_prim[tri], h.t, h.shape, h.prim)) {
+                        h.hit = true; h.t = t; h.shape = S.tri_shape[tri]; h.prim = S.tri_prim[tri]; h.b1 = u; h.b2 = v;
+                        if (any_hit) return h;
+                    }
+                }
+            } else {
+                stack[sp++] = n.left; stack[sp++] = n.right;
+            }
+        }
+    }
+    return h;
+}
+
+/* Shape::fill_surface_interaction for the three shape kinds + the kd-tree's shading frame
+   (include/mitsuba/render/kdtree.h:2335-2364) */
+static SI make_si(const Scene &S, const Ray &ray, const Hit &h) {
+    if (!h.hit) return si_invalid();
+    SI si = si_invalid();
+    si.valid = true; si.t = h.t; si.shape = h.shape; si.prim = h.prim;
+    const Shape &sh = S.shapes[h.shape];
+    if (sh.type == SMS_SHAPE_SPHERE) {
+        /* sphere.cpp:329-374 */
+        V3 n = normalize(ray_at(ray, si.t) - sh.center);
+        si.sh_n = n;
+        si.p = fmadd3(n, sh.radius, sh.center);
+        V3 local = (si.p - sh.center) * sh.inv_radius;
+        float rd_2 = local.x * local.x + local.y * local.y;
+        si.uv = v2(0.f, 0.f);
+        si.dp_du = v3(-local.y, local.x, 0.f);
+        float rd = sqrtf(rd_2), inv_rd = 1.f / rd;
+        float cos_phi = local.x * inv_rd, sin_phi = local.y * inv_rd;
+        si.dp_dv = v3(local.z * cos_phi, local.z * sin_phi, -rd);
+        if (rd == 0.f) si.dp_dv = v3(1.f, 0.f, 0.f);
+        si.dp_du = (si.dp_du * sh.radius) * (2.f * PI_F);
+        si.dp_dv = (si.dp_dv * sh.radius) * PI_F;
+        si.n = si.sh_n;
+    } else if (sh.type == SMS_SHAPE_RECTANGLE) {
+        /* rectangle.cpp:175-202 */
+        si.n = sh.fr_n; si.sh_n = sh.fr_n;
+        si.dp_du = sh.fr_s; si.dp_dv = sh.fr_t;
+        si.p = ray_at(ray, si.t);
+        si.uv = v2(fmaf(h.lx, .5f, .5f), fmaf(h.ly, .5f, .5f));
+        si.lx = h.lx; si.ly = h.ly;
+    } else {
+        /* mesh.cpp:620-691 */
+        float b1 = h.b1, b2 = h.b2, b0 = 1.f - b1 - b2;
+        si.b1 = b1; si.b2 = b2;
+        int tri = sh.fbase + h.prim;
+        const uint32_t *f = &S.faces[3 * tri];
+        V3 p0 = v3(S.pos[3 * f[0]], S.pos[3 * f[0] + 1], S.pos[3 * f[0] + 2]);
+        V3 p1 = v3(S.pos[3 * f[1]], S.pos[3 * f[1] + 1], S.pos[3 * f[1] + 2]);
+        V3 p2 = v3(S.pos[3 * f[2]], S.pos[3 * f[2] + 1], S.pos[3 * f[2] + 2]);
+        V3 dp0 = p1 - p0, dp1 = p2 - p0;
+        si.p = (p0 * b0 + p1 * b1) + p2 * b2;
+        V3 n = normalize(cross(dp0, dp1));
+        si.n = n;
+        V3 dpdu, dpdv; coordinate_system(n, &dpdu, &dpdv);
+        V2 uv = v2(b1, b2);
+        if (sh.has_uv) {
+            V2 uv0 = v2(S.uvs[2 * f[0]], S.uvs[2 * f[0] + 1]), uv1 = v2(S.uvs[2 * f[1]], S.uvs[2 * f[1] + 1]),
+               uv2 = v2(S.uvs[2 * f[2]], S.uvs[2 * f[2] + 1]);
+            uv = (uv0 * b0 + uv1 * b1) + uv2 * b2;
+            V2 duv0 = uv1 - uv0, duv1 = uv2 - uv0;
+            float det = fmaf(duv0.x, duv1.y, -(duv0.y * duv1.x)), inv_det = 1.f / det;
+            if (det != 0.f) {
+                V3 a = duv0.y * dp1;
+                dpdu = v3(fmaf(duv1.y, dp0.x, -a.x), fmaf(duv1.y, dp0.y, -a.y), fmaf(duv1.y, dp0.z, -a.z)) * inv_det;
+                V3 c = duv0.x * dp1;
+                dpdv = v3(fmaf(-duv1.x, dp0.x, c.x), fmaf(-duv1.x, dp0.y, c.y), fmaf(-duv1.x, dp0.z, c.z)) * inv_det;
+            }
+        }
+        si.uv = uv;
+        if (sh.has_n) {
+            V3 n0 = v3(S.nrm[3 * f[0]], S.nrm[3 * f[0] + 1], S.nrm[3 * f[0] + 2]);
+            V3 n1 = v3(S.nrm[3 * f[1]], S.nrm[3 * f[1] + 1], S.nrm[3 * f[1] + 2]);
+            V3 n2 = v3(S.nrm[3 * f[2]], S.nrm[3 * f[2] + 1], S.nrm[3 * f[2] + 2]);
+            n = normalize((n0 * b0 + n1 * b1) + n2 * b2);
+        }
+        si.sh_n = n;
+        si.dp_du = dpdu; si.dp_dv = dpdv;
+    }
+    si.sh_s = normalize(fnmadd3(si.sh_n, dot(si.sh_n, si.dp_du), si.dp_du));
+    si.sh_t = cross(si.sh_n, si.sh_s);
+    V3 mw = -ray.d;
+    si.wi = v3(dot(mw, si.sh_s), dot(mw, si.sh_t), dot(mw, si.sh_n));
+    si.duv_dx = si.duv_dy = v2(0.f, 0.f);
+    return si;
+}
+
+static SI ray_intersect(const Scene &S, const Ray &ray) { return make_si(S, ray, scene_hit(S, ray, false)); }
+static bool ray_test(const Scene &S, const Ray &ray) { return scene_hit(S, ray, true).hit; }
+
+/* ---------------------------------------------------------------- normal derivatives */
+static void normal_derivative(const Scene &S, const SI &si, V3 *dn_du, V3 *dn_dv) {
+    const Shape &sh = S.shapes[si.shape];
+    if (sh.type == SMS_SHAPE_SPHERE) {
+        /* sphere.cpp:376-382 */
+        *dn_du = si.dp_du * sh.inv_radius; *dn_dv = si.dp_dv * sh.inv_radius;
+    } else if (sh.type == SMS_SHAPE_RECTANGLE) {
+        *dn_du = v3(0.f, 0.f, 0.f); *dn_dv = v3(0.f, 0.f, 0.f);
+    } else {
+        /* mesh.cpp:694-760 */
+        if (!sh.has_n) { *dn_du = v3(0.f, 0.f, 0.f); *dn_dv = v3(0.f, 0.f, 0.f); return; }
+        int tri = sh.fbase + si.prim;
+        const uint32_t *f = &S.faces[3 * tri];
+        V3 p0 = v3(S.pos[3 * f[0]], S.pos[3 * f[0] + 1], S.pos[3 * f[0] + 2]);
+        V3 p1 = v3(S.pos[3 * f[1]], S.pos[3 * f[1] + 1], S.pos[3 * f[1] + 2]);
+        V3 p2 = v3(S.pos[3 * f[2]], S.pos[3 * f[2] + 1], S.pos[3 * f[2] + 2]);
+        V3 n0 = v3(S.nrm[3 * f[0]], S.nrm[3 * f[0] + 1], S.nrm[3 * f[0] + 2]);
+        V3 n1 = v3(S.nrm[3 * f[1]], S.nrm[3 * f[1] + 1], S.nrm[3 * f[1] + 2]);
+        V3 n2 = v3(S.nrm[3 * f[2]], S.nrm[3 * f[2] + 1], S.nrm[3 * f[2] + 2]);
+        V3 rel = si.p - p0, du = p1 - p0, dv = p2 - p0;
+        float b1 = dot(du, rel), b2 = dot(dv, rel), a11 = dot(du, du), a12 = dot(du, dv), a22 = dot(dv, dv),
+              inv_det = 1.f / (a11 * a22 - a12 * a12);
+        float u = fmaf(a22, b1, -(a12 * b2)) * inv_det, v = fmaf(-a12, b1, a11 * b2) * inv_det, w = 1.f - u - v;
+        V3 N = (u * n1 + v * n2) + w * n0;
+        float il = 1.f / sqrtf(sqnorm(N));
+        N = N * il;
+        V3 dndu = (n1 - n0) * il, dndv = (n2 - n0) * il;
+        dndu = fnmadd3(N, dot(N, dndu), dndu);
+        dndv = fnmadd3(N, dot(N, dndv), dndv);
+        if (sh.has_uv) {
+            V2 uv0 = v2(S.uvs[2 * f[0]], S.uvs[2 * f[0] + 1]), uv1 = v2(S.uvs[2 * f[1]], S.uvs[2 * f[1] + 1]),
+               uv2 = v2(S.uvs[2 * f[2]], S.uvs[2 * f[2] + 1]);
+            V2 duv1 = uv1 - uv0, duv2 = uv2 - uv0;
+            inv_det = 1.f / (duv1.x * duv2.y - duv1.y * duv2.x);
+            V3 a = (duv2.y * dndu - duv1.y * dndv) * inv_det, b = (-duv2.x * dndu + duv1.x * dndv) * inv_det;
+            dndu = a; dndv = b;
+        }
+        *dn_du = dndu; *dn_dv = dndv;
+    }
+}
+
+/* ---------------------------------------------------------------- normal map (level 0) */
+struct Texel4 { V2 v00, v10, v01, v11; V2 w; };
+/* Normalmap::eval_normal / eval_normal_derivatives index computation (normalmap.h:62-80,148-163) */
+static Texel4 nm_fetch(const Normalmap &nm, V2 uv) {
+    int res = nm.res;
+    float fres = (float) res;
+    float duv = 1.f / fres;
+    V2 pos = v2((uv.x - 0.5f * duv) * fres, (uv.y - 0.5f * duv) * fres);
+    float hi = (float) (res - 1);
+    pos.x = std::min(std::max(pos.x, 0.f), hi); pos.y = std::min(std::max(pos.y, 0.f), hi);
+    uint32_t xmin = (uint32_t) floorf(pos.x), ymin = (uint32_t) floorf(pos.y);
+    uint32_t xmax = xmin + 1, ymax = ymin + 1;
+    uint32_t lim = (uint32_t) (res - 1);
+    xmin = std::min(xmin, lim); ymin = std::min(ymin, lim); xmax = std::min(xmax, lim); ymax = std::min(ymax, lim);
+    Texel4 t;
+    const float *s = nm.slope.data();
+    uint32_t i00 = ymin * res + xmin, i10 = ymin * res + xmax, i01 = ymax * res + xmin, i11 = ymax * res + xmax;
+    t.v00 = v2(s[2 * i00], s[2 * i00 + 1]); t.v10 = v2(s[2 * i10], s[2 * i10 + 1]);
+    t.v01 = v2(s[2 * i01], s[2 * i01 + 1]); t.v11 = v2(s[2 * i11], s[2 * i11 + 1]);
+    t.w = v2(pos.x - (float) xmin, pos.y - (float) ymin);
+    return t;
+}
+/* interpolated slope (v[0], v[1]) of eval_normal with use_slopes (normalmap.h:82-92) */
+static V2 nm_slope(const Texel4 &t) {
+    V2 w1 = t.w, w0 = v2(1.f - w1.x, 1.f - w1.y);
+    V2 a = v2(fmaf(w0.x, t.v00.x, w1.x * t.v10.x), fmaf(w0.x, t.v00.y, w1.x * t.v10.y));
+    V2 b = v2(fmaf(w0.x, t.v01.x, w1.x * t.v11.x), fmaf(w0.x, t.v01.y, w1.x * t.v11.y));
+    return v2(fmaf(w0.y, a.x, w1.y * b.x), fmaf(w0.y, a.y, w1.y * b.y));
+}
+/* tmp_u, tmp_v of eval_normal_derivatives with use_slopes (normalmap.h:170-176) */
+static void nm_slope_deriv(const Texel4 &t, float res, V2 *tu, V2 *tv) {
+    V2 tmp = (t.v01 + t.v10) - t.v11;
+    *tu = ((t.v10 + t.v00 * (t.w.y - 1.f)) - tmp * t.w.y) * res;
+    *tv = ((t.v01 + t.v00 * (t.w.x - 1.f)) - tmp * t.w.x) * res;
+}
+static inline V2 tile_uv(V2 uv, float tiles) {
+    uv = uv * tiles;
+    return v2(uv.x - floorf(uv.x), uv.y - floorf(uv.y));
+}
+
+/* ---------------------------------------------------------------- ManifoldVertex */
+struct MV {
+    V3 p, dp_du, dp_dv, n, gn, dn_du, dn_dv, s, t, ds_du, ds_dv, dt_du, dt_dv;
+    float eta; V2 uv; int shape; bool fixed_direction;
+};
+static MV mv_point(V3 p) {
+    MV v; memset(&v, 0, sizeof(v)); v.p = p; v.eta = 1.f; v.shape = -1; v.fixed_direction = false; return v;
+}
+
+/* compute_shading_frame_derivative (include/mitsuba/core/frame.h:224-244) */
+static void shading_frame_derivative(V3 n, V3 dp_du, V3 dn_du, V3 dn_dv, MV &v) {
+    V3 s = dp_du - n * dot(n, dp_du);
+    float inv_len_s = 1.f / norm(s);
+    s = s * inv_len_s;
+    v.ds_du = inv_len_s * (-dn_du * dot(n, dp_du) - n * dot(dn_du, dp_du));
+    v.ds_dv = inv_len_s * (-dn_dv * dot(n, dp_du) - n * dot(dn_dv, dp_du));
+    v.ds_du = v.ds_du - s * dot(v.ds_du, s);
+    v.ds_dv = v.ds_dv - s * dot(v.ds_dv, s);
+    v.dt_du = cross(dn_du, s) + cross(n, v.ds_du);
+    v.dt_dv = cross(dn_dv, s) + cross(n, v.ds_dv);
+    v.dn_du = dn_du; v.dn_dv = dn_dv;
+}
+
+static inline V3 to_world(V3 s, V3 t, V3 n, V3 v) { return (s * v.x + t * v.y) + n * v.z; }
+
+/* ManifoldVertex(si, smoothing=0) (include/mitsuba/render/manifold.h:89-113) */
+static MV mv_from_si(const Scene &S, const SI &si) {
+    MV v = mv_point(si.p);
+    v.dp_du = si.dp_du; v.dp_dv = si.dp_dv; v.gn = si.n; v.uv = si.uv; v.shape = si.shape;
+    const Shape &sh = S.shapes[si.shape];
+    v.eta = sh.eta_vtx;
+    V3 bn = si.sh_n;
+    V3 bs = normalize(fnmadd3(bn, dot(bn, si.dp_du), si.dp_du));
+    V3 bt = cross(bn, bs);
+    V3 sdn_du, sdn_dv;
+    normal_derivative(S, si, &sdn_du, &sdn_dv);
+    if (sh.bsdf != SMS_BSDF_NORMALMAP) {
+        /* BSDF::frame / frame_derivative (src/librender/bsdf.cpp:17-31) */
+        v.n = bn; v.s = bs; v.t = bt;
+        shading_frame_derivative(bn, si.dp_du, sdn_du, sdn_dv, v);
+    } else {
+        /* NormalmapBSDF::frame / frame_derivative at smoothing 0 (src/bsdfs/normalmap.cpp:218-290) */
+        const Normalmap &nm = S.nmaps[sh.normalmap];
+        V2 uv = tile_uv(si.uv, sh.tiles);
+        Texel4 tx = nm_fetch(nm, uv);
+        V2 sl = nm_slope(tx);
+        V3 nl = v3(-sl.x, -sl.y, 1.f);
+        V2 tu, tv; nm_slope_deriv(tx, (float) nm.res, &tu, &tv);
+        V3 dn_du = v3(-tu.x, -tu.y, 0.f) * sh.tiles, dn_dv = v3(-tv.x, -tv.y, 0.f) * sh.tiles;
+        V3 rn = normalize(to_world(bs, bt, bn, nl));
+        v.n = rn;
+        v.s = normalize(si.dp_du - rn * dot(rn, si.dp_du));
+        v.t = cross(rn, v.s);
+        MV db; shading_frame_derivative(bn, si.dp_du, sdn_du, sdn_dv, db);
+        V3 world_n = to_world(bs, bt, bn, nl);
+        float inv_length_n = 1.f / norm(world_n);
+        world_n = world_n * inv_length_n;
+        V3 dnu = inv_length_n * (to_world(bs, bt, bn, dn_du) + to_world(db.ds_du, db.dt_du, db.dn_du, nl));
+        V3 dnv = inv_length_n * (to_world(bs, bt, bn, dn_dv) + to_world(db.ds_dv, db.dt_dv, db.dn_dv, nl));
+        dnu = dnu - world_n * dot(dnu, world_n);
+        dnv = dnv - world_n * dot(dnv, world_n);
+        V3 s = si.dp_du - world_n * dot(world_n, si.dp_du);
+        float inv_length_s = 1.f / norm(s);
+        s = s * inv_length_s;
+        V3 dsu = inv_length_s * (-dnu * dot(world_n, si.dp_du) - world_n * dot(dnu, si.dp_du));
+        V3 dsv = inv_length_s * (-dnv * dot(world_n, si.dp_du) - world_n * dot(dnv, si.dp_du));
+        dsu = dsu - s * dot(dsu, s);
+        dsv = dsv - s * dot(dsv, s);
+        v.dn_du = dnu; v.dn_dv = dnv; v.ds_du = dsu; v.ds_dv = dsv;
+        v.dt_du = cross(dnu, s) + cross(world_n, dsu);
+        v.dt_dv = cross(dnv, s) + cross(world_n, dsv);
+    }
+    if (dot(v.n, v.gn) < 0.f) v.gn = v.gn * -1.f;
+    return v;
+}
+
+/* ManifoldVertex::make_orthonormal (manifold.h:115-126) */
+static void make_orthonormal(MV &v) {
+    float inv_norm = 1.f / norm(v.dp_du);
+    v.dp_du = v.dp_du * inv_norm;
+    v.dn_du = v.dn_du * inv_norm;
+    float dp = dot(v.dp_du, v.dp_dv);
+    V3 dp_dv_tmp = v.dp_dv - dp * v.dp_du;
+    V3 dn_dv_tmp = v.dn_dv - dp * v.dn_du;
+    inv_norm = 1.f / norm(dp_dv_tmp);
+    v.dp_dv = dp_dv_tmp * inv_norm;
+    v.dn_dv = dn_dv_tmp * inv_norm;
+}
+
+/* ---------------------------------------------------------------- specular transforms */
+/* SpecularManifold::reflect / d_reflect / refract / d_refract (manifold.h:347-417) */
+static inline V3 reflect(V3 w, V3 n) { return 2.f * dot(w, n) * n - w; }
+static inline void d_reflect(V3 w, V3 dw_du, V3 dw_dv, V3 n, V3 dn_du, V3 dn_dv, V3 *du, V3 *dv) {
+    float dot_w_n = dot(w, n), dot_dwdu_n = dot(dw_du, n), dot_dwdv_n = dot(dw_dv, n),
+          dot_w_dndu = dot(w, dn_du), dot_w_dndv = dot(w, dn_dv);
+    *du = 2.f * ((dot_dwdu_n + dot_w_dndu) * n + dot_w_n * dn_du) - dw_du;
+    *dv = 2.f * ((dot_dwdv_n + dot_w_dndv) * n + dot_w_n * dn_dv) - dw_dv;
+}
+static inline bool refract(V3 w, V3 n_, float eta_, V3 *out) {
+    V3 n = n_;
+    float eta = 1.f / eta_;
+    if (dot(w, n) < 0.f) { eta = 1.f / eta; n = n * -1.f; }
+    float dot_w_n = dot(w, n);
+    float root_term = 1.f - eta * eta * (1.f - dot_w_n * dot_w_n);
+    if (root_term < 0.f) { *out = v3(0.f, 0.f, 0.f); return false; }
+    *out = -eta * (w - dot_w_n * n) - n * sqrtf(root_term);
+    return true;
+}
+static inline void d_refract(V3 w, V3 dw_du, V3 dw_dv, V3 n_, V3 dn_du_, V3 dn_dv_, float eta_, V3 *du, V3 *dv) {
+    V3 n = n_, dn_du = dn_du_, dn_dv = dn_dv_;
+    float eta = 1.f / eta_;
+    if (dot(w, n) < 0.f) { eta = 1.f / eta; n = n * -1.f; dn_du = dn_du * -1.f; dn_dv = dn_dv * -1.f; }
+    float dot_w_n = dot(w, n), dot_dwdu_n = dot(dw_du, n), dot_dwdv_n = dot(dw_dv, n),
+          dot_w_dndu = dot(w, dn_du), dot_w_dndv = dot(w, dn_dv);
+    float root = sqrtf(1.f - eta * eta * (1.f - dot_w_n * dot_w_n));
+    V3 a_u = -eta * (dw_du - ((dot_dwdu_n + dot_w_dndu) * n + dot_w_n * dn_du));
+    V3 b1_u = dn_du * root;
+    V3 b2_u = n * (1.f / (2.f * root)) * (-eta * eta * (-2.f * dot_w_n * (dot_dwdu_n + dot_w_dndu)));
+    V3 b_u = -(b1_u + b2_u);
+    V3 a_v = -eta * (dw_dv - ((dot_dwdv_n + dot_w_dndv) * n + dot_w_n * dn_dv));
+    V3 b1_v = dn_dv * root;
+    V3 b2_v = n * (1.f / (2.f * root)) * (-eta * eta * (-2.f * dot_w_n * (dot_dwdv_n + dot_w_dndv)));
+    V3 b_v = -(b1_v + b2_v);
+    *du = a_u + b_u; *dv = a_v + b_v;
+}
+/* sphcoords / d_sphcoords (manifold.h:419-444) */
+static inline void sphcoords(V3 w, float *theta, float *phi) {
+    *theta = safe_acos(w.z);
+    *phi = c_atan2f(w.y, w.x);
+    if (*phi < 0.f) *phi += 2.f * PI_F;
+}
+static inline void d_sphcoords(V3 w, V3 dw_du, V3 dw_dv, float *dt_du, float *dp_du, float *dt_dv, float *dp_dv) {
+    float d_acos = -(1.f / safe_sqrt(1.f - w.z * w.z));
+    V2 d_theta = d_acos * v2(dw_du.z, dw_dv.z);
+    float yx = w.y / w.x;
+    float d_atan = 1.f / (1.f + yx * yx);
+    V2 d_phi = (d_atan * v2(w.x * dw_du.y - w.y * dw_du.x, w.x * dw_dv.y - w.y * dw_dv.x)) * (1.f / (w.x * w.x));
+    if (w.x == 0.f) d_phi = v2(0.f, 0.f);
+    *dt_du = d_theta.x; *dp_du = d_phi.x; *dt_dv = d_theta.y; *dp_dv = d_phi.y;
+}
+
+/* ---------------------------------------------------------------- emitter interaction */
+struct EI { V3 p, n, d, weight; float pdf; int kind, emitter; };
+static inline bool ei_is_directional(const EI &e) { return e.kind == SMS_EI_DIRECTIONAL; }
+static inline bool ei_is_delta(const EI &e) { return e.kind == SMS_EI_POINT || e.kind == SMS_EI_DIRECTIONAL; }
+
+static EI ei_from_c(const sms_emitter_interaction &c) {
+    EI e; e.p = v3(c.p[0], c.p[1], c.p[2]); e.n = v3(c.n[0], c.n[1], c.n[2]); e.d = v3(c.d[0], c.d[1], c.d[2]);
+    e.weight = v3(c.weight[0], c.weight[1], c.weight[2]); e.pdf = c.pdf; e.kind = c.kind; e.emitter = c.emitter;
+    return e;
+}
+
+/* warp::square_to_uniform_sphere (include/mitsuba/core/warp.h:262-267) */
+static inline V3 square_to_uniform_sphere(V2 s) {
+    float z = fmaf(-2.f, s.y, 1.f), r = safe_sqrt(fmaf(-z, z, 1.f));
+    float sn, cs; c_sincosf((2.f * PI_F) * s.x, &sn, &cs);
+    return v3(r * cs, r * sn, z);
+}
+
+struct PS { V3 p, n; float pdf; };
+/* Shape::sample_position for sphere (sphere.cpp:153-171), rectangle (rectangle.cpp:114-126),
+   mesh (mesh.cpp:542-585) */
+static PS sample_position(const Scene &S, int shape, V2 sample) {
+    const Shape &sh = S.shapes[shape];
+    PS ps;
+    if (sh.type == SMS_SHAPE_SPHERE) {
+        V3 local = square_to_uniform_sphere(sample);
+        ps.p = fmadd3(local, sh.radius, sh.center); ps.n = local; ps.pdf = sh.inv_surface_area;
+    } else if (sh.type == SMS_SHAPE_RECTANGLE) {
+        ps.p = xform_p(sh.M, v3(sample.x * 2.f - 1.f, sample.y * 2.f - 1.f, 0.f));
+        ps.n = sh.fr_n; ps.pdf = sh.inv_surface_area;
+    } else {
+        float value = sample.y;
+        float scaled = value * sh.area_sum;
+        /* enoki::binary_search over [0, nf-1]: first index whose cdf is not < value */
+        int lo = 0, hi = sh.nf - 1;
+        while (lo < hi) { int mid = (lo + hi) / 2; if (sh.cdf[mid] < scaled) lo = mid + 1; else hi = mid; }
+        int idx = lo;
+        float pmf = sh.pmf[idx] * sh.area_norm;
+        float cdf = idx > 0 ? sh.cdf[idx - 1] * sh.area_norm : 0.f;
+        float y = (value - cdf) / pmf;
+        const uint32_t *f = &S.faces[3 * (sh.fbase + idx)];
+        V3 p0 = v3(S.pos[3 * f[0]], S.pos[3 * f[0] + 1], S.pos[3 * f[0] + 2]);
+        V3 p1 = v3(S.pos[3 * f[1]], S.pos[3 * f[1] + 1], S.pos[3 * f[1] + 2]);
+        V3 p2 = v3(S.pos[3 * f[2]], S.pos[3 * f[2] + 1], S.pos[3 * f[2] + 2]);
+        V3 e0 = p1 - p0, e1 = p2 - p0;
+        float t = safe_sqrt(1.f - sample.x);
+        V2 b = v2(1.f - t, t * y);
+        ps.p = (p0 + e0 * b.x) + e1 * b.y;
+        if (sh.has_n) {
+            V3 n0 = v3(S.nrm[3 * f[0]], S.nrm[3 * f[0] + 1], S.nrm[3 * f[0] + 2]);
+            V3 n1 = v3(S.nrm[3 * f[1]], S.nrm[3 * f[1] + 1], S.nrm[3 * f[1] + 2]);
+            V3 n2 = v3(S.nrm[3 * f[2]], S.nrm[3 * f[2] + 1], S.nrm[3 * f[2] + 2]);
+            ps.n = normalize((n0 * (1.f - b.x - b.y) + n1 * b.x) + n2 * b.y);
+        } else ps.n = normalize(cross(e0, e1));
+        ps.pdf = sh.area_norm;
+    }
+    return ps;
+}
+
+/* SpecularManifold::sample_emitter_interaction (manifold.h:203-260) */
+static EI sample_emitter_interaction(const Scene &S, V3 si_p, const std::vector<int> &emitters, Pcg32 *rng) {
+    EI ei; memset(&ei, 0, sizeof(ei)); ei.emitter = -1; ei.kind = SMS_EI_AREA;
+    if (emitters.empty()) return ei;
+    float emitter_sample = pcg32_float(rng);
+    float emitter_pdf = 1.f / (float) emitters.size();
+    uint32_t index = std::min((uint32_t) (emitter_sample * (float) emitters.size()), (uint32_t) emitters.size() - 1);
+    const Emitter &em = S.emitters[emitters[index]];
+    ei.emitter = emitters[index];
+    V3 spec = v3(0.f, 0.f, 0.f);
+    V2 s2; s2.x = pcg32_float(rng); s2.y = pcg32_float(rng);
+    if (em.type == SMS_EMITTER_AREA) {
+        ei.kind = SMS_EI_AREA;
+        PS ps = sample_position(S, em.shape, s2);
+        if (ps.pdf > 0.f) {
+            spec = em.radiance / ps.pdf;        // area.cpp:71-78 with cos(wi=(0,0,1)) > 0
+            ei.p = ps.p; ei.n = ps.n; ei.d = normalize(ps.p - si_p); ei.pdf = ps.pdf;
+        }
+    } else {
+        ei.kind = SMS_EI_POINT;
+        /* PointLight::sample_direction (src/emitters/point.cpp:74-98) */
+        V3 d = em.position - si_p;
+        float dist = norm(d), inv_dist = 1.f / dist;
+        d = d * inv_dist;
+        ei.p = em.position; ei.d = d; ei.n = d; ei.pdf = 1.f;
+        spec = em.intensity * (inv_dist * inv_dist);
+        spec = spec * dist * dist;
+    }
+    ei.pdf *= emitter_pdf;
+    ei.weight = spec * (1.f / emitter_pdf);
+    return ei;
+}
+
+/* SpecularManifold::emitter_interaction_to_vertex (manifold.h:263-302) */
+static bool emitter_interaction_to_vertex(const Scene &S, const EI &ei, V3 p, MV *vy) {
+    if (ei.kind == SMS_EI_AREA) {
+        V3 d_tmp = normalize(ei.p - p);
+        Ray r = make_ray(p + SHADOW_EPS_F * d_tmp, d_tmp);
+        SI si_y = ray_intersect(S, r);
+        if (!si_y.valid) { *vy = mv_point(v3(0.f, 0.f, 0.f)); return false; }
+        *vy = mv_from_si(S, si_y);
+        make_orthonormal(*vy);
+        return true;
+    } else if (ei.kind == SMS_EI_DIRECTIONAL) {
+        *vy = mv_point(ei.p);
+        V3 d = normalize(ei.p - p);
+        vy->p = p + d; vy->n = -d;
+        coordinate_system(vy->n, &vy->dp_du, &vy->dp_dv);
+        vy->fixed_direction = true;
+        return true;
+    } else {
+        *vy = mv_point(ei.p);
+        V3 d = normalize(p - ei.p);
+        vy->n = d; vy->gn = d;
+        coordinate_system(d, &vy->dp_du, &vy->dp_dv);
+        return true;
+    }
+}
+
+/* ---------------------------------------------------------------- Fresnel */
+/* fresnel() (include/mitsuba/render/fresnel.h:34-75) */
+static float fresnel(float cos_theta_i, float eta) {
+    bool outside = cos_theta_i >= 0.f;
+    float rcp_eta = 1.f / eta, eta_it = outside ? eta : rcp_eta, eta_ti = outside ? rcp_eta : eta;
+    float cos_theta_t_sqr = fmaf(-fmaf(-cos_theta_i, cos_theta_i, 1.f), eta_ti * eta_ti, 1.f);
+    float cos_theta_i_abs = fabsf(cos_theta_i), cos_theta_t_abs = safe_sqrt(cos_theta_t_sqr);
+    bool index_matched = eta == 1.f, special = index_matched || cos_theta_i_abs == 0.f;
+    float r_sc = index_matched ? 0.f : 1.f;
+    float a_s = fmaf(-eta_it, cos_theta_t_abs, cos_theta_i_abs) / fmaf(eta_it, cos_theta_t_abs, cos_theta_i_abs);
+    float a_p = fmaf(-eta_it, cos_theta_i_abs, cos_theta_t_abs) / fmaf(eta_it, cos_theta_i_abs, cos_theta_t_abs);
+    float r = .5f * (a_s * a_s + a_p * a_p);
+    return special ? r_sc : r;
+}
+/* fresnel_conductor (fresnel.h:96-122) */
+static float fresnel_conductor(float cos_theta_i, float eta_r, float eta_i) {
+    float c2 = cos_theta_i * cos_theta_i, s2 = 1.f - c2, s4 = s2 * s2;
+    float temp_1 = eta_r * eta_r - eta_i * eta_i - s2;
+    float a_2_pb_2 = safe_sqrt(temp_1 * temp_1 + 4.f * eta_i * eta_i * eta_r * eta_r);
+    float a = safe_sqrt(.5f * (a_2_pb_2 + temp_1));
+    float term_1 = a_2_pb_2 + c2, term_2 = 2.f * cos_theta_i * a;
+    float r_s = (term_1 - term_2) / (term_1 + term_2);
+    float term_3 = a_2_pb_2 * c2 + s4, term_4 = term_2 * s2;
+    float r_p = r_s * (term_3 - term_4) / (term_3 + term_4);
+    return .5f * (r_s + r_p);
+}
+
+/* ================================================================ single scatter */
+struct Step { bool ok; V2 C, dX; };
+
+/* compute_step_anglediff (src/librender/manifold_ss.cpp:494-636) */
+static Step step_anglediff(V3 v0p, const MV &v1, const EI &v2e, V3 n_offset) {
+    Step r; r.ok = false; r.C = v2(INF_F, INF_F); r.dX = v2(0.f, 0.f);
+    V3 wi = v0p - v1.p;
+    float ili = norm(wi);
+    if (ili < 1e-3f) return r;
+    ili = 1.f / ili;
+    wi = wi * ili;
+    V3 dwi_du = -ili * (v1.dp_du - wi * dot(wi, v1.dp_du));
+    V3 dwi_dv = -ili * (v1.dp_dv - wi * dot(wi, v1.dp_dv));
+    V3 wo = ei_is_directional(v2e) ? v2e.d : v2e.p - v1.p;
+    float ilo = norm(wo);
+    if (ilo < 1e-3f) return r;
+    ilo = 1.f / ilo;
+    wo = wo * ilo;
+    V3 dwo_du, dwo_dv;
+    if (ei_is_directional(v2e)) { dwo_du = v3(0.f, 0.f, 0.f); dwo_dv = v3(0.f, 0.f, 0.f); }
+    else {
+        dwo_du = -ilo * (v1.dp_du - wo * dot(wo, v1.dp_du));
+        dwo_dv = -ilo * (v1.dp_dv - wo * dot(wo, v1.dp_dv));
+    }
+    V2 C = v2(0.f, 0.f);
+    M2 dC = m2(0.f, 0.f, 0.f, 0.f);
+    V3 n = (v1.s * n_offset.x + v1.t * n_offset.y) + v1.n * n_offset.z;
+    V3 dn_du = (v1.ds_du * n_offset.x + v1.dt_du * n_offset.y) + v1.dn_du * n_offset.z;
+    V3 dn_dv = (v1.ds_dv * n_offset.x + v1.dt_dv * n_offset.y) + v1.dn_dv * n_offset.z;
+    bool success = false;
+    V3 wio; bool valid_i;
+    if (v1.eta == 1.f) { wio = reflect(wi, n); valid_i = true; } else valid_i = refract(wi, n, v1.eta, &wio);
+    if (valid_i) {
+        V3 dwio_du, dwio_dv;
+        if (v1.eta == 1.f) d_reflect(wi, dwi_du, dwi_dv, n, dn_du, dn_dv, &dwio_du, &dwio_dv);
+        else d_refract(wi, dwi_du, dwi_dv, n, dn_du, dn_dv, v1.eta, &dwio_du, &dwio_dv);
+        float to, po, tio, pio;
+        sphcoords(wo, &to, &po); sphcoords(wio, &tio, &pio);
+        float dt = to - tio, dp = po - pio;
+        if (dp < -PI_F) dp += 2.f * PI_F; else if (dp > PI_F) dp -= 2.f * PI_F;
+        C = v2(dt, dp);
+        float dto_du, dpo_du, dto_dv, dpo_dv, dtio_du, dpio_du, dtio_dv, dpio_dv;
+        d_sphcoords(wo, dwo_du, dwo_dv, &dto_du, &dpo_du, &dto_dv, &dpo_dv);
+        d_sphcoords(wio, dwio_du, dwio_dv, &dtio_du, &dpio_du, &dtio_dv, &dpio_dv);
+        dC = m2(dto_du - dtio_du, dto_dv - dtio_dv, dpo_du - dpio_du, dpo_dv - dpio_dv);
+        success = true;
+    }
+    V3 woi; bool valid_o;
+    if (v1.eta == 1.f) { woi = reflect(wo, n); valid_o = true; } else valid_o = refract(wo, n, v1.eta, &woi);
+    if (valid_o && !success) {
+        V3 dwoi_du, dwoi_dv;
+        if (v1.eta == 1.f) d_reflect(wo, dwo_du, dwo_dv, n, dn_du, dn_dv, &dwoi_du, &dwoi_dv);
+        else d_refract(wo, dwo_du, dwo_dv, n, dn_du, dn_dv, v1.eta, &dwoi_du, &dwoi_dv);
+        float ti, pi, toi, poi;
+        sphcoords(wi, &ti, &pi); sphcoords(woi, &toi, &poi);
+        float dt = ti - toi, dp = pi - poi;
+        if (dp < -PI_F) dp += 2.f * PI_F; else if (dp > PI_F) dp -= 2.f * PI_F;
+        C = v2(dt, dp);
+        float dti_du, dpi_du, dti_dv, dpi_dv, dtoi_du, dpoi_du, dtoi_dv, dpoi_dv;
+        d_sphcoords(wi, dwi_du, dwi_dv, &dti_du, &dpi_du, &dti_dv, &dpi_dv);
+        d_sphcoords(woi, dwoi_du, dwoi_dv, &dtoi_du, &dpoi_du, &dtoi_dv, &dpoi_dv);
+        dC = m2(dti_du - dtoi_du, dti_dv - dtoi_dv, dpi_du - dpoi_du, dpi_dv - dpoi_dv);
+        success = true;
+    }
+    float determinant = det2(dC);
+    if (fabsf(determinant) < 1e-6f) return r;
+    M2 inv = inverse2(dC);
+    r.ok = success; r.C = C; r.dX = mulv2(inv, C);
+    return r;
+}
+
+/* compute_step_halfvector (manifold_ss.cpp:407-492) */
+static Step step_halfvector(V3 v0p, const MV &v1, const EI &v2e, V3 n_offset) {
+    Step r; r.ok = false; r.C = v2(INF_F, INF_F); r.dX = v2(0.f, 0.f);
+    V3 wo = ei_is_directional(v2e) ? v2e.d : v2e.p - v1.p;
+    float ilo = norm(wo);
+    if (ilo < 1e-3f) return r;
+    ilo = 1.f / ilo; wo = wo * ilo;
+    V3 wi = v0p - v1.p;
+    float ili = norm(wi);
+    if (ili < 1e-3f) return r;
+    ili = 1.f / ili; wi = wi * ili;
+    float eta = v1.eta;
+    if (dot(wi, v1.gn) < 0.f) eta = 1.f / eta;
+    V3 h = wi + eta * wo;
+    if (eta != 1.f) h = h * -1.f;
+    float ilh = 1.f / norm(h);
+    h = h * ilh;
+    ilo *= eta * ilh; ili *= ilh;
+    V3 dh_du, dh_dv;
+    if (ei_is_directional(v2e)) {
+        dh_du = ili * (-v1.dp_du + wi * dot(wi, v1.dp_du));
+        dh_dv = ili * (-v1.dp_dv + wi * dot(wi, v1.dp_dv));
+    } else {
+        dh_du = (-v1.dp_du * (ili + ilo) + wi * (dot(wi, v1.dp_du) * ili)) + wo * (dot(wo, v1.dp_du) * ilo);
+        dh_dv = (-v1.dp_dv * (ili + ilo) + wi * (dot(wi, v1.dp_dv) * ili)) + wo * (dot(wo, v1.dp_dv) * ilo);
+    }
+    dh_du = dh_du - h * dot(dh_du, h);
+    dh_dv = dh_dv - h * dot(dh_dv, h);
+    if (eta != 1.f) { dh_du = dh_du * -1.f; dh_dv = dh_dv * -1.f; }
+    M2 dH = m2(dot(v1.ds_du, h) + dot(v1.s, dh_du), dot(v1.ds_dv, h) + dot(v1.s, dh_dv),
+               dot(v1.dt_du, h) + dot(v1.t, dh_du), dot(v1.dt_dv, h) + dot(v1.t, dh_dv));
+    float determinant = det2(dH);
+    if (fabsf(determinant) < 1e-6f) return r;
+    M2 inv = inverse2(dH);
+    V2 H = v2(dot(v1.s, h), dot(v1.t, h));
+    V2 dHv = H - v2(n_offset.x, n_offset.y);
+    r.ok = true; r.C = dHv; r.dX = mulv2(inv, dHv);
+    return r;
+}
+
+struct NewtonOut { bool ok; SI si; int iterations; };
+
+/* newton_solver SS (manifold_ss.cpp:329-405) */
+static NewtonOut newton_ss(const Scene &S, V3 si_p, const MV &vtx_init, const EI &ei, V3 n_offset) {
+    const sms_config &cfg = S.cfg;
+    MV vtx = vtx_init;
+    bool success = false;
+    int iterations = 0;
+    float beta = 1.f;
+    SI si_current = si_invalid();
+    while (iterations < cfg.max_iterations) {
+        Step st = cfg.halfvector_constraints ? step_halfvector(si_p, vtx, ei, n_offset)
+                                             : step_anglediff(si_p, vtx, ei, n_offset);
+        if (!st.ok) break;
+        if (norm2(st.C) < cfg.solver_threshold) { success = true; break; }
+        V3 p_prop = vtx.p - cfg.step_scale * beta * (vtx.dp_du * st.dX.x + vtx.dp_dv * st.dX.y);
+        V3 d_prop = normalize(p_prop - si_p);
+        Ray ray = make_ray(si_p, d_prop);
+        si_current = ray_intersect(S, ray);
+        if (!si_current.valid || vtx.shape != si_current.shape) {
+            beta = 0.5f * beta;
+            iterations += 1;
+            continue;
+        }
+        beta = std::min(1.f, 2.f * beta);
+        vtx = mv_from_si(S, si_current);
+        iterations++;
+    }
+    NewtonOut out; out.iterations = iterations; out.si = si_current;
+    if (!success) { out.ok = false; return out; }
+    V3 wx = normalize(si_p - vtx.p);
+    V3 wy = ei_is_directional(ei) ? ei.d : normalize(ei.p - vtx.p);
+    float cos_theta_x = dot(vtx.gn, wx), cos_theta_y = dot(vtx.gn, wy);
+    bool refraction = cos_theta_x * cos_theta_y < 0.f, reflection = !refraction;
+    if ((vtx.eta == 1.f && !reflection) || (vtx.eta != 1.f && !refraction)) { out.ok = false; return out; }
+    out.ok = true;
+    return out;
+}
+
+struct PathOut { bool ok; SI si_final, si_init; int iterations; };
+
+/* sample_path SS (manifold_ss.cpp:203-279), uniform seed (twostage not built) */
+static PathOut sample_path_ss(const Scene &S, int shape, V3 si_p, const EI &ei, V2 sample, V3 n_offset, V3 p_start) {
+    PathOut out; out.ok = false; out.si_final = si_invalid(); out.si_init = si_invalid(); out.iterations = 0;
+    PS ps = sample_position(S, shape, sample);
+    if (p_start.x != 0.f || p_start.y != 0.f || p_start.z != 0.f) ps.p = p_start;
+    V3 d_tmp = normalize(ps.p - si_p);
+    if (S.cfg.mnee_init) {
+        const Shape &sh = S.shapes[shape];
+        bool reflection = sh.k != 0.f;                 // any(imag(ior) != 0)
+        if (reflection) d_tmp = normalize(sh.bbox_center - si_p);
+        else d_tmp = normalize(ei.p - si_p);
+    }
+    Ray ray = make_ray(si_p, d_tmp);
+    SI si_init = ray_intersect(S, ray);
+    if (!si_init.valid || shape != si_init.shape) return out;
+    out.si_init = si_init;
+    MV vtx_init = mv_from_si(S, si_init);
+    NewtonOut nr = newton_ss(S, si_p, vtx_init, ei, n_offset);
+    out.iterations = nr.iterations;
+    out.si_final = nr.si;
+    out.ok = nr.ok && nr.si.valid;
+    return out;
+}
+
+/* specular_reflectance SS, smooth branch (manifold_ss.cpp:638-693) */
+static V3 specular_reflectance_ss(const Scene &S, const SI &si, V3 wo) {
+    if (!si.valid) return v3(0.f, 0.f, 0.f);
+    const Shape &sh = S.shapes[si.shape];
+    bool reflection = sh.k != 0.f;      // any(imag(ior) != 0); the all(imag == 0) sub-case is unreachable
+    V3 n = si.sh_n;       // BSDF::frame(si).n == sh_frame.n for non-normal-mapped shapes
+    float cos_theta = dot(n, wo);
+    float val;
+    if (reflection) {
+        val = fresnel_conductor(fabsf(cos_theta), sh.eta_r, sh.k);
+    } else {
+        float eta = hmean_same(sh.eta_r);
+        if (cos_theta < 0.f) eta = 1.f / eta;
+        float F = fresnel(cos_theta, eta);
+        val = 1.f - F;
+        val = val * (eta * eta);
+    }
+    return v3(val, val, val);
+}
+
+/* geometric_term SS (manifold_ss.cpp:695-857) */
+static float geometric_term_ss(const MV &v0, const MV &v1, const MV &v2v) {
+    V3 wi = v0.p - v1.p;
+    float ili = norm(wi);
+    if (ili < 1e-3f) return 0.f;
+    ili = 1.f / ili; wi = wi * ili;
+    V3 wo = v2v.p - v1.p;
+    float ilo = norm(wo);
+    if (ilo < 1e-3f) return 0.f;
+    ilo = 1.f / ilo; wo = wo * ilo;
+    M2 dc1_dx0 = m2(0, 0, 0, 0), dc2_dx1 = m2(0, 0, 0, 0), dc2_dx2 = m2(0, 0, 0, 0);
+    if (v2v.fixed_direction) {
+        V3 dc2_du1 = ilo * (v1.dp_du - wo * dot(wo, v1.dp_du)), dc2_dv1 = ilo * (v1.dp_dv - wo * dot(wo, v1.dp_dv));
+        dc2_dx1 = m2(dot(dc2_du1, v2v.dp_du), dot(dc2_dv1, v2v.dp_du), dot(dc2_du1, v2v.dp_dv), dot(dc2_dv1, v2v.dp_dv));
+        V3 dc2_du2 = -ilo * (v2v.dp_du - wo * dot(wo, v2v.dp_du)), dc2_dv2 = -ilo * (v2v.dp_dv - wo * dot(wo, v2v.dp_dv));
+        dc2_dx2 = m2(dot(dc2_du2, v2v.dp_du), dot(dc2_dv2, v2v.dp_du), dot(dc2_du2, v2v.dp_dv), dot(dc2_dv2, v2v.dp_dv));
+    }
+    float eta = v1.eta;
+    if (dot(wi, v1.gn) < 0.f) eta = 1.f / eta;
+    V3 h = wi + eta * wo;
+    if (eta != 1.f) h = h * -1.f;
+    float ilh = 1.f / norm(h);
+    h = h * ilh;
+    ilo *= eta * ilh; ili *= ilh;
+    float dot_dpdu_n = dot(v1.dp_du, v1.n), dot_dpdv_n = dot(v1.dp_dv, v1.n);
+    V3 s = v1.dp_du - dot_dpdu_n * v1.n, t = v1.dp_dv - dot_dpdv_n * v1.n;
+    V3 dh_du, dh_dv;
+    if (v2v.fixed_direction) {
+        dh_du = ili * (v0.dp_du - wi * dot(wi, v0.dp_du));
+        dh_dv = ili * (v0.dp_dv - wi * dot(wi, v0.dp_dv));
+        dh_du = dh_du - h * dot(dh_du, h); dh_dv = dh_dv - h * dot(dh_dv, h);
+        if (eta != 1.f) { dh_du = dh_du * -1.f; dh_dv = dh_dv * -1.f; }
+        dc1_dx0 = m2(dot(dh_du, s), dot(dh_dv, s), dot(dh_du, t), dot(dh_dv, t));
+    }
+    dh_du = (-v1.dp_du * (ili + ilo) + wi * (dot(wi, v1.dp_du) * ili)) + wo * (dot(wo, v1.dp_du) * ilo);
+    dh_dv = (-v1.dp_dv * (ili + ilo) + wi * (dot(wi, v1.dp_dv) * ili)) + wo * (dot(wo, v1.dp_dv) * ilo);
+    dh_du = dh_du - h * dot(dh_du, h); dh_dv = dh_dv - h * dot(dh_dv, h);
+    if (eta != 1.f) { dh_du = dh_du * -1.f; dh_dv = dh_dv * -1.f; }
+    float dot_h_n = dot(h, v1.n), dot_h_dndu = dot(h, v1.dn_du), dot_h_dndv = dot(h, v1.dn_dv);
+    M2 dc1_dx1 = m2((dot(dh_du, s) - dot(v1.dp_du, v1.dn_du) * dot_h_n) - dot_dpdu_n * dot_h_dndu,
+                    (dot(dh_dv, s) - dot(v1.dp_du, v1.dn_dv) * dot_h_n) - dot_dpdu_n * dot_h_dndv,
+                    (dot(dh_du, t) - dot(v1.dp_dv, v1.dn_du) * dot_h_n) - dot_dpdv_n * dot_h_dndu,
+                    (dot(dh_dv, t) - dot(v1.dp_dv, v1.dn_dv) * dot_h_n) - dot_dpdv_n * dot_h_dndv);
+    dh_du = ilo * (v2v.dp_du - wo * dot(wo, v2v.dp_du));
+    dh_dv = ilo * (v2v.dp_dv - wo * dot(wo, v2v.dp_dv));
+    dh_du = dh_du - h * dot(dh_du, h); dh_dv = dh_dv - h * dot(dh_dv, h);
+    if (eta != 1.f) { dh_du = dh_du * -1.f; dh_dv = dh_dv * -1.f; }
+    M2 dc1_dx2 = m2(dot(dh_du, s), dot(dh_dv, s), dot(dh_du, t), dot(dh_dv, t));
+    if (v2v.fixed_direction) {
+        if (fabsf(det2(dc2_dx2)) < 1e-6f) return 0.f;
+        M2 Li = inverse2(dc2_dx2);
+        M2 tmp = mulm2(Li, dc2_dx1);
+        M2 m = subm2(dc1_dx1, mulm2(dc1_dx2, tmp));
+        if (fabsf(det2(m)) < 1e-6f) return 0.f;
+        Li = inverse2(m);
+        M2 sol1 = mulm2(negm2(Li), dc1_dx0);
+        M2 sol0 = mulm2(negm2(tmp), sol1);
+        float G = fabsf(det2(negm2(sol0)));
+        G = std::min(G, 10.f);
+        G /= fabsf(dot(wi, v0.n));
+        G *= eta * eta;
+        return G;
+    }
+    if (fabsf(det2(dc1_dx1)) < 1e-6f) return 0.f;
+    M2 inv = inverse2(dc1_dx1);
+    float dx1_dx2 = fabsf(det2(mulm2(inv, dc1_dx2)));
+    dx1_dx2 = std::min(dx1_dx2, 1.f);
+    V3 d = v0.p - v1.p;
+    float inv_r2 = 1.f / sqnorm(d);
+    d = d * sqrtf(inv_r2);
+    float dw0_dx1 = fabsf(dot(d, v1.gn)) * inv_r2;
+    return dw0_dx1 * dx1_dx2;
+}
+
+/* evaluate_path_contribution SS (manifold_ss.cpp:281-327) */
+static V3 path_contribution_ss(const Scene &S, const SI &si, const EI &ei_, const SI &si_final) {
+    MV vtx = mv_from_si(S, si_final);
+    make_orthonormal(vtx);
+    EI ei = ei_;
+    if (ei.kind == SMS_EI_POINT) {
+        const Emitter &em = S.emitters[ei.emitter];
+        V3 d = em.position - si_final.p;
+        float dist = norm(d), inv_dist = 1.f / dist;
+        d = d * inv_dist;
+        float emitter_pdf = ei.pdf;
+        ei.p = em.position; ei.d = d; ei.n = d; ei.pdf = 1.f;
+        ei.weight = (em.intensity * (inv_dist * inv_dist)) * dist * dist;
+        ei.pdf *= emitter_pdf;
+        ei.weight = ei.weight * (1.f / emitter_pdf);
+    }
+    MV vy;
+    if (!emitter_interaction_to_vertex(S, ei, vtx.p, &vy)) return v3(0.f, 0.f, 0.f);
+    MV vx = mv_from_si(S, si);
+    make_orthonormal(vx);
+    V3 thr = specular_reflectance_ss(S, si_final, normalize(ei.p - si_final.p));
+    float G = geometric_term_ss(vx, vtx, vy);
+    thr = thr * G;
+    return v3(thr.x * ei.weight.x, thr.y * ei.weight.y, thr.z * ei.weight.z);
+}
+
+/* Diffuse::eval (src/bsdfs/diffuse.cpp:104-120) at the receiver */
+static V3 receiver_bsdf_eval(const Scene &S, const SI &si, V3 wo_local) {
+    const Shape &sh = S.shapes[si.shape];
+    if (sh.bsdf != SMS_BSDF_DIFFUSE) return v3(0.f, 0.f, 0.f);
+    float ci = si.wi.z, co = wo_local.z;
+    if (!(ci > 0.f && co > 0.f)) return v3(0.f, 0.f, 0.f);
+    return (sh.refl * INV_PI_F) * co;
+}
+
+static SI si_from_query(const sms_ss_query &q) {
+    SI si = si_invalid();
+    si.valid = true; si.t = 1.f;
+    si.p = v3(q.p[0], q.p[1], q.p[2]); si.n = v3(q.n[0], q.n[1], q.n[2]);
+    si.sh_n = v3(q.sh_n[0], q.sh_n[1], q.sh_n[2]); si.sh_s = v3(q.sh_s[0], q.sh_s[1], q.sh_s[2]);
+    si.sh_t = v3(q.sh_t[0], q.sh_t[1], q.sh_t[2]); si.wi = v3(q.wi[0], q.wi[1], q.wi[2]);
+    si.dp_du = si.sh_s; si.dp_dv = si.sh_t;
+    si.shape = q.shape;
+    return si;
+}
+
+struct Stats { uint64_t walks, conv, iters, bcalls, btrials, bmax; };
+
+/* specular_manifold_sampling SS (manifold_ss.cpp:28-201), per-(query, walk) PCG32 streams */
+static V3 estimate_ss(const Scene &S, const sms_ss_query &q, uint64_t seed, uint64_t qid, Stats &st) {
+    const sms_config &cfg = S.cfg;
+    SI si = si_from_query(q);
+    Pcg32 rq; stream_seed(&rq, seed, qid, 0);
+    EI ei = sample_emitter_interaction(S, si.p, S.emitters_ss, &rq);
+    V3 result = v3(0.f, 0.f, 0.f);
+    if (S.casters_ss.empty()) return result;
+    V3 n_offset = v3(0.f, 0.f, 1.f);
+    for (size_t c = 0; c < S.casters_ss.size(); ++c) {
+        int shape = S.casters_ss[c];
+        uint64_t wbase = ((uint64_t) c << 32);
+        V3 value = v3(0.f, 0.f, 0.f);
+        auto run_walk = [&](uint64_t k) {
+            Pcg32 rw; stream_seed(&rw, seed, qid, wbase + k + 1);
+            V2 s; s.x = pcg32_float(&rw); s.y = pcg32_float(&rw);
+            PathOut po = sample_path_ss(S, shape, si.p, ei, s, n_offset, v3(0.f, 0.f, 0.f));
+            st.walks++; st.iters += po.iterations; if (po.ok) st.conv++;
+            return po;
+        };
+        auto visible = [&](const SI &sf) {
+            V3 d = sf.p - ei.p;
+            float dist = norm(d);
+            d = d * (1.f / dist);
+            Ray rv = make_ray(ei.p, d, RAY_EPS_F * (1.f + hmax_abs(ei.p)), dist * (1.f - RAY_EPS_F));
+            return !ray_test(S, rv);
+        };
+        if (!cfg.biased) {
+            PathOut po = run_walk(0);
+            if (!po.ok) continue;
+            V3 direction = normalize(po.si_final.p - si.p);
+            if (!visible(po.si_final)) continue;
+            V3 specular_val = path_contribution_ss(S, si, ei, po.si_final);
+            V3 dl = v3(dot(direction, si.sh_s), dot(direction, si.sh_t), dot(direction, si.sh_n));
+            V3 bsdf_val = receiver_bsdf_eval(S, si, dl);
+            float inv_prob = 1.f;
+            int iterations = 1;
+            uint64_t k = 1;
+            st.bcalls++;
+            while (true) {
+                PathOut pt = run_walk(k++);
+                V3 dtr = normalize(pt.si_final.p - si.p);
+                if (pt.ok && fabsf(dot(direction, dtr) - 1.f) < cfg.uniqueness_threshold) break;
+                inv_prob += 1.f;
+                iterations++;
+                if (cfg.max_trials > 0 && iterations > cfg.max_trials) { inv_prob = 0.f; break; }
+            }
+            st.btrials += iterations;
+            if ((uint64_t) iterations > st.bmax) st.bmax = iterations;
+            value = v3(bsdf_val.x * specular_val.x * inv_prob, bsdf_val.y * specular_val.y * inv_prob,
+                       bsdf_val.z * specular_val.z * inv_prob);
+        } else {
+            std::vector<V3> solutions;
+            for (int m = 0; m < cfg.max_trials; ++m) {
+                PathOut po = run_walk((uint64_t) m);
+                if (!po.ok) continue;
+                V3 direction = normalize(po.si_final.p - si.p);
+                if (!visible(po.si_final)) continue;
+                bool dup = false;
+                for (size_t k = 0; k < solutions.size(); ++k)
+                    if (fabsf(dot(direction, solutions[k]) - 1.f) < cfg.uniqueness_threshold) { dup = true; break; }
+                if (dup) continue;
+                solutions.push_back(direction);
+                V3 dl = v3(dot(direction, si.sh_s), dot(direction, si.sh_t), dot(direction, si.sh_n));
+                V3 bsdf_val = receiver_bsdf_eval(S, si, dl);
+                V3 specular_val = path_contribution_ss(S, si, ei, po.si_final);
+                value = value + v3(bsdf_val.x * specular_val.x, bsdf_val.y * specular_val.y, bsdf_val.z * specular_val.z);
+            }
+        }
+        result = result + value / 1.f;
+    }
+    return result;
+}
+
+/* ================================================================ glints */
+static inline bool inside_triangle(V2 p, V2 a, V2 b, V2 c) {
+    V2 v0 = c - a, v1 = b - a, v2_ = p - a;
+    float dot00 = dot2(v0, v0), dot01 = dot2(v0, v1), dot02 = dot2(v0, v2_), dot11 = dot2(v1, v1), dot12 = dot2(v1, v2_);
+    float inv_denom = 1.f / (dot00 * dot11 - dot01 * dot01);
+    float u = (dot11 * dot02 - dot01 * dot12) * inv_denom, v = (dot00 * dot12 - dot01 * dot02) * inv_denom;
+    return (u >= 0.f) && (v >= 0.f) && (u + v < 1.f);
+}
+
+struct GlintSI { V3 p, gn, n, s, t, dp_du, dp_dv, wi; V2 uv, duv_dx, duv_dy; int shape; };
+
+/* newton_solver + compute_step glints (manifold_glints.cpp:296-373) */
+static bool newton_glint(const Scene &S, const GlintSI &si, V2 target, V2 uv_init, V2 *uv_out, int *iters) {
+    const sms_config &cfg = S.cfg;
+    const Shape &sh = S.shapes[si.shape];
+    const Normalmap &nm = S.nmaps[sh.normalmap];
+    V2 uv = uv_init;
+    V2 fp_a = (si.uv + 0.5f * si.duv_dx) + 0.5f * si.duv_dy, fp_b = (si.uv + 0.5f * si.duv_dx) - 0.5f * si.duv_dy,
+       fp_c = (si.uv - 0.5f * si.duv_dx) - 0.5f * si.duv_dy, fp_d = (si.uv - 0.5f * si.duv_dx) + 0.5f * si.duv_dy;
+    bool success = false;
+    int iterations = 0;
+    float beta = 1.f;
+    while (iterations < cfg.max_iterations) {
+        V2 tuv = tile_uv(uv, sh.tiles);
+        Texel4 tx = nm_fetch(nm, tuv);
+        V2 slope = nm_slope(tx);
+        V2 tu, tv; nm_slope_deriv(tx, (float) nm.res, &tu, &tv);
+        V2 du = tu * sh.tiles, dv = tv * sh.tiles;
+        V2 dS = slope - target;
+        M2 J = m2(du.x, dv.x, du.y, dv.y);
+        if (fabsf(det2(J)) < 1e-6f) break;
+        V2 step = mulv2(inverse2(J), dS);
+        if (norm2(dS) < cfg.solver_threshold) { success = true; break; }
+        V2 uv_prop = uv - cfg.step_scale * beta * step;
+        if (!(inside_triangle(uv_prop, fp_a, fp_b, fp_c) || inside_triangle(uv_prop, fp_a, fp_c, fp_d))) {
+            beta = 0.5f * beta;
+            iterations++;
+            continue;
+        }
+        beta = std::min(1.f, 2.f * beta);
+        uv = uv_prop;
+        iterations++;
+    }
+    *uv_out = uv; *iters = iterations;
+    return success;
+}
+
+static inline V3 to_local(const GlintSI &si, V3 v) { return v3(dot(v, si.s), dot(v, si.t), dot(v, si.n)); }
+
+/* sample_glint (manifold_glints.cpp:225-265) */
+static bool sample_glint(const Scene &S, V3 sensor_p, V3 ei_p, const GlintSI &si, V2 sample, V3 n_offset,
+                         V2 xi_start, V2 *uv_final, V2 *uv_init, int *iters) {
+    *iters = 0;
+    V3 wi = to_local(si, normalize(sensor_p - si.p)), wo = to_local(si, normalize(ei_p - si.p));
+    if (wi.z <= 0.f || wo.z <= 0.f) { *uv_final = v2(0.f, 0.f); *uv_init = v2(0.f, 0.f); return false; }
+    V3 h = normalize(wi + wo);
+    V2 h_s = v2(-h.x / h.z, -h.y / h.z), o_s = v2(-n_offset.x / n_offset.z, -n_offset.y / n_offset.z);
+    V2 target = h_s + o_s;
+    V2 fp = sample;
+    if (xi_start.x != 1.f || xi_start.y != 1.f) fp = xi_start;
+    V2 duv = (fp.x - 0.5f) * si.duv_dx + (fp.y - 0.5f) * si.duv_dy;
+    *uv_init = si.uv + duv;
+    return newton_glint(S, si, target, *uv_init, uv_final, iters);
+}
+
+/* specular_reflectance glints, smooth branch (manifold_glints.cpp:375-409) */
+static float specular_reflectance_glint(const Shape &sh, V3 n, V3 wo) {
+    float cos_theta = dot(n, wo);
+    if (sh.k == 0.f) return fresnel(fabsf(cos_theta), sh.eta_r);
+    return fresnel_conductor(fabsf(cos_theta), sh.eta_r, sh.k);
+}
+
+/* geometric_term glints (manifold_glints.cpp:411-554) */
+static float geometric_term_glint(const MV &v0, const MV &v1, const MV &v2v) {
+    V3 wi = v0.p - v1.p;
+    float ili = norm(wi);
+    if (ili < 1e-3f) return 0.f;
+    ili = 1.f / ili; wi = wi * ili;
+    V3 wo = v2v.p - v1.p;
+    float ilo = norm(wo);
+    if (ilo < 1e-3f) return 0.f;
+    ilo = 1.f / ilo; wo = wo * ilo;
+    M2 dc1_dx0 = m2(0, 0, 0, 0), dc2_dx1 = m2(0, 0, 0, 0), dc2_dx2 = m2(0, 0, 0, 0);
+    if (v2v.fixed_direction) {
+        V3 dc2_du1 = ilo * (v1.dp_du - wo * dot(wo, v1.dp_du)), dc2_dv1 = ilo * (v1.dp_dv - wo * dot(wo, v1.dp_dv));
+        dc2_dx1 = m2(dot(dc2_du1, v2v.dp_du), dot(dc2_dv1, v2v.dp_du), dot(dc2_du1, v2v.dp_dv), dot(dc2_dv1, v2v.dp_dv));
+        V3 dc2_du2 = -ilo * (v2v.dp_du - wo * dot(wo, v2v.dp_du)), dc2_dv2 = -ilo * (v2v.dp_dv - wo * dot(wo, v2v.dp_dv));
+        dc2_dx2 = m2(dot(dc2_du2, v2v.dp_du), dot(dc2_dv2, v2v.dp_du), dot(dc2_du2, v2v.dp_dv), dot(dc2_dv2, v2v.dp_dv));
+    }
+    V3 h = wi + wo;
+    float ilh = 1.f / norm(h);
+    h = h * ilh;
+    ilo *= ilh; ili *= ilh;
+    float dot_dpdu_n = dot(v1.dp_du, v1.n), dot_dpdv_n = dot(v1.dp_dv, v1.n);
+    V3 s = v1.dp_du - dot_dpdu_n * v1.n, t = v1.dp_dv - dot_dpdv_n * v1.n;
+    V3 dh_du, dh_dv;
+    if (v2v.fixed_direction) {
+        dh_du = ili * (v0.dp_du - wi * dot(wi, v0.dp_du));
+        dh_dv = ili * (v0.dp_dv - wi * dot(wi, v0.dp_dv));
+        dh_du = dh_du - h * dot(dh_du, h); dh_dv = dh_dv - h * dot(dh_dv, h);
+        dc1_dx0 = m2(dot(dh_du, s), dot(dh_dv, s), dot(dh_du, t), dot(dh_dv, t));
+    }
+    dh_du = (-v1.dp_du * (ili + ilo) + wi * (dot(wi, v1.dp_du) * ili)) + wo * (dot(wo, v1.dp_du) * ilo);
+    dh_dv = (-v1.dp_dv * (ili + ilo) + wi * (dot(wi, v1.dp_dv) * ili)) + wo * (dot(wo, v1.dp_dv) * ilo);
+    dh_du = dh_du - h * dot(dh_du, h); dh_dv = dh_dv - h * dot(dh_dv, h);
+    float dot_h_n = dot(h, v1.n), dot_h_dndu = dot(h, v1.dn_du), dot_h_dndv = dot(h, v1.dn_dv);
+    M2 dc1_dx1 = m2((dot(dh_du, s) - dot(v1.dp_du, v1.dn_du) * dot_h_n) - dot_dpdu_n * dot_h_dndu,
+                    (dot(dh_dv, s) - dot(v1.dp_du, v1.dn_dv) * dot_h_n) - dot_dpdu_n * dot_h_dndv,
+                    (dot(dh_du, t) - dot(v1.dp_dv, v1.dn_du) * dot_h_n) - dot_dpdv_n * dot_h_dndu,
+                    (dot(dh_dv, t) - dot(v1.dp_dv, v1.dn_dv) * dot_h_n) - dot_dpdv_n * dot_h_dndv);
+    dh_du = ilo * (v2v.dp_du - wo * dot(wo, v2v.dp_du));
+    dh_dv = ilo * (v2v.dp_dv - wo * dot(wo, v2v.dp_dv));
+    dh_du = dh_du - h * dot(dh_du, h); dh_dv = dh_dv - h * dot(dh_dv, h);
+    M2 dc1_dx2 = m2(dot(dh_du, s), dot(dh_dv, s), dot(dh_du, t), dot(dh_dv, t));
+    float G = 0.f;
+    if (v2v.fixed_direction) {
+        if (fabsf(det2(dc2_dx2)) < 1e-6f) return 0.f;
+        M2 Li = inverse2(dc2_dx2);
+        M2 tmp = mulm2(Li, dc2_dx1);
+        M2 m = subm2(dc1_dx1, mulm2(dc1_dx2, tmp));
+        if (fabsf(det2(m)) < 1e-6f) return 0.f;
+        Li = inverse2(m);
+        M2 sol1 = mulm2(negm2(Li), dc1_dx0);
+        M2 sol0 = mulm2(negm2(tmp), sol1);
+        G = fabsf(det2(negm2(sol0)));
+    } else {
+        if (fabsf(det2(dc1_dx1)) < 1e-6f) return 0.f;
+        M2 inv = inverse2(dc1_dx1);
+        float dx1_dx2 = fabsf(det2(mulm2(inv, dc1_dx2)));
+        V3 d = v0.p - v1.p;
+        float inv_r2 = 1.f / sqnorm(d);
+        d = d * sqrtf(inv_r2);
+        float dw0_dx1 = fabsf(dot(d, v1.gn)) * inv_r2;
+        G = dw0_dx1 * dx1_dx2;
+    }
+    G = std::min(1e-5f, G);
+    return G;
+}
+
+/* evaluate_glint_contribution (manifold_glints.cpp:267-294) */
+static V3 glint_contribution(const Scene &S, V3 sensor_p, const EI &ei, const GlintSI &gsi, V2 uv) {
+    MV vy;
+    if (!emitter_interaction_to_vertex(S, ei, gsi.p, &vy)) return v3(0.f, 0.f, 0.f);
+    MV vx = mv_point(sensor_p);
+    V3 dx = normalize(gsi.p - sensor_p);
+    vx.n = dx; vx.gn = dx;
+    coordinate_system(vx.n, &vx.dp_du, &vx.dp_dv);
+    SI si = si_invalid();
+    si.valid = true; si.p = gsi.p; si.n = gsi.gn; si.sh_n = gsi.n; si.sh_s = gsi.s; si.sh_t = gsi.t;
+    si.dp_du = gsi.dp_du; si.dp_dv = gsi.dp_dv; si.uv = uv; si.shape = gsi.shape;
+    MV vtx = mv_from_si(S, si);
+    make_orthonormal(vtx);
+    float F = specular_reflectance_glint(S.shapes[gsi.shape], vtx.n, normalize(ei.p - gsi.p));
+    float G = geometric_term_glint(vx, vtx, vy);
+    float v = F * G;
+    return v3(v, v, v);
+}
+
+static GlintSI gsi_from_query(const sms_glint_query &q) {
+    GlintSI g;
+    g.p = v3(q.p[0], q.p[1], q.p[2]); g.gn = v3(q.n[0], q.n[1], q.n[2]); g.n = v3(q.sh_n[0], q.sh_n[1], q.sh_n[2]);
+    g.s = v3(q.sh_s[0], q.sh_s[1], q.sh_s[2]); g.t = v3(q.sh_t[0], q.sh_t[1], q.sh_t[2]);
+    g.dp_du = v3(q.dp_du[0], q.dp_du[1], q.dp_du[2]); g.dp_dv = v3(q.dp_dv[0], q.dp_dv[1], q.dp_dv[2]);
+    g.wi = v3(q.wi[0], q.wi[1], q.wi[2]);
+    g.uv = v2(q.uv[0], q.uv[1]); g.duv_dx = v2(q.duv_dx[0], q.duv_dx[1]); g.duv_dy = v2(q.duv_dy[0], q.duv_dy[1]);
+    g.shape = q.shape;
+    return g;
+}
+
+/* sensor importance We * film area (manifold_glints.cpp:146-153; perspective.cpp:224-276) */
+static float sensor_we(const Scene &S, V3 sensor_p, V3 p) {
+    const Camera &c = S.cam;
+    V3 d = normalize(p - sensor_p);
+    V3 dl = v3(dot(d, c.left), dot(d, c.up), dot(d, c.dir));
+    float cos_theta = dl.z;
+    if (!(cos_theta > 0.f)) return 0.f;
+    float inv_cos = 1.f / cos_theta;
+    float px = dl.x * inv_cos, py = dl.y * inv_cos;
+    if (!(px >= -c.rect_x && px <= c.rect_x && py >= -c.rect_y && py <= c.rect_y)) return 0.f;
+    float We = c.normalization * inv_cos * inv_cos * inv_cos;
+    return We * (float) (c.w * c.h);
+}
+
+/* SMS strategy of the glint estimator (manifold_glints.cpp:28-181) */
+static V3 estimate_glint(const Scene &S, const sms_glint_query &q, uint64_t seed, uint64_t qid, Stats &st) {
+    const sms_config &cfg = S.cfg;
+    V3 result = v3(0.f, 0.f, 0.f);
+    if (q.shape < 0) return result;
+    GlintSI si = gsi_from_query(q);
+    V3 sensor_p = v3(q.sensor_p[0], q.sensor_p[1], q.sensor_p[2]);
+    Pcg32 rq; stream_seed(&rq, seed, qid, 0);
+    EI ei = sample_emitter_interaction(S, si.p, S.emitters_all, &rq);
+    V3 edir = ei.p - si.p;
+    float edist = norm(edir);
+    edir = edir * (1.f / edist);
+    Ray re = make_ray(si.p, edir, RAY_EPS_F * (1.f + hmax_abs(si.p)), edist * (1.f - RAY_EPS_F));
+    if (ray_test(S, re)) return result;
+    V3 n_offset = v3(0.f, 0.f, 1.f);
+    bool sms_success = false;
+    V3 sms_weight = v3(0.f, 0.f, 0.f);
+    auto walk = [&](uint64_t k, V2 *uvf) {
+        Pcg32 rw; stream_seed(&rw, seed, qid, k + 1);
+        V2 s; s.x = pcg32_float(&rw); s.y = pcg32_float(&rw);
+        V2 uvi; int it;
+        bool ok = sample_glint(S, sensor_p, ei.p, si, s, n_offset, v2(1.f, 1.f), uvf, &uvi, &it);
+        st.walks++; st.iters += it; if (ok) st.conv++;
+        return ok;
+    };
+    if (!cfg.biased) {
+        V2 uv_final;
+        if (walk(0, &uv_final)) {
+            sms_success = true;
+            V3 specular_val = glint_contribution(S, sensor_p, ei, si, uv_final);
+            float inv_prob = 1.f;
+            int iterations = 1;
+            uint64_t k = 1;
+            st.bcalls++;
+            while (true) {
+                V2 uvt;
+                bool ok = walk(k++, &uvt);
+                if (ok && norm2(uvt - uv_final) < cfg.uniqueness_threshold) break;
+                inv_prob += 1.f;
+                iterations++;
+                if (cfg.max_trials > 0 && iterations > cfg.max_trials) { inv_prob = 0.f; break; }
+            }
+            st.btrials += iterations;
+            if ((uint64_t) iterations > st.bmax) st.bmax = iterations;
+            sms_weight = specular_val * inv_prob;
+        }
+    } else {
+        std::vector<V2> sols;
+        for (int m = 0; m < cfg.max_trials; ++m) {
+            V2 uvf;
+            if (!walk((uint64_t) m, &uvf)) continue;
+            bool dup = false;
+            for (size_t k = 0; k < sols.size(); ++k) if (norm2(uvf - sols[k]) < cfg.uniqueness_threshold) { dup = true; break; }
+            if (dup) continue;
+            sols.push_back(uvf);
+            sms_weight = sms_weight + glint_contribution(S, sensor_p, ei, si, uvf);
+        }
+        sms_success = !sols.empty();
+    }
+    if (sms_success) {
+        float We = sensor_we(S, sensor_p, si.p);
+        float p_sms = 1.f;
+        /* p_bsdf = lean_pdf + 1 is not built yet: the MIS weight is taken as 1 (sms_strategy_only) */
+        float mis = 1.f;
+        if (cfg.bsdf_strategy_only) mis = 0.f;
+        (void) p_sms;
+        float f = mis * We;
+        result = result + v3(f * sms_weight.x * ei.weight.x, f * sms_weight.y * ei.weight.y, f * sms_weight.z * ei.weight.z);
+    }
+    return result;
+}
+
+/* Camera ray + compute_partials for one pixel sample of a glint frame
+   (integrator.cpp:243-262, perspective.cpp:191-221, interaction.h:216-248). */
+static void make_glint_query(const Scene &S, int64_t idx, int spp, uint64_t seed, sms_glint_query *q) {
+    memset(q, 0, sizeof(*q));
+    q->shape = -1;
+    const Camera &c = S.cam;
+    int64_t pixel = idx / spp;
+    int px = (int) (pixel % c.w), py = (int) (pixel / c.w);
+    q->pixel = (int32_t) pixel;
+    Pcg32 r; stream_seed(&r, seed, (uint64_t) idx, 0xFFFFFFFFull);
+    float sx = pcg32_float(&r), sy = pcg32_float(&r);
+    V2 pos = v2(((float) px + sx) / (float) c.w, ((float) py + sy) / (float) c.h);
+    V3 near_p = v3((1.f - 2.f * pos.x) * c.tan_half, (1.f - 2.f * pos.y) * c.rect_y, 1.f);
+    V3 dloc = normalize(near_p);
+    V3 d = to_world(c.left, c.up, c.dir, dloc);
+    V3 dxl = normalize(near_p + c.dx), dyl = normalize(near_p + c.dy);
+    V3 d_x = to_world(c.left, c.up, c.dir, dxl), d_y = to_world(c.left, c.up, c.dir, dyl);
+    float scale = 1.f / sqrtf((float) spp);
+    d_x = fmadd3(d_x - d, scale, d); d_y = fmadd3(d_y - d, scale, d);
+    Ray ray = make_ray(c.o, d, 1e-2f * (1.f / dloc.z), 1e4f * (1.f / dloc.z));
+    SI si = ray_intersect(S, ray);
+    for (int a = 0; a < 3; ++a) (&q->sensor_p[0])[a] = (&c.o.x)[a];
+    if (!si.valid) return;
+    const Shape &sh = S.shapes[si.shape];
+    if (!(sh.flags & SMS_FLAG_GLINTY)) return;
+    /* compute_partials (interaction.h:216-248); o_x = o_y = o */
+    float dd = dot(si.n, si.p);
+    float t_x = (dd - dot(si.n, c.o)) / dot(si.n, d_x), t_y = (dd - dot(si.n, c.o)) / dot(si.n, d_y);
+    V3 dp_dx = fmadd3(d_x, t_x, c.o) - si.p, dp_dy = fmadd3(d_y, t_y, c.o) - si.p;
+    float a00 = dot(si.dp_du, si.dp_du), a01 = dot(si.dp_du, si.dp_dv), a11 = dot(si.dp_dv, si.dp_dv),
+          inv_det = 1.f / (a00 * a11 - a01 * a01);
+    float b0x = dot(si.dp_du, dp_dx), b1x = dot(si.dp_dv, dp_dx), b0y = dot(si.dp_du, dp_dy), b1y = dot(si.dp_dv, dp_dy);
+    if (!isfinite(inv_det)) inv_det = 0.f;
+    V2 duv_dx = v2(fmaf(a11, b0x, -(a01 * b1x)) * inv_det, fmaf(a00, b1x, -(a01 * b0x)) * inv_det);
+    V2 duv_dy = v2(fmaf(a11, b0y, -(a01 * b1y)) * inv_det, fmaf(a00, b1y, -(a01 * b0y)) * inv_det);
+    q->shape = si.shape;
+    for (int a = 0; a < 3; ++a) {
+        q->p[a] = (&si.p.x)[a]; q->n[a] = (&si.n.x)[a]; q->sh_n[a] = (&si.sh_n.x)[a]; q->sh_s[a] = (&si.sh_s.x)[a];
+        q->sh_t[a] = (&si.sh_t.x)[a]; q->dp_du[a] = (&si.dp_du.x)[a]; q->dp_dv[a] = (&si.dp_dv.x)[a]; q->wi[a] = (&si.wi.x)[a];
+    }
+    q->uv[0] = si.uv.x; q->uv[1] = si.uv.y;
+    q->duv_dx[0] = duv_dx.x; q->duv_dx[1] = duv_dx.y; q->duv_dy[0] = duv_dy.x; q->duv_dy[1] = duv_dy.y;
+}
+
+template <typename F>
+static void parallel_for(int64_t n, int threads, F f) {
+    if (threads <= 1 || n < 64) { for (int64_t i = 0; i < n; ++i) f(i); return; }
+    std::atomic<int64_t> next(0);
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; ++t)
+        pool.emplace_back([&]() {
+            while (true) {
+                int64_t b = next.fetch_add(64);
+                if (b >= n) break;
+                int64_t e = std::min(n, b + 64);
+                for (int64_t i = b; i < e; ++i) f(i);
+            }
+        });
+    for (auto &th : pool) th.join();
+}
+
+static void stats_out(const Stats &s, sms_stats *o) {
+    if (!o) return;
+    memset(o, 0, sizeof(*o));
+    o->walks = s.walks; o->walks_converged = s.conv; o->newton_iterations = s.iters;
+    o->bernoulli_calls = s.bcalls; o->bernoulli_trials = s.btrials; o->bernoulli_max = s.bmax;
+}
+
+} // namespace
+
+/* ================================================================ C entry points */
+extern "C" {
+
+void *oracle_create(const sms_scene_desc *desc, const sms_config *cfg) { return scene_create(desc, cfg); }
+void oracle_destroy(void *h) { delete (Scene *) h; }
+void oracle_set_config(void *h, const sms_config *cfg) { ((Scene *) h)->cfg = *cfg; }
+
+int oracle_ss_walk_batch(void *h, const sms_ss_walk_in *in, int64_t n, sms_ss_walk_out *out, int threads) {
+    const Scene &S = *(Scene *) h;
+    parallel_for(n, threads, [&](int64_t i) {
+        const sms_ss_walk_in &w = in[i];
+        EI ei = ei_from_c(w.ei);
+        PathOut po = sample_path_ss(S, w.shape, v3(w.si_p[0], w.si_p[1], w.si_p[2]), ei, v2(w.sample[0], w.sample[1]),
+                                    v3(w.n_offset[0], w.n_offset[1], w.n_offset[2]),
+                                    v3(w.p_start[0], w.p_start[1], w.p_start[2]));
+        sms_ss_walk_out &o = out[i];
+        memset(&o, 0, sizeof(o));
+        o.success = po.ok; o.iterations = po.iterations;
+        o.final_shape = po.si_final.valid ? po.si_final.shape : -1;
+        o.init_valid = po.si_init.valid;
+        for (int a = 0; a < 3; ++a) {
+            o.p[a] = (&po.si_final.p.x)[a]; o.n[a] = (&po.si_final.sh_n.x)[a]; o.p_init[a] = (&po.si_init.p.x)[a];
+        }
+    });
+    return 0;
+}
+
+int oracle_glint_walk_batch(void *h, const sms_glint_walk_in *in, int64_t n, sms_glint_walk_out *out, int threads) {
+    const Scene &S = *(Scene *) h;
+    parallel_for(n, threads, [&](int64_t i) {
+        const sms_glint_walk_in &w = in[i];
+        GlintSI g;
+        g.p = v3(w.si_p[0], w.si_p[1], w.si_p[2]);
+        g.s = v3(w.frame_s[0], w.frame_s[1], w.frame_s[2]); g.t = v3(w.frame_t[0], w.frame_t[1], w.frame_t[2]);
+        g.n = v3(w.frame_n[0], w.frame_n[1], w.frame_n[2]); g.gn = g.n;
+        g.uv = v2(w.uv[0], w.uv[1]); g.duv_dx = v2(w.duv_dx[0], w.duv_dx[1]); g.duv_dy = v2(w.duv_dy[0], w.duv_dy[1]);
+        g.shape = w.shape; g.dp_du = g.s; g.dp_dv = g.t; g.wi = v3(0.f, 0.f, 1.f);
+        V2 uvf, uvi; int it;
+        bool ok = sample_glint(S, v3(w.sensor_p[0], w.sensor_p[1], w.sensor_p[2]), v3(w.ei_p[0], w.ei_p[1], w.ei_p[2]), g,
+                               v2(w.sample[0], w.sample[1]), v3(w.n_offset[0], w.n_offset[1], w.n_offset[2]),
+                               v2(w.xi_start[0], w.xi_start[1]), &uvf, &uvi, &it);
+        sms_glint_walk_out &o = out[i];
+        o.success = ok; o.iterations = it; o.uv[0] = uvf.x; o.uv[1] = uvf.y; o.uv_init[0] = uvi.x; o.uv_init[1] = uvi.y;
+    });
+    return 0;
+}
+
+int oracle_ss_estimate_batch(void *h, const sms_ss_query *q, int64_t n, uint64_t seed, float *out_rgb,
+                             sms_stats *stats, int threads) {
+    const Scene &S = *(Scene *) h;
+    std::vector<Stats> per((size_t) n);
+    parallel_for(n, threads, [&](int64_t i) {
+        Stats st; memset(&st, 0, sizeof(st));
+        V3 r = estimate_ss(S, q[i], seed, (uint64_t) i, st);
+        out_rgb[3 * i] = r.x; out_rgb[3 * i + 1] = r.y; out_rgb[3 * i + 2] = r.z;
+        per[(size_t) i] = st;
+    });
+    Stats tot; memset(&tot, 0, sizeof(tot));
+    for (auto &s : per) { tot.walks += s.walks; tot.conv += s.conv; tot.iters += s.iters; tot.bcalls += s.bcalls;
+        tot.btrials += s.btrials; if (s.bmax > tot.bmax) tot.bmax = s.bmax; }
+    stats_out(tot, stats);
+    return 0;
+}
+
+int oracle_glint_estimate_batch(void *h, const sms_glint_query *q, int64_t n, uint64_t seed, float *out_rgb,
+                                sms_stats *stats, int threads) {
+    const Scene &S = *(Scene *) h;
+    std::vector<Stats> per((size_t) n);
+    parallel_for(n, threads, [&](int64_t i) {
+        Stats st; memset(&st, 0, sizeof(st));
+        V3 r = estimate_glint(S, q[i], seed, (uint64_t) i, st);
+        out_rgb[3 * i] = r.x; out_rgb[3 * i + 1] = r.y; out_rgb[3 * i + 2] = r.z;
+        per[(size_t) i] = st;
+    });
+    Stats tot; memset(&tot, 0, sizeof(tot));
+    for (auto &s : per) { tot.walks += s.walks; tot.conv += s.conv; tot.iters += s.iters; tot.bcalls += s.bcalls;
+        tot.btrials += s.btrials; if (s.bmax > tot.bmax) tot.bmax = s.bmax; }
+    stats_out(tot, stats);
+    return 0;
+}
+
+int oracle_glint_make_queries(void *h, int spp, uint64_t seed, sms_glint_query *q, int threads) {
+    const Scene &S = *(Scene *) h;
+    if (!S.has_cam) return -1;
+    int64_t n = (int64_t) S.cam.w * S.cam.h * spp;
+    parallel_for(n, threads, [&](int64_t i) { make_glint_query(S, i, spp, seed, &q[i]); });
+    return 0;
+}
+
+/* Scene ray cast exposed for the BVH-vs-brute-force test. */
+int oracle_ray_intersect(void *h, const float *o, const float *d, float *t, int *shape, int *prim, float *p) {
+    const Scene &S = *(Scene *) h;
+    Ray r = make_ray(v3(o[0], o[1], o[2]), v3(d[0], d[1], d[2]));
+    SI si = ray_intersect(S, r);
+    *t = si.t; *shape = si.valid ? si.shape : -1; *prim = si.prim;
+    p[0] = si.p.x; p[1] = si.p.y; p[2] = si.p.z;
+    return si.valid;
+}
+
+/* Constraint value and Jacobian of the angle-difference / half-vector step at a point on a
+   specular shape (for finite-difference tests).  Returns step.ok. */
+int oracle_ss_step_at(void *h, int halfvector, const float *x, const float *o, const float *d,
+                      const sms_emitter_interaction *e, float *C, float *dX, float *vtx_p) {
+    const Scene &S = *(Scene *) h;
+    SI si = ray_intersect(S, make_ray(v3(o[0], o[1], o[2]), v3(d[0], d[1], d[2])));
+    if (!si.valid) return -1;
+    MV v = mv_from_si(S, si);
+    EI ei = ei_from_c(*e);
+    Step st = halfvector ? step_halfvector(v3(x[0], x[1], x[2]), v, ei, v3(0.f, 0.f, 1.f))
+                         : step_anglediff(v3(x[0], x[1], x[2]), v, ei, v3(0.f, 0.f, 1.f));
+    C[0] = st.C.x; C[1] = st.C.y; dX[0] = st.dX.x; dX[1] = st.dX.y;
+    vtx_p[0] = v.p.x; vtx_p[1] = v.p.y; vtx_p[2] = v.p.z;
+    return st.ok;
+}
+
+/* Math routines exposed for pinning against libm. */
+float oracle_acosf(float x) { return safe_acos(x); }
+float oracle_atan2f(float y, float x) { return c_atan2f(y, x); }
+void oracle_sincosf(float x, float *s, float *c) { c_sincosf(x, s, c); }
+float oracle_fresnel(float c, float eta) { return fresnel(c, eta); }
+float oracle_fresnel_conductor(float c, float er, float ei) { return fresnel_conductor(c, er, ei); }
+void oracle_square_to_uniform_sphere(float u, float v, float *out) {
+    V3 r = square_to_uniform_sphere(v2(u, v)); out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+uint32_t oracle_pcg32_first(uint64_t initstate, uint64_t initseq, uint32_t *out, int n) {
+    Pcg32 r; pcg32_seed(&r, initstate, initseq);
+    for (int i = 0; i < n; ++i) out[i] = pcg32_next(&r);
+    return 0;
+}
+int oracle_sample_emitter_interaction(void *h, const float *p, uint64_t seed, uint64_t qid, sms_emitter_interaction *out) {
+    const Scene &S = *(Scene *) h;
+    Pcg32 rq; stream_seed(&rq, seed, qid, 0);
+    EI ei = sample_emitter_interaction(S, v3(p[0], p[1], p[2]), S.emitters_ss.empty() ? S.emitters_all : S.emitters_ss, &rq);
+    memset(out, 0, sizeof(*out));
+    for (int a = 0; a < 3; ++a) { out->p[a] = (&ei.p.x)[a]; out->n[a] = (&ei.n.x)[a]; out->d[a] = (&ei.d.x)[a]; out->weight[a] = (&ei.weight.x)[a]; }
+    out->pdf = ei.pdf; out->kind = ei.kind; out->emitter = ei.emitter;
+    return 0;
+}
+
+} // extern "C"

@@ -1,0 +1,340 @@
+// dev_ss.h -- single-scatter manifold walk on the GPU: angle-difference / half-vector
+// constraints with their 2x2 Jacobians, the Newton solver with reprojection by whole-scene
+// ray casts, the seed, and the path contribution (SpecularManifoldSingleScatter,
+// src/librender/manifold_ss.cpp).
+#pragma once
+#include "dev_scene.h"
+
+namespace smsdev {
+
+// ---------------------------------------------------------------- specular transforms (manifold.h:347-444)
+SMS_DEV V3 reflect(V3 w, V3 n) { return 2.f * dot(w, n) * n - w; }
+SMS_DEV void d_reflect(V3 w, V3 dw_du, V3 dw_dv, V3 n, V3 dn_du, V3 dn_dv, V3 &du, V3 &dv) {
+    float dot_w_n = dot(w, n), dot_dwdu_n = dot(dw_du, n), dot_dwdv_n = dot(dw_dv, n),
+          dot_w_dndu = dot(w, dn_du), dot_w_dndv = dot(w, dn_dv);
+    du = 2.f * ((dot_dwdu_n + dot_w_dndu) * n + dot_w_n * dn_du) - dw_du;
+    dv = 2.f * ((dot_dwdv_n + dot_w_dndv) * n + dot_w_n * dn_dv) - dw_dv;
+}
+SMS_DEV bool refract(V3 w, V3 n_, float eta_, V3 &out) {
+    V3 n = n_;
+    float eta = 1.f / eta_;
+    if (dot(w, n) < 0.f) { eta = 1.f / eta; n = n * -1.f; }
+    float dot_w_n = dot(w, n);
+    float root_term = 1.f - eta * eta * (1.f - dot_w_n * dot_w_n);
+    if (root_term < 0.f) { out = v3(0.f, 0.f, 0.f); return false; }
+    out = -eta * (w - dot_w_n * n) - n * sqrtf(root_term);
+    return true;
+}
+SMS_DEV void d_refract(V3 w, V3 dw_du, V3 dw_dv, V3 n_, V3 dn_du_, V3 dn_dv_, float eta_, V3 &du, V3 &dv) {
+    V3 n = n_, dn_du = dn_du_, dn_dv = dn_dv_;
+    float eta = 1.f / eta_;
+    if (dot(w, n) < 0.f) { eta = 1.f / eta; n = n * -1.f; dn_du = dn_du * -1.f; dn_dv = dn_dv * -1.f; }
+    float dot_w_n = dot(w, n), dot_dwdu_n = dot(dw_du, n), dot_dwdv_n = dot(dw_dv, n),
+          dot_w_dndu = dot(w, dn_du), dot_w_dndv = dot(w, dn_dv);
+    float root = sqrtf(1.f - eta * eta * (1.f - dot_w_n * dot_w_n));
+    V3 a_u = -eta * (dw_du - ((dot_dwdu_n + dot_w_dndu) * n + dot_w_n * dn_du));
+    V3 b1_u = dn_du * root;
+    V3 b2_u = n * (1.f / (2.f * root)) * (-eta * eta * (-2.f * dot_w_n * (dot_dwdu_n + dot_w_dndu)));
+    V3 b_u = -(b1_u + b2_u);
+    V3 a_v = -eta * (dw_dv - ((dot_dwdv_n + dot_w_dndv) * n + dot_w_n * dn_dv));
+    V3 b1_v = dn_dv * root;
+    V3 b2_v = n * (1.f / (2.f * root)) * (-eta * eta * (-2.f * dot_w_n * (dot_dwdv_n + dot_w_dndv)));
+    V3 b_v = -(b1_v + b2_v);
+    du = a_u + b_u; dv = a_v + b_v;
+}
+SMS_DEV void sphcoords(V3 w, float &theta, float &phi) {
+    theta = safe_acos(w.z);
+    phi = c_atan2f(w.y, w.x);
+    if (phi < 0.f) phi += 2.f * PI_F;
+}
+SMS_DEV void d_sphcoords(V3 w, V3 dw_du, V3 dw_dv, float &dt_du, float &dp_du, float &dt_dv, float &dp_dv) {
+    float d_acos = -(1.f / safe_sqrt(1.f - w.z * w.z));
+    V2 d_theta = d_acos * v2(dw_du.z, dw_dv.z);
+    float yx = w.y / w.x;
+    float d_atan = 1.f / (1.f + yx * yx);
+    V2 d_phi = (d_atan * v2(w.x * dw_du.y - w.y * dw_du.x, w.x * dw_dv.y - w.y * dw_dv.x)) * (1.f / (w.x * w.x));
+    if (w.x == 0.f) d_phi = v2(0.f, 0.f);
+    dt_du = d_theta.x; dp_du = d_phi.x; dt_dv = d_theta.y; dp_dv = d_phi.y;
+}
+
+struct Step { bool ok; V2 C, dX; };
+
+// compute_step_anglediff (manifold_ss.cpp:494-636)
+SMS_DEV Step step_anglediff(V3 v0p, const MV &v1, const EI &v2e, V3 n_offset) {
+    Step r; r.ok = false; r.C = v2(__builtin_inff(), __builtin_inff()); r.dX = v2(0.f, 0.f);
+    bool directional = v2e.kind == SMS_EI_DIRECTIONAL;
+    V3 wi = v0p - v1.p;
+    float ili = norm(wi);
+    if (ili < 1e-3f) return r;
+    ili = 1.f / ili;
+    wi = wi * ili;
+    V3 dwi_du = -ili * (v1.dp_du - wi * dot(wi, v1.dp_du));
+    V3 dwi_dv = -ili * (v1.dp_dv - wi * dot(wi, v1.dp_dv));
+    V3 wo = directional ? v2e.d : v2e.p - v1.p;
+    float ilo = norm(wo);
+    if (ilo < 1e-3f) return r;
+    ilo = 1.f / ilo;
+    wo = wo * ilo;
+    V3 dwo_du, dwo_dv;
+    if (directional) { dwo_du = v3(0.f, 0.f, 0.f); dwo_dv = v3(0.f, 0.f, 0.f); }
+    else {
+        dwo_du = -ilo * (v1.dp_du - wo * dot(wo, v1.dp_du));
+        dwo_dv = -ilo * (v1.dp_dv - wo * dot(wo, v1.dp_dv));
+    }
+    V2 C = v2(0.f, 0.f);
+    M2 dC = m2(0.f, 0.f, 0.f, 0.f);
+    V3 n = (v1.s * n_offset.x + v1.t * n_offset.y) + v1.n * n_offset.z;
+    V3 dn_du = (v1.ds_du * n_offset.x + v1.dt_du * n_offset.y) + v1.dn_du * n_offset.z;
+    V3 dn_dv = (v1.ds_dv * n_offset.x + v1.dt_dv * n_offset.y) + v1.dn_dv * n_offset.z;
+    bool success = false;
+    bool refl = v1.eta == 1.f;
+    V3 wio; bool valid_i;
+    if (refl) { wio = reflect(wi, n); valid_i = true; } else valid_i = refract(wi, n, v1.eta, wio);
+    if (valid_i) {
+        V3 dwio_du, dwio_dv;
+        if (refl) d_reflect(wi, dwi_du, dwi_dv, n, dn_du, dn_dv, dwio_du, dwio_dv);
+        else d_refract(wi, dwi_du, dwi_dv, n, dn_du, dn_dv, v1.eta, dwio_du, dwio_dv);
+        float to, po, tio, pio;
+        sphcoords(wo, to, po); sphcoords(wio, tio, pio);
+        float dt = to - tio, dp = po - pio;
+        if (dp < -PI_F) dp += 2.f * PI_F; else if (dp > PI_F) dp -= 2.f * PI_F;
+        C = v2(dt, dp);
+        float dto_du, dpo_du, dto_dv, dpo_dv, dtio_du, dpio_du, dtio_dv, dpio_dv;
+        d_sphcoords(wo, dwo_du, dwo_dv, dto_du, dpo_du, dto_dv, dpo_dv);
+        d_sphcoords(wio, dwio_du, dwio_dv, dtio_du, dpio_du, dtio_dv, dpio_dv);
+        dC = m2(dto_du - dtio_du, dto_dv - dtio_dv, dpo_du - dpio_du, dpo_dv - dpio_dv);
+        success = true;
+    }
+    V3 woi; bool valid_o;
+    if (refl) { woi = reflect(wo, n); valid_o = true; } else valid_o = refract(wo, n, v1.eta, woi);
+    if (valid_o && !success) {
+        V3 dwoi_du, dwoi_dv;
+        if (refl) d_reflect(wo, dwo_du, dwo_dv, n, dn_du, dn_dv, dwoi_du, dwoi_dv);
+        else d_refract(wo, dwo_du, dwo_dv, n, dn_du, dn_dv, v1.eta, dwoi_du, dwoi_dv);
+        float ti, pi, toi, poi;
+        sphcoords(wi, ti, pi); sphcoords(woi, toi, poi);
+        float dt = ti - toi, dp = pi - poi;
+        if (dp < -PI_F) dp += 2.f * PI_F; else if (dp > PI_F) dp -= 2.f * PI_F;
+        C = v2(dt, dp);
+        float dti_du, dpi_du, dti_dv, dpi_dv, dtoi_du, dpoi_du, dtoi_dv, dpoi_dv;
+        d_sphcoords(wi, dwi_du, dwi_dv, dti_du, dpi_du, dti_dv, dpi_dv);
+        d_sphcoords(woi, dwoi_du, dwoi_dv, dtoi_du, dpoi_du, dtoi_dv, dpoi_dv);
+        dC = m2(dti_du - dtoi_du, dti_dv - dtoi_dv, dpi_du - dpoi_du, dpi_dv - dpoi_dv);
+        success = true;
+    }
+    float determinant = det2(dC);
+    if (fabsf(determinant) < 1e-6f) return r;
+    M2 inv = inverse2(dC);
+    r.ok = success; r.C = C; r.dX = mulv2(inv, C);
+    return r;
+}
+
+// compute_step_halfvector (manifold_ss.cpp:407-492)
+SMS_DEV Step step_halfvector(V3 v0p, const MV &v1, const EI &v2e, V3 n_offset) {
+    Step r; r.ok = false; r.C = v2(__builtin_inff(), __builtin_inff()); r.dX = v2(0.f, 0.f);
+    bool directional = v2e.kind == SMS_EI_DIRECTIONAL;
+    V3 wo = directional ? v2e.d : v2e.p - v1.p;
+    float ilo = norm(wo);
+    if (ilo < 1e-3f) return r;
+    ilo = 1.f / ilo; wo = wo * ilo;
+    V3 wi = v0p - v1.p;
+    float ili = norm(wi);
+    if (ili < 1e-3f) return r;
+    ili = 1.f / ili; wi = wi * ili;
+    float eta = v1.eta;
+    if (dot(wi, v1.gn) < 0.f) eta = 1.f / eta;
+    V3 h = wi + eta * wo;
+    if (eta != 1.f) h = h * -1.f;
+    float ilh = 1.f / norm(h);
+    h = h * ilh;
+    ilo *= eta * ilh; ili *= ilh;
+    V3 dh_du, dh_dv;
+    if (directional) {
+        dh_du = ili * (-v1.dp_du + wi * dot(wi, v1.dp_du));
+        dh_dv = ili * (-v1.dp_dv + wi * dot(wi, v1.dp_dv));
+    } else {
+        dh_du = (-v1.dp_du * (ili + ilo) + wi * (dot(wi, v1.dp_du) * ili)) + wo * (dot(wo, v1.dp_du) * ilo);
+        dh_dv = (-v1.dp_dv * (ili + ilo) + wi * (dot(wi, v1.dp_dv) * ili)) + wo * (dot(wo, v1.dp_dv) * ilo);
+    }
+    dh_du = dh_du - h * dot(dh_du, h);
+    dh_dv = dh_dv - h * dot(dh_dv, h);
+    if (eta != 1.f) { dh_du = dh_du * -1.f; dh_dv = dh_dv * -1.f; }
+    M2 dH = m2(dot(v1.ds_du, h) + dot(v1.s, dh_du), dot(v1.ds_dv, h) + dot(v1.s, dh_dv),
+               dot(v1.dt_du, h) + dot(v1.t, dh_du), dot(v1.dt_dv, h) + dot(v1.t, dh_dv));
+    if (fabsf(det2(dH)) < 1e-6f) return r;
+    M2 inv = inverse2(dH);
+    V2 H = v2(dot(v1.s, h), dot(v1.t, h));
+    V2 dHv = H - v2(n_offset.x, n_offset.y);
+    r.ok = true; r.C = dHv; r.dX = mulv2(inv, dHv);
+    return r;
+}
+
+struct PathResult { bool ok; bool init_valid; SI si_final; V3 p_init; int iterations; };
+
+// sample_path SS (manifold_ss.cpp:203-279) with newton_solver (manifold_ss.cpp:329-405)
+SMS_DEV PathResult sample_path_ss(const DScene &S, int shape, V3 si_p, const EI &ei, V2 sample, V3 n_offset, V3 p_start) {
+    const sms_config &cfg = S.cfg;
+    PathResult out;
+    out.ok = false; out.init_valid = false; out.si_final = si_invalid(); out.p_init = v3(0.f, 0.f, 0.f);
+    out.iterations = 0;
+    PS ps = sample_position(S, shape, sample);
+    if (p_start.x != 0.f || p_start.y != 0.f || p_start.z != 0.f) ps.p = p_start;
+    V3 d_tmp = normalize(ps.p - si_p);
+    if (cfg.mnee_init) {
+        const DShape &sh = S.shapes[shape];
+        if (sh.k != 0.f) d_tmp = normalize(ld3(sh.bbox_center) - si_p);
+        else d_tmp = normalize(ei.p - si_p);
+    }
+    SI si_init = ray_intersect(S, make_ray(si_p, d_tmp));
+    if (!si_init.valid || shape != si_init.shape) return out;
+    out.init_valid = true; out.p_init = si_init.p;
+    MV vtx = mv_from_si(S, si_init);
+    // newton_solver
+    bool success = false;
+    int iterations = 0;
+    float beta = 1.f;
+    SI si_current = si_invalid();
+    while (iterations < cfg.max_iterations) {
+        Step st = cfg.halfvector_constraints ? step_halfvector(si_p, vtx, ei, n_offset)
+                                             : step_anglediff(si_p, vtx, ei, n_offset);
+        if (!st.ok) break;
+        if (norm2(st.C) < cfg.solver_threshold) { success = true; break; }
+        V3 p_prop = vtx.p - cfg.step_scale * beta * (vtx.dp_du * st.dX.x + vtx.dp_dv * st.dX.y);
+        V3 d_prop = normalize(p_prop - si_p);
+        si_current = ray_intersect(S, make_ray(si_p, d_prop));
+        if (!si_current.valid || vtx.shape != si_current.shape) {
+            beta = 0.5f * beta;
+            iterations += 1;
+            continue;
+        }
+        beta = fmin_(1.f, 2.f * beta);
+        vtx = mv_from_si(S, si_current);
+        iterations++;
+    }
+    out.iterations = iterations;
+    out.si_final = si_current;
+    if (!success) return out;
+    V3 wx = normalize(si_p - vtx.p);
+    V3 wy = ei.kind == SMS_EI_DIRECTIONAL ? ei.d : normalize(ei.p - vtx.p);
+    float cos_theta_x = dot(vtx.gn, wx), cos_theta_y = dot(vtx.gn, wy);
+    bool refraction = cos_theta_x * cos_theta_y < 0.f, reflection = !refraction;
+    if ((vtx.eta == 1.f && !reflection) || (vtx.eta != 1.f && !refraction)) return out;
+    out.ok = si_current.valid;
+    return out;
+}
+
+// specular_reflectance SS, smooth branch (manifold_ss.cpp:638-693)
+SMS_DEV float specular_reflectance_ss(const DScene &S, const SI &si, V3 wo) {
+    if (!si.valid) return 0.f;
+    const DShape &sh = S.shapes[si.shape];
+    float cos_theta = dot(si.sh_n, wo);
+    if (sh.k != 0.f) return fresnel_conductor(fabsf(cos_theta), sh.eta_r, sh.k);
+    float eta = sh.eta_vtx;
+    if (cos_theta < 0.f) eta = 1.f / eta;
+    float F = fresnel(cos_theta, eta);
+    float val = 1.f - F;
+    return val * (eta * eta);
+}
+
+// geometric_term SS (manifold_ss.cpp:695-857)
+SMS_DEV float geometric_term_ss(const MV &v0, const MV &v1, const MV &v2v) {
+    V3 wi = v0.p - v1.p;
+    float ili = norm(wi);
+    if (ili < 1e-3f) return 0.f;
+    ili = 1.f / ili; wi = wi * ili;
+    V3 wo = v2v.p - v1.p;
+    float ilo = norm(wo);
+    if (ilo < 1e-3f) return 0.f;
+    ilo = 1.f / ilo; wo = wo * ilo;
+    M2 dc1_dx0 = m2(0, 0, 0, 0), dc2_dx1 = m2(0, 0, 0, 0), dc2_dx2 = m2(0, 0, 0, 0);
+    if (v2v.fixed_direction) {
+        V3 dc2_du1 = ilo * (v1.dp_du - wo * dot(wo, v1.dp_du)), dc2_dv1 = ilo * (v1.dp_dv - wo * dot(wo, v1.dp_dv));
+        dc2_dx1 = m2(dot(dc2_du1, v2v.dp_du), dot(dc2_dv1, v2v.dp_du), dot(dc2_du1, v2v.dp_dv), dot(dc2_dv1, v2v.dp_dv));
+        V3 dc2_du2 = -ilo * (v2v.dp_du - wo * dot(wo, v2v.dp_du)), dc2_dv2 = -ilo * (v2v.dp_dv - wo * dot(wo, v2v.dp_dv));
+        dc2_dx2 = m2(dot(dc2_du2, v2v.dp_du), dot(dc2_dv2, v2v.dp_du), dot(dc2_du2, v2v.dp_dv), dot(dc2_dv2, v2v.dp_dv));
+    }
+    float eta = v1.eta;
+    if (dot(wi, v1.gn) < 0.f) eta = 1.f / eta;
+    V3 h = wi + eta * wo;
+    if (eta != 1.f) h = h * -1.f;
+    float ilh = 1.f / norm(h);
+    h = h * ilh;
+    ilo *= eta * ilh; ili *= ilh;
+    float dot_dpdu_n = dot(v1.dp_du, v1.n), dot_dpdv_n = dot(v1.dp_dv, v1.n);
+    V3 s = v1.dp_du - dot_dpdu_n * v1.n, t = v1.dp_dv - dot_dpdv_n * v1.n;
+    V3 dh_du, dh_dv;
+    if (v2v.fixed_direction) {
+        dh_du = ili * (v0.dp_du - wi * dot(wi, v0.dp_du));
+        dh_dv = ili * (v0.dp_dv - wi * dot(wi, v0.dp_dv));
+        dh_du = dh_du - h * dot(dh_du, h); dh_dv = dh_dv - h * dot(dh_dv, h);
+        if (eta != 1.f) { dh_du = dh_du * -1.f; dh_dv = dh_dv * -1.f; }
+        dc1_dx0 = m2(dot(dh_du, s), dot(dh_dv, s), dot(dh_du, t), dot(dh_dv, t));
+    }
+    dh_du = (-v1.dp_du * (ili + ilo) + wi * (dot(wi, v1.dp_du) * ili)) + wo * (dot(wo, v1.dp_du) * ilo);
+    dh_dv = (-v1.dp_dv * (ili + ilo) + wi * (dot(wi, v1.dp_dv) * ili)) + wo * (dot(wo, v1.dp_dv) * ilo);
+    dh_du = dh_du - h * dot(dh_du, h); dh_dv = dh_dv - h * dot(dh_dv, h);
+    if (eta != 1.f) { dh_du = dh_du * -1.f; dh_dv = dh_dv * -1.f; }
+    float dot_h_n = dot(h, v1.n), dot_h_dndu = dot(h, v1.dn_du), dot_h_dndv = dot(h, v1.dn_dv);
+    M2 dc1_dx1 = m2((dot(dh_du, s) - dot(v1.dp_du, v1.dn_du) * dot_h_n) - dot_dpdu_n * dot_h_dndu,
+                    (dot(dh_dv, s) - dot(v1.dp_du, v1.dn_dv) * dot_h_n) - dot_dpdu_n * dot_h_dndv,
+                    (dot(dh_du, t) - dot(v1.dp_dv, v1.dn_du) * dot_h_n) - dot_dpdv_n * dot_h_dndu,
+                    (dot(dh_dv, t) - dot(v1.dp_dv, v1.dn_dv) * dot_h_n) - dot_dpdv_n * dot_h_dndv);
+    dh_du = ilo * (v2v.dp_du - wo * dot(wo, v2v.dp_du));
+    dh_dv = ilo * (v2v.dp_dv - wo * dot(wo, v2v.dp_dv));
+    dh_du = dh_du - h * dot(dh_du, h); dh_dv = dh_dv - h * dot(dh_dv, h);
+    if (eta != 1.f) { dh_du = dh_du * -1.f; dh_dv = dh_dv * -1.f; }
+    M2 dc1_dx2 = m2(dot(dh_du, s), dot(dh_dv, s), dot(dh_du, t), dot(dh_dv, t));
+    if (v2v.fixed_direction) {
+        if (fabsf(det2(dc2_dx2)) < 1e-6f) return 0.f;
+        M2 Li = inverse2(dc2_dx2);
+        M2 tmp = mulm2(Li, dc2_dx1);
+        M2 m = subm2(dc1_dx1, mulm2(dc1_dx2, tmp));
+        if (fabsf(det2(m)) < 1e-6f) return 0.f;
+        Li = inverse2(m);
+        M2 sol1 = mulm2(negm2(Li), dc1_dx0);
+        M2 sol0 = mulm2(negm2(tmp), sol1);
+        float G = fabsf(det2(negm2(sol0)));
+        G = fmin_(G, 10.f);
+        G /= fabsf(dot(wi, v0.n));
+        G *= eta * eta;
+        return G;
+    }
+    if (fabsf(det2(dc1_dx1)) < 1e-6f) return 0.f;
+    M2 inv = inverse2(dc1_dx1);
+    float dx1_dx2 = fabsf(det2(mulm2(inv, dc1_dx2)));
+    dx1_dx2 = fmin_(dx1_dx2, 1.f);
+    V3 d = v0.p - v1.p;
+    float inv_r2 = 1.f / sqnorm(d);
+    d = d * sqrtf(inv_r2);
+    float dw0_dx1 = fabsf(dot(d, v1.gn)) * inv_r2;
+    return dw0_dx1 * dx1_dx2;
+}
+
+// evaluate_path_contribution SS (manifold_ss.cpp:281-327); si = receiver (query) interaction
+SMS_DEV V3 path_contribution_ss(const DScene &S, const SI &si, const EI &ei_, const SI &si_final) {
+    MV vtx = mv_from_si(S, si_final);
+    make_orthonormal(vtx);
+    EI ei = ei_;
+    if (ei.kind == SMS_EI_POINT) {
+        const DEmitter &em = S.emitters[ei.emitter];
+        V3 pos = ld3(em.position);
+        V3 d = pos - si_final.p;
+        float dist = norm(d), inv_dist = 1.f / dist;
+        d = d * inv_dist;
+        float emitter_pdf = ei.pdf;
+        ei.p = pos; ei.d = d; ei.n = d; ei.pdf = 1.f;
+        ei.weight = (ld3(em.intensity) * (inv_dist * inv_dist)) * dist * dist;
+        ei.pdf *= emitter_pdf;
+        ei.weight = ei.weight * (1.f / emitter_pdf);
+    }
+    MV vy;
+    if (!emitter_interaction_to_vertex(S, ei, vtx.p, vy)) return v3(0.f, 0.f, 0.f);
+    MV vx = mv_from_si(S, si);
+    make_orthonormal(vx);
+    float F = specular_reflectance_ss(S, si_final, normalize(ei.p - si_final.p));
+    float G = geometric_term_ss(vx, vtx, vy);
+    float thr = F * G;
+    return v3(thr * ei.weight.x, thr * ei.weight.y, thr * ei.weight.z);
+}
+
+} // namespace smsdev

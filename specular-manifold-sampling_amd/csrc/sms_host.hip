@@ -1,0 +1,734 @@
+// sms_host.hip -- C ABI (include/sms.h) of the gfx950 SMS library: context, scene upload,
+// BVH construction, normal-map decoding and kernel launches.
+//
+// One context lives on the HIP device that is current when sms_ctx_create is called and
+// owns one stream (default priority).  Estimator calls run two persistent passes on that
+// stream; their work-queue buffers grow on demand and are reused across calls.
+#include "kernels.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+using namespace smsdev;
+
+struct sms_ctx {
+    int device = 0;
+    int n_cu = 0;
+    char dev_name[256] = {0};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+    float last_ms = 0.f, last_ms2 = 0.f;
+    std::string err;
+    DScene scene;
+    std::vector<void *> scene_allocs;
+    // estimator work buffers
+    unsigned long long *counters = nullptr;     // [0] cursor [1] n_overflow [2] coop cursor [8..15] stats
+    OverflowRec *overflow = nullptr; size_t overflow_cap = 0;
+    float *part = nullptr; float *invp = nullptr; size_t part_cap = 0;
+    int lane_trials = 256;
+    int blocks_per_cu = 0;
+    int occ_glint = 0, occ_ss = 0, occ_coop = 0, occ_biased = 0, occ_sscoop = 0;
+};
+
+namespace {
+
+sms_status fail(sms_ctx *ctx, sms_status code, const std::string &msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+#define HIP_TRY(ctx, call)                                                                      \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return fail(ctx, e_ == hipErrorOutOfMemory ? SMS_E_OOM : SMS_E_HIP,                 \
+                        std::string(#call) + ": " + hipGetErrorString(e_));                     \
+    } while (0)
+
+template <typename T>
+sms_status upload(sms_ctx *ctx, const std::vector<T> &v, const T **dst) {
+    if (v.empty()) { *dst = nullptr; return SMS_OK; }
+    void *p = nullptr;
+    HIP_TRY(ctx, hipMalloc(&p, v.size() * sizeof(T)));
+    ctx->scene_allocs.push_back(p);
+    HIP_TRY(ctx, hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    *dst = (const T *) p;
+    return SMS_OK;
+}
+
+void invert_affine(const float *M, float *Minv) {
+    double a[3][3], inv[3][3];
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) a[i][j] = M[i * 4 + j];
+    double det = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) - a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0])
+               + a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+    double id = 1.0 / det;
+    inv[0][0] = (a[1][1] * a[2][2] - a[1][2] * a[2][1]) * id;
+    inv[0][1] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) * id;
+    inv[0][2] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) * id;
+    inv[1][0] = (a[1][2] * a[2][0] - a[1][0] * a[2][2]) * id;
+    inv[1][1] = (a[0][0] * a[2][2] - a[0][2] * a[2][0]) * id;
+    inv[1][2] = (a[0][2] * a[1][0] - a[0][0] * a[1][2]) * id;
+    inv[2][0] = (a[1][0] * a[2][1] - a[1][1] * a[2][0]) * id;
+    inv[2][1] = (a[0][1] * a[2][0] - a[0][0] * a[2][1]) * id;
+    inv[2][2] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) * id;
+    double t[3] = {M[3], M[7], M[11]};
+    for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) Minv[i * 4 + j] = (float) inv[i][j];
+        Minv[i * 4 + 3] = (float) (-(inv[i][0] * t[0] + inv[i][1] * t[1] + inv[i][2] * t[2]));
+    }
+}
+
+float hmean_same(float e) { return ((e + e) + e) / 3.f; }
+
+float i2f(int x) { float f; memcpy(&f, &x, 4); return f; }
+
+// ---------------------------------------------------------------- binned-SAH BVH (host build)
+struct BItem { float bmin[3], bmax[3], c[3]; int tri; };
+
+struct BvhOut { std::vector<DNode> nodes; std::vector<int> order; };
+
+void pad_box(float *bmin, float *bmax) {
+    for (int a = 0; a < 3; ++a) {
+        float ext = bmax[a] - bmin[a];
+        float pad = 1e-5f * ext + 1e-5f * std::max(std::fabs(bmin[a]), std::fabs(bmax[a])) + 1e-7f;
+        bmin[a] -= pad; bmax[a] += pad;
+    }
+}
+
+BvhOut build_bvh(std::vector<BItem> &items) {
+    BvhOut out;
+    int n = (int) items.size();
+    if (n == 0) return out;
+    const int BINS = 16, MAX_LEAF = 8;
+    struct Task { int node, lo, hi; };
+    std::vector<Task> stack;
+    out.nodes.push_back(DNode());
+    stack.push_back({0, 0, n});
+    while (!stack.empty()) {
+        Task tk = stack.back(); stack.pop_back();
+        float bmin[3] = {INFINITY, INFINITY, INFINITY}, bmax[3] = {-INFINITY, -INFINITY, -INFINITY};
+        float cmin[3] = {INFINITY, INFINITY, INFINITY}, cmax[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int i = tk.lo; i < tk.hi; ++i)
+            for (int a = 0; a < 3; ++a) {
+                bmin[a] = std::min(bmin[a], items[i].bmin[a]); bmax[a] = std::max(bmax[a], items[i].bmax[a]);
+                cmin[a] = std::min(cmin[a], items[i].c[a]); cmax[a] = std::max(cmax[a], items[i].c[a]);
+            }
+        float pb0[3] = {bmin[0], bmin[1], bmin[2]}, pb1[3] = {bmax[0], bmax[1], bmax[2]};
+        pad_box(pb0, pb1);
+        int count = tk.hi - tk.lo;
+        DNode nd;
+        nd.a = make_float4(pb0[0], pb0[1], pb0[2], 0.f);
+        nd.b = make_float4(pb1[0], pb1[1], pb1[2], 0.f);
+        auto make_leaf = [&]() {
+            nd.a.w = i2f(tk.lo);
+            nd.b.w = i2f(count);
+            out.nodes[tk.node] = nd;
+        };
+        if (count <= 2) { make_leaf(); continue; }
+        // SAH over BINS bins per axis
+        auto area = [](const float *lo, const float *hi) {
+            float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+            return dx * dy + dy * dz + dz * dx;
+        };
+        int best_axis = -1, best_split = -1;
+        float best_cost = INFINITY;
+        for (int a = 0; a < 3; ++a) {
+            float ext = cmax[a] - cmin[a];
+            if (!(ext > 0.f)) continue;
+            int cnt[BINS] = {0};
+            float lo[BINS][3], hi[BINS][3];
+            for (int b = 0; b < BINS; ++b) for (int k = 0; k < 3; ++k) { lo[b][k] = INFINITY; hi[b][k] = -INFINITY; }
+            float scale = BINS / ext;
+            for (int i = tk.lo; i < tk.hi; ++i) {
+                int b = std::min(BINS - 1, (int) ((items[i].c[a] - cmin[a]) * scale));
+                cnt[b]++;
+                for (int k = 0; k < 3; ++k) { lo[b][k] = std::min(lo[b][k], items[i].bmin[k]); hi[b][k] = std::max(hi[b][k], items[i].bmax[k]); }
+            }
+            float rl[BINS][3], rh[BINS][3]; int rc[BINS];
+            float al[3] = {INFINITY, INFINITY, INFINITY}, ah[3] = {-INFINITY, -INFINITY, -INFINITY}; int ac = 0;
+            for (int b = BINS - 1; b > 0; --b) {
+                ac += cnt[b];
+                for (int k = 0; k < 3; ++k) { al[k] = std::min(al[k], lo[b][k]); ah[k] = std::max(ah[k], hi[b][k]); }
+                rc[b] = ac; for (int k = 0; k < 3; ++k) { rl[b][k] = al[k]; rh[b][k] = ah[k]; }
+            }
+            float ll[3] = {INFINITY, INFINITY, INFINITY}, lh[3] = {-INFINITY, -INFINITY, -INFINITY}; int lc = 0;
+            for (int b = 0; b < BINS - 1; ++b) {
+                lc += cnt[b];
+                for (int k = 0; k < 3; ++k) { ll[k] = std::min(ll[k], lo[b][k]); lh[k] = std::max(lh[k], hi[b][k]); }
+                if (lc == 0 || rc[b + 1] == 0) continue;
+                float cost = lc * area(ll, lh) + rc[b + 1] * area(rl[b + 1], rh[b + 1]);
+                if (cost < best_cost) { best_cost = cost; best_axis = a; best_split = b; }
+            }
+        }
+        float leaf_cost = count * area(bmin, bmax);
+        if (best_axis < 0 || (count <= MAX_LEAF && best_cost >= leaf_cost)) {
+            if (best_axis < 0 && count > MAX_LEAF) {
+                // degenerate centroids: split in the middle of the item range
+                int mid = (tk.lo + tk.hi) / 2;
+                nd.a.w = i2f((int) out.nodes.size());
+                nd.b.w = i2f(0);
+                out.nodes[tk.node] = nd;
+                int l = (int) out.nodes.size();
+                out.nodes.push_back(DNode()); out.nodes.push_back(DNode());
+                stack.push_back({l, tk.lo, mid}); stack.push_back({l + 1, mid, tk.hi});
+                continue;
+            }
+            make_leaf();
+            continue;
+        }
+        float ext = cmax[best_axis] - cmin[best_axis], scale = BINS / ext;
+        auto mid_it = std::partition(items.begin() + tk.lo, items.begin() + tk.hi, [&](const BItem &it) {
+            return std::min(BINS - 1, (int) ((it.c[best_axis] - cmin[best_axis]) * scale)) <= best_split;
+        });
+        int mid = (int) (mid_it - items.begin());
+        if (mid == tk.lo || mid == tk.hi) mid = (tk.lo + tk.hi) / 2;
+        int l = (int) out.nodes.size();
+        nd.a.w = i2f(l);
+        nd.b.w = i2f(0);
+        out.nodes[tk.node] = nd;
+        out.nodes.push_back(DNode()); out.nodes.push_back(DNode());
+        stack.push_back({l, tk.lo, mid}); stack.push_back({l + 1, mid, tk.hi});
+    }
+    out.order.resize(n);
+    for (int i = 0; i < n; ++i) out.order[i] = items[i].tri;
+    return out;
+}
+
+void camera_setup(DCamera &c, const sms_camera_desc &d) {
+    double o[3] = {d.origin[0], d.origin[1], d.origin[2]};
+    double tg[3] = {d.target[0], d.target[1], d.target[2]};
+    double up[3] = {d.up[0], d.up[1], d.up[2]};
+    double dir[3] = {tg[0] - o[0], tg[1] - o[1], tg[2] - o[2]};
+    double l = std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+    for (int i = 0; i < 3; ++i) dir[i] /= l;
+    double left[3] = {up[1] * dir[2] - up[2] * dir[1], up[2] * dir[0] - up[0] * dir[2], up[0] * dir[1] - up[1] * dir[0]};
+    l = std::sqrt(left[0] * left[0] + left[1] * left[1] + left[2] * left[2]);
+    for (int i = 0; i < 3; ++i) left[i] /= l;
+    double nup[3] = {dir[1] * left[2] - dir[2] * left[1], dir[2] * left[0] - dir[0] * left[2], dir[0] * left[1] - dir[1] * left[0]};
+    for (int i = 0; i < 3; ++i) {
+        c.o[i] = (float) o[i]; c.left[i] = (float) left[i]; c.up[i] = (float) nup[i]; c.dir[i] = (float) dir[i];
+    }
+    double th = std::tan(0.5 * d.fov_deg * M_PI / 180.0);
+    double aspect = (double) d.width / (double) d.height;
+    c.tan_half = (float) th;
+    c.dx[0] = (float) (-2.0 * th / d.width); c.dx[1] = 0.f; c.dx[2] = 0.f;
+    c.dy[0] = 0.f; c.dy[1] = (float) (-2.0 * th / aspect / d.height); c.dy[2] = 0.f;
+    c.rect_x = (float) th; c.rect_y = (float) (th / aspect);
+    c.normalization = (float) (1.0 / ((2.0 * th) * (2.0 * th / aspect)));
+    c.w = d.width; c.h = d.height;
+}
+
+sms_status build_scene(sms_ctx *ctx, const sms_scene_desc *desc, const sms_config *cfg) {
+    DScene &S = ctx->scene;
+    memset(&S, 0, sizeof(S));
+    S.cfg = *cfg;
+    if (desc->n_shapes <= 0 || !desc->shapes) return fail(ctx, SMS_E_INVALID, "scene has no shapes");
+    // normal maps: level-0 slopes (src/librender/normalmap.cpp:46-66)
+    std::vector<DNormalmap> nmaps;
+    for (int i = 0; i < desc->n_normalmaps; ++i) {
+        const sms_normalmap_desc &nd = desc->normalmaps[i];
+        if (nd.res <= 0 || (nd.res & (nd.res - 1)) != 0 || !nd.rgb)
+            return fail(ctx, SMS_E_INVALID, "normal map must be square power-of-two");
+        size_t N = (size_t) nd.res * nd.res;
+        std::vector<float2> sl(N);
+        for (size_t s = 0; s < N; ++s) {
+            float r = (float) nd.rgb[3 * s] / 255.f, g = (float) nd.rgb[3 * s + 1] / 255.f, b = (float) nd.rgb[3 * s + 2] / 255.f;
+            float x = 2.f * r - 1.f, y = 2.f * g - 1.f, z = 2.f * b - 1.f;
+            float inv = 1.f / std::sqrt((x * x + y * y) + z * z);
+            float nx = x * inv, ny = y * inv, nz = z * inv;
+            sl[s] = make_float2(-nx / nz, -ny / nz);
+        }
+        const float2 *dp;
+        sms_status st = upload(ctx, sl, &dp);
+        if (st) return st;
+        DNormalmap m; m.res = nd.res; m.pad = 0; m.slope = dp;
+        nmaps.push_back(m);
+    }
+    std::vector<DEmitter> emitters;
+    std::vector<int> em_ss, em_all;
+    for (int i = 0; i < desc->n_emitters; ++i) {
+        const sms_emitter_desc &e = desc->emitters[i];
+        DEmitter d;
+        memset(&d, 0, sizeof(d));
+        d.type = e.type; d.shape = e.shape;
+        for (int a = 0; a < 3; ++a) { d.radiance[a] = e.radiance[a]; d.position[a] = e.position[a]; d.intensity[a] = e.intensity[a]; }
+        d.ss = e.caustic_ss; d.ms = e.caustic_ms;
+        if (e.type == SMS_EMITTER_AREA && (e.shape < 0 || e.shape >= desc->n_shapes ||
+                                           desc->shapes[e.shape].type != SMS_SHAPE_SPHERE))
+            return fail(ctx, SMS_E_INVALID, "area emitters must sit on a sphere shape");
+        emitters.push_back(d);
+        em_all.push_back(i);
+        if (e.caustic_ss) em_ss.push_back(i);
+    }
+    std::vector<DShape> shapes;
+    std::vector<int> analytic, casters;
+    std::vector<float4> vpos, vnrm;
+    std::vector<float2> vuv;
+    std::vector<uint4> faces;
+    std::vector<float> cdf, pmf;
+    for (int i = 0; i < desc->n_shapes; ++i) {
+        const sms_shape_desc &d = desc->shapes[i];
+        DShape sh;
+        memset(&sh, 0, sizeof(sh));
+        sh.type = d.type; sh.bsdf = d.bsdf; sh.flags = d.flags; sh.emitter = d.emitter;
+        sh.cx = d.center[0]; sh.cy = d.center[1]; sh.cz = d.center[2];
+        sh.radius = d.radius; sh.inv_radius = 1.f / d.radius;
+        memcpy(sh.M, d.to_world, sizeof(sh.M));
+        if (d.bsdf == SMS_BSDF_DIFFUSE) { sh.eta_r = 0.f; sh.k = 1.f; }
+        else if (d.bsdf == SMS_BSDF_DIELECTRIC) { sh.eta_r = d.eta; sh.k = 0.f; }
+        else { sh.eta_r = d.eta; sh.k = d.k; }
+        sh.eta_vtx = sh.k == 0.f ? hmean_same(sh.eta_r) : 1.f;
+        for (int a = 0; a < 3; ++a) sh.refl[a] = d.reflectance[a];
+        sh.normalmap = d.normalmap; sh.tiles = d.tiles;
+        if (d.bsdf == SMS_BSDF_NORMALMAP && (d.normalmap < 0 || d.normalmap >= desc->n_normalmaps))
+            return fail(ctx, SMS_E_INVALID, "normal-mapped shape without a normal map");
+        sh.bbox_center[0] = sh.cx; sh.bbox_center[1] = sh.cy; sh.bbox_center[2] = sh.cz;
+        if (d.type == SMS_SHAPE_SPHERE) {
+            sh.inv_surface_area = 1.f / (4.f * PI_F * sh.radius * sh.radius);
+            analytic.push_back(i);
+        } else if (d.type == SMS_SHAPE_RECTANGLE) {
+            invert_affine(sh.M, sh.Minv);
+            sh.frs[0] = 2.f * sh.M[0]; sh.frs[1] = 2.f * sh.M[4]; sh.frs[2] = 2.f * sh.M[8];
+            sh.frt[0] = 2.f * sh.M[1]; sh.frt[1] = 2.f * sh.M[5]; sh.frt[2] = 2.f * sh.M[9];
+            double nx = sh.Minv[8], ny = sh.Minv[9], nz = sh.Minv[10];
+            double l = std::sqrt(nx * nx + ny * ny + nz * nz);
+            sh.frn[0] = (float) (nx / l); sh.frn[1] = (float) (ny / l); sh.frn[2] = (float) (nz / l);
+            float cs[3] = {sh.frs[1] * sh.frt[2] - sh.frs[2] * sh.frt[1], sh.frs[2] * sh.frt[0] - sh.frs[0] * sh.frt[2],
+                           sh.frs[0] * sh.frt[1] - sh.frs[1] * sh.frt[0]};
+            sh.inv_surface_area = 1.f / std::sqrt((cs[0] * cs[0] + cs[1] * cs[1]) + cs[2] * cs[2]);
+            sh.bbox_center[0] = sh.M[3]; sh.bbox_center[1] = sh.M[7]; sh.bbox_center[2] = sh.M[11];
+            analytic.push_back(i);
+        } else if (d.type == SMS_SHAPE_MESH) {
+            if (d.n_faces <= 0 || d.n_vertices <= 0 || !d.positions || !d.faces)
+                return fail(ctx, SMS_E_INVALID, "mesh without faces/positions");
+            int vbase = (int) vpos.size();
+            sh.fbase = (int) faces.size(); sh.nf = d.n_faces;
+            sh.has_n = d.normals != nullptr; sh.has_uv = d.texcoords != nullptr;
+            float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+            for (int v = 0; v < d.n_vertices; ++v) {
+                vpos.push_back(make_float4(d.positions[3 * v], d.positions[3 * v + 1], d.positions[3 * v + 2], 0.f));
+                vnrm.push_back(d.normals ? make_float4(d.normals[3 * v], d.normals[3 * v + 1], d.normals[3 * v + 2], 0.f)
+                                         : make_float4(0.f, 0.f, 0.f, 0.f));
+                vuv.push_back(d.texcoords ? make_float2(d.texcoords[2 * v], d.texcoords[2 * v + 1]) : make_float2(0.f, 0.f));
+                for (int a = 0; a < 3; ++a) { lo[a] = std::min(lo[a], d.positions[3 * v + a]); hi[a] = std::max(hi[a], d.positions[3 * v + a]); }
+            }
+            for (int a = 0; a < 3; ++a) sh.bbox_center[a] = 0.5f * (lo[a] + hi[a]);
+            sh.cdf_offset = (int) cdf.size();
+            double sum = 0.0;
+            for (int f = 0; f < d.n_faces; ++f) {
+                uint32_t i0 = d.faces[3 * f], i1 = d.faces[3 * f + 1], i2 = d.faces[3 * f + 2];
+                if (i0 >= (uint32_t) d.n_vertices || i1 >= (uint32_t) d.n_vertices || i2 >= (uint32_t) d.n_vertices)
+                    return fail(ctx, SMS_E_INVALID, "mesh face index out of range");
+                faces.push_back(make_uint4(vbase + i0, vbase + i1, vbase + i2, (uint32_t) i));
+                const float *p0 = &d.positions[3 * i0], *p1 = &d.positions[3 * i1], *p2 = &d.positions[3 * i2];
+                float e0[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]}, e1[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+                float cr[3] = {e0[1] * e1[2] - e0[2] * e1[1], e0[2] * e1[0] - e0[0] * e1[2], e0[0] * e1[1] - e0[1] * e1[0]};
+                float a = .5f * std::sqrt((cr[0] * cr[0] + cr[1] * cr[1]) + cr[2] * cr[2]);
+                pmf.push_back(a);
+                sum += (double) a;
+                cdf.push_back((float) sum);
+            }
+            sh.area_sum = (float) sum; sh.area_norm = (float) (1.0 / sum);
+            sh.inv_surface_area = sh.area_norm;
+        } else {
+            return fail(ctx, SMS_E_INVALID, "unknown shape type");
+        }
+        if (d.flags & SMS_FLAG_CAUSTIC_CASTER_SS) casters.push_back(i);
+        shapes.push_back(sh);
+    }
+    // BVH over all mesh triangles
+    std::vector<BItem> items(faces.size());
+    for (size_t t = 0; t < faces.size(); ++t) {
+        BItem &it = items[t];
+        it.tri = (int) t;
+        uint32_t ids[3] = {faces[t].x, faces[t].y, faces[t].z};
+        for (int a = 0; a < 3; ++a) { it.bmin[a] = INFINITY; it.bmax[a] = -INFINITY; }
+        for (int k = 0; k < 3; ++k) {
+            float p[3] = {vpos[ids[k]].x, vpos[ids[k]].y, vpos[ids[k]].z};
+            for (int a = 0; a < 3; ++a) { it.bmin[a] = std::min(it.bmin[a], p[a]); it.bmax[a] = std::max(it.bmax[a], p[a]); }
+        }
+        for (int a = 0; a < 3; ++a) it.c[a] = 0.5f * (it.bmin[a] + it.bmax[a]);
+    }
+    BvhOut bvh = build_bvh(items);
+    std::vector<float4> tris(3 * bvh.order.size());
+    std::vector<int> face_prim_base(desc->n_shapes, 0);
+    for (int i = 0; i < desc->n_shapes; ++i) face_prim_base[i] = shapes[i].fbase;
+    for (size_t k = 0; k < bvh.order.size(); ++k) {
+        const uint4 f = faces[bvh.order[k]];
+        float4 p0 = vpos[f.x], p1 = vpos[f.y], p2 = vpos[f.z];
+        int shape = (int) f.w, prim = bvh.order[k] - face_prim_base[shape];
+        tris[3 * k] = make_float4(p0.x, p0.y, p0.z, i2f(shape));
+        tris[3 * k + 1] = make_float4(p1.x - p0.x, p1.y - p0.y, p1.z - p0.z, i2f(prim));
+        tris[3 * k + 2] = make_float4(p2.x - p0.x, p2.y - p0.y, p2.z - p0.z, 0.f);
+    }
+    sms_status st;
+    if ((st = upload(ctx, shapes, &S.shapes))) return st;
+    S.n_shapes = (int) shapes.size();
+    if ((st = upload(ctx, analytic, &S.analytic))) return st;
+    S.n_analytic = (int) analytic.size();
+    if ((st = upload(ctx, emitters, &S.emitters))) return st;
+    S.n_emitters = (int) emitters.size();
+    if ((st = upload(ctx, em_ss, &S.emitters_ss))) return st;
+    S.n_emitters_ss = (int) em_ss.size();
+    if ((st = upload(ctx, em_all, &S.emitters_all))) return st;
+    S.n_emitters_all = (int) em_all.size();
+    if ((st = upload(ctx, casters, &S.casters_ss))) return st;
+    S.n_casters_ss = (int) casters.size();
+    if ((st = upload(ctx, nmaps, &S.nmaps))) return st;
+    if ((st = upload(ctx, vpos, &S.vpos))) return st;
+    if ((st = upload(ctx, vnrm, &S.vnrm))) return st;
+    if ((st = upload(ctx, vuv, &S.vuv))) return st;
+    if ((st = upload(ctx, faces, &S.faces))) return st;
+    if ((st = upload(ctx, bvh.nodes, &S.nodes))) return st;
+    S.n_nodes = (int) bvh.nodes.size();
+    if ((st = upload(ctx, tris, &S.tris))) return st;
+    if ((st = upload(ctx, cdf, &S.cdf))) return st;
+    if ((st = upload(ctx, pmf, &S.pmf))) return st;
+    S.has_cam = desc->camera.width > 0 && desc->camera.height > 0;
+    if (S.has_cam) camera_setup(S.cam, desc->camera);
+    return SMS_OK;
+}
+
+sms_status check_config(sms_ctx *ctx, const sms_config *cfg) {
+    if (!cfg) return fail(ctx, SMS_E_INVALID, "null config");
+    if (cfg->twostage) return fail(ctx, SMS_E_UNSUPPORTED, "twostage solver is not built");
+    if (cfg->max_iterations < 0) return fail(ctx, SMS_E_INVALID, "max_iterations < 0");
+    return SMS_OK;
+}
+
+__global__ void k_zero_u64(unsigned long long *p, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+constexpr int N_COUNTERS = 16;
+
+sms_status ensure_work(sms_ctx *ctx, size_t n_overflow, size_t n_part) {
+    if (!ctx->counters) HIP_TRY(ctx, hipMalloc((void **) &ctx->counters, N_COUNTERS * sizeof(unsigned long long)));
+    if (n_overflow > ctx->overflow_cap) {
+        if (ctx->overflow) HIP_TRY(ctx, hipFree(ctx->overflow));
+        ctx->overflow = nullptr;
+        size_t cap = std::max(n_overflow, (size_t) 1 << 16);
+        HIP_TRY(ctx, hipMalloc((void **) &ctx->overflow, cap * sizeof(OverflowRec)));
+        ctx->overflow_cap = cap;
+    }
+    if (n_part > ctx->part_cap) {
+        if (ctx->part) HIP_TRY(ctx, hipFree(ctx->part));
+        if (ctx->invp) HIP_TRY(ctx, hipFree(ctx->invp));
+        ctx->part = ctx->invp = nullptr;
+        HIP_TRY(ctx, hipMalloc((void **) &ctx->part, 3 * n_part * sizeof(float)));
+        HIP_TRY(ctx, hipMalloc((void **) &ctx->invp, n_part * sizeof(float)));
+        ctx->part_cap = n_part;
+    }
+    return SMS_OK;
+}
+
+EstimatorWork work_of(sms_ctx *ctx) {
+    EstimatorWork W;
+    W.cursor = ctx->counters + 0; W.n_overflow = ctx->counters + 1; W.coop_cursor = ctx->counters + 2;
+    W.stats = ctx->counters + 8; W.overflow = ctx->overflow; W.lane_trials = ctx->lane_trials;
+    return W;
+}
+
+sms_status read_stats(sms_ctx *ctx, sms_stats *stats) {
+    unsigned long long h[N_COUNTERS];
+    HIP_TRY(ctx, hipMemcpyAsync(h, ctx->counters, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (stats) {
+        memset(stats, 0, sizeof(*stats));
+        stats->walks = h[8]; stats->walks_converged = h[9]; stats->newton_iterations = h[10];
+        stats->bernoulli_calls = h[11]; stats->bernoulli_trials = h[12]; stats->bernoulli_max = h[13];
+        stats->overflow_queries = h[1];
+        stats->tail_iterations = h[14];
+    }
+    return SMS_OK;
+}
+
+int grid_for(sms_ctx *ctx, int occ) {
+    int per_cu = ctx->blocks_per_cu > 0 ? std::min(ctx->blocks_per_cu, std::max(1, occ)) : std::max(1, occ);
+    return per_cu * ctx->n_cu;
+}
+
+template <typename K>
+sms_status occupancy(sms_ctx *ctx, K kernel, int *out) {
+    int nb = 0;
+    HIP_TRY(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, 256, 0));
+    *out = std::max(1, nb);
+    return SMS_OK;
+}
+
+} // namespace
+
+// ==================================================================== C ABI
+extern "C" {
+
+void sms_config_default(sms_config *cfg) {
+    if (!cfg) return;
+    cfg->biased = 0; cfg->twostage = 0; cfg->halfvector_constraints = 0; cfg->mnee_init = 0;
+    cfg->step_scale = 1.f; cfg->max_iterations = 20; cfg->solver_threshold = 1e-5f;
+    cfg->uniqueness_threshold = 1e-4f; cfg->max_trials = -1; cfg->bounces = 1;
+    cfg->bsdf_strategy_only = 0; cfg->sms_strategy_only = 0;
+}
+
+sms_status sms_ctx_create(const sms_scene_desc *scene, const sms_config *cfg, sms_ctx **out) {
+    if (!out || !scene || !cfg) return SMS_E_INVALID;
+    *out = nullptr;
+    int n_dev = 0;
+    if (hipGetDeviceCount(&n_dev) != hipSuccess || n_dev == 0) return SMS_E_NO_DEVICE;
+    sms_ctx *ctx = new (std::nothrow) sms_ctx();
+    if (!ctx) return SMS_E_OOM;
+    sms_status st = check_config(ctx, cfg);
+    if (st) { delete ctx; return st; }
+    if (hipGetDevice(&ctx->device) != hipSuccess) { delete ctx; return SMS_E_HIP; }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, ctx->device) != hipSuccess) { delete ctx; return SMS_E_HIP; }
+    ctx->n_cu = prop.multiProcessorCount;
+    snprintf(ctx->dev_name, sizeof(ctx->dev_name), "%s (%s)", prop.name, prop.gcnArchName);
+    if (hipStreamCreate(&ctx->stream) != hipSuccess || hipEventCreate(&ctx->ev0) != hipSuccess ||
+        hipEventCreate(&ctx->ev1) != hipSuccess || hipEventCreate(&ctx->ev2) != hipSuccess) {
+        sms_ctx_destroy(ctx);
+        return SMS_E_HIP;
+    }
+    st = build_scene(ctx, scene, cfg);
+    if (!st) st = occupancy(ctx, k_glint_estimate, &ctx->occ_glint);
+    if (!st) st = occupancy(ctx, k_glint_coop, &ctx->occ_coop);
+    if (!st) st = occupancy(ctx, k_ss_estimate, &ctx->occ_ss);
+    if (!st) st = occupancy(ctx, k_ss_coop, &ctx->occ_sscoop);
+    if (!st) st = occupancy(ctx, k_ss_biased, &ctx->occ_biased);
+    if (!st) st = ensure_work(ctx, 0, 0);
+    if (st) {
+        fprintf(stderr, "sms_ctx_create: %s\n", ctx->err.c_str());
+        sms_ctx_destroy(ctx);
+        return st;
+    }
+    *out = ctx;
+    return SMS_OK;
+}
+
+sms_status sms_ctx_destroy(sms_ctx *ctx) {
+    if (!ctx) return SMS_E_INVALID;
+    if (ctx->stream) (void) hipStreamSynchronize(ctx->stream);
+    for (void *p : ctx->scene_allocs) (void) hipFree(p);
+    if (ctx->counters) (void) hipFree(ctx->counters);
+    if (ctx->overflow) (void) hipFree(ctx->overflow);
+    if (ctx->part) (void) hipFree(ctx->part);
+    if (ctx->invp) (void) hipFree(ctx->invp);
+    if (ctx->ev0) (void) hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void) hipEventDestroy(ctx->ev1);
+    if (ctx->ev2) (void) hipEventDestroy(ctx->ev2);
+    if (ctx->stream) (void) hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return SMS_OK;
+}
+
+sms_status sms_ctx_set_config(sms_ctx *ctx, const sms_config *cfg) {
+    if (!ctx) return SMS_E_INVALID;
+    sms_status st = check_config(ctx, cfg);
+    if (st) return st;
+    ctx->scene.cfg = *cfg;
+    return SMS_OK;
+}
+
+const char *sms_ctx_last_error(const sms_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+sms_status sms_ctx_set_schedule(sms_ctx *ctx, const sms_schedule *s) {
+    if (!ctx || !s) return SMS_E_INVALID;
+    if (s->lane_trials < 1 || s->blocks_per_cu < 0) return fail(ctx, SMS_E_INVALID, "bad schedule");
+    ctx->lane_trials = s->lane_trials;
+    ctx->blocks_per_cu = s->blocks_per_cu;
+    return SMS_OK;
+}
+
+sms_status sms_ctx_get_schedule(const sms_ctx *ctx, sms_schedule *s) {
+    if (!ctx || !s) return SMS_E_INVALID;
+    memset(s, 0, sizeof(*s));
+    s->lane_trials = ctx->lane_trials;
+    s->blocks_per_cu = ctx->blocks_per_cu;
+    return SMS_OK;
+}
+
+sms_status sms_ctx_device_info(const sms_ctx *ctx, char *name, int32_t name_len, int32_t *n_cu) {
+    if (!ctx) return SMS_E_INVALID;
+    if (name && name_len > 0) snprintf(name, (size_t) name_len, "%s", ctx->dev_name);
+    if (n_cu) *n_cu = ctx->n_cu;
+    return SMS_OK;
+}
+
+sms_status sms_ctx_sync(sms_ctx *ctx) {
+    if (!ctx) return SMS_E_INVALID;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return SMS_OK;
+}
+
+sms_status sms_ctx_last_kernel_ms(const sms_ctx *ctx, float *ms) {
+    if (!ctx || !ms) return SMS_E_INVALID;
+    ms[0] = ctx->last_ms;
+    ms[1] = ctx->last_ms2;
+    return SMS_OK;
+}
+
+sms_status sms_ss_walk_batch(sms_ctx *ctx, const sms_ss_walk_in *in, int64_t n, sms_ss_walk_out *out) {
+    if (!ctx || n < 0 || (n > 0 && (!in || !out))) return fail(ctx, SMS_E_INVALID, "bad walk batch arguments");
+    if (n == 0) return SMS_OK;
+    for (int i = 0; i < 1; ++i) {}
+    int blocks = (int) ((n + 255) / 256);
+    hipLaunchKernelGGL(k_ss_walk, dim3(blocks), dim3(256), 0, ctx->stream, ctx->scene, in, n, out);
+    HIP_TRY(ctx, hipGetLastError());
+    return SMS_OK;
+}
+
+sms_status sms_glint_walk_batch(sms_ctx *ctx, const sms_glint_walk_in *in, int64_t n, sms_glint_walk_out *out) {
+    if (!ctx || n < 0 || (n > 0 && (!in || !out))) return fail(ctx, SMS_E_INVALID, "bad walk batch arguments");
+    if (n == 0) return SMS_OK;
+    if (!ctx->scene.nmaps) return fail(ctx, SMS_E_INVALID, "scene has no normal map");
+    int blocks = (int) ((n + 255) / 256);
+    hipLaunchKernelGGL(k_glint_walk, dim3(blocks), dim3(256), 0, ctx->stream, ctx->scene, in, n, out);
+    HIP_TRY(ctx, hipGetLastError());
+    return SMS_OK;
+}
+
+sms_status sms_glint_make_queries(sms_ctx *ctx, int32_t spp, uint64_t seed, sms_glint_query *q) {
+    if (!ctx || spp <= 0 || !q) return fail(ctx, SMS_E_INVALID, "bad query arguments");
+    if (!ctx->scene.has_cam) return fail(ctx, SMS_E_INVALID, "scene has no camera");
+    int64_t n = (int64_t) ctx->scene.cam.w * ctx->scene.cam.h * spp;
+    int blocks = (int) ((n + 255) / 256);
+    hipLaunchKernelGGL(k_glint_queries, dim3(blocks), dim3(256), 0, ctx->stream, ctx->scene, (int) spp, seed, q, n);
+    HIP_TRY(ctx, hipGetLastError());
+    return SMS_OK;
+}
+
+sms_status sms_glint_estimate_batch(sms_ctx *ctx, const sms_glint_query *q, int64_t n, uint64_t seed, float *out_rgb,
+                                    sms_stats *stats) {
+    if (!ctx || n < 0 || (n > 0 && (!q || !out_rgb))) return fail(ctx, SMS_E_INVALID, "bad estimator arguments");
+    if (ctx->scene.cfg.biased) return fail(ctx, SMS_E_UNSUPPORTED, "biased glint estimator is not built");
+    if (!ctx->scene.nmaps) return fail(ctx, SMS_E_INVALID, "scene has no normal map");
+    sms_status st = ensure_work(ctx, (size_t) std::max<int64_t>(n, 1), 0);
+    if (st) return st;
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream, ctx->counters, N_COUNTERS);
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    if (n > 0) {
+        EstimatorWork W = work_of(ctx);
+        hipLaunchKernelGGL(k_glint_estimate, dim3(grid_for(ctx, ctx->occ_glint)), dim3(256), 0, ctx->stream,
+                           ctx->scene, q, n, seed, out_rgb, W);
+        HIP_TRY(ctx, hipGetLastError());
+        HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+        hipLaunchKernelGGL(k_glint_coop, dim3(grid_for(ctx, ctx->occ_coop)), dim3(256), 0, ctx->stream,
+                           ctx->scene, q, seed, out_rgb, W);
+        HIP_TRY(ctx, hipGetLastError());
+    } else {
+        HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    }
+    HIP_TRY(ctx, hipEventRecord(ctx->ev2, ctx->stream));
+    st = read_stats(ctx, stats);
+    if (st) return st;
+    HIP_TRY(ctx, hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1));
+    HIP_TRY(ctx, hipEventElapsedTime(&ctx->last_ms2, ctx->ev1, ctx->ev2));
+    return SMS_OK;
+}
+
+sms_status sms_ss_estimate_batch(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *out_rgb,
+                                 sms_stats *stats) {
+    if (!ctx || n < 0 || (n > 0 && (!q || !out_rgb))) return fail(ctx, SMS_E_INVALID, "bad estimator arguments");
+    const sms_config &cfg = ctx->scene.cfg;
+    int nc = ctx->scene.n_casters_ss;
+    if (cfg.biased && cfg.max_trials > 128) return fail(ctx, SMS_E_UNSUPPORTED, "biased trial sets above 128 are not built");
+    sms_status st = ensure_work(ctx, cfg.biased ? 0 : (size_t) std::max<int64_t>(n * nc, 1),
+                                cfg.biased ? 0 : (size_t) std::max<int64_t>(n * nc, 1));
+    if (st) return st;
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream, ctx->counters, N_COUNTERS);
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    if (n > 0) {
+        EstimatorWork W = work_of(ctx);
+        if (cfg.biased) {
+            hipLaunchKernelGGL(k_ss_biased, dim3(grid_for(ctx, ctx->occ_biased)), dim3(256), 0, ctx->stream,
+                               ctx->scene, q, n, seed, out_rgb, W);
+            HIP_TRY(ctx, hipGetLastError());
+            HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+        } else {
+            hipLaunchKernelGGL(k_ss_estimate, dim3(grid_for(ctx, ctx->occ_ss)), dim3(256), 0, ctx->stream,
+                               ctx->scene, q, n, seed, ctx->part, ctx->invp, W);
+            HIP_TRY(ctx, hipGetLastError());
+            HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+            hipLaunchKernelGGL(k_ss_coop, dim3(grid_for(ctx, ctx->occ_sscoop)), dim3(256), 0, ctx->stream,
+                               ctx->scene, q, seed, ctx->invp, W);
+            HIP_TRY(ctx, hipGetLastError());
+            hipLaunchKernelGGL(k_ss_finalize, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                               ctx->part, ctx->invp, nc, n, out_rgb);
+            HIP_TRY(ctx, hipGetLastError());
+        }
+    } else {
+        HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    }
+    HIP_TRY(ctx, hipEventRecord(ctx->ev2, ctx->stream));
+    st = read_stats(ctx, stats);
+    if (st) return st;
+    HIP_TRY(ctx, hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1));
+    HIP_TRY(ctx, hipEventElapsedTime(&ctx->last_ms2, ctx->ev1, ctx->ev2));
+    return SMS_OK;
+}
+
+// ---------------------------------------------------------------- device memory helpers
+sms_status sms_device_alloc(sms_ctx *ctx, size_t bytes, void **ptr) {
+    if (!ctx || !ptr) return SMS_E_INVALID;
+    HIP_TRY(ctx, hipMalloc(ptr, bytes == 0 ? 1 : bytes));
+    return SMS_OK;
+}
+sms_status sms_device_free(sms_ctx *ctx, void *ptr) {
+    if (!ctx) return SMS_E_INVALID;
+    if (ptr) HIP_TRY(ctx, hipFree(ptr));
+    return SMS_OK;
+}
+sms_status sms_copy_h2d(sms_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx) return SMS_E_INVALID;
+    if (bytes) HIP_TRY(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return SMS_OK;
+}
+sms_status sms_copy_d2h(sms_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx) return SMS_E_INVALID;
+    if (bytes) HIP_TRY(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return SMS_OK;
+}
+
+// ---------------------------------------------------------------- host-buffer variants
+#define HOST_VARIANT_BODY(INT, OUTT, CALL)                                                      \
+    if (!ctx || n < 0) return SMS_E_INVALID;                                                     \
+    if (n == 0) return SMS_OK;                                                                   \
+    void *d_in = nullptr, *d_out = nullptr;                                                      \
+    sms_status st = sms_device_alloc(ctx, (size_t) n * sizeof(INT), &d_in);                      \
+    if (!st) st = sms_device_alloc(ctx, (size_t) n * (OUTT), &d_out);                            \
+    if (!st) st = sms_copy_h2d(ctx, d_in, in, (size_t) n * sizeof(INT));                         \
+    if (!st) st = CALL;                                                                          \
+    if (!st) st = sms_copy_d2h(ctx, out, d_out, (size_t) n * (OUTT));                            \
+    if (d_in) sms_device_free(ctx, d_in);                                                        \
+    if (d_out) sms_device_free(ctx, d_out);                                                      \
+    return st;
+
+sms_status sms_ss_walk_batch_host(sms_ctx *ctx, const sms_ss_walk_in *in, int64_t n, sms_ss_walk_out *out) {
+    HOST_VARIANT_BODY(sms_ss_walk_in, sizeof(sms_ss_walk_out),
+                      sms_ss_walk_batch(ctx, (const sms_ss_walk_in *) d_in, n, (sms_ss_walk_out *) d_out))
+}
+sms_status sms_glint_walk_batch_host(sms_ctx *ctx, const sms_glint_walk_in *in, int64_t n, sms_glint_walk_out *out) {
+    HOST_VARIANT_BODY(sms_glint_walk_in, sizeof(sms_glint_walk_out),
+                      sms_glint_walk_batch(ctx, (const sms_glint_walk_in *) d_in, n, (sms_glint_walk_out *) d_out))
+}
+sms_status sms_ss_estimate_batch_host(sms_ctx *ctx, const sms_ss_query *in, int64_t n, uint64_t seed, float *out,
+                                      sms_stats *stats) {
+    HOST_VARIANT_BODY(sms_ss_query, 3 * sizeof(float),
+                      sms_ss_estimate_batch(ctx, (const sms_ss_query *) d_in, n, seed, (float *) d_out, stats))
+}
+sms_status sms_glint_estimate_batch_host(sms_ctx *ctx, const sms_glint_query *in, int64_t n, uint64_t seed, float *out,
+                                         sms_stats *stats) {
+    HOST_VARIANT_BODY(sms_glint_query, 3 * sizeof(float),
+                      sms_glint_estimate_batch(ctx, (const sms_glint_query *) d_in, n, seed, (float *) d_out, stats))
+}
+
+} // extern "C"

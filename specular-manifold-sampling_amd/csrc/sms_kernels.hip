@@ -1,0 +1,573 @@
+// sms_kernels.hip -- gfx950 kernels of the SMS solver path.
+//
+//   k_ss_walk / k_glint_walk      parity entry points: one lane per walk record
+//   k_glint_queries               camera ray differentials -> glint queries (footprints)
+//   k_glint_estimate              persistent unbiased glint estimator: every lane owns a
+//                                 query, advances its walk one Newton iteration per loop
+//                                 trip and refills from a device query counter when the
+//                                 query is resolved; queries that need more than
+//                                 `lane_trials` Bernoulli trials are pushed to an overflow
+//                                 list
+//   k_glint_coop                  overflow pass: one wavefront per query, 64 trials per
+//                                 round, the first match found by a ballot
+//   k_ss_estimate / k_ss_coop     the same two passes for unbiased single-scatter SMS
+//   k_ss_biased                   biased single-scatter SMS: one wavefront per query,
+//                                 lanes = trial-set members, in-order de-duplication
+//
+// Random numbers: query q draws its emitter sample from PCG32 stream (q, 0) and walk k of
+// caster c from stream (q, (c << 32) + k + 1), so every result is independent of which
+// lane or pass ran the walk.
+#include "kernels.h"
+
+namespace smsdev {
+
+// ---------------------------------------------------------------- stats
+SMS_DEV void wave_add_u64(unsigned long long *dst, unsigned long long v) {
+    // reduce across the wavefront, one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, v);
+}
+SMS_DEV void wave_max_u64(unsigned long long *dst, unsigned long long v) {
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    if ((threadIdx.x & 63) == 0 && v) atomicMax(dst, v);
+}
+
+// ================================================================ parity walks
+__global__ void k_ss_walk(DScene S, const sms_ss_walk_in *__restrict__ in, int64_t n, sms_ss_walk_out *__restrict__ out) {
+    int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const sms_ss_walk_in w = in[i];
+    EI ei;
+    ei.p = ld3(w.ei.p); ei.n = ld3(w.ei.n); ei.d = ld3(w.ei.d); ei.weight = ld3(w.ei.weight);
+    ei.pdf = w.ei.pdf; ei.kind = w.ei.kind; ei.emitter = w.ei.emitter;
+    PathResult r = sample_path_ss(S, w.shape, ld3(w.si_p), ei, v2(w.sample[0], w.sample[1]), ld3(w.n_offset), ld3(w.p_start));
+    sms_ss_walk_out o;
+    o.success = r.ok; o.iterations = r.iterations;
+    o.final_shape = r.si_final.valid ? r.si_final.shape : -1;
+    o.init_valid = r.init_valid;
+    st3(o.p, r.si_final.p); st3(o.n, r.si_final.sh_n); st3(o.p_init, r.p_init);
+    out[i] = o;
+}
+
+__global__ void k_glint_walk(DScene S, const sms_glint_walk_in *__restrict__ in, int64_t n, sms_glint_walk_out *__restrict__ out) {
+    int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const sms_glint_walk_in w = in[i];
+    sms_glint_walk_out o;
+    o.success = 0; o.iterations = 0; o.uv[0] = o.uv[1] = 0.f; o.uv_init[0] = o.uv_init[1] = 0.f;
+    GlintFootprint fp;
+    if (!glint_target(ld3(w.sensor_p), ld3(w.ei_p), ld3(w.si_p), ld3(w.frame_s), ld3(w.frame_t), ld3(w.frame_n),
+                      ld3(w.n_offset), fp.target)) { out[i] = o; return; }
+    const DShape &sh = S.shapes[w.shape];
+    const DNormalmap nm = S.nmaps[sh.normalmap];
+    fp.uv = v2(w.uv[0], w.uv[1]); fp.duv_dx = v2(w.duv_dx[0], w.duv_dx[1]); fp.duv_dy = v2(w.duv_dy[0], w.duv_dy[1]);
+    fp.slope = nm.slope; fp.res = nm.res; fp.fres = (float) nm.res; fp.tiles = sh.tiles;
+    footprint_setup(fp);
+    V2 xi = v2(w.sample[0], w.sample[1]);
+    if (w.xi_start[0] != 1.f || w.xi_start[1] != 1.f) xi = v2(w.xi_start[0], w.xi_start[1]);
+    GlintWalk gw;
+    glint_walk_start(fp, xi, gw);
+    o.uv_init[0] = gw.uv.x; o.uv_init[1] = gw.uv.y;
+    int st;
+    while ((st = glint_walk_iter(fp, S.cfg, gw)) == WALK_RUNNING) {}
+    o.success = st == WALK_CONVERGED; o.iterations = gw.it; o.uv[0] = gw.uv.x; o.uv[1] = gw.uv.y;
+    out[i] = o;
+}
+
+// ================================================================ glint queries (camera)
+__global__ void k_glint_queries(DScene S, int spp, uint64_t seed, sms_glint_query *__restrict__ q, int64_t n) {
+    int64_t idx = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    const DCamera &c = S.cam;
+    sms_glint_query o;
+    float *z = (float *) &o;
+    for (int k = 0; k < (int) (sizeof(o) / 4); ++k) z[k] = 0.f;
+    o.shape = -1;
+    int64_t pixel = idx / spp;
+    int px = (int) (pixel % c.w), py = (int) (pixel / c.w);
+    o.pixel = (int32_t) pixel;
+    Pcg32 r;
+    stream_seed(r, query_key(seed, (uint64_t) idx), 0xFFFFFFFFull);
+    float sx = pcg32_float(r), sy = pcg32_float(r);
+    V2 pos = v2(((float) px + sx) / (float) c.w, ((float) py + sy) / (float) c.h);
+    V3 near_p = v3((1.f - 2.f * pos.x) * c.tan_half, (1.f - 2.f * pos.y) * c.rect_y, 1.f);
+    V3 dloc = normalize(near_p);
+    V3 L = ld3(c.left), U = ld3(c.up), D = ld3(c.dir), O = ld3(c.o);
+    V3 d = to_world(L, U, D, dloc);
+    V3 d_x = to_world(L, U, D, normalize(near_p + ld3(c.dx))), d_y = to_world(L, U, D, normalize(near_p + ld3(c.dy)));
+    float scale = 1.f / sqrtf((float) spp);
+    d_x = fmadd3(d_x - d, scale, d); d_y = fmadd3(d_y - d, scale, d);
+    Ray ray = make_ray(O, d, 1e-2f * (1.f / dloc.z), 1e4f * (1.f / dloc.z));
+    SI si = ray_intersect(S, ray);
+    st3(o.sensor_p, O);
+    if (si.valid && (S.shapes[si.shape].flags & SMS_FLAG_GLINTY)) {
+        // SurfaceInteraction::compute_partials (interaction.h:216-248)
+        float dd = dot(si.n, si.p);
+        float t_x = (dd - dot(si.n, O)) / dot(si.n, d_x), t_y = (dd - dot(si.n, O)) / dot(si.n, d_y);
+        V3 dp_dx = fmadd3(d_x, t_x, O) - si.p, dp_dy = fmadd3(d_y, t_y, O) - si.p;
+        float a00 = dot(si.dp_du, si.dp_du), a01 = dot(si.dp_du, si.dp_dv), a11 = dot(si.dp_dv, si.dp_dv),
+              inv_det = 1.f / (a00 * a11 - a01 * a01);
+        float b0x = dot(si.dp_du, dp_dx), b1x = dot(si.dp_dv, dp_dx), b0y = dot(si.dp_du, dp_dy), b1y = dot(si.dp_dv, dp_dy);
+        if (!isfinite(inv_det)) inv_det = 0.f;
+        o.duv_dx[0] = fmaf(a11, b0x, -(a01 * b1x)) * inv_det; o.duv_dx[1] = fmaf(a00, b1x, -(a01 * b0x)) * inv_det;
+        o.duv_dy[0] = fmaf(a11, b0y, -(a01 * b1y)) * inv_det; o.duv_dy[1] = fmaf(a00, b1y, -(a01 * b0y)) * inv_det;
+        o.shape = si.shape;
+        st3(o.p, si.p); st3(o.n, si.n); st3(o.sh_n, si.sh_n); st3(o.sh_s, si.sh_s); st3(o.sh_t, si.sh_t);
+        st3(o.dp_du, si.dp_du); st3(o.dp_dv, si.dp_dv);
+        V3 mw = -ray.d;
+        st3(o.wi, v3(dot(mw, si.sh_s), dot(mw, si.sh_t), dot(mw, si.sh_n)));
+        o.uv[0] = si.uv.x; o.uv[1] = si.uv.y;
+    }
+    q[idx] = o;
+}
+
+// ================================================================ glint estimator
+struct GlintLane {
+    int64_t q;                  // query index or -1
+    uint64_t qkey;
+    int k;                      // walk index (0 = first walk, k >= 1 = Bernoulli trial k)
+    GlintFootprint fp;
+    GlintWalk w;
+    V2 uv_sol;
+    float val, f;               // F*G of the solution, mis*We
+    V3 ew;                      // emitter weight
+    V3 ei_p;                    // emitter sample position (all the contribution needs of it)
+    int ei_kind;
+};
+
+SMS_DEV SI glint_query_si(const sms_glint_query &g) {
+    SI si = si_invalid();
+    si.valid = true; si.p = ld3(g.p); si.n = ld3(g.n); si.sh_n = ld3(g.sh_n); si.sh_s = ld3(g.sh_s); si.sh_t = ld3(g.sh_t);
+    si.dp_du = ld3(g.dp_du); si.dp_dv = ld3(g.dp_dv); si.uv = v2(g.uv[0], g.uv[1]); si.shape = g.shape;
+    return si;
+}
+
+SMS_DEV void write_rgb(float *out, int64_t q, V3 v) { out[3 * q] = v.x; out[3 * q + 1] = v.y; out[3 * q + 2] = v.z; }
+
+// result of the SMS strategy for a resolved query: (f * (val * inv_prob)) * ew per channel
+SMS_DEV V3 glint_result(float val, float inv_prob, float f, V3 ew) {
+    float sw = val * inv_prob;
+    return v3((f * sw) * ew.x, (f * sw) * ew.y, (f * sw) * ew.z);
+}
+
+__global__ void __launch_bounds__(256) k_glint_estimate(DScene S, const sms_glint_query *__restrict__ Q, int64_t n, uint64_t seed,
+                                                        float *__restrict__ out, EstimatorWork W) {
+    const sms_config &cfg = S.cfg;
+    unsigned long long st_walks = 0, st_conv = 0, st_iters = 0, st_bcalls = 0, st_btrials = 0, st_bmax = 0;
+    GlintLane L;
+    L.q = -1;
+    while (true) {
+        if (L.q < 0) {
+            // ---- refill: fetch the next query and set it up (emitter, visibility, target, walk 0)
+            int64_t q = (int64_t) atomicAdd(W.cursor, 1ull);
+            if (q >= n) break;
+            const sms_glint_query g = Q[q];
+            write_rgb(out, q, v3(0.f, 0.f, 0.f));
+            if (g.shape < 0) continue;
+            SI si = glint_query_si(g);
+            uint64_t qkey = query_key(seed, (uint64_t) q);
+            Pcg32 rq;
+            stream_seed(rq, qkey, 0);
+            EI ei = sample_emitter_interaction(S, si.p, S.emitters_all, S.n_emitters_all, rq);
+            V3 edir = ei.p - si.p;
+            float edist = norm(edir);
+            edir = edir * (1.f / edist);
+            if (ray_test(S, make_ray(si.p, edir, RAY_EPS_F * (1.f + hmax_abs(si.p)), edist * (1.f - RAY_EPS_F)))) continue;
+            GlintFootprint &fp = L.fp;
+            if (!glint_target(ld3(g.sensor_p), ei.p, si.p, si.sh_s, si.sh_t, si.sh_n, v3(0.f, 0.f, 1.f), fp.target)) {
+                st_walks++;                    // sample_glint returns failure before the solve
+                continue;
+            }
+            const DShape &sh = S.shapes[g.shape];
+            const DNormalmap nm = S.nmaps[sh.normalmap];
+            fp.uv = si.uv; fp.duv_dx = v2(g.duv_dx[0], g.duv_dx[1]); fp.duv_dy = v2(g.duv_dy[0], g.duv_dy[1]);
+            fp.slope = nm.slope; fp.res = nm.res; fp.fres = (float) nm.res; fp.tiles = sh.tiles;
+            footprint_setup(fp);
+            L.q = q; L.qkey = qkey; L.k = 0;
+            L.ew = ei.weight;
+            L.f = 0.f; L.val = 0.f;
+            Pcg32 rw;
+            stream_seed(rw, qkey, 1);
+            V2 xi; xi.x = pcg32_float(rw); xi.y = pcg32_float(rw);
+            glint_walk_start(fp, xi, L.w);
+            L.ei_p = ei.p; L.ei_kind = ei.kind;
+        }
+        // ---- one Newton iteration of the lane's current walk
+        int st = glint_walk_iter(L.fp, cfg, L.w);
+        if (st == WALK_RUNNING) continue;
+        st_walks++; st_iters += (unsigned long long) L.w.it;
+        bool ok = st == WALK_CONVERGED;
+        if (ok) st_conv++;
+        if (L.k == 0) {
+            if (!ok) { L.q = -1; continue; }
+            // first solution: contribution and MIS/sensor factor (manifold_glints.cpp:73-79,144-179)
+            const sms_glint_query g = Q[L.q];
+            SI si = glint_query_si(g);
+            EI ei;
+            ei.p = L.ei_p; ei.kind = L.ei_kind;
+            ei.n = ei.d = v3(0.f, 0.f, 0.f); ei.weight = L.ew; ei.pdf = 0.f; ei.emitter = -1;
+            V3 sensor_p = ld3(g.sensor_p);
+            L.val = glint_contribution(S, sensor_p, ei, si, L.w.uv);
+            float We = sensor_we(S.cam, sensor_p, si.p);
+            float mis = cfg.bsdf_strategy_only ? 0.f : 1.f;
+            L.f = mis * We;
+            L.uv_sol = L.w.uv;
+            st_bcalls++;
+            L.k = 1;
+        } else {
+            if (ok && norm2(L.w.uv - L.uv_sol) < cfg.uniqueness_threshold) {
+                // match at trial k: inv_prob = k (manifold_glints.cpp:85-104)
+                st_btrials += (unsigned long long) L.k;
+                st_bmax = st_bmax > (unsigned long long) L.k ? st_bmax : (unsigned long long) L.k;
+                write_rgb(out, L.q, glint_result(L.val, (float) L.k, L.f, L.ew));
+                L.q = -1;
+                continue;
+            }
+            L.k++;
+            if (cfg.max_trials > 0 && L.k > cfg.max_trials) {
+                st_btrials += (unsigned long long) L.k;
+                st_bmax = st_bmax > (unsigned long long) L.k ? st_bmax : (unsigned long long) L.k;
+                write_rgb(out, L.q, glint_result(L.val, 0.f, L.f, L.ew));
+                L.q = -1;
+                continue;
+            }
+            if (L.k > W.lane_trials) {
+                unsigned long long slot = atomicAdd(W.n_overflow, 1ull);
+                OverflowRec rec;
+                rec.q = L.q; rec.k = L.k; rec.sol[0] = L.uv_sol.x; rec.sol[1] = L.uv_sol.y; rec.sol[2] = 0.f;
+                rec.val[0] = L.val; rec.val[1] = L.val; rec.val[2] = L.val; rec.f = L.f;
+                rec.ew[0] = L.ew.x; rec.ew[1] = L.ew.y; rec.ew[2] = L.ew.z;
+                W.overflow[slot] = rec;
+                L.q = -1;
+                continue;
+            }
+        }
+        Pcg32 rw;
+        stream_seed(rw, L.qkey, (uint64_t) L.k + 1);
+        V2 xi; xi.x = pcg32_float(rw); xi.y = pcg32_float(rw);
+        glint_walk_start(L.fp, xi, L.w);
+    }
+    wave_add_u64(W.stats + 0, st_walks); wave_add_u64(W.stats + 1, st_conv); wave_add_u64(W.stats + 2, st_iters);
+    wave_add_u64(W.stats + 3, st_bcalls); wave_add_u64(W.stats + 4, st_btrials); wave_max_u64(W.stats + 5, st_bmax);
+}
+
+// Overflow pass: one wavefront per query; lane j runs trial k0 + j; the smallest matching
+// trial index wins (identical to the sequential loop because each trial has its own stream).
+__global__ void __launch_bounds__(256) k_glint_coop(DScene S, const sms_glint_query *__restrict__ Q, uint64_t seed,
+                                                    float *__restrict__ out, EstimatorWork W) {
+    const sms_config &cfg = S.cfg;
+    const int lane = threadIdx.x & 63;
+    unsigned long long st_walks = 0, st_conv = 0, st_iters = 0, st_btrials = 0, st_bmax = 0, st_tail = 0;
+    const unsigned long long n_over = *W.n_overflow;
+    while (true) {
+        unsigned long long item = 0;
+        if (lane == 0) item = atomicAdd(W.coop_cursor, 1ull);
+        item = __shfl(item, 0, 64);
+        if (item >= n_over) break;
+        const OverflowRec rec = W.overflow[item];
+        const sms_glint_query g = Q[rec.q];
+        SI si = glint_query_si(g);
+        uint64_t qkey = query_key(seed, (uint64_t) rec.q);
+        Pcg32 rq;
+        stream_seed(rq, qkey, 0);
+        EI ei = sample_emitter_interaction(S, si.p, S.emitters_all, S.n_emitters_all, rq);
+        GlintFootprint fp;
+        glint_target(ld3(g.sensor_p), ei.p, si.p, si.sh_s, si.sh_t, si.sh_n, v3(0.f, 0.f, 1.f), fp.target);
+        const DShape &sh = S.shapes[g.shape];
+        const DNormalmap nm = S.nmaps[sh.normalmap];
+        fp.uv = si.uv; fp.duv_dx = v2(g.duv_dx[0], g.duv_dx[1]); fp.duv_dy = v2(g.duv_dy[0], g.duv_dy[1]);
+        fp.slope = nm.slope; fp.res = nm.res; fp.fres = (float) nm.res; fp.tiles = sh.tiles;
+        footprint_setup(fp);
+        V2 uv_sol = v2(rec.sol[0], rec.sol[1]);
+        int64_t kcap = cfg.max_trials > 0 ? (int64_t) cfg.max_trials : (int64_t) 1 << 40;   // last admissible trial
+        int64_t found = -1;
+        for (int64_t base = rec.k; base <= kcap && found < 0; base += 64) {
+            int64_t k = base + lane;
+            bool match = false, ran = false, conv = false;
+            int its = 0;
+            if (k <= kcap) {
+                Pcg32 rw;
+                stream_seed(rw, qkey, (uint64_t) k + 1);
+                V2 xi; xi.x = pcg32_float(rw); xi.y = pcg32_float(rw);
+                GlintWalk w;
+                glint_walk_start(fp, xi, w);
+                int s;
+                while ((s = glint_walk_iter(fp, cfg, w)) == WALK_RUNNING) {}
+                ran = true; its = w.it; conv = s == WALK_CONVERGED;
+                match = conv && norm2(w.uv - uv_sol) < cfg.uniqueness_threshold;
+            }
+            unsigned long long ballot = __ballot(match);
+            if (ballot) found = base + (int64_t) __ffsll((long long) ballot) - 1;
+            // count only the walks the sequential trial loop would have run (k <= first match)
+            if (ran && (found < 0 || k <= found)) {
+                st_walks++; st_iters += (unsigned long long) its; st_tail += (unsigned long long) its;
+                if (conv) st_conv++;
+            }
+        }
+        float inv_prob = found >= 0 ? (float) found : 0.f;
+        unsigned long long trials = found >= 0 ? (unsigned long long) found : (unsigned long long) kcap + 1ull;
+        if (lane == 0) {
+            write_rgb(out, rec.q, glint_result(rec.val[0], inv_prob, rec.f, v3(rec.ew[0], rec.ew[1], rec.ew[2])));
+            st_btrials += trials;
+            st_bmax = st_bmax > trials ? st_bmax : trials;
+        }
+    }
+    wave_add_u64(W.stats + 0, st_walks); wave_add_u64(W.stats + 1, st_conv); wave_add_u64(W.stats + 2, st_iters);
+    wave_add_u64(W.stats + 4, st_btrials); wave_max_u64(W.stats + 5, st_bmax); wave_add_u64(W.stats + 6, st_tail);
+}
+
+// ================================================================ single scatter, unbiased
+SMS_DEV SI ss_query_si(const sms_ss_query &g) {
+    SI si = si_invalid();
+    si.valid = true; si.t = 1.f;
+    si.p = ld3(g.p); si.n = ld3(g.n); si.sh_n = ld3(g.sh_n); si.sh_s = ld3(g.sh_s); si.sh_t = ld3(g.sh_t);
+    si.dp_du = si.sh_s; si.dp_dv = si.sh_t; si.shape = g.shape;
+    return si;
+}
+
+// Diffuse::eval at the receiver (src/bsdfs/diffuse.cpp:104-120)
+SMS_DEV V3 receiver_bsdf_eval(const DScene &S, const SI &si, V3 wi_local, V3 wo_local) {
+    const DShape &sh = S.shapes[si.shape];
+    if (sh.bsdf != SMS_BSDF_DIFFUSE) return v3(0.f, 0.f, 0.f);
+    if (!(wi_local.z > 0.f && wo_local.z > 0.f)) return v3(0.f, 0.f, 0.f);
+    return (ld3(sh.refl) * INV_PI_F) * wo_local.z;
+}
+
+SMS_DEV bool ss_visible(const DScene &S, const EI &ei, V3 p) {
+    V3 d = p - ei.p;
+    float dist = norm(d);
+    d = d * (1.f / dist);
+    return !ray_test(S, make_ray(ei.p, d, RAY_EPS_F * (1.f + hmax_abs(ei.p)), dist * (1.f - RAY_EPS_F)));
+}
+
+SMS_DEV PathResult ss_walk(const DScene &S, uint64_t qkey, int c, int shape, uint64_t k, V3 si_p, const EI &ei) {
+    Pcg32 rw;
+    stream_seed(rw, qkey, ((uint64_t) c << 32) + k + 1);
+    V2 s; s.x = pcg32_float(rw); s.y = pcg32_float(rw);
+    return sample_path_ss(S, shape, si_p, ei, s, v3(0.f, 0.f, 1.f), v3(0.f, 0.f, 0.f));
+}
+
+// Pass 1: lane per query; for every caster the first walk, visibility, contribution and up
+// to lane_trials Bernoulli trials.  Per (query, caster) the product bsdf*specular (bxs) and
+// the trial count are stored; unresolved (query, caster) pairs go to the overflow list.
+__global__ void __launch_bounds__(256) k_ss_estimate(DScene S, const sms_ss_query *__restrict__ Q, int64_t n, uint64_t seed,
+                                                     float *__restrict__ part, float *__restrict__ invp, EstimatorWork W) {
+    const sms_config &cfg = S.cfg;
+    unsigned long long st_walks = 0, st_conv = 0, st_iters = 0, st_bcalls = 0, st_btrials = 0, st_bmax = 0;
+    const int nc = S.n_casters_ss;
+    while (true) {
+        int64_t q = (int64_t) atomicAdd(W.cursor, 1ull);
+        if (q >= n) break;
+        const sms_ss_query g = Q[q];
+        SI si = ss_query_si(g);
+        V3 wi_local = ld3(g.wi);
+        uint64_t qkey = query_key(seed, (uint64_t) q);
+        Pcg32 rq;
+        stream_seed(rq, qkey, 0);
+        EI ei = sample_emitter_interaction(S, si.p, S.emitters_ss, S.n_emitters_ss, rq);
+        for (int c = 0; c < nc; ++c) {
+            int64_t slot = q * nc + c;
+            part[3 * slot] = 0.f; part[3 * slot + 1] = 0.f; part[3 * slot + 2] = 0.f;
+            invp[slot] = 0.f;
+            int shape = S.casters_ss[c];
+            PathResult r = ss_walk(S, qkey, c, shape, 0, si.p, ei);
+            st_walks++; st_iters += (unsigned long long) r.iterations;
+            if (!r.ok) continue;
+            st_conv++;
+            V3 direction = normalize(r.si_final.p - si.p);
+            if (!ss_visible(S, ei, r.si_final.p)) continue;
+            V3 spec = path_contribution_ss(S, si, ei, r.si_final);
+            V3 dl = v3(dot(direction, si.sh_s), dot(direction, si.sh_t), dot(direction, si.sh_n));
+            V3 bsdf = receiver_bsdf_eval(S, si, wi_local, dl);
+            part[3 * slot] = bsdf.x * spec.x; part[3 * slot + 1] = bsdf.y * spec.y; part[3 * slot + 2] = bsdf.z * spec.z;
+            st_bcalls++;
+            int64_t k = 1;
+            float inv_prob = -1.f;
+            while (true) {
+                PathResult t = ss_walk(S, qkey, c, shape, (uint64_t) k, si.p, ei);
+                st_walks++; st_iters += (unsigned long long) t.iterations;
+                if (t.ok) st_conv++;
+                V3 dtr = normalize(t.si_final.p - si.p);
+                if (t.ok && fabsf(dot(direction, dtr) - 1.f) < cfg.uniqueness_threshold) { inv_prob = (float) k; break; }
+                k++;
+                if (cfg.max_trials > 0 && k > cfg.max_trials) { inv_prob = 0.f; break; }
+                if (k > W.lane_trials) break;
+            }
+            if (inv_prob >= 0.f) {
+                unsigned long long tr = (unsigned long long) k;
+                st_btrials += tr; st_bmax = st_bmax > tr ? st_bmax : tr;
+                invp[slot] = inv_prob;
+            } else {
+                unsigned long long o = atomicAdd(W.n_overflow, 1ull);
+                OverflowRec rec;
+                rec.q = slot; rec.k = k;
+                rec.sol[0] = direction.x; rec.sol[1] = direction.y; rec.sol[2] = direction.z;
+                W.overflow[o] = rec;
+            }
+        }
+    }
+    wave_add_u64(W.stats + 0, st_walks); wave_add_u64(W.stats + 1, st_conv); wave_add_u64(W.stats + 2, st_iters);
+    wave_add_u64(W.stats + 3, st_bcalls); wave_add_u64(W.stats + 4, st_btrials); wave_max_u64(W.stats + 5, st_bmax);
+}
+
+__global__ void __launch_bounds__(256) k_ss_coop(DScene S, const sms_ss_query *__restrict__ Q, uint64_t seed,
+                                                 float *__restrict__ invp, EstimatorWork W) {
+    const sms_config &cfg = S.cfg;
+    const int lane = threadIdx.x & 63;
+    const int nc = S.n_casters_ss;
+    unsigned long long st_walks = 0, st_conv = 0, st_iters = 0, st_btrials = 0, st_bmax = 0, st_tail = 0;
+    const unsigned long long n_over = *W.n_overflow;
+    while (true) {
+        unsigned long long item = 0;
+        if (lane == 0) item = atomicAdd(W.coop_cursor, 1ull);
+        item = __shfl(item, 0, 64);
+        if (item >= n_over) break;
+        const OverflowRec rec = W.overflow[item];
+        int64_t q = rec.q / nc;
+        int c = (int) (rec.q % nc);
+        const sms_ss_query g = Q[q];
+        SI si = ss_query_si(g);
+        uint64_t qkey = query_key(seed, (uint64_t) q);
+        Pcg32 rq;
+        stream_seed(rq, qkey, 0);
+        EI ei = sample_emitter_interaction(S, si.p, S.emitters_ss, S.n_emitters_ss, rq);
+        V3 direction = v3(rec.sol[0], rec.sol[1], rec.sol[2]);
+        int shape = S.casters_ss[c];
+        int64_t kcap = cfg.max_trials > 0 ? (int64_t) cfg.max_trials : (int64_t) 1 << 40;
+        int64_t found = -1;
+        for (int64_t base = rec.k; base <= kcap && found < 0; base += 64) {
+            int64_t k = base + lane;
+            bool match = false, ran = false, conv = false;
+            int its = 0;
+            if (k <= kcap) {
+                PathResult t = ss_walk(S, qkey, c, shape, (uint64_t) k, si.p, ei);
+                ran = true; its = t.iterations; conv = t.ok;
+                V3 dtr = normalize(t.si_final.p - si.p);
+                match = t.ok && fabsf(dot(direction, dtr) - 1.f) < cfg.uniqueness_threshold;
+            }
+            unsigned long long ballot = __ballot(match);
+            if (ballot) found = base + (int64_t) __ffsll((long long) ballot) - 1;
+            if (ran && (found < 0 || k <= found)) {
+                st_walks++; st_iters += (unsigned long long) its; st_tail += (unsigned long long) its;
+                if (conv) st_conv++;
+            }
+        }
+        if (lane == 0) {
+            invp[rec.q] = found >= 0 ? (float) found : 0.f;
+            unsigned long long tr = found >= 0 ? (unsigned long long) found : (unsigned long long) kcap + 1ull;
+            st_btrials += tr; st_bmax = st_bmax > tr ? st_bmax : tr;
+        }
+    }
+    wave_add_u64(W.stats + 0, st_walks); wave_add_u64(W.stats + 1, st_conv); wave_add_u64(W.stats + 2, st_iters);
+    wave_add_u64(W.stats + 4, st_btrials); wave_max_u64(W.stats + 5, st_bmax); wave_add_u64(W.stats + 6, st_tail);
+}
+
+// result = sum over casters (in order) of bxs * inv_prob  (manifold_ss.cpp:139-140,198)
+__global__ void k_ss_finalize(const float *__restrict__ part, const float *__restrict__ invp, int nc, int64_t n,
+                              float *__restrict__ out) {
+    int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    V3 r = v3(0.f, 0.f, 0.f);
+    for (int c = 0; c < nc; ++c) {
+        int64_t s = q * nc + c;
+        float ip = invp[s];
+        r = r + v3(part[3 * s] * ip, part[3 * s + 1] * ip, part[3 * s + 2] * ip) / 1.f;
+    }
+    write_rgb(out, q, r);
+}
+
+// ================================================================ single scatter, biased
+// One wavefront per query: lane j runs trial-set member m = chunk*64 + j; solutions are
+// de-duplicated in m order through an LDS list and their contributions summed in m order
+// (manifold_ss.cpp:142-197).
+constexpr int BIASED_MAX_SOL = 128;
+__global__ void __launch_bounds__(256) k_ss_biased(DScene S, const sms_ss_query *__restrict__ Q, int64_t n, uint64_t seed,
+                                                   float *__restrict__ out, EstimatorWork W) {
+    __shared__ float sol_lds[4][BIASED_MAX_SOL][3];
+    const sms_config &cfg = S.cfg;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int nc = S.n_casters_ss;
+    unsigned long long st_walks = 0, st_conv = 0, st_iters = 0;
+    while (true) {
+        unsigned long long qq = 0;
+        if (lane == 0) qq = atomicAdd(W.cursor, 1ull);
+        qq = __shfl(qq, 0, 64);
+        if ((int64_t) qq >= n) break;
+        int64_t q = (int64_t) qq;
+        const sms_ss_query g = Q[q];
+        SI si = ss_query_si(g);
+        V3 wi_local = ld3(g.wi);
+        uint64_t qkey = query_key(seed, (uint64_t) q);
+        Pcg32 rq;
+        stream_seed(rq, qkey, 0);
+        EI ei = sample_emitter_interaction(S, si.p, S.emitters_ss, S.n_emitters_ss, rq);
+        V3 result = v3(0.f, 0.f, 0.f);
+        for (int c = 0; c < nc; ++c) {
+            int shape = S.casters_ss[c];
+            V3 value = v3(0.f, 0.f, 0.f);
+            int n_sol = 0;
+            for (int base = 0; base < cfg.max_trials; base += 64) {
+                int m = base + lane;
+                bool cand = false;
+                V3 direction = v3(0.f, 0.f, 0.f);
+                PathResult r;
+                r.ok = false;
+                if (m < cfg.max_trials) {
+                    r = ss_walk(S, qkey, c, shape, (uint64_t) m, si.p, ei);
+                    st_walks++; st_iters += (unsigned long long) r.iterations;
+                    if (r.ok) {
+                        st_conv++;
+                        direction = normalize(r.si_final.p - si.p);
+                        cand = ss_visible(S, ei, r.si_final.p);
+                    }
+                }
+                // in-order de-duplication against the solution list
+                bool unique = false;
+                unsigned long long cmask = __ballot(cand);
+                while (cmask) {
+                    int j = __ffsll((long long) cmask) - 1;
+                    cmask &= cmask - 1ull;
+                    float dx = __shfl(direction.x, j, 64), dy = __shfl(direction.y, j, 64), dz = __shfl(direction.z, j, 64);
+                    V3 dj = v3(dx, dy, dz);
+                    bool dup = false;
+                    for (int s0 = 0; s0 < n_sol; s0 += 64) {
+                        int s = s0 + lane;
+                        bool d = false;
+                        if (s < n_sol) d = fabsf(dot(dj, v3(sol_lds[wv][s][0], sol_lds[wv][s][1], sol_lds[wv][s][2])) - 1.f)
+                                           < cfg.uniqueness_threshold;
+                        if (__ballot(d)) { dup = true; break; }
+                    }
+                    if (!dup) {
+                        if (n_sol < BIASED_MAX_SOL && lane == 0) {
+                            sol_lds[wv][n_sol][0] = dx; sol_lds[wv][n_sol][1] = dy; sol_lds[wv][n_sol][2] = dz;
+                        }
+                        if (n_sol < BIASED_MAX_SOL) n_sol++;
+                        if (lane == j) unique = true;
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                }
+                V3 contrib = v3(0.f, 0.f, 0.f);
+                if (unique) {
+                    V3 dl = v3(dot(direction, si.sh_s), dot(direction, si.sh_t), dot(direction, si.sh_n));
+                    V3 bsdf = receiver_bsdf_eval(S, si, wi_local, dl);
+                    V3 spec = path_contribution_ss(S, si, ei, r.si_final);
+                    contrib = v3(bsdf.x * spec.x, bsdf.y * spec.y, bsdf.z * spec.z);
+                }
+                unsigned long long umask = __ballot(unique);
+                while (umask) {
+                    int j = __ffsll((long long) umask) - 1;
+                    umask &= umask - 1ull;
+                    value = value + v3(__shfl(contrib.x, j, 64), __shfl(contrib.y, j, 64), __shfl(contrib.z, j, 64));
+                }
+            }
+            result = result + value / 1.f;
+        }
+        if (lane == 0) write_rgb(out, q, result);
+    }
+    wave_add_u64(W.stats + 0, st_walks); wave_add_u64(W.stats + 1, st_conv); wave_add_u64(W.stats + 2, st_iters);
+}
+
+} // namespace smsdev

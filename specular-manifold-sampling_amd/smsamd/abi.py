@@ -1,0 +1,165 @@
+"""ctypes / numpy mirror of ``include/sms.h`` (the C ABI of the gfx950 SMS library).
+
+Record types used in batches (walk inputs/outputs, estimator queries) are numpy structured
+dtypes whose layout equals the C structs (all members are 4-byte scalars, so there is no
+padding); scene-description structs that carry pointers are ctypes Structures.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+SMS_OK = 0
+STATUS_NAMES = {0: "SMS_OK", -1: "SMS_E_INVALID", -2: "SMS_E_HIP", -3: "SMS_E_OOM",
+                -4: "SMS_E_QUEUE_FULL", -5: "SMS_E_UNSUPPORTED", -6: "SMS_E_NO_DEVICE"}
+
+SHAPE_SPHERE, SHAPE_RECTANGLE, SHAPE_MESH = 0, 1, 2
+BSDF_DIFFUSE, BSDF_DIELECTRIC, BSDF_CONDUCTOR, BSDF_NORMALMAP = 0, 1, 2, 3
+FLAG_CAUSTIC_CASTER_SS, FLAG_CAUSTIC_CASTER_MS = 0x01, 0x02
+FLAG_CAUSTIC_RECEIVER, FLAG_CAUSTIC_BOUNCER, FLAG_GLINTY = 0x04, 0x08, 0x10
+EMITTER_AREA, EMITTER_POINT = 0, 1
+EI_AREA, EI_POINT, EI_DIRECTIONAL = 0, 1, 2
+
+
+class SmsConfig(C.Structure):
+    """sms_config: field-for-field SMSConfig (include/mitsuba/render/manifold.h:11-28)."""
+    _fields_ = [("biased", C.c_int32), ("twostage", C.c_int32),
+                ("halfvector_constraints", C.c_int32), ("mnee_init", C.c_int32),
+                ("step_scale", C.c_float), ("max_iterations", C.c_int32),
+                ("solver_threshold", C.c_float), ("uniqueness_threshold", C.c_float),
+                ("max_trials", C.c_int32), ("bounces", C.c_int32),
+                ("bsdf_strategy_only", C.c_int32), ("sms_strategy_only", C.c_int32)]
+
+    @classmethod
+    def default(cls, **kw):
+        """SMSConfig() plus the integrator property defaults (path_sms_ss.cpp:49-57)."""
+        c = cls(0, 0, 0, 0, 1.0, 20, 1e-5, 1e-4, -1, 1, 0, 0)
+        for k, v in kw.items():
+            setattr(c, k, v)
+        return c
+
+
+class SmsShapeDesc(C.Structure):
+    _fields_ = [("type", C.c_int32), ("bsdf", C.c_int32), ("flags", C.c_int32), ("emitter", C.c_int32),
+                ("center", C.c_float * 3), ("radius", C.c_float), ("to_world", C.c_float * 12),
+                ("eta", C.c_float), ("k", C.c_float), ("reflectance", C.c_float * 3),
+                ("normalmap", C.c_int32), ("tiles", C.c_float),
+                ("n_vertices", C.c_int32), ("n_faces", C.c_int32),
+                ("positions", C.POINTER(C.c_float)), ("normals", C.POINTER(C.c_float)),
+                ("texcoords", C.POINTER(C.c_float)), ("faces", C.POINTER(C.c_uint32))]
+
+
+class SmsEmitterDesc(C.Structure):
+    _fields_ = [("type", C.c_int32), ("shape", C.c_int32), ("radiance", C.c_float * 3),
+                ("position", C.c_float * 3), ("intensity", C.c_float * 3),
+                ("caustic_ss", C.c_int32), ("caustic_ms", C.c_int32)]
+
+
+class SmsNormalmapDesc(C.Structure):
+    _fields_ = [("res", C.c_int32), ("rgb", C.POINTER(C.c_uint8))]
+
+
+class SmsCameraDesc(C.Structure):
+    _fields_ = [("origin", C.c_float * 3), ("target", C.c_float * 3), ("up", C.c_float * 3),
+                ("fov_deg", C.c_float), ("width", C.c_int32), ("height", C.c_int32)]
+
+
+class SmsSceneDesc(C.Structure):
+    _fields_ = [("n_shapes", C.c_int32), ("shapes", C.POINTER(SmsShapeDesc)),
+                ("n_emitters", C.c_int32), ("emitters", C.POINTER(SmsEmitterDesc)),
+                ("n_normalmaps", C.c_int32), ("normalmaps", C.POINTER(SmsNormalmapDesc)),
+                ("camera", SmsCameraDesc)]
+
+
+class SmsStats(C.Structure):
+    _fields_ = [("walks", C.c_uint64), ("walks_converged", C.c_uint64),
+                ("newton_iterations", C.c_uint64), ("bernoulli_calls", C.c_uint64),
+                ("bernoulli_trials", C.c_uint64), ("bernoulli_max", C.c_uint64),
+                ("overflow_queries", C.c_uint64), ("tail_iterations", C.c_uint64)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+class SmsSchedule(C.Structure):
+    _fields_ = [("lane_trials", C.c_int32), ("blocks_per_cu", C.c_int32), ("pad", C.c_int32 * 2)]
+
+
+F3 = (np.float32, 3)
+F2 = (np.float32, 2)
+EI_DTYPE = np.dtype([("p", F3), ("n", F3), ("d", F3), ("weight", F3), ("pdf", np.float32),
+                     ("kind", np.int32), ("emitter", np.int32)])
+SS_WALK_IN = np.dtype([("si_p", F3), ("shape", np.int32), ("ei", EI_DTYPE), ("n_offset", F3),
+                       ("sample", F2), ("p_start", F3)])
+SS_WALK_OUT = np.dtype([("success", np.int32), ("iterations", np.int32), ("final_shape", np.int32),
+                        ("init_valid", np.int32), ("p", F3), ("n", F3), ("p_init", F3)])
+GLINT_WALK_IN = np.dtype([("sensor_p", F3), ("ei_p", F3), ("si_p", F3), ("frame_s", F3),
+                          ("frame_t", F3), ("frame_n", F3), ("uv", F2), ("duv_dx", F2),
+                          ("duv_dy", F2), ("shape", np.int32), ("n_offset", F3), ("sample", F2),
+                          ("xi_start", F2)])
+GLINT_WALK_OUT = np.dtype([("success", np.int32), ("iterations", np.int32), ("uv", F2),
+                           ("uv_init", F2)])
+SS_QUERY = np.dtype([("p", F3), ("n", F3), ("sh_n", F3), ("sh_s", F3), ("sh_t", F3), ("wi", F3),
+                     ("shape", np.int32), ("pad", np.int32)])
+GLINT_QUERY = np.dtype([("sensor_p", F3), ("p", F3), ("n", F3), ("sh_n", F3), ("sh_s", F3),
+                        ("sh_t", F3), ("dp_du", F3), ("dp_dv", F3), ("wi", F3), ("uv", F2),
+                        ("duv_dx", F2), ("duv_dy", F2), ("shape", np.int32), ("pixel", np.int32)])
+
+# sizes the C header implies (checked against the compiled library in the tests)
+C_SIZES = {"sms_emitter_interaction": 60, "sms_ss_walk_in": 108, "sms_ss_walk_out": 52,
+           "sms_glint_walk_in": 128, "sms_glint_walk_out": 24, "sms_ss_query": 80,
+           "sms_glint_query": 140, "sms_config": 48, "sms_stats": 64}
+DTYPES = {"sms_emitter_interaction": EI_DTYPE, "sms_ss_walk_in": SS_WALK_IN,
+          "sms_ss_walk_out": SS_WALK_OUT, "sms_glint_walk_in": GLINT_WALK_IN,
+          "sms_glint_walk_out": GLINT_WALK_OUT, "sms_ss_query": SS_QUERY,
+          "sms_glint_query": GLINT_QUERY}
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "libsms_amd.so")
+
+# every symbol include/sms.h declares
+EXPORTS = ["sms_config_default", "sms_ctx_create", "sms_ctx_destroy", "sms_ctx_set_config",
+           "sms_ctx_set_schedule", "sms_ctx_get_schedule",
+           "sms_ctx_last_error", "sms_ctx_device_info", "sms_ctx_sync", "sms_ss_walk_batch",
+           "sms_glint_walk_batch", "sms_ss_estimate_batch", "sms_glint_estimate_batch",
+           "sms_glint_make_queries", "sms_ctx_last_kernel_ms", "sms_ss_walk_batch_host",
+           "sms_glint_walk_batch_host", "sms_ss_estimate_batch_host",
+           "sms_glint_estimate_batch_host", "sms_device_alloc", "sms_device_free",
+           "sms_copy_h2d", "sms_copy_d2h"]
+
+
+def load_library(path=LIB_PATH):
+    """Load the HIP library.  Raises if it is missing: there is no CPU fallback."""
+    if not os.path.exists(path):
+        raise RuntimeError(f"libsms_amd.so not built ({path}); run __graft_entry__.build()")
+    lib = C.CDLL(path)
+    vp = C.c_void_p
+    i64 = C.c_int64
+    lib.sms_config_default.argtypes = [C.POINTER(SmsConfig)]
+    lib.sms_config_default.restype = None
+    lib.sms_ctx_create.argtypes = [C.POINTER(SmsSceneDesc), C.POINTER(SmsConfig), C.POINTER(vp)]
+    lib.sms_ctx_destroy.argtypes = [vp]
+    lib.sms_ctx_set_config.argtypes = [vp, C.POINTER(SmsConfig)]
+    lib.sms_ctx_set_schedule.argtypes = [vp, C.POINTER(SmsSchedule)]
+    lib.sms_ctx_get_schedule.argtypes = [vp, C.POINTER(SmsSchedule)]
+    lib.sms_ctx_last_error.argtypes = [vp]
+    lib.sms_ctx_last_error.restype = C.c_char_p
+    lib.sms_ctx_device_info.argtypes = [vp, C.c_char_p, C.c_int32, C.POINTER(C.c_int32)]
+    lib.sms_ctx_sync.argtypes = [vp]
+    for name in ("sms_ss_walk_batch", "sms_glint_walk_batch", "sms_ss_walk_batch_host",
+                 "sms_glint_walk_batch_host"):
+        getattr(lib, name).argtypes = [vp, vp, i64, vp]
+    for name in ("sms_ss_estimate_batch", "sms_glint_estimate_batch",
+                 "sms_ss_estimate_batch_host", "sms_glint_estimate_batch_host"):
+        getattr(lib, name).argtypes = [vp, vp, i64, C.c_uint64, vp, C.POINTER(SmsStats)]
+    lib.sms_glint_make_queries.argtypes = [vp, C.c_int32, C.c_uint64, vp]
+    lib.sms_ctx_last_kernel_ms.argtypes = [vp, vp]
+    lib.sms_device_alloc.argtypes = [vp, C.c_size_t, C.POINTER(vp)]
+    lib.sms_device_free.argtypes = [vp, vp]
+    lib.sms_copy_h2d.argtypes = [vp, vp, vp, C.c_size_t]
+    lib.sms_copy_d2h.argtypes = [vp, vp, vp, C.c_size_t]
+    for name in EXPORTS:
+        fn = getattr(lib, name)
+        if fn.restype is C.c_int and name not in ("sms_ctx_last_error", "sms_config_default"):
+            fn.restype = C.c_int32
+    return lib

@@ -1,0 +1,99 @@
+"""ctypes loader of the CPU oracle (oracle/liboracle.so) -- the checker of the parity tests.
+
+Only the test-suite, __graft_entry__.smoke() and bench.py's CPU-baseline leg load it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from smsamd import abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+def load():
+    if not os.path.exists(ORACLE_SO):
+        build()
+    lib = C.CDLL(ORACLE_SO)
+    vp, i64 = C.c_void_p, C.c_int64
+    lib.oracle_create.argtypes = [C.POINTER(abi.SmsSceneDesc), C.POINTER(abi.SmsConfig)]
+    lib.oracle_create.restype = vp
+    lib.oracle_destroy.argtypes = [vp]
+    lib.oracle_set_config.argtypes = [vp, C.POINTER(abi.SmsConfig)]
+    lib.oracle_ss_walk_batch.argtypes = [vp, vp, i64, vp, C.c_int]
+    lib.oracle_glint_walk_batch.argtypes = [vp, vp, i64, vp, C.c_int]
+    lib.oracle_ss_estimate_batch.argtypes = [vp, vp, i64, C.c_uint64, vp, C.POINTER(abi.SmsStats), C.c_int]
+    lib.oracle_glint_estimate_batch.argtypes = [vp, vp, i64, C.c_uint64, vp, C.POINTER(abi.SmsStats), C.c_int]
+    lib.oracle_glint_make_queries.argtypes = [vp, C.c_int, C.c_uint64, vp, C.c_int]
+    lib.oracle_ray_intersect.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    lib.oracle_ss_step_at.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, vp, vp]
+    lib.oracle_acosf.argtypes = [C.c_float]
+    lib.oracle_acosf.restype = C.c_float
+    lib.oracle_atan2f.argtypes = [C.c_float, C.c_float]
+    lib.oracle_atan2f.restype = C.c_float
+    lib.oracle_sincosf.argtypes = [C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    lib.oracle_fresnel.argtypes = [C.c_float, C.c_float]
+    lib.oracle_fresnel.restype = C.c_float
+    lib.oracle_fresnel_conductor.argtypes = [C.c_float, C.c_float, C.c_float]
+    lib.oracle_fresnel_conductor.restype = C.c_float
+    lib.oracle_square_to_uniform_sphere.argtypes = [C.c_float, C.c_float, vp]
+    lib.oracle_pcg32_first.argtypes = [C.c_uint64, C.c_uint64, vp, C.c_int]
+    lib.oracle_sample_emitter_interaction.argtypes = [vp, vp, C.c_uint64, C.c_uint64, vp]
+    return lib
+
+
+class Oracle:
+    """One oracle scene (mirrors sms_ctx)."""
+
+    def __init__(self, scene, cfg, threads=None):
+        self.lib = load()
+        self.scene = scene
+        self.desc = scene.desc()
+        self.cfg = cfg
+        self.h = self.lib.oracle_create(C.byref(self.desc), C.byref(cfg))
+        self.threads = threads or min(8, os.cpu_count() or 1)
+
+    def close(self):
+        if self.h:
+            self.lib.oracle_destroy(self.h)
+            self.h = None
+
+    def set_config(self, cfg):
+        self.cfg = cfg
+        self.lib.oracle_set_config(self.h, C.byref(cfg))
+
+    def ss_walks(self, win):
+        out = np.zeros(len(win), abi.SS_WALK_OUT)
+        self.lib.oracle_ss_walk_batch(self.h, win.ctypes.data, len(win), out.ctypes.data, self.threads)
+        return out
+
+    def glint_walks(self, win):
+        out = np.zeros(len(win), abi.GLINT_WALK_OUT)
+        self.lib.oracle_glint_walk_batch(self.h, win.ctypes.data, len(win), out.ctypes.data, self.threads)
+        return out
+
+    def ss_estimate(self, q, seed):
+        out = np.zeros((len(q), 3), np.float32)
+        st = abi.SmsStats()
+        self.lib.oracle_ss_estimate_batch(self.h, q.ctypes.data, len(q), seed, out.ctypes.data, C.byref(st), self.threads)
+        return out, st.as_dict()
+
+    def glint_estimate(self, q, seed):
+        out = np.zeros((len(q), 3), np.float32)
+        st = abi.SmsStats()
+        self.lib.oracle_glint_estimate_batch(self.h, q.ctypes.data, len(q), seed, out.ctypes.data, C.byref(st), self.threads)
+        return out, st.as_dict()
+
+    def glint_queries(self, spp, seed):
+        cam = self.scene.camera
+        q = np.zeros(cam["width"] * cam["height"] * spp, abi.GLINT_QUERY)
+        self.lib.oracle_glint_make_queries(self.h, spp, seed, q.ctypes.data, self.threads)
+        return q

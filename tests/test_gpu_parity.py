@@ -1,0 +1,168 @@
+"""GPU-vs-oracle parity of the HIP kernels, called through the C ABI.
+
+The kernels follow the oracle's arithmetic contract (same op order, no FP contraction,
+the same Cephes transcendentals and PCG32 streams), so walks must agree exactly: identical
+converge/fail flags and iteration counts, solution points within 1e-4 relative world-space
+distance (expected bit-equal).  Estimator outputs use the same per-(query, walk) random
+streams and must agree to 1e-5 relative.
+"""
+import numpy as np
+import pytest
+
+from smsamd import abi, scenes, workloads
+
+pytestmark = pytest.mark.gpu
+
+REL_TOL = 1e-4
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    from smsamd import gpu as g
+    g.lib()
+    return g
+
+
+@pytest.fixture(scope="module")
+def oracle_mod():
+    import oracle_lib
+    oracle_lib.build()
+    return oracle_lib
+
+
+def _rel_dist(a, b):
+    d = np.linalg.norm(a.astype(np.float64) - b.astype(np.float64), axis=-1)
+    return d / np.maximum(np.linalg.norm(b.astype(np.float64), axis=-1), 1e-6)
+
+
+def _check_ss_walks(go, oo):
+    print(f"walks={len(go)} success={oo['success'].mean():.3f} iters={oo['iterations'].mean():.2f}", flush=True)
+    assert np.array_equal(go["success"], oo["success"]), "converge/fail flags differ"
+    assert np.array_equal(go["init_valid"], oo["init_valid"])
+    assert np.array_equal(go["iterations"], oo["iterations"])
+    assert np.array_equal(go["final_shape"], oo["final_shape"])
+    ok = oo["success"] == 1
+    assert _rel_dist(go["p"][ok], oo["p"][ok]).max(initial=0) <= REL_TOL
+    assert np.array_equal(go["p"], oo["p"]), "solution vertices not bit-equal"
+
+
+@pytest.mark.parametrize("scene_fn,extent", [(scenes.sphere_caustic, 1.2), (scenes.sphere_dome, 0.9)])
+def test_ss_walks_sphere(gpu, oracle_mod, scene_fn, extent):
+    sc = scene_fn()
+    cfg = abi.SmsConfig.default()
+    ctx = gpu.Context(sc, cfg)
+    orc = oracle_mod.Oracle(sc, cfg)
+    w = workloads.ss_walk_inputs(sc, 20000, 11, extent=extent)
+    _check_ss_walks(ctx.ss_walks(w), orc.ss_walks(w))
+    # explicit seeds (p_start override, manifold_ss.cpp:224-226)
+    w2 = workloads.ss_walk_inputs(sc, 4000, 12, extent=extent)
+    rng = np.random.default_rng(3)
+    w2["p_start"] = (np.array([0.0, 1.5 if scene_fn is scenes.sphere_caustic else 0.0, 0.0])
+                     + workloads.uniform_sphere(rng.random(4000), rng.random(4000))).astype(np.float32)
+    _check_ss_walks(ctx.ss_walks(w2), orc.ss_walks(w2))
+    # half-vector constraints
+    cfg_h = abi.SmsConfig.default(halfvector_constraints=1)
+    ctx.set_config(cfg_h)
+    orc.set_config(cfg_h)
+    _check_ss_walks(ctx.ss_walks(w), orc.ss_walks(w))
+    ctx.close()
+    orc.close()
+
+
+def test_glint_walks(gpu, oracle_mod):
+    sc = scenes.glint_plane(64, 64, nm_res=256, n_scratches=64, seed=3)
+    cfg = abi.SmsConfig.default(max_trials=100000)
+    ctx = gpu.Context(sc, cfg)
+    orc = oracle_mod.Oracle(sc, cfg)
+    for fp in (2.0, 6.0):
+        w = workloads.glint_walk_inputs(sc, 30000, 21, footprint_texels=fp)
+        go, oo = ctx.glint_walks(w), orc.glint_walks(w)
+        print(f"glint walks fp={fp} success={oo['success'].mean():.4f} iters={oo['iterations'].mean():.2f}", flush=True)
+        assert np.array_equal(go["success"], oo["success"])
+        assert np.array_equal(go["iterations"], oo["iterations"])
+        assert np.array_equal(go["uv"], oo["uv"])
+        assert np.array_equal(go["uv_init"], oo["uv_init"])
+    ctx.close()
+    orc.close()
+
+
+def test_glint_queries_and_estimate(gpu, oracle_mod):
+    sc = scenes.glint_plane(64, 48, nm_res=256, n_scratches=64, seed=3)
+    cfg = abi.SmsConfig.default(max_trials=100000)
+    ctx = gpu.Context(sc, cfg)
+    orc = oracle_mod.Oracle(sc, cfg)
+    qg, qo = ctx.glint_queries(2, 5), orc.glint_queries(2, 5)
+    assert np.array_equal(qg.view(np.uint8), qo.view(np.uint8)), "glint queries differ"
+    rg, sg = ctx.glint_estimate(qo, 9)
+    ro, so = orc.glint_estimate(qo, 9)
+    print("gpu", sg, "oracle", so, flush=True)
+    assert so["bernoulli_calls"] > 20
+    assert np.array_equal(rg, ro)
+    for k in ("walks", "walks_converged", "newton_iterations", "bernoulli_calls", "bernoulli_trials", "bernoulli_max"):
+        assert sg[k] == so[k], k
+    # force the one-wavefront-per-query tail pass for every query with a solution
+    ctx.set_schedule(lane_trials=1)
+    rg2, sg2 = ctx.glint_estimate(qo, 9)
+    assert sg2["overflow_queries"] > 0
+    assert np.array_equal(rg2, ro)
+    assert sg2["bernoulli_trials"] == so["bernoulli_trials"]
+    # trial cap: inv_prob = 0 past max_trials (manifold_glints.cpp:96-103)
+    cfg2 = abi.SmsConfig.default(max_trials=3)
+    ctx.set_config(cfg2)
+    orc.set_config(cfg2)
+    ctx.set_schedule(lane_trials=2)
+    rg3, _ = ctx.glint_estimate(qo, 9)
+    ro3, _ = orc.glint_estimate(qo, 9)
+    assert np.array_equal(rg3, ro3)
+    ctx.close()
+    orc.close()
+
+
+@pytest.mark.parametrize("biased", [1, 0])
+def test_ss_estimate_dome(gpu, oracle_mod, biased):
+    sc = scenes.sphere_dome()
+    cfg = abi.SmsConfig.default(biased=biased, max_trials=64 if biased else 100000)
+    ctx = gpu.Context(sc, cfg)
+    orc = oracle_mod.Oracle(sc, cfg)
+    q = workloads.plane_queries(sc, 3000, 2, extent=0.9)
+    rg, sg = ctx.ss_estimate(q, 7)
+    ro, so = orc.ss_estimate(q, 7)
+    print("gpu", sg, "oracle", so, flush=True)
+    assert (ro[:, 0] > 0).mean() > 0.2
+    np.testing.assert_allclose(rg, ro, rtol=1e-5, atol=0)
+    assert np.array_equal(rg, ro)
+    for k in ("walks", "walks_converged", "newton_iterations"):
+        assert sg[k] == so[k], k
+    if not biased:
+        ctx.set_schedule(lane_trials=1)
+        rg2, sg2 = ctx.ss_estimate(q, 7)
+        assert sg2["overflow_queries"] > 0
+        assert np.array_equal(rg2, ro)
+    ctx.close()
+    orc.close()
+
+
+def test_ss_estimate_sphere_above_plane_is_zero(gpu, oracle_mod):
+    """Sphere wholly above the plane: every refraction solution is occluded from the light by
+    the sphere itself (manifold_ss.cpp:88-102), on both sides."""
+    sc = scenes.sphere_caustic()
+    cfg = abi.SmsConfig.default(biased=1, max_trials=64)
+    ctx = gpu.Context(sc, cfg)
+    orc = oracle_mod.Oracle(sc, cfg)
+    q = workloads.plane_queries(sc, 1000, 4, extent=1.2)
+    rg, sg = ctx.ss_estimate(q, 3)
+    ro, so = orc.ss_estimate(q, 3)
+    assert so["walks_converged"] > 0
+    assert np.array_equal(rg, ro)
+    assert not ro.any()
+    ctx.close()
+    orc.close()
+
+
+def test_empty_batches(gpu):
+    sc = scenes.sphere_dome()
+    ctx = gpu.Context(sc, abi.SmsConfig.default())
+    assert len(ctx.ss_walks(np.zeros(0, abi.SS_WALK_IN))) == 0
+    rg, st = ctx.ss_estimate(np.zeros(0, abi.SS_QUERY), 1)
+    assert rg.shape == (0, 3)
+    ctx.close()
