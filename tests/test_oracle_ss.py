@@ -1,0 +1,238 @@
+"""Pins of the oracle's single-scatter solver (a4-a11 of the hot path) by mathematics that does
+not reuse its code:
+
+* brute force: on the glass dome every converged walk must land on a refraction solution
+  found by an FP64 root search on the great circle through receiver, light and centre;
+* closed form: on a plane mirror the walk must converge to the image-point construction;
+* finite differences: the Newton step J^-1 C of the angle-difference constraint must match an
+  FP64 finite-difference Jacobian of an independent numpy statement of the constraint;
+* statistics: the unbiased (Bernoulli) estimator and the biased (M = 64) estimator agree in
+  expectation when the trial set finds every solution.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib
+from smsamd import abi, scenes, workloads
+
+ETA = 1.5
+
+
+def _refract(w, n, eta_):
+    eta = 1.0 / eta_
+    if np.dot(w, n) < 0:
+        eta = 1.0 / eta
+        n = -n
+    c = np.dot(w, n)
+    root = 1.0 - eta * eta * (1.0 - c * c)
+    if root < 0:
+        return None
+    return -eta * (w - c * n) - n * math.sqrt(root)
+
+
+def _dome_roots(x, y, c=np.zeros(3), r=1.0):
+    """All refraction solutions p on the sphere in the plane of (x, y, c), FP64."""
+    e1 = (x - c) / np.linalg.norm(x - c)
+    t = (y - c) - np.dot(y - c, e1) * e1
+    e2 = t / np.linalg.norm(t)
+    nrm = np.cross(e1, e2)
+
+    def f(th):
+        p = c + r * (math.cos(th) * e1 + math.sin(th) * e2)
+        n = (p - c) / r
+        wi = (x - p) / np.linalg.norm(x - p)
+        wo = (y - p) / np.linalg.norm(y - p)
+        wt = _refract(wi, n, ETA)
+        if wt is None:
+            return None
+        return float(np.dot(np.cross(wt, wo), nrm))
+
+    ths = np.linspace(0, 2 * math.pi, 20001)
+    vals = [f(t) for t in ths]
+    roots = []
+    for i in range(len(ths) - 1):
+        a, b = vals[i], vals[i + 1]
+        if a is None or b is None or a * b > 0:
+            continue
+        lo, hi, flo = ths[i], ths[i + 1], a
+        for _ in range(60):
+            mid = 0.5 * (lo + hi)
+            fm = f(mid)
+            if fm is None:
+                break
+            if flo * fm <= 0:
+                hi = mid
+            else:
+                lo, flo = mid, fm
+        th = 0.5 * (lo + hi)
+        p = c + r * (math.cos(th) * e1 + math.sin(th) * e2)
+        n = (p - c) / r
+        wi = (x - p) / np.linalg.norm(x - p)
+        wo = (y - p) / np.linalg.norm(y - p)
+        wt = _refract(wi, n, ETA)
+        if wt is not None and np.dot(wt, wo) > 0.999:
+            roots.append(p)
+    return roots
+
+
+@pytest.fixture(scope="module")
+def dome():
+    sc = scenes.sphere_dome()
+    o = oracle_lib.Oracle(sc, abi.SmsConfig.default())
+    yield sc, o
+    o.close()
+
+
+def test_dome_walks_land_on_brute_force_roots(dome):
+    sc, o = dome
+    rng = np.random.default_rng(5)
+    checked = found_all = cases = 0
+    for case in range(16):
+        x = np.array([rng.uniform(-0.6, 0.6), 0.0, rng.uniform(-0.6, 0.6)])
+        y = np.array([0.0, 5.0, 0.0]) + 0.1 * workloads.uniform_sphere(rng.random(1), rng.random(1))[0]
+        roots = _dome_roots(x, y)
+        if not roots:
+            continue
+        cases += 1
+        w = np.zeros(400, abi.SS_WALK_IN)
+        w["si_p"] = x
+        w["shape"] = 0
+        w["ei"]["p"] = y
+        w["ei"]["kind"] = abi.EI_AREA
+        w["n_offset"] = (0, 0, 1)
+        w["sample"] = rng.random((400, 2))
+        out = o.ss_walks(w)
+        ok = out["success"] == 1
+        assert ok.mean() > 0.3
+        hit = set()
+        for p in out["p"][ok].astype(np.float64):
+            d = [np.linalg.norm(p - q) / np.linalg.norm(q) for q in roots]
+            k = int(np.argmin(d))
+            assert d[k] < 1e-4, (case, p, roots)
+            hit.add(k)
+            checked += 1
+        found_all += len(hit) == len(roots)
+    assert cases >= 10
+    assert checked > 1000
+    assert found_all >= cases - 2
+
+
+def test_plane_mirror_closed_form():
+    """Conductor mirror x = 1.2 (facing -x); receiver on the floor, point light: the solution is
+    where the segment receiver -> mirrored light crosses the mirror plane."""
+    sc = scenes.Scene()
+    M = np.array([[0, 0, -1, 1.2], [0, 2.0, 0, 1.0], [2.0, 0, 0, 0.0]], np.float32)   # local z -> -x
+    sc.add_rectangle(M, abi.BSDF_CONDUCTOR, flags=abi.FLAG_CAUSTIC_CASTER_SS, eta=0.0, k=1.0)
+    sc.add_rectangle(scenes.plane_to_world((0, 0, 0), 10.0), abi.BSDF_DIFFUSE, flags=abi.FLAG_CAUSTIC_RECEIVER)
+    sc.add_point_light((0.2, 2.0, 0.3), (1, 1, 1))
+    o = oracle_lib.Oracle(sc, abi.SmsConfig.default())
+    rng = np.random.default_rng(2)
+    n = 300
+    w = np.zeros(n, abi.SS_WALK_IN)
+    xs = np.stack([rng.uniform(-1, 0.8, n), np.zeros(n), rng.uniform(-1, 1, n)], -1)
+    w["si_p"] = xs
+    w["shape"] = 0
+    light = np.array([0.2, 2.0, 0.3])
+    w["ei"]["p"] = light
+    w["ei"]["kind"] = abi.EI_POINT
+    w["n_offset"] = (0, 0, 1)
+    w["sample"] = rng.random((n, 2))
+    out = o.ss_walks(w)
+    ok = out["success"] == 1
+    assert ok.mean() > 0.5
+    img = light.copy()
+    img[0] = 2 * 1.2 - img[0]
+    for x, p in zip(xs[ok], out["p"][ok].astype(np.float64)):
+        t = (1.2 - x[0]) / (img[0] - x[0])
+        ref = x + t * (img - x)
+        assert np.linalg.norm(p - ref) / np.linalg.norm(ref) < 1e-4
+    o.close()
+
+
+def _constraint(u, v, x, y, c, r, eta):
+    """Angle-difference constraint of a vertex at sphere parameters (u, v), FP64."""
+    phi, th = 2 * math.pi * u, math.pi * v
+    loc = np.array([math.sin(th) * math.cos(phi), math.sin(th) * math.sin(phi), math.cos(th)])
+    p = c + r * loc
+    n = loc
+    wi = (x - p) / np.linalg.norm(x - p)
+    wo = (y - p) / np.linalg.norm(y - p)
+    wt = _refract(wi, n, eta)
+    if wt is None:
+        return None, p
+
+    def sph(w):
+        t = math.acos(max(-1, min(1, w[2])))
+        ph = math.atan2(w[1], w[0])
+        return t, ph + (2 * math.pi if ph < 0 else 0)
+
+    to, po = sph(wo)
+    ti, pi_ = sph(wt)
+    dp = po - pi_
+    if dp < -math.pi:
+        dp += 2 * math.pi
+    elif dp > math.pi:
+        dp -= 2 * math.pi
+    return np.array([to - ti, dp]), p
+
+
+def test_newton_step_matches_finite_differences(dome):
+    sc, o = dome
+    rng = np.random.default_rng(9)
+    c = np.zeros(3)
+    cases = 0
+    for _ in range(60):
+        x = np.array([rng.uniform(-0.6, 0.6), 0.0, rng.uniform(-0.6, 0.6)])
+        y = np.array([rng.uniform(-1, 1), 5.0, rng.uniform(-1, 1)])
+        q = workloads.uniform_sphere(rng.random(1), rng.uniform(0.0, 0.35, 1))[0]   # upper cap about +z
+        p = np.array([q[0], q[2], q[1]])                                             # rotate the cap to +y
+        d = (p - x) / np.linalg.norm(p - x)
+        e = np.zeros(1, abi.EI_DTYPE)
+        e["p"] = y
+        e["kind"] = abi.EI_AREA
+        Co = np.zeros(2, np.float32)
+        dXo = np.zeros(2, np.float32)
+        vp = np.zeros(3, np.float32)
+        xo = x.astype(np.float32)
+        do = d.astype(np.float32)
+        ok = o.lib.oracle_ss_step_at(o.h, 0, xo.ctypes.data, xo.ctypes.data, do.ctypes.data, e.ctypes.data,
+                                     Co.ctypes.data, dXo.ctypes.data, vp.ctypes.data)
+        if ok != 1:
+            continue
+        # re-evaluate at the float32 vertex the oracle actually used
+        loc = (vp.astype(np.float64) - c) / np.linalg.norm(vp.astype(np.float64) - c)
+        v0 = math.acos(loc[2]) / math.pi
+        u0 = (math.atan2(loc[1], loc[0]) % (2 * math.pi)) / (2 * math.pi)
+        C0, _ = _constraint(u0, v0, x, y, c, 1.0, ETA)
+        h = 1e-6
+        ev = [_constraint(u0 + h, v0, x, y, c, 1.0, ETA)[0], _constraint(u0 - h, v0, x, y, c, 1.0, ETA)[0],
+              _constraint(u0, v0 + h, x, y, c, 1.0, ETA)[0], _constraint(u0, v0 - h, x, y, c, 1.0, ETA)[0]]
+        if C0 is None or any(e is None for e in ev):
+            continue
+        J = np.zeros((2, 2))
+        J[:, 0] = (ev[0] - ev[1]) / (2 * h)
+        J[:, 1] = (ev[2] - ev[3]) / (2 * h)
+        if abs(np.linalg.det(J)) < 1e-3 or np.linalg.cond(J) > 1e4:
+            continue
+        dX = np.linalg.solve(J, C0)
+        assert np.allclose(Co, C0, atol=2e-5), (Co, C0)
+        assert np.linalg.norm(dXo - dX) <= 2e-3 * max(np.linalg.norm(dX), 1e-2), (dXo, dX)
+        cases += 1
+    assert cases >= 30
+
+
+def test_unbiased_and_biased_estimators_agree(dome):
+    sc, _ = dome
+    q = workloads.plane_queries(sc, 1500, 3, extent=0.8)
+    ob = oracle_lib.Oracle(sc, abi.SmsConfig.default(biased=1, max_trials=64))
+    rb, _ = ob.ss_estimate(q, 1)
+    ou = oracle_lib.Oracle(sc, abi.SmsConfig.default(max_trials=1000000))
+    runs = np.stack([ou.ss_estimate(q, s)[0][:, 0] for s in range(1, 9)])
+    mean_u = runs.mean()
+    se = runs.mean(0).std() / math.sqrt(runs.shape[1])
+    assert abs(mean_u - rb[:, 0].mean()) < 4 * se + 1e-4 * abs(mean_u)
+    ob.close()
+    ou.close()
