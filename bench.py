@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--spp", type=int, default=4)
     ap.add_argument("--nm-res", type=int, default=2048)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample length")
+    ap.add_argument("--lane-trials", type=int, default=None, help="override the tuned schedule")
+    ap.add_argument("--blocks-per-cu", type=int, default=None, help="override the tuned schedule")
     ap.add_argument("--no-cpu", action="store_true")
     return ap.parse_args()
 
@@ -58,6 +60,12 @@ def main():
     dev, ncu = ctx.device_info()
     qbuf, n = ctx.glint_queries_device(args.spp, seed=1)
     out = gpu.DeviceBuffer(ctx, n * 12)
+    sched = ctx.get_schedule()
+    if args.lane_trials is not None:
+        sched["lane_trials"] = args.lane_trials
+    if args.blocks_per_cu is not None:
+        sched["blocks_per_cu"] = args.blocks_per_cu
+    ctx.set_schedule(**sched)
     print(f"# setup {time.time() - t0:.1f}s device={dev} cus={ncu} queries={n}", file=sys.stderr, flush=True)
 
     for w in range(args.warmup):
@@ -112,7 +120,8 @@ def main():
         "config": {"workload": f"glint SMS, {args.width}x{args.height}x{args.spp}spp footprint queries, "
                                f"{args.nm_res}^2 RGB8 scratch+bump normal map, unbiased (max_trials 1e5)",
                    "queries_per_step": n, "walks_per_step": tot["walks"] / args.steps,
-                   "walks_per_s": walks_per_s, "device": dev},
+                   "walks_per_s": walks_per_s, "device": dev, "schedule": sched,
+                   "tail_and_setup_ms": round(k2_ms / args.steps, 3)},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
@@ -133,22 +142,21 @@ def cpu_baseline(sc, cfg, ctx, n, args):
     q = ctx.glint_queries(args.spp, seed=1)
     threads = max(1, min(16, os.cpu_count() or 1))
     orc = oracle_lib.Oracle(sc, cfg, threads=threads)
-    # calibrate on a small sample, then size the timed sample to ~cpu_seconds
-    stride = 4096
+    # whole frames (same queries, per-frame seeds as the GPU steps) until ~cpu_seconds elapsed
+    walks = conv = frames = 0
     t = time.perf_counter()
-    _, st = orc.glint_estimate(np.ascontiguousarray(q[::stride]), 1)
-    dt = max(time.perf_counter() - t, 1e-3)
-    per_query = dt / len(q[::stride])
-    m = int(min(len(q), max(1000, args.cpu_seconds / per_query)))
-    stride = max(1, len(q) // m)
-    sample = np.ascontiguousarray(q[::stride])
-    t = time.perf_counter()
-    _, st = orc.glint_estimate(sample, 1)
-    dt = time.perf_counter() - t
+    while True:
+        _, st = orc.glint_estimate(q, 1 + frames)
+        walks += st["walks"]
+        conv += st["walks_converged"]
+        frames += 1
+        dt = time.perf_counter() - t
+        if dt >= args.cpu_seconds or frames >= 1000:
+            break
     orc.close()
-    return {"value": st["walks_converged"] / dt, "unit": "walks/s", "cores": threads, "kind": "port",
-            "sample": f"every {stride}th query of the frame ({len(sample)} queries, {st['walks']} walks, "
-                      f"{dt:.1f}s on {threads} threads)", "walks_per_s": st["walks"] / dt}
+    return {"value": conv / dt, "unit": "walks/s", "cores": threads, "kind": "port",
+            "sample": f"{frames} whole frames of the same workload ({len(q)} queries each, {walks} walks, "
+                      f"{dt:.1f}s on {threads} threads)", "walks_per_s": walks / dt}
 
 
 if __name__ == "__main__":

@@ -178,6 +178,31 @@ typedef struct sms_ss_walk_out {
     float p_init[3];           /* si_init.p                                                */
 } sms_ss_walk_out;
 
+/* One multi-scatter walk = SpecularManifoldMultiScatter::sample_path
+   (src/librender/manifold_ms.cpp:203-227): seed point on `shape`, a ray traced through it
+   scattering (reflect/refract) at every caster/bouncer for `bounces` vertices, then the
+   block-tridiagonal Newton solve with chain reprojection.  first_path = 0 requires the
+   chain to hit seed_shapes[i] at vertex i (manifold_ms.cpp:346-351,884-887). */
+#define SMS_MS_MAX_BOUNCES 4
+typedef struct sms_ms_walk_in {
+    float si_p[3];
+    int32_t shape;
+    sms_emitter_interaction ei;
+    float sample[2];
+    float p_start[3];
+    int32_t first_path;
+    int32_t seed_shapes[SMS_MS_MAX_BOUNCES];
+} sms_ms_walk_in;
+
+typedef struct sms_ms_walk_out {
+    int32_t success;
+    int32_t iterations;
+    int32_t seed_valid;        /* sample_seed_path succeeded                               */
+    int32_t n_vertices;
+    float p[SMS_MS_MAX_BOUNCES][3];      /* solution chain (valid when success)            */
+    int32_t shapes[SMS_MS_MAX_BOUNCES];  /* shapes of the seed chain                       */
+} sms_ms_walk_out;
+
 /* One glint walk = SpecularManifoldGlints::sample_glint (manifold_glints.cpp:225-265).
    xi_start == (1,1) means "no override" (:253-255). */
 typedef struct sms_glint_walk_in {
@@ -274,6 +299,8 @@ sms_status sms_ss_walk_batch(sms_ctx *ctx, const sms_ss_walk_in *in, int64_t n,
                              sms_ss_walk_out *out);
 sms_status sms_glint_walk_batch(sms_ctx *ctx, const sms_glint_walk_in *in, int64_t n,
                                 sms_glint_walk_out *out);
+sms_status sms_ms_walk_batch(sms_ctx *ctx, const sms_ms_walk_in *in, int64_t n,
+                             sms_ms_walk_out *out);
 
 /* Throughput entry points: the whole estimator per query, run by a persistent kernel that
    refills idle lanes from a device work queue.  Random numbers are drawn from per-(query,
@@ -283,6 +310,10 @@ sms_status sms_ss_estimate_batch(sms_ctx *ctx, const sms_ss_query *q, int64_t n,
                                  float *out_rgb, sms_stats *stats);
 sms_status sms_glint_estimate_batch(sms_ctx *ctx, const sms_glint_query *q, int64_t n,
                                     uint64_t seed, float *out_rgb, sms_stats *stats);
+/* Multi-scatter estimator (SpecularManifoldMultiScatter::specular_manifold_sampling,
+   manifold_ms.cpp:27-201) over receiver queries; config.bounces vertices per chain. */
+sms_status sms_ms_estimate_batch(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed,
+                                 float *out_rgb, sms_stats *stats);
 
 /* Camera-ray generation for glint frames: for every pixel sample of a width x height x spp
    frame, trace the primary ray differential, intersect the glinty shape and write the
@@ -300,6 +331,10 @@ sms_status sms_ss_walk_batch_host(sms_ctx *ctx, const sms_ss_walk_in *in, int64_
                                   sms_ss_walk_out *out);
 sms_status sms_glint_walk_batch_host(sms_ctx *ctx, const sms_glint_walk_in *in, int64_t n,
                                      sms_glint_walk_out *out);
+sms_status sms_ms_walk_batch_host(sms_ctx *ctx, const sms_ms_walk_in *in, int64_t n,
+                                  sms_ms_walk_out *out);
+sms_status sms_ms_estimate_batch_host(sms_ctx *ctx, const sms_ss_query *q, int64_t n,
+                                      uint64_t seed, float *out_rgb, sms_stats *stats);
 sms_status sms_ss_estimate_batch_host(sms_ctx *ctx, const sms_ss_query *q, int64_t n,
                                       uint64_t seed, float *out_rgb, sms_stats *stats);
 sms_status sms_glint_estimate_batch_host(sms_ctx *ctx, const sms_glint_query *q, int64_t n,

@@ -1598,6 +1598,558 @@ static void stats_out(const Stats &s, sms_stats *o) {
     o->bernoulli_calls = s.bcalls; o->bernoulli_trials = s.btrials; o->bernoulli_max = s.bmax;
 }
 
+/* ================================================================ multi scatter */
+static const int MS_MAXB = 4;
+
+struct MSVert { MV v; V2 C; M2 dprev, dcur, dnext, tmp, inv_lambda; V2 dx; };
+struct MSPath { int n; MSVert v[MS_MAXB]; int shapes[MS_MAXB]; };
+
+/* Interaction::spawn_ray (include/mitsuba/render/interaction.h:58-61) */
+static inline Ray spawn_ray(V3 p, V3 d) { return make_ray(p, d, (1.f + hmax_abs(p)) * RAY_EPS_F, INF_F); }
+
+static inline V3 offset_normal_world(const MV &v) { return (v.s * 0.f + v.t * 0.f) + v.n * 1.f; }
+
+/* sample_seed_path (manifold_ms.cpp:280-408) with smooth interfaces (offset normal (0,0,1)).
+   The reference's bounce-0 target-shape test compares the shape with itself (:332) and so
+   never rejects; this restatement keeps that behaviour. */
+static bool ms_seed_path(const Scene &S, int shape, V3 si_p, const EI &ei, V2 sample, bool first_path,
+                         const int *seed_shapes, V3 p_start, MSPath &P) {
+    P.n = 0;
+    V3 x0 = si_p;
+    V3 x1 = sample_position(S, shape, sample).p;
+    if (p_start.x != 0.f || p_start.y != 0.f || p_start.z != 0.f) x1 = p_start;
+    V3 wo = normalize(x1 - x0);
+    if (S.cfg.mnee_init) {
+        const Shape &sh = S.shapes[shape];
+        wo = sh.k != 0.f ? normalize(sh.bbox_center - x0) : normalize(ei.p - x0);
+    }
+    Ray ray = make_ray(x0, wo);
+    while (true) {
+        int bounce = P.n;
+        if (bounce >= S.cfg.bounces) break;
+        SI si = ray_intersect(S, ray);
+        if (!si.valid) return false;
+        int flags = S.shapes[si.shape].flags;
+        if (!(flags & SMS_FLAG_CAUSTIC_CASTER_MS) && !(flags & SMS_FLAG_CAUSTIC_BOUNCER)) return false;
+        if (!first_path && si.shape != seed_shapes[bounce]) return false;
+        MV vertex = mv_from_si(S, si);
+        P.v[bounce].v = vertex;
+        P.shapes[bounce] = si.shape;
+        P.n++;
+        if (!S.cfg.mnee_init) {
+            V3 m = offset_normal_world(vertex);
+            V3 wi = -wo;
+            if (vertex.eta == 1.f) wo = reflect(wi, m);
+            else if (!refract(wi, m, vertex.eta, &wo)) return false;
+        }
+        ray = spawn_ray(si.p, wo);
+    }
+    return true;
+}
+
+static inline bool invert_exact(M2 a, M2 *inv) {
+    if (fabsf(det2(a)) == 0.f) return false;
+    *inv = inverse2(a);
+    return true;
+}
+
+/* invert_tridiagonal_step (manifold_ms.cpp:1192-1232): block Thomas elimination */
+static bool ms_invert_tridiagonal_step(MSPath &P) {
+    int n = P.n;
+    if (n == 0) return true;
+    P.v[0].tmp = P.v[0].dprev;
+    if (!invert_exact(P.v[0].dcur, &P.v[0].inv_lambda)) return false;
+    for (int i = 1; i < n; ++i) {
+        P.v[i].tmp = mulm2(P.v[i].dprev, P.v[i - 1].inv_lambda);
+        M2 m = subm2(P.v[i].dcur, mulm2(P.v[i].tmp, P.v[i - 1].dnext));
+        if (!invert_exact(m, &P.v[i].inv_lambda)) return false;
+    }
+    P.v[0].dx = P.v[0].C;
+    for (int i = 1; i < n; ++i) P.v[i].dx = P.v[i].C - mulv2(P.v[i].tmp, P.v[i - 1].dx);
+    P.v[n - 1].dx = mulv2(P.v[n - 1].inv_lambda, P.v[n - 1].dx);
+    for (int i = n - 2; i >= 0; --i) P.v[i].dx = mulv2(P.v[i].inv_lambda, P.v[i].dx - mulv2(P.v[i].dnext, P.v[i + 1].dx));
+    return true;
+}
+
+static inline float wrap_pi(float dp) {
+    if (dp < -PI_F) dp += 2.f * PI_F; else if (dp > PI_F) dp -= 2.f * PI_F;
+    return dp;
+}
+static inline bool transform_w(V3 w, V3 n, float eta, V3 *out) {
+    if (eta == 1.f) { *out = reflect(w, n); return true; }
+    return refract(w, n, eta, out);
+}
+static inline void d_transform_w(V3 w, V3 du, V3 dv, V3 n, V3 dn_du, V3 dn_dv, float eta, V3 *ou, V3 *ov) {
+    if (eta == 1.f) d_reflect(w, du, dv, n, dn_du, dn_dv, ou, ov);
+    else d_refract(w, du, dv, n, dn_du, dn_dv, eta, ou, ov);
+}
+
+/* compute_step_anglediff MS (manifold_ms.cpp:631-857) */
+static bool ms_step_anglediff(MSPath &P, V3 x0, const EI &ei) {
+    int k = P.n;
+    bool success = true;
+    const V3 zero = v3(0.f, 0.f, 0.f);
+    for (int i = 0; i < k; ++i) {
+        MSVert &vi = P.v[i];
+        const MV &v = vi.v;
+        vi.C = v2(0.f, 0.f);
+        vi.dprev = vi.dcur = vi.dnext = m2(0.f, 0.f, 0.f, 0.f);
+        V3 x_prev = i == 0 ? x0 : P.v[i - 1].v.p;
+        V3 x_next = i == k - 1 ? ei.p : P.v[i + 1].v.p;
+        V3 x_cur = v.p;
+        bool fixed = i == k - 1 && ei_is_directional(ei);
+        V3 wo = fixed ? ei.d : x_next - x_cur;
+        float ilo = norm(wo);
+        if (ilo < 1e-3f) return false;
+        ilo = 1.f / ilo;
+        wo = wo * ilo;
+        V3 dwo_du_cur, dwo_dv_cur;
+        if (fixed) { dwo_du_cur = zero; dwo_dv_cur = zero; }
+        else {
+            dwo_du_cur = -ilo * (v.dp_du - wo * dot(wo, v.dp_du));
+            dwo_dv_cur = -ilo * (v.dp_dv - wo * dot(wo, v.dp_dv));
+        }
+        V3 wi = x_prev - x_cur;
+        float ili = norm(wi);
+        if (ili < 1e-3f) return false;
+        ili = 1.f / ili;
+        wi = wi * ili;
+        V3 dwi_du_cur = -ili * (v.dp_du - wi * dot(wi, v.dp_du)), dwi_dv_cur = -ili * (v.dp_dv - wi * dot(wi, v.dp_dv));
+        bool success_i = false;
+        V3 n = offset_normal_world(v);
+        V3 dn_du = (v.ds_du * 0.f + v.dt_du * 0.f) + v.dn_du * 1.f;
+        V3 dn_dv = (v.ds_dv * 0.f + v.dt_dv * 0.f) + v.dn_dv * 1.f;
+        V3 wio;
+        if (transform_w(wi, n, v.eta, &wio)) {
+            float to, po, tio, pio;
+            sphcoords(wo, &to, &po); sphcoords(wio, &tio, &pio);
+            vi.C = v2(to - tio, wrap_pi(po - pio));
+            float dto_du, dpo_du, dto_dv, dpo_dv, dtio_du, dpio_du, dtio_dv, dpio_dv;
+            if (i > 0) {
+                const MV &vp = P.v[i - 1].v;
+                V3 dwi_du_prev = ili * (vp.dp_du - wi * dot(wi, vp.dp_du)), dwi_dv_prev = ili * (vp.dp_dv - wi * dot(wi, vp.dp_dv));
+                V3 a, b;
+                d_transform_w(wi, dwi_du_prev, dwi_dv_prev, n, zero, zero, v.eta, &a, &b);
+                d_sphcoords(wio, a, b, &dtio_du, &dpio_du, &dtio_dv, &dpio_dv);
+                vi.dprev = m2(-dtio_du, -dtio_dv, -dpio_du, -dpio_dv);
+            }
+            V3 a, b;
+            d_transform_w(wi, dwi_du_cur, dwi_dv_cur, n, dn_du, dn_dv, v.eta, &a, &b);
+            d_sphcoords(wo, dwo_du_cur, dwo_dv_cur, &dto_du, &dpo_du, &dto_dv, &dpo_dv);
+            d_sphcoords(wio, a, b, &dtio_du, &dpio_du, &dtio_dv, &dpio_dv);
+            vi.dcur = m2(dto_du - dtio_du, dto_dv - dtio_dv, dpo_du - dpio_du, dpo_dv - dpio_dv);
+            if (i < k - 1) {
+                const MV &vn = P.v[i + 1].v;
+                V3 dwo_du_next = ilo * (vn.dp_du - wo * dot(wo, vn.dp_du)), dwo_dv_next = ilo * (vn.dp_dv - wo * dot(wo, vn.dp_dv));
+                d_sphcoords(wo, dwo_du_next, dwo_dv_next, &dto_du, &dpo_du, &dto_dv, &dpo_dv);
+                vi.dnext = m2(dto_du, dto_dv, dpo_du, dpo_dv);
+            }
+            success_i = true;
+        }
+        V3 woi;
+        if (transform_w(wo, n, v.eta, &woi) && !success_i) {
+            float ti, pi, toi, poi;
+            sphcoords(wi, &ti, &pi); sphcoords(woi, &toi, &poi);
+            vi.C = v2(ti - toi, wrap_pi(pi - poi));
+            float dti_du, dpi_du, dti_dv, dpi_dv, dtoi_du, dpoi_du, dtoi_dv, dpoi_dv;
+            if (i > 0) {
+                const MV &vp = P.v[i - 1].v;
+                V3 dwi_du_prev = ili * (vp.dp_du - wi * dot(wi, vp.dp_du)), dwi_dv_prev = ili * (vp.dp_dv - wi * dot(wi, vp.dp_dv));
+                d_sphcoords(wi, dwi_du_prev, dwi_dv_prev, &dti_du, &dpi_du, &dti_dv, &dpi_dv);
+                vi.dprev = m2(dti_du, dti_dv, dpi_du, dpi_dv);
+            }
+            V3 a, b;
+            d_transform_w(wo, dwo_du_cur, dwo_dv_cur, n, dn_du, dn_dv, v.eta, &a, &b);
+            d_sphcoords(wi, dwi_du_cur, dwi_dv_cur, &dti_du, &dpi_du, &dti_dv, &dpi_dv);
+            d_sphcoords(woi, a, b, &dtoi_du, &dpoi_du, &dtoi_dv, &dpoi_dv);
+            vi.dcur = m2(dti_du - dtoi_du, dti_dv - dtoi_dv, dpi_du - dpoi_du, dpi_dv - dpoi_dv);
+            if (i < k - 1) {
+                const MV &vn = P.v[i + 1].v;
+                V3 dwo_du_next = ilo * (vn.dp_du - wo * dot(wo, vn.dp_du)), dwo_dv_next = ilo * (vn.dp_dv - wo * dot(wo, vn.dp_dv));
+                V3 c, d;
+                d_transform_w(wo, dwo_du_next, dwo_dv_next, n, zero, zero, v.eta, &c, &d);
+                d_sphcoords(woi, c, d, &dtoi_du, &dpoi_du, &dtoi_dv, &dpoi_dv);
+                vi.dnext = m2(-dtoi_du, -dtoi_dv, -dpoi_du, -dpoi_dv);
+            }
+            success_i = true;
+        }
+        success = success && success_i;
+    }
+    if (!success || !ms_invert_tridiagonal_step(P)) return false;
+    return true;
+}
+
+/* compute_step_halfvector MS (manifold_ms.cpp:500-629), smooth interfaces: offset normal
+   (0,0,1), so the constraint is the tangential part of the generalized half-vector */
+static bool ms_step_halfvector(MSPath &P, V3 x0, const EI &ei) {
+    int k = P.n;
+    for (int i = 0; i < k; ++i) {
+        MSVert &vi = P.v[i];
+        const MV &v = vi.v;
+        vi.C = v2(0.f, 0.f);
+        vi.dprev = vi.dcur = vi.dnext = m2(0.f, 0.f, 0.f, 0.f);
+        V3 x_prev = i == 0 ? x0 : P.v[i - 1].v.p;
+        V3 x_next = i == k - 1 ? ei.p : P.v[i + 1].v.p;
+        bool fixed = i == k - 1 && ei_is_directional(ei);
+        V3 wo = fixed ? ei.d : x_next - v.p;
+        float ilo = norm(wo);
+        if (ilo < 1e-3f) return false;
+        ilo = 1.f / ilo;
+        wo = wo * ilo;
+        V3 wi = x_prev - v.p;
+        float ili = norm(wi);
+        if (ili < 1e-3f) return false;
+        ili = 1.f / ili;
+        wi = wi * ili;
+        float eta = v.eta;
+        if (dot(wi, v.gn) < 0.f) eta = 1.f / eta;
+        V3 h = wi + eta * wo;
+        if (eta != 1.f) h = h * -1.f;
+        float ilh = 1.f / norm(h);
+        h = h * ilh;
+        ilo *= eta * ilh; ili *= ilh;
+        V3 dh_du, dh_dv;
+        if (i > 0) {
+            const MV &vp = P.v[i - 1].v;
+            dh_du = ili * (vp.dp_du - wi * dot(wi, vp.dp_du));
+            dh_dv = ili * (vp.dp_dv - wi * dot(wi, vp.dp_dv));
+            dh_du = dh_du - h * dot(dh_du, h); dh_dv = dh_dv - h * dot(dh_dv, h);
+            if (eta != 1.f) { dh_du = dh_du * -1.f; dh_dv = dh_dv * -1.f; }
+            vi.dprev = m2(dot(v.s, dh_du), dot(v.s, dh_dv), dot(v.t, dh_du), dot(v.t, dh_dv));
+        }
+        if (fixed) {
+            dh_du = ili * (-v.dp_du + wi * dot(wi, v.dp_du));
+            dh_dv = ili * (-v.dp_dv + wi * dot(wi, v.dp_dv));
+        } else {
+            dh_du = (-v.dp_du * (ili + ilo) + wi * (dot(wi, v.dp_du) * ili)) + wo * (dot(wo, v.dp_du) * ilo);
+            dh_dv = (-v.dp_dv * (ili + ilo) + wi * (dot(wi, v.dp_dv) * ili)) + wo * (dot(wo, v.dp_dv) * ilo);
+        }
+        dh_du = dh_du - h * dot(dh_du, h); dh_dv = dh_dv - h * dot(dh_dv, h);
+        if (eta != 1.f) { dh_du = dh_du * -1.f; dh_dv = dh_dv * -1.f; }
+        vi.dcur = m2(dot(v.ds_du, h) + dot(v.s, dh_du), dot(v.ds_dv, h) + dot(v.s, dh_dv),
+                     dot(v.dt_du, h) + dot(v.t, dh_du), dot(v.dt_dv, h) + dot(v.t, dh_dv));
+        if (i < k - 1) {
+            const MV &vn = P.v[i + 1].v;
+            dh_du = ilo * (vn.dp_du - wo * dot(wo, vn.dp_du));
+            dh_dv = ilo * (vn.dp_dv - wo * dot(wo, vn.dp_dv));
+            dh_du = dh_du - h * dot(dh_du, h); dh_dv = dh_dv - h * dot(dh_dv, h);
+            if (eta != 1.f) { dh_du = dh_du * -1.f; dh_dv = dh_dv * -1.f; }
+            vi.dnext = m2(dot(v.s, dh_du), dot(v.s, dh_dv), dot(v.t, dh_du), dot(v.t, dh_dv));
+        }
+        vi.C = v2(dot(v.s, h) - 0.f, dot(v.t, h) - 0.f);
+    }
+    return ms_invert_tridiagonal_step(P);
+}
+
+/* reproject (manifold_ms.cpp:859-918): re-trace the chain through the proposed first vertex */
+static bool ms_reproject(const Scene &S, V3 si_p, V3 x1, const int *seed_shapes, int n_seed, MSPath &Q) {
+    Q.n = 0;
+    V3 wo = normalize(x1 - si_p);
+    Ray ray = make_ray(si_p, wo);
+    while (true) {
+        int bounce = Q.n;
+        if (bounce >= S.cfg.bounces) break;
+        SI si = ray_intersect(S, ray);
+        if (!si.valid) return false;
+        if (si.shape != seed_shapes[bounce]) return false;
+        MV vertex = mv_from_si(S, si);
+        Q.v[bounce].v = vertex;
+        Q.shapes[bounce] = si.shape;
+        Q.n++;
+        V3 m = offset_normal_world(vertex);
+        V3 wi = -wo;
+        if (vertex.eta == 1.f) wo = reflect(wi, m);
+        else if (!refract(wi, m, vertex.eta, &wo)) return false;
+        ray = spawn_ray(si.p, wo);
+    }
+    return Q.n == n_seed;
+}
+
+struct MSWalk { bool ok, seed_ok; int iterations; MSPath path; int seed_shapes[MS_MAXB]; };
+
+/* sample_path MS (manifold_ms.cpp:203-227) + newton_solver (:410-498).  With first_path the
+   seed chain's shapes become the template later reprojections must hit. */
+static MSWalk ms_sample_path(const Scene &S, int shape, V3 si_p, const EI &ei, V2 sample, bool first_path,
+                             const int *seed_shapes_in, V3 p_start) {
+    const sms_config &cfg = S.cfg;
+    MSWalk W;
+    W.ok = false; W.seed_ok = false; W.iterations = 0;
+    for (int i = 0; i < MS_MAXB; ++i) W.seed_shapes[i] = seed_shapes_in ? seed_shapes_in[i] : -1;
+    if (!ms_seed_path(S, shape, si_p, ei, sample, first_path, W.seed_shapes, p_start, W.path)) return W;
+    W.seed_ok = true;
+    if (first_path) for (int i = 0; i < W.path.n; ++i) W.seed_shapes[i] = W.path.shapes[i];
+    int n_seed = W.path.n;
+    bool success = false, needs_step_update = true;
+    int iterations = 0;
+    float beta = 1.f;
+    MSPath Q;
+    while (iterations < cfg.max_iterations) {
+        bool step_success = true;
+        if (needs_step_update)
+            step_success = cfg.halfvector_constraints ? ms_step_halfvector(W.path, si_p, ei)
+                                                      : ms_step_anglediff(W.path, si_p, ei);
+        if (!step_success) break;
+        bool converged = true;
+        for (int i = 0; i < W.path.n; ++i)
+            if (norm2(W.path.v[i].C) > cfg.solver_threshold) { converged = false; break; }
+        if (converged) { success = true; break; }
+        V3 prop0 = W.path.v[0].v.p - cfg.step_scale * beta * (W.path.v[0].v.dp_du * W.path.v[0].dx.x +
+                                                              W.path.v[0].v.dp_dv * W.path.v[0].dx.y);
+        if (!ms_reproject(S, si_p, prop0, W.seed_shapes, n_seed, Q)) {
+            beta = 0.5f * beta;
+            needs_step_update = false;
+        } else {
+            beta = std::min(1.f, 2.f * beta);
+            W.path = Q;
+            needs_step_update = true;
+        }
+        iterations++;
+    }
+    W.iterations = iterations;
+    if (!success) return W;
+    int n = W.path.n;
+    for (int i = 0; i < n; ++i) {
+        V3 x_prev = i == 0 ? si_p : W.path.v[i - 1].v.p;
+        V3 x_next = i == n - 1 ? ei.p : W.path.v[i + 1].v.p;
+        V3 x_cur = W.path.v[i].v.p;
+        bool fixed = i == n - 1 && ei_is_directional(ei);
+        V3 wi = normalize(x_prev - x_cur);
+        V3 wo = fixed ? ei.d : normalize(x_next - x_cur);
+        float ci = dot(W.path.v[i].v.gn, wi), co = dot(W.path.v[i].v.gn, wo);
+        bool refraction = ci * co < 0.f, reflection = !refraction;
+        float eta = W.path.v[i].v.eta;
+        if ((eta == 1.f && !reflection) || (eta != 1.f && !refraction)) return W;
+    }
+    W.ok = true;
+    return W;
+}
+
+/* specular_reflectance MS, smooth interfaces (manifold_ms.cpp:920-1023), with the final
+   visibility test towards the light */
+static V3 ms_specular_reflectance(const Scene &S, V3 si_p, const EI &ei, const MSPath &P) {
+    if (P.n == 0) return v3(0.f, 0.f, 0.f);
+    V3 wo = normalize(P.v[0].v.p - si_p);
+    Ray ray = make_ray(si_p, wo);
+    float val = 1.f;
+    SI si = si_invalid();
+    for (int k = 0; k < P.n; ++k) {
+        si = ray_intersect(S, ray);
+        if (!si.valid) return v3(0.f, 0.f, 0.f);
+        const Shape &sh = S.shapes[si.shape];
+        V3 w;
+        if (k < P.n - 1) w = normalize(P.v[k + 1].v.p - si.p);
+        else w = ei_is_directional(ei) ? ei.d : normalize(ei.p - si.p);
+        float cos_theta = dot(si.sh_n, w);
+        float f;
+        if (sh.k != 0.f) f = fresnel_conductor(fabsf(cos_theta), sh.eta_r, sh.k);
+        else {
+            float eta = hmean_same(sh.eta_r);
+            if (cos_theta < 0.f) eta = 1.f / eta;
+            f = (1.f - fresnel(cos_theta, eta)) * (eta * eta);
+        }
+        val = val * f;
+        ray = make_ray(si.p, w);
+    }
+    if (!ei_is_directional(ei)) {
+        V3 d = ei.p - si.p;
+        ray = make_ray(si.p, normalize(d), RAY_EPS_F * (1.f + hmax_abs(si.p)), norm(d) * (1.f - RAY_EPS_F));
+    }
+    if (ray_test(S, ray)) return v3(0.f, 0.f, 0.f);
+    return v3(val, val, val);
+}
+
+/* invert_tridiagonal_geo (manifold_ms.cpp:1234-1268) */
+static float ms_invert_tridiagonal_geo(MSVert *v, int n) {
+    if (n == 0) return 0.f;
+    M2 Li;
+    if (!invert_exact(v[0].dcur, &Li)) return 0.f;
+    for (int i = 0; i < n - 2; ++i) {
+        v[i].tmp = mulm2(Li, v[i].dnext);
+        M2 m = subm2(v[i + 1].dcur, mulm2(v[i + 1].dprev, v[i].tmp));
+        if (!invert_exact(m, &Li)) return 0.f;
+    }
+    v[n - 2].inv_lambda = mulm2(negm2(Li), v[n - 2].dnext);
+    for (int i = n - 3; i >= 0; --i) v[i].inv_lambda = mulm2(negm2(v[i].tmp), v[i + 1].inv_lambda);
+    return fabsf(det2(negm2(v[0].inv_lambda)));
+}
+
+/* geometric_term MS (manifold_ms.cpp:1025-1190) */
+static float ms_geometric_term(const MV &vx, const MV &vy, const MSPath &P) {
+    MSVert v[MS_MAXB + 2];
+    int k = 0;
+    if (vy.fixed_direction) {
+        v[k++].v = vy;
+        for (int i = P.n - 1; i >= 0; --i) v[k++].v = P.v[i].v;
+        v[k++].v = vx;
+    } else {
+        for (int i = 0; i < P.n; ++i) v[k++].v = P.v[i].v;
+        v[k++].v = vy;
+    }
+    for (int i = 0; i < k - 1; ++i) {
+        v[i].dprev = v[i].dcur = v[i].dnext = m2(0.f, 0.f, 0.f, 0.f);
+        const MV &c = v[i].v, &nx = v[i + 1].v;
+        V3 wo = nx.p - c.p;
+        float ilo = norm(wo);
+        if (ilo < 1e-3f) return 0.f;
+        ilo = 1.f / ilo;
+        wo = wo * ilo;
+        if (c.fixed_direction) {
+            V3 dc_du_cur = -ilo * (c.dp_du - wo * dot(wo, c.dp_du)), dc_dv_cur = -ilo * (c.dp_dv - wo * dot(wo, c.dp_dv));
+            v[i].dcur = m2(dot(dc_du_cur, c.dp_du), dot(dc_dv_cur, c.dp_du), dot(dc_du_cur, c.dp_dv), dot(dc_dv_cur, c.dp_dv));
+            V3 dc_du_next = ilo * (nx.dp_du - wo * dot(wo, nx.dp_du)), dc_dv_next = ilo * (nx.dp_dv - wo * dot(wo, nx.dp_dv));
+            v[i].dnext = m2(dot(dc_du_next, c.dp_du), dot(dc_dv_next, c.dp_du), dot(dc_du_next, c.dp_dv), dot(dc_dv_next, c.dp_dv));
+            continue;
+        }
+        V3 x_prev = i == 0 ? vx.p : v[i - 1].v.p;
+        V3 wi = x_prev - c.p;
+        float ili = norm(wi);
+        if (ili < 1e-3f) return 0.f;
+        ili = 1.f / ili;
+        wi = wi * ili;
+        float eta = c.eta;
+        if (dot(wi, c.gn) < 0.f) eta = 1.f / eta;
+        V3 h = wi + eta * wo;
+        if (eta != 1.f) h = h * -1.f;
+        float ilh = 1.f / norm(h);
+        h = h * ilh;
+        ilo *= eta * ilh; ili *= ilh;
+        float dot_dpdu_n = dot(c.dp_du, c.n), dot_dpdv_n = dot(c.dp_dv, c.n);
+        V3 s = c.dp_du - dot_dpdu_n * c.n, t = c.dp_dv - dot_dpdv_n * c.n;
+        V3 dh_du, dh_dv;
+        if (i > 0) {
+            const MV &pv = v[i - 1].v;
+            dh_du = ili * (pv.dp_du - wi * dot(wi, pv.dp_du));
+            dh_dv = ili * (pv.dp_dv - wi * dot(wi, pv.dp_dv));
+            dh_du = dh_du - h * dot(dh_du, h); dh_dv = dh_dv - h * dot(dh_dv, h);
+            if (eta != 1.f) { dh_du = dh_du * -1.f; dh_dv = dh_dv * -1.f; }
+            v[i].dprev = m2(dot(s, dh_du), dot(s, dh_dv), dot(t, dh_du), dot(t, dh_dv));
+        }
+        dh_du = (-c.dp_du * (ili + ilo) + wi * (dot(wi, c.dp_du) * ili)) + wo * (dot(wo, c.dp_du) * ilo);
+        dh_dv = (-c.dp_dv * (ili + ilo) + wi * (dot(wi, c.dp_dv) * ili)) + wo * (dot(wo, c.dp_dv) * ilo);
+        dh_du = dh_du - h * dot(dh_du, h); dh_dv = dh_dv - h * dot(dh_dv, h);
+        if (eta != 1.f) { dh_du = dh_du * -1.f; dh_dv = dh_dv * -1.f; }
+        float dot_h_n = dot(h, c.n), dot_h_dndu = dot(h, c.dn_du), dot_h_dndv = dot(h, c.dn_dv);
+        v[i].dcur = m2((dot(dh_du, s) - dot(c.dp_du, c.dn_du) * dot_h_n) - dot_dpdu_n * dot_h_dndu,
+                       (dot(dh_dv, s) - dot(c.dp_du, c.dn_dv) * dot_h_n) - dot_dpdu_n * dot_h_dndv,
+                       (dot(dh_du, t) - dot(c.dp_dv, c.dn_du) * dot_h_n) - dot_dpdv_n * dot_h_dndu,
+                       (dot(dh_dv, t) - dot(c.dp_dv, c.dn_dv) * dot_h_n) - dot_dpdv_n * dot_h_dndv);
+        dh_du = ilo * (nx.dp_du - wo * dot(wo, nx.dp_du));
+        dh_dv = ilo * (nx.dp_dv - wo * dot(wo, nx.dp_dv));
+        dh_du = dh_du - h * dot(dh_du, h); dh_dv = dh_dv - h * dot(dh_dv, h);
+        if (eta != 1.f) { dh_du = dh_du * -1.f; dh_dv = dh_dv * -1.f; }
+        v[i].dnext = m2(dot(s, dh_du), dot(s, dh_dv), dot(t, dh_du), dot(t, dh_dv));
+    }
+    if (vy.fixed_direction) {
+        float G = ms_invert_tridiagonal_geo(v, k);
+        V3 d = normalize(v[k - 1].v.p - v[k - 2].v.p);
+        return G / fabsf(dot(d, v[k - 1].v.n));
+    }
+    float dx1_dxend = ms_invert_tridiagonal_geo(v, k);
+    V3 d = vx.p - v[0].v.p;
+    float inv_r2 = 1.f / sqnorm(d);
+    d = d * sqrtf(inv_r2);
+    float dw0_dx1 = fabsf(dot(d, v[0].v.gn)) * inv_r2;
+    return dw0_dx1 * dx1_dxend;
+}
+
+/* evaluate_path_contribution MS (manifold_ms.cpp:229-278) */
+static V3 ms_path_contribution(const Scene &S, const SI &si, const EI &ei_, MSPath P) {
+    const MV &last = P.v[P.n - 1].v;
+    EI ei = ei_;
+    if (ei.kind == SMS_EI_POINT) {
+        const Emitter &em = S.emitters[ei.emitter];
+        V3 d = em.position - last.p;
+        float dist = norm(d), inv_dist = 1.f / dist;
+        d = d * inv_dist;
+        float emitter_pdf = ei.pdf;
+        ei.p = em.position; ei.d = d; ei.n = d; ei.pdf = 1.f;
+        ei.weight = (em.intensity * (inv_dist * inv_dist)) * dist * dist;
+        ei.pdf *= emitter_pdf;
+        ei.weight = ei.weight * (1.f / emitter_pdf);
+    }
+    MV vy;
+    if (!emitter_interaction_to_vertex(S, ei, last.p, &vy)) return v3(0.f, 0.f, 0.f);
+    for (int k = 0; k < P.n; ++k) make_orthonormal(P.v[k].v);
+    MV vx = mv_from_si(S, si);
+    make_orthonormal(vx);
+    V3 f = ms_specular_reflectance(S, si.p, ei, P);
+    float G = ms_geometric_term(vx, vy, P);
+    V3 thr = f * G;
+    return v3(thr.x * ei.weight.x, thr.y * ei.weight.y, thr.z * ei.weight.z);
+}
+
+/* specular_manifold_sampling MS (manifold_ms.cpp:27-201), per-(query, walk) PCG32 streams */
+static V3 estimate_ms(const Scene &S, const sms_ss_query &q, uint64_t seed, uint64_t qid, Stats &st) {
+    const sms_config &cfg = S.cfg;
+    SI si = si_from_query(q);
+    Pcg32 rq; stream_seed(&rq, seed, qid, 0);
+    std::vector<int> em_ms;
+    for (int i = 0; i < (int) S.emitters.size(); ++i) if (S.emitters[i].ms) em_ms.push_back(i);
+    EI ei = sample_emitter_interaction(S, si.p, em_ms, &rq);
+    V3 result = v3(0.f, 0.f, 0.f);
+    std::vector<int> casters;
+    for (int i = 0; i < (int) S.shapes.size(); ++i) if (S.shapes[i].flags & SMS_FLAG_CAUSTIC_CASTER_MS) casters.push_back(i);
+    for (size_t c = 0; c < casters.size(); ++c) {
+        int shape = casters[c];
+        uint64_t wbase = (uint64_t) c << 32;
+        V3 value = v3(0.f, 0.f, 0.f);
+        auto run = [&](uint64_t k, bool first, const int *shapes) {
+            Pcg32 rw; stream_seed(&rw, seed, qid, wbase + k + 1);
+            V2 s; s.x = pcg32_float(&rw); s.y = pcg32_float(&rw);
+            MSWalk w = ms_sample_path(S, shape, si.p, ei, s, first, shapes, v3(0.f, 0.f, 0.f));
+            st.walks++; st.iters += w.iterations; if (w.ok) st.conv++;
+            return w;
+        };
+        if (!cfg.biased) {
+            MSWalk w0 = run(0, true, nullptr);
+            if (!w0.ok) continue;
+            V3 direction = normalize(w0.path.v[0].v.p - si.p);
+            V3 spec = ms_path_contribution(S, si, ei, w0.path);
+            V3 dl = v3(dot(direction, si.sh_s), dot(direction, si.sh_t), dot(direction, si.sh_n));
+            V3 bsdf = receiver_bsdf_eval(S, si, dl);
+            float inv_prob = 1.f;
+            int iterations = 1;
+            uint64_t k = 1;
+            st.bcalls++;
+            while (true) {
+                MSWalk t = run(k++, false, w0.seed_shapes);
+                V3 dtr = t.path.n > 0 ? normalize(t.path.v[0].v.p - si.p) : v3(0.f, 0.f, 0.f);
+                if (t.ok && fabsf(dot(direction, dtr) - 1.f) < cfg.uniqueness_threshold) break;
+                inv_prob += 1.f;
+                iterations++;
+                if (cfg.max_trials > 0 && iterations > cfg.max_trials) { inv_prob = 0.f; break; }
+            }
+            st.btrials += iterations;
+            if ((uint64_t) iterations > st.bmax) st.bmax = iterations;
+            value = v3(bsdf.x * spec.x * inv_prob, bsdf.y * spec.y * inv_prob, bsdf.z * spec.z * inv_prob);
+        } else {
+            std::vector<V3> solutions;
+            bool have_seed = false;
+            int seed_shapes[MS_MAXB];
+            for (int m = 0; m < cfg.max_trials; ++m) {
+                MSWalk w = run((uint64_t) m, !have_seed, have_seed ? seed_shapes : nullptr);
+                if (!have_seed && w.seed_ok) {
+                    have_seed = true;
+                    for (int i = 0; i < MS_MAXB; ++i) seed_shapes[i] = w.seed_shapes[i];
+                }
+                if (!w.ok) continue;
+                V3 direction = normalize(w.path.v[0].v.p - si.p);
+                bool dup = false;
+                for (size_t j = 0; j < solutions.size(); ++j)
+                    if (fabsf(dot(direction, solutions[j]) - 1.f) < cfg.uniqueness_threshold) { dup = true; break; }
+                if (dup) continue;
+                solutions.push_back(direction);
+                V3 dl = v3(dot(direction, si.sh_s), dot(direction, si.sh_t), dot(direction, si.sh_n));
+                V3 bsdf = receiver_bsdf_eval(S, si, dl);
+                V3 spec = ms_path_contribution(S, si, ei, w.path);
+                value = value + v3(bsdf.x * spec.x, bsdf.y * spec.y, bsdf.z * spec.z);
+            }
+        }
+        result = result + value / 1.f;
+    }
+    return result;
+}
+
 } // namespace
 
 /* ================================================================ C entry points */
@@ -1681,6 +2233,43 @@ int oracle_glint_estimate_batch(void *h, const sms_glint_query *q, int64_t n, ui
     return 0;
 }
 
+int oracle_ms_walk_batch(void *h, const sms_ms_walk_in *in, int64_t n, sms_ms_walk_out *out, int threads) {
+    const Scene &S = *(Scene *) h;
+    if (S.cfg.bounces < 1 || S.cfg.bounces > MS_MAXB) return -5;
+    parallel_for(n, threads, [&](int64_t i) {
+        const sms_ms_walk_in &w = in[i];
+        EI ei = ei_from_c(w.ei);
+        MSWalk r = ms_sample_path(S, w.shape, v3(w.si_p[0], w.si_p[1], w.si_p[2]), ei, v2(w.sample[0], w.sample[1]),
+                                  w.first_path != 0, w.seed_shapes, v3(w.p_start[0], w.p_start[1], w.p_start[2]));
+        sms_ms_walk_out &o = out[i];
+        memset(&o, 0, sizeof(o));
+        o.success = r.ok; o.iterations = r.iterations; o.seed_valid = r.seed_ok; o.n_vertices = r.path.n;
+        for (int k = 0; k < MS_MAXB; ++k) {
+            o.shapes[k] = r.seed_ok ? r.seed_shapes[k] : -1;
+            if (k < r.path.n) { o.p[k][0] = r.path.v[k].v.p.x; o.p[k][1] = r.path.v[k].v.p.y; o.p[k][2] = r.path.v[k].v.p.z; }
+        }
+    });
+    return 0;
+}
+
+int oracle_ms_estimate_batch(void *h, const sms_ss_query *q, int64_t n, uint64_t seed, float *out_rgb,
+                             sms_stats *stats, int threads) {
+    const Scene &S = *(Scene *) h;
+    if (S.cfg.bounces < 1 || S.cfg.bounces > MS_MAXB) return -5;
+    std::vector<Stats> per((size_t) n);
+    parallel_for(n, threads, [&](int64_t i) {
+        Stats st; memset(&st, 0, sizeof(st));
+        V3 r = estimate_ms(S, q[i], seed, (uint64_t) i, st);
+        out_rgb[3 * i] = r.x; out_rgb[3 * i + 1] = r.y; out_rgb[3 * i + 2] = r.z;
+        per[(size_t) i] = st;
+    });
+    Stats tot; memset(&tot, 0, sizeof(tot));
+    for (auto &s : per) { tot.walks += s.walks; tot.conv += s.conv; tot.iters += s.iters; tot.bcalls += s.bcalls;
+        tot.btrials += s.btrials; if (s.bmax > tot.bmax) tot.bmax = s.bmax; }
+    stats_out(tot, stats);
+    return 0;
+}
+
 int oracle_glint_make_queries(void *h, int spp, uint64_t seed, sms_glint_query *q, int threads) {
     const Scene &S = *(Scene *) h;
     if (!S.has_cam) return -1;
@@ -1713,6 +2302,35 @@ int oracle_ss_step_at(void *h, int halfvector, const float *x, const float *o, c
     C[0] = st.C.x; C[1] = st.C.y; dX[0] = st.dX.x; dX[1] = st.dX.y;
     vtx_p[0] = v.p.x; vtx_p[1] = v.p.y; vtx_p[2] = v.p.z;
     return st.ok;
+}
+
+/* Block-tridiagonal kernels of the multi-scatter solver on caller-given blocks (row-major 2x2
+   per vertex: dprev, dcur, dnext; C two floats per vertex), exposed for checks against dense
+   linear algebra.  step: dx = A^{-1} C (returns 0 when a pivot block is singular);
+   geo: |det| of the vertex-0 block of -A'^{-1} B (manifold_ms.cpp:1234-1268). */
+int oracle_ms_tridiag_step(int n, const float *dprev, const float *dcur, const float *dnext, const float *C, float *dx) {
+    if (n < 1 || n > MS_MAXB) return -1;
+    MSPath P;
+    P.n = n;
+    for (int i = 0; i < n; ++i) {
+        P.v[i].dprev = m2(dprev[4 * i], dprev[4 * i + 1], dprev[4 * i + 2], dprev[4 * i + 3]);
+        P.v[i].dcur = m2(dcur[4 * i], dcur[4 * i + 1], dcur[4 * i + 2], dcur[4 * i + 3]);
+        P.v[i].dnext = m2(dnext[4 * i], dnext[4 * i + 1], dnext[4 * i + 2], dnext[4 * i + 3]);
+        P.v[i].C = v2(C[2 * i], C[2 * i + 1]);
+    }
+    if (!ms_invert_tridiagonal_step(P)) return 0;
+    for (int i = 0; i < n; ++i) { dx[2 * i] = P.v[i].dx.x; dx[2 * i + 1] = P.v[i].dx.y; }
+    return 1;
+}
+float oracle_ms_tridiag_geo(int n, const float *dprev, const float *dcur, const float *dnext) {
+    if (n < 2 || n > MS_MAXB + 2) return -1.f;
+    MSVert v[MS_MAXB + 2];
+    for (int i = 0; i < n; ++i) {
+        v[i].dprev = m2(dprev[4 * i], dprev[4 * i + 1], dprev[4 * i + 2], dprev[4 * i + 3]);
+        v[i].dcur = m2(dcur[4 * i], dcur[4 * i + 1], dcur[4 * i + 2], dcur[4 * i + 3]);
+        v[i].dnext = m2(dnext[4 * i], dnext[4 * i + 1], dnext[4 * i + 2], dnext[4 * i + 3]);
+    }
+    return ms_invert_tridiagonal_geo(v, n);
 }
 
 /* Math routines exposed for pinning against libm. */

@@ -31,8 +31,33 @@ SMS_DEV bool tri_test(const TriTest &t, V2 p) {
 struct GlintFootprint {
     V2 target, uv, duv_dx, duv_dy;
     TriTest t0, t1;              // (a,b,c) and (a,c,d) of the footprint parallelogram
-    const float2 *slope;
-    int res; float fres, tiles;
+    const float2 *slope;         // padded level-0 slopes
+    int res, stride; float fres, tiles;
+};
+
+// texel source of a walk: the padded slope array in global memory
+struct GlobalTexels {
+    const float2 *slope; int res, stride;
+    SMS_DEV Texel4 operator()(V2 uv) const { return nm_fetch(slope, res, stride, uv); }
+};
+
+// texel source of a wavefront whose walks share one footprint: a (T+1)^2 window of the padded
+// slope array staged in LDS at texel (x0, y0); lookups outside it read global memory
+template <int T>
+struct LdsTexels {
+    const float2 (*tile)[T + 1];
+    int x0, y0;
+    GlobalTexels g;
+    SMS_DEV Texel4 operator()(V2 uv) const {
+        NmCoord c = nm_coord(g.res, uv);
+        int lx = c.x - x0, ly = c.y - y0;
+        if (lx < 0 || ly < 0 || lx + 1 > T || ly + 1 > T) return g(uv);
+        float2 a = tile[ly][lx], b = tile[ly][lx + 1], d0 = tile[ly + 1][lx], d1 = tile[ly + 1][lx + 1];
+        Texel4 t;
+        t.v00 = v2(a.x, a.y); t.v10 = v2(b.x, b.y); t.v01 = v2(d0.x, d0.y); t.v11 = v2(d1.x, d1.y);
+        t.w = c.w;
+        return t;
+    }
 };
 
 // footprint corners (manifold_glints.cpp:303-307)
@@ -64,10 +89,11 @@ SMS_DEV void glint_walk_start(const GlintFootprint &fp, V2 xi, GlintWalk &w) {
 }
 
 // One iteration of newton_solver + compute_step (manifold_glints.cpp:313-373).
-SMS_DEV int glint_walk_iter(const GlintFootprint &fp, const sms_config &cfg, GlintWalk &w) {
+template <class Fetch>
+SMS_DEV int glint_walk_iter_f(const GlintFootprint &fp, const sms_config &cfg, GlintWalk &w, const Fetch &fetch) {
     if (w.it >= cfg.max_iterations) return WALK_FAILED;
     V2 tuv = tile_uv(w.uv, fp.tiles);
-    Texel4 tx = nm_fetch(fp.slope, fp.res, tuv);
+    Texel4 tx = fetch(tuv);
     V2 slope = nm_slope(tx);
     V2 tu, tv;
     nm_slope_deriv(tx, fp.fres, tu, tv);
@@ -87,6 +113,11 @@ SMS_DEV int glint_walk_iter(const GlintFootprint &fp, const sms_config &cfg, Gli
     w.uv = uv_prop;
     w.it++;
     return WALK_RUNNING;
+}
+SMS_DEV int glint_walk_iter(const GlintFootprint &fp, const sms_config &cfg, GlintWalk &w) {
+    GlobalTexels f;
+    f.slope = fp.slope; f.res = fp.res; f.stride = fp.stride;
+    return glint_walk_iter_f(fp, cfg, w, f);
 }
 
 // geometric_term glints (manifold_glints.cpp:411-554)

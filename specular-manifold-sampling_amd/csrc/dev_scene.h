@@ -31,7 +31,11 @@ struct DShape {
 
 struct DEmitter { int type, shape; float radiance[3], position[3], intensity[3]; int ss, ms; };
 
-struct DNormalmap { int res; int pad; const float2 *slope; };
+// Level-0 slopes stored with one replicated pad column and row ((res+1)^2 float2, row stride
+// res+1): texels (x, y), (x+1, y) of a bilinear footprint are adjacent for every x <= res-1,
+// and the pad reproduces the edge clamp of Normalmap::eval_normal (normalmap.h:67-72), so a
+// fetch is two 16-byte loads.
+struct DNormalmap { int res; int stride; const float2 *slope; };
 
 struct DCamera {
     float o[3], left[3], up[3], dir[3];
@@ -49,6 +53,8 @@ struct DScene {
     const int *emitters_ss; int n_emitters_ss;
     const int *emitters_all; int n_emitters_all;
     const int *casters_ss; int n_casters_ss;
+    const int *emitters_ms; int n_emitters_ms;
+    const int *casters_ms; int n_casters_ms;
     const DNormalmap *nmaps;
     const float4 *vpos;                  // mesh vertex positions (xyz, 0)
     const float4 *vnrm;                  // mesh vertex normals
@@ -352,8 +358,11 @@ SMS_DEV void normal_derivative(const DScene &S, const SI &si, V3 &dn_du, V3 &dn_
 
 // ---------------------------------------------------------------- normal map (level 0)
 struct Texel4 { V2 v00, v10, v01, v11, w; };
-// Normalmap::eval_normal / eval_normal_derivatives index computation (normalmap.h:62-80)
-SMS_DEV Texel4 nm_fetch(const float2 *__restrict__ slope, int res, V2 uv) {
+struct NmCoord { int x, y; V2 w; };
+// Normalmap::eval_normal / eval_normal_derivatives index computation (normalmap.h:62-80):
+// the lower-left texel (clamped to res-1) and the bilinear weights; the upper texel is
+// x+1 / y+1 in the padded layout.
+SMS_DEV NmCoord nm_coord(int res, V2 uv) {
     float fres = (float) res;
     float duv = 1.f / fres;
     V2 pos = v2((uv.x - 0.5f * duv) * fres, (uv.y - 0.5f * duv) * fres);
@@ -361,14 +370,20 @@ SMS_DEV Texel4 nm_fetch(const float2 *__restrict__ slope, int res, V2 uv) {
     pos.x = fmin_(fmax_(pos.x, 0.f), hi); pos.y = fmin_(fmax_(pos.y, 0.f), hi);
     uint32_t xmin = (uint32_t) floorf(pos.x), ymin = (uint32_t) floorf(pos.y);
     uint32_t lim = (uint32_t) (res - 1);
-    uint32_t xmax = xmin + 1u, ymax = ymin + 1u;
     xmin = xmin < lim ? xmin : lim; ymin = ymin < lim ? ymin : lim;
-    xmax = xmax < lim ? xmax : lim; ymax = ymax < lim ? ymax : lim;
+    NmCoord c;
+    c.x = (int) xmin; c.y = (int) ymin;
+    c.w = v2(pos.x - (float) xmin, pos.y - (float) ymin);
+    return c;
+}
+struct alignas(8) F4 { float x, y, z, w; };
+SMS_DEV Texel4 nm_fetch(const float2 *__restrict__ slope, int res, int stride, V2 uv) {
+    NmCoord c = nm_coord(res, uv);
+    const float2 *row = slope + (size_t) c.y * stride + c.x;
+    F4 a = *(const F4 *) row, b = *(const F4 *) (row + stride);
     Texel4 t;
-    float2 a = slope[ymin * res + xmin], b = slope[ymin * res + xmax];
-    float2 c = slope[ymax * res + xmin], d = slope[ymax * res + xmax];
-    t.v00 = v2(a.x, a.y); t.v10 = v2(b.x, b.y); t.v01 = v2(c.x, c.y); t.v11 = v2(d.x, d.y);
-    t.w = v2(pos.x - (float) xmin, pos.y - (float) ymin);
+    t.v00 = v2(a.x, a.y); t.v10 = v2(a.z, a.w); t.v01 = v2(b.x, b.y); t.v11 = v2(b.z, b.w);
+    t.w = c.w;
     return t;
 }
 SMS_DEV V2 nm_slope(const Texel4 &t) {
@@ -433,7 +448,7 @@ SMS_DEV MV mv_from_si(const DScene &S, const SI &si) {
     } else {
         const DNormalmap nm = S.nmaps[sh.normalmap];
         V2 uv = tile_uv(si.uv, sh.tiles);
-        Texel4 tx = nm_fetch(nm.slope, nm.res, uv);
+        Texel4 tx = nm_fetch(nm.slope, nm.res, nm.stride, uv);
         V2 sl = nm_slope(tx);
         V3 nl = v3(-sl.x, -sl.y, 1.f);
         V2 tu, tv;

@@ -23,7 +23,7 @@ struct sms_ctx {
     int n_cu = 0;
     char dev_name[256] = {0};
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
     float last_ms = 0.f, last_ms2 = 0.f;
     std::string err;
     DScene scene;
@@ -32,9 +32,11 @@ struct sms_ctx {
     unsigned long long *counters = nullptr;     // [0] cursor [1] n_overflow [2] coop cursor [8..15] stats
     OverflowRec *overflow = nullptr; size_t overflow_cap = 0;
     float *part = nullptr; float *invp = nullptr; size_t part_cap = 0;
+    GlintRec *rec = nullptr; GlintRes *res = nullptr; size_t rec_cap = 0;
     int lane_trials = 256;
     int blocks_per_cu = 0;
     int occ_glint = 0, occ_ss = 0, occ_coop = 0, occ_biased = 0, occ_sscoop = 0;
+    int occ_ms[2] = {0, 0}, occ_mscoop[2] = {0, 0}, occ_msbiased[2] = {0, 0};   // [0]: chains <= 2, [1]: <= 4
 };
 
 namespace {
@@ -236,23 +238,28 @@ sms_status build_scene(sms_ctx *ctx, const sms_scene_desc *desc, const sms_confi
         const sms_normalmap_desc &nd = desc->normalmaps[i];
         if (nd.res <= 0 || (nd.res & (nd.res - 1)) != 0 || !nd.rgb)
             return fail(ctx, SMS_E_INVALID, "normal map must be square power-of-two");
-        size_t N = (size_t) nd.res * nd.res;
-        std::vector<float2> sl(N);
-        for (size_t s = 0; s < N; ++s) {
-            float r = (float) nd.rgb[3 * s] / 255.f, g = (float) nd.rgb[3 * s + 1] / 255.f, b = (float) nd.rgb[3 * s + 2] / 255.f;
-            float x = 2.f * r - 1.f, y = 2.f * g - 1.f, z = 2.f * b - 1.f;
-            float inv = 1.f / std::sqrt((x * x + y * y) + z * z);
-            float nx = x * inv, ny = y * inv, nz = z * inv;
-            sl[s] = make_float2(-nx / nz, -ny / nz);
+        const size_t res = (size_t) nd.res, stride = res + 1;
+        // padded layout: column res and row res replicate column/row res-1 (edge clamp)
+        std::vector<float2> sl(stride * stride);
+        for (size_t y = 0; y < stride; ++y) {
+            for (size_t x = 0; x < stride; ++x) {
+                size_t s = std::min(y, res - 1) * res + std::min(x, res - 1);
+                float r = (float) nd.rgb[3 * s] / 255.f, g = (float) nd.rgb[3 * s + 1] / 255.f,
+                      b = (float) nd.rgb[3 * s + 2] / 255.f;
+                float vx = 2.f * r - 1.f, vy = 2.f * g - 1.f, vz = 2.f * b - 1.f;
+                float inv = 1.f / std::sqrt((vx * vx + vy * vy) + vz * vz);
+                float nx = vx * inv, ny = vy * inv, nz = vz * inv;
+                sl[y * stride + x] = make_float2(-nx / nz, -ny / nz);
+            }
         }
         const float2 *dp;
         sms_status st = upload(ctx, sl, &dp);
         if (st) return st;
-        DNormalmap m; m.res = nd.res; m.pad = 0; m.slope = dp;
+        DNormalmap m; m.res = nd.res; m.stride = (int) stride; m.slope = dp;
         nmaps.push_back(m);
     }
     std::vector<DEmitter> emitters;
-    std::vector<int> em_ss, em_all;
+    std::vector<int> em_ss, em_ms, em_all;
     for (int i = 0; i < desc->n_emitters; ++i) {
         const sms_emitter_desc &e = desc->emitters[i];
         DEmitter d;
@@ -266,9 +273,10 @@ sms_status build_scene(sms_ctx *ctx, const sms_scene_desc *desc, const sms_confi
         emitters.push_back(d);
         em_all.push_back(i);
         if (e.caustic_ss) em_ss.push_back(i);
+        if (e.caustic_ms) em_ms.push_back(i);
     }
     std::vector<DShape> shapes;
-    std::vector<int> analytic, casters;
+    std::vector<int> analytic, casters, casters_ms;
     std::vector<float4> vpos, vnrm;
     std::vector<float2> vuv;
     std::vector<uint4> faces;
@@ -341,6 +349,7 @@ sms_status build_scene(sms_ctx *ctx, const sms_scene_desc *desc, const sms_confi
             return fail(ctx, SMS_E_INVALID, "unknown shape type");
         }
         if (d.flags & SMS_FLAG_CAUSTIC_CASTER_SS) casters.push_back(i);
+        if (d.flags & SMS_FLAG_CAUSTIC_CASTER_MS) casters_ms.push_back(i);
         shapes.push_back(sh);
     }
     // BVH over all mesh triangles
@@ -381,6 +390,10 @@ sms_status build_scene(sms_ctx *ctx, const sms_scene_desc *desc, const sms_confi
     S.n_emitters_all = (int) em_all.size();
     if ((st = upload(ctx, casters, &S.casters_ss))) return st;
     S.n_casters_ss = (int) casters.size();
+    if ((st = upload(ctx, em_ms, &S.emitters_ms))) return st;
+    S.n_emitters_ms = (int) em_ms.size();
+    if ((st = upload(ctx, casters_ms, &S.casters_ms))) return st;
+    S.n_casters_ms = (int) casters_ms.size();
     if ((st = upload(ctx, nmaps, &S.nmaps))) return st;
     if ((st = upload(ctx, vpos, &S.vpos))) return st;
     if ((st = upload(ctx, vnrm, &S.vnrm))) return st;
@@ -400,6 +413,14 @@ sms_status check_config(sms_ctx *ctx, const sms_config *cfg) {
     if (!cfg) return fail(ctx, SMS_E_INVALID, "null config");
     if (cfg->twostage) return fail(ctx, SMS_E_UNSUPPORTED, "twostage solver is not built");
     if (cfg->max_iterations < 0) return fail(ctx, SMS_E_INVALID, "max_iterations < 0");
+    return SMS_OK;
+}
+
+// Multi-scatter is built for smooth specular chains of 1..SMS_MS_MAX_BOUNCES vertices.
+sms_status check_ms(sms_ctx *ctx) {
+    const sms_config &cfg = ctx->scene.cfg;
+    if (cfg.bounces < 1 || cfg.bounces > SMS_MS_MAX_BOUNCES)
+        return fail(ctx, SMS_E_UNSUPPORTED, "multi-scatter chains of 1..4 vertices are built");
     return SMS_OK;
 }
 
@@ -430,10 +451,24 @@ sms_status ensure_work(sms_ctx *ctx, size_t n_overflow, size_t n_part) {
     return SMS_OK;
 }
 
+sms_status ensure_glint_work(sms_ctx *ctx, size_t n) {
+    if (n > ctx->rec_cap) {
+        if (ctx->rec) HIP_TRY(ctx, hipFree(ctx->rec));
+        if (ctx->res) HIP_TRY(ctx, hipFree(ctx->res));
+        ctx->rec = nullptr; ctx->res = nullptr;
+        HIP_TRY(ctx, hipMalloc((void **) &ctx->rec, n * sizeof(GlintRec)));
+        HIP_TRY(ctx, hipMalloc((void **) &ctx->res, n * sizeof(GlintRes)));
+        ctx->rec_cap = n;
+    }
+    return SMS_OK;
+}
+
 EstimatorWork work_of(sms_ctx *ctx) {
     EstimatorWork W;
     W.cursor = ctx->counters + 0; W.n_overflow = ctx->counters + 1; W.coop_cursor = ctx->counters + 2;
+    W.n_rec = ctx->counters + 3;
     W.stats = ctx->counters + 8; W.overflow = ctx->overflow; W.lane_trials = ctx->lane_trials;
+    W.rec = ctx->rec; W.res = ctx->res;
     return W;
 }
 
@@ -451,8 +486,11 @@ sms_status read_stats(sms_ctx *ctx, sms_stats *stats) {
     return SMS_OK;
 }
 
+// Persistent kernels take their work from device counters, so blocks beyond the resident
+// limit simply start later and find the queue drained; the default is the occupancy limit,
+// at least 8 blocks per CU.
 int grid_for(sms_ctx *ctx, int occ) {
-    int per_cu = ctx->blocks_per_cu > 0 ? std::min(ctx->blocks_per_cu, std::max(1, occ)) : std::max(1, occ);
+    int per_cu = ctx->blocks_per_cu > 0 ? ctx->blocks_per_cu : std::max(8, occ);
     return per_cu * ctx->n_cu;
 }
 
@@ -492,16 +530,23 @@ sms_status sms_ctx_create(const sms_scene_desc *scene, const sms_config *cfg, sm
     ctx->n_cu = prop.multiProcessorCount;
     snprintf(ctx->dev_name, sizeof(ctx->dev_name), "%s (%s)", prop.name, prop.gcnArchName);
     if (hipStreamCreate(&ctx->stream) != hipSuccess || hipEventCreate(&ctx->ev0) != hipSuccess ||
-        hipEventCreate(&ctx->ev1) != hipSuccess || hipEventCreate(&ctx->ev2) != hipSuccess) {
+        hipEventCreate(&ctx->ev1) != hipSuccess || hipEventCreate(&ctx->ev2) != hipSuccess ||
+        hipEventCreate(&ctx->ev3) != hipSuccess) {
         sms_ctx_destroy(ctx);
         return SMS_E_HIP;
     }
     st = build_scene(ctx, scene, cfg);
-    if (!st) st = occupancy(ctx, k_glint_estimate, &ctx->occ_glint);
+    if (!st) st = occupancy(ctx, k_glint_walks, &ctx->occ_glint);
     if (!st) st = occupancy(ctx, k_glint_coop, &ctx->occ_coop);
     if (!st) st = occupancy(ctx, k_ss_estimate, &ctx->occ_ss);
     if (!st) st = occupancy(ctx, k_ss_coop, &ctx->occ_sscoop);
     if (!st) st = occupancy(ctx, k_ss_biased, &ctx->occ_biased);
+    if (!st) st = occupancy(ctx, k_ms_estimate_2, &ctx->occ_ms[0]);
+    if (!st) st = occupancy(ctx, k_ms_estimate_4, &ctx->occ_ms[1]);
+    if (!st) st = occupancy(ctx, k_ms_coop_2, &ctx->occ_mscoop[0]);
+    if (!st) st = occupancy(ctx, k_ms_coop_4, &ctx->occ_mscoop[1]);
+    if (!st) st = occupancy(ctx, k_ms_biased_2, &ctx->occ_msbiased[0]);
+    if (!st) st = occupancy(ctx, k_ms_biased_4, &ctx->occ_msbiased[1]);
     if (!st) st = ensure_work(ctx, 0, 0);
     if (st) {
         fprintf(stderr, "sms_ctx_create: %s\n", ctx->err.c_str());
@@ -520,6 +565,9 @@ sms_status sms_ctx_destroy(sms_ctx *ctx) {
     if (ctx->overflow) (void) hipFree(ctx->overflow);
     if (ctx->part) (void) hipFree(ctx->part);
     if (ctx->invp) (void) hipFree(ctx->invp);
+    if (ctx->rec) (void) hipFree(ctx->rec);
+    if (ctx->res) (void) hipFree(ctx->res);
+    if (ctx->ev3) (void) hipEventDestroy(ctx->ev3);
     if (ctx->ev0) (void) hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void) hipEventDestroy(ctx->ev1);
     if (ctx->ev2) (void) hipEventDestroy(ctx->ev2);
@@ -577,7 +625,6 @@ sms_status sms_ctx_last_kernel_ms(const sms_ctx *ctx, float *ms) {
 sms_status sms_ss_walk_batch(sms_ctx *ctx, const sms_ss_walk_in *in, int64_t n, sms_ss_walk_out *out) {
     if (!ctx || n < 0 || (n > 0 && (!in || !out))) return fail(ctx, SMS_E_INVALID, "bad walk batch arguments");
     if (n == 0) return SMS_OK;
-    for (int i = 0; i < 1; ++i) {}
     int blocks = (int) ((n + 255) / 256);
     hipLaunchKernelGGL(k_ss_walk, dim3(blocks), dim3(256), 0, ctx->stream, ctx->scene, in, n, out);
     HIP_TRY(ctx, hipGetLastError());
@@ -610,26 +657,38 @@ sms_status sms_glint_estimate_batch(sms_ctx *ctx, const sms_glint_query *q, int6
     if (ctx->scene.cfg.biased) return fail(ctx, SMS_E_UNSUPPORTED, "biased glint estimator is not built");
     if (!ctx->scene.nmaps) return fail(ctx, SMS_E_INVALID, "scene has no normal map");
     sms_status st = ensure_work(ctx, (size_t) std::max<int64_t>(n, 1), 0);
+    if (!st) st = ensure_glint_work(ctx, (size_t) std::max<int64_t>(n, 1));
     if (st) return st;
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream, ctx->counters, N_COUNTERS);
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     if (n > 0) {
         EstimatorWork W = work_of(ctx);
-        hipLaunchKernelGGL(k_glint_estimate, dim3(grid_for(ctx, ctx->occ_glint)), dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL(k_glint_setup, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream,
                            ctx->scene, q, n, seed, out_rgb, W);
+        HIP_TRY(ctx, hipGetLastError());
+        HIP_TRY(ctx, hipEventRecord(ctx->ev3, ctx->stream));
+        hipLaunchKernelGGL(k_glint_walks, dim3(grid_for(ctx, ctx->occ_glint)), dim3(256), 0, ctx->stream,
+                           ctx->scene, seed, W);
         HIP_TRY(ctx, hipGetLastError());
         HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
         hipLaunchKernelGGL(k_glint_coop, dim3(grid_for(ctx, ctx->occ_coop)), dim3(256), 0, ctx->stream,
-                           ctx->scene, q, seed, out_rgb, W);
+                           ctx->scene, seed, W);
+        HIP_TRY(ctx, hipGetLastError());
+        hipLaunchKernelGGL(k_glint_finalize, dim3(4 * ctx->n_cu), dim3(256), 0, ctx->stream, ctx->scene, q, seed,
+                           out_rgb, W);
         HIP_TRY(ctx, hipGetLastError());
     } else {
+        HIP_TRY(ctx, hipEventRecord(ctx->ev3, ctx->stream));
         HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     }
     HIP_TRY(ctx, hipEventRecord(ctx->ev2, ctx->stream));
     st = read_stats(ctx, stats);
     if (st) return st;
-    HIP_TRY(ctx, hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1));
+    float setup_ms = 0.f;
+    HIP_TRY(ctx, hipEventElapsedTime(&setup_ms, ctx->ev0, ctx->ev3));
+    HIP_TRY(ctx, hipEventElapsedTime(&ctx->last_ms, ctx->ev3, ctx->ev1));
     HIP_TRY(ctx, hipEventElapsedTime(&ctx->last_ms2, ctx->ev1, ctx->ev2));
+    ctx->last_ms2 += setup_ms;
     return SMS_OK;
 }
 
@@ -658,6 +717,66 @@ sms_status sms_ss_estimate_batch(sms_ctx *ctx, const sms_ss_query *q, int64_t n,
             HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
             hipLaunchKernelGGL(k_ss_coop, dim3(grid_for(ctx, ctx->occ_sscoop)), dim3(256), 0, ctx->stream,
                                ctx->scene, q, seed, ctx->invp, W);
+            HIP_TRY(ctx, hipGetLastError());
+            hipLaunchKernelGGL(k_ss_finalize, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                               ctx->part, ctx->invp, nc, n, out_rgb);
+            HIP_TRY(ctx, hipGetLastError());
+        }
+    } else {
+        HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    }
+    HIP_TRY(ctx, hipEventRecord(ctx->ev2, ctx->stream));
+    st = read_stats(ctx, stats);
+    if (st) return st;
+    HIP_TRY(ctx, hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1));
+    HIP_TRY(ctx, hipEventElapsedTime(&ctx->last_ms2, ctx->ev1, ctx->ev2));
+    return SMS_OK;
+}
+
+sms_status sms_ms_walk_batch(sms_ctx *ctx, const sms_ms_walk_in *in, int64_t n, sms_ms_walk_out *out) {
+    if (!ctx || n < 0 || (n > 0 && (!in || !out))) return fail(ctx, SMS_E_INVALID, "bad walk batch arguments");
+    sms_status st = check_ms(ctx);
+    if (st) return st;
+    if (n == 0) return SMS_OK;
+    int blocks = (int) ((n + 255) / 256);
+    if (ctx->scene.cfg.bounces <= 2)
+        hipLaunchKernelGGL(k_ms_walk_2, dim3(blocks), dim3(256), 0, ctx->stream, ctx->scene, in, n, out);
+    else
+        hipLaunchKernelGGL(k_ms_walk_4, dim3(blocks), dim3(256), 0, ctx->stream, ctx->scene, in, n, out);
+    HIP_TRY(ctx, hipGetLastError());
+    return SMS_OK;
+}
+
+sms_status sms_ms_estimate_batch(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *out_rgb,
+                                 sms_stats *stats) {
+    if (!ctx || n < 0 || (n > 0 && (!q || !out_rgb))) return fail(ctx, SMS_E_INVALID, "bad estimator arguments");
+    sms_status st = check_ms(ctx);
+    if (st) return st;
+    const sms_config &cfg = ctx->scene.cfg;
+    const int nc = ctx->scene.n_casters_ms;
+    const int b = cfg.bounces <= 2 ? 0 : 1;
+    if (cfg.biased && cfg.max_trials > 128) return fail(ctx, SMS_E_UNSUPPORTED, "biased trial sets above 128 are not built");
+    st = ensure_work(ctx, cfg.biased ? 0 : (size_t) std::max<int64_t>(n * nc, 1),
+                     cfg.biased ? 0 : (size_t) std::max<int64_t>(n * nc, 1));
+    if (st) return st;
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream, ctx->counters, N_COUNTERS);
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    if (n > 0) {
+        EstimatorWork W = work_of(ctx);
+        if (cfg.biased) {
+            dim3 grid(grid_for(ctx, ctx->occ_msbiased[b]));
+            if (b == 0) hipLaunchKernelGGL(k_ms_biased_2, grid, dim3(256), 0, ctx->stream, ctx->scene, q, n, seed, out_rgb, W);
+            else hipLaunchKernelGGL(k_ms_biased_4, grid, dim3(256), 0, ctx->stream, ctx->scene, q, n, seed, out_rgb, W);
+            HIP_TRY(ctx, hipGetLastError());
+            HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+        } else {
+            dim3 g1(grid_for(ctx, ctx->occ_ms[b])), g2(grid_for(ctx, ctx->occ_mscoop[b]));
+            if (b == 0) hipLaunchKernelGGL(k_ms_estimate_2, g1, dim3(256), 0, ctx->stream, ctx->scene, q, n, seed, ctx->part, ctx->invp, W);
+            else hipLaunchKernelGGL(k_ms_estimate_4, g1, dim3(256), 0, ctx->stream, ctx->scene, q, n, seed, ctx->part, ctx->invp, W);
+            HIP_TRY(ctx, hipGetLastError());
+            HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+            if (b == 0) hipLaunchKernelGGL(k_ms_coop_2, g2, dim3(256), 0, ctx->stream, ctx->scene, q, seed, ctx->invp, W);
+            else hipLaunchKernelGGL(k_ms_coop_4, g2, dim3(256), 0, ctx->stream, ctx->scene, q, seed, ctx->invp, W);
             HIP_TRY(ctx, hipGetLastError());
             hipLaunchKernelGGL(k_ss_finalize, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream,
                                ctx->part, ctx->invp, nc, n, out_rgb);
@@ -729,6 +848,16 @@ sms_status sms_glint_estimate_batch_host(sms_ctx *ctx, const sms_glint_query *in
                                          sms_stats *stats) {
     HOST_VARIANT_BODY(sms_glint_query, 3 * sizeof(float),
                       sms_glint_estimate_batch(ctx, (const sms_glint_query *) d_in, n, seed, (float *) d_out, stats))
+}
+
+sms_status sms_ms_walk_batch_host(sms_ctx *ctx, const sms_ms_walk_in *in, int64_t n, sms_ms_walk_out *out) {
+    HOST_VARIANT_BODY(sms_ms_walk_in, sizeof(sms_ms_walk_out),
+                      sms_ms_walk_batch(ctx, (const sms_ms_walk_in *) d_in, n, (sms_ms_walk_out *) d_out))
+}
+sms_status sms_ms_estimate_batch_host(sms_ctx *ctx, const sms_ss_query *in, int64_t n, uint64_t seed, float *out,
+                                      sms_stats *stats) {
+    HOST_VARIANT_BODY(sms_ss_query, 3 * sizeof(float),
+                      sms_ms_estimate_batch(ctx, (const sms_ss_query *) d_in, n, seed, (float *) d_out, stats))
 }
 
 } // extern "C"

@@ -2,15 +2,12 @@
 //
 //   k_ss_walk / k_glint_walk      parity entry points: one lane per walk record
 //   k_glint_queries               camera ray differentials -> glint queries (footprints)
-//   k_glint_estimate              persistent unbiased glint estimator: every lane owns a
-//                                 query, advances its walk one Newton iteration per loop
-//                                 trip and refills from a device query counter when the
-//                                 query is resolved; queries that need more than
-//                                 `lane_trials` Bernoulli trials are pushed to an overflow
-//                                 list
-//   k_glint_coop                  overflow pass: one wavefront per query, 64 trials per
-//                                 round, the first match found by a ballot
-//   k_ss_estimate / k_ss_coop     the same two passes for unbiased single-scatter SMS
+//   (the unbiased glint estimator lives in sms_glint_kernels.hip)
+//   k_ss_estimate                 unbiased single-scatter SMS, lane per query: first walk,
+//                                 contribution and up to `lane_trials` Bernoulli trials;
+//                                 longer trial loops go to an overflow list
+//   k_ss_coop                     overflow pass: one wavefront per (query, caster), 64 trials
+//                                 per round, the first match found by a ballot
 //   k_ss_biased                   biased single-scatter SMS: one wavefront per query,
 //                                 lanes = trial-set members, in-order de-duplication
 //
@@ -20,20 +17,6 @@
 #include "kernels.h"
 
 namespace smsdev {
-
-// ---------------------------------------------------------------- stats
-SMS_DEV void wave_add_u64(unsigned long long *dst, unsigned long long v) {
-    // reduce across the wavefront, one atomic per wave
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, v);
-}
-SMS_DEV void wave_max_u64(unsigned long long *dst, unsigned long long v) {
-    for (int off = 32; off > 0; off >>= 1) {
-        unsigned long long o = __shfl_xor(v, off, 64);
-        v = o > v ? o : v;
-    }
-    if ((threadIdx.x & 63) == 0 && v) atomicMax(dst, v);
-}
 
 // ================================================================ parity walks
 __global__ void k_ss_walk(DScene S, const sms_ss_walk_in *__restrict__ in, int64_t n, sms_ss_walk_out *__restrict__ out) {
@@ -64,7 +47,7 @@ __global__ void k_glint_walk(DScene S, const sms_glint_walk_in *__restrict__ in,
     const DShape &sh = S.shapes[w.shape];
     const DNormalmap nm = S.nmaps[sh.normalmap];
     fp.uv = v2(w.uv[0], w.uv[1]); fp.duv_dx = v2(w.duv_dx[0], w.duv_dx[1]); fp.duv_dy = v2(w.duv_dy[0], w.duv_dy[1]);
-    fp.slope = nm.slope; fp.res = nm.res; fp.fres = (float) nm.res; fp.tiles = sh.tiles;
+    fp.slope = nm.slope; fp.res = nm.res; fp.stride = nm.stride; fp.fres = (float) nm.res; fp.tiles = sh.tiles;
     footprint_setup(fp);
     V2 xi = v2(w.sample[0], w.sample[1]);
     if (w.xi_start[0] != 1.f || w.xi_start[1] != 1.f) xi = v2(w.xi_start[0], w.xi_start[1]);
@@ -122,201 +105,6 @@ __global__ void k_glint_queries(DScene S, int spp, uint64_t seed, sms_glint_quer
         o.uv[0] = si.uv.x; o.uv[1] = si.uv.y;
     }
     q[idx] = o;
-}
-
-// ================================================================ glint estimator
-struct GlintLane {
-    int64_t q;                  // query index or -1
-    uint64_t qkey;
-    int k;                      // walk index (0 = first walk, k >= 1 = Bernoulli trial k)
-    GlintFootprint fp;
-    GlintWalk w;
-    V2 uv_sol;
-    float val, f;               // F*G of the solution, mis*We
-    V3 ew;                      // emitter weight
-    V3 ei_p;                    // emitter sample position (all the contribution needs of it)
-    int ei_kind;
-};
-
-SMS_DEV SI glint_query_si(const sms_glint_query &g) {
-    SI si = si_invalid();
-    si.valid = true; si.p = ld3(g.p); si.n = ld3(g.n); si.sh_n = ld3(g.sh_n); si.sh_s = ld3(g.sh_s); si.sh_t = ld3(g.sh_t);
-    si.dp_du = ld3(g.dp_du); si.dp_dv = ld3(g.dp_dv); si.uv = v2(g.uv[0], g.uv[1]); si.shape = g.shape;
-    return si;
-}
-
-SMS_DEV void write_rgb(float *out, int64_t q, V3 v) { out[3 * q] = v.x; out[3 * q + 1] = v.y; out[3 * q + 2] = v.z; }
-
-// result of the SMS strategy for a resolved query: (f * (val * inv_prob)) * ew per channel
-SMS_DEV V3 glint_result(float val, float inv_prob, float f, V3 ew) {
-    float sw = val * inv_prob;
-    return v3((f * sw) * ew.x, (f * sw) * ew.y, (f * sw) * ew.z);
-}
-
-__global__ void __launch_bounds__(256) k_glint_estimate(DScene S, const sms_glint_query *__restrict__ Q, int64_t n, uint64_t seed,
-                                                        float *__restrict__ out, EstimatorWork W) {
-    const sms_config &cfg = S.cfg;
-    unsigned long long st_walks = 0, st_conv = 0, st_iters = 0, st_bcalls = 0, st_btrials = 0, st_bmax = 0;
-    GlintLane L;
-    L.q = -1;
-    while (true) {
-        if (L.q < 0) {
-            // ---- refill: fetch the next query and set it up (emitter, visibility, target, walk 0)
-            int64_t q = (int64_t) atomicAdd(W.cursor, 1ull);
-            if (q >= n) break;
-            const sms_glint_query g = Q[q];
-            write_rgb(out, q, v3(0.f, 0.f, 0.f));
-            if (g.shape < 0) continue;
-            SI si = glint_query_si(g);
-            uint64_t qkey = query_key(seed, (uint64_t) q);
-            Pcg32 rq;
-            stream_seed(rq, qkey, 0);
-            EI ei = sample_emitter_interaction(S, si.p, S.emitters_all, S.n_emitters_all, rq);
-            V3 edir = ei.p - si.p;
-            float edist = norm(edir);
-            edir = edir * (1.f / edist);
-            if (ray_test(S, make_ray(si.p, edir, RAY_EPS_F * (1.f + hmax_abs(si.p)), edist * (1.f - RAY_EPS_F)))) continue;
-            GlintFootprint &fp = L.fp;
-            if (!glint_target(ld3(g.sensor_p), ei.p, si.p, si.sh_s, si.sh_t, si.sh_n, v3(0.f, 0.f, 1.f), fp.target)) {
-                st_walks++;                    // sample_glint returns failure before the solve
-                continue;
-            }
-            const DShape &sh = S.shapes[g.shape];
-            const DNormalmap nm = S.nmaps[sh.normalmap];
-            fp.uv = si.uv; fp.duv_dx = v2(g.duv_dx[0], g.duv_dx[1]); fp.duv_dy = v2(g.duv_dy[0], g.duv_dy[1]);
-            fp.slope = nm.slope; fp.res = nm.res; fp.fres = (float) nm.res; fp.tiles = sh.tiles;
-            footprint_setup(fp);
-            L.q = q; L.qkey = qkey; L.k = 0;
-            L.ew = ei.weight;
-            L.f = 0.f; L.val = 0.f;
-            Pcg32 rw;
-            stream_seed(rw, qkey, 1);
-            V2 xi; xi.x = pcg32_float(rw); xi.y = pcg32_float(rw);
-            glint_walk_start(fp, xi, L.w);
-            L.ei_p = ei.p; L.ei_kind = ei.kind;
-        }
-        // ---- one Newton iteration of the lane's current walk
-        int st = glint_walk_iter(L.fp, cfg, L.w);
-        if (st == WALK_RUNNING) continue;
-        st_walks++; st_iters += (unsigned long long) L.w.it;
-        bool ok = st == WALK_CONVERGED;
-        if (ok) st_conv++;
-        if (L.k == 0) {
-            if (!ok) { L.q = -1; continue; }
-            // first solution: contribution and MIS/sensor factor (manifold_glints.cpp:73-79,144-179)
-            const sms_glint_query g = Q[L.q];
-            SI si = glint_query_si(g);
-            EI ei;
-            ei.p = L.ei_p; ei.kind = L.ei_kind;
-            ei.n = ei.d = v3(0.f, 0.f, 0.f); ei.weight = L.ew; ei.pdf = 0.f; ei.emitter = -1;
-            V3 sensor_p = ld3(g.sensor_p);
-            L.val = glint_contribution(S, sensor_p, ei, si, L.w.uv);
-            float We = sensor_we(S.cam, sensor_p, si.p);
-            float mis = cfg.bsdf_strategy_only ? 0.f : 1.f;
-            L.f = mis * We;
-            L.uv_sol = L.w.uv;
-            st_bcalls++;
-            L.k = 1;
-        } else {
-            if (ok && norm2(L.w.uv - L.uv_sol) < cfg.uniqueness_threshold) {
-                // match at trial k: inv_prob = k (manifold_glints.cpp:85-104)
-                st_btrials += (unsigned long long) L.k;
-                st_bmax = st_bmax > (unsigned long long) L.k ? st_bmax : (unsigned long long) L.k;
-                write_rgb(out, L.q, glint_result(L.val, (float) L.k, L.f, L.ew));
-                L.q = -1;
-                continue;
-            }
-            L.k++;
-            if (cfg.max_trials > 0 && L.k > cfg.max_trials) {
-                st_btrials += (unsigned long long) L.k;
-                st_bmax = st_bmax > (unsigned long long) L.k ? st_bmax : (unsigned long long) L.k;
-                write_rgb(out, L.q, glint_result(L.val, 0.f, L.f, L.ew));
-                L.q = -1;
-                continue;
-            }
-            if (L.k > W.lane_trials) {
-                unsigned long long slot = atomicAdd(W.n_overflow, 1ull);
-                OverflowRec rec;
-                rec.q = L.q; rec.k = L.k; rec.sol[0] = L.uv_sol.x; rec.sol[1] = L.uv_sol.y; rec.sol[2] = 0.f;
-                rec.val[0] = L.val; rec.val[1] = L.val; rec.val[2] = L.val; rec.f = L.f;
-                rec.ew[0] = L.ew.x; rec.ew[1] = L.ew.y; rec.ew[2] = L.ew.z;
-                W.overflow[slot] = rec;
-                L.q = -1;
-                continue;
-            }
-        }
-        Pcg32 rw;
-        stream_seed(rw, L.qkey, (uint64_t) L.k + 1);
-        V2 xi; xi.x = pcg32_float(rw); xi.y = pcg32_float(rw);
-        glint_walk_start(L.fp, xi, L.w);
-    }
-    wave_add_u64(W.stats + 0, st_walks); wave_add_u64(W.stats + 1, st_conv); wave_add_u64(W.stats + 2, st_iters);
-    wave_add_u64(W.stats + 3, st_bcalls); wave_add_u64(W.stats + 4, st_btrials); wave_max_u64(W.stats + 5, st_bmax);
-}
-
-// Overflow pass: one wavefront per query; lane j runs trial k0 + j; the smallest matching
-// trial index wins (identical to the sequential loop because each trial has its own stream).
-__global__ void __launch_bounds__(256) k_glint_coop(DScene S, const sms_glint_query *__restrict__ Q, uint64_t seed,
-                                                    float *__restrict__ out, EstimatorWork W) {
-    const sms_config &cfg = S.cfg;
-    const int lane = threadIdx.x & 63;
-    unsigned long long st_walks = 0, st_conv = 0, st_iters = 0, st_btrials = 0, st_bmax = 0, st_tail = 0;
-    const unsigned long long n_over = *W.n_overflow;
-    while (true) {
-        unsigned long long item = 0;
-        if (lane == 0) item = atomicAdd(W.coop_cursor, 1ull);
-        item = __shfl(item, 0, 64);
-        if (item >= n_over) break;
-        const OverflowRec rec = W.overflow[item];
-        const sms_glint_query g = Q[rec.q];
-        SI si = glint_query_si(g);
-        uint64_t qkey = query_key(seed, (uint64_t) rec.q);
-        Pcg32 rq;
-        stream_seed(rq, qkey, 0);
-        EI ei = sample_emitter_interaction(S, si.p, S.emitters_all, S.n_emitters_all, rq);
-        GlintFootprint fp;
-        glint_target(ld3(g.sensor_p), ei.p, si.p, si.sh_s, si.sh_t, si.sh_n, v3(0.f, 0.f, 1.f), fp.target);
-        const DShape &sh = S.shapes[g.shape];
-        const DNormalmap nm = S.nmaps[sh.normalmap];
-        fp.uv = si.uv; fp.duv_dx = v2(g.duv_dx[0], g.duv_dx[1]); fp.duv_dy = v2(g.duv_dy[0], g.duv_dy[1]);
-        fp.slope = nm.slope; fp.res = nm.res; fp.fres = (float) nm.res; fp.tiles = sh.tiles;
-        footprint_setup(fp);
-        V2 uv_sol = v2(rec.sol[0], rec.sol[1]);
-        int64_t kcap = cfg.max_trials > 0 ? (int64_t) cfg.max_trials : (int64_t) 1 << 40;   // last admissible trial
-        int64_t found = -1;
-        for (int64_t base = rec.k; base <= kcap && found < 0; base += 64) {
-            int64_t k = base + lane;
-            bool match = false, ran = false, conv = false;
-            int its = 0;
-            if (k <= kcap) {
-                Pcg32 rw;
-                stream_seed(rw, qkey, (uint64_t) k + 1);
-                V2 xi; xi.x = pcg32_float(rw); xi.y = pcg32_float(rw);
-                GlintWalk w;
-                glint_walk_start(fp, xi, w);
-                int s;
-                while ((s = glint_walk_iter(fp, cfg, w)) == WALK_RUNNING) {}
-                ran = true; its = w.it; conv = s == WALK_CONVERGED;
-                match = conv && norm2(w.uv - uv_sol) < cfg.uniqueness_threshold;
-            }
-            unsigned long long ballot = __ballot(match);
-            if (ballot) found = base + (int64_t) __ffsll((long long) ballot) - 1;
-            // count only the walks the sequential trial loop would have run (k <= first match)
-            if (ran && (found < 0 || k <= found)) {
-                st_walks++; st_iters += (unsigned long long) its; st_tail += (unsigned long long) its;
-                if (conv) st_conv++;
-            }
-        }
-        float inv_prob = found >= 0 ? (float) found : 0.f;
-        unsigned long long trials = found >= 0 ? (unsigned long long) found : (unsigned long long) kcap + 1ull;
-        if (lane == 0) {
-            write_rgb(out, rec.q, glint_result(rec.val[0], inv_prob, rec.f, v3(rec.ew[0], rec.ew[1], rec.ew[2])));
-            st_btrials += trials;
-            st_bmax = st_bmax > trials ? st_bmax : trials;
-        }
-    }
-    wave_add_u64(W.stats + 0, st_walks); wave_add_u64(W.stats + 1, st_conv); wave_add_u64(W.stats + 2, st_iters);
-    wave_add_u64(W.stats + 4, st_btrials); wave_max_u64(W.stats + 5, st_bmax); wave_add_u64(W.stats + 6, st_tail);
 }
 
 // ================================================================ single scatter, unbiased

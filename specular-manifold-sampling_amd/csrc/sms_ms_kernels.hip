@@ -1,0 +1,340 @@
+// sms_ms_kernels.hip -- multi-scatter SMS (SpecularManifoldMultiScatter,
+// src/librender/manifold_ms.cpp) on gfx950.
+//
+//   k_ms_walk_<B>      parity entry point: one lane per walk record
+//   k_ms_estimate_<B>  unbiased estimator, lane per query: for every multi-scatter caster the
+//                      first walk (which fixes the chain's shape sequence), its contribution and
+//                      up to `lane_trials` Bernoulli trials; longer loops go to the overflow list
+//   k_ms_coop_<B>      overflow pass: one wavefront per (query, caster), 64 trials per round,
+//                      the first match found by a ballot
+//   k_ms_biased_<B>    biased estimator: one wavefront per query, lanes = trial-set members;
+//                      the earliest member whose seed chain is valid fixes the shape sequence
+//                      of all later members (manifold_ms.cpp:150-163), solutions de-duplicated
+//                      in member order
+//
+// <B> is the largest chain the kernel holds (2 or 4 specular vertices); the host picks the
+// smallest that fits `bounces`.  Walk k of caster c at query q draws from PCG32 stream
+// (q, (c << 32) + k + 1); the emitter sample comes from stream (q, 0).
+#include "kernels.h"
+#include "dev_ms.h"
+
+namespace smsdev {
+
+SMS_DEV EI ei_from_abi(const sms_emitter_interaction &e) {
+    EI ei;
+    ei.p = ld3(e.p); ei.n = ld3(e.n); ei.d = ld3(e.d); ei.weight = ld3(e.weight);
+    ei.pdf = e.pdf; ei.kind = e.kind; ei.emitter = e.emitter;
+    return ei;
+}
+
+SMS_DEV SI ms_query_si(const sms_ss_query &g) {
+    SI si = si_invalid();
+    si.valid = true; si.t = 1.f;
+    si.p = ld3(g.p); si.n = ld3(g.n); si.sh_n = ld3(g.sh_n); si.sh_s = ld3(g.sh_s); si.sh_t = ld3(g.sh_t);
+    si.dp_du = si.sh_s; si.dp_dv = si.sh_t; si.shape = g.shape;
+    return si;
+}
+
+// Diffuse::eval at the receiver (src/bsdfs/diffuse.cpp:104-120)
+SMS_DEV V3 ms_receiver_bsdf(const DScene &S, const SI &si, V3 wi_local, V3 wo_local) {
+    const DShape &sh = S.shapes[si.shape];
+    if (sh.bsdf != SMS_BSDF_DIFFUSE) return v3(0.f, 0.f, 0.f);
+    if (!(wi_local.z > 0.f && wo_local.z > 0.f)) return v3(0.f, 0.f, 0.f);
+    return (ld3(sh.refl) * INV_PI_F) * wo_local.z;
+}
+
+template <int MAXB>
+SMS_DEV void ms_walk_k(const DScene &S, uint64_t qkey, int c, int shape, uint64_t k, V3 si_p, const EI &ei, bool first,
+                       const int *shapes, MSWalk<MAXB> &w) {
+    Pcg32 rw;
+    stream_seed(rw, qkey, ((uint64_t) c << 32) + k + 1);
+    V2 s; s.x = pcg32_float(rw); s.y = pcg32_float(rw);
+    ms_sample_path<MAXB>(S, shape, si_p, ei, s, first, shapes, v3(0.f, 0.f, 0.f), w);
+}
+
+// ================================================================ parity walks
+template <int MAXB>
+SMS_DEV void ms_walk_body(const DScene &S, const sms_ms_walk_in *__restrict__ in, int64_t n, sms_ms_walk_out *__restrict__ out) {
+    int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const sms_ms_walk_in w = in[i];
+    EI ei = ei_from_abi(w.ei);
+    MSWalk<MAXB> r;
+    ms_sample_path<MAXB>(S, w.shape, ld3(w.si_p), ei, v2(w.sample[0], w.sample[1]), w.first_path != 0, w.seed_shapes,
+                         ld3(w.p_start), r);
+    sms_ms_walk_out o;
+    o.success = r.ok; o.iterations = r.iterations; o.seed_valid = r.seed_ok; o.n_vertices = r.path.n;
+    for (int k = 0; k < SMS_MS_MAX_BOUNCES; ++k) {
+        int sk = k < MAXB ? r.seed_shapes[k < MAXB ? k : 0] : w.seed_shapes[k];
+        o.shapes[k] = r.seed_ok ? sk : -1;
+        V3 p = v3(0.f, 0.f, 0.f);
+        if (k < r.path.n) p = r.path.v[k < MAXB ? k : 0].v.p;
+        st3(o.p[k], p);
+    }
+    out[i] = o;
+}
+
+// ================================================================ unbiased estimator
+template <int MAXB>
+SMS_DEV void ms_estimate_body(const DScene &S, const sms_ss_query *__restrict__ Q, int64_t n, uint64_t seed,
+                              float *__restrict__ part, float *__restrict__ invp, EstimatorWork W) {
+    const sms_config &cfg = S.cfg;
+    unsigned long long st_walks = 0, st_conv = 0, st_iters = 0, st_bcalls = 0, st_btrials = 0, st_bmax = 0;
+    const int nc = S.n_casters_ms;
+    while (true) {
+        int64_t q = (int64_t) atomicAdd(W.cursor, 1ull);
+        if (q >= n) break;
+        const sms_ss_query g = Q[q];
+        SI si = ms_query_si(g);
+        V3 wi_local = ld3(g.wi);
+        uint64_t qkey = query_key(seed, (uint64_t) q);
+        Pcg32 rq;
+        stream_seed(rq, qkey, 0);
+        EI ei = sample_emitter_interaction(S, si.p, S.emitters_ms, S.n_emitters_ms, rq);
+        for (int c = 0; c < nc; ++c) {
+            int64_t slot = q * nc + c;
+            part[3 * slot] = 0.f; part[3 * slot + 1] = 0.f; part[3 * slot + 2] = 0.f;
+            invp[slot] = 0.f;
+            const int shape = S.casters_ms[c];
+            MSWalk<MAXB> w0;
+            ms_walk_k<MAXB>(S, qkey, c, shape, 0, si.p, ei, true, nullptr, w0);
+            st_walks++; st_iters += (unsigned long long) w0.iterations;
+            if (!w0.ok) continue;
+            st_conv++;
+            V3 direction = normalize(w0.path.v[0].v.p - si.p);
+            V3 spec = ms_path_contribution<MAXB>(S, si, ei, w0.path);
+            V3 dl = v3(dot(direction, si.sh_s), dot(direction, si.sh_t), dot(direction, si.sh_n));
+            V3 bsdf = ms_receiver_bsdf(S, si, wi_local, dl);
+            part[3 * slot] = bsdf.x * spec.x; part[3 * slot + 1] = bsdf.y * spec.y; part[3 * slot + 2] = bsdf.z * spec.z;
+            st_bcalls++;
+            int64_t k = 1;
+            float inv_prob = -1.f;
+            while (true) {
+                MSWalk<MAXB> t;
+                ms_walk_k<MAXB>(S, qkey, c, shape, (uint64_t) k, si.p, ei, false, w0.seed_shapes, t);
+                st_walks++; st_iters += (unsigned long long) t.iterations;
+                if (t.ok) {
+                    st_conv++;
+                    V3 dtr = normalize(t.path.v[0].v.p - si.p);
+                    if (fabsf(dot(direction, dtr) - 1.f) < cfg.uniqueness_threshold) { inv_prob = (float) k; break; }
+                }
+                k++;
+                if (cfg.max_trials > 0 && k > cfg.max_trials) { inv_prob = 0.f; break; }
+                if (k > W.lane_trials) break;
+            }
+            if (inv_prob >= 0.f) {
+                unsigned long long tr = (unsigned long long) k;
+                st_btrials += tr; st_bmax = st_bmax > tr ? st_bmax : tr;
+                invp[slot] = inv_prob;
+            } else {
+                unsigned long long o = atomicAdd(W.n_overflow, 1ull);
+                OverflowRec rec;
+                rec.q = slot; rec.k = k;
+                rec.sol[0] = direction.x; rec.sol[1] = direction.y; rec.sol[2] = direction.z;
+                for (int b = 0; b < 4; ++b) rec.shapes[b] = b < MAXB ? w0.seed_shapes[b < MAXB ? b : 0] : -1;
+                rec.pad = 0;
+                W.overflow[o] = rec;
+            }
+        }
+    }
+    wave_add_u64(W.stats + 0, st_walks); wave_add_u64(W.stats + 1, st_conv); wave_add_u64(W.stats + 2, st_iters);
+    wave_add_u64(W.stats + 3, st_bcalls); wave_add_u64(W.stats + 4, st_btrials); wave_max_u64(W.stats + 5, st_bmax);
+}
+
+template <int MAXB>
+SMS_DEV void ms_coop_body(const DScene &S, const sms_ss_query *__restrict__ Q, uint64_t seed, float *__restrict__ invp,
+                          EstimatorWork W) {
+    const sms_config &cfg = S.cfg;
+    const int lane = threadIdx.x & 63;
+    const int nc = S.n_casters_ms;
+    unsigned long long st_walks = 0, st_conv = 0, st_iters = 0, st_btrials = 0, st_bmax = 0, st_tail = 0;
+    const unsigned long long n_over = *W.n_overflow;
+    while (true) {
+        unsigned long long item = 0;
+        if (lane == 0) item = atomicAdd(W.coop_cursor, 1ull);
+        item = __shfl(item, 0, 64);
+        if (item >= n_over) break;
+        const OverflowRec rec = W.overflow[item];
+        int64_t q = rec.q / nc;
+        int c = (int) (rec.q % nc);
+        const sms_ss_query g = Q[q];
+        SI si = ms_query_si(g);
+        uint64_t qkey = query_key(seed, (uint64_t) q);
+        Pcg32 rq;
+        stream_seed(rq, qkey, 0);
+        EI ei = sample_emitter_interaction(S, si.p, S.emitters_ms, S.n_emitters_ms, rq);
+        V3 direction = v3(rec.sol[0], rec.sol[1], rec.sol[2]);
+        int shapes[4] = {rec.shapes[0], rec.shapes[1], rec.shapes[2], rec.shapes[3]};
+        const int shape = S.casters_ms[c];
+        int64_t kcap = cfg.max_trials > 0 ? (int64_t) cfg.max_trials : (int64_t) 1 << 40;
+        int64_t found = -1;
+        for (int64_t base = rec.k; base <= kcap && found < 0; base += 64) {
+            int64_t k = base + lane;
+            bool match = false, ran = false, conv = false;
+            int its = 0;
+            if (k <= kcap) {
+                MSWalk<MAXB> t;
+                ms_walk_k<MAXB>(S, qkey, c, shape, (uint64_t) k, si.p, ei, false, shapes, t);
+                ran = true; its = t.iterations; conv = t.ok;
+                if (t.ok) {
+                    V3 dtr = normalize(t.path.v[0].v.p - si.p);
+                    match = fabsf(dot(direction, dtr) - 1.f) < cfg.uniqueness_threshold;
+                }
+            }
+            unsigned long long ballot = __ballot(match);
+            if (ballot) found = base + (int64_t) __ffsll((long long) ballot) - 1;
+            if (ran && (found < 0 || k <= found)) {
+                st_walks++; st_iters += (unsigned long long) its; st_tail += (unsigned long long) its;
+                if (conv) st_conv++;
+            }
+        }
+        if (lane == 0) {
+            invp[rec.q] = found >= 0 ? (float) found : 0.f;
+            unsigned long long tr = found >= 0 ? (unsigned long long) found : (unsigned long long) kcap + 1ull;
+            st_btrials += tr; st_bmax = st_bmax > tr ? st_bmax : tr;
+        }
+    }
+    wave_add_u64(W.stats + 0, st_walks); wave_add_u64(W.stats + 1, st_conv); wave_add_u64(W.stats + 2, st_iters);
+    wave_add_u64(W.stats + 4, st_btrials); wave_max_u64(W.stats + 5, st_bmax); wave_add_u64(W.stats + 6, st_tail);
+}
+
+// ================================================================ biased estimator
+constexpr int MS_BIASED_MAX_SOL = 128;
+
+template <int MAXB>
+SMS_DEV void ms_biased_body(const DScene &S, const sms_ss_query *__restrict__ Q, int64_t n, uint64_t seed,
+                            float *__restrict__ out, EstimatorWork W, float (*sol_lds)[3]) {
+    const sms_config &cfg = S.cfg;
+    const int lane = threadIdx.x & 63;
+    const int nc = S.n_casters_ms;
+    unsigned long long st_walks = 0, st_conv = 0, st_iters = 0;
+    while (true) {
+        unsigned long long qq = 0;
+        if (lane == 0) qq = atomicAdd(W.cursor, 1ull);
+        qq = __shfl(qq, 0, 64);
+        if ((int64_t) qq >= n) break;
+        int64_t q = (int64_t) qq;
+        const sms_ss_query g = Q[q];
+        SI si = ms_query_si(g);
+        V3 wi_local = ld3(g.wi);
+        uint64_t qkey = query_key(seed, (uint64_t) q);
+        Pcg32 rq;
+        stream_seed(rq, qkey, 0);
+        EI ei = sample_emitter_interaction(S, si.p, S.emitters_ms, S.n_emitters_ms, rq);
+        V3 result = v3(0.f, 0.f, 0.f);
+        for (int c = 0; c < nc; ++c) {
+            const int shape = S.casters_ms[c];
+            V3 value = v3(0.f, 0.f, 0.f);
+            int n_sol = 0;
+            bool have_seed = false;
+            int seed_shapes[MAXB];
+            for (int b = 0; b < MAXB; ++b) seed_shapes[b] = -1;
+            for (int base = 0; base < cfg.max_trials; base += 64) {
+                const int m = base + lane;
+                const bool active = m < cfg.max_trials;
+                // members before the first valid seed chain run as first paths (and fail);
+                // the first valid one fixes the shape sequence of all later members
+                int j0 = -1;
+                if (!have_seed) {
+                    bool sok = false;
+                    MSPath<MAXB> P;
+                    P.n = 0;
+                    if (active) {
+                        Pcg32 rw;
+                        stream_seed(rw, qkey, ((uint64_t) c << 32) + (uint64_t) m + 1);
+                        V2 s; s.x = pcg32_float(rw); s.y = pcg32_float(rw);
+                        sok = ms_seed_path<MAXB>(S, shape, si.p, ei, s, true, seed_shapes, v3(0.f, 0.f, 0.f), P);
+                    }
+                    unsigned long long sb = __ballot(sok);
+                    if (sb) {
+                        j0 = __ffsll((long long) sb) - 1;
+                        for (int b = 0; b < MAXB; ++b) {
+                            int sh = (sok && b < P.n) ? P.shapes[b] : -1;
+                            seed_shapes[b] = __shfl(sh, j0, 64);
+                        }
+                    }
+                }
+                const bool first = !have_seed && (j0 < 0 || lane <= j0);
+                MSWalk<MAXB> r;
+                r.ok = false; r.path.n = 0;
+                bool cand = false;
+                V3 direction = v3(0.f, 0.f, 0.f);
+                if (active) {
+                    ms_walk_k<MAXB>(S, qkey, c, shape, (uint64_t) m, si.p, ei, first, first ? nullptr : seed_shapes, r);
+                    st_walks++; st_iters += (unsigned long long) r.iterations;
+                    if (r.ok) {
+                        st_conv++;
+                        cand = true;
+                        direction = normalize(r.path.v[0].v.p - si.p);
+                    }
+                }
+                if (j0 >= 0) have_seed = true;
+                bool unique = false;
+                unsigned long long cmask = __ballot(cand);
+                while (cmask) {
+                    int j = __ffsll((long long) cmask) - 1;
+                    cmask &= cmask - 1ull;
+                    float dx = __shfl(direction.x, j, 64), dy = __shfl(direction.y, j, 64), dz = __shfl(direction.z, j, 64);
+                    V3 dj = v3(dx, dy, dz);
+                    bool dup = false;
+                    for (int s0 = 0; s0 < n_sol; s0 += 64) {
+                        int s = s0 + lane;
+                        bool d = false;
+                        if (s < n_sol) d = fabsf(dot(dj, v3(sol_lds[s][0], sol_lds[s][1], sol_lds[s][2])) - 1.f)
+                                           < cfg.uniqueness_threshold;
+                        if (__ballot(d)) { dup = true; break; }
+                    }
+                    if (!dup) {
+                        if (n_sol < MS_BIASED_MAX_SOL && lane == 0) {
+                            sol_lds[n_sol][0] = dx; sol_lds[n_sol][1] = dy; sol_lds[n_sol][2] = dz;
+                        }
+                        if (n_sol < MS_BIASED_MAX_SOL) n_sol++;
+                        if (lane == j) unique = true;
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                }
+                V3 contrib = v3(0.f, 0.f, 0.f);
+                if (unique) {
+                    V3 dl = v3(dot(direction, si.sh_s), dot(direction, si.sh_t), dot(direction, si.sh_n));
+                    V3 bsdf = ms_receiver_bsdf(S, si, wi_local, dl);
+                    V3 spec = ms_path_contribution<MAXB>(S, si, ei, r.path);
+                    contrib = v3(bsdf.x * spec.x, bsdf.y * spec.y, bsdf.z * spec.z);
+                }
+                unsigned long long umask = __ballot(unique);
+                while (umask) {
+                    int j = __ffsll((long long) umask) - 1;
+                    umask &= umask - 1ull;
+                    value = value + v3(__shfl(contrib.x, j, 64), __shfl(contrib.y, j, 64), __shfl(contrib.z, j, 64));
+                }
+            }
+            result = result + value / 1.f;
+        }
+        if (lane == 0) write_rgb(out, q, result);
+    }
+    wave_add_u64(W.stats + 0, st_walks); wave_add_u64(W.stats + 1, st_conv); wave_add_u64(W.stats + 2, st_iters);
+}
+
+#define SMS_MS_KERNELS(B)                                                                                           \
+    __global__ void __launch_bounds__(256) k_ms_walk_##B(DScene S, const sms_ms_walk_in *in, int64_t n,            \
+                                                         sms_ms_walk_out *out) {                                   \
+        ms_walk_body<B>(S, in, n, out);                                                                             \
+    }                                                                                                               \
+    __global__ void __launch_bounds__(256) k_ms_estimate_##B(DScene S, const sms_ss_query *Q, int64_t n,           \
+                                                             uint64_t seed, float *part, float *invp,               \
+                                                             EstimatorWork W) {                                     \
+        ms_estimate_body<B>(S, Q, n, seed, part, invp, W);                                                          \
+    }                                                                                                               \
+    __global__ void __launch_bounds__(256) k_ms_coop_##B(DScene S, const sms_ss_query *Q, uint64_t seed,           \
+                                                         float *invp, EstimatorWork W) {                           \
+        ms_coop_body<B>(S, Q, seed, invp, W);                                                                       \
+    }                                                                                                               \
+    __global__ void __launch_bounds__(256) k_ms_biased_##B(DScene S, const sms_ss_query *Q, int64_t n,             \
+                                                           uint64_t seed, float *out, EstimatorWork W) {           \
+        __shared__ float sol_lds[4][MS_BIASED_MAX_SOL][3];                                                          \
+        ms_biased_body<B>(S, Q, n, seed, out, W, sol_lds[threadIdx.x >> 6]);                                        \
+    }
+
+SMS_MS_KERNELS(2)
+SMS_MS_KERNELS(4)
+
+} // namespace smsdev

@@ -99,6 +99,12 @@ GLINT_WALK_IN = np.dtype([("sensor_p", F3), ("ei_p", F3), ("si_p", F3), ("frame_
                           ("xi_start", F2)])
 GLINT_WALK_OUT = np.dtype([("success", np.int32), ("iterations", np.int32), ("uv", F2),
                            ("uv_init", F2)])
+MS_MAX_BOUNCES = 4
+MS_WALK_IN = np.dtype([("si_p", F3), ("shape", np.int32), ("ei", EI_DTYPE), ("sample", F2), ("p_start", F3),
+                       ("first_path", np.int32), ("seed_shapes", (np.int32, MS_MAX_BOUNCES))])
+MS_WALK_OUT = np.dtype([("success", np.int32), ("iterations", np.int32), ("seed_valid", np.int32),
+                        ("n_vertices", np.int32), ("p", (np.float32, (MS_MAX_BOUNCES, 3))),
+                        ("shapes", (np.int32, MS_MAX_BOUNCES))])
 SS_QUERY = np.dtype([("p", F3), ("n", F3), ("sh_n", F3), ("sh_s", F3), ("sh_t", F3), ("wi", F3),
                      ("shape", np.int32), ("pad", np.int32)])
 GLINT_QUERY = np.dtype([("sensor_p", F3), ("p", F3), ("n", F3), ("sh_n", F3), ("sh_s", F3),
@@ -108,8 +114,10 @@ GLINT_QUERY = np.dtype([("sensor_p", F3), ("p", F3), ("n", F3), ("sh_n", F3), ("
 # sizes the C header implies (checked against the compiled library in the tests)
 C_SIZES = {"sms_emitter_interaction": 60, "sms_ss_walk_in": 108, "sms_ss_walk_out": 52,
            "sms_glint_walk_in": 128, "sms_glint_walk_out": 24, "sms_ss_query": 80,
-           "sms_glint_query": 140, "sms_config": 48, "sms_stats": 64}
+           "sms_glint_query": 140, "sms_config": 48, "sms_stats": 64, "sms_ms_walk_in": 116,
+           "sms_ms_walk_out": 80}
 DTYPES = {"sms_emitter_interaction": EI_DTYPE, "sms_ss_walk_in": SS_WALK_IN,
+          "sms_ms_walk_in": MS_WALK_IN, "sms_ms_walk_out": MS_WALK_OUT,
           "sms_ss_walk_out": SS_WALK_OUT, "sms_glint_walk_in": GLINT_WALK_IN,
           "sms_glint_walk_out": GLINT_WALK_OUT, "sms_ss_query": SS_QUERY,
           "sms_glint_query": GLINT_QUERY}
@@ -125,7 +133,8 @@ EXPORTS = ["sms_config_default", "sms_ctx_create", "sms_ctx_destroy", "sms_ctx_s
            "sms_glint_make_queries", "sms_ctx_last_kernel_ms", "sms_ss_walk_batch_host",
            "sms_glint_walk_batch_host", "sms_ss_estimate_batch_host",
            "sms_glint_estimate_batch_host", "sms_device_alloc", "sms_device_free",
-           "sms_copy_h2d", "sms_copy_d2h"]
+           "sms_copy_h2d", "sms_copy_d2h", "sms_ms_walk_batch", "sms_ms_estimate_batch",
+           "sms_ms_walk_batch_host", "sms_ms_estimate_batch_host"]
 
 
 def load_library(path=LIB_PATH):
@@ -147,10 +156,11 @@ def load_library(path=LIB_PATH):
     lib.sms_ctx_device_info.argtypes = [vp, C.c_char_p, C.c_int32, C.POINTER(C.c_int32)]
     lib.sms_ctx_sync.argtypes = [vp]
     for name in ("sms_ss_walk_batch", "sms_glint_walk_batch", "sms_ss_walk_batch_host",
-                 "sms_glint_walk_batch_host"):
+                 "sms_glint_walk_batch_host", "sms_ms_walk_batch", "sms_ms_walk_batch_host"):
         getattr(lib, name).argtypes = [vp, vp, i64, vp]
     for name in ("sms_ss_estimate_batch", "sms_glint_estimate_batch",
-                 "sms_ss_estimate_batch_host", "sms_glint_estimate_batch_host"):
+                 "sms_ss_estimate_batch_host", "sms_glint_estimate_batch_host",
+                 "sms_ms_estimate_batch", "sms_ms_estimate_batch_host"):
         getattr(lib, name).argtypes = [vp, vp, i64, C.c_uint64, vp, C.POINTER(SmsStats)]
     lib.sms_glint_make_queries.argtypes = [vp, C.c_int32, C.c_uint64, vp]
     lib.sms_ctx_last_kernel_ms.argtypes = [vp, vp]

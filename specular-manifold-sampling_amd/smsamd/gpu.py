@@ -131,6 +131,18 @@ class Context:
                                                    C.byref(st)), self.h, "glint_estimate")
         return out, st.as_dict()
 
+    def ms_walks(self, win):
+        out = np.zeros(len(win), abi.MS_WALK_OUT)
+        _check(lib().sms_ms_walk_batch_host(self.h, win.ctypes.data, len(win), out.ctypes.data), self.h, "ms_walks")
+        return out
+
+    def ms_estimate(self, q, seed):
+        out = np.zeros((len(q), 3), np.float32)
+        st = abi.SmsStats()
+        _check(lib().sms_ms_estimate_batch_host(self.h, q.ctypes.data, len(q), seed, out.ctypes.data, C.byref(st)),
+               self.h, "ms_estimate")
+        return out, st.as_dict()
+
     def glint_queries(self, spp, seed):
         cam = self.scene.camera
         n = cam["width"] * cam["height"] * spp

@@ -161,6 +161,127 @@ def sphere_dome(width=256, height=256):
     return sphere_caustic(width, height, center_y=0.0)
 
 
+# ==================================================================== water pool
+def gerstner_grid(n=512, extent=1.0, n_waves=64, seed=3, amp_range=(1e-3, 2e-2), steepness=0.5):
+    """(n+1)^2 grid over [-extent, extent]^2 (xz) displaced by ``n_waves`` Gerstner waves with
+    amplitudes log-uniform in ``amp_range`` and random direction and phase; returns positions,
+    analytic vertex normals, grid uvs and 2 n^2 triangles (FP64 evaluation, stored as float32)."""
+    rng = np.random.default_rng(seed)
+    amp = np.exp(rng.uniform(math.log(amp_range[0]), math.log(amp_range[1]), n_waves))
+    ang = rng.uniform(0, 2 * np.pi, n_waves)
+    dirs = np.stack([np.cos(ang), np.sin(ang)], -1)
+    wavelen = np.clip(amp * rng.uniform(80, 160, n_waves), 0.05, 2.0)
+    k = 2 * np.pi / wavelen
+    phase = rng.uniform(0, 2 * np.pi, n_waves)
+    q = steepness / (k * amp * n_waves)
+    t = np.linspace(-extent, extent, n + 1)
+    X0, Z0 = np.meshgrid(t, t, indexing="xy")
+    P = np.stack([X0, np.zeros_like(X0), Z0], -1)
+    dPdx = np.zeros_like(P)
+    dPdz = np.zeros_like(P)
+    dPdx[..., 0] = 1.0
+    dPdz[..., 2] = 1.0
+    for i in range(n_waves):
+        th = k[i] * (dirs[i, 0] * X0 + dirs[i, 1] * Z0) + phase[i]
+        c, s = np.cos(th), np.sin(th)
+        P[..., 0] += q[i] * amp[i] * dirs[i, 0] * c
+        P[..., 2] += q[i] * amp[i] * dirs[i, 1] * c
+        P[..., 1] += amp[i] * s
+        for d, ax in ((dPdx, 0), (dPdz, 1)):
+            dth = k[i] * dirs[i, ax]
+            d[..., 0] += -q[i] * amp[i] * dirs[i, 0] * s * dth
+            d[..., 2] += -q[i] * amp[i] * dirs[i, 1] * s * dth
+            d[..., 1] += amp[i] * c * dth
+    N = np.cross(dPdz, dPdx)
+    N /= np.linalg.norm(N, axis=-1, keepdims=True)
+    uu, vv = np.meshgrid(np.linspace(0, 1, n + 1), np.linspace(0, 1, n + 1), indexing="xy")
+    uv = np.stack([uu, vv], -1)
+    idx = np.arange((n + 1) * (n + 1)).reshape(n + 1, n + 1)
+    a, b = idx[:-1, :-1].ravel(), idx[:-1, 1:].ravel()
+    c_, d = idx[1:, :-1].ravel(), idx[1:, 1:].ravel()
+    faces = np.concatenate([np.stack([a, c_, b], -1), np.stack([b, c_, d], -1)])
+    return (P.reshape(-1, 3).astype(np.float32), N.reshape(-1, 3).astype(np.float32),
+            uv.reshape(-1, 2).astype(np.float32), faces.astype(np.uint32))
+
+
+def pool(width=1024, height=1024, n=512, n_waves=64, seed=3):
+    """Water surface (IOR 1.33, caustic caster) over a diffuse floor 1 unit below (receiver),
+    sphere light of radius 0.05 above; single-scatter refractive caustics on the floor."""
+    sc = Scene()
+    P, N, UV, F = gerstner_grid(n, 1.0, n_waves, seed)
+    sc.add_mesh(P, F, normals=N, texcoords=UV, bsdf=abi.BSDF_DIELECTRIC, flags=abi.FLAG_CAUSTIC_CASTER_SS, eta=1.33)
+    sc.add_rectangle(plane_to_world((0, -1.0, 0), 1.0), abi.BSDF_DIFFUSE, flags=abi.FLAG_CAUSTIC_RECEIVER,
+                     reflectance=(0.5, 0.5, 0.5))
+    light = sc.add_sphere((0.25, 2.5, 0.1), 0.05, abi.BSDF_DIFFUSE, reflectance=(0.0, 0.0, 0.0))
+    sc.add_area_light(light, (2000.0, 2000.0, 2000.0))
+    sc.set_camera((0.0, 2.2, 2.6), (0.0, -1.0, 0.0), (0, 1, 0), 40.0, width, height)
+    return sc
+
+
+# ==================================================================== double-refraction slab
+def value_noise(n, seed, octaves=(8, 16, 32), gains=(1.0, 0.5, 0.25)):
+    """(n+1)^2 smooth value noise in [-1, 1]: random lattice values, smoothstep-interpolated,
+    summed over octaves and normalised by the maximum magnitude."""
+    rng = np.random.default_rng(seed)
+    t = np.linspace(0.0, 1.0, n + 1)
+    h = np.zeros((n + 1, n + 1))
+    for cells, g in zip(octaves, gains):
+        lat = rng.uniform(-1, 1, (cells + 1, cells + 1))
+        x = t * cells
+        i = np.minimum(np.floor(x).astype(int), cells - 1)
+        f = x - i
+        f = f * f * (3 - 2 * f)
+        a = lat[np.ix_(i, i)]
+        b = lat[np.ix_(i, i + 1)]
+        c = lat[np.ix_(i + 1, i)]
+        d = lat[np.ix_(i + 1, i + 1)]
+        fy, fx = f[:, None], f[None, :]
+        h += g * ((a * (1 - fx) + b * fx) * (1 - fy) + (c * (1 - fx) + d * fx) * fy)
+    return h / np.max(np.abs(h))
+
+
+def height_mesh(h, extent, y0, amplitude, flip):
+    """Grid mesh over [-extent, extent]^2 at height y0 + amplitude*h with central-difference
+    vertex normals (pointing down when ``flip``) and grid uvs."""
+    n = h.shape[0] - 1
+    t = np.linspace(-extent, extent, n + 1)
+    X, Z = np.meshgrid(t, t, indexing="xy")
+    Y = y0 + amplitude * h
+    P = np.stack([X, Y, Z], -1)
+    gy, gx = np.gradient(Y, t, t)
+    N = np.stack([-gx, np.ones_like(Y), -gy], -1)
+    N /= np.linalg.norm(N, axis=-1, keepdims=True)
+    uu, vv = np.meshgrid(np.linspace(0, 1, n + 1), np.linspace(0, 1, n + 1), indexing="xy")
+    idx = np.arange((n + 1) * (n + 1)).reshape(n + 1, n + 1)
+    a, b = idx[:-1, :-1].ravel(), idx[:-1, 1:].ravel()
+    c, d = idx[1:, :-1].ravel(), idx[1:, 1:].ravel()
+    faces = np.concatenate([np.stack([a, c, b], -1), np.stack([b, c, d], -1)])
+    if flip:
+        N = -N
+        faces = faces[:, [0, 2, 1]]
+    return (P.reshape(-1, 3).astype(np.float32), N.reshape(-1, 3).astype(np.float32),
+            np.stack([uu, vv], -1).reshape(-1, 2).astype(np.float32), faces.astype(np.uint32))
+
+
+def slab(width=1024, height=1024, n=256, thickness=0.2, amplitude=0.01, seed=5):
+    """Glass slab (IOR 1.5) whose top (multi-scatter caster) and bottom (bouncer) faces carry
+    independent value-noise heightfields; diffuse floor below, sphere light above.  Chains
+    floor -> bottom face -> top face -> light have two specular vertices."""
+    sc = Scene()
+    top = height_mesh(value_noise(n, seed), 0.5, 0.5 + thickness / 2, amplitude, flip=False)
+    bot = height_mesh(value_noise(n, seed + 1), 0.5, 0.5 - thickness / 2, amplitude, flip=True)
+    sc.add_mesh(top[0], top[3], normals=top[1], texcoords=top[2], bsdf=abi.BSDF_DIELECTRIC,
+                flags=abi.FLAG_CAUSTIC_CASTER_MS, eta=1.5)
+    sc.add_mesh(bot[0], bot[3], normals=bot[1], texcoords=bot[2], bsdf=abi.BSDF_DIELECTRIC,
+                flags=abi.FLAG_CAUSTIC_BOUNCER, eta=1.5)
+    sc.add_rectangle(plane_to_world((0, 0, 0), 2.0), abi.BSDF_DIFFUSE, flags=abi.FLAG_CAUSTIC_RECEIVER,
+                     reflectance=(0.5, 0.5, 0.5))
+    light = sc.add_sphere((0.1, 2.0, -0.1), 0.05, abi.BSDF_DIFFUSE, reflectance=(0.0, 0.0, 0.0))
+    sc.add_area_light(light, (2000.0, 2000.0, 2000.0))
+    sc.set_camera((0.0, 1.8, 2.2), (0.0, 0.2, 0.0), (0, 1, 0), 40.0, width, height)
+    return sc
+
+
 # ==================================================================== glint normal map
 def scratch_bump_normalmap(res=2048, n_scratches=4096, n_bumps=None, seed=1, slope_std=0.3):
     """RGB8 normal map: Gaussian bumps (sigma 2-8 texels) scaled to a slope standard deviation

@@ -93,3 +93,47 @@ def glint_walk_inputs(scene, n, seed, footprint_texels=4.0, nm_res=None):
     w["sample"] = rng.random((n, 2)).astype(np.float32)
     w["xi_start"] = (1.0, 1.0)
     return w
+
+
+def ms_walk_inputs(scene, n, seed, caster=0, light_shape=3, extent=0.3, p_start=None, first_path=True,
+                   seed_shapes=(-1, -1, -1, -1)):
+    """Multi-scatter walks of the slab scene: receiver x uniform in a disc of radius ``extent``
+    on the floor (y=0), emitter sample uniform on the light sphere, seed sample uniform."""
+    rng = np.random.default_rng(seed)
+    w = np.zeros(n, abi.MS_WALK_IN)
+    r = extent * np.sqrt(rng.random(n))
+    a = 2 * np.pi * rng.random(n)
+    w["si_p"][:, 0] = r * np.cos(a)
+    w["si_p"][:, 2] = r * np.sin(a)
+    ls = scene.shapes[light_shape]
+    c = np.asarray(ls["center"], np.float64)
+    ld = uniform_sphere(rng.random(n), rng.random(n))
+    w["ei"]["p"] = (c + ls["radius"] * ld).astype(np.float32)
+    w["ei"]["n"] = ld.astype(np.float32)
+    w["ei"]["d"] = _normalize(w["ei"]["p"].astype(np.float64) - w["si_p"]).astype(np.float32)
+    w["ei"]["weight"] = 1.0
+    w["ei"]["pdf"] = 1.0
+    w["ei"]["kind"] = abi.EI_AREA
+    w["ei"]["emitter"] = 0
+    w["shape"] = caster
+    w["sample"] = rng.random((n, 2)).astype(np.float32)
+    w["first_path"] = 1 if first_path else 0
+    w["seed_shapes"] = seed_shapes
+    if p_start is not None:
+        w["p_start"] = p_start
+    return w
+
+
+def floor_queries_at(points, camera_origin, shape):
+    """Estimator queries at given floor points (y=0, normal +y), viewed from ``camera_origin``."""
+    p = np.asarray(points, np.float64).reshape(-1, 3)
+    q = np.zeros(len(p), abi.SS_QUERY)
+    q["p"] = p
+    q["n"] = (0.0, 1.0, 0.0)
+    q["sh_n"] = (0.0, 1.0, 0.0)
+    q["sh_s"] = (1.0, 0.0, 0.0)
+    q["sh_t"] = (0.0, 0.0, -1.0)
+    wi = _normalize(np.asarray(camera_origin, np.float64) - p)
+    q["wi"] = np.stack([wi[:, 0], -wi[:, 2], wi[:, 1]], -1)
+    q["shape"] = shape
+    return q

@@ -33,8 +33,13 @@ def load():
     lib.oracle_ss_estimate_batch.argtypes = [vp, vp, i64, C.c_uint64, vp, C.POINTER(abi.SmsStats), C.c_int]
     lib.oracle_glint_estimate_batch.argtypes = [vp, vp, i64, C.c_uint64, vp, C.POINTER(abi.SmsStats), C.c_int]
     lib.oracle_glint_make_queries.argtypes = [vp, C.c_int, C.c_uint64, vp, C.c_int]
+    lib.oracle_ms_walk_batch.argtypes = [vp, vp, i64, vp, C.c_int]
+    lib.oracle_ms_estimate_batch.argtypes = [vp, vp, i64, C.c_uint64, vp, C.POINTER(abi.SmsStats), C.c_int]
     lib.oracle_ray_intersect.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     lib.oracle_ss_step_at.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, vp, vp]
+    lib.oracle_ms_tridiag_step.argtypes = [C.c_int, vp, vp, vp, vp, vp]
+    lib.oracle_ms_tridiag_geo.argtypes = [C.c_int, vp, vp, vp]
+    lib.oracle_ms_tridiag_geo.restype = C.c_float
     lib.oracle_acosf.argtypes = [C.c_float]
     lib.oracle_acosf.restype = C.c_float
     lib.oracle_atan2f.argtypes = [C.c_float, C.c_float]
@@ -90,6 +95,20 @@ class Oracle:
         out = np.zeros((len(q), 3), np.float32)
         st = abi.SmsStats()
         self.lib.oracle_glint_estimate_batch(self.h, q.ctypes.data, len(q), seed, out.ctypes.data, C.byref(st), self.threads)
+        return out, st.as_dict()
+
+    def ms_walks(self, win):
+        out = np.zeros(len(win), abi.MS_WALK_OUT)
+        rc = self.lib.oracle_ms_walk_batch(self.h, win.ctypes.data, len(win), out.ctypes.data, self.threads)
+        assert rc == 0, rc
+        return out
+
+    def ms_estimate(self, q, seed):
+        out = np.zeros((len(q), 3), np.float32)
+        st = abi.SmsStats()
+        rc = self.lib.oracle_ms_estimate_batch(self.h, q.ctypes.data, len(q), seed, out.ctypes.data, C.byref(st),
+                                               self.threads)
+        assert rc == 0, rc
         return out, st.as_dict()
 
     def glint_queries(self, spp, seed):

@@ -159,6 +159,106 @@ def test_ss_estimate_sphere_above_plane_is_zero(gpu, oracle_mod):
     orc.close()
 
 
+def _pool_walks(sc, n, seed):
+    rng = np.random.default_rng(seed)
+    w = np.zeros(n, abi.SS_WALK_IN)
+    w["si_p"] = np.stack([rng.uniform(-0.9, 0.9, n), np.full(n, -1.0), rng.uniform(-0.9, 0.9, n)], -1)
+    w["shape"] = 0
+    ls = sc.shapes[2]
+    w["ei"]["p"] = np.asarray(ls["center"]) + ls["radius"] * workloads.uniform_sphere(rng.random(n), rng.random(n))
+    w["ei"]["kind"] = abi.EI_AREA
+    w["n_offset"] = (0, 0, 1)
+    w["sample"] = rng.random((n, 2))
+    return w
+
+
+def test_pool_walks_through_bvh(gpu, oracle_mod):
+    """524,288-triangle Gerstner pool: every reprojection traverses the GPU BVH; the oracle
+    traverses its own BVH, so agreement also checks closest-hit equivalence."""
+    sc = scenes.pool(n=512)
+    cfg = abi.SmsConfig.default(max_trials=100000)
+    ctx = gpu.Context(sc, cfg)
+    orc = oracle_mod.Oracle(sc, cfg)
+    w = _pool_walks(sc, 20000, 5)
+    _check_ss_walks(ctx.ss_walks(w), orc.ss_walks(w))
+    q = workloads.plane_queries(sc, 4000, 2, extent=0.9, shape=1)
+    q["p"][:, 1] = -1.0
+    rg, sg = ctx.ss_estimate(q, 7)
+    ro, so = orc.ss_estimate(q, 7)
+    print("gpu", sg, "oracle", so, flush=True)
+    assert (ro[:, 0] > 0).mean() > 0.05
+    assert np.array_equal(rg, ro)
+    ctx.close()
+    orc.close()
+
+
+def _check_ms_walks(go, oo):
+    print(f"ms walks={len(go)} seed={oo['seed_valid'].mean():.3f} success={oo['success'].mean():.3f} "
+          f"iters={oo['iterations'].mean():.2f}", flush=True)
+    for k in ("success", "seed_valid", "iterations", "n_vertices", "shapes"):
+        assert np.array_equal(go[k], oo[k]), k
+    ok = oo["success"] == 1
+    assert _rel_dist(go["p"][ok].reshape(-1, 3), oo["p"][ok].reshape(-1, 3)).max(initial=0) <= REL_TOL
+    assert np.array_equal(go["p"], oo["p"]), "chain vertices not bit-equal"
+
+
+@pytest.fixture(scope="module")
+def slab_scene():
+    return scenes.slab(width=64, height=64, n=256)
+
+
+@pytest.mark.parametrize("bounces,halfvector", [(2, 0), (2, 1), (1, 0), (3, 0)])
+def test_ms_walks_slab(gpu, oracle_mod, slab_scene, bounces, halfvector):
+    """Two-vertex chains through the displaced slab (bottom face -> top face); bounces=1 and 3
+    exercise chains that stop early or run out of specular surfaces, and the 4-vertex kernel."""
+    cfg = abi.SmsConfig.default(bounces=bounces, halfvector_constraints=halfvector)
+    ctx = gpu.Context(slab_scene, cfg)
+    orc = oracle_mod.Oracle(slab_scene, cfg)
+    w = workloads.ms_walk_inputs(slab_scene, 20000, 21, extent=0.35)
+    oo = orc.ms_walks(w)
+    _check_ms_walks(ctx.ms_walks(w), oo)
+    if bounces == 2 and not halfvector:
+        assert oo["success"].mean() > 0.02
+        w2 = workloads.ms_walk_inputs(slab_scene, 20000, 22, extent=0.35, first_path=False, seed_shapes=(1, 0, -1, -1))
+        _check_ms_walks(ctx.ms_walks(w2), orc.ms_walks(w2))
+    ctx.close()
+    orc.close()
+
+
+@pytest.mark.parametrize("biased", [1, 0])
+def test_ms_estimate_slab(gpu, oracle_mod, slab_scene, biased):
+    cfg = abi.SmsConfig.default(bounces=2, biased=biased, max_trials=32 if biased else 100000)
+    ctx = gpu.Context(slab_scene, cfg)
+    orc = oracle_mod.Oracle(slab_scene, cfg)
+    q = workloads.plane_queries(slab_scene, 1500, 5, extent=0.35, shape=2)
+    rg, sg = ctx.ms_estimate(q, 9)
+    ro, so = orc.ms_estimate(q, 9)
+    print("gpu", sg, "oracle", so, flush=True)
+    assert (ro[:, 0] > 0).mean() > 0.02
+    assert np.array_equal(rg, ro)
+    for k in ("walks", "walks_converged", "newton_iterations"):
+        assert sg[k] == so[k], k
+    if not biased:
+        for k in ("bernoulli_calls", "bernoulli_trials", "bernoulli_max"):
+            assert sg[k] == so[k], k
+        ctx.set_schedule(lane_trials=1)
+        rg2, sg2 = ctx.ms_estimate(q, 9)
+        assert sg2["overflow_queries"] > 0
+        assert np.array_equal(rg2, ro)
+        assert sg2["bernoulli_trials"] == so["bernoulli_trials"]
+    ctx.close()
+    orc.close()
+
+
+def test_ms_rejects_unbuilt_chain_lengths(gpu, slab_scene):
+    ctx = gpu.Context(slab_scene, abi.SmsConfig.default(bounces=5))
+    with pytest.raises(gpu.SmsError):
+        ctx.ms_walks(workloads.ms_walk_inputs(slab_scene, 4, 1))
+    with pytest.raises(gpu.SmsError):
+        ctx.ms_estimate(workloads.plane_queries(slab_scene, 4, 1, extent=0.3, shape=2), 1)
+    ctx.close()
+
+
 def test_empty_batches(gpu):
     sc = scenes.sphere_dome()
     ctx = gpu.Context(sc, abi.SmsConfig.default())
