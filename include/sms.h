@@ -273,12 +273,22 @@ sms_status sms_ctx_create(const sms_scene_desc *scene, const sms_config *cfg, sm
 sms_status sms_ctx_destroy(sms_ctx *ctx);
 sms_status sms_ctx_set_config(sms_ctx *ctx, const sms_config *cfg);
 /* Scheduling knobs of the persistent estimator kernels (chosen by the autotuner).
-   lane_trials: Bernoulli trials one lane runs for a query before the query is handed to the
-   one-wavefront-per-query tail pass; blocks_per_cu: persistent blocks per compute unit
-   (0 = the occupancy limit).  No counterpart in the reference (its trial loops are serial). */
+   lane_trials: Bernoulli trials one lane (glints: one lane group) runs for a query before the
+   query is handed on to the wavefront-wide tail passes; blocks_per_cu: persistent blocks per
+   compute unit (0 = the occupancy limit); trial_group (glints): lanes that run one query's
+   trials side by side in the first trial pass (1, 2, 4, 8 or 16); tail_rounds (glints):
+   64-trial rounds of the one-wavefront pass before a chain moves to the multi-wavefront stage
+   (0 = straight to it); long_chains / long_trials (glints): chains the multi-wavefront stage
+   holds (at most 1024) and trials it runs per chain (a multiple of 64, at most 32768); the
+   rest continue one wavefront per chain.  No counterpart in the reference (its trial loops
+   are serial). */
 typedef struct sms_schedule {
     int32_t lane_trials;
     int32_t blocks_per_cu;
+    int32_t trial_group;
+    int32_t tail_rounds;
+    int32_t long_chains;
+    int32_t long_trials;
     int32_t pad[2];
 } sms_schedule;
 sms_status sms_ctx_set_schedule(sms_ctx *ctx, const sms_schedule *s);
@@ -310,6 +320,12 @@ sms_status sms_ss_estimate_batch(sms_ctx *ctx, const sms_ss_query *q, int64_t n,
                                  float *out_rgb, sms_stats *stats);
 sms_status sms_glint_estimate_batch(sms_ctx *ctx, const sms_glint_query *q, int64_t n,
                                     uint64_t seed, float *out_rgb, sms_stats *stats);
+/* Autotuner (no counterpart in the reference): times sms_glint_estimate_batch on the given
+   device-resident queries for every candidate schedule (trial_group x lane_trials x
+   tail_rounds x blocks_per_cu), keeps the fastest in the context and returns it in *best.
+   out_rgb is device scratch of 3 * n floats; results do not depend on the schedule. */
+sms_status sms_glint_autotune(sms_ctx *ctx, const sms_glint_query *q, int64_t n, uint64_t seed,
+                              float *out_rgb, sms_schedule *best);
 /* Multi-scatter estimator (SpecularManifoldMultiScatter::specular_manifold_sampling,
    manifold_ms.cpp:27-201) over receiver queries; config.bounces vertices per chain. */
 sms_status sms_ms_estimate_batch(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed,

@@ -194,7 +194,11 @@ struct Hit { bool hit; float t; int shape, prim; float b1, b2, lx, ly; };
 
 // Scene::ray_intersect / ray_test (scene_native.inl:17-55): closest hit over every shape,
 // ties broken by (t, shape, prim) so the result does not depend on traversal order.
-SMS_DEV Hit scene_hit(const DScene &S, const Ray &ray, bool any_hit) {
+// BVH = false drops the triangle-BVH traversal (and its per-lane stack) for scenes made of
+// analytic shapes only.  The host rejects BVHs deeper than SMS_BVH_STACK.
+constexpr int SMS_BVH_STACK = 64;
+template <bool BVH>
+SMS_DEV Hit scene_hit_t(const DScene &S, const Ray &ray, bool any_hit) {
     Hit h; h.hit = false; h.t = __builtin_inff(); h.shape = 1 << 30; h.prim = 1 << 30;
     h.b1 = h.b2 = h.lx = h.ly = 0.f;
     for (int k = 0; k < S.n_analytic; ++k) {
@@ -210,8 +214,8 @@ SMS_DEV Hit scene_hit(const DScene &S, const Ray &ray, bool any_hit) {
             if (any_hit) return h;
         }
     }
-    if (S.n_nodes > 0) {
-        int stack[64];
+    if (BVH && S.n_nodes > 0) {
+        int stack[SMS_BVH_STACK];
         int sp = 0;
         int node = 0;
         while (true) {
@@ -318,8 +322,10 @@ SMS_DEV SI make_si(const DScene &S, const Ray &ray, const Hit &h) {
     return si;
 }
 
+SMS_DEV Hit scene_hit(const DScene &S, const Ray &ray, bool any_hit) { return scene_hit_t<true>(S, ray, any_hit); }
 SMS_DEV SI ray_intersect(const DScene &S, const Ray &ray) { return make_si(S, ray, scene_hit(S, ray, false)); }
 SMS_DEV bool ray_test(const DScene &S, const Ray &ray) { return scene_hit(S, ray, true).hit; }
+template <bool BVH> SMS_DEV bool ray_test_t(const DScene &S, const Ray &ray) { return scene_hit_t<BVH>(S, ray, true).hit; }
 
 // ---------------------------------------------------------------- normal derivatives
 SMS_DEV void normal_derivative(const DScene &S, const SI &si, V3 &dn_du, V3 &dn_dv) {

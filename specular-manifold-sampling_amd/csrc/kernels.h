@@ -15,20 +15,78 @@ struct OverflowRec {
     int32_t pad;
 };
 
-// Compacted glint footprint (one per query whose setup succeeded), 40 bytes.
+// Glint footprint of a query that reaches the solve (block-local compaction in LDS), 40 bytes.
 struct GlintRec {
     int32_t q;            // query index
     float target[2];      // target slope (manifold_glints.cpp:241-247)
     float uv[2], duv_dx[2], duv_dy[2];
-    int32_t pad;
+    int32_t shape;
 };
 
-// Per-record outcome of the glint walks: status 0 = first walk failed, 1 = resolved,
-// 2 = handed to the tail pass (inv_prob written there).
-struct GlintRes {
+// A glint query whose first walk converged: the footprint and the solution its Bernoulli
+// trials must find again, 48 bytes.
+struct TrialRec {
+    GlintRec f;
     float uv_sol[2];
+};
+
+// Outcome of a trial item: status 1 = resolved (inv_prob final), 2 = handed to the tail pass.
+struct GlintRes {
     float inv_prob;
     int32_t status;
+};
+
+// Hot device counters exist in SPREAD copies SPREAD_STRIDE u64 apart (separate 128-byte
+// lines): a wavefront updates the copy picked by its block and wave index, so thousands of
+// wavefronts finishing together do not serialise on one address.  The host sums the copies.
+constexpr int SPREAD = 64;
+constexpr int SPREAD_STRIDE = 16;
+SMS_DEV int spread_slot() { return (int) (((blockIdx.x << 2) + (threadIdx.x >> 6)) & (SPREAD - 1)); }
+
+// stats counter j of copy 0 is stats[j]; add / max a wavefront's values into its copy
+SMS_DEV void stat_add(unsigned long long *dst, unsigned long long v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst + spread_slot() * SPREAD_STRIDE, v);
+}
+SMS_DEV void stat_max(unsigned long long *dst, unsigned long long v) {
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    if ((threadIdx.x & 63) == 0 && v) atomicMax(dst + spread_slot() * SPREAD_STRIDE, v);
+}
+
+// A list split into SPREAD sub-lists of sub_cap items, each with its own counter; producers
+// append to a sub-list they are assigned, consumers walk the sub-lists by a static mapping
+// (no claim counter).  Items are addressed by slot = sub-list * sub_cap + position.
+template <class T>
+struct SpreadList {
+    T *items;
+    unsigned long long *counts;       // SPREAD counters, SPREAD_STRIDE apart
+    unsigned long long sub_cap;
+};
+template <class T>
+SMS_DEV unsigned long long spread_append(const SpreadList<T> &L, int sub, unsigned long long n) {
+    return atomicAdd(L.counts + sub * SPREAD_STRIDE, n);
+}
+// Static consumer mapping: unit u of n_units (n_units >= SPREAD) walks sub-list u % SPREAD at
+// positions u / SPREAD, u / SPREAD + m, ... where m = units on that sub-list.
+struct SpreadWalk { int sub; unsigned long long pos, step, count; };
+template <class T>
+SMS_DEV SpreadWalk spread_walk(const SpreadList<T> &L, unsigned long long u, unsigned long long n_units) {
+    SpreadWalk w;
+    w.sub = (int) (u % SPREAD);
+    w.pos = u / SPREAD;
+    w.step = (n_units - (unsigned long long) w.sub + SPREAD - 1) / SPREAD;
+    w.count = L.counts[w.sub * SPREAD_STRIDE];
+    return w;
+}
+
+// A work list of OverflowRec items with its length and claim cursor.
+struct WorkList {
+    OverflowRec *items;
+    unsigned long long *count;
+    unsigned long long *cursor;
 };
 
 // Device work-queue state of one estimator call.
@@ -36,25 +94,25 @@ struct EstimatorWork {
     unsigned long long *cursor;       // next item of the persistent pass
     unsigned long long *n_overflow;   // overflow list length
     unsigned long long *coop_cursor;  // next overflow record (tail pass)
-    unsigned long long *n_rec;        // compacted glint records
-    unsigned long long *stats;        // sms_stats counters (walks, converged, iterations, calls, trials, max, tail its)
+    unsigned long long *stats;        // copy 0 of the spread sms_stats counters (walks, converged, iterations,
+                                      // calls, trials, max, tail iterations)
     OverflowRec *overflow;
-    GlintRec *rec;
-    GlintRes *res;
-    int lane_trials;                  // trials a lane runs before handing a query to the tail pass
+    GlintRes *res;                    // per trial slot
+    SpreadList<TrialRec> trials;      // glints: queries whose first walk converged
+    SpreadList<OverflowRec> over;     // glints: trial items left after the grouped trial pass
+    int lane_trials;                  // trials a lane (or lane group) runs before handing a query on
+    int group;                        // glints: lanes per item in the first trial pass
+    int tail_rounds;                  // glints: 64-trial rounds of the one-wavefront tail pass
+    // glints, multi-wavefront stage for long trial chains
+    WorkList long_list;               // items left after the one-wavefront rounds
+    WorkList seq_list;                // items left after the multi-wavefront stage
+    long long *mw_found;              // first matching trial per long item (atomic min)
+    unsigned long long *mw_next;      // next 64-trial chunk per long item
+    uint16_t *mw_log;                 // per trial: iterations | converged << 15
+    int mw_items;                     // long items the stage holds (the rest go to seq_list)
+    int mw_trials;                    // trials per long item the stage may run
 };
 
-SMS_DEV void wave_add_u64(unsigned long long *dst, unsigned long long v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, v);
-}
-SMS_DEV void wave_max_u64(unsigned long long *dst, unsigned long long v) {
-    for (int off = 32; off > 0; off >>= 1) {
-        unsigned long long o = __shfl_xor(v, off, 64);
-        v = o > v ? o : v;
-    }
-    if ((threadIdx.x & 63) == 0 && v) atomicMax(dst, v);
-}
 SMS_DEV void write_rgb(float *out, int64_t q, V3 v) { out[3 * q] = v.x; out[3 * q + 1] = v.y; out[3 * q + 2] = v.z; }
 
 SMS_DEV SI glint_query_si(const sms_glint_query &g) {
@@ -74,9 +132,17 @@ SMS_DEV V3 glint_result(float val, float inv_prob, float f, V3 ew) {
 __global__ void k_ss_walk(DScene S, const sms_ss_walk_in *in, int64_t n, sms_ss_walk_out *out);
 __global__ void k_glint_walk(DScene S, const sms_glint_walk_in *in, int64_t n, sms_glint_walk_out *out);
 __global__ void k_glint_queries(DScene S, int spp, uint64_t seed, sms_glint_query *q, int64_t n);
-__global__ void k_glint_setup(DScene S, const sms_glint_query *Q, int64_t n, uint64_t seed, float *out, EstimatorWork W);
-__global__ void k_glint_walks(DScene S, uint64_t seed, EstimatorWork W);
-__global__ void k_glint_coop(DScene S, uint64_t seed, EstimatorWork W);
+__global__ void k_glint_first_bvh(DScene S, const sms_glint_query *Q, int64_t n, uint64_t seed, float *out, EstimatorWork W);
+__global__ void k_glint_first_nobvh(DScene S, const sms_glint_query *Q, int64_t n, uint64_t seed, float *out, EstimatorWork W);
+__global__ void k_glint_trials_g1(DScene S, uint64_t seed, EstimatorWork W);
+__global__ void k_glint_trials_g2(DScene S, uint64_t seed, EstimatorWork W);
+__global__ void k_glint_trials_g4(DScene S, uint64_t seed, EstimatorWork W);
+__global__ void k_glint_trials_g8(DScene S, uint64_t seed, EstimatorWork W);
+__global__ void k_glint_trials_g16(DScene S, uint64_t seed, EstimatorWork W);
+__global__ void k_glint_coop_rounds(DScene S, uint64_t seed, EstimatorWork W);
+__global__ void k_glint_multiwave(DScene S, uint64_t seed, EstimatorWork W);
+__global__ void k_glint_multiwave_resolve(DScene S, EstimatorWork W);
+__global__ void k_glint_coop_seq(DScene S, uint64_t seed, EstimatorWork W);
 __global__ void k_glint_finalize(DScene S, const sms_glint_query *Q, uint64_t seed, float *out, EstimatorWork W);
 __global__ void k_ss_estimate(DScene S, const sms_ss_query *Q, int64_t n, uint64_t seed, float *part, float *invp, EstimatorWork W);
 __global__ void k_ss_coop(DScene S, const sms_ss_query *Q, uint64_t seed, float *invp, EstimatorWork W);

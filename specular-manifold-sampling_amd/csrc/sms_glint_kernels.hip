@@ -1,21 +1,28 @@
 // sms_glint_kernels.hip -- unbiased glint SMS estimator (SpecularManifoldGlints::
-// specular_manifold_sampling, src/librender/manifold_glints.cpp:28-181) as four passes:
+// specular_manifold_sampling, src/librender/manifold_glints.cpp:28-181) as a pass sequence
+// in which every Bernoulli trial chain, however long, finishes within a few kernel-lengths:
 //
-//   k_glint_setup     one lane per query: zero the output, sample the emitter, test its
-//                     visibility, compute the footprint's target slope; queries that reach the
-//                     solve are compacted (one atomic per wavefront) into GlintRec records
-//   k_glint_walks     persistent: every lane owns a record and advances its current walk one
-//                     Newton iteration per loop trip -- the first walk, then Bernoulli trials
-//                     until one lands on the same solution; a lane whose query needs more than
-//                     `lane_trials` trials hands it to the overflow list and refills
-//   k_glint_coop      tail pass: one wavefront per overflow record; lane j runs trial k0 + j,
-//                     the footprint's texels are staged in LDS, the first match comes from a
-//                     ballot
-//   k_glint_finalize  one lane per record with a solution: contribution (Fresnel x generalized
-//                     geometric term), sensor importance, output (f * (val * inv_prob)) * ew
+//   k_glint_first      one lane per query, 256 consecutive queries per block: zero the output,
+//                      sample the emitter, test its visibility, compute the footprint's target
+//                      slope; the queries that reach the solve are compacted inside the block
+//                      (LDS) so whole wavefronts run first walks and empty ones retire; a
+//                      converged first walk becomes a trial item (one atomic per block, on the
+//                      block's sub-list of the spread trial list)
+//   k_glint_trials_gG  G lanes per trial item run G trials side by side, `lane_trials` trials
+//                      per item; most chains end here (the mean chain is about 3 trials)
+//   k_glint_coop_rounds one wavefront per remaining item, 64 trials per round, `tail_rounds`
+//                      rounds, the footprint's texels staged in LDS
+//   k_glint_multiwave  the few long chains left get up to 64 wavefronts each, which claim
+//                      64-trial chunks of their item; the first match is an atomic min and a
+//                      per-trial log keeps the counters exact (k_glint_multiwave_resolve)
+//   k_glint_coop_seq   one wavefront per item for chains longer than the multi-wave range
+//   k_glint_finalize   one lane per trial item: contribution (Fresnel x generalized geometric
+//                      term), sensor importance, output (f * (val * inv_prob)) * ew
 //
-// Walk k of query q draws from PCG32 stream (q, k + 1), so the result of every query equals
-// the sequential trial loop whatever lane or pass runs which trial.
+// Work is assigned by static mappings over spread lists and counters are spread over copies
+// (kernels.h), so no pass serialises thousands of wavefronts on one atomic address.  Walk k
+// of query q draws from PCG32 stream (q, k + 1), so the result of every query equals the
+// sequential trial loop whatever lane, group, wavefront or pass runs which trial.
 #include "kernels.h"
 
 namespace smsdev {
@@ -28,17 +35,32 @@ SMS_DEV void footprint_from_rec(const GlintRec &r, GlintFootprint &fp) {
     footprint_setup(fp);
 }
 
-// ---------------------------------------------------------------- pass 1: setup + compaction
-__global__ void __launch_bounds__(256) k_glint_setup(DScene S, const sms_glint_query *__restrict__ Q, int64_t n,
-                                                     uint64_t seed, float *__restrict__ out, EstimatorWork W) {
-    int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+SMS_DEV void footprint_texels(const DScene &S, int shape, GlintFootprint &fp) {
+    const DShape &gs = S.shapes[shape];
+    const DNormalmap nm = S.nmaps[gs.normalmap];
+    fp.slope = nm.slope; fp.res = nm.res; fp.stride = nm.stride; fp.fres = (float) nm.res; fp.tiles = gs.tiles;
+}
+
+SMS_DEV unsigned long long wave_sum_u64(unsigned long long v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+
+// ---------------------------------------------------------------- pass 1: setup + first walks
+template <bool BVH>
+SMS_DEV void glint_first_body(const DScene &S, const sms_glint_query *__restrict__ Q, int64_t n, uint64_t seed,
+                              float *__restrict__ out, EstimatorWork W, GlintRec *lrec, int *wcount,
+                              unsigned long long *lbase, unsigned long long (*lstats)[4]) {
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int64_t q = (int64_t) blockIdx.x * blockDim.x + tid;
     bool valid = false;
-    unsigned long long st_walks = 0;
+    unsigned long long st_walks = 0, st_conv = 0, st_iters = 0;
     GlintRec rec;
     if (q < n) {
         const sms_glint_query g = Q[q];
         write_rgb(out, q, v3(0.f, 0.f, 0.f));
-        if (g.shape >= 0) {
+        if (g.shape >= 0 && g.shape < S.n_shapes && S.shapes[g.shape].bsdf == SMS_BSDF_NORMALMAP) {
             SI si = glint_query_si(g);
             Pcg32 rq;
             stream_seed(rq, query_key(seed, (uint64_t) q), 0);
@@ -46,7 +68,7 @@ __global__ void __launch_bounds__(256) k_glint_setup(DScene S, const sms_glint_q
             V3 edir = ei.p - si.p;
             float edist = norm(edir);
             edir = edir * (1.f / edist);
-            bool occluded = ray_test(S, make_ray(si.p, edir, RAY_EPS_F * (1.f + hmax_abs(si.p)), edist * (1.f - RAY_EPS_F)));
+            bool occluded = ray_test_t<BVH>(S, make_ray(si.p, edir, RAY_EPS_F * (1.f + hmax_abs(si.p)), edist * (1.f - RAY_EPS_F)));
             if (!occluded) {
                 V2 target;
                 if (glint_target(ld3(g.sensor_p), ei.p, si.p, si.sh_s, si.sh_t, si.sh_n, v3(0.f, 0.f, 1.f), target)) {
@@ -56,230 +78,469 @@ __global__ void __launch_bounds__(256) k_glint_setup(DScene S, const sms_glint_q
                     rec.uv[0] = g.uv[0]; rec.uv[1] = g.uv[1];
                     rec.duv_dx[0] = g.duv_dx[0]; rec.duv_dx[1] = g.duv_dx[1];
                     rec.duv_dy[0] = g.duv_dy[0]; rec.duv_dy[1] = g.duv_dy[1];
-                    rec.pad = g.shape;
+                    rec.shape = g.shape;
                 } else {
                     st_walks = 1;          // sample_glint fails its side test before the solve
                 }
             }
         }
     }
-    // wavefront-aggregated append
+    // block-local compaction of the solvable queries
     unsigned long long ballot = __ballot(valid);
-    unsigned long long base = 0;
-    int lane = threadIdx.x & 63;
-    if (lane == 0 && ballot) base = atomicAdd(W.n_rec, (unsigned long long) __popcll(ballot));
-    base = __shfl(base, 0, 64);
-    if (valid) {
-        unsigned long long rank = __popcll(ballot & ((1ull << lane) - 1ull));
-        W.rec[base + rank] = rec;
+    if (lane == 0) wcount[wv] = (int) __popcll(ballot);
+    __syncthreads();
+    int base = 0, total = 0;
+    for (int i = 0; i < 4; ++i) {
+        int c = wcount[i];
+        base += i < wv ? c : 0;
+        total += c;
     }
-    wave_add_u64(W.stats + 0, st_walks);
+    if (valid) lrec[base + (int) __popcll(ballot & ((1ull << lane) - 1ull))] = rec;
+    __syncthreads();
+    // first walks: slot tid of the compacted block
+    bool conv = false;
+    GlintRec r;
+    V2 uv_sol = v2(0.f, 0.f);
+    if (tid < total) {
+        r = lrec[tid];
+        GlintFootprint fp;
+        footprint_texels(S, r.shape, fp);
+        footprint_from_rec(r, fp);
+        Pcg32 rw;
+        stream_seed(rw, query_key(seed, (uint64_t) r.q), 1);
+        V2 xi;
+        xi.x = pcg32_float(rw);
+        xi.y = pcg32_float(rw);
+        GlintWalk w;
+        glint_walk_start(fp, xi, w);
+        int st;
+        while ((st = glint_walk_iter(fp, S.cfg, w)) == WALK_RUNNING) {}
+        st_walks += 1;
+        st_iters += (unsigned long long) w.it;
+        conv = st == WALK_CONVERGED;
+        uv_sol = w.uv;
+    }
+    if (conv) st_conv = 1;
+    // append converged first walks as trial items: one atomic per block, on the block's own
+    // sub-list of the spread trial list; the block's counters go to its copy of the stats
+    unsigned long long cb = __ballot(conv);
+    __syncthreads();                                   // wcount reuse
+    if (lane == 0) wcount[wv] = (int) __popcll(cb);
+    unsigned long long sw = wave_sum_u64(st_walks), sc = wave_sum_u64(st_conv), si_ = wave_sum_u64(st_iters);
+    if (lane == 0) { lstats[wv][0] = sw; lstats[wv][1] = sc; lstats[wv][2] = si_; }
+    __syncthreads();
+    const int sub = (int) (blockIdx.x % SPREAD);
+    if (tid == 0) {
+        int nb = wcount[0] + wcount[1] + wcount[2] + wcount[3];
+        *lbase = nb ? spread_append(W.trials, sub, (unsigned long long) nb) : 0ull;
+        unsigned long long *st = W.stats + sub * SPREAD_STRIDE;
+        unsigned long long v0 = (lstats[0][0] + lstats[1][0]) + (lstats[2][0] + lstats[3][0]);
+        unsigned long long v1 = (lstats[0][1] + lstats[1][1]) + (lstats[2][1] + lstats[3][1]);
+        unsigned long long v2_ = (lstats[0][2] + lstats[1][2]) + (lstats[2][2] + lstats[3][2]);
+        if (v0) atomicAdd(st + 0, v0);
+        if (v1) atomicAdd(st + 1, v1);
+        if (v2_) atomicAdd(st + 2, v2_);
+        if (nb) atomicAdd(st + 3, (unsigned long long) nb);   // Bernoulli loops started (bernoulli_calls)
+    }
+    __syncthreads();
+    if (conv) {
+        int wb = 0;
+        for (int i = 0; i < 4; ++i) wb += i < wv ? wcount[i] : 0;
+        unsigned long long pos = *lbase + (unsigned long long) wb + (unsigned long long) __popcll(cb & ((1ull << lane) - 1ull));
+        TrialRec t;
+        t.f = r;
+        t.uv_sol[0] = uv_sol.x; t.uv_sol[1] = uv_sol.y;
+        W.trials.items[(size_t) sub * W.trials.sub_cap + pos] = t;
+    }
 }
 
-// ---------------------------------------------------------------- pass 2: persistent walks
-struct WalkLane {
-    int32_t r;                  // record index or -1
-    int32_t k;                  // 0 = first walk, k >= 1 = Bernoulli trial k
-    uint64_t qkey;
-    GlintFootprint fp;
-    GlintWalk w;
-    V2 uv_sol;
-};
+#define SMS_GLINT_FIRST_KERNEL(NAME, BVH)                                                                        \
+    __global__ void __launch_bounds__(256) NAME(DScene S, const sms_glint_query *__restrict__ Q, int64_t n,      \
+                                                uint64_t seed, float *__restrict__ out, EstimatorWork W) {       \
+        __shared__ GlintRec lrec[256];                                                                           \
+        __shared__ int wcount[4];                                                                                \
+        __shared__ unsigned long long lbase;                                                                     \
+        __shared__ unsigned long long lstats[4][4];                                                              \
+        glint_first_body<BVH>(S, Q, n, seed, out, W, lrec, wcount, &lbase, lstats);                              \
+    }
+SMS_GLINT_FIRST_KERNEL(k_glint_first_bvh, true)
+SMS_GLINT_FIRST_KERNEL(k_glint_first_nobvh, false)
 
-SMS_DEV void start_walk(WalkLane &L) {
+// ---------------------------------------------------------------- trial walks
+// Trial k (>= 1) of a trial item: the walk from stream (q, k + 1).  Returns the walk status.
+template <class Fetch>
+SMS_DEV int glint_trial(const GlintFootprint &fp, const sms_config &cfg, uint64_t qkey, int64_t k, GlintWalk &w,
+                        const Fetch &fetch) {
     Pcg32 rw;
-    stream_seed(rw, L.qkey, (uint64_t) L.k + 1);
+    stream_seed(rw, qkey, (uint64_t) k + 1);
     V2 xi;
     xi.x = pcg32_float(rw);
     xi.y = pcg32_float(rw);
-    glint_walk_start(L.fp, xi, L.w);
+    glint_walk_start(fp, xi, w);
+    int s;
+    while ((s = glint_walk_iter_f(fp, cfg, w, fetch)) == WALK_RUNNING) {}
+    return s;
 }
 
-__global__ void __launch_bounds__(256) k_glint_walks(DScene S, uint64_t seed, EstimatorWork W) {
-    const sms_config &cfg = S.cfg;
-    const unsigned long long n_rec = *W.n_rec;
-    unsigned long long st_walks = 0, st_conv = 0, st_iters = 0, st_bcalls = 0, st_btrials = 0, st_bmax = 0;
-    WalkLane L;
-    L.r = -1;
-    while (true) {
-        if (L.r < 0) {
-            unsigned long long r = atomicAdd(W.cursor, 1ull);
-            if (r >= n_rec) break;
-            const GlintRec rec = W.rec[r];
-            const DShape &gs = S.shapes[rec.pad];
-            const DNormalmap nm = S.nmaps[gs.normalmap];
-            L.fp.slope = nm.slope; L.fp.res = nm.res; L.fp.stride = nm.stride; L.fp.fres = (float) nm.res;
-            L.fp.tiles = gs.tiles;
-            footprint_from_rec(rec, L.fp);
-            L.r = (int32_t) r;
-            L.k = 0;
-            L.qkey = query_key(seed, (uint64_t) rec.q);
-            start_walk(L);
-        }
-        int st = glint_walk_iter(L.fp, cfg, L.w);
-        if (st == WALK_RUNNING) continue;
-        st_walks++;
-        st_iters += (unsigned long long) L.w.it;
-        bool ok = st == WALK_CONVERGED;
-        if (ok) st_conv++;
-        if (L.k == 0) {
-            if (!ok) {
-                GlintRes o; o.uv_sol[0] = o.uv_sol[1] = 0.f; o.inv_prob = 0.f; o.status = 0;
-                W.res[L.r] = o;
-                L.r = -1;
-                continue;
-            }
-            L.uv_sol = L.w.uv;
-            st_bcalls++;
-            L.k = 1;
-        } else {
-            bool done = false;
-            float inv_prob = 0.f;
-            if (ok && norm2(L.w.uv - L.uv_sol) < cfg.uniqueness_threshold) {
-                inv_prob = (float) L.k;          // match at trial k (manifold_glints.cpp:85-104)
-                done = true;
-            } else {
-                L.k++;
-                if (cfg.max_trials > 0 && L.k > cfg.max_trials) done = true;     // inv_prob = 0
-            }
-            if (done) {
-                st_btrials += (unsigned long long) L.k;
-                st_bmax = st_bmax > (unsigned long long) L.k ? st_bmax : (unsigned long long) L.k;
-                GlintRes o; o.uv_sol[0] = L.uv_sol.x; o.uv_sol[1] = L.uv_sol.y; o.inv_prob = inv_prob; o.status = 1;
-                W.res[L.r] = o;
-                L.r = -1;
-                continue;
-            }
-            if (L.k > W.lane_trials) {
-                unsigned long long slot = atomicAdd(W.n_overflow, 1ull);
-                OverflowRec ov;
-                ov.q = L.r; ov.k = L.k; ov.sol[0] = L.uv_sol.x; ov.sol[1] = L.uv_sol.y; ov.sol[2] = 0.f;
-                ov.shapes[0] = ov.shapes[1] = ov.shapes[2] = ov.shapes[3] = -1; ov.pad = 0;
-                W.overflow[slot] = ov;
-                GlintRes o; o.uv_sol[0] = L.uv_sol.x; o.uv_sol[1] = L.uv_sol.y; o.inv_prob = 0.f; o.status = 2;
-                W.res[L.r] = o;
-                L.r = -1;
-                continue;
-            }
-        }
-        start_walk(L);
-    }
-    wave_add_u64(W.stats + 0, st_walks); wave_add_u64(W.stats + 1, st_conv); wave_add_u64(W.stats + 2, st_iters);
-    wave_add_u64(W.stats + 3, st_bcalls); wave_add_u64(W.stats + 4, st_btrials); wave_max_u64(W.stats + 5, st_bmax);
+SMS_DEV int64_t trial_cap(const sms_config &cfg) {
+    return cfg.max_trials > 0 ? (int64_t) cfg.max_trials : (int64_t) 1 << 40;
 }
 
-// ---------------------------------------------------------------- pass 3: tail (wave per query)
-// The 64 lanes of a wavefront iterate inside one footprint, so its texels (plus the bilinear
-// margin) are staged in LDS when they fit; otherwise the walk reads the global slope array.
-constexpr int COOP_TILE = 24;                 // texels per side of the staged window
-__global__ void __launch_bounds__(256) k_glint_coop(DScene S, uint64_t seed, EstimatorWork W) {
-    __shared__ float2 tile[4][COOP_TILE + 1][COOP_TILE + 1];
+SMS_DEV void append_item(const WorkList &L, const OverflowRec &r) {
+    unsigned long long slot = atomicAdd(L.count, 1ull);
+    L.items[slot] = r;
+}
+
+SMS_DEV OverflowRec make_item(int64_t item, int64_t k, V2 uv_sol) {
+    OverflowRec ov;
+    ov.q = item; ov.k = k; ov.sol[0] = uv_sol.x; ov.sol[1] = uv_sol.y; ov.sol[2] = 0.f;
+    ov.shapes[0] = ov.shapes[1] = ov.shapes[2] = ov.shapes[3] = -1; ov.pad = 0;
+    return ov;
+}
+
+
+// ---------------------------------------------------------------- pass 2: grouped trials
+// G lanes per trial item run trials k, k+1, ..., k+G-1 side by side; the first match of a
+// round comes from the group's ballot bits.  A group runs ceil(lane_trials / G) rounds; items
+// still open go to the overflow list (the sub-list of their trial slot).  Groups walk the
+// trial list by a static mapping (no claim counter).
+template <int G>
+SMS_DEV void glint_trials_body(const DScene &S, uint64_t seed, EstimatorWork W) {
     const sms_config &cfg = S.cfg;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    unsigned long long st_walks = 0, st_conv = 0, st_iters = 0, st_btrials = 0, st_bmax = 0, st_tail = 0;
-    const unsigned long long n_over = *W.n_overflow;
-    while (true) {
-        unsigned long long item = 0;
-        if (lane == 0) item = atomicAdd(W.coop_cursor, 1ull);
-        item = __shfl(item, 0, 64);
-        if (item >= n_over) break;
-        const OverflowRec ov = W.overflow[item];
-        const GlintRec rec = W.rec[ov.q];
-        const DShape &gs = S.shapes[rec.pad];
-        const DNormalmap nm = S.nmaps[gs.normalmap];
+    const int lane = threadIdx.x & 63;
+    const int gl = lane & (G - 1), gfirst = lane & ~(G - 1);
+    const unsigned long long gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+    const unsigned long long gid = ((unsigned long long) blockIdx.x * blockDim.x + threadIdx.x) / G;
+    const unsigned long long n_groups = ((unsigned long long) gridDim.x * blockDim.x) / G;
+    const int64_t kcap = trial_cap(cfg);
+    const int rounds = (W.lane_trials + G - 1) / G;
+    unsigned long long st_walks = 0, st_conv = 0, st_iters = 0, st_btrials = 0, st_bmax = 0;
+    const SpreadWalk sw = spread_walk(W.trials, gid, n_groups);
+    for (unsigned long long p = sw.pos; p < sw.count; p += sw.step) {
+        const unsigned long long slot = (unsigned long long) sw.sub * W.trials.sub_cap + p;
+        const TrialRec t = W.trials.items[slot];
         GlintFootprint fp;
-        fp.slope = nm.slope; fp.res = nm.res; fp.stride = nm.stride; fp.fres = (float) nm.res; fp.tiles = gs.tiles;
-        footprint_from_rec(rec, fp);
-        // Stage the footprint's texel window (bounding box of the parallelogram, one texel of
-        // margin each side) when it fits; with tiling only footprints inside one tile qualify.
-        LdsTexels<COOP_TILE> lds;
-        lds.tile = tile[wv];
-        lds.g.slope = fp.slope; lds.g.res = fp.res; lds.g.stride = fp.stride;
-        bool staged = false;
-        if (gs.tiles == 1.f) {
-            float ex = 0.5f * (fabsf(fp.duv_dx.x) + fabsf(fp.duv_dy.x)), ey = 0.5f * (fabsf(fp.duv_dx.y) + fabsf(fp.duv_dy.y));
-            float u0 = fp.uv.x - ex, u1 = fp.uv.x + ex, v0 = fp.uv.y - ey, v1 = fp.uv.y + ey;
-            if (u0 > 0.f && v0 > 0.f && u1 < 1.f && v1 < 1.f) {
-                int x0 = (int) floorf(u0 * fp.fres - 0.5f) - 1, y0 = (int) floorf(v0 * fp.fres - 0.5f) - 1;
-                int x1 = (int) floorf(u1 * fp.fres - 0.5f) + 2, y1 = (int) floorf(v1 * fp.fres - 0.5f) + 2;
-                x0 = x0 < 0 ? 0 : x0; y0 = y0 < 0 ? 0 : y0;
-                if (x1 - x0 < COOP_TILE && y1 - y0 < COOP_TILE) {
-                    for (int t = lane; t < (COOP_TILE + 1) * (COOP_TILE + 1); t += 64) {
-                        int ty = t / (COOP_TILE + 1), tx = t % (COOP_TILE + 1);
-                        int gx = x0 + tx, gy = y0 + ty;
-                        gx = gx > fp.res ? fp.res : gx; gy = gy > fp.res ? fp.res : gy;   // pad column/row
-                        tile[wv][ty][tx] = fp.slope[(size_t) gy * fp.stride + gx];
-                    }
-                    lds.x0 = x0; lds.y0 = y0;
-                    staged = true;
-                }
-            }
-        }
-        // The window belongs to this wavefront only (waves of a block leave the item loop at
-        // different times, so no block barrier here): order its LDS writes before the reads.
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        uint64_t qkey = query_key(seed, (uint64_t) rec.q);
-        V2 uv_sol = v2(ov.sol[0], ov.sol[1]);
-        int64_t kcap = cfg.max_trials > 0 ? (int64_t) cfg.max_trials : (int64_t) 1 << 40;
-        int64_t found = -1;
-        for (int64_t base = ov.k; base <= kcap && found < 0; base += 64) {
-            int64_t k = base + lane;
+        footprint_texels(S, t.f.shape, fp);
+        footprint_from_rec(t.f, fp);
+        GlobalTexels tex; tex.slope = fp.slope; tex.res = fp.res; tex.stride = fp.stride;
+        const uint64_t qkey = query_key(seed, (uint64_t) t.f.q);
+        const V2 uv_sol = v2(t.uv_sol[0], t.uv_sol[1]);
+        int64_t found = -1, base = 1;
+        for (int r = 0; r < rounds && found < 0 && base <= kcap; ++r, base += G) {
+            const int64_t k = base + gl;
             bool match = false, ran = false, conv = false;
             int its = 0;
             if (k <= kcap) {
-                Pcg32 rw;
-                stream_seed(rw, qkey, (uint64_t) k + 1);
-                V2 xi; xi.x = pcg32_float(rw); xi.y = pcg32_float(rw);
                 GlintWalk w;
-                glint_walk_start(fp, xi, w);
-                int s;
-                if (staged) { while ((s = glint_walk_iter_f(fp, cfg, w, lds)) == WALK_RUNNING) {} }
-                else { while ((s = glint_walk_iter(fp, cfg, w)) == WALK_RUNNING) {} }
+                int s = glint_trial(fp, cfg, qkey, k, w, tex);
                 ran = true; its = w.it; conv = s == WALK_CONVERGED;
                 match = conv && norm2(w.uv - uv_sol) < cfg.uniqueness_threshold;
             }
-            unsigned long long ballot = __ballot(match);
-            if (ballot) found = base + (int64_t) __ffsll((long long) ballot) - 1;
-            // count only the walks the sequential trial loop would have run (k <= first match)
+            unsigned long long b = (__ballot(match) >> gfirst) & gmask;
+            if (b) found = base + (int64_t) __ffsll((long long) b) - 1;
             if (ran && (found < 0 || k <= found)) {
-                st_walks++; st_iters += (unsigned long long) its; st_tail += (unsigned long long) its;
+                st_walks++; st_iters += (unsigned long long) its;
                 if (conv) st_conv++;
             }
         }
-        // every lane is done reading the window before the next item overwrites it
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        if (lane == 0) {
-            W.res[ov.q].inv_prob = found >= 0 ? (float) found : 0.f;
-            unsigned long long trials = found >= 0 ? (unsigned long long) found : (unsigned long long) kcap + 1ull;
-            st_btrials += trials;
-            st_bmax = st_bmax > trials ? st_bmax : trials;
+        if (gl == 0) {
+            if (found >= 0 || base > kcap) {
+                unsigned long long tr = found >= 0 ? (unsigned long long) found : (unsigned long long) kcap + 1ull;
+                st_btrials += tr; st_bmax = st_bmax > tr ? st_bmax : tr;
+                GlintRes o; o.inv_prob = found >= 0 ? (float) found : 0.f; o.status = 1;
+                W.res[slot] = o;
+            } else {
+                unsigned long long pos = spread_append(W.over, sw.sub, 1ull);
+                W.over.items[(size_t) sw.sub * W.over.sub_cap + pos] = make_item((int64_t) slot, base, uv_sol);
+                GlintRes o; o.inv_prob = 0.f; o.status = 2;
+                W.res[slot] = o;
+            }
         }
     }
-    wave_add_u64(W.stats + 0, st_walks); wave_add_u64(W.stats + 1, st_conv); wave_add_u64(W.stats + 2, st_iters);
-    wave_add_u64(W.stats + 4, st_btrials); wave_max_u64(W.stats + 5, st_bmax); wave_add_u64(W.stats + 6, st_tail);
+    stat_add(W.stats + 0, st_walks); stat_add(W.stats + 1, st_conv); stat_add(W.stats + 2, st_iters);
+    stat_add(W.stats + 4, st_btrials); stat_max(W.stats + 5, st_bmax); stat_add(W.stats + 6, st_iters);
 }
 
-// ---------------------------------------------------------------- pass 4: contributions
+#define SMS_GLINT_TRIALS_KERNEL(G)                                                                               \
+    __global__ void __launch_bounds__(256) k_glint_trials_g##G(DScene S, uint64_t seed, EstimatorWork W) {       \
+        glint_trials_body<G>(S, seed, W);                                                                        \
+    }
+SMS_GLINT_TRIALS_KERNEL(1)
+SMS_GLINT_TRIALS_KERNEL(2)
+SMS_GLINT_TRIALS_KERNEL(4)
+SMS_GLINT_TRIALS_KERNEL(8)
+SMS_GLINT_TRIALS_KERNEL(16)
+
+// ---------------------------------------------------------------- one-wavefront tail passes
+// The 64 lanes of a wavefront iterate inside one footprint, so its texels (plus the bilinear
+// margin) are staged in LDS when they fit; otherwise the walk reads the global slope array.
+constexpr int COOP_TILE = 24;                 // texels per side of the staged window
+
+// Stage the texel window of a footprint into this wavefront's tile; false if it does not fit.
+SMS_DEV bool stage_window(const GlintFootprint &fp, float tiles, float2 (*tile)[COOP_TILE + 1], int lane,
+                          LdsTexels<COOP_TILE> &lds) {
+    lds.tile = tile;
+    lds.g.slope = fp.slope; lds.g.res = fp.res; lds.g.stride = fp.stride;
+    bool staged = false;
+    if (tiles == 1.f) {
+        float ex = 0.5f * (fabsf(fp.duv_dx.x) + fabsf(fp.duv_dy.x)), ey = 0.5f * (fabsf(fp.duv_dx.y) + fabsf(fp.duv_dy.y));
+        float u0 = fp.uv.x - ex, u1 = fp.uv.x + ex, v0 = fp.uv.y - ey, v1 = fp.uv.y + ey;
+        if (u0 > 0.f && v0 > 0.f && u1 < 1.f && v1 < 1.f) {
+            int x0 = (int) floorf(u0 * fp.fres - 0.5f) - 1, y0 = (int) floorf(v0 * fp.fres - 0.5f) - 1;
+            int x1 = (int) floorf(u1 * fp.fres - 0.5f) + 2, y1 = (int) floorf(v1 * fp.fres - 0.5f) + 2;
+            x0 = x0 < 0 ? 0 : x0; y0 = y0 < 0 ? 0 : y0;
+            if (x1 - x0 < COOP_TILE && y1 - y0 < COOP_TILE) {
+                for (int t = lane; t < (COOP_TILE + 1) * (COOP_TILE + 1); t += 64) {
+                    int ty = t / (COOP_TILE + 1), tx = t % (COOP_TILE + 1);
+                    int gx = x0 + tx, gy = y0 + ty;
+                    gx = gx > fp.res ? fp.res : gx; gy = gy > fp.res ? fp.res : gy;   // pad column/row
+                    tile[ty][tx] = fp.slope[(size_t) gy * fp.stride + gx];
+                }
+                lds.x0 = x0; lds.y0 = y0;
+                staged = true;
+            }
+        }
+    }
+    // The window belongs to this wavefront only (waves of a block leave their item loops at
+    // different times, so no block barrier): order its LDS writes before the reads.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    return staged;
+}
+
+SMS_DEV void window_done() {
+    // every lane is done reading the window before it is overwritten
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+
+// One wavefront per overflow item: lane j runs trial k0 + j, at most `max_rounds` rounds of 64
+// (negative = until resolved).  Unresolved items go to `out` with the next trial index.
+SMS_DEV void glint_coop_item(const DScene &S, uint64_t seed, EstimatorWork W, const OverflowRec &ov, int max_rounds,
+                             const WorkList &out, bool init_long, float2 (*tile)[COOP_TILE + 1],
+                             unsigned long long st[5]) {
+    const sms_config &cfg = S.cfg;
+    const int lane = threadIdx.x & 63;
+    const int64_t kcap = trial_cap(cfg);
+    const TrialRec tr = W.trials.items[ov.q];
+    GlintFootprint fp;
+    footprint_texels(S, tr.f.shape, fp);
+    footprint_from_rec(tr.f, fp);
+    LdsTexels<COOP_TILE> lds;
+    bool staged = stage_window(fp, S.shapes[tr.f.shape].tiles, tile, lane, lds);
+    GlobalTexels tex; tex.slope = fp.slope; tex.res = fp.res; tex.stride = fp.stride;
+    const uint64_t qkey = query_key(seed, (uint64_t) tr.f.q);
+    const V2 uv_sol = v2(ov.sol[0], ov.sol[1]);
+    int64_t found = -1, base = ov.k;
+    for (int round = 0; base <= kcap && found < 0 && (max_rounds < 0 || round < max_rounds); base += 64, ++round) {
+        const int64_t k = base + lane;
+        bool match = false, ran = false, conv = false;
+        int its = 0;
+        if (k <= kcap) {
+            GlintWalk w;
+            int s = staged ? glint_trial(fp, cfg, qkey, k, w, lds) : glint_trial(fp, cfg, qkey, k, w, tex);
+            ran = true; its = w.it; conv = s == WALK_CONVERGED;
+            match = conv && norm2(w.uv - uv_sol) < cfg.uniqueness_threshold;
+        }
+        unsigned long long ballot = __ballot(match);
+        if (ballot) found = base + (int64_t) __ffsll((long long) ballot) - 1;
+        // count only the walks the sequential trial loop would have run (k <= first match)
+        if (ran && (found < 0 || k <= found)) {
+            st[0]++; st[2] += (unsigned long long) its;
+            if (conv) st[1]++;
+        }
+    }
+    window_done();
+    if (lane == 0) {
+        if (found >= 0 || base > kcap) {
+            W.res[ov.q].inv_prob = found >= 0 ? (float) found : 0.f;
+            W.res[ov.q].status = 1;
+            unsigned long long trials = found >= 0 ? (unsigned long long) found : (unsigned long long) kcap + 1ull;
+            st[3] += trials;
+            st[4] = st[4] > trials ? st[4] : trials;
+        } else {
+            unsigned long long slot = atomicAdd(out.count, 1ull);
+            out.items[slot] = make_item(ov.q, base, uv_sol);
+            if (init_long && slot < (unsigned long long) W.mw_items) {
+                W.mw_found[slot] = 0x7fffffffffffffffLL;
+                W.mw_next[slot] = 0ull;
+            }
+        }
+    }
+}
+
+SMS_DEV void coop_stats(EstimatorWork W, const unsigned long long st[5]) {
+    stat_add(W.stats + 0, st[0]); stat_add(W.stats + 1, st[1]); stat_add(W.stats + 2, st[2]);
+    stat_add(W.stats + 4, st[3]); stat_max(W.stats + 5, st[4]); stat_add(W.stats + 6, st[2]);
+}
+
+// overflow items, one wavefront each, by a static mapping over the spread overflow list
+__global__ void __launch_bounds__(256) k_glint_coop_rounds(DScene S, uint64_t seed, EstimatorWork W) {
+    __shared__ float2 tile[4][COOP_TILE + 1][COOP_TILE + 1];
+    unsigned long long st[5] = {0ull, 0ull, 0ull, 0ull, 0ull};
+    const unsigned long long wave = ((unsigned long long) blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const unsigned long long n_waves = ((unsigned long long) gridDim.x * blockDim.x) >> 6;
+    const SpreadWalk sw = spread_walk(W.over, wave, n_waves);
+    for (unsigned long long p = sw.pos; p < sw.count; p += sw.step) {
+        const OverflowRec ov = W.over.items[(size_t) sw.sub * W.over.sub_cap + p];
+        glint_coop_item(S, seed, W, ov, W.tail_rounds, W.long_list, true, tile[threadIdx.x >> 6], st);
+    }
+    coop_stats(W, st);
+}
+
+// chains longer than the multi-wavefront range, one wavefront each
+__global__ void __launch_bounds__(256) k_glint_coop_seq(DScene S, uint64_t seed, EstimatorWork W) {
+    __shared__ float2 tile[4][COOP_TILE + 1][COOP_TILE + 1];
+    unsigned long long st[5] = {0ull, 0ull, 0ull, 0ull, 0ull};
+    const unsigned long long n_items = *W.seq_list.count;
+    const unsigned long long n_waves = ((unsigned long long) gridDim.x * blockDim.x) >> 6;
+    for (unsigned long long i = ((unsigned long long) blockIdx.x * blockDim.x + threadIdx.x) >> 6; i < n_items; i += n_waves)
+        glint_coop_item(S, seed, W, W.seq_list.items[i], -1, W.seq_list, false, tile[threadIdx.x >> 6], st);
+    coop_stats(W, st);
+}
+
+// ---------------------------------------------------------------- multi-wavefront stage
+// The few items with long trial chains get several wavefronts each: a wavefront claims the
+// item's next 64-trial chunk (per-item counter), runs it and lowers the item's first match
+// (atomic min).  Chunks past the current first match are not claimed; chunks that ran past
+// the final one are discarded by the resolve pass, which sums the per-trial log only up to
+// the first match, so the counters equal the sequential trial loop's.
+__global__ void __launch_bounds__(256) k_glint_multiwave(DScene S, uint64_t seed, EstimatorWork W) {
+    __shared__ float2 tile_all[4][COOP_TILE + 1][COOP_TILE + 1];
+    const sms_config &cfg = S.cfg;
+    const int lane = threadIdx.x & 63;
+    float2 (*tile)[COOP_TILE + 1] = tile_all[threadIdx.x >> 6];
+    const unsigned long long n_long = *W.long_list.count;
+    const int n = (int) (n_long < (unsigned long long) W.mw_items ? n_long : (unsigned long long) W.mw_items);
+    if (n == 0) return;
+    const int wave = (int) ((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int n_waves = (int) ((gridDim.x * blockDim.x) >> 6);
+    int per_item = n_waves / n;
+    per_item = per_item < 1 ? 1 : (per_item > 64 ? 64 : per_item);
+    if (wave >= n * per_item) return;
+    const int64_t kcap = trial_cap(cfg);
+    int j = wave % n, exhausted = 0, staged_item = -1;
+    bool staged = false;
+    LdsTexels<COOP_TILE> lds;
+    GlintFootprint fp;
+    GlobalTexels tex;
+    uint64_t qkey = 0;
+    V2 uv_sol = v2(0.f, 0.f);
+    int64_t kstart = 0;
+    while (exhausted < n) {
+        if (staged_item != j) {
+            if (staged_item >= 0) window_done();
+            const OverflowRec ov = W.long_list.items[j];
+            const TrialRec tr = W.trials.items[ov.q];
+            footprint_texels(S, tr.f.shape, fp);
+            footprint_from_rec(tr.f, fp);
+            staged = stage_window(fp, S.shapes[tr.f.shape].tiles, tile, lane, lds);
+            tex.slope = fp.slope; tex.res = fp.res; tex.stride = fp.stride;
+            qkey = query_key(seed, (uint64_t) tr.f.q);
+            uv_sol = v2(ov.sol[0], ov.sol[1]);
+            kstart = ov.k;
+            staged_item = j;
+        }
+        unsigned long long c = 0;
+        if (lane == 0) c = atomicAdd(&W.mw_next[j], 1ull);
+        c = __shfl(c, 0, 64);
+        long long found_now = __hip_atomic_load(&W.mw_found[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int64_t lim = kstart + (int64_t) W.mw_trials - 1;
+        lim = lim < kcap ? lim : kcap;
+        lim = lim < (int64_t) found_now ? lim : (int64_t) found_now;
+        const int64_t k0 = kstart + 64 * (int64_t) c;
+        if (k0 > lim) {                          // nothing left to claim on this item
+            j = j + 1 == n ? 0 : j + 1;
+            exhausted++;
+            continue;
+        }
+        exhausted = 0;
+        const int64_t k = k0 + lane;
+        bool match = false;
+        if (k <= kcap && k < kstart + (int64_t) W.mw_trials) {
+            GlintWalk w;
+            int s = staged ? glint_trial(fp, cfg, qkey, k, w, lds) : glint_trial(fp, cfg, qkey, k, w, tex);
+            bool conv = s == WALK_CONVERGED;
+            match = conv && norm2(w.uv - uv_sol) < cfg.uniqueness_threshold;
+            W.mw_log[(size_t) j * (size_t) W.mw_trials + (size_t) (k - kstart)] =
+                (uint16_t) ((unsigned) w.it | (conv ? 0x8000u : 0u));
+        }
+        unsigned long long ballot = __ballot(match);
+        if (ballot && lane == 0) atomicMin(&W.mw_found[j], (long long) (k0 + (int64_t) __ffsll((long long) ballot) - 1));
+    }
+    if (staged_item >= 0) window_done();
+}
+
+// One wavefront per long item: sum the log up to the first match (or the end of the stage's
+// range), resolve the item or pass it to the sequential tail.
+__global__ void __launch_bounds__(256) k_glint_multiwave_resolve(DScene S, EstimatorWork W) {
+    const sms_config &cfg = S.cfg;
+    const int lane = threadIdx.x & 63;
+    const unsigned long long n_long = *W.long_list.count;
+    const int64_t kcap = trial_cap(cfg);
+    unsigned long long st_walks = 0, st_conv = 0, st_iters = 0, st_btrials = 0, st_bmax = 0;
+    const unsigned long long n_waves = (gridDim.x * blockDim.x) >> 6;
+    for (unsigned long long j = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; j < n_long; j += n_waves) {
+        const OverflowRec ov = W.long_list.items[j];
+        if (j >= (unsigned long long) W.mw_items) {
+            if (lane == 0) append_item(W.seq_list, ov);
+            continue;
+        }
+        const long long found = W.mw_found[j];
+        const int64_t kstart = ov.k;
+        int64_t last = kstart + (int64_t) W.mw_trials - 1;
+        last = last < kcap ? last : kcap;
+        const bool matched = found != 0x7fffffffffffffffLL;
+        const int64_t upto = matched ? (int64_t) found : last;
+        unsigned long long w_its = 0, w_conv = 0;
+        for (int64_t t = lane; t <= upto - kstart; t += 64) {
+            unsigned v = W.mw_log[(size_t) j * (size_t) W.mw_trials + (size_t) t];
+            w_its += v & 0x7fffu;
+            w_conv += v >> 15;
+        }
+        if (lane == 0) {
+            st_walks += (unsigned long long) (upto - kstart + 1);
+            if (matched || last == kcap) {
+                W.res[ov.q].inv_prob = matched ? (float) found : 0.f;
+                W.res[ov.q].status = 1;
+                unsigned long long trials = matched ? (unsigned long long) found : (unsigned long long) kcap + 1ull;
+                st_btrials += trials;
+                st_bmax = st_bmax > trials ? st_bmax : trials;
+            } else {
+                OverflowRec nx = ov;
+                nx.k = last + 1;
+                append_item(W.seq_list, nx);
+            }
+        }
+        st_iters += w_its;
+        st_conv += w_conv;
+    }
+    stat_add(W.stats + 0, st_walks); stat_add(W.stats + 1, st_conv); stat_add(W.stats + 2, st_iters);
+    stat_add(W.stats + 4, st_btrials); stat_max(W.stats + 5, st_bmax); stat_add(W.stats + 6, st_iters);
+}
+
+
+// ---------------------------------------------------------------- contributions
 __global__ void __launch_bounds__(256) k_glint_finalize(DScene S, const sms_glint_query *__restrict__ Q, uint64_t seed,
                                                         float *__restrict__ out, EstimatorWork W) {
-    const unsigned long long n_rec = *W.n_rec;
     const sms_config &cfg = S.cfg;
-    for (unsigned long long r = (unsigned long long) blockIdx.x * blockDim.x + threadIdx.x; r < n_rec;
-         r += (unsigned long long) gridDim.x * blockDim.x) {
-        const GlintRes res = W.res[r];
-        if (res.status == 0) continue;
-        const int64_t q = W.rec[r].q;
+    const unsigned long long u = (unsigned long long) blockIdx.x * blockDim.x + threadIdx.x;
+    const SpreadWalk sw = spread_walk(W.trials, u, (unsigned long long) gridDim.x * blockDim.x);
+    for (unsigned long long p = sw.pos; p < sw.count; p += sw.step) {
+        const unsigned long long slot = (unsigned long long) sw.sub * W.trials.sub_cap + p;
+        const GlintRes res = W.res[slot];
+        const TrialRec tr = W.trials.items[slot];
+        const int64_t q = tr.f.q;
         const sms_glint_query g = Q[q];
         SI si = glint_query_si(g);
         Pcg32 rq;
         stream_seed(rq, query_key(seed, (uint64_t) q), 0);
         EI ei = sample_emitter_interaction(S, si.p, S.emitters_all, S.n_emitters_all, rq);
         V3 sensor_p = ld3(g.sensor_p);
-        float val = glint_contribution(S, sensor_p, ei, si, v2(res.uv_sol[0], res.uv_sol[1]));
+        float val = glint_contribution(S, sensor_p, ei, si, v2(tr.uv_sol[0], tr.uv_sol[1]));
         float We = sensor_we(S.cam, sensor_p, si.p);
         float mis = cfg.bsdf_strategy_only ? 0.f : 1.f;
         write_rgb(out, q, glint_result(val, res.inv_prob, mis * We, ei.weight));

@@ -32,10 +32,18 @@ struct sms_ctx {
     unsigned long long *counters = nullptr;     // [0] cursor [1] n_overflow [2] coop cursor [8..15] stats
     OverflowRec *overflow = nullptr; size_t overflow_cap = 0;
     float *part = nullptr; float *invp = nullptr; size_t part_cap = 0;
-    GlintRec *rec = nullptr; GlintRes *res = nullptr; size_t rec_cap = 0;
-    int lane_trials = 256;
+    TrialRec *trial = nullptr; GlintRes *res = nullptr; size_t trial_cap = 0, sub_cap = 0;
+    OverflowRec *gover = nullptr; size_t gover_cap = 0, over_sub = 0;   // spread overflow list
+    int lane_trials = 8;
     int blocks_per_cu = 0;
-    int occ_glint = 0, occ_ss = 0, occ_coop = 0, occ_biased = 0, occ_sscoop = 0;
+    int trial_group = 8;
+    int tail_rounds = 2;
+    int long_chains = 1024, long_trials = 32768;
+    // glint tail lists and the multi-wavefront stage (allocated with the glint work buffers)
+    OverflowRec *long_items = nullptr; OverflowRec *seq_items = nullptr;
+    long long *mw_found = nullptr; unsigned long long *mw_next = nullptr; uint16_t *mw_log = nullptr;
+    int mw_items = 0, mw_trials = 0;
+    int occ_glint = 0, occ_ss = 0, occ_coop = 0, occ_biased = 0, occ_sscoop = 0, occ_mw = 0;   // occ_glint: trial pass
     int occ_ms[2] = {0, 0}, occ_mscoop[2] = {0, 0}, occ_msbiased[2] = {0, 0};   // [0]: chains <= 2, [1]: <= 4
 };
 
@@ -94,7 +102,7 @@ float i2f(int x) { float f; memcpy(&f, &x, 4); return f; }
 // ---------------------------------------------------------------- binned-SAH BVH (host build)
 struct BItem { float bmin[3], bmax[3], c[3]; int tri; };
 
-struct BvhOut { std::vector<DNode> nodes; std::vector<int> order; };
+struct BvhOut { std::vector<DNode> nodes; std::vector<int> order; int max_depth = 0; };
 
 void pad_box(float *bmin, float *bmax) {
     for (int a = 0; a < 3; ++a) {
@@ -109,12 +117,13 @@ BvhOut build_bvh(std::vector<BItem> &items) {
     int n = (int) items.size();
     if (n == 0) return out;
     const int BINS = 16, MAX_LEAF = 8;
-    struct Task { int node, lo, hi; };
+    struct Task { int node, lo, hi, depth; };
     std::vector<Task> stack;
     out.nodes.push_back(DNode());
-    stack.push_back({0, 0, n});
+    stack.push_back({0, 0, n, 0});
     while (!stack.empty()) {
         Task tk = stack.back(); stack.pop_back();
+        out.max_depth = std::max(out.max_depth, tk.depth);
         float bmin[3] = {INFINITY, INFINITY, INFINITY}, bmax[3] = {-INFINITY, -INFINITY, -INFINITY};
         float cmin[3] = {INFINITY, INFINITY, INFINITY}, cmax[3] = {-INFINITY, -INFINITY, -INFINITY};
         for (int i = tk.lo; i < tk.hi; ++i)
@@ -179,7 +188,7 @@ BvhOut build_bvh(std::vector<BItem> &items) {
                 out.nodes[tk.node] = nd;
                 int l = (int) out.nodes.size();
                 out.nodes.push_back(DNode()); out.nodes.push_back(DNode());
-                stack.push_back({l, tk.lo, mid}); stack.push_back({l + 1, mid, tk.hi});
+                stack.push_back({l, tk.lo, mid, tk.depth + 1}); stack.push_back({l + 1, mid, tk.hi, tk.depth + 1});
                 continue;
             }
             make_leaf();
@@ -196,7 +205,7 @@ BvhOut build_bvh(std::vector<BItem> &items) {
         nd.b.w = i2f(0);
         out.nodes[tk.node] = nd;
         out.nodes.push_back(DNode()); out.nodes.push_back(DNode());
-        stack.push_back({l, tk.lo, mid}); stack.push_back({l + 1, mid, tk.hi});
+        stack.push_back({l, tk.lo, mid, tk.depth + 1}); stack.push_back({l + 1, mid, tk.hi, tk.depth + 1});
     }
     out.order.resize(n);
     for (int i = 0; i < n; ++i) out.order[i] = items[i].tri;
@@ -295,6 +304,8 @@ sms_status build_scene(sms_ctx *ctx, const sms_scene_desc *desc, const sms_confi
         sh.eta_vtx = sh.k == 0.f ? hmean_same(sh.eta_r) : 1.f;
         for (int a = 0; a < 3; ++a) sh.refl[a] = d.reflectance[a];
         sh.normalmap = d.normalmap; sh.tiles = d.tiles;
+        if ((d.flags & SMS_FLAG_GLINTY) && d.bsdf != SMS_BSDF_NORMALMAP)
+            return fail(ctx, SMS_E_INVALID, "glinty shapes need the normal-mapped BSDF");
         if (d.bsdf == SMS_BSDF_NORMALMAP && (d.normalmap < 0 || d.normalmap >= desc->n_normalmaps))
             return fail(ctx, SMS_E_INVALID, "normal-mapped shape without a normal map");
         sh.bbox_center[0] = sh.cx; sh.bbox_center[1] = sh.cy; sh.bbox_center[2] = sh.cz;
@@ -366,6 +377,8 @@ sms_status build_scene(sms_ctx *ctx, const sms_scene_desc *desc, const sms_confi
         for (int a = 0; a < 3; ++a) it.c[a] = 0.5f * (it.bmin[a] + it.bmax[a]);
     }
     BvhOut bvh = build_bvh(items);
+    if (bvh.max_depth > SMS_BVH_STACK)
+        return fail(ctx, SMS_E_UNSUPPORTED, "BVH deeper than the traversal stack");
     std::vector<float4> tris(3 * bvh.order.size());
     std::vector<int> face_prim_base(desc->n_shapes, 0);
     for (int i = 0; i < desc->n_shapes; ++i) face_prim_base[i] = shapes[i].fbase;
@@ -429,7 +442,12 @@ __global__ void k_zero_u64(unsigned long long *p, int n) {
     if (i < n) p[i] = 0ull;
 }
 
-constexpr int N_COUNTERS = 16;
+// counters buffer: queue counters 0..31, then the spread stats copies, then the spread
+// counters of the glint trial and overflow lists
+constexpr int STATS_OFF = 32;
+constexpr int TRIALS_OFF = STATS_OFF + SPREAD * SPREAD_STRIDE;
+constexpr int OVER_OFF = TRIALS_OFF + SPREAD * SPREAD_STRIDE;
+constexpr int N_COUNTERS = OVER_OFF + SPREAD * SPREAD_STRIDE;
 
 sms_status ensure_work(sms_ctx *ctx, size_t n_overflow, size_t n_part) {
     if (!ctx->counters) HIP_TRY(ctx, hipMalloc((void **) &ctx->counters, N_COUNTERS * sizeof(unsigned long long)));
@@ -451,37 +469,111 @@ sms_status ensure_work(sms_ctx *ctx, size_t n_overflow, size_t n_part) {
     return SMS_OK;
 }
 
-sms_status ensure_glint_work(sms_ctx *ctx, size_t n) {
-    if (n > ctx->rec_cap) {
-        if (ctx->rec) HIP_TRY(ctx, hipFree(ctx->rec));
-        if (ctx->res) HIP_TRY(ctx, hipFree(ctx->res));
-        ctx->rec = nullptr; ctx->res = nullptr;
-        HIP_TRY(ctx, hipMalloc((void **) &ctx->rec, n * sizeof(GlintRec)));
-        HIP_TRY(ctx, hipMalloc((void **) &ctx->res, n * sizeof(GlintRes)));
-        ctx->rec_cap = n;
+// Spread overflow list with sub-lists of sub_cap items (kept for the call that asks).
+sms_status ensure_over(sms_ctx *ctx, size_t sub_cap) {
+    if (SPREAD * sub_cap > ctx->gover_cap) {
+        if (ctx->gover) HIP_TRY(ctx, hipFree(ctx->gover));
+        ctx->gover = nullptr;
+        HIP_TRY(ctx, hipMalloc((void **) &ctx->gover, SPREAD * sub_cap * sizeof(OverflowRec)));
+        ctx->gover_cap = SPREAD * sub_cap;
     }
+    ctx->over_sub = sub_cap;
     return SMS_OK;
+}
+
+// Per-sub-list capacity for a lane-per-query pass over n queries with nc items each, one
+// block per 256 queries, block b appending to sub-list b % SPREAD.
+size_t query_sub_cap(int64_t n, int nc) {
+    const size_t items = (size_t) std::max<int64_t>(n, 1) * (size_t) std::max(nc, 1);
+    return (items + SPREAD - 1) / SPREAD + 256 * (size_t) std::max(nc, 1);
+}
+
+// Multi-wavefront stage of the glint tail: long chains it holds and trials per chain.
+constexpr int MW_ITEMS = 1024;
+constexpr int MW_TRIALS = 32768;
+
+sms_status ensure_glint_work(sms_ctx *ctx, size_t n) {
+    if (n > ctx->trial_cap) {
+        void **bufs[4] = {(void **) &ctx->trial, (void **) &ctx->res, (void **) &ctx->long_items,
+                          (void **) &ctx->seq_items};
+        for (void **b : bufs) {
+            if (*b) HIP_TRY(ctx, hipFree(*b));
+            *b = nullptr;
+        }
+        // spread lists: SPREAD sub-lists; a sub-list takes the items of every SPREAD-th block
+        // (at most 256 each) or of every SPREAD-th trial slot
+        const size_t sub_cap = (n + SPREAD - 1) / SPREAD + 256;
+        HIP_TRY(ctx, hipMalloc((void **) &ctx->trial, SPREAD * sub_cap * sizeof(TrialRec)));
+        HIP_TRY(ctx, hipMalloc((void **) &ctx->res, SPREAD * sub_cap * sizeof(GlintRes)));
+        HIP_TRY(ctx, hipMalloc((void **) &ctx->long_items, n * sizeof(OverflowRec)));
+        HIP_TRY(ctx, hipMalloc((void **) &ctx->seq_items, n * sizeof(OverflowRec)));
+        ctx->trial_cap = n;
+        ctx->sub_cap = sub_cap;
+    }
+    {
+        sms_status st = ensure_over(ctx, ctx->sub_cap);
+        if (st) return st;
+    }
+    if (!ctx->mw_log) {
+        HIP_TRY(ctx, hipMalloc((void **) &ctx->mw_found, MW_ITEMS * sizeof(long long)));
+        HIP_TRY(ctx, hipMalloc((void **) &ctx->mw_next, MW_ITEMS * sizeof(unsigned long long)));
+        HIP_TRY(ctx, hipMalloc((void **) &ctx->mw_log, (size_t) MW_ITEMS * MW_TRIALS * sizeof(uint16_t)));
+    }
+    // the per-trial log stores iteration counts in 15 bits
+    ctx->mw_items = ctx->scene.cfg.max_iterations < 0x8000 ? ctx->long_chains : 0;
+    ctx->mw_trials = ctx->long_trials;
+    return SMS_OK;
+}
+
+sms_schedule schedule_of(const sms_ctx *ctx) {
+    sms_schedule s;
+    memset(&s, 0, sizeof(s));
+    s.lane_trials = ctx->lane_trials; s.blocks_per_cu = ctx->blocks_per_cu;
+    s.trial_group = ctx->trial_group; s.tail_rounds = ctx->tail_rounds;
+    s.long_chains = ctx->long_chains; s.long_trials = ctx->long_trials;
+    return s;
+}
+
+void apply_schedule(sms_ctx *ctx, const sms_schedule &s) {
+    ctx->lane_trials = s.lane_trials; ctx->blocks_per_cu = s.blocks_per_cu;
+    ctx->trial_group = s.trial_group; ctx->tail_rounds = s.tail_rounds;
+    ctx->long_chains = s.long_chains; ctx->long_trials = s.long_trials;
 }
 
 EstimatorWork work_of(sms_ctx *ctx) {
     EstimatorWork W;
-    W.cursor = ctx->counters + 0; W.n_overflow = ctx->counters + 1; W.coop_cursor = ctx->counters + 2;
-    W.n_rec = ctx->counters + 3;
-    W.stats = ctx->counters + 8; W.overflow = ctx->overflow; W.lane_trials = ctx->lane_trials;
-    W.rec = ctx->rec; W.res = ctx->res;
+    memset(&W, 0, sizeof(W));
+    unsigned long long *c = ctx->counters;
+    W.cursor = c + 0; W.n_overflow = c + 1; W.coop_cursor = c + 2;
+    W.stats = c + STATS_OFF; W.overflow = ctx->overflow; W.lane_trials = ctx->lane_trials;
+    W.res = ctx->res;
+    W.trials.items = ctx->trial; W.trials.counts = c + TRIALS_OFF; W.trials.sub_cap = ctx->sub_cap;
+    W.over.items = ctx->gover; W.over.counts = c + OVER_OFF; W.over.sub_cap = ctx->over_sub;
+    W.group = ctx->trial_group; W.tail_rounds = ctx->tail_rounds;
+    W.long_list.items = ctx->long_items; W.long_list.count = c + 4; W.long_list.cursor = c + 5;
+    W.seq_list.items = ctx->seq_items; W.seq_list.count = c + 6; W.seq_list.cursor = c + 7;
+    W.mw_found = ctx->mw_found; W.mw_next = ctx->mw_next; W.mw_log = ctx->mw_log;
+    W.mw_items = ctx->mw_items; W.mw_trials = ctx->mw_trials;
     return W;
 }
 
 sms_status read_stats(sms_ctx *ctx, sms_stats *stats) {
-    unsigned long long h[N_COUNTERS];
-    HIP_TRY(ctx, hipMemcpyAsync(h, ctx->counters, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<unsigned long long> h(N_COUNTERS);
+    HIP_TRY(ctx, hipMemcpyAsync(h.data(), ctx->counters, N_COUNTERS * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     if (stats) {
+        unsigned long long v[8] = {0, 0, 0, 0, 0, 0, 0, 0}, over = 0;
+        for (int s = 0; s < SPREAD; ++s) {
+            const unsigned long long *cp = &h[STATS_OFF + s * SPREAD_STRIDE];
+            for (int j = 0; j < 8; ++j) v[j] = j == 5 ? std::max(v[j], cp[j]) : v[j] + cp[j];
+            over += h[OVER_OFF + s * SPREAD_STRIDE];
+        }
         memset(stats, 0, sizeof(*stats));
-        stats->walks = h[8]; stats->walks_converged = h[9]; stats->newton_iterations = h[10];
-        stats->bernoulli_calls = h[11]; stats->bernoulli_trials = h[12]; stats->bernoulli_max = h[13];
-        stats->overflow_queries = h[1];
-        stats->tail_iterations = h[14];
+        stats->walks = v[0]; stats->walks_converged = v[1]; stats->newton_iterations = v[2];
+        stats->bernoulli_calls = v[3]; stats->bernoulli_trials = v[4]; stats->bernoulli_max = v[5];
+        stats->overflow_queries = h[1] + over;
+        stats->tail_iterations = v[6];
     }
     return SMS_OK;
 }
@@ -536,8 +628,9 @@ sms_status sms_ctx_create(const sms_scene_desc *scene, const sms_config *cfg, sm
         return SMS_E_HIP;
     }
     st = build_scene(ctx, scene, cfg);
-    if (!st) st = occupancy(ctx, k_glint_walks, &ctx->occ_glint);
-    if (!st) st = occupancy(ctx, k_glint_coop, &ctx->occ_coop);
+    if (!st) st = occupancy(ctx, k_glint_trials_g8, &ctx->occ_glint);
+    if (!st) st = occupancy(ctx, k_glint_coop_rounds, &ctx->occ_coop);
+    if (!st) st = occupancy(ctx, k_glint_multiwave, &ctx->occ_mw);
     if (!st) st = occupancy(ctx, k_ss_estimate, &ctx->occ_ss);
     if (!st) st = occupancy(ctx, k_ss_coop, &ctx->occ_sscoop);
     if (!st) st = occupancy(ctx, k_ss_biased, &ctx->occ_biased);
@@ -565,7 +658,13 @@ sms_status sms_ctx_destroy(sms_ctx *ctx) {
     if (ctx->overflow) (void) hipFree(ctx->overflow);
     if (ctx->part) (void) hipFree(ctx->part);
     if (ctx->invp) (void) hipFree(ctx->invp);
-    if (ctx->rec) (void) hipFree(ctx->rec);
+    if (ctx->trial) (void) hipFree(ctx->trial);
+    if (ctx->long_items) (void) hipFree(ctx->long_items);
+    if (ctx->seq_items) (void) hipFree(ctx->seq_items);
+    if (ctx->mw_found) (void) hipFree(ctx->mw_found);
+    if (ctx->mw_next) (void) hipFree(ctx->mw_next);
+    if (ctx->mw_log) (void) hipFree(ctx->mw_log);
+    if (ctx->gover) (void) hipFree(ctx->gover);
     if (ctx->res) (void) hipFree(ctx->res);
     if (ctx->ev3) (void) hipEventDestroy(ctx->ev3);
     if (ctx->ev0) (void) hipEventDestroy(ctx->ev0);
@@ -588,17 +687,21 @@ const char *sms_ctx_last_error(const sms_ctx *ctx) { return ctx ? ctx->err.c_str
 
 sms_status sms_ctx_set_schedule(sms_ctx *ctx, const sms_schedule *s) {
     if (!ctx || !s) return SMS_E_INVALID;
-    if (s->lane_trials < 1 || s->blocks_per_cu < 0) return fail(ctx, SMS_E_INVALID, "bad schedule");
-    ctx->lane_trials = s->lane_trials;
-    ctx->blocks_per_cu = s->blocks_per_cu;
+    const int g = s->trial_group;
+    if (s->lane_trials < 1 || s->blocks_per_cu < 0 || s->tail_rounds < 0 ||
+        !(g == 0 || g == 1 || g == 2 || g == 4 || g == 8 || g == 16) ||
+        s->long_chains < 0 || s->long_chains > MW_ITEMS || s->long_trials < 64 || s->long_trials > MW_TRIALS ||
+        s->long_trials % 64 != 0)
+        return fail(ctx, SMS_E_INVALID, "bad schedule");
+    sms_schedule v = *s;
+    if (v.trial_group == 0) v.trial_group = 8;
+    apply_schedule(ctx, v);
     return SMS_OK;
 }
 
 sms_status sms_ctx_get_schedule(const sms_ctx *ctx, sms_schedule *s) {
     if (!ctx || !s) return SMS_E_INVALID;
-    memset(s, 0, sizeof(*s));
-    s->lane_trials = ctx->lane_trials;
-    s->blocks_per_cu = ctx->blocks_per_cu;
+    *s = schedule_of(ctx);
     return SMS_OK;
 }
 
@@ -651,44 +754,102 @@ sms_status sms_glint_make_queries(sms_ctx *ctx, int32_t spp, uint64_t seed, sms_
     return SMS_OK;
 }
 
-sms_status sms_glint_estimate_batch(sms_ctx *ctx, const sms_glint_query *q, int64_t n, uint64_t seed, float *out_rgb,
-                                    sms_stats *stats) {
+} // extern "C"
+
+namespace {
+
+sms_status glint_prepare(sms_ctx *ctx, const sms_glint_query *q, int64_t n, const float *out_rgb) {
     if (!ctx || n < 0 || (n > 0 && (!q || !out_rgb))) return fail(ctx, SMS_E_INVALID, "bad estimator arguments");
     if (ctx->scene.cfg.biased) return fail(ctx, SMS_E_UNSUPPORTED, "biased glint estimator is not built");
     if (!ctx->scene.nmaps) return fail(ctx, SMS_E_INVALID, "scene has no normal map");
     sms_status st = ensure_work(ctx, (size_t) std::max<int64_t>(n, 1), 0);
     if (!st) st = ensure_glint_work(ctx, (size_t) std::max<int64_t>(n, 1));
-    if (st) return st;
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream, ctx->counters, N_COUNTERS);
+    return st;
+}
+
+// Enqueue the glint pass sequence on the context stream (no host synchronisation); events
+// ev0 / ev1 / ev2 bracket the first-walk pass and the rest.
+sms_status glint_enqueue(sms_ctx *ctx, const sms_glint_query *q, int64_t n, uint64_t seed, float *out_rgb) {
+    hipLaunchKernelGGL(k_zero_u64, dim3((N_COUNTERS + 255) / 256), dim3(256), 0, ctx->stream, ctx->counters, N_COUNTERS);
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     if (n > 0) {
         EstimatorWork W = work_of(ctx);
-        hipLaunchKernelGGL(k_glint_setup, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream,
-                           ctx->scene, q, n, seed, out_rgb, W);
-        HIP_TRY(ctx, hipGetLastError());
-        HIP_TRY(ctx, hipEventRecord(ctx->ev3, ctx->stream));
-        hipLaunchKernelGGL(k_glint_walks, dim3(grid_for(ctx, ctx->occ_glint)), dim3(256), 0, ctx->stream,
-                           ctx->scene, seed, W);
+        const unsigned blocks = (unsigned) ((n + 255) / 256);
+        if (ctx->scene.n_nodes > 0)
+            hipLaunchKernelGGL(k_glint_first_bvh, dim3(blocks), dim3(256), 0, ctx->stream, ctx->scene, q, n, seed, out_rgb, W);
+        else
+            hipLaunchKernelGGL(k_glint_first_nobvh, dim3(blocks), dim3(256), 0, ctx->stream, ctx->scene, q, n, seed, out_rgb, W);
         HIP_TRY(ctx, hipGetLastError());
         HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-        hipLaunchKernelGGL(k_glint_coop, dim3(grid_for(ctx, ctx->occ_coop)), dim3(256), 0, ctx->stream,
-                           ctx->scene, seed, W);
+        const dim3 gt(grid_for(ctx, ctx->occ_glint)), gc(grid_for(ctx, ctx->occ_coop)), gm(grid_for(ctx, ctx->occ_mw));
+        switch (ctx->trial_group) {
+        case 1: hipLaunchKernelGGL(k_glint_trials_g1, gt, dim3(256), 0, ctx->stream, ctx->scene, seed, W); break;
+        case 2: hipLaunchKernelGGL(k_glint_trials_g2, gt, dim3(256), 0, ctx->stream, ctx->scene, seed, W); break;
+        case 4: hipLaunchKernelGGL(k_glint_trials_g4, gt, dim3(256), 0, ctx->stream, ctx->scene, seed, W); break;
+        case 16: hipLaunchKernelGGL(k_glint_trials_g16, gt, dim3(256), 0, ctx->stream, ctx->scene, seed, W); break;
+        default: hipLaunchKernelGGL(k_glint_trials_g8, gt, dim3(256), 0, ctx->stream, ctx->scene, seed, W); break;
+        }
+        HIP_TRY(ctx, hipGetLastError());
+        hipLaunchKernelGGL(k_glint_coop_rounds, gc, dim3(256), 0, ctx->stream, ctx->scene, seed, W);
+        HIP_TRY(ctx, hipGetLastError());
+        hipLaunchKernelGGL(k_glint_multiwave, gm, dim3(256), 0, ctx->stream, ctx->scene, seed, W);
+        HIP_TRY(ctx, hipGetLastError());
+        hipLaunchKernelGGL(k_glint_multiwave_resolve, dim3(ctx->n_cu), dim3(256), 0, ctx->stream, ctx->scene, W);
+        HIP_TRY(ctx, hipGetLastError());
+        hipLaunchKernelGGL(k_glint_coop_seq, gc, dim3(256), 0, ctx->stream, ctx->scene, seed, W);
         HIP_TRY(ctx, hipGetLastError());
         hipLaunchKernelGGL(k_glint_finalize, dim3(4 * ctx->n_cu), dim3(256), 0, ctx->stream, ctx->scene, q, seed,
                            out_rgb, W);
         HIP_TRY(ctx, hipGetLastError());
     } else {
-        HIP_TRY(ctx, hipEventRecord(ctx->ev3, ctx->stream));
         HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     }
     HIP_TRY(ctx, hipEventRecord(ctx->ev2, ctx->stream));
-    st = read_stats(ctx, stats);
+    return SMS_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+sms_status sms_glint_estimate_batch(sms_ctx *ctx, const sms_glint_query *q, int64_t n, uint64_t seed, float *out_rgb,
+                                    sms_stats *stats) {
+    sms_status st = glint_prepare(ctx, q, n, out_rgb);
+    if (!st) st = glint_enqueue(ctx, q, n, seed, out_rgb);
+    if (!st) st = read_stats(ctx, stats);
     if (st) return st;
-    float setup_ms = 0.f;
-    HIP_TRY(ctx, hipEventElapsedTime(&setup_ms, ctx->ev0, ctx->ev3));
-    HIP_TRY(ctx, hipEventElapsedTime(&ctx->last_ms, ctx->ev3, ctx->ev1));
+    HIP_TRY(ctx, hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1));
     HIP_TRY(ctx, hipEventElapsedTime(&ctx->last_ms2, ctx->ev1, ctx->ev2));
-    ctx->last_ms2 += setup_ms;
+    return SMS_OK;
+}
+
+sms_status sms_glint_autotune(sms_ctx *ctx, const sms_glint_query *q, int64_t n, uint64_t seed, float *out_rgb,
+                              sms_schedule *best) {
+    if (!best) return fail(ctx, SMS_E_INVALID, "null schedule");
+    sms_status st = glint_prepare(ctx, q, n, out_rgb);
+    if (st) return st;
+    const sms_schedule keep = schedule_of(ctx);
+    sms_schedule win = keep;
+    float win_ms = INFINITY;
+    const int groups[4] = {2, 4, 8, 16}, mults[2] = {1, 2}, rounds[2] = {1, 2}, bpcs[2] = {0, 2 * ctx->occ_glint};
+    for (int g : groups) for (int m : mults) for (int r : rounds) for (int b : bpcs) {
+        ctx->trial_group = g; ctx->lane_trials = g * m; ctx->tail_rounds = r; ctx->blocks_per_cu = b;
+        float ms = 0.f;
+        for (int rep = 0; rep < 2; ++rep) {           // the first run warms caches and allocations
+            st = glint_enqueue(ctx, q, n, seed + (uint64_t) rep, out_rgb);
+            if (st) break;
+            HIP_TRY(ctx, hipEventSynchronize(ctx->ev2));
+            HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev2));
+        }
+        if (st) break;
+        if (ms < win_ms) {
+            win_ms = ms;
+            win = schedule_of(ctx);
+        }
+    }
+    apply_schedule(ctx, st ? keep : win);
+    if (st) return st;
+    *best = win;
     return SMS_OK;
 }
 
@@ -698,10 +859,10 @@ sms_status sms_ss_estimate_batch(sms_ctx *ctx, const sms_ss_query *q, int64_t n,
     const sms_config &cfg = ctx->scene.cfg;
     int nc = ctx->scene.n_casters_ss;
     if (cfg.biased && cfg.max_trials > 128) return fail(ctx, SMS_E_UNSUPPORTED, "biased trial sets above 128 are not built");
-    sms_status st = ensure_work(ctx, cfg.biased ? 0 : (size_t) std::max<int64_t>(n * nc, 1),
-                                cfg.biased ? 0 : (size_t) std::max<int64_t>(n * nc, 1));
+    sms_status st = cfg.biased ? SMS_OK : ensure_over(ctx, query_sub_cap(n, nc));
+    if (!st) st = ensure_work(ctx, 0, cfg.biased ? 0 : (size_t) std::max<int64_t>(n * nc, 1));
     if (st) return st;
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream, ctx->counters, N_COUNTERS);
+    hipLaunchKernelGGL(k_zero_u64, dim3((N_COUNTERS + 255) / 256), dim3(256), 0, ctx->stream, ctx->counters, N_COUNTERS);
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     if (n > 0) {
         EstimatorWork W = work_of(ctx);
@@ -711,7 +872,7 @@ sms_status sms_ss_estimate_batch(sms_ctx *ctx, const sms_ss_query *q, int64_t n,
             HIP_TRY(ctx, hipGetLastError());
             HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
         } else {
-            hipLaunchKernelGGL(k_ss_estimate, dim3(grid_for(ctx, ctx->occ_ss)), dim3(256), 0, ctx->stream,
+            hipLaunchKernelGGL(k_ss_estimate, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream,
                                ctx->scene, q, n, seed, ctx->part, ctx->invp, W);
             HIP_TRY(ctx, hipGetLastError());
             HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
@@ -756,10 +917,10 @@ sms_status sms_ms_estimate_batch(sms_ctx *ctx, const sms_ss_query *q, int64_t n,
     const int nc = ctx->scene.n_casters_ms;
     const int b = cfg.bounces <= 2 ? 0 : 1;
     if (cfg.biased && cfg.max_trials > 128) return fail(ctx, SMS_E_UNSUPPORTED, "biased trial sets above 128 are not built");
-    st = ensure_work(ctx, cfg.biased ? 0 : (size_t) std::max<int64_t>(n * nc, 1),
-                     cfg.biased ? 0 : (size_t) std::max<int64_t>(n * nc, 1));
+    st = cfg.biased ? SMS_OK : ensure_over(ctx, query_sub_cap(n, nc));
+    if (!st) st = ensure_work(ctx, 0, cfg.biased ? 0 : (size_t) std::max<int64_t>(n * nc, 1));
     if (st) return st;
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream, ctx->counters, N_COUNTERS);
+    hipLaunchKernelGGL(k_zero_u64, dim3((N_COUNTERS + 255) / 256), dim3(256), 0, ctx->stream, ctx->counters, N_COUNTERS);
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     if (n > 0) {
         EstimatorWork W = work_of(ctx);
@@ -770,7 +931,7 @@ sms_status sms_ms_estimate_batch(sms_ctx *ctx, const sms_ss_query *q, int64_t n,
             HIP_TRY(ctx, hipGetLastError());
             HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
         } else {
-            dim3 g1(grid_for(ctx, ctx->occ_ms[b])), g2(grid_for(ctx, ctx->occ_mscoop[b]));
+            dim3 g1((unsigned) ((n + 255) / 256)), g2(grid_for(ctx, ctx->occ_mscoop[b]));
             if (b == 0) hipLaunchKernelGGL(k_ms_estimate_2, g1, dim3(256), 0, ctx->stream, ctx->scene, q, n, seed, ctx->part, ctx->invp, W);
             else hipLaunchKernelGGL(k_ms_estimate_4, g1, dim3(256), 0, ctx->stream, ctx->scene, q, n, seed, ctx->part, ctx->invp, W);
             HIP_TRY(ctx, hipGetLastError());

@@ -81,9 +81,10 @@ SMS_DEV void ms_estimate_body(const DScene &S, const sms_ss_query *__restrict__ 
     const sms_config &cfg = S.cfg;
     unsigned long long st_walks = 0, st_conv = 0, st_iters = 0, st_bcalls = 0, st_btrials = 0, st_bmax = 0;
     const int nc = S.n_casters_ms;
-    while (true) {
-        int64_t q = (int64_t) atomicAdd(W.cursor, 1ull);
-        if (q >= n) break;
+    // one lane per query; the launch covers n with one block per 256 queries, so a block's
+    // overflow items fit its sub-list of the spread overflow list
+    const int sub = (int) (blockIdx.x % SPREAD);
+    for (int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t) gridDim.x * blockDim.x) {
         const sms_ss_query g = Q[q];
         SI si = ms_query_si(g);
         V3 wi_local = ld3(g.wi);
@@ -127,18 +128,18 @@ SMS_DEV void ms_estimate_body(const DScene &S, const sms_ss_query *__restrict__ 
                 st_btrials += tr; st_bmax = st_bmax > tr ? st_bmax : tr;
                 invp[slot] = inv_prob;
             } else {
-                unsigned long long o = atomicAdd(W.n_overflow, 1ull);
+                unsigned long long o = spread_append(W.over, sub, 1ull);
                 OverflowRec rec;
                 rec.q = slot; rec.k = k;
                 rec.sol[0] = direction.x; rec.sol[1] = direction.y; rec.sol[2] = direction.z;
                 for (int b = 0; b < 4; ++b) rec.shapes[b] = b < MAXB ? w0.seed_shapes[b < MAXB ? b : 0] : -1;
                 rec.pad = 0;
-                W.overflow[o] = rec;
+                W.over.items[(size_t) sub * W.over.sub_cap + o] = rec;
             }
         }
     }
-    wave_add_u64(W.stats + 0, st_walks); wave_add_u64(W.stats + 1, st_conv); wave_add_u64(W.stats + 2, st_iters);
-    wave_add_u64(W.stats + 3, st_bcalls); wave_add_u64(W.stats + 4, st_btrials); wave_max_u64(W.stats + 5, st_bmax);
+    stat_add(W.stats + 0, st_walks); stat_add(W.stats + 1, st_conv); stat_add(W.stats + 2, st_iters);
+    stat_add(W.stats + 3, st_bcalls); stat_add(W.stats + 4, st_btrials); stat_max(W.stats + 5, st_bmax);
 }
 
 template <int MAXB>
@@ -148,13 +149,10 @@ SMS_DEV void ms_coop_body(const DScene &S, const sms_ss_query *__restrict__ Q, u
     const int lane = threadIdx.x & 63;
     const int nc = S.n_casters_ms;
     unsigned long long st_walks = 0, st_conv = 0, st_iters = 0, st_btrials = 0, st_bmax = 0, st_tail = 0;
-    const unsigned long long n_over = *W.n_overflow;
-    while (true) {
-        unsigned long long item = 0;
-        if (lane == 0) item = atomicAdd(W.coop_cursor, 1ull);
-        item = __shfl(item, 0, 64);
-        if (item >= n_over) break;
-        const OverflowRec rec = W.overflow[item];
+    const unsigned long long wave = ((unsigned long long) blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const SpreadWalk sw = spread_walk(W.over, wave, ((unsigned long long) gridDim.x * blockDim.x) >> 6);
+    for (unsigned long long p = sw.pos; p < sw.count; p += sw.step) {
+        const OverflowRec rec = W.over.items[(size_t) sw.sub * W.over.sub_cap + p];
         int64_t q = rec.q / nc;
         int c = (int) (rec.q % nc);
         const sms_ss_query g = Q[q];
@@ -194,8 +192,8 @@ SMS_DEV void ms_coop_body(const DScene &S, const sms_ss_query *__restrict__ Q, u
             st_btrials += tr; st_bmax = st_bmax > tr ? st_bmax : tr;
         }
     }
-    wave_add_u64(W.stats + 0, st_walks); wave_add_u64(W.stats + 1, st_conv); wave_add_u64(W.stats + 2, st_iters);
-    wave_add_u64(W.stats + 4, st_btrials); wave_max_u64(W.stats + 5, st_bmax); wave_add_u64(W.stats + 6, st_tail);
+    stat_add(W.stats + 0, st_walks); stat_add(W.stats + 1, st_conv); stat_add(W.stats + 2, st_iters);
+    stat_add(W.stats + 4, st_btrials); stat_max(W.stats + 5, st_bmax); stat_add(W.stats + 6, st_tail);
 }
 
 // ================================================================ biased estimator
@@ -208,12 +206,8 @@ SMS_DEV void ms_biased_body(const DScene &S, const sms_ss_query *__restrict__ Q,
     const int lane = threadIdx.x & 63;
     const int nc = S.n_casters_ms;
     unsigned long long st_walks = 0, st_conv = 0, st_iters = 0;
-    while (true) {
-        unsigned long long qq = 0;
-        if (lane == 0) qq = atomicAdd(W.cursor, 1ull);
-        qq = __shfl(qq, 0, 64);
-        if ((int64_t) qq >= n) break;
-        int64_t q = (int64_t) qq;
+    const int64_t n_waves = ((int64_t) gridDim.x * blockDim.x) >> 6;
+    for (int64_t q = ((int64_t) blockIdx.x * blockDim.x + threadIdx.x) >> 6; q < n; q += n_waves) {
         const sms_ss_query g = Q[q];
         SI si = ms_query_si(g);
         V3 wi_local = ld3(g.wi);
@@ -311,7 +305,7 @@ SMS_DEV void ms_biased_body(const DScene &S, const sms_ss_query *__restrict__ Q,
         }
         if (lane == 0) write_rgb(out, q, result);
     }
-    wave_add_u64(W.stats + 0, st_walks); wave_add_u64(W.stats + 1, st_conv); wave_add_u64(W.stats + 2, st_iters);
+    stat_add(W.stats + 0, st_walks); stat_add(W.stats + 1, st_conv); stat_add(W.stats + 2, st_iters);
 }
 
 #define SMS_MS_KERNELS(B)                                                                                           \

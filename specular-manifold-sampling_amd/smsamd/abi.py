@@ -82,7 +82,21 @@ class SmsStats(C.Structure):
 
 
 class SmsSchedule(C.Structure):
-    _fields_ = [("lane_trials", C.c_int32), ("blocks_per_cu", C.c_int32), ("pad", C.c_int32 * 2)]
+    _fields_ = [("lane_trials", C.c_int32), ("blocks_per_cu", C.c_int32), ("trial_group", C.c_int32),
+                ("tail_rounds", C.c_int32), ("long_chains", C.c_int32), ("long_trials", C.c_int32),
+                ("pad", C.c_int32 * 2)]
+
+    KEYS = ("lane_trials", "blocks_per_cu", "trial_group", "tail_rounds", "long_chains", "long_trials")
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k in self.KEYS}
+
+    @classmethod
+    def from_dict(cls, d):
+        s = cls()
+        for k in cls.KEYS:
+            setattr(s, k, int(d[k]))
+        return s
 
 
 F3 = (np.float32, 3)
@@ -134,7 +148,7 @@ EXPORTS = ["sms_config_default", "sms_ctx_create", "sms_ctx_destroy", "sms_ctx_s
            "sms_glint_walk_batch_host", "sms_ss_estimate_batch_host",
            "sms_glint_estimate_batch_host", "sms_device_alloc", "sms_device_free",
            "sms_copy_h2d", "sms_copy_d2h", "sms_ms_walk_batch", "sms_ms_estimate_batch",
-           "sms_ms_walk_batch_host", "sms_ms_estimate_batch_host"]
+           "sms_ms_walk_batch_host", "sms_ms_estimate_batch_host", "sms_glint_autotune"]
 
 
 def load_library(path=LIB_PATH):
@@ -163,6 +177,7 @@ def load_library(path=LIB_PATH):
                  "sms_ms_estimate_batch", "sms_ms_estimate_batch_host"):
         getattr(lib, name).argtypes = [vp, vp, i64, C.c_uint64, vp, C.POINTER(SmsStats)]
     lib.sms_glint_make_queries.argtypes = [vp, C.c_int32, C.c_uint64, vp]
+    lib.sms_glint_autotune.argtypes = [vp, vp, i64, C.c_uint64, vp, C.POINTER(SmsSchedule)]
     lib.sms_ctx_last_kernel_ms.argtypes = [vp, vp]
     lib.sms_device_alloc.argtypes = [vp, C.c_size_t, C.POINTER(vp)]
     lib.sms_device_free.argtypes = [vp, vp]

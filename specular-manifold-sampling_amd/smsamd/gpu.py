@@ -82,14 +82,21 @@ class Context:
         self.cfg = cfg
         _check(lib().sms_ctx_set_config(self.h, C.byref(cfg)), self.h, "set_config")
 
-    def set_schedule(self, lane_trials=256, blocks_per_cu=0):
-        s = abi.SmsSchedule(lane_trials, blocks_per_cu)
+    def set_schedule(self, **kw):
+        """Change some of the sms_schedule fields (abi.SmsSchedule.KEYS); the others keep their
+        current values."""
+        cur = self.get_schedule()
+        unknown = set(kw) - set(cur)
+        if unknown:
+            raise ValueError(f"unknown schedule keys {sorted(unknown)}")
+        cur.update(kw)
+        s = abi.SmsSchedule.from_dict(cur)
         _check(lib().sms_ctx_set_schedule(self.h, C.byref(s)), self.h, "set_schedule")
 
     def get_schedule(self):
         s = abi.SmsSchedule()
         _check(lib().sms_ctx_get_schedule(self.h, C.byref(s)), self.h, "get_schedule")
-        return {"lane_trials": s.lane_trials, "blocks_per_cu": s.blocks_per_cu}
+        return s.as_dict()
 
     def device_info(self):
         name = C.create_string_buffer(256)
@@ -169,6 +176,12 @@ class Context:
         _check(lib().sms_glint_estimate_batch(self.h, qbuf.ptr, n, seed, outbuf.ptr, C.byref(st)), self.h,
                "glint_estimate")
         return st.as_dict()
+
+    def glint_autotune_device(self, qbuf, n, seed, outbuf):
+        """Pick the fastest glint schedule on these queries (sms_glint_autotune); returns it."""
+        s = abi.SmsSchedule()
+        _check(lib().sms_glint_autotune(self.h, qbuf.ptr, n, seed, outbuf.ptr, C.byref(s)), self.h, "glint_autotune")
+        return s.as_dict()
 
     def ss_estimate_device(self, qbuf, n, seed, outbuf):
         st = abi.SmsStats()

@@ -79,3 +79,30 @@ def test_struct_layouts_match_c():
             assert dt.itemsize == abi.C_SIZES[n], n
             for f in dt.names:
                 assert int(got[f"{n}.{f}"]) == dt.fields[f][1], (n, f)
+
+
+CTYPES_STRUCTS = {"sms_config": abi.SmsConfig, "sms_stats": abi.SmsStats, "sms_schedule": abi.SmsSchedule,
+                  "sms_shape_desc": abi.SmsShapeDesc, "sms_emitter_desc": abi.SmsEmitterDesc,
+                  "sms_normalmap_desc": abi.SmsNormalmapDesc, "sms_camera_desc": abi.SmsCameraDesc,
+                  "sms_scene_desc": abi.SmsSceneDesc}
+
+
+def test_ctypes_structs_match_c():
+    """Every ctypes mirror has the C struct's size and field offsets."""
+    src = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
+    for n, cls in CTYPES_STRUCTS.items():
+        src.append(f'printf("{n} %zu\\n", sizeof({n}));')
+        for f, _ in cls._fields_:
+            src.append(f'printf("{n}.{f} %zu\\n", offsetof({n}, {f}));')
+    src.append("return 0;}")
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "probe.c")
+        exe = os.path.join(d, "probe")
+        open(c, "w").write("\n".join(src))
+        subprocess.run(["gcc", "-std=c99", "-o", exe, c], check=True)
+        out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout
+    got = dict(line.split() for line in out.strip().splitlines())
+    for n, cls in CTYPES_STRUCTS.items():
+        assert int(got[n]) == C.sizeof(cls), n
+        for f, _ in cls._fields_:
+            assert int(got[f"{n}.{f}"]) == getattr(cls, f).offset, (n, f)

@@ -100,20 +100,40 @@ def test_glint_queries_and_estimate(gpu, oracle_mod):
     assert np.array_equal(rg, ro)
     for k in ("walks", "walks_converged", "newton_iterations", "bernoulli_calls", "bernoulli_trials", "bernoulli_max"):
         assert sg[k] == so[k], k
-    # force the one-wavefront-per-query tail pass for every query with a solution
-    ctx.set_schedule(lane_trials=1)
-    rg2, sg2 = ctx.glint_estimate(qo, 9)
-    assert sg2["overflow_queries"] > 0
-    assert np.array_equal(rg2, ro)
-    assert sg2["bernoulli_trials"] == so["bernoulli_trials"]
+    # Every schedule splits the trial chains differently between the grouped lane pass, the
+    # one-wavefront rounds, the multi-wavefront stage and the sequential tail; the result and
+    # the counters must not change.  tail_rounds = 0 sends every chain left after the lane pass
+    # to the multi-wavefront stage; long_chains / long_trials small force chains on to the
+    # sequential tail.
+    assert so["bernoulli_max"] > 8
+    schedules = [dict(trial_group=1, lane_trials=1, tail_rounds=0),
+                 dict(trial_group=1, lane_trials=1, tail_rounds=0, long_chains=8, long_trials=64),
+                 dict(trial_group=2, lane_trials=2, tail_rounds=0, long_chains=1024, long_trials=128),
+                 dict(trial_group=2, lane_trials=2, tail_rounds=1, long_chains=1024, long_trials=32768),
+                 dict(trial_group=4, lane_trials=8, tail_rounds=2),
+                 dict(trial_group=8, lane_trials=8, tail_rounds=1),
+                 dict(trial_group=16, lane_trials=16, tail_rounds=3),
+                 dict(trial_group=8, lane_trials=8, tail_rounds=2, blocks_per_cu=1)]
+    for sch in schedules:
+        ctx.set_schedule(**sch)
+        rg2, sg2 = ctx.glint_estimate(qo, 9)
+        print("schedule", sch, sg2, flush=True)
+        assert np.array_equal(rg2, ro), sch
+        for k in ("walks", "walks_converged", "newton_iterations", "bernoulli_trials", "bernoulli_max"):
+            assert sg2[k] == so[k], (k, sch)
+        if sch["lane_trials"] < 8:
+            assert sg2["overflow_queries"] > 0
     # trial cap: inv_prob = 0 past max_trials (manifold_glints.cpp:96-103)
-    cfg2 = abi.SmsConfig.default(max_trials=3)
-    ctx.set_config(cfg2)
-    orc.set_config(cfg2)
-    ctx.set_schedule(lane_trials=2)
-    rg3, _ = ctx.glint_estimate(qo, 9)
-    ro3, _ = orc.glint_estimate(qo, 9)
-    assert np.array_equal(rg3, ro3)
+    for cap, group, lane_trials in [(3, 2, 2), (100, 1, 1), (100, 4, 4)]:
+        cfg2 = abi.SmsConfig.default(max_trials=cap)
+        ctx.set_config(cfg2)
+        orc.set_config(cfg2)
+        ctx.set_schedule(trial_group=group, lane_trials=lane_trials, tail_rounds=0, long_chains=4, long_trials=64)
+        rg3, sg3 = ctx.glint_estimate(qo, 9)
+        ro3, so3 = orc.glint_estimate(qo, 9)
+        assert np.array_equal(rg3, ro3), cap
+        for k in ("walks", "newton_iterations", "bernoulli_trials", "bernoulli_max"):
+            assert sg3[k] == so3[k], (k, cap)
     ctx.close()
     orc.close()
 
