@@ -25,7 +25,12 @@ sys.path.insert(0, os.path.join(ROOT, "specular-manifold-sampling_amd"))
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 FP32_PEAK_TFLOPS = 157.3       # MI355X FP32 vector
-BYTES_PER_ITER = 32            # 4 texels x (2 x fp32 slope) per Newton iteration
+BYTES_PER_ITER = 12            # algorithmic: 4 RGB8 texels per Newton iteration (the map's source format)
+# HBM bytes per launch of k_glint_first on this workload (default sizes and schedule), from
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs scaled by a plain 256 MiB copy's counters
+# (FETCH_SIZE reads half the copied bytes: factor 2; WRITE_SIZE factor 1); see
+# profiles/glint_first_512x512x4spp_pmc.md.
+TRAFFIC_FIRST_PASS = {"fetch_kib": 115055.6, "write_kib": 13509.6, "fetch_factor": 2.0, "write_factor": 1.0}
 FLOPS_PER_ITER = 120           # FP32 ops per glint Newton iteration (fetch, solve, footprint test)
 
 
@@ -45,11 +50,29 @@ def parse():
     ap.add_argument("--trial-group", type=int, default=None, help="override the tuned schedule")
     ap.add_argument("--tail-rounds", type=int, default=None, help="override the tuned schedule")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain 256 MiB device-to-device copy (HBM counter calibration)")
     return ap.parse_args()
+
+
+def copy_calibration():
+    """A plain copy kernel of known size: FETCH_SIZE / WRITE_SIZE of the same rocprofv3 --pmc
+    run, divided by the bytes below, give the factor that turns those counters into bytes."""
+    import torch
+    n = 256 << 20
+    src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "launches": 3}), flush=True)
 
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
@@ -102,8 +125,14 @@ def main():
     k1_s = k1_ms / 1e3 / args.steps
     achieved_gbs = BYTES_PER_ITER * (iters_pass1 / args.steps) / k1_s / 1e9
     achieved_tflops = FLOPS_PER_ITER * (iters_pass1 / args.steps) / k1_s / 1e12
+    default_workload = (args.width, args.height, args.spp, args.nm_res) == (512, 512, 4, 2048)
+    traffic = None
+    if default_workload:
+        t = TRAFFIC_FIRST_PASS
+        traffic = round(1024.0 * (t["fetch_kib"] * t["fetch_factor"] + t["write_kib"] * t["write_factor"]))
     roofline = {"bound": "hbm", "achieved": round(achieved_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved_gbs / HBM_PEAK_GBS, 5), "traffic": None,
+                "frac": round(achieved_gbs / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "bytes_per_iter": BYTES_PER_ITER, "iters_per_launch": round(iters_pass1 / args.steps),
                 "kernel": "k_glint_first", "kernel_ms": round(k1_s * 1e3, 3),
                 "valu": {"achieved": round(achieved_tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                          "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 5),

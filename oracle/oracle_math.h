@@ -13,7 +13,8 @@
  *   fmadd/fnmadd/fmsub map to fmaf wherever the reference writes them; every other
  *   expression is evaluated as written, without contraction (-ffp-contract=off).
  * Transcendentals follow S. L. Moshier's Cephes single-precision routines (sinf/cosf,
- * atanf, asinf/acosf, logf, expf, erff), re-implemented here from the published method.
+ * atanf, asinf/acosf), re-implemented here from the published method; exp, erf and log2 of
+ * the LEAN pdf are FP64 series rounded to float (see below).
  */
 #pragma once
 #include <math.h>
@@ -185,5 +186,61 @@ static inline uint64_t mix64(uint64_t z) {
 static inline void stream_seed(Pcg32 *r, uint64_t seed, uint64_t query, uint64_t walk) {
     pcg32_seed(r, mix64(seed ^ mix64(query + 0x9E3779B97F4A7C15ULL)), walk);
 }
+
+/* ---------------------------------------------------------------- exp / erf / log2
+   For the LEAN pdf (the glint MIS weight).  Evaluated in FP64 with fixed-length series and
+   rounded to float, so they stand in for the correctly rounded float functions; the GPU code
+   runs the same operations.  exp: range reduction by ln 2 (split constant) and a degree-13
+   Taylor polynomial; erf: Maclaurin series below 2, Laplace continued fraction of erfc
+   above; log2: exponent split and the atanh series of ln((1+s)/(1-s)). */
+static inline double d_from_bits(uint64_t b) { double d; memcpy(&d, &b, 8); return d; }
+static inline uint64_t d_bits(double d) { uint64_t b; memcpy(&b, &d, 8); return b; }
+static inline double ldexp2(double p, int n) {
+    int n1 = n / 2, n2 = n - n1;
+    return (p * d_from_bits((uint64_t) (n1 + 1023) << 52)) * d_from_bits((uint64_t) (n2 + 1023) << 52);
+}
+static inline double exp_d(double x) {
+    if (x != x) return x;
+    if (x > 709.0) return INFINITY;
+    if (x < -745.0) return 0.0;
+    double k = floor(x * 1.4426950408889634 + 0.5);
+    double r = (x - k * 6.93147180369123816490e-01) - k * 1.90821492927058770002e-10;
+    double p = 1.0 / 6227020800.0;
+    p = p * r + 1.0 / 479001600.0; p = p * r + 1.0 / 39916800.0; p = p * r + 1.0 / 3628800.0;
+    p = p * r + 1.0 / 362880.0; p = p * r + 1.0 / 40320.0; p = p * r + 1.0 / 5040.0; p = p * r + 1.0 / 720.0;
+    p = p * r + 1.0 / 120.0; p = p * r + 1.0 / 24.0; p = p * r + 1.0 / 6.0; p = p * r + 0.5; p = p * r + 1.0;
+    p = p * r + 1.0;
+    return ldexp2(p, (int) k);
+}
+static inline double erf_d(double x) {
+    double ax = fabs(x), e;
+    if (ax < 2.0) {
+        double x2 = ax * ax, term = ax, s = ax;
+        for (int n = 1; n <= 36; ++n) {
+            term = term * (-x2 / (double) n);
+            s = s + term / (double) (2 * n + 1);
+        }
+        e = s * 1.12837916709551257390;          /* 2 / sqrt(pi) */
+    } else if (ax < 6.0) {
+        double f = ax;                            /* erfc = exp(-x^2)/sqrt(pi) / (x + (1/2)/(x + 1/(x + (3/2)/(x + ...)))) */
+        for (int k = 40; k >= 1; --k) f = ax + (0.5 * (double) k) / f;
+        e = 1.0 - exp_d(-ax * ax) * 0.56418958354775628695 / f;
+    } else {
+        e = 1.0;
+    }
+    return x < 0.0 ? -e : e;
+}
+static inline double log2_d(double x) {          /* x positive, normal */
+    uint64_t b = d_bits(x);
+    int e = (int) ((b >> 52) & 0x7ff) - 1022;
+    double m = d_from_bits((b & 0x000fffffffffffffULL) | 0x3fe0000000000000ULL);   /* [0.5, 1) */
+    if (m < 0.70710678118654752440) { m = m + m; e -= 1; }
+    double s = (m - 1.0) / (m + 1.0), s2 = s * s, t = 0.0;
+    for (int k = 39; k >= 1; k -= 2) t = t * s2 + 1.0 / (double) k;
+    return (double) e + (2.0 * s * t) * 1.44269504088896340736;
+}
+static inline float expf_l(float x) { return (float) exp_d((double) x); }
+static inline float erff_l(float x) { return (float) erf_d((double) x); }
+static inline float log2f_l(float x) { return (float) log2_d((double) x); }
 
 } // namespace orc

@@ -46,6 +46,9 @@ struct Emitter { int type, shape; V3 radiance, position, intensity; bool ss, ms;
 struct Normalmap {
     int res;
     std::vector<float> slope;       // level-0 slopes (res*res*2), normalmap.cpp:46-66
+    /* LEAN mip pyramid: level l has (res >> l)^2 texels of (E[x], E[y], E[x^2], E[y^2], E[xy])
+       (normalmap.cpp:36-87) */
+    std::vector<std::vector<float>> lean;
 };
 
 struct Camera {
@@ -214,6 +217,39 @@ static void camera_setup(Camera &c, const sms_camera_desc &d) {
     c.w = d.width; c.h = d.height;
 }
 
+/* LEAN map (normalmap.cpp:36-87): level 0 = (s_x, s_y, s_x^2 + b, s_y^2 + b, s_x s_y) with the
+   base Beckmann variance b = 0.5 * 1e-4^2; each coarser level is the box-filter resample to half
+   resolution (Bitmap::resample, bitmap.cpp:234-312, rfilter.h:107-203): two taps of weight 0.5
+   per axis, first along x, then along y. */
+static void build_lean(Normalmap &nm) {
+    const int res = nm.res;
+    const float base_alpha = 0.0001f, base_sigma_2 = 0.5f * base_alpha * base_alpha;
+    std::vector<float> l0((size_t) res * res * 5);
+    for (size_t s = 0; s < (size_t) res * res; ++s) {
+        float sx = nm.slope[2 * s], sy = nm.slope[2 * s + 1];
+        float *d = &l0[5 * s];
+        d[0] = sx; d[1] = sy; d[2] = sx * sx + base_sigma_2; d[3] = sy * sy + base_sigma_2; d[4] = sx * sy;
+    }
+    nm.lean.clear();
+    nm.lean.push_back(std::move(l0));
+    for (int r = res; r > 1; r /= 2) {
+        const std::vector<float> &src = nm.lean.back();
+        const int h = r / 2;
+        std::vector<float> tmp((size_t) h * r * 5), dst((size_t) h * h * 5);
+        for (int y = 0; y < r; ++y)
+            for (int x = 0; x < h; ++x)
+                for (int c = 0; c < 5; ++c)
+                    tmp[((size_t) y * h + x) * 5 + c] = src[((size_t) y * r + 2 * x) * 5 + c] * 0.5f +
+                                                        src[((size_t) y * r + 2 * x + 1) * 5 + c] * 0.5f;
+        for (int y = 0; y < h; ++y)
+            for (int x = 0; x < h; ++x)
+                for (int c = 0; c < 5; ++c)
+                    dst[((size_t) y * h + x) * 5 + c] = tmp[((size_t) (2 * y) * h + x) * 5 + c] * 0.5f +
+                                                        tmp[((size_t) (2 * y + 1) * h + x) * 5 + c] * 0.5f;
+        nm.lean.push_back(std::move(dst));
+    }
+}
+
 static Scene *scene_create(const sms_scene_desc *desc, const sms_config *cfg) {
     Scene *S = new Scene();
     S->cfg = *cfg;
@@ -230,6 +266,7 @@ static Scene *scene_create(const sms_scene_desc *desc, const sms_config *cfg) {
             nm.slope[2 * s + 0] = -n.x / n.z;
             nm.slope[2 * s + 1] = -n.y / n.z;
         }
+        build_lean(nm);
         S->nmaps.push_back(nm);
     }
     for (int i = 0; i < desc->n_emitters; ++i) {
@@ -1458,6 +1495,153 @@ static float sensor_we(const Scene &S, V3 sensor_p, V3 p) {
     return We * (float) (c.w * c.h);
 }
 
+/* ---------------------------------------------------------------- LEAN pdf (glint MIS) */
+struct LeanParams { V3 n; float ax, ay, rho, sqrt_one_minus_rho_sqr, tx_n, ty_n; };
+
+static const float INV_SQRT_PI_F = 0.56418958354775628695f;
+
+/* NormalmapBSDF::mipmap_level (bsdfs/normalmap.cpp:361-374).  A negative ceil() converts to a
+   huge size_t in the reference and min() then selects the coarsest level. */
+static int lean_level(float tiles, V2 duv_dx, V2 duv_dy, int levels) {
+    V2 a = duv_dx * tiles, b = duv_dy * tiles;
+    float width = 2.f * std::max(std::max(fabsf(a.x), fabsf(a.y)), std::max(fabsf(b.x), fabsf(b.y)));
+    float l = ceilf((float) (levels - 1) + log2f_l(std::max(width, 1e-8f)));
+    if (l < 0.f) return levels - 1;
+    return std::min((int) l, levels - 1);
+}
+
+/* eval_normal (use_slopes) and eval_lean_sigma at one level (normalmap.h:61-92,198-228) */
+static void lean_moments(const Normalmap &nm, V2 uv, int level, V2 *mu, float sig[3]) {
+    const int res = nm.res >> level;
+    const float fres = (float) res, duv = 1.f / fres;
+    V2 pos = v2((uv.x - 0.5f * duv) * fres, (uv.y - 0.5f * duv) * fres);
+    pos.x = std::min(std::max(pos.x, 0.f), fres - 1.f); pos.y = std::min(std::max(pos.y, 0.f), fres - 1.f);
+    uint32_t xmin = (uint32_t) floorf(pos.x), ymin = (uint32_t) floorf(pos.y);
+    uint32_t xmax = xmin + 1, ymax = ymin + 1, lim = (uint32_t) (res - 1);
+    xmin = std::min(xmin, lim); ymin = std::min(ymin, lim); xmax = std::min(xmax, lim); ymax = std::min(ymax, lim);
+    V2 w1 = v2(pos.x - (float) xmin, pos.y - (float) ymin), w0 = v2(1.f - w1.x, 1.f - w1.y);
+    const float *L = nm.lean[level].data();
+    const float *v00 = L + 5 * ((size_t) ymin * res + xmin), *v10 = L + 5 * ((size_t) ymin * res + xmax),
+                *v01 = L + 5 * ((size_t) ymax * res + xmin), *v11 = L + 5 * ((size_t) ymax * res + xmax);
+    float v[5];
+    for (int c = 0; c < 5; ++c) {
+        float a = fmaf(w0.x, v00[c], w1.x * v10[c]), b = fmaf(w0.x, v01[c], w1.x * v11[c]);
+        v[c] = fmaf(w0.y, a, w1.y * b);
+    }
+    V3 n = v3(-v[0], -v[1], 1.f);
+    *mu = v2(-n.x / n.z, -n.y / n.z);
+    sig[0] = std::max(0.f, v[2] - v[0] * v[0]);     /* sigma(0,0) */
+    sig[1] = std::max(0.f, v[3] - v[1] * v[1]);     /* sigma(1,1) */
+    sig[2] = std::max(0.f, v[4] - v[0] * v[1]);     /* sigma(0,1) = sigma(1,0) */
+}
+
+/* LEAN::p_from_lean_and_base (lean.h:44-77) */
+static LeanParams lean_params(V2 mu, const float sig[3], float alpha_u, float alpha_v) {
+    float lean_e1 = mu.x, lean_e2 = mu.y;
+    float lean_ax = sqrtf(2.f * sig[0]), lean_ay = sqrtf(2.f * sig[1]);
+    float rho = 2.f * sig[2] / (lean_ax * lean_ay);
+    float lean_e3 = 0.5f * lean_ax * lean_ax + lean_e1 * lean_e1;
+    float lean_e4 = 0.5f * lean_ay * lean_ay + lean_e2 * lean_e2;
+    float lean_e5 = 0.5f * rho * lean_ax * lean_ay + lean_e1 * lean_e2;
+    float base_e3 = 0.5f * alpha_u * alpha_u, base_e4 = 0.5f * alpha_v * alpha_v;
+    float e1 = lean_e1, e2 = lean_e2, e3 = lean_e3 + base_e3, e4 = lean_e4 + base_e4, e5 = lean_e5;
+    LeanParams p;
+    p.tx_n = e1; p.ty_n = e2;
+    float tmp3 = std::max(0.f, e3 - e1 * e1), tmp4 = std::max(0.f, e4 - e2 * e2);
+    p.ax = sqrtf(2.f * tmp3);
+    p.ay = sqrtf(2.f * tmp4);
+    p.rho = 2.f * (e5 - e1 * e2) / (p.ax * p.ay);
+    p.sqrt_one_minus_rho_sqr = sqrtf(1.f - p.rho * p.rho);
+    p.n = normalize(v3(-p.tx_n, -p.ty_n, 1.f));
+    return p;
+}
+
+/* LEAN::sigma_std, sigma, g1, gaf, p22, ndf, vndf (lean.h:79-212) */
+static float lean_sigma_std(float cos_theta) {
+    if (cos_theta == 1.f) return 1.f;
+    float sin_theta = sqrtf(1.f - cos_theta * cos_theta);
+    float nu = cos_theta / sin_theta;
+    float tmp = expf_l(-nu * nu) * INV_SQRT_PI_F;
+    return 0.5f * (cos_theta * (1.f + erff_l(nu)) + sin_theta * tmp);
+}
+static float lean_sigma(V3 k, const LeanParams &p) {
+    float a = k.x * p.ax + k.y * p.ay * p.rho;
+    float b = k.y * p.ay * p.sqrt_one_minus_rho_sqr;
+    float c = k.z - k.x * p.tx_n - k.y * p.ty_n;
+    float nrm = sqrtf(a * a + b * b + c * c);
+    V3 kp = v3(a, b, c) * (1.f / nrm);
+    return nrm * lean_sigma_std(kp.z);
+}
+static float lean_g1(V3 k, const LeanParams &p) {
+    if (dot(k, p.n) > 0.f) return k.z / lean_sigma(k, p);
+    return 0.f;
+}
+static float lean_p22(float sx, float sy, const LeanParams &p) {
+    sx -= p.tx_n;
+    sy -= p.ty_n;
+    float nrm = p.ax * p.ay * p.sqrt_one_minus_rho_sqr;
+    float x = sx / p.ax;
+    float tmp1 = p.ax * sy - p.rho * p.ay * sx;
+    float tmp2 = p.ax * p.ay * p.sqrt_one_minus_rho_sqr;
+    float y = tmp1 / tmp2;
+    float r = x * x + y * y;
+    return (INV_PI_F * expf_l(-r)) / nrm;
+}
+static float lean_ndf(V3 h, const LeanParams &p) {
+    if (h.z > 5.9604644775390625e-08f) {          /* math::Epsilon<float> */
+        float tmp1 = h.z * h.z, tmp2 = tmp1 * tmp1;
+        return lean_p22(-h.x / h.z, -h.y / h.z, p) / tmp2;
+    }
+    return 0.f;
+}
+static float lean_vndf(V3 h, V3 wi, const LeanParams &p) {
+    float dp = dot(wi, h);
+    if (dp > 0.f) return dp * lean_ndf(h, p) / lean_sigma(wi, p);
+    return 0.f;
+}
+
+/* SpecularManifoldGlints::lean_pdf (manifold_glints.cpp:556-579) with the smooth conductor's
+   roughness 0 (conductor.cpp:288-290); wi, wo in the query's shading frame */
+static float lean_pdf(const Scene &S, const GlintSI &si, V3 wo) {
+    if (si.wi.z < 0.f || wo.z < 0.f) return 0.f;
+    const Shape &sh = S.shapes[si.shape];
+    const Normalmap &nm = S.nmaps[sh.normalmap];
+    int level = lean_level(sh.tiles, si.duv_dx, si.duv_dy, (int) nm.lean.size());
+    V2 uv = si.uv * sh.tiles;
+    uv = v2(uv.x - floorf(uv.x), uv.y - floorf(uv.y));
+    V2 mu; float sig[3];
+    lean_moments(nm, uv, level, &mu, sig);
+    LeanParams p = lean_params(mu, sig, 0.f, 0.f);
+    V3 H = normalize(wo + si.wi);
+    float G1i = lean_g1(wo, p), G1o = lean_g1(si.wi, p);
+    float G = G1i * G1o;
+    if (!(G > 0.f)) return 0.f;
+    return lean_vndf(H, si.wi, p) / (4.f * dot(si.wi, H));
+}
+
+/* power heuristic (manifold_glints.h:155-159) */
+static float mis_power(float a, float b) {
+    a *= a;
+    b *= b;
+    return a > 0.f ? a / (a + b) : 0.f;
+}
+
+/* Scene::pdf_emitter_direction for the SMS strategy's emitter sample (scene.cpp:206-219,
+   area.cpp:119-125, sphere.cpp:254-267); 1 for delta emitters (manifold_glints.cpp:158-165) */
+static float sms_emitter_pdf(const Scene &S, const EI &ei, V3 p) {
+    if (ei.kind != SMS_EI_AREA) return 1.f;
+    const Emitter &em = S.emitters[ei.emitter];
+    const Shape &sh = S.shapes[em.shape];
+    float sin_alpha = sh.radius * (1.f / norm(sh.center - p));
+    float cos_alpha = safe_sqrt(1.f - sin_alpha * sin_alpha);
+    float pdf;
+    if (sin_alpha < 0.99999994039535522461f) pdf = INV_TWOPI_F / (1.f - cos_alpha);
+    else pdf = sh.inv_surface_area * (norm(ei.p - p) * norm(ei.p - p)) / fabsf(dot(ei.d, ei.n));
+    if (!(dot(ei.d, ei.n) < 0.f)) pdf = 0.f;
+    if (S.emitters.size() > 1) pdf = pdf * (1.f / (float) S.emitters.size());
+    return pdf;
+}
+
 /* SMS strategy of the glint estimator (manifold_glints.cpp:28-181) */
 static V3 estimate_glint(const Scene &S, const sms_glint_query &q, uint64_t seed, uint64_t qid, Stats &st) {
     const sms_config &cfg = S.cfg;
@@ -1519,11 +1703,14 @@ static V3 estimate_glint(const Scene &S, const sms_glint_query &q, uint64_t seed
     }
     if (sms_success) {
         float We = sensor_we(S, sensor_p, si.p);
-        float p_sms = 1.f;
-        /* p_bsdf = lean_pdf + 1 is not built yet: the MIS weight is taken as 1 (sms_strategy_only) */
-        float mis = 1.f;
+        /* MIS against the BSDF strategy (manifold_glints.cpp:155-179): p_sms is the emitter
+           sampling density, p_bsdf the LEAN pdf of the emitter direction plus 1 */
+        float p_sms = sms_emitter_pdf(S, ei, si.p);
+        V3 wo_l = v3(dot(edir, si.s), dot(edir, si.t), dot(edir, si.n));
+        float p_bsdf = lean_pdf(S, si, wo_l);
+        p_bsdf += 1.f;
+        float mis = mis_power(p_sms, p_bsdf);
         if (cfg.bsdf_strategy_only) mis = 0.f;
-        (void) p_sms;
         float f = mis * We;
         result = result + v3(f * sms_weight.x * ei.weight.x, f * sms_weight.y * ei.weight.y, f * sms_weight.z * ei.weight.z);
     }
@@ -2332,6 +2519,29 @@ float oracle_ms_tridiag_geo(int n, const float *dprev, const float *dcur, const 
     }
     return ms_invert_tridiagonal_geo(v, n);
 }
+
+/* LEAN pieces exposed for the pdf-normalisation and mip-pyramid checks. */
+float oracle_lean_pdf_params(const float *mu, const float *sig, const float *wi, const float *wo) {
+    LeanParams p = lean_params(v2(mu[0], mu[1]), sig, 0.f, 0.f);
+    V3 i = v3(wi[0], wi[1], wi[2]), o = v3(wo[0], wo[1], wo[2]);
+    if (i.z < 0.f || o.z < 0.f) return 0.f;
+    V3 H = normalize(o + i);
+    float G = lean_g1(o, p) * lean_g1(i, p);
+    if (!(G > 0.f)) return 0.f;
+    return lean_vndf(H, i, p) / (4.f * dot(i, H));
+}
+int oracle_lean_levels(void *h, int nmap) { return (int) ((Scene *) h)->nmaps[nmap].lean.size(); }
+int oracle_lean_level(float tiles, const float *duv_dx, const float *duv_dy, int levels) {
+    return lean_level(tiles, v2(duv_dx[0], duv_dx[1]), v2(duv_dy[0], duv_dy[1]), levels);
+}
+void oracle_lean_texel(void *h, int nmap, int level, int x, int y, float *out5) {
+    const Normalmap &nm = ((Scene *) h)->nmaps[nmap];
+    const int res = nm.res >> level;
+    for (int c = 0; c < 5; ++c) out5[c] = nm.lean[level][((size_t) y * res + x) * 5 + c];
+}
+double oracle_exp_d(double x) { return exp_d(x); }
+double oracle_erf_d(double x) { return erf_d(x); }
+double oracle_log2_d(double x) { return log2_d(x); }
 
 /* Math routines exposed for pinning against libm. */
 float oracle_acosf(float x) { return safe_acos(x); }

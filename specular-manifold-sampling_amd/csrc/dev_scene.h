@@ -35,7 +35,13 @@ struct DEmitter { int type, shape; float radiance[3], position[3], intensity[3];
 // res+1): texels (x, y), (x+1, y) of a bilinear footprint are adjacent for every x <= res-1,
 // and the pad reproduces the edge clamp of Normalmap::eval_normal (normalmap.h:67-72), so a
 // fetch is two 16-byte loads.
-struct DNormalmap { int res; int stride; const float2 *slope; };
+// LEAN mip pyramid: level l holds (res >> l)^2 texels of (E[x], E[y], E[x^2], E[y^2], E[xy]),
+// unpadded, starting at float offset lean_off[l] of `lean`.
+constexpr int SMS_MAX_LEAN_LEVELS = 16;
+struct DNormalmap {
+    int res; int stride; const float2 *slope;
+    const float *lean; int levels; long long lean_off[SMS_MAX_LEAN_LEVELS];
+};
 
 struct DCamera {
     float o[3], left[3], up[3], dir[3];

@@ -3,6 +3,7 @@
 #pragma once
 #include "dev_ss.h"
 #include "dev_glint.h"
+#include "dev_lean.h"
 
 namespace smsdev {
 

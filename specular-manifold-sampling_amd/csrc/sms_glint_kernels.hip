@@ -542,7 +542,18 @@ __global__ void __launch_bounds__(256) k_glint_finalize(DScene S, const sms_glin
         V3 sensor_p = ld3(g.sensor_p);
         float val = glint_contribution(S, sensor_p, ei, si, v2(tr.uv_sol[0], tr.uv_sol[1]));
         float We = sensor_we(S.cam, sensor_p, si.p);
-        float mis = cfg.bsdf_strategy_only ? 0.f : 1.f;
+        // MIS against the BSDF strategy (manifold_glints.cpp:155-179): p_sms is the emitter
+        // sampling density, p_bsdf the LEAN pdf of the emitter direction plus 1
+        V3 edir = ei.p - si.p;
+        float edist = norm(edir);
+        edir = edir * (1.f / edist);
+        float p_sms = sms_emitter_pdf(S, ei, si.p);
+        V3 wo_l = v3(dot(edir, si.sh_s), dot(edir, si.sh_t), dot(edir, si.sh_n));
+        float p_bsdf = lean_pdf(S, g.shape, v2(g.uv[0], g.uv[1]), v2(g.duv_dx[0], g.duv_dx[1]),
+                                v2(g.duv_dy[0], g.duv_dy[1]), ld3(g.wi), wo_l);
+        p_bsdf += 1.f;
+        float mis = mis_power(p_sms, p_bsdf);
+        if (cfg.bsdf_strategy_only) mis = 0.f;
         write_rgb(out, q, glint_result(val, res.inv_prob, mis * We, ei.weight));
     }
 }

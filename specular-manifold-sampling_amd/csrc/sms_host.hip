@@ -264,7 +264,47 @@ sms_status build_scene(sms_ctx *ctx, const sms_scene_desc *desc, const sms_confi
         const float2 *dp;
         sms_status st = upload(ctx, sl, &dp);
         if (st) return st;
-        DNormalmap m; m.res = nd.res; m.stride = (int) stride; m.slope = dp;
+        DNormalmap m;
+        memset(&m, 0, sizeof(m));
+        m.res = nd.res; m.stride = (int) stride; m.slope = dp;
+        // LEAN map (normalmap.cpp:36-87): level 0 = (s_x, s_y, s_x^2 + b, s_y^2 + b, s_x s_y) with
+        // the base Beckmann variance b = 0.5 * 1e-4^2; coarser levels are the two-tap box resample
+        // to half resolution (weights 0.5), along x first, then along y (bitmap.cpp:234-312)
+        const float base_alpha = 0.0001f, base_sigma_2 = 0.5f * base_alpha * base_alpha;
+        std::vector<float> lean((size_t) res * res * 5);
+        for (size_t y = 0; y < res; ++y)
+            for (size_t x = 0; x < res; ++x) {
+                float2 v = sl[y * stride + x];
+                float *d = &lean[(y * res + x) * 5];
+                d[0] = v.x; d[1] = v.y; d[2] = v.x * v.x + base_sigma_2; d[3] = v.y * v.y + base_sigma_2;
+                d[4] = v.x * v.y;
+            }
+        m.levels = 1;
+        size_t off = 0;
+        for (size_t r = res; r > 1; r /= 2) {
+            if (m.levels >= SMS_MAX_LEAN_LEVELS) return fail(ctx, SMS_E_INVALID, "normal map too large");
+            const size_t h = r / 2;
+            std::vector<float> tmp(h * r * 5);
+            for (size_t y = 0; y < r; ++y)
+                for (size_t x = 0; x < h; ++x)
+                    for (int c = 0; c < 5; ++c)
+                        tmp[(y * h + x) * 5 + c] = lean[off + (y * r + 2 * x) * 5 + c] * 0.5f +
+                                                   lean[off + (y * r + 2 * x + 1) * 5 + c] * 0.5f;
+            const size_t next = off + r * r * 5;
+            lean.resize(next + h * h * 5);
+            for (size_t y = 0; y < h; ++y)
+                for (size_t x = 0; x < h; ++x)
+                    for (int c = 0; c < 5; ++c)
+                        lean[next + (y * h + x) * 5 + c] = tmp[((2 * y) * h + x) * 5 + c] * 0.5f +
+                                                           tmp[((2 * y + 1) * h + x) * 5 + c] * 0.5f;
+            m.lean_off[m.levels] = (long long) next;
+            m.levels++;
+            off = next;
+        }
+        const float *dl;
+        st = upload(ctx, lean, &dl);
+        if (st) return st;
+        m.lean = dl;
         nmaps.push_back(m);
     }
     std::vector<DEmitter> emitters;

@@ -40,6 +40,14 @@ def load():
     lib.oracle_ms_tridiag_step.argtypes = [C.c_int, vp, vp, vp, vp, vp]
     lib.oracle_ms_tridiag_geo.argtypes = [C.c_int, vp, vp, vp]
     lib.oracle_ms_tridiag_geo.restype = C.c_float
+    lib.oracle_lean_pdf_params.argtypes = [vp, vp, vp, vp]
+    lib.oracle_lean_pdf_params.restype = C.c_float
+    lib.oracle_lean_levels.argtypes = [vp, C.c_int]
+    lib.oracle_lean_level.argtypes = [C.c_float, vp, vp, C.c_int]
+    lib.oracle_lean_texel.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp]
+    for fn in ("oracle_exp_d", "oracle_erf_d", "oracle_log2_d"):
+        getattr(lib, fn).argtypes = [C.c_double]
+        getattr(lib, fn).restype = C.c_double
     lib.oracle_acosf.argtypes = [C.c_float]
     lib.oracle_acosf.restype = C.c_float
     lib.oracle_atan2f.argtypes = [C.c_float, C.c_float]
