@@ -356,6 +356,59 @@ sms_status sms_ss_estimate_batch_host(sms_ctx *ctx, const sms_ss_query *q, int64
 sms_status sms_glint_estimate_batch_host(sms_ctx *ctx, const sms_glint_query *q, int64_t n,
                                          uint64_t seed, float *out_rgb, sms_stats *stats);
 
+/* ---------------------------------------------------------------- learned seed proposal
+   No counterpart in the reference, whose seeds are uniform over the caster's area
+   (manifold_ss.cpp:228-235).  A small network maps (shading point, shading normal, light
+   sample), positions normalised to the scene's bounding box, through 8-frequency Fourier
+   features and 4 ReLU layers of width 64 (BF16 weights and activations, FP32 accumulation on
+   the matrix cores) to an 8-component Gaussian mixture over the UV of one mesh caster
+   (48 raw outputs: per component weight logit, mean u, mean v (sigmoid), log sigma u, log sigma
+   v (clamped to [-7, 0]), correlation (0.95 tanh)).  With a proposal set, the seed of every
+   walk on that caster -- the first walk and every Bernoulli trial -- is drawn from
+   defensive * (uniform area seed of the reference) + (1 - defensive) * mixture, so the
+   inverse-PDF estimate stays unbiased; a mixture seed whose uv falls outside the mesh's UV
+   domain is a failed walk with 0 iterations.  Training minimises the mixture's negative
+   log-likelihood of converged solutions' uv (Adam). */
+#define SMS_MLP_INPUTS 9
+#define SMS_MLP_OUTPUTS 48
+#define SMS_MLP_PARAMS 25904
+typedef struct sms_mlp sms_mlp;
+/* He-initialised network on ctx's device (the output layer scaled by 0.1). */
+sms_status sms_mlp_create(sms_ctx *ctx, uint64_t seed, sms_mlp **out);
+sms_status sms_mlp_destroy(sms_mlp *mlp);
+/* FP32 master parameters (SMS_MLP_PARAMS floats, host memory): per layer the weights
+   output-major W[n][k] (layer 1: k < 160, the 153 features zero-padded), then the bias. */
+sms_status sms_mlp_get_params(sms_mlp *mlp, float *params);
+sms_status sms_mlp_set_params(sms_mlp *mlp, const float *params);
+/* x: n x 9 inputs, out: n x 48 raw outputs (device memory). */
+sms_status sms_mlp_infer(sms_mlp *mlp, const float *x, int64_t n, float *out);
+/* Mean negative log-likelihood of uv (n x 2, device) under the mixture of x (n x 9, device)
+   and its gradient; the gradient is kept for sms_mlp_adam and, if grad is not NULL, copied to
+   grad (SMS_MLP_PARAMS host floats).  *loss receives the mean loss. */
+sms_status sms_mlp_grad(sms_mlp *mlp, const float *x, const float *uv, int64_t n, float *grad, float *loss);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) with the kept gradient, which is cleared. */
+sms_status sms_mlp_adam(sms_mlp *mlp, float lr);
+/* Use `mlp` as the seed proposal of single-scatter caster mesh `shape` (with uvs) in
+   sms_ss_estimate_batch; defensive in (0, 1].  mlp NULL: uniform seeds, `shape` still names the
+   caster whose solutions the training capture records; shape -1 clears the proposal. */
+sms_status sms_ctx_set_proposal(sms_ctx *ctx, sms_mlp *mlp, int32_t shape, float defensive);
+/* Record (inputs, solution uv) of converged, unoccluded first walks on the proposal shape in
+   later sms_ss_estimate_batch calls: 64 lists of ceil(capacity / 64) samples, filled by the
+   estimator's blocks in turn, so up to 64 ceil(capacity / 64) samples in total; a full list
+   drops further samples (0 disables and frees). */
+sms_status sms_ctx_set_training_capture(sms_ctx *ctx, int64_t capacity);
+/* Number of recorded samples; when x / uv are not NULL, copies them to device buffers of
+   count x 9 and count x 2 floats. */
+sms_status sms_ctx_training_samples(sms_ctx *ctx, int64_t *count, float *x, float *uv);
+sms_status sms_ctx_clear_training(sms_ctx *ctx);
+/* Network inputs (device, n x 9) of the queries of an estimate with this seed: shading point,
+   shading normal and the estimator's light sample, positions mapped by (p - c) / h with c the
+   centre and h the largest half extent of the scene's bounding box. */
+sms_status sms_ss_proposal_inputs(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *x);
+/* Raw network outputs (device, n x 48) the proposal gives the queries of an estimate with this
+   seed -- the exact values sms_ss_estimate_batch turns into its seed mixtures. */
+sms_status sms_ss_proposal_outputs(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *out);
+
 /* ---------------------------------------------------------------- device memory helpers */
 sms_status sms_device_alloc(sms_ctx *ctx, size_t bytes, void **ptr);
 sms_status sms_device_free(sms_ctx *ctx, void *ptr);

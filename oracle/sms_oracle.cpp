@@ -73,6 +73,7 @@ struct Scene {
     std::vector<int> casters_ss, emitters_ss, emitters_all;
     Camera cam;
     bool has_cam;
+    float box_c[3], box_inv;             // bounding box centre, 1 / largest half extent
     sms_config cfg;
 };
 
@@ -347,6 +348,28 @@ static Scene *scene_create(const sms_scene_desc *desc, const sms_config *cfg) {
         if (d.flags & SMS_FLAG_CAUSTIC_CASTER_SS) S->casters_ss.push_back(i);
     }
     bvh_build(*S);
+    /* scene bounding box over mesh vertices, spheres and rectangle corners (inputs of the
+       learned proposal are normalised to it) */
+    float lo[3] = {INF_F, INF_F, INF_F}, hi[3] = {-INF_F, -INF_F, -INF_F};
+    auto grow = [&](float x, float y, float z) {
+        const float v[3] = {x, y, z};
+        for (int a = 0; a < 3; ++a) { lo[a] = std::min(lo[a], v[a]); hi[a] = std::max(hi[a], v[a]); }
+    };
+    for (size_t v = 0; v < S->pos.size() / 3; ++v) grow(S->pos[3 * v], S->pos[3 * v + 1], S->pos[3 * v + 2]);
+    for (const Shape &sh : S->shapes) {
+        if (sh.type == SMS_SHAPE_SPHERE) {
+            grow(sh.center.x - sh.radius, sh.center.y - sh.radius, sh.center.z - sh.radius);
+            grow(sh.center.x + sh.radius, sh.center.y + sh.radius, sh.center.z + sh.radius);
+        } else if (sh.type == SMS_SHAPE_RECTANGLE) {
+            for (int cy = -1; cy <= 1; cy += 2)
+                for (int cx = -1; cx <= 1; cx += 2)
+                    grow(sh.M[0] * cx + sh.M[1] * cy + sh.M[3], sh.M[4] * cx + sh.M[5] * cy + sh.M[7],
+                         sh.M[8] * cx + sh.M[9] * cy + sh.M[11]);
+        }
+    }
+    float half = 0.f;
+    for (int a = 0; a < 3; ++a) { S->box_c[a] = 0.5f * (lo[a] + hi[a]); half = std::max(half, 0.5f * (hi[a] - lo[a])); }
+    S->box_inv = half > 0.f ? 1.f / half : 1.f;
     S->has_cam = desc->camera.width > 0;
     if (S->has_cam) camera_setup(S->cam, desc->camera);
     return S;
@@ -1238,8 +1261,140 @@ static SI si_from_query(const sms_ss_query &q) {
 
 struct Stats { uint64_t walks, conv, iters, bcalls, btrials, bmax; };
 
+/* ================================================================ learned seed proposal
+   No counterpart in the reference, whose seeds are uniform over the caster's area
+   (manifold_ss.cpp:228-235).  Restates the library's defensive-mixture seed so that estimates
+   with a proposal are checked walk for walk: the raw network outputs of a query are the
+   input; everything after them (mixture preparation, seed draw, UV lookup, the walk) is
+   restated here. */
+static const int PROP_COMP = 8, PROP_RAW = 48;
+
+struct UVGrid { int g = 0, shape = -1; std::vector<int> start, faces; };
+
+/* G x G cells over the UV square (G the power of two with 2 G^2 >= faces, at most 1024);
+   cell (x, y) lists in ascending order the faces whose UV bounding box spans it, spans taken
+   as floor(lo G) .. floor(hi G) clamped to the grid */
+static void build_uv_grid(const Scene &S, int shape, UVGrid &G) {
+    const Shape &sh = S.shapes[shape];
+    int g = 1;
+    while (2 * (int64_t) g * g < sh.nf && g < 1024) g *= 2;
+    auto cell_of = [&](float t) { int c = (int) floorf(t * (float) g); return std::min(std::max(c, 0), g - 1); };
+    std::vector<std::vector<int>> cells((size_t) g * g);
+    for (int f = 0; f < sh.nf; ++f) {
+        const uint32_t *fc = &S.faces[3 * (size_t) (sh.fbase + f)];
+        float ux[3], uy[3];
+        for (int k = 0; k < 3; ++k) { ux[k] = S.uvs[2 * (size_t) fc[k]]; uy[k] = S.uvs[2 * (size_t) fc[k] + 1]; }
+        const float lx = std::min(std::min(ux[0], ux[1]), ux[2]), hx = std::max(std::max(ux[0], ux[1]), ux[2]);
+        const float ly = std::min(std::min(uy[0], uy[1]), uy[2]), hy = std::max(std::max(uy[0], uy[1]), uy[2]);
+        for (int y = cell_of(ly); y <= cell_of(hy); ++y)
+            for (int x = cell_of(lx); x <= cell_of(hx); ++x) cells[(size_t) y * g + x].push_back(sh.fbase + f);
+    }
+    G.g = g; G.shape = shape;
+    G.start.assign(1, 0); G.faces.clear();
+    for (auto &c : cells) { G.faces.insert(G.faces.end(), c.begin(), c.end()); G.start.push_back((int) G.faces.size()); }
+}
+
+/* surface point of uv: first listed face of uv's cell whose UV triangle holds it (UV-space
+   barycentrics), positions interpolated with the same weights */
+static bool uv_to_point(const Scene &S, const UVGrid &G, V2 uv, V3 *p) {
+    int gx = (int) (uv.x * (float) G.g), gy = (int) (uv.y * (float) G.g);
+    gx = gx < G.g - 1 ? gx : G.g - 1;
+    gy = gy < G.g - 1 ? gy : G.g - 1;
+    const int cell = gy * G.g + gx;
+    for (int i = G.start[cell]; i < G.start[cell + 1]; ++i) {
+        const uint32_t *fc = &S.faces[3 * (size_t) G.faces[i]];
+        V2 t0 = v2(S.uvs[2 * fc[0]], S.uvs[2 * fc[0] + 1]), t1 = v2(S.uvs[2 * fc[1]], S.uvs[2 * fc[1] + 1]),
+           t2 = v2(S.uvs[2 * fc[2]], S.uvs[2 * fc[2] + 1]);
+        V2 e1 = t1 - t0, e2 = t2 - t0, d = uv - t0;
+        float det = e1.x * e2.y - e1.y * e2.x;
+        float b1 = (d.x * e2.y - d.y * e2.x) / det;
+        float b2 = (e1.x * d.y - e1.y * d.x) / det;
+        float b0 = (1.f - b1) - b2;
+        if (b0 >= 0.f && b1 >= 0.f && b2 >= 0.f) {
+            V3 q0 = v3(S.pos[3 * fc[0]], S.pos[3 * fc[0] + 1], S.pos[3 * fc[0] + 2]);
+            V3 q1 = v3(S.pos[3 * fc[1]], S.pos[3 * fc[1] + 1], S.pos[3 * fc[1] + 2]);
+            V3 q2 = v3(S.pos[3 * fc[2]], S.pos[3 * fc[2] + 1], S.pos[3 * fc[2] + 2]);
+            *p = (q0 * b0 + q1 * b1) + q2 * b2;
+            return true;
+        }
+    }
+    return false;
+}
+
+static inline float omin(float a, float b) { return b < a ? b : a; }
+static inline float omax(float a, float b) { return a < b ? b : a; }
+static inline float sigmoid_l(float x) { return 1.f / (1.f + expf_l(-x)); }
+static inline float tanh_l(float x) {
+    float t = expf_l(-2.f * fabsf(x));
+    float r = (1.f - t) / (1.f + t);
+    return x < 0.f ? -r : r;
+}
+static inline float logf_l(float x) { return (float) (log2_d((double) x) * 0.69314718055994530942); }
+
+/* raw outputs (per component: weight logit, mean u, mean v, log sigma u, log sigma v,
+   correlation) -> cdf of the softmax weights, sigmoid means, sigmas exp(clamp(., -7, 0)),
+   correlation 0.95 tanh */
+static void gmm_prepare(const float *o, float *g) {
+    float mx = o[0];
+    for (int c = 1; c < PROP_COMP; ++c) mx = omax(mx, o[6 * c]);
+    float e[PROP_COMP], sum = 0.f;
+    for (int c = 0; c < PROP_COMP; ++c) { e[c] = expf_l(o[6 * c] - mx); sum = sum + e[c]; }
+    float cdf = 0.f;
+    for (int c = 0; c < PROP_COMP; ++c) {
+        cdf = cdf + e[c] / sum;
+        g[c] = cdf;
+        g[8 + c] = sigmoid_l(o[6 * c + 1]);
+        g[16 + c] = sigmoid_l(o[6 * c + 2]);
+        g[24 + c] = expf_l(omin(omax(o[6 * c + 3], -7.f), 0.f));
+        g[32 + c] = expf_l(omin(omax(o[6 * c + 4], -7.f), 0.f));
+        g[40 + c] = 0.95f * tanh_l(o[6 * c + 5]);
+    }
+}
+
+static void proposal_inputs(const Scene &S, V3 p, V3 n, V3 light, float *x) {
+    x[0] = (p.x - S.box_c[0]) * S.box_inv; x[1] = (p.y - S.box_c[1]) * S.box_inv; x[2] = (p.z - S.box_c[2]) * S.box_inv;
+    x[3] = n.x; x[4] = n.y; x[5] = n.z;
+    x[6] = (light.x - S.box_c[0]) * S.box_inv; x[7] = (light.y - S.box_c[1]) * S.box_inv;
+    x[8] = (light.z - S.box_c[2]) * S.box_inv;
+}
+
+/* seed of one walk from its stream: t0 < defensive -> the reference's uniform seed from the
+   next two draws (kind 0, uv = the 2D sample); else component by t1 against the cdf and a
+   Box-Muller draw from (t2, t3) through the Cholesky factor (kind 1) */
+static int proposal_seed(const float *g, float defensive, Pcg32 *rw, V2 *out) {
+    const float t0 = pcg32_float(rw);
+    if (t0 < defensive) { out->x = pcg32_float(rw); out->y = pcg32_float(rw); return 0; }
+    const float t1 = pcg32_float(rw), t2 = pcg32_float(rw), t3 = pcg32_float(rw);
+    int c = PROP_COMP - 1;
+    for (int j = 0; j < PROP_COMP; ++j) if (t1 < g[j]) { c = j; break; }
+    const float r = sqrtf(-2.f * logf_l(1.f - t2));
+    float sn, cs;
+    c_sincosf((2.f * PI_F) * t3, &sn, &cs);
+    const float z1 = r * cs, z2 = r * sn;
+    const float rho = g[40 + c], q = sqrtf(1.f - rho * rho);
+    *out = v2(g[8 + c] + g[24 + c] * z1, g[16 + c] + g[32 + c] * (rho * z1 + q * z2));
+    return 1;
+}
+
+struct Prop { const float *raw; int shape; float defensive; UVGrid grid; };
+
+/* one walk of a query whose mixture is g: uniform seed, or the mixture's uv mapped to the
+   surface and used as p_start; uv outside [0,1)^2 or in no face -> failed walk, 0 iterations */
+static PathOut proposal_walk(const Scene &S, const Prop &P, const float *g, int shape, V3 si_p, const EI &ei, Pcg32 *rw) {
+    V2 s;
+    if (proposal_seed(g, P.defensive, rw, &s) == 0)
+        return sample_path_ss(S, shape, si_p, ei, s, v3(0.f, 0.f, 1.f), v3(0.f, 0.f, 0.f));
+    PathOut po;
+    po.ok = false; po.iterations = 0; po.si_final = si_invalid(); po.si_init = si_invalid();
+    if (!(s.x >= 0.f && s.x < 1.f && s.y >= 0.f && s.y < 1.f)) return po;
+    V3 x1;
+    if (!uv_to_point(S, P.grid, s, &x1)) return po;
+    return sample_path_ss(S, shape, si_p, ei, v2(0.f, 0.f), v3(0.f, 0.f, 1.f), x1);
+}
+
 /* specular_manifold_sampling SS (manifold_ss.cpp:28-201), per-(query, walk) PCG32 streams */
-static V3 estimate_ss(const Scene &S, const sms_ss_query &q, uint64_t seed, uint64_t qid, Stats &st) {
+static V3 estimate_ss(const Scene &S, const sms_ss_query &q, uint64_t seed, uint64_t qid, Stats &st,
+                      const Prop *P = nullptr) {
     const sms_config &cfg = S.cfg;
     SI si = si_from_query(q);
     Pcg32 rq; stream_seed(&rq, seed, qid, 0);
@@ -1251,10 +1406,18 @@ static V3 estimate_ss(const Scene &S, const sms_ss_query &q, uint64_t seed, uint
         int shape = S.casters_ss[c];
         uint64_t wbase = ((uint64_t) c << 32);
         V3 value = v3(0.f, 0.f, 0.f);
+        float gq[PROP_RAW];
+        const bool use_prop = P && shape == P->shape;
+        if (use_prop) gmm_prepare(P->raw + qid * PROP_RAW, gq);
         auto run_walk = [&](uint64_t k) {
             Pcg32 rw; stream_seed(&rw, seed, qid, wbase + k + 1);
-            V2 s; s.x = pcg32_float(&rw); s.y = pcg32_float(&rw);
-            PathOut po = sample_path_ss(S, shape, si.p, ei, s, n_offset, v3(0.f, 0.f, 0.f));
+            PathOut po;
+            if (use_prop) {
+                po = proposal_walk(S, *P, gq, shape, si.p, ei, &rw);
+            } else {
+                V2 s; s.x = pcg32_float(&rw); s.y = pcg32_float(&rw);
+                po = sample_path_ss(S, shape, si.p, ei, s, n_offset, v3(0.f, 0.f, 0.f));
+            }
             st.walks++; st.iters += po.iterations; if (po.ok) st.conv++;
             return po;
         };
@@ -2400,6 +2563,81 @@ int oracle_ss_estimate_batch(void *h, const sms_ss_query *q, int64_t n, uint64_t
     for (auto &s : per) { tot.walks += s.walks; tot.conv += s.conv; tot.iters += s.iters; tot.bcalls += s.bcalls;
         tot.btrials += s.btrials; if (s.bmax > tot.bmax) tot.bmax = s.bmax; }
     stats_out(tot, stats);
+    return 0;
+}
+
+/* estimator with the learned proposal on caster `shape`: raw = n x 48 raw network outputs */
+int oracle_ss_estimate_batch_prop(void *h, const sms_ss_query *q, int64_t n, uint64_t seed, const float *raw, int shape,
+                                  float defensive, float *out_rgb, sms_stats *stats, int threads) {
+    const Scene &S = *(Scene *) h;
+    if (shape < 0 || shape >= (int) S.shapes.size() || S.shapes[shape].type != SMS_SHAPE_MESH) return -1;
+    Prop P;
+    P.raw = raw; P.shape = shape; P.defensive = defensive;
+    build_uv_grid(S, shape, P.grid);
+    std::vector<Stats> per((size_t) n);
+    parallel_for(n, threads, [&](int64_t i) {
+        Stats st; memset(&st, 0, sizeof(st));
+        V3 r = estimate_ss(S, q[i], seed, (uint64_t) i, st, &P);
+        out_rgb[3 * i] = r.x; out_rgb[3 * i + 1] = r.y; out_rgb[3 * i + 2] = r.z;
+        per[(size_t) i] = st;
+    });
+    Stats tot; memset(&tot, 0, sizeof(tot));
+    for (auto &s : per) { tot.walks += s.walks; tot.conv += s.conv; tot.iters += s.iters; tot.bcalls += s.bcalls;
+        tot.btrials += s.btrials; if (s.bmax > tot.bmax) tot.bmax = s.bmax; }
+    stats_out(tot, stats);
+    return 0;
+}
+
+/* network inputs of the queries of an estimate (same light sample as the estimator) */
+void oracle_prop_inputs(void *h, const sms_ss_query *q, int64_t n, uint64_t seed, float *x) {
+    const Scene &S = *(Scene *) h;
+    for (int64_t i = 0; i < n; ++i) {
+        SI si = si_from_query(q[i]);
+        Pcg32 rq; stream_seed(&rq, seed, (uint64_t) i, 0);
+        EI ei = sample_emitter_interaction(S, si.p, S.emitters_ss, &rq);
+        proposal_inputs(S, si.p, si.sh_n, ei.p, x + 9 * i);
+    }
+}
+
+void oracle_scene_box(void *h, float *c, float *inv) {
+    const Scene &S = *(Scene *) h;
+    for (int a = 0; a < 3; ++a) c[a] = S.box_c[a];
+    *inv = S.box_inv;
+}
+
+void oracle_gmm_prepare(const float *raw, int64_t n, float *g) {
+    for (int64_t i = 0; i < n; ++i) gmm_prepare(raw + PROP_RAW * i, g + PROP_RAW * i);
+}
+
+/* seed draw of walk `walk` of query `qid`: out = (kind, u, v) per record */
+void oracle_prop_seed(const float *g, float defensive, uint64_t seed, uint64_t qid, uint64_t walk0, int64_t n,
+                      float *out) {
+    for (int64_t i = 0; i < n; ++i) {
+        Pcg32 rw; stream_seed(&rw, seed, qid, walk0 + (uint64_t) i);
+        V2 s;
+        int kind = proposal_seed(g, defensive, &rw, &s);
+        out[3 * i] = (float) kind; out[3 * i + 1] = s.x; out[3 * i + 2] = s.y;
+    }
+}
+
+/* uv -> surface point through the proposal's grid of mesh `shape`; ok[i] = 0 when no face
+   holds uv; brute = 1 scans every face of the mesh in order instead of the grid */
+int oracle_uv_to_point(void *h, int shape, const float *uv, int64_t n, int brute, float *p, int32_t *ok) {
+    const Scene &S = *(Scene *) h;
+    if (shape < 0 || shape >= (int) S.shapes.size() || S.shapes[shape].type != SMS_SHAPE_MESH) return -1;
+    UVGrid G;
+    if (brute) {
+        const Shape &sh = S.shapes[shape];
+        G.g = 1; G.start = {0, sh.nf};
+        for (int f = 0; f < sh.nf; ++f) G.faces.push_back(sh.fbase + f);
+    } else {
+        build_uv_grid(S, shape, G);
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        V3 x = v3(0.f, 0.f, 0.f);
+        ok[i] = uv_to_point(S, G, v2(uv[2 * i], uv[2 * i + 1]), &x) ? 1 : 0;
+        p[3 * i] = x.x; p[3 * i + 1] = x.y; p[3 * i + 2] = x.z;
+    }
     return 0;
 }
 

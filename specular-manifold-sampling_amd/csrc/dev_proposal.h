@@ -1,0 +1,118 @@
+// dev_proposal.h -- learned seed proposal on the walk side: network inputs of a query, the
+// mixture prepared from the network's raw outputs, the defensive-mixture seed of a walk, and
+// the UV -> surface-point lookup of the proposal's mesh caster.
+//
+// Everything a seed depends on is computed with operations that round identically on the host
+// (exp through the FP64 series of dev_lean.h, Cephes sincos, IEEE sqrt / division), so a walk
+// seeded from the same raw network outputs is bit-identical on every device and on the CPU.
+#pragma once
+#include "dev_lean.h"
+#include "dev_ss.h"
+
+namespace smsdev {
+
+constexpr int PROP_COMP = 8;
+constexpr int PROP_RAW = 48;
+
+// Uniform grid over a mesh's UV square [0,1)^2: cell (x, y) lists, in face order, the faces
+// whose UV bounding box overlaps it (CSR: start[G*G + 1], faces = global face indices).
+struct DUVGrid { int g; const int *start; const int *faces; };
+
+struct Proposal {
+    const float *gmm;                 // per query: cdf[8], mu_u[8], mu_v[8], sx[8], sy[8], rho[8]
+    int shape;                        // mesh caster the proposal seeds (-1: none)
+    float defensive;                  // weight of the reference's uniform seed
+    DUVGrid grid;
+    // capture of (inputs, solution uv) of converged first walks
+    float *cap;                       // SPREAD sub-lists of cap_sub records of 12 floats
+    unsigned long long *cap_counts;   // SPREAD counters, SPREAD_STRIDE apart
+    unsigned long long cap_sub;
+};
+
+SMS_DEV float sigmoid_l(float x) { return 1.f / (1.f + expf_l(-x)); }
+SMS_DEV float tanh_l(float x) {
+    float t = expf_l(-2.f * fabsf(x));
+    float r = (1.f - t) / (1.f + t);
+    return x < 0.f ? -r : r;
+}
+SMS_DEV float logf_l(float x) { return (float) (log2_d((double) x) * 0.69314718055994530942); }
+
+// raw network outputs -> mixture parameters used by the seeds
+SMS_DEV void gmm_prepare(const float *o, float *g) {
+    float mx = o[0];
+    for (int c = 1; c < PROP_COMP; ++c) mx = fmax_(mx, o[6 * c]);
+    float e[PROP_COMP], sum = 0.f;
+    for (int c = 0; c < PROP_COMP; ++c) { e[c] = expf_l(o[6 * c] - mx); sum = sum + e[c]; }
+    float cdf = 0.f;
+    for (int c = 0; c < PROP_COMP; ++c) {
+        cdf = cdf + e[c] / sum;
+        g[c] = cdf;
+        g[8 + c] = sigmoid_l(o[6 * c + 1]);
+        g[16 + c] = sigmoid_l(o[6 * c + 2]);
+        g[24 + c] = expf_l(fmin_(fmax_(o[6 * c + 3], -7.f), 0.f));
+        g[32 + c] = expf_l(fmin_(fmax_(o[6 * c + 4], -7.f), 0.f));
+        g[40 + c] = 0.95f * tanh_l(o[6 * c + 5]);
+    }
+}
+
+// network inputs of a query: shading point and light sample normalised to the scene box,
+// shading normal as is
+SMS_DEV void proposal_inputs(const DScene &S, V3 p, V3 n, V3 light, float *x) {
+    x[0] = (p.x - S.box_c[0]) * S.box_inv; x[1] = (p.y - S.box_c[1]) * S.box_inv; x[2] = (p.z - S.box_c[2]) * S.box_inv;
+    x[3] = n.x; x[4] = n.y; x[5] = n.z;
+    x[6] = (light.x - S.box_c[0]) * S.box_inv; x[7] = (light.y - S.box_c[1]) * S.box_inv;
+    x[8] = (light.z - S.box_c[2]) * S.box_inv;
+}
+
+// point of the mesh whose uv is `uv` (first face in grid order that contains it)
+SMS_DEV bool uv_to_point(const DScene &S, const DUVGrid &G, V2 uv, V3 &p) {
+    int gx = (int) (uv.x * (float) G.g), gy = (int) (uv.y * (float) G.g);
+    gx = gx < G.g - 1 ? gx : G.g - 1;
+    gy = gy < G.g - 1 ? gy : G.g - 1;
+    const int cell = gy * G.g + gx;
+    for (int i = G.start[cell]; i < G.start[cell + 1]; ++i) {
+        const uint4 f = S.faces[G.faces[i]];
+        const float2 t0 = S.vuv[f.x], t1 = S.vuv[f.y], t2 = S.vuv[f.z];
+        V2 e1 = v2(t1.x - t0.x, t1.y - t0.y), e2 = v2(t2.x - t0.x, t2.y - t0.y), d = v2(uv.x - t0.x, uv.y - t0.y);
+        float det = e1.x * e2.y - e1.y * e2.x;
+        float b1 = (d.x * e2.y - d.y * e2.x) / det;
+        float b2 = (e1.x * d.y - e1.y * d.x) / det;
+        float b0 = (1.f - b1) - b2;
+        if (b0 >= 0.f && b1 >= 0.f && b2 >= 0.f) {
+            const float4 q0 = S.vpos[f.x], q1 = S.vpos[f.y], q2 = S.vpos[f.z];
+            p = (v3(q0.x, q0.y, q0.z) * b0 + v3(q1.x, q1.y, q1.z) * b1) + v3(q2.x, q2.y, q2.z) * b2;
+            return true;
+        }
+    }
+    return false;
+}
+
+// One single-scatter walk seeded from the defensive mixture.  Draws from the walk's stream:
+// t0 chooses the component (t0 < defensive: the reference's uniform seed from (t1, t2));
+// otherwise t1 picks the Gaussian, (t2, t3) give it a Box-Muller sample.
+SMS_DEV PathResult proposal_walk(const DScene &S, const Proposal &P, const float *g, int shape, V3 si_p, const EI &ei,
+                                 Pcg32 &rw) {
+    const float t0 = pcg32_float(rw);
+    if (t0 < P.defensive) {
+        V2 s; s.x = pcg32_float(rw); s.y = pcg32_float(rw);
+        return sample_path_ss(S, shape, si_p, ei, s, v3(0.f, 0.f, 1.f), v3(0.f, 0.f, 0.f));
+    }
+    const float t1 = pcg32_float(rw), t2 = pcg32_float(rw), t3 = pcg32_float(rw);
+    int c = PROP_COMP - 1;
+    for (int j = 0; j < PROP_COMP; ++j) if (t1 < g[j]) { c = j; break; }
+    const float r = sqrtf(-2.f * logf_l(1.f - t2));
+    float sn, cs;
+    c_sincosf((2.f * PI_F) * t3, &sn, &cs);
+    const float z1 = r * cs, z2 = r * sn;
+    const float rho = g[40 + c], q = sqrtf(1.f - rho * rho);
+    V2 uv = v2(g[8 + c] + g[24 + c] * z1, g[16 + c] + g[32 + c] * (rho * z1 + q * z2));
+    PathResult res;
+    res.ok = false; res.init_valid = false; res.iterations = 0;
+    res.si_final = si_invalid(); res.p_init = v3(0.f, 0.f, 0.f);
+    if (!(uv.x >= 0.f && uv.x < 1.f && uv.y >= 0.f && uv.y < 1.f)) return res;
+    V3 x1;
+    if (!uv_to_point(S, P.grid, uv, x1)) return res;
+    return sample_path_ss(S, shape, si_p, ei, v2(0.f, 0.f), v3(0.f, 0.f, 1.f), x1);
+}
+
+} // namespace smsdev

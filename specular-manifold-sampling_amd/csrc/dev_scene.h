@@ -70,6 +70,7 @@ struct DScene {
     const float4 *tris;                  // leaf-ordered triangle records (3 float4 each)
     const float *cdf, *pmf;              // mesh area distributions
     DCamera cam; int has_cam;
+    float box_c[3], box_inv;             // scene bounding box centre, 1 / largest half extent
     sms_config cfg;
 };
 

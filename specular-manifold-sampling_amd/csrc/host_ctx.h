@@ -1,0 +1,77 @@
+// host_ctx.h -- host-side state of an SMS context, shared by the library's host sources.
+#pragma once
+#include "kernels.h"
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+using namespace smsdev;
+
+struct sms_mlp;
+
+struct sms_ctx {
+    int device = 0;
+    int n_cu = 0;
+    char dev_name[256] = {0};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+    float last_ms = 0.f, last_ms2 = 0.f;
+    std::string err;
+    DScene scene;
+    std::vector<void *> scene_allocs;
+    // estimator work buffers
+    unsigned long long *counters = nullptr;     // [0] cursor [1] n_overflow [2] coop cursor [8..15] stats
+    OverflowRec *overflow = nullptr; size_t overflow_cap = 0;
+    float *part = nullptr; float *invp = nullptr; size_t part_cap = 0;
+    TrialRec *trial = nullptr; GlintRes *res = nullptr; size_t trial_cap = 0, sub_cap = 0;
+    OverflowRec *gover = nullptr; size_t gover_cap = 0, over_sub = 0;   // spread overflow list
+    int lane_trials = 8;
+    int blocks_per_cu = 0;
+    int trial_group = 8;
+    int tail_rounds = 2;
+    int long_chains = 1024, long_trials = 32768;
+    // glint tail lists and the multi-wavefront stage (allocated with the glint work buffers)
+    OverflowRec *long_items = nullptr; OverflowRec *seq_items = nullptr;
+    long long *mw_found = nullptr; unsigned long long *mw_next = nullptr; uint16_t *mw_log = nullptr;
+    int mw_items = 0, mw_trials = 0;
+    int occ_glint = 0, occ_ss = 0, occ_coop = 0, occ_biased = 0, occ_sscoop = 0, occ_mw = 0;   // occ_glint: trial pass
+    int occ_ms[2] = {0, 0}, occ_mscoop[2] = {0, 0}, occ_msbiased[2] = {0, 0};   // [0]: chains <= 2, [1]: <= 4
+    // host copies the learned proposal's UV grid is built from
+    std::vector<float2> h_vuv;
+    std::vector<uint4> h_faces;
+    std::vector<DShape> h_shapes;
+    // learned seed proposal (single scatter)
+    sms_mlp *prop_mlp = nullptr;
+    int prop_shape = -1;
+    float prop_defensive = 1.f;
+    int grid_g = 0, grid_shape = -1;
+    int *grid_start = nullptr, *grid_faces = nullptr;
+    float *prop_x = nullptr, *prop_raw = nullptr, *prop_g = nullptr; size_t prop_cap = 0;
+    // capture of (inputs, solution uv) of converged first walks
+    float *cap = nullptr; unsigned long long *cap_counts = nullptr; size_t cap_sub = 0;
+};
+
+
+namespace smshost {
+
+inline sms_status fail(sms_ctx *ctx, sms_status code, const std::string &msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+#define HIP_TRY(ctx, call)                                                                      \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return fail(ctx, e_ == hipErrorOutOfMemory ? SMS_E_OOM : SMS_E_HIP,                 \
+                        std::string(#call) + ": " + hipGetErrorString(e_));                     \
+    } while (0)
+
+} // namespace smshost
+
+// learned seed proposal (sms_mlp_host.hip)
+Proposal proposal_of(sms_ctx *ctx);
+sms_status proposal_prepare(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed);
+void proposal_release(sms_ctx *ctx);

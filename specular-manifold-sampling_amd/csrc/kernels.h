@@ -4,6 +4,7 @@
 #include "dev_ss.h"
 #include "dev_glint.h"
 #include "dev_lean.h"
+#include "dev_proposal.h"
 
 namespace smsdev {
 
@@ -112,6 +113,7 @@ struct EstimatorWork {
     uint16_t *mw_log;                 // per trial: iterations | converged << 15
     int mw_items;                     // long items the stage holds (the rest go to seq_list)
     int mw_trials;                    // trials per long item the stage may run
+    Proposal prop;                    // single scatter: learned seed proposal (prop.gmm null: none)
 };
 
 SMS_DEV void write_rgb(float *out, int64_t q, V3 v) { out[3 * q] = v.x; out[3 * q + 1] = v.y; out[3 * q + 2] = v.z; }
@@ -148,6 +150,15 @@ __global__ void k_glint_finalize(DScene S, const sms_glint_query *Q, uint64_t se
 __global__ void k_ss_estimate(DScene S, const sms_ss_query *Q, int64_t n, uint64_t seed, float *part, float *invp, EstimatorWork W);
 __global__ void k_ss_coop(DScene S, const sms_ss_query *Q, uint64_t seed, float *invp, EstimatorWork W);
 __global__ void k_ss_finalize(const float *part, const float *invp, int nc, int64_t n, float *out);
+__global__ void k_prop_inputs(DScene S, const sms_ss_query *Q, int64_t n, uint64_t seed, float *x);
+__global__ void k_gmm_prepare(const float *raw, int64_t n, float *g);
+__global__ void k_capture_gather(const float *cap, const unsigned long long *counts, unsigned long long cap_sub,
+                                 float *x, float *uv);
+__global__ void k_mlp_infer(const float *p, const uint16_t *wb, const float *x9, int64_t n, float *out);
+__global__ void k_mlp_grad(const float *p, const uint16_t *wb, const float *x9, const float *uv, int64_t n, float scale,
+                           float *grad, float *loss_sum);
+__global__ void k_mlp_adam(float *p, float *g, float *m, float *v, uint16_t *wb, int n, float lr, float b1, float b2,
+                           float eps, float bc1, float bc2);
 __global__ void k_ss_biased(DScene S, const sms_ss_query *Q, int64_t n, uint64_t seed, float *out, EstimatorWork W);
 #define SMS_MS_KERNEL_DECLS(B)                                                                                   \
     __global__ void k_ms_walk_##B(DScene S, const sms_ms_walk_in *in, int64_t n, sms_ms_walk_out *out);         \

@@ -4,7 +4,7 @@
 // One context lives on the HIP device that is current when sms_ctx_create is called and
 // owns one stream (default priority).  Estimator calls run two persistent passes on that
 // stream; their work-queue buffers grow on demand and are reused across calls.
-#include "kernels.h"
+#include "host_ctx.h"
 
 #include <hip/hip_runtime.h>
 
@@ -17,50 +17,10 @@
 #include <vector>
 
 using namespace smsdev;
+using namespace smshost;
 
-struct sms_ctx {
-    int device = 0;
-    int n_cu = 0;
-    char dev_name[256] = {0};
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
-    float last_ms = 0.f, last_ms2 = 0.f;
-    std::string err;
-    DScene scene;
-    std::vector<void *> scene_allocs;
-    // estimator work buffers
-    unsigned long long *counters = nullptr;     // [0] cursor [1] n_overflow [2] coop cursor [8..15] stats
-    OverflowRec *overflow = nullptr; size_t overflow_cap = 0;
-    float *part = nullptr; float *invp = nullptr; size_t part_cap = 0;
-    TrialRec *trial = nullptr; GlintRes *res = nullptr; size_t trial_cap = 0, sub_cap = 0;
-    OverflowRec *gover = nullptr; size_t gover_cap = 0, over_sub = 0;   // spread overflow list
-    int lane_trials = 8;
-    int blocks_per_cu = 0;
-    int trial_group = 8;
-    int tail_rounds = 2;
-    int long_chains = 1024, long_trials = 32768;
-    // glint tail lists and the multi-wavefront stage (allocated with the glint work buffers)
-    OverflowRec *long_items = nullptr; OverflowRec *seq_items = nullptr;
-    long long *mw_found = nullptr; unsigned long long *mw_next = nullptr; uint16_t *mw_log = nullptr;
-    int mw_items = 0, mw_trials = 0;
-    int occ_glint = 0, occ_ss = 0, occ_coop = 0, occ_biased = 0, occ_sscoop = 0, occ_mw = 0;   // occ_glint: trial pass
-    int occ_ms[2] = {0, 0}, occ_mscoop[2] = {0, 0}, occ_msbiased[2] = {0, 0};   // [0]: chains <= 2, [1]: <= 4
-};
 
 namespace {
-
-sms_status fail(sms_ctx *ctx, sms_status code, const std::string &msg) {
-    if (ctx) ctx->err = msg;
-    return code;
-}
-
-#define HIP_TRY(ctx, call)                                                                      \
-    do {                                                                                        \
-        hipError_t e_ = (call);                                                                 \
-        if (e_ != hipSuccess)                                                                   \
-            return fail(ctx, e_ == hipErrorOutOfMemory ? SMS_E_OOM : SMS_E_HIP,                 \
-                        std::string(#call) + ": " + hipGetErrorString(e_));                     \
-    } while (0)
 
 template <typename T>
 sms_status upload(sms_ctx *ctx, const std::vector<T> &v, const T **dst) {
@@ -430,6 +390,28 @@ sms_status build_scene(sms_ctx *ctx, const sms_scene_desc *desc, const sms_confi
         tris[3 * k + 1] = make_float4(p1.x - p0.x, p1.y - p0.y, p1.z - p0.z, i2f(prim));
         tris[3 * k + 2] = make_float4(p2.x - p0.x, p2.y - p0.y, p2.z - p0.z, 0.f);
     }
+    // scene bounding box (positions of the learned proposal's inputs are normalised to it)
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    auto grow = [&](float x, float y, float z) {
+        const float v[3] = {x, y, z};
+        for (int a = 0; a < 3; ++a) { lo[a] = std::min(lo[a], v[a]); hi[a] = std::max(hi[a], v[a]); }
+    };
+    for (const float4 &v : vpos) grow(v.x, v.y, v.z);
+    for (const DShape &sh : shapes) {
+        if (sh.type == SMS_SHAPE_SPHERE) {
+            grow(sh.cx - sh.radius, sh.cy - sh.radius, sh.cz - sh.radius);
+            grow(sh.cx + sh.radius, sh.cy + sh.radius, sh.cz + sh.radius);
+        } else if (sh.type == SMS_SHAPE_RECTANGLE) {
+            for (int cy = -1; cy <= 1; cy += 2)
+                for (int cx = -1; cx <= 1; cx += 2)
+                    grow(sh.M[0] * cx + sh.M[1] * cy + sh.M[3], sh.M[4] * cx + sh.M[5] * cy + sh.M[7],
+                         sh.M[8] * cx + sh.M[9] * cy + sh.M[11]);
+        }
+    }
+    float half = 0.f;
+    for (int a = 0; a < 3; ++a) { S.box_c[a] = 0.5f * (lo[a] + hi[a]); half = std::max(half, 0.5f * (hi[a] - lo[a])); }
+    S.box_inv = half > 0.f ? 1.f / half : 1.f;
+    ctx->h_vuv = vuv; ctx->h_faces = faces; ctx->h_shapes = shapes;
     sms_status st;
     if ((st = upload(ctx, shapes, &S.shapes))) return st;
     S.n_shapes = (int) shapes.size();
@@ -594,6 +576,7 @@ EstimatorWork work_of(sms_ctx *ctx) {
     W.seq_list.items = ctx->seq_items; W.seq_list.count = c + 6; W.seq_list.cursor = c + 7;
     W.mw_found = ctx->mw_found; W.mw_next = ctx->mw_next; W.mw_log = ctx->mw_log;
     W.mw_items = ctx->mw_items; W.mw_trials = ctx->mw_trials;
+    W.prop = proposal_of(ctx);
     return W;
 }
 
@@ -706,6 +689,7 @@ sms_status sms_ctx_destroy(sms_ctx *ctx) {
     if (ctx->mw_log) (void) hipFree(ctx->mw_log);
     if (ctx->gover) (void) hipFree(ctx->gover);
     if (ctx->res) (void) hipFree(ctx->res);
+    proposal_release(ctx);
     if (ctx->ev3) (void) hipEventDestroy(ctx->ev3);
     if (ctx->ev0) (void) hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void) hipEventDestroy(ctx->ev1);
@@ -905,6 +889,8 @@ sms_status sms_ss_estimate_batch(sms_ctx *ctx, const sms_ss_query *q, int64_t n,
     hipLaunchKernelGGL(k_zero_u64, dim3((N_COUNTERS + 255) / 256), dim3(256), 0, ctx->stream, ctx->counters, N_COUNTERS);
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     if (n > 0) {
+        st = proposal_prepare(ctx, q, n, seed);
+        if (st) return st;
         EstimatorWork W = work_of(ctx);
         if (cfg.biased) {
             hipLaunchKernelGGL(k_ss_biased, dim3(grid_for(ctx, ctx->occ_biased)), dim3(256), 0, ctx->stream,

@@ -131,11 +131,23 @@ SMS_DEV bool ss_visible(const DScene &S, const EI &ei, V3 p) {
     return !ray_test(S, make_ray(ei.p, d, RAY_EPS_F * (1.f + hmax_abs(ei.p)), dist * (1.f - RAY_EPS_F)));
 }
 
-SMS_DEV PathResult ss_walk(const DScene &S, uint64_t qkey, int c, int shape, uint64_t k, V3 si_p, const EI &ei) {
+SMS_DEV PathResult ss_walk(const DScene &S, const Proposal &P, int64_t q, uint64_t qkey, int c, int shape, uint64_t k,
+                           V3 si_p, const EI &ei) {
     Pcg32 rw;
     stream_seed(rw, qkey, ((uint64_t) c << 32) + k + 1);
+    if (P.gmm && shape == P.shape) return proposal_walk(S, P, P.gmm + q * PROP_RAW, shape, si_p, ei, rw);
     V2 s; s.x = pcg32_float(rw); s.y = pcg32_float(rw);
     return sample_path_ss(S, shape, si_p, ei, s, v3(0.f, 0.f, 1.f), v3(0.f, 0.f, 0.f));
+}
+
+// record (network inputs, solution uv) of a converged first walk on the proposal's caster
+SMS_DEV void capture_sample(const DScene &S, const Proposal &P, const SI &si, const EI &ei, V2 uv) {
+    const int sub = (int) (blockIdx.x % SPREAD);
+    unsigned long long pos = atomicAdd(P.cap_counts + sub * SPREAD_STRIDE, 1ull);
+    if (pos >= P.cap_sub) return;
+    float *r = P.cap + ((size_t) sub * P.cap_sub + pos) * 12;
+    proposal_inputs(S, si.p, si.sh_n, ei.p, r);
+    r[9] = uv.x; r[10] = uv.y; r[11] = 0.f;
 }
 
 // Pass 1: lane per query; for every caster the first walk, visibility, contribution and up
@@ -162,12 +174,13 @@ __global__ void __launch_bounds__(256) k_ss_estimate(DScene S, const sms_ss_quer
             part[3 * slot] = 0.f; part[3 * slot + 1] = 0.f; part[3 * slot + 2] = 0.f;
             invp[slot] = 0.f;
             int shape = S.casters_ss[c];
-            PathResult r = ss_walk(S, qkey, c, shape, 0, si.p, ei);
+            PathResult r = ss_walk(S, W.prop, q, qkey, c, shape, 0, si.p, ei);
             st_walks++; st_iters += (unsigned long long) r.iterations;
             if (!r.ok) continue;
             st_conv++;
             V3 direction = normalize(r.si_final.p - si.p);
             if (!ss_visible(S, ei, r.si_final.p)) continue;
+            if (W.prop.cap && shape == W.prop.shape) capture_sample(S, W.prop, si, ei, r.si_final.uv);
             V3 spec = path_contribution_ss(S, si, ei, r.si_final);
             V3 dl = v3(dot(direction, si.sh_s), dot(direction, si.sh_t), dot(direction, si.sh_n));
             V3 bsdf = receiver_bsdf_eval(S, si, wi_local, dl);
@@ -176,7 +189,7 @@ __global__ void __launch_bounds__(256) k_ss_estimate(DScene S, const sms_ss_quer
             int64_t k = 1;
             float inv_prob = -1.f;
             while (true) {
-                PathResult t = ss_walk(S, qkey, c, shape, (uint64_t) k, si.p, ei);
+                PathResult t = ss_walk(S, W.prop, q, qkey, c, shape, (uint64_t) k, si.p, ei);
                 st_walks++; st_iters += (unsigned long long) t.iterations;
                 if (t.ok) st_conv++;
                 V3 dtr = normalize(t.si_final.p - si.p);
@@ -229,7 +242,7 @@ __global__ void __launch_bounds__(256) k_ss_coop(DScene S, const sms_ss_query *_
             bool match = false, ran = false, conv = false;
             int its = 0;
             if (k <= kcap) {
-                PathResult t = ss_walk(S, qkey, c, shape, (uint64_t) k, si.p, ei);
+                PathResult t = ss_walk(S, W.prop, q, qkey, c, shape, (uint64_t) k, si.p, ei);
                 ran = true; its = t.iterations; conv = t.ok;
                 V3 dtr = normalize(t.si_final.p - si.p);
                 match = t.ok && fabsf(dot(direction, dtr) - 1.f) < cfg.uniqueness_threshold;
@@ -298,7 +311,7 @@ __global__ void __launch_bounds__(256) k_ss_biased(DScene S, const sms_ss_query 
                 PathResult r;
                 r.ok = false;
                 if (m < cfg.max_trials) {
-                    r = ss_walk(S, qkey, c, shape, (uint64_t) m, si.p, ei);
+                    r = ss_walk(S, W.prop, q, qkey, c, shape, (uint64_t) m, si.p, ei);
                     st_walks++; st_iters += (unsigned long long) r.iterations;
                     if (r.ok) {
                         st_conv++;
@@ -350,6 +363,51 @@ __global__ void __launch_bounds__(256) k_ss_biased(DScene S, const sms_ss_query 
         if (lane == 0) write_rgb(out, q, result);
     }
     stat_add(W.stats + 0, st_walks); stat_add(W.stats + 1, st_conv); stat_add(W.stats + 2, st_iters);
+}
+
+// ================================================================ learned seed proposal
+// network inputs of every query of an estimate: the shading point, its shading normal and the
+// light sample the estimator draws for it (same stream, same arithmetic as k_ss_estimate)
+__global__ void k_prop_inputs(DScene S, const sms_ss_query *__restrict__ Q, int64_t n, uint64_t seed,
+                              float *__restrict__ x) {
+    const int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const sms_ss_query g = Q[q];
+    SI si = ss_query_si(g);
+    Pcg32 rq;
+    stream_seed(rq, query_key(seed, (uint64_t) q), 0);
+    EI ei = sample_emitter_interaction(S, si.p, S.emitters_ss, S.n_emitters_ss, rq);
+    float r[9];
+    proposal_inputs(S, si.p, si.sh_n, ei.p, r);
+    for (int j = 0; j < 9; ++j) x[q * 9 + j] = r[j];
+}
+
+__global__ void k_gmm_prepare(const float *__restrict__ raw, int64_t n, float *__restrict__ g) {
+    const int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    float o[PROP_RAW], r[PROP_RAW];
+    for (int j = 0; j < PROP_RAW; ++j) o[j] = raw[q * PROP_RAW + j];
+    gmm_prepare(o, r);
+    for (int j = 0; j < PROP_RAW; ++j) g[q * PROP_RAW + j] = r[j];
+}
+
+// compact the capture sub-lists (sub s: min(count_s, cap_sub) records) into x (n x 9) and
+// uv (n x 2), sub-lists in order; one block per sub-list
+__global__ void k_capture_gather(const float *__restrict__ cap, const unsigned long long *__restrict__ counts,
+                                 unsigned long long cap_sub, float *__restrict__ x, float *__restrict__ uv) {
+    const int sub = blockIdx.x;
+    unsigned long long off = 0;
+    for (int s = 0; s < sub; ++s) {
+        const unsigned long long c = counts[s * SPREAD_STRIDE];
+        off += c < cap_sub ? c : cap_sub;
+    }
+    const unsigned long long c0 = counts[sub * SPREAD_STRIDE], cnt = c0 < cap_sub ? c0 : cap_sub;
+    for (unsigned long long i = threadIdx.x; i < cnt; i += blockDim.x) {
+        const float *r = cap + ((size_t) sub * cap_sub + i) * 12;
+        for (int j = 0; j < 9; ++j) x[(off + i) * 9 + j] = r[j];
+        uv[(off + i) * 2] = r[9];
+        uv[(off + i) * 2 + 1] = r[10];
+    }
 }
 
 } // namespace smsdev

@@ -1,0 +1,370 @@
+// sms_mlp_host.hip -- C ABI of the learned seed proposal (include/sms.h): network parameters,
+// training steps, the proposal's UV grid, the per-estimate seed preparation and the capture
+// of training samples.  No counterpart in the reference, whose seeds are uniform over the
+// caster's area (src/librender/manifold_ss.cpp:228-235).
+#include "host_ctx.h"
+#include "dev_mlp.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+using namespace smshost;
+
+struct sms_mlp {
+    sms_ctx *ctx = nullptr;
+    float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr, *loss = nullptr;   // device, FP32
+    uint16_t *wb = nullptr;                                                          // device, BF16 copy of p
+    int64_t step = 0;                                                                // Adam steps taken
+};
+
+namespace {
+
+static_assert(MLP_PARAMS == SMS_MLP_PARAMS, "parameter count of the ABI and the kernels differ");
+static_assert(MLP_IN == SMS_MLP_INPUTS && MLP_OUT == SMS_MLP_OUTPUTS, "network shape of the ABI and the kernels differ");
+static_assert(PROP_RAW == MLP_OUT, "mixture layout");
+
+uint16_t host_bf16(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t) ((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t) (u >> 16);
+}
+
+// splitmix64 stream for the initial weights
+struct InitRng {
+    uint64_t s;
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return ((double) (next() >> 11) + 0.5) * (1.0 / 9007199254740992.0); }
+    double normal() {
+        const double u1 = uniform(), u2 = uniform();
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    }
+};
+
+// He-normal weights (std sqrt(2 / fan_in)); the output layer scaled by 0.1; zero biases
+std::vector<float> init_params(uint64_t seed) {
+    std::vector<float> p(MLP_PARAMS, 0.f);
+    InitRng r{seed ^ 0x5eed5eed12345678ULL};
+    auto layer = [&](int w_off, int n, int k_used, int k_stride, double gain) {
+        const double sd = gain * std::sqrt(2.0 / (double) k_used);
+        for (int o = 0; o < n; ++o)
+            for (int k = 0; k < k_used; ++k) p[(size_t) w_off + (size_t) o * k_stride + k] = (float) (sd * r.normal());
+    };
+    layer(MLP_W1, MLP_H, MLP_FEAT, MLP_K1, 1.0);
+    layer(MLP_W2, MLP_H, MLP_H, MLP_H, 1.0);
+    layer(MLP_W3, MLP_H, MLP_H, MLP_H, 1.0);
+    layer(MLP_W4, MLP_H, MLP_H, MLP_H, 1.0);
+    layer(MLP_W5, MLP_OUT, MLP_H, MLP_H, 0.1);
+    return p;
+}
+
+sms_status upload_params(sms_mlp *mlp, const float *h) {
+    sms_ctx *ctx = mlp->ctx;
+    std::vector<uint16_t> b(MLP_PARAMS);
+    for (int i = 0; i < MLP_PARAMS; ++i) b[i] = host_bf16(h[i]);
+    HIP_TRY(ctx, hipMemcpyAsync(mlp->p, h, MLP_PARAMS * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(mlp->wb, b.data(), MLP_PARAMS * sizeof(uint16_t), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return SMS_OK;
+}
+
+void free_mlp(sms_mlp *mlp) {
+    for (void *q : {(void *) mlp->p, (void *) mlp->g, (void *) mlp->m, (void *) mlp->v, (void *) mlp->loss, (void *) mlp->wb})
+        if (q) (void) hipFree(q);
+    delete mlp;
+}
+
+sms_status launch_infer(sms_mlp *mlp, const float *x, int64_t n, float *out) {
+    sms_ctx *ctx = mlp->ctx;
+    if (n == 0) return SMS_OK;
+    const int64_t tiles = (n + MLP_TILE - 1) / MLP_TILE;
+    const unsigned grid = (unsigned) std::min<int64_t>(tiles, (int64_t) ctx->n_cu * 2);
+    hipLaunchKernelGGL(k_mlp_infer, dim3(grid), dim3(256), 0, ctx->stream, mlp->p, mlp->wb, x, n, out);
+    HIP_TRY(ctx, hipGetLastError());
+    return SMS_OK;
+}
+
+sms_status ensure_prop(sms_ctx *ctx, size_t n) {
+    if (n <= ctx->prop_cap) return SMS_OK;
+    const size_t cap = std::max<size_t>(n, 2 * ctx->prop_cap);
+    for (float **b : {&ctx->prop_x, &ctx->prop_raw, &ctx->prop_g})
+        if (*b) { (void) hipFree(*b); *b = nullptr; }
+    ctx->prop_cap = 0;
+    HIP_TRY(ctx, hipMalloc(&ctx->prop_x, cap * MLP_IN * sizeof(float)));
+    HIP_TRY(ctx, hipMalloc(&ctx->prop_raw, cap * MLP_OUT * sizeof(float)));
+    HIP_TRY(ctx, hipMalloc(&ctx->prop_g, cap * MLP_OUT * sizeof(float)));
+    ctx->prop_cap = cap;
+    return SMS_OK;
+}
+
+// Uniform grid over the UV square of mesh `shape`: G x G cells (G the power of two with
+// 2 G^2 >= faces, at most 1024); cell (x, y) lists, in ascending face order, every face whose
+// UV bounding box [lo, hi] has floor(lo * G) <= x <= floor(hi * G) (clamped to the grid) and the
+// same along y.  G is a power of two, so uv * G is exact and a point of a face's UV triangle
+// always falls in a cell that lists the face.
+sms_status build_uv_grid(sms_ctx *ctx, int shape) {
+    const DShape &sh = ctx->h_shapes[shape];
+    int G = 1;
+    while (2 * (int64_t) G * G < sh.nf && G < 1024) G *= 2;
+    auto cell_of = [&](float t) {
+        int c = (int) std::floor(t * (float) G);
+        return std::min(std::max(c, 0), G - 1);
+    };
+    std::vector<int> count((size_t) G * G + 1, 0);
+    std::vector<int4> span(sh.nf);
+    for (int f = 0; f < sh.nf; ++f) {
+        const uint4 fc = ctx->h_faces[sh.fbase + f];
+        const float2 a = ctx->h_vuv[fc.x], b = ctx->h_vuv[fc.y], c = ctx->h_vuv[fc.z];
+        const float lx = std::min(std::min(a.x, b.x), c.x), hx = std::max(std::max(a.x, b.x), c.x);
+        const float ly = std::min(std::min(a.y, b.y), c.y), hy = std::max(std::max(a.y, b.y), c.y);
+        span[f] = make_int4(cell_of(lx), cell_of(hx), cell_of(ly), cell_of(hy));
+        for (int y = span[f].z; y <= span[f].w; ++y)
+            for (int x = span[f].x; x <= span[f].y; ++x) count[(size_t) y * G + x + 1]++;
+    }
+    for (size_t i = 1; i < count.size(); ++i) count[i] += count[i - 1];
+    std::vector<int> fill(count.begin(), count.end() - 1), faces((size_t) count.back());
+    for (int f = 0; f < sh.nf; ++f)
+        for (int y = span[f].z; y <= span[f].w; ++y)
+            for (int x = span[f].x; x <= span[f].y; ++x) faces[(size_t) fill[(size_t) y * G + x]++] = sh.fbase + f;
+    if (ctx->grid_start) { (void) hipFree(ctx->grid_start); ctx->grid_start = nullptr; }
+    if (ctx->grid_faces) { (void) hipFree(ctx->grid_faces); ctx->grid_faces = nullptr; }
+    ctx->grid_shape = -1;
+    HIP_TRY(ctx, hipMalloc(&ctx->grid_start, count.size() * sizeof(int)));
+    HIP_TRY(ctx, hipMalloc(&ctx->grid_faces, std::max<size_t>(faces.size(), 1) * sizeof(int)));
+    HIP_TRY(ctx, hipMemcpy(ctx->grid_start, count.data(), count.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!faces.empty())
+        HIP_TRY(ctx, hipMemcpy(ctx->grid_faces, faces.data(), faces.size() * sizeof(int), hipMemcpyHostToDevice));
+    ctx->grid_g = G;
+    ctx->grid_shape = shape;
+    return SMS_OK;
+}
+
+sms_status launch_inputs(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *x) {
+    if (n == 0) return SMS_OK;
+    hipLaunchKernelGGL(k_prop_inputs, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream, ctx->scene, q, n,
+                       seed, x);
+    HIP_TRY(ctx, hipGetLastError());
+    return SMS_OK;
+}
+
+} // namespace
+
+// shared with sms_host.hip
+Proposal proposal_of(sms_ctx *ctx) {
+    Proposal P;
+    memset(&P, 0, sizeof(P));
+    P.gmm = ctx->prop_mlp && ctx->prop_shape >= 0 ? ctx->prop_g : nullptr;
+    P.shape = ctx->prop_shape;
+    P.defensive = ctx->prop_defensive;
+    P.grid.g = ctx->grid_g; P.grid.start = ctx->grid_start; P.grid.faces = ctx->grid_faces;
+    P.cap = ctx->cap_sub ? ctx->cap : nullptr;
+    P.cap_counts = ctx->cap_counts;
+    P.cap_sub = ctx->cap_sub;
+    return P;
+}
+
+// network inputs -> raw outputs -> mixtures of the n queries of an estimate (prop_g)
+sms_status proposal_prepare(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed) {
+    if (!ctx->prop_mlp || ctx->prop_shape < 0 || n == 0) return SMS_OK;
+    sms_status st = ensure_prop(ctx, (size_t) n);
+    if (!st) st = launch_inputs(ctx, q, n, seed, ctx->prop_x);
+    if (!st) st = launch_infer(ctx->prop_mlp, ctx->prop_x, n, ctx->prop_raw);
+    if (st) return st;
+    hipLaunchKernelGGL(k_gmm_prepare, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream, ctx->prop_raw, n,
+                       ctx->prop_g);
+    HIP_TRY(ctx, hipGetLastError());
+    return SMS_OK;
+}
+
+void proposal_release(sms_ctx *ctx) {
+    for (void *q : {(void *) ctx->grid_start, (void *) ctx->grid_faces, (void *) ctx->prop_x, (void *) ctx->prop_raw,
+                    (void *) ctx->prop_g, (void *) ctx->cap, (void *) ctx->cap_counts})
+        if (q) (void) hipFree(q);
+    if (ctx->prop_mlp) ctx->prop_mlp = nullptr;
+}
+
+extern "C" {
+
+sms_status sms_mlp_create(sms_ctx *ctx, uint64_t seed, sms_mlp **out) {
+    if (!ctx || !out) return fail(ctx, SMS_E_INVALID, "sms_mlp_create: null argument");
+    *out = nullptr;
+    sms_mlp *mlp = new (std::nothrow) sms_mlp();
+    if (!mlp) return fail(ctx, SMS_E_OOM, "sms_mlp_create: host allocation");
+    mlp->ctx = ctx;
+    const size_t fb = MLP_PARAMS * sizeof(float);
+    hipError_t e = hipMalloc(&mlp->p, fb);
+    if (e == hipSuccess) e = hipMalloc(&mlp->g, fb);
+    if (e == hipSuccess) e = hipMalloc(&mlp->m, fb);
+    if (e == hipSuccess) e = hipMalloc(&mlp->v, fb);
+    if (e == hipSuccess) e = hipMalloc(&mlp->loss, sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&mlp->wb, MLP_PARAMS * sizeof(uint16_t));
+    if (e == hipSuccess) e = hipMemsetAsync(mlp->g, 0, fb, ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(mlp->m, 0, fb, ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(mlp->v, 0, fb, ctx->stream);
+    if (e != hipSuccess) {
+        free_mlp(mlp);
+        return fail(ctx, e == hipErrorOutOfMemory ? SMS_E_OOM : SMS_E_HIP, std::string("sms_mlp_create: ") + hipGetErrorString(e));
+    }
+    std::vector<float> h = init_params(seed);
+    sms_status st = upload_params(mlp, h.data());
+    if (st) { free_mlp(mlp); return st; }
+    *out = mlp;
+    return SMS_OK;
+}
+
+sms_status sms_mlp_destroy(sms_mlp *mlp) {
+    if (!mlp) return SMS_E_INVALID;
+    sms_ctx *ctx = mlp->ctx;
+    (void) hipStreamSynchronize(ctx->stream);
+    if (ctx->prop_mlp == mlp) ctx->prop_mlp = nullptr;
+    free_mlp(mlp);
+    return SMS_OK;
+}
+
+sms_status sms_mlp_get_params(sms_mlp *mlp, float *params) {
+    if (!mlp || !params) return SMS_E_INVALID;
+    sms_ctx *ctx = mlp->ctx;
+    HIP_TRY(ctx, hipMemcpyAsync(params, mlp->p, MLP_PARAMS * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return SMS_OK;
+}
+
+sms_status sms_mlp_set_params(sms_mlp *mlp, const float *params) {
+    if (!mlp || !params) return SMS_E_INVALID;
+    return upload_params(mlp, params);
+}
+
+sms_status sms_mlp_infer(sms_mlp *mlp, const float *x, int64_t n, float *out) {
+    if (!mlp) return SMS_E_INVALID;
+    if (n < 0 || (n > 0 && (!x || !out))) return fail(mlp->ctx, SMS_E_INVALID, "sms_mlp_infer: bad arguments");
+    return launch_infer(mlp, x, n, out);
+}
+
+sms_status sms_mlp_grad(sms_mlp *mlp, const float *x, const float *uv, int64_t n, float *grad, float *loss) {
+    if (!mlp) return SMS_E_INVALID;
+    sms_ctx *ctx = mlp->ctx;
+    if (n <= 0 || !x || !uv) return fail(ctx, SMS_E_INVALID, "sms_mlp_grad: bad arguments");
+    HIP_TRY(ctx, hipMemsetAsync(mlp->g, 0, MLP_PARAMS * sizeof(float), ctx->stream));
+    HIP_TRY(ctx, hipMemsetAsync(mlp->loss, 0, sizeof(float), ctx->stream));
+    hipLaunchKernelGGL(k_mlp_grad, dim3((unsigned) ((n + MLP_TILE - 1) / MLP_TILE)), dim3(256), 0, ctx->stream, mlp->p,
+                       mlp->wb, x, uv, n, 1.f / (float) n, mlp->g, mlp->loss);
+    HIP_TRY(ctx, hipGetLastError());
+    if (grad)
+        HIP_TRY(ctx, hipMemcpyAsync(grad, mlp->g, MLP_PARAMS * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    if (loss) {
+        float s = 0.f;
+        HIP_TRY(ctx, hipMemcpyAsync(&s, mlp->loss, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        *loss = s / (float) n;
+    } else if (grad) {
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return SMS_OK;
+}
+
+sms_status sms_mlp_adam(sms_mlp *mlp, float lr) {
+    if (!mlp) return SMS_E_INVALID;
+    sms_ctx *ctx = mlp->ctx;
+    if (!(lr > 0.f)) return fail(ctx, SMS_E_INVALID, "sms_mlp_adam: learning rate must be positive");
+    mlp->step++;
+    const double b1 = 0.9, b2 = 0.999;
+    const float bc1 = (float) (1.0 - std::pow(b1, (double) mlp->step));
+    const float bc2 = (float) (1.0 - std::pow(b2, (double) mlp->step));
+    hipLaunchKernelGGL(k_mlp_adam, dim3((MLP_PARAMS + 255) / 256), dim3(256), 0, ctx->stream, mlp->p, mlp->g, mlp->m,
+                       mlp->v, mlp->wb, MLP_PARAMS, lr, (float) b1, (float) b2, 1e-8f, bc1, bc2);
+    HIP_TRY(ctx, hipGetLastError());
+    return SMS_OK;
+}
+
+sms_status sms_ctx_set_proposal(sms_ctx *ctx, sms_mlp *mlp, int32_t shape, float defensive) {
+    if (!ctx) return SMS_E_INVALID;
+    if (shape < 0) {
+        if (mlp) return fail(ctx, SMS_E_INVALID, "sms_ctx_set_proposal: a network needs a caster shape");
+        ctx->prop_mlp = nullptr; ctx->prop_shape = -1; ctx->prop_defensive = 1.f;
+        return SMS_OK;
+    }
+    if (shape >= (int) ctx->h_shapes.size()) return fail(ctx, SMS_E_INVALID, "sms_ctx_set_proposal: shape out of range");
+    const DShape &sh = ctx->h_shapes[shape];
+    if (sh.type != SMS_SHAPE_MESH || !sh.has_uv || !(sh.flags & SMS_FLAG_CAUSTIC_CASTER_SS))
+        return fail(ctx, SMS_E_INVALID, "sms_ctx_set_proposal: the shape must be a single-scatter caster mesh with uvs");
+    if (mlp && mlp->ctx != ctx) return fail(ctx, SMS_E_INVALID, "sms_ctx_set_proposal: network of another context");
+    if (mlp && !(defensive > 0.f && defensive <= 1.f))
+        return fail(ctx, SMS_E_INVALID, "sms_ctx_set_proposal: defensive weight must be in (0, 1]");
+    if (ctx->grid_shape != shape) {
+        sms_status st = build_uv_grid(ctx, shape);
+        if (st) return st;
+    }
+    ctx->prop_mlp = mlp;
+    ctx->prop_shape = shape;
+    ctx->prop_defensive = mlp ? defensive : 1.f;
+    return SMS_OK;
+}
+
+sms_status sms_ctx_set_training_capture(sms_ctx *ctx, int64_t capacity) {
+    if (!ctx || capacity < 0) return fail(ctx, SMS_E_INVALID, "sms_ctx_set_training_capture: bad capacity");
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->cap) { (void) hipFree(ctx->cap); ctx->cap = nullptr; }
+    if (ctx->cap_counts) { (void) hipFree(ctx->cap_counts); ctx->cap_counts = nullptr; }
+    ctx->cap_sub = 0;
+    if (capacity == 0) return SMS_OK;
+    const size_t sub = (size_t) ((capacity + SPREAD - 1) / SPREAD);
+    HIP_TRY(ctx, hipMalloc(&ctx->cap, (size_t) SPREAD * sub * 12 * sizeof(float)));
+    HIP_TRY(ctx, hipMalloc(&ctx->cap_counts, (size_t) SPREAD * SPREAD_STRIDE * sizeof(unsigned long long)));
+    HIP_TRY(ctx, hipMemset(ctx->cap_counts, 0, (size_t) SPREAD * SPREAD_STRIDE * sizeof(unsigned long long)));
+    ctx->cap_sub = sub;
+    return SMS_OK;
+}
+
+sms_status sms_ctx_training_samples(sms_ctx *ctx, int64_t *count, float *x, float *uv) {
+    if (!ctx || !count) return SMS_E_INVALID;
+    *count = 0;
+    if (!ctx->cap_sub) return SMS_OK;
+    std::vector<unsigned long long> h((size_t) SPREAD * SPREAD_STRIDE);
+    HIP_TRY(ctx, hipMemcpyAsync(h.data(), ctx->cap_counts, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    int64_t total = 0;
+    for (int s = 0; s < SPREAD; ++s) total += (int64_t) std::min<unsigned long long>(h[(size_t) s * SPREAD_STRIDE], ctx->cap_sub);
+    *count = total;
+    if (x && uv && total > 0) {
+        hipLaunchKernelGGL(k_capture_gather, dim3(SPREAD), dim3(256), 0, ctx->stream, ctx->cap, ctx->cap_counts,
+                           (unsigned long long) ctx->cap_sub, x, uv);
+        HIP_TRY(ctx, hipGetLastError());
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return SMS_OK;
+}
+
+sms_status sms_ctx_clear_training(sms_ctx *ctx) {
+    if (!ctx) return SMS_E_INVALID;
+    if (!ctx->cap_counts) return SMS_OK;
+    HIP_TRY(ctx, hipMemsetAsync(ctx->cap_counts, 0, (size_t) SPREAD * SPREAD_STRIDE * sizeof(unsigned long long),
+                                ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return SMS_OK;
+}
+
+sms_status sms_ss_proposal_inputs(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *x) {
+    if (!ctx || n < 0 || (n > 0 && (!q || !x))) return fail(ctx, SMS_E_INVALID, "sms_ss_proposal_inputs: bad arguments");
+    return launch_inputs(ctx, q, n, seed, x);
+}
+
+sms_status sms_ss_proposal_outputs(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *out) {
+    if (!ctx || n < 0 || (n > 0 && (!q || !out))) return fail(ctx, SMS_E_INVALID, "sms_ss_proposal_outputs: bad arguments");
+    if (!ctx->prop_mlp) return fail(ctx, SMS_E_INVALID, "sms_ss_proposal_outputs: no proposal network set");
+    sms_status st = ensure_prop(ctx, (size_t) std::max<int64_t>(n, 1));
+    if (!st) st = launch_inputs(ctx, q, n, seed, ctx->prop_x);
+    if (!st) st = launch_infer(ctx->prop_mlp, ctx->prop_x, n, out);
+    return st;
+}
+
+} // extern "C"

@@ -188,3 +188,138 @@ class Context:
         _check(lib().sms_ss_estimate_batch(self.h, qbuf.ptr, n, seed, outbuf.ptr, C.byref(st)), self.h,
                "ss_estimate")
         return st.as_dict()
+
+
+    # ---------------------------------------------------------------- learned seed proposal
+    def set_proposal(self, mlp, shape, defensive=0.2):
+        _check(lib().sms_ctx_set_proposal(self.h, mlp.h if mlp is not None else None, shape, defensive), self.h,
+               "sms_ctx_set_proposal")
+
+    def set_training_capture(self, capacity):
+        _check(lib().sms_ctx_set_training_capture(self.h, capacity), self.h, "sms_ctx_set_training_capture")
+
+    def training_count(self):
+        n = C.c_int64(0)
+        _check(lib().sms_ctx_training_samples(self.h, C.byref(n), None, None), self.h, "sms_ctx_training_samples")
+        return int(n.value)
+
+    def training_samples_device(self):
+        """(x buffer, uv buffer, count) of the captured samples, left in device memory."""
+        n = self.training_count()
+        xb = DeviceBuffer(self, max(n * abi.MLP_INPUTS * 4, 4))
+        ub = DeviceBuffer(self, max(n * 8, 4))
+        m = C.c_int64(0)
+        _check(lib().sms_ctx_training_samples(self.h, C.byref(m), xb.ptr, ub.ptr), self.h, "sms_ctx_training_samples")
+        assert m.value == n
+        return xb, ub, n
+
+    def training_samples(self):
+        xb, ub, n = self.training_samples_device()
+        try:
+            x = np.zeros((n, abi.MLP_INPUTS), np.float32)
+            uv = np.zeros((n, 2), np.float32)
+            if n:
+                xb.download(x)
+                ub.download(uv)
+        finally:
+            xb.free()
+            ub.free()
+        return x, uv
+
+    def clear_training(self):
+        _check(lib().sms_ctx_clear_training(self.h), self.h, "sms_ctx_clear_training")
+
+    def _per_query(self, fn, q, seed, width, what):
+        n = len(q)
+        qb = DeviceBuffer(self, max(q.nbytes, 4))
+        ob = DeviceBuffer(self, max(n * width * 4, 4))
+        try:
+            qb.upload(q)
+            _check(fn(self.h, qb.ptr, n, seed, ob.ptr), self.h, what)
+            out = np.zeros((n, width), np.float32)
+            ob.download(out)
+        finally:
+            qb.free()
+            ob.free()
+        return out
+
+    def proposal_inputs(self, q, seed):
+        return self._per_query(lib().sms_ss_proposal_inputs, q, seed, abi.MLP_INPUTS, "sms_ss_proposal_inputs")
+
+    def proposal_outputs(self, q, seed):
+        return self._per_query(lib().sms_ss_proposal_outputs, q, seed, abi.MLP_OUTPUTS, "sms_ss_proposal_outputs")
+
+
+class Mlp:
+    """The learned seed proposal's network (sms_mlp_*) on a context's device."""
+
+    def __init__(self, ctx, seed=0):
+        self.ctx = ctx
+        h = C.c_void_p()
+        _check(lib().sms_mlp_create(ctx.h, seed, C.byref(h)), ctx.h, "sms_mlp_create")
+        self.h = h.value
+
+    def close(self):
+        if self.h:
+            lib().sms_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        p = np.zeros(abi.MLP_PARAMS, np.float32)
+        _check(lib().sms_mlp_get_params(self.h, p.ctypes.data), self.ctx.h, "sms_mlp_get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.shape == (abi.MLP_PARAMS,)
+        _check(lib().sms_mlp_set_params(self.h, p.ctypes.data), self.ctx.h, "sms_mlp_set_params")
+
+    def infer_device(self, xbuf, n, outbuf):
+        _check(lib().sms_mlp_infer(self.h, xbuf.ptr, n, outbuf.ptr), self.ctx.h, "sms_mlp_infer")
+
+    def infer(self, x):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, abi.MLP_INPUTS)
+        n = len(x)
+        xb = DeviceBuffer(self.ctx, max(x.nbytes, 4))
+        ob = DeviceBuffer(self.ctx, max(n * abi.MLP_OUTPUTS * 4, 4))
+        try:
+            xb.upload(x)
+            self.infer_device(xb, n, ob)
+            out = np.zeros((n, abi.MLP_OUTPUTS), np.float32)
+            ob.download(out)
+        finally:
+            xb.free()
+            ob.free()
+        return out
+
+    def grad_device(self, xbuf, uvbuf, n, want_grad=False, want_loss=True, offset=0):
+        """Gradient of the mean mixture NLL over samples [offset, offset + n) of device buffers."""
+        g = np.zeros(abi.MLP_PARAMS, np.float32) if want_grad else None
+        loss = C.c_float(0.0)
+        _check(lib().sms_mlp_grad(self.h, xbuf.ptr + 4 * abi.MLP_INPUTS * offset, uvbuf.ptr + 8 * offset, n,
+                                  g.ctypes.data if want_grad else None, C.byref(loss) if want_loss else None),
+               self.ctx.h, "sms_mlp_grad")
+        return (float(loss.value) if want_loss else None), g
+
+    def grad(self, x, uv):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, abi.MLP_INPUTS)
+        uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+        xb = DeviceBuffer(self.ctx, x.nbytes)
+        ub = DeviceBuffer(self.ctx, uv.nbytes)
+        try:
+            xb.upload(x)
+            ub.upload(uv)
+            return self.grad_device(xb, ub, len(x), want_grad=True)
+        finally:
+            xb.free()
+            ub.free()
+
+    def adam(self, lr):
+        _check(lib().sms_mlp_adam(self.h, lr), self.ctx.h, "sms_mlp_adam")
+

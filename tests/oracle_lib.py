@@ -60,6 +60,13 @@ def load():
     lib.oracle_square_to_uniform_sphere.argtypes = [C.c_float, C.c_float, vp]
     lib.oracle_pcg32_first.argtypes = [C.c_uint64, C.c_uint64, vp, C.c_int]
     lib.oracle_sample_emitter_interaction.argtypes = [vp, vp, C.c_uint64, C.c_uint64, vp]
+    lib.oracle_ss_estimate_batch_prop.argtypes = [vp, vp, i64, C.c_uint64, vp, C.c_int, C.c_float, vp,
+                                                  C.POINTER(abi.SmsStats), C.c_int]
+    lib.oracle_prop_inputs.argtypes = [vp, vp, i64, C.c_uint64, vp]
+    lib.oracle_scene_box.argtypes = [vp, vp, vp]
+    lib.oracle_gmm_prepare.argtypes = [vp, i64, vp]
+    lib.oracle_prop_seed.argtypes = [vp, C.c_float, C.c_uint64, C.c_uint64, C.c_uint64, i64, vp]
+    lib.oracle_uv_to_point.argtypes = [vp, C.c_int, vp, i64, C.c_int, vp, vp]
     return lib
 
 
@@ -124,3 +131,52 @@ class Oracle:
         q = np.zeros(cam["width"] * cam["height"] * spp, abi.GLINT_QUERY)
         self.lib.oracle_glint_make_queries(self.h, spp, seed, q.ctypes.data, self.threads)
         return q
+
+    # ---------------------------------------------------------------- learned seed proposal
+    def ss_estimate_prop(self, q, seed, raw, shape, defensive):
+        """Estimator with the defensive-mixture seed of caster ``shape``; raw: (n, 48) outputs."""
+        raw = np.ascontiguousarray(raw, np.float32)
+        assert raw.shape == (len(q), 48)
+        out = np.zeros((len(q), 3), np.float32)
+        st = abi.SmsStats()
+        rc = self.lib.oracle_ss_estimate_batch_prop(self.h, q.ctypes.data, len(q), seed, raw.ctypes.data, shape,
+                                                    defensive, out.ctypes.data, C.byref(st), self.threads)
+        assert rc == 0, rc
+        return out, st.as_dict()
+
+    def prop_inputs(self, q, seed):
+        x = np.zeros((len(q), 9), np.float32)
+        self.lib.oracle_prop_inputs(self.h, q.ctypes.data, len(q), seed, x.ctypes.data)
+        return x
+
+    def scene_box(self):
+        c = np.zeros(3, np.float32)
+        inv = C.c_float(0)
+        self.lib.oracle_scene_box(self.h, c.ctypes.data, C.byref(inv))
+        return c, float(inv.value)
+
+    def uv_to_point(self, shape, uv, brute=False):
+        uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+        p = np.zeros((len(uv), 3), np.float32)
+        ok = np.zeros(len(uv), np.int32)
+        rc = self.lib.oracle_uv_to_point(self.h, shape, uv.ctypes.data, len(uv), int(brute), p.ctypes.data,
+                                         ok.ctypes.data)
+        assert rc == 0, rc
+        return p, ok.astype(bool)
+
+
+def gmm_prepare(raw):
+    lib = load()
+    raw = np.ascontiguousarray(raw, np.float32).reshape(-1, 48)
+    g = np.zeros_like(raw)
+    lib.oracle_gmm_prepare(raw.ctypes.data, len(raw), g.ctypes.data)
+    return g
+
+
+def prop_seeds(g48, defensive, seed, qid, walk0, n):
+    """(kind, u, v) of walks walk0 .. walk0 + n - 1 of query qid drawn from prepared mixture g48."""
+    lib = load()
+    g = np.ascontiguousarray(g48, np.float32).reshape(48)
+    out = np.zeros((n, 3), np.float32)
+    lib.oracle_prop_seed(g.ctypes.data, defensive, seed, qid, walk0, n, out.ctypes.data)
+    return out
