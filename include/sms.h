@@ -336,6 +336,20 @@ sms_status sms_ms_estimate_batch(sms_ctx *ctx, const sms_ss_query *q, int64_t n,
    glint query (shape = -1 when the first hit is not glinty).  q: device, n = w*h*spp. */
 sms_status sms_glint_make_queries(sms_ctx *ctx, int32_t spp, uint64_t seed, sms_glint_query *q);
 
+/* Camera pass of the single-scatter integrator (path_sms_ss.cpp:94-251), SMS term only: every
+   pixel sample of a width x height x spp frame follows smooth dielectric interfaces
+   (reflection with probability F, else refraction with weight eta_ti^2, dielectric.cpp:201-313)
+   until it reaches a caustic receiver that is not a caster (path_sms_ss.cpp:118-124), which
+   becomes its query; weight receives the path throughput.  Samples that reach no receiver get
+   shape -1 (the estimators return 0 for them, without walks) and weight 0.
+   q: device, n = w*h*spp queries; weight: device, n floats. */
+sms_status sms_ss_make_queries(sms_ctx *ctx, int32_t spp, uint64_t seed, sms_ss_query *q, float *weight);
+/* Box-filter film (device memory): for each of `pixels` pixels, adds the sum over its spp
+   consecutive samples of weight * rgb to image (pixels x 3) and, if image_sq is not NULL, the
+   sum of squares to image_sq.  The caller divides by the number of samples accumulated. */
+sms_status sms_film_accumulate(sms_ctx *ctx, const float *rgb, const float *weight, int64_t pixels, int32_t spp,
+                               float *image, float *image_sq);
+
 /* Kernel timing of the most recent estimator call, measured with HIP events on the context
    stream the kernels ran on: ms[0] = first (persistent lane-per-query) pass, ms[1] = the
    tail / finalize passes.  ms must hold 2 floats. */

@@ -83,6 +83,8 @@ static inline Ray make_ray(V3 o, V3 d, float mint = RAY_EPS_F, float maxt = INF_
     return r;
 }
 static inline V3 ray_at(const Ray &r, float t) { return fmadd3(r.d, t, r.o); }
+/* Interaction::spawn_ray (include/mitsuba/render/interaction.h:58-61) */
+static inline Ray spawn_ray(V3 p, V3 d) { return make_ray(p, d, (1.f + hmax_abs(p)) * RAY_EPS_F, INF_F); }
 
 struct SI {
     bool valid; float t;
@@ -473,7 +475,11 @@ static Hit scene_hit(const Scene &S, const Ray &ray, bool any_hit) {
             if (any_hit) return h;
         }
     }
-    if (!S.nodes.empty()) {
+    /* a ray with a non-finite origin or direction hits no triangle (tri_intersect rejects it)
+       but passes every box test: skip the traversal, the result is the same */
+    const bool finite = std::isfinite(ray.o.x) && std::isfinite(ray.o.y) && std::isfinite(ray.o.z) &&
+                        std::isfinite(ray.d.x) && std::isfinite(ray.d.y) && std::isfinite(ray.d.z);
+    if (!S.nodes.empty() && finite) {
         int stack[128]; int sp = 0; stack[sp++] = 0;
         while (sp > 0) {
             const Node &n = S.nodes[stack[--sp]];
@@ -1396,6 +1402,7 @@ static PathOut proposal_walk(const Scene &S, const Prop &P, const float *g, int 
 static V3 estimate_ss(const Scene &S, const sms_ss_query &q, uint64_t seed, uint64_t qid, Stats &st,
                       const Prop *P = nullptr) {
     const sms_config &cfg = S.cfg;
+    if (q.shape < 0) return v3(0.f, 0.f, 0.f);     /* no receiver: zero estimate, no walks */
     SI si = si_from_query(q);
     Pcg32 rq; stream_seed(&rq, seed, qid, 0);
     EI ei = sample_emitter_interaction(S, si.p, S.emitters_ss, &rq);
@@ -1924,6 +1931,64 @@ static void make_glint_query(const Scene &S, int64_t idx, int spp, uint64_t seed
     q->duv_dx[0] = duv_dx.x; q->duv_dx[1] = duv_dx.y; q->duv_dy[0] = duv_dy.x; q->duv_dy[1] = duv_dy.y;
 }
 
+/* Camera pass of the single-scatter integrator (path_sms_ss.cpp:94-251), SMS term only: the
+   pixel sample's ray (same camera model and jitter stream as the glint queries) follows smooth
+   dielectric interfaces -- reflection with probability F, else refraction
+   (dielectric.cpp:201-313: weight 1 for reflection, eta_ti^2 for refraction) -- for up to
+   SS_CAMERA_DEPTH vertices; the first caustic receiver that is not a caster becomes the query
+   (path_sms_ss.cpp:118-124) with the path throughput as its weight.  Paths that leave the
+   scene or stop on another surface give shape -1 and weight 0. */
+static const int SS_CAMERA_DEPTH = 8;
+static void make_ss_query(const Scene &S, int64_t idx, int spp, uint64_t seed, sms_ss_query *q, float *w) {
+    memset(q, 0, sizeof(*q));
+    q->shape = -1;
+    *w = 0.f;
+    const Camera &c = S.cam;
+    int64_t pixel = idx / spp;
+    int px = (int) (pixel % c.w), py = (int) (pixel / c.w);
+    Pcg32 r; stream_seed(&r, seed, (uint64_t) idx, 0xFFFFFFFFull);
+    float sx = pcg32_float(&r), sy = pcg32_float(&r);
+    V2 pos = v2(((float) px + sx) / (float) c.w, ((float) py + sy) / (float) c.h);
+    V3 near_p = v3((1.f - 2.f * pos.x) * c.tan_half, (1.f - 2.f * pos.y) * c.rect_y, 1.f);
+    V3 dloc = normalize(near_p);
+    V3 d = to_world(c.left, c.up, c.dir, dloc);
+    Ray ray = make_ray(c.o, d, 1e-2f * (1.f / dloc.z), 1e4f * (1.f / dloc.z));
+    float throughput = 1.f;
+    for (int depth = 0; depth < SS_CAMERA_DEPTH; ++depth) {
+        SI si = ray_intersect(S, ray);
+        if (!si.valid) return;
+        const Shape &sh = S.shapes[si.shape];
+        V3 wi = v3(-dot(ray.d, si.sh_s), -dot(ray.d, si.sh_t), -dot(ray.d, si.sh_n));
+        if ((sh.flags & SMS_FLAG_CAUSTIC_RECEIVER) && !(sh.flags & SMS_FLAG_CAUSTIC_CASTER_SS)) {
+            q->shape = si.shape;
+            for (int a = 0; a < 3; ++a) {
+                q->p[a] = (&si.p.x)[a]; q->n[a] = (&si.n.x)[a]; q->sh_n[a] = (&si.sh_n.x)[a];
+                q->sh_s[a] = (&si.sh_s.x)[a]; q->sh_t[a] = (&si.sh_t.x)[a]; q->wi[a] = (&wi.x)[a];
+            }
+            *w = throughput;
+            return;
+        }
+        if (sh.bsdf != SMS_BSDF_DIELECTRIC) return;
+        /* fresnel (fresnel.h:34-80) with the transmitted cosine and relative IOR */
+        const float cos_i = wi.z, eta = sh.eta_r;
+        const bool outside = cos_i >= 0.f;
+        const float rcp_eta = 1.f / eta, eta_ti = outside ? rcp_eta : eta;
+        const float cos_t_sqr = fmaf(-fmaf(-cos_i, cos_i, 1.f), eta_ti * eta_ti, 1.f);
+        const float cos_t_abs = safe_sqrt(cos_t_sqr);
+        const float F = fresnel(cos_i, eta);
+        V3 wo;
+        if (pcg32_float(&r) <= F) {
+            wo = v3(-wi.x, -wi.y, wi.z);
+        } else {
+            const float cos_t = outside ? -cos_t_abs : cos_t_abs;
+            wo = v3(-eta_ti * wi.x, -eta_ti * wi.y, cos_t);
+            throughput = throughput * (eta_ti * eta_ti);
+        }
+        V3 dw = to_world(si.sh_s, si.sh_t, si.sh_n, wo);
+        ray = spawn_ray(si.p, dw);
+    }
+}
+
 template <typename F>
 static void parallel_for(int64_t n, int threads, F f) {
     if (threads <= 1 || n < 64) { for (int64_t i = 0; i < n; ++i) f(i); return; }
@@ -1953,9 +2018,6 @@ static const int MS_MAXB = 4;
 
 struct MSVert { MV v; V2 C; M2 dprev, dcur, dnext, tmp, inv_lambda; V2 dx; };
 struct MSPath { int n; MSVert v[MS_MAXB]; int shapes[MS_MAXB]; };
-
-/* Interaction::spawn_ray (include/mitsuba/render/interaction.h:58-61) */
-static inline Ray spawn_ray(V3 p, V3 d) { return make_ray(p, d, (1.f + hmax_abs(p)) * RAY_EPS_F, INF_F); }
 
 static inline V3 offset_normal_world(const MV &v) { return (v.s * 0.f + v.t * 0.f) + v.n * 1.f; }
 
@@ -2430,6 +2492,7 @@ static V3 ms_path_contribution(const Scene &S, const SI &si, const EI &ei_, MSPa
 
 /* specular_manifold_sampling MS (manifold_ms.cpp:27-201), per-(query, walk) PCG32 streams */
 static V3 estimate_ms(const Scene &S, const sms_ss_query &q, uint64_t seed, uint64_t qid, Stats &st) {
+    if (q.shape < 0) return v3(0.f, 0.f, 0.f);     /* no receiver: zero estimate, no walks */
     const sms_config &cfg = S.cfg;
     SI si = si_from_query(q);
     Pcg32 rq; stream_seed(&rq, seed, qid, 0);
@@ -2563,6 +2626,14 @@ int oracle_ss_estimate_batch(void *h, const sms_ss_query *q, int64_t n, uint64_t
     for (auto &s : per) { tot.walks += s.walks; tot.conv += s.conv; tot.iters += s.iters; tot.bcalls += s.bcalls;
         tot.btrials += s.btrials; if (s.bmax > tot.bmax) tot.bmax = s.bmax; }
     stats_out(tot, stats);
+    return 0;
+}
+
+int oracle_ss_make_queries(void *h, int spp, uint64_t seed, sms_ss_query *q, float *w, int threads) {
+    const Scene &S = *(Scene *) h;
+    if (!S.has_cam) return -1;
+    int64_t n = (int64_t) S.cam.w * S.cam.h * spp;
+    parallel_for(n, threads, [&](int64_t i) { make_ss_query(S, i, spp, seed, q + i, w + i); });
     return 0;
 }
 

@@ -14,8 +14,6 @@ struct MSVert { MV v; V2 C; M2 dprev, dcur, dnext, tmp, inv_lambda; V2 dx; };
 template <int MAXB>
 struct MSPath { int n; MSVert v[MAXB]; int shapes[MAXB]; };
 
-// Interaction::spawn_ray (include/mitsuba/render/interaction.h:58-61)
-SMS_DEV Ray spawn_ray(V3 p, V3 d) { return make_ray(p, d, (1.f + hmax_abs(p)) * RAY_EPS_F, __builtin_inff()); }
 
 // offset normal (0,0,1) in world space (manifold_ms.cpp:386-388)
 SMS_DEV V3 offset_normal_world(const MV &v) { return (v.s * 0.f + v.t * 0.f) + v.n * 1.f; }

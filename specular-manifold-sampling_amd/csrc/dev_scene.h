@@ -79,6 +79,8 @@ SMS_DEV Ray make_ray(V3 o, V3 d, float mint, float maxt) {
     Ray r; r.o = o; r.d = d; r.d_rcp = v3(1.f / d.x, 1.f / d.y, 1.f / d.z); r.mint = mint; r.maxt = maxt;
     return r;
 }
+// Interaction::spawn_ray (include/mitsuba/render/interaction.h:58-61)
+SMS_DEV Ray spawn_ray(V3 p, V3 d) { return make_ray(p, d, (1.f + hmax_abs(p)) * RAY_EPS_F, __builtin_inff()); }
 SMS_DEV Ray make_ray(V3 o, V3 d) { return make_ray(o, d, RAY_EPS_F, __builtin_inff()); }
 
 SMS_DEV V3 xform_p(const float *M, V3 p) {
@@ -221,7 +223,12 @@ SMS_DEV Hit scene_hit_t(const DScene &S, const Ray &ray, bool any_hit) {
             if (any_hit) return h;
         }
     }
-    if (BVH && S.n_nodes > 0) {
+    // A ray with a non-finite origin or direction (a Newton step that produced NaN) hits no
+    // triangle -- tri_intersect rejects it -- but passes every box test, so it would walk the
+    // whole tree; skip the traversal (the result is the same).
+    const bool finite = isfinite(ray.o.x) && isfinite(ray.o.y) && isfinite(ray.o.z) && isfinite(ray.d.x) &&
+                        isfinite(ray.d.y) && isfinite(ray.d.z);
+    if (BVH && S.n_nodes > 0 && finite) {
         int stack[SMS_BVH_STACK];
         int sp = 0;
         int node = 0;

@@ -150,6 +150,8 @@ __global__ void k_glint_finalize(DScene S, const sms_glint_query *Q, uint64_t se
 __global__ void k_ss_estimate(DScene S, const sms_ss_query *Q, int64_t n, uint64_t seed, float *part, float *invp, EstimatorWork W);
 __global__ void k_ss_coop(DScene S, const sms_ss_query *Q, uint64_t seed, float *invp, EstimatorWork W);
 __global__ void k_ss_finalize(const float *part, const float *invp, int nc, int64_t n, float *out);
+__global__ void k_ss_queries(DScene S, int spp, uint64_t seed, sms_ss_query *q, float *w, int64_t n);
+__global__ void k_film_accumulate(const float *rgb, const float *w, int64_t pixels, int spp, float *image, float *image_sq);
 __global__ void k_prop_inputs(DScene S, const sms_ss_query *Q, int64_t n, uint64_t seed, float *x);
 __global__ void k_gmm_prepare(const float *raw, int64_t n, float *g);
 __global__ void k_capture_gather(const float *cap, const unsigned long long *counts, unsigned long long cap_sub,

@@ -768,6 +768,27 @@ sms_status sms_glint_walk_batch(sms_ctx *ctx, const sms_glint_walk_in *in, int64
     return SMS_OK;
 }
 
+sms_status sms_ss_make_queries(sms_ctx *ctx, int32_t spp, uint64_t seed, sms_ss_query *q, float *weight) {
+    if (!ctx || spp <= 0 || !q || !weight) return fail(ctx, SMS_E_INVALID, "bad query arguments");
+    if (!ctx->scene.has_cam) return fail(ctx, SMS_E_INVALID, "scene has no camera");
+    int64_t n = (int64_t) ctx->scene.cam.w * ctx->scene.cam.h * spp;
+    hipLaunchKernelGGL(k_ss_queries, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream, ctx->scene, (int) spp,
+                       seed, q, weight, n);
+    HIP_TRY(ctx, hipGetLastError());
+    return SMS_OK;
+}
+
+sms_status sms_film_accumulate(sms_ctx *ctx, const float *rgb, const float *weight, int64_t pixels, int32_t spp,
+                               float *image, float *image_sq) {
+    if (!ctx || pixels < 0 || spp <= 0 || (pixels > 0 && (!rgb || !weight || !image)))
+        return fail(ctx, SMS_E_INVALID, "bad film arguments");
+    if (pixels == 0) return SMS_OK;
+    hipLaunchKernelGGL(k_film_accumulate, dim3((unsigned) ((pixels + 255) / 256)), dim3(256), 0, ctx->stream, rgb, weight,
+                       pixels, (int) spp, image, image_sq);
+    HIP_TRY(ctx, hipGetLastError());
+    return SMS_OK;
+}
+
 sms_status sms_glint_make_queries(sms_ctx *ctx, int32_t spp, uint64_t seed, sms_glint_query *q) {
     if (!ctx || spp <= 0 || !q) return fail(ctx, SMS_E_INVALID, "bad query arguments");
     if (!ctx->scene.has_cam) return fail(ctx, SMS_E_INVALID, "scene has no camera");

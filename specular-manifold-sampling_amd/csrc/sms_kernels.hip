@@ -173,6 +173,7 @@ __global__ void __launch_bounds__(256) k_ss_estimate(DScene S, const sms_ss_quer
             int64_t slot = q * nc + c;
             part[3 * slot] = 0.f; part[3 * slot + 1] = 0.f; part[3 * slot + 2] = 0.f;
             invp[slot] = 0.f;
+            if (g.shape < 0) continue;                 // no receiver: zero estimate, no walks
             int shape = S.casters_ss[c];
             PathResult r = ss_walk(S, W.prop, q, qkey, c, shape, 0, si.p, ei);
             st_walks++; st_iters += (unsigned long long) r.iterations;
@@ -300,7 +301,8 @@ __global__ void __launch_bounds__(256) k_ss_biased(DScene S, const sms_ss_query 
         stream_seed(rq, qkey, 0);
         EI ei = sample_emitter_interaction(S, si.p, S.emitters_ss, S.n_emitters_ss, rq);
         V3 result = v3(0.f, 0.f, 0.f);
-        for (int c = 0; c < nc; ++c) {
+        const int ncq = g.shape < 0 ? 0 : nc;        // no receiver: zero estimate, no walks
+        for (int c = 0; c < ncq; ++c) {
             int shape = S.casters_ss[c];
             V3 value = v3(0.f, 0.f, 0.f);
             int n_sol = 0;
@@ -363,6 +365,92 @@ __global__ void __launch_bounds__(256) k_ss_biased(DScene S, const sms_ss_query 
         if (lane == 0) write_rgb(out, q, result);
     }
     stat_add(W.stats + 0, st_walks); stat_add(W.stats + 1, st_conv); stat_add(W.stats + 2, st_iters);
+}
+
+// ================================================================ single-scatter camera pass
+// Camera pass of the single-scatter integrator (path_sms_ss.cpp:94-251), SMS term only: the
+// pixel sample's ray (camera model and jitter stream of k_glint_queries) follows smooth
+// dielectric interfaces -- reflection with probability F, else refraction
+// (dielectric.cpp:201-313: weight 1 for reflection, eta_ti^2 for refraction) -- for up to
+// SS_CAMERA_DEPTH vertices; the first caustic receiver that is not a caster becomes the query
+// (path_sms_ss.cpp:118-124) with the path throughput as its weight.  Paths that leave the
+// scene or stop on another surface give shape -1 and weight 0.
+constexpr int SS_CAMERA_DEPTH = 8;
+__global__ void k_ss_queries(DScene S, int spp, uint64_t seed, sms_ss_query *__restrict__ q, float *__restrict__ w,
+                             int64_t n) {
+    const int64_t idx = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    const DCamera &c = S.cam;
+    sms_ss_query o;
+    float *z = (float *) &o;
+    for (int k = 0; k < (int) (sizeof(o) / 4); ++k) z[k] = 0.f;
+    o.shape = -1;
+    float weight = 0.f;
+    const int64_t pixel = idx / spp;
+    const int px = (int) (pixel % c.w), py = (int) (pixel / c.w);
+    Pcg32 r;
+    stream_seed(r, query_key(seed, (uint64_t) idx), 0xFFFFFFFFull);
+    float sx = pcg32_float(r), sy = pcg32_float(r);
+    V2 pos = v2(((float) px + sx) / (float) c.w, ((float) py + sy) / (float) c.h);
+    V3 near_p = v3((1.f - 2.f * pos.x) * c.tan_half, (1.f - 2.f * pos.y) * c.rect_y, 1.f);
+    V3 dloc = normalize(near_p);
+    V3 d = to_world(ld3(c.left), ld3(c.up), ld3(c.dir), dloc);
+    Ray ray = make_ray(ld3(c.o), d, 1e-2f * (1.f / dloc.z), 1e4f * (1.f / dloc.z));
+    float throughput = 1.f;
+    for (int depth = 0; depth < SS_CAMERA_DEPTH; ++depth) {
+        SI si = ray_intersect(S, ray);
+        if (!si.valid) break;
+        const DShape &sh = S.shapes[si.shape];
+        V3 wi = v3(-dot(ray.d, si.sh_s), -dot(ray.d, si.sh_t), -dot(ray.d, si.sh_n));
+        if ((sh.flags & SMS_FLAG_CAUSTIC_RECEIVER) && !(sh.flags & SMS_FLAG_CAUSTIC_CASTER_SS)) {
+            o.shape = si.shape;
+            st3(o.p, si.p); st3(o.n, si.n); st3(o.sh_n, si.sh_n); st3(o.sh_s, si.sh_s); st3(o.sh_t, si.sh_t);
+            st3(o.wi, wi);
+            weight = throughput;
+            break;
+        }
+        if (sh.bsdf != SMS_BSDF_DIELECTRIC) break;
+        // fresnel (fresnel.h:34-80) with the transmitted cosine and relative IOR
+        const float cos_i = wi.z, eta = sh.eta_r;
+        const bool outside = cos_i >= 0.f;
+        const float rcp_eta = 1.f / eta, eta_ti = outside ? rcp_eta : eta;
+        const float cos_t_sqr = fmaf(-fmaf(-cos_i, cos_i, 1.f), eta_ti * eta_ti, 1.f);
+        const float cos_t_abs = safe_sqrt(cos_t_sqr);
+        const float F = fresnel(cos_i, eta);
+        V3 wo;
+        if (pcg32_float(r) <= F) {
+            wo = v3(-wi.x, -wi.y, wi.z);
+        } else {
+            const float cos_t = outside ? -cos_t_abs : cos_t_abs;
+            wo = v3(-eta_ti * wi.x, -eta_ti * wi.y, cos_t);
+            throughput = throughput * (eta_ti * eta_ti);
+        }
+        ray = spawn_ray(si.p, to_world(si.sh_s, si.sh_t, si.sh_n, wo));
+    }
+    q[idx] = o;
+    w[idx] = weight;
+}
+
+// Film: per pixel, the sums over its spp samples (in sample order) of weight * rgb and of its
+// square, added to image / image_sq (box filter; the caller divides by the samples taken).
+__global__ void k_film_accumulate(const float *__restrict__ rgb, const float *__restrict__ w, int64_t pixels, int spp,
+                                  float *__restrict__ image, float *__restrict__ image_sq) {
+    const int64_t p = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= pixels) return;
+    float s[3] = {0.f, 0.f, 0.f}, s2[3] = {0.f, 0.f, 0.f};
+    for (int k = 0; k < spp; ++k) {
+        const int64_t i = p * spp + k;
+        const float wi = w[i];
+        for (int a = 0; a < 3; ++a) {
+            const float v = wi * rgb[3 * i + a];
+            s[a] = s[a] + v;
+            s2[a] = s2[a] + v * v;
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        image[3 * p + a] += s[a];
+        if (image_sq) image_sq[3 * p + a] += s2[a];
+    }
 }
 
 // ================================================================ learned seed proposal

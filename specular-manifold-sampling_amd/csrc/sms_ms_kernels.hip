@@ -96,6 +96,7 @@ SMS_DEV void ms_estimate_body(const DScene &S, const sms_ss_query *__restrict__ 
             int64_t slot = q * nc + c;
             part[3 * slot] = 0.f; part[3 * slot + 1] = 0.f; part[3 * slot + 2] = 0.f;
             invp[slot] = 0.f;
+            if (g.shape < 0) continue;                 // no receiver: zero estimate, no walks
             const int shape = S.casters_ms[c];
             MSWalk<MAXB> w0;
             ms_walk_k<MAXB>(S, qkey, c, shape, 0, si.p, ei, true, nullptr, w0);
@@ -216,7 +217,8 @@ SMS_DEV void ms_biased_body(const DScene &S, const sms_ss_query *__restrict__ Q,
         stream_seed(rq, qkey, 0);
         EI ei = sample_emitter_interaction(S, si.p, S.emitters_ms, S.n_emitters_ms, rq);
         V3 result = v3(0.f, 0.f, 0.f);
-        for (int c = 0; c < nc; ++c) {
+        const int ncq = g.shape < 0 ? 0 : nc;        // no receiver: zero estimate, no walks
+        for (int c = 0; c < ncq; ++c) {
             const int shape = S.casters_ms[c];
             V3 value = v3(0.f, 0.f, 0.f);
             int n_sol = 0;

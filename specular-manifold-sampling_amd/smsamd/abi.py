@@ -152,7 +152,8 @@ EXPORTS = ["sms_config_default", "sms_ctx_create", "sms_ctx_destroy", "sms_ctx_s
            "sms_mlp_create", "sms_mlp_destroy", "sms_mlp_get_params", "sms_mlp_set_params",
            "sms_mlp_infer", "sms_mlp_grad", "sms_mlp_adam", "sms_ctx_set_proposal",
            "sms_ctx_set_training_capture", "sms_ctx_training_samples", "sms_ctx_clear_training",
-           "sms_ss_proposal_inputs", "sms_ss_proposal_outputs"]
+           "sms_ss_proposal_inputs", "sms_ss_proposal_outputs", "sms_ss_make_queries",
+           "sms_film_accumulate"]
 
 # learned seed proposal (include/sms.h)
 MLP_INPUTS = 9
@@ -205,6 +206,8 @@ def load_library(path=LIB_PATH):
     lib.sms_ctx_clear_training.argtypes = [vp]
     lib.sms_ss_proposal_inputs.argtypes = [vp, vp, i64, C.c_uint64, vp]
     lib.sms_ss_proposal_outputs.argtypes = [vp, vp, i64, C.c_uint64, vp]
+    lib.sms_ss_make_queries.argtypes = [vp, C.c_int32, C.c_uint64, vp, vp]
+    lib.sms_film_accumulate.argtypes = [vp, vp, vp, i64, C.c_int32, vp, vp]
     for name in EXPORTS:
         fn = getattr(lib, name)
         if fn.restype is C.c_int and name not in ("sms_ctx_last_error", "sms_config_default"):

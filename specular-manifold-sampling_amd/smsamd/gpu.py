@@ -162,7 +162,32 @@ class Context:
             buf.free()
         return q
 
+    def ss_queries(self, spp, seed):
+        """Camera pass of the single-scatter integrator: (queries, path weights) on the host."""
+        qb, wb, n = self.ss_queries_device(spp, seed)
+        try:
+            q = np.zeros(n, abi.SS_QUERY)
+            w = np.zeros(n, np.float32)
+            qb.download(q)
+            wb.download(w)
+        finally:
+            qb.free()
+            wb.free()
+        return q, w
+
     # ---------------------------------------------------------------- device-resident API
+    def ss_queries_device(self, spp, seed, qbuf=None, wbuf=None):
+        cam = self.scene.camera
+        n = cam["width"] * cam["height"] * spp
+        qb = qbuf or DeviceBuffer(self, n * abi.SS_QUERY.itemsize)
+        wb = wbuf or DeviceBuffer(self, n * 4)
+        _check(lib().sms_ss_make_queries(self.h, spp, seed, qb.ptr, wb.ptr), self.h, "ss_queries")
+        return qb, wb, n
+
+    def film_accumulate_device(self, rgbbuf, wbuf, pixels, spp, imgbuf, imgsqbuf=None):
+        _check(lib().sms_film_accumulate(self.h, rgbbuf.ptr, wbuf.ptr, pixels, spp, imgbuf.ptr,
+                                         imgsqbuf.ptr if imgsqbuf is not None else None), self.h, "film_accumulate")
+
     def glint_queries_device(self, spp, seed):
         cam = self.scene.camera
         n = cam["width"] * cam["height"] * spp

@@ -63,6 +63,7 @@ def load():
     lib.oracle_ss_estimate_batch_prop.argtypes = [vp, vp, i64, C.c_uint64, vp, C.c_int, C.c_float, vp,
                                                   C.POINTER(abi.SmsStats), C.c_int]
     lib.oracle_prop_inputs.argtypes = [vp, vp, i64, C.c_uint64, vp]
+    lib.oracle_ss_make_queries.argtypes = [vp, C.c_int, C.c_uint64, vp, vp, C.c_int]
     lib.oracle_scene_box.argtypes = [vp, vp, vp]
     lib.oracle_gmm_prepare.argtypes = [vp, i64, vp]
     lib.oracle_prop_seed.argtypes = [vp, C.c_float, C.c_uint64, C.c_uint64, C.c_uint64, i64, vp]
@@ -131,6 +132,15 @@ class Oracle:
         q = np.zeros(cam["width"] * cam["height"] * spp, abi.GLINT_QUERY)
         self.lib.oracle_glint_make_queries(self.h, spp, seed, q.ctypes.data, self.threads)
         return q
+
+    def ss_queries(self, spp, seed):
+        cam = self.scene.camera
+        n = cam["width"] * cam["height"] * spp
+        q = np.zeros(n, abi.SS_QUERY)
+        w = np.zeros(n, np.float32)
+        rc = self.lib.oracle_ss_make_queries(self.h, spp, seed, q.ctypes.data, w.ctypes.data, self.threads)
+        assert rc == 0, rc
+        return q, w
 
     # ---------------------------------------------------------------- learned seed proposal
     def ss_estimate_prop(self, q, seed, raw, shape, defensive):

@@ -212,6 +212,28 @@ def test_pool_walks_through_bvh(gpu, oracle_mod):
     orc.close()
 
 
+def test_nonfinite_walk_is_fast(gpu, oracle_mod):
+    """The pool walk whose Newton step turns NaN (tests/golden/pool_nan_walk.npy) matches the
+    oracle and finishes quickly: NaN reprojection rays skip the BVH (a full traversal by one
+    wavefront took about 0.4 s per ray)."""
+    import os
+    import time
+    w = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "pool_nan_walk.npy"))
+    sc = scenes.pool(512, 512)
+    cfg = abi.SmsConfig.default(max_trials=1000000, uniqueness_threshold=1e-5)
+    ctx = gpu.Context(sc, cfg)
+    orc = oracle_mod.Oracle(sc, cfg)
+    ctx.ss_walks(w)                      # first launch of the kernel
+    t = time.perf_counter()
+    go = ctx.ss_walks(w)
+    dt = time.perf_counter() - t
+    print(f"nan walk: {dt * 1e3:.2f} ms", flush=True)
+    _check_ss_walks(go, orc.ss_walks(w))
+    assert dt < 0.1
+    ctx.close()
+    orc.close()
+
+
 def _check_ms_walks(go, oo):
     print(f"ms walks={len(go)} seed={oo['seed_valid'].mean():.3f} success={oo['success'].mean():.3f} "
           f"iters={oo['iterations'].mean():.2f}", flush=True)

@@ -236,3 +236,71 @@ def test_unbiased_and_biased_estimators_agree(dome):
     assert abs(mean_u - rb[:, 0].mean()) < 4 * se + 1e-4 * abs(mean_u)
     ob.close()
     ou.close()
+
+
+def test_camera_pass_refracts_by_snell_law():
+    """Flat water (no waves): every camera sample that reaches the floor through one
+    refraction carries weight (1/1.33)^2, and tracing its receiver's incident direction back to
+    the water plane meets the camera ray under Snell's law; weight-1 samples pass beside the
+    water square."""
+    sc = scenes.pool(width=48, height=48, n=8, n_waves=0)
+    orc = oracle_lib.Oracle(sc, abi.SmsConfig.default())
+    q, w = orc.ss_queries(4, 3)
+    floor = q["shape"] == 1
+    assert floor.mean() > 0.3
+    assert np.all(w[~floor] == 0) and np.all(q["shape"][~floor] == -1)
+    o = np.asarray(sc.camera["origin"], np.float64)
+    # the pool has no walls: samples that pass beside the water square reach the floor directly
+    direct = np.isclose(w, 1.0)
+    assert np.all(np.isclose(w[floor], 1.0) | np.isclose(w[floor], (1 / 1.33) ** 2, rtol=1e-6))
+    pd = q["p"][floor & direct].astype(np.float64)
+    s_ = (0.0 - o[1]) / (pd[:, 1] - o[1])
+    x0 = o + s_[:, None] * (pd - o)                      # where the straight ray crosses y = 0
+    assert (np.abs(x0[:, [0, 2]]).max(1) > 1.0 - 1e-5).all()
+    sel = floor & ~direct
+    assert sel.mean() > 0.2
+    p = q["p"][sel].astype(np.float64)
+    np.testing.assert_allclose(p[:, 1], -1.0, atol=1e-6)
+    # floor frame: sh_s, sh_t, sh_n -> world direction of wi (toward the water)
+    wi = q["wi"][sel].astype(np.float64)
+    d = (q["sh_s"][sel] * wi[:, :1] + q["sh_t"][sel] * wi[:, 1:2] + q["sh_n"][sel] * wi[:, 2:3]).astype(np.float64)
+    t = (0.0 - p[:, 1]) / d[:, 1]
+    h = p + t[:, None] * d
+    a = (h - o) / np.linalg.norm(h - o, axis=1, keepdims=True)          # incident, in air
+    b = -d / np.linalg.norm(d, axis=1, keepdims=True)                    # transmitted, in water
+    n = np.array([0.0, 1.0, 0.0])
+    sin_a = np.linalg.norm(np.cross(a, n), axis=1)
+    sin_b = np.linalg.norm(np.cross(b, n), axis=1)
+    np.testing.assert_allclose(sin_a, 1.33 * sin_b, atol=2e-5)
+    # coplanar: the refracted ray stays in the plane of incidence
+    assert np.abs(np.einsum("ij,ij->i", np.cross(a, n), b)).max() < 2e-5
+    orc.close()
+
+
+def test_camera_pass_through_glass_sphere():
+    """Glass sphere over the floor: samples that pass through the sphere refract twice,
+    (1/1.5)^2 * 1.5^2 = 1; direct floor hits have weight 1; internal reflections keep 1."""
+    sc = scenes.sphere_caustic(48, 48)
+    orc = oracle_lib.Oracle(sc, abi.SmsConfig.default())
+    q, w = orc.ss_queries(4, 5)
+    hit = q["shape"] >= 0
+    assert hit.mean() > 0.5
+    np.testing.assert_allclose(w[hit], 1.0, rtol=1e-6)
+    orc.close()
+
+
+def test_nonfinite_newton_step_walk_fixture():
+    """A pool walk whose Newton step turns NaN (tests/golden/pool_nan_walk.npy): its
+    reprojection rays hit nothing, every iteration is consumed, the walk fails; the rays skip
+    the BVH instead of visiting every node."""
+    import os
+    import time
+    w = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "pool_nan_walk.npy"))
+    sc = scenes.pool(512, 512)
+    orc = oracle_lib.Oracle(sc, abi.SmsConfig.default(max_trials=1000000, uniqueness_threshold=1e-5), threads=1)
+    t = time.perf_counter()
+    o = orc.ss_walks(w)
+    dt = time.perf_counter() - t
+    assert o["success"][0] == 0 and o["iterations"][0] == 20 and o["init_valid"][0] == 1
+    assert dt < 0.05, dt
+    orc.close()
