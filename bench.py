@@ -52,6 +52,15 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 256 MiB device-to-device copy (HBM counter calibration)")
+    ap.add_argument("--no-rmse", action="store_true", help="skip the equal-time RMSE experiment")
+    ap.add_argument("--rmse-only", action="store_true", help="run only the equal-time RMSE experiment")
+    ap.add_argument("--rmse-res", type=int, default=1024, help="pool image width = height")
+    ap.add_argument("--rmse-spp", type=int, default=64, help="uniform-seed samples per pixel (sets the time budget)")
+    ap.add_argument("--rmse-ref-spp", type=int, default=512, help="reference samples per pixel (uniform seeds)")
+    ap.add_argument("--rmse-pass-spp", type=int, default=8)
+    ap.add_argument("--train-samples", type=int, default=2_000_000)
+    ap.add_argument("--train-epochs", type=int, default=32, help="Adam steps per captured batch")
+    ap.add_argument("--defensive", type=float, default=0.2)
     return ap.parse_args()
 
 
@@ -77,6 +86,9 @@ def main():
         print(json.dumps({"error": "this build measures one GPU only; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.rmse_only:
+        print(json.dumps(equal_time_rmse(args)), flush=True)
+        return
     import numpy as np
     from smsamd import abi, gpu, scenes
 
@@ -168,6 +180,117 @@ def main():
     out.free()
     qbuf.free()
     ctx.close()
+
+
+def equal_time_rmse(args):
+    """Water pool (512^2-cell Gerstner mesh, IOR 1.33, floor 1 below, sphere light r 0.05),
+    caustic term of the camera image at rmse_res^2.  Uniform seeding renders rmse_spp samples
+    per pixel; its wall-clock time T is the budget.  The learned run gets the same T: it renders
+    uniform-seed passes while capturing their converged solutions and training the network on
+    them online (train_samples samples, batch 8192, Adam lr 1e-3), then renders passes with the
+    defensive mixture until T is spent; every pass counts in its image.  The amortized run
+    renders with the trained network for T.  RMSE against a uniform-seed reference of
+    rmse_ref_spp samples, with the reference's own variance removed."""
+    import numpy as np
+    from smsamd import abi, gpu, render, scenes
+    res, pass_spp = args.rmse_res, args.rmse_pass_spp
+    sc = scenes.pool(res, res)
+    cfg = abi.SmsConfig.default(max_trials=10_000_000, uniqueness_threshold=1e-5, solver_threshold=1e-5)
+    ctx = gpu.Context(sc, cfg)
+    runner = render.PassRunner(ctx, pass_spp)
+    film = render.Film(ctx, runner.pixels)
+    stats = {}
+    runner.run(999_999, film)                                   # first launches
+    ctx.sync()
+
+    def frame(seed0, n_passes, st=None):
+        film.clear()
+        for p in range(n_passes):
+            runner.run(seed0 + p, film, st)
+        ctx.sync()
+        return film.download()
+
+    (ref, ref_var), t_ref = render.timed(lambda: frame(10_000_000, args.rmse_ref_spp // pass_spp))
+    ref_var_mean = ref_var / max(film.spp, 1)
+    print(f"# rmse: reference {film.spp} spp in {t_ref:.1f}s", file=sys.stderr, flush=True)
+    st_u = {}
+    (uni, _), T = render.timed(lambda: frame(20_000_000, args.rmse_spp // pass_spp, st_u))
+    spp_u = film.spp
+
+    def until_budget(seed0, t_start, st):
+        times, p = [], 0
+        while True:
+            est = sum(times) / len(times) if times else 0.0
+            if time.perf_counter() - t_start + est > T:
+                break
+            t = time.perf_counter()
+            runner.run(seed0 + p, film, st)
+            ctx.sync()
+            times.append(time.perf_counter() - t)
+            p += 1
+        return p
+
+    # learned, training charged to the frame
+    film.clear()
+    mlp = gpu.Mlp(ctx, seed=1)
+    t0 = time.perf_counter()
+    passes_tr, steps, losses = render.train_online(ctx, runner, film, mlp, caster=0, samples=args.train_samples,
+                                                   seed0=30_000_000, epochs=args.train_epochs)
+    ctx.sync()
+    t_train = time.perf_counter() - t0
+    spp_tr = film.spp
+    ctx.set_proposal(mlp, 0, args.defensive)
+    st_l = {}
+    until_budget(40_000_000, t0, st_l)
+    t_l = time.perf_counter() - t0
+    learned, _ = film.download()
+    spp_l = film.spp
+    # learned, network already trained
+    film.clear()
+    t0 = time.perf_counter()
+    st_a = {}
+    until_budget(50_000_000, t0, st_a)
+    t_a = time.perf_counter() - t0
+    amort, _ = film.download()
+    spp_a = film.spp
+    ctx.set_proposal(None, -1)
+
+    lum = ref.mean(1)
+    mask = lum > 2.0 * lum.mean()
+    mean_ref = float(ref.mean())
+    r_u, r_l, r_a = (render.rmse(x, ref, ref_var_mean) for x in (uni, learned, amort))
+    m_u, m_l, m_a = (render.rmse(x, ref, ref_var_mean, mask) for x in (uni, learned, amort))
+
+    def conv_rate(st):
+        return st.get("walks_converged", 0) / max(st.get("walks", 1), 1)
+
+    def trials(st):
+        return st.get("bernoulli_trials", 0) / max(st.get("bernoulli_calls", 1), 1)
+    out = {
+        "workload": f"water pool caustics, {res}x{res} camera image, unbiased (max_trials 1e7, uniqueness 1e-5), "
+                    f"passes of {pass_spp} spp",
+        "budget_s": round(T, 3), "uniform_spp": spp_u,
+        "learned_spp": spp_l, "learned_spp_during_training": spp_tr, "learned_s": round(t_l, 3),
+        "train_s": round(t_train, 3), "train_samples": args.train_samples, "train_steps": steps,
+        "train_epochs": args.train_epochs,
+        "train_loss_first_last": [round(losses[0], 4), round(losses[-1], 4)] if losses else None,
+        "amortized_spp": spp_a, "amortized_s": round(t_a, 3),
+        "reference_spp": args.rmse_ref_spp, "reference_s": round(t_ref, 2), "mean_pixel": mean_ref,
+        "rmse_uniform": r_u, "rmse_learned": r_l, "rmse_amortized": r_a,
+        "ratio": r_u / r_l if r_l > 0 else None, "ratio_amortized": r_u / r_a if r_a > 0 else None,
+        "caustic_mask_pixels": int(mask.sum()),
+        "ratio_caustic_mask": m_u / m_l if m_l > 0 else None,
+        "ratio_caustic_mask_amortized": m_u / m_a if m_a > 0 else None,
+        "converged_per_walk": {"uniform": round(conv_rate(st_u), 4), "learned": round(conv_rate(st_a), 4)},
+        "trials_per_estimate": {"uniform": round(trials(st_u), 3), "learned": round(trials(st_a), 3)},
+        "defensive": args.defensive,
+    }
+    print(f"# rmse: {json.dumps(out)}", file=sys.stderr, flush=True)
+    mlp.close()
+    runner.free()
+    film.free()
+    ctx.close()
+    return out
 
 
 def cpu_baseline(sc, cfg, ctx, n, args):

@@ -406,10 +406,11 @@ sms_status sms_mlp_adam(sms_mlp *mlp, float lr);
    sms_ss_estimate_batch; defensive in (0, 1].  mlp NULL: uniform seeds, `shape` still names the
    caster whose solutions the training capture records; shape -1 clears the proposal. */
 sms_status sms_ctx_set_proposal(sms_ctx *ctx, sms_mlp *mlp, int32_t shape, float defensive);
-/* Record (inputs, solution uv) of converged, unoccluded first walks on the proposal shape in
-   later sms_ss_estimate_batch calls: 64 lists of ceil(capacity / 64) samples, filled by the
-   estimator's blocks in turn, so up to 64 ceil(capacity / 64) samples in total; a full list
-   drops further samples (0 disables and frees). */
+/* Record (inputs, solution uv) of every converged walk on the proposal shape -- first walks
+   and Bernoulli trials, whose seeds follow the same distribution -- in later
+   sms_ss_estimate_batch calls (unbiased estimator): 64 lists of ceil(capacity / 64) samples,
+   filled by the estimator's blocks in turn, so up to 64 ceil(capacity / 64) samples in total;
+   a full list drops further samples (0 disables and frees). */
 sms_status sms_ctx_set_training_capture(sms_ctx *ctx, int64_t capacity);
 /* Number of recorded samples; when x / uv are not NULL, copies them to device buffers of
    count x 9 and count x 2 floats. */

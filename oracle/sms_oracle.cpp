@@ -1365,8 +1365,8 @@ static void proposal_inputs(const Scene &S, V3 p, V3 n, V3 light, float *x) {
 }
 
 /* seed of one walk from its stream: t0 < defensive -> the reference's uniform seed from the
-   next two draws (kind 0, uv = the 2D sample); else component by t1 against the cdf and a
-   Box-Muller draw from (t2, t3) through the Cholesky factor (kind 1) */
+   next two draws (kind 0, *out = the 2D sample); else component by t1 against the cdf and a
+   Box-Muller draw from (t2, t3) through the Cholesky factor (kind 1, *out = uv) */
 static int proposal_seed(const float *g, float defensive, Pcg32 *rw, V2 *out) {
     const float t0 = pcg32_float(rw);
     if (t0 < defensive) { out->x = pcg32_float(rw); out->y = pcg32_float(rw); return 0; }
@@ -1388,14 +1388,15 @@ struct Prop { const float *raw; int shape; float defensive; UVGrid grid; };
    surface and used as p_start; uv outside [0,1)^2 or in no face -> failed walk, 0 iterations */
 static PathOut proposal_walk(const Scene &S, const Prop &P, const float *g, int shape, V3 si_p, const EI &ei, Pcg32 *rw) {
     V2 s;
-    if (proposal_seed(g, P.defensive, rw, &s) == 0)
-        return sample_path_ss(S, shape, si_p, ei, s, v3(0.f, 0.f, 1.f), v3(0.f, 0.f, 0.f));
-    PathOut po;
-    po.ok = false; po.iterations = 0; po.si_final = si_invalid(); po.si_init = si_invalid();
-    if (!(s.x >= 0.f && s.x < 1.f && s.y >= 0.f && s.y < 1.f)) return po;
-    V3 x1;
-    if (!uv_to_point(S, P.grid, s, &x1)) return po;
-    return sample_path_ss(S, shape, si_p, ei, v2(0.f, 0.f), v3(0.f, 0.f, 1.f), x1);
+    V3 p_start = v3(0.f, 0.f, 0.f);
+    if (proposal_seed(g, P.defensive, rw, &s) == 1) {
+        PathOut po;
+        po.ok = false; po.iterations = 0; po.si_final = si_invalid(); po.si_init = si_invalid();
+        if (!(s.x >= 0.f && s.x < 1.f && s.y >= 0.f && s.y < 1.f)) return po;
+        if (!uv_to_point(S, P.grid, s, &p_start)) return po;
+        s = v2(0.f, 0.f);
+    }
+    return sample_path_ss(S, shape, si_p, ei, s, v3(0.f, 0.f, 1.f), p_start);
 }
 
 /* specular_manifold_sampling SS (manifold_ss.cpp:28-201), per-(query, walk) PCG32 streams */

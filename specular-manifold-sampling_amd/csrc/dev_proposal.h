@@ -87,15 +87,17 @@ SMS_DEV bool uv_to_point(const DScene &S, const DUVGrid &G, V2 uv, V3 &p) {
     return false;
 }
 
-// One single-scatter walk seeded from the defensive mixture.  Draws from the walk's stream:
-// t0 chooses the component (t0 < defensive: the reference's uniform seed from (t1, t2));
-// otherwise t1 picks the Gaussian, (t2, t3) give it a Box-Muller sample.
-SMS_DEV PathResult proposal_walk(const DScene &S, const Proposal &P, const float *g, int shape, V3 si_p, const EI &ei,
-                                 Pcg32 &rw) {
+// Seed of one single-scatter walk from the defensive mixture.  Draws from the walk's stream:
+// t0 chooses the component (t0 < defensive: the reference's uniform seed, whose 2D sample is
+// the next two draws, returned in `sample`); otherwise t1 picks the Gaussian and (t2, t3) give
+// it a Box-Muller sample, whose surface point is returned in `p_start`.  False: the mixture's
+// uv is outside [0,1)^2 or in no face (a failed walk with 0 iterations).  The caller runs one
+// sample_path_ss for both kinds, so the lanes of a wavefront share one Newton solver.
+SMS_DEV bool proposal_seed(const DScene &S, const Proposal &P, const float *g, Pcg32 &rw, V2 &sample, V3 &p_start) {
     const float t0 = pcg32_float(rw);
     if (t0 < P.defensive) {
-        V2 s; s.x = pcg32_float(rw); s.y = pcg32_float(rw);
-        return sample_path_ss(S, shape, si_p, ei, s, v3(0.f, 0.f, 1.f), v3(0.f, 0.f, 0.f));
+        sample.x = pcg32_float(rw); sample.y = pcg32_float(rw);
+        return true;
     }
     const float t1 = pcg32_float(rw), t2 = pcg32_float(rw), t3 = pcg32_float(rw);
     int c = PROP_COMP - 1;
@@ -106,13 +108,8 @@ SMS_DEV PathResult proposal_walk(const DScene &S, const Proposal &P, const float
     const float z1 = r * cs, z2 = r * sn;
     const float rho = g[40 + c], q = sqrtf(1.f - rho * rho);
     V2 uv = v2(g[8 + c] + g[24 + c] * z1, g[16 + c] + g[32 + c] * (rho * z1 + q * z2));
-    PathResult res;
-    res.ok = false; res.init_valid = false; res.iterations = 0;
-    res.si_final = si_invalid(); res.p_init = v3(0.f, 0.f, 0.f);
-    if (!(uv.x >= 0.f && uv.x < 1.f && uv.y >= 0.f && uv.y < 1.f)) return res;
-    V3 x1;
-    if (!uv_to_point(S, P.grid, uv, x1)) return res;
-    return sample_path_ss(S, shape, si_p, ei, v2(0.f, 0.f), v3(0.f, 0.f, 1.f), x1);
+    if (!(uv.x >= 0.f && uv.x < 1.f && uv.y >= 0.f && uv.y < 1.f)) return false;
+    return uv_to_point(S, P.grid, uv, p_start);
 }
 
 } // namespace smsdev

@@ -135,12 +135,22 @@ SMS_DEV PathResult ss_walk(const DScene &S, const Proposal &P, int64_t q, uint64
                            V3 si_p, const EI &ei) {
     Pcg32 rw;
     stream_seed(rw, qkey, ((uint64_t) c << 32) + k + 1);
-    if (P.gmm && shape == P.shape) return proposal_walk(S, P, P.gmm + q * PROP_RAW, shape, si_p, ei, rw);
-    V2 s; s.x = pcg32_float(rw); s.y = pcg32_float(rw);
-    return sample_path_ss(S, shape, si_p, ei, s, v3(0.f, 0.f, 1.f), v3(0.f, 0.f, 0.f));
+    V2 s = v2(0.f, 0.f);
+    V3 p_start = v3(0.f, 0.f, 0.f);
+    if (P.gmm && shape == P.shape) {
+        if (!proposal_seed(S, P, P.gmm + q * PROP_RAW, rw, s, p_start)) {
+            PathResult res;
+            res.ok = false; res.init_valid = false; res.iterations = 0;
+            res.si_final = si_invalid(); res.p_init = v3(0.f, 0.f, 0.f);
+            return res;
+        }
+    } else {
+        s.x = pcg32_float(rw); s.y = pcg32_float(rw);
+    }
+    return sample_path_ss(S, shape, si_p, ei, s, v3(0.f, 0.f, 1.f), p_start);
 }
 
-// record (network inputs, solution uv) of a converged first walk on the proposal's caster
+// record (network inputs, solution uv) of a converged walk on the proposal's caster
 SMS_DEV void capture_sample(const DScene &S, const Proposal &P, const SI &si, const EI &ei, V2 uv) {
     const int sub = (int) (blockIdx.x % SPREAD);
     unsigned long long pos = atomicAdd(P.cap_counts + sub * SPREAD_STRIDE, 1ull);
@@ -180,8 +190,8 @@ __global__ void __launch_bounds__(256) k_ss_estimate(DScene S, const sms_ss_quer
             if (!r.ok) continue;
             st_conv++;
             V3 direction = normalize(r.si_final.p - si.p);
-            if (!ss_visible(S, ei, r.si_final.p)) continue;
             if (W.prop.cap && shape == W.prop.shape) capture_sample(S, W.prop, si, ei, r.si_final.uv);
+            if (!ss_visible(S, ei, r.si_final.p)) continue;
             V3 spec = path_contribution_ss(S, si, ei, r.si_final);
             V3 dl = v3(dot(direction, si.sh_s), dot(direction, si.sh_t), dot(direction, si.sh_n));
             V3 bsdf = receiver_bsdf_eval(S, si, wi_local, dl);
@@ -193,6 +203,7 @@ __global__ void __launch_bounds__(256) k_ss_estimate(DScene S, const sms_ss_quer
                 PathResult t = ss_walk(S, W.prop, q, qkey, c, shape, (uint64_t) k, si.p, ei);
                 st_walks++; st_iters += (unsigned long long) t.iterations;
                 if (t.ok) st_conv++;
+                if (t.ok && W.prop.cap && shape == W.prop.shape) capture_sample(S, W.prop, si, ei, t.si_final.uv);
                 V3 dtr = normalize(t.si_final.p - si.p);
                 if (t.ok && fabsf(dot(direction, dtr) - 1.f) < cfg.uniqueness_threshold) { inv_prob = (float) k; break; }
                 k++;
@@ -245,6 +256,7 @@ __global__ void __launch_bounds__(256) k_ss_coop(DScene S, const sms_ss_query *_
             if (k <= kcap) {
                 PathResult t = ss_walk(S, W.prop, q, qkey, c, shape, (uint64_t) k, si.p, ei);
                 ran = true; its = t.iterations; conv = t.ok;
+                if (t.ok && W.prop.cap && shape == W.prop.shape) capture_sample(S, W.prop, si, ei, t.si_final.uv);
                 V3 dtr = normalize(t.si_final.p - si.p);
                 match = t.ok && fabsf(dot(direction, dtr) - 1.f) < cfg.uniqueness_threshold;
             }
