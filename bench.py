@@ -61,6 +61,9 @@ def parse():
     ap.add_argument("--train-samples", type=int, default=2_000_000)
     ap.add_argument("--train-epochs", type=int, default=32, help="Adam steps per captured batch")
     ap.add_argument("--defensive", type=float, default=0.2)
+    ap.add_argument("--capture", choices=["solutions", "estimates"], default="solutions",
+                    help="training records: every converged solution, or estimate-weighted solutions")
+    ap.add_argument("--weight-power", type=float, default=1.0, help="exponent on estimate weights")
     return ap.parse_args()
 
 
@@ -234,8 +237,10 @@ def equal_time_rmse(args):
     film.clear()
     mlp = gpu.Mlp(ctx, seed=1)
     t0 = time.perf_counter()
+    mode = abi.CAPTURE_ESTIMATES if args.capture == "estimates" else abi.CAPTURE_SOLUTIONS
     passes_tr, steps, losses = render.train_online(ctx, runner, film, mlp, caster=0, samples=args.train_samples,
-                                                   seed0=30_000_000, epochs=args.train_epochs)
+                                                   seed0=30_000_000, epochs=args.train_epochs, mode=mode,
+                                                   weight_power=args.weight_power)
     ctx.sync()
     t_train = time.perf_counter() - t0
     spp_tr = film.spp
@@ -272,7 +277,7 @@ def equal_time_rmse(args):
         "budget_s": round(T, 3), "uniform_spp": spp_u,
         "learned_spp": spp_l, "learned_spp_during_training": spp_tr, "learned_s": round(t_l, 3),
         "train_s": round(t_train, 3), "train_samples": args.train_samples, "train_steps": steps,
-        "train_epochs": args.train_epochs,
+        "train_epochs": args.train_epochs, "capture": args.capture, "weight_power": args.weight_power,
         "train_loss_first_last": [round(losses[0], 4), round(losses[-1], 4)] if losses else None,
         "amortized_spp": spp_a, "amortized_s": round(t_a, 3),
         "reference_spp": args.rmse_ref_spp, "reference_s": round(t_ref, 2), "mean_pixel": mean_ref,

@@ -381,8 +381,8 @@ sms_status sms_glint_estimate_batch_host(sms_ctx *ctx, const sms_glint_query *q,
    walk on that caster -- the first walk and every Bernoulli trial -- is drawn from
    defensive * (uniform area seed of the reference) + (1 - defensive) * mixture, so the
    inverse-PDF estimate stays unbiased; a mixture seed whose uv falls outside the mesh's UV
-   domain is a failed walk with 0 iterations.  Training minimises the mixture's negative
-   log-likelihood of converged solutions' uv (Adam). */
+   domain is a failed walk with 0 iterations.  Training minimises the mixture's (weighted)
+   negative log-likelihood of captured solutions' uv (Adam). */
 #define SMS_MLP_INPUTS 9
 #define SMS_MLP_OUTPUTS 48
 #define SMS_MLP_PARAMS 25904
@@ -396,25 +396,35 @@ sms_status sms_mlp_get_params(sms_mlp *mlp, float *params);
 sms_status sms_mlp_set_params(sms_mlp *mlp, const float *params);
 /* x: n x 9 inputs, out: n x 48 raw outputs (device memory). */
 sms_status sms_mlp_infer(sms_mlp *mlp, const float *x, int64_t n, float *out);
-/* Mean negative log-likelihood of uv (n x 2, device) under the mixture of x (n x 9, device)
+/* Weighted mean negative log-likelihood sum_j w_j NLL_j / sum_j w_j of uv (n x 2, device)
+   under the mixtures of x (n x 9, device), w (n, device; NULL: all 1, non-negative, not all 0)
    and its gradient; the gradient is kept for sms_mlp_adam and, if grad is not NULL, copied to
-   grad (SMS_MLP_PARAMS host floats).  *loss receives the mean loss. */
-sms_status sms_mlp_grad(sms_mlp *mlp, const float *x, const float *uv, int64_t n, float *grad, float *loss);
+   grad (SMS_MLP_PARAMS host floats).  *loss (if not NULL) receives the loss. */
+sms_status sms_mlp_grad(sms_mlp *mlp, const float *x, const float *uv, const float *w, int64_t n, float *grad,
+                        float *loss);
 /* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) with the kept gradient, which is cleared. */
 sms_status sms_mlp_adam(sms_mlp *mlp, float lr);
 /* Use `mlp` as the seed proposal of single-scatter caster mesh `shape` (with uvs) in
    sms_ss_estimate_batch; defensive in (0, 1].  mlp NULL: uniform seeds, `shape` still names the
    caster whose solutions the training capture records; shape -1 clears the proposal. */
 sms_status sms_ctx_set_proposal(sms_ctx *ctx, sms_mlp *mlp, int32_t shape, float defensive);
-/* Record (inputs, solution uv) of every converged walk on the proposal shape -- first walks
-   and Bernoulli trials, whose seeds follow the same distribution -- in later
-   sms_ss_estimate_batch calls (unbiased estimator): 64 lists of ceil(capacity / 64) samples,
-   filled by the estimator's blocks in turn, so up to 64 ceil(capacity / 64) samples in total;
-   a full list drops further samples (0 disables and frees). */
-sms_status sms_ctx_set_training_capture(sms_ctx *ctx, int64_t capacity);
+/* Training capture of later sms_ss_estimate_batch calls (unbiased estimator) on the proposal
+   shape, as records (network inputs, solution uv, weight):
+     SMS_CAPTURE_SOLUTIONS  every converged walk -- first walks and Bernoulli trials, whose seeds
+                            follow the same distribution -- with weight 1;
+     SMS_CAPTURE_ESTIMATES  every (query, caster) whose estimate has a positive RGB mean: the
+                            first walk's solution with that mean as weight (bsdf * contribution
+                            * inverse probability), so a weighted fit follows the solutions'
+                            contributions instead of their basins.
+   64 lists of ceil(capacity / 64) records, filled by the estimator's blocks in turn, so up to
+   64 ceil(capacity / 64) in total; a full list drops further records (capacity 0 disables
+   and frees). */
+#define SMS_CAPTURE_SOLUTIONS 0
+#define SMS_CAPTURE_ESTIMATES 1
+sms_status sms_ctx_set_training_capture(sms_ctx *ctx, int64_t capacity, int32_t mode);
 /* Number of recorded samples; when x / uv are not NULL, copies them to device buffers of
-   count x 9 and count x 2 floats. */
-sms_status sms_ctx_training_samples(sms_ctx *ctx, int64_t *count, float *x, float *uv);
+   count x 9 and count x 2 floats, and the weights to w (count floats) if w is not NULL. */
+sms_status sms_ctx_training_samples(sms_ctx *ctx, int64_t *count, float *x, float *uv, float *w);
 sms_status sms_ctx_clear_training(sms_ctx *ctx);
 /* Network inputs (device, n x 9) of the queries of an estimate with this seed: shading point,
    shading normal and the estimator's light sample, positions mapped by (p - c) / h with c the

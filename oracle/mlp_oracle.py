@@ -9,7 +9,7 @@ the bias; layers 9->160 features -> 64 -> 64 -> 64 -> 64 -> 48.
 
 Two precisions:
   * ``"f64"``  -- exact arithmetic in float64 (pinned against torch autograd and central finite
-    differences in tests/test_oracle_mlp.py);
+    differences in tests/test_oracle_proposal.py);
   * ``"bf16"`` -- the rounding points of the gfx950 kernels: features, weights, hidden
     activations and back-propagated errors rounded to BF16 (round to nearest even), products
     accumulated in FP32, biases / head / loss in FP32.
@@ -155,15 +155,17 @@ def forward(params, x9, mode="f64"):
     return h, acts
 
 
-def loss_and_grad(params, x9, uv, mode="f64"):
-    """Mean NLL over the batch and its gradient w.r.t. the parameter vector."""
+def loss_and_grad(params, x9, uv, mode="f64", w=None):
+    """Weighted mean NLL sum_j w_j NLL_j / sum_j w_j over the batch (w None: unit weights) and
+    its gradient w.r.t. the parameter vector."""
     o, acts = forward(params, x9, mode)
     n = len(o)
     loss, d_o = nll_grad(o, uv)
+    wt = np.ones(n) if w is None else np.asarray(w, np.float64)
     grad = np.zeros(PARAMS, np.float64)
     if mode == "f64":
         Ws = unpack(params, np.float64)
-        dy = d_o / n
+        dy = d_o * (wt / wt.sum())[:, None]
         for i in range(4, -1, -1):
             W, _ = Ws[i]
             w0, b0 = OFFSETS[i]
@@ -171,9 +173,11 @@ def loss_and_grad(params, x9, uv, mode="f64"):
             grad[b0:b0 + W.shape[0]] = dy.sum(0)
             if i > 0:
                 dy = (dy @ W) * (acts[i] > 0)
-        return float(loss.mean()), grad
+        return float((wt * loss).sum() / wt.sum()), grad
     Ws = [(bf16(W), b) for W, b in unpack(params, np.float32)]
-    dy = bf16((d_o * (1.0 / n)).astype(np.float32))
+    w32 = wt.astype(np.float32)
+    scale = (w32 * (np.float32(1) / w32.sum(dtype=np.float32))).astype(np.float32)
+    dy = bf16((d_o * scale[:, None]).astype(np.float32))
     for i in range(4, -1, -1):
         W, _ = Ws[i]
         w0, b0 = OFFSETS[i]
@@ -181,7 +185,7 @@ def loss_and_grad(params, x9, uv, mode="f64"):
         grad[b0:b0 + W.shape[0]] = dy.sum(0, dtype=np.float32)
         if i > 0:
             dy = bf16(((dy @ W).astype(np.float32)) * (acts[i] > 0))
-    return float(loss.astype(np.float32).mean()), grad
+    return float((w32 * loss.astype(np.float32)).sum() / w32.sum()), grad
 
 
 def adam(params, grad, m, v, t, lr, b1=0.9, b2=0.999, eps=1e-8):

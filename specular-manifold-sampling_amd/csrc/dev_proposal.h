@@ -23,10 +23,11 @@ struct Proposal {
     int shape;                        // mesh caster the proposal seeds (-1: none)
     float defensive;                  // weight of the reference's uniform seed
     DUVGrid grid;
-    // capture of (inputs, solution uv) of converged first walks
+    // training capture: records (inputs[9], uv[2], weight)
     float *cap;                       // SPREAD sub-lists of cap_sub records of 12 floats
     unsigned long long *cap_counts;   // SPREAD counters, SPREAD_STRIDE apart
     unsigned long long cap_sub;
+    int cap_mode;                     // SMS_CAPTURE_SOLUTIONS or SMS_CAPTURE_ESTIMATES
 };
 
 SMS_DEV float sigmoid_l(float x) { return 1.f / (1.f + expf_l(-x)); }

@@ -50,7 +50,7 @@ struct sms_ctx {
     int *grid_start = nullptr, *grid_faces = nullptr;
     float *prop_x = nullptr, *prop_raw = nullptr, *prop_g = nullptr; size_t prop_cap = 0;
     // capture of (inputs, solution uv) of converged first walks
-    float *cap = nullptr; unsigned long long *cap_counts = nullptr; size_t cap_sub = 0;
+    float *cap = nullptr; unsigned long long *cap_counts = nullptr; size_t cap_sub = 0; int cap_mode = 0;
 };
 
 

@@ -924,7 +924,7 @@ sms_status sms_ss_estimate_batch(sms_ctx *ctx, const sms_ss_query *q, int64_t n,
             HIP_TRY(ctx, hipGetLastError());
             HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
             hipLaunchKernelGGL(k_ss_coop, dim3(grid_for(ctx, ctx->occ_sscoop)), dim3(256), 0, ctx->stream,
-                               ctx->scene, q, seed, ctx->invp, W);
+                               ctx->scene, q, seed, ctx->part, ctx->invp, W);
             HIP_TRY(ctx, hipGetLastError());
             hipLaunchKernelGGL(k_ss_finalize, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream,
                                ctx->part, ctx->invp, nc, n, out_rgb);

@@ -150,15 +150,18 @@ SMS_DEV PathResult ss_walk(const DScene &S, const Proposal &P, int64_t q, uint64
     return sample_path_ss(S, shape, si_p, ei, s, v3(0.f, 0.f, 1.f), p_start);
 }
 
-// record (network inputs, solution uv) of a converged walk on the proposal's caster
-SMS_DEV void capture_sample(const DScene &S, const Proposal &P, const SI &si, const EI &ei, V2 uv) {
+// record (network inputs, solution uv, weight) for the proposal's training
+SMS_DEV void capture_sample(const DScene &S, const Proposal &P, const SI &si, const EI &ei, V2 uv, float w) {
     const int sub = (int) (blockIdx.x % SPREAD);
     unsigned long long pos = atomicAdd(P.cap_counts + sub * SPREAD_STRIDE, 1ull);
     if (pos >= P.cap_sub) return;
     float *r = P.cap + ((size_t) sub * P.cap_sub + pos) * 12;
     proposal_inputs(S, si.p, si.sh_n, ei.p, r);
-    r[9] = uv.x; r[10] = uv.y; r[11] = 0.f;
+    r[9] = uv.x; r[10] = uv.y; r[11] = w;
 }
+
+SMS_DEV bool capturing(const Proposal &P, int shape, int mode) { return P.cap && shape == P.shape && P.cap_mode == mode; }
+SMS_DEV float mean3(const float *v) { return ((v[0] + v[1]) + v[2]) * (1.f / 3.f); }
 
 // Pass 1: lane per query; for every caster the first walk, visibility, contribution and up
 // to lane_trials Bernoulli trials.  Per (query, caster) the product bsdf*specular (bxs) and
@@ -190,7 +193,7 @@ __global__ void __launch_bounds__(256) k_ss_estimate(DScene S, const sms_ss_quer
             if (!r.ok) continue;
             st_conv++;
             V3 direction = normalize(r.si_final.p - si.p);
-            if (W.prop.cap && shape == W.prop.shape) capture_sample(S, W.prop, si, ei, r.si_final.uv);
+            if (capturing(W.prop, shape, SMS_CAPTURE_SOLUTIONS)) capture_sample(S, W.prop, si, ei, r.si_final.uv, 1.f);
             if (!ss_visible(S, ei, r.si_final.p)) continue;
             V3 spec = path_contribution_ss(S, si, ei, r.si_final);
             V3 dl = v3(dot(direction, si.sh_s), dot(direction, si.sh_t), dot(direction, si.sh_n));
@@ -203,7 +206,8 @@ __global__ void __launch_bounds__(256) k_ss_estimate(DScene S, const sms_ss_quer
                 PathResult t = ss_walk(S, W.prop, q, qkey, c, shape, (uint64_t) k, si.p, ei);
                 st_walks++; st_iters += (unsigned long long) t.iterations;
                 if (t.ok) st_conv++;
-                if (t.ok && W.prop.cap && shape == W.prop.shape) capture_sample(S, W.prop, si, ei, t.si_final.uv);
+                if (t.ok && capturing(W.prop, shape, SMS_CAPTURE_SOLUTIONS))
+                    capture_sample(S, W.prop, si, ei, t.si_final.uv, 1.f);
                 V3 dtr = normalize(t.si_final.p - si.p);
                 if (t.ok && fabsf(dot(direction, dtr) - 1.f) < cfg.uniqueness_threshold) { inv_prob = (float) k; break; }
                 k++;
@@ -214,11 +218,17 @@ __global__ void __launch_bounds__(256) k_ss_estimate(DScene S, const sms_ss_quer
                 unsigned long long tr = (unsigned long long) k;
                 st_btrials += tr; st_bmax = st_bmax > tr ? st_bmax : tr;
                 invp[slot] = inv_prob;
+                const float wgt = mean3(&part[3 * slot]) * inv_prob;
+                if (wgt > 0.f && capturing(W.prop, shape, SMS_CAPTURE_ESTIMATES))
+                    capture_sample(S, W.prop, si, ei, r.si_final.uv, wgt);
             } else {
                 unsigned long long o = spread_append(W.over, sub, 1ull);
                 OverflowRec rec;
                 rec.q = slot; rec.k = k;
                 rec.sol[0] = direction.x; rec.sol[1] = direction.y; rec.sol[2] = direction.z;
+                // the first walk's uv, for a training capture at the chain's end
+                rec.shapes[0] = __float_as_int(r.si_final.uv.x); rec.shapes[1] = __float_as_int(r.si_final.uv.y);
+                rec.shapes[2] = rec.shapes[3] = 0;
                 W.over.items[(size_t) sub * W.over.sub_cap + o] = rec;
             }
         }
@@ -228,7 +238,8 @@ __global__ void __launch_bounds__(256) k_ss_estimate(DScene S, const sms_ss_quer
 }
 
 __global__ void __launch_bounds__(256) k_ss_coop(DScene S, const sms_ss_query *__restrict__ Q, uint64_t seed,
-                                                 float *__restrict__ invp, EstimatorWork W) {
+                                                 const float *__restrict__ part, float *__restrict__ invp,
+                                                 EstimatorWork W) {
     const sms_config &cfg = S.cfg;
     const int lane = threadIdx.x & 63;
     const int nc = S.n_casters_ss;
@@ -256,7 +267,8 @@ __global__ void __launch_bounds__(256) k_ss_coop(DScene S, const sms_ss_query *_
             if (k <= kcap) {
                 PathResult t = ss_walk(S, W.prop, q, qkey, c, shape, (uint64_t) k, si.p, ei);
                 ran = true; its = t.iterations; conv = t.ok;
-                if (t.ok && W.prop.cap && shape == W.prop.shape) capture_sample(S, W.prop, si, ei, t.si_final.uv);
+                if (t.ok && capturing(W.prop, shape, SMS_CAPTURE_SOLUTIONS))
+                    capture_sample(S, W.prop, si, ei, t.si_final.uv, 1.f);
                 V3 dtr = normalize(t.si_final.p - si.p);
                 match = t.ok && fabsf(dot(direction, dtr) - 1.f) < cfg.uniqueness_threshold;
             }
@@ -271,6 +283,9 @@ __global__ void __launch_bounds__(256) k_ss_coop(DScene S, const sms_ss_query *_
             invp[rec.q] = found >= 0 ? (float) found : 0.f;
             unsigned long long tr = found >= 0 ? (unsigned long long) found : (unsigned long long) kcap + 1ull;
             st_btrials += tr; st_bmax = st_bmax > tr ? st_bmax : tr;
+            const float wgt = found > 0 ? mean3(&part[3 * rec.q]) * (float) found : 0.f;
+            if (wgt > 0.f && capturing(W.prop, shape, SMS_CAPTURE_ESTIMATES))
+                capture_sample(S, W.prop, si, ei, v2(__int_as_float(rec.shapes[0]), __int_as_float(rec.shapes[1])), wgt);
         }
     }
     stat_add(W.stats + 0, st_walks); stat_add(W.stats + 1, st_conv); stat_add(W.stats + 2, st_iters);
@@ -491,10 +506,11 @@ __global__ void k_gmm_prepare(const float *__restrict__ raw, int64_t n, float *_
     for (int j = 0; j < PROP_RAW; ++j) g[q * PROP_RAW + j] = r[j];
 }
 
-// compact the capture sub-lists (sub s: min(count_s, cap_sub) records) into x (n x 9) and
-// uv (n x 2), sub-lists in order; one block per sub-list
+// compact the capture sub-lists (sub s: min(count_s, cap_sub) records) into x (n x 9), uv
+// (n x 2) and, if w is not null, the weights (n), sub-lists in order; one block per sub-list
 __global__ void k_capture_gather(const float *__restrict__ cap, const unsigned long long *__restrict__ counts,
-                                 unsigned long long cap_sub, float *__restrict__ x, float *__restrict__ uv) {
+                                 unsigned long long cap_sub, float *__restrict__ x, float *__restrict__ uv,
+                                 float *__restrict__ w) {
     const int sub = blockIdx.x;
     unsigned long long off = 0;
     for (int s = 0; s < sub; ++s) {
@@ -507,6 +523,7 @@ __global__ void k_capture_gather(const float *__restrict__ cap, const unsigned l
         for (int j = 0; j < 9; ++j) x[(off + i) * 9 + j] = r[j];
         uv[(off + i) * 2] = r[9];
         uv[(off + i) * 2 + 1] = r[10];
+        if (w) w[off + i] = r[11];
     }
 }
 

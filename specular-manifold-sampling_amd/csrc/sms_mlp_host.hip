@@ -169,6 +169,7 @@ Proposal proposal_of(sms_ctx *ctx) {
     P.cap = ctx->cap_sub ? ctx->cap : nullptr;
     P.cap_counts = ctx->cap_counts;
     P.cap_sub = ctx->cap_sub;
+    P.cap_mode = ctx->cap_mode;
     return P;
 }
 
@@ -205,7 +206,7 @@ sms_status sms_mlp_create(sms_ctx *ctx, uint64_t seed, sms_mlp **out) {
     if (e == hipSuccess) e = hipMalloc(&mlp->g, fb);
     if (e == hipSuccess) e = hipMalloc(&mlp->m, fb);
     if (e == hipSuccess) e = hipMalloc(&mlp->v, fb);
-    if (e == hipSuccess) e = hipMalloc(&mlp->loss, sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&mlp->loss, 2 * sizeof(float));
     if (e == hipSuccess) e = hipMalloc(&mlp->wb, MLP_PARAMS * sizeof(uint16_t));
     if (e == hipSuccess) e = hipMemsetAsync(mlp->g, 0, fb, ctx->stream);
     if (e == hipSuccess) e = hipMemsetAsync(mlp->m, 0, fb, ctx->stream);
@@ -249,22 +250,27 @@ sms_status sms_mlp_infer(sms_mlp *mlp, const float *x, int64_t n, float *out) {
     return launch_infer(mlp, x, n, out);
 }
 
-sms_status sms_mlp_grad(sms_mlp *mlp, const float *x, const float *uv, int64_t n, float *grad, float *loss) {
+sms_status sms_mlp_grad(sms_mlp *mlp, const float *x, const float *uv, const float *w, int64_t n, float *grad,
+                        float *loss) {
     if (!mlp) return SMS_E_INVALID;
     sms_ctx *ctx = mlp->ctx;
     if (n <= 0 || !x || !uv) return fail(ctx, SMS_E_INVALID, "sms_mlp_grad: bad arguments");
+    // mlp->loss: [0] sum of w * nll, [1] sum of w; the kernels scale the gradient by 1 / [1]
+    HIP_TRY(ctx, hipMemsetAsync(mlp->loss, 0, 2 * sizeof(float), ctx->stream));
+    const unsigned tiles = (unsigned) ((n + MLP_TILE - 1) / MLP_TILE);
+    hipLaunchKernelGGL(k_mlp_weight_sum, dim3(tiles), dim3(64), 0, ctx->stream, w, n, mlp->loss + 1);
+    HIP_TRY(ctx, hipGetLastError());
     HIP_TRY(ctx, hipMemsetAsync(mlp->g, 0, MLP_PARAMS * sizeof(float), ctx->stream));
-    HIP_TRY(ctx, hipMemsetAsync(mlp->loss, 0, sizeof(float), ctx->stream));
-    hipLaunchKernelGGL(k_mlp_grad, dim3((unsigned) ((n + MLP_TILE - 1) / MLP_TILE)), dim3(256), 0, ctx->stream, mlp->p,
-                       mlp->wb, x, uv, n, 1.f / (float) n, mlp->g, mlp->loss);
+    hipLaunchKernelGGL(k_mlp_grad, dim3(tiles), dim3(256), 0, ctx->stream, mlp->p, mlp->wb, x, uv, w, n,
+                       (const float *) (mlp->loss + 1), mlp->g, mlp->loss);
     HIP_TRY(ctx, hipGetLastError());
     if (grad)
         HIP_TRY(ctx, hipMemcpyAsync(grad, mlp->g, MLP_PARAMS * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
     if (loss) {
-        float s = 0.f;
-        HIP_TRY(ctx, hipMemcpyAsync(&s, mlp->loss, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        float s[2] = {0.f, 0.f};
+        HIP_TRY(ctx, hipMemcpyAsync(s, mlp->loss, 2 * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-        *loss = s / (float) n;
+        *loss = s[0] / s[1];
     } else if (grad) {
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     }
@@ -309,8 +315,11 @@ sms_status sms_ctx_set_proposal(sms_ctx *ctx, sms_mlp *mlp, int32_t shape, float
     return SMS_OK;
 }
 
-sms_status sms_ctx_set_training_capture(sms_ctx *ctx, int64_t capacity) {
+sms_status sms_ctx_set_training_capture(sms_ctx *ctx, int64_t capacity, int32_t mode) {
     if (!ctx || capacity < 0) return fail(ctx, SMS_E_INVALID, "sms_ctx_set_training_capture: bad capacity");
+    if (mode != SMS_CAPTURE_SOLUTIONS && mode != SMS_CAPTURE_ESTIMATES)
+        return fail(ctx, SMS_E_INVALID, "sms_ctx_set_training_capture: unknown mode");
+    ctx->cap_mode = mode;
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     if (ctx->cap) { (void) hipFree(ctx->cap); ctx->cap = nullptr; }
     if (ctx->cap_counts) { (void) hipFree(ctx->cap_counts); ctx->cap_counts = nullptr; }
@@ -324,7 +333,7 @@ sms_status sms_ctx_set_training_capture(sms_ctx *ctx, int64_t capacity) {
     return SMS_OK;
 }
 
-sms_status sms_ctx_training_samples(sms_ctx *ctx, int64_t *count, float *x, float *uv) {
+sms_status sms_ctx_training_samples(sms_ctx *ctx, int64_t *count, float *x, float *uv, float *w) {
     if (!ctx || !count) return SMS_E_INVALID;
     *count = 0;
     if (!ctx->cap_sub) return SMS_OK;
@@ -337,7 +346,7 @@ sms_status sms_ctx_training_samples(sms_ctx *ctx, int64_t *count, float *x, floa
     *count = total;
     if (x && uv && total > 0) {
         hipLaunchKernelGGL(k_capture_gather, dim3(SPREAD), dim3(256), 0, ctx->stream, ctx->cap, ctx->cap_counts,
-                           (unsigned long long) ctx->cap_sub, x, uv);
+                           (unsigned long long) ctx->cap_sub, x, uv, w);
         HIP_TRY(ctx, hipGetLastError());
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     }

@@ -3,7 +3,7 @@
 //
 //   k_mlp_infer   persistent over 64-sample tiles: weights loaded to LDS once per block, then
 //                 features -> 4 ReLU layers -> 48 raw mixture outputs per sample
-//   k_mlp_grad    one block per 64-sample tile of a batch: forward, mixture negative
+//   k_mlp_grad    one block per 64-sample tile of a batch: forward, weighted mixture negative
 //                 log-likelihood of the target uv, backward through the five layers on the
 //                 matrix cores; weight gradients are added to the FP32 gradient vector
 //   k_mlp_adam    Adam on the FP32 master parameters, refreshes the BF16 copy, clears the
@@ -88,9 +88,20 @@ SMS_DEV void mlp_input_grad(const uint16_t *dy, int dys, const uint16_t *w, cons
     }
 }
 
+// sum of the batch's sample weights (w null: the sample count), one 64-sample tile per block
+__global__ void __launch_bounds__(64) k_mlp_weight_sum(const float *__restrict__ w, int64_t n, float *__restrict__ wsum) {
+    const int64_t i = (int64_t) blockIdx.x * MLP_TILE + threadIdx.x;
+    float v = i < n ? (w ? w[i] : 1.f) : 0.f;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    if (threadIdx.x == 0) atomicAdd(wsum, v);
+}
+
+// Weighted negative log-likelihood: sample j's head gradient is scaled by w_j / sum(w);
+// loss_sum accumulates sum_j w_j NLL_j.
 __global__ void __launch_bounds__(256) k_mlp_grad(const float *__restrict__ p, const uint16_t *__restrict__ wb,
-                                                  const float *__restrict__ x9, const float *__restrict__ uv, int64_t n,
-                                                  float scale, float *__restrict__ grad, float *__restrict__ loss_sum) {
+                                                  const float *__restrict__ x9, const float *__restrict__ uv,
+                                                  const float *__restrict__ w, int64_t n, const float *__restrict__ wsum,
+                                                  float *__restrict__ grad, float *__restrict__ loss_sum) {
     __shared__ MlpGradLds G;
     MlpLds &L = G.f;
     mlp_load_weights(wb, p, L);
@@ -99,12 +110,17 @@ __global__ void __launch_bounds__(256) k_mlp_grad(const float *__restrict__ p, c
     __syncthreads();
     mlp_forward_tile(L);
     // mixture head: loss and d loss / d raw outputs (BF16 rows of dy, stride MLP_H)
+    const float inv_wsum = 1.f / *wsum;
     for (int s = threadIdx.x; s < MLP_TILE; s += blockDim.x) {
         const int64_t i = base + s;
         float d_o[MLP_OUT];
         float l = 0.f;
-        if (i < n) l = gmm_nll_grad(&L.o[s][0], uv[2 * i], uv[2 * i + 1], scale, d_o);
-        else for (int c = 0; c < MLP_OUT; ++c) d_o[c] = 0.f;
+        if (i < n) {
+            const float wi = w ? w[i] : 1.f;
+            l = wi * gmm_nll_grad(&L.o[s][0], uv[2 * i], uv[2 * i + 1], wi * inv_wsum, d_o);
+        } else {
+            for (int c = 0; c < MLP_OUT; ++c) d_o[c] = 0.f;
+        }
         for (int c = 0; c < MLP_OUT; ++c) G.dy[s][c] = to_bf16(d_o[c]);
         G.loss[s] = l;
     }

@@ -159,6 +159,7 @@ EXPORTS = ["sms_config_default", "sms_ctx_create", "sms_ctx_destroy", "sms_ctx_s
 MLP_INPUTS = 9
 MLP_OUTPUTS = 48
 MLP_PARAMS = 25904
+CAPTURE_SOLUTIONS, CAPTURE_ESTIMATES = 0, 1
 
 
 def load_library(path=LIB_PATH):
@@ -198,11 +199,11 @@ def load_library(path=LIB_PATH):
     lib.sms_mlp_get_params.argtypes = [vp, vp]
     lib.sms_mlp_set_params.argtypes = [vp, vp]
     lib.sms_mlp_infer.argtypes = [vp, vp, i64, vp]
-    lib.sms_mlp_grad.argtypes = [vp, vp, vp, i64, vp, C.POINTER(C.c_float)]
+    lib.sms_mlp_grad.argtypes = [vp, vp, vp, vp, i64, vp, C.POINTER(C.c_float)]
     lib.sms_mlp_adam.argtypes = [vp, C.c_float]
     lib.sms_ctx_set_proposal.argtypes = [vp, vp, C.c_int32, C.c_float]
-    lib.sms_ctx_set_training_capture.argtypes = [vp, i64]
-    lib.sms_ctx_training_samples.argtypes = [vp, C.POINTER(i64), vp, vp]
+    lib.sms_ctx_set_training_capture.argtypes = [vp, i64, C.c_int32]
+    lib.sms_ctx_training_samples.argtypes = [vp, C.POINTER(i64), vp, vp, vp]
     lib.sms_ctx_clear_training.argtypes = [vp]
     lib.sms_ss_proposal_inputs.argtypes = [vp, vp, i64, C.c_uint64, vp]
     lib.sms_ss_proposal_outputs.argtypes = [vp, vp, i64, C.c_uint64, vp]

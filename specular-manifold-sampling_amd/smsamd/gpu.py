@@ -220,36 +220,42 @@ class Context:
         _check(lib().sms_ctx_set_proposal(self.h, mlp.h if mlp is not None else None, shape, defensive), self.h,
                "sms_ctx_set_proposal")
 
-    def set_training_capture(self, capacity):
-        _check(lib().sms_ctx_set_training_capture(self.h, capacity), self.h, "sms_ctx_set_training_capture")
+    def set_training_capture(self, capacity, mode=abi.CAPTURE_SOLUTIONS):
+        _check(lib().sms_ctx_set_training_capture(self.h, capacity, mode), self.h, "sms_ctx_set_training_capture")
 
     def training_count(self):
         n = C.c_int64(0)
-        _check(lib().sms_ctx_training_samples(self.h, C.byref(n), None, None), self.h, "sms_ctx_training_samples")
+        _check(lib().sms_ctx_training_samples(self.h, C.byref(n), None, None, None), self.h,
+               "sms_ctx_training_samples")
         return int(n.value)
 
     def training_samples_device(self):
-        """(x buffer, uv buffer, count) of the captured samples, left in device memory."""
+        """(x buffer, uv buffer, weight buffer, count) of the captured samples, in device memory."""
         n = self.training_count()
         xb = DeviceBuffer(self, max(n * abi.MLP_INPUTS * 4, 4))
         ub = DeviceBuffer(self, max(n * 8, 4))
+        wb = DeviceBuffer(self, max(n * 4, 4))
         m = C.c_int64(0)
-        _check(lib().sms_ctx_training_samples(self.h, C.byref(m), xb.ptr, ub.ptr), self.h, "sms_ctx_training_samples")
+        _check(lib().sms_ctx_training_samples(self.h, C.byref(m), xb.ptr, ub.ptr, wb.ptr), self.h,
+               "sms_ctx_training_samples")
         assert m.value == n
-        return xb, ub, n
+        return xb, ub, wb, n
 
     def training_samples(self):
-        xb, ub, n = self.training_samples_device()
+        xb, ub, wb, n = self.training_samples_device()
         try:
             x = np.zeros((n, abi.MLP_INPUTS), np.float32)
             uv = np.zeros((n, 2), np.float32)
+            w = np.zeros(n, np.float32)
             if n:
                 xb.download(x)
                 ub.download(uv)
+                wb.download(w)
         finally:
             xb.free()
             ub.free()
-        return x, uv
+            wb.free()
+        return x, uv, w
 
     def clear_training(self):
         _check(lib().sms_ctx_clear_training(self.h), self.h, "sms_ctx_clear_training")
@@ -323,27 +329,32 @@ class Mlp:
             ob.free()
         return out
 
-    def grad_device(self, xbuf, uvbuf, n, want_grad=False, want_loss=True, offset=0):
-        """Gradient of the mean mixture NLL over samples [offset, offset + n) of device buffers."""
+    def grad_device(self, xbuf, uvbuf, n, want_grad=False, want_loss=True, offset=0, wbuf=None):
+        """Gradient of the (weighted) mean mixture NLL over samples [offset, offset + n) of device
+        buffers; wbuf None: unit weights."""
         g = np.zeros(abi.MLP_PARAMS, np.float32) if want_grad else None
         loss = C.c_float(0.0)
-        _check(lib().sms_mlp_grad(self.h, xbuf.ptr + 4 * abi.MLP_INPUTS * offset, uvbuf.ptr + 8 * offset, n,
+        _check(lib().sms_mlp_grad(self.h, xbuf.ptr + 4 * abi.MLP_INPUTS * offset, uvbuf.ptr + 8 * offset,
+                                  wbuf.ptr + 4 * offset if wbuf is not None else None, n,
                                   g.ctypes.data if want_grad else None, C.byref(loss) if want_loss else None),
                self.ctx.h, "sms_mlp_grad")
         return (float(loss.value) if want_loss else None), g
 
-    def grad(self, x, uv):
+    def grad(self, x, uv, w=None):
         x = np.ascontiguousarray(x, np.float32).reshape(-1, abi.MLP_INPUTS)
         uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
-        xb = DeviceBuffer(self.ctx, x.nbytes)
-        ub = DeviceBuffer(self.ctx, uv.nbytes)
+        bufs = [DeviceBuffer(self.ctx, x.nbytes), DeviceBuffer(self.ctx, uv.nbytes)]
         try:
-            xb.upload(x)
-            ub.upload(uv)
-            return self.grad_device(xb, ub, len(x), want_grad=True)
+            bufs[0].upload(x)
+            bufs[1].upload(uv)
+            if w is not None:
+                w = np.ascontiguousarray(w, np.float32).reshape(-1)
+                bufs.append(DeviceBuffer(self.ctx, w.nbytes))
+                bufs[2].upload(w)
+            return self.grad_device(bufs[0], bufs[1], len(x), want_grad=True, wbuf=bufs[2] if w is not None else None)
         finally:
-            xb.free()
-            ub.free()
+            for b in bufs:
+                b.free()
 
     def adam(self, lr):
         _check(lib().sms_mlp_adam(self.h, lr), self.ctx.h, "sms_mlp_adam")

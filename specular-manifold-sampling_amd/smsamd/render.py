@@ -77,26 +77,34 @@ class PassRunner:
 
 
 def train_online(ctx, runner, film, mlp, caster, samples=2_000_000, batch=8192, lr=1e-3, seed0=1 << 20,
-                 stats=None, rng_seed=0, epochs=1):
-    """Render uniform-seed passes into `film` while capturing the converged solutions they find
-    on mesh `caster`; after each pass, `epochs` x (captures // batch) Adam steps on batches of
-    that pass's captures (windows of `batch` consecutive samples at random offsets), until
-    `samples` captured samples were trained on.  Returns (passes, steps, losses)."""
+                 stats=None, rng_seed=0, epochs=1, mode=abi.CAPTURE_SOLUTIONS, weight_power=1.0):
+    """Render uniform-seed passes into `film` while capturing training records on mesh
+    `caster` (`mode`: every converged solution with weight 1, or the solutions behind non-zero
+    estimates weighted by the estimate, raised to `weight_power`); after each pass,
+    `epochs` x (records // batch) Adam steps on windows of `batch` consecutive records at random
+    offsets of that pass's capture, until `samples` records were trained on.
+    Returns (passes, steps, losses)."""
     rng = np.random.default_rng(rng_seed)
     ctx.set_proposal(None, caster)
-    ctx.set_training_capture(runner.n)
+    ctx.set_training_capture(runner.n, mode)
     used, steps, passes, losses = 0, 0, 0, []
+    weighted = mode == abi.CAPTURE_ESTIMATES
     try:
         while used < samples:
             runner.run(seed0 + passes, film, stats)
             passes += 1
-            xb, ub, have = ctx.training_samples_device()
+            xb, ub, wb, have = ctx.training_samples_device()
             ctx.clear_training()
             try:
+                if weighted and weight_power != 1.0 and have:
+                    w = np.zeros(have, np.float32)
+                    wb.download(w)
+                    wb.upload(np.power(np.maximum(w, 0.0), weight_power).astype(np.float32))
                 n_new = min(have, samples - used) // batch * batch
                 for _ in range(epochs * (n_new // batch)):
                     off = int(rng.integers(0, have - batch + 1))
-                    loss, _ = mlp.grad_device(xb, ub, batch, want_loss=(steps % 32 == 0), offset=off)
+                    loss, _ = mlp.grad_device(xb, ub, batch, want_loss=(steps % 32 == 0), offset=off,
+                                              wbuf=wb if weighted else None)
                     if loss is not None:
                         losses.append(loss)
                     mlp.adam(lr)
@@ -105,6 +113,7 @@ def train_online(ctx, runner, film, mlp, caster, samples=2_000_000, batch=8192, 
             finally:
                 xb.free()
                 ub.free()
+                wb.free()
             if have < batch and passes > 1000:
                 raise RuntimeError("the caster yields too few converged solutions to train on")
     finally:

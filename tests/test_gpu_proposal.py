@@ -154,6 +154,15 @@ def test_mlp_gradient(gpu, pool_scene):
     for i, ((n, k), (w0, b0)) in enumerate(zip(M.LAYERS, M.OFFSETS)):
         sl = slice(w0, b0 + n)
         assert cos(g[sl], g_bf[sl]) > 0.995, i
+    # weighted negative log-likelihood (heavy-tailed weights, as estimates are)
+    wts = np.random.default_rng(9).lognormal(0.0, 1.5, len(x)).astype(np.float32)
+    loss_w, gw = mlp.grad(x, uv, wts)
+    lw_bf, gw_bf = M.loss_and_grad(p, x, uv, "bf16", wts)
+    lw_64, gw_64 = M.loss_and_grad(p, x, uv, "f64", wts)
+    print(f"weighted: loss gpu {loss_w:.6f} bf16 {lw_bf:.6f} f64 {lw_64:.6f}; grad rel err vs bf16 "
+          f"{rel(gw.astype(np.float64), gw_bf):.2e} vs f64 {rel(gw.astype(np.float64), gw_64):.2e}", flush=True)
+    assert abs(loss_w - lw_bf) < 1e-3 * abs(lw_bf)
+    assert rel(gw.astype(np.float64), gw_bf) < 2e-2 and cos(gw.astype(np.float64), gw_64) > 0.99
     mlp.close()
     ctx.close()
 
@@ -237,7 +246,8 @@ def test_capture(gpu, oracle_mod, pool_scene):
     ctx.set_training_capture(1 << 16)
     q = _floor_queries(pool_scene, 4000, 5)
     rg, _ = ctx.ss_estimate(q, 3)
-    x, uv = ctx.training_samples()
+    x, uv, w = ctx.training_samples()
+    assert (w == 1).all()
     xin = ctx.proposal_inputs(q, 3)
     rows = {r.tobytes(): i for i, r in enumerate(xin)}
     idx = np.array([rows.get(r.tobytes(), -1) for r in x])
@@ -262,6 +272,15 @@ def test_capture(gpu, oracle_mod, pool_scene):
     assert ctx.training_count() > n0
     ctx.clear_training()
     assert ctx.training_count() == 0
+    # estimate-weighted records: one per non-zero estimate, weight = the estimate's RGB mean
+    ctx.set_training_capture(1 << 16, abi.CAPTURE_ESTIMATES)
+    rg, _ = ctx.ss_estimate(q, 3)
+    x, uv, w = ctx.training_samples()
+    idx = np.array([rows.get(r.tobytes(), -1) for r in x])
+    nz = np.nonzero(rg[:, 0] > 0)[0]
+    assert (idx >= 0).all() and sorted(idx.tolist()) == sorted(nz.tolist())
+    np.testing.assert_allclose(w, rg[idx].mean(1), rtol=1e-5)
+    ctx.set_training_capture(0)
     ctx.close()
     orc.close()
 
@@ -279,7 +298,8 @@ def test_training_beats_uniform(gpu, pool_scene):
     while ctx.training_count() < 150000 and seed < 200:
         ctx.ss_estimate(_floor_queries(pool_scene, 65536, seed), seed)
         seed += 1
-    xb, ub, n = ctx.training_samples_device()
+    xb, ub, wb, n = ctx.training_samples_device()
+    wb.free()
     mlp = gpu.Mlp(ctx, seed=3)
     losses = []
     rng = np.random.default_rng(0)

@@ -288,3 +288,18 @@ def test_mixture_estimator_is_unbiased(small_pool):
     # the concentrated proposal converges more often per walk
     assert sm["walks_converged"] / sm["walks"] > su["walks_converged"] / su["walks"]
     orc.close()
+
+
+def test_weighted_gradient_matches_torch_and_reduces_to_unweighted():
+    """Weighted NLL: unit weights give the unweighted gradient; integer weights equal repeating
+    samples; the weighted gradient matches torch autograd of the weighted loss."""
+    pytest.importorskip("torch")
+    p = M.init_like_library(4).astype(np.float64)
+    x, uv = _batch(48, 11)
+    l1, g1 = M.loss_and_grad(p, x, uv, "f64")
+    lu, gu = M.loss_and_grad(p, x, uv, "f64", np.ones(48))
+    assert abs(l1 - lu) < 1e-12 and np.abs(g1 - gu).max() < 1e-12
+    reps = np.random.default_rng(3).integers(1, 4, 48)
+    lw, gw = M.loss_and_grad(p, x, uv, "f64", reps.astype(np.float64))
+    lr_, gr = M.loss_and_grad(p, np.repeat(x, reps, 0), np.repeat(uv, reps, 0), "f64")
+    assert abs(lw - lr_) < 1e-10 and np.abs(gw - gr).max() < 1e-10 * max(1.0, np.abs(gr).max())
