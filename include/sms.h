@@ -280,8 +280,10 @@ sms_status sms_ctx_set_config(sms_ctx *ctx, const sms_config *cfg);
    64-trial rounds of the one-wavefront pass before a chain moves to the multi-wavefront stage
    (0 = straight to it); long_chains / long_trials (glints): chains the multi-wavefront stage
    holds (at most 1024) and trials it runs per chain (a multiple of 64, at most 32768); the
-   rest continue one wavefront per chain.  No counterpart in the reference (its trial loops
-   are serial). */
+   rest continue one wavefront per chain; ss_first_pass (single scatter, unbiased): 0 = the
+   persistent walker, whose lanes take their next walk as soon as one ends, 1 = one lane per
+   query for its first walk and up to lane_trials trials.  No counterpart in the reference (its
+   trial loops are serial). */
 typedef struct sms_schedule {
     int32_t lane_trials;
     int32_t blocks_per_cu;
@@ -289,7 +291,8 @@ typedef struct sms_schedule {
     int32_t tail_rounds;
     int32_t long_chains;
     int32_t long_trials;
-    int32_t pad[2];
+    int32_t ss_first_pass;
+    int32_t pad;
 } sms_schedule;
 sms_status sms_ctx_set_schedule(sms_ctx *ctx, const sms_schedule *s);
 sms_status sms_ctx_get_schedule(const sms_ctx *ctx, sms_schedule *s);

@@ -32,10 +32,12 @@ struct sms_ctx {
     int trial_group = 8;
     int tail_rounds = 2;
     int long_chains = 1024, long_trials = 32768;
+    int ss_first_pass = 0;                      // 0: persistent walker, 1: lane per query
     // glint tail lists and the multi-wavefront stage (allocated with the glint work buffers)
     OverflowRec *long_items = nullptr; OverflowRec *seq_items = nullptr;
     long long *mw_found = nullptr; unsigned long long *mw_next = nullptr; uint16_t *mw_log = nullptr;
     int mw_items = 0, mw_trials = 0;
+    int occ_walker = 0;
     int occ_glint = 0, occ_ss = 0, occ_coop = 0, occ_biased = 0, occ_sscoop = 0, occ_mw = 0;   // occ_glint: trial pass
     int occ_ms[2] = {0, 0}, occ_mscoop[2] = {0, 0}, occ_msbiased[2] = {0, 0};   // [0]: chains <= 2, [1]: <= 4
     // host copies the learned proposal's UV grid is built from

@@ -148,6 +148,8 @@ __global__ void k_glint_multiwave_resolve(DScene S, EstimatorWork W);
 __global__ void k_glint_coop_seq(DScene S, uint64_t seed, EstimatorWork W);
 __global__ void k_glint_finalize(DScene S, const sms_glint_query *Q, uint64_t seed, float *out, EstimatorWork W);
 __global__ void k_ss_estimate(DScene S, const sms_ss_query *Q, int64_t n, uint64_t seed, float *part, float *invp, EstimatorWork W);
+__global__ void k_ss_walker(DScene S, const sms_ss_query *Q, int64_t n, uint64_t seed, float *part, float *invp,
+                            EstimatorWork W);
 __global__ void k_ss_coop(DScene S, const sms_ss_query *Q, uint64_t seed, const float *part, float *invp, EstimatorWork W);
 __global__ void k_ss_finalize(const float *part, const float *invp, int nc, int64_t n, float *out);
 __global__ void k_ss_queries(DScene S, int spp, uint64_t seed, sms_ss_query *q, float *w, int64_t n);

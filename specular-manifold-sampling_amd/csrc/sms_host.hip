@@ -553,6 +553,7 @@ sms_schedule schedule_of(const sms_ctx *ctx) {
     s.lane_trials = ctx->lane_trials; s.blocks_per_cu = ctx->blocks_per_cu;
     s.trial_group = ctx->trial_group; s.tail_rounds = ctx->tail_rounds;
     s.long_chains = ctx->long_chains; s.long_trials = ctx->long_trials;
+    s.ss_first_pass = ctx->ss_first_pass;
     return s;
 }
 
@@ -560,6 +561,7 @@ void apply_schedule(sms_ctx *ctx, const sms_schedule &s) {
     ctx->lane_trials = s.lane_trials; ctx->blocks_per_cu = s.blocks_per_cu;
     ctx->trial_group = s.trial_group; ctx->tail_rounds = s.tail_rounds;
     ctx->long_chains = s.long_chains; ctx->long_trials = s.long_trials;
+    ctx->ss_first_pass = s.ss_first_pass;
 }
 
 EstimatorWork work_of(sms_ctx *ctx) {
@@ -655,6 +657,7 @@ sms_status sms_ctx_create(const sms_scene_desc *scene, const sms_config *cfg, sm
     if (!st) st = occupancy(ctx, k_glint_coop_rounds, &ctx->occ_coop);
     if (!st) st = occupancy(ctx, k_glint_multiwave, &ctx->occ_mw);
     if (!st) st = occupancy(ctx, k_ss_estimate, &ctx->occ_ss);
+    if (!st) st = occupancy(ctx, k_ss_walker, &ctx->occ_walker);
     if (!st) st = occupancy(ctx, k_ss_coop, &ctx->occ_sscoop);
     if (!st) st = occupancy(ctx, k_ss_biased, &ctx->occ_biased);
     if (!st) st = occupancy(ctx, k_ms_estimate_2, &ctx->occ_ms[0]);
@@ -715,7 +718,7 @@ sms_status sms_ctx_set_schedule(sms_ctx *ctx, const sms_schedule *s) {
     if (s->lane_trials < 1 || s->blocks_per_cu < 0 || s->tail_rounds < 0 ||
         !(g == 0 || g == 1 || g == 2 || g == 4 || g == 8 || g == 16) ||
         s->long_chains < 0 || s->long_chains > MW_ITEMS || s->long_trials < 64 || s->long_trials > MW_TRIALS ||
-        s->long_trials % 64 != 0)
+        s->long_trials % 64 != 0 || s->ss_first_pass < 0 || s->ss_first_pass > 1)
         return fail(ctx, SMS_E_INVALID, "bad schedule");
     sms_schedule v = *s;
     if (v.trial_group == 0) v.trial_group = 8;
@@ -919,8 +922,15 @@ sms_status sms_ss_estimate_batch(sms_ctx *ctx, const sms_ss_query *q, int64_t n,
             HIP_TRY(ctx, hipGetLastError());
             HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
         } else {
-            hipLaunchKernelGGL(k_ss_estimate, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream,
-                               ctx->scene, q, n, seed, ctx->part, ctx->invp, W);
+            if (ctx->ss_first_pass == 0) {
+                // persistent: the resident blocks take queries from the device queue
+                const int per_cu = ctx->blocks_per_cu > 0 ? ctx->blocks_per_cu : ctx->occ_walker;
+                hipLaunchKernelGGL(k_ss_walker, dim3((unsigned) (per_cu * ctx->n_cu)), dim3(256), 0, ctx->stream,
+                                   ctx->scene, q, n, seed, ctx->part, ctx->invp, W);
+            } else {
+                hipLaunchKernelGGL(k_ss_estimate, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                                   ctx->scene, q, n, seed, ctx->part, ctx->invp, W);
+            }
             HIP_TRY(ctx, hipGetLastError());
             HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
             hipLaunchKernelGGL(k_ss_coop, dim3(grid_for(ctx, ctx->occ_sscoop)), dim3(256), 0, ctx->stream,

@@ -1,0 +1,260 @@
+// sms_ss_walker.hip -- first pass of the unbiased single-scatter estimator
+// (SpecularManifoldSingleScatter::specular_manifold_sampling, manifold_ss.cpp:28-141) as a
+// persistent walker.
+//
+// Every lane holds one walk and advances it by exactly one ray cast per loop trip: the seed
+// ray of sample_path (manifold_ss.cpp:243-249), a Newton reprojection (:369-371) or the
+// shadow ray of the first solution (:96-103).  A lane whose walk ends takes its next walk in
+// the same trip -- the chain's next Bernoulli trial, the next caster, or the next query from
+// the wavefront's share of a device queue -- so the 64 lanes stay busy while walks of 0 to 20
+// iterations and chains of geometric length end at different times.  All lanes share one
+// ray-cast site, one surface-interaction build and one Newton-step evaluation per trip.
+//
+// Walks, random streams and arithmetic are those of sample_path_ss / k_ss_estimate, so every
+// estimate, solution and counter is bit-identical; only the order in which walks run differs.
+// Chains still unresolved after `lane_trials` trials go to the spread overflow list for the
+// cooperative pass, with the same record layout.
+#include "kernels.h"
+#include "dev_ss_est.h"
+
+namespace smsdev {
+
+namespace {
+
+enum : int { WK_NONE = 0, WK_SEED = 1, WK_NEWTON = 2, WK_VIS = 3 };
+constexpr int WK_CHUNK = 64;   // queries a wavefront claims from the queue at a time
+
+struct WalkerLane {
+    // query
+    int64_t q;
+    SI si;
+    V3 wi_local;
+    EI ei;
+    uint64_t qkey;
+    // chain (query, caster)
+    int c, shape;
+    int64_t slot, k;
+    V3 direction;
+    SI si_first;                 // first solution, for the contribution after the shadow ray
+    // walk
+    int stage;
+    MV vtx;
+    SI si_cur;
+    float beta;
+    int its;
+    // counters
+    unsigned long long walks, conv, iters, bcalls, btrials, bmax;
+};
+
+} // namespace
+
+__global__ void __launch_bounds__(256) k_ss_walker(DScene S, const sms_ss_query *__restrict__ Q, int64_t n,
+                                                   uint64_t seed, float *__restrict__ part, float *__restrict__ invp,
+                                                   EstimatorWork W) {
+    const sms_config &cfg = S.cfg;
+    const int nc = S.n_casters_ss;
+    const int lane = threadIdx.x & 63;
+    const V3 n_offset = v3(0.f, 0.f, 1.f);
+    const Proposal &P = W.prop;
+
+    WalkerLane L;
+    L.stage = WK_NONE; L.q = -1; L.c = 0; L.shape = -1; L.slot = 0; L.k = 0; L.its = 0; L.beta = 1.f;
+    L.walks = L.conv = L.iters = L.bcalls = L.btrials = L.bmax = 0;
+    L.qkey = 0;
+    // wavefront-uniform query pool
+    int64_t pool_next = 0, pool_end = 0;
+    bool exhausted = false;
+
+    // ---- chain / walk transitions (no ray casts)
+    auto begin_chain = [&]() {
+        L.slot = L.q * nc + L.c;
+        part[3 * L.slot] = 0.f; part[3 * L.slot + 1] = 0.f; part[3 * L.slot + 2] = 0.f;
+        invp[L.slot] = 0.f;
+        L.shape = S.casters_ss[L.c];
+        L.k = 0;
+        L.stage = WK_SEED;
+    };
+    auto end_chain = [&]() {
+        L.c++;
+        if (L.c < nc) begin_chain();
+        else L.stage = WK_NONE;
+    };
+    auto resolve = [&](float inv_prob) {
+        const unsigned long long tr = (unsigned long long) L.k;
+        L.btrials += tr; L.bmax = L.bmax > tr ? L.bmax : tr;
+        invp[L.slot] = inv_prob;
+        const float wgt = mean3(&part[3 * L.slot]) * inv_prob;
+        if (wgt > 0.f && capturing(P, L.shape, SMS_CAPTURE_ESTIMATES))
+            capture_sample(S, P, L.si, L.ei, L.si_first.uv, wgt);
+        end_chain();
+    };
+    // a walk ended: ok, its final surface interaction, its iterations
+    auto finish_walk = [&](bool ok, const SI &si_final, int iterations) {
+        L.walks++; L.iters += (unsigned long long) iterations;
+        if (ok) L.conv++;
+        if (ok && capturing(P, L.shape, SMS_CAPTURE_SOLUTIONS)) capture_sample(S, P, L.si, L.ei, si_final.uv, 1.f);
+        if (L.k == 0) {
+            if (!ok) { end_chain(); return; }
+            L.direction = normalize(si_final.p - L.si.p);
+            L.si_first = si_final;
+            L.stage = WK_VIS;
+            return;
+        }
+        V3 dtr = normalize(si_final.p - L.si.p);
+        if (ok && fabsf(dot(L.direction, dtr) - 1.f) < cfg.uniqueness_threshold) { resolve((float) L.k); return; }
+        L.k++;
+        if (cfg.max_trials > 0 && L.k > cfg.max_trials) { resolve(0.f); return; }
+        if (L.k > W.lane_trials) {
+            const int sub = (int) ((L.q / 256) % SPREAD);
+            unsigned long long o = spread_append(W.over, sub, 1ull);
+            OverflowRec rec;
+            rec.q = L.slot; rec.k = L.k;
+            rec.sol[0] = L.direction.x; rec.sol[1] = L.direction.y; rec.sol[2] = L.direction.z;
+            rec.shapes[0] = __float_as_int(L.si_first.uv.x); rec.shapes[1] = __float_as_int(L.si_first.uv.y);
+            rec.shapes[2] = rec.shapes[3] = 0;
+            W.over.items[(size_t) sub * W.over.sub_cap + o] = rec;
+            end_chain();
+            return;
+        }
+        L.stage = WK_SEED;
+    };
+    auto start_query = [&](int64_t q) {
+        L.q = q;
+        const sms_ss_query g = Q[q];
+        if (g.shape < 0 || nc == 0) {          // no receiver: zero estimate, no walks
+            for (int c = 0; c < nc; ++c) {
+                const int64_t s = q * nc + c;
+                part[3 * s] = 0.f; part[3 * s + 1] = 0.f; part[3 * s + 2] = 0.f;
+                invp[s] = 0.f;
+            }
+            L.stage = WK_NONE;
+            return;
+        }
+        L.si = ss_query_si(g);
+        L.wi_local = ld3(g.wi);
+        L.qkey = query_key(seed, (uint64_t) q);
+        Pcg32 rq;
+        stream_seed(rq, L.qkey, 0);
+        L.ei = sample_emitter_interaction(S, L.si.p, S.emitters_ss, S.n_emitters_ss, rq);
+        L.c = 0;
+        begin_chain();
+    };
+
+    while (true) {
+        Ray ray;
+        ray.o = ray.d = ray.d_rcp = v3(0.f, 0.f, 0.f); ray.mint = 0.f; ray.maxt = 0.f;
+        bool has_ray = false, any_hit = false;
+        while (true) {
+            // advance this lane's state machine to its next ray (or to needing a query)
+            while (!has_ray && L.stage != WK_NONE) {
+                if (L.stage == WK_SEED) {
+                    // ss_walk + the seed of sample_path_ss
+                    Pcg32 rw;
+                    stream_seed(rw, L.qkey, ((uint64_t) L.c << 32) + (uint64_t) L.k + 1);
+                    V2 s = v2(0.f, 0.f);
+                    V3 p_start = v3(0.f, 0.f, 0.f);
+                    if (P.gmm && L.shape == P.shape) {
+                        if (!proposal_seed(S, P, P.gmm + L.q * PROP_RAW, rw, s, p_start)) {
+                            finish_walk(false, si_invalid(), 0);
+                            continue;
+                        }
+                    } else {
+                        s.x = pcg32_float(rw); s.y = pcg32_float(rw);
+                    }
+                    PS ps = sample_position(S, L.shape, s);
+                    if (p_start.x != 0.f || p_start.y != 0.f || p_start.z != 0.f) ps.p = p_start;
+                    V3 d_tmp = normalize(ps.p - L.si.p);
+                    if (cfg.mnee_init) {
+                        const DShape &sh = S.shapes[L.shape];
+                        if (sh.k != 0.f) d_tmp = normalize(ld3(sh.bbox_center) - L.si.p);
+                        else d_tmp = normalize(L.ei.p - L.si.p);
+                    }
+                    ray = make_ray(L.si.p, d_tmp);
+                    has_ray = true; any_hit = false;
+                } else if (L.stage == WK_NEWTON) {
+                    // one iteration of newton_solver (manifold_ss.cpp:329-405)
+                    if (L.its >= cfg.max_iterations) { finish_walk(false, L.si_cur, L.its); continue; }
+                    Step st = cfg.halfvector_constraints ? step_halfvector(L.si.p, L.vtx, L.ei, n_offset)
+                                                         : step_anglediff(L.si.p, L.vtx, L.ei, n_offset);
+                    if (!st.ok) { finish_walk(false, L.si_cur, L.its); continue; }
+                    if (norm2(st.C) < cfg.solver_threshold) {
+                        V3 wx = normalize(L.si.p - L.vtx.p);
+                        V3 wy = L.ei.kind == SMS_EI_DIRECTIONAL ? L.ei.d : normalize(L.ei.p - L.vtx.p);
+                        float cos_theta_x = dot(L.vtx.gn, wx), cos_theta_y = dot(L.vtx.gn, wy);
+                        bool refraction = cos_theta_x * cos_theta_y < 0.f, reflection = !refraction;
+                        bool type_ok = !((L.vtx.eta == 1.f && !reflection) || (L.vtx.eta != 1.f && !refraction));
+                        finish_walk(type_ok && L.si_cur.valid, L.si_cur, L.its);
+                        continue;
+                    }
+                    V3 p_prop = L.vtx.p - cfg.step_scale * L.beta * (L.vtx.dp_du * st.dX.x + L.vtx.dp_dv * st.dX.y);
+                    V3 d_prop = normalize(p_prop - L.si.p);
+                    ray = make_ray(L.si.p, d_prop);
+                    has_ray = true; any_hit = false;
+                } else {
+                    // shadow ray of the first solution (ss_visible)
+                    V3 p = L.si_first.p;
+                    V3 d = p - L.ei.p;
+                    float dist = norm(d);
+                    d = d * (1.f / dist);
+                    ray = make_ray(L.ei.p, d, RAY_EPS_F * (1.f + hmax_abs(L.ei.p)), dist * (1.f - RAY_EPS_F));
+                    has_ray = true; any_hit = true;
+                }
+            }
+            // lanes without work take queries from the wavefront's pool (wavefront-uniform)
+            const unsigned long long req = __ballot(L.stage == WK_NONE && !has_ray);
+            if (!req || exhausted) break;
+            if (pool_next >= pool_end) {
+                unsigned long long base = 0;
+                if (lane == 0) base = atomicAdd(W.cursor, (unsigned long long) WK_CHUNK);
+                base = __shfl(base, 0, 64);
+                if ((int64_t) base >= n) { exhausted = true; break; }
+                pool_next = (int64_t) base;
+                pool_end = (int64_t) base + WK_CHUNK < n ? (int64_t) base + WK_CHUNK : n;
+            }
+            const int rank = __popcll(req & ((1ull << lane) - 1ull));
+            const int64_t avail = pool_end - pool_next;
+            if (((req >> lane) & 1ull) && rank < avail) start_query(pool_next + rank);
+            const int64_t taken = (int64_t) __popcll(req) < avail ? (int64_t) __popcll(req) : avail;
+            pool_next += taken;
+        }
+        if (!__ballot(has_ray)) break;
+        // one ray cast per lane
+        Hit h;
+        h.hit = false; h.t = __builtin_inff(); h.shape = 1 << 30; h.prim = 1 << 30; h.b1 = h.b2 = h.lx = h.ly = 0.f;
+        if (has_ray) h = scene_hit(S, ray, any_hit);
+        if (!has_ray) continue;
+        if (L.stage == WK_VIS) {
+            if (h.hit) { end_chain(); continue; }
+            // contribution of the first solution (manifold_ss.cpp:104-118)
+            V3 spec = path_contribution_ss(S, L.si, L.ei, L.si_first);
+            V3 dl = v3(dot(L.direction, L.si.sh_s), dot(L.direction, L.si.sh_t), dot(L.direction, L.si.sh_n));
+            V3 bsdf = receiver_bsdf_eval(S, L.si, L.wi_local, dl);
+            part[3 * L.slot] = bsdf.x * spec.x; part[3 * L.slot + 1] = bsdf.y * spec.y;
+            part[3 * L.slot + 2] = bsdf.z * spec.z;
+            L.bcalls++;
+            L.k = 1;
+            L.stage = WK_SEED;
+            continue;
+        }
+        // seed ray or reprojection: the surface interaction and, if it is on the walk's shape,
+        // its manifold vertex
+        SI hit_si = make_si(S, ray, h);
+        const int want = L.stage == WK_SEED ? L.shape : L.vtx.shape;
+        const bool on_shape = hit_si.valid && want == hit_si.shape;
+        if (on_shape) L.vtx = mv_from_si(S, hit_si);
+        if (L.stage == WK_SEED) {
+            if (!on_shape) { finish_walk(false, si_invalid(), 0); continue; }
+            L.beta = 1.f; L.its = 0; L.si_cur = si_invalid();
+            L.stage = WK_NEWTON;
+        } else {
+            L.si_cur = hit_si;
+            if (!on_shape) L.beta = 0.5f * L.beta;
+            else L.beta = fmin_(1.f, 2.f * L.beta);
+            L.its += 1;
+        }
+    }
+    stat_add(W.stats + 0, L.walks); stat_add(W.stats + 1, L.conv); stat_add(W.stats + 2, L.iters);
+    stat_add(W.stats + 3, L.bcalls); stat_add(W.stats + 4, L.btrials); stat_max(W.stats + 5, L.bmax);
+}
+
+} // namespace smsdev

@@ -84,9 +84,10 @@ class SmsStats(C.Structure):
 class SmsSchedule(C.Structure):
     _fields_ = [("lane_trials", C.c_int32), ("blocks_per_cu", C.c_int32), ("trial_group", C.c_int32),
                 ("tail_rounds", C.c_int32), ("long_chains", C.c_int32), ("long_trials", C.c_int32),
-                ("pad", C.c_int32 * 2)]
+                ("ss_first_pass", C.c_int32), ("pad", C.c_int32)]
 
-    KEYS = ("lane_trials", "blocks_per_cu", "trial_group", "tail_rounds", "long_chains", "long_trials")
+    KEYS = ("lane_trials", "blocks_per_cu", "trial_group", "tail_rounds", "long_chains", "long_trials",
+            "ss_first_pass")
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k in self.KEYS}

@@ -42,13 +42,15 @@ def test_camera_queries_and_estimates_match_oracle(gpu, oracle_mod, scene_fn):
     print(f"{scene_fn}: receivers {(qo['shape'] >= 0).mean():.3f} weights {np.unique(wo)[:6]}", flush=True)
     assert np.array_equal(qg.view(np.uint8), qo.view(np.uint8)), "camera queries differ"
     assert np.array_equal(wg, wo)
-    rg, sg = ctx.ss_estimate(qg, 5)
     ro, so = orc.ss_estimate(qo, 5)
-    print("gpu", sg, "oracle", so, flush=True)
-    assert np.array_equal(rg, ro, equal_nan=True)
-    for k in ("walks", "walks_converged", "newton_iterations", "bernoulli_calls", "bernoulli_trials"):
-        assert sg[k] == so[k], k
-    assert not rg[qg["shape"] < 0].any()
+    for first_pass in (0, 1):
+        ctx.set_schedule(ss_first_pass=first_pass)
+        rg, sg = ctx.ss_estimate(qg, 5)
+        print("gpu", first_pass, sg, "oracle", so, flush=True)
+        assert np.array_equal(rg, ro, equal_nan=True)
+        for k in ("walks", "walks_converged", "newton_iterations", "bernoulli_calls", "bernoulli_trials"):
+            assert sg[k] == so[k], k
+        assert not rg[qg["shape"] < 0].any()
     ctx.close()
     orc.close()
 

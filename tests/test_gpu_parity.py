@@ -154,10 +154,16 @@ def test_ss_estimate_dome(gpu, oracle_mod, biased):
     for k in ("walks", "walks_converged", "newton_iterations"):
         assert sg[k] == so[k], k
     if not biased:
-        ctx.set_schedule(lane_trials=1)
-        rg2, sg2 = ctx.ss_estimate(q, 7)
-        assert sg2["overflow_queries"] > 0
-        assert np.array_equal(rg2, ro, equal_nan=True)
+        for first_pass in (0, 1):
+            for lane_trials in (1, 8, 64):
+                ctx.set_schedule(lane_trials=lane_trials, ss_first_pass=first_pass)
+                rg2, sg2 = ctx.ss_estimate(q, 7)
+                if lane_trials == 1:
+                    assert sg2["overflow_queries"] > 0
+                assert np.array_equal(rg2, ro, equal_nan=True), (first_pass, lane_trials)
+                for k in ("walks", "walks_converged", "newton_iterations", "bernoulli_calls", "bernoulli_trials",
+                          "bernoulli_max"):
+                    assert sg2[k] == so[k], (k, first_pass, lane_trials)
     ctx.close()
     orc.close()
 
@@ -203,11 +209,15 @@ def test_pool_walks_through_bvh(gpu, oracle_mod):
     _check_ss_walks(ctx.ss_walks(w), orc.ss_walks(w))
     q = workloads.plane_queries(sc, 4000, 2, extent=0.9, shape=1)
     q["p"][:, 1] = -1.0
-    rg, sg = ctx.ss_estimate(q, 7)
     ro, so = orc.ss_estimate(q, 7)
-    print("gpu", sg, "oracle", so, flush=True)
-    assert (ro[:, 0] > 0).mean() > 0.05
-    assert np.array_equal(rg, ro, equal_nan=True)
+    for first_pass in (0, 1):
+        ctx.set_schedule(ss_first_pass=first_pass)
+        rg, sg = ctx.ss_estimate(q, 7)
+        print("gpu", first_pass, sg, "oracle", so, flush=True)
+        assert (ro[:, 0] > 0).mean() > 0.05
+        assert np.array_equal(rg, ro, equal_nan=True)
+        for k in ("walks", "walks_converged", "newton_iterations", "bernoulli_calls", "bernoulli_trials"):
+            assert sg[k] == so[k], (k, first_pass)
     ctx.close()
     orc.close()
 

@@ -218,13 +218,16 @@ def test_estimator_with_proposal_matches_oracle(gpu, oracle_mod, pool_scene, kin
     ctx.set_proposal(mlp, 0, 0.2)
     q = _floor_queries(pool_scene, 3000, 4, extent=0.7)
     raw = ctx.proposal_outputs(q, 9)
-    rg, sg = ctx.ss_estimate(q, 9)
     ro, so = orc.ss_estimate_prop(q, 9, raw, 0, 0.2)
-    print(kind, "gpu", sg, "oracle", so, flush=True)
-    for k in ("walks", "walks_converged", "newton_iterations", "bernoulli_calls", "bernoulli_trials", "bernoulli_max"):
-        assert sg[k] == so[k], k
-    assert (ro[:, 0] > 0).mean() > 0.05
-    assert np.array_equal(rg, ro, equal_nan=True)
+    for first_pass in (0, 1):
+        ctx.set_schedule(ss_first_pass=first_pass)
+        rg, sg = ctx.ss_estimate(q, 9)
+        print(kind, first_pass, "gpu", sg, "oracle", so, flush=True)
+        for k in ("walks", "walks_converged", "newton_iterations", "bernoulli_calls", "bernoulli_trials",
+                  "bernoulli_max"):
+            assert sg[k] == so[k], k
+        assert (ro[:, 0] > 0).mean() > 0.05
+        assert np.array_equal(rg, ro, equal_nan=True)
     # biased estimator with the same seeds
     cfg_b = abi.SmsConfig.default(biased=1, max_trials=32)
     ctx.set_config(cfg_b)
