@@ -13,6 +13,12 @@ whole timed region; `roofline` refers to the dominant kernel (k_glint_first: emi
 visibility, footprint target and the first Newton walk of every query) timed with HIP events
 on the stream it runs on; `cpu_baseline` times the CPU oracle on a bounded sample
 of the same queries on this host's cores.
+
+`equal_time_rmse` is the second half of the metric, on the water-pool config (see
+equal_time_rmse): RMSE of uniform-seed and learned-proposal renders given the same wall-clock
+budget, against a uniform-seed reference; `ratio_amortized` compares with the trained network
+(training done before the frame), `ratio` charges the training (its capture passes count in
+the image) to the same frame.
 """
 import argparse
 import json
@@ -59,9 +65,9 @@ def parse():
     ap.add_argument("--rmse-ref-spp", type=int, default=512, help="reference samples per pixel (uniform seeds)")
     ap.add_argument("--rmse-pass-spp", type=int, default=8)
     ap.add_argument("--train-samples", type=int, default=2_000_000)
-    ap.add_argument("--train-epochs", type=int, default=32, help="Adam steps per captured batch")
+    ap.add_argument("--train-epochs", type=int, default=64, help="Adam steps per captured batch")
     ap.add_argument("--defensive", type=float, default=0.2)
-    ap.add_argument("--capture", choices=["solutions", "estimates"], default="solutions",
+    ap.add_argument("--capture", choices=["solutions", "estimates"], default="estimates",
                     help="training records: every converged solution, or estimate-weighted solutions")
     ap.add_argument("--weight-power", type=float, default=1.0, help="exponent on estimate weights")
     return ap.parse_args()
@@ -156,8 +162,12 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(sc, cfg, ctx, n, args)
+    out.free()
+    qbuf.free()
+    ctx.close()
+    rmse = None if args.no_rmse else equal_time_rmse(args)
     result = {
-        "metric": "converged walks/s at 1 GPU",
+        "metric": "converged walks/s at 1 GPU; equal-time RMSE learned vs uniform seeds",
         "value": conv_per_s,
         "unit": "walks/s",
         "n_gpus": 1,
@@ -176,13 +186,11 @@ def main():
                    "trials_tail_finalize_ms": round(k2_ms / args.steps, 3)},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "equal_time_rmse": rmse,
     }
     if cpu and cpu.get("value"):
         result["gpu_over_cpu"] = conv_per_s / cpu["value"]
     print(json.dumps(result), flush=True)
-    out.free()
-    qbuf.free()
-    ctx.close()
 
 
 def equal_time_rmse(args):
@@ -205,6 +213,9 @@ def equal_time_rmse(args):
     stats = {}
     runner.run(999_999, film)                                   # first launches
     ctx.sync()
+    ctx.ss_queries_device(pass_spp, 999_998, runner.q, runner.w)
+    sched = ctx.ss_autotune_device(runner.q, runner.n, 999_998, runner.rgb)
+    print(f"# rmse: schedule {sched}", file=sys.stderr, flush=True)
 
     def frame(seed0, n_passes, st=None):
         film.clear()
@@ -288,7 +299,7 @@ def equal_time_rmse(args):
         "ratio_caustic_mask_amortized": m_u / m_a if m_a > 0 else None,
         "converged_per_walk": {"uniform": round(conv_rate(st_u), 4), "learned": round(conv_rate(st_a), 4)},
         "trials_per_estimate": {"uniform": round(trials(st_u), 3), "learned": round(trials(st_a), 3)},
-        "defensive": args.defensive,
+        "defensive": args.defensive, "schedule": sched,
     }
     print(f"# rmse: {json.dumps(out)}", file=sys.stderr, flush=True)
     mlp.close()

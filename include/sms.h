@@ -329,6 +329,12 @@ sms_status sms_glint_estimate_batch(sms_ctx *ctx, const sms_glint_query *q, int6
    out_rgb is device scratch of 3 * n floats; results do not depend on the schedule. */
 sms_status sms_glint_autotune(sms_ctx *ctx, const sms_glint_query *q, int64_t n, uint64_t seed,
                               float *out_rgb, sms_schedule *best);
+/* Autotuner of the unbiased single-scatter estimator (no counterpart in the reference): times
+   sms_ss_estimate_batch on the given device-resident queries for every candidate first pass
+   and lane_trials, keeps the fastest in the context and returns it in *best.  out_rgb is
+   device scratch of 3 * n floats; results do not depend on the schedule. */
+sms_status sms_ss_autotune(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *out_rgb,
+                           sms_schedule *best);
 /* Multi-scatter estimator (SpecularManifoldMultiScatter::specular_manifold_sampling,
    manifold_ms.cpp:27-201) over receiver queries; config.bounces vertices per chain. */
 sms_status sms_ms_estimate_batch(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed,

@@ -901,6 +901,37 @@ sms_status sms_glint_autotune(sms_ctx *ctx, const sms_glint_query *q, int64_t n,
     return SMS_OK;
 }
 
+sms_status sms_ss_autotune(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *out_rgb,
+                           sms_schedule *best) {
+    if (!ctx) return SMS_E_INVALID;
+    if (!best) return fail(ctx, SMS_E_INVALID, "null schedule");
+    if (ctx->scene.cfg.biased) return fail(ctx, SMS_E_INVALID, "the autotuner times the unbiased estimator");
+    const sms_schedule keep = schedule_of(ctx);
+    sms_schedule win = keep;
+    float win_ms = INFINITY;
+    sms_status st = SMS_OK;
+    struct Cand { int first_pass, lane_trials; };
+    const Cand cands[8] = {{0, 8}, {0, 16}, {0, 32}, {0, 64}, {0, 128}, {1, 4}, {1, 8}, {1, 16}};
+    for (const Cand &cd : cands) {
+        ctx->ss_first_pass = cd.first_pass; ctx->lane_trials = cd.lane_trials; ctx->blocks_per_cu = 0;
+        float ms = 0.f;
+        for (int rep = 0; rep < 2; ++rep) {           // the first run warms caches and allocations
+            st = sms_ss_estimate_batch(ctx, q, n, seed + (uint64_t) rep, out_rgb, nullptr);
+            if (st) break;
+            ms = ctx->last_ms + ctx->last_ms2;
+        }
+        if (st) break;
+        if (ms < win_ms) {
+            win_ms = ms;
+            win = schedule_of(ctx);
+        }
+    }
+    apply_schedule(ctx, st ? keep : win);
+    if (st) return st;
+    *best = win;
+    return SMS_OK;
+}
+
 sms_status sms_ss_estimate_batch(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *out_rgb,
                                  sms_stats *stats) {
     if (!ctx || n < 0 || (n > 0 && (!q || !out_rgb))) return fail(ctx, SMS_E_INVALID, "bad estimator arguments");

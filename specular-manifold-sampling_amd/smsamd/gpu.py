@@ -208,6 +208,12 @@ class Context:
         _check(lib().sms_glint_autotune(self.h, qbuf.ptr, n, seed, outbuf.ptr, C.byref(s)), self.h, "glint_autotune")
         return s.as_dict()
 
+    def ss_autotune_device(self, qbuf, n, seed, outbuf):
+        """Pick the fastest unbiased single-scatter schedule on these queries (sms_ss_autotune)."""
+        s = abi.SmsSchedule()
+        _check(lib().sms_ss_autotune(self.h, qbuf.ptr, n, seed, outbuf.ptr, C.byref(s)), self.h, "ss_autotune")
+        return s.as_dict()
+
     def ss_estimate_device(self, qbuf, n, seed, outbuf):
         st = abi.SmsStats()
         _check(lib().sms_ss_estimate_batch(self.h, qbuf.ptr, n, seed, outbuf.ptr, C.byref(st)), self.h,
