@@ -224,16 +224,19 @@ SMS_DEV PathResult sample_path_ss(const DScene &S, int shape, V3 si_p, const EI 
 }
 
 // specular_reflectance SS, smooth branch (manifold_ss.cpp:638-693)
-SMS_DEV float specular_reflectance_ss(const DScene &S, const SI &si, V3 wo) {
-    if (!si.valid) return 0.f;
-    const DShape &sh = S.shapes[si.shape];
-    float cos_theta = dot(si.sh_n, wo);
+SMS_DEV float specular_reflectance_ss_n(const DScene &S, int shape, V3 sh_n, V3 wo) {
+    const DShape &sh = S.shapes[shape];
+    float cos_theta = dot(sh_n, wo);
     if (sh.k != 0.f) return fresnel_conductor(fabsf(cos_theta), sh.eta_r, sh.k);
     float eta = sh.eta_vtx;
     if (cos_theta < 0.f) eta = 1.f / eta;
     float F = fresnel(cos_theta, eta);
     float val = 1.f - F;
     return val * (eta * eta);
+}
+SMS_DEV float specular_reflectance_ss(const DScene &S, const SI &si, V3 wo) {
+    if (!si.valid) return 0.f;
+    return specular_reflectance_ss_n(S, si.shape, si.sh_n, wo);
 }
 
 // geometric_term SS (manifold_ss.cpp:695-857)
@@ -311,14 +314,15 @@ SMS_DEV float geometric_term_ss(const MV &v0, const MV &v1, const MV &v2v) {
 }
 
 // evaluate_path_contribution SS (manifold_ss.cpp:281-327); si = receiver (query) interaction
-SMS_DEV V3 path_contribution_ss(const DScene &S, const SI &si, const EI &ei_, const SI &si_final) {
-    MV vtx = mv_from_si(S, si_final);
+// vfinal = mv_from_si of the (valid) solution surface interaction, sh_n_final its shading normal
+SMS_DEV V3 path_contribution_ss_mv(const DScene &S, const SI &si, const EI &ei_, const MV &vfinal, V3 sh_n_final) {
+    MV vtx = vfinal;
     make_orthonormal(vtx);
     EI ei = ei_;
     if (ei.kind == SMS_EI_POINT) {
         const DEmitter &em = S.emitters[ei.emitter];
         V3 pos = ld3(em.position);
-        V3 d = pos - si_final.p;
+        V3 d = pos - vfinal.p;
         float dist = norm(d), inv_dist = 1.f / dist;
         d = d * inv_dist;
         float emitter_pdf = ei.pdf;
@@ -331,10 +335,16 @@ SMS_DEV V3 path_contribution_ss(const DScene &S, const SI &si, const EI &ei_, co
     if (!emitter_interaction_to_vertex(S, ei, vtx.p, vy)) return v3(0.f, 0.f, 0.f);
     MV vx = mv_from_si(S, si);
     make_orthonormal(vx);
-    float F = specular_reflectance_ss(S, si_final, normalize(ei.p - si_final.p));
+    float F = specular_reflectance_ss_n(S, vfinal.shape, sh_n_final, normalize(ei.p - vfinal.p));
     float G = geometric_term_ss(vx, vtx, vy);
     float thr = F * G;
     return v3(thr * ei.weight.x, thr * ei.weight.y, thr * ei.weight.z);
+}
+
+// contribution of a walk's solution (manifold_ss.cpp:281-327)
+SMS_DEV V3 path_contribution_ss(const DScene &S, const SI &si, const EI &ei, const SI &si_final) {
+    if (!si_final.valid) return v3(0.f, 0.f, 0.f);
+    return path_contribution_ss_mv(S, si, ei, mv_from_si(S, si_final), si_final.sh_n);
 }
 
 } // namespace smsdev

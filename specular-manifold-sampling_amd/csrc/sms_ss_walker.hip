@@ -24,6 +24,10 @@ namespace {
 enum : int { WK_NONE = 0, WK_SEED = 1, WK_NEWTON = 2, WK_VIS = 3 };
 constexpr int WK_CHUNK = 64;   // queries a wavefront claims from the queue at a time
 
+// Per-lane state.  A walk keeps its manifold vertex and, of the surface interaction that
+// produced it, the uv and shading normal: a walk can only converge right after an accepted
+// reprojection (the constraint depends on the vertex alone), so on success that interaction
+// is the solution, valid exactly when at least one iteration ran.
 struct WalkerLane {
     // query
     int64_t q;
@@ -35,31 +39,33 @@ struct WalkerLane {
     int c, shape;
     int64_t slot, k;
     V3 direction;
-    SI si_first;                 // first solution, for the contribution after the shadow ray
+    V3 part_first;               // bsdf * contribution of the first solution, kept until its shadow ray
+    V3 p_first;
+    V2 uv_first;
     // walk
     int stage;
     MV vtx;
-    SI si_cur;
+    V2 uv_cur;
+    V3 shn_cur;
     float beta;
     int its;
-    // counters
-    unsigned long long walks, conv, iters, bcalls, btrials, bmax;
+    // counters (per lane and call; far below 2^32)
+    unsigned walks, conv, iters, bcalls, btrials, bmax;
 };
 
 } // namespace
 
-__global__ void __launch_bounds__(256) k_ss_walker(DScene S, const sms_ss_query *__restrict__ Q, int64_t n,
-                                                   uint64_t seed, float *__restrict__ part, float *__restrict__ invp,
-                                                   EstimatorWork W) {
+__device__ __forceinline__ void ss_walker_body(const DScene &S, const sms_ss_query *__restrict__ Q, int64_t n,
+                                               uint64_t seed, float *__restrict__ part, float *__restrict__ invp,
+                                               const EstimatorWork &W) {
     const sms_config &cfg = S.cfg;
     const int nc = S.n_casters_ss;
     const int lane = threadIdx.x & 63;
     const V3 n_offset = v3(0.f, 0.f, 1.f);
     const Proposal &P = W.prop;
-
     WalkerLane L;
     L.stage = WK_NONE; L.q = -1; L.c = 0; L.shape = -1; L.slot = 0; L.k = 0; L.its = 0; L.beta = 1.f;
-    L.walks = L.conv = L.iters = L.bcalls = L.btrials = L.bmax = 0;
+    L.walks = L.conv = L.iters = L.bcalls = L.btrials = L.bmax = 0u;
     L.qkey = 0;
     // wavefront-uniform query pool
     int64_t pool_next = 0, pool_end = 0;
@@ -80,27 +86,34 @@ __global__ void __launch_bounds__(256) k_ss_walker(DScene S, const sms_ss_query 
         else L.stage = WK_NONE;
     };
     auto resolve = [&](float inv_prob) {
-        const unsigned long long tr = (unsigned long long) L.k;
+        const unsigned tr = (unsigned) L.k;
         L.btrials += tr; L.bmax = L.bmax > tr ? L.bmax : tr;
         invp[L.slot] = inv_prob;
         const float wgt = mean3(&part[3 * L.slot]) * inv_prob;
         if (wgt > 0.f && capturing(P, L.shape, SMS_CAPTURE_ESTIMATES))
-            capture_sample(S, P, L.si, L.ei, L.si_first.uv, wgt);
+            capture_sample(S, P, L.si, L.ei, L.uv_first, wgt);
         end_chain();
     };
-    // a walk ended: ok, its final surface interaction, its iterations
-    auto finish_walk = [&](bool ok, const SI &si_final, int iterations) {
-        L.walks++; L.iters += (unsigned long long) iterations;
+    // a walk ended: ok and its iterations; on success the solution is L.vtx (uv L.uv_cur)
+    auto finish_walk = [&](bool ok, int iterations) {
+        L.walks++; L.iters += (unsigned) iterations;
         if (ok) L.conv++;
-        if (ok && capturing(P, L.shape, SMS_CAPTURE_SOLUTIONS)) capture_sample(S, P, L.si, L.ei, si_final.uv, 1.f);
+        if (ok && capturing(P, L.shape, SMS_CAPTURE_SOLUTIONS)) capture_sample(S, P, L.si, L.ei, L.uv_cur, 1.f);
         if (L.k == 0) {
             if (!ok) { end_chain(); return; }
-            L.direction = normalize(si_final.p - L.si.p);
-            L.si_first = si_final;
+            L.direction = normalize(L.vtx.p - L.si.p);
+            // the contribution is evaluated now (before the shadow ray) so the solution's
+            // interaction need not be kept; it is discarded if the shadow ray is blocked
+            V3 spec = path_contribution_ss_mv(S, L.si, L.ei, L.vtx, L.shn_cur);
+            V3 dl = v3(dot(L.direction, L.si.sh_s), dot(L.direction, L.si.sh_t), dot(L.direction, L.si.sh_n));
+            V3 bsdf = receiver_bsdf_eval(S, L.si, L.wi_local, dl);
+            L.part_first = v3(bsdf.x * spec.x, bsdf.y * spec.y, bsdf.z * spec.z);
+            L.p_first = L.vtx.p;
+            L.uv_first = L.uv_cur;
             L.stage = WK_VIS;
             return;
         }
-        V3 dtr = normalize(si_final.p - L.si.p);
+        V3 dtr = normalize(L.vtx.p - L.si.p);
         if (ok && fabsf(dot(L.direction, dtr) - 1.f) < cfg.uniqueness_threshold) { resolve((float) L.k); return; }
         L.k++;
         if (cfg.max_trials > 0 && L.k > cfg.max_trials) { resolve(0.f); return; }
@@ -110,7 +123,7 @@ __global__ void __launch_bounds__(256) k_ss_walker(DScene S, const sms_ss_query 
             OverflowRec rec;
             rec.q = L.slot; rec.k = L.k;
             rec.sol[0] = L.direction.x; rec.sol[1] = L.direction.y; rec.sol[2] = L.direction.z;
-            rec.shapes[0] = __float_as_int(L.si_first.uv.x); rec.shapes[1] = __float_as_int(L.si_first.uv.y);
+            rec.shapes[0] = __float_as_int(L.uv_first.x); rec.shapes[1] = __float_as_int(L.uv_first.y);
             rec.shapes[2] = rec.shapes[3] = 0;
             W.over.items[(size_t) sub * W.over.sub_cap + o] = rec;
             end_chain();
@@ -155,7 +168,7 @@ __global__ void __launch_bounds__(256) k_ss_walker(DScene S, const sms_ss_query 
                     V3 p_start = v3(0.f, 0.f, 0.f);
                     if (P.gmm && L.shape == P.shape) {
                         if (!proposal_seed(S, P, P.gmm + L.q * PROP_RAW, rw, s, p_start)) {
-                            finish_walk(false, si_invalid(), 0);
+                            finish_walk(false, 0);
                             continue;
                         }
                     } else {
@@ -173,17 +186,18 @@ __global__ void __launch_bounds__(256) k_ss_walker(DScene S, const sms_ss_query 
                     has_ray = true; any_hit = false;
                 } else if (L.stage == WK_NEWTON) {
                     // one iteration of newton_solver (manifold_ss.cpp:329-405)
-                    if (L.its >= cfg.max_iterations) { finish_walk(false, L.si_cur, L.its); continue; }
+                    if (L.its >= cfg.max_iterations) { finish_walk(false, L.its); continue; }
                     Step st = cfg.halfvector_constraints ? step_halfvector(L.si.p, L.vtx, L.ei, n_offset)
                                                          : step_anglediff(L.si.p, L.vtx, L.ei, n_offset);
-                    if (!st.ok) { finish_walk(false, L.si_cur, L.its); continue; }
+                    if (!st.ok) { finish_walk(false, L.its); continue; }
                     if (norm2(st.C) < cfg.solver_threshold) {
                         V3 wx = normalize(L.si.p - L.vtx.p);
                         V3 wy = L.ei.kind == SMS_EI_DIRECTIONAL ? L.ei.d : normalize(L.ei.p - L.vtx.p);
                         float cos_theta_x = dot(L.vtx.gn, wx), cos_theta_y = dot(L.vtx.gn, wy);
                         bool refraction = cos_theta_x * cos_theta_y < 0.f, reflection = !refraction;
                         bool type_ok = !((L.vtx.eta == 1.f && !reflection) || (L.vtx.eta != 1.f && !refraction));
-                        finish_walk(type_ok && L.si_cur.valid, L.si_cur, L.its);
+                        // converged at iteration 0 reports failure (no reprojection has run yet)
+                        finish_walk(type_ok && L.its > 0, L.its);
                         continue;
                     }
                     V3 p_prop = L.vtx.p - cfg.step_scale * L.beta * (L.vtx.dp_du * st.dX.x + L.vtx.dp_dv * st.dX.y);
@@ -192,7 +206,7 @@ __global__ void __launch_bounds__(256) k_ss_walker(DScene S, const sms_ss_query 
                     has_ray = true; any_hit = false;
                 } else {
                     // shadow ray of the first solution (ss_visible)
-                    V3 p = L.si_first.p;
+                    V3 p = L.p_first;
                     V3 d = p - L.ei.p;
                     float dist = norm(d);
                     d = d * (1.f / dist);
@@ -225,12 +239,9 @@ __global__ void __launch_bounds__(256) k_ss_walker(DScene S, const sms_ss_query 
         if (!has_ray) continue;
         if (L.stage == WK_VIS) {
             if (h.hit) { end_chain(); continue; }
-            // contribution of the first solution (manifold_ss.cpp:104-118)
-            V3 spec = path_contribution_ss(S, L.si, L.ei, L.si_first);
-            V3 dl = v3(dot(L.direction, L.si.sh_s), dot(L.direction, L.si.sh_t), dot(L.direction, L.si.sh_n));
-            V3 bsdf = receiver_bsdf_eval(S, L.si, L.wi_local, dl);
-            part[3 * L.slot] = bsdf.x * spec.x; part[3 * L.slot + 1] = bsdf.y * spec.y;
-            part[3 * L.slot + 2] = bsdf.z * spec.z;
+            // visible: the first solution's bsdf * contribution (manifold_ss.cpp:104-118)
+            part[3 * L.slot] = L.part_first.x; part[3 * L.slot + 1] = L.part_first.y;
+            part[3 * L.slot + 2] = L.part_first.z;
             L.bcalls++;
             L.k = 1;
             L.stage = WK_SEED;
@@ -241,20 +252,29 @@ __global__ void __launch_bounds__(256) k_ss_walker(DScene S, const sms_ss_query 
         SI hit_si = make_si(S, ray, h);
         const int want = L.stage == WK_SEED ? L.shape : L.vtx.shape;
         const bool on_shape = hit_si.valid && want == hit_si.shape;
-        if (on_shape) L.vtx = mv_from_si(S, hit_si);
+        if (on_shape) {
+            L.vtx = mv_from_si(S, hit_si);
+            L.uv_cur = hit_si.uv;
+            L.shn_cur = hit_si.sh_n;
+        }
         if (L.stage == WK_SEED) {
-            if (!on_shape) { finish_walk(false, si_invalid(), 0); continue; }
-            L.beta = 1.f; L.its = 0; L.si_cur = si_invalid();
+            if (!on_shape) { finish_walk(false, 0); continue; }
+            L.beta = 1.f; L.its = 0;
             L.stage = WK_NEWTON;
         } else {
-            L.si_cur = hit_si;
             if (!on_shape) L.beta = 0.5f * L.beta;
             else L.beta = fmin_(1.f, 2.f * L.beta);
             L.its += 1;
         }
     }
     stat_add(W.stats + 0, L.walks); stat_add(W.stats + 1, L.conv); stat_add(W.stats + 2, L.iters);
-    stat_add(W.stats + 3, L.bcalls); stat_add(W.stats + 4, L.btrials); stat_max(W.stats + 5, L.bmax);
+    stat_add(W.stats + 3, L.bcalls); stat_add(W.stats + 4, L.btrials); stat_max(W.stats + 5, (unsigned long long) L.bmax);
+}
+
+__global__ void __launch_bounds__(256) k_ss_walker(DScene S, const sms_ss_query *__restrict__ Q, int64_t n,
+                                                   uint64_t seed, float *__restrict__ part, float *__restrict__ invp,
+                                                   EstimatorWork W) {
+    ss_walker_body(S, Q, n, seed, part, invp, W);
 }
 
 } // namespace smsdev

@@ -77,20 +77,20 @@ class PassRunner:
 
 
 def train_online(ctx, runner, film, mlp, caster, samples=2_000_000, batch=8192, lr=1e-3, seed0=1 << 20,
-                 stats=None, rng_seed=0, epochs=1, mode=abi.CAPTURE_SOLUTIONS, weight_power=1.0):
+                 stats=None, rng_seed=0, epochs=1, mode=abi.CAPTURE_SOLUTIONS, weight_power=1.0, deadline=None):
     """Render uniform-seed passes into `film` while capturing training records on mesh
     `caster` (`mode`: every converged solution with weight 1, or the solutions behind non-zero
     estimates weighted by the estimate, raised to `weight_power`); after each pass,
     `epochs` x (records // batch) Adam steps on windows of `batch` consecutive records at random
-    offsets of that pass's capture, until `samples` records were trained on.
-    Returns (passes, steps, losses)."""
+    offsets of that pass's capture, until `samples` records were trained on or, if given, the
+    time.perf_counter() `deadline` has passed.  Returns (passes, steps, losses)."""
     rng = np.random.default_rng(rng_seed)
     ctx.set_proposal(None, caster)
     ctx.set_training_capture(runner.n, mode)
     used, steps, passes, losses = 0, 0, 0, []
     weighted = mode == abi.CAPTURE_ESTIMATES
     try:
-        while used < samples:
+        while used < samples and (deadline is None or time.perf_counter() < deadline):
             runner.run(seed0 + passes, film, stats)
             passes += 1
             xb, ub, wb, have = ctx.training_samples_device()
