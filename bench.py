@@ -59,6 +59,9 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 256 MiB device-to-device copy (HBM counter calibration)")
     ap.add_argument("--no-rmse", action="store_true", help="skip the equal-time RMSE experiment")
+    ap.add_argument("--no-pool", action="store_true", help="skip the pool single-scatter throughput line")
+    ap.add_argument("--pool-only", action="store_true", help="run only the pool single-scatter throughput")
+    ap.add_argument("--ss-first-pass", type=int, default=None, help="pool: 0 walker, 1 lane per query (default: tuned)")
     ap.add_argument("--rmse-only", action="store_true", help="run only the equal-time RMSE experiment")
     ap.add_argument("--rmse-res", type=int, default=1024, help="pool image width = height")
     ap.add_argument("--rmse-spp", type=int, default=64, help="uniform-seed samples per pixel (sets the time budget)")
@@ -97,6 +100,9 @@ def main():
         sys.exit(2)
     if args.rmse_only:
         print(json.dumps(equal_time_rmse(args)), flush=True)
+        return
+    if args.pool_only:
+        print(json.dumps(pool_throughput(args)), flush=True)
         return
     import numpy as np
     from smsamd import abi, gpu, scenes
@@ -165,6 +171,7 @@ def main():
     out.free()
     qbuf.free()
     ctx.close()
+    pool = None if args.no_pool else pool_throughput(args)
     rmse = None if args.no_rmse else equal_time_rmse(args)
     result = {
         "metric": "converged walks/s at 1 GPU; equal-time RMSE learned vs uniform seeds",
@@ -186,11 +193,62 @@ def main():
                    "trials_tail_finalize_ms": round(k2_ms / args.steps, 3)},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "pool_ss": pool,
         "equal_time_rmse": rmse,
     }
     if cpu and cpu.get("value"):
         result["gpu_over_cpu"] = conv_per_s / cpu["value"]
     print(json.dumps(result), flush=True)
+
+
+def pool_throughput(args):
+    """Unbiased single-scatter SMS on the water pool (uniform seeds): one 1024^2 x 8 spp pass of
+    camera queries resident in HBM, `steps` timed estimates after `warmup` (per-step seeds);
+    schedule from sms_ss_autotune unless --no-tune / --ss-first-pass / --lane-trials are given."""
+    from smsamd import abi, gpu, scenes
+    res, spp = args.rmse_res, args.rmse_pass_spp
+    sc = scenes.pool(res, res)
+    cfg = abi.SmsConfig.default(max_trials=10_000_000, uniqueness_threshold=1e-5, solver_threshold=1e-5)
+    ctx = gpu.Context(sc, cfg)
+    qb, wb, n = ctx.ss_queries_device(spp, 1)
+    out = gpu.DeviceBuffer(ctx, n * 12)
+    if args.no_tune or args.ss_first_pass is not None or args.lane_trials is not None:
+        kw = {}
+        if args.ss_first_pass is not None:
+            kw["ss_first_pass"] = args.ss_first_pass
+        if args.lane_trials is not None:
+            kw["lane_trials"] = args.lane_trials
+        if kw:
+            ctx.set_schedule(**kw)
+        sched = ctx.get_schedule()
+    else:
+        sched = ctx.ss_autotune_device(qb, n, 7777, out)
+    for w in range(args.warmup):
+        ctx.ss_estimate_device(qb, n, 2000 + w, out)
+    ctx.sync()
+    tot = {}
+    k1 = k2 = 0.0
+    t0 = time.perf_counter()
+    for s_ in range(args.steps):
+        st = ctx.ss_estimate_device(qb, n, 1 + s_, out)
+        a, b = ctx.last_kernel_ms()
+        k1 += a
+        k2 += b
+        for k, v in st.items():
+            tot[k] = tot.get(k, 0) + v
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    res_ = {"workload": f"water pool caustics, {res}x{res}x{spp}spp camera queries ({n}), unbiased, uniform seeds",
+            "converged_walks_per_s": tot["walks_converged"] / dt, "walks_per_s": tot["walks"] / dt,
+            "newton_iterations_per_s": tot["newton_iterations"] / dt, "ms_per_step": 1e3 * dt / args.steps,
+            "first_pass_ms": k1 / args.steps, "tail_ms": k2 / args.steps, "steps": args.steps,
+            "walks_per_step": tot["walks"] / args.steps, "schedule": sched}
+    print(f"# pool: {json.dumps(res_)}", file=sys.stderr, flush=True)
+    out.free()
+    qb.free()
+    wb.free()
+    ctx.close()
+    return res_
 
 
 def equal_time_rmse(args):
