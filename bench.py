@@ -16,9 +16,9 @@ of the same queries on this host's cores.
 
 `equal_time_rmse` is the second half of the metric, on the water-pool config (see
 equal_time_rmse): RMSE of uniform-seed and learned-proposal renders given the same wall-clock
-budget, against a uniform-seed reference; `ratio_amortized` compares with the trained network
-(training done before the frame), `ratio` charges the training (its capture passes count in
-the image) to the same frame.
+budget, against a uniform-seed reference; `ratio` charges the training to the same frame (the
+network is trained online between its passes, every pass counts in the image),
+`ratio_amortized` compares with the trained network (training done before the frame).
 """
 import argparse
 import json
@@ -68,11 +68,8 @@ def parse():
     ap.add_argument("--rmse-ref-spp", type=int, default=512, help="reference samples per pixel (uniform seeds)")
     ap.add_argument("--rmse-pass-spp", type=int, default=8)
     ap.add_argument("--train-samples", type=int, default=2_000_000)
-    ap.add_argument("--train-epochs", type=int, default=64, help="Adam steps per captured batch")
+    ap.add_argument("--train-epochs", type=int, default=16, help="Adam steps per 8192 captured records")
     ap.add_argument("--defensive", type=float, default=0.2)
-    ap.add_argument("--capture", choices=["solutions", "estimates"], default="estimates",
-                    help="training records: every converged solution, or estimate-weighted solutions")
-    ap.add_argument("--weight-power", type=float, default=1.0, help="exponent on estimate weights")
     return ap.parse_args()
 
 
@@ -255,12 +252,13 @@ def equal_time_rmse(args):
     """Water pool (512^2-cell Gerstner mesh, IOR 1.33, floor 1 below, sphere light r 0.05),
     caustic term of the camera image at rmse_res^2.  Uniform seeding renders rmse_spp samples
     per pixel; its wall-clock time T is the budget.  The charged run gets the same T for
-    everything: it renders uniform-seed passes while capturing training records and training
-    the network online (batch 8192, Adam lr 1e-3), stopping at T if train_samples records are
-    not yet used, then renders passes with the defensive mixture until T is spent; every pass
-    counts in its image.  Training then continues to train_samples records, and the amortized
-    run renders with that network for T.  RMSE against a uniform-seed reference of
-    rmse_ref_spp samples, with the reference's own variance removed."""
+    everything, training included: its first pass uses uniform seeds, every later pass the
+    defensive mixture with the network as trained so far; after each pass the network trains on
+    the estimate-weighted solutions that pass captured (batch 8192, Adam lr 1e-3, train_epochs
+    steps per 8192 records) until train_samples records were used; every pass counts in the
+    image.  The amortized run renders T with the trained network (training done before the
+    frame).  RMSE against a uniform-seed reference of rmse_ref_spp samples, with the
+    reference's own variance removed."""
     import numpy as np
     from smsamd import abi, gpu, render, scenes
     res, pass_spp = args.rmse_res, args.rmse_pass_spp
@@ -269,7 +267,6 @@ def equal_time_rmse(args):
     ctx = gpu.Context(sc, cfg)
     runner = render.PassRunner(ctx, pass_spp)
     film = render.Film(ctx, runner.pixels)
-    stats = {}
     runner.run(999_999, film)                                   # first launches
     ctx.sync()
     ctx.ss_queries_device(pass_spp, 999_998, runner.q, runner.w)
@@ -285,65 +282,49 @@ def equal_time_rmse(args):
 
     (ref, ref_var), t_ref = render.timed(lambda: frame(10_000_000, args.rmse_ref_spp // pass_spp))
     ref_var_mean = ref_var / max(film.spp, 1)
+    dropped_ref = film.dropped()
     print(f"# rmse: reference {film.spp} spp in {t_ref:.1f}s", file=sys.stderr, flush=True)
     st_u = {}
     (uni, _), T = render.timed(lambda: frame(20_000_000, args.rmse_spp // pass_spp, st_u))
-    spp_u = film.spp
-
-    def until_budget(seed0, t_start, st):
-        times, p = [], 0
-        while True:
-            est = sum(times) / len(times) if times else 0.0
-            if time.perf_counter() - t_start + est > T:
-                break
-            t = time.perf_counter()
-            runner.run(seed0 + p, film, st)
-            ctx.sync()
-            times.append(time.perf_counter() - t)
-            p += 1
-        return p
+    spp_u, dropped_u = film.spp, film.dropped()
 
     # learned, training charged to the frame
     film.clear()
     mlp = gpu.Mlp(ctx, seed=1)
-    t0 = time.perf_counter()
-    mode = abi.CAPTURE_ESTIMATES if args.capture == "estimates" else abi.CAPTURE_SOLUTIONS
-    passes_tr, steps, losses = render.train_online(ctx, runner, film, mlp, caster=0, samples=args.train_samples,
-                                                   seed0=30_000_000, epochs=args.train_epochs, mode=mode,
-                                                   weight_power=args.weight_power, deadline=t0 + T)
-    ctx.sync()
-    t_train = time.perf_counter() - t0
-    spp_charged_uniform = film.spp
-    steps_charged = steps
-    # the charged frame keeps what it rendered within T; training then continues to the full
-    # sample count for the amortized run (its time is reported, not charged)
-    ctx.set_proposal(mlp, 0, args.defensive)
     st_l = {}
-    until_budget(40_000_000, t0, st_l)
-    t_l = time.perf_counter() - t0
+    tr, t_l = render.timed(lambda: render.render_online(
+        ctx, runner, film, mlp, 0, T, samples=args.train_samples, epochs=args.train_epochs, defensive=args.defensive,
+        seed0=30_000_000, stats=st_l))
     learned, _ = film.download()
-    spp_l = film.spp
-    t1 = time.perf_counter()
-    if steps * 8192 // max(args.train_epochs, 1) < args.train_samples:
+    spp_l, dropped_l = film.spp, film.dropped()
+    steps, losses, train_s = tr["steps"], tr["losses"], tr["train_s"]
+    if args.train_samples - tr["used"] >= 8192:
+        # training goes on (outside any budget) to the full record count for the amortized run
         film.clear()
-        p2, s2, l2 = render.train_online(ctx, runner, film, mlp, caster=0,
-                                         samples=args.train_samples - steps * 8192 // max(args.train_epochs, 1),
-                                         seed0=35_000_000, epochs=args.train_epochs, mode=mode,
-                                         weight_power=args.weight_power, rng_seed=1)
-        steps += s2
-        losses += l2
-    ctx.sync()
-    t_train += time.perf_counter() - t1
-    spp_tr = spp_charged_uniform
-    ctx.set_proposal(mlp, 0, args.defensive)
+        tr2 = render.render_online(ctx, runner, film, mlp, 0, float("inf"), samples=args.train_samples - tr["used"],
+                                   epochs=args.train_epochs, defensive=args.defensive, seed0=35_000_000,
+                                   uniform_passes=0, rng_seed=1, stop_when_trained=True)
+        steps += tr2["steps"]
+        losses += tr2["losses"]
+        train_s += tr2["train_s"]
     # learned, network already trained
+    ctx.set_proposal(mlp, 0, args.defensive)
     film.clear()
-    t0 = time.perf_counter()
     st_a = {}
-    until_budget(50_000_000, t0, st_a)
+    t0 = time.perf_counter()
+    times, p = [], 0
+    while True:
+        est = sum(times) / len(times) if times else 0.0
+        if time.perf_counter() - t0 + est > T:
+            break
+        t = time.perf_counter()
+        runner.run(50_000_000 + p, film, st_a)
+        ctx.sync()
+        times.append(time.perf_counter() - t)
+        p += 1
     t_a = time.perf_counter() - t0
     amort, _ = film.download()
-    spp_a = film.spp
+    spp_a, dropped_a = film.spp, film.dropped()
     ctx.set_proposal(None, -1)
 
     lum = ref.mean(1)
@@ -361,10 +342,11 @@ def equal_time_rmse(args):
         "workload": f"water pool caustics, {res}x{res} camera image, unbiased (max_trials 1e7, uniqueness 1e-5), "
                     f"passes of {pass_spp} spp",
         "budget_s": round(T, 3), "uniform_spp": spp_u,
-        "learned_spp": spp_l, "learned_spp_during_training": spp_tr, "learned_s": round(t_l, 3),
-        "train_s": round(t_train, 3), "train_samples": args.train_samples, "train_steps": steps,
-        "train_steps_within_budget": steps_charged,
-        "train_epochs": args.train_epochs, "capture": args.capture, "weight_power": args.weight_power,
+        "learned_spp": spp_l, "learned_s": round(t_l, 3), "learned_passes_uniform": 1,
+        "train_s_within_budget": round(tr["train_s"], 3), "train_steps_within_budget": tr["steps"],
+        "train_records_within_budget": tr["used"],
+        "train_s": round(train_s, 3), "train_samples": args.train_samples, "train_steps": steps,
+        "train_epochs": args.train_epochs,
         "train_loss_first_last": [round(losses[0], 4), round(losses[-1], 4)] if losses else None,
         "amortized_spp": spp_a, "amortized_s": round(t_a, 3),
         "reference_spp": args.rmse_ref_spp, "reference_s": round(t_ref, 2), "mean_pixel": mean_ref,
@@ -375,6 +357,8 @@ def equal_time_rmse(args):
         "ratio_caustic_mask_amortized": m_u / m_a if m_a > 0 else None,
         "converged_per_walk": {"uniform": round(conv_rate(st_u), 4), "learned": round(conv_rate(st_a), 4)},
         "trials_per_estimate": {"uniform": round(trials(st_u), 3), "learned": round(trials(st_a), 3)},
+        "invalid_samples_dropped": {"reference": dropped_ref, "uniform": dropped_u, "learned": dropped_l,
+                                    "amortized": dropped_a},
         "defensive": args.defensive, "schedule": sched,
     }
     print(f"# rmse: {json.dumps(out)}", file=sys.stderr, flush=True)

@@ -355,9 +355,12 @@ sms_status sms_glint_make_queries(sms_ctx *ctx, int32_t spp, uint64_t seed, sms_
 sms_status sms_ss_make_queries(sms_ctx *ctx, int32_t spp, uint64_t seed, sms_ss_query *q, float *weight);
 /* Box-filter film (device memory): for each of `pixels` pixels, adds the sum over its spp
    consecutive samples of weight * rgb to image (pixels x 3) and, if image_sq is not NULL, the
-   sum of squares to image_sq.  The caller divides by the number of samples accumulated. */
+   sum of squares to image_sq.  A sample with a non-finite or negative (below -1e-5) channel is
+   dropped, as the reference's ImageBlock::put drops it (src/librender/imageblock.cpp:85-107);
+   image_n (pixels floats, may be NULL) receives the number of samples kept, by which the
+   caller divides. */
 sms_status sms_film_accumulate(sms_ctx *ctx, const float *rgb, const float *weight, int64_t pixels, int32_t spp,
-                               float *image, float *image_sq);
+                               float *image, float *image_sq, float *image_n);
 
 /* Kernel timing of the most recent estimator call, measured with HIP events on the context
    stream the kernels ran on: ms[0] = first (persistent lane-per-query) pass, ms[1] = the
@@ -413,6 +416,16 @@ sms_status sms_mlp_grad(sms_mlp *mlp, const float *x, const float *uv, const flo
                         float *loss);
 /* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) with the kept gradient, which is cleared. */
 sms_status sms_mlp_adam(sms_mlp *mlp, float lr);
+/* n_steps training steps, each the gradient of sms_mlp_grad over the batch window
+   [offsets[i], offsets[i] + batch) of the device sample arrays x (records x 9), uv (records x
+   2), w (records; NULL: all 1) followed by an Adam step with learning rate lr; offsets is a host
+   array whose windows must lie inside the arrays.  The steps are queued on the context's
+   stream without host synchronisation; losses (if not NULL, n_steps host floats) receives each
+   step's weighted mean NLL (then the call waits for the steps).  Gradients are summed in a
+   fixed order: the same inputs give bit-identical parameters.  The reference has no trained
+   component; this is the online training loop of the learned proposal. */
+sms_status sms_mlp_train(sms_mlp *mlp, const float *x, const float *uv, const float *w, int32_t batch,
+                         const int64_t *offsets, int32_t n_steps, float lr, float *losses);
 /* Use `mlp` as the seed proposal of single-scatter caster mesh `shape` (with uvs) in
    sms_ss_estimate_batch; defensive in (0, 1].  mlp NULL: uniform seeds, `shape` still names the
    caster whose solutions the training capture records; shape -1 clears the proposal. */

@@ -174,15 +174,17 @@ def loss_and_grad(params, x9, uv, mode="f64", w=None):
             if i > 0:
                 dy = (dy @ W) * (acts[i] > 0)
         return float((wt * loss).sum() / wt.sum()), grad
+    # the kernels' order: head gradients times w_j, rounded to BF16, back-propagated with FP32
+    # sums, and the summed gradient times 1 / sum(w) at the end
     Ws = [(bf16(W), b) for W, b in unpack(params, np.float32)]
     w32 = wt.astype(np.float32)
-    scale = (w32 * (np.float32(1) / w32.sum(dtype=np.float32))).astype(np.float32)
-    dy = bf16((d_o * scale[:, None]).astype(np.float32))
+    inv = np.float32(1) / w32.sum(dtype=np.float32)
+    dy = bf16((d_o * w32[:, None]).astype(np.float32))
     for i in range(4, -1, -1):
         W, _ = Ws[i]
         w0, b0 = OFFSETS[i]
-        grad[w0:w0 + W.size] = (dy.T @ acts[i]).astype(np.float32).ravel()
-        grad[b0:b0 + W.shape[0]] = dy.sum(0, dtype=np.float32)
+        grad[w0:w0 + W.size] = ((dy.T @ acts[i]).astype(np.float32) * inv).ravel()
+        grad[b0:b0 + W.shape[0]] = dy.sum(0, dtype=np.float32) * inv
         if i > 0:
             dy = bf16(((dy @ W).astype(np.float32)) * (acts[i] > 0))
     return float((w32 * loss.astype(np.float32)).sum() / w32.sum()), grad

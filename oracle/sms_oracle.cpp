@@ -1382,7 +1382,8 @@ static int proposal_seed(const float *g, float defensive, Pcg32 *rw, V2 *out) {
     return 1;
 }
 
-struct Prop { const float *raw; int shape; float defensive; UVGrid grid; };
+/* raw: network outputs, row (qid - raw_base) for query qid */
+struct Prop { const float *raw; int shape; float defensive; UVGrid grid; int64_t raw_base = 0; };
 
 /* one walk of a query whose mixture is g: uniform seed, or the mixture's uv mapped to the
    surface and used as p_start; uv outside [0,1)^2 or in no face -> failed walk, 0 iterations */
@@ -1416,7 +1417,7 @@ static V3 estimate_ss(const Scene &S, const sms_ss_query &q, uint64_t seed, uint
         V3 value = v3(0.f, 0.f, 0.f);
         float gq[PROP_RAW];
         const bool use_prop = P && shape == P->shape;
-        if (use_prop) gmm_prepare(P->raw + qid * PROP_RAW, gq);
+        if (use_prop) gmm_prepare(P->raw + ((int64_t) qid - P->raw_base) * PROP_RAW, gq);
         auto run_walk = [&](uint64_t k) {
             Pcg32 rw; stream_seed(&rw, seed, qid, wbase + k + 1);
             PathOut po;
@@ -2675,6 +2676,30 @@ void oracle_scene_box(void *h, float *c, float *inv) {
     const Scene &S = *(Scene *) h;
     for (int a = 0; a < 3; ++a) c[a] = S.box_c[a];
     *inv = S.box_inv;
+}
+
+/* estimates of selected queries of a batch: q[i] is query ids[i] of the batch (its random
+   streams), raw row i its network outputs; sequential */
+int oracle_ss_estimate_ids_prop(void *h, const sms_ss_query *q, const int64_t *ids, int64_t n, uint64_t seed,
+                                const float *raw, int shape, float defensive, float *out_rgb, sms_stats *stats) {
+    const Scene &S = *(Scene *) h;
+    if (shape < 0 || shape >= (int) S.shapes.size() || S.shapes[shape].type != SMS_SHAPE_MESH) return -1;
+    Prop P;
+    P.shape = shape; P.defensive = defensive;
+    build_uv_grid(S, shape, P.grid);
+    Stats st; memset(&st, 0, sizeof(st));
+    for (int64_t i = 0; i < n; ++i) {
+        P.raw = raw + i * PROP_RAW;
+        P.raw_base = ids[i];
+        V3 r = estimate_ss(S, q[i], seed, (uint64_t) ids[i], st, &P);
+        out_rgb[3 * i] = r.x; out_rgb[3 * i + 1] = r.y; out_rgb[3 * i + 2] = r.z;
+    }
+    if (stats) {
+        memset(stats, 0, sizeof(*stats));
+        stats->walks = st.walks; stats->walks_converged = st.conv; stats->newton_iterations = st.iters;
+        stats->bernoulli_calls = st.bcalls; stats->bernoulli_trials = st.btrials; stats->bernoulli_max = st.bmax;
+    }
+    return 0;
 }
 
 void oracle_gmm_prepare(const float *raw, int64_t n, float *g) {

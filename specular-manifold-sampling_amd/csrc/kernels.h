@@ -153,15 +153,19 @@ __global__ void k_ss_walker(DScene S, const sms_ss_query *Q, int64_t n, uint64_t
 __global__ void k_ss_coop(DScene S, const sms_ss_query *Q, uint64_t seed, const float *part, float *invp, EstimatorWork W);
 __global__ void k_ss_finalize(const float *part, const float *invp, int nc, int64_t n, float *out);
 __global__ void k_ss_queries(DScene S, int spp, uint64_t seed, sms_ss_query *q, float *w, int64_t n);
-__global__ void k_film_accumulate(const float *rgb, const float *w, int64_t pixels, int spp, float *image, float *image_sq);
+__global__ void k_film_accumulate(const float *rgb, const float *w, int64_t pixels, int spp, float *image, float *image_sq,
+                                  float *image_n);
 __global__ void k_prop_inputs(DScene S, const sms_ss_query *Q, int64_t n, uint64_t seed, float *x);
 __global__ void k_gmm_prepare(const float *raw, int64_t n, float *g);
 __global__ void k_capture_gather(const float *cap, const unsigned long long *counts, unsigned long long cap_sub,
                                  float *x, float *uv, float *w);
 __global__ void k_mlp_infer(const float *p, const uint16_t *wb, const float *x9, int64_t n, float *out);
-__global__ void k_mlp_weight_sum(const float *w, int64_t n, float *wsum);
 __global__ void k_mlp_grad(const float *p, const uint16_t *wb, const float *x9, const float *uv, const float *w, int64_t n,
-                           const float *wsum, float *grad, float *loss_sum);
+                           float *partial, float *aux);
+__global__ void k_mlp_reduce(const float *partial, const float *aux, int nb, float *g, float *loss);
+__global__ void k_mlp_reduce_adam(const float *partial, const float *aux, int nb, float *p, float *m, float *v,
+                                  uint16_t *wb, float lr, float b1, float b2, float eps, float bc1, float bc2,
+                                  float *loss);
 __global__ void k_mlp_adam(float *p, float *g, float *m, float *v, uint16_t *wb, int n, float lr, float b1, float b2,
                            float eps, float bc1, float bc2);
 __global__ void k_ss_biased(DScene S, const sms_ss_query *Q, int64_t n, uint64_t seed, float *out, EstimatorWork W);

@@ -782,12 +782,12 @@ sms_status sms_ss_make_queries(sms_ctx *ctx, int32_t spp, uint64_t seed, sms_ss_
 }
 
 sms_status sms_film_accumulate(sms_ctx *ctx, const float *rgb, const float *weight, int64_t pixels, int32_t spp,
-                               float *image, float *image_sq) {
+                               float *image, float *image_sq, float *image_n) {
     if (!ctx || pixels < 0 || spp <= 0 || (pixels > 0 && (!rgb || !weight || !image)))
         return fail(ctx, SMS_E_INVALID, "bad film arguments");
     if (pixels == 0) return SMS_OK;
     hipLaunchKernelGGL(k_film_accumulate, dim3((unsigned) ((pixels + 255) / 256)), dim3(256), 0, ctx->stream, rgb, weight,
-                       pixels, (int) spp, image, image_sq);
+                       pixels, (int) spp, image, image_sq, image_n);
     HIP_TRY(ctx, hipGetLastError());
     return SMS_OK;
 }

@@ -406,24 +406,35 @@ __global__ void k_ss_queries(DScene S, int spp, uint64_t seed, sms_ss_query *__r
 
 // Film: per pixel, the sums over its spp samples (in sample order) of weight * rgb and of its
 // square, added to image / image_sq (box filter; the caller divides by the samples taken).
+// A sample whose weighted value has a non-finite or negative (below -1e-5) channel is dropped,
+// as ImageBlock::put drops it (src/librender/imageblock.cpp:85-107); image_n counts the rest.
 __global__ void k_film_accumulate(const float *__restrict__ rgb, const float *__restrict__ w, int64_t pixels, int spp,
-                                  float *__restrict__ image, float *__restrict__ image_sq) {
+                                  float *__restrict__ image, float *__restrict__ image_sq, float *__restrict__ image_n) {
     const int64_t p = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= pixels) return;
     float s[3] = {0.f, 0.f, 0.f}, s2[3] = {0.f, 0.f, 0.f};
+    int valid = 0;
     for (int k = 0; k < spp; ++k) {
         const int64_t i = p * spp + k;
         const float wi = w[i];
+        float v[3];
+        bool ok = true;
         for (int a = 0; a < 3; ++a) {
-            const float v = wi * rgb[3 * i + a];
-            s[a] = s[a] + v;
-            s2[a] = s2[a] + v * v;
+            v[a] = wi * rgb[3 * i + a];
+            ok = ok && isfinite(v[a]) && v[a] >= -1e-5f;
+        }
+        if (!ok) continue;
+        ++valid;
+        for (int a = 0; a < 3; ++a) {
+            s[a] = s[a] + v[a];
+            s2[a] = s2[a] + v[a] * v[a];
         }
     }
     for (int a = 0; a < 3; ++a) {
         image[3 * p + a] += s[a];
         if (image_sq) image_sq[3 * p + a] += s2[a];
     }
+    if (image_n) image_n[p] += (float) valid;
 }
 
 // ================================================================ learned seed proposal

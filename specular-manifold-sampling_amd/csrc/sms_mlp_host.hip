@@ -13,10 +13,16 @@
 
 using namespace smshost;
 
+// Most blocks of a gradient pass: rows of the partial-gradient matrix (256 x 25,904 floats).
+constexpr int MLP_GRAD_ROWS = 256;
+
 struct sms_mlp {
     sms_ctx *ctx = nullptr;
     float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr, *loss = nullptr;   // device, FP32
     uint16_t *wb = nullptr;                                                          // device, BF16 copy of p
+    float *partial = nullptr, *aux = nullptr;   // device: per-block gradient rows, (sum w, sum w nll) per row
+    float *loss_hist = nullptr;                 // device: per-step losses of sms_mlp_train
+    int64_t loss_cap = 0;
     int64_t step = 0;                                                                // Adam steps taken
 };
 
@@ -78,7 +84,8 @@ sms_status upload_params(sms_mlp *mlp, const float *h) {
 }
 
 void free_mlp(sms_mlp *mlp) {
-    for (void *q : {(void *) mlp->p, (void *) mlp->g, (void *) mlp->m, (void *) mlp->v, (void *) mlp->loss, (void *) mlp->wb})
+    for (void *q : {(void *) mlp->p, (void *) mlp->g, (void *) mlp->m, (void *) mlp->v, (void *) mlp->loss,
+                    (void *) mlp->wb, (void *) mlp->partial, (void *) mlp->aux, (void *) mlp->loss_hist})
         if (q) (void) hipFree(q);
     delete mlp;
 }
@@ -91,6 +98,20 @@ sms_status launch_infer(sms_mlp *mlp, const float *x, int64_t n, float *out) {
     hipLaunchKernelGGL(k_mlp_infer, dim3(grid), dim3(256), 0, ctx->stream, mlp->p, mlp->wb, x, n, out);
     HIP_TRY(ctx, hipGetLastError());
     return SMS_OK;
+}
+
+constexpr float ADAM_B1 = 0.9f, ADAM_B2 = 0.999f, ADAM_EPS = 1e-8f;
+
+// next Adam step's bias corrections 1 - beta^t (beta and the power in double, rounded once)
+void adam_corrections(sms_mlp *mlp, float *bc1, float *bc2) {
+    mlp->step++;
+    *bc1 = (float) (1.0 - std::pow(0.9, (double) mlp->step));
+    *bc2 = (float) (1.0 - std::pow(0.999, (double) mlp->step));
+}
+
+// blocks (partial rows) of a gradient pass over n samples
+int grad_rows(int64_t n) {
+    return (int) std::min<int64_t>((n + MLP_TILE - 1) / MLP_TILE, MLP_GRAD_ROWS);
 }
 
 sms_status ensure_prop(sms_ctx *ctx, size_t n) {
@@ -207,6 +228,8 @@ sms_status sms_mlp_create(sms_ctx *ctx, uint64_t seed, sms_mlp **out) {
     if (e == hipSuccess) e = hipMalloc(&mlp->m, fb);
     if (e == hipSuccess) e = hipMalloc(&mlp->v, fb);
     if (e == hipSuccess) e = hipMalloc(&mlp->loss, 2 * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&mlp->partial, (size_t) MLP_GRAD_ROWS * fb);
+    if (e == hipSuccess) e = hipMalloc(&mlp->aux, 2 * MLP_GRAD_ROWS * sizeof(float));
     if (e == hipSuccess) e = hipMalloc(&mlp->wb, MLP_PARAMS * sizeof(uint16_t));
     if (e == hipSuccess) e = hipMemsetAsync(mlp->g, 0, fb, ctx->stream);
     if (e == hipSuccess) e = hipMemsetAsync(mlp->m, 0, fb, ctx->stream);
@@ -255,25 +278,17 @@ sms_status sms_mlp_grad(sms_mlp *mlp, const float *x, const float *uv, const flo
     if (!mlp) return SMS_E_INVALID;
     sms_ctx *ctx = mlp->ctx;
     if (n <= 0 || !x || !uv) return fail(ctx, SMS_E_INVALID, "sms_mlp_grad: bad arguments");
-    // mlp->loss: [0] sum of w * nll, [1] sum of w; the kernels scale the gradient by 1 / [1]
-    HIP_TRY(ctx, hipMemsetAsync(mlp->loss, 0, 2 * sizeof(float), ctx->stream));
-    const unsigned tiles = (unsigned) ((n + MLP_TILE - 1) / MLP_TILE);
-    hipLaunchKernelGGL(k_mlp_weight_sum, dim3(tiles), dim3(64), 0, ctx->stream, w, n, mlp->loss + 1);
+    const int rows = grad_rows(n);
+    hipLaunchKernelGGL(k_mlp_grad, dim3(rows), dim3(256), 0, ctx->stream, mlp->p, mlp->wb, x, uv, w, n, mlp->partial,
+                       mlp->aux);
     HIP_TRY(ctx, hipGetLastError());
-    HIP_TRY(ctx, hipMemsetAsync(mlp->g, 0, MLP_PARAMS * sizeof(float), ctx->stream));
-    hipLaunchKernelGGL(k_mlp_grad, dim3(tiles), dim3(256), 0, ctx->stream, mlp->p, mlp->wb, x, uv, w, n,
-                       (const float *) (mlp->loss + 1), mlp->g, mlp->loss);
+    hipLaunchKernelGGL(k_mlp_reduce, dim3((MLP_PARAMS + 63) / 64), dim3(256), 0, ctx->stream,
+                       (const float *) mlp->partial, (const float *) mlp->aux, rows, mlp->g, mlp->loss);
     HIP_TRY(ctx, hipGetLastError());
     if (grad)
         HIP_TRY(ctx, hipMemcpyAsync(grad, mlp->g, MLP_PARAMS * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
-    if (loss) {
-        float s[2] = {0.f, 0.f};
-        HIP_TRY(ctx, hipMemcpyAsync(s, mlp->loss, 2 * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
-        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-        *loss = s[0] / s[1];
-    } else if (grad) {
-        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    }
+    if (loss) HIP_TRY(ctx, hipMemcpyAsync(loss, mlp->loss, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    if (grad || loss) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return SMS_OK;
 }
 
@@ -281,13 +296,48 @@ sms_status sms_mlp_adam(sms_mlp *mlp, float lr) {
     if (!mlp) return SMS_E_INVALID;
     sms_ctx *ctx = mlp->ctx;
     if (!(lr > 0.f)) return fail(ctx, SMS_E_INVALID, "sms_mlp_adam: learning rate must be positive");
-    mlp->step++;
-    const double b1 = 0.9, b2 = 0.999;
-    const float bc1 = (float) (1.0 - std::pow(b1, (double) mlp->step));
-    const float bc2 = (float) (1.0 - std::pow(b2, (double) mlp->step));
+    float bc1, bc2;
+    adam_corrections(mlp, &bc1, &bc2);
     hipLaunchKernelGGL(k_mlp_adam, dim3((MLP_PARAMS + 255) / 256), dim3(256), 0, ctx->stream, mlp->p, mlp->g, mlp->m,
-                       mlp->v, mlp->wb, MLP_PARAMS, lr, (float) b1, (float) b2, 1e-8f, bc1, bc2);
+                       mlp->v, mlp->wb, MLP_PARAMS, lr, ADAM_B1, ADAM_B2, ADAM_EPS, bc1, bc2);
     HIP_TRY(ctx, hipGetLastError());
+    return SMS_OK;
+}
+
+sms_status sms_mlp_train(sms_mlp *mlp, const float *x, const float *uv, const float *w, int32_t batch,
+                         const int64_t *offsets, int32_t n_steps, float lr, float *losses) {
+    if (!mlp) return SMS_E_INVALID;
+    sms_ctx *ctx = mlp->ctx;
+    if (n_steps < 0 || (n_steps > 0 && (!x || !uv || !offsets || batch <= 0)))
+        return fail(ctx, SMS_E_INVALID, "sms_mlp_train: bad arguments");
+    if (!(lr > 0.f)) return fail(ctx, SMS_E_INVALID, "sms_mlp_train: learning rate must be positive");
+    for (int32_t i = 0; i < n_steps; ++i)
+        if (offsets[i] < 0) return fail(ctx, SMS_E_INVALID, "sms_mlp_train: negative batch offset");
+    if (losses && n_steps > mlp->loss_cap) {
+        if (mlp->loss_hist) HIP_TRY(ctx, hipFree(mlp->loss_hist));
+        mlp->loss_hist = nullptr;
+        mlp->loss_cap = 0;
+        HIP_TRY(ctx, hipMalloc(&mlp->loss_hist, (size_t) n_steps * sizeof(float)));
+        mlp->loss_cap = n_steps;
+    }
+    const int rows = grad_rows(batch);
+    for (int32_t i = 0; i < n_steps; ++i) {
+        const int64_t o = offsets[i];
+        hipLaunchKernelGGL(k_mlp_grad, dim3(rows), dim3(256), 0, ctx->stream, mlp->p, mlp->wb, x + MLP_IN * o,
+                           uv + 2 * o, w ? w + o : nullptr, (int64_t) batch, mlp->partial, mlp->aux);
+        HIP_TRY(ctx, hipGetLastError());
+        float bc1, bc2;
+        adam_corrections(mlp, &bc1, &bc2);
+        hipLaunchKernelGGL(k_mlp_reduce_adam, dim3((MLP_PARAMS + 63) / 64), dim3(256), 0, ctx->stream,
+                           (const float *) mlp->partial, (const float *) mlp->aux, rows, mlp->p, mlp->m, mlp->v,
+                           mlp->wb, lr, ADAM_B1, ADAM_B2, ADAM_EPS, bc1, bc2, losses ? mlp->loss_hist + i : nullptr);
+        HIP_TRY(ctx, hipGetLastError());
+    }
+    if (losses && n_steps > 0) {
+        HIP_TRY(ctx, hipMemcpyAsync(losses, mlp->loss_hist, (size_t) n_steps * sizeof(float), hipMemcpyDeviceToHost,
+                                    ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    }
     return SMS_OK;
 }
 

@@ -151,7 +151,7 @@ EXPORTS = ["sms_config_default", "sms_ctx_create", "sms_ctx_destroy", "sms_ctx_s
            "sms_copy_h2d", "sms_copy_d2h", "sms_ms_walk_batch", "sms_ms_estimate_batch",
            "sms_ms_walk_batch_host", "sms_ms_estimate_batch_host", "sms_glint_autotune",
            "sms_mlp_create", "sms_mlp_destroy", "sms_mlp_get_params", "sms_mlp_set_params",
-           "sms_mlp_infer", "sms_mlp_grad", "sms_mlp_adam", "sms_ctx_set_proposal",
+           "sms_mlp_infer", "sms_mlp_grad", "sms_mlp_adam", "sms_mlp_train", "sms_ctx_set_proposal",
            "sms_ctx_set_training_capture", "sms_ctx_training_samples", "sms_ctx_clear_training",
            "sms_ss_proposal_inputs", "sms_ss_proposal_outputs", "sms_ss_make_queries",
            "sms_film_accumulate", "sms_ss_autotune"]
@@ -203,6 +203,7 @@ def load_library(path=LIB_PATH):
     lib.sms_mlp_infer.argtypes = [vp, vp, i64, vp]
     lib.sms_mlp_grad.argtypes = [vp, vp, vp, vp, i64, vp, C.POINTER(C.c_float)]
     lib.sms_mlp_adam.argtypes = [vp, C.c_float]
+    lib.sms_mlp_train.argtypes = [vp, vp, vp, vp, C.c_int32, vp, C.c_int32, C.c_float, vp]
     lib.sms_ctx_set_proposal.argtypes = [vp, vp, C.c_int32, C.c_float]
     lib.sms_ctx_set_training_capture.argtypes = [vp, i64, C.c_int32]
     lib.sms_ctx_training_samples.argtypes = [vp, C.POINTER(i64), vp, vp, vp]
@@ -210,7 +211,7 @@ def load_library(path=LIB_PATH):
     lib.sms_ss_proposal_inputs.argtypes = [vp, vp, i64, C.c_uint64, vp]
     lib.sms_ss_proposal_outputs.argtypes = [vp, vp, i64, C.c_uint64, vp]
     lib.sms_ss_make_queries.argtypes = [vp, C.c_int32, C.c_uint64, vp, vp]
-    lib.sms_film_accumulate.argtypes = [vp, vp, vp, i64, C.c_int32, vp, vp]
+    lib.sms_film_accumulate.argtypes = [vp, vp, vp, i64, C.c_int32, vp, vp, vp]
     for name in EXPORTS:
         fn = getattr(lib, name)
         if fn.restype is C.c_int and name not in ("sms_ctx_last_error", "sms_config_default"):

@@ -184,9 +184,10 @@ class Context:
         _check(lib().sms_ss_make_queries(self.h, spp, seed, qb.ptr, wb.ptr), self.h, "ss_queries")
         return qb, wb, n
 
-    def film_accumulate_device(self, rgbbuf, wbuf, pixels, spp, imgbuf, imgsqbuf=None):
+    def film_accumulate_device(self, rgbbuf, wbuf, pixels, spp, imgbuf, imgsqbuf=None, imgnbuf=None):
         _check(lib().sms_film_accumulate(self.h, rgbbuf.ptr, wbuf.ptr, pixels, spp, imgbuf.ptr,
-                                         imgsqbuf.ptr if imgsqbuf is not None else None), self.h, "film_accumulate")
+                                         imgsqbuf.ptr if imgsqbuf is not None else None,
+                                         imgnbuf.ptr if imgnbuf is not None else None), self.h, "film_accumulate")
 
     def glint_queries_device(self, spp, seed):
         cam = self.scene.camera
@@ -364,4 +365,20 @@ class Mlp:
 
     def adam(self, lr):
         _check(lib().sms_mlp_adam(self.h, lr), self.ctx.h, "sms_mlp_adam")
+
+    def train_device(self, xbuf, uvbuf, records, batch, offsets, lr, wbuf=None, want_losses=False):
+        """len(offsets) training steps (gradient over the batch window at each offset of the
+        device sample buffers, then Adam), queued without host synchronisation; returns the
+        per-step losses if asked (then waits)."""
+        offs = np.ascontiguousarray(offsets, np.int64)
+        if len(offs) and (offs.min() < 0 or offs.max() + batch > records):
+            raise ValueError("batch window outside the sample buffers")
+        for b, per in ((xbuf, 4 * abi.MLP_INPUTS), (uvbuf, 8), (wbuf, 4)):
+            if b is not None and b.nbytes < records * per:
+                raise ValueError("sample buffer smaller than the record count")
+        losses = np.zeros(len(offs), np.float32) if want_losses else None
+        _check(lib().sms_mlp_train(self.h, xbuf.ptr, uvbuf.ptr, wbuf.ptr if wbuf is not None else None, batch,
+                                   offs.ctypes.data if len(offs) else None, len(offs), lr,
+                                   losses.ctypes.data if want_losses else None), self.ctx.h, "sms_mlp_train")
+        return losses
 

@@ -15,13 +15,15 @@ from . import abi, gpu
 
 
 class Film:
-    """Per-pixel sums of weight * rgb and of its square over the samples taken (device)."""
+    """Per-pixel sums of weight * rgb, of its square and the count of samples kept (device);
+    invalid samples are dropped by the film kernel, as the reference's image block drops them."""
 
     def __init__(self, ctx, pixels):
         self.ctx = ctx
         self.pixels = pixels
         self.sum = gpu.DeviceBuffer(ctx, pixels * 12)
         self.sq = gpu.DeviceBuffer(ctx, pixels * 12)
+        self.n = gpu.DeviceBuffer(ctx, pixels * 4)
         self.spp = 0
         self.clear()
 
@@ -29,22 +31,32 @@ class Film:
         z = np.zeros((self.pixels, 3), np.float32)
         self.sum.upload(z)
         self.sq.upload(z)
+        self.n.upload(np.zeros(self.pixels, np.float32))
         self.spp = 0
 
     def download(self):
         """(mean, per-sample variance) images, float64, shape (pixels, 3)."""
         s = np.zeros((self.pixels, 3), np.float32)
         s2 = np.zeros((self.pixels, 3), np.float32)
+        cnt = np.zeros(self.pixels, np.float32)
         self.sum.download(s)
         self.sq.download(s2)
-        n = max(self.spp, 1)
+        self.n.download(cnt)
+        n = np.maximum(cnt.astype(np.float64), 1.0)[:, None]
         mean = s.astype(np.float64) / n
-        var = np.maximum(s2.astype(np.float64) / n - mean * mean, 0.0) * (n / max(n - 1, 1))
+        var = np.maximum(s2.astype(np.float64) / n - mean * mean, 0.0) * (n / np.maximum(n - 1, 1))
         return mean, var
+
+    def dropped(self):
+        """samples the film dropped as invalid"""
+        cnt = np.zeros(self.pixels, np.float32)
+        self.n.download(cnt)
+        return int(self.spp * self.pixels - cnt.astype(np.int64).sum())
 
     def free(self):
         self.sum.free()
         self.sq.free()
+        self.n.free()
 
 
 class PassRunner:
@@ -64,7 +76,7 @@ class PassRunner:
         """One pass: camera samples of seed `seed`, estimates with the same seed, into film."""
         self.ctx.ss_queries_device(self.pass_spp, seed, self.q, self.w)
         st = self.ctx.ss_estimate_device(self.q, self.n, seed, self.rgb)
-        self.ctx.film_accumulate_device(self.rgb, self.w, self.pixels, self.pass_spp, film.sum, film.sq)
+        self.ctx.film_accumulate_device(self.rgb, self.w, self.pixels, self.pass_spp, film.sum, film.sq, film.n)
         film.spp += self.pass_spp
         if stats is not None:
             for k, v in st.items():
@@ -76,49 +88,64 @@ class PassRunner:
             b.free()
 
 
-def train_online(ctx, runner, film, mlp, caster, samples=2_000_000, batch=8192, lr=1e-3, seed0=1 << 20,
-                 stats=None, rng_seed=0, epochs=1, mode=abi.CAPTURE_SOLUTIONS, weight_power=1.0, deadline=None):
-    """Render uniform-seed passes into `film` while capturing training records on mesh
-    `caster` (`mode`: every converged solution with weight 1, or the solutions behind non-zero
-    estimates weighted by the estimate, raised to `weight_power`); after each pass,
-    `epochs` x (records // batch) Adam steps on windows of `batch` consecutive records at random
-    offsets of that pass's capture, until `samples` records were trained on or, if given, the
-    time.perf_counter() `deadline` has passed.  Returns (passes, steps, losses)."""
+def render_online(ctx, runner, film, mlp, caster, budget_s, samples=2_000_000, batch=8192, lr=1e-3, epochs=16,
+                  uniform_passes=1, defensive=0.2, seed0=1 << 24, rng_seed=0, stats=None, max_steps_per_pass=None,
+                  stop_when_trained=False):
+    """One frame of `budget_s` seconds whose seed proposal is trained while it renders.  The
+    first `uniform_passes` passes use uniform seeds, every later pass the defensive mixture with
+    the network as trained so far.  Each pass captures estimate-weighted records on mesh
+    `caster`; a record's weight is its estimate, so the expected total weight at a solution is
+    the solution's contribution whatever proposal found it, and records of learned passes are
+    as valid as uniform ones.  After a pass, epochs x (records // batch) Adam steps (at most
+    max_steps_per_pass) on windows at random offsets of its records run before the next pass,
+    until fewer than `batch` of `samples` records are left (stop_when_trained: the frame ends
+    then).  Every pass
+    counts in `film`; passes stop when the next one would end after the budget.  Returns a dict
+    of passes, steps, losses (every 32nd step), used (records trained on) and train_s (seconds
+    spent training)."""
     rng = np.random.default_rng(rng_seed)
     ctx.set_proposal(None, caster)
-    ctx.set_training_capture(runner.n, mode)
-    used, steps, passes, losses = 0, 0, 0, []
-    weighted = mode == abi.CAPTURE_ESTIMATES
+    ctx.set_training_capture(runner.n, abi.CAPTURE_ESTIMATES)
+    t0 = time.perf_counter()
+    out = {"passes": 0, "steps": 0, "losses": [], "used": 0, "train_s": 0.0}
+    capturing, last = True, 0.0
     try:
-        while used < samples and (deadline is None or time.perf_counter() < deadline):
-            runner.run(seed0 + passes, film, stats)
-            passes += 1
-            xb, ub, wb, have = ctx.training_samples_device()
-            ctx.clear_training()
-            try:
-                if weighted and weight_power != 1.0 and have:
-                    w = np.zeros(have, np.float32)
-                    wb.download(w)
-                    wb.upload(np.power(np.maximum(w, 0.0), weight_power).astype(np.float32))
-                n_new = min(have, samples - used) // batch * batch
-                for _ in range(epochs * (n_new // batch)):
-                    off = int(rng.integers(0, have - batch + 1))
-                    loss, _ = mlp.grad_device(xb, ub, batch, want_loss=(steps % 32 == 0), offset=off,
-                                              wbuf=wb if weighted else None)
-                    if loss is not None:
-                        losses.append(loss)
-                    mlp.adam(lr)
-                    steps += 1
-                used += max(n_new, batch if have >= batch else 0)
-            finally:
-                xb.free()
-                ub.free()
-                wb.free()
-            if have < batch and passes > 1000:
-                raise RuntimeError("the caster yields too few converged solutions to train on")
+        while True:
+            t = time.perf_counter()
+            if out["passes"] and (t - t0 + last > budget_s or (stop_when_trained and not capturing)):
+                break
+            runner.run(seed0 + out["passes"], film, stats)
+            out["passes"] += 1
+            if capturing:
+                xb, ub, wb, have = ctx.training_samples_device()
+                ctx.clear_training()
+                try:
+                    n_new = min(have, samples - out["used"]) // batch * batch
+                    n_steps = epochs * (n_new // batch)
+                    if max_steps_per_pass is not None:
+                        n_steps = min(n_steps, max_steps_per_pass)
+                    if n_steps:
+                        offs = rng.integers(0, have - batch + 1, n_steps)
+                        t1 = time.perf_counter()
+                        hist = mlp.train_device(xb, ub, have, batch, offs, lr, wbuf=wb, want_losses=True)
+                        out["train_s"] += time.perf_counter() - t1
+                        out["losses"].extend(float(v) for v in hist[(-out["steps"]) % 32::32])
+                        out["steps"] += n_steps
+                    out["used"] += n_new
+                finally:
+                    xb.free()
+                    ub.free()
+                    wb.free()
+                if samples - out["used"] < batch:          # no whole batch left to train on
+                    capturing = False
+                    ctx.set_training_capture(0)
+                if out["passes"] >= uniform_passes:
+                    ctx.set_proposal(mlp, caster, defensive)
+            ctx.sync()
+            last = time.perf_counter() - t
     finally:
         ctx.set_training_capture(0)
-    return passes, steps, losses
+    return out
 
 
 def rmse(img, ref, ref_var_of_mean, mask=None):

@@ -56,28 +56,47 @@ def test_camera_queries_and_estimates_match_oracle(gpu, oracle_mod, scene_fn):
 
 
 def test_film_accumulate(gpu):
+    """Per-pixel sums in sample order; samples with a non-finite or negative weighted channel are
+    dropped and the count of kept samples is accumulated (src/librender/imageblock.cpp:85-107)."""
     sc = scenes.sphere_caustic(32, 16)
     ctx = gpu.Context(sc, abi.SmsConfig.default())
     pixels, spp = 32 * 16, 5
     rng = np.random.default_rng(1)
     rgb = rng.random((pixels * spp, 3)).astype(np.float32)
     w = rng.random(pixels * spp).astype(np.float32)
+    rgb[7, 1] = np.nan
+    rgb[30, 0] = np.inf
+    rgb[44, 2] = -1.0
+    rgb[45, 2] = -1e-6                   # within the reference's tolerance: kept
+    w[61] = 0.0
+    rgb[61, 0] = np.inf                  # 0 * inf = NaN: dropped
     img0 = rng.random((pixels, 3)).astype(np.float32)
-    bufs = [gpu.DeviceBuffer(ctx, a.nbytes) for a in (rgb, w, img0, img0)]
-    for b, a in zip(bufs, (rgb, w, img0, np.zeros_like(img0))):
+    n0 = np.full(pixels, 3.0, np.float32)
+    bufs = [gpu.DeviceBuffer(ctx, a.nbytes) for a in (rgb, w, img0, img0, n0)]
+    for b, a in zip(bufs, (rgb, w, img0, np.zeros_like(img0), n0)):
         b.upload(a)
-    ctx.film_accumulate_device(bufs[0], bufs[1], pixels, spp, bufs[2], bufs[3])
-    img, sq = np.zeros_like(img0), np.zeros_like(img0)
+    ctx.film_accumulate_device(bufs[0], bufs[1], pixels, spp, bufs[2], bufs[3], bufs[4])
+    img, sq, cnt = np.zeros_like(img0), np.zeros_like(img0), np.zeros_like(n0)
     bufs[2].download(img)
     bufs[3].download(sq)
-    v = (w[:, None] * rgb).reshape(pixels, spp, 3)
+    bufs[4].download(cnt)
+    with np.errstate(invalid="ignore"):
+        v = (w[:, None] * rgb).reshape(pixels, spp, 3)
+    keep = (np.isfinite(v) & (v >= np.float32(-1e-5))).all(2)
+    assert keep.sum() == pixels * spp - 4
     s = np.zeros((pixels, 3), np.float32)
     s2 = np.zeros((pixels, 3), np.float32)
     for k in range(spp):
-        s = s + v[:, k]
-        s2 = s2 + v[:, k] * v[:, k]
+        vk = np.where(keep[:, k, None], v[:, k], np.float32(0))
+        s = s + vk
+        s2 = s2 + vk * vk
     assert np.array_equal(img, img0 + s)
     assert np.array_equal(sq, s2)
+    assert np.array_equal(cnt, n0 + keep.sum(1).astype(np.float32))
+    # without the count buffer
+    ctx.film_accumulate_device(bufs[0], bufs[1], pixels, spp, bufs[2])
+    bufs[2].download(img)
+    assert np.array_equal(img, (img0 + s) + s)
     for b in bufs:
         b.free()
     ctx.close()

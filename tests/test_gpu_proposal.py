@@ -191,6 +191,53 @@ def test_adam_step(gpu, pool_scene):
     ctx.close()
 
 
+def test_train_loop_matches_steps_and_oracle(gpu, pool_scene):
+    """sms_mlp_train: bit-identical to the same steps through sms_mlp_grad + sms_mlp_adam,
+    bit-reproducible from run to run, and following the oracle's BF16 training trajectory."""
+    ctx = gpu.Context(pool_scene, abi.SmsConfig.default())
+    x, uv = _batch(3000, 12)
+    wts = np.random.default_rng(13).lognormal(0.0, 1.0, len(x)).astype(np.float32)
+    bufs = [gpu.DeviceBuffer(ctx, a.nbytes) for a in (x, uv, wts)]
+    for b, a in zip(bufs, (x, uv, wts)):
+        b.upload(a)
+    batch, offs = 1024, np.array([0, 1500, 977, 1976, 3], np.int64)
+    mlps = [gpu.Mlp(ctx, seed=21) for _ in range(3)]
+    p0 = mlps[0].get_params()
+    losses = mlps[0].train_device(bufs[0], bufs[1], len(x), batch, offs, 1e-3, wbuf=bufs[2], want_losses=True)
+    step_losses = []
+    for o in offs:
+        loss, _ = mlps[1].grad_device(bufs[0], bufs[1], batch, want_loss=True, offset=int(o), wbuf=bufs[2])
+        step_losses.append(loss)
+        mlps[1].adam(1e-3)
+    mlps[2].train_device(bufs[0], bufs[1], len(x), batch, offs, 1e-3, wbuf=bufs[2])
+    pa, pb, pc = (m.get_params() for m in mlps)
+    assert np.array_equal(pa, pb), "train loop differs from grad + adam steps"
+    assert np.array_equal(pa, pc), "training is not reproducible"
+    assert np.array_equal(losses, np.array(step_losses, np.float32))
+    # oracle trajectory (BF16 restatement, float32 Adam)
+    p, m, v = p0.copy(), np.zeros_like(p0), np.zeros_like(p0)
+    ref_losses = []
+    for t, o in enumerate(offs, 1):
+        sl = slice(int(o), int(o) + batch)
+        lo, g = M.loss_and_grad(p, x[sl], uv[sl], "bf16", wts[sl])
+        ref_losses.append(lo)
+        p, m, v = M.adam(p, g, m, v, t, 1e-3)
+    print(f"losses gpu {losses} oracle {np.array(ref_losses)}; params max diff {np.abs(pa - p).max():.2e}", flush=True)
+    np.testing.assert_allclose(losses, ref_losses, rtol=2e-3)
+    # Adam moves every parameter by about lr per step whatever the gradient's size, so a
+    # gradient entry near zero may take the other sign: most parameters must agree closely
+    close = np.abs(pa - p) < 2e-4
+    assert close.mean() > 0.97, close.mean()
+    assert np.abs(pa - p).max() < 1.2e-2
+    with pytest.raises(ValueError):
+        mlps[0].train_device(bufs[0], bufs[1], len(x), batch, [len(x) - batch + 1], 1e-3)
+    for b in bufs:
+        b.free()
+    for mm in mlps:
+        mm.close()
+    ctx.close()
+
+
 def _constant_mixture(mlp, centre=(0.5, 0.5), sigma=0.08):
     """Network whose output is one fixed mixture for every query (last layer: zero weights)."""
     p = mlp.get_params()
@@ -303,15 +350,11 @@ def test_training_beats_uniform(gpu, pool_scene):
         seed += 1
     xb, ub, wb, n = ctx.training_samples_device()
     mlp = gpu.Mlp(ctx, seed=3)
-    losses = []
     rng = np.random.default_rng(0)
     bs = 8192
-    for step in range(400):
-        off = int(rng.integers(0, n - bs))
-        loss, _ = mlp.grad_device(xb, ub, bs, want_loss=(step % 50 == 0 or step == 399), offset=off, wbuf=wb)
-        if loss is not None:
-            losses.append(loss)
-        mlp.adam(1e-3)
+    offs = rng.integers(0, n - bs, 400)
+    hist = mlp.train_device(xb, ub, n, bs, offs, 1e-3, wbuf=wb, want_losses=True)
+    losses = [float(hist[i]) for i in list(range(0, 400, 50)) + [399]]
     xb.free()
     ub.free()
     wb.free()
