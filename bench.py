@@ -9,7 +9,10 @@ Workload (one step = the SMS estimate of one frame, queries resident in HBM):
   render_shoes.py:84).
 
 Prints ONE JSON line.  `value` = converged walks/s (walks that returned success) over the
-whole timed region; `roofline` refers to the dominant kernel (k_glint_first: emitter sample,
+whole timed region, in which each frame is one replay of the frame's captured HIP graph
+(sms_glint_graph_create) with that frame's seed; the same frames are first run as eager
+estimator calls, whose HIP events time each kernel and whose statistics the replays must
+reproduce.  `roofline` refers to the dominant kernel (k_glint_first: emitter sample,
 visibility, footprint target and the first Newton walk of every query) timed with HIP events
 on the stream it runs on; `cpu_baseline` times the CPU oracle on a bounded sample
 of the same queries on this host's cores.
@@ -63,11 +66,14 @@ def parse():
     ap.add_argument("--pool-only", action="store_true", help="run only the pool single-scatter throughput")
     ap.add_argument("--ss-first-pass", type=int, default=None, help="pool: 0 walker, 1 lane per query (default: tuned)")
     ap.add_argument("--rmse-only", action="store_true", help="run only the equal-time RMSE experiment")
+    ap.add_argument("--mlp-only", action="store_true", help="run only the proposal network's training / inference timing")
+    ap.add_argument("--no-mlp", action="store_true", help="skip the proposal network's timing")
+    ap.add_argument("--mlp-steps", type=int, default=2000, help="timed training steps of the network leg")
     ap.add_argument("--rmse-res", type=int, default=1024, help="pool image width = height")
     ap.add_argument("--rmse-spp", type=int, default=64, help="uniform-seed samples per pixel (sets the time budget)")
     ap.add_argument("--rmse-ref-spp", type=int, default=512, help="reference samples per pixel (uniform seeds)")
     ap.add_argument("--rmse-pass-spp", type=int, default=8)
-    ap.add_argument("--train-samples", type=int, default=2_000_000)
+    ap.add_argument("--train-samples", type=int, default=4_000_000, help="records the online training uses")
     ap.add_argument("--train-epochs", type=int, default=16, help="Adam steps per 8192 captured records")
     ap.add_argument("--defensive", type=float, default=0.2)
     return ap.parse_args()
@@ -100,6 +106,9 @@ def main():
         return
     if args.pool_only:
         print(json.dumps(pool_throughput(args)), flush=True)
+        return
+    if args.mlp_only:
+        print(json.dumps(mlp_throughput(args)), flush=True)
         return
     import numpy as np
     from smsamd import abi, gpu, scenes
@@ -139,12 +148,29 @@ def main():
         for k, v in st.items():
             tot[k] = tot.get(k, 0) + v
     ctx.sync()
+    eager_s = time.perf_counter() - t_start
+    # the same frames (same seeds) replayed from a captured graph: one launch per frame, no host
+    # synchronisation between frames, statistics folded on the device
+    graph = ctx.glint_graph(qbuf, n, out)
+    graph.launch(1000)
+    graph.stats(reset=True)
+    ctx.sync()
+    t_start = time.perf_counter()
+    for s in range(args.steps):
+        graph.launch(1 + s)
+    ctx.sync()
     elapsed = time.perf_counter() - t_start
+    gtot = graph.stats(reset=True)
+    graph.close()
+    for k in ("walks", "walks_converged", "newton_iterations"):
+        if gtot[k] != tot[k]:
+            raise RuntimeError(f"graph replays differ from the eager frames in {k}: {gtot[k]} != {tot[k]}")
     ms_per_step = 1e3 * elapsed / args.steps
     conv_per_s = tot["walks_converged"] / elapsed
     walks_per_s = tot["walks"] / elapsed
-    print(f"# steps={args.steps} {ms_per_step:.2f} ms/step walks/s={walks_per_s:.4g} conv/s={conv_per_s:.4g} "
-          f"first={k1_ms / args.steps:.3f}ms rest={k2_ms / args.steps:.3f}ms stats={tot}", file=sys.stderr, flush=True)
+    print(f"# steps={args.steps} graph {ms_per_step:.3f} ms/step (eager {1e3 * eager_s / args.steps:.3f}) "
+          f"walks/s={walks_per_s:.4g} conv/s={conv_per_s:.4g} first={k1_ms / args.steps:.3f}ms "
+          f"rest={k2_ms / args.steps:.3f}ms stats={tot}", file=sys.stderr, flush=True)
 
     k1_s = k1_ms / 1e3 / args.steps
     achieved_gbs = BYTES_PER_ITER * (iters_pass1 / args.steps) / k1_s / 1e9
@@ -169,6 +195,7 @@ def main():
     qbuf.free()
     ctx.close()
     pool = None if args.no_pool else pool_throughput(args)
+    mlp = None if args.no_mlp else mlp_throughput(args)
     rmse = None if args.no_rmse else equal_time_rmse(args)
     result = {
         "metric": "converged walks/s at 1 GPU; equal-time RMSE learned vs uniform seeds",
@@ -187,10 +214,13 @@ def main():
                                f"{args.nm_res}^2 RGB8 scratch+bump normal map, unbiased (max_trials 1e5)",
                    "queries_per_step": n, "walks_per_step": tot["walks"] / args.steps,
                    "walks_per_s": walks_per_s, "device": dev, "schedule": sched,
+                   "launch": "hipGraph replay per frame",
+                   "eager_ms_per_step": round(1e3 * eager_s / args.steps, 4),
                    "trials_tail_finalize_ms": round(k2_ms / args.steps, 3)},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "pool_ss": pool,
+        "mlp": mlp,
         "equal_time_rmse": rmse,
     }
     if cpu and cpu.get("value"):
@@ -246,6 +276,68 @@ def pool_throughput(args):
     wb.free()
     ctx.close()
     return res_
+
+
+BF16_PEAK_TFLOPS = 2500.0      # MI355X dense BF16 matrix peak
+# algorithmic multiply-adds per sample: forward 153x64 + 3x64x64 + 64x48 (features unpadded);
+# training adds the weight gradients (same count) and the input gradients of layers 2-5
+MLP_FWD_MAC = 153 * 64 + 3 * 64 * 64 + 64 * 48
+MLP_TRAIN_MAC = 2 * MLP_FWD_MAC + 3 * 64 * 64 + 48 * 64
+
+
+def mlp_throughput(args):
+    """The learned proposal's network on synthetic records resident in HBM: `mlp_steps` training
+    steps (sms_mlp_train: gradient of the weighted mixture NLL over a batch of 8192, fused
+    reduction + Adam) and one inference over 1024^2 x 8 queries (sms_mlp_infer).  Wall-clock
+    per step / per inference (the calls are queued on the context stream and timed to the
+    synchronisation that ends them)."""
+    import numpy as np
+    from smsamd import abi, gpu, scenes
+    ctx = gpu.Context(scenes.sphere_dome(), abi.SmsConfig.default())
+    rng = np.random.default_rng(0)
+    n_rec, batch = 1 << 21, 8192
+    x = rng.uniform(-1.0, 1.0, (n_rec, abi.MLP_INPUTS)).astype(np.float32)
+    uv = rng.uniform(0.0, 1.0, (n_rec, 2)).astype(np.float32)
+    w = rng.lognormal(0.0, 1.0, n_rec).astype(np.float32)
+    bufs = [gpu.DeviceBuffer(ctx, a.nbytes) for a in (x, uv, w)]
+    for b, a in zip(bufs, (x, uv, w)):
+        b.upload(a)
+    mlp = gpu.Mlp(ctx, seed=1)
+    offs = rng.integers(0, n_rec - batch + 1, args.mlp_steps)
+    mlp.train_device(bufs[0], bufs[1], n_rec, batch, offs[:50], 1e-3, wbuf=bufs[2], want_losses=True)
+    t0 = time.perf_counter()
+    losses = mlp.train_device(bufs[0], bufs[1], n_rec, batch, offs, 1e-3, wbuf=bufs[2], want_losses=True)
+    train_s = (time.perf_counter() - t0) / args.mlp_steps
+    n_inf = 1024 * 1024 * 8
+    xi = gpu.DeviceBuffer(ctx, n_inf * abi.MLP_INPUTS * 4)
+    oi = gpu.DeviceBuffer(ctx, n_inf * abi.MLP_OUTPUTS * 4)
+    for k in range(n_inf // n_rec):                         # the records, repeated
+        gpu.lib().sms_copy_h2d(ctx.h, xi.ptr + k * x.nbytes, x.ctypes.data, x.nbytes)
+    mlp.infer_device(xi, n_inf, oi)
+    ctx.sync()
+    t0 = time.perf_counter()
+    reps = 5
+    for _ in range(reps):
+        mlp.infer_device(xi, n_inf, oi)
+    ctx.sync()
+    infer_s = (time.perf_counter() - t0) / reps
+    train_tflops = 2.0 * MLP_TRAIN_MAC * batch / train_s / 1e12
+    infer_tflops = 2.0 * MLP_FWD_MAC * n_inf / infer_s / 1e12
+    out = {"workload": "proposal network 9 -> 153 Fourier features -> 4x64 ReLU -> 48 (8-component mixture), "
+                       "BF16 MFMA with FP32 accumulation",
+           "train_batch": batch, "train_steps": args.mlp_steps, "train_us_per_step": round(1e6 * train_s, 2),
+           "train_samples_per_s": batch / train_s, "train_loss_first_last": [float(losses[0]), float(losses[-1])],
+           "infer_queries": n_inf, "infer_ms": round(1e3 * infer_s, 3), "infer_queries_per_s": n_inf / infer_s,
+           "roofline": {"bound": "mfma", "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "train_achieved": round(train_tflops, 2), "train_frac": round(train_tflops / BF16_PEAK_TFLOPS, 5),
+                        "infer_achieved": round(infer_tflops, 2), "infer_frac": round(infer_tflops / BF16_PEAK_TFLOPS, 5),
+                        "flops_per_sample_train": 2 * MLP_TRAIN_MAC, "flops_per_query_infer": 2 * MLP_FWD_MAC}}
+    print(f"# mlp: {json.dumps(out)}", file=sys.stderr, flush=True)
+    for b in bufs + [xi, oi]:
+        b.free()
+    mlp.close()
+    ctx.close()
+    return out
 
 
 def equal_time_rmse(args):

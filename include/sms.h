@@ -456,6 +456,38 @@ sms_status sms_ss_proposal_inputs(sms_ctx *ctx, const sms_ss_query *q, int64_t n
    seed -- the exact values sms_ss_estimate_batch turns into its seed mixtures. */
 sms_status sms_ss_proposal_outputs(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *out);
 
+/* ---------------------------------------------------------------- frame graphs
+   No counterpart in the reference (which renders through per-thread scalar calls).  A graph
+   holds one estimator call's launch sequence on the context stream -- frame start (seed and
+   work counters), the estimator's kernels, a statistics fold -- captured once, then replayed by
+   sms_graph_launch with a new seed and no per-kernel host work; launches are asynchronous and
+   may follow each other without host synchronisation.  Each launch gives the results of the
+   matching sms_*_estimate_batch call with that seed, bit for bit.  The graph keeps the
+   context's buffers, configuration, schedule, proposal and training capture as they were at
+   creation: any later change of them (including a larger estimate on the same context, which
+   grows the work buffers) makes sms_graph_launch return SMS_E_INVALID until the graph is
+   recreated.  Queries, outputs and (for the pass graph) film buffers are device memory that
+   must outlive the graph; the proposal network's weights are read at each launch, so
+   training between launches is seen by the next one. */
+typedef struct sms_graph sms_graph;
+/* Graph of sms_glint_estimate_batch(ctx, q, n, seed, out_rgb, ...). */
+sms_status sms_glint_graph_create(sms_ctx *ctx, const sms_glint_query *q, int64_t n, float *out_rgb, sms_graph **out);
+/* Graph of sms_ss_estimate_batch(ctx, q, n, seed, out_rgb, ...). */
+sms_status sms_ss_graph_create(sms_ctx *ctx, const sms_ss_query *q, int64_t n, float *out_rgb, sms_graph **out);
+/* Graph of sms_ms_estimate_batch(ctx, q, n, seed, out_rgb, ...). */
+sms_status sms_ms_graph_create(sms_ctx *ctx, const sms_ss_query *q, int64_t n, float *out_rgb, sms_graph **out);
+/* Graph of one single-scatter frame pass: sms_ss_make_queries(ctx, spp, seed, q, weight), then
+   sms_ss_estimate_batch(ctx, q, w*h*spp, seed, out_rgb), then sms_film_accumulate(ctx, out_rgb,
+   weight, w*h, spp, image, image_sq, image_n) (image_sq and image_n may be NULL). */
+sms_status sms_ss_pass_graph_create(sms_ctx *ctx, int32_t spp, sms_ss_query *q, float *weight, float *out_rgb,
+                                    float *image, float *image_sq, float *image_n, sms_graph **out);
+/* Enqueue one replay with `seed` on the context stream. */
+sms_status sms_graph_launch(sms_graph *g, uint64_t seed);
+/* Waits for the stream; *stats (if not NULL) = the statistics summed over the launches since
+   creation or the last reset (bernoulli_max: their maximum); reset != 0 starts a new sum. */
+sms_status sms_graph_stats(sms_graph *g, sms_stats *stats, int32_t reset);
+sms_status sms_graph_destroy(sms_graph *g);
+
 /* ---------------------------------------------------------------- device memory helpers */
 sms_status sms_device_alloc(sms_ctx *ctx, size_t bytes, void **ptr);
 sms_status sms_device_free(sms_ctx *ctx, void *ptr);

@@ -53,7 +53,25 @@ struct sms_ctx {
     float *prop_x = nullptr, *prop_raw = nullptr, *prop_g = nullptr; size_t prop_cap = 0;
     // capture of (inputs, solution uv) of converged first walks
     float *cap = nullptr; unsigned long long *cap_counts = nullptr; size_t cap_sub = 0; int cap_mode = 0;
+    // seeds the kernels read from device memory (slots SEED_*), so that a captured frame graph
+    // takes a new seed per launch by updating its first node only
+    uint64_t *seeds = nullptr;
+    // bumped whenever buffers or settings that a captured graph holds by value change
+    uint64_t gen = 1;
+    bool capturing = false;                     // launch sequence being captured: no event records
 };
+
+// seed slots: estimator, camera pass, the proposal's input / output queries
+constexpr int SEED_EST = 0, SEED_CAM = 1, SEED_API = 2, SEED_SLOTS = 4;
+
+// counters buffer: queue counters 0..31, then the spread stats copies, then the spread
+// counters of the glint trial and overflow lists
+constexpr int STATS_OFF = 32;
+constexpr int TRIALS_OFF = STATS_OFF + SPREAD * SPREAD_STRIDE;
+constexpr int OVER_OFF = TRIALS_OFF + SPREAD * SPREAD_STRIDE;
+constexpr int N_COUNTERS = OVER_OFF + SPREAD * SPREAD_STRIDE;
+// folded frame statistics (k_stats_fold): sms_stats fields 0..7 in order, then overflow queries
+constexpr int N_FOLDED = 9;
 
 
 namespace smshost {
@@ -73,7 +91,31 @@ inline sms_status fail(sms_ctx *ctx, sms_status code, const std::string &msg) {
 
 } // namespace smshost
 
+// record an event unless the launch sequence is being captured into a graph
+#define EV_RECORD(ctx, ev)                                                                      \
+    do {                                                                                        \
+        if (!(ctx)->capturing) HIP_TRY(ctx, hipEventRecord(ev, (ctx)->stream));                 \
+    } while (0)
+
 // learned seed proposal (sms_mlp_host.hip)
 Proposal proposal_of(sms_ctx *ctx);
-sms_status proposal_prepare(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed);
+sms_status proposal_alloc(sms_ctx *ctx, int64_t n);
+sms_status proposal_enqueue(sms_ctx *ctx, const sms_ss_query *q, int64_t n);
 void proposal_release(sms_ctx *ctx);
+
+// launch sequences shared by the estimator calls and the frame graphs (sms_host.hip).  The
+// *_prepare functions validate and allocate (never while capturing); the *_enqueue functions
+// only launch kernels, reading the seed of slot SEED_EST (SEED_CAM for the camera passes).
+__global__ void k_frame_begin(unsigned long long *counters, int n, uint64_t *seeds, uint64_t seed);
+__global__ void k_set_seed(uint64_t *slot, uint64_t seed);
+__global__ void k_stats_fold(const unsigned long long *counters, unsigned long long *total);
+sms_status frame_begin(sms_ctx *ctx, uint64_t seed);
+sms_status glint_prepare(sms_ctx *ctx, const sms_glint_query *q, int64_t n, const float *out_rgb);
+sms_status glint_enqueue(sms_ctx *ctx, const sms_glint_query *q, int64_t n, float *out_rgb);
+sms_status ss_prepare(sms_ctx *ctx, const sms_ss_query *q, int64_t n, const float *out_rgb);
+sms_status ss_enqueue(sms_ctx *ctx, const sms_ss_query *q, int64_t n, float *out_rgb);
+sms_status ms_prepare(sms_ctx *ctx, const sms_ss_query *q, int64_t n, const float *out_rgb);
+sms_status ms_enqueue(sms_ctx *ctx, const sms_ss_query *q, int64_t n, float *out_rgb);
+sms_status ss_queries_enqueue(sms_ctx *ctx, int32_t spp, sms_ss_query *q, float *weight);
+sms_status film_enqueue(sms_ctx *ctx, const float *rgb, const float *weight, int64_t pixels, int32_t spp, float *image,
+                        float *image_sq, float *image_n);

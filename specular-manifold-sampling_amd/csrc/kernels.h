@@ -134,28 +134,28 @@ SMS_DEV V3 glint_result(float val, float inv_prob, float f, V3 ew) {
 
 __global__ void k_ss_walk(DScene S, const sms_ss_walk_in *in, int64_t n, sms_ss_walk_out *out);
 __global__ void k_glint_walk(DScene S, const sms_glint_walk_in *in, int64_t n, sms_glint_walk_out *out);
-__global__ void k_glint_queries(DScene S, int spp, uint64_t seed, sms_glint_query *q, int64_t n);
-__global__ void k_glint_first_bvh(DScene S, const sms_glint_query *Q, int64_t n, uint64_t seed, float *out, EstimatorWork W);
-__global__ void k_glint_first_nobvh(DScene S, const sms_glint_query *Q, int64_t n, uint64_t seed, float *out, EstimatorWork W);
-__global__ void k_glint_trials_g1(DScene S, uint64_t seed, EstimatorWork W);
-__global__ void k_glint_trials_g2(DScene S, uint64_t seed, EstimatorWork W);
-__global__ void k_glint_trials_g4(DScene S, uint64_t seed, EstimatorWork W);
-__global__ void k_glint_trials_g8(DScene S, uint64_t seed, EstimatorWork W);
-__global__ void k_glint_trials_g16(DScene S, uint64_t seed, EstimatorWork W);
-__global__ void k_glint_coop_rounds(DScene S, uint64_t seed, EstimatorWork W);
-__global__ void k_glint_multiwave(DScene S, uint64_t seed, EstimatorWork W);
+__global__ void k_glint_queries(DScene S, int spp, const uint64_t *seedp, sms_glint_query *q, int64_t n);
+__global__ void k_glint_first_bvh(DScene S, const sms_glint_query *Q, int64_t n, const uint64_t *seedp, float *out, EstimatorWork W);
+__global__ void k_glint_first_nobvh(DScene S, const sms_glint_query *Q, int64_t n, const uint64_t *seedp, float *out, EstimatorWork W);
+__global__ void k_glint_trials_g1(DScene S, const uint64_t *seedp, EstimatorWork W);
+__global__ void k_glint_trials_g2(DScene S, const uint64_t *seedp, EstimatorWork W);
+__global__ void k_glint_trials_g4(DScene S, const uint64_t *seedp, EstimatorWork W);
+__global__ void k_glint_trials_g8(DScene S, const uint64_t *seedp, EstimatorWork W);
+__global__ void k_glint_trials_g16(DScene S, const uint64_t *seedp, EstimatorWork W);
+__global__ void k_glint_coop_rounds(DScene S, const uint64_t *seedp, EstimatorWork W);
+__global__ void k_glint_multiwave(DScene S, const uint64_t *seedp, EstimatorWork W);
 __global__ void k_glint_multiwave_resolve(DScene S, EstimatorWork W);
-__global__ void k_glint_coop_seq(DScene S, uint64_t seed, EstimatorWork W);
-__global__ void k_glint_finalize(DScene S, const sms_glint_query *Q, uint64_t seed, float *out, EstimatorWork W);
-__global__ void k_ss_estimate(DScene S, const sms_ss_query *Q, int64_t n, uint64_t seed, float *part, float *invp, EstimatorWork W);
-__global__ void k_ss_walker(DScene S, const sms_ss_query *Q, int64_t n, uint64_t seed, float *part, float *invp,
+__global__ void k_glint_coop_seq(DScene S, const uint64_t *seedp, EstimatorWork W);
+__global__ void k_glint_finalize(DScene S, const sms_glint_query *Q, const uint64_t *seedp, float *out, EstimatorWork W);
+__global__ void k_ss_estimate(DScene S, const sms_ss_query *Q, int64_t n, const uint64_t *seedp, float *part, float *invp, EstimatorWork W);
+__global__ void k_ss_walker(DScene S, const sms_ss_query *Q, int64_t n, const uint64_t *seedp, float *part, float *invp,
                             EstimatorWork W);
-__global__ void k_ss_coop(DScene S, const sms_ss_query *Q, uint64_t seed, const float *part, float *invp, EstimatorWork W);
+__global__ void k_ss_coop(DScene S, const sms_ss_query *Q, const uint64_t *seedp, const float *part, float *invp, EstimatorWork W);
 __global__ void k_ss_finalize(const float *part, const float *invp, int nc, int64_t n, float *out);
-__global__ void k_ss_queries(DScene S, int spp, uint64_t seed, sms_ss_query *q, float *w, int64_t n);
+__global__ void k_ss_queries(DScene S, int spp, const uint64_t *seedp, sms_ss_query *q, float *w, int64_t n);
 __global__ void k_film_accumulate(const float *rgb, const float *w, int64_t pixels, int spp, float *image, float *image_sq,
                                   float *image_n);
-__global__ void k_prop_inputs(DScene S, const sms_ss_query *Q, int64_t n, uint64_t seed, float *x);
+__global__ void k_prop_inputs(DScene S, const sms_ss_query *Q, int64_t n, const uint64_t *seedp, float *x);
 __global__ void k_gmm_prepare(const float *raw, int64_t n, float *g);
 __global__ void k_capture_gather(const float *cap, const unsigned long long *counts, unsigned long long cap_sub,
                                  float *x, float *uv, float *w);
@@ -168,13 +168,13 @@ __global__ void k_mlp_reduce_adam(const float *partial, const float *aux, int nb
                                   float *loss);
 __global__ void k_mlp_adam(float *p, float *g, float *m, float *v, uint16_t *wb, int n, float lr, float b1, float b2,
                            float eps, float bc1, float bc2);
-__global__ void k_ss_biased(DScene S, const sms_ss_query *Q, int64_t n, uint64_t seed, float *out, EstimatorWork W);
+__global__ void k_ss_biased(DScene S, const sms_ss_query *Q, int64_t n, const uint64_t *seedp, float *out, EstimatorWork W);
 #define SMS_MS_KERNEL_DECLS(B)                                                                                   \
     __global__ void k_ms_walk_##B(DScene S, const sms_ms_walk_in *in, int64_t n, sms_ms_walk_out *out);         \
-    __global__ void k_ms_estimate_##B(DScene S, const sms_ss_query *Q, int64_t n, uint64_t seed, float *part,   \
+    __global__ void k_ms_estimate_##B(DScene S, const sms_ss_query *Q, int64_t n, const uint64_t *seedp, float *part,   \
                                       float *invp, EstimatorWork W);                                             \
-    __global__ void k_ms_coop_##B(DScene S, const sms_ss_query *Q, uint64_t seed, float *invp, EstimatorWork W); \
-    __global__ void k_ms_biased_##B(DScene S, const sms_ss_query *Q, int64_t n, uint64_t seed, float *out,      \
+    __global__ void k_ms_coop_##B(DScene S, const sms_ss_query *Q, const uint64_t *seedp, float *invp, EstimatorWork W); \
+    __global__ void k_ms_biased_##B(DScene S, const sms_ss_query *Q, int64_t n, const uint64_t *seedp, float *out,      \
                                     EstimatorWork W);
 SMS_MS_KERNEL_DECLS(2)
 SMS_MS_KERNEL_DECLS(4)

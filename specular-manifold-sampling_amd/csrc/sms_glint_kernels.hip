@@ -156,12 +156,12 @@ SMS_DEV void glint_first_body(const DScene &S, const sms_glint_query *__restrict
 
 #define SMS_GLINT_FIRST_KERNEL(NAME, BVH)                                                                        \
     __global__ void __launch_bounds__(256) NAME(DScene S, const sms_glint_query *__restrict__ Q, int64_t n,      \
-                                                uint64_t seed, float *__restrict__ out, EstimatorWork W) {       \
+                                                const uint64_t *__restrict__ seedp, float *__restrict__ out, EstimatorWork W) {       \
         __shared__ GlintRec lrec[256];                                                                           \
         __shared__ int wcount[4];                                                                                \
         __shared__ unsigned long long lbase;                                                                     \
         __shared__ unsigned long long lstats[4][4];                                                              \
-        glint_first_body<BVH>(S, Q, n, seed, out, W, lrec, wcount, &lbase, lstats);                              \
+        glint_first_body<BVH>(S, Q, n, *seedp, out, W, lrec, wcount, &lbase, lstats);                              \
     }
 SMS_GLINT_FIRST_KERNEL(k_glint_first_bvh, true)
 SMS_GLINT_FIRST_KERNEL(k_glint_first_nobvh, false)
@@ -262,8 +262,8 @@ SMS_DEV void glint_trials_body(const DScene &S, uint64_t seed, EstimatorWork W) 
 }
 
 #define SMS_GLINT_TRIALS_KERNEL(G)                                                                               \
-    __global__ void __launch_bounds__(256) k_glint_trials_g##G(DScene S, uint64_t seed, EstimatorWork W) {       \
-        glint_trials_body<G>(S, seed, W);                                                                        \
+    __global__ void __launch_bounds__(256) k_glint_trials_g##G(DScene S, const uint64_t *__restrict__ seedp, EstimatorWork W) {       \
+        glint_trials_body<G>(S, *seedp, W);                                                                        \
     }
 SMS_GLINT_TRIALS_KERNEL(1)
 SMS_GLINT_TRIALS_KERNEL(2)
@@ -378,7 +378,8 @@ SMS_DEV void coop_stats(EstimatorWork W, const unsigned long long st[5]) {
 }
 
 // overflow items, one wavefront each, by a static mapping over the spread overflow list
-__global__ void __launch_bounds__(256) k_glint_coop_rounds(DScene S, uint64_t seed, EstimatorWork W) {
+__global__ void __launch_bounds__(256) k_glint_coop_rounds(DScene S, const uint64_t *__restrict__ seedp, EstimatorWork W) {
+    const uint64_t seed = *seedp;
     __shared__ float2 tile[4][COOP_TILE + 1][COOP_TILE + 1];
     unsigned long long st[5] = {0ull, 0ull, 0ull, 0ull, 0ull};
     const unsigned long long wave = ((unsigned long long) blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -392,7 +393,8 @@ __global__ void __launch_bounds__(256) k_glint_coop_rounds(DScene S, uint64_t se
 }
 
 // chains longer than the multi-wavefront range, one wavefront each
-__global__ void __launch_bounds__(256) k_glint_coop_seq(DScene S, uint64_t seed, EstimatorWork W) {
+__global__ void __launch_bounds__(256) k_glint_coop_seq(DScene S, const uint64_t *__restrict__ seedp, EstimatorWork W) {
+    const uint64_t seed = *seedp;
     __shared__ float2 tile[4][COOP_TILE + 1][COOP_TILE + 1];
     unsigned long long st[5] = {0ull, 0ull, 0ull, 0ull, 0ull};
     const unsigned long long n_items = *W.seq_list.count;
@@ -408,7 +410,8 @@ __global__ void __launch_bounds__(256) k_glint_coop_seq(DScene S, uint64_t seed,
 // (atomic min).  Chunks past the current first match are not claimed; chunks that ran past
 // the final one are discarded by the resolve pass, which sums the per-trial log only up to
 // the first match, so the counters equal the sequential trial loop's.
-__global__ void __launch_bounds__(256) k_glint_multiwave(DScene S, uint64_t seed, EstimatorWork W) {
+__global__ void __launch_bounds__(256) k_glint_multiwave(DScene S, const uint64_t *__restrict__ seedp, EstimatorWork W) {
+    const uint64_t seed = *seedp;
     __shared__ float2 tile_all[4][COOP_TILE + 1][COOP_TILE + 1];
     const sms_config &cfg = S.cfg;
     const int lane = threadIdx.x & 63;
@@ -524,8 +527,9 @@ __global__ void __launch_bounds__(256) k_glint_multiwave_resolve(DScene S, Estim
 
 
 // ---------------------------------------------------------------- contributions
-__global__ void __launch_bounds__(256) k_glint_finalize(DScene S, const sms_glint_query *__restrict__ Q, uint64_t seed,
+__global__ void __launch_bounds__(256) k_glint_finalize(DScene S, const sms_glint_query *__restrict__ Q, const uint64_t *__restrict__ seedp,
                                                         float *__restrict__ out, EstimatorWork W) {
+    const uint64_t seed = *seedp;
     const sms_config &cfg = S.cfg;
     const unsigned long long u = (unsigned long long) blockIdx.x * blockDim.x + threadIdx.x;
     const SpreadWalk sw = spread_walk(W.trials, u, (unsigned long long) gridDim.x * blockDim.x);

@@ -464,16 +464,59 @@ __global__ void k_zero_u64(unsigned long long *p, int n) {
     if (i < n) p[i] = 0ull;
 }
 
-// counters buffer: queue counters 0..31, then the spread stats copies, then the spread
-// counters of the glint trial and overflow lists
-constexpr int STATS_OFF = 32;
-constexpr int TRIALS_OFF = STATS_OFF + SPREAD * SPREAD_STRIDE;
-constexpr int OVER_OFF = TRIALS_OFF + SPREAD * SPREAD_STRIDE;
-constexpr int N_COUNTERS = OVER_OFF + SPREAD * SPREAD_STRIDE;
+
+} // namespace
+
+// Frame start: the estimator and camera seed slots take `seed` and the work counters start at
+// zero -- a kernel rather than a memset, so that a captured frame graph holds kernel nodes only.
+__global__ void k_frame_begin(unsigned long long *counters, int n, uint64_t *seeds, uint64_t seed) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) counters[i] = 0ull;
+    if (i == 0) {
+        seeds[SEED_EST] = seed;
+        seeds[SEED_CAM] = seed;
+    }
+}
+
+__global__ void k_set_seed(uint64_t *slot, uint64_t seed) {
+    if (threadIdx.x == 0) *slot = seed;
+}
+
+// Add a frame's counters to total[0..8]: the spread copies of the stats summed (bernoulli_max
+// as a maximum) into total[0..7] in sms_stats order, the overflow count into total[8].  One
+// wavefront: lane s reads copy s of every counter, then butterfly reductions.
+static_assert(SPREAD == 64, "k_stats_fold maps one lane to each spread copy");
+__global__ void k_stats_fold(const unsigned long long *counters, unsigned long long *total) {
+    const int s = threadIdx.x;
+    unsigned long long v[9];
+    for (int j = 0; j < 8; ++j) v[j] = counters[STATS_OFF + s * SPREAD_STRIDE + j];
+    v[8] = counters[OVER_OFF + s * SPREAD_STRIDE] + (s == 0 ? counters[1] : 0ull);
+    for (int off = 32; off > 0; off >>= 1)
+        for (int j = 0; j < 9; ++j) {
+            const unsigned long long o = __shfl_xor(v[j], off, 64);
+            v[j] = j == 5 ? (o > v[j] ? o : v[j]) : v[j] + o;
+        }
+    if (s == 0)
+        for (int j = 0; j < 9; ++j) total[j] = j == 5 ? (v[j] > total[j] ? v[j] : total[j]) : total[j] + v[j];
+}
+
+sms_status frame_begin(sms_ctx *ctx, uint64_t seed) {
+    hipLaunchKernelGGL(k_frame_begin, dim3((N_COUNTERS + 255) / 256), dim3(256), 0, ctx->stream, ctx->counters,
+                       N_COUNTERS, ctx->seeds, seed);
+    HIP_TRY(ctx, hipGetLastError());
+    return SMS_OK;
+}
+
+namespace {
 
 sms_status ensure_work(sms_ctx *ctx, size_t n_overflow, size_t n_part) {
     if (!ctx->counters) HIP_TRY(ctx, hipMalloc((void **) &ctx->counters, N_COUNTERS * sizeof(unsigned long long)));
+    if (!ctx->seeds) {
+        HIP_TRY(ctx, hipMalloc((void **) &ctx->seeds, SEED_SLOTS * sizeof(uint64_t)));
+        HIP_TRY(ctx, hipMemset(ctx->seeds, 0, SEED_SLOTS * sizeof(uint64_t)));
+    }
     if (n_overflow > ctx->overflow_cap) {
+        ctx->gen++;
         if (ctx->overflow) HIP_TRY(ctx, hipFree(ctx->overflow));
         ctx->overflow = nullptr;
         size_t cap = std::max(n_overflow, (size_t) 1 << 16);
@@ -481,6 +524,7 @@ sms_status ensure_work(sms_ctx *ctx, size_t n_overflow, size_t n_part) {
         ctx->overflow_cap = cap;
     }
     if (n_part > ctx->part_cap) {
+        ctx->gen++;
         if (ctx->part) HIP_TRY(ctx, hipFree(ctx->part));
         if (ctx->invp) HIP_TRY(ctx, hipFree(ctx->invp));
         ctx->part = ctx->invp = nullptr;
@@ -494,11 +538,13 @@ sms_status ensure_work(sms_ctx *ctx, size_t n_overflow, size_t n_part) {
 // Spread overflow list with sub-lists of sub_cap items (kept for the call that asks).
 sms_status ensure_over(sms_ctx *ctx, size_t sub_cap) {
     if (SPREAD * sub_cap > ctx->gover_cap) {
+        ctx->gen++;
         if (ctx->gover) HIP_TRY(ctx, hipFree(ctx->gover));
         ctx->gover = nullptr;
         HIP_TRY(ctx, hipMalloc((void **) &ctx->gover, SPREAD * sub_cap * sizeof(OverflowRec)));
         ctx->gover_cap = SPREAD * sub_cap;
     }
+    if (ctx->over_sub != sub_cap) ctx->gen++;
     ctx->over_sub = sub_cap;
     return SMS_OK;
 }
@@ -516,6 +562,7 @@ constexpr int MW_TRIALS = 32768;
 
 sms_status ensure_glint_work(sms_ctx *ctx, size_t n) {
     if (n > ctx->trial_cap) {
+        ctx->gen++;
         void **bufs[4] = {(void **) &ctx->trial, (void **) &ctx->res, (void **) &ctx->long_items,
                           (void **) &ctx->seq_items};
         for (void **b : bufs) {
@@ -537,12 +584,15 @@ sms_status ensure_glint_work(sms_ctx *ctx, size_t n) {
         if (st) return st;
     }
     if (!ctx->mw_log) {
+        ctx->gen++;
         HIP_TRY(ctx, hipMalloc((void **) &ctx->mw_found, MW_ITEMS * sizeof(long long)));
         HIP_TRY(ctx, hipMalloc((void **) &ctx->mw_next, MW_ITEMS * sizeof(unsigned long long)));
         HIP_TRY(ctx, hipMalloc((void **) &ctx->mw_log, (size_t) MW_ITEMS * MW_TRIALS * sizeof(uint16_t)));
     }
     // the per-trial log stores iteration counts in 15 bits
-    ctx->mw_items = ctx->scene.cfg.max_iterations < 0x8000 ? ctx->long_chains : 0;
+    const int mw_items = ctx->scene.cfg.max_iterations < 0x8000 ? ctx->long_chains : 0;
+    if (mw_items != ctx->mw_items || ctx->long_trials != ctx->mw_trials) ctx->gen++;
+    ctx->mw_items = mw_items;
     ctx->mw_trials = ctx->long_trials;
     return SMS_OK;
 }
@@ -558,6 +608,7 @@ sms_schedule schedule_of(const sms_ctx *ctx) {
 }
 
 void apply_schedule(sms_ctx *ctx, const sms_schedule &s) {
+    ctx->gen++;
     ctx->lane_trials = s.lane_trials; ctx->blocks_per_cu = s.blocks_per_cu;
     ctx->trial_group = s.trial_group; ctx->tail_rounds = s.tail_rounds;
     ctx->long_chains = s.long_chains; ctx->long_trials = s.long_trials;
@@ -692,6 +743,7 @@ sms_status sms_ctx_destroy(sms_ctx *ctx) {
     if (ctx->mw_log) (void) hipFree(ctx->mw_log);
     if (ctx->gover) (void) hipFree(ctx->gover);
     if (ctx->res) (void) hipFree(ctx->res);
+    if (ctx->seeds) (void) hipFree(ctx->seeds);
     proposal_release(ctx);
     if (ctx->ev3) (void) hipEventDestroy(ctx->ev3);
     if (ctx->ev0) (void) hipEventDestroy(ctx->ev0);
@@ -707,6 +759,7 @@ sms_status sms_ctx_set_config(sms_ctx *ctx, const sms_config *cfg) {
     sms_status st = check_config(ctx, cfg);
     if (st) return st;
     ctx->scene.cfg = *cfg;
+    ctx->gen++;
     return SMS_OK;
 }
 
@@ -774,22 +827,16 @@ sms_status sms_glint_walk_batch(sms_ctx *ctx, const sms_glint_walk_in *in, int64
 sms_status sms_ss_make_queries(sms_ctx *ctx, int32_t spp, uint64_t seed, sms_ss_query *q, float *weight) {
     if (!ctx || spp <= 0 || !q || !weight) return fail(ctx, SMS_E_INVALID, "bad query arguments");
     if (!ctx->scene.has_cam) return fail(ctx, SMS_E_INVALID, "scene has no camera");
-    int64_t n = (int64_t) ctx->scene.cam.w * ctx->scene.cam.h * spp;
-    hipLaunchKernelGGL(k_ss_queries, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream, ctx->scene, (int) spp,
-                       seed, q, weight, n);
+    hipLaunchKernelGGL(k_set_seed, dim3(1), dim3(64), 0, ctx->stream, ctx->seeds + SEED_CAM, seed);
     HIP_TRY(ctx, hipGetLastError());
-    return SMS_OK;
+    return ss_queries_enqueue(ctx, spp, q, weight);
 }
 
 sms_status sms_film_accumulate(sms_ctx *ctx, const float *rgb, const float *weight, int64_t pixels, int32_t spp,
                                float *image, float *image_sq, float *image_n) {
     if (!ctx || pixels < 0 || spp <= 0 || (pixels > 0 && (!rgb || !weight || !image)))
         return fail(ctx, SMS_E_INVALID, "bad film arguments");
-    if (pixels == 0) return SMS_OK;
-    hipLaunchKernelGGL(k_film_accumulate, dim3((unsigned) ((pixels + 255) / 256)), dim3(256), 0, ctx->stream, rgb, weight,
-                       pixels, (int) spp, image, image_sq, image_n);
-    HIP_TRY(ctx, hipGetLastError());
-    return SMS_OK;
+    return film_enqueue(ctx, rgb, weight, pixels, spp, image, image_sq, image_n);
 }
 
 sms_status sms_glint_make_queries(sms_ctx *ctx, int32_t spp, uint64_t seed, sms_glint_query *q) {
@@ -797,14 +844,14 @@ sms_status sms_glint_make_queries(sms_ctx *ctx, int32_t spp, uint64_t seed, sms_
     if (!ctx->scene.has_cam) return fail(ctx, SMS_E_INVALID, "scene has no camera");
     int64_t n = (int64_t) ctx->scene.cam.w * ctx->scene.cam.h * spp;
     int blocks = (int) ((n + 255) / 256);
-    hipLaunchKernelGGL(k_glint_queries, dim3(blocks), dim3(256), 0, ctx->stream, ctx->scene, (int) spp, seed, q, n);
+    hipLaunchKernelGGL(k_set_seed, dim3(1), dim3(64), 0, ctx->stream, ctx->seeds + SEED_CAM, seed);
+    hipLaunchKernelGGL(k_glint_queries, dim3(blocks), dim3(256), 0, ctx->stream, ctx->scene, (int) spp,
+                       (const uint64_t *) ctx->seeds + SEED_CAM, q, n);
     HIP_TRY(ctx, hipGetLastError());
     return SMS_OK;
 }
 
 } // extern "C"
-
-namespace {
 
 sms_status glint_prepare(sms_ctx *ctx, const sms_glint_query *q, int64_t n, const float *out_rgb) {
     if (!ctx || n < 0 || (n > 0 && (!q || !out_rgb))) return fail(ctx, SMS_E_INVALID, "bad estimator arguments");
@@ -815,11 +862,11 @@ sms_status glint_prepare(sms_ctx *ctx, const sms_glint_query *q, int64_t n, cons
     return st;
 }
 
-// Enqueue the glint pass sequence on the context stream (no host synchronisation); events
-// ev0 / ev1 / ev2 bracket the first-walk pass and the rest.
-sms_status glint_enqueue(sms_ctx *ctx, const sms_glint_query *q, int64_t n, uint64_t seed, float *out_rgb) {
-    hipLaunchKernelGGL(k_zero_u64, dim3((N_COUNTERS + 255) / 256), dim3(256), 0, ctx->stream, ctx->counters, N_COUNTERS);
-    HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+// The glint pass sequence on the context stream (no host synchronisation; the frame's seed and
+// zeroed counters come first); events ev0 / ev1 / ev2 bracket the first-walk pass and the rest.
+sms_status glint_enqueue(sms_ctx *ctx, const sms_glint_query *q, int64_t n, float *out_rgb) {
+    const uint64_t *seed = ctx->seeds + SEED_EST;
+    EV_RECORD(ctx, ctx->ev0);
     if (n > 0) {
         EstimatorWork W = work_of(ctx);
         const unsigned blocks = (unsigned) ((n + 255) / 256);
@@ -828,7 +875,7 @@ sms_status glint_enqueue(sms_ctx *ctx, const sms_glint_query *q, int64_t n, uint
         else
             hipLaunchKernelGGL(k_glint_first_nobvh, dim3(blocks), dim3(256), 0, ctx->stream, ctx->scene, q, n, seed, out_rgb, W);
         HIP_TRY(ctx, hipGetLastError());
-        HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+        EV_RECORD(ctx, ctx->ev1);
         const dim3 gt(grid_for(ctx, ctx->occ_glint)), gc(grid_for(ctx, ctx->occ_coop)), gm(grid_for(ctx, ctx->occ_mw));
         switch (ctx->trial_group) {
         case 1: hipLaunchKernelGGL(k_glint_trials_g1, gt, dim3(256), 0, ctx->stream, ctx->scene, seed, W); break;
@@ -850,9 +897,136 @@ sms_status glint_enqueue(sms_ctx *ctx, const sms_glint_query *q, int64_t n, uint
                            out_rgb, W);
         HIP_TRY(ctx, hipGetLastError());
     } else {
-        HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+        EV_RECORD(ctx, ctx->ev1);
     }
-    HIP_TRY(ctx, hipEventRecord(ctx->ev2, ctx->stream));
+    EV_RECORD(ctx, ctx->ev2);
+    return SMS_OK;
+}
+
+sms_status ss_prepare(sms_ctx *ctx, const sms_ss_query *q, int64_t n, const float *out_rgb) {
+    if (!ctx || n < 0 || (n > 0 && (!q || !out_rgb))) return fail(ctx, SMS_E_INVALID, "bad estimator arguments");
+    const sms_config &cfg = ctx->scene.cfg;
+    const int nc = ctx->scene.n_casters_ss;
+    if (cfg.biased && cfg.max_trials > 128) return fail(ctx, SMS_E_UNSUPPORTED, "biased trial sets above 128 are not built");
+    sms_status st = cfg.biased ? SMS_OK : ensure_over(ctx, query_sub_cap(n, nc));
+    if (!st) st = ensure_work(ctx, 0, cfg.biased ? 0 : (size_t) std::max<int64_t>(n * nc, 1));
+    if (!st) st = proposal_alloc(ctx, n);
+    return st;
+}
+
+// single-scatter estimator: proposal mixtures (if a network is set), then the biased pass or
+// the unbiased first pass (persistent walker or lane per query), cooperative tail, finalize
+sms_status ss_enqueue(sms_ctx *ctx, const sms_ss_query *q, int64_t n, float *out_rgb) {
+    const sms_config &cfg = ctx->scene.cfg;
+    const int nc = ctx->scene.n_casters_ss;
+    const uint64_t *seed = ctx->seeds + SEED_EST;
+    EV_RECORD(ctx, ctx->ev0);
+    if (n > 0) {
+        sms_status st = proposal_enqueue(ctx, q, n);
+        if (st) return st;
+        EstimatorWork W = work_of(ctx);
+        if (cfg.biased) {
+            hipLaunchKernelGGL(k_ss_biased, dim3(grid_for(ctx, ctx->occ_biased)), dim3(256), 0, ctx->stream,
+                               ctx->scene, q, n, seed, out_rgb, W);
+            HIP_TRY(ctx, hipGetLastError());
+            EV_RECORD(ctx, ctx->ev1);
+        } else {
+            if (ctx->ss_first_pass == 0) {
+                // persistent: the resident blocks take queries from the device queue
+                const int per_cu = ctx->blocks_per_cu > 0 ? ctx->blocks_per_cu : ctx->occ_walker;
+                hipLaunchKernelGGL(k_ss_walker, dim3((unsigned) (per_cu * ctx->n_cu)), dim3(256), 0, ctx->stream,
+                                   ctx->scene, q, n, seed, ctx->part, ctx->invp, W);
+            } else {
+                hipLaunchKernelGGL(k_ss_estimate, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                                   ctx->scene, q, n, seed, ctx->part, ctx->invp, W);
+            }
+            HIP_TRY(ctx, hipGetLastError());
+            EV_RECORD(ctx, ctx->ev1);
+            hipLaunchKernelGGL(k_ss_coop, dim3(grid_for(ctx, ctx->occ_sscoop)), dim3(256), 0, ctx->stream,
+                               ctx->scene, q, seed, ctx->part, ctx->invp, W);
+            HIP_TRY(ctx, hipGetLastError());
+            hipLaunchKernelGGL(k_ss_finalize, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                               ctx->part, ctx->invp, nc, n, out_rgb);
+            HIP_TRY(ctx, hipGetLastError());
+        }
+    } else {
+        EV_RECORD(ctx, ctx->ev1);
+    }
+    EV_RECORD(ctx, ctx->ev2);
+    return SMS_OK;
+}
+
+sms_status ms_prepare(sms_ctx *ctx, const sms_ss_query *q, int64_t n, const float *out_rgb) {
+    if (!ctx || n < 0 || (n > 0 && (!q || !out_rgb))) return fail(ctx, SMS_E_INVALID, "bad estimator arguments");
+    sms_status st = check_ms(ctx);
+    if (st) return st;
+    const sms_config &cfg = ctx->scene.cfg;
+    const int nc = ctx->scene.n_casters_ms;
+    if (cfg.biased && cfg.max_trials > 128) return fail(ctx, SMS_E_UNSUPPORTED, "biased trial sets above 128 are not built");
+    st = cfg.biased ? SMS_OK : ensure_over(ctx, query_sub_cap(n, nc));
+    if (!st) st = ensure_work(ctx, 0, cfg.biased ? 0 : (size_t) std::max<int64_t>(n * nc, 1));
+    return st;
+}
+
+sms_status ms_enqueue(sms_ctx *ctx, const sms_ss_query *q, int64_t n, float *out_rgb) {
+    const sms_config &cfg = ctx->scene.cfg;
+    const int nc = ctx->scene.n_casters_ms;
+    const int b = cfg.bounces <= 2 ? 0 : 1;
+    const uint64_t *seed = ctx->seeds + SEED_EST;
+    EV_RECORD(ctx, ctx->ev0);
+    if (n > 0) {
+        EstimatorWork W = work_of(ctx);
+        if (cfg.biased) {
+            dim3 grid(grid_for(ctx, ctx->occ_msbiased[b]));
+            if (b == 0) hipLaunchKernelGGL(k_ms_biased_2, grid, dim3(256), 0, ctx->stream, ctx->scene, q, n, seed, out_rgb, W);
+            else hipLaunchKernelGGL(k_ms_biased_4, grid, dim3(256), 0, ctx->stream, ctx->scene, q, n, seed, out_rgb, W);
+            HIP_TRY(ctx, hipGetLastError());
+            EV_RECORD(ctx, ctx->ev1);
+        } else {
+            dim3 g1((unsigned) ((n + 255) / 256)), g2(grid_for(ctx, ctx->occ_mscoop[b]));
+            if (b == 0) hipLaunchKernelGGL(k_ms_estimate_2, g1, dim3(256), 0, ctx->stream, ctx->scene, q, n, seed, ctx->part, ctx->invp, W);
+            else hipLaunchKernelGGL(k_ms_estimate_4, g1, dim3(256), 0, ctx->stream, ctx->scene, q, n, seed, ctx->part, ctx->invp, W);
+            HIP_TRY(ctx, hipGetLastError());
+            EV_RECORD(ctx, ctx->ev1);
+            if (b == 0) hipLaunchKernelGGL(k_ms_coop_2, g2, dim3(256), 0, ctx->stream, ctx->scene, q, seed, ctx->invp, W);
+            else hipLaunchKernelGGL(k_ms_coop_4, g2, dim3(256), 0, ctx->stream, ctx->scene, q, seed, ctx->invp, W);
+            HIP_TRY(ctx, hipGetLastError());
+            hipLaunchKernelGGL(k_ss_finalize, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                               ctx->part, ctx->invp, nc, n, out_rgb);
+            HIP_TRY(ctx, hipGetLastError());
+        }
+    } else {
+        EV_RECORD(ctx, ctx->ev1);
+    }
+    EV_RECORD(ctx, ctx->ev2);
+    return SMS_OK;
+}
+
+// camera pass of the single-scatter integrator (seed slot SEED_CAM)
+sms_status ss_queries_enqueue(sms_ctx *ctx, int32_t spp, sms_ss_query *q, float *weight) {
+    const int64_t n = (int64_t) ctx->scene.cam.w * ctx->scene.cam.h * spp;
+    hipLaunchKernelGGL(k_ss_queries, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream, ctx->scene, (int) spp,
+                       (const uint64_t *) ctx->seeds + SEED_CAM, q, weight, n);
+    HIP_TRY(ctx, hipGetLastError());
+    return SMS_OK;
+}
+
+sms_status film_enqueue(sms_ctx *ctx, const float *rgb, const float *weight, int64_t pixels, int32_t spp, float *image,
+                        float *image_sq, float *image_n) {
+    if (pixels == 0) return SMS_OK;
+    hipLaunchKernelGGL(k_film_accumulate, dim3((unsigned) ((pixels + 255) / 256)), dim3(256), 0, ctx->stream, rgb, weight,
+                       pixels, (int) spp, image, image_sq, image_n);
+    HIP_TRY(ctx, hipGetLastError());
+    return SMS_OK;
+}
+
+namespace {
+
+sms_status estimator_done(sms_ctx *ctx, sms_stats *stats) {
+    sms_status st = read_stats(ctx, stats);
+    if (st) return st;
+    HIP_TRY(ctx, hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1));
+    HIP_TRY(ctx, hipEventElapsedTime(&ctx->last_ms2, ctx->ev1, ctx->ev2));
     return SMS_OK;
 }
 
@@ -863,12 +1037,9 @@ extern "C" {
 sms_status sms_glint_estimate_batch(sms_ctx *ctx, const sms_glint_query *q, int64_t n, uint64_t seed, float *out_rgb,
                                     sms_stats *stats) {
     sms_status st = glint_prepare(ctx, q, n, out_rgb);
-    if (!st) st = glint_enqueue(ctx, q, n, seed, out_rgb);
-    if (!st) st = read_stats(ctx, stats);
-    if (st) return st;
-    HIP_TRY(ctx, hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1));
-    HIP_TRY(ctx, hipEventElapsedTime(&ctx->last_ms2, ctx->ev1, ctx->ev2));
-    return SMS_OK;
+    if (!st) st = frame_begin(ctx, seed);
+    if (!st) st = glint_enqueue(ctx, q, n, out_rgb);
+    return st ? st : estimator_done(ctx, stats);
 }
 
 sms_status sms_glint_autotune(sms_ctx *ctx, const sms_glint_query *q, int64_t n, uint64_t seed, float *out_rgb,
@@ -884,7 +1055,8 @@ sms_status sms_glint_autotune(sms_ctx *ctx, const sms_glint_query *q, int64_t n,
         ctx->trial_group = g; ctx->lane_trials = g * m; ctx->tail_rounds = r; ctx->blocks_per_cu = b;
         float ms = 0.f;
         for (int rep = 0; rep < 2; ++rep) {           // the first run warms caches and allocations
-            st = glint_enqueue(ctx, q, n, seed + (uint64_t) rep, out_rgb);
+            st = frame_begin(ctx, seed + (uint64_t) rep);
+            if (!st) st = glint_enqueue(ctx, q, n, out_rgb);
             if (st) break;
             HIP_TRY(ctx, hipEventSynchronize(ctx->ev2));
             HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev2));
@@ -934,52 +1106,10 @@ sms_status sms_ss_autotune(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint6
 
 sms_status sms_ss_estimate_batch(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *out_rgb,
                                  sms_stats *stats) {
-    if (!ctx || n < 0 || (n > 0 && (!q || !out_rgb))) return fail(ctx, SMS_E_INVALID, "bad estimator arguments");
-    const sms_config &cfg = ctx->scene.cfg;
-    int nc = ctx->scene.n_casters_ss;
-    if (cfg.biased && cfg.max_trials > 128) return fail(ctx, SMS_E_UNSUPPORTED, "biased trial sets above 128 are not built");
-    sms_status st = cfg.biased ? SMS_OK : ensure_over(ctx, query_sub_cap(n, nc));
-    if (!st) st = ensure_work(ctx, 0, cfg.biased ? 0 : (size_t) std::max<int64_t>(n * nc, 1));
-    if (st) return st;
-    hipLaunchKernelGGL(k_zero_u64, dim3((N_COUNTERS + 255) / 256), dim3(256), 0, ctx->stream, ctx->counters, N_COUNTERS);
-    HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    if (n > 0) {
-        st = proposal_prepare(ctx, q, n, seed);
-        if (st) return st;
-        EstimatorWork W = work_of(ctx);
-        if (cfg.biased) {
-            hipLaunchKernelGGL(k_ss_biased, dim3(grid_for(ctx, ctx->occ_biased)), dim3(256), 0, ctx->stream,
-                               ctx->scene, q, n, seed, out_rgb, W);
-            HIP_TRY(ctx, hipGetLastError());
-            HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-        } else {
-            if (ctx->ss_first_pass == 0) {
-                // persistent: the resident blocks take queries from the device queue
-                const int per_cu = ctx->blocks_per_cu > 0 ? ctx->blocks_per_cu : ctx->occ_walker;
-                hipLaunchKernelGGL(k_ss_walker, dim3((unsigned) (per_cu * ctx->n_cu)), dim3(256), 0, ctx->stream,
-                                   ctx->scene, q, n, seed, ctx->part, ctx->invp, W);
-            } else {
-                hipLaunchKernelGGL(k_ss_estimate, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream,
-                                   ctx->scene, q, n, seed, ctx->part, ctx->invp, W);
-            }
-            HIP_TRY(ctx, hipGetLastError());
-            HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-            hipLaunchKernelGGL(k_ss_coop, dim3(grid_for(ctx, ctx->occ_sscoop)), dim3(256), 0, ctx->stream,
-                               ctx->scene, q, seed, ctx->part, ctx->invp, W);
-            HIP_TRY(ctx, hipGetLastError());
-            hipLaunchKernelGGL(k_ss_finalize, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream,
-                               ctx->part, ctx->invp, nc, n, out_rgb);
-            HIP_TRY(ctx, hipGetLastError());
-        }
-    } else {
-        HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-    }
-    HIP_TRY(ctx, hipEventRecord(ctx->ev2, ctx->stream));
-    st = read_stats(ctx, stats);
-    if (st) return st;
-    HIP_TRY(ctx, hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1));
-    HIP_TRY(ctx, hipEventElapsedTime(&ctx->last_ms2, ctx->ev1, ctx->ev2));
-    return SMS_OK;
+    sms_status st = ss_prepare(ctx, q, n, out_rgb);
+    if (!st) st = frame_begin(ctx, seed);
+    if (!st) st = ss_enqueue(ctx, q, n, out_rgb);
+    return st ? st : estimator_done(ctx, stats);
 }
 
 sms_status sms_ms_walk_batch(sms_ctx *ctx, const sms_ms_walk_in *in, int64_t n, sms_ms_walk_out *out) {
@@ -998,48 +1128,10 @@ sms_status sms_ms_walk_batch(sms_ctx *ctx, const sms_ms_walk_in *in, int64_t n, 
 
 sms_status sms_ms_estimate_batch(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *out_rgb,
                                  sms_stats *stats) {
-    if (!ctx || n < 0 || (n > 0 && (!q || !out_rgb))) return fail(ctx, SMS_E_INVALID, "bad estimator arguments");
-    sms_status st = check_ms(ctx);
-    if (st) return st;
-    const sms_config &cfg = ctx->scene.cfg;
-    const int nc = ctx->scene.n_casters_ms;
-    const int b = cfg.bounces <= 2 ? 0 : 1;
-    if (cfg.biased && cfg.max_trials > 128) return fail(ctx, SMS_E_UNSUPPORTED, "biased trial sets above 128 are not built");
-    st = cfg.biased ? SMS_OK : ensure_over(ctx, query_sub_cap(n, nc));
-    if (!st) st = ensure_work(ctx, 0, cfg.biased ? 0 : (size_t) std::max<int64_t>(n * nc, 1));
-    if (st) return st;
-    hipLaunchKernelGGL(k_zero_u64, dim3((N_COUNTERS + 255) / 256), dim3(256), 0, ctx->stream, ctx->counters, N_COUNTERS);
-    HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    if (n > 0) {
-        EstimatorWork W = work_of(ctx);
-        if (cfg.biased) {
-            dim3 grid(grid_for(ctx, ctx->occ_msbiased[b]));
-            if (b == 0) hipLaunchKernelGGL(k_ms_biased_2, grid, dim3(256), 0, ctx->stream, ctx->scene, q, n, seed, out_rgb, W);
-            else hipLaunchKernelGGL(k_ms_biased_4, grid, dim3(256), 0, ctx->stream, ctx->scene, q, n, seed, out_rgb, W);
-            HIP_TRY(ctx, hipGetLastError());
-            HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-        } else {
-            dim3 g1((unsigned) ((n + 255) / 256)), g2(grid_for(ctx, ctx->occ_mscoop[b]));
-            if (b == 0) hipLaunchKernelGGL(k_ms_estimate_2, g1, dim3(256), 0, ctx->stream, ctx->scene, q, n, seed, ctx->part, ctx->invp, W);
-            else hipLaunchKernelGGL(k_ms_estimate_4, g1, dim3(256), 0, ctx->stream, ctx->scene, q, n, seed, ctx->part, ctx->invp, W);
-            HIP_TRY(ctx, hipGetLastError());
-            HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-            if (b == 0) hipLaunchKernelGGL(k_ms_coop_2, g2, dim3(256), 0, ctx->stream, ctx->scene, q, seed, ctx->invp, W);
-            else hipLaunchKernelGGL(k_ms_coop_4, g2, dim3(256), 0, ctx->stream, ctx->scene, q, seed, ctx->invp, W);
-            HIP_TRY(ctx, hipGetLastError());
-            hipLaunchKernelGGL(k_ss_finalize, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream,
-                               ctx->part, ctx->invp, nc, n, out_rgb);
-            HIP_TRY(ctx, hipGetLastError());
-        }
-    } else {
-        HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-    }
-    HIP_TRY(ctx, hipEventRecord(ctx->ev2, ctx->stream));
-    st = read_stats(ctx, stats);
-    if (st) return st;
-    HIP_TRY(ctx, hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1));
-    HIP_TRY(ctx, hipEventElapsedTime(&ctx->last_ms2, ctx->ev1, ctx->ev2));
-    return SMS_OK;
+    sms_status st = ms_prepare(ctx, q, n, out_rgb);
+    if (!st) st = frame_begin(ctx, seed);
+    if (!st) st = ms_enqueue(ctx, q, n, out_rgb);
+    return st ? st : estimator_done(ctx, stats);
 }
 
 // ---------------------------------------------------------------- device memory helpers

@@ -62,7 +62,8 @@ __global__ void k_glint_walk(DScene S, const sms_glint_walk_in *__restrict__ in,
 }
 
 // ================================================================ glint queries (camera)
-__global__ void k_glint_queries(DScene S, int spp, uint64_t seed, sms_glint_query *__restrict__ q, int64_t n) {
+__global__ void k_glint_queries(DScene S, int spp, const uint64_t *__restrict__ seedp, sms_glint_query *__restrict__ q, int64_t n) {
+    const uint64_t seed = *seedp;
     int64_t idx = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n) return;
     const DCamera &c = S.cam;
@@ -112,8 +113,9 @@ __global__ void k_glint_queries(DScene S, int spp, uint64_t seed, sms_glint_quer
 // Pass 1: lane per query; for every caster the first walk, visibility, contribution and up
 // to lane_trials Bernoulli trials.  Per (query, caster) the product bsdf*specular (bxs) and
 // the trial count are stored; unresolved (query, caster) pairs go to the overflow list.
-__global__ void __launch_bounds__(256) k_ss_estimate(DScene S, const sms_ss_query *__restrict__ Q, int64_t n, uint64_t seed,
+__global__ void __launch_bounds__(256) k_ss_estimate(DScene S, const sms_ss_query *__restrict__ Q, int64_t n, const uint64_t *__restrict__ seedp,
                                                      float *__restrict__ part, float *__restrict__ invp, EstimatorWork W) {
+    const uint64_t seed = *seedp;
     const sms_config &cfg = S.cfg;
     unsigned long long st_walks = 0, st_conv = 0, st_iters = 0, st_bcalls = 0, st_btrials = 0, st_bmax = 0;
     const int nc = S.n_casters_ss;
@@ -183,9 +185,10 @@ __global__ void __launch_bounds__(256) k_ss_estimate(DScene S, const sms_ss_quer
     stat_add(W.stats + 3, st_bcalls); stat_add(W.stats + 4, st_btrials); stat_max(W.stats + 5, st_bmax);
 }
 
-__global__ void __launch_bounds__(256) k_ss_coop(DScene S, const sms_ss_query *__restrict__ Q, uint64_t seed,
+__global__ void __launch_bounds__(256) k_ss_coop(DScene S, const sms_ss_query *__restrict__ Q, const uint64_t *__restrict__ seedp,
                                                  const float *__restrict__ part, float *__restrict__ invp,
                                                  EstimatorWork W) {
+    const uint64_t seed = *seedp;
     const sms_config &cfg = S.cfg;
     const int lane = threadIdx.x & 63;
     const int nc = S.n_casters_ss;
@@ -257,8 +260,9 @@ __global__ void k_ss_finalize(const float *__restrict__ part, const float *__res
 // de-duplicated in m order through an LDS list and their contributions summed in m order
 // (manifold_ss.cpp:142-197).
 constexpr int BIASED_MAX_SOL = 128;
-__global__ void __launch_bounds__(256) k_ss_biased(DScene S, const sms_ss_query *__restrict__ Q, int64_t n, uint64_t seed,
+__global__ void __launch_bounds__(256) k_ss_biased(DScene S, const sms_ss_query *__restrict__ Q, int64_t n, const uint64_t *__restrict__ seedp,
                                                    float *__restrict__ out, EstimatorWork W) {
+    const uint64_t seed = *seedp;
     __shared__ float sol_lds[4][BIASED_MAX_SOL][3];
     const sms_config &cfg = S.cfg;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -349,8 +353,9 @@ __global__ void __launch_bounds__(256) k_ss_biased(DScene S, const sms_ss_query 
 // (path_sms_ss.cpp:118-124) with the path throughput as its weight.  Paths that leave the
 // scene or stop on another surface give shape -1 and weight 0.
 constexpr int SS_CAMERA_DEPTH = 8;
-__global__ void k_ss_queries(DScene S, int spp, uint64_t seed, sms_ss_query *__restrict__ q, float *__restrict__ w,
+__global__ void k_ss_queries(DScene S, int spp, const uint64_t *__restrict__ seedp, sms_ss_query *__restrict__ q, float *__restrict__ w,
                              int64_t n) {
+    const uint64_t seed = *seedp;
     const int64_t idx = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n) return;
     const DCamera &c = S.cam;
@@ -440,8 +445,9 @@ __global__ void k_film_accumulate(const float *__restrict__ rgb, const float *__
 // ================================================================ learned seed proposal
 // network inputs of every query of an estimate: the shading point, its shading normal and the
 // light sample the estimator draws for it (same stream, same arithmetic as k_ss_estimate)
-__global__ void k_prop_inputs(DScene S, const sms_ss_query *__restrict__ Q, int64_t n, uint64_t seed,
+__global__ void k_prop_inputs(DScene S, const sms_ss_query *__restrict__ Q, int64_t n, const uint64_t *__restrict__ seedp,
                               float *__restrict__ x) {
+    const uint64_t seed = *seedp;
     const int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
     const sms_ss_query g = Q[q];

@@ -116,6 +116,7 @@ int grad_rows(int64_t n) {
 
 sms_status ensure_prop(sms_ctx *ctx, size_t n) {
     if (n <= ctx->prop_cap) return SMS_OK;
+    ctx->gen++;
     const size_t cap = std::max<size_t>(n, 2 * ctx->prop_cap);
     for (float **b : {&ctx->prop_x, &ctx->prop_raw, &ctx->prop_g})
         if (*b) { (void) hipFree(*b); *b = nullptr; }
@@ -169,7 +170,8 @@ sms_status build_uv_grid(sms_ctx *ctx, int shape) {
     return SMS_OK;
 }
 
-sms_status launch_inputs(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *x) {
+// network inputs of n queries with the seed of device slot `seed`
+sms_status launch_inputs(sms_ctx *ctx, const sms_ss_query *q, int64_t n, const uint64_t *seed, float *x) {
     if (n == 0) return SMS_OK;
     hipLaunchKernelGGL(k_prop_inputs, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream, ctx->scene, q, n,
                        seed, x);
@@ -194,11 +196,18 @@ Proposal proposal_of(sms_ctx *ctx) {
     return P;
 }
 
-// network inputs -> raw outputs -> mixtures of the n queries of an estimate (prop_g)
-sms_status proposal_prepare(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed) {
+// buffers of the proposal's per-query mixtures for estimates of n queries
+sms_status proposal_alloc(sms_ctx *ctx, int64_t n) {
     if (!ctx->prop_mlp || ctx->prop_shape < 0 || n == 0) return SMS_OK;
-    sms_status st = ensure_prop(ctx, (size_t) n);
-    if (!st) st = launch_inputs(ctx, q, n, seed, ctx->prop_x);
+    return ensure_prop(ctx, (size_t) n);
+}
+
+// network inputs -> raw outputs -> mixtures of the n queries of an estimate (prop_g), with the
+// estimator's seed slot
+sms_status proposal_enqueue(sms_ctx *ctx, const sms_ss_query *q, int64_t n) {
+    if (!ctx->prop_mlp || ctx->prop_shape < 0 || n == 0) return SMS_OK;
+    if ((size_t) n > ctx->prop_cap) return fail(ctx, SMS_E_INVALID, "proposal buffers not prepared");
+    sms_status st = launch_inputs(ctx, q, n, ctx->seeds + SEED_EST, ctx->prop_x);
     if (!st) st = launch_infer(ctx->prop_mlp, ctx->prop_x, n, ctx->prop_raw);
     if (st) return st;
     hipLaunchKernelGGL(k_gmm_prepare, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream, ctx->prop_raw, n,
@@ -249,7 +258,10 @@ sms_status sms_mlp_destroy(sms_mlp *mlp) {
     if (!mlp) return SMS_E_INVALID;
     sms_ctx *ctx = mlp->ctx;
     (void) hipStreamSynchronize(ctx->stream);
-    if (ctx->prop_mlp == mlp) ctx->prop_mlp = nullptr;
+    if (ctx->prop_mlp == mlp) {
+        ctx->prop_mlp = nullptr;
+        ctx->gen++;
+    }
     free_mlp(mlp);
     return SMS_OK;
 }
@@ -346,6 +358,7 @@ sms_status sms_ctx_set_proposal(sms_ctx *ctx, sms_mlp *mlp, int32_t shape, float
     if (shape < 0) {
         if (mlp) return fail(ctx, SMS_E_INVALID, "sms_ctx_set_proposal: a network needs a caster shape");
         ctx->prop_mlp = nullptr; ctx->prop_shape = -1; ctx->prop_defensive = 1.f;
+        ctx->gen++;
         return SMS_OK;
     }
     if (shape >= (int) ctx->h_shapes.size()) return fail(ctx, SMS_E_INVALID, "sms_ctx_set_proposal: shape out of range");
@@ -362,6 +375,7 @@ sms_status sms_ctx_set_proposal(sms_ctx *ctx, sms_mlp *mlp, int32_t shape, float
     ctx->prop_mlp = mlp;
     ctx->prop_shape = shape;
     ctx->prop_defensive = mlp ? defensive : 1.f;
+    ctx->gen++;
     return SMS_OK;
 }
 
@@ -370,6 +384,7 @@ sms_status sms_ctx_set_training_capture(sms_ctx *ctx, int64_t capacity, int32_t 
     if (mode != SMS_CAPTURE_SOLUTIONS && mode != SMS_CAPTURE_ESTIMATES)
         return fail(ctx, SMS_E_INVALID, "sms_ctx_set_training_capture: unknown mode");
     ctx->cap_mode = mode;
+    ctx->gen++;
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     if (ctx->cap) { (void) hipFree(ctx->cap); ctx->cap = nullptr; }
     if (ctx->cap_counts) { (void) hipFree(ctx->cap_counts); ctx->cap_counts = nullptr; }
@@ -414,14 +429,17 @@ sms_status sms_ctx_clear_training(sms_ctx *ctx) {
 
 sms_status sms_ss_proposal_inputs(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *x) {
     if (!ctx || n < 0 || (n > 0 && (!q || !x))) return fail(ctx, SMS_E_INVALID, "sms_ss_proposal_inputs: bad arguments");
-    return launch_inputs(ctx, q, n, seed, x);
+    hipLaunchKernelGGL(k_set_seed, dim3(1), dim3(64), 0, ctx->stream, ctx->seeds + SEED_API, seed);
+    HIP_TRY(ctx, hipGetLastError());
+    return launch_inputs(ctx, q, n, ctx->seeds + SEED_API, x);
 }
 
 sms_status sms_ss_proposal_outputs(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *out) {
     if (!ctx || n < 0 || (n > 0 && (!q || !out))) return fail(ctx, SMS_E_INVALID, "sms_ss_proposal_outputs: bad arguments");
     if (!ctx->prop_mlp) return fail(ctx, SMS_E_INVALID, "sms_ss_proposal_outputs: no proposal network set");
     sms_status st = ensure_prop(ctx, (size_t) std::max<int64_t>(n, 1));
-    if (!st) st = launch_inputs(ctx, q, n, seed, ctx->prop_x);
+    hipLaunchKernelGGL(k_set_seed, dim3(1), dim3(64), 0, ctx->stream, ctx->seeds + SEED_API, seed);
+    if (!st) st = launch_inputs(ctx, q, n, ctx->seeds + SEED_API, ctx->prop_x);
     if (!st) st = launch_infer(ctx->prop_mlp, ctx->prop_x, n, out);
     return st;
 }

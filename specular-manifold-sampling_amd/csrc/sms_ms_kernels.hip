@@ -316,18 +316,18 @@ SMS_DEV void ms_biased_body(const DScene &S, const sms_ss_query *__restrict__ Q,
         ms_walk_body<B>(S, in, n, out);                                                                             \
     }                                                                                                               \
     __global__ void __launch_bounds__(256) k_ms_estimate_##B(DScene S, const sms_ss_query *Q, int64_t n,           \
-                                                             uint64_t seed, float *part, float *invp,               \
+                                                             const uint64_t *__restrict__ seedp, float *part, float *invp,               \
                                                              EstimatorWork W) {                                     \
-        ms_estimate_body<B>(S, Q, n, seed, part, invp, W);                                                          \
+        ms_estimate_body<B>(S, Q, n, *seedp, part, invp, W);                                                          \
     }                                                                                                               \
-    __global__ void __launch_bounds__(256) k_ms_coop_##B(DScene S, const sms_ss_query *Q, uint64_t seed,           \
+    __global__ void __launch_bounds__(256) k_ms_coop_##B(DScene S, const sms_ss_query *Q, const uint64_t *__restrict__ seedp,           \
                                                          float *invp, EstimatorWork W) {                           \
-        ms_coop_body<B>(S, Q, seed, invp, W);                                                                       \
+        ms_coop_body<B>(S, Q, *seedp, invp, W);                                                                       \
     }                                                                                                               \
     __global__ void __launch_bounds__(256) k_ms_biased_##B(DScene S, const sms_ss_query *Q, int64_t n,             \
-                                                           uint64_t seed, float *out, EstimatorWork W) {           \
+                                                           const uint64_t *__restrict__ seedp, float *out, EstimatorWork W) {           \
         __shared__ float sol_lds[4][MS_BIASED_MAX_SOL][3];                                                          \
-        ms_biased_body<B>(S, Q, n, seed, out, W, sol_lds[threadIdx.x >> 6]);                                        \
+        ms_biased_body<B>(S, Q, n, *seedp, out, W, sol_lds[threadIdx.x >> 6]);                                        \
     }
 
 SMS_MS_KERNELS(2)

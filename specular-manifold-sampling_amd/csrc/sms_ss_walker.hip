@@ -272,8 +272,9 @@ __device__ __forceinline__ void ss_walker_body(const DScene &S, const sms_ss_que
 }
 
 __global__ void __launch_bounds__(256) k_ss_walker(DScene S, const sms_ss_query *__restrict__ Q, int64_t n,
-                                                   uint64_t seed, float *__restrict__ part, float *__restrict__ invp,
+                                                   const uint64_t *__restrict__ seedp, float *__restrict__ part, float *__restrict__ invp,
                                                    EstimatorWork W) {
+    const uint64_t seed = *seedp;
     ss_walker_body(S, Q, n, seed, part, invp, W);
 }
 

@@ -151,7 +151,9 @@ EXPORTS = ["sms_config_default", "sms_ctx_create", "sms_ctx_destroy", "sms_ctx_s
            "sms_copy_h2d", "sms_copy_d2h", "sms_ms_walk_batch", "sms_ms_estimate_batch",
            "sms_ms_walk_batch_host", "sms_ms_estimate_batch_host", "sms_glint_autotune",
            "sms_mlp_create", "sms_mlp_destroy", "sms_mlp_get_params", "sms_mlp_set_params",
-           "sms_mlp_infer", "sms_mlp_grad", "sms_mlp_adam", "sms_mlp_train", "sms_ctx_set_proposal",
+           "sms_mlp_infer", "sms_mlp_grad", "sms_mlp_adam", "sms_mlp_train", "sms_glint_graph_create",
+           "sms_ss_graph_create", "sms_ms_graph_create", "sms_ss_pass_graph_create", "sms_graph_launch",
+           "sms_graph_stats", "sms_graph_destroy", "sms_ctx_set_proposal",
            "sms_ctx_set_training_capture", "sms_ctx_training_samples", "sms_ctx_clear_training",
            "sms_ss_proposal_inputs", "sms_ss_proposal_outputs", "sms_ss_make_queries",
            "sms_film_accumulate", "sms_ss_autotune"]
@@ -204,6 +206,12 @@ def load_library(path=LIB_PATH):
     lib.sms_mlp_grad.argtypes = [vp, vp, vp, vp, i64, vp, C.POINTER(C.c_float)]
     lib.sms_mlp_adam.argtypes = [vp, C.c_float]
     lib.sms_mlp_train.argtypes = [vp, vp, vp, vp, C.c_int32, vp, C.c_int32, C.c_float, vp]
+    for name in ("sms_glint_graph_create", "sms_ss_graph_create", "sms_ms_graph_create"):
+        getattr(lib, name).argtypes = [vp, vp, i64, vp, C.POINTER(vp)]
+    lib.sms_ss_pass_graph_create.argtypes = [vp, C.c_int32, vp, vp, vp, vp, vp, vp, C.POINTER(vp)]
+    lib.sms_graph_launch.argtypes = [vp, C.c_uint64]
+    lib.sms_graph_stats.argtypes = [vp, C.POINTER(SmsStats), C.c_int32]
+    lib.sms_graph_destroy.argtypes = [vp]
     lib.sms_ctx_set_proposal.argtypes = [vp, vp, C.c_int32, C.c_float]
     lib.sms_ctx_set_training_capture.argtypes = [vp, i64, C.c_int32]
     lib.sms_ctx_training_samples.argtypes = [vp, C.POINTER(i64), vp, vp, vp]

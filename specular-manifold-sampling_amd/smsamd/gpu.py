@@ -221,6 +221,12 @@ class Context:
                "ss_estimate")
         return st.as_dict()
 
+    def ms_estimate_device(self, qbuf, n, seed, outbuf):
+        st = abi.SmsStats()
+        _check(lib().sms_ms_estimate_batch(self.h, qbuf.ptr, n, seed, outbuf.ptr, C.byref(st)), self.h,
+               "ms_estimate")
+        return st.as_dict()
+
 
     # ---------------------------------------------------------------- learned seed proposal
     def set_proposal(self, mlp, shape, defensive=0.2):
@@ -281,11 +287,51 @@ class Context:
             ob.free()
         return out
 
+    def _graph(self, fn, *args):
+        h = C.c_void_p()
+        _check(fn(self.h, *args, C.byref(h)), self.h, fn.__name__)
+        return Graph(self, h.value)
+
+    def glint_graph(self, qbuf, n, outbuf):
+        return self._graph(lib().sms_glint_graph_create, qbuf.ptr, n, outbuf.ptr)
+
+    def ss_graph(self, qbuf, n, outbuf):
+        return self._graph(lib().sms_ss_graph_create, qbuf.ptr, n, outbuf.ptr)
+
+    def ms_graph(self, qbuf, n, outbuf):
+        return self._graph(lib().sms_ms_graph_create, qbuf.ptr, n, outbuf.ptr)
+
+    def ss_pass_graph(self, spp, qbuf, wbuf, rgbbuf, imgbuf, imgsqbuf=None, imgnbuf=None):
+        return self._graph(lib().sms_ss_pass_graph_create, spp, qbuf.ptr, wbuf.ptr, rgbbuf.ptr, imgbuf.ptr,
+                           imgsqbuf.ptr if imgsqbuf is not None else None,
+                           imgnbuf.ptr if imgnbuf is not None else None)
+
     def proposal_inputs(self, q, seed):
         return self._per_query(lib().sms_ss_proposal_inputs, q, seed, abi.MLP_INPUTS, "sms_ss_proposal_inputs")
 
     def proposal_outputs(self, q, seed):
         return self._per_query(lib().sms_ss_proposal_outputs, q, seed, abi.MLP_OUTPUTS, "sms_ss_proposal_outputs")
+
+
+class Graph:
+    """A captured frame launch sequence (sms_graph_*): launch(seed) replays it asynchronously."""
+
+    def __init__(self, ctx, h):
+        self.ctx = ctx
+        self.h = h
+
+    def launch(self, seed):
+        _check(lib().sms_graph_launch(self.h, seed), self.ctx.h, "sms_graph_launch")
+
+    def stats(self, reset=False):
+        st = abi.SmsStats()
+        _check(lib().sms_graph_stats(self.h, C.byref(st), 1 if reset else 0), self.ctx.h, "sms_graph_stats")
+        return st.as_dict()
+
+    def close(self):
+        if self.h:
+            lib().sms_graph_destroy(self.h)
+            self.h = None
 
 
 class Mlp:
