@@ -66,6 +66,10 @@ def parse():
     ap.add_argument("--pool-only", action="store_true", help="run only the pool single-scatter throughput")
     ap.add_argument("--ss-first-pass", type=int, default=None, help="pool: 0 walker, 1 lane per query (default: tuned)")
     ap.add_argument("--rmse-only", action="store_true", help="run only the equal-time RMSE experiment")
+    ap.add_argument("--ms-only", action="store_true", help="run only the multi-scatter slab experiment")
+    ap.add_argument("--no-ms", action="store_true", help="skip the multi-scatter slab experiment")
+    ap.add_argument("--ms-spp", type=int, default=32, help="uniform-seed samples per pixel of the slab budget")
+    ap.add_argument("--ms-ref-spp", type=int, default=256, help="slab reference samples per pixel")
     ap.add_argument("--mlp-only", action="store_true", help="run only the proposal network's training / inference timing")
     ap.add_argument("--no-mlp", action="store_true", help="skip the proposal network's timing")
     ap.add_argument("--mlp-steps", type=int, default=2000, help="timed training steps of the network leg")
@@ -106,6 +110,9 @@ def main():
         return
     if args.pool_only:
         print(json.dumps(pool_throughput(args)), flush=True)
+        return
+    if args.ms_only:
+        print(json.dumps(equal_time_rmse(args, "slab")), flush=True)
         return
     if args.mlp_only:
         print(json.dumps(mlp_throughput(args)), flush=True)
@@ -197,6 +204,7 @@ def main():
     pool = None if args.no_pool else pool_throughput(args)
     mlp = None if args.no_mlp else mlp_throughput(args)
     rmse = None if args.no_rmse else equal_time_rmse(args)
+    ms_rmse = None if args.no_ms else equal_time_rmse(args, "slab")
     result = {
         "metric": "converged walks/s at 1 GPU; equal-time RMSE learned vs uniform seeds",
         "value": conv_per_s,
@@ -222,6 +230,7 @@ def main():
         "pool_ss": pool,
         "mlp": mlp,
         "equal_time_rmse": rmse,
+        "ms_slab_equal_time_rmse": ms_rmse,
     }
     if cpu and cpu.get("value"):
         result["gpu_over_cpu"] = conv_per_s / cpu["value"]
@@ -340,10 +349,14 @@ def mlp_throughput(args):
     return out
 
 
-def equal_time_rmse(args):
-    """Water pool (512^2-cell Gerstner mesh, IOR 1.33, floor 1 below, sphere light r 0.05),
-    caustic term of the camera image at rmse_res^2.  Uniform seeding renders rmse_spp samples
-    per pixel; its wall-clock time T is the budget.  The charged run gets the same T for
+def equal_time_rmse(args, case="pool"):
+    """case "pool": water pool (512^2-cell Gerstner mesh, IOR 1.33, floor 1 below, sphere light
+    r 0.05), single-scatter caustic term of the camera image at rmse_res^2, unbiased (cap 1e7,
+    thresholds 1e-5 as pool_sms.xml); budget rmse_spp, reference rmse_ref_spp.  case "slab":
+    glass slab 0.2 thick (IOR 1.5) with value-noise faces over a diffuse floor, two-vertex
+    multi-scatter chains (bounces 2, unbiased, cap 1e7, thresholds 1e-5 as slab_sms.xml), the
+    proposal on the chain's seed point on the top face; budget ms_spp, reference ms_ref_spp.
+    Uniform seeding renders the budget's samples per pixel; its wall-clock time T is the budget.  The charged run gets the same T for
     everything, training included: its first pass uses uniform seeds, every later pass the
     defensive mixture with the network as trained so far; after each pass the network trains on
     the estimate-weighted solutions that pass captured (batch 8192, Adam lr 1e-3, train_epochs
@@ -354,16 +367,26 @@ def equal_time_rmse(args):
     import numpy as np
     from smsamd import abi, gpu, render, scenes
     res, pass_spp = args.rmse_res, args.rmse_pass_spp
-    sc = scenes.pool(res, res)
-    cfg = abi.SmsConfig.default(max_trials=10_000_000, uniqueness_threshold=1e-5, solver_threshold=1e-5)
+    if case == "pool":
+        sc = scenes.pool(res, res)
+        cfg = abi.SmsConfig.default(max_trials=10_000_000, uniqueness_threshold=1e-5, solver_threshold=1e-5)
+        budget_spp, ref_spp, estimator = args.rmse_spp, args.rmse_ref_spp, "ss"
+    else:
+        sc = scenes.slab(res, res, n=256)
+        cfg = abi.SmsConfig.default(bounces=2, max_trials=10_000_000, uniqueness_threshold=1e-5,
+                                    solver_threshold=1e-5)
+        budget_spp, ref_spp, estimator = args.ms_spp, args.ms_ref_spp, "ms"
     ctx = gpu.Context(sc, cfg)
-    runner = render.PassRunner(ctx, pass_spp)
+    runner = render.PassRunner(ctx, pass_spp, estimator)
     film = render.Film(ctx, runner.pixels)
     runner.run(999_999, film)                                   # first launches
     ctx.sync()
-    ctx.ss_queries_device(pass_spp, 999_998, runner.q, runner.w)
-    sched = ctx.ss_autotune_device(runner.q, runner.n, 999_998, runner.rgb)
-    print(f"# rmse: schedule {sched}", file=sys.stderr, flush=True)
+    if estimator == "ss":
+        ctx.ss_queries_device(pass_spp, 999_998, runner.q, runner.w)
+        sched = ctx.ss_autotune_device(runner.q, runner.n, 999_998, runner.rgb)
+    else:
+        sched = ctx.get_schedule()
+    print(f"# rmse {case}: schedule {sched}", file=sys.stderr, flush=True)
 
     def frame(seed0, n_passes, st=None):
         film.clear()
@@ -372,12 +395,12 @@ def equal_time_rmse(args):
         ctx.sync()
         return film.download()
 
-    (ref, ref_var), t_ref = render.timed(lambda: frame(10_000_000, args.rmse_ref_spp // pass_spp))
+    (ref, ref_var), t_ref = render.timed(lambda: frame(10_000_000, ref_spp // pass_spp))
     ref_var_mean = ref_var / max(film.spp, 1)
     dropped_ref = film.dropped()
-    print(f"# rmse: reference {film.spp} spp in {t_ref:.1f}s", file=sys.stderr, flush=True)
+    print(f"# rmse {case}: reference {film.spp} spp in {t_ref:.1f}s", file=sys.stderr, flush=True)
     st_u = {}
-    (uni, _), T = render.timed(lambda: frame(20_000_000, args.rmse_spp // pass_spp, st_u))
+    (uni, _), T = render.timed(lambda: frame(20_000_000, budget_spp // pass_spp, st_u))
     spp_u, dropped_u = film.spp, film.dropped()
 
     # learned, training charged to the frame
@@ -430,9 +453,10 @@ def equal_time_rmse(args):
 
     def trials(st):
         return st.get("bernoulli_trials", 0) / max(st.get("bernoulli_calls", 1), 1)
+    workload = (f"water pool caustics (single scatter), {res}x{res} camera image" if case == "pool" else
+                f"glass slab double refraction (multi scatter, 2 vertices), {res}x{res} camera image")
     out = {
-        "workload": f"water pool caustics, {res}x{res} camera image, unbiased (max_trials 1e7, uniqueness 1e-5), "
-                    f"passes of {pass_spp} spp",
+        "workload": f"{workload}, unbiased (max_trials 1e7, uniqueness 1e-5), passes of {pass_spp} spp",
         "budget_s": round(T, 3), "uniform_spp": spp_u,
         "learned_spp": spp_l, "learned_s": round(t_l, 3), "learned_passes_uniform": 1,
         "train_s_within_budget": round(tr["train_s"], 3), "train_steps_within_budget": tr["steps"],
@@ -451,9 +475,12 @@ def equal_time_rmse(args):
         "trials_per_estimate": {"uniform": round(trials(st_u), 3), "learned": round(trials(st_a), 3)},
         "invalid_samples_dropped": {"reference": dropped_ref, "uniform": dropped_u, "learned": dropped_l,
                                     "amortized": dropped_a},
+        "uniform_frame": {"walks_per_s": st_u.get("walks", 0) / T, "converged_walks_per_s": st_u.get("walks_converged", 0) / T,
+                          "newton_iterations_per_s": st_u.get("newton_iterations", 0) / T,
+                          "note": "whole frames: camera passes, estimates and film"},
         "defensive": args.defensive, "schedule": sched,
     }
-    print(f"# rmse: {json.dumps(out)}", file=sys.stderr, flush=True)
+    print(f"# rmse {case}: {json.dumps(out)}", file=sys.stderr, flush=True)
     mlp.close()
     runner.free()
     film.free()

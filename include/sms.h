@@ -426,12 +426,15 @@ sms_status sms_mlp_adam(sms_mlp *mlp, float lr);
    component; this is the online training loop of the learned proposal. */
 sms_status sms_mlp_train(sms_mlp *mlp, const float *x, const float *uv, const float *w, int32_t batch,
                          const int64_t *offsets, int32_t n_steps, float lr, float *losses);
-/* Use `mlp` as the seed proposal of single-scatter caster mesh `shape` (with uvs) in
-   sms_ss_estimate_batch; defensive in (0, 1].  mlp NULL: uniform seeds, `shape` still names the
-   caster whose solutions the training capture records; shape -1 clears the proposal. */
+/* Use `mlp` as the seed proposal of caster mesh `shape` (with uvs): in sms_ss_estimate_batch if
+   it is a single-scatter caster, in the unbiased sms_ms_estimate_batch if it is a multi-scatter
+   caster (the seed point of the chain on it; the biased multi-scatter estimator keeps uniform
+   seeds); defensive in (0, 1].  mlp NULL: uniform seeds, `shape` still names the caster whose
+   solutions the training capture records; shape -1 clears the proposal. */
 sms_status sms_ctx_set_proposal(sms_ctx *ctx, sms_mlp *mlp, int32_t shape, float defensive);
-/* Training capture of later sms_ss_estimate_batch calls (unbiased estimator) on the proposal
-   shape, as records (network inputs, solution uv, weight):
+/* Training capture of later unbiased sms_ss_estimate_batch / sms_ms_estimate_batch calls on the
+   proposal shape, as records (network inputs, solution uv -- for a multi-scatter chain the uv
+   of its vertex on that shape --, weight):
      SMS_CAPTURE_SOLUTIONS  every converged walk -- first walks and Bernoulli trials, whose seeds
                             follow the same distribution -- with weight 1;
      SMS_CAPTURE_ESTIMATES  every (query, caster) whose estimate has a positive RGB mean: the
@@ -455,6 +458,10 @@ sms_status sms_ss_proposal_inputs(sms_ctx *ctx, const sms_ss_query *q, int64_t n
 /* Raw network outputs (device, n x 48) the proposal gives the queries of an estimate with this
    seed -- the exact values sms_ss_estimate_batch turns into its seed mixtures. */
 sms_status sms_ss_proposal_outputs(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *out);
+/* The same for the multi-scatter estimator (its light samples come from the multi-scatter
+   emitters). */
+sms_status sms_ms_proposal_inputs(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *x);
+sms_status sms_ms_proposal_outputs(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *out);
 
 /* ---------------------------------------------------------------- frame graphs
    No counterpart in the reference (which renders through per-thread scalar calls).  A graph

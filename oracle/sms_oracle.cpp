@@ -2493,7 +2493,11 @@ static V3 ms_path_contribution(const Scene &S, const SI &si, const EI &ei_, MSPa
 }
 
 /* specular_manifold_sampling MS (manifold_ms.cpp:27-201), per-(query, walk) PCG32 streams */
-static V3 estimate_ms(const Scene &S, const sms_ss_query &q, uint64_t seed, uint64_t qid, Stats &st) {
+/* specular_manifold_sampling MS (manifold_ms.cpp:28-201).  With a proposal on a caster, the
+   unbiased estimator's walks on it seed the chain through the defensive mixture's point
+   (uniform branch: the reference's area sample); the biased estimator keeps uniform seeds. */
+static V3 estimate_ms(const Scene &S, const sms_ss_query &q, uint64_t seed, uint64_t qid, Stats &st,
+                      const Prop *P = nullptr) {
     if (q.shape < 0) return v3(0.f, 0.f, 0.f);     /* no receiver: zero estimate, no walks */
     const sms_config &cfg = S.cfg;
     SI si = si_from_query(q);
@@ -2508,10 +2512,29 @@ static V3 estimate_ms(const Scene &S, const sms_ss_query &q, uint64_t seed, uint
         int shape = casters[c];
         uint64_t wbase = (uint64_t) c << 32;
         V3 value = v3(0.f, 0.f, 0.f);
+        float gq[PROP_RAW];
+        const bool use_prop = P && shape == P->shape && !cfg.biased;
+        if (use_prop) gmm_prepare(P->raw + ((int64_t) qid - P->raw_base) * PROP_RAW, gq);
         auto run = [&](uint64_t k, bool first, const int *shapes) {
             Pcg32 rw; stream_seed(&rw, seed, qid, wbase + k + 1);
-            V2 s; s.x = pcg32_float(&rw); s.y = pcg32_float(&rw);
-            MSWalk w = ms_sample_path(S, shape, si.p, ei, s, first, shapes, v3(0.f, 0.f, 0.f));
+            V2 s = v2(0.f, 0.f);
+            V3 p_start = v3(0.f, 0.f, 0.f);
+            bool seeded = true;
+            if (use_prop) {
+                if (proposal_seed(gq, P->defensive, &rw, &s) == 1) {
+                    seeded = s.x >= 0.f && s.x < 1.f && s.y >= 0.f && s.y < 1.f && uv_to_point(S, P->grid, s, &p_start);
+                    s = v2(0.f, 0.f);
+                }
+            } else {
+                s.x = pcg32_float(&rw); s.y = pcg32_float(&rw);
+            }
+            MSWalk w;
+            if (seeded) {
+                w = ms_sample_path(S, shape, si.p, ei, s, first, shapes, p_start);
+            } else {                                   /* mixture uv outside the caster: failed, 0 iterations */
+                w.ok = false; w.seed_ok = false; w.iterations = 0; w.path.n = 0;
+                for (int i = 0; i < MS_MAXB; ++i) w.seed_shapes[i] = shapes ? shapes[i] : -1;
+            }
             st.walks++; st.iters += w.iterations; if (w.ok) st.conv++;
             return w;
         };
@@ -2670,6 +2693,42 @@ void oracle_prop_inputs(void *h, const sms_ss_query *q, int64_t n, uint64_t seed
         EI ei = sample_emitter_interaction(S, si.p, S.emitters_ss, &rq);
         proposal_inputs(S, si.p, si.sh_n, ei.p, x + 9 * i);
     }
+}
+
+/* the same for the multi-scatter estimator's light samples */
+void oracle_prop_inputs_ms(void *h, const sms_ss_query *q, int64_t n, uint64_t seed, float *x) {
+    const Scene &S = *(Scene *) h;
+    std::vector<int> em_ms;
+    for (int i = 0; i < (int) S.emitters.size(); ++i) if (S.emitters[i].ms) em_ms.push_back(i);
+    for (int64_t i = 0; i < n; ++i) {
+        SI si = si_from_query(q[i]);
+        Pcg32 rq; stream_seed(&rq, seed, (uint64_t) i, 0);
+        EI ei = sample_emitter_interaction(S, si.p, em_ms, &rq);
+        proposal_inputs(S, si.p, si.sh_n, ei.p, x + 9 * i);
+    }
+}
+
+/* multi-scatter estimates with the proposal of raw network outputs raw (n x 48) on `shape` */
+int oracle_ms_estimate_batch_prop(void *h, const sms_ss_query *q, int64_t n, uint64_t seed, const float *raw, int shape,
+                                  float defensive, float *out_rgb, sms_stats *stats, int threads) {
+    const Scene &S = *(Scene *) h;
+    if (S.cfg.bounces < 1 || S.cfg.bounces > MS_MAXB) return -5;
+    if (shape < 0 || shape >= (int) S.shapes.size() || S.shapes[shape].type != SMS_SHAPE_MESH) return -1;
+    Prop P;
+    P.raw = raw; P.shape = shape; P.defensive = defensive;
+    build_uv_grid(S, shape, P.grid);
+    std::vector<Stats> per((size_t) n);
+    parallel_for(n, threads, [&](int64_t i) {
+        Stats st; memset(&st, 0, sizeof(st));
+        V3 r = estimate_ms(S, q[i], seed, (uint64_t) i, st, &P);
+        out_rgb[3 * i] = r.x; out_rgb[3 * i + 1] = r.y; out_rgb[3 * i + 2] = r.z;
+        per[(size_t) i] = st;
+    });
+    Stats tot; memset(&tot, 0, sizeof(tot));
+    for (auto &s : per) { tot.walks += s.walks; tot.conv += s.conv; tot.iters += s.iters; tot.bcalls += s.bcalls;
+        tot.btrials += s.btrials; if (s.bmax > tot.bmax) tot.bmax = s.bmax; }
+    stats_out(tot, stats);
+    return 0;
 }
 
 void oracle_scene_box(void *h, float *c, float *inv) {

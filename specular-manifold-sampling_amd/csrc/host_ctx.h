@@ -100,7 +100,7 @@ inline sms_status fail(sms_ctx *ctx, sms_status code, const std::string &msg) {
 // learned seed proposal (sms_mlp_host.hip)
 Proposal proposal_of(sms_ctx *ctx);
 sms_status proposal_alloc(sms_ctx *ctx, int64_t n);
-sms_status proposal_enqueue(sms_ctx *ctx, const sms_ss_query *q, int64_t n);
+sms_status proposal_enqueue(sms_ctx *ctx, const sms_ss_query *q, int64_t n, int ms);
 void proposal_release(sms_ctx *ctx);
 
 // launch sequences shared by the estimator calls and the frame graphs (sms_host.hip).  The

@@ -155,7 +155,7 @@ __global__ void k_ss_finalize(const float *part, const float *invp, int nc, int6
 __global__ void k_ss_queries(DScene S, int spp, const uint64_t *seedp, sms_ss_query *q, float *w, int64_t n);
 __global__ void k_film_accumulate(const float *rgb, const float *w, int64_t pixels, int spp, float *image, float *image_sq,
                                   float *image_n);
-__global__ void k_prop_inputs(DScene S, const sms_ss_query *Q, int64_t n, const uint64_t *seedp, float *x);
+__global__ void k_prop_inputs(DScene S, const sms_ss_query *Q, int64_t n, const uint64_t *seedp, int ms, float *x);
 __global__ void k_gmm_prepare(const float *raw, int64_t n, float *g);
 __global__ void k_capture_gather(const float *cap, const unsigned long long *counts, unsigned long long cap_sub,
                                  float *x, float *uv, float *w);
@@ -169,12 +169,13 @@ __global__ void k_mlp_reduce_adam(const float *partial, const float *aux, int nb
 __global__ void k_mlp_adam(float *p, float *g, float *m, float *v, uint16_t *wb, int n, float lr, float b1, float b2,
                            float eps, float bc1, float bc2);
 __global__ void k_ss_biased(DScene S, const sms_ss_query *Q, int64_t n, const uint64_t *seedp, float *out, EstimatorWork W);
-#define SMS_MS_KERNEL_DECLS(B)                                                                                   \
-    __global__ void k_ms_walk_##B(DScene S, const sms_ms_walk_in *in, int64_t n, sms_ms_walk_out *out);         \
-    __global__ void k_ms_estimate_##B(DScene S, const sms_ss_query *Q, int64_t n, const uint64_t *seedp, float *part,   \
-                                      float *invp, EstimatorWork W);                                             \
-    __global__ void k_ms_coop_##B(DScene S, const sms_ss_query *Q, const uint64_t *seedp, float *invp, EstimatorWork W); \
-    __global__ void k_ms_biased_##B(DScene S, const sms_ss_query *Q, int64_t n, const uint64_t *seedp, float *out,      \
+#define SMS_MS_KERNEL_DECLS(B)                                                                                        \
+    __global__ void k_ms_walk_##B(DScene S, const sms_ms_walk_in *in, int64_t n, sms_ms_walk_out *out);               \
+    __global__ void k_ms_estimate_##B(DScene S, const sms_ss_query *Q, int64_t n, const uint64_t *seedp, float *part, \
+                                      float *invp, EstimatorWork W);                                                  \
+    __global__ void k_ms_coop_##B(DScene S, const sms_ss_query *Q, const uint64_t *seedp, const float *part,          \
+                                  float *invp, EstimatorWork W);                                                      \
+    __global__ void k_ms_biased_##B(DScene S, const sms_ss_query *Q, int64_t n, const uint64_t *seedp, float *out,    \
                                     EstimatorWork W);
 SMS_MS_KERNEL_DECLS(2)
 SMS_MS_KERNEL_DECLS(4)

@@ -154,14 +154,15 @@ SMS_DEV void glint_first_body(const DScene &S, const sms_glint_query *__restrict
     }
 }
 
-#define SMS_GLINT_FIRST_KERNEL(NAME, BVH)                                                                        \
-    __global__ void __launch_bounds__(256) NAME(DScene S, const sms_glint_query *__restrict__ Q, int64_t n,      \
-                                                const uint64_t *__restrict__ seedp, float *__restrict__ out, EstimatorWork W) {       \
-        __shared__ GlintRec lrec[256];                                                                           \
-        __shared__ int wcount[4];                                                                                \
-        __shared__ unsigned long long lbase;                                                                     \
-        __shared__ unsigned long long lstats[4][4];                                                              \
-        glint_first_body<BVH>(S, Q, n, *seedp, out, W, lrec, wcount, &lbase, lstats);                              \
+#define SMS_GLINT_FIRST_KERNEL(NAME, BVH)                                                                    \
+    __global__ void __launch_bounds__(256) NAME(DScene S, const sms_glint_query *__restrict__ Q, int64_t n,  \
+                                                const uint64_t *__restrict__ seedp, float *__restrict__ out, \
+                                                EstimatorWork W) {                                           \
+        __shared__ GlintRec lrec[256];                                                                       \
+        __shared__ int wcount[4];                                                                            \
+        __shared__ unsigned long long lbase;                                                                 \
+        __shared__ unsigned long long lstats[4][4];                                                          \
+        glint_first_body<BVH>(S, Q, n, *seedp, out, W, lrec, wcount, &lbase, lstats);                        \
     }
 SMS_GLINT_FIRST_KERNEL(k_glint_first_bvh, true)
 SMS_GLINT_FIRST_KERNEL(k_glint_first_nobvh, false)
@@ -261,9 +262,9 @@ SMS_DEV void glint_trials_body(const DScene &S, uint64_t seed, EstimatorWork W) 
     stat_add(W.stats + 4, st_btrials); stat_max(W.stats + 5, st_bmax); stat_add(W.stats + 6, st_iters);
 }
 
-#define SMS_GLINT_TRIALS_KERNEL(G)                                                                               \
-    __global__ void __launch_bounds__(256) k_glint_trials_g##G(DScene S, const uint64_t *__restrict__ seedp, EstimatorWork W) {       \
-        glint_trials_body<G>(S, *seedp, W);                                                                        \
+#define SMS_GLINT_TRIALS_KERNEL(G)                                                                                              \
+    __global__ void __launch_bounds__(256) k_glint_trials_g##G(DScene S, const uint64_t *__restrict__ seedp, EstimatorWork W) { \
+        glint_trials_body<G>(S, *seedp, W);                                                                                     \
     }
 SMS_GLINT_TRIALS_KERNEL(1)
 SMS_GLINT_TRIALS_KERNEL(2)

@@ -922,7 +922,7 @@ sms_status ss_enqueue(sms_ctx *ctx, const sms_ss_query *q, int64_t n, float *out
     const uint64_t *seed = ctx->seeds + SEED_EST;
     EV_RECORD(ctx, ctx->ev0);
     if (n > 0) {
-        sms_status st = proposal_enqueue(ctx, q, n);
+        sms_status st = proposal_enqueue(ctx, q, n, 0);
         if (st) return st;
         EstimatorWork W = work_of(ctx);
         if (cfg.biased) {
@@ -965,6 +965,7 @@ sms_status ms_prepare(sms_ctx *ctx, const sms_ss_query *q, int64_t n, const floa
     if (cfg.biased && cfg.max_trials > 128) return fail(ctx, SMS_E_UNSUPPORTED, "biased trial sets above 128 are not built");
     st = cfg.biased ? SMS_OK : ensure_over(ctx, query_sub_cap(n, nc));
     if (!st) st = ensure_work(ctx, 0, cfg.biased ? 0 : (size_t) std::max<int64_t>(n * nc, 1));
+    if (!st && !cfg.biased) st = proposal_alloc(ctx, n);
     return st;
 }
 
@@ -975,6 +976,10 @@ sms_status ms_enqueue(sms_ctx *ctx, const sms_ss_query *q, int64_t n, float *out
     const uint64_t *seed = ctx->seeds + SEED_EST;
     EV_RECORD(ctx, ctx->ev0);
     if (n > 0) {
+        if (!cfg.biased) {
+            sms_status st = proposal_enqueue(ctx, q, n, 1);
+            if (st) return st;
+        }
         EstimatorWork W = work_of(ctx);
         if (cfg.biased) {
             dim3 grid(grid_for(ctx, ctx->occ_msbiased[b]));
@@ -988,8 +993,9 @@ sms_status ms_enqueue(sms_ctx *ctx, const sms_ss_query *q, int64_t n, float *out
             else hipLaunchKernelGGL(k_ms_estimate_4, g1, dim3(256), 0, ctx->stream, ctx->scene, q, n, seed, ctx->part, ctx->invp, W);
             HIP_TRY(ctx, hipGetLastError());
             EV_RECORD(ctx, ctx->ev1);
-            if (b == 0) hipLaunchKernelGGL(k_ms_coop_2, g2, dim3(256), 0, ctx->stream, ctx->scene, q, seed, ctx->invp, W);
-            else hipLaunchKernelGGL(k_ms_coop_4, g2, dim3(256), 0, ctx->stream, ctx->scene, q, seed, ctx->invp, W);
+            const float *part = ctx->part;
+            if (b == 0) hipLaunchKernelGGL(k_ms_coop_2, g2, dim3(256), 0, ctx->stream, ctx->scene, q, seed, part, ctx->invp, W);
+            else hipLaunchKernelGGL(k_ms_coop_4, g2, dim3(256), 0, ctx->stream, ctx->scene, q, seed, part, ctx->invp, W);
             HIP_TRY(ctx, hipGetLastError());
             hipLaunchKernelGGL(k_ss_finalize, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream,
                                ctx->part, ctx->invp, nc, n, out_rgb);

@@ -213,19 +213,23 @@ __global__ void __launch_bounds__(256) k_ss_coop(DScene S, const sms_ss_query *_
             int64_t k = base + lane;
             bool match = false, ran = false, conv = false;
             int its = 0;
+            V2 uv_t = v2(0.f, 0.f);
             if (k <= kcap) {
                 PathResult t = ss_walk(S, W.prop, q, qkey, c, shape, (uint64_t) k, si.p, ei);
                 ran = true; its = t.iterations; conv = t.ok;
-                if (t.ok && capturing(W.prop, shape, SMS_CAPTURE_SOLUTIONS))
-                    capture_sample(S, W.prop, si, ei, t.si_final.uv, 1.f);
+                uv_t = t.si_final.uv;
                 V3 dtr = normalize(t.si_final.p - si.p);
                 match = t.ok && fabsf(dot(direction, dtr) - 1.f) < cfg.uniqueness_threshold;
             }
             unsigned long long ballot = __ballot(match);
             if (ballot) found = base + (int64_t) __ffsll((long long) ballot) - 1;
+            // the trials the sequential loop runs: up to the first match
             if (ran && (found < 0 || k <= found)) {
                 st_walks++; st_iters += (unsigned long long) its; st_tail += (unsigned long long) its;
-                if (conv) st_conv++;
+                if (conv) {
+                    st_conv++;
+                    if (capturing(W.prop, shape, SMS_CAPTURE_SOLUTIONS)) capture_sample(S, W.prop, si, ei, uv_t, 1.f);
+                }
             }
         }
         if (lane == 0) {
@@ -444,9 +448,10 @@ __global__ void k_film_accumulate(const float *__restrict__ rgb, const float *__
 
 // ================================================================ learned seed proposal
 // network inputs of every query of an estimate: the shading point, its shading normal and the
-// light sample the estimator draws for it (same stream, same arithmetic as k_ss_estimate)
+// light sample the estimator draws for it (same stream, same arithmetic as k_ss_estimate; ms:
+// the multi-scatter estimator's emitters)
 __global__ void k_prop_inputs(DScene S, const sms_ss_query *__restrict__ Q, int64_t n, const uint64_t *__restrict__ seedp,
-                              float *__restrict__ x) {
+                              int ms, float *__restrict__ x) {
     const uint64_t seed = *seedp;
     const int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
@@ -454,7 +459,8 @@ __global__ void k_prop_inputs(DScene S, const sms_ss_query *__restrict__ Q, int6
     SI si = ss_query_si(g);
     Pcg32 rq;
     stream_seed(rq, query_key(seed, (uint64_t) q), 0);
-    EI ei = sample_emitter_interaction(S, si.p, S.emitters_ss, S.n_emitters_ss, rq);
+    EI ei = ms ? sample_emitter_interaction(S, si.p, S.emitters_ms, S.n_emitters_ms, rq)
+               : sample_emitter_interaction(S, si.p, S.emitters_ss, S.n_emitters_ss, rq);
     float r[9];
     proposal_inputs(S, si.p, si.sh_n, ei.p, r);
     for (int j = 0; j < 9; ++j) x[q * 9 + j] = r[j];

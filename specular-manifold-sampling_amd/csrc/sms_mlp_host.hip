@@ -170,11 +170,11 @@ sms_status build_uv_grid(sms_ctx *ctx, int shape) {
     return SMS_OK;
 }
 
-// network inputs of n queries with the seed of device slot `seed`
-sms_status launch_inputs(sms_ctx *ctx, const sms_ss_query *q, int64_t n, const uint64_t *seed, float *x) {
+// network inputs of n queries with the seed of device slot `seed` (ms: multi-scatter emitters)
+sms_status launch_inputs(sms_ctx *ctx, const sms_ss_query *q, int64_t n, const uint64_t *seed, int ms, float *x) {
     if (n == 0) return SMS_OK;
     hipLaunchKernelGGL(k_prop_inputs, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream, ctx->scene, q, n,
-                       seed, x);
+                       seed, ms, x);
     HIP_TRY(ctx, hipGetLastError());
     return SMS_OK;
 }
@@ -203,11 +203,14 @@ sms_status proposal_alloc(sms_ctx *ctx, int64_t n) {
 }
 
 // network inputs -> raw outputs -> mixtures of the n queries of an estimate (prop_g), with the
-// estimator's seed slot
-sms_status proposal_enqueue(sms_ctx *ctx, const sms_ss_query *q, int64_t n) {
+// estimator's seed slot (ms: the multi-scatter estimator's light samples)
+sms_status proposal_enqueue(sms_ctx *ctx, const sms_ss_query *q, int64_t n, int ms) {
     if (!ctx->prop_mlp || ctx->prop_shape < 0 || n == 0) return SMS_OK;
+    // only an estimator whose casters include the proposal's shape uses the mixtures
+    if (!(ctx->h_shapes[ctx->prop_shape].flags & (ms ? SMS_FLAG_CAUSTIC_CASTER_MS : SMS_FLAG_CAUSTIC_CASTER_SS)))
+        return SMS_OK;
     if ((size_t) n > ctx->prop_cap) return fail(ctx, SMS_E_INVALID, "proposal buffers not prepared");
-    sms_status st = launch_inputs(ctx, q, n, ctx->seeds + SEED_EST, ctx->prop_x);
+    sms_status st = launch_inputs(ctx, q, n, ctx->seeds + SEED_EST, ms, ctx->prop_x);
     if (!st) st = launch_infer(ctx->prop_mlp, ctx->prop_x, n, ctx->prop_raw);
     if (st) return st;
     hipLaunchKernelGGL(k_gmm_prepare, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream, ctx->prop_raw, n,
@@ -363,8 +366,8 @@ sms_status sms_ctx_set_proposal(sms_ctx *ctx, sms_mlp *mlp, int32_t shape, float
     }
     if (shape >= (int) ctx->h_shapes.size()) return fail(ctx, SMS_E_INVALID, "sms_ctx_set_proposal: shape out of range");
     const DShape &sh = ctx->h_shapes[shape];
-    if (sh.type != SMS_SHAPE_MESH || !sh.has_uv || !(sh.flags & SMS_FLAG_CAUSTIC_CASTER_SS))
-        return fail(ctx, SMS_E_INVALID, "sms_ctx_set_proposal: the shape must be a single-scatter caster mesh with uvs");
+    if (sh.type != SMS_SHAPE_MESH || !sh.has_uv || !(sh.flags & (SMS_FLAG_CAUSTIC_CASTER_SS | SMS_FLAG_CAUSTIC_CASTER_MS)))
+        return fail(ctx, SMS_E_INVALID, "sms_ctx_set_proposal: the shape must be a caster mesh with uvs");
     if (mlp && mlp->ctx != ctx) return fail(ctx, SMS_E_INVALID, "sms_ctx_set_proposal: network of another context");
     if (mlp && !(defensive > 0.f && defensive <= 1.f))
         return fail(ctx, SMS_E_INVALID, "sms_ctx_set_proposal: defensive weight must be in (0, 1]");
@@ -427,21 +430,37 @@ sms_status sms_ctx_clear_training(sms_ctx *ctx) {
     return SMS_OK;
 }
 
-sms_status sms_ss_proposal_inputs(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *x) {
-    if (!ctx || n < 0 || (n > 0 && (!q || !x))) return fail(ctx, SMS_E_INVALID, "sms_ss_proposal_inputs: bad arguments");
+namespace {
+
+sms_status proposal_io(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, int ms, float *x, float *out) {
+    if (!ctx || n < 0 || (n > 0 && (!q || !(x || out)))) return fail(ctx, SMS_E_INVALID, "proposal inputs: bad arguments");
+    if (out && !ctx->prop_mlp) return fail(ctx, SMS_E_INVALID, "proposal outputs: no proposal network set");
+    sms_status st = out ? ensure_prop(ctx, (size_t) std::max<int64_t>(n, 1)) : SMS_OK;
+    if (st) return st;
     hipLaunchKernelGGL(k_set_seed, dim3(1), dim3(64), 0, ctx->stream, ctx->seeds + SEED_API, seed);
     HIP_TRY(ctx, hipGetLastError());
-    return launch_inputs(ctx, q, n, ctx->seeds + SEED_API, x);
+    if (!out) return launch_inputs(ctx, q, n, ctx->seeds + SEED_API, ms, x);
+    st = launch_inputs(ctx, q, n, ctx->seeds + SEED_API, ms, ctx->prop_x);
+    if (!st) st = launch_infer(ctx->prop_mlp, ctx->prop_x, n, out);
+    return st;
+}
+
+} // namespace
+
+sms_status sms_ss_proposal_inputs(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *x) {
+    return proposal_io(ctx, q, n, seed, 0, x, nullptr);
 }
 
 sms_status sms_ss_proposal_outputs(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *out) {
-    if (!ctx || n < 0 || (n > 0 && (!q || !out))) return fail(ctx, SMS_E_INVALID, "sms_ss_proposal_outputs: bad arguments");
-    if (!ctx->prop_mlp) return fail(ctx, SMS_E_INVALID, "sms_ss_proposal_outputs: no proposal network set");
-    sms_status st = ensure_prop(ctx, (size_t) std::max<int64_t>(n, 1));
-    hipLaunchKernelGGL(k_set_seed, dim3(1), dim3(64), 0, ctx->stream, ctx->seeds + SEED_API, seed);
-    if (!st) st = launch_inputs(ctx, q, n, ctx->seeds + SEED_API, ctx->prop_x);
-    if (!st) st = launch_infer(ctx->prop_mlp, ctx->prop_x, n, out);
-    return st;
+    return proposal_io(ctx, q, n, seed, 0, nullptr, out);
+}
+
+sms_status sms_ms_proposal_inputs(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *x) {
+    return proposal_io(ctx, q, n, seed, 1, x, nullptr);
+}
+
+sms_status sms_ms_proposal_outputs(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *out) {
+    return proposal_io(ctx, q, n, seed, 1, nullptr, out);
 }
 
 } // extern "C"

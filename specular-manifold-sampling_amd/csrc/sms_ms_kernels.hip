@@ -14,9 +14,13 @@
 //
 // <B> is the largest chain the kernel holds (2 or 4 specular vertices); the host picks the
 // smallest that fits `bounces`.  Walk k of caster c at query q draws from PCG32 stream
-// (q, (c << 32) + k + 1); the emitter sample comes from stream (q, 0).
+// (q, (c << 32) + k + 1); the emitter sample comes from stream (q, 0).  With a learned
+// proposal on caster c, the seed point of every unbiased-estimator walk on it (first walk and
+// trials) comes from the defensive mixture, exactly as for single scatter; the biased
+// estimator keeps the reference's uniform seeds.
 #include "kernels.h"
 #include "dev_ms.h"
+#include "dev_ss_est.h"
 
 namespace smsdev {
 
@@ -43,13 +47,54 @@ SMS_DEV V3 ms_receiver_bsdf(const DScene &S, const SI &si, V3 wi_local, V3 wo_lo
     return (ld3(sh.refl) * INV_PI_F) * wo_local.z;
 }
 
+// Walk k of caster c at query q.  g: the query's prepared mixture when the proposal seeds this
+// caster, else null (uniform seed); a mixture seed outside the caster's UV domain is a failed
+// walk with 0 iterations.
 template <int MAXB>
-SMS_DEV void ms_walk_k(const DScene &S, uint64_t qkey, int c, int shape, uint64_t k, V3 si_p, const EI &ei, bool first,
-                       const int *shapes, MSWalk<MAXB> &w) {
+SMS_DEV void ms_walk_k(const DScene &S, const Proposal &P, const float *g, uint64_t qkey, int c, int shape, uint64_t k,
+                       V3 si_p, const EI &ei, bool first, const int *shapes, MSWalk<MAXB> &w) {
     Pcg32 rw;
     stream_seed(rw, qkey, ((uint64_t) c << 32) + k + 1);
-    V2 s; s.x = pcg32_float(rw); s.y = pcg32_float(rw);
-    ms_sample_path<MAXB>(S, shape, si_p, ei, s, first, shapes, v3(0.f, 0.f, 0.f), w);
+    V2 s = v2(0.f, 0.f);
+    V3 p_start = v3(0.f, 0.f, 0.f);
+    if (g) {
+        if (!proposal_seed(S, P, g, rw, s, p_start)) {
+            w.ok = false; w.seed_ok = false; w.iterations = 0; w.path.n = 0;
+            for (int i = 0; i < MAXB; ++i) w.seed_shapes[i] = shapes ? shapes[i] : -1;
+            return;
+        }
+    } else {
+        s.x = pcg32_float(rw); s.y = pcg32_float(rw);
+    }
+    ms_sample_path<MAXB>(S, shape, si_p, ei, s, first, shapes, p_start, w);
+}
+
+// mixture of query q when the proposal seeds `shape`, else null
+SMS_DEV const float *ms_mixture(const Proposal &P, int64_t q, int shape) {
+    return P.gmm && shape == P.shape ? P.gmm + q * PROP_RAW : nullptr;
+}
+
+// uv of the chain's vertex on `shape` (the training target of the proposal): the first hit of
+// the chain's segment into that vertex, traced as the reprojection traces it
+template <int MAXB>
+SMS_DEV bool ms_caster_uv(const DScene &S, V3 si_p, const MSPath<MAXB> &path, int shape, V2 &uv) {
+    for (int i = 0; i < path.n && i < MAXB; ++i) {
+        if (path.shapes[i] != shape) continue;
+        const V3 o = i == 0 ? si_p : path.v[i > 0 ? i - 1 : 0].v.p;
+        const V3 d = normalize(path.v[i].v.p - o);
+        const SI h = ray_intersect(S, i == 0 ? make_ray(o, d) : spawn_ray(o, d));
+        if (!h.valid || h.shape != shape) return false;
+        uv = h.uv;
+        return true;
+    }
+    return false;
+}
+
+template <int MAXB>
+SMS_DEV void ms_capture(const DScene &S, const Proposal &P, const SI &si, const EI &ei, const MSPath<MAXB> &path,
+                        int shape, float w) {
+    V2 uv;
+    if (ms_caster_uv<MAXB>(S, si.p, path, shape, uv)) capture_sample(S, P, si, ei, uv, w);
 }
 
 // ================================================================ parity walks
@@ -98,11 +143,13 @@ SMS_DEV void ms_estimate_body(const DScene &S, const sms_ss_query *__restrict__ 
             invp[slot] = 0.f;
             if (g.shape < 0) continue;                 // no receiver: zero estimate, no walks
             const int shape = S.casters_ms[c];
+            const float *gq = ms_mixture(W.prop, q, shape);
             MSWalk<MAXB> w0;
-            ms_walk_k<MAXB>(S, qkey, c, shape, 0, si.p, ei, true, nullptr, w0);
+            ms_walk_k<MAXB>(S, W.prop, gq, qkey, c, shape, 0, si.p, ei, true, nullptr, w0);
             st_walks++; st_iters += (unsigned long long) w0.iterations;
             if (!w0.ok) continue;
             st_conv++;
+            if (capturing(W.prop, shape, SMS_CAPTURE_SOLUTIONS)) ms_capture<MAXB>(S, W.prop, si, ei, w0.path, shape, 1.f);
             V3 direction = normalize(w0.path.v[0].v.p - si.p);
             V3 spec = ms_path_contribution<MAXB>(S, si, ei, w0.path);
             V3 dl = v3(dot(direction, si.sh_s), dot(direction, si.sh_t), dot(direction, si.sh_n));
@@ -113,10 +160,12 @@ SMS_DEV void ms_estimate_body(const DScene &S, const sms_ss_query *__restrict__ 
             float inv_prob = -1.f;
             while (true) {
                 MSWalk<MAXB> t;
-                ms_walk_k<MAXB>(S, qkey, c, shape, (uint64_t) k, si.p, ei, false, w0.seed_shapes, t);
+                ms_walk_k<MAXB>(S, W.prop, gq, qkey, c, shape, (uint64_t) k, si.p, ei, false, w0.seed_shapes, t);
                 st_walks++; st_iters += (unsigned long long) t.iterations;
                 if (t.ok) {
                     st_conv++;
+                    if (capturing(W.prop, shape, SMS_CAPTURE_SOLUTIONS))
+                        ms_capture<MAXB>(S, W.prop, si, ei, t.path, shape, 1.f);
                     V3 dtr = normalize(t.path.v[0].v.p - si.p);
                     if (fabsf(dot(direction, dtr) - 1.f) < cfg.uniqueness_threshold) { inv_prob = (float) k; break; }
                 }
@@ -128,6 +177,9 @@ SMS_DEV void ms_estimate_body(const DScene &S, const sms_ss_query *__restrict__ 
                 unsigned long long tr = (unsigned long long) k;
                 st_btrials += tr; st_bmax = st_bmax > tr ? st_bmax : tr;
                 invp[slot] = inv_prob;
+                const float wgt = mean3(&part[3 * slot]) * inv_prob;
+                if (wgt > 0.f && capturing(W.prop, shape, SMS_CAPTURE_ESTIMATES))
+                    ms_capture<MAXB>(S, W.prop, si, ei, w0.path, shape, wgt);
             } else {
                 unsigned long long o = spread_append(W.over, sub, 1ull);
                 OverflowRec rec;
@@ -144,8 +196,8 @@ SMS_DEV void ms_estimate_body(const DScene &S, const sms_ss_query *__restrict__ 
 }
 
 template <int MAXB>
-SMS_DEV void ms_coop_body(const DScene &S, const sms_ss_query *__restrict__ Q, uint64_t seed, float *__restrict__ invp,
-                          EstimatorWork W) {
+SMS_DEV void ms_coop_body(const DScene &S, const sms_ss_query *__restrict__ Q, uint64_t seed,
+                          const float *__restrict__ part, float *__restrict__ invp, EstimatorWork W) {
     const sms_config &cfg = S.cfg;
     const int lane = threadIdx.x & 63;
     const int nc = S.n_casters_ms;
@@ -165,15 +217,17 @@ SMS_DEV void ms_coop_body(const DScene &S, const sms_ss_query *__restrict__ Q, u
         V3 direction = v3(rec.sol[0], rec.sol[1], rec.sol[2]);
         int shapes[4] = {rec.shapes[0], rec.shapes[1], rec.shapes[2], rec.shapes[3]};
         const int shape = S.casters_ms[c];
+        const float *gq = ms_mixture(W.prop, q, shape);
         int64_t kcap = cfg.max_trials > 0 ? (int64_t) cfg.max_trials : (int64_t) 1 << 40;
         int64_t found = -1;
         for (int64_t base = rec.k; base <= kcap && found < 0; base += 64) {
             int64_t k = base + lane;
             bool match = false, ran = false, conv = false;
             int its = 0;
+            MSWalk<MAXB> t;
+            t.ok = false;
             if (k <= kcap) {
-                MSWalk<MAXB> t;
-                ms_walk_k<MAXB>(S, qkey, c, shape, (uint64_t) k, si.p, ei, false, shapes, t);
+                ms_walk_k<MAXB>(S, W.prop, gq, qkey, c, shape, (uint64_t) k, si.p, ei, false, shapes, t);
                 ran = true; its = t.iterations; conv = t.ok;
                 if (t.ok) {
                     V3 dtr = normalize(t.path.v[0].v.p - si.p);
@@ -182,15 +236,27 @@ SMS_DEV void ms_coop_body(const DScene &S, const sms_ss_query *__restrict__ Q, u
             }
             unsigned long long ballot = __ballot(match);
             if (ballot) found = base + (int64_t) __ffsll((long long) ballot) - 1;
+            // the trials the sequential loop runs: up to the first match
             if (ran && (found < 0 || k <= found)) {
                 st_walks++; st_iters += (unsigned long long) its; st_tail += (unsigned long long) its;
-                if (conv) st_conv++;
+                if (conv) {
+                    st_conv++;
+                    if (capturing(W.prop, shape, SMS_CAPTURE_SOLUTIONS))
+                        ms_capture<MAXB>(S, W.prop, si, ei, t.path, shape, 1.f);
+                }
             }
         }
         if (lane == 0) {
             invp[rec.q] = found >= 0 ? (float) found : 0.f;
             unsigned long long tr = found >= 0 ? (unsigned long long) found : (unsigned long long) kcap + 1ull;
             st_btrials += tr; st_bmax = st_bmax > tr ? st_bmax : tr;
+            const float wgt = found > 0 ? mean3(&part[3 * rec.q]) * (float) found : 0.f;
+            if (wgt > 0.f && capturing(W.prop, shape, SMS_CAPTURE_ESTIMATES)) {
+                // the first walk again (same stream), for the uv of its caster vertex
+                MSWalk<MAXB> w0;
+                ms_walk_k<MAXB>(S, W.prop, gq, qkey, c, shape, 0, si.p, ei, true, nullptr, w0);
+                if (w0.ok) ms_capture<MAXB>(S, W.prop, si, ei, w0.path, shape, wgt);
+            }
         }
     }
     stat_add(W.stats + 0, st_walks); stat_add(W.stats + 1, st_conv); stat_add(W.stats + 2, st_iters);
@@ -256,7 +322,8 @@ SMS_DEV void ms_biased_body(const DScene &S, const sms_ss_query *__restrict__ Q,
                 bool cand = false;
                 V3 direction = v3(0.f, 0.f, 0.f);
                 if (active) {
-                    ms_walk_k<MAXB>(S, qkey, c, shape, (uint64_t) m, si.p, ei, first, first ? nullptr : seed_shapes, r);
+                    ms_walk_k<MAXB>(S, W.prop, nullptr, qkey, c, shape, (uint64_t) m, si.p, ei, first,
+                                    first ? nullptr : seed_shapes, r);
                     st_walks++; st_iters += (unsigned long long) r.iterations;
                     if (r.ok) {
                         st_conv++;
@@ -310,24 +377,26 @@ SMS_DEV void ms_biased_body(const DScene &S, const sms_ss_query *__restrict__ Q,
     stat_add(W.stats + 0, st_walks); stat_add(W.stats + 1, st_conv); stat_add(W.stats + 2, st_iters);
 }
 
-#define SMS_MS_KERNELS(B)                                                                                           \
-    __global__ void __launch_bounds__(256) k_ms_walk_##B(DScene S, const sms_ms_walk_in *in, int64_t n,            \
-                                                         sms_ms_walk_out *out) {                                   \
-        ms_walk_body<B>(S, in, n, out);                                                                             \
-    }                                                                                                               \
-    __global__ void __launch_bounds__(256) k_ms_estimate_##B(DScene S, const sms_ss_query *Q, int64_t n,           \
-                                                             const uint64_t *__restrict__ seedp, float *part, float *invp,               \
-                                                             EstimatorWork W) {                                     \
-        ms_estimate_body<B>(S, Q, n, *seedp, part, invp, W);                                                          \
-    }                                                                                                               \
-    __global__ void __launch_bounds__(256) k_ms_coop_##B(DScene S, const sms_ss_query *Q, const uint64_t *__restrict__ seedp,           \
-                                                         float *invp, EstimatorWork W) {                           \
-        ms_coop_body<B>(S, Q, *seedp, invp, W);                                                                       \
-    }                                                                                                               \
-    __global__ void __launch_bounds__(256) k_ms_biased_##B(DScene S, const sms_ss_query *Q, int64_t n,             \
-                                                           const uint64_t *__restrict__ seedp, float *out, EstimatorWork W) {           \
-        __shared__ float sol_lds[4][MS_BIASED_MAX_SOL][3];                                                          \
-        ms_biased_body<B>(S, Q, n, *seedp, out, W, sol_lds[threadIdx.x >> 6]);                                        \
+#define SMS_MS_KERNELS(B)                                                                                       \
+    __global__ void __launch_bounds__(256) k_ms_walk_##B(DScene S, const sms_ms_walk_in *in, int64_t n,         \
+                                                         sms_ms_walk_out *out) {                                \
+        ms_walk_body<B>(S, in, n, out);                                                                         \
+    }                                                                                                           \
+    __global__ void __launch_bounds__(256) k_ms_estimate_##B(DScene S, const sms_ss_query *Q, int64_t n,        \
+                                                             const uint64_t *__restrict__ seedp, float *part,   \
+                                                             float *invp, EstimatorWork W) {                    \
+        ms_estimate_body<B>(S, Q, n, *seedp, part, invp, W);                                                    \
+    }                                                                                                           \
+    __global__ void __launch_bounds__(256) k_ms_coop_##B(DScene S, const sms_ss_query *Q,                       \
+                                                         const uint64_t *__restrict__ seedp, const float *part, \
+                                                         float *invp, EstimatorWork W) {                        \
+        ms_coop_body<B>(S, Q, *seedp, part, invp, W);                                                           \
+    }                                                                                                           \
+    __global__ void __launch_bounds__(256) k_ms_biased_##B(DScene S, const sms_ss_query *Q, int64_t n,          \
+                                                           const uint64_t *__restrict__ seedp, float *out,      \
+                                                           EstimatorWork W) {                                   \
+        __shared__ float sol_lds[4][MS_BIASED_MAX_SOL][3];                                                      \
+        ms_biased_body<B>(S, Q, n, *seedp, out, W, sol_lds[threadIdx.x >> 6]);                                  \
     }
 
 SMS_MS_KERNELS(2)

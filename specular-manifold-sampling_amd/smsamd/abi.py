@@ -152,6 +152,7 @@ EXPORTS = ["sms_config_default", "sms_ctx_create", "sms_ctx_destroy", "sms_ctx_s
            "sms_ms_walk_batch_host", "sms_ms_estimate_batch_host", "sms_glint_autotune",
            "sms_mlp_create", "sms_mlp_destroy", "sms_mlp_get_params", "sms_mlp_set_params",
            "sms_mlp_infer", "sms_mlp_grad", "sms_mlp_adam", "sms_mlp_train", "sms_glint_graph_create",
+           "sms_ms_proposal_inputs", "sms_ms_proposal_outputs",
            "sms_ss_graph_create", "sms_ms_graph_create", "sms_ss_pass_graph_create", "sms_graph_launch",
            "sms_graph_stats", "sms_graph_destroy", "sms_ctx_set_proposal",
            "sms_ctx_set_training_capture", "sms_ctx_training_samples", "sms_ctx_clear_training",
@@ -218,6 +219,8 @@ def load_library(path=LIB_PATH):
     lib.sms_ctx_clear_training.argtypes = [vp]
     lib.sms_ss_proposal_inputs.argtypes = [vp, vp, i64, C.c_uint64, vp]
     lib.sms_ss_proposal_outputs.argtypes = [vp, vp, i64, C.c_uint64, vp]
+    lib.sms_ms_proposal_inputs.argtypes = [vp, vp, i64, C.c_uint64, vp]
+    lib.sms_ms_proposal_outputs.argtypes = [vp, vp, i64, C.c_uint64, vp]
     lib.sms_ss_make_queries.argtypes = [vp, C.c_int32, C.c_uint64, vp, vp]
     lib.sms_film_accumulate.argtypes = [vp, vp, vp, i64, C.c_int32, vp, vp, vp]
     for name in EXPORTS:

@@ -312,6 +312,12 @@ class Context:
     def proposal_outputs(self, q, seed):
         return self._per_query(lib().sms_ss_proposal_outputs, q, seed, abi.MLP_OUTPUTS, "sms_ss_proposal_outputs")
 
+    def ms_proposal_inputs(self, q, seed):
+        return self._per_query(lib().sms_ms_proposal_inputs, q, seed, abi.MLP_INPUTS, "sms_ms_proposal_inputs")
+
+    def ms_proposal_outputs(self, q, seed):
+        return self._per_query(lib().sms_ms_proposal_outputs, q, seed, abi.MLP_OUTPUTS, "sms_ms_proposal_outputs")
+
 
 class Graph:
     """A captured frame launch sequence (sms_graph_*): launch(seed) replays it asynchronously."""

@@ -60,9 +60,10 @@ class Film:
 
 
 class PassRunner:
-    """Device buffers for passes of ``pass_spp`` samples per pixel over the camera's frame."""
+    """Device buffers for passes of ``pass_spp`` samples per pixel over the camera's frame;
+    ``estimator``: "ss" (single scatter) or "ms" (multi scatter) at the first receiver."""
 
-    def __init__(self, ctx, pass_spp):
+    def __init__(self, ctx, pass_spp, estimator="ss"):
         cam = ctx.scene.camera
         self.ctx = ctx
         self.pixels = cam["width"] * cam["height"]
@@ -71,11 +72,14 @@ class PassRunner:
         self.q = gpu.DeviceBuffer(ctx, self.n * abi.SS_QUERY.itemsize)
         self.w = gpu.DeviceBuffer(ctx, self.n * 4)
         self.rgb = gpu.DeviceBuffer(ctx, self.n * 12)
+        if estimator not in ("ss", "ms"):
+            raise ValueError(estimator)
+        self.estimate = ctx.ss_estimate_device if estimator == "ss" else ctx.ms_estimate_device
 
     def run(self, seed, film, stats=None):
         """One pass: camera samples of seed `seed`, estimates with the same seed, into film."""
         self.ctx.ss_queries_device(self.pass_spp, seed, self.q, self.w)
-        st = self.ctx.ss_estimate_device(self.q, self.n, seed, self.rgb)
+        st = self.estimate(self.q, self.n, seed, self.rgb)
         self.ctx.film_accumulate_device(self.rgb, self.w, self.pixels, self.pass_spp, film.sum, film.sq, film.n)
         film.spp += self.pass_spp
         if stats is not None:

@@ -63,6 +63,9 @@ def load():
     lib.oracle_ss_estimate_batch_prop.argtypes = [vp, vp, i64, C.c_uint64, vp, C.c_int, C.c_float, vp,
                                                   C.POINTER(abi.SmsStats), C.c_int]
     lib.oracle_prop_inputs.argtypes = [vp, vp, i64, C.c_uint64, vp]
+    lib.oracle_prop_inputs_ms.argtypes = [vp, vp, i64, C.c_uint64, vp]
+    lib.oracle_ms_estimate_batch_prop.argtypes = [vp, vp, i64, C.c_uint64, vp, C.c_int, C.c_float, vp,
+                                                  C.POINTER(abi.SmsStats), C.c_int]
     lib.oracle_ss_make_queries.argtypes = [vp, C.c_int, C.c_uint64, vp, vp, C.c_int]
     lib.oracle_scene_box.argtypes = [vp, vp, vp]
     lib.oracle_gmm_prepare.argtypes = [vp, i64, vp]
@@ -158,6 +161,22 @@ class Oracle:
         x = np.zeros((len(q), 9), np.float32)
         self.lib.oracle_prop_inputs(self.h, q.ctypes.data, len(q), seed, x.ctypes.data)
         return x
+
+    def ms_prop_inputs(self, q, seed):
+        x = np.zeros((len(q), 9), np.float32)
+        self.lib.oracle_prop_inputs_ms(self.h, q.ctypes.data, len(q), seed, x.ctypes.data)
+        return x
+
+    def ms_estimate_prop(self, q, seed, raw, shape, defensive):
+        """Multi-scatter estimator with the defensive-mixture seed on caster ``shape``."""
+        raw = np.ascontiguousarray(raw, np.float32)
+        assert raw.shape == (len(q), 48)
+        out = np.zeros((len(q), 3), np.float32)
+        st = abi.SmsStats()
+        rc = self.lib.oracle_ms_estimate_batch_prop(self.h, q.ctypes.data, len(q), seed, raw.ctypes.data, shape,
+                                                    defensive, out.ctypes.data, C.byref(st), self.threads)
+        assert rc == 0, rc
+        return out, st.as_dict()
 
     def scene_box(self):
         c = np.zeros(3, np.float32)

@@ -194,3 +194,33 @@ def test_flat_slab_estimate_matches_brute_force(flat_slab, point):
     assert st["walks_converged"] > 0
     assert abs(m.mean() - ref) < 4 * se + 4 * 0.5 / math.pi * irr_err + 1e-3 * ref, (m.mean(), ref, se)
     np.testing.assert_array_equal(est[:, 0], est[:, 1])
+
+
+def test_ms_mixture_estimator_is_unbiased():
+    """Two-vertex slab chains seeded through a concentrated defensive mixture on the top face:
+    same mean as the reference's uniform seeds within the standard errors, more converged walks."""
+    sc = scenes.slab(width=32, height=32, n=64)
+    cfg = abi.SmsConfig.default(bounces=2, max_trials=100000)
+    orc = oracle_lib.Oracle(sc, cfg)
+    q = workloads.plane_queries(sc, 24, 5, extent=0.3, shape=2)
+    light = np.asarray(sc.shapes[3]["center"], np.float64)
+    p = q["p"].astype(np.float64)
+    hit = p + (0.6 / light[1]) * (light - p)               # toward the light, at the top face
+    u, v = np.clip(hit[:, 0] + 0.5, 0.02, 0.98), np.clip(hit[:, 2] + 0.5, 0.02, 0.98)
+    raw = np.zeros((len(q), 48), np.float32)
+    for c in range(8):
+        raw[:, 6 * c + 1] = np.log(u / (1 - u)) + 0.03 * (c - 3.5)
+        raw[:, 6 * c + 2] = np.log(v / (1 - v)) - 0.03 * (c - 3.5)
+        raw[:, 6 * c + 3] = raw[:, 6 * c + 4] = np.log(0.08)
+    reps = 600
+    qq, rr = np.concatenate([q] * reps), np.concatenate([raw] * reps)
+    uni, su = orc.ms_estimate(qq, 3)
+    mix, sm = orc.ms_estimate_prop(qq, 3, rr, 0, 0.2)
+    lu, lm = uni[:, 0].astype(np.float64), mix[:, 0].astype(np.float64)
+    se = np.sqrt(lu.var() / len(lu) + lm.var() / len(lm))
+    print(f"uniform {lu.mean():.5f} mixture {lm.mean():.5f} se {se:.5f} conv {su['walks_converged'] / su['walks']:.3f} "
+          f"{sm['walks_converged'] / sm['walks']:.3f}")
+    assert lu.mean() > 0
+    assert abs(lu.mean() - lm.mean()) < 4 * se
+    assert sm["walks_converged"] / sm["walks"] > su["walks_converged"] / su["walks"]
+    orc.close()
