@@ -69,7 +69,10 @@ def parse():
     ap.add_argument("--ms-only", action="store_true", help="run only the multi-scatter slab experiment")
     ap.add_argument("--no-ms", action="store_true", help="skip the multi-scatter slab experiment")
     ap.add_argument("--ms-spp", type=int, default=32, help="uniform-seed samples per pixel of the slab budget")
-    ap.add_argument("--ms-ref-spp", type=int, default=256, help="slab reference samples per pixel")
+    ap.add_argument("--rmse-reps", type=int, default=3, help="frames per method of the pool experiment")
+    ap.add_argument("--ms-reps", type=int, default=2, help="frames per method of the slab experiment")
+    ap.add_argument("--ms-ref-spp", type=int, default=128, help="slab reference samples per pixel")
+    ap.add_argument("--ms-pass-spp", type=int, default=2, help="samples per pixel of a slab pass")
     ap.add_argument("--mlp-only", action="store_true", help="run only the proposal network's training / inference timing")
     ap.add_argument("--no-mlp", action="store_true", help="skip the proposal network's timing")
     ap.add_argument("--mlp-steps", type=int, default=2000, help="timed training steps of the network leg")
@@ -362,11 +365,13 @@ def equal_time_rmse(args, case="pool"):
     the estimate-weighted solutions that pass captured (batch 8192, Adam lr 1e-3, train_epochs
     steps per 8192 records) until train_samples records were used; every pass counts in the
     image.  The amortized run renders T with the trained network (training done before the
-    frame).  RMSE against a uniform-seed reference of rmse_ref_spp samples, with the
-    reference's own variance removed."""
+    frame).  RMSE against a uniform-seed reference, with the reference's own variance removed.
+    Each method renders `frames` independent frames (the charged ones each training a fresh
+    network); a ratio is sqrt(mean MSE uniform / mean MSE learned) over them, the per-frame
+    ratios are reported beside it -- single frames are dominated by a few firefly pixels."""
     import numpy as np
     from smsamd import abi, gpu, render, scenes
-    res, pass_spp = args.rmse_res, args.rmse_pass_spp
+    res, pass_spp = args.rmse_res, (args.rmse_pass_spp if case == "pool" else args.ms_pass_spp)
     if case == "pool":
         sc = scenes.pool(res, res)
         cfg = abi.SmsConfig.default(max_trials=10_000_000, uniqueness_threshold=1e-5, solver_threshold=1e-5)
@@ -385,7 +390,8 @@ def equal_time_rmse(args, case="pool"):
         ctx.ss_queries_device(pass_spp, 999_998, runner.q, runner.w)
         sched = ctx.ss_autotune_device(runner.q, runner.n, 999_998, runner.rgb)
     else:
-        sched = ctx.get_schedule()
+        ctx.ss_queries_device(pass_spp, 999_998, runner.q, runner.w)
+        sched = ctx.ms_autotune_device(runner.q, runner.n, 999_998, runner.rgb)
     print(f"# rmse {case}: schedule {sched}", file=sys.stderr, flush=True)
 
     def frame(seed0, n_passes, st=None):
@@ -399,54 +405,82 @@ def equal_time_rmse(args, case="pool"):
     ref_var_mean = ref_var / max(film.spp, 1)
     dropped_ref = film.dropped()
     print(f"# rmse {case}: reference {film.spp} spp in {t_ref:.1f}s", file=sys.stderr, flush=True)
-    st_u = {}
-    (uni, _), T = render.timed(lambda: frame(20_000_000, budget_spp // pass_spp, st_u))
-    spp_u, dropped_u = film.spp, film.dropped()
+    lum = ref.mean(1)
+    mask = lum > 2.0 * lum.mean()
+    reps = args.rmse_reps if case == "pool" else args.ms_reps
 
-    # learned, training charged to the frame
-    film.clear()
-    mlp = gpu.Mlp(ctx, seed=1)
-    st_l = {}
-    tr, t_l = render.timed(lambda: render.render_online(
-        ctx, runner, film, mlp, 0, T, samples=args.train_samples, epochs=args.train_epochs, defensive=args.defensive,
-        seed0=30_000_000, stats=st_l))
-    learned, _ = film.download()
-    spp_l, dropped_l = film.spp, film.dropped()
-    steps, losses, train_s = tr["steps"], tr["losses"], tr["train_s"]
-    if args.train_samples - tr["used"] >= 8192:
-        # training goes on (outside any budget) to the full record count for the amortized run
+    def errors(img):
+        return (render.rmse(img, ref, ref_var_mean) ** 2, render.rmse(img, ref, ref_var_mean, mask) ** 2,
+                render.rmse_trimmed(img, ref, ref_var_mean) ** 2)
+
+    def until_budget(seed0, st):
+        t0 = time.perf_counter()
+        times, p = [], 0
+        while True:
+            est = sum(times) / len(times) if times else 0.0
+            if time.perf_counter() - t0 + est > T:
+                break
+            t = time.perf_counter()
+            runner.run(seed0 + p, film, st)
+            ctx.sync()
+            times.append(time.perf_counter() - t)
+            p += 1
+        return time.perf_counter() - t0
+
+    # uniform seeds: the first frame's time is the budget T
+    st_u, err_u, dropped = {}, [], {"reference": dropped_ref, "uniform": 0, "learned": 0, "amortized": 0}
+    (img, _), T = render.timed(lambda: frame(20_000_000, budget_spp // pass_spp, st_u))
+    spp_u = film.spp
+    err_u.append(errors(img))
+    dropped["uniform"] += film.dropped()
+    for r in range(1, reps):
         film.clear()
-        tr2 = render.render_online(ctx, runner, film, mlp, 0, float("inf"), samples=args.train_samples - tr["used"],
+        until_budget(20_000_000 + 1000 * r, None)
+        err_u.append(errors(film.download()[0]))
+        dropped["uniform"] += film.dropped()
+    # learned, training charged to each frame (a fresh network per frame)
+    err_l, spp_l, tr0, mlp = [], [], None, None
+    for r in range(reps):
+        film.clear()
+        m = gpu.Mlp(ctx, seed=1 + r)
+        st_l = {}
+        tr = render.render_online(ctx, runner, film, m, 0, T, samples=args.train_samples, epochs=args.train_epochs,
+                                  defensive=args.defensive, seed0=30_000_000 + 1000 * r, rng_seed=r, stats=st_l)
+        err_l.append(errors(film.download()[0]))
+        spp_l.append(film.spp)
+        dropped["learned"] += film.dropped()
+        if r == 0:
+            tr0, mlp = tr, m
+        else:
+            m.close()
+    steps, losses, train_s = tr0["steps"], tr0["losses"], tr0["train_s"]
+    if args.train_samples - tr0["used"] >= 8192:
+        # training of the first frame's network goes on (outside any budget) to the full record
+        # count for the amortized frames
+        film.clear()
+        tr2 = render.render_online(ctx, runner, film, mlp, 0, float("inf"), samples=args.train_samples - tr0["used"],
                                    epochs=args.train_epochs, defensive=args.defensive, seed0=35_000_000,
-                                   uniform_passes=0, rng_seed=1, stop_when_trained=True)
+                                   uniform_passes=0, rng_seed=99, stop_when_trained=True)
         steps += tr2["steps"]
         losses += tr2["losses"]
         train_s += tr2["train_s"]
     # learned, network already trained
     ctx.set_proposal(mlp, 0, args.defensive)
-    film.clear()
-    st_a = {}
-    t0 = time.perf_counter()
-    times, p = [], 0
-    while True:
-        est = sum(times) / len(times) if times else 0.0
-        if time.perf_counter() - t0 + est > T:
-            break
-        t = time.perf_counter()
-        runner.run(50_000_000 + p, film, st_a)
-        ctx.sync()
-        times.append(time.perf_counter() - t)
-        p += 1
-    t_a = time.perf_counter() - t0
-    amort, _ = film.download()
-    spp_a, dropped_a = film.spp, film.dropped()
+    err_a, spp_a, st_a = [], [], {}
+    for r in range(reps):
+        film.clear()
+        t_a = until_budget(50_000_000 + 1000 * r, st_a)
+        err_a.append(errors(film.download()[0]))
+        spp_a.append(film.spp)
+        dropped["amortized"] += film.dropped()
     ctx.set_proposal(None, -1)
 
-    lum = ref.mean(1)
-    mask = lum > 2.0 * lum.mean()
+    def ratio(eu, el, j):
+        return float(np.sqrt(np.mean([e[j] for e in eu]) / np.mean([e[j] for e in el])))
+
+    def per_frame(el):
+        return [round(float(np.sqrt(err_u[i][0] / el[i][0])), 3) for i in range(len(el))]
     mean_ref = float(ref.mean())
-    r_u, r_l, r_a = (render.rmse(x, ref, ref_var_mean) for x in (uni, learned, amort))
-    m_u, m_l, m_a = (render.rmse(x, ref, ref_var_mean, mask) for x in (uni, learned, amort))
 
     def conv_rate(st):
         return st.get("walks_converged", 0) / max(st.get("walks", 1), 1)
@@ -457,24 +491,27 @@ def equal_time_rmse(args, case="pool"):
                 f"glass slab double refraction (multi scatter, 2 vertices), {res}x{res} camera image")
     out = {
         "workload": f"{workload}, unbiased (max_trials 1e7, uniqueness 1e-5), passes of {pass_spp} spp",
-        "budget_s": round(T, 3), "uniform_spp": spp_u,
-        "learned_spp": spp_l, "learned_s": round(t_l, 3), "learned_passes_uniform": 1,
-        "train_s_within_budget": round(tr["train_s"], 3), "train_steps_within_budget": tr["steps"],
-        "train_records_within_budget": tr["used"],
+        "frames": reps, "budget_s": round(T, 3), "uniform_spp": spp_u,
+        "learned_spp": spp_l, "learned_passes_uniform": 1,
+        "train_s_within_budget": round(tr0["train_s"], 3), "train_steps_within_budget": tr0["steps"],
+        "train_records_within_budget": tr0["used"],
         "train_s": round(train_s, 3), "train_samples": args.train_samples, "train_steps": steps,
         "train_epochs": args.train_epochs,
         "train_loss_first_last": [round(losses[0], 4), round(losses[-1], 4)] if losses else None,
-        "amortized_spp": spp_a, "amortized_s": round(t_a, 3),
-        "reference_spp": args.rmse_ref_spp, "reference_s": round(t_ref, 2), "mean_pixel": mean_ref,
-        "rmse_uniform": r_u, "rmse_learned": r_l, "rmse_amortized": r_a,
-        "ratio": r_u / r_l if r_l > 0 else None, "ratio_amortized": r_u / r_a if r_a > 0 else None,
+        "amortized_spp": spp_a,
+        "reference_spp": ref_spp, "reference_s": round(t_ref, 2), "mean_pixel": mean_ref,
+        "rmse_uniform": float(np.sqrt(np.mean([e[0] for e in err_u]))),
+        "rmse_learned": float(np.sqrt(np.mean([e[0] for e in err_l]))),
+        "rmse_amortized": float(np.sqrt(np.mean([e[0] for e in err_a]))),
+        "ratio": ratio(err_u, err_l, 0), "ratio_amortized": ratio(err_u, err_a, 0),
+        "ratio_per_frame": per_frame(err_l), "ratio_amortized_per_frame": per_frame(err_a),
         "caustic_mask_pixels": int(mask.sum()),
-        "ratio_caustic_mask": m_u / m_l if m_l > 0 else None,
-        "ratio_caustic_mask_amortized": m_u / m_a if m_a > 0 else None,
+        "ratio_caustic_mask": ratio(err_u, err_l, 1), "ratio_caustic_mask_amortized": ratio(err_u, err_a, 1),
+        "ratio_trimmed": ratio(err_u, err_l, 2), "ratio_trimmed_amortized": ratio(err_u, err_a, 2),
+        "trimmed": "RMSE without the 0.1 % of pixels with the largest squared error",
         "converged_per_walk": {"uniform": round(conv_rate(st_u), 4), "learned": round(conv_rate(st_a), 4)},
         "trials_per_estimate": {"uniform": round(trials(st_u), 3), "learned": round(trials(st_a), 3)},
-        "invalid_samples_dropped": {"reference": dropped_ref, "uniform": dropped_u, "learned": dropped_l,
-                                    "amortized": dropped_a},
+        "invalid_samples_dropped": dropped,
         "uniform_frame": {"walks_per_s": st_u.get("walks", 0) / T, "converged_walks_per_s": st_u.get("walks_converged", 0) / T,
                           "newton_iterations_per_s": st_u.get("newton_iterations", 0) / T,
                           "note": "whole frames: camera passes, estimates and film"},

@@ -1110,6 +1110,37 @@ sms_status sms_ss_autotune(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint6
     return SMS_OK;
 }
 
+sms_status sms_ms_autotune(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *out_rgb,
+                           sms_schedule *best) {
+    if (!ctx) return SMS_E_INVALID;
+    if (!best) return fail(ctx, SMS_E_INVALID, "null schedule");
+    if (ctx->scene.cfg.biased) return fail(ctx, SMS_E_INVALID, "the autotuner times the unbiased estimator");
+    const sms_schedule keep = schedule_of(ctx);
+    sms_schedule win = keep;
+    float win_ms = INFINITY;
+    sms_status st = SMS_OK;
+    const int cands[6] = {1, 2, 4, 8, 16, 32};
+    for (int lt : cands) {
+        ctx->lane_trials = lt;
+        ctx->blocks_per_cu = 0;
+        float ms = 0.f;
+        for (int rep = 0; rep < 2; ++rep) {           // the first run warms caches and allocations
+            st = sms_ms_estimate_batch(ctx, q, n, seed + (uint64_t) rep, out_rgb, nullptr);
+            if (st) break;
+            ms = ctx->last_ms + ctx->last_ms2;
+        }
+        if (st) break;
+        if (ms < win_ms) {
+            win_ms = ms;
+            win = schedule_of(ctx);
+        }
+    }
+    apply_schedule(ctx, st ? keep : win);
+    if (st) return st;
+    *best = win;
+    return SMS_OK;
+}
+
 sms_status sms_ss_estimate_batch(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *out_rgb,
                                  sms_stats *stats) {
     sms_status st = ss_prepare(ctx, q, n, out_rgb);

@@ -157,7 +157,7 @@ EXPORTS = ["sms_config_default", "sms_ctx_create", "sms_ctx_destroy", "sms_ctx_s
            "sms_graph_stats", "sms_graph_destroy", "sms_ctx_set_proposal",
            "sms_ctx_set_training_capture", "sms_ctx_training_samples", "sms_ctx_clear_training",
            "sms_ss_proposal_inputs", "sms_ss_proposal_outputs", "sms_ss_make_queries",
-           "sms_film_accumulate", "sms_ss_autotune"]
+           "sms_film_accumulate", "sms_ss_autotune", "sms_ms_autotune"]
 
 # learned seed proposal (include/sms.h)
 MLP_INPUTS = 9
@@ -194,6 +194,7 @@ def load_library(path=LIB_PATH):
     lib.sms_glint_make_queries.argtypes = [vp, C.c_int32, C.c_uint64, vp]
     lib.sms_glint_autotune.argtypes = [vp, vp, i64, C.c_uint64, vp, C.POINTER(SmsSchedule)]
     lib.sms_ss_autotune.argtypes = [vp, vp, i64, C.c_uint64, vp, C.POINTER(SmsSchedule)]
+    lib.sms_ms_autotune.argtypes = [vp, vp, i64, C.c_uint64, vp, C.POINTER(SmsSchedule)]
     lib.sms_ctx_last_kernel_ms.argtypes = [vp, vp]
     lib.sms_device_alloc.argtypes = [vp, C.c_size_t, C.POINTER(vp)]
     lib.sms_device_free.argtypes = [vp, vp]

@@ -215,6 +215,12 @@ class Context:
         _check(lib().sms_ss_autotune(self.h, qbuf.ptr, n, seed, outbuf.ptr, C.byref(s)), self.h, "ss_autotune")
         return s.as_dict()
 
+    def ms_autotune_device(self, qbuf, n, seed, outbuf):
+        """Pick the fastest unbiased multi-scatter schedule on these queries (sms_ms_autotune)."""
+        s = abi.SmsSchedule()
+        _check(lib().sms_ms_autotune(self.h, qbuf.ptr, n, seed, outbuf.ptr, C.byref(s)), self.h, "ms_autotune")
+        return s.as_dict()
+
     def ss_estimate_device(self, qbuf, n, seed, outbuf):
         st = abi.SmsStats()
         _check(lib().sms_ss_estimate_batch(self.h, qbuf.ptr, n, seed, outbuf.ptr, C.byref(st)), self.h,

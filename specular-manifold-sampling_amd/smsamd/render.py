@@ -152,6 +152,14 @@ def render_online(ctx, runner, film, mlp, caster, budget_s, samples=2_000_000, b
     return out
 
 
+def rmse_trimmed(img, ref, ref_var_of_mean, drop=1e-3):
+    """rmse() over the pixels left after dropping the fraction `drop` with the largest squared
+    error (a firefly-robust companion of the plain RMSE)."""
+    d2 = (((img - ref) ** 2 - ref_var_of_mean).mean(1))
+    keep = max(1, int(len(d2) * (1.0 - drop)))
+    return float(np.sqrt(max(np.partition(d2, keep - 1)[:keep].mean(), 0.0)))
+
+
 def rmse(img, ref, ref_var_of_mean, mask=None):
     """RMSE of img against ref with the reference's own noise removed: mean over pixels (and
     channels) of (img - ref)^2 - Var[ref mean], floored at 0."""
