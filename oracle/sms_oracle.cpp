@@ -1623,19 +1623,27 @@ static float geometric_term_glint(const MV &v0, const MV &v1, const MV &v2v) {
     return G;
 }
 
-/* evaluate_glint_contribution (manifold_glints.cpp:267-294) */
-static V3 glint_contribution(const Scene &S, V3 sensor_p, const EI &ei, const GlintSI &gsi, V2 uv) {
-    MV vy;
-    if (!emitter_interaction_to_vertex(S, ei, gsi.p, &vy)) return v3(0.f, 0.f, 0.f);
-    MV vx = mv_point(sensor_p);
+/* the three vertices of a glint path: sensor point, the footprint's point with the normal map
+   at uv, the emitter vertex (manifold_glints.cpp:267-283) */
+static bool glint_vertices(const Scene &S, V3 sensor_p, const EI &ei, const GlintSI &gsi, V2 uv, MV *vx, MV *vtx,
+                           MV *vy) {
+    if (!emitter_interaction_to_vertex(S, ei, gsi.p, vy)) return false;
+    *vx = mv_point(sensor_p);
     V3 dx = normalize(gsi.p - sensor_p);
-    vx.n = dx; vx.gn = dx;
-    coordinate_system(vx.n, &vx.dp_du, &vx.dp_dv);
+    vx->n = dx; vx->gn = dx;
+    coordinate_system(vx->n, &vx->dp_du, &vx->dp_dv);
     SI si = si_invalid();
     si.valid = true; si.p = gsi.p; si.n = gsi.gn; si.sh_n = gsi.n; si.sh_s = gsi.s; si.sh_t = gsi.t;
     si.dp_du = gsi.dp_du; si.dp_dv = gsi.dp_dv; si.uv = uv; si.shape = gsi.shape;
-    MV vtx = mv_from_si(S, si);
-    make_orthonormal(vtx);
+    *vtx = mv_from_si(S, si);
+    make_orthonormal(*vtx);
+    return true;
+}
+
+/* evaluate_glint_contribution (manifold_glints.cpp:267-294) */
+static V3 glint_contribution(const Scene &S, V3 sensor_p, const EI &ei, const GlintSI &gsi, V2 uv) {
+    MV vx, vtx, vy;
+    if (!glint_vertices(S, sensor_p, ei, gsi, uv, &vx, &vtx, &vy)) return v3(0.f, 0.f, 0.f);
     float F = specular_reflectance_glint(S.shapes[gsi.shape], vtx.n, normalize(ei.p - gsi.p));
     float G = geometric_term_glint(vx, vtx, vy);
     float v = F * G;
@@ -2729,6 +2737,20 @@ int oracle_ms_estimate_batch_prop(void *h, const sms_ss_query *q, int64_t n, uin
         tot.btrials += s.btrials; if (s.bmax > tot.bmax) tot.bmax = s.bmax; }
     stats_out(tot, stats);
     return 0;
+}
+
+/* generalized geometric term of a glint path (tests): the vertex at query q's point with the
+   normal map at uv, the sensor at x0, a point light at x2 */
+float oracle_glint_geometric(void *h, const sms_glint_query *q, const float *uv, const float *x0, const float *x2) {
+    const Scene &S = *(Scene *) h;
+    GlintSI gsi = gsi_from_query(*q);
+    EI ei;
+    memset(&ei, 0, sizeof(ei));
+    ei.kind = SMS_EI_POINT;
+    ei.p = v3(x2[0], x2[1], x2[2]);
+    MV vx, vtx, vy;
+    if (!glint_vertices(S, v3(x0[0], x0[1], x0[2]), ei, gsi, v2(uv[0], uv[1]), &vx, &vtx, &vy)) return -1.f;
+    return geometric_term_glint(vx, vtx, vy);
 }
 
 void oracle_scene_box(void *h, float *c, float *inv) {

@@ -2,6 +2,7 @@
 "bilinearly interpolated level-0 slope = target slope" found per texel cell in FP64 from the
 decoded RGB8 map (normalmap.cpp:46-66, normalmap.h:62-104); walks on a map whose slopes vary
 linearly across a cell converge to the closed-form point."""
+import ctypes
 import math
 
 import numpy as np
@@ -9,6 +10,9 @@ import pytest
 
 import oracle_lib
 from smsamd import abi, scenes, workloads
+
+
+C_FLOAT, C_VP = ctypes.c_float, ctypes.c_void_p
 
 
 def decode_slopes(rgb):
@@ -115,3 +119,110 @@ def test_glint_linear_cell_closed_form():
         assert min(np.linalg.norm(uv - r) for r in roots) < 1e-4
         assert out["iterations"][i] <= 6
     o.close()
+
+
+def _bilinear(S, uv, res):
+    """Texel-centre bilinear interpolation of the decoded slopes (interior uv)."""
+    x, y = uv[0] * res - 0.5, uv[1] * res - 0.5
+    i, j = int(math.floor(x)), int(math.floor(y))
+    fx, fy = x - i, y - j
+    return (S[j, i] * (1 - fx) + S[j, i + 1] * fx) * (1 - fy) + (S[j + 1, i] * (1 - fx) + S[j + 1, i + 1] * fx) * fy
+
+
+def test_glint_geometric_term_matches_finite_differences():
+    """The glint path's generalized geometric term for a point light (manifold_glints.cpp:411-554)
+    is |cos| / r0^2 x |det dx1/dx2|: how the specular vertex x1 -- moving on the plane, its normal
+    the normal map's at its uv -- follows the light x2 over the light's tangent plane.  An
+    independent FP64 solve of "half-vector parallel to the mapped normal", differentiated by
+    central differences over light moves, must give the same value.  Both sides use orthonormal
+    bases, so the determinant does not depend on which ones."""
+    res = 16
+    g = (np.arange(res) + 0.5) / res
+    U, V = np.meshgrid(g, g, indexing="xy")
+    sl = np.stack([0.12 * np.sin(2 * np.pi * U) + 0.05 * V, 0.10 * np.cos(2 * np.pi * V) - 0.04 * U], -1)
+    n = np.concatenate([-sl, np.ones((res, res, 1))], -1)
+    n /= np.linalg.norm(n, axis=-1, keepdims=True)
+    rgb = np.clip(np.round((n * 0.5 + 0.5) * 255), 0, 255).astype(np.uint8)
+    S = decode_slopes(rgb)
+    sc = scenes.glint_plane(16, 16, normalmap=rgb)
+    orc = oracle_lib.Oracle(sc, abi.SmsConfig.default())
+    geo = orc.lib.oracle_glint_geometric
+    geo.restype, geo.argtypes = C_FLOAT, [C_VP] * 5
+    q0 = orc.glint_queries(1, 3)
+    q0 = q0[q0["shape"] >= 0][:1].copy()
+    f64 = lambda k: q0[k][0].astype(np.float64)
+    p_ref, uv_ref = f64("p"), f64("uv")
+    dpu, dpv, bn, gn = f64("dp_du"), f64("dp_dv"), f64("sh_n"), f64("n")
+    nz = lambda v: v / np.linalg.norm(v)
+    bs = nz(dpu - bn * (bn @ dpu))
+    bt = np.cross(bn, bs)
+
+    def x1_of(uv):
+        return p_ref + dpu * (uv[0] - uv_ref[0]) + dpv * (uv[1] - uv_ref[1])
+
+    def normal(uv):
+        s = _bilinear(S, uv, res)
+        return nz(-s[0] * bs - s[1] * bt + bn)
+
+    def residual(uv, x0, x2):
+        nn = normal(uv)
+        e1 = nz(bs - nn * (nn @ bs))
+        e2 = np.cross(nn, e1)
+        x1 = x1_of(uv)
+        h = nz(nz(x0 - x1) + nz(x2 - x1))
+        return np.array([h @ e1, h @ e2])
+
+    def solve(uv, x0, x2):
+        """Newton in uv, kept inside the bilinear cell it starts in (None if it leaves)."""
+        uv = np.array(uv, np.float64)
+        cell = np.floor(uv * res - 0.5)
+        for _ in range(60):
+            if np.any(np.floor(uv * res - 0.5) != cell):
+                return None
+            J = np.zeros((2, 2))
+            for k in range(2):
+                d = np.zeros(2)
+                d[k] = 1e-7
+                J[:, k] = (residual(uv + d, x0, x2) - residual(uv - d, x0, x2)) / 2e-7
+            step = np.linalg.solve(J, residual(uv, x0, x2))
+            uv = uv - step
+            if np.abs(step).max() < 1e-15:
+                break
+        return uv if np.abs(residual(uv, x0, x2)).max() < 1e-13 else None
+
+    a1 = nz(dpu)                                                  # orthonormal basis of the plane
+    a2 = nz(dpv - a1 * (a1 @ dpv))
+    rng = np.random.default_rng(5)
+    errs = []
+    for trial in range(40):
+        cell = rng.integers(2, res - 3, 2)
+        uv = (cell + 0.5 + rng.uniform(0.1, 0.9, 2)) / res         # inside one bilinear cell
+        x1, nn = x1_of(uv), normal(uv)
+        w0 = nz(np.array([rng.uniform(-0.5, 0.5), 1.0, rng.uniform(0.2, 0.8)]))
+        w2 = 2 * (w0 @ nn) * nn - w0
+        if w2 @ gn < 0.2:
+            continue
+        x0, x2 = x1 + rng.uniform(150, 300) * w0, x1 + rng.uniform(150, 300) * w2
+        d = nz(x1 - x2)
+        b1 = nz(np.cross(d, [0.0, 0.0, 1.0]))
+        b2 = np.cross(d, b1)
+        eps = 0.5
+        moved = [solve(uv, x0, x2 + sg * eps * b) for b in (b1, b2) for sg in (1, -1)]
+        if any(m is None for m in moved):                          # near a fold of the slope map
+            continue
+        J = np.array([[(x1_of(moved[2 * k]) - x1_of(moved[2 * k + 1])) @ a / (2 * eps) for k in range(2)]
+                      for a in (a1, a2)])
+        d0 = x0 - x1
+        expect = abs(nz(d0) @ gn) / (d0 @ d0) * abs(np.linalg.det(J))
+        if not expect < 1e-5:                                      # the reference clamps G at 1e-5
+            continue
+        qq = q0.copy()
+        qq["p"], qq["uv"] = x1, uv
+        uv32, x032, x232 = (np.ascontiguousarray(v, np.float32) for v in (uv, x0, x2))
+        G = geo(orc.h, qq.ctypes.data, uv32.ctypes.data, x032.ctypes.data, x232.ctypes.data)
+        # FP32 oracle against FP64 differences: about 1e-5 typically, a few 1e-3 near folds
+        assert G == pytest.approx(expect, rel=5e-3), (trial, uv, G, expect)
+        errs.append(abs(G / expect - 1))
+    assert len(errs) >= 15
+    assert np.median(errs) < 1e-4
+    orc.close()
