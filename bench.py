@@ -69,7 +69,7 @@ def parse():
     ap.add_argument("--ms-only", action="store_true", help="run only the multi-scatter slab experiment")
     ap.add_argument("--no-ms", action="store_true", help="skip the multi-scatter slab experiment")
     ap.add_argument("--ms-spp", type=int, default=32, help="uniform-seed samples per pixel of the slab budget")
-    ap.add_argument("--rmse-reps", type=int, default=3, help="frames per method of the pool experiment")
+    ap.add_argument("--rmse-reps", type=int, default=8, help="frames per method of the pool experiment")
     ap.add_argument("--ms-reps", type=int, default=2, help="frames per method of the slab experiment")
     ap.add_argument("--ms-ref-spp", type=int, default=128, help="slab reference samples per pixel")
     ap.add_argument("--ms-pass-spp", type=int, default=2, help="samples per pixel of a slab pass")
@@ -80,8 +80,10 @@ def parse():
     ap.add_argument("--rmse-spp", type=int, default=64, help="uniform-seed samples per pixel (sets the time budget)")
     ap.add_argument("--rmse-ref-spp", type=int, default=512, help="reference samples per pixel (uniform seeds)")
     ap.add_argument("--rmse-pass-spp", type=int, default=8)
+    ap.add_argument("--first-pass-spp", type=int, default=2,
+                    help="samples per pixel of the charged frame's uniform first pass")
     ap.add_argument("--train-samples", type=int, default=4_000_000, help="records the online training uses")
-    ap.add_argument("--train-epochs", type=int, default=16, help="Adam steps per 8192 captured records")
+    ap.add_argument("--train-epochs", type=int, default=8, help="Adam steps per 8192 captured records")
     ap.add_argument("--defensive", type=float, default=0.2)
     return ap.parse_args()
 
@@ -359,8 +361,9 @@ def equal_time_rmse(args, case="pool"):
     glass slab 0.2 thick (IOR 1.5) with value-noise faces over a diffuse floor, two-vertex
     multi-scatter chains (bounces 2, unbiased, cap 1e7, thresholds 1e-5 as slab_sms.xml), the
     proposal on the chain's seed point on the top face; budget ms_spp, reference ms_ref_spp.
-    Uniform seeding renders the budget's samples per pixel; its wall-clock time T is the budget.  The charged run gets the same T for
-    everything, training included: its first pass uses uniform seeds, every later pass the
+    Uniform seeding renders the budget's samples per pixel; its wall-clock time T is the budget.
+    The charged run gets the same T for everything, training included: its first pass (of
+    first_pass_spp samples per pixel) uses uniform seeds, every later pass the
     defensive mixture with the network as trained so far; after each pass the network trains on
     the estimate-weighted solutions that pass captured (batch 8192, Adam lr 1e-3, train_epochs
     steps per 8192 records) until train_samples records were used; every pass counts in the
@@ -368,7 +371,10 @@ def equal_time_rmse(args, case="pool"):
     frame).  RMSE against a uniform-seed reference, with the reference's own variance removed.
     Each method renders `frames` independent frames (the charged ones each training a fresh
     network); a ratio is sqrt(mean MSE uniform / mean MSE learned) over them, the per-frame
-    ratios are reported beside it -- single frames are dominated by a few firefly pixels."""
+    ratios and their median are reported beside it -- the errors are heavy-tailed: a frame
+    that draws one of the rare samples whose solution the mixture proposes with little mass
+    (at worst 0.2 x the uniform density, so up to 5x the uniform inverse-PDF) can hold most
+    of its squared error in a few pixels."""
     import numpy as np
     from smsamd import abi, gpu, render, scenes
     res, pass_spp = args.rmse_res, (args.rmse_pass_spp if case == "pool" else args.ms_pass_spp)
@@ -383,8 +389,12 @@ def equal_time_rmse(args, case="pool"):
         budget_spp, ref_spp, estimator = args.ms_spp, args.ms_ref_spp, "ms"
     ctx = gpu.Context(sc, cfg)
     runner = render.PassRunner(ctx, pass_spp, estimator)
+    first_spp = args.first_pass_spp or pass_spp
+    first_runner = render.PassRunner(ctx, first_spp, estimator) if first_spp != pass_spp else None
     film = render.Film(ctx, runner.pixels)
     runner.run(999_999, film)                                   # first launches
+    if first_runner is not None:
+        first_runner.run(999_997, film)
     ctx.sync()
     if estimator == "ss":
         ctx.ss_queries_device(pass_spp, 999_998, runner.q, runner.w)
@@ -445,7 +455,8 @@ def equal_time_rmse(args, case="pool"):
         m = gpu.Mlp(ctx, seed=1 + r)
         st_l = {}
         tr = render.render_online(ctx, runner, film, m, 0, T, samples=args.train_samples, epochs=args.train_epochs,
-                                  defensive=args.defensive, seed0=30_000_000 + 1000 * r, rng_seed=r, stats=st_l)
+                                  defensive=args.defensive, seed0=30_000_000 + 1000 * r, rng_seed=r, stats=st_l,
+                                  uniform_runner=first_runner)
         err_l.append(errors(film.download()[0]))
         spp_l.append(film.spp)
         dropped["learned"] += film.dropped()
@@ -492,7 +503,7 @@ def equal_time_rmse(args, case="pool"):
     out = {
         "workload": f"{workload}, unbiased (max_trials 1e7, uniqueness 1e-5), passes of {pass_spp} spp",
         "frames": reps, "budget_s": round(T, 3), "uniform_spp": spp_u,
-        "learned_spp": spp_l, "learned_passes_uniform": 1,
+        "learned_spp": spp_l, "learned_passes_uniform": 1, "first_pass_spp": first_spp,
         "train_s_within_budget": round(tr0["train_s"], 3), "train_steps_within_budget": tr0["steps"],
         "train_records_within_budget": tr0["used"],
         "train_s": round(train_s, 3), "train_samples": args.train_samples, "train_steps": steps,
@@ -505,6 +516,8 @@ def equal_time_rmse(args, case="pool"):
         "rmse_amortized": float(np.sqrt(np.mean([e[0] for e in err_a]))),
         "ratio": ratio(err_u, err_l, 0), "ratio_amortized": ratio(err_u, err_a, 0),
         "ratio_per_frame": per_frame(err_l), "ratio_amortized_per_frame": per_frame(err_a),
+        "ratio_median_frame": float(np.median(per_frame(err_l))),
+        "ratio_amortized_median_frame": float(np.median(per_frame(err_a))),
         "caustic_mask_pixels": int(mask.sum()),
         "ratio_caustic_mask": ratio(err_u, err_l, 1), "ratio_caustic_mask_amortized": ratio(err_u, err_a, 1),
         "ratio_trimmed": ratio(err_u, err_l, 2), "ratio_trimmed_amortized": ratio(err_u, err_a, 2),
@@ -520,6 +533,8 @@ def equal_time_rmse(args, case="pool"):
     print(f"# rmse {case}: {json.dumps(out)}", file=sys.stderr, flush=True)
     mlp.close()
     runner.free()
+    if first_runner is not None:
+        first_runner.free()
     film.free()
     ctx.close()
     return out

@@ -94,10 +94,11 @@ class PassRunner:
 
 def render_online(ctx, runner, film, mlp, caster, budget_s, samples=2_000_000, batch=8192, lr=1e-3, epochs=16,
                   uniform_passes=1, defensive=0.2, seed0=1 << 24, rng_seed=0, stats=None, max_steps_per_pass=None,
-                  stop_when_trained=False):
+                  stop_when_trained=False, uniform_runner=None):
     """One frame of `budget_s` seconds whose seed proposal is trained while it renders.  The
     first `uniform_passes` passes use uniform seeds, every later pass the defensive mixture with
-    the network as trained so far.  Each pass captures estimate-weighted records on mesh
+    the network as trained so far (the uniform passes with `uniform_runner`, a runner of fewer
+    samples per pixel, when given).  Each pass captures estimate-weighted records on mesh
     `caster`; a record's weight is its estimate, so the expected total weight at a solution is
     the solution's contribution whatever proposal found it, and records of learned passes are
     as valid as uniform ones.  After a pass, epochs x (records // batch) Adam steps (at most
@@ -109,7 +110,7 @@ def render_online(ctx, runner, film, mlp, caster, budget_s, samples=2_000_000, b
     spent training)."""
     rng = np.random.default_rng(rng_seed)
     ctx.set_proposal(None, caster)
-    ctx.set_training_capture(runner.n, abi.CAPTURE_ESTIMATES)
+    ctx.set_training_capture(max(runner.n, uniform_runner.n if uniform_runner else 0), abi.CAPTURE_ESTIMATES)
     t0 = time.perf_counter()
     out = {"passes": 0, "steps": 0, "losses": [], "used": 0, "train_s": 0.0}
     capturing, last = True, 0.0
@@ -118,7 +119,8 @@ def render_online(ctx, runner, film, mlp, caster, budget_s, samples=2_000_000, b
             t = time.perf_counter()
             if out["passes"] and (t - t0 + last > budget_s or (stop_when_trained and not capturing)):
                 break
-            runner.run(seed0 + out["passes"], film, stats)
+            first = uniform_runner is not None and out["passes"] < uniform_passes
+            (uniform_runner if first else runner).run(seed0 + out["passes"], film, stats)
             out["passes"] += 1
             if capturing:
                 xb, ub, wb, have = ctx.training_samples_device()
