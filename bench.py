@@ -441,19 +441,23 @@ def equal_time_rmse(args, case="pool"):
     st_u, err_u, dropped = {}, [], {"reference": dropped_ref, "uniform": 0, "learned": 0, "amortized": 0}
     (img, _), T = render.timed(lambda: frame(20_000_000, budget_spp // pass_spp, st_u))
     spp_u = film.spp
+    max_trials_u = [st_u.get("bernoulli_max", 0)]
     err_u.append(errors(img))
     dropped["uniform"] += film.dropped()
     for r in range(1, reps):
         film.clear()
-        until_budget(20_000_000 + 1000 * r, None)
+        st_f = {}
+        until_budget(20_000_000 + 1000 * r, st_f)
+        max_trials_u.append(st_f.get("bernoulli_max", 0))
         err_u.append(errors(film.download()[0]))
         dropped["uniform"] += film.dropped()
     # learned, training charged to each frame (a fresh network per frame)
-    err_l, spp_l, tr0, mlp = [], [], None, None
+    err_l, spp_l, tr0, mlp, frame_stats_l = [], [], None, None, []
     for r in range(reps):
         film.clear()
         m = gpu.Mlp(ctx, seed=1 + r)
         st_l = {}
+        frame_stats_l.append(st_l)
         tr = render.render_online(ctx, runner, film, m, 0, T, samples=args.train_samples, epochs=args.train_epochs,
                                   defensive=args.defensive, seed0=30_000_000 + 1000 * r, rng_seed=r, stats=st_l,
                                   uniform_runner=first_runner)
@@ -477,10 +481,14 @@ def equal_time_rmse(args, case="pool"):
         train_s += tr2["train_s"]
     # learned, network already trained
     ctx.set_proposal(mlp, 0, args.defensive)
-    err_a, spp_a, st_a = [], [], {}
+    err_a, spp_a, st_a, max_trials_a = [], [], {}, []
     for r in range(reps):
         film.clear()
-        t_a = until_budget(50_000_000 + 1000 * r, st_a)
+        st_f = {}
+        t_a = until_budget(50_000_000 + 1000 * r, st_f)
+        max_trials_a.append(st_f.get("bernoulli_max", 0))
+        for k, v in st_f.items():
+            st_a[k] = max(st_a.get(k, 0), v) if k == "bernoulli_max" else st_a.get(k, 0) + v
         err_a.append(errors(film.download()[0]))
         spp_a.append(film.spp)
         dropped["amortized"] += film.dropped()
@@ -516,6 +524,9 @@ def equal_time_rmse(args, case="pool"):
         "rmse_amortized": float(np.sqrt(np.mean([e[0] for e in err_a]))),
         "ratio": ratio(err_u, err_l, 0), "ratio_amortized": ratio(err_u, err_a, 0),
         "ratio_per_frame": per_frame(err_l), "ratio_amortized_per_frame": per_frame(err_a),
+        "max_trials_per_frame": {"uniform": max_trials_u,
+                                 "learned": [s.get("bernoulli_max", 0) for s in frame_stats_l],
+                                 "amortized": max_trials_a},
         "ratio_median_frame": float(np.median(per_frame(err_l))),
         "ratio_amortized_median_frame": float(np.median(per_frame(err_a))),
         "caustic_mask_pixels": int(mask.sum()),

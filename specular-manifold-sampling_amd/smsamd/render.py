@@ -84,7 +84,7 @@ class PassRunner:
         film.spp += self.pass_spp
         if stats is not None:
             for k, v in st.items():
-                stats[k] = stats.get(k, 0) + v
+                stats[k] = max(stats.get(k, 0), v) if k == "bernoulli_max" else stats.get(k, 0) + v
         return st
 
     def free(self):
