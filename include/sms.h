@@ -321,6 +321,10 @@ sms_status sms_ms_walk_batch(sms_ctx *ctx, const sms_ms_walk_in *in, int64_t n,
    out_rgb: n x 3 floats.  stats: host pointer, filled after completion (may be NULL). */
 sms_status sms_ss_estimate_batch(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed,
                                  float *out_rgb, sms_stats *stats);
+/* Glint estimator (SpecularManifoldGlints::specular_manifold_sampling, manifold_glints.cpp:
+   28-223): the SMS strategy -- unbiased Bernoulli trials, or with config.biased the distinct
+   solutions of a trial set of max_trials <= 128 walks -- and, in scenes with an area emitter,
+   the BSDF-sampling strategy at a footprint offset (:183-220), MIS-weighted as the reference. */
 sms_status sms_glint_estimate_batch(sms_ctx *ctx, const sms_glint_query *q, int64_t n,
                                     uint64_t seed, float *out_rgb, sms_stats *stats);
 /* Autotuner (no counterpart in the reference): times sms_glint_estimate_batch on the given

@@ -1822,9 +1822,74 @@ static float sms_emitter_pdf(const Scene &S, const EI &ei, V3 p) {
     return pdf;
 }
 
-/* SMS strategy of the glint estimator (manifold_glints.cpp:28-181) */
+/* BSDF-sampling strategy of the glint estimator (manifold_glints.cpp:183-220): an offset uv
+   inside the pixel footprint, NormalmapBSDF::sample there (bsdfs/normalmap.cpp:118-136: the
+   nested smooth conductor, conductor.cpp:217-275, in the level-0 perturbed frame, the reflected
+   direction kept only if it stays on the same side of the unperturbed frame), one ray
+   (Interaction::spawn_ray, interaction.h:58-61); a hit on an area emitter adds
+   mis(p_bsdf, p_sms) * F * L with p_bsdf the LEAN pdf at the offset uv plus 1 and p_sms the
+   emitter-sampling density of the hit (AreaLight::eval, area.cpp:71-79: front side only).
+   The footprint offset is drawn from PCG32 stream GLINT_BSDF_STREAM of the query. */
+static const uint64_t GLINT_BSDF_STREAM = 0xFFFFFFFEull;
+static bool scene_has_area_emitter(const Scene &S) {
+    for (const Emitter &e : S.emitters)
+        if (e.type == SMS_EMITTER_AREA) return true;
+    return false;
+}
+/* parts (tests, may be null): [0..1] footprint sample, [2..4] ray direction, [5] emitter hit
+   (-1: none), [6] F, [7] p_bsdf, [8] p_sms, [9] mis */
+static V3 glint_bsdf_strategy(const Scene &S, const GlintSI &si, uint64_t seed, uint64_t qid, float *parts = nullptr) {
+    const V3 zero = v3(0.f, 0.f, 0.f);
+    Pcg32 r; stream_seed(&r, seed, qid, GLINT_BSDF_STREAM);
+    float fx = pcg32_float(&r), fy = pcg32_float(&r);
+    if (parts) {
+        for (int i = 0; i < 10; ++i) parts[i] = 0.f;
+        parts[0] = fx; parts[1] = fy; parts[5] = -1.f;
+    }
+    V2 duv = (fx - 0.5f) * si.duv_dx + (fy - 0.5f) * si.duv_dy;
+    SI so = si_invalid();
+    so.valid = true; so.p = si.p; so.n = si.gn; so.sh_n = si.n; so.sh_s = si.s; so.sh_t = si.t;
+    so.dp_du = si.dp_du; so.dp_dv = si.dp_dv; so.uv = si.uv + duv; so.shape = si.shape;
+    MV f = mv_from_si(S, so);                     /* NormalmapBSDF::frame (n, s, t) at the offset uv */
+    V3 wi_w = to_world(si.s, si.t, si.n, si.wi);
+    V3 wi_p = v3(dot(wi_w, f.s), dot(wi_w, f.t), dot(wi_w, f.n));
+    if (!(wi_p.z > 0.f)) return zero;
+    V3 wo_p = v3(-wi_p.x, -wi_p.y, wi_p.z);
+    const Shape &sh = S.shapes[si.shape];
+    float F = fresnel_conductor(wi_p.z, sh.eta_r, sh.k);
+    V3 wo_w = to_world(f.s, f.t, f.n, wo_p);
+    V3 wo = v3(dot(wo_w, si.s), dot(wo_w, si.t), dot(wo_w, si.n));
+    if (!(wo_p.z * wo.z > 0.f)) return zero;
+    V3 d = to_world(si.s, si.t, si.n, wo);
+    if (parts) { parts[2] = d.x; parts[3] = d.y; parts[4] = d.z; parts[6] = F; }
+    SI h = ray_intersect(S, spawn_ray(si.p, d));
+    if (!h.valid) return zero;
+    int e = S.shapes[h.shape].emitter;
+    if (e < 0 || S.emitters[e].type != SMS_EMITTER_AREA) return zero;
+    if (parts) parts[5] = (float) e;
+    GlintSI sio = si;
+    sio.uv = so.uv;
+    float p_bsdf = lean_pdf(S, sio, wo) + 1.f;
+    EI eh;
+    memset(&eh, 0, sizeof(eh));
+    eh.kind = SMS_EI_AREA; eh.emitter = e; eh.p = h.p; eh.n = h.sh_n;
+    V3 dh = h.p - si.p;
+    eh.d = dh * (1.f / norm(dh));
+    float p_sms = sms_emitter_pdf(S, eh, si.p);
+    float mis = mis_power(p_bsdf, p_sms);
+    if (S.cfg.sms_strategy_only) mis = 0.f;
+    if (parts) { parts[7] = p_bsdf; parts[8] = p_sms; parts[9] = mis; }
+    if (!(dot(d * -1.f, h.sh_n) > 0.f)) return zero;
+    const V3 L = S.emitters[e].radiance;
+    float w = mis * F;
+    return v3(w * L.x, w * L.y, w * L.z);
+}
+
+/* The glint estimator (manifold_glints.cpp:28-223): SMS strategy, then the BSDF strategy,
+   which only an area emitter can receive (skipped in scenes without one) */
+static V3 estimate_glint_sms(const Scene &S, const sms_glint_query &q, const GlintSI &si, V3 sensor_p, const EI &ei,
+                             V3 edir, uint64_t seed, uint64_t qid, Stats &st);
 static V3 estimate_glint(const Scene &S, const sms_glint_query &q, uint64_t seed, uint64_t qid, Stats &st) {
-    const sms_config &cfg = S.cfg;
     V3 result = v3(0.f, 0.f, 0.f);
     if (q.shape < 0) return result;
     GlintSI si = gsi_from_query(q);
@@ -1835,7 +1900,16 @@ static V3 estimate_glint(const Scene &S, const sms_glint_query &q, uint64_t seed
     float edist = norm(edir);
     edir = edir * (1.f / edist);
     Ray re = make_ray(si.p, edir, RAY_EPS_F * (1.f + hmax_abs(si.p)), edist * (1.f - RAY_EPS_F));
-    if (ray_test(S, re)) return result;
+    if (!ray_test(S, re)) result = estimate_glint_sms(S, q, si, sensor_p, ei, edir, seed, qid, st);
+    if (scene_has_area_emitter(S)) result = result + glint_bsdf_strategy(S, si, seed, qid);
+    return result;
+}
+
+/* SMS strategy of the glint estimator (manifold_glints.cpp:50-181), emitter sample visible */
+static V3 estimate_glint_sms(const Scene &S, const sms_glint_query &q, const GlintSI &si, V3 sensor_p, const EI &ei,
+                             V3 edir, uint64_t seed, uint64_t qid, Stats &st) {
+    const sms_config &cfg = S.cfg;
+    V3 result = v3(0.f, 0.f, 0.f);
     V3 n_offset = v3(0.f, 0.f, 1.f);
     bool sms_success = false;
     V3 sms_weight = v3(0.f, 0.f, 0.f);
@@ -2751,6 +2825,13 @@ float oracle_glint_geometric(void *h, const sms_glint_query *q, const float *uv,
     MV vx, vtx, vy;
     if (!glint_vertices(S, v3(x0[0], x0[1], x0[2]), ei, gsi, v2(uv[0], uv[1]), &vx, &vtx, &vy)) return -1.f;
     return geometric_term_glint(vx, vtx, vy);
+}
+
+/* the glint BSDF-sampling strategy of query q (index qid) alone, with its parts (tests) */
+void oracle_glint_bsdf_parts(void *h, const sms_glint_query *q, uint64_t seed, uint64_t qid, float *rgb, float *parts) {
+    const Scene &S = *(Scene *) h;
+    V3 v = glint_bsdf_strategy(S, gsi_from_query(*q), seed, qid, parts);
+    rgb[0] = v.x; rgb[1] = v.y; rgb[2] = v.z;
 }
 
 void oracle_scene_box(void *h, float *c, float *inv) {

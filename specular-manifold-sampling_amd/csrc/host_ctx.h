@@ -39,6 +39,8 @@ struct sms_ctx {
     int mw_items = 0, mw_trials = 0;
     int occ_walker = 0;
     int occ_glint = 0, occ_ss = 0, occ_coop = 0, occ_biased = 0, occ_sscoop = 0, occ_mw = 0;   // occ_glint: trial pass
+    int occ_glint_biased = 0;
+    bool area_emitters = false;                 // glint BSDF-sampling strategy can contribute
     int occ_ms[2] = {0, 0}, occ_mscoop[2] = {0, 0}, occ_msbiased[2] = {0, 0};   // [0]: chains <= 2, [1]: <= 4
     // host copies the learned proposal's UV grid is built from
     std::vector<float2> h_vuv;

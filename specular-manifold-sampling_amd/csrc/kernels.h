@@ -136,6 +136,9 @@ __global__ void k_ss_walk(DScene S, const sms_ss_walk_in *in, int64_t n, sms_ss_
 __global__ void k_glint_walk(DScene S, const sms_glint_walk_in *in, int64_t n, sms_glint_walk_out *out);
 __global__ void k_glint_queries(DScene S, int spp, const uint64_t *seedp, sms_glint_query *q, int64_t n);
 __global__ void k_glint_first_bvh(DScene S, const sms_glint_query *Q, int64_t n, const uint64_t *seedp, float *out, EstimatorWork W);
+__global__ void k_glint_biased_bvh(DScene S, const sms_glint_query *Q, int64_t n, const uint64_t *seedp, float *out, EstimatorWork W);
+__global__ void k_glint_biased_nobvh(DScene S, const sms_glint_query *Q, int64_t n, const uint64_t *seedp, float *out, EstimatorWork W);
+__global__ void k_glint_bsdf(DScene S, const sms_glint_query *Q, int64_t n, const uint64_t *seedp, float *out);
 __global__ void k_glint_first_nobvh(DScene S, const sms_glint_query *Q, int64_t n, const uint64_t *seedp, float *out, EstimatorWork W);
 __global__ void k_glint_trials_g1(DScene S, const uint64_t *seedp, EstimatorWork W);
 __global__ void k_glint_trials_g2(DScene S, const uint64_t *seedp, EstimatorWork W);

@@ -563,4 +563,186 @@ __global__ void __launch_bounds__(256) k_glint_finalize(DScene S, const sms_glin
     }
 }
 
+// ---------------------------------------------------------------- biased estimator
+// Biased glint SMS (manifold_glints.cpp:112-140): one wavefront per query; lane m of a round
+// runs walk m of the trial set (stream (q, m + 1), as the unbiased trial k); converged walks
+// are de-duplicated in m order by uv distance through an LDS list of the solutions so far and
+// the contributions of the new ones are summed in m order.  MIS and sensor importance as the
+// unbiased finalize pass.
+constexpr int GLINT_BIASED_MAX_SOL = 128;
+template <bool BVH>
+SMS_DEV void glint_biased_body(const DScene &S, const sms_glint_query *__restrict__ Q, int64_t n, uint64_t seed,
+                               float *__restrict__ out, EstimatorWork W, float2 (*sol_lds)[GLINT_BIASED_MAX_SOL]) {
+    const sms_config &cfg = S.cfg;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    unsigned long long st_walks = 0, st_conv = 0, st_iters = 0;
+    const int64_t n_waves = ((int64_t) gridDim.x * blockDim.x) >> 6;
+    for (int64_t q = ((int64_t) blockIdx.x * blockDim.x + threadIdx.x) >> 6; q < n; q += n_waves) {
+        const sms_glint_query g = Q[q];
+        V3 result = v3(0.f, 0.f, 0.f);
+        if (g.shape >= 0 && g.shape < S.n_shapes && S.shapes[g.shape].bsdf == SMS_BSDF_NORMALMAP) {
+            SI si = glint_query_si(g);
+            const uint64_t qkey = query_key(seed, (uint64_t) q);
+            Pcg32 rq;
+            stream_seed(rq, qkey, 0);
+            EI ei = sample_emitter_interaction(S, si.p, S.emitters_all, S.n_emitters_all, rq);
+            V3 edir = ei.p - si.p;
+            float edist = norm(edir);
+            edir = edir * (1.f / edist);
+            const bool occluded =
+                ray_test_t<BVH>(S, make_ray(si.p, edir, RAY_EPS_F * (1.f + hmax_abs(si.p)), edist * (1.f - RAY_EPS_F)));
+            if (!occluded) {
+                const V3 sensor_p = ld3(g.sensor_p);
+                GlintFootprint fp;
+                // sample_glint's side test fails every walk of the set before its solve
+                const bool solvable = glint_target(sensor_p, ei.p, si.p, si.sh_s, si.sh_t, si.sh_n, v3(0.f, 0.f, 1.f), fp.target);
+                if (solvable) {
+                    footprint_texels(S, g.shape, fp);
+                    fp.uv = v2(g.uv[0], g.uv[1]);
+                    fp.duv_dx = v2(g.duv_dx[0], g.duv_dx[1]);
+                    fp.duv_dy = v2(g.duv_dy[0], g.duv_dy[1]);
+                    footprint_setup(fp);
+                }
+                float sum = 0.f;
+                int n_sol = 0;
+                for (int base = 0; base < cfg.max_trials; base += 64) {
+                    const int m = base + lane;
+                    bool cand = false;
+                    V2 uvf = v2(0.f, 0.f);
+                    if (m < cfg.max_trials) {
+                        st_walks++;
+                        if (solvable) {
+                            Pcg32 rw;
+                            stream_seed(rw, qkey, (uint64_t) m + 1);
+                            V2 xi;
+                            xi.x = pcg32_float(rw);
+                            xi.y = pcg32_float(rw);
+                            GlintWalk w;
+                            glint_walk_start(fp, xi, w);
+                            int stw;
+                            while ((stw = glint_walk_iter(fp, cfg, w)) == WALK_RUNNING) {}
+                            st_iters += (unsigned long long) w.it;
+                            if (stw == WALK_CONVERGED) {
+                                st_conv++;
+                                cand = true;
+                                uvf = w.uv;
+                            }
+                        }
+                    }
+                    // in-order de-duplication against the solution list
+                    bool unique = false;
+                    unsigned long long cmask = __ballot(cand);
+                    while (cmask) {
+                        const int j = __ffsll((long long) cmask) - 1;
+                        cmask &= cmask - 1ull;
+                        const V2 uj = v2(__shfl(uvf.x, j, 64), __shfl(uvf.y, j, 64));
+                        bool dup = false;
+                        for (int s0 = 0; s0 < n_sol; s0 += 64) {
+                            const int s = s0 + lane;
+                            bool d = false;
+                            if (s < n_sol) {
+                                const float2 o = sol_lds[wv][s];
+                                d = norm2(uj - v2(o.x, o.y)) < cfg.uniqueness_threshold;
+                            }
+                            if (__ballot(d)) { dup = true; break; }
+                        }
+                        if (!dup) {
+                            if (lane == 0) sol_lds[wv][n_sol] = make_float2(uj.x, uj.y);
+                            n_sol++;
+                            if (lane == j) unique = true;
+                        }
+                        __builtin_amdgcn_wave_barrier();
+                    }
+                    const float contrib = unique ? glint_contribution(S, sensor_p, ei, si, uvf) : 0.f;
+                    unsigned long long umask = __ballot(unique);
+                    while (umask) {
+                        const int j = __ffsll((long long) umask) - 1;
+                        umask &= umask - 1ull;
+                        sum += __shfl(contrib, j, 64);
+                    }
+                }
+                if (n_sol > 0) {
+                    const float We = sensor_we(S.cam, sensor_p, si.p);
+                    const float p_sms = sms_emitter_pdf(S, ei, si.p);
+                    const V3 wo_l = v3(dot(edir, si.sh_s), dot(edir, si.sh_t), dot(edir, si.sh_n));
+                    float p_bsdf = lean_pdf(S, g.shape, v2(g.uv[0], g.uv[1]), v2(g.duv_dx[0], g.duv_dx[1]),
+                                            v2(g.duv_dy[0], g.duv_dy[1]), ld3(g.wi), wo_l);
+                    p_bsdf += 1.f;
+                    float mis = mis_power(p_sms, p_bsdf);
+                    if (cfg.bsdf_strategy_only) mis = 0.f;
+                    result = glint_result(sum, 1.f, mis * We, ei.weight);
+                }
+            }
+        }
+        if (lane == 0) write_rgb(out, q, result);
+    }
+    stat_add(W.stats + 0, st_walks); stat_add(W.stats + 1, st_conv); stat_add(W.stats + 2, st_iters);
+}
+
+#define SMS_GLINT_BIASED_KERNEL(NAME, BVH)                                                                   \
+    __global__ void __launch_bounds__(256) NAME(DScene S, const sms_glint_query *__restrict__ Q, int64_t n,  \
+                                                const uint64_t *__restrict__ seedp, float *__restrict__ out, \
+                                                EstimatorWork W) {                                           \
+        __shared__ float2 sol_lds[4][GLINT_BIASED_MAX_SOL];                                                  \
+        glint_biased_body<BVH>(S, Q, n, *seedp, out, W, sol_lds);                                            \
+    }
+SMS_GLINT_BIASED_KERNEL(k_glint_biased_bvh, true)
+SMS_GLINT_BIASED_KERNEL(k_glint_biased_nobvh, false)
+
+// ---------------------------------------------------------------- BSDF-sampling strategy
+// manifold_glints.cpp:183-220, one lane per query, added to the SMS strategy's result in out:
+// an offset uv inside the pixel footprint (PCG32 stream GLINT_BSDF_STREAM of the query), the
+// nested smooth conductor's mirror direction in the normal map's level-0 frame there
+// (bsdfs/normalmap.cpp:118-136, conductor.cpp:217-275), kept only if it stays on the same side
+// of the unperturbed frame, one ray (Interaction::spawn_ray); a hit on the front of an area
+// emitter adds mis(p_bsdf, p_sms) * F * L (area.cpp:71-79).  Only an area emitter can receive
+// this strategy, so the host launches it for scenes that have one.
+constexpr uint64_t GLINT_BSDF_STREAM = 0xFFFFFFFEull;
+__global__ void __launch_bounds__(256) k_glint_bsdf(DScene S, const sms_glint_query *__restrict__ Q, int64_t n,
+                                                    const uint64_t *__restrict__ seedp, float *__restrict__ out) {
+    const int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const sms_glint_query g = Q[q];
+    if (g.shape < 0 || g.shape >= S.n_shapes || S.shapes[g.shape].bsdf != SMS_BSDF_NORMALMAP) return;
+    Pcg32 r;
+    stream_seed(r, query_key(*seedp, (uint64_t) q), GLINT_BSDF_STREAM);
+    const float fx = pcg32_float(r), fy = pcg32_float(r);
+    const V2 duv_dx = v2(g.duv_dx[0], g.duv_dx[1]), duv_dy = v2(g.duv_dy[0], g.duv_dy[1]);
+    const V2 duv = (fx - 0.5f) * duv_dx + (fy - 0.5f) * duv_dy;
+    SI so = glint_query_si(g);
+    so.uv = so.uv + duv;
+    const MV f = mv_from_si(S, so);                       // NormalmapBSDF::frame at the offset uv
+    const V3 wi = ld3(g.wi);
+    const V3 wi_w = to_world(so.sh_s, so.sh_t, so.sh_n, wi);
+    const V3 wi_p = v3(dot(wi_w, f.s), dot(wi_w, f.t), dot(wi_w, f.n));
+    if (!(wi_p.z > 0.f)) return;
+    const V3 wo_p = v3(-wi_p.x, -wi_p.y, wi_p.z);
+    const DShape &sh = S.shapes[g.shape];
+    const float F = fresnel_conductor(wi_p.z, sh.eta_r, sh.k);
+    const V3 wo_w = to_world(f.s, f.t, f.n, wo_p);
+    const V3 wo = v3(dot(wo_w, so.sh_s), dot(wo_w, so.sh_t), dot(wo_w, so.sh_n));
+    if (!(wo_p.z * wo.z > 0.f)) return;
+    const V3 d = to_world(so.sh_s, so.sh_t, so.sh_n, wo);
+    const SI h = ray_intersect(S, spawn_ray(so.p, d));
+    if (!h.valid) return;
+    const int e = S.shapes[h.shape].emitter;
+    if (e < 0 || S.emitters[e].type != SMS_EMITTER_AREA) return;
+    float p_bsdf = lean_pdf(S, g.shape, so.uv, duv_dx, duv_dy, wi, wo);
+    p_bsdf += 1.f;
+    EI eh;
+    eh.p = h.p; eh.n = h.sh_n;
+    const V3 dh = h.p - so.p;
+    eh.d = dh * (1.f / norm(dh));
+    eh.weight = v3(0.f, 0.f, 0.f); eh.pdf = 0.f; eh.kind = SMS_EI_AREA; eh.emitter = e;
+    const float p_sms = sms_emitter_pdf(S, eh, so.p);
+    float mis = mis_power(p_bsdf, p_sms);
+    if (S.cfg.sms_strategy_only) mis = 0.f;
+    if (!(dot(d * -1.f, h.sh_n) > 0.f)) return;
+    const DEmitter &em = S.emitters[e];
+    const float w = mis * F;
+    out[3 * q] = out[3 * q] + w * em.radiance[0];
+    out[3 * q + 1] = out[3 * q + 1] + w * em.radiance[1];
+    out[3 * q + 2] = out[3 * q + 2] + w * em.radiance[2];
+}
+
 } // namespace smsdev

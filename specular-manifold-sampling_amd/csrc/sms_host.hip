@@ -280,6 +280,7 @@ sms_status build_scene(sms_ctx *ctx, const sms_scene_desc *desc, const sms_confi
                                            desc->shapes[e.shape].type != SMS_SHAPE_SPHERE))
             return fail(ctx, SMS_E_INVALID, "area emitters must sit on a sphere shape");
         emitters.push_back(d);
+        if (e.type == SMS_EMITTER_AREA) ctx->area_emitters = true;
         em_all.push_back(i);
         if (e.caustic_ss) em_ss.push_back(i);
         if (e.caustic_ms) em_ms.push_back(i);
@@ -711,6 +712,7 @@ sms_status sms_ctx_create(const sms_scene_desc *scene, const sms_config *cfg, sm
     if (!st) st = occupancy(ctx, k_ss_walker, &ctx->occ_walker);
     if (!st) st = occupancy(ctx, k_ss_coop, &ctx->occ_sscoop);
     if (!st) st = occupancy(ctx, k_ss_biased, &ctx->occ_biased);
+    if (!st) st = occupancy(ctx, k_glint_biased_bvh, &ctx->occ_glint_biased);
     if (!st) st = occupancy(ctx, k_ms_estimate_2, &ctx->occ_ms[0]);
     if (!st) st = occupancy(ctx, k_ms_estimate_4, &ctx->occ_ms[1]);
     if (!st) st = occupancy(ctx, k_ms_coop_2, &ctx->occ_mscoop[0]);
@@ -855,8 +857,10 @@ sms_status sms_glint_make_queries(sms_ctx *ctx, int32_t spp, uint64_t seed, sms_
 
 sms_status glint_prepare(sms_ctx *ctx, const sms_glint_query *q, int64_t n, const float *out_rgb) {
     if (!ctx || n < 0 || (n > 0 && (!q || !out_rgb))) return fail(ctx, SMS_E_INVALID, "bad estimator arguments");
-    if (ctx->scene.cfg.biased) return fail(ctx, SMS_E_UNSUPPORTED, "biased glint estimator is not built");
+    if (ctx->scene.cfg.biased && ctx->scene.cfg.max_trials > 128)
+        return fail(ctx, SMS_E_UNSUPPORTED, "biased trial sets above 128 are not built");
     if (!ctx->scene.nmaps) return fail(ctx, SMS_E_INVALID, "scene has no normal map");
+    if (ctx->scene.cfg.biased) return ensure_work(ctx, 0, 0);
     sms_status st = ensure_work(ctx, (size_t) std::max<int64_t>(n, 1), 0);
     if (!st) st = ensure_glint_work(ctx, (size_t) std::max<int64_t>(n, 1));
     return st;
@@ -867,7 +871,21 @@ sms_status glint_prepare(sms_ctx *ctx, const sms_glint_query *q, int64_t n, cons
 sms_status glint_enqueue(sms_ctx *ctx, const sms_glint_query *q, int64_t n, float *out_rgb) {
     const uint64_t *seed = ctx->seeds + SEED_EST;
     EV_RECORD(ctx, ctx->ev0);
-    if (n > 0) {
+    if (n > 0 && ctx->scene.cfg.biased) {
+        EstimatorWork W = work_of(ctx);
+        const dim3 grid(grid_for(ctx, ctx->occ_glint_biased));
+        if (ctx->scene.n_nodes > 0)
+            hipLaunchKernelGGL(k_glint_biased_bvh, grid, dim3(256), 0, ctx->stream, ctx->scene, q, n, seed, out_rgb, W);
+        else
+            hipLaunchKernelGGL(k_glint_biased_nobvh, grid, dim3(256), 0, ctx->stream, ctx->scene, q, n, seed, out_rgb, W);
+        HIP_TRY(ctx, hipGetLastError());
+        EV_RECORD(ctx, ctx->ev1);
+        if (ctx->area_emitters) {
+            hipLaunchKernelGGL(k_glint_bsdf, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream, ctx->scene, q,
+                               n, seed, out_rgb);
+            HIP_TRY(ctx, hipGetLastError());
+        }
+    } else if (n > 0) {
         EstimatorWork W = work_of(ctx);
         const unsigned blocks = (unsigned) ((n + 255) / 256);
         if (ctx->scene.n_nodes > 0)
@@ -896,6 +914,11 @@ sms_status glint_enqueue(sms_ctx *ctx, const sms_glint_query *q, int64_t n, floa
         hipLaunchKernelGGL(k_glint_finalize, dim3(4 * ctx->n_cu), dim3(256), 0, ctx->stream, ctx->scene, q, seed,
                            out_rgb, W);
         HIP_TRY(ctx, hipGetLastError());
+        if (ctx->area_emitters) {
+            hipLaunchKernelGGL(k_glint_bsdf, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream, ctx->scene, q,
+                               n, seed, out_rgb);
+            HIP_TRY(ctx, hipGetLastError());
+        }
     } else {
         EV_RECORD(ctx, ctx->ev1);
     }

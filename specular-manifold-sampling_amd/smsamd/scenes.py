@@ -338,13 +338,20 @@ def scratch_bump_normalmap(res=2048, n_scratches=4096, n_bumps=None, seed=1, slo
     return rgb
 
 
-def glint_plane(width=512, height=512, nm_res=2048, n_scratches=4096, seed=1, normalmap=None):
-    """1x1 normal-mapped mirror plane (conductor eta=0, k=1 -> F=1) at y=0 under a point light."""
+def glint_plane(width=512, height=512, nm_res=2048, n_scratches=4096, seed=1, normalmap=None, area_light=None,
+                point_light=True, eta=0.0, k=1.0):
+    """1x1 normal-mapped mirror plane (conductor eta=0, k=1 -> F=1) at y=0 under a point light;
+    area_light = (centre, radius, radiance) adds a spherical area light (diffuse sphere)."""
     sc = Scene()
     rgb = normalmap if normalmap is not None else scratch_bump_normalmap(nm_res, n_scratches, seed=seed)
     nm = sc.add_normalmap(rgb)
     sc.add_rectangle(plane_to_world((0, 0, 0), 0.5), abi.BSDF_NORMALMAP, flags=abi.FLAG_GLINTY,
-                     eta=0.0, k=1.0, normalmap=nm, tiles=1.0)
-    sc.add_point_light((0.35, 1.2, -0.3), (3.0, 3.0, 3.0))
+                     eta=eta, k=k, normalmap=nm, tiles=1.0)
+    if point_light:
+        sc.add_point_light((0.35, 1.2, -0.3), (3.0, 3.0, 3.0))
+    if area_light is not None:
+        centre, radius, radiance = area_light
+        light = sc.add_sphere(centre, radius, abi.BSDF_DIFFUSE)
+        sc.add_area_light(light, radiance)
     sc.set_camera((0.0, 0.95, 0.9), (0.0, 0.0, 0.02), (0, 1, 0), 40.0, width, height)
     return sc

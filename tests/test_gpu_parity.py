@@ -86,6 +86,58 @@ def test_glint_walks(gpu, oracle_mod):
     orc.close()
 
 
+@pytest.mark.parametrize("max_trials", [1, 40, 128])
+def test_glint_biased_estimate(gpu, oracle_mod, max_trials):
+    """Biased glint estimator: trial sets of max_trials walks, uv de-duplication in walk order."""
+    sc = scenes.glint_plane(64, 48, nm_res=256, n_scratches=64, seed=3)
+    cfg = abi.SmsConfig.default(biased=1, max_trials=max_trials)
+    ctx = gpu.Context(sc, cfg)
+    orc = oracle_mod.Oracle(sc, cfg)
+    q = orc.glint_queries(2, 5)
+    rg, sg = ctx.glint_estimate(q, 9)
+    ro, so = orc.glint_estimate(q, 9)
+    print(max_trials, "gpu", sg, "oracle", so, flush=True)
+    assert (ro[:, 0] > 0).sum() > 20
+    assert np.array_equal(rg, ro, equal_nan=True)
+    for k in ("walks", "walks_converged", "newton_iterations", "bernoulli_calls"):
+        assert sg[k] == so[k], k
+    if max_trials == 128:
+        with pytest.raises(gpu.SmsError):
+            ctx.set_config(abi.SmsConfig.default(biased=1, max_trials=129))
+            ctx.glint_estimate(q, 9)
+    ctx.close()
+    orc.close()
+
+
+@pytest.mark.parametrize("biased", [0, 1])
+def test_glint_area_light_estimate(gpu, oracle_mod, biased):
+    """Point light plus a spherical area light seen in the mirror: the SMS strategy samples
+    either emitter, and the BSDF-sampling strategy (manifold_glints.cpp:183-220) adds the area
+    light where a footprint's mirror direction reaches it."""
+    area = ((0.0, 1.1, -1.05), 0.35, (4.0, 3.0, 2.0))
+    sc = scenes.glint_plane(64, 48, nm_res=256, n_scratches=64, seed=3, area_light=area)
+    cfg = abi.SmsConfig.default(biased=biased, max_trials=32 if biased else 100000)
+    ctx = gpu.Context(sc, cfg)
+    orc = oracle_mod.Oracle(sc, cfg)
+    q = orc.glint_queries(2, 5)
+    rg, sg = ctx.glint_estimate(q, 9)
+    ro, so = orc.glint_estimate(q, 9)
+    print(biased, "gpu", sg, "oracle", so, flush=True)
+    assert np.array_equal(rg, ro, equal_nan=True)
+    for k in ("walks", "walks_converged", "newton_iterations", "bernoulli_calls", "bernoulli_trials", "bernoulli_max"):
+        assert sg[k] == so[k], k
+    # without the BSDF strategy's share the result changes where the mirror sees the light
+    cfg2 = abi.SmsConfig.default(biased=biased, max_trials=cfg.max_trials, sms_strategy_only=1)
+    ctx.set_config(cfg2)
+    orc.set_config(cfg2)
+    rg2, _ = ctx.glint_estimate(q, 9)
+    ro2, _ = orc.glint_estimate(q, 9)
+    assert np.array_equal(rg2, ro2, equal_nan=True)
+    assert (rg2[:, 0] != rg[:, 0]).sum() > 20
+    ctx.close()
+    orc.close()
+
+
 def test_glint_queries_and_estimate(gpu, oracle_mod):
     sc = scenes.glint_plane(64, 48, nm_res=256, n_scratches=64, seed=3)
     cfg = abi.SmsConfig.default(max_trials=100000)

@@ -226,3 +226,127 @@ def test_glint_geometric_term_matches_finite_differences():
     assert len(errs) >= 15
     assert np.median(errs) < 1e-4
     orc.close()
+
+
+def test_glint_biased_and_unbiased_estimators_agree():
+    """The biased glint estimator (manifold_glints.cpp:112-140) sums the contributions of the
+    distinct solutions a trial set of M walks finds; with M large enough to find every solution
+    of a footprint its mean equals the unbiased Bernoulli estimator's (:70-111)."""
+    sc = scenes.glint_plane(32, 32, nm_res=256, n_scratches=64, seed=3)
+    ob = oracle_lib.Oracle(sc, abi.SmsConfig.default(biased=1, max_trials=128))
+    q = ob.glint_queries(4, 5)
+    rb, sb = ob.glint_estimate(q, 1)
+    ou = oracle_lib.Oracle(sc, abi.SmsConfig.default(max_trials=100000))
+    runs = np.stack([ou.glint_estimate(q, s)[0][:, 0] for s in range(1, 9)])
+    mean_u, mean_b = runs.mean(), rb[:, 0].astype(np.float64).mean()
+    se = runs.mean(0).std() / math.sqrt(runs.shape[1])
+    print(f"unbiased {mean_u:.6g} biased {mean_b:.6g} se {se:.3g} walks {sb['walks']} converged {sb['walks_converged']}")
+    assert (rb[:, 0] > 0).sum() > 50
+    assert sb["walks"] > 128 * 50 and sb["bernoulli_calls"] == 0
+    assert abs(mean_u - mean_b) < 4 * se + 1e-3 * abs(mean_u)
+    ob.close()
+    ou.close()
+
+
+def _smooth_map(res=16):
+    g = (np.arange(res) + 0.5) / res
+    U, V = np.meshgrid(g, g, indexing="xy")
+    sl = np.stack([0.12 * np.sin(2 * np.pi * U) + 0.05 * V, 0.10 * np.cos(2 * np.pi * V) - 0.04 * U], -1)
+    n = np.concatenate([-sl, np.ones((res, res, 1))], -1)
+    n /= np.linalg.norm(n, axis=-1, keepdims=True)
+    return np.clip(np.round((n * 0.5 + 0.5) * 255), 0, 255).astype(np.uint8)
+
+
+def _bilinear_clamped(S, uv, res):
+    """Normalmap::eval_normal's texel-centre bilinear with edge clamp (normalmap.h:67-80)."""
+    x = min(max(uv[0] * res - 0.5, 0.0), res - 1.0)
+    y = min(max(uv[1] * res - 0.5, 0.0), res - 1.0)
+    i0, j0 = int(math.floor(x)), int(math.floor(y))
+    i1, j1 = min(i0 + 1, res - 1), min(j0 + 1, res - 1)
+    fx, fy = x - i0, y - j0
+    return (S[j0, i0] * (1 - fx) + S[j0, i1] * fx) * (1 - fy) + (S[j1, i0] * (1 - fx) + S[j1, i1] * fx) * fy
+
+
+AREA_LIGHT = ((0.0, 1.1, -1.05), 0.35, (4.0, 3.0, 2.0))
+
+
+def test_glint_bsdf_strategy_geometry():
+    """BSDF-sampling strategy of the glint estimator (manifold_glints.cpp:183-220) on a mirror
+    plane with a smooth normal map under a point light and a spherical area light.  An
+    independent FP64 restatement of its geometry -- the footprint offset, the level-0 bilinear
+    normal there, the mirror direction and its side tests, the ray against the light sphere --
+    must agree with the oracle on every query's direction and on whether it reaches the light;
+    where it does, the emitter density is the sphere's solid-angle cone density over the two
+    emitters, and the value is the power heuristic of the two densities times F (1 for this
+    conductor) times the radiance.  The full estimate is the SMS strategy plus this term."""
+    rgb = _smooth_map()
+    res = rgb.shape[0]
+    S = decode_slopes(rgb)
+    sc = scenes.glint_plane(48, 48, normalmap=rgb, area_light=AREA_LIGHT)
+    orc = oracle_lib.Oracle(sc, abi.SmsConfig.default(max_trials=100000))
+    lib = orc.lib
+    lib.oracle_glint_bsdf_parts.argtypes = [C_VP, C_VP, ctypes.c_uint64, ctypes.c_uint64, C_VP, C_VP]
+    q = orc.glint_queries(2, 7)
+    centre, radius, L = np.array(AREA_LIGHT[0]), AREA_LIGHT[1], np.array(AREA_LIGHT[2])
+    nz = lambda v: v / np.linalg.norm(v)
+    seed, hits, checked = 3, 0, 0
+    values = np.zeros((len(q), 3), np.float32)
+    for i in range(len(q)):
+        if q["shape"][i] < 0:
+            continue
+        out, parts = np.zeros(3, np.float32), np.zeros(10, np.float32)
+        lib.oracle_glint_bsdf_parts(orc.h, q[i:i + 1].ctypes.data, seed, i, out.ctypes.data, parts.ctypes.data)
+        values[i] = out
+        f64 = lambda k: q[k][i].astype(np.float64)
+        uv = f64("uv") + (parts[0] - 0.5) * f64("duv_dx") + (parts[1] - 0.5) * f64("duv_dy")
+        uv = uv - np.floor(uv)
+        s_, t_, n_ = f64("sh_s"), f64("sh_t"), f64("sh_n")
+        bn, dpu = n_, f64("dp_du")
+        bs = nz(dpu - bn * (bn @ dpu))
+        bt = np.cross(bn, bs)
+        sl = _bilinear_clamped(S, uv, res)
+        m = nz(-sl[0] * bs - sl[1] * bt + bn)
+        wi = f64("wi")
+        wi_w = s_ * wi[0] + t_ * wi[1] + n_ * wi[2]
+        if abs(wi_w @ m) < 1e-4:
+            continue
+        d = 2 * (wi_w @ m) * m - wi_w
+        reflected = wi_w @ m > 0 and d @ n_ > 0
+        if not reflected:
+            assert parts[5] == -1 and not out.any(), i
+            continue
+        checked += 1
+        assert np.allclose(parts[2:5], d, atol=2e-5), (i, parts[2:5], d)
+        p = f64("p")
+        oc = p - centre
+        b = oc @ d
+        disc = b * b - (oc @ oc - radius * radius)
+        if abs(disc) < 1e-4:                                   # grazing: FP32 may decide either way
+            continue
+        hit = disc > 0 and -b - math.sqrt(disc) > 0
+        assert (parts[5] == 1) == hit, (i, parts[5], disc)
+        if not hit:
+            assert not out.any()
+            continue
+        hits += 1
+        sin_a = radius / np.linalg.norm(centre - p)
+        p_sms = 1.0 / (2 * math.pi * (1 - math.sqrt(1 - sin_a * sin_a))) / 2.0
+        assert parts[8] == pytest.approx(p_sms, rel=1e-4)
+        assert parts[7] >= 1.0 and parts[6] == pytest.approx(1.0, abs=1e-6)
+        pb, ps = float(parts[7]), p_sms
+        expect = pb * pb / (pb * pb + ps * ps) * L
+        assert np.allclose(out, expect, rtol=1e-4), (i, out, expect)
+    print(f"queries {int((q['shape'] >= 0).sum())} reflected {checked} reaching the light {hits}")
+    assert checked > 500 and hits > 100
+    # the estimator adds the BSDF strategy to the SMS strategy's result
+    full, _ = orc.glint_estimate(q, seed)
+    orc.set_config(abi.SmsConfig.default(max_trials=100000, sms_strategy_only=1))
+    sms_only, _ = orc.glint_estimate(q, seed)
+    zero_bsdf = np.zeros_like(values)
+    for i in np.nonzero(values.any(1))[0][:50]:
+        out, parts = np.zeros(3, np.float32), np.zeros(10, np.float32)
+        lib.oracle_glint_bsdf_parts(orc.h, q[i:i + 1].ctypes.data, seed, int(i), out.ctypes.data, parts.ctypes.data)
+        zero_bsdf[i] = out                                     # mis = 0 under sms_strategy_only
+    assert not zero_bsdf.any()
+    assert np.array_equal(full, sms_only + values)
+    orc.close()
