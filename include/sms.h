@@ -47,7 +47,7 @@ typedef int32_t sms_status;
 /* Field-for-field mirror of SMSConfig (include/mitsuba/render/manifold.h:11-28). */
 typedef struct sms_config {
     int32_t biased;                  /* biased (trial set of size max_trials) vs unbiased SMS  */
-    int32_t twostage;                /* two-stage solver for normal maps (not built: must be 0) */
+    int32_t twostage;                /* single scatter: LEAN-smoothed pre-solve on normal maps  */
     int32_t halfvector_constraints;  /* half-vector constraint instead of angle difference     */
     int32_t mnee_init;               /* deterministic MNEE seed instead of a random one        */
     float   step_scale;              /* Newton step scale                                      */
@@ -166,6 +166,7 @@ typedef struct sms_ss_walk_in {
     float n_offset[3];         /* (0,0,1) for smooth casters                               */
     float sample[2];           /* 2D sample for Shape::sample_position                     */
     float p_start[3];          /* seed override                                            */
+    float lean_sample[2];      /* twostage: 2D sample of the LEAN offset slope             */
 } sms_ss_walk_in;
 
 typedef struct sms_ss_walk_out {

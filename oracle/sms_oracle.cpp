@@ -688,8 +688,10 @@ static void shading_frame_derivative(V3 n, V3 dp_du, V3 dn_du, V3 dn_dv, MV &v) 
 
 static inline V3 to_world(V3 s, V3 t, V3 n, V3 v) { return (s * v.x + t * v.y) + n * v.z; }
 
-/* ManifoldVertex(si, smoothing=0) (include/mitsuba/render/manifold.h:89-113) */
-static MV mv_from_si(const Scene &S, const SI &si) {
+/* ManifoldVertex(si, smoothing) (include/mitsuba/render/manifold.h:89-113) for smoothing 0 and
+   1.  Smoothing 1 evaluates a normal map at its coarsest mip level (normalmap.h:236-268 with
+   level L - 1): one texel, so the normal is the map's mean slope and its derivatives vanish. */
+static MV mv_from_si(const Scene &S, const SI &si, bool smooth = false) {
     MV v = mv_point(si.p);
     v.dp_du = si.dp_du; v.dp_dv = si.dp_dv; v.gn = si.n; v.uv = si.uv; v.shape = si.shape;
     const Shape &sh = S.shapes[si.shape];
@@ -706,11 +708,18 @@ static MV mv_from_si(const Scene &S, const SI &si) {
     } else {
         /* NormalmapBSDF::frame / frame_derivative at smoothing 0 (src/bsdfs/normalmap.cpp:218-290) */
         const Normalmap &nm = S.nmaps[sh.normalmap];
-        V2 uv = tile_uv(si.uv, sh.tiles);
-        Texel4 tx = nm_fetch(nm, uv);
-        V2 sl = nm_slope(tx);
+        V2 sl, tu, tv;
+        if (smooth) {
+            const std::vector<float> &top = nm.lean.back();
+            sl = v2(top[0], top[1]);
+            tu = v2(0.f, 0.f); tv = v2(0.f, 0.f);
+        } else {
+            V2 uv = tile_uv(si.uv, sh.tiles);
+            Texel4 tx = nm_fetch(nm, uv);
+            sl = nm_slope(tx);
+            nm_slope_deriv(tx, (float) nm.res, &tu, &tv);
+        }
         V3 nl = v3(-sl.x, -sl.y, 1.f);
-        V2 tu, tv; nm_slope_deriv(tx, (float) nm.res, &tu, &tv);
         V3 dn_du = v3(-tu.x, -tu.y, 0.f) * sh.tiles, dn_dv = v3(-tv.x, -tv.y, 0.f) * sh.tiles;
         V3 rn = normalize(to_world(bs, bt, bn, nl));
         v.n = rn;
@@ -1063,7 +1072,7 @@ static Step step_halfvector(V3 v0p, const MV &v1, const EI &v2e, V3 n_offset) {
 struct NewtonOut { bool ok; SI si; int iterations; };
 
 /* newton_solver SS (manifold_ss.cpp:329-405) */
-static NewtonOut newton_ss(const Scene &S, V3 si_p, const MV &vtx_init, const EI &ei, V3 n_offset) {
+static NewtonOut newton_ss(const Scene &S, V3 si_p, const MV &vtx_init, const EI &ei, V3 n_offset, bool smooth = false) {
     const sms_config &cfg = S.cfg;
     MV vtx = vtx_init;
     bool success = false;
@@ -1085,7 +1094,7 @@ static NewtonOut newton_ss(const Scene &S, V3 si_p, const MV &vtx_init, const EI
             continue;
         }
         beta = std::min(1.f, 2.f * beta);
-        vtx = mv_from_si(S, si_current);
+        vtx = mv_from_si(S, si_current, smooth);
         iterations++;
     }
     NewtonOut out; out.iterations = iterations; out.si = si_current;
@@ -1101,8 +1110,42 @@ static NewtonOut newton_ss(const Scene &S, V3 si_p, const MV &vtx_init, const EI
 
 struct PathOut { bool ok; SI si_final, si_init; int iterations; };
 
-/* sample_path SS (manifold_ss.cpp:203-279), uniform seed (twostage not built) */
-static PathOut sample_path_ss(const Scene &S, int shape, V3 si_p, const EI &ei, V2 sample, V3 n_offset, V3 p_start) {
+static inline float logf_l(float x);
+static void lean_moments(const Normalmap &nm, V2 uv, int level, V2 *mu, float sig[3]);
+
+/* SpecularManifold::sample_gaussian (manifold.h:328-345) with warp::square_to_std_normal
+   (warp.h:184-190); sigma = (s00, s11, s01) */
+static V2 sample_gaussian(V2 mu, const float sig[3], V2 s) {
+    const float r = sqrtf(-2.f * logf_l(1.f - s.x));
+    float sn, cs;
+    c_sincosf((2.f * PI_F) * s.y, &sn, &cs);
+    V2 p = v2(cs * r, sn * r);
+    const float sx = sqrtf(sig[0]), sy = sqrtf(sig[1]);
+    const float rho = sig[2] / (sx * sy);
+    const float a = sqrtf(1.f + rho), b = sqrtf(1.f - rho);
+    const float d = a * 0.5f + b * 0.5f, o = a * 0.5f + b * -0.5f;     /* A = a P + b Q */
+    V2 q = v2(d * p.x + o * p.y, o * p.x + d * p.y);
+    q.x = q.x * sx; q.y = q.y * sy;
+    return q + mu;
+}
+
+/* BSDF::lean at a surface interaction without uv partials: NormalmapBSDF::lean
+   (bsdfs/normalmap.cpp:292-303) at the coarsest mip level (mipmap_level, :361-364); (0, 0)
+   for other BSDFs (bsdf.cpp:34-38) */
+static void lean_coarsest(const Scene &S, const SI &si, V2 *mu, float sig[3]) {
+    const Shape &sh = S.shapes[si.shape];
+    if (sh.bsdf != SMS_BSDF_NORMALMAP) { *mu = v2(0.f, 0.f); sig[0] = sig[1] = sig[2] = 0.f; return; }
+    const Normalmap &nm = S.nmaps[sh.normalmap];
+    V2 uv = si.uv * sh.tiles;
+    uv = v2(uv.x - floorf(uv.x), uv.y - floorf(uv.y));
+    lean_moments(nm, uv, (int) nm.lean.size() - 1, mu, sig);
+}
+
+/* sample_path SS (manifold_ss.cpp:203-279): uniform seed, optional MNEE direction, and with
+   config.twostage the LEAN-smoothed pre-solve (:251-270) from an offset slope drawn with
+   lean_sample; iterations count both solves */
+static PathOut sample_path_ss(const Scene &S, int shape, V3 si_p, const EI &ei, V2 sample, V3 n_offset, V3 p_start,
+                              V2 lean_sample) {
     PathOut out; out.ok = false; out.si_final = si_invalid(); out.si_init = si_invalid(); out.iterations = 0;
     PS ps = sample_position(S, shape, sample);
     if (p_start.x != 0.f || p_start.y != 0.f || p_start.z != 0.f) ps.p = p_start;
@@ -1117,9 +1160,22 @@ static PathOut sample_path_ss(const Scene &S, int shape, V3 si_p, const EI &ei, 
     SI si_init = ray_intersect(S, ray);
     if (!si_init.valid || shape != si_init.shape) return out;
     out.si_init = si_init;
-    MV vtx_init = mv_from_si(S, si_init);
+    const bool twostage = S.cfg.twostage != 0;
+    MV vtx_init = mv_from_si(S, si_init, twostage);
+    int smooth_iterations = 0;
+    if (twostage) {
+        V2 mu_offset = v2(-n_offset.x / n_offset.z, -n_offset.y / n_offset.z);
+        V2 mu; float sig[3];
+        lean_coarsest(S, si_init, &mu, sig);
+        V2 slope = sample_gaussian(0.5f * (mu + mu_offset), sig, lean_sample);
+        V3 lean_normal = normalize(v3(-slope.x, -slope.y, 1.f));
+        NewtonOut sm = newton_ss(S, si_p, vtx_init, ei, lean_normal, true);
+        smooth_iterations = sm.iterations;
+        if (!(sm.ok && sm.si.valid)) { out.iterations = smooth_iterations; return out; }
+        vtx_init = mv_from_si(S, sm.si);
+    }
     NewtonOut nr = newton_ss(S, si_p, vtx_init, ei, n_offset);
-    out.iterations = nr.iterations;
+    out.iterations = smooth_iterations + nr.iterations;
     out.si_final = nr.si;
     out.ok = nr.ok && nr.si.valid;
     return out;
@@ -1397,7 +1453,9 @@ static PathOut proposal_walk(const Scene &S, const Prop &P, const float *g, int 
         if (!uv_to_point(S, P.grid, s, &p_start)) return po;
         s = v2(0.f, 0.f);
     }
-    return sample_path_ss(S, shape, si_p, ei, s, v3(0.f, 0.f, 1.f), p_start);
+    V2 ls = v2(0.f, 0.f);
+    if (S.cfg.twostage) { ls.x = pcg32_float(rw); ls.y = pcg32_float(rw); }
+    return sample_path_ss(S, shape, si_p, ei, s, v3(0.f, 0.f, 1.f), p_start, ls);
 }
 
 /* specular_manifold_sampling SS (manifold_ss.cpp:28-201), per-(query, walk) PCG32 streams */
@@ -1425,7 +1483,9 @@ static V3 estimate_ss(const Scene &S, const sms_ss_query &q, uint64_t seed, uint
                 po = proposal_walk(S, *P, gq, shape, si.p, ei, &rw);
             } else {
                 V2 s; s.x = pcg32_float(&rw); s.y = pcg32_float(&rw);
-                po = sample_path_ss(S, shape, si.p, ei, s, n_offset, v3(0.f, 0.f, 0.f));
+                V2 ls = v2(0.f, 0.f);
+                if (cfg.twostage) { ls.x = pcg32_float(&rw); ls.y = pcg32_float(&rw); }
+                po = sample_path_ss(S, shape, si.p, ei, s, n_offset, v3(0.f, 0.f, 0.f), ls);
             }
             st.walks++; st.iters += po.iterations; if (po.ok) st.conv++;
             return po;
@@ -2686,7 +2746,7 @@ int oracle_ss_walk_batch(void *h, const sms_ss_walk_in *in, int64_t n, sms_ss_wa
         EI ei = ei_from_c(w.ei);
         PathOut po = sample_path_ss(S, w.shape, v3(w.si_p[0], w.si_p[1], w.si_p[2]), ei, v2(w.sample[0], w.sample[1]),
                                     v3(w.n_offset[0], w.n_offset[1], w.n_offset[2]),
-                                    v3(w.p_start[0], w.p_start[1], w.p_start[2]));
+                                    v3(w.p_start[0], w.p_start[1], w.p_start[2]), v2(w.lean_sample[0], w.lean_sample[1]));
         sms_ss_walk_out &o = out[i];
         memset(&o, 0, sizeof(o));
         o.success = po.ok; o.iterations = po.iterations;
@@ -2832,6 +2892,12 @@ void oracle_glint_bsdf_parts(void *h, const sms_glint_query *q, uint64_t seed, u
     const Scene &S = *(Scene *) h;
     V3 v = glint_bsdf_strategy(S, gsi_from_query(*q), seed, qid, parts);
     rgb[0] = v.x; rgb[1] = v.y; rgb[2] = v.z;
+}
+
+/* SpecularManifold::sample_gaussian of the two-stage solver (tests): mu[2], sig = (s00, s11, s01) */
+void oracle_sample_gaussian(const float *mu, const float *sig, const float *s, float *out) {
+    V2 r = sample_gaussian(v2(mu[0], mu[1]), sig, v2(s[0], s[1]));
+    out[0] = r.x; out[1] = r.y;
 }
 
 void oracle_scene_box(void *h, float *c, float *inv) {

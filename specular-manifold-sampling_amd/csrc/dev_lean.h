@@ -59,6 +59,7 @@ SMS_DEV double log2_d(double x) {
 SMS_DEV float expf_l(float x) { return (float) exp_d((double) x); }
 SMS_DEV float erff_l(float x) { return (float) erf_d((double) x); }
 SMS_DEV float log2f_l(float x) { return (float) log2_d((double) x); }
+SMS_DEV float logf_l(float x) { return (float) (log2_d((double) x) * 0.69314718055994530942); }
 
 constexpr float INV_SQRT_PI_F = 0.56418958354775628695f;
 constexpr float INV_TWOPI_F = 0.15915494309189533577f;

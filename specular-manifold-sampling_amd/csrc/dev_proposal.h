@@ -36,7 +36,6 @@ SMS_DEV float tanh_l(float x) {
     float r = (1.f - t) / (1.f + t);
     return x < 0.f ? -r : r;
 }
-SMS_DEV float logf_l(float x) { return (float) (log2_d((double) x) * 0.69314718055994530942); }
 
 // raw network outputs -> mixture parameters used by the seeds
 SMS_DEV void gmm_prepare(const float *o, float *g) {

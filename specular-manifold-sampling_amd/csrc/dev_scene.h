@@ -451,7 +451,9 @@ SMS_DEV void shading_frame_derivative(V3 n, V3 dp_du, V3 dn_du, V3 dn_dv, V3 &ds
 
 // ManifoldVertex(si, smoothing = 0) (include/mitsuba/render/manifold.h:89-113) with the BSDF
 // frame / frame_derivative of bsdf.cpp:17-31 or of the normal-map BSDF (bsdfs/normalmap.cpp:218-290)
-SMS_DEV MV mv_from_si(const DScene &S, const SI &si) {
+// smooth: ManifoldVertex(si, smoothing = 1) -- a normal map at its coarsest mip level (one
+// texel: the mean slope, zero derivatives; normalmap.h:236-268); otherwise smoothing 0
+SMS_DEV MV mv_from_si(const DScene &S, const SI &si, bool smooth = false) {
     MV v = mv_point(si.p);
     v.dp_du = si.dp_du; v.dp_dv = si.dp_dv; v.gn = si.n; v.shape = si.shape;
     const DShape &sh = S.shapes[si.shape];
@@ -467,12 +469,18 @@ SMS_DEV MV mv_from_si(const DScene &S, const SI &si) {
         v.dn_du = sdn_du; v.dn_dv = sdn_dv;
     } else {
         const DNormalmap nm = S.nmaps[sh.normalmap];
-        V2 uv = tile_uv(si.uv, sh.tiles);
-        Texel4 tx = nm_fetch(nm.slope, nm.res, nm.stride, uv);
-        V2 sl = nm_slope(tx);
+        V2 sl, tu, tv;
+        if (smooth) {
+            const float *top = nm.lean + nm.lean_off[nm.levels - 1];
+            sl = v2(top[0], top[1]);
+            tu = v2(0.f, 0.f); tv = v2(0.f, 0.f);
+        } else {
+            V2 uv = tile_uv(si.uv, sh.tiles);
+            Texel4 tx = nm_fetch(nm.slope, nm.res, nm.stride, uv);
+            sl = nm_slope(tx);
+            nm_slope_deriv(tx, (float) nm.res, tu, tv);
+        }
         V3 nl = v3(-sl.x, -sl.y, 1.f);
-        V2 tu, tv;
-        nm_slope_deriv(tx, (float) nm.res, tu, tv);
         V3 dn_du = v3(-tu.x, -tu.y, 0.f) * sh.tiles, dn_dv = v3(-tv.x, -tv.y, 0.f) * sh.tiles;
         V3 rn = normalize(to_world(bs, bt, bn, nl));
         v.n = rn;

@@ -3,6 +3,7 @@
 // ray casts, the seed, and the path contribution (SpecularManifoldSingleScatter,
 // src/librender/manifold_ss.cpp).
 #pragma once
+#include "dev_lean.h"
 #include "dev_scene.h"
 
 namespace smsdev {
@@ -171,8 +172,71 @@ SMS_DEV Step step_halfvector(V3 v0p, const MV &v1, const EI &v2e, V3 n_offset) {
 
 struct PathResult { bool ok; bool init_valid; SI si_final; V3 p_init; int iterations; };
 
-// sample_path SS (manifold_ss.cpp:203-279) with newton_solver (manifold_ss.cpp:329-405)
-SMS_DEV PathResult sample_path_ss(const DScene &S, int shape, V3 si_p, const EI &ei, V2 sample, V3 n_offset, V3 p_start) {
+// SpecularManifold::sample_gaussian (manifold.h:328-345) with warp::square_to_std_normal
+// (warp.h:184-190); sigma = (s00, s11, s01)
+SMS_DEV V2 sample_gaussian(V2 mu, const float sig[3], V2 s) {
+    const float r = sqrtf(-2.f * logf_l(1.f - s.x));
+    float sn, cs;
+    c_sincosf((2.f * PI_F) * s.y, &sn, &cs);
+    const V2 p = v2(cs * r, sn * r);
+    const float sx = sqrtf(sig[0]), sy = sqrtf(sig[1]);
+    const float rho = sig[2] / (sx * sy);
+    const float a = sqrtf(1.f + rho), b = sqrtf(1.f - rho);
+    const float d = a * 0.5f + b * 0.5f, o = a * 0.5f + b * -0.5f;     // A = a P + b Q
+    V2 q = v2(d * p.x + o * p.y, o * p.x + d * p.y);
+    q.x = q.x * sx; q.y = q.y * sy;
+    return q + mu;
+}
+
+// BSDF::lean at an interaction without uv partials: NormalmapBSDF::lean (bsdfs/normalmap.cpp:
+// 292-303) at the coarsest mip level (mipmap_level, :361-364); (0, 0) for other BSDFs
+SMS_DEV void lean_coarsest(const DScene &S, const SI &si, V2 &mu, float sig[3]) {
+    const DShape &sh = S.shapes[si.shape];
+    if (sh.bsdf != SMS_BSDF_NORMALMAP) { mu = v2(0.f, 0.f); sig[0] = sig[1] = sig[2] = 0.f; return; }
+    const DNormalmap nm = S.nmaps[sh.normalmap];
+    V2 uv = si.uv * sh.tiles;
+    uv = v2(uv.x - floorf(uv.x), uv.y - floorf(uv.y));
+    lean_moments(nm, uv, nm.levels - 1, mu, sig);
+}
+
+// newton_solver SS (manifold_ss.cpp:329-405) from vtx; smooth: vertices at smoothing 1
+SMS_DEV bool newton_ss(const DScene &S, V3 si_p, MV &vtx, const EI &ei, V3 n_offset, bool smooth, SI &si_current,
+                       int &iterations) {
+    const sms_config &cfg = S.cfg;
+    bool success = false;
+    iterations = 0;
+    float beta = 1.f;
+    si_current = si_invalid();
+    while (iterations < cfg.max_iterations) {
+        Step st = cfg.halfvector_constraints ? step_halfvector(si_p, vtx, ei, n_offset)
+                                             : step_anglediff(si_p, vtx, ei, n_offset);
+        if (!st.ok) break;
+        if (norm2(st.C) < cfg.solver_threshold) { success = true; break; }
+        V3 p_prop = vtx.p - cfg.step_scale * beta * (vtx.dp_du * st.dX.x + vtx.dp_dv * st.dX.y);
+        V3 d_prop = normalize(p_prop - si_p);
+        si_current = ray_intersect(S, make_ray(si_p, d_prop));
+        if (!si_current.valid || vtx.shape != si_current.shape) {
+            beta = 0.5f * beta;
+            iterations += 1;
+            continue;
+        }
+        beta = fmin_(1.f, 2.f * beta);
+        vtx = mv_from_si(S, si_current, smooth);
+        iterations++;
+    }
+    if (!success) return false;
+    V3 wx = normalize(si_p - vtx.p);
+    V3 wy = ei.kind == SMS_EI_DIRECTIONAL ? ei.d : normalize(ei.p - vtx.p);
+    float cos_theta_x = dot(vtx.gn, wx), cos_theta_y = dot(vtx.gn, wy);
+    bool refraction = cos_theta_x * cos_theta_y < 0.f, reflection = !refraction;
+    return !((vtx.eta == 1.f && !reflection) || (vtx.eta != 1.f && !refraction));
+}
+
+// sample_path SS (manifold_ss.cpp:203-279): uniform seed, optional MNEE direction, and with
+// config.twostage the LEAN-smoothed pre-solve (:251-270) from an offset slope drawn with
+// lean_sample; iterations count both solves
+SMS_DEV PathResult sample_path_ss(const DScene &S, int shape, V3 si_p, const EI &ei, V2 sample, V3 n_offset, V3 p_start,
+                                  V2 lean_sample) {
     const sms_config &cfg = S.cfg;
     PathResult out;
     out.ok = false; out.init_valid = false; out.si_final = si_invalid(); out.p_init = v3(0.f, 0.f, 0.f);
@@ -188,38 +252,25 @@ SMS_DEV PathResult sample_path_ss(const DScene &S, int shape, V3 si_p, const EI 
     SI si_init = ray_intersect(S, make_ray(si_p, d_tmp));
     if (!si_init.valid || shape != si_init.shape) return out;
     out.init_valid = true; out.p_init = si_init.p;
-    MV vtx = mv_from_si(S, si_init);
-    // newton_solver
-    bool success = false;
-    int iterations = 0;
-    float beta = 1.f;
-    SI si_current = si_invalid();
-    while (iterations < cfg.max_iterations) {
-        Step st = cfg.halfvector_constraints ? step_halfvector(si_p, vtx, ei, n_offset)
-                                             : step_anglediff(si_p, vtx, ei, n_offset);
-        if (!st.ok) break;
-        if (norm2(st.C) < cfg.solver_threshold) { success = true; break; }
-        V3 p_prop = vtx.p - cfg.step_scale * beta * (vtx.dp_du * st.dX.x + vtx.dp_dv * st.dX.y);
-        V3 d_prop = normalize(p_prop - si_p);
-        si_current = ray_intersect(S, make_ray(si_p, d_prop));
-        if (!si_current.valid || vtx.shape != si_current.shape) {
-            beta = 0.5f * beta;
-            iterations += 1;
-            continue;
-        }
-        beta = fmin_(1.f, 2.f * beta);
+    MV vtx = mv_from_si(S, si_init, cfg.twostage != 0);
+    int smooth_iterations = 0;
+    SI si_current;
+    int iterations;
+    if (cfg.twostage) {
+        const V2 mu_offset = v2(-n_offset.x / n_offset.z, -n_offset.y / n_offset.z);
+        V2 mu;
+        float sig[3];
+        lean_coarsest(S, si_init, mu, sig);
+        const V2 slope = sample_gaussian(0.5f * (mu + mu_offset), sig, lean_sample);
+        const V3 lean_normal = normalize(v3(-slope.x, -slope.y, 1.f));
+        const bool ok = newton_ss(S, si_p, vtx, ei, lean_normal, true, si_current, smooth_iterations);
+        if (!(ok && si_current.valid)) { out.iterations = smooth_iterations; return out; }
         vtx = mv_from_si(S, si_current);
-        iterations++;
     }
-    out.iterations = iterations;
+    const bool ok = newton_ss(S, si_p, vtx, ei, n_offset, false, si_current, iterations);
+    out.iterations = smooth_iterations + iterations;
     out.si_final = si_current;
-    if (!success) return out;
-    V3 wx = normalize(si_p - vtx.p);
-    V3 wy = ei.kind == SMS_EI_DIRECTIONAL ? ei.d : normalize(ei.p - vtx.p);
-    float cos_theta_x = dot(vtx.gn, wx), cos_theta_y = dot(vtx.gn, wy);
-    bool refraction = cos_theta_x * cos_theta_y < 0.f, reflection = !refraction;
-    if ((vtx.eta == 1.f && !reflection) || (vtx.eta != 1.f && !refraction)) return out;
-    out.ok = si_current.valid;
+    out.ok = ok && si_current.valid;
     return out;
 }
 

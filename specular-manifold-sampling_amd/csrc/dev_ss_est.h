@@ -45,7 +45,9 @@ SMS_DEV PathResult ss_walk(const DScene &S, const Proposal &P, int64_t q, uint64
     } else {
         s.x = pcg32_float(rw); s.y = pcg32_float(rw);
     }
-    return sample_path_ss(S, shape, si_p, ei, s, v3(0.f, 0.f, 1.f), p_start);
+    V2 ls = v2(0.f, 0.f);
+    if (S.cfg.twostage) { ls.x = pcg32_float(rw); ls.y = pcg32_float(rw); }
+    return sample_path_ss(S, shape, si_p, ei, s, v3(0.f, 0.f, 1.f), p_start, ls);
 }
 
 // record (network inputs, solution uv, weight) for the proposal's training

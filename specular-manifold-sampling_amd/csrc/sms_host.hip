@@ -447,7 +447,6 @@ sms_status build_scene(sms_ctx *ctx, const sms_scene_desc *desc, const sms_confi
 
 sms_status check_config(sms_ctx *ctx, const sms_config *cfg) {
     if (!cfg) return fail(ctx, SMS_E_INVALID, "null config");
-    if (cfg->twostage) return fail(ctx, SMS_E_UNSUPPORTED, "twostage solver is not built");
     if (cfg->max_iterations < 0) return fail(ctx, SMS_E_INVALID, "max_iterations < 0");
     return SMS_OK;
 }
@@ -954,8 +953,9 @@ sms_status ss_enqueue(sms_ctx *ctx, const sms_ss_query *q, int64_t n, float *out
             HIP_TRY(ctx, hipGetLastError());
             EV_RECORD(ctx, ctx->ev1);
         } else {
-            if (ctx->ss_first_pass == 0) {
+            if (ctx->ss_first_pass == 0 && !cfg.twostage) {
                 // persistent: the resident blocks take queries from the device queue
+                // (the two-stage solver runs in the lane-per-query pass)
                 const int per_cu = ctx->blocks_per_cu > 0 ? ctx->blocks_per_cu : ctx->occ_walker;
                 hipLaunchKernelGGL(k_ss_walker, dim3((unsigned) (per_cu * ctx->n_cu)), dim3(256), 0, ctx->stream,
                                    ctx->scene, q, n, seed, ctx->part, ctx->invp, W);

@@ -27,7 +27,8 @@ __global__ void k_ss_walk(DScene S, const sms_ss_walk_in *__restrict__ in, int64
     EI ei;
     ei.p = ld3(w.ei.p); ei.n = ld3(w.ei.n); ei.d = ld3(w.ei.d); ei.weight = ld3(w.ei.weight);
     ei.pdf = w.ei.pdf; ei.kind = w.ei.kind; ei.emitter = w.ei.emitter;
-    PathResult r = sample_path_ss(S, w.shape, ld3(w.si_p), ei, v2(w.sample[0], w.sample[1]), ld3(w.n_offset), ld3(w.p_start));
+    PathResult r = sample_path_ss(S, w.shape, ld3(w.si_p), ei, v2(w.sample[0], w.sample[1]), ld3(w.n_offset), ld3(w.p_start),
+                                  v2(w.lean_sample[0], w.lean_sample[1]));
     sms_ss_walk_out o;
     o.success = r.ok; o.iterations = r.iterations;
     o.final_shape = r.si_final.valid ? r.si_final.shape : -1;

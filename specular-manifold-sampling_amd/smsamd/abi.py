@@ -105,7 +105,7 @@ F2 = (np.float32, 2)
 EI_DTYPE = np.dtype([("p", F3), ("n", F3), ("d", F3), ("weight", F3), ("pdf", np.float32),
                      ("kind", np.int32), ("emitter", np.int32)])
 SS_WALK_IN = np.dtype([("si_p", F3), ("shape", np.int32), ("ei", EI_DTYPE), ("n_offset", F3),
-                       ("sample", F2), ("p_start", F3)])
+                       ("sample", F2), ("p_start", F3), ("lean_sample", F2)])
 SS_WALK_OUT = np.dtype([("success", np.int32), ("iterations", np.int32), ("final_shape", np.int32),
                         ("init_valid", np.int32), ("p", F3), ("n", F3), ("p_init", F3)])
 GLINT_WALK_IN = np.dtype([("sensor_p", F3), ("ei_p", F3), ("si_p", F3), ("frame_s", F3),
@@ -127,7 +127,7 @@ GLINT_QUERY = np.dtype([("sensor_p", F3), ("p", F3), ("n", F3), ("sh_n", F3), ("
                         ("duv_dx", F2), ("duv_dy", F2), ("shape", np.int32), ("pixel", np.int32)])
 
 # sizes the C header implies (checked against the compiled library in the tests)
-C_SIZES = {"sms_emitter_interaction": 60, "sms_ss_walk_in": 108, "sms_ss_walk_out": 52,
+C_SIZES = {"sms_emitter_interaction": 60, "sms_ss_walk_in": 116, "sms_ss_walk_out": 52,
            "sms_glint_walk_in": 128, "sms_glint_walk_out": 24, "sms_ss_query": 80,
            "sms_glint_query": 140, "sms_config": 48, "sms_stats": 64, "sms_ms_walk_in": 116,
            "sms_ms_walk_out": 80}

@@ -135,6 +135,24 @@ def plane_to_world(center, half_size, normal_axis="y"):
                      [0, -s, 0, cz]], np.float32)
 
 
+def mirror_wall(normalmap=None, nm_res=64, n_scratches=16, seed=1, tiles=1.0):
+    """Reflective single-scatter caustic from a normal-mapped mirror: a vertical 1x1 conductor
+    wall (eta=0, k=1) at x = 0.8 facing -x, its normal map as glint_plane's, casts the light of
+    a sphere area light (r 0.1, radiance 50) at (-0.4, 1.4, 0) onto a diffuse floor (y = 0,
+    2x2, receiver).  Camera above the floor.  Shapes: 0 floor, 1 wall, 2 light."""
+    sc = Scene()
+    rgb = normalmap if normalmap is not None else scratch_bump_normalmap(nm_res, n_scratches, seed=seed)
+    nm = sc.add_normalmap(rgb)
+    sc.add_rectangle(plane_to_world((0, 0, 0), 1.0), abi.BSDF_DIFFUSE, flags=abi.FLAG_CAUSTIC_RECEIVER)
+    wall = np.array([[0, 0, -1, 0.8], [0, 0.5, 0, 0.6], [0.5, 0, 0, 0.0]], np.float32)
+    sc.add_rectangle(wall, abi.BSDF_NORMALMAP, flags=abi.FLAG_CAUSTIC_CASTER_SS, eta=0.0, k=1.0, normalmap=nm,
+                     tiles=tiles)
+    light = sc.add_sphere((-0.4, 1.4, 0.0), 0.1, abi.BSDF_DIFFUSE)
+    sc.add_area_light(light, (50.0, 50.0, 50.0))
+    sc.set_camera((-1.2, 2.2, 0.0), (0.2, 0.0, 0.0), (0, 1, 0), 50.0, 64, 64)
+    return sc
+
+
 # ==================================================================== sphere caustic
 def sphere_caustic(width=256, height=256, center_y=1.5):
     """Glass sphere (r=1, IOR 1.5) at (0,center_y,0) over a diffuse plane (albedo 0.5), sphere

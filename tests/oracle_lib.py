@@ -209,3 +209,15 @@ def prop_seeds(g48, defensive, seed, qid, walk0, n):
     out = np.zeros((n, 3), np.float32)
     lib.oracle_prop_seed(g.ctypes.data, defensive, seed, qid, walk0, n, out.ctypes.data)
     return out
+
+
+def upgrade_ss_walk_in(a):
+    """Fixtures written before sms_ss_walk_in gained lean_sample: copy into the current layout
+    (lean_sample 0); other arrays pass through."""
+    names = a.dtype.names
+    if not names or "sample" not in names or "p_start" not in names or "lean_sample" in names or "first_path" in names:
+        return a
+    out = np.zeros(a.shape, abi.SS_WALK_IN)
+    for k in names:
+        out[k] = a[k]
+    return out

@@ -17,7 +17,8 @@ import mlp_oracle as M  # noqa: E402
 
 
 def _load(name):
-    return np.load(os.path.join(GOLD, name), allow_pickle=False)
+    d = np.load(os.path.join(GOLD, name), allow_pickle=False)
+    return {k: oracle_lib.upgrade_ss_walk_in(d[k]) for k in d.files}
 
 
 def test_fixture_inputs_are_regenerated_identically():

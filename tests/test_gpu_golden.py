@@ -6,6 +6,7 @@ import sys
 import numpy as np
 import pytest
 
+import oracle_lib
 from smsamd import abi
 
 pytestmark = pytest.mark.gpu
@@ -22,7 +23,8 @@ def gpu():
 
 
 def _load(name):
-    return np.load(os.path.join(GOLD, name), allow_pickle=False)
+    d = np.load(os.path.join(GOLD, name), allow_pickle=False)
+    return {k: oracle_lib.upgrade_ss_walk_in(d[k]) for k in d.files}
 
 
 @pytest.mark.parametrize("name,scene,cfg,fn", [

@@ -9,6 +9,7 @@ streams and must agree to 1e-5 relative.
 import numpy as np
 import pytest
 
+import oracle_lib
 from smsamd import abi, scenes, workloads
 
 pytestmark = pytest.mark.gpu
@@ -134,6 +135,33 @@ def test_glint_area_light_estimate(gpu, oracle_mod, biased):
     ro2, _ = orc.glint_estimate(q, 9)
     assert np.array_equal(rg2, ro2, equal_nan=True)
     assert (rg2[:, 0] != rg[:, 0]).sum() > 20
+    ctx.close()
+    orc.close()
+
+
+@pytest.mark.parametrize("biased", [0, 1])
+def test_ss_twostage(gpu, oracle_mod, biased):
+    """Two-stage solver (manifold_ss.cpp:251-270) on a normal-mapped mirror wall: walks with
+    explicit LEAN samples and whole estimates (the lane-per-query pass) equal the oracle."""
+    sc = scenes.mirror_wall(nm_res=64, n_scratches=16, seed=4)
+    cfg = abi.SmsConfig.default(max_trials=32 if biased else 100000, biased=biased, twostage=1)
+    ctx = gpu.Context(sc, cfg)
+    orc = oracle_mod.Oracle(sc, cfg)
+    w = workloads.ss_walk_inputs(sc, 4000, 7, caster=1, light_shape=2, extent=0.9)
+    w["lean_sample"] = np.random.default_rng(8).random((len(w), 2)).astype(np.float32)
+    go, oo = ctx.ss_walks(w), orc.ss_walks(w)
+    print("two-stage walks converged", oo["success"].mean(), flush=True)
+    assert oo["success"].sum() > 200
+    for k in ("success", "iterations", "final_shape", "init_valid", "p", "n", "p_init"):
+        assert np.array_equal(go[k], oo[k]), k
+    q = workloads.plane_queries(sc, 2000, 5, extent=0.9, shape=0)
+    rg, sg = ctx.ss_estimate(q, 3)
+    ro, so = orc.ss_estimate(q, 3)
+    print(biased, "gpu", sg, "oracle", so, flush=True)
+    assert so["walks_converged"] > 100 and (ro[:, 0] > 0).sum() > 20
+    assert np.array_equal(rg, ro, equal_nan=True)
+    for k in ("walks", "walks_converged", "newton_iterations", "bernoulli_calls", "bernoulli_trials", "bernoulli_max"):
+        assert sg[k] == so[k], k
     ctx.close()
     orc.close()
 
@@ -280,7 +308,8 @@ def test_nonfinite_walk_is_fast(gpu, oracle_mod):
     wavefront took about 0.4 s per ray)."""
     import os
     import time
-    w = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "pool_nan_walk.npy"))
+    w = oracle_lib.upgrade_ss_walk_in(
+        np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "pool_nan_walk.npy")))
     sc = scenes.pool(512, 512)
     cfg = abi.SmsConfig.default(max_trials=1000000, uniqueness_threshold=1e-5)
     ctx = gpu.Context(sc, cfg)

@@ -295,7 +295,8 @@ def test_nonfinite_newton_step_walk_fixture():
     the BVH instead of visiting every node."""
     import os
     import time
-    w = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "pool_nan_walk.npy"))
+    w = oracle_lib.upgrade_ss_walk_in(
+        np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "pool_nan_walk.npy")))
     sc = scenes.pool(512, 512)
     orc = oracle_lib.Oracle(sc, abi.SmsConfig.default(max_trials=1000000, uniqueness_threshold=1e-5), threads=1)
     t = time.perf_counter()
@@ -304,3 +305,55 @@ def test_nonfinite_newton_step_walk_fixture():
     assert o["success"][0] == 0 and o["iterations"][0] == 20 and o["init_valid"][0] == 1
     assert dt < 0.05, dt
     orc.close()
+
+
+@pytest.mark.parametrize("mu,sig", [((0.1, -0.2), (0.04, 0.01, -0.012)), ((0.0, 0.3), (0.02, 0.05, 0.018))])
+def test_sample_gaussian_moments(mu, sig):
+    """SpecularManifold::sample_gaussian (manifold.h:328-345), the two-stage solver's offset
+    slope: samples have the requested mean and covariance (a sign slip in the mixing matrix
+    flips the correlation)."""
+    lib = oracle_lib.load()
+    lib.oracle_sample_gaussian.argtypes = [C.c_void_p] * 4
+    rng = np.random.default_rng(2)
+    n = 40000
+    s = rng.random((n, 2)).astype(np.float32)
+    m, sg = np.asarray(mu, np.float32), np.asarray(sig, np.float32)
+    out = np.zeros((n, 2), np.float32)
+    o = np.zeros(2, np.float32)
+    for i in range(n):
+        lib.oracle_sample_gaussian(m.ctypes.data, sg.ctypes.data, s[i].ctypes.data, o.ctypes.data)
+        out[i] = o
+    x = out.astype(np.float64)
+    cov = np.cov(x.T)
+    se = np.sqrt(np.array([sig[0], sig[1]]) / n)
+    assert np.all(np.abs(x.mean(0) - mu) < 4 * se)
+    assert cov[0, 0] == pytest.approx(sig[0], rel=0.04)
+    assert cov[1, 1] == pytest.approx(sig[1], rel=0.04)
+    assert cov[0, 1] == pytest.approx(sig[2], abs=0.04 * math.sqrt(sig[0] * sig[1]))
+
+
+def _wall_queries(sc, n, seed):
+    return workloads.plane_queries(sc, n, seed, extent=0.9, shape=0)
+
+
+def test_twostage_estimator_agrees_with_single_stage():
+    """The two-stage solver (manifold_ss.cpp:251-270) only changes how a walk reaches a
+    solution of the normal-mapped mirror; the unbiased estimate's expectation is the same with
+    and without it."""
+    sc = scenes.mirror_wall(nm_res=64, n_scratches=16, seed=4)
+    q = _wall_queries(sc, 600, 5)
+    means, stats = {}, {}
+    for ts in (0, 1):
+        o = oracle_lib.Oracle(sc, abi.SmsConfig.default(max_trials=100000, twostage=ts))
+        runs = np.stack([o.ss_estimate(q, s)[0][:, 0] for s in range(1, 13)]).astype(np.float64)
+        st = o.ss_estimate(q, 1)[1]
+        means[ts], stats[ts] = runs, st
+        o.close()
+    mu0, mu1 = means[0].mean(), means[1].mean()
+    d = means[1].mean(0) - means[0].mean(0)            # paired by query: the queries' own spread cancels
+    se = d.std() / math.sqrt(d.size)
+    print(f"single {mu0:.5g} two-stage {mu1:.5g} se {se:.3g} converged/walk "
+          f"{stats[0]['walks_converged'] / stats[0]['walks']:.3f} {stats[1]['walks_converged'] / stats[1]['walks']:.3f}")
+    assert (means[0] > 0).mean() > 0.05 and (means[1] > 0).mean() > 0.05
+    assert not np.array_equal(means[0], means[1])
+    assert abs(mu0 - mu1) < 4 * se + 1e-3 * abs(mu0)
