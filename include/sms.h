@@ -281,10 +281,12 @@ sms_status sms_ctx_set_config(sms_ctx *ctx, const sms_config *cfg);
    64-trial rounds of the one-wavefront pass before a chain moves to the multi-wavefront stage
    (0 = straight to it); long_chains / long_trials (glints): chains the multi-wavefront stage
    holds (at most 1024) and trials it runs per chain (a multiple of 64, at most 32768); the
-   rest continue one wavefront per chain; ss_first_pass (single scatter, unbiased): 0 = the
-   persistent walker, whose lanes take their next walk as soon as one ends, 1 = one lane per
-   query for its first walk and up to lane_trials trials.  No counterpart in the reference (its
-   trial loops are serial). */
+   rest continue one wavefront per chain; ss_first_pass (unbiased single and multi scatter): 0 =
+   persistent -- single scatter: the walker, whose lanes take their next walk as soon as one
+   ends (the two-stage solver uses 1); multi scatter: resident lanes that take their next
+   (query, caster) item from a device queue as soon as a chain ends --, 1 = one lane per query
+   for its first walk and up to lane_trials trials.  No counterpart in the reference (its trial
+   loops are serial). */
 typedef struct sms_schedule {
     int32_t lane_trials;
     int32_t blocks_per_cu;
@@ -339,8 +341,9 @@ sms_status sms_glint_autotune(sms_ctx *ctx, const sms_glint_query *q, int64_t n,
    and lane_trials, keeps the fastest in the context and returns it in *best.  out_rgb is
    device scratch of 3 * n floats; results do not depend on the schedule. */
 /* Autotuner of the unbiased multi-scatter estimator (no counterpart in the reference): times
-   sms_ms_estimate_batch for lane_trials 1 .. 32 (trials a lane runs before a chain moves to the
-   one-wavefront pass), keeps the fastest and returns it in *best. */
+   sms_ms_estimate_batch for both first passes (ss_first_pass) and lane_trials 1 .. 32 (trials a
+   lane runs before a chain moves to the one-wavefront pass), keeps the fastest and returns it
+   in *best. */
 sms_status sms_ms_autotune(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *out_rgb,
                            sms_schedule *best);
 sms_status sms_ss_autotune(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *out_rgb,

@@ -42,6 +42,7 @@ struct sms_ctx {
     int occ_glint_biased = 0;
     bool area_emitters = false;                 // glint BSDF-sampling strategy can contribute
     int occ_ms[2] = {0, 0}, occ_mscoop[2] = {0, 0}, occ_msbiased[2] = {0, 0};   // [0]: chains <= 2, [1]: <= 4
+    int occ_mspersist[2] = {0, 0};
     // host copies the learned proposal's UV grid is built from
     std::vector<float2> h_vuv;
     std::vector<uint4> h_faces;

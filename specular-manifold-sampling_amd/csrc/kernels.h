@@ -176,6 +176,8 @@ __global__ void k_ss_biased(DScene S, const sms_ss_query *Q, int64_t n, const ui
     __global__ void k_ms_walk_##B(DScene S, const sms_ms_walk_in *in, int64_t n, sms_ms_walk_out *out);               \
     __global__ void k_ms_estimate_##B(DScene S, const sms_ss_query *Q, int64_t n, const uint64_t *seedp, float *part, \
                                       float *invp, EstimatorWork W);                                                  \
+    __global__ void k_ms_persist_##B(DScene S, const sms_ss_query *Q, int64_t n, const uint64_t *seedp, float *part, \
+                                     float *invp, EstimatorWork W);                                                  \
     __global__ void k_ms_coop_##B(DScene S, const sms_ss_query *Q, const uint64_t *seedp, const float *part,          \
                                   float *invp, EstimatorWork W);                                                      \
     __global__ void k_ms_biased_##B(DScene S, const sms_ss_query *Q, int64_t n, const uint64_t *seedp, float *out,    \

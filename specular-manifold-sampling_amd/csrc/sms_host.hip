@@ -714,6 +714,8 @@ sms_status sms_ctx_create(const sms_scene_desc *scene, const sms_config *cfg, sm
     if (!st) st = occupancy(ctx, k_glint_biased_bvh, &ctx->occ_glint_biased);
     if (!st) st = occupancy(ctx, k_ms_estimate_2, &ctx->occ_ms[0]);
     if (!st) st = occupancy(ctx, k_ms_estimate_4, &ctx->occ_ms[1]);
+    if (!st) st = occupancy(ctx, k_ms_persist_2, &ctx->occ_mspersist[0]);
+    if (!st) st = occupancy(ctx, k_ms_persist_4, &ctx->occ_mspersist[1]);
     if (!st) st = occupancy(ctx, k_ms_coop_2, &ctx->occ_mscoop[0]);
     if (!st) st = occupancy(ctx, k_ms_coop_4, &ctx->occ_mscoop[1]);
     if (!st) st = occupancy(ctx, k_ms_biased_2, &ctx->occ_msbiased[0]);
@@ -1012,8 +1014,15 @@ sms_status ms_enqueue(sms_ctx *ctx, const sms_ss_query *q, int64_t n, float *out
             EV_RECORD(ctx, ctx->ev1);
         } else {
             dim3 g1((unsigned) ((n + 255) / 256)), g2(grid_for(ctx, ctx->occ_mscoop[b]));
-            if (b == 0) hipLaunchKernelGGL(k_ms_estimate_2, g1, dim3(256), 0, ctx->stream, ctx->scene, q, n, seed, ctx->part, ctx->invp, W);
-            else hipLaunchKernelGGL(k_ms_estimate_4, g1, dim3(256), 0, ctx->stream, ctx->scene, q, n, seed, ctx->part, ctx->invp, W);
+            if (ctx->ss_first_pass == 0) {
+                // persistent: resident lanes take (query, caster) items from the device cursor
+                const dim3 gp(grid_for(ctx, ctx->occ_mspersist[b]));
+                if (b == 0) hipLaunchKernelGGL(k_ms_persist_2, gp, dim3(256), 0, ctx->stream, ctx->scene, q, n, seed, ctx->part, ctx->invp, W);
+                else hipLaunchKernelGGL(k_ms_persist_4, gp, dim3(256), 0, ctx->stream, ctx->scene, q, n, seed, ctx->part, ctx->invp, W);
+            } else {
+                if (b == 0) hipLaunchKernelGGL(k_ms_estimate_2, g1, dim3(256), 0, ctx->stream, ctx->scene, q, n, seed, ctx->part, ctx->invp, W);
+                else hipLaunchKernelGGL(k_ms_estimate_4, g1, dim3(256), 0, ctx->stream, ctx->scene, q, n, seed, ctx->part, ctx->invp, W);
+            }
             HIP_TRY(ctx, hipGetLastError());
             EV_RECORD(ctx, ctx->ev1);
             const float *part = ctx->part;
@@ -1143,8 +1152,9 @@ sms_status sms_ms_autotune(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint6
     float win_ms = INFINITY;
     sms_status st = SMS_OK;
     const int cands[6] = {1, 2, 4, 8, 16, 32};
-    for (int lt : cands) {
-        ctx->lane_trials = lt;
+    for (int cand = 0; cand < 12; ++cand) {
+        ctx->ss_first_pass = cand / 6;
+        ctx->lane_trials = cands[cand % 6];
         ctx->blocks_per_cu = 0;
         float ms = 0.f;
         for (int rep = 0; rep < 2; ++rep) {           // the first run warms caches and allocations

@@ -195,6 +195,109 @@ SMS_DEV void ms_estimate_body(const DScene &S, const sms_ss_query *__restrict__ 
     stat_add(W.stats + 3, st_bcalls); stat_add(W.stats + 4, st_btrials); stat_max(W.stats + 5, st_bmax);
 }
 
+// Persistent first pass: resident lanes take (query, caster) items i = q * nc + c from the
+// device cursor and run one walk per loop trip -- the item's first walk, then its Bernoulli
+// trials -- from one walk call site, so a lane whose chain ends takes the next item at once
+// instead of idling until its wavefront's slowest chain ends.  Streams, results, statistics
+// and overflow items equal ms_estimate_body's: item i goes to the overflow sub-list of
+// (q / 256) % SPREAD, the sub-list of the lane pass's block for q, so the capacities hold.
+template <int MAXB>
+SMS_DEV void ms_persistent_body(const DScene &S, const sms_ss_query *__restrict__ Q, int64_t n, uint64_t seed,
+                                float *__restrict__ part, float *__restrict__ invp, EstimatorWork W) {
+    const sms_config &cfg = S.cfg;
+    unsigned long long st_walks = 0, st_conv = 0, st_iters = 0, st_bcalls = 0, st_btrials = 0, st_bmax = 0;
+    const int nc = S.n_casters_ms;
+    const unsigned long long n_items = (unsigned long long) n * (unsigned long long) nc;
+    int64_t item = -1, q = 0, k = 0;
+    int c = 0, shape = -1;
+    SI si;
+    EI ei;
+    V3 wi_local = v3(0.f, 0.f, 0.f), direction = v3(0.f, 0.f, 0.f);
+    uint64_t qkey = 0;
+    const float *gq = nullptr;
+    int seed_shapes[MAXB];
+    for (int b = 0; b < MAXB; ++b) seed_shapes[b] = -1;
+    V2 uv0 = v2(0.f, 0.f);
+    bool have_uv0 = false;
+    while (true) {
+        if (item < 0) {
+            const unsigned long long it = atomicAdd(W.cursor, 1ull);
+            if (it >= n_items) break;
+            item = (int64_t) it;
+            q = item / nc;
+            c = (int) (item % nc);
+            part[3 * item] = 0.f; part[3 * item + 1] = 0.f; part[3 * item + 2] = 0.f;
+            invp[item] = 0.f;
+            const sms_ss_query g = Q[q];
+            if (g.shape < 0) { item = -1; continue; }     // no receiver: zero estimate, no walks
+            si = ms_query_si(g);
+            wi_local = ld3(g.wi);
+            qkey = query_key(seed, (uint64_t) q);
+            Pcg32 rq;
+            stream_seed(rq, qkey, 0);
+            ei = sample_emitter_interaction(S, si.p, S.emitters_ms, S.n_emitters_ms, rq);
+            shape = S.casters_ms[c];
+            gq = ms_mixture(W.prop, q, shape);
+            k = 0;
+        }
+        MSWalk<MAXB> t;
+        ms_walk_k<MAXB>(S, W.prop, gq, qkey, c, shape, (uint64_t) k, si.p, ei, k == 0, k == 0 ? nullptr : seed_shapes, t);
+        st_walks++; st_iters += (unsigned long long) t.iterations;
+        if (k == 0) {
+            if (!t.ok) { item = -1; continue; }
+            st_conv++;
+            if (capturing(W.prop, shape, SMS_CAPTURE_SOLUTIONS)) ms_capture<MAXB>(S, W.prop, si, ei, t.path, shape, 1.f);
+            have_uv0 = capturing(W.prop, shape, SMS_CAPTURE_ESTIMATES) && ms_caster_uv<MAXB>(S, si.p, t.path, shape, uv0);
+            direction = normalize(t.path.v[0].v.p - si.p);
+            V3 spec = ms_path_contribution<MAXB>(S, si, ei, t.path);
+            V3 dl = v3(dot(direction, si.sh_s), dot(direction, si.sh_t), dot(direction, si.sh_n));
+            V3 bsdf = ms_receiver_bsdf(S, si, wi_local, dl);
+            part[3 * item] = bsdf.x * spec.x; part[3 * item + 1] = bsdf.y * spec.y; part[3 * item + 2] = bsdf.z * spec.z;
+            st_bcalls++;
+            for (int b = 0; b < MAXB; ++b) seed_shapes[b] = t.seed_shapes[b];
+            k = 1;
+            continue;
+        }
+        bool match = false;
+        if (t.ok) {
+            st_conv++;
+            if (capturing(W.prop, shape, SMS_CAPTURE_SOLUTIONS)) ms_capture<MAXB>(S, W.prop, si, ei, t.path, shape, 1.f);
+            V3 dtr = normalize(t.path.v[0].v.p - si.p);
+            match = fabsf(dot(direction, dtr) - 1.f) < cfg.uniqueness_threshold;
+        }
+        float inv_prob;
+        if (match) {
+            inv_prob = (float) k;
+        } else {
+            k++;
+            if (cfg.max_trials > 0 && k > cfg.max_trials) {
+                inv_prob = 0.f;
+            } else if (k > W.lane_trials) {
+                const int sub = (int) ((q >> 8) % SPREAD);
+                unsigned long long o = spread_append(W.over, sub, 1ull);
+                OverflowRec rec;
+                rec.q = item; rec.k = k;
+                rec.sol[0] = direction.x; rec.sol[1] = direction.y; rec.sol[2] = direction.z;
+                for (int b = 0; b < 4; ++b) rec.shapes[b] = b < MAXB ? seed_shapes[b < MAXB ? b : 0] : -1;
+                rec.pad = 0;
+                W.over.items[(size_t) sub * W.over.sub_cap + o] = rec;
+                item = -1;
+                continue;
+            } else {
+                continue;
+            }
+        }
+        const unsigned long long tr = (unsigned long long) k;
+        st_btrials += tr; st_bmax = st_bmax > tr ? st_bmax : tr;
+        invp[item] = inv_prob;
+        const float wgt = mean3(&part[3 * item]) * inv_prob;
+        if (wgt > 0.f && have_uv0) capture_sample(S, W.prop, si, ei, uv0, wgt);
+        item = -1;
+    }
+    stat_add(W.stats + 0, st_walks); stat_add(W.stats + 1, st_conv); stat_add(W.stats + 2, st_iters);
+    stat_add(W.stats + 3, st_bcalls); stat_add(W.stats + 4, st_btrials); stat_max(W.stats + 5, st_bmax);
+}
+
 template <int MAXB>
 SMS_DEV void ms_coop_body(const DScene &S, const sms_ss_query *__restrict__ Q, uint64_t seed,
                           const float *__restrict__ part, float *__restrict__ invp, EstimatorWork W) {
@@ -386,6 +489,11 @@ SMS_DEV void ms_biased_body(const DScene &S, const sms_ss_query *__restrict__ Q,
                                                              const uint64_t *__restrict__ seedp, float *part,   \
                                                              float *invp, EstimatorWork W) {                    \
         ms_estimate_body<B>(S, Q, n, *seedp, part, invp, W);                                                    \
+    }                                                                                                           \
+    __global__ void __launch_bounds__(256) k_ms_persist_##B(DScene S, const sms_ss_query *Q, int64_t n,         \
+                                                            const uint64_t *__restrict__ seedp, float *part,    \
+                                                            float *invp, EstimatorWork W) {                     \
+        ms_persistent_body<B>(S, Q, n, *seedp, part, invp, W);                                                  \
     }                                                                                                           \
     __global__ void __launch_bounds__(256) k_ms_coop_##B(DScene S, const sms_ss_query *Q,                       \
                                                          const uint64_t *__restrict__ seedp, const float *part, \

@@ -374,11 +374,18 @@ def test_ms_estimate_slab(gpu, oracle_mod, slab_scene, biased):
     if not biased:
         for k in ("bernoulli_calls", "bernoulli_trials", "bernoulli_max"):
             assert sg[k] == so[k], k
-        ctx.set_schedule(lane_trials=1)
-        rg2, sg2 = ctx.ms_estimate(q, 9)
-        assert sg2["overflow_queries"] > 0
-        assert np.array_equal(rg2, ro, equal_nan=True)
-        assert sg2["bernoulli_trials"] == so["bernoulli_trials"]
+        # persistent first pass (0) and lane per query (1), long and short lane trial runs
+        for first_pass in (0, 1):
+            for lane_trials in (8, 1):
+                ctx.set_schedule(lane_trials=lane_trials, ss_first_pass=first_pass)
+                rg2, sg2 = ctx.ms_estimate(q, 9)
+                print("first pass", first_pass, "lane_trials", lane_trials, sg2, flush=True)
+                if lane_trials == 1:
+                    assert sg2["overflow_queries"] > 0
+                assert np.array_equal(rg2, ro, equal_nan=True), (first_pass, lane_trials)
+                for k in ("walks", "walks_converged", "newton_iterations", "bernoulli_calls", "bernoulli_trials",
+                          "bernoulli_max"):
+                    assert sg2[k] == so[k], (k, first_pass, lane_trials)
     ctx.close()
     orc.close()
 
