@@ -160,6 +160,9 @@ __global__ void k_film_accumulate(const float *rgb, const float *w, int64_t pixe
                                   float *image_n);
 __global__ void k_prop_inputs(DScene S, const sms_ss_query *Q, int64_t n, const uint64_t *seedp, int ms, float *x);
 __global__ void k_gmm_prepare(const float *raw, int64_t n, float *g);
+__global__ void k_capture_keys(const float *cap, const unsigned long long *counts, unsigned long long cap_sub, float *rec,
+                               uint64_t *keys, uint32_t *idx);
+__global__ void k_capture_scatter(const float *rec, const uint32_t *idx, int64_t n, float *x, float *uv, float *w);
 __global__ void k_capture_gather(const float *cap, const unsigned long long *counts, unsigned long long cap_sub,
                                  float *x, float *uv, float *w);
 __global__ void k_mlp_infer(const float *p, const uint16_t *wb, const float *x9, int64_t n, float *out);

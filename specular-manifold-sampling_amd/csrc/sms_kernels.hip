@@ -497,4 +497,41 @@ __global__ void k_capture_gather(const float *__restrict__ cap, const unsigned l
     }
 }
 
+// Deterministic order of the captured records: lanes append them in scheduling order, so
+// k_capture_keys copies them into one array with a 64-bit hash of each record's twelve words,
+// the records are sorted by that key (radix sort, stable), and k_capture_scatter writes them
+// out in key order.  Equal records are interchangeable, so the output depends only on the set
+// of records captured.
+__global__ void k_capture_keys(const float *__restrict__ cap, const unsigned long long *__restrict__ counts,
+                               unsigned long long cap_sub, float *__restrict__ rec, uint64_t *__restrict__ keys,
+                               uint32_t *__restrict__ idx) {
+    const int sub = blockIdx.x;
+    unsigned long long off = 0;
+    for (int s = 0; s < sub; ++s) {
+        const unsigned long long c = counts[s * SPREAD_STRIDE];
+        off += c < cap_sub ? c : cap_sub;
+    }
+    const unsigned long long c0 = counts[sub * SPREAD_STRIDE], cnt = c0 < cap_sub ? c0 : cap_sub;
+    for (unsigned long long i = threadIdx.x; i < cnt; i += blockDim.x) {
+        const float *r = cap + ((size_t) sub * cap_sub + i) * 12;
+        uint64_t h = 0x9E3779B97F4A7C15ull;
+        for (int j = 0; j < 12; ++j) {
+            rec[(off + i) * 12 + j] = r[j];
+            h = mix64(h ^ (uint64_t) __float_as_uint(r[j]));
+        }
+        keys[off + i] = h;
+        idx[off + i] = (uint32_t) (off + i);
+    }
+}
+__global__ void k_capture_scatter(const float *__restrict__ rec, const uint32_t *__restrict__ idx, int64_t n,
+                                  float *__restrict__ x, float *__restrict__ uv, float *__restrict__ w) {
+    const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float *r = rec + (size_t) idx[i] * 12;
+    for (int j = 0; j < 9; ++j) x[i * 9 + j] = r[j];
+    uv[i * 2] = r[9];
+    uv[i * 2 + 1] = r[10];
+    if (w) w[i] = r[11];
+}
+
 } // namespace smsdev

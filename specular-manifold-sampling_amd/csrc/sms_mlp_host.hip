@@ -3,6 +3,8 @@
 // of training samples.  No counterpart in the reference, whose seeds are uniform over the
 // caster's area (src/librender/manifold_ss.cpp:228-235).
 #include "host_ctx.h"
+
+#include <hipcub/device/device_radix_sort.hpp>
 #include "dev_mlp.h"
 
 #include <algorithm>
@@ -413,10 +415,39 @@ sms_status sms_ctx_training_samples(sms_ctx *ctx, int64_t *count, float *x, floa
     for (int s = 0; s < SPREAD; ++s) total += (int64_t) std::min<unsigned long long>(h[(size_t) s * SPREAD_STRIDE], ctx->cap_sub);
     *count = total;
     if (x && uv && total > 0) {
-        hipLaunchKernelGGL(k_capture_gather, dim3(SPREAD), dim3(256), 0, ctx->stream, ctx->cap, ctx->cap_counts,
-                           (unsigned long long) ctx->cap_sub, x, uv, w);
-        HIP_TRY(ctx, hipGetLastError());
-        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        // records in an order that does not depend on scheduling: sorted by a hash of their words
+        float *rec = nullptr;
+        uint64_t *keys = nullptr;
+        uint32_t *idx = nullptr;
+        void *tmp = nullptr;
+        size_t tmp_bytes = 0;
+        const int nn = (int) total;
+        hipError_t e = hipMalloc(&rec, (size_t) total * 12 * sizeof(float));
+        if (e == hipSuccess) e = hipMalloc(&keys, (size_t) total * 2 * sizeof(uint64_t));
+        if (e == hipSuccess) e = hipMalloc(&idx, (size_t) total * 2 * sizeof(uint32_t));
+        if (e == hipSuccess)
+            e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys, keys + total, idx, idx + total, nn, 0, 64,
+                                                   ctx->stream);
+        if (e == hipSuccess) e = hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 1));
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_capture_keys, dim3(SPREAD), dim3(256), 0, ctx->stream, ctx->cap, ctx->cap_counts,
+                               (unsigned long long) ctx->cap_sub, rec, keys, idx);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess)
+            e = hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys, keys + total, idx, idx + total, nn, 0, 64,
+                                                   ctx->stream);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_capture_scatter, dim3((unsigned) ((total + 255) / 256)), dim3(256), 0, ctx->stream, rec,
+                               idx + total, (int64_t) total, x, uv, w);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        for (void *b : {(void *) rec, (void *) keys, (void *) idx, tmp})
+            if (b) (void) hipFree(b);
+        if (e != hipSuccess)
+            return fail(ctx, e == hipErrorOutOfMemory ? SMS_E_OOM : SMS_E_HIP,
+                        std::string("sms_ctx_training_samples: ") + hipGetErrorString(e));
     }
     return SMS_OK;
 }

@@ -330,6 +330,17 @@ def test_capture(gpu, oracle_mod, pool_scene):
     nz = np.nonzero(rg[:, 0] > 0)[0]
     assert (idx >= 0).all() and sorted(idx.tolist()) == sorted(nz.tolist())
     np.testing.assert_allclose(w, rg[idx].mean(1), rtol=1e-5)
+    # records come out in an order that does not depend on scheduling: the walker and the lane
+    # pass append them in different orders and return them identically
+    outs = []
+    for first_pass in (0, 1, 0):
+        ctx.set_schedule(ss_first_pass=first_pass)
+        ctx.clear_training()
+        ctx.ss_estimate(q, 3)
+        outs.append(ctx.training_samples())
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            assert np.array_equal(a, b)
     ctx.set_training_capture(0)
     ctx.close()
     orc.close()
