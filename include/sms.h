@@ -461,7 +461,9 @@ sms_status sms_ctx_set_proposal(sms_ctx *ctx, sms_mlp *mlp, int32_t shape, float
 #define SMS_CAPTURE_ESTIMATES 1
 sms_status sms_ctx_set_training_capture(sms_ctx *ctx, int64_t capacity, int32_t mode);
 /* Number of recorded samples; when x / uv are not NULL, copies them to device buffers of
-   count x 9 and count x 2 floats, and the weights to w (count floats) if w is not NULL. */
+   count x 9 and count x 2 floats, and the weights to w (count floats) if w is not NULL, in an
+   order that depends only on the records (sorted by a hash of their contents), not on the
+   order the estimator's lanes recorded them. */
 sms_status sms_ctx_training_samples(sms_ctx *ctx, int64_t *count, float *x, float *uv, float *w);
 sms_status sms_ctx_clear_training(sms_ctx *ctx);
 /* Network inputs (device, n x 9) of the queries of an estimate with this seed: shading point,
