@@ -341,9 +341,9 @@ sms_status sms_glint_autotune(sms_ctx *ctx, const sms_glint_query *q, int64_t n,
    and lane_trials, keeps the fastest in the context and returns it in *best.  out_rgb is
    device scratch of 3 * n floats; results do not depend on the schedule. */
 /* Autotuner of the unbiased multi-scatter estimator (no counterpart in the reference): times
-   sms_ms_estimate_batch for both first passes (ss_first_pass) and lane_trials 1 .. 32 (trials a
-   lane runs before a chain moves to the one-wavefront pass), keeps the fastest and returns it
-   in *best. */
+   sms_ms_estimate_batch for both first passes (ss_first_pass: persistent with lane_trials 4 ..
+   1024, lane per query with 1 .. 32; lane_trials = trials a lane runs before a chain moves to
+   the one-wavefront pass), keeps the fastest and returns it in *best. */
 sms_status sms_ms_autotune(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *out_rgb,
                            sms_schedule *best);
 sms_status sms_ss_autotune(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *out_rgb,

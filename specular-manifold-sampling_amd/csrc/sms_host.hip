@@ -1155,10 +1155,13 @@ sms_status sms_ms_autotune(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint6
     sms_schedule win = keep;
     float win_ms = INFINITY;
     sms_status st = SMS_OK;
-    const int cands[6] = {1, 2, 4, 8, 16, 32};
-    for (int cand = 0; cand < 12; ++cand) {
-        ctx->ss_first_pass = cand / 6;
-        ctx->lane_trials = cands[cand % 6];
+    // the persistent pass refills a lane whose chain runs long, so longer lane runs pay off there
+    struct Cand { int first_pass, lane_trials; };
+    const Cand cands[11] = {{0, 4}, {0, 16}, {0, 64}, {0, 256}, {0, 1024},
+                            {1, 1}, {1, 2}, {1, 4}, {1, 8}, {1, 16}, {1, 32}};
+    for (const Cand &cd : cands) {
+        ctx->ss_first_pass = cd.first_pass;
+        ctx->lane_trials = cd.lane_trials;
         ctx->blocks_per_cu = 0;
         float ms = 0.f;
         for (int rep = 0; rep < 2; ++rep) {           // the first run warms caches and allocations

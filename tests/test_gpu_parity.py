@@ -376,7 +376,7 @@ def test_ms_estimate_slab(gpu, oracle_mod, slab_scene, biased):
             assert sg[k] == so[k], k
         # persistent first pass (0) and lane per query (1), long and short lane trial runs
         for first_pass in (0, 1):
-            for lane_trials in (8, 1):
+            for lane_trials in (256, 8, 1):
                 ctx.set_schedule(lane_trials=lane_trials, ss_first_pass=first_pass)
                 rg2, sg2 = ctx.ms_estimate(q, 9)
                 print("first pass", first_pass, "lane_trials", lane_trials, sg2, flush=True)
