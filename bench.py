@@ -82,7 +82,7 @@ def parse():
     ap.add_argument("--rmse-pass-spp", type=int, default=8)
     ap.add_argument("--first-pass-spp", type=int, default=2,
                     help="samples per pixel of the charged frame's uniform first pass")
-    ap.add_argument("--train-samples", type=int, default=4_000_000, help="records the online training uses")
+    ap.add_argument("--train-samples", type=int, default=2_000_000, help="records the online training uses")
     ap.add_argument("--train-epochs", type=int, default=8, help="Adam steps per 8192 captured records")
     ap.add_argument("--defensive", type=float, default=0.2)
     return ap.parse_args()
