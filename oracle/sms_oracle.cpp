@@ -25,6 +25,23 @@ using namespace orc;
 
 namespace {
 
+/* ---------------------------------------------------------------- planted-error hooks
+   Factors the tests set to plant a known mistake into one term of the oracle and check that the
+   pin of that term fails.  Every factor is 1 unless a test sets it; x * 1.0f == x exactly, so
+   the unplanted oracle is bit-identical to one without the hooks. */
+enum {
+    PLANT_SS_G = 0,         /* first row of dC1/dx2 in the single-scatter geometric term      */
+    PLANT_SS_TRANSMIT,      /* specular_reflectance of a single-scatter dielectric             */
+    PLANT_SS_AD_JAC,        /* dC/du_0 of the single-scatter angle-difference Jacobian          */
+    PLANT_SS_HV_JAC,        /* dH/du_0 of the single-scatter half-vector Jacobian               */
+    PLANT_MS_AD_JAC,        /* dC_i/dx_i (0,0) of the multi-scatter angle-difference blocks     */
+    PLANT_MS_HV_JAC,        /* dC_i/dx_i (0,0) of the multi-scatter half-vector blocks          */
+    PLANT_MNEE_TARGET,      /* scale of the light point the MNEE seed ray aims at              */
+    PLANT_MS_G,             /* the multi-scatter generalized geometric term                     */
+    PLANT_N = 16
+};
+static float g_plant[PLANT_N] = {1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f};
+
 /* ---------------------------------------------------------------- scene */
 struct Shape {
     int type, bsdf, flags, emitter;
@@ -1022,6 +1039,7 @@ static Step step_anglediff(V3 v0p, const MV &v1, const EI &v2e, V3 n_offset) {
         dC = m2(dti_du - dtoi_du, dti_dv - dtoi_dv, dpi_du - dpoi_du, dpi_dv - dpoi_dv);
         success = true;
     }
+    dC.m00 = dC.m00 * g_plant[PLANT_SS_AD_JAC];
     float determinant = det2(dC);
     if (fabsf(determinant) < 1e-6f) return r;
     M2 inv = inverse2(dC);
@@ -1060,6 +1078,7 @@ static Step step_halfvector(V3 v0p, const MV &v1, const EI &v2e, V3 n_offset) {
     if (eta != 1.f) { dh_du = dh_du * -1.f; dh_dv = dh_dv * -1.f; }
     M2 dH = m2(dot(v1.ds_du, h) + dot(v1.s, dh_du), dot(v1.ds_dv, h) + dot(v1.s, dh_dv),
                dot(v1.dt_du, h) + dot(v1.t, dh_du), dot(v1.dt_dv, h) + dot(v1.t, dh_dv));
+    dH.m00 = dH.m00 * g_plant[PLANT_SS_HV_JAC];
     float determinant = det2(dH);
     if (fabsf(determinant) < 1e-6f) return r;
     M2 inv = inverse2(dH);
@@ -1154,7 +1173,7 @@ static PathOut sample_path_ss(const Scene &S, int shape, V3 si_p, const EI &ei, 
         const Shape &sh = S.shapes[shape];
         bool reflection = sh.k != 0.f;                 // any(imag(ior) != 0)
         if (reflection) d_tmp = normalize(sh.bbox_center - si_p);
-        else d_tmp = normalize(ei.p - si_p);
+        else d_tmp = normalize(ei.p * g_plant[PLANT_MNEE_TARGET] - si_p);
     }
     Ray ray = make_ray(si_p, d_tmp);
     SI si_init = ray_intersect(S, ray);
@@ -1197,6 +1216,7 @@ static V3 specular_reflectance_ss(const Scene &S, const SI &si, V3 wo) {
         float F = fresnel(cos_theta, eta);
         val = 1.f - F;
         val = val * (eta * eta);
+        val = val * g_plant[PLANT_SS_TRANSMIT];
     }
     return v3(val, val, val);
 }
@@ -1248,7 +1268,8 @@ static float geometric_term_ss(const MV &v0, const MV &v1, const MV &v2v) {
     dh_dv = ilo * (v2v.dp_dv - wo * dot(wo, v2v.dp_dv));
     dh_du = dh_du - h * dot(dh_du, h); dh_dv = dh_dv - h * dot(dh_dv, h);
     if (eta != 1.f) { dh_du = dh_du * -1.f; dh_dv = dh_dv * -1.f; }
-    M2 dc1_dx2 = m2(dot(dh_du, s), dot(dh_dv, s), dot(dh_du, t), dot(dh_dv, t));
+    M2 dc1_dx2 = m2(dot(dh_du, s) * g_plant[PLANT_SS_G], dot(dh_dv, s) * g_plant[PLANT_SS_G], dot(dh_du, t),
+                    dot(dh_dv, t));
     if (v2v.fixed_direction) {
         if (fabsf(det2(dc2_dx2)) < 1e-6f) return 0.f;
         M2 Li = inverse2(dc2_dx2);
@@ -2177,7 +2198,7 @@ static bool ms_seed_path(const Scene &S, int shape, V3 si_p, const EI &ei, V2 sa
     V3 wo = normalize(x1 - x0);
     if (S.cfg.mnee_init) {
         const Shape &sh = S.shapes[shape];
-        wo = sh.k != 0.f ? normalize(sh.bbox_center - x0) : normalize(ei.p - x0);
+        wo = sh.k != 0.f ? normalize(sh.bbox_center - x0) : normalize(ei.p * g_plant[PLANT_MNEE_TARGET] - x0);
     }
     Ray ray = make_ray(x0, wo);
     while (true) {
@@ -2329,6 +2350,7 @@ static bool ms_step_anglediff(MSPath &P, V3 x0, const EI &ei) {
             }
             success_i = true;
         }
+        vi.dcur.m00 = vi.dcur.m00 * g_plant[PLANT_MS_AD_JAC];
         success = success && success_i;
     }
     if (!success || !ms_invert_tridiagonal_step(P)) return false;
@@ -2384,6 +2406,7 @@ static bool ms_step_halfvector(MSPath &P, V3 x0, const EI &ei) {
         if (eta != 1.f) { dh_du = dh_du * -1.f; dh_dv = dh_dv * -1.f; }
         vi.dcur = m2(dot(v.ds_du, h) + dot(v.s, dh_du), dot(v.ds_dv, h) + dot(v.s, dh_dv),
                      dot(v.dt_du, h) + dot(v.t, dh_du), dot(v.dt_dv, h) + dot(v.t, dh_dv));
+        vi.dcur.m00 = vi.dcur.m00 * g_plant[PLANT_MS_HV_JAC];
         if (i < k - 1) {
             const MV &vn = P.v[i + 1].v;
             dh_du = ilo * (vn.dp_du - wo * dot(wo, vn.dp_du));
@@ -2600,7 +2623,7 @@ static float ms_geometric_term(const MV &vx, const MV &vy, const MSPath &P) {
         V3 d = normalize(v[k - 1].v.p - v[k - 2].v.p);
         return G / fabsf(dot(d, v[k - 1].v.n));
     }
-    float dx1_dxend = ms_invert_tridiagonal_geo(v, k);
+    float dx1_dxend = ms_invert_tridiagonal_geo(v, k) * g_plant[PLANT_MS_G];
     V3 d = vx.p - v[0].v.p;
     float inv_r2 = 1.f / sqnorm(d);
     d = d * sqrtf(inv_r2);
@@ -3102,6 +3125,89 @@ void oracle_lean_texel(void *h, int nmap, int level, int x, int y, float *out5) 
     const int res = nm.res >> level;
     for (int c = 0; c < 5; ++c) out5[c] = nm.lean[level][((size_t) y * res + x) * 5 + c];
 }
+/* Set a planted-error factor (tests only; see the PLANT_* sites).  Returns 0, or -1 for an
+   unknown site. */
+int oracle_plant(int site, float factor) {
+    if (site < 0 || site >= PLANT_N) return -1;
+    g_plant[site] = factor;
+    return 0;
+}
+
+/* The pieces of evaluate_path_contribution SS (manifold_ss.cpp:281-327) for a receiver query, an
+   emitter interaction and the specular vertex hit by the ray from the query point toward p1:
+   out[0] = geometric term, out[1] = specular_reflectance (first channel), out[2] = emitter
+   weight (first channel), out[3..5] = the contribution, out[6..8] = the vertex used.  Returns 0
+   when the ray does not reach a caster. */
+int oracle_ss_contribution(void *h, const sms_ss_query *q, const sms_emitter_interaction *e, const float *p1,
+                           float *out) {
+    const Scene &S = *(Scene *) h;
+    SI si = si_from_query(*q);
+    V3 x = si.p;
+    SI sf = ray_intersect(S, make_ray(x, normalize(v3(p1[0], p1[1], p1[2]) - x)));
+    if (!sf.valid || !(S.shapes[sf.shape].flags & SMS_FLAG_CAUSTIC_CASTER_SS)) return 0;
+    EI ei = ei_from_c(*e);
+    V3 c = path_contribution_ss(S, si, ei, sf);
+    /* the geometric term and the emitter weight as path_contribution_ss forms them */
+    MV vtx = mv_from_si(S, sf);
+    make_orthonormal(vtx);
+    EI e2 = ei;
+    if (e2.kind == SMS_EI_POINT) {
+        const Emitter &em = S.emitters[e2.emitter];
+        V3 d = em.position - sf.p;
+        float dist = norm(d), inv_dist = 1.f / dist;
+        d = d * inv_dist;
+        float emitter_pdf = e2.pdf;
+        e2.p = em.position; e2.d = d; e2.n = d; e2.pdf = 1.f;
+        e2.weight = (em.intensity * (inv_dist * inv_dist)) * dist * dist;
+        e2.pdf *= emitter_pdf;
+        e2.weight = e2.weight * (1.f / emitter_pdf);
+    }
+    MV vy;
+    float G = 0.f;
+    if (emitter_interaction_to_vertex(S, e2, vtx.p, &vy)) {
+        MV vx = mv_from_si(S, si);
+        make_orthonormal(vx);
+        G = geometric_term_ss(vx, vtx, vy);
+    }
+    out[0] = G;
+    out[1] = specular_reflectance_ss(S, sf, normalize(e2.p - sf.p)).x;
+    out[2] = e2.weight.x;
+    out[3] = c.x; out[4] = c.y; out[5] = c.z;
+    out[6] = sf.p.x; out[7] = sf.p.y; out[8] = sf.p.z;
+    return 1;
+}
+
+/* Constraint values and Newton step of the multi-scatter solver (angle-difference or half-vector
+   constraints, manifold_ms.cpp:500-857, block-tridiagonal solve :1192-1232) for the chain that a
+   ray from x0 through p[0] and then through p[1], ... hits (k vertices), with emitter
+   interaction e.  Writes C (k x 2), dx (k x 2) and the vertices used (k x 3).  Returns 1 when the
+   step exists, 0 when the constraints or the solve fail, -1 when the chain is not hit. */
+int oracle_ms_step_at(void *h, int halfvector, const float *x0, int k, const float *p,
+                      const sms_emitter_interaction *e, float *C, float *dx, float *vp) {
+    const Scene &S = *(Scene *) h;
+    if (k < 1 || k > MS_MAXB) return -1;
+    MSPath P;
+    memset(&P, 0, sizeof(P));
+    V3 o = v3(x0[0], x0[1], x0[2]);
+    Ray ray = make_ray(o, normalize(v3(p[0], p[1], p[2]) - o));
+    for (int i = 0; i < k; ++i) {
+        SI si = ray_intersect(S, ray);
+        if (!si.valid) return -1;
+        P.v[i].v = mv_from_si(S, si);
+        P.shapes[i] = si.shape;
+        P.n++;
+        if (i + 1 < k) ray = spawn_ray(si.p, normalize(v3(p[3 * (i + 1)], p[3 * (i + 1) + 1], p[3 * (i + 1) + 2]) - si.p));
+    }
+    EI ei = ei_from_c(*e);
+    bool ok = halfvector ? ms_step_halfvector(P, o, ei) : ms_step_anglediff(P, o, ei);
+    for (int i = 0; i < k; ++i) {
+        C[2 * i] = P.v[i].C.x; C[2 * i + 1] = P.v[i].C.y;
+        dx[2 * i] = P.v[i].dx.x; dx[2 * i + 1] = P.v[i].dx.y;
+        vp[3 * i] = P.v[i].v.p.x; vp[3 * i + 1] = P.v[i].v.p.y; vp[3 * i + 2] = P.v[i].v.p.z;
+    }
+    return ok ? 1 : 0;
+}
+
 double oracle_exp_d(double x) { return exp_d(x); }
 double oracle_erf_d(double x) { return erf_d(x); }
 double oracle_log2_d(double x) { return log2_d(x); }

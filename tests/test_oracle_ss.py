@@ -4,8 +4,7 @@ not reuse its code:
 * brute force: on the glass dome every converged walk must land on a refraction solution
   found by an FP64 root search on the great circle through receiver, light and centre;
 * closed form: on a plane mirror the walk must converge to the image-point construction;
-* finite differences: the Newton step J^-1 C of the angle-difference constraint must match an
-  FP64 finite-difference Jacobian of an independent numpy statement of the constraint;
+* finite differences of both constraints' Newton steps: tests/test_oracle_pins.py;
 * statistics: the unbiased (Bernoulli) estimator and the biased (M = 64) estimator agree in
   expectation when the trial set finds every solution.
 """
@@ -150,78 +149,6 @@ def test_plane_mirror_closed_form():
         ref = x + t * (img - x)
         assert np.linalg.norm(p - ref) / np.linalg.norm(ref) < 1e-4
     o.close()
-
-
-def _constraint(u, v, x, y, c, r, eta):
-    """Angle-difference constraint of a vertex at sphere parameters (u, v), FP64."""
-    phi, th = 2 * math.pi * u, math.pi * v
-    loc = np.array([math.sin(th) * math.cos(phi), math.sin(th) * math.sin(phi), math.cos(th)])
-    p = c + r * loc
-    n = loc
-    wi = (x - p) / np.linalg.norm(x - p)
-    wo = (y - p) / np.linalg.norm(y - p)
-    wt = _refract(wi, n, eta)
-    if wt is None:
-        return None, p
-
-    def sph(w):
-        t = math.acos(max(-1, min(1, w[2])))
-        ph = math.atan2(w[1], w[0])
-        return t, ph + (2 * math.pi if ph < 0 else 0)
-
-    to, po = sph(wo)
-    ti, pi_ = sph(wt)
-    dp = po - pi_
-    if dp < -math.pi:
-        dp += 2 * math.pi
-    elif dp > math.pi:
-        dp -= 2 * math.pi
-    return np.array([to - ti, dp]), p
-
-
-def test_newton_step_matches_finite_differences(dome):
-    sc, o = dome
-    rng = np.random.default_rng(9)
-    c = np.zeros(3)
-    cases = 0
-    for _ in range(60):
-        x = np.array([rng.uniform(-0.6, 0.6), 0.0, rng.uniform(-0.6, 0.6)])
-        y = np.array([rng.uniform(-1, 1), 5.0, rng.uniform(-1, 1)])
-        q = workloads.uniform_sphere(rng.random(1), rng.uniform(0.0, 0.35, 1))[0]   # upper cap about +z
-        p = np.array([q[0], q[2], q[1]])                                             # rotate the cap to +y
-        d = (p - x) / np.linalg.norm(p - x)
-        e = np.zeros(1, abi.EI_DTYPE)
-        e["p"] = y
-        e["kind"] = abi.EI_AREA
-        Co = np.zeros(2, np.float32)
-        dXo = np.zeros(2, np.float32)
-        vp = np.zeros(3, np.float32)
-        xo = x.astype(np.float32)
-        do = d.astype(np.float32)
-        ok = o.lib.oracle_ss_step_at(o.h, 0, xo.ctypes.data, xo.ctypes.data, do.ctypes.data, e.ctypes.data,
-                                     Co.ctypes.data, dXo.ctypes.data, vp.ctypes.data)
-        if ok != 1:
-            continue
-        # re-evaluate at the float32 vertex the oracle actually used
-        loc = (vp.astype(np.float64) - c) / np.linalg.norm(vp.astype(np.float64) - c)
-        v0 = math.acos(loc[2]) / math.pi
-        u0 = (math.atan2(loc[1], loc[0]) % (2 * math.pi)) / (2 * math.pi)
-        C0, _ = _constraint(u0, v0, x, y, c, 1.0, ETA)
-        h = 1e-6
-        ev = [_constraint(u0 + h, v0, x, y, c, 1.0, ETA)[0], _constraint(u0 - h, v0, x, y, c, 1.0, ETA)[0],
-              _constraint(u0, v0 + h, x, y, c, 1.0, ETA)[0], _constraint(u0, v0 - h, x, y, c, 1.0, ETA)[0]]
-        if C0 is None or any(e is None for e in ev):
-            continue
-        J = np.zeros((2, 2))
-        J[:, 0] = (ev[0] - ev[1]) / (2 * h)
-        J[:, 1] = (ev[2] - ev[3]) / (2 * h)
-        if abs(np.linalg.det(J)) < 1e-3 or np.linalg.cond(J) > 1e4:
-            continue
-        dX = np.linalg.solve(J, C0)
-        assert np.allclose(Co, C0, atol=2e-5), (Co, C0)
-        assert np.linalg.norm(dXo - dX) <= 2e-3 * max(np.linalg.norm(dX), 1e-2), (dXo, dX)
-        cases += 1
-    assert cases >= 30
 
 
 def test_unbiased_and_biased_estimators_agree(dome):
