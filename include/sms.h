@@ -269,8 +269,17 @@ typedef struct sms_stats {
 /* ---------------------------------------------------------------- context */
 typedef struct sms_ctx sms_ctx;
 
-/* Copy the scene to the current HIP device, build the BVH, decode normal maps. */
+/* Copy the scene to the current HIP device, build the BVH, decode normal maps.  The scene is
+   validated and built on the host before any device call, so a bad description returns
+   SMS_E_INVALID (and a host allocation failure SMS_E_OOM) also where no device is visible. */
 sms_status sms_ctx_create(const sms_scene_desc *scene, const sms_config *cfg, sms_ctx **out);
+/* Host only, no device needed: validates a scene description as sms_ctx_create does and
+   returns in *device_bytes the device memory its scene arrays would take (normal-map slopes and
+   LEAN pyramid, shape records, mesh vertices and faces, BVH nodes, triangle records, area
+   CDFs), for capacity planning.  err (if not NULL) receives the failure reason.  No
+   counterpart in the reference (Scene construction, src/librender/scene.cpp:18-112, has no
+   dry run). */
+sms_status sms_scene_footprint(const sms_scene_desc *scene, uint64_t *device_bytes, char *err, int32_t err_len);
 sms_status sms_ctx_destroy(sms_ctx *ctx);
 sms_status sms_ctx_set_config(sms_ctx *ctx, const sms_config *cfg);
 /* Scheduling knobs of the persistent estimator kernels (chosen by the autotuner).

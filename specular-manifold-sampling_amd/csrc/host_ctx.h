@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <exception>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -82,6 +84,29 @@ namespace smshost {
 inline sms_status fail(sms_ctx *ctx, sms_status code, const std::string &msg) {
     if (ctx) ctx->err = msg;
     return code;
+}
+
+inline sms_status fail_noexcept(sms_ctx *ctx, sms_status code, const char *msg) noexcept {
+    if (ctx) {
+        try { ctx->err = msg; } catch (...) {}
+    }
+    return code;
+}
+
+// Runs the body of a C entry point with every C++ exception turned into a status code, so that
+// none crosses the C ABI: std::bad_alloc -> SMS_E_OOM (host allocation), anything else ->
+// SMS_E_INVALID with the exception's message as the context's last error.
+template <typename F>
+inline sms_status guarded(sms_ctx *ctx, F &&body) noexcept {
+    try {
+        return body();
+    } catch (const std::bad_alloc &) {
+        return fail_noexcept(ctx, SMS_E_OOM, "host allocation failed");
+    } catch (const std::exception &e) {
+        return fail_noexcept(ctx, SMS_E_INVALID, e.what());
+    } catch (...) {
+        return fail_noexcept(ctx, SMS_E_INVALID, "unexpected host exception");
+    }
 }
 
 #define HIP_TRY(ctx, call)                                                                      \

@@ -102,82 +102,96 @@ sms_status capture(sms_ctx *ctx, Body &&body, sms_graph **out) {
 extern "C" {
 
 sms_status sms_glint_graph_create(sms_ctx *ctx, const sms_glint_query *q, int64_t n, float *out_rgb, sms_graph **out) {
-    if (!ctx || !out) return fail(ctx, SMS_E_INVALID, "sms_glint_graph_create: null argument");
-    sms_status st = glint_prepare(ctx, q, n, out_rgb);
-    if (st) return st;
-    return capture(ctx, [&] { return glint_enqueue(ctx, q, n, out_rgb); }, out);
+    return guarded(ctx, [&]() -> sms_status {
+        if (!ctx || !out) return fail(ctx, SMS_E_INVALID, "sms_glint_graph_create: null argument");
+        sms_status st = glint_prepare(ctx, q, n, out_rgb);
+        if (st) return st;
+        return capture(ctx, [&] { return glint_enqueue(ctx, q, n, out_rgb); }, out);
+    });
 }
 
 sms_status sms_ss_graph_create(sms_ctx *ctx, const sms_ss_query *q, int64_t n, float *out_rgb, sms_graph **out) {
-    if (!ctx || !out) return fail(ctx, SMS_E_INVALID, "sms_ss_graph_create: null argument");
-    sms_status st = ss_prepare(ctx, q, n, out_rgb);
-    if (st) return st;
-    return capture(ctx, [&] { return ss_enqueue(ctx, q, n, out_rgb); }, out);
+    return guarded(ctx, [&]() -> sms_status {
+        if (!ctx || !out) return fail(ctx, SMS_E_INVALID, "sms_ss_graph_create: null argument");
+        sms_status st = ss_prepare(ctx, q, n, out_rgb);
+        if (st) return st;
+        return capture(ctx, [&] { return ss_enqueue(ctx, q, n, out_rgb); }, out);
+    });
 }
 
 sms_status sms_ms_graph_create(sms_ctx *ctx, const sms_ss_query *q, int64_t n, float *out_rgb, sms_graph **out) {
-    if (!ctx || !out) return fail(ctx, SMS_E_INVALID, "sms_ms_graph_create: null argument");
-    sms_status st = ms_prepare(ctx, q, n, out_rgb);
-    if (st) return st;
-    return capture(ctx, [&] { return ms_enqueue(ctx, q, n, out_rgb); }, out);
+    return guarded(ctx, [&]() -> sms_status {
+        if (!ctx || !out) return fail(ctx, SMS_E_INVALID, "sms_ms_graph_create: null argument");
+        sms_status st = ms_prepare(ctx, q, n, out_rgb);
+        if (st) return st;
+        return capture(ctx, [&] { return ms_enqueue(ctx, q, n, out_rgb); }, out);
+    });
 }
 
 sms_status sms_ss_pass_graph_create(sms_ctx *ctx, int32_t spp, sms_ss_query *q, float *weight, float *out_rgb,
                                     float *image, float *image_sq, float *image_n, sms_graph **out) {
-    if (!ctx || !out) return fail(ctx, SMS_E_INVALID, "sms_ss_pass_graph_create: null argument");
-    if (spp <= 0 || !q || !weight || !out_rgb || !image)
-        return fail(ctx, SMS_E_INVALID, "sms_ss_pass_graph_create: bad arguments");
-    if (!ctx->scene.has_cam) return fail(ctx, SMS_E_INVALID, "scene has no camera");
-    const int64_t pixels = (int64_t) ctx->scene.cam.w * ctx->scene.cam.h, n = pixels * spp;
-    sms_status st = ss_prepare(ctx, q, n, out_rgb);
-    if (st) return st;
-    return capture(ctx, [&] {
-        sms_status s2 = ss_queries_enqueue(ctx, spp, q, weight);
-        if (!s2) s2 = ss_enqueue(ctx, q, n, out_rgb);
-        if (!s2) s2 = film_enqueue(ctx, out_rgb, weight, pixels, spp, image, image_sq, image_n);
-        return s2;
-    }, out);
+    return guarded(ctx, [&]() -> sms_status {
+        if (!ctx || !out) return fail(ctx, SMS_E_INVALID, "sms_ss_pass_graph_create: null argument");
+        if (spp <= 0 || !q || !weight || !out_rgb || !image)
+            return fail(ctx, SMS_E_INVALID, "sms_ss_pass_graph_create: bad arguments");
+        if (!ctx->scene.has_cam) return fail(ctx, SMS_E_INVALID, "scene has no camera");
+        const int64_t pixels = (int64_t) ctx->scene.cam.w * ctx->scene.cam.h, n = pixels * spp;
+        sms_status st = ss_prepare(ctx, q, n, out_rgb);
+        if (st) return st;
+        return capture(ctx, [&] {
+            sms_status s2 = ss_queries_enqueue(ctx, spp, q, weight);
+            if (!s2) s2 = ss_enqueue(ctx, q, n, out_rgb);
+            if (!s2) s2 = film_enqueue(ctx, out_rgb, weight, pixels, spp, image, image_sq, image_n);
+            return s2;
+        }, out);
+    });
 }
 
 sms_status sms_graph_launch(sms_graph *g, uint64_t seed) {
-    if (!g) return SMS_E_INVALID;
-    sms_ctx *ctx = g->ctx;
-    if (g->gen != ctx->gen)
-        return fail(ctx, SMS_E_INVALID, "sms_graph_launch: the context's buffers or settings changed since the capture");
-    g->a_seed = seed;
-    void *args[4] = {(void *) &g->a_counters, (void *) &g->a_n, (void *) &g->a_seeds, (void *) &g->a_seed};
-    hipKernelNodeParams p = g->begin_params;
-    p.kernelParams = args;
-    p.extra = nullptr;
-    HIP_TRY(ctx, hipGraphExecKernelNodeSetParams(g->exec, g->begin, &p));
-    HIP_TRY(ctx, hipGraphLaunch(g->exec, ctx->stream));
-    return SMS_OK;
+    return guarded(g ? g->ctx : nullptr, [&]() -> sms_status {
+        if (!g) return SMS_E_INVALID;
+        sms_ctx *ctx = g->ctx;
+        if (g->gen != ctx->gen)
+            return fail(ctx, SMS_E_INVALID, "sms_graph_launch: the context's buffers or settings changed since the capture");
+        g->a_seed = seed;
+        void *args[4] = {(void *) &g->a_counters, (void *) &g->a_n, (void *) &g->a_seeds, (void *) &g->a_seed};
+        hipKernelNodeParams p = g->begin_params;
+        p.kernelParams = args;
+        p.extra = nullptr;
+        HIP_TRY(ctx, hipGraphExecKernelNodeSetParams(g->exec, g->begin, &p));
+        HIP_TRY(ctx, hipGraphLaunch(g->exec, ctx->stream));
+        return SMS_OK;
+    });
 }
 
 sms_status sms_graph_stats(sms_graph *g, sms_stats *stats, int32_t reset) {
-    if (!g) return SMS_E_INVALID;
-    sms_ctx *ctx = g->ctx;
-    unsigned long long h[N_FOLDED];
-    HIP_TRY(ctx, hipMemcpyAsync(h, g->total, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    if (stats) {
-        memset(stats, 0, sizeof(*stats));
-        stats->walks = h[0]; stats->walks_converged = h[1]; stats->newton_iterations = h[2];
-        stats->bernoulli_calls = h[3]; stats->bernoulli_trials = h[4]; stats->bernoulli_max = h[5];
-        stats->tail_iterations = h[6]; stats->overflow_queries = h[8];
-    }
-    if (reset) {
-        HIP_TRY(ctx, hipMemsetAsync(g->total, 0, sizeof(h), ctx->stream));
+    return guarded(g ? g->ctx : nullptr, [&]() -> sms_status {
+        if (!g) return SMS_E_INVALID;
+        sms_ctx *ctx = g->ctx;
+        unsigned long long h[N_FOLDED];
+        HIP_TRY(ctx, hipMemcpyAsync(h, g->total, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    }
-    return SMS_OK;
+        if (stats) {
+            memset(stats, 0, sizeof(*stats));
+            stats->walks = h[0]; stats->walks_converged = h[1]; stats->newton_iterations = h[2];
+            stats->bernoulli_calls = h[3]; stats->bernoulli_trials = h[4]; stats->bernoulli_max = h[5];
+            stats->tail_iterations = h[6]; stats->overflow_queries = h[8];
+        }
+        if (reset) {
+            HIP_TRY(ctx, hipMemsetAsync(g->total, 0, sizeof(h), ctx->stream));
+            HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        }
+        return SMS_OK;
+    });
 }
 
 sms_status sms_graph_destroy(sms_graph *g) {
-    if (!g) return SMS_E_INVALID;
-    (void) hipStreamSynchronize(g->ctx->stream);
-    free_graph(g);
-    return SMS_OK;
+    return guarded(g ? g->ctx : nullptr, [&]() -> sms_status {
+        if (!g) return SMS_E_INVALID;
+        (void) hipStreamSynchronize(g->ctx->stream);
+        free_graph(g);
+        return SMS_OK;
+    });
 }
 
 } // extern "C"

@@ -196,17 +196,51 @@ void camera_setup(DCamera &c, const sms_camera_desc &d) {
     c.w = d.width; c.h = d.height;
 }
 
-sms_status build_scene(sms_ctx *ctx, const sms_scene_desc *desc, const sms_config *cfg) {
-    DScene &S = ctx->scene;
-    memset(&S, 0, sizeof(S));
-    S.cfg = *cfg;
-    if (desc->n_shapes <= 0 || !desc->shapes) return fail(ctx, SMS_E_INVALID, "scene has no shapes");
+// Host-side form of a scene: everything sms_ctx_create uploads, built without a device.
+struct HostNormalmap { DNormalmap m; std::vector<float2> slope; std::vector<float> lean; };
+struct HostScene {
+    std::vector<HostNormalmap> nmaps;
+    std::vector<DEmitter> emitters;
+    std::vector<int> em_ss, em_ms, em_all;
+    bool area_emitters = false;
+    std::vector<DShape> shapes;
+    std::vector<int> analytic, casters, casters_ms;
+    std::vector<float4> vpos, vnrm;
+    std::vector<float2> vuv;
+    std::vector<uint4> faces;
+    std::vector<float> cdf, pmf;
+    std::vector<DNode> nodes;
+    std::vector<float4> tris;
+    float box_c[3] = {0.f, 0.f, 0.f}, box_inv = 1.f;
+    bool has_cam = false;
+    DCamera cam;
+
+    // device bytes of the scene arrays (what upload_scene allocates)
+    uint64_t device_bytes() const {
+        uint64_t b = 0;
+        for (const HostNormalmap &m : nmaps)
+            b += m.slope.size() * sizeof(float2) + m.lean.size() * sizeof(float) + sizeof(DNormalmap);
+        b += emitters.size() * sizeof(DEmitter) + (em_ss.size() + em_ms.size() + em_all.size()) * sizeof(int);
+        b += shapes.size() * sizeof(DShape) + (analytic.size() + casters.size() + casters_ms.size()) * sizeof(int);
+        b += (vpos.size() + vnrm.size()) * sizeof(float4) + vuv.size() * sizeof(float2) + faces.size() * sizeof(uint4);
+        b += (cdf.size() + pmf.size()) * sizeof(float) + nodes.size() * sizeof(DNode) + tris.size() * sizeof(float4);
+        return b;
+    }
+};
+
+// Validates a scene description and builds its host form (normal-map slopes and LEAN pyramid,
+// shape records, mesh arrays, the BVH); no device call.  On failure *err names the problem.
+sms_status prepare_scene(const sms_scene_desc *desc, HostScene &H, std::string &err) {
+    auto bad = [&](sms_status code, const char *msg) { err = msg; return code; };
+    if (desc->n_shapes <= 0 || !desc->shapes) return bad(SMS_E_INVALID, "scene has no shapes");
+    if (desc->n_emitters < 0 || (desc->n_emitters > 0 && !desc->emitters) || desc->n_normalmaps < 0 ||
+        (desc->n_normalmaps > 0 && !desc->normalmaps))
+        return bad(SMS_E_INVALID, "bad emitter or normal-map arrays");
     // normal maps: level-0 slopes (src/librender/normalmap.cpp:46-66)
-    std::vector<DNormalmap> nmaps;
     for (int i = 0; i < desc->n_normalmaps; ++i) {
         const sms_normalmap_desc &nd = desc->normalmaps[i];
         if (nd.res <= 0 || (nd.res & (nd.res - 1)) != 0 || !nd.rgb)
-            return fail(ctx, SMS_E_INVALID, "normal map must be square power-of-two");
+            return bad(SMS_E_INVALID, "normal map must be square power-of-two");
         const size_t res = (size_t) nd.res, stride = res + 1;
         // padded layout: column res and row res replicate column/row res-1 (edge clamp)
         std::vector<float2> sl(stride * stride);
@@ -221,12 +255,9 @@ sms_status build_scene(sms_ctx *ctx, const sms_scene_desc *desc, const sms_confi
                 sl[y * stride + x] = make_float2(-nx / nz, -ny / nz);
             }
         }
-        const float2 *dp;
-        sms_status st = upload(ctx, sl, &dp);
-        if (st) return st;
         DNormalmap m;
         memset(&m, 0, sizeof(m));
-        m.res = nd.res; m.stride = (int) stride; m.slope = dp;
+        m.res = nd.res; m.stride = (int) stride;
         // LEAN map (normalmap.cpp:36-87): level 0 = (s_x, s_y, s_x^2 + b, s_y^2 + b, s_x s_y) with
         // the base Beckmann variance b = 0.5 * 1e-4^2; coarser levels are the two-tap box resample
         // to half resolution (weights 0.5), along x first, then along y (bitmap.cpp:234-312)
@@ -242,7 +273,7 @@ sms_status build_scene(sms_ctx *ctx, const sms_scene_desc *desc, const sms_confi
         m.levels = 1;
         size_t off = 0;
         for (size_t r = res; r > 1; r /= 2) {
-            if (m.levels >= SMS_MAX_LEAN_LEVELS) return fail(ctx, SMS_E_INVALID, "normal map too large");
+            if (m.levels >= SMS_MAX_LEAN_LEVELS) return bad(SMS_E_INVALID, "normal map too large");
             const size_t h = r / 2;
             std::vector<float> tmp(h * r * 5);
             for (size_t y = 0; y < r; ++y)
@@ -261,14 +292,10 @@ sms_status build_scene(sms_ctx *ctx, const sms_scene_desc *desc, const sms_confi
             m.levels++;
             off = next;
         }
-        const float *dl;
-        st = upload(ctx, lean, &dl);
-        if (st) return st;
-        m.lean = dl;
-        nmaps.push_back(m);
+        H.nmaps.push_back(HostNormalmap{m, std::move(sl), std::move(lean)});
     }
-    std::vector<DEmitter> emitters;
-    std::vector<int> em_ss, em_ms, em_all;
+    std::vector<DEmitter> &emitters = H.emitters;
+    std::vector<int> &em_ss = H.em_ss, &em_ms = H.em_ms, &em_all = H.em_all;
     for (int i = 0; i < desc->n_emitters; ++i) {
         const sms_emitter_desc &e = desc->emitters[i];
         DEmitter d;
@@ -278,19 +305,19 @@ sms_status build_scene(sms_ctx *ctx, const sms_scene_desc *desc, const sms_confi
         d.ss = e.caustic_ss; d.ms = e.caustic_ms;
         if (e.type == SMS_EMITTER_AREA && (e.shape < 0 || e.shape >= desc->n_shapes ||
                                            desc->shapes[e.shape].type != SMS_SHAPE_SPHERE))
-            return fail(ctx, SMS_E_INVALID, "area emitters must sit on a sphere shape");
+            return bad(SMS_E_INVALID, "area emitters must sit on a sphere shape");
         emitters.push_back(d);
-        if (e.type == SMS_EMITTER_AREA) ctx->area_emitters = true;
+        if (e.type == SMS_EMITTER_AREA) H.area_emitters = true;
         em_all.push_back(i);
         if (e.caustic_ss) em_ss.push_back(i);
         if (e.caustic_ms) em_ms.push_back(i);
     }
-    std::vector<DShape> shapes;
-    std::vector<int> analytic, casters, casters_ms;
-    std::vector<float4> vpos, vnrm;
-    std::vector<float2> vuv;
-    std::vector<uint4> faces;
-    std::vector<float> cdf, pmf;
+    std::vector<DShape> &shapes = H.shapes;
+    std::vector<int> &analytic = H.analytic, &casters = H.casters, &casters_ms = H.casters_ms;
+    std::vector<float4> &vpos = H.vpos, &vnrm = H.vnrm;
+    std::vector<float2> &vuv = H.vuv;
+    std::vector<uint4> &faces = H.faces;
+    std::vector<float> &cdf = H.cdf, &pmf = H.pmf;
     for (int i = 0; i < desc->n_shapes; ++i) {
         const sms_shape_desc &d = desc->shapes[i];
         DShape sh;
@@ -306,9 +333,9 @@ sms_status build_scene(sms_ctx *ctx, const sms_scene_desc *desc, const sms_confi
         for (int a = 0; a < 3; ++a) sh.refl[a] = d.reflectance[a];
         sh.normalmap = d.normalmap; sh.tiles = d.tiles;
         if ((d.flags & SMS_FLAG_GLINTY) && d.bsdf != SMS_BSDF_NORMALMAP)
-            return fail(ctx, SMS_E_INVALID, "glinty shapes need the normal-mapped BSDF");
+            return bad(SMS_E_INVALID, "glinty shapes need the normal-mapped BSDF");
         if (d.bsdf == SMS_BSDF_NORMALMAP && (d.normalmap < 0 || d.normalmap >= desc->n_normalmaps))
-            return fail(ctx, SMS_E_INVALID, "normal-mapped shape without a normal map");
+            return bad(SMS_E_INVALID, "normal-mapped shape without a normal map");
         sh.bbox_center[0] = sh.cx; sh.bbox_center[1] = sh.cy; sh.bbox_center[2] = sh.cz;
         if (d.type == SMS_SHAPE_SPHERE) {
             sh.inv_surface_area = 1.f / (4.f * PI_F * sh.radius * sh.radius);
@@ -327,7 +354,7 @@ sms_status build_scene(sms_ctx *ctx, const sms_scene_desc *desc, const sms_confi
             analytic.push_back(i);
         } else if (d.type == SMS_SHAPE_MESH) {
             if (d.n_faces <= 0 || d.n_vertices <= 0 || !d.positions || !d.faces)
-                return fail(ctx, SMS_E_INVALID, "mesh without faces/positions");
+                return bad(SMS_E_INVALID, "mesh without faces/positions");
             int vbase = (int) vpos.size();
             sh.fbase = (int) faces.size(); sh.nf = d.n_faces;
             sh.has_n = d.normals != nullptr; sh.has_uv = d.texcoords != nullptr;
@@ -345,7 +372,7 @@ sms_status build_scene(sms_ctx *ctx, const sms_scene_desc *desc, const sms_confi
             for (int f = 0; f < d.n_faces; ++f) {
                 uint32_t i0 = d.faces[3 * f], i1 = d.faces[3 * f + 1], i2 = d.faces[3 * f + 2];
                 if (i0 >= (uint32_t) d.n_vertices || i1 >= (uint32_t) d.n_vertices || i2 >= (uint32_t) d.n_vertices)
-                    return fail(ctx, SMS_E_INVALID, "mesh face index out of range");
+                    return bad(SMS_E_INVALID, "mesh face index out of range");
                 faces.push_back(make_uint4(vbase + i0, vbase + i1, vbase + i2, (uint32_t) i));
                 const float *p0 = &d.positions[3 * i0], *p1 = &d.positions[3 * i1], *p2 = &d.positions[3 * i2];
                 float e0[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]}, e1[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
@@ -358,7 +385,7 @@ sms_status build_scene(sms_ctx *ctx, const sms_scene_desc *desc, const sms_confi
             sh.area_sum = (float) sum; sh.area_norm = (float) (1.0 / sum);
             sh.inv_surface_area = sh.area_norm;
         } else {
-            return fail(ctx, SMS_E_INVALID, "unknown shape type");
+            return bad(SMS_E_INVALID, "unknown shape type");
         }
         if (d.flags & SMS_FLAG_CAUSTIC_CASTER_SS) casters.push_back(i);
         if (d.flags & SMS_FLAG_CAUSTIC_CASTER_MS) casters_ms.push_back(i);
@@ -379,8 +406,10 @@ sms_status build_scene(sms_ctx *ctx, const sms_scene_desc *desc, const sms_confi
     }
     BvhOut bvh = build_bvh(items);
     if (bvh.max_depth > SMS_BVH_STACK)
-        return fail(ctx, SMS_E_UNSUPPORTED, "BVH deeper than the traversal stack");
-    std::vector<float4> tris(3 * bvh.order.size());
+        return bad(SMS_E_UNSUPPORTED, "BVH deeper than the traversal stack");
+    H.nodes = std::move(bvh.nodes);
+    std::vector<float4> &tris = H.tris;
+    tris.resize(3 * bvh.order.size());
     std::vector<int> face_prim_base(desc->n_shapes, 0);
     for (int i = 0; i < desc->n_shapes; ++i) face_prim_base[i] = shapes[i].fbase;
     for (size_t k = 0; k < bvh.order.size(); ++k) {
@@ -410,38 +439,58 @@ sms_status build_scene(sms_ctx *ctx, const sms_scene_desc *desc, const sms_confi
         }
     }
     float half = 0.f;
-    for (int a = 0; a < 3; ++a) { S.box_c[a] = 0.5f * (lo[a] + hi[a]); half = std::max(half, 0.5f * (hi[a] - lo[a])); }
-    S.box_inv = half > 0.f ? 1.f / half : 1.f;
-    ctx->h_vuv = vuv; ctx->h_faces = faces; ctx->h_shapes = shapes;
+    for (int a = 0; a < 3; ++a) { H.box_c[a] = 0.5f * (lo[a] + hi[a]); half = std::max(half, 0.5f * (hi[a] - lo[a])); }
+    H.box_inv = half > 0.f ? 1.f / half : 1.f;
+    H.has_cam = desc->camera.width > 0 && desc->camera.height > 0;
+    if (H.has_cam) camera_setup(H.cam, desc->camera);
+    return SMS_OK;
+}
+
+// Copies a prepared scene to the context's device.
+sms_status upload_scene(sms_ctx *ctx, HostScene &H, const sms_config *cfg) {
+    DScene &S = ctx->scene;
+    memset(&S, 0, sizeof(S));
+    S.cfg = *cfg;
     sms_status st;
-    if ((st = upload(ctx, shapes, &S.shapes))) return st;
-    S.n_shapes = (int) shapes.size();
-    if ((st = upload(ctx, analytic, &S.analytic))) return st;
-    S.n_analytic = (int) analytic.size();
-    if ((st = upload(ctx, emitters, &S.emitters))) return st;
-    S.n_emitters = (int) emitters.size();
-    if ((st = upload(ctx, em_ss, &S.emitters_ss))) return st;
-    S.n_emitters_ss = (int) em_ss.size();
-    if ((st = upload(ctx, em_all, &S.emitters_all))) return st;
-    S.n_emitters_all = (int) em_all.size();
-    if ((st = upload(ctx, casters, &S.casters_ss))) return st;
-    S.n_casters_ss = (int) casters.size();
-    if ((st = upload(ctx, em_ms, &S.emitters_ms))) return st;
-    S.n_emitters_ms = (int) em_ms.size();
-    if ((st = upload(ctx, casters_ms, &S.casters_ms))) return st;
-    S.n_casters_ms = (int) casters_ms.size();
+    std::vector<DNormalmap> nmaps;
+    for (HostNormalmap &hm : H.nmaps) {
+        DNormalmap m = hm.m;
+        if ((st = upload(ctx, hm.slope, &m.slope))) return st;
+        if ((st = upload(ctx, hm.lean, &m.lean))) return st;
+        nmaps.push_back(m);
+    }
+    ctx->area_emitters = H.area_emitters;
+    for (int a = 0; a < 3; ++a) S.box_c[a] = H.box_c[a];
+    S.box_inv = H.box_inv;
+    ctx->h_vuv = H.vuv; ctx->h_faces = H.faces; ctx->h_shapes = H.shapes;
+    if ((st = upload(ctx, H.shapes, &S.shapes))) return st;
+    S.n_shapes = (int) H.shapes.size();
+    if ((st = upload(ctx, H.analytic, &S.analytic))) return st;
+    S.n_analytic = (int) H.analytic.size();
+    if ((st = upload(ctx, H.emitters, &S.emitters))) return st;
+    S.n_emitters = (int) H.emitters.size();
+    if ((st = upload(ctx, H.em_ss, &S.emitters_ss))) return st;
+    S.n_emitters_ss = (int) H.em_ss.size();
+    if ((st = upload(ctx, H.em_all, &S.emitters_all))) return st;
+    S.n_emitters_all = (int) H.em_all.size();
+    if ((st = upload(ctx, H.casters, &S.casters_ss))) return st;
+    S.n_casters_ss = (int) H.casters.size();
+    if ((st = upload(ctx, H.em_ms, &S.emitters_ms))) return st;
+    S.n_emitters_ms = (int) H.em_ms.size();
+    if ((st = upload(ctx, H.casters_ms, &S.casters_ms))) return st;
+    S.n_casters_ms = (int) H.casters_ms.size();
     if ((st = upload(ctx, nmaps, &S.nmaps))) return st;
-    if ((st = upload(ctx, vpos, &S.vpos))) return st;
-    if ((st = upload(ctx, vnrm, &S.vnrm))) return st;
-    if ((st = upload(ctx, vuv, &S.vuv))) return st;
-    if ((st = upload(ctx, faces, &S.faces))) return st;
-    if ((st = upload(ctx, bvh.nodes, &S.nodes))) return st;
-    S.n_nodes = (int) bvh.nodes.size();
-    if ((st = upload(ctx, tris, &S.tris))) return st;
-    if ((st = upload(ctx, cdf, &S.cdf))) return st;
-    if ((st = upload(ctx, pmf, &S.pmf))) return st;
-    S.has_cam = desc->camera.width > 0 && desc->camera.height > 0;
-    if (S.has_cam) camera_setup(S.cam, desc->camera);
+    if ((st = upload(ctx, H.vpos, &S.vpos))) return st;
+    if ((st = upload(ctx, H.vnrm, &S.vnrm))) return st;
+    if ((st = upload(ctx, H.vuv, &S.vuv))) return st;
+    if ((st = upload(ctx, H.faces, &S.faces))) return st;
+    if ((st = upload(ctx, H.nodes, &S.nodes))) return st;
+    S.n_nodes = (int) H.nodes.size();
+    if ((st = upload(ctx, H.tris, &S.tris))) return st;
+    if ((st = upload(ctx, H.cdf, &S.cdf))) return st;
+    if ((st = upload(ctx, H.pmf, &S.pmf))) return st;
+    S.has_cam = H.has_cam;
+    if (S.has_cam) S.cam = H.cam;
     return SMS_OK;
 }
 
@@ -672,38 +721,22 @@ sms_status occupancy(sms_ctx *ctx, K kernel, int *out) {
 
 } // namespace
 
-// ==================================================================== C ABI
-extern "C" {
+namespace {
 
-void sms_config_default(sms_config *cfg) {
-    if (!cfg) return;
-    cfg->biased = 0; cfg->twostage = 0; cfg->halfvector_constraints = 0; cfg->mnee_init = 0;
-    cfg->step_scale = 1.f; cfg->max_iterations = 20; cfg->solver_threshold = 1e-5f;
-    cfg->uniqueness_threshold = 1e-4f; cfg->max_trials = -1; cfg->bounces = 1;
-    cfg->bsdf_strategy_only = 0; cfg->sms_strategy_only = 0;
-}
-
-sms_status sms_ctx_create(const sms_scene_desc *scene, const sms_config *cfg, sms_ctx **out) {
-    if (!out || !scene || !cfg) return SMS_E_INVALID;
-    *out = nullptr;
-    int n_dev = 0;
-    if (hipGetDeviceCount(&n_dev) != hipSuccess || n_dev == 0) return SMS_E_NO_DEVICE;
-    sms_ctx *ctx = new (std::nothrow) sms_ctx();
-    if (!ctx) return SMS_E_OOM;
-    sms_status st = check_config(ctx, cfg);
-    if (st) { delete ctx; return st; }
-    if (hipGetDevice(&ctx->device) != hipSuccess) { delete ctx; return SMS_E_HIP; }
+// Device part of sms_ctx_create: stream, events, scene upload, occupancy queries, work buffers.
+sms_status create_on_device(sms_ctx *ctx, HostScene &H, const sms_config *cfg) {
+    sms_status st;
+    HIP_TRY(ctx, hipGetDevice(&ctx->device));
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, ctx->device) != hipSuccess) { delete ctx; return SMS_E_HIP; }
+    HIP_TRY(ctx, hipGetDeviceProperties(&prop, ctx->device));
     ctx->n_cu = prop.multiProcessorCount;
     snprintf(ctx->dev_name, sizeof(ctx->dev_name), "%s (%s)", prop.name, prop.gcnArchName);
-    if (hipStreamCreate(&ctx->stream) != hipSuccess || hipEventCreate(&ctx->ev0) != hipSuccess ||
-        hipEventCreate(&ctx->ev1) != hipSuccess || hipEventCreate(&ctx->ev2) != hipSuccess ||
-        hipEventCreate(&ctx->ev3) != hipSuccess) {
-        sms_ctx_destroy(ctx);
-        return SMS_E_HIP;
-    }
-    st = build_scene(ctx, scene, cfg);
+    HIP_TRY(ctx, hipStreamCreate(&ctx->stream));
+    HIP_TRY(ctx, hipEventCreate(&ctx->ev0));
+    HIP_TRY(ctx, hipEventCreate(&ctx->ev1));
+    HIP_TRY(ctx, hipEventCreate(&ctx->ev2));
+    HIP_TRY(ctx, hipEventCreate(&ctx->ev3));
+    st = upload_scene(ctx, H, cfg);
     if (!st) st = occupancy(ctx, k_glint_trials_g8, &ctx->occ_glint);
     if (!st) st = occupancy(ctx, k_glint_coop_rounds, &ctx->occ_coop);
     if (!st) st = occupancy(ctx, k_glint_multiwave, &ctx->occ_mw);
@@ -721,137 +754,223 @@ sms_status sms_ctx_create(const sms_scene_desc *scene, const sms_config *cfg, sm
     if (!st) st = occupancy(ctx, k_ms_biased_2, &ctx->occ_msbiased[0]);
     if (!st) st = occupancy(ctx, k_ms_biased_4, &ctx->occ_msbiased[1]);
     if (!st) st = ensure_work(ctx, 0, 0);
-    if (st) {
-        fprintf(stderr, "sms_ctx_create: %s\n", ctx->err.c_str());
-        sms_ctx_destroy(ctx);
-        return st;
-    }
-    *out = ctx;
-    return SMS_OK;
+    return st;
 }
 
+} // namespace
+
+// ==================================================================== C ABI
+extern "C" {
+
+void sms_config_default(sms_config *cfg) {
+    if (!cfg) return;
+    cfg->biased = 0; cfg->twostage = 0; cfg->halfvector_constraints = 0; cfg->mnee_init = 0;
+    cfg->step_scale = 1.f; cfg->max_iterations = 20; cfg->solver_threshold = 1e-5f;
+    cfg->uniqueness_threshold = 1e-4f; cfg->max_trials = -1; cfg->bounces = 1;
+    cfg->bsdf_strategy_only = 0; cfg->sms_strategy_only = 0;
+}
+
+sms_status sms_ctx_create(const sms_scene_desc *scene, const sms_config *cfg, sms_ctx **out) {
+    if (!out || !scene || !cfg) return SMS_E_INVALID;
+    *out = nullptr;
+    sms_ctx *ctx = nullptr;
+    try {
+        sms_status st = check_config(nullptr, cfg);
+        if (st) return st;
+        // the scene's host form first: a bad description or a host allocation failure is reported
+        // before any device is touched
+        HostScene H;
+        std::string err;
+        st = prepare_scene(scene, H, err);
+        if (st) {
+            fprintf(stderr, "sms_ctx_create: %s\n", err.c_str());
+            return st;
+        }
+        int n_dev = 0;
+        if (hipGetDeviceCount(&n_dev) != hipSuccess || n_dev == 0) return SMS_E_NO_DEVICE;
+        ctx = new sms_ctx();
+        st = create_on_device(ctx, H, cfg);
+        if (st) {
+            fprintf(stderr, "sms_ctx_create: %s\n", ctx->err.c_str());
+            sms_ctx_destroy(ctx);
+            return st;
+        }
+        *out = ctx;
+        return SMS_OK;
+    } catch (const std::bad_alloc &) {
+        if (ctx) sms_ctx_destroy(ctx);
+        return SMS_E_OOM;
+    } catch (...) {
+        if (ctx) sms_ctx_destroy(ctx);
+        return SMS_E_INVALID;
+    }
+}
+
+sms_status sms_scene_footprint(const sms_scene_desc *scene, uint64_t *device_bytes, char *err, int32_t err_len) {
+    if (!scene || !device_bytes) return SMS_E_INVALID;
+    try {
+        HostScene H;
+        std::string msg;
+        sms_status st = prepare_scene(scene, H, msg);
+        if (err && err_len > 0) snprintf(err, (size_t) err_len, "%s", msg.c_str());
+        *device_bytes = st ? 0 : H.device_bytes();
+        return st;
+    } catch (const std::bad_alloc &) {
+        if (err && err_len > 0) snprintf(err, (size_t) err_len, "host allocation failed");
+        return SMS_E_OOM;
+    } catch (...) {
+        if (err && err_len > 0) snprintf(err, (size_t) err_len, "unexpected host exception");
+        return SMS_E_INVALID;
+    }
+}
+
+
 sms_status sms_ctx_destroy(sms_ctx *ctx) {
-    if (!ctx) return SMS_E_INVALID;
-    if (ctx->stream) (void) hipStreamSynchronize(ctx->stream);
-    for (void *p : ctx->scene_allocs) (void) hipFree(p);
-    if (ctx->counters) (void) hipFree(ctx->counters);
-    if (ctx->overflow) (void) hipFree(ctx->overflow);
-    if (ctx->part) (void) hipFree(ctx->part);
-    if (ctx->invp) (void) hipFree(ctx->invp);
-    if (ctx->trial) (void) hipFree(ctx->trial);
-    if (ctx->long_items) (void) hipFree(ctx->long_items);
-    if (ctx->seq_items) (void) hipFree(ctx->seq_items);
-    if (ctx->mw_found) (void) hipFree(ctx->mw_found);
-    if (ctx->mw_next) (void) hipFree(ctx->mw_next);
-    if (ctx->mw_log) (void) hipFree(ctx->mw_log);
-    if (ctx->gover) (void) hipFree(ctx->gover);
-    if (ctx->res) (void) hipFree(ctx->res);
-    if (ctx->seeds) (void) hipFree(ctx->seeds);
-    proposal_release(ctx);
-    if (ctx->ev3) (void) hipEventDestroy(ctx->ev3);
-    if (ctx->ev0) (void) hipEventDestroy(ctx->ev0);
-    if (ctx->ev1) (void) hipEventDestroy(ctx->ev1);
-    if (ctx->ev2) (void) hipEventDestroy(ctx->ev2);
-    if (ctx->stream) (void) hipStreamDestroy(ctx->stream);
-    delete ctx;
-    return SMS_OK;
+    return guarded(ctx, [&]() -> sms_status {
+        if (!ctx) return SMS_E_INVALID;
+        if (ctx->stream) (void) hipStreamSynchronize(ctx->stream);
+        for (void *p : ctx->scene_allocs) (void) hipFree(p);
+        if (ctx->counters) (void) hipFree(ctx->counters);
+        if (ctx->overflow) (void) hipFree(ctx->overflow);
+        if (ctx->part) (void) hipFree(ctx->part);
+        if (ctx->invp) (void) hipFree(ctx->invp);
+        if (ctx->trial) (void) hipFree(ctx->trial);
+        if (ctx->long_items) (void) hipFree(ctx->long_items);
+        if (ctx->seq_items) (void) hipFree(ctx->seq_items);
+        if (ctx->mw_found) (void) hipFree(ctx->mw_found);
+        if (ctx->mw_next) (void) hipFree(ctx->mw_next);
+        if (ctx->mw_log) (void) hipFree(ctx->mw_log);
+        if (ctx->gover) (void) hipFree(ctx->gover);
+        if (ctx->res) (void) hipFree(ctx->res);
+        if (ctx->seeds) (void) hipFree(ctx->seeds);
+        proposal_release(ctx);
+        if (ctx->ev3) (void) hipEventDestroy(ctx->ev3);
+        if (ctx->ev0) (void) hipEventDestroy(ctx->ev0);
+        if (ctx->ev1) (void) hipEventDestroy(ctx->ev1);
+        if (ctx->ev2) (void) hipEventDestroy(ctx->ev2);
+        if (ctx->stream) (void) hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return SMS_OK;
+    });
 }
 
 sms_status sms_ctx_set_config(sms_ctx *ctx, const sms_config *cfg) {
-    if (!ctx) return SMS_E_INVALID;
-    sms_status st = check_config(ctx, cfg);
-    if (st) return st;
-    ctx->scene.cfg = *cfg;
-    ctx->gen++;
-    return SMS_OK;
+    return guarded(ctx, [&]() -> sms_status {
+        if (!ctx) return SMS_E_INVALID;
+        sms_status st = check_config(ctx, cfg);
+        if (st) return st;
+        ctx->scene.cfg = *cfg;
+        ctx->gen++;
+        return SMS_OK;
+    });
 }
 
 const char *sms_ctx_last_error(const sms_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
 sms_status sms_ctx_set_schedule(sms_ctx *ctx, const sms_schedule *s) {
-    if (!ctx || !s) return SMS_E_INVALID;
-    const int g = s->trial_group;
-    if (s->lane_trials < 1 || s->blocks_per_cu < 0 || s->tail_rounds < 0 ||
-        !(g == 0 || g == 1 || g == 2 || g == 4 || g == 8 || g == 16) ||
-        s->long_chains < 0 || s->long_chains > MW_ITEMS || s->long_trials < 64 || s->long_trials > MW_TRIALS ||
-        s->long_trials % 64 != 0 || s->ss_first_pass < 0 || s->ss_first_pass > 1)
-        return fail(ctx, SMS_E_INVALID, "bad schedule");
-    sms_schedule v = *s;
-    if (v.trial_group == 0) v.trial_group = 8;
-    apply_schedule(ctx, v);
-    return SMS_OK;
+    return guarded(ctx, [&]() -> sms_status {
+        if (!ctx || !s) return SMS_E_INVALID;
+        const int g = s->trial_group;
+        if (s->lane_trials < 1 || s->blocks_per_cu < 0 || s->tail_rounds < 0 ||
+            !(g == 0 || g == 1 || g == 2 || g == 4 || g == 8 || g == 16) ||
+            s->long_chains < 0 || s->long_chains > MW_ITEMS || s->long_trials < 64 || s->long_trials > MW_TRIALS ||
+            s->long_trials % 64 != 0 || s->ss_first_pass < 0 || s->ss_first_pass > 1)
+            return fail(ctx, SMS_E_INVALID, "bad schedule");
+        sms_schedule v = *s;
+        if (v.trial_group == 0) v.trial_group = 8;
+        apply_schedule(ctx, v);
+        return SMS_OK;
+    });
 }
 
 sms_status sms_ctx_get_schedule(const sms_ctx *ctx, sms_schedule *s) {
-    if (!ctx || !s) return SMS_E_INVALID;
-    *s = schedule_of(ctx);
-    return SMS_OK;
+    return guarded(const_cast<sms_ctx *>(ctx), [&]() -> sms_status {
+        if (!ctx || !s) return SMS_E_INVALID;
+        *s = schedule_of(ctx);
+        return SMS_OK;
+    });
 }
 
 sms_status sms_ctx_device_info(const sms_ctx *ctx, char *name, int32_t name_len, int32_t *n_cu) {
-    if (!ctx) return SMS_E_INVALID;
-    if (name && name_len > 0) snprintf(name, (size_t) name_len, "%s", ctx->dev_name);
-    if (n_cu) *n_cu = ctx->n_cu;
-    return SMS_OK;
+    return guarded(const_cast<sms_ctx *>(ctx), [&]() -> sms_status {
+        if (!ctx) return SMS_E_INVALID;
+        if (name && name_len > 0) snprintf(name, (size_t) name_len, "%s", ctx->dev_name);
+        if (n_cu) *n_cu = ctx->n_cu;
+        return SMS_OK;
+    });
 }
 
 sms_status sms_ctx_sync(sms_ctx *ctx) {
-    if (!ctx) return SMS_E_INVALID;
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    return SMS_OK;
+    return guarded(ctx, [&]() -> sms_status {
+        if (!ctx) return SMS_E_INVALID;
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        return SMS_OK;
+    });
 }
 
 sms_status sms_ctx_last_kernel_ms(const sms_ctx *ctx, float *ms) {
-    if (!ctx || !ms) return SMS_E_INVALID;
-    ms[0] = ctx->last_ms;
-    ms[1] = ctx->last_ms2;
-    return SMS_OK;
+    return guarded(const_cast<sms_ctx *>(ctx), [&]() -> sms_status {
+        if (!ctx || !ms) return SMS_E_INVALID;
+        ms[0] = ctx->last_ms;
+        ms[1] = ctx->last_ms2;
+        return SMS_OK;
+    });
 }
 
 sms_status sms_ss_walk_batch(sms_ctx *ctx, const sms_ss_walk_in *in, int64_t n, sms_ss_walk_out *out) {
-    if (!ctx || n < 0 || (n > 0 && (!in || !out))) return fail(ctx, SMS_E_INVALID, "bad walk batch arguments");
-    if (n == 0) return SMS_OK;
-    int blocks = (int) ((n + 255) / 256);
-    hipLaunchKernelGGL(k_ss_walk, dim3(blocks), dim3(256), 0, ctx->stream, ctx->scene, in, n, out);
-    HIP_TRY(ctx, hipGetLastError());
-    return SMS_OK;
+    return guarded(ctx, [&]() -> sms_status {
+        if (!ctx || n < 0 || (n > 0 && (!in || !out))) return fail(ctx, SMS_E_INVALID, "bad walk batch arguments");
+        if (n == 0) return SMS_OK;
+        int blocks = (int) ((n + 255) / 256);
+        hipLaunchKernelGGL(k_ss_walk, dim3(blocks), dim3(256), 0, ctx->stream, ctx->scene, in, n, out);
+        HIP_TRY(ctx, hipGetLastError());
+        return SMS_OK;
+    });
 }
 
 sms_status sms_glint_walk_batch(sms_ctx *ctx, const sms_glint_walk_in *in, int64_t n, sms_glint_walk_out *out) {
-    if (!ctx || n < 0 || (n > 0 && (!in || !out))) return fail(ctx, SMS_E_INVALID, "bad walk batch arguments");
-    if (n == 0) return SMS_OK;
-    if (!ctx->scene.nmaps) return fail(ctx, SMS_E_INVALID, "scene has no normal map");
-    int blocks = (int) ((n + 255) / 256);
-    hipLaunchKernelGGL(k_glint_walk, dim3(blocks), dim3(256), 0, ctx->stream, ctx->scene, in, n, out);
-    HIP_TRY(ctx, hipGetLastError());
-    return SMS_OK;
+    return guarded(ctx, [&]() -> sms_status {
+        if (!ctx || n < 0 || (n > 0 && (!in || !out))) return fail(ctx, SMS_E_INVALID, "bad walk batch arguments");
+        if (n == 0) return SMS_OK;
+        if (!ctx->scene.nmaps) return fail(ctx, SMS_E_INVALID, "scene has no normal map");
+        int blocks = (int) ((n + 255) / 256);
+        hipLaunchKernelGGL(k_glint_walk, dim3(blocks), dim3(256), 0, ctx->stream, ctx->scene, in, n, out);
+        HIP_TRY(ctx, hipGetLastError());
+        return SMS_OK;
+    });
 }
 
 sms_status sms_ss_make_queries(sms_ctx *ctx, int32_t spp, uint64_t seed, sms_ss_query *q, float *weight) {
-    if (!ctx || spp <= 0 || !q || !weight) return fail(ctx, SMS_E_INVALID, "bad query arguments");
-    if (!ctx->scene.has_cam) return fail(ctx, SMS_E_INVALID, "scene has no camera");
-    hipLaunchKernelGGL(k_set_seed, dim3(1), dim3(64), 0, ctx->stream, ctx->seeds + SEED_CAM, seed);
-    HIP_TRY(ctx, hipGetLastError());
-    return ss_queries_enqueue(ctx, spp, q, weight);
+    return guarded(ctx, [&]() -> sms_status {
+        if (!ctx || spp <= 0 || !q || !weight) return fail(ctx, SMS_E_INVALID, "bad query arguments");
+        if (!ctx->scene.has_cam) return fail(ctx, SMS_E_INVALID, "scene has no camera");
+        hipLaunchKernelGGL(k_set_seed, dim3(1), dim3(64), 0, ctx->stream, ctx->seeds + SEED_CAM, seed);
+        HIP_TRY(ctx, hipGetLastError());
+        return ss_queries_enqueue(ctx, spp, q, weight);
+    });
 }
 
 sms_status sms_film_accumulate(sms_ctx *ctx, const float *rgb, const float *weight, int64_t pixels, int32_t spp,
                                float *image, float *image_sq, float *image_n) {
-    if (!ctx || pixels < 0 || spp <= 0 || (pixels > 0 && (!rgb || !weight || !image)))
-        return fail(ctx, SMS_E_INVALID, "bad film arguments");
-    return film_enqueue(ctx, rgb, weight, pixels, spp, image, image_sq, image_n);
+    return guarded(ctx, [&]() -> sms_status {
+        if (!ctx || pixels < 0 || spp <= 0 || (pixels > 0 && (!rgb || !weight || !image)))
+            return fail(ctx, SMS_E_INVALID, "bad film arguments");
+        return film_enqueue(ctx, rgb, weight, pixels, spp, image, image_sq, image_n);
+    });
 }
 
 sms_status sms_glint_make_queries(sms_ctx *ctx, int32_t spp, uint64_t seed, sms_glint_query *q) {
-    if (!ctx || spp <= 0 || !q) return fail(ctx, SMS_E_INVALID, "bad query arguments");
-    if (!ctx->scene.has_cam) return fail(ctx, SMS_E_INVALID, "scene has no camera");
-    int64_t n = (int64_t) ctx->scene.cam.w * ctx->scene.cam.h * spp;
-    int blocks = (int) ((n + 255) / 256);
-    hipLaunchKernelGGL(k_set_seed, dim3(1), dim3(64), 0, ctx->stream, ctx->seeds + SEED_CAM, seed);
-    hipLaunchKernelGGL(k_glint_queries, dim3(blocks), dim3(256), 0, ctx->stream, ctx->scene, (int) spp,
-                       (const uint64_t *) ctx->seeds + SEED_CAM, q, n);
-    HIP_TRY(ctx, hipGetLastError());
-    return SMS_OK;
+    return guarded(ctx, [&]() -> sms_status {
+        if (!ctx || spp <= 0 || !q) return fail(ctx, SMS_E_INVALID, "bad query arguments");
+        if (!ctx->scene.has_cam) return fail(ctx, SMS_E_INVALID, "scene has no camera");
+        int64_t n = (int64_t) ctx->scene.cam.w * ctx->scene.cam.h * spp;
+        int blocks = (int) ((n + 255) / 256);
+        hipLaunchKernelGGL(k_set_seed, dim3(1), dim3(64), 0, ctx->stream, ctx->seeds + SEED_CAM, seed);
+        hipLaunchKernelGGL(k_glint_queries, dim3(blocks), dim3(256), 0, ctx->stream, ctx->scene, (int) spp,
+                           (const uint64_t *) ctx->seeds + SEED_CAM, q, n);
+        HIP_TRY(ctx, hipGetLastError());
+        return SMS_OK;
+    });
 }
 
 } // extern "C"
@@ -1074,165 +1193,187 @@ extern "C" {
 
 sms_status sms_glint_estimate_batch(sms_ctx *ctx, const sms_glint_query *q, int64_t n, uint64_t seed, float *out_rgb,
                                     sms_stats *stats) {
-    sms_status st = glint_prepare(ctx, q, n, out_rgb);
-    if (!st) st = frame_begin(ctx, seed);
-    if (!st) st = glint_enqueue(ctx, q, n, out_rgb);
-    return st ? st : estimator_done(ctx, stats);
+    return guarded(ctx, [&]() -> sms_status {
+        sms_status st = glint_prepare(ctx, q, n, out_rgb);
+        if (!st) st = frame_begin(ctx, seed);
+        if (!st) st = glint_enqueue(ctx, q, n, out_rgb);
+        return st ? st : estimator_done(ctx, stats);
+    });
 }
 
 sms_status sms_glint_autotune(sms_ctx *ctx, const sms_glint_query *q, int64_t n, uint64_t seed, float *out_rgb,
                               sms_schedule *best) {
-    if (!best) return fail(ctx, SMS_E_INVALID, "null schedule");
-    sms_status st = glint_prepare(ctx, q, n, out_rgb);
-    if (st) return st;
-    const sms_schedule keep = schedule_of(ctx);
-    sms_schedule win = keep;
-    float win_ms = INFINITY;
-    const int groups[4] = {2, 4, 8, 16}, mults[2] = {1, 2}, rounds[2] = {1, 2}, bpcs[2] = {0, 2 * ctx->occ_glint};
-    for (int g : groups) for (int m : mults) for (int r : rounds) for (int b : bpcs) {
-        ctx->trial_group = g; ctx->lane_trials = g * m; ctx->tail_rounds = r; ctx->blocks_per_cu = b;
-        // the first run warms caches and allocations; the best of the next three is kept, as a
-        // frame of a few hundred microseconds varies by several percent from run to run
-        float ms = INFINITY;
-        for (int rep = 0; rep < 4; ++rep) {
-            st = frame_begin(ctx, seed + (uint64_t) rep);
-            if (!st) st = glint_enqueue(ctx, q, n, out_rgb);
+    return guarded(ctx, [&]() -> sms_status {
+        if (!best) return fail(ctx, SMS_E_INVALID, "null schedule");
+        sms_status st = glint_prepare(ctx, q, n, out_rgb);
+        if (st) return st;
+        const sms_schedule keep = schedule_of(ctx);
+        sms_schedule win = keep;
+        float win_ms = INFINITY;
+        const int groups[4] = {2, 4, 8, 16}, mults[2] = {1, 2}, rounds[2] = {1, 2}, bpcs[2] = {0, 2 * ctx->occ_glint};
+        for (int g : groups) for (int m : mults) for (int r : rounds) for (int b : bpcs) {
+            ctx->trial_group = g; ctx->lane_trials = g * m; ctx->tail_rounds = r; ctx->blocks_per_cu = b;
+            // the first run warms caches and allocations; the best of the next three is kept, as a
+            // frame of a few hundred microseconds varies by several percent from run to run
+            float ms = INFINITY;
+            for (int rep = 0; rep < 4; ++rep) {
+                st = frame_begin(ctx, seed + (uint64_t) rep);
+                if (!st) st = glint_enqueue(ctx, q, n, out_rgb);
+                if (st) break;
+                HIP_TRY(ctx, hipEventSynchronize(ctx->ev2));
+                float t = 0.f;
+                HIP_TRY(ctx, hipEventElapsedTime(&t, ctx->ev0, ctx->ev2));
+                if (rep > 0) ms = std::min(ms, t);
+            }
             if (st) break;
-            HIP_TRY(ctx, hipEventSynchronize(ctx->ev2));
-            float t = 0.f;
-            HIP_TRY(ctx, hipEventElapsedTime(&t, ctx->ev0, ctx->ev2));
-            if (rep > 0) ms = std::min(ms, t);
+            if (ms < win_ms) {
+                win_ms = ms;
+                win = schedule_of(ctx);
+            }
         }
-        if (st) break;
-        if (ms < win_ms) {
-            win_ms = ms;
-            win = schedule_of(ctx);
-        }
-    }
-    apply_schedule(ctx, st ? keep : win);
-    if (st) return st;
-    *best = win;
-    return SMS_OK;
+        apply_schedule(ctx, st ? keep : win);
+        if (st) return st;
+        *best = win;
+        return SMS_OK;
+    });
 }
 
 sms_status sms_ss_autotune(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *out_rgb,
                            sms_schedule *best) {
-    if (!ctx) return SMS_E_INVALID;
-    if (!best) return fail(ctx, SMS_E_INVALID, "null schedule");
-    if (ctx->scene.cfg.biased) return fail(ctx, SMS_E_INVALID, "the autotuner times the unbiased estimator");
-    const sms_schedule keep = schedule_of(ctx);
-    sms_schedule win = keep;
-    float win_ms = INFINITY;
-    sms_status st = SMS_OK;
-    struct Cand { int first_pass, lane_trials; };
-    const Cand cands[8] = {{0, 8}, {0, 16}, {0, 32}, {0, 64}, {0, 128}, {1, 4}, {1, 8}, {1, 16}};
-    for (const Cand &cd : cands) {
-        ctx->ss_first_pass = cd.first_pass; ctx->lane_trials = cd.lane_trials; ctx->blocks_per_cu = 0;
-        float ms = 0.f;
-        for (int rep = 0; rep < 2; ++rep) {           // the first run warms caches and allocations
-            st = sms_ss_estimate_batch(ctx, q, n, seed + (uint64_t) rep, out_rgb, nullptr);
+    return guarded(ctx, [&]() -> sms_status {
+        if (!ctx) return SMS_E_INVALID;
+        if (!best) return fail(ctx, SMS_E_INVALID, "null schedule");
+        if (ctx->scene.cfg.biased) return fail(ctx, SMS_E_INVALID, "the autotuner times the unbiased estimator");
+        const sms_schedule keep = schedule_of(ctx);
+        sms_schedule win = keep;
+        float win_ms = INFINITY;
+        sms_status st = SMS_OK;
+        struct Cand { int first_pass, lane_trials; };
+        const Cand cands[8] = {{0, 8}, {0, 16}, {0, 32}, {0, 64}, {0, 128}, {1, 4}, {1, 8}, {1, 16}};
+        for (const Cand &cd : cands) {
+            ctx->ss_first_pass = cd.first_pass; ctx->lane_trials = cd.lane_trials; ctx->blocks_per_cu = 0;
+            float ms = 0.f;
+            for (int rep = 0; rep < 2; ++rep) {           // the first run warms caches and allocations
+                st = sms_ss_estimate_batch(ctx, q, n, seed + (uint64_t) rep, out_rgb, nullptr);
+                if (st) break;
+                ms = ctx->last_ms + ctx->last_ms2;
+            }
             if (st) break;
-            ms = ctx->last_ms + ctx->last_ms2;
+            if (ms < win_ms) {
+                win_ms = ms;
+                win = schedule_of(ctx);
+            }
         }
-        if (st) break;
-        if (ms < win_ms) {
-            win_ms = ms;
-            win = schedule_of(ctx);
-        }
-    }
-    apply_schedule(ctx, st ? keep : win);
-    if (st) return st;
-    *best = win;
-    return SMS_OK;
+        apply_schedule(ctx, st ? keep : win);
+        if (st) return st;
+        *best = win;
+        return SMS_OK;
+    });
 }
 
 sms_status sms_ms_autotune(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *out_rgb,
                            sms_schedule *best) {
-    if (!ctx) return SMS_E_INVALID;
-    if (!best) return fail(ctx, SMS_E_INVALID, "null schedule");
-    if (ctx->scene.cfg.biased) return fail(ctx, SMS_E_INVALID, "the autotuner times the unbiased estimator");
-    const sms_schedule keep = schedule_of(ctx);
-    sms_schedule win = keep;
-    float win_ms = INFINITY;
-    sms_status st = SMS_OK;
-    // the persistent pass refills a lane whose chain runs long, so longer lane runs pay off there
-    struct Cand { int first_pass, lane_trials; };
-    const Cand cands[11] = {{0, 4}, {0, 16}, {0, 64}, {0, 256}, {0, 1024},
-                            {1, 1}, {1, 2}, {1, 4}, {1, 8}, {1, 16}, {1, 32}};
-    for (const Cand &cd : cands) {
-        ctx->ss_first_pass = cd.first_pass;
-        ctx->lane_trials = cd.lane_trials;
-        ctx->blocks_per_cu = 0;
-        float ms = 0.f;
-        for (int rep = 0; rep < 2; ++rep) {           // the first run warms caches and allocations
-            st = sms_ms_estimate_batch(ctx, q, n, seed + (uint64_t) rep, out_rgb, nullptr);
+    return guarded(ctx, [&]() -> sms_status {
+        if (!ctx) return SMS_E_INVALID;
+        if (!best) return fail(ctx, SMS_E_INVALID, "null schedule");
+        if (ctx->scene.cfg.biased) return fail(ctx, SMS_E_INVALID, "the autotuner times the unbiased estimator");
+        const sms_schedule keep = schedule_of(ctx);
+        sms_schedule win = keep;
+        float win_ms = INFINITY;
+        sms_status st = SMS_OK;
+        // the persistent pass refills a lane whose chain runs long, so longer lane runs pay off there
+        struct Cand { int first_pass, lane_trials; };
+        const Cand cands[11] = {{0, 4}, {0, 16}, {0, 64}, {0, 256}, {0, 1024},
+                                {1, 1}, {1, 2}, {1, 4}, {1, 8}, {1, 16}, {1, 32}};
+        for (const Cand &cd : cands) {
+            ctx->ss_first_pass = cd.first_pass;
+            ctx->lane_trials = cd.lane_trials;
+            ctx->blocks_per_cu = 0;
+            float ms = 0.f;
+            for (int rep = 0; rep < 2; ++rep) {           // the first run warms caches and allocations
+                st = sms_ms_estimate_batch(ctx, q, n, seed + (uint64_t) rep, out_rgb, nullptr);
+                if (st) break;
+                ms = ctx->last_ms + ctx->last_ms2;
+            }
             if (st) break;
-            ms = ctx->last_ms + ctx->last_ms2;
+            if (ms < win_ms) {
+                win_ms = ms;
+                win = schedule_of(ctx);
+            }
         }
-        if (st) break;
-        if (ms < win_ms) {
-            win_ms = ms;
-            win = schedule_of(ctx);
-        }
-    }
-    apply_schedule(ctx, st ? keep : win);
-    if (st) return st;
-    *best = win;
-    return SMS_OK;
+        apply_schedule(ctx, st ? keep : win);
+        if (st) return st;
+        *best = win;
+        return SMS_OK;
+    });
 }
 
 sms_status sms_ss_estimate_batch(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *out_rgb,
                                  sms_stats *stats) {
-    sms_status st = ss_prepare(ctx, q, n, out_rgb);
-    if (!st) st = frame_begin(ctx, seed);
-    if (!st) st = ss_enqueue(ctx, q, n, out_rgb);
-    return st ? st : estimator_done(ctx, stats);
+    return guarded(ctx, [&]() -> sms_status {
+        sms_status st = ss_prepare(ctx, q, n, out_rgb);
+        if (!st) st = frame_begin(ctx, seed);
+        if (!st) st = ss_enqueue(ctx, q, n, out_rgb);
+        return st ? st : estimator_done(ctx, stats);
+    });
 }
 
 sms_status sms_ms_walk_batch(sms_ctx *ctx, const sms_ms_walk_in *in, int64_t n, sms_ms_walk_out *out) {
-    if (!ctx || n < 0 || (n > 0 && (!in || !out))) return fail(ctx, SMS_E_INVALID, "bad walk batch arguments");
-    sms_status st = check_ms(ctx);
-    if (st) return st;
-    if (n == 0) return SMS_OK;
-    int blocks = (int) ((n + 255) / 256);
-    if (ctx->scene.cfg.bounces <= 2)
-        hipLaunchKernelGGL(k_ms_walk_2, dim3(blocks), dim3(256), 0, ctx->stream, ctx->scene, in, n, out);
-    else
-        hipLaunchKernelGGL(k_ms_walk_4, dim3(blocks), dim3(256), 0, ctx->stream, ctx->scene, in, n, out);
-    HIP_TRY(ctx, hipGetLastError());
-    return SMS_OK;
+    return guarded(ctx, [&]() -> sms_status {
+        if (!ctx || n < 0 || (n > 0 && (!in || !out))) return fail(ctx, SMS_E_INVALID, "bad walk batch arguments");
+        sms_status st = check_ms(ctx);
+        if (st) return st;
+        if (n == 0) return SMS_OK;
+        int blocks = (int) ((n + 255) / 256);
+        if (ctx->scene.cfg.bounces <= 2)
+            hipLaunchKernelGGL(k_ms_walk_2, dim3(blocks), dim3(256), 0, ctx->stream, ctx->scene, in, n, out);
+        else
+            hipLaunchKernelGGL(k_ms_walk_4, dim3(blocks), dim3(256), 0, ctx->stream, ctx->scene, in, n, out);
+        HIP_TRY(ctx, hipGetLastError());
+        return SMS_OK;
+    });
 }
 
 sms_status sms_ms_estimate_batch(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *out_rgb,
                                  sms_stats *stats) {
-    sms_status st = ms_prepare(ctx, q, n, out_rgb);
-    if (!st) st = frame_begin(ctx, seed);
-    if (!st) st = ms_enqueue(ctx, q, n, out_rgb);
-    return st ? st : estimator_done(ctx, stats);
+    return guarded(ctx, [&]() -> sms_status {
+        sms_status st = ms_prepare(ctx, q, n, out_rgb);
+        if (!st) st = frame_begin(ctx, seed);
+        if (!st) st = ms_enqueue(ctx, q, n, out_rgb);
+        return st ? st : estimator_done(ctx, stats);
+    });
 }
 
 // ---------------------------------------------------------------- device memory helpers
 sms_status sms_device_alloc(sms_ctx *ctx, size_t bytes, void **ptr) {
-    if (!ctx || !ptr) return SMS_E_INVALID;
-    HIP_TRY(ctx, hipMalloc(ptr, bytes == 0 ? 1 : bytes));
-    return SMS_OK;
+    return guarded(ctx, [&]() -> sms_status {
+        if (!ctx || !ptr) return SMS_E_INVALID;
+        HIP_TRY(ctx, hipMalloc(ptr, bytes == 0 ? 1 : bytes));
+        return SMS_OK;
+    });
 }
 sms_status sms_device_free(sms_ctx *ctx, void *ptr) {
-    if (!ctx) return SMS_E_INVALID;
-    if (ptr) HIP_TRY(ctx, hipFree(ptr));
-    return SMS_OK;
+    return guarded(ctx, [&]() -> sms_status {
+        if (!ctx) return SMS_E_INVALID;
+        if (ptr) HIP_TRY(ctx, hipFree(ptr));
+        return SMS_OK;
+    });
 }
 sms_status sms_copy_h2d(sms_ctx *ctx, void *dst, const void *src, size_t bytes) {
-    if (!ctx) return SMS_E_INVALID;
-    if (bytes) HIP_TRY(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    return SMS_OK;
+    return guarded(ctx, [&]() -> sms_status {
+        if (!ctx) return SMS_E_INVALID;
+        if (bytes) HIP_TRY(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        return SMS_OK;
+    });
 }
 sms_status sms_copy_d2h(sms_ctx *ctx, void *dst, const void *src, size_t bytes) {
-    if (!ctx) return SMS_E_INVALID;
-    if (bytes) HIP_TRY(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    return SMS_OK;
+    return guarded(ctx, [&]() -> sms_status {
+        if (!ctx) return SMS_E_INVALID;
+        if (bytes) HIP_TRY(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        return SMS_OK;
+    });
 }
 
 // ---------------------------------------------------------------- host-buffer variants
@@ -1250,32 +1391,44 @@ sms_status sms_copy_d2h(sms_ctx *ctx, void *dst, const void *src, size_t bytes) 
     return st;
 
 sms_status sms_ss_walk_batch_host(sms_ctx *ctx, const sms_ss_walk_in *in, int64_t n, sms_ss_walk_out *out) {
-    HOST_VARIANT_BODY(sms_ss_walk_in, sizeof(sms_ss_walk_out),
-                      sms_ss_walk_batch(ctx, (const sms_ss_walk_in *) d_in, n, (sms_ss_walk_out *) d_out))
+    return guarded(ctx, [&]() -> sms_status {
+        HOST_VARIANT_BODY(sms_ss_walk_in, sizeof(sms_ss_walk_out),
+                          sms_ss_walk_batch(ctx, (const sms_ss_walk_in *) d_in, n, (sms_ss_walk_out *) d_out))
+    });
 }
 sms_status sms_glint_walk_batch_host(sms_ctx *ctx, const sms_glint_walk_in *in, int64_t n, sms_glint_walk_out *out) {
-    HOST_VARIANT_BODY(sms_glint_walk_in, sizeof(sms_glint_walk_out),
-                      sms_glint_walk_batch(ctx, (const sms_glint_walk_in *) d_in, n, (sms_glint_walk_out *) d_out))
+    return guarded(ctx, [&]() -> sms_status {
+        HOST_VARIANT_BODY(sms_glint_walk_in, sizeof(sms_glint_walk_out),
+                          sms_glint_walk_batch(ctx, (const sms_glint_walk_in *) d_in, n, (sms_glint_walk_out *) d_out))
+    });
 }
 sms_status sms_ss_estimate_batch_host(sms_ctx *ctx, const sms_ss_query *in, int64_t n, uint64_t seed, float *out,
                                       sms_stats *stats) {
-    HOST_VARIANT_BODY(sms_ss_query, 3 * sizeof(float),
-                      sms_ss_estimate_batch(ctx, (const sms_ss_query *) d_in, n, seed, (float *) d_out, stats))
+    return guarded(ctx, [&]() -> sms_status {
+        HOST_VARIANT_BODY(sms_ss_query, 3 * sizeof(float),
+                          sms_ss_estimate_batch(ctx, (const sms_ss_query *) d_in, n, seed, (float *) d_out, stats))
+    });
 }
 sms_status sms_glint_estimate_batch_host(sms_ctx *ctx, const sms_glint_query *in, int64_t n, uint64_t seed, float *out,
                                          sms_stats *stats) {
-    HOST_VARIANT_BODY(sms_glint_query, 3 * sizeof(float),
-                      sms_glint_estimate_batch(ctx, (const sms_glint_query *) d_in, n, seed, (float *) d_out, stats))
+    return guarded(ctx, [&]() -> sms_status {
+        HOST_VARIANT_BODY(sms_glint_query, 3 * sizeof(float),
+                          sms_glint_estimate_batch(ctx, (const sms_glint_query *) d_in, n, seed, (float *) d_out, stats))
+    });
 }
 
 sms_status sms_ms_walk_batch_host(sms_ctx *ctx, const sms_ms_walk_in *in, int64_t n, sms_ms_walk_out *out) {
-    HOST_VARIANT_BODY(sms_ms_walk_in, sizeof(sms_ms_walk_out),
-                      sms_ms_walk_batch(ctx, (const sms_ms_walk_in *) d_in, n, (sms_ms_walk_out *) d_out))
+    return guarded(ctx, [&]() -> sms_status {
+        HOST_VARIANT_BODY(sms_ms_walk_in, sizeof(sms_ms_walk_out),
+                          sms_ms_walk_batch(ctx, (const sms_ms_walk_in *) d_in, n, (sms_ms_walk_out *) d_out))
+    });
 }
 sms_status sms_ms_estimate_batch_host(sms_ctx *ctx, const sms_ss_query *in, int64_t n, uint64_t seed, float *out,
                                       sms_stats *stats) {
-    HOST_VARIANT_BODY(sms_ss_query, 3 * sizeof(float),
-                      sms_ms_estimate_batch(ctx, (const sms_ss_query *) d_in, n, seed, (float *) d_out, stats))
+    return guarded(ctx, [&]() -> sms_status {
+        HOST_VARIANT_BODY(sms_ss_query, 3 * sizeof(float),
+                          sms_ms_estimate_batch(ctx, (const sms_ss_query *) d_in, n, seed, (float *) d_out, stats))
+    });
 }
 
 } // extern "C"

@@ -231,267 +231,301 @@ void proposal_release(sms_ctx *ctx) {
 extern "C" {
 
 sms_status sms_mlp_create(sms_ctx *ctx, uint64_t seed, sms_mlp **out) {
-    if (!ctx || !out) return fail(ctx, SMS_E_INVALID, "sms_mlp_create: null argument");
-    *out = nullptr;
-    sms_mlp *mlp = new (std::nothrow) sms_mlp();
-    if (!mlp) return fail(ctx, SMS_E_OOM, "sms_mlp_create: host allocation");
-    mlp->ctx = ctx;
-    const size_t fb = MLP_PARAMS * sizeof(float);
-    hipError_t e = hipMalloc(&mlp->p, fb);
-    if (e == hipSuccess) e = hipMalloc(&mlp->g, fb);
-    if (e == hipSuccess) e = hipMalloc(&mlp->m, fb);
-    if (e == hipSuccess) e = hipMalloc(&mlp->v, fb);
-    if (e == hipSuccess) e = hipMalloc(&mlp->loss, 2 * sizeof(float));
-    if (e == hipSuccess) e = hipMalloc(&mlp->partial, (size_t) MLP_GRAD_ROWS * fb);
-    if (e == hipSuccess) e = hipMalloc(&mlp->aux, 2 * MLP_GRAD_ROWS * sizeof(float));
-    if (e == hipSuccess) e = hipMalloc(&mlp->wb, MLP_PARAMS * sizeof(uint16_t));
-    if (e == hipSuccess) e = hipMemsetAsync(mlp->g, 0, fb, ctx->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(mlp->m, 0, fb, ctx->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(mlp->v, 0, fb, ctx->stream);
-    if (e != hipSuccess) {
-        free_mlp(mlp);
-        return fail(ctx, e == hipErrorOutOfMemory ? SMS_E_OOM : SMS_E_HIP, std::string("sms_mlp_create: ") + hipGetErrorString(e));
-    }
-    std::vector<float> h = init_params(seed);
-    sms_status st = upload_params(mlp, h.data());
-    if (st) { free_mlp(mlp); return st; }
-    *out = mlp;
-    return SMS_OK;
+    return guarded(ctx, [&]() -> sms_status {
+        if (!ctx || !out) return fail(ctx, SMS_E_INVALID, "sms_mlp_create: null argument");
+        *out = nullptr;
+        sms_mlp *mlp = new (std::nothrow) sms_mlp();
+        if (!mlp) return fail(ctx, SMS_E_OOM, "sms_mlp_create: host allocation");
+        mlp->ctx = ctx;
+        const size_t fb = MLP_PARAMS * sizeof(float);
+        hipError_t e = hipMalloc(&mlp->p, fb);
+        if (e == hipSuccess) e = hipMalloc(&mlp->g, fb);
+        if (e == hipSuccess) e = hipMalloc(&mlp->m, fb);
+        if (e == hipSuccess) e = hipMalloc(&mlp->v, fb);
+        if (e == hipSuccess) e = hipMalloc(&mlp->loss, 2 * sizeof(float));
+        if (e == hipSuccess) e = hipMalloc(&mlp->partial, (size_t) MLP_GRAD_ROWS * fb);
+        if (e == hipSuccess) e = hipMalloc(&mlp->aux, 2 * MLP_GRAD_ROWS * sizeof(float));
+        if (e == hipSuccess) e = hipMalloc(&mlp->wb, MLP_PARAMS * sizeof(uint16_t));
+        if (e == hipSuccess) e = hipMemsetAsync(mlp->g, 0, fb, ctx->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(mlp->m, 0, fb, ctx->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(mlp->v, 0, fb, ctx->stream);
+        if (e != hipSuccess) {
+            free_mlp(mlp);
+            return fail(ctx, e == hipErrorOutOfMemory ? SMS_E_OOM : SMS_E_HIP, std::string("sms_mlp_create: ") + hipGetErrorString(e));
+        }
+        std::vector<float> h = init_params(seed);
+        sms_status st = upload_params(mlp, h.data());
+        if (st) { free_mlp(mlp); return st; }
+        *out = mlp;
+        return SMS_OK;
+    });
 }
 
 sms_status sms_mlp_destroy(sms_mlp *mlp) {
-    if (!mlp) return SMS_E_INVALID;
-    sms_ctx *ctx = mlp->ctx;
-    (void) hipStreamSynchronize(ctx->stream);
-    if (ctx->prop_mlp == mlp) {
-        ctx->prop_mlp = nullptr;
-        ctx->gen++;
-    }
-    free_mlp(mlp);
-    return SMS_OK;
+    return guarded(mlp ? mlp->ctx : nullptr, [&]() -> sms_status {
+        if (!mlp) return SMS_E_INVALID;
+        sms_ctx *ctx = mlp->ctx;
+        (void) hipStreamSynchronize(ctx->stream);
+        if (ctx->prop_mlp == mlp) {
+            ctx->prop_mlp = nullptr;
+            ctx->gen++;
+        }
+        free_mlp(mlp);
+        return SMS_OK;
+    });
 }
 
 sms_status sms_mlp_get_params(sms_mlp *mlp, float *params) {
-    if (!mlp || !params) return SMS_E_INVALID;
-    sms_ctx *ctx = mlp->ctx;
-    HIP_TRY(ctx, hipMemcpyAsync(params, mlp->p, MLP_PARAMS * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    return SMS_OK;
+    return guarded(mlp ? mlp->ctx : nullptr, [&]() -> sms_status {
+        if (!mlp || !params) return SMS_E_INVALID;
+        sms_ctx *ctx = mlp->ctx;
+        HIP_TRY(ctx, hipMemcpyAsync(params, mlp->p, MLP_PARAMS * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        return SMS_OK;
+    });
 }
 
 sms_status sms_mlp_set_params(sms_mlp *mlp, const float *params) {
-    if (!mlp || !params) return SMS_E_INVALID;
-    return upload_params(mlp, params);
+    return guarded(mlp ? mlp->ctx : nullptr, [&]() -> sms_status {
+        if (!mlp || !params) return SMS_E_INVALID;
+        return upload_params(mlp, params);
+    });
 }
 
 sms_status sms_mlp_infer(sms_mlp *mlp, const float *x, int64_t n, float *out) {
-    if (!mlp) return SMS_E_INVALID;
-    if (n < 0 || (n > 0 && (!x || !out))) return fail(mlp->ctx, SMS_E_INVALID, "sms_mlp_infer: bad arguments");
-    return launch_infer(mlp, x, n, out);
+    return guarded(mlp ? mlp->ctx : nullptr, [&]() -> sms_status {
+        if (!mlp) return SMS_E_INVALID;
+        if (n < 0 || (n > 0 && (!x || !out))) return fail(mlp->ctx, SMS_E_INVALID, "sms_mlp_infer: bad arguments");
+        return launch_infer(mlp, x, n, out);
+    });
 }
 
 sms_status sms_mlp_grad(sms_mlp *mlp, const float *x, const float *uv, const float *w, int64_t n, float *grad,
                         float *loss) {
-    if (!mlp) return SMS_E_INVALID;
-    sms_ctx *ctx = mlp->ctx;
-    if (n <= 0 || !x || !uv) return fail(ctx, SMS_E_INVALID, "sms_mlp_grad: bad arguments");
-    const int rows = grad_rows(n);
-    hipLaunchKernelGGL(k_mlp_grad, dim3(rows), dim3(256), 0, ctx->stream, mlp->p, mlp->wb, x, uv, w, n, mlp->partial,
-                       mlp->aux);
-    HIP_TRY(ctx, hipGetLastError());
-    hipLaunchKernelGGL(k_mlp_reduce, dim3((MLP_PARAMS + 63) / 64), dim3(256), 0, ctx->stream,
-                       (const float *) mlp->partial, (const float *) mlp->aux, rows, mlp->g, mlp->loss);
-    HIP_TRY(ctx, hipGetLastError());
-    if (grad)
-        HIP_TRY(ctx, hipMemcpyAsync(grad, mlp->g, MLP_PARAMS * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
-    if (loss) HIP_TRY(ctx, hipMemcpyAsync(loss, mlp->loss, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
-    if (grad || loss) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    return SMS_OK;
+    return guarded(mlp ? mlp->ctx : nullptr, [&]() -> sms_status {
+        if (!mlp) return SMS_E_INVALID;
+        sms_ctx *ctx = mlp->ctx;
+        if (n <= 0 || !x || !uv) return fail(ctx, SMS_E_INVALID, "sms_mlp_grad: bad arguments");
+        const int rows = grad_rows(n);
+        hipLaunchKernelGGL(k_mlp_grad, dim3(rows), dim3(256), 0, ctx->stream, mlp->p, mlp->wb, x, uv, w, n, mlp->partial,
+                           mlp->aux);
+        HIP_TRY(ctx, hipGetLastError());
+        hipLaunchKernelGGL(k_mlp_reduce, dim3((MLP_PARAMS + 63) / 64), dim3(256), 0, ctx->stream,
+                           (const float *) mlp->partial, (const float *) mlp->aux, rows, mlp->g, mlp->loss);
+        HIP_TRY(ctx, hipGetLastError());
+        if (grad)
+            HIP_TRY(ctx, hipMemcpyAsync(grad, mlp->g, MLP_PARAMS * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        if (loss) HIP_TRY(ctx, hipMemcpyAsync(loss, mlp->loss, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        if (grad || loss) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        return SMS_OK;
+    });
 }
 
 sms_status sms_mlp_adam(sms_mlp *mlp, float lr) {
-    if (!mlp) return SMS_E_INVALID;
-    sms_ctx *ctx = mlp->ctx;
-    if (!(lr > 0.f)) return fail(ctx, SMS_E_INVALID, "sms_mlp_adam: learning rate must be positive");
-    float bc1, bc2;
-    adam_corrections(mlp, &bc1, &bc2);
-    hipLaunchKernelGGL(k_mlp_adam, dim3((MLP_PARAMS + 255) / 256), dim3(256), 0, ctx->stream, mlp->p, mlp->g, mlp->m,
-                       mlp->v, mlp->wb, MLP_PARAMS, lr, ADAM_B1, ADAM_B2, ADAM_EPS, bc1, bc2);
-    HIP_TRY(ctx, hipGetLastError());
-    return SMS_OK;
+    return guarded(mlp ? mlp->ctx : nullptr, [&]() -> sms_status {
+        if (!mlp) return SMS_E_INVALID;
+        sms_ctx *ctx = mlp->ctx;
+        if (!(lr > 0.f)) return fail(ctx, SMS_E_INVALID, "sms_mlp_adam: learning rate must be positive");
+        float bc1, bc2;
+        adam_corrections(mlp, &bc1, &bc2);
+        hipLaunchKernelGGL(k_mlp_adam, dim3((MLP_PARAMS + 255) / 256), dim3(256), 0, ctx->stream, mlp->p, mlp->g, mlp->m,
+                           mlp->v, mlp->wb, MLP_PARAMS, lr, ADAM_B1, ADAM_B2, ADAM_EPS, bc1, bc2);
+        HIP_TRY(ctx, hipGetLastError());
+        return SMS_OK;
+    });
 }
 
 sms_status sms_mlp_train(sms_mlp *mlp, const float *x, const float *uv, const float *w, int32_t batch,
                          const int64_t *offsets, int32_t n_steps, float lr, float *losses) {
-    if (!mlp) return SMS_E_INVALID;
-    sms_ctx *ctx = mlp->ctx;
-    if (n_steps < 0 || (n_steps > 0 && (!x || !uv || !offsets || batch <= 0)))
-        return fail(ctx, SMS_E_INVALID, "sms_mlp_train: bad arguments");
-    if (!(lr > 0.f)) return fail(ctx, SMS_E_INVALID, "sms_mlp_train: learning rate must be positive");
-    for (int32_t i = 0; i < n_steps; ++i)
-        if (offsets[i] < 0) return fail(ctx, SMS_E_INVALID, "sms_mlp_train: negative batch offset");
-    if (losses && n_steps > mlp->loss_cap) {
-        if (mlp->loss_hist) HIP_TRY(ctx, hipFree(mlp->loss_hist));
-        mlp->loss_hist = nullptr;
-        mlp->loss_cap = 0;
-        HIP_TRY(ctx, hipMalloc(&mlp->loss_hist, (size_t) n_steps * sizeof(float)));
-        mlp->loss_cap = n_steps;
-    }
-    const int rows = grad_rows(batch);
-    for (int32_t i = 0; i < n_steps; ++i) {
-        const int64_t o = offsets[i];
-        hipLaunchKernelGGL(k_mlp_grad, dim3(rows), dim3(256), 0, ctx->stream, mlp->p, mlp->wb, x + MLP_IN * o,
-                           uv + 2 * o, w ? w + o : nullptr, (int64_t) batch, mlp->partial, mlp->aux);
-        HIP_TRY(ctx, hipGetLastError());
-        float bc1, bc2;
-        adam_corrections(mlp, &bc1, &bc2);
-        hipLaunchKernelGGL(k_mlp_reduce_adam, dim3((MLP_PARAMS + 63) / 64), dim3(256), 0, ctx->stream,
-                           (const float *) mlp->partial, (const float *) mlp->aux, rows, mlp->p, mlp->m, mlp->v,
-                           mlp->wb, lr, ADAM_B1, ADAM_B2, ADAM_EPS, bc1, bc2, losses ? mlp->loss_hist + i : nullptr);
-        HIP_TRY(ctx, hipGetLastError());
-    }
-    if (losses && n_steps > 0) {
-        HIP_TRY(ctx, hipMemcpyAsync(losses, mlp->loss_hist, (size_t) n_steps * sizeof(float), hipMemcpyDeviceToHost,
-                                    ctx->stream));
-        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    }
-    return SMS_OK;
+    return guarded(mlp ? mlp->ctx : nullptr, [&]() -> sms_status {
+        if (!mlp) return SMS_E_INVALID;
+        sms_ctx *ctx = mlp->ctx;
+        if (n_steps < 0 || (n_steps > 0 && (!x || !uv || !offsets || batch <= 0)))
+            return fail(ctx, SMS_E_INVALID, "sms_mlp_train: bad arguments");
+        if (!(lr > 0.f)) return fail(ctx, SMS_E_INVALID, "sms_mlp_train: learning rate must be positive");
+        for (int32_t i = 0; i < n_steps; ++i)
+            if (offsets[i] < 0) return fail(ctx, SMS_E_INVALID, "sms_mlp_train: negative batch offset");
+        if (losses && n_steps > mlp->loss_cap) {
+            if (mlp->loss_hist) HIP_TRY(ctx, hipFree(mlp->loss_hist));
+            mlp->loss_hist = nullptr;
+            mlp->loss_cap = 0;
+            HIP_TRY(ctx, hipMalloc(&mlp->loss_hist, (size_t) n_steps * sizeof(float)));
+            mlp->loss_cap = n_steps;
+        }
+        const int rows = grad_rows(batch);
+        for (int32_t i = 0; i < n_steps; ++i) {
+            const int64_t o = offsets[i];
+            hipLaunchKernelGGL(k_mlp_grad, dim3(rows), dim3(256), 0, ctx->stream, mlp->p, mlp->wb, x + MLP_IN * o,
+                               uv + 2 * o, w ? w + o : nullptr, (int64_t) batch, mlp->partial, mlp->aux);
+            HIP_TRY(ctx, hipGetLastError());
+            float bc1, bc2;
+            adam_corrections(mlp, &bc1, &bc2);
+            hipLaunchKernelGGL(k_mlp_reduce_adam, dim3((MLP_PARAMS + 63) / 64), dim3(256), 0, ctx->stream,
+                               (const float *) mlp->partial, (const float *) mlp->aux, rows, mlp->p, mlp->m, mlp->v,
+                               mlp->wb, lr, ADAM_B1, ADAM_B2, ADAM_EPS, bc1, bc2, losses ? mlp->loss_hist + i : nullptr);
+            HIP_TRY(ctx, hipGetLastError());
+        }
+        if (losses && n_steps > 0) {
+            HIP_TRY(ctx, hipMemcpyAsync(losses, mlp->loss_hist, (size_t) n_steps * sizeof(float), hipMemcpyDeviceToHost,
+                                        ctx->stream));
+            HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        }
+        return SMS_OK;
+    });
 }
 
 sms_status sms_ctx_set_proposal(sms_ctx *ctx, sms_mlp *mlp, int32_t shape, float defensive) {
-    if (!ctx) return SMS_E_INVALID;
-    if (shape < 0) {
-        if (mlp) return fail(ctx, SMS_E_INVALID, "sms_ctx_set_proposal: a network needs a caster shape");
-        ctx->prop_mlp = nullptr; ctx->prop_shape = -1; ctx->prop_defensive = 1.f;
+    return guarded(ctx, [&]() -> sms_status {
+        if (!ctx) return SMS_E_INVALID;
+        if (shape < 0) {
+            if (mlp) return fail(ctx, SMS_E_INVALID, "sms_ctx_set_proposal: a network needs a caster shape");
+            ctx->prop_mlp = nullptr; ctx->prop_shape = -1; ctx->prop_defensive = 1.f;
+            ctx->gen++;
+            return SMS_OK;
+        }
+        if (shape >= (int) ctx->h_shapes.size()) return fail(ctx, SMS_E_INVALID, "sms_ctx_set_proposal: shape out of range");
+        const DShape &sh = ctx->h_shapes[shape];
+        if (sh.type != SMS_SHAPE_MESH || !sh.has_uv || !(sh.flags & (SMS_FLAG_CAUSTIC_CASTER_SS | SMS_FLAG_CAUSTIC_CASTER_MS)))
+            return fail(ctx, SMS_E_INVALID, "sms_ctx_set_proposal: the shape must be a caster mesh with uvs");
+        if (mlp && mlp->ctx != ctx) return fail(ctx, SMS_E_INVALID, "sms_ctx_set_proposal: network of another context");
+        if (mlp && !(defensive > 0.f && defensive <= 1.f))
+            return fail(ctx, SMS_E_INVALID, "sms_ctx_set_proposal: defensive weight must be in (0, 1]");
+        if (ctx->grid_shape != shape) {
+            sms_status st = build_uv_grid(ctx, shape);
+            if (st) return st;
+        }
+        ctx->prop_mlp = mlp;
+        ctx->prop_shape = shape;
+        ctx->prop_defensive = mlp ? defensive : 1.f;
         ctx->gen++;
         return SMS_OK;
-    }
-    if (shape >= (int) ctx->h_shapes.size()) return fail(ctx, SMS_E_INVALID, "sms_ctx_set_proposal: shape out of range");
-    const DShape &sh = ctx->h_shapes[shape];
-    if (sh.type != SMS_SHAPE_MESH || !sh.has_uv || !(sh.flags & (SMS_FLAG_CAUSTIC_CASTER_SS | SMS_FLAG_CAUSTIC_CASTER_MS)))
-        return fail(ctx, SMS_E_INVALID, "sms_ctx_set_proposal: the shape must be a caster mesh with uvs");
-    if (mlp && mlp->ctx != ctx) return fail(ctx, SMS_E_INVALID, "sms_ctx_set_proposal: network of another context");
-    if (mlp && !(defensive > 0.f && defensive <= 1.f))
-        return fail(ctx, SMS_E_INVALID, "sms_ctx_set_proposal: defensive weight must be in (0, 1]");
-    if (ctx->grid_shape != shape) {
-        sms_status st = build_uv_grid(ctx, shape);
-        if (st) return st;
-    }
-    ctx->prop_mlp = mlp;
-    ctx->prop_shape = shape;
-    ctx->prop_defensive = mlp ? defensive : 1.f;
-    ctx->gen++;
-    return SMS_OK;
+    });
 }
 
 sms_status sms_ctx_set_training_capture(sms_ctx *ctx, int64_t capacity, int32_t mode) {
-    if (!ctx || capacity < 0) return fail(ctx, SMS_E_INVALID, "sms_ctx_set_training_capture: bad capacity");
-    if (mode != SMS_CAPTURE_SOLUTIONS && mode != SMS_CAPTURE_ESTIMATES)
-        return fail(ctx, SMS_E_INVALID, "sms_ctx_set_training_capture: unknown mode");
-    ctx->cap_mode = mode;
-    ctx->gen++;
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    if (ctx->cap) { (void) hipFree(ctx->cap); ctx->cap = nullptr; }
-    if (ctx->cap_counts) { (void) hipFree(ctx->cap_counts); ctx->cap_counts = nullptr; }
-    ctx->cap_sub = 0;
-    if (capacity == 0) return SMS_OK;
-    const size_t sub = (size_t) ((capacity + SPREAD - 1) / SPREAD);
-    HIP_TRY(ctx, hipMalloc(&ctx->cap, (size_t) SPREAD * sub * 12 * sizeof(float)));
-    HIP_TRY(ctx, hipMalloc(&ctx->cap_counts, (size_t) SPREAD * SPREAD_STRIDE * sizeof(unsigned long long)));
-    HIP_TRY(ctx, hipMemset(ctx->cap_counts, 0, (size_t) SPREAD * SPREAD_STRIDE * sizeof(unsigned long long)));
-    ctx->cap_sub = sub;
-    return SMS_OK;
+    return guarded(ctx, [&]() -> sms_status {
+        if (!ctx || capacity < 0) return fail(ctx, SMS_E_INVALID, "sms_ctx_set_training_capture: bad capacity");
+        if (mode != SMS_CAPTURE_SOLUTIONS && mode != SMS_CAPTURE_ESTIMATES)
+            return fail(ctx, SMS_E_INVALID, "sms_ctx_set_training_capture: unknown mode");
+        ctx->cap_mode = mode;
+        ctx->gen++;
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        if (ctx->cap) { (void) hipFree(ctx->cap); ctx->cap = nullptr; }
+        if (ctx->cap_counts) { (void) hipFree(ctx->cap_counts); ctx->cap_counts = nullptr; }
+        ctx->cap_sub = 0;
+        if (capacity == 0) return SMS_OK;
+        const size_t sub = (size_t) ((capacity + SPREAD - 1) / SPREAD);
+        HIP_TRY(ctx, hipMalloc(&ctx->cap, (size_t) SPREAD * sub * 12 * sizeof(float)));
+        HIP_TRY(ctx, hipMalloc(&ctx->cap_counts, (size_t) SPREAD * SPREAD_STRIDE * sizeof(unsigned long long)));
+        HIP_TRY(ctx, hipMemset(ctx->cap_counts, 0, (size_t) SPREAD * SPREAD_STRIDE * sizeof(unsigned long long)));
+        ctx->cap_sub = sub;
+        return SMS_OK;
+    });
 }
 
 sms_status sms_ctx_training_samples(sms_ctx *ctx, int64_t *count, float *x, float *uv, float *w) {
-    if (!ctx || !count) return SMS_E_INVALID;
-    *count = 0;
-    if (!ctx->cap_sub) return SMS_OK;
-    std::vector<unsigned long long> h((size_t) SPREAD * SPREAD_STRIDE);
-    HIP_TRY(ctx, hipMemcpyAsync(h.data(), ctx->cap_counts, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                                ctx->stream));
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    int64_t total = 0;
-    for (int s = 0; s < SPREAD; ++s) total += (int64_t) std::min<unsigned long long>(h[(size_t) s * SPREAD_STRIDE], ctx->cap_sub);
-    *count = total;
-    if (x && uv && total > 0) {
-        // records in an order that does not depend on scheduling: sorted by a hash of their words
-        float *rec = nullptr;
-        uint64_t *keys = nullptr;
-        uint32_t *idx = nullptr;
-        void *tmp = nullptr;
-        size_t tmp_bytes = 0;
-        const int nn = (int) total;
-        hipError_t e = hipMalloc(&rec, (size_t) total * 12 * sizeof(float));
-        if (e == hipSuccess) e = hipMalloc(&keys, (size_t) total * 2 * sizeof(uint64_t));
-        if (e == hipSuccess) e = hipMalloc(&idx, (size_t) total * 2 * sizeof(uint32_t));
-        if (e == hipSuccess)
-            e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys, keys + total, idx, idx + total, nn, 0, 64,
-                                                   ctx->stream);
-        if (e == hipSuccess) e = hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 1));
-        if (e == hipSuccess) {
-            hipLaunchKernelGGL(k_capture_keys, dim3(SPREAD), dim3(256), 0, ctx->stream, ctx->cap, ctx->cap_counts,
-                               (unsigned long long) ctx->cap_sub, rec, keys, idx);
-            e = hipGetLastError();
+    return guarded(ctx, [&]() -> sms_status {
+        if (!ctx || !count) return SMS_E_INVALID;
+        *count = 0;
+        if (!ctx->cap_sub) return SMS_OK;
+        std::vector<unsigned long long> h((size_t) SPREAD * SPREAD_STRIDE);
+        HIP_TRY(ctx, hipMemcpyAsync(h.data(), ctx->cap_counts, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                    ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        int64_t total = 0;
+        for (int s = 0; s < SPREAD; ++s) total += (int64_t) std::min<unsigned long long>(h[(size_t) s * SPREAD_STRIDE], ctx->cap_sub);
+        *count = total;
+        if (x && uv && total > 0) {
+            // records in an order that does not depend on scheduling: sorted by a hash of their words
+            float *rec = nullptr;
+            uint64_t *keys = nullptr;
+            uint32_t *idx = nullptr;
+            void *tmp = nullptr;
+            size_t tmp_bytes = 0;
+            const int nn = (int) total;
+            hipError_t e = hipMalloc(&rec, (size_t) total * 12 * sizeof(float));
+            if (e == hipSuccess) e = hipMalloc(&keys, (size_t) total * 2 * sizeof(uint64_t));
+            if (e == hipSuccess) e = hipMalloc(&idx, (size_t) total * 2 * sizeof(uint32_t));
+            if (e == hipSuccess)
+                e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys, keys + total, idx, idx + total, nn, 0, 64,
+                                                       ctx->stream);
+            if (e == hipSuccess) e = hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 1));
+            if (e == hipSuccess) {
+                hipLaunchKernelGGL(k_capture_keys, dim3(SPREAD), dim3(256), 0, ctx->stream, ctx->cap, ctx->cap_counts,
+                                   (unsigned long long) ctx->cap_sub, rec, keys, idx);
+                e = hipGetLastError();
+            }
+            if (e == hipSuccess)
+                e = hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys, keys + total, idx, idx + total, nn, 0, 64,
+                                                       ctx->stream);
+            if (e == hipSuccess) {
+                hipLaunchKernelGGL(k_capture_scatter, dim3((unsigned) ((total + 255) / 256)), dim3(256), 0, ctx->stream, rec,
+                                   idx + total, (int64_t) total, x, uv, w);
+                e = hipGetLastError();
+            }
+            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+            for (void *b : {(void *) rec, (void *) keys, (void *) idx, tmp})
+                if (b) (void) hipFree(b);
+            if (e != hipSuccess)
+                return fail(ctx, e == hipErrorOutOfMemory ? SMS_E_OOM : SMS_E_HIP,
+                            std::string("sms_ctx_training_samples: ") + hipGetErrorString(e));
         }
-        if (e == hipSuccess)
-            e = hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys, keys + total, idx, idx + total, nn, 0, 64,
-                                                   ctx->stream);
-        if (e == hipSuccess) {
-            hipLaunchKernelGGL(k_capture_scatter, dim3((unsigned) ((total + 255) / 256)), dim3(256), 0, ctx->stream, rec,
-                               idx + total, (int64_t) total, x, uv, w);
-            e = hipGetLastError();
-        }
-        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-        for (void *b : {(void *) rec, (void *) keys, (void *) idx, tmp})
-            if (b) (void) hipFree(b);
-        if (e != hipSuccess)
-            return fail(ctx, e == hipErrorOutOfMemory ? SMS_E_OOM : SMS_E_HIP,
-                        std::string("sms_ctx_training_samples: ") + hipGetErrorString(e));
-    }
-    return SMS_OK;
+        return SMS_OK;
+    });
 }
 
 sms_status sms_ctx_clear_training(sms_ctx *ctx) {
-    if (!ctx) return SMS_E_INVALID;
-    if (!ctx->cap_counts) return SMS_OK;
-    HIP_TRY(ctx, hipMemsetAsync(ctx->cap_counts, 0, (size_t) SPREAD * SPREAD_STRIDE * sizeof(unsigned long long),
-                                ctx->stream));
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    return SMS_OK;
+    return guarded(ctx, [&]() -> sms_status {
+        if (!ctx) return SMS_E_INVALID;
+        if (!ctx->cap_counts) return SMS_OK;
+        HIP_TRY(ctx, hipMemsetAsync(ctx->cap_counts, 0, (size_t) SPREAD * SPREAD_STRIDE * sizeof(unsigned long long),
+                                    ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        return SMS_OK;
+    });
 }
 
 namespace {
 
 sms_status proposal_io(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, int ms, float *x, float *out) {
-    if (!ctx || n < 0 || (n > 0 && (!q || !(x || out)))) return fail(ctx, SMS_E_INVALID, "proposal inputs: bad arguments");
-    if (out && !ctx->prop_mlp) return fail(ctx, SMS_E_INVALID, "proposal outputs: no proposal network set");
-    sms_status st = out ? ensure_prop(ctx, (size_t) std::max<int64_t>(n, 1)) : SMS_OK;
-    if (st) return st;
-    hipLaunchKernelGGL(k_set_seed, dim3(1), dim3(64), 0, ctx->stream, ctx->seeds + SEED_API, seed);
-    HIP_TRY(ctx, hipGetLastError());
-    if (!out) return launch_inputs(ctx, q, n, ctx->seeds + SEED_API, ms, x);
-    st = launch_inputs(ctx, q, n, ctx->seeds + SEED_API, ms, ctx->prop_x);
-    if (!st) st = launch_infer(ctx->prop_mlp, ctx->prop_x, n, out);
-    return st;
+    return guarded(ctx, [&]() -> sms_status {
+        if (!ctx || n < 0 || (n > 0 && (!q || !(x || out)))) return fail(ctx, SMS_E_INVALID, "proposal inputs: bad arguments");
+        if (out && !ctx->prop_mlp) return fail(ctx, SMS_E_INVALID, "proposal outputs: no proposal network set");
+        sms_status st = out ? ensure_prop(ctx, (size_t) std::max<int64_t>(n, 1)) : SMS_OK;
+        if (st) return st;
+        hipLaunchKernelGGL(k_set_seed, dim3(1), dim3(64), 0, ctx->stream, ctx->seeds + SEED_API, seed);
+        HIP_TRY(ctx, hipGetLastError());
+        if (!out) return launch_inputs(ctx, q, n, ctx->seeds + SEED_API, ms, x);
+        st = launch_inputs(ctx, q, n, ctx->seeds + SEED_API, ms, ctx->prop_x);
+        if (!st) st = launch_infer(ctx->prop_mlp, ctx->prop_x, n, out);
+        return st;
+    });
 }
 
 } // namespace
 
 sms_status sms_ss_proposal_inputs(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *x) {
-    return proposal_io(ctx, q, n, seed, 0, x, nullptr);
+    return guarded(ctx, [&]() -> sms_status {
+        return proposal_io(ctx, q, n, seed, 0, x, nullptr);
+    });
 }
 
 sms_status sms_ss_proposal_outputs(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *out) {
-    return proposal_io(ctx, q, n, seed, 0, nullptr, out);
+    return guarded(ctx, [&]() -> sms_status {
+        return proposal_io(ctx, q, n, seed, 0, nullptr, out);
+    });
 }
 
 sms_status sms_ms_proposal_inputs(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *x) {
-    return proposal_io(ctx, q, n, seed, 1, x, nullptr);
+    return guarded(ctx, [&]() -> sms_status {
+        return proposal_io(ctx, q, n, seed, 1, x, nullptr);
+    });
 }
 
 sms_status sms_ms_proposal_outputs(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *out) {
-    return proposal_io(ctx, q, n, seed, 1, nullptr, out);
+    return guarded(ctx, [&]() -> sms_status {
+        return proposal_io(ctx, q, n, seed, 1, nullptr, out);
+    });
 }
 
 } // extern "C"

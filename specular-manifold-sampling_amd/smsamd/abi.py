@@ -157,7 +157,7 @@ EXPORTS = ["sms_config_default", "sms_ctx_create", "sms_ctx_destroy", "sms_ctx_s
            "sms_graph_stats", "sms_graph_destroy", "sms_ctx_set_proposal",
            "sms_ctx_set_training_capture", "sms_ctx_training_samples", "sms_ctx_clear_training",
            "sms_ss_proposal_inputs", "sms_ss_proposal_outputs", "sms_ss_make_queries",
-           "sms_film_accumulate", "sms_ss_autotune", "sms_ms_autotune"]
+           "sms_film_accumulate", "sms_ss_autotune", "sms_ms_autotune", "sms_scene_footprint"]
 
 # learned seed proposal (include/sms.h)
 MLP_INPUTS = 9
@@ -176,6 +176,7 @@ def load_library(path=LIB_PATH):
     lib.sms_config_default.argtypes = [C.POINTER(SmsConfig)]
     lib.sms_config_default.restype = None
     lib.sms_ctx_create.argtypes = [C.POINTER(SmsSceneDesc), C.POINTER(SmsConfig), C.POINTER(vp)]
+    lib.sms_scene_footprint.argtypes = [C.POINTER(SmsSceneDesc), C.POINTER(C.c_uint64), C.c_char_p, C.c_int32]
     lib.sms_ctx_destroy.argtypes = [vp]
     lib.sms_ctx_set_config.argtypes = [vp, C.POINTER(SmsConfig)]
     lib.sms_ctx_set_schedule.argtypes = [vp, C.POINTER(SmsSchedule)]

@@ -320,7 +320,9 @@ def test_nonfinite_walk_is_fast(gpu, oracle_mod):
     dt = time.perf_counter() - t
     print(f"nan walk: {dt * 1e3:.2f} ms", flush=True)
     _check_ss_walks(go, orc.ss_walks(w))
-    assert dt < 0.1
+    # a walk whose 20 rays each visited the whole tree took about 8 s; skipping the tree takes
+    # well under a millisecond, so a generous bound separates the two without timing noise
+    assert dt < 2.0
     ctx.close()
     orc.close()
 
