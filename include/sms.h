@@ -333,6 +333,19 @@ sms_status sms_ms_walk_batch(sms_ctx *ctx, const sms_ms_walk_in *in, int64_t n,
    out_rgb: n x 3 floats.  stats: host pointer, filled after completion (may be NULL). */
 sms_status sms_ss_estimate_batch(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed,
                                  float *out_rgb, sms_stats *stats);
+/* sms_ss_estimate_batch without any allocation: no counterpart in the reference; for callers
+   that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+   that forbid hipMalloc).  Same results and statistics as sms_ss_estimate_batch, bit for bit.  The
+   call's scratch (work counters, seed slots, per-(query, caster) partial results, the overflow
+   list of the Bernoulli tail) is a fixed device array of the library, about 20 MB on every device
+   the library runs on, held until the process ends: at most 262144 (query, caster) pairs per call
+   (n x number of single-scatter casters), larger calls return SMS_E_INVALID.  Every call on a
+   device shares that device's array, so calls that overlap on one device (other contexts' or
+   other streams') race; calls on one context's stream follow each other.  Runs without a learned
+   proposal or training capture (SMS_E_UNSUPPORTED otherwise).  Synchronises like
+   sms_ss_estimate_batch (stats are read back). */
+sms_status sms_ss_estimate_batch_noalloc(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed,
+                                         float *out_rgb, sms_stats *stats);
 /* Glint estimator (SpecularManifoldGlints::specular_manifold_sampling, manifold_glints.cpp:
    28-223): the SMS strategy -- unbiased Bernoulli trials, or with config.biased the distinct
    solutions of a trial set of max_trials <= 128 walks -- and, in scenes with an area emitter,

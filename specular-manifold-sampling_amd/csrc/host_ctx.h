@@ -64,6 +64,8 @@ struct sms_ctx {
     // bumped whenever buffers or settings that a captured graph holds by value change
     uint64_t gen = 1;
     bool capturing = false;                     // launch sequence being captured: no event records
+    // device address of the allocation-free estimator's scratch array on this context's device
+    void *scratch = nullptr;
 };
 
 // seed slots: estimator, camera pass, the proposal's input / output queries
@@ -124,6 +126,9 @@ inline sms_status guarded(sms_ctx *ctx, F &&body) noexcept {
     do {                                                                                        \
         if (!(ctx)->capturing) HIP_TRY(ctx, hipEventRecord(ev, (ctx)->stream));                 \
     } while (0)
+
+// allocation-free estimator (sms_noalloc_host.hip): per-device scratch address lookup
+sms_status noalloc_init_device(sms_ctx *ctx);
 
 // learned seed proposal (sms_mlp_host.hip)
 Proposal proposal_of(sms_ctx *ctx);

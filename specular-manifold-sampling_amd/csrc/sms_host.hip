@@ -727,6 +727,7 @@ namespace {
 sms_status create_on_device(sms_ctx *ctx, HostScene &H, const sms_config *cfg) {
     sms_status st;
     HIP_TRY(ctx, hipGetDevice(&ctx->device));
+    if ((st = noalloc_init_device(ctx))) return st;
     hipDeviceProp_t prop;
     HIP_TRY(ctx, hipGetDeviceProperties(&prop, ctx->device));
     ctx->n_cu = prop.multiProcessorCount;

@@ -157,7 +157,8 @@ EXPORTS = ["sms_config_default", "sms_ctx_create", "sms_ctx_destroy", "sms_ctx_s
            "sms_graph_stats", "sms_graph_destroy", "sms_ctx_set_proposal",
            "sms_ctx_set_training_capture", "sms_ctx_training_samples", "sms_ctx_clear_training",
            "sms_ss_proposal_inputs", "sms_ss_proposal_outputs", "sms_ss_make_queries",
-           "sms_film_accumulate", "sms_ss_autotune", "sms_ms_autotune", "sms_scene_footprint"]
+           "sms_film_accumulate", "sms_ss_autotune", "sms_ms_autotune", "sms_scene_footprint",
+           "sms_ss_estimate_batch_noalloc"]
 
 # learned seed proposal (include/sms.h)
 MLP_INPUTS = 9
@@ -190,7 +191,7 @@ def load_library(path=LIB_PATH):
         getattr(lib, name).argtypes = [vp, vp, i64, vp]
     for name in ("sms_ss_estimate_batch", "sms_glint_estimate_batch",
                  "sms_ss_estimate_batch_host", "sms_glint_estimate_batch_host",
-                 "sms_ms_estimate_batch", "sms_ms_estimate_batch_host"):
+                 "sms_ms_estimate_batch", "sms_ms_estimate_batch_host", "sms_ss_estimate_batch_noalloc"):
         getattr(lib, name).argtypes = [vp, vp, i64, C.c_uint64, vp, C.POINTER(SmsStats)]
     lib.sms_glint_make_queries.argtypes = [vp, C.c_int32, C.c_uint64, vp]
     lib.sms_glint_autotune.argtypes = [vp, vp, i64, C.c_uint64, vp, C.POINTER(SmsSchedule)]
