@@ -64,7 +64,7 @@ static inline V3 fmadd3(V3 a, float b, V3 c) { return v3(fmaf(a.x, b, c.x), fmaf
 /* fnmadd(a, b, c) = -a*b + c */
 static inline V3 fnmadd3(V3 a, float b, V3 c) { return v3(fmaf(-a.x, b, c.x), fmaf(-a.y, b, c.y), fmaf(-a.z, b, c.z)); }
 static inline float hmax_abs(V3 a) { float m = fabsf(a.x); if (fabsf(a.y) > m) m = fabsf(a.y); if (fabsf(a.z) > m) m = fabsf(a.z); return m; }
-static inline float safe_sqrt(float x) { return sqrtf(x > 0.f ? x : 0.f); }
+static inline float safe_sqrt(float x) { return sqrtf(x > 0.f ? x : (float) 0.f); }
 static inline float mulsign(float a, float b) { return signbit(b) ? -a : a; }
 
 /* 2x2 matrix, m[row][col] */
@@ -153,7 +153,7 @@ static inline float c_acosf(float x) {
     if (x > 0.5f) return 2.0f * c_asinf(sqrtf(0.5f * (1.0f - x)));
     return PIO2_F - c_asinf(x);
 }
-static inline float safe_acos(float x) { return c_acosf(x < -1.f ? -1.f : (x > 1.f ? 1.f : x)); }
+static inline float safe_acos(float x) { return c_acosf(x < -1.f ? (float) -1.f : (x > 1.f ? (float) 1.f : x)); }
 
 /* ---------------------------------------------------------------- PCG32 (O'Neill, pcg-random.org) */
 struct Pcg32 { uint64_t state, inc; };

@@ -317,7 +317,7 @@ static Scene *scene_create(const sms_scene_desc *desc, const sms_config *cfg) {
         sh.area_sum = sh.area_norm = 0.f;
         /* ManifoldVertex eta: dielectric real IOR, conductors (k != 0) and others 1
            (manifold.h:93-95; bsdf.cpp:59-62 default ior (0,1)) */
-        sh.eta_vtx = sh.k == 0.f ? hmean_same(sh.eta_r) : 1.f;
+        sh.eta_vtx = sh.k == 0.f ? hmean_same(sh.eta_r) : (float) 1.f;
         sh.bbox_center = sh.center;
         if (d.type == SMS_SHAPE_SPHERE) {
             sh.inv_surface_area = 1.f / (4.f * PI_F * sh.radius * sh.radius);
@@ -337,8 +337,8 @@ static Scene *scene_create(const sms_scene_desc *desc, const sms_config *cfg) {
             sh.has_n = d.normals != nullptr; sh.has_uv = d.texcoords != nullptr;
             for (int v = 0; v < d.n_vertices; ++v) {
                 for (int a = 0; a < 3; ++a) S->pos.push_back(d.positions[3 * v + a]);
-                for (int a = 0; a < 3; ++a) S->nrm.push_back(d.normals ? d.normals[3 * v + a] : 0.f);
-                for (int a = 0; a < 2; ++a) S->uvs.push_back(d.texcoords ? d.texcoords[2 * v + a] : 0.f);
+                for (int a = 0; a < 3; ++a) S->nrm.push_back(d.normals ? d.normals[3 * v + a] : (float) 0.f);
+                for (int a = 0; a < 2; ++a) S->uvs.push_back(d.texcoords ? d.texcoords[2 * v + a] : (float) 0.f);
             }
             /* area distribution: mesh.cpp area_distr_ensure + DiscreteDistribution::update
                (include/mitsuba/core/distr_1d.h: double running sum, float cdf) */
@@ -388,7 +388,7 @@ static Scene *scene_create(const sms_scene_desc *desc, const sms_config *cfg) {
     }
     float half = 0.f;
     for (int a = 0; a < 3; ++a) { S->box_c[a] = 0.5f * (lo[a] + hi[a]); half = std::max(half, 0.5f * (hi[a] - lo[a])); }
-    S->box_inv = half > 0.f ? 1.f / half : 1.f;
+    S->box_inv = half > 0.f ? 1.f / half : (float) 1.f;
     S->has_cam = desc->camera.width > 0;
     if (S->has_cam) camera_setup(S->cam, desc->camera);
     return S;
@@ -868,7 +868,7 @@ static PS sample_position(const Scene &S, int shape, V2 sample) {
         while (lo < hi) { int mid = (lo + hi) / 2; if (sh.cdf[mid] < scaled) lo = mid + 1; else hi = mid; }
         int idx = lo;
         float pmf = sh.pmf[idx] * sh.area_norm;
-        float cdf = idx > 0 ? sh.cdf[idx - 1] * sh.area_norm : 0.f;
+        float cdf = idx > 0 ? sh.cdf[idx - 1] * sh.area_norm : (float) 0.f;
         float y = (value - cdf) / pmf;
         const uint32_t *f = &S.faces[3 * (sh.fbase + idx)];
         V3 p0 = v3(S.pos[3 * f[0]], S.pos[3 * f[0] + 1], S.pos[3 * f[0] + 2]);
@@ -1884,7 +1884,7 @@ static float lean_pdf(const Scene &S, const GlintSI &si, V3 wo) {
 static float mis_power(float a, float b) {
     a *= a;
     b *= b;
-    return a > 0.f ? a / (a + b) : 0.f;
+    return a > 0.f ? a / (a + b) : (float) 0.f;
 }
 
 /* Scene::pdf_emitter_direction for the SMS strategy's emitter sample (scene.cpp:206-219,

@@ -13,16 +13,18 @@ from smsamd import abi
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
+# the op-counting build of the same source (counting float/double types, oracle/opcount.h)
+OPCOUNT_SO = os.path.join(ORACLE_DIR, "liboracle_opcount.so")
 
 
 def build():
     subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
 
 
-def load():
-    if not os.path.exists(ORACLE_SO):
-        build()
-    lib = C.CDLL(ORACLE_SO)
+def load(path=ORACLE_SO):
+    if not os.path.exists(path):
+        subprocess.run(["make", "-s", "-C", ORACLE_DIR, os.path.basename(path)], check=True)
+    lib = C.CDLL(path)
     vp, i64 = C.c_void_p, C.c_int64
     lib.oracle_create.argtypes = [C.POINTER(abi.SmsSceneDesc), C.POINTER(abi.SmsConfig)]
     lib.oracle_create.restype = vp
@@ -77,8 +79,8 @@ def load():
 class Oracle:
     """One oracle scene (mirrors sms_ctx)."""
 
-    def __init__(self, scene, cfg, threads=None):
-        self.lib = load()
+    def __init__(self, scene, cfg, threads=None, lib_path=ORACLE_SO):
+        self.lib = load(lib_path)
         self.scene = scene
         self.desc = scene.desc()
         self.cfg = cfg
