@@ -17,6 +17,13 @@ visibility, footprint target and the first Newton walk of every query) timed wit
 on the stream it runs on; `cpu_baseline` times the CPU oracle on a bounded sample
 of the same queries on this host's cores.
 
+Beside it: `dome_ss_biased` (the plain single-scatter caustic config: glass dome, 256^2 x 16
+spp, biased M = 64) and `pool_ss` (the water pool, 1024^2 x 8 spp pass, unbiased), each with an
+FP32 roofline of its walk kernel (FLOP per Newton iteration frozen by the op-counting oracle,
+tests/golden/flop_counts.json) and a CPU-oracle baseline on 16 threads and on one core.  The
+glint CPU baseline runs whole frames with the GPU steps' seeds and must reproduce their
+statistics (and the first frames' RGB, bit for bit), or the benchmark fails.
+
 `equal_time_rmse` is the second half of the metric, on the water-pool config (see
 equal_time_rmse): RMSE of uniform-seed and learned-proposal renders given the same wall-clock
 budget, against a uniform-seed reference; `ratio` charges the training to the same frame (the
@@ -40,7 +47,36 @@ BYTES_PER_ITER = 12            # algorithmic: 4 RGB8 texels per Newton iteration
 # (FETCH_SIZE reads half the copied bytes: factor 2; WRITE_SIZE factor 1); see
 # profiles/glint_first_512x512x4spp_pmc.md.
 TRAFFIC_FIRST_PASS = {"fetch_kib": 115055.6, "write_kib": 13509.6, "fetch_factor": 2.0, "write_factor": 1.0}
-FLOPS_PER_ITER = 120           # FP32 ops per glint Newton iteration (fetch, solve, footprint test)
+# FP32 FLOP per Newton iteration of each estimator (its whole work divided by its iterations),
+# frozen by the op-counting build of the CPU oracle (tests/golden/make_flop_counts.py)
+FLOP_COUNTS = json.load(open(os.path.join(ROOT, "tests", "golden", "flop_counts.json")))
+FLOPS_PER_ITER = FLOP_COUNTS["glint"]["flop32_per_iteration"]
+CPU_THREADS = 16               # the host share of one GPU on the benchmark machine
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def timed_cpu(run, seconds, max_reps=1000):
+    """Calls run(rep) -> (walks, converged) until `seconds` have passed; returns the rates."""
+    walks = conv = reps = 0
+    t = time.perf_counter()
+    while True:
+        w, c = run(reps)
+        walks += w
+        conv += c
+        reps += 1
+        dt = time.perf_counter() - t
+        if dt >= seconds or reps >= max_reps:
+            return {"walks_per_s": walks / dt, "converged_per_s": conv / dt, "reps": reps, "walks": walks,
+                    "seconds": round(dt, 2)}
 
 
 def parse():
@@ -64,6 +100,8 @@ def parse():
     ap.add_argument("--no-rmse", action="store_true", help="skip the equal-time RMSE experiment")
     ap.add_argument("--no-pool", action="store_true", help="skip the pool single-scatter throughput line")
     ap.add_argument("--pool-only", action="store_true", help="run only the pool single-scatter throughput")
+    ap.add_argument("--dome-only", action="store_true", help="run only the glass-dome biased single-scatter line")
+    ap.add_argument("--no-dome", action="store_true", help="skip the glass-dome biased single-scatter line")
     ap.add_argument("--ss-first-pass", type=int, default=None, help="pool: 0 walker, 1 lane per query (default: tuned)")
     ap.add_argument("--rmse-only", action="store_true", help="run only the equal-time RMSE experiment")
     ap.add_argument("--ms-only", action="store_true", help="run only the multi-scatter slab experiment")
@@ -116,6 +154,9 @@ def main():
     if args.pool_only:
         print(json.dumps(pool_throughput(args)), flush=True)
         return
+    if args.dome_only:
+        print(json.dumps(dome_throughput(args)), flush=True)
+        return
     if args.ms_only:
         print(json.dumps(equal_time_rmse(args, "slab")), flush=True)
         return
@@ -150,6 +191,7 @@ def main():
     tot = {}
     k1_ms = k2_ms = 0.0
     iters_pass1 = 0
+    step_stats = []
     t_start = time.perf_counter()
     for s in range(args.steps):
         st = ctx.glint_estimate_device(qbuf, n, 1 + s, out)
@@ -157,6 +199,7 @@ def main():
         k1_ms += a
         k2_ms += b
         iters_pass1 += st["newton_iterations"] - st["tail_iterations"]
+        step_stats.append(st)
         for k, v in st.items():
             tot[k] = tot.get(k, 0) + v
     ctx.sync()
@@ -202,10 +245,11 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(sc, cfg, ctx, n, args)
+        cpu = cpu_baseline(sc, cfg, ctx, qbuf, n, out, step_stats, args)
     out.free()
     qbuf.free()
     ctx.close()
+    dome = None if args.no_dome else dome_throughput(args)
     pool = None if args.no_pool else pool_throughput(args)
     mlp = None if args.no_mlp else mlp_throughput(args)
     rmse = None if args.no_rmse else equal_time_rmse(args)
@@ -232,6 +276,7 @@ def main():
                    "trials_tail_finalize_ms": round(k2_ms / args.steps, 3)},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "dome_ss_biased": dome,
         "pool_ss": pool,
         "mlp": mlp,
         "equal_time_rmse": rmse,
@@ -284,7 +329,76 @@ def pool_throughput(args):
             "newton_iterations_per_s": tot["newton_iterations"] / dt, "ms_per_step": 1e3 * dt / args.steps,
             "first_pass_ms": k1 / args.steps, "tail_ms": k2 / args.steps, "steps": args.steps,
             "walks_per_step": tot["walks"] / args.steps, "schedule": sched}
+    kernel = "k_ss_walker" if sched.get("ss_first_pass", 0) == 0 else "k_ss_estimate"
+    res_["roofline"] = valu_roofline(kernel, "pool", (tot["newton_iterations"] - tot["tail_iterations"]) / args.steps,
+                                     k1 / args.steps)
     print(f"# pool: {json.dumps(res_)}", file=sys.stderr, flush=True)
+    if not args.no_cpu:
+        import numpy as np
+        from smsamd import abi
+        q = np.zeros(n, abi.SS_QUERY)
+        qb.download(q)
+        res_["cpu_baseline"] = query_sample_baseline(sc, cfg, q, "ss", args.cpu_seconds / 1.5, 256)
+        res_["gpu_over_cpu"] = res_["converged_walks_per_s"] / res_["cpu_baseline"]["value"]
+        print(f"# pool cpu: {json.dumps(res_['cpu_baseline'])}", file=sys.stderr, flush=True)
+    out.free()
+    qb.free()
+    wb.free()
+    ctx.close()
+    return res_
+
+
+def valu_roofline(kernel, flop_key, iters_per_launch, kernel_ms):
+    """FP32 roofline of a walk kernel: frozen FLOP per Newton iteration x the iterations one
+    launch ran / its HIP-event time."""
+    fpi = FLOP_COUNTS[flop_key]["flop32_per_iteration"]
+    tflops = fpi * iters_per_launch / (kernel_ms * 1e-3) / 1e12
+    return {"bound": "valu", "achieved": round(tflops, 4), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(tflops / FP32_PEAK_TFLOPS, 6), "traffic": None, "kernel": kernel,
+            "kernel_ms": round(kernel_ms, 3), "flop32_per_iteration": round(fpi, 1),
+            "iters_per_launch": round(iters_per_launch)}
+
+
+def dome_throughput(args):
+    """The plain single-scatter caustic config: glass sphere (r 1, IOR 1.5) centred on a diffuse
+    floor (a dome, as the reference's sphere scene places it), sphere light r 0.1 radiance 100 at
+    (0,5,0), pinhole 45 deg; 256^2 x 16 spp camera queries resident in HBM, biased SMS with trial
+    sets of 64 walks, Newton max 20, threshold 1e-5, uniform seeds.  `steps` timed estimates after
+    `warmup`; the CPU oracle runs strided samples of the same queries."""
+    from smsamd import abi, gpu, scenes
+    sc = scenes.sphere_dome(256, 256)
+    cfg = abi.SmsConfig.default(biased=1, max_trials=64)
+    ctx = gpu.Context(sc, cfg)
+    qb, wb, n = ctx.ss_queries_device(16, 1)
+    out = gpu.DeviceBuffer(ctx, n * 12)
+    for w in range(args.warmup):
+        ctx.ss_estimate_device(qb, n, 3000 + w, out)
+    ctx.sync()
+    tot, k1 = {}, 0.0
+    t0 = time.perf_counter()
+    for s_ in range(args.steps):
+        st = ctx.ss_estimate_device(qb, n, 1 + s_, out)
+        k1 += ctx.last_kernel_ms()[0]
+        for k, v in st.items():
+            tot[k] = tot.get(k, 0) + v
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    res_ = {"workload": f"glass dome caustics, 256x256x16spp camera queries ({n}), biased M=64, uniform seeds",
+            "converged_walks_per_s": tot["walks_converged"] / dt, "walks_per_s": tot["walks"] / dt,
+            "newton_iterations_per_s": tot["newton_iterations"] / dt, "ms_per_step": 1e3 * dt / args.steps,
+            "steps": args.steps, "walks_per_step": tot["walks"] / args.steps}
+    res_["roofline"] = valu_roofline("k_ss_biased", "dome_biased", tot["newton_iterations"] / args.steps,
+                                     k1 / args.steps)
+    print(f"# dome: {json.dumps(res_)}", file=sys.stderr, flush=True)
+    if not args.no_cpu:
+        import numpy as np
+        q = np.zeros(n, abi.SS_QUERY)
+        qb.download(q)
+        res_["cpu_baseline"] = query_sample_baseline(sc, cfg, q, "ss", args.cpu_seconds / 1.5, 256)
+        cb = res_["cpu_baseline"]
+        res_["gpu_over_cpu"] = {"walks": res_["walks_per_s"] / cb["walks_per_s"],
+                                "walks_one_core": res_["walks_per_s"] / cb["one_core"]["walks_per_s"]}
+        print(f"# dome cpu: {json.dumps(cb)}", file=sys.stderr, flush=True)
     out.free()
     qb.free()
     wb.free()
@@ -551,30 +665,77 @@ def equal_time_rmse(args, case="pool"):
     return out
 
 
-def cpu_baseline(sc, cfg, ctx, n, args):
-    """CPU oracle on a strided sample of the same queries (same seeds per query)."""
+def cpu_baseline(sc, cfg, ctx, qbuf, n, out, step_stats, args):
+    """CPU oracle on whole frames of the same queries and per-frame seeds as the GPU steps, on
+    CPU_THREADS host threads for ~cpu_seconds, then on one thread.  Every frame both ran must
+    agree: the oracle's statistics equal the GPU step's, and for the first frames the oracle's
+    RGB equals the GPU's bit for bit (raises otherwise)."""
+    import numpy as np
+    from smsamd import abi
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    q = np.zeros(n, abi.GLINT_QUERY)
+    qbuf.download(q)
+    checked_rgb = 0
+    orc = oracle_lib.Oracle(sc, cfg, threads=CPU_THREADS)
+    keys = ("walks", "walks_converged", "newton_iterations", "bernoulli_calls", "bernoulli_trials", "bernoulli_max")
+
+    def frame(rep):
+        nonlocal checked_rgb
+        rgb, st = orc.glint_estimate(q, 1 + rep)
+        if rep < len(step_stats):
+            for k in keys:
+                if st[k] != step_stats[rep][k]:
+                    raise RuntimeError(f"CPU oracle and GPU differ on frame seed {1 + rep} in {k}: "
+                                       f"{st[k]} != {step_stats[rep][k]}")
+        if rep < 3:
+            ctx.glint_estimate_device(qbuf, n, 1 + rep, out)
+            g = out.download(np.zeros((n, 3), np.float32))
+            if not np.array_equal(g, rgb, equal_nan=True):
+                raise RuntimeError(f"CPU oracle and GPU RGB differ on frame seed {1 + rep}: "
+                                   f"{int((g != rgb).any(1).sum())} queries")
+            checked_rgb += 1
+        return st["walks"], st["walks_converged"]
+
+    r16 = timed_cpu(frame, args.cpu_seconds)
+    orc.close()
+    orc1 = oracle_lib.Oracle(sc, cfg, threads=1)
+    r1 = timed_cpu(lambda rep: (lambda st: (st["walks"], st["walks_converged"]))(orc1.glint_estimate(q, 1 + rep)[1]),
+                   args.cpu_seconds / 3)
+    orc1.close()
+    return {"value": r16["converged_per_s"], "unit": "walks/s", "cores": CPU_THREADS, "kind": "port",
+            "sample": f"{r16['reps']} whole frames of the same workload ({n} queries each, {r16['walks']} walks, "
+                      f"{r16['seconds']}s on {CPU_THREADS} threads); frames checked equal to the GPU's: "
+                      f"{min(r16['reps'], len(step_stats))} by statistics, {checked_rgb} bit for bit in RGB",
+            "walks_per_s": r16["walks_per_s"], "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+            "one_core": {"value": r1["converged_per_s"], "walks_per_s": r1["walks_per_s"], "cores": 1,
+                         "sample": f"{r1['reps']} whole frames, {r1['seconds']}s"}}
+
+
+def query_sample_baseline(sc, cfg, q_all, kind, seconds, stride):
+    """CPU oracle on a strided sample of the given queries (every stride-th; its own random
+    streams), CPU_THREADS threads for `seconds`, then one thread for a third of that."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
-    from smsamd import abi
-    q = ctx.glint_queries(args.spp, seed=1)
-    threads = max(1, min(16, os.cpu_count() or 1))
-    orc = oracle_lib.Oracle(sc, cfg, threads=threads)
-    # whole frames (same queries, per-frame seeds as the GPU steps) until ~cpu_seconds elapsed
-    walks = conv = frames = 0
-    t = time.perf_counter()
-    while True:
-        _, st = orc.glint_estimate(q, 1 + frames)
-        walks += st["walks"]
-        conv += st["walks_converged"]
-        frames += 1
-        dt = time.perf_counter() - t
-        if dt >= args.cpu_seconds or frames >= 1000:
-            break
-    orc.close()
-    return {"value": conv / dt, "unit": "walks/s", "cores": threads, "kind": "port",
-            "sample": f"{frames} whole frames of the same workload ({len(q)} queries each, {walks} walks, "
-                      f"{dt:.1f}s on {threads} threads)", "walks_per_s": walks / dt}
+    res = {}
+    for threads, secs in ((CPU_THREADS, seconds), (1, seconds / 3)):
+        orc = oracle_lib.Oracle(sc, cfg, threads=threads)
+        fn = orc.ss_estimate if kind == "ss" else orc.ms_estimate
+
+        def run(rep):
+            qs = np.ascontiguousarray(q_all[(rep * 7919) % stride::stride])
+            _, st = fn(qs, 1 + rep)
+            return st["walks"], st["walks_converged"]
+        res[threads] = timed_cpu(run, secs)
+        orc.close()
+    r16, r1 = res[CPU_THREADS], res[1]
+    return {"value": r16["converged_per_s"], "unit": "walks/s", "cores": CPU_THREADS, "kind": "port",
+            "sample": f"{r16['reps']} strided samples (1/{stride}) of the same queries, {r16['walks']} walks in "
+                      f"{r16['seconds']}s on {CPU_THREADS} threads",
+            "walks_per_s": r16["walks_per_s"], "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+            "one_core": {"value": r1["converged_per_s"], "walks_per_s": r1["walks_per_s"], "cores": 1,
+                         "sample": f"{r1['reps']} samples, {r1['walks']} walks, {r1['seconds']}s"}}
 
 
 if __name__ == "__main__":

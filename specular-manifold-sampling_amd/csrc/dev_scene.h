@@ -201,12 +201,40 @@ SMS_DEV bool better(float t, int shape, int prim, float bt, int bshape, int bpri
 
 struct Hit { bool hit; float t; int shape, prim; float b1, b2, lx, ly; };
 
+// Entry distance of a ray into a (padded) box, or +inf if the conservative slab test misses it;
+// the test is box_hit's, so a box is entered exactly when box_hit accepts it.
+SMS_DEV float box_enter(float4 a, float4 b, const Ray &r, float tmax) {
+    float t0 = r.mint, t1 = tmax;
+    float ta, tb, lo, hi;
+    ta = (a.x - r.o.x) * r.d_rcp.x; tb = (b.x - r.o.x) * r.d_rcp.x;
+    if (ta != ta) ta = -__builtin_inff();
+    if (tb != tb) tb = __builtin_inff();
+    lo = fmin_(ta, tb); hi = fmax_(ta, tb); t0 = fmax_(t0, lo); t1 = fmin_(t1, hi);
+    ta = (a.y - r.o.y) * r.d_rcp.y; tb = (b.y - r.o.y) * r.d_rcp.y;
+    if (ta != ta) ta = -__builtin_inff();
+    if (tb != tb) tb = __builtin_inff();
+    lo = fmin_(ta, tb); hi = fmax_(ta, tb); t0 = fmax_(t0, lo); t1 = fmin_(t1, hi);
+    ta = (a.z - r.o.z) * r.d_rcp.z; tb = (b.z - r.o.z) * r.d_rcp.z;
+    if (ta != ta) ta = -__builtin_inff();
+    if (tb != tb) tb = __builtin_inff();
+    lo = fmin_(ta, tb); hi = fmax_(ta, tb); t0 = fmax_(t0, lo); t1 = fmin_(t1, hi);
+    return t0 <= t1 * (1.f + 4.f * EPS_F) ? t0 : __builtin_inff();
+}
+
 // Scene::ray_intersect / ray_test (scene_native.inl:17-55): closest hit over every shape,
 // ties broken by (t, shape, prim) so the result does not depend on traversal order.
 // BVH = false drops the triangle-BVH traversal (and its per-lane stack) for scenes made of
-// analytic shapes only.  The host rejects BVHs deeper than SMS_BVH_STACK.
+// analytic shapes only.  LDS_STACK = true keeps the traversal stack in LDS (SMS_BVH_STACK
+// entries per lane, interleaved by lane; the kernel's blocks must have 256 threads) instead of
+// per-lane scratch memory.  The host rejects BVHs deeper than SMS_BVH_STACK.
+//
+// Traversal: the two children of a node sit side by side in one 64-byte aligned pair; both
+// boxes are tested against the current closest hit, the nearer entered box is visited next and
+// the other pushed; a popped node is re-tested against the closest hit found since.  A box is
+// skipped only when its entry lies beyond the closest hit, so every triangle that could be the
+// closest hit or tie with it is tested and the result is that of an exhaustive search.
 constexpr int SMS_BVH_STACK = 64;
-template <bool BVH>
+template <bool BVH, bool LDS_STACK = false>
 SMS_DEV Hit scene_hit_t(const DScene &S, const Ray &ray, bool any_hit) {
     Hit h; h.hit = false; h.t = __builtin_inff(); h.shape = 1 << 30; h.prim = 1 << 30;
     h.b1 = h.b2 = h.lx = h.ly = 0.f;
@@ -229,14 +257,21 @@ SMS_DEV Hit scene_hit_t(const DScene &S, const Ray &ray, bool any_hit) {
     const bool finite = isfinite(ray.o.x) && isfinite(ray.o.y) && isfinite(ray.o.z) && isfinite(ray.d.x) &&
                         isfinite(ray.d.y) && isfinite(ray.d.z);
     if (BVH && S.n_nodes > 0 && finite) {
-        int stack[SMS_BVH_STACK];
+        int local_stack[LDS_STACK ? 1 : SMS_BVH_STACK];
+        int *stack = local_stack;
+        int stride = 1;
+        if (LDS_STACK) {
+            __shared__ int lds_stack[SMS_BVH_STACK * 256];
+            stack = lds_stack + threadIdx.x;
+            stride = 256;
+        }
         int sp = 0;
-        int node = 0;
+        DNode nd = S.nodes[0];
+        bool have = box_enter(nd.a, nd.b, ray, fmin_(ray.maxt, h.t)) < __builtin_inff();
         while (true) {
-            DNode nd = S.nodes[node];
-            int count = __float_as_int(nd.b.w);
-            int link = __float_as_int(nd.a.w);
-            if (box_hit(nd.a, nd.b, ray, fmin_(ray.maxt, h.t))) {
+            if (have) {
+                const int count = __float_as_int(nd.b.w);
+                const int link = __float_as_int(nd.a.w);
                 if (count > 0) {
                     for (int k = link; k < link + count; ++k) {
                         float4 r0 = S.tris[3 * k], r1 = S.tris[3 * k + 1], r2 = S.tris[3 * k + 2];
@@ -251,18 +286,25 @@ SMS_DEV Hit scene_hit_t(const DScene &S, const Ray &ray, bool any_hit) {
                         }
                     }
                 } else {
-                    // visit the nearer child first (by box centre along the ray direction)
-                    DNode c0 = S.nodes[link], c1 = S.nodes[link + 1];
-                    float d0 = ((c0.a.x + c0.b.x) * ray.d.x + (c0.a.y + c0.b.y) * ray.d.y) + (c0.a.z + c0.b.z) * ray.d.z;
-                    float d1 = ((c1.a.x + c1.b.x) * ray.d.x + (c1.a.y + c1.b.y) * ray.d.y) + (c1.a.z + c1.b.z) * ray.d.z;
-                    int first = d0 <= d1 ? link : link + 1;
-                    stack[sp++] = first == link ? link + 1 : link;
-                    node = first;
-                    continue;
+                    const DNode c0 = S.nodes[link], c1 = S.nodes[link + 1];
+                    const float tmax = fmin_(ray.maxt, h.t);
+                    const float e0 = box_enter(c0.a, c0.b, ray, tmax), e1 = box_enter(c1.a, c1.b, ray, tmax);
+                    const bool h0 = e0 < __builtin_inff(), h1 = e1 < __builtin_inff();
+                    if (h0 && h1) {
+                        const bool first0 = e0 <= e1;
+                        stack[stride * sp++] = first0 ? link + 1 : link;
+                        nd = first0 ? c0 : c1;
+                        continue;
+                    }
+                    if (h0 || h1) {
+                        nd = h0 ? c0 : c1;
+                        continue;
+                    }
                 }
             }
             if (sp == 0) break;
-            node = stack[--sp];
+            nd = S.nodes[stack[stride * --sp]];
+            have = box_enter(nd.a, nd.b, ray, fmin_(ray.maxt, h.t)) < __builtin_inff();
         }
     }
     return h;

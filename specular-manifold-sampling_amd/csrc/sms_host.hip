@@ -79,6 +79,9 @@ BvhOut build_bvh(std::vector<BItem> &items) {
     const int BINS = 16, MAX_LEAF = 8;
     struct Task { int node, lo, hi, depth; };
     std::vector<Task> stack;
+    // node 0 is the root and node 1 unused, so every sibling pair starts at an even index: the
+    // traversal reads both children of a node as one 64-byte aligned pair
+    out.nodes.push_back(DNode());
     out.nodes.push_back(DNode());
     stack.push_back({0, 0, n, 0});
     while (!stack.empty()) {

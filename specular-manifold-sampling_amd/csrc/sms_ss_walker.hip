@@ -235,7 +235,7 @@ __device__ __forceinline__ void ss_walker_body(const DScene &S, const sms_ss_que
         // one ray cast per lane
         Hit h;
         h.hit = false; h.t = __builtin_inff(); h.shape = 1 << 30; h.prim = 1 << 30; h.b1 = h.b2 = h.lx = h.ly = 0.f;
-        if (has_ray) h = scene_hit(S, ray, any_hit);
+        if (has_ray) h = scene_hit_t<true, true>(S, ray, any_hit);
         if (!has_ray) continue;
         if (L.stage == WK_VIS) {
             if (h.hit) { end_chain(); continue; }
