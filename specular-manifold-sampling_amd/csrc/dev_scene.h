@@ -648,12 +648,13 @@ SMS_DEV EI sample_emitter_interaction(const DScene &S, V3 si_p, const int *emitt
     return ei;
 }
 
-// SpecularManifold::emitter_interaction_to_vertex (manifold.h:263-302)
+// SpecularManifold::emitter_interaction_to_vertex (manifold.h:263-302); LDS_STACK as scene_hit_t
+template <bool LDS_STACK = false>
 SMS_DEV bool emitter_interaction_to_vertex(const DScene &S, const EI &ei, V3 p, MV &vy) {
     if (ei.kind == SMS_EI_AREA) {
         V3 d_tmp = normalize(ei.p - p);
         Ray r = make_ray(p + SHADOW_EPS_F * d_tmp, d_tmp);
-        SI si_y = ray_intersect(S, r);
+        SI si_y = make_si(S, r, scene_hit_t<true, LDS_STACK>(S, r, false));
         if (!si_y.valid) { vy = mv_point(v3(0.f, 0.f, 0.f)); return false; }
         vy = mv_from_si(S, si_y);
         make_orthonormal(vy);

@@ -24,16 +24,18 @@ namespace {
 enum : int { WK_NONE = 0, WK_SEED = 1, WK_NEWTON = 2, WK_VIS = 3 };
 constexpr int WK_CHUNK = 64;   // queries a wavefront claims from the queue at a time
 
-// Per-lane state.  A walk keeps its manifold vertex and, of the surface interaction that
-// produced it, the uv and shading normal: a walk can only converge right after an accepted
-// reprojection (the constraint depends on the vertex alone), so on success that interaction
-// is the solution, valid exactly when at least one iteration ran.
+// Per-lane state, kept small so that it is all a lane carries through the ray cast: of the
+// query only its index, random-stream key and shading point (the rest is re-read from the query
+// array when a walk ends), of the walk's manifold vertex only the hit that produced it -- the
+// vertex is rebuilt from that record (make_si + mv_from_si, deterministic) at every Newton step,
+// which is where the old code built it after each accepted reprojection.  The emitter
+// interaction lives in LDS.  A walk can only converge right after an accepted reprojection (the
+// constraint depends on the vertex alone), so on success the rebuilt interaction is the
+// solution, valid exactly when at least one iteration ran.
 struct WalkerLane {
     // query
     int64_t q;
-    SI si;
-    V3 wi_local;
-    EI ei;
+    V3 si_p;
     uint64_t qkey;
     // chain (query, caster)
     int c, shape;
@@ -42,11 +44,10 @@ struct WalkerLane {
     V3 part_first;               // bsdf * contribution of the first solution, kept until its shadow ray
     V3 p_first;
     V2 uv_first;
-    // walk
+    // walk: the record of the hit that made the current vertex (a ray from si_p along key_d)
     int stage;
-    MV vtx;
-    V2 uv_cur;
-    V3 shn_cur;
+    Hit key;
+    V3 key_d;
     float beta;
     int its;
     // counters (per lane and call; far below 2^32)
@@ -63,10 +64,15 @@ __device__ __forceinline__ void ss_walker_body(const DScene &S, const sms_ss_que
     const int lane = threadIdx.x & 63;
     const V3 n_offset = v3(0.f, 0.f, 1.f);
     const Proposal &P = W.prop;
+    __shared__ EI ei_lds[256];
+    EI &ei_lane = ei_lds[threadIdx.x];
     WalkerLane L;
     L.stage = WK_NONE; L.q = -1; L.c = 0; L.shape = -1; L.slot = 0; L.k = 0; L.its = 0; L.beta = 1.f;
     L.walks = L.conv = L.iters = L.bcalls = L.btrials = L.bmax = 0u;
     L.qkey = 0;
+    L.si_p = v3(0.f, 0.f, 0.f);
+    L.key_d = v3(0.f, 0.f, 0.f);
+    L.key.hit = false; L.key.t = 0.f; L.key.shape = L.key.prim = 0; L.key.b1 = L.key.b2 = L.key.lx = L.key.ly = 0.f;
     // wavefront-uniform query pool
     int64_t pool_next = 0, pool_end = 0;
     bool exhausted = false;
@@ -91,30 +97,12 @@ __device__ __forceinline__ void ss_walker_body(const DScene &S, const sms_ss_que
         invp[L.slot] = inv_prob;
         const float wgt = mean3(&part[3 * L.slot]) * inv_prob;
         if (wgt > 0.f && capturing(P, L.shape, SMS_CAPTURE_ESTIMATES))
-            capture_sample(S, P, L.si, L.ei, L.uv_first, wgt);
+            capture_sample(S, P, ss_query_si(Q[L.q]), ei_lane, L.uv_first, wgt);
         end_chain();
     };
-    // a walk ended: ok and its iterations; on success the solution is L.vtx (uv L.uv_cur)
-    auto finish_walk = [&](bool ok, int iterations) {
-        L.walks++; L.iters += (unsigned) iterations;
-        if (ok) L.conv++;
-        if (ok && capturing(P, L.shape, SMS_CAPTURE_SOLUTIONS)) capture_sample(S, P, L.si, L.ei, L.uv_cur, 1.f);
-        if (L.k == 0) {
-            if (!ok) { end_chain(); return; }
-            L.direction = normalize(L.vtx.p - L.si.p);
-            // the contribution is evaluated now (before the shadow ray) so the solution's
-            // interaction need not be kept; it is discarded if the shadow ray is blocked
-            V3 spec = path_contribution_ss_mv(S, L.si, L.ei, L.vtx, L.shn_cur);
-            V3 dl = v3(dot(L.direction, L.si.sh_s), dot(L.direction, L.si.sh_t), dot(L.direction, L.si.sh_n));
-            V3 bsdf = receiver_bsdf_eval(S, L.si, L.wi_local, dl);
-            L.part_first = v3(bsdf.x * spec.x, bsdf.y * spec.y, bsdf.z * spec.z);
-            L.p_first = L.vtx.p;
-            L.uv_first = L.uv_cur;
-            L.stage = WK_VIS;
-            return;
-        }
-        V3 dtr = normalize(L.vtx.p - L.si.p);
-        if (ok && fabsf(dot(L.direction, dtr) - 1.f) < cfg.uniqueness_threshold) { resolve((float) L.k); return; }
+    // the chain's trial k did not re-find its solution: the next trial, or the cap, or the
+    // overflow list for the cooperative pass
+    auto next_trial = [&]() {
         L.k++;
         if (cfg.max_trials > 0 && L.k > cfg.max_trials) { resolve(0.f); return; }
         if (L.k > W.lane_trials) {
@@ -131,6 +119,36 @@ __device__ __forceinline__ void ss_walker_body(const DScene &S, const sms_ss_que
         }
         L.stage = WK_SEED;
     };
+    // a walk ended without a solution after `iterations` Newton iterations
+    auto walk_failed = [&](int iterations) {
+        L.walks++; L.iters += (unsigned) iterations;
+        if (L.k == 0) { end_chain(); return; }
+        next_trial();
+    };
+    // a walk converged: its solution is vtx, made by the surface interaction hs
+    auto walk_converged = [&](int iterations, const MV &vtx, const SI &hs) {
+        L.walks++; L.iters += (unsigned) iterations;
+        L.conv++;
+        if (capturing(P, L.shape, SMS_CAPTURE_SOLUTIONS)) capture_sample(S, P, ss_query_si(Q[L.q]), ei_lane, hs.uv, 1.f);
+        if (L.k == 0) {
+            L.direction = normalize(vtx.p - L.si_p);
+            // the contribution is evaluated now (before the shadow ray) so the solution's
+            // interaction need not be kept; it is discarded if the shadow ray is blocked
+            const sms_ss_query g = Q[L.q];
+            const SI si = ss_query_si(g);
+            V3 spec = path_contribution_ss_mv<true>(S, si, ei_lane, vtx, hs.sh_n);
+            V3 dl = v3(dot(L.direction, si.sh_s), dot(L.direction, si.sh_t), dot(L.direction, si.sh_n));
+            V3 bsdf = receiver_bsdf_eval(S, si, ld3(g.wi), dl);
+            L.part_first = v3(bsdf.x * spec.x, bsdf.y * spec.y, bsdf.z * spec.z);
+            L.p_first = vtx.p;
+            L.uv_first = hs.uv;
+            L.stage = WK_VIS;
+            return;
+        }
+        V3 dtr = normalize(vtx.p - L.si_p);
+        if (fabsf(dot(L.direction, dtr) - 1.f) < cfg.uniqueness_threshold) { resolve((float) L.k); return; }
+        next_trial();
+    };
     auto start_query = [&](int64_t q) {
         L.q = q;
         const sms_ss_query g = Q[q];
@@ -143,12 +161,11 @@ __device__ __forceinline__ void ss_walker_body(const DScene &S, const sms_ss_que
             L.stage = WK_NONE;
             return;
         }
-        L.si = ss_query_si(g);
-        L.wi_local = ld3(g.wi);
+        L.si_p = ld3(g.p);
         L.qkey = query_key(seed, (uint64_t) q);
         Pcg32 rq;
         stream_seed(rq, L.qkey, 0);
-        L.ei = sample_emitter_interaction(S, L.si.p, S.emitters_ss, S.n_emitters_ss, rq);
+        ei_lane = sample_emitter_interaction(S, L.si_p, S.emitters_ss, S.n_emitters_ss, rq);
         L.c = 0;
         begin_chain();
     };
@@ -168,7 +185,7 @@ __device__ __forceinline__ void ss_walker_body(const DScene &S, const sms_ss_que
                     V3 p_start = v3(0.f, 0.f, 0.f);
                     if (P.gmm && L.shape == P.shape) {
                         if (!proposal_seed(S, P, P.gmm + L.q * PROP_RAW, rw, s, p_start)) {
-                            finish_walk(false, 0);
+                            walk_failed(0);
                             continue;
                         }
                     } else {
@@ -176,41 +193,47 @@ __device__ __forceinline__ void ss_walker_body(const DScene &S, const sms_ss_que
                     }
                     PS ps = sample_position(S, L.shape, s);
                     if (p_start.x != 0.f || p_start.y != 0.f || p_start.z != 0.f) ps.p = p_start;
-                    V3 d_tmp = normalize(ps.p - L.si.p);
+                    V3 d_tmp = normalize(ps.p - L.si_p);
                     if (cfg.mnee_init) {
                         const DShape &sh = S.shapes[L.shape];
-                        if (sh.k != 0.f) d_tmp = normalize(ld3(sh.bbox_center) - L.si.p);
-                        else d_tmp = normalize(L.ei.p - L.si.p);
+                        if (sh.k != 0.f) d_tmp = normalize(ld3(sh.bbox_center) - L.si_p);
+                        else d_tmp = normalize(ei_lane.p - L.si_p);
                     }
-                    ray = make_ray(L.si.p, d_tmp);
+                    ray = make_ray(L.si_p, d_tmp);
                     has_ray = true; any_hit = false;
                 } else if (L.stage == WK_NEWTON) {
-                    // one iteration of newton_solver (manifold_ss.cpp:329-405)
-                    if (L.its >= cfg.max_iterations) { finish_walk(false, L.its); continue; }
-                    Step st = cfg.halfvector_constraints ? step_halfvector(L.si.p, L.vtx, L.ei, n_offset)
-                                                         : step_anglediff(L.si.p, L.vtx, L.ei, n_offset);
-                    if (!st.ok) { finish_walk(false, L.its); continue; }
+                    // one iteration of newton_solver (manifold_ss.cpp:329-405) at the vertex the
+                    // last accepted hit made
+                    const SI hs = make_si(S, make_ray(L.si_p, L.key_d), L.key);
+                    const MV vtx = mv_from_si(S, hs);
+                    if (L.its >= cfg.max_iterations) { walk_failed(L.its); continue; }
+                    const EI ei = ei_lane;
+                    Step st = cfg.halfvector_constraints ? step_halfvector(L.si_p, vtx, ei, n_offset)
+                                                         : step_anglediff(L.si_p, vtx, ei, n_offset);
+                    if (!st.ok) { walk_failed(L.its); continue; }
                     if (norm2(st.C) < cfg.solver_threshold) {
-                        V3 wx = normalize(L.si.p - L.vtx.p);
-                        V3 wy = L.ei.kind == SMS_EI_DIRECTIONAL ? L.ei.d : normalize(L.ei.p - L.vtx.p);
-                        float cos_theta_x = dot(L.vtx.gn, wx), cos_theta_y = dot(L.vtx.gn, wy);
+                        V3 wx = normalize(L.si_p - vtx.p);
+                        V3 wy = ei.kind == SMS_EI_DIRECTIONAL ? ei.d : normalize(ei.p - vtx.p);
+                        float cos_theta_x = dot(vtx.gn, wx), cos_theta_y = dot(vtx.gn, wy);
                         bool refraction = cos_theta_x * cos_theta_y < 0.f, reflection = !refraction;
-                        bool type_ok = !((L.vtx.eta == 1.f && !reflection) || (L.vtx.eta != 1.f && !refraction));
+                        bool type_ok = !((vtx.eta == 1.f && !reflection) || (vtx.eta != 1.f && !refraction));
                         // converged at iteration 0 reports failure (no reprojection has run yet)
-                        finish_walk(type_ok && L.its > 0, L.its);
+                        if (type_ok && L.its > 0) walk_converged(L.its, vtx, hs);
+                        else walk_failed(L.its);
                         continue;
                     }
-                    V3 p_prop = L.vtx.p - cfg.step_scale * L.beta * (L.vtx.dp_du * st.dX.x + L.vtx.dp_dv * st.dX.y);
-                    V3 d_prop = normalize(p_prop - L.si.p);
-                    ray = make_ray(L.si.p, d_prop);
+                    V3 p_prop = vtx.p - cfg.step_scale * L.beta * (vtx.dp_du * st.dX.x + vtx.dp_dv * st.dX.y);
+                    V3 d_prop = normalize(p_prop - L.si_p);
+                    ray = make_ray(L.si_p, d_prop);
                     has_ray = true; any_hit = false;
                 } else {
                     // shadow ray of the first solution (ss_visible)
+                    const V3 ep = ei_lane.p;
                     V3 p = L.p_first;
-                    V3 d = p - L.ei.p;
+                    V3 d = p - ep;
                     float dist = norm(d);
                     d = d * (1.f / dist);
-                    ray = make_ray(L.ei.p, d, RAY_EPS_F * (1.f + hmax_abs(L.ei.p)), dist * (1.f - RAY_EPS_F));
+                    ray = make_ray(ep, d, RAY_EPS_F * (1.f + hmax_abs(ep)), dist * (1.f - RAY_EPS_F));
                     has_ray = true; any_hit = true;
                 }
             }
@@ -247,18 +270,13 @@ __device__ __forceinline__ void ss_walker_body(const DScene &S, const sms_ss_que
             L.stage = WK_SEED;
             continue;
         }
-        // seed ray or reprojection: the surface interaction and, if it is on the walk's shape,
-        // its manifold vertex
-        SI hit_si = make_si(S, ray, h);
-        const int want = L.stage == WK_SEED ? L.shape : L.vtx.shape;
-        const bool on_shape = hit_si.valid && want == hit_si.shape;
-        if (on_shape) {
-            L.vtx = mv_from_si(S, hit_si);
-            L.uv_cur = hit_si.uv;
-            L.shn_cur = hit_si.sh_n;
-        }
+        // seed ray or reprojection: a hit on the walk's shape becomes the walk's vertex (kept as
+        // its hit record); the seed ray's target is the chain's shape, a reprojection's the shape
+        // of the current vertex, which is the same shape
+        const bool on_shape = h.hit && h.shape == L.shape;
+        if (on_shape) { L.key = h; L.key_d = ray.d; }
         if (L.stage == WK_SEED) {
-            if (!on_shape) { finish_walk(false, 0); continue; }
+            if (!on_shape) { walk_failed(0); continue; }
             L.beta = 1.f; L.its = 0;
             L.stage = WK_NEWTON;
         } else {
