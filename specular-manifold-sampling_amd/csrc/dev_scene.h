@@ -224,9 +224,10 @@ SMS_DEV float box_enter(float4 a, float4 b, const Ray &r, float tmax) {
 // Scene::ray_intersect / ray_test (scene_native.inl:17-55): closest hit over every shape,
 // ties broken by (t, shape, prim) so the result does not depend on traversal order.
 // BVH = false drops the triangle-BVH traversal (and its per-lane stack) for scenes made of
-// analytic shapes only.  LDS_STACK = true keeps the traversal stack in LDS (SMS_BVH_STACK
-// entries per lane, interleaved by lane; the kernel's blocks must have 256 threads) instead of
-// per-lane scratch memory.  The host rejects BVHs deeper than SMS_BVH_STACK.
+// analytic shapes only.  LDS_STACK = true keeps the traversal stack in LDS (LDS_DEPTH entries
+// per lane, interleaved by lane; the kernel's blocks must have 256 threads and the scene's BVH
+// must be at most LDS_DEPTH deep) instead of per-lane scratch memory.  The host rejects BVHs
+// deeper than SMS_BVH_STACK.
 //
 // Traversal: the two children of a node sit side by side in one 64-byte aligned pair; both
 // boxes are tested against the current closest hit, the nearer entered box is visited next and
@@ -234,7 +235,7 @@ SMS_DEV float box_enter(float4 a, float4 b, const Ray &r, float tmax) {
 // skipped only when its entry lies beyond the closest hit, so every triangle that could be the
 // closest hit or tie with it is tested and the result is that of an exhaustive search.
 constexpr int SMS_BVH_STACK = 64;
-template <bool BVH, bool LDS_STACK = false>
+template <bool BVH, bool LDS_STACK = false, int LDS_DEPTH = SMS_BVH_STACK>
 SMS_DEV Hit scene_hit_t(const DScene &S, const Ray &ray, bool any_hit) {
     Hit h; h.hit = false; h.t = __builtin_inff(); h.shape = 1 << 30; h.prim = 1 << 30;
     h.b1 = h.b2 = h.lx = h.ly = 0.f;
@@ -261,7 +262,7 @@ SMS_DEV Hit scene_hit_t(const DScene &S, const Ray &ray, bool any_hit) {
         int *stack = local_stack;
         int stride = 1;
         if (LDS_STACK) {
-            __shared__ int lds_stack[SMS_BVH_STACK * 256];
+            __shared__ int lds_stack[LDS_DEPTH * 256];
             stack = lds_stack + threadIdx.x;
             stride = 256;
         }
@@ -649,12 +650,12 @@ SMS_DEV EI sample_emitter_interaction(const DScene &S, V3 si_p, const int *emitt
 }
 
 // SpecularManifold::emitter_interaction_to_vertex (manifold.h:263-302); LDS_STACK as scene_hit_t
-template <bool LDS_STACK = false>
+template <bool LDS_STACK = false, int LDS_DEPTH = SMS_BVH_STACK>
 SMS_DEV bool emitter_interaction_to_vertex(const DScene &S, const EI &ei, V3 p, MV &vy) {
     if (ei.kind == SMS_EI_AREA) {
         V3 d_tmp = normalize(ei.p - p);
         Ray r = make_ray(p + SHADOW_EPS_F * d_tmp, d_tmp);
-        SI si_y = make_si(S, r, scene_hit_t<true, LDS_STACK>(S, r, false));
+        SI si_y = make_si(S, r, scene_hit_t<true, LDS_STACK, LDS_DEPTH>(S, r, false));
         if (!si_y.valid) { vy = mv_point(v3(0.f, 0.f, 0.f)); return false; }
         vy = mv_from_si(S, si_y);
         make_orthonormal(vy);

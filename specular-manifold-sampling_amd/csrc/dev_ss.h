@@ -366,7 +366,7 @@ SMS_DEV float geometric_term_ss(const MV &v0, const MV &v1, const MV &v2v) {
 
 // evaluate_path_contribution SS (manifold_ss.cpp:281-327); si = receiver (query) interaction
 // vfinal = mv_from_si of the (valid) solution surface interaction, sh_n_final its shading normal
-template <bool LDS_STACK = false>
+template <bool LDS_STACK = false, int LDS_DEPTH = SMS_BVH_STACK>
 SMS_DEV V3 path_contribution_ss_mv(const DScene &S, const SI &si, const EI &ei_, const MV &vfinal, V3 sh_n_final) {
     MV vtx = vfinal;
     make_orthonormal(vtx);
@@ -384,7 +384,7 @@ SMS_DEV V3 path_contribution_ss_mv(const DScene &S, const SI &si, const EI &ei_,
         ei.weight = ei.weight * (1.f / emitter_pdf);
     }
     MV vy;
-    if (!emitter_interaction_to_vertex<LDS_STACK>(S, ei, vtx.p, vy)) return v3(0.f, 0.f, 0.f);
+    if (!emitter_interaction_to_vertex<LDS_STACK, LDS_DEPTH>(S, ei, vtx.p, vy)) return v3(0.f, 0.f, 0.f);
     MV vx = mv_from_si(S, si);
     make_orthonormal(vx);
     float F = specular_reflectance_ss_n(S, vfinal.shape, sh_n_final, normalize(ei.p - vfinal.p));

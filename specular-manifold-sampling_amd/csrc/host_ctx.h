@@ -43,6 +43,7 @@ struct sms_ctx {
     int occ_glint = 0, occ_ss = 0, occ_coop = 0, occ_biased = 0, occ_sscoop = 0, occ_mw = 0;   // occ_glint: trial pass
     int occ_glint_biased = 0;
     bool area_emitters = false;                 // glint BSDF-sampling strategy can contribute
+    int bvh_depth = 0;                          // depth of the triangle BVH (0: no meshes)
     int occ_ms[2] = {0, 0}, occ_mscoop[2] = {0, 0}, occ_msbiased[2] = {0, 0};   // [0]: chains <= 2, [1]: <= 4
     int occ_mspersist[2] = {0, 0};
     // host copies the learned proposal's UV grid is built from

@@ -151,6 +151,9 @@ __global__ void k_glint_multiwave_resolve(DScene S, EstimatorWork W);
 __global__ void k_glint_coop_seq(DScene S, const uint64_t *seedp, EstimatorWork W);
 __global__ void k_glint_finalize(DScene S, const sms_glint_query *Q, const uint64_t *seedp, float *out, EstimatorWork W);
 __global__ void k_ss_estimate(DScene S, const sms_ss_query *Q, int64_t n, const uint64_t *seedp, float *part, float *invp, EstimatorWork W);
+// LDS traversal-stack entries per lane of the persistent walker; scenes whose BVH is deeper run
+// the lane-per-query first pass instead
+constexpr int WALKER_STACK = 32;
 __global__ void k_ss_walker(DScene S, const sms_ss_query *Q, int64_t n, const uint64_t *seedp, float *part, float *invp,
                             EstimatorWork W);
 __global__ void k_ss_coop(DScene S, const sms_ss_query *Q, const uint64_t *seedp, const float *part, float *invp, EstimatorWork W);

@@ -213,6 +213,7 @@ struct HostScene {
     std::vector<uint4> faces;
     std::vector<float> cdf, pmf;
     std::vector<DNode> nodes;
+    int bvh_depth = 0;
     std::vector<float4> tris;
     float box_c[3] = {0.f, 0.f, 0.f}, box_inv = 1.f;
     bool has_cam = false;
@@ -411,6 +412,7 @@ sms_status prepare_scene(const sms_scene_desc *desc, HostScene &H, std::string &
     if (bvh.max_depth > SMS_BVH_STACK)
         return bad(SMS_E_UNSUPPORTED, "BVH deeper than the traversal stack");
     H.nodes = std::move(bvh.nodes);
+    H.bvh_depth = bvh.max_depth;
     std::vector<float4> &tris = H.tris;
     tris.resize(3 * bvh.order.size());
     std::vector<int> face_prim_base(desc->n_shapes, 0);
@@ -463,6 +465,7 @@ sms_status upload_scene(sms_ctx *ctx, HostScene &H, const sms_config *cfg) {
         nmaps.push_back(m);
     }
     ctx->area_emitters = H.area_emitters;
+    ctx->bvh_depth = H.bvh_depth;
     for (int a = 0; a < 3; ++a) S.box_c[a] = H.box_c[a];
     S.box_inv = H.box_inv;
     ctx->h_vuv = H.vuv; ctx->h_faces = H.faces; ctx->h_shapes = H.shapes;
@@ -1078,9 +1081,10 @@ sms_status ss_enqueue(sms_ctx *ctx, const sms_ss_query *q, int64_t n, float *out
             HIP_TRY(ctx, hipGetLastError());
             EV_RECORD(ctx, ctx->ev1);
         } else {
-            if (ctx->ss_first_pass == 0 && !cfg.twostage) {
+            if (ctx->ss_first_pass == 0 && !cfg.twostage && ctx->bvh_depth <= WALKER_STACK) {
                 // persistent: the resident blocks take queries from the device queue
-                // (the two-stage solver runs in the lane-per-query pass)
+                // (the two-stage solver, and scenes whose BVH is deeper than the walker's LDS
+                // stack, run the lane-per-query pass)
                 const int per_cu = ctx->blocks_per_cu > 0 ? ctx->blocks_per_cu : ctx->occ_walker;
                 hipLaunchKernelGGL(k_ss_walker, dim3((unsigned) (per_cu * ctx->n_cu)), dim3(256), 0, ctx->stream,
                                    ctx->scene, q, n, seed, ctx->part, ctx->invp, W);

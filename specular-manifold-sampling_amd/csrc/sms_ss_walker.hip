@@ -136,7 +136,7 @@ __device__ __forceinline__ void ss_walker_body(const DScene &S, const sms_ss_que
             // interaction need not be kept; it is discarded if the shadow ray is blocked
             const sms_ss_query g = Q[L.q];
             const SI si = ss_query_si(g);
-            V3 spec = path_contribution_ss_mv<true>(S, si, ei_lane, vtx, hs.sh_n);
+            V3 spec = path_contribution_ss_mv<true, WALKER_STACK>(S, si, ei_lane, vtx, hs.sh_n);
             V3 dl = v3(dot(L.direction, si.sh_s), dot(L.direction, si.sh_t), dot(L.direction, si.sh_n));
             V3 bsdf = receiver_bsdf_eval(S, si, ld3(g.wi), dl);
             L.part_first = v3(bsdf.x * spec.x, bsdf.y * spec.y, bsdf.z * spec.z);
@@ -258,7 +258,7 @@ __device__ __forceinline__ void ss_walker_body(const DScene &S, const sms_ss_que
         // one ray cast per lane
         Hit h;
         h.hit = false; h.t = __builtin_inff(); h.shape = 1 << 30; h.prim = 1 << 30; h.b1 = h.b2 = h.lx = h.ly = 0.f;
-        if (has_ray) h = scene_hit_t<true, true>(S, ray, any_hit);
+        if (has_ray) h = scene_hit_t<true, true, WALKER_STACK>(S, ray, any_hit);
         if (!has_ray) continue;
         if (L.stage == WK_VIS) {
             if (h.hit) { end_chain(); continue; }
@@ -289,7 +289,7 @@ __device__ __forceinline__ void ss_walker_body(const DScene &S, const sms_ss_que
     stat_add(W.stats + 3, L.bcalls); stat_add(W.stats + 4, L.btrials); stat_max(W.stats + 5, (unsigned long long) L.bmax);
 }
 
-__global__ void __launch_bounds__(256) k_ss_walker(DScene S, const sms_ss_query *__restrict__ Q, int64_t n,
+__global__ void __launch_bounds__(256, 3) k_ss_walker(DScene S, const sms_ss_query *__restrict__ Q, int64_t n,
                                                    const uint64_t *__restrict__ seedp, float *__restrict__ part, float *__restrict__ invp,
                                                    EstimatorWork W) {
     const uint64_t seed = *seedp;
