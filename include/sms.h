@@ -106,7 +106,7 @@ typedef struct sms_shape_desc {
     const uint32_t *faces;      /* n_faces x 3                                            */
 } sms_shape_desc;
 
-enum sms_emitter_type { SMS_EMITTER_AREA = 0, SMS_EMITTER_POINT = 1 };
+enum sms_emitter_type { SMS_EMITTER_AREA = 0, SMS_EMITTER_POINT = 1, SMS_EMITTER_DIRECTIONAL = 2 };
 
 typedef struct sms_emitter_desc {
     int32_t type;              /* sms_emitter_type                                         */
@@ -116,6 +116,8 @@ typedef struct sms_emitter_desc {
     float   intensity[3];      /* POINT: intensity (src/emitters/point.cpp:74-105)         */
     int32_t caustic_ss;        /* emitter is an SMS single-scatter target                  */
     int32_t caustic_ms;        /* emitter is an SMS multi-scatter target                   */
+    float   direction[3];      /* DIRECTIONAL: direction the light travels (unit length)   */
+    float   irradiance[3];     /* DIRECTIONAL: irradiance (src/emitters/directional.cpp:105-128) */
 } sms_emitter_desc;
 
 typedef struct sms_normalmap_desc {

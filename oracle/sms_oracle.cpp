@@ -38,6 +38,7 @@ enum {
     PLANT_MS_HV_JAC,        /* dC_i/dx_i (0,0) of the multi-scatter half-vector blocks          */
     PLANT_MNEE_TARGET,      /* scale of the light point the MNEE seed ray aims at              */
     PLANT_MS_G,             /* the multi-scatter generalized geometric term                     */
+    PLANT_SS_G_DIR,         /* the single-scatter geometric term toward a directional emitter  */
     PLANT_N = 16
 };
 static float g_plant[PLANT_N] = {1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f};
@@ -58,7 +59,7 @@ struct Shape {
     float inv_surface_area;
 };
 
-struct Emitter { int type, shape; V3 radiance, position, intensity; bool ss, ms; };
+struct Emitter { int type, shape; V3 radiance, position, intensity, direction, irradiance; bool ss, ms; };
 
 struct Normalmap {
     int res;
@@ -91,6 +92,7 @@ struct Scene {
     Camera cam;
     bool has_cam;
     float box_c[3], box_inv;             // bounding box centre, 1 / largest half extent
+    float bs_dist;                       // directional emitters: 2 x the scene's bounding-sphere radius
     sms_config cfg;
 };
 
@@ -295,6 +297,8 @@ static Scene *scene_create(const sms_scene_desc *desc, const sms_config *cfg) {
         em.radiance = v3(e.radiance[0], e.radiance[1], e.radiance[2]);
         em.position = v3(e.position[0], e.position[1], e.position[2]);
         em.intensity = v3(e.intensity[0], e.intensity[1], e.intensity[2]);
+        em.direction = v3(e.direction[0], e.direction[1], e.direction[2]);
+        em.irradiance = v3(e.irradiance[0], e.irradiance[1], e.irradiance[2]);
         em.ss = e.caustic_ss != 0; em.ms = e.caustic_ms != 0;
         S->emitters.push_back(em);
         S->emitters_all.push_back(i);
@@ -389,6 +393,19 @@ static Scene *scene_create(const sms_scene_desc *desc, const sms_config *cfg) {
     float half = 0.f;
     for (int a = 0; a < 3; ++a) { S->box_c[a] = 0.5f * (lo[a] + hi[a]); half = std::max(half, 0.5f * (hi[a] - lo[a])); }
     S->box_inv = half > 0.f ? 1.f / half : (float) 1.f;
+    /* BoundingBox::bounding_sphere of the scene (radius = |max - centre|), enlarged by
+       DirectionalEmitter::set_scene (src/emitters/directional.cpp:63-68); sample_direction places
+       the emitter 2 radii away (:112) */
+    {
+        double r2 = 0.0;
+        for (int a = 0; a < 3; ++a) {
+            double c = 0.5 * ((double) lo[a] + (double) hi[a]), d = (double) hi[a] - c;
+            r2 += d * d;
+        }
+        double r = sqrt(r2) * (1.0 + (double) RAY_EPS_F);
+        if (!(r > (double) RAY_EPS_F)) r = (double) RAY_EPS_F;
+        S->bs_dist = (float) (2.0 * r);
+    }
     S->has_cam = desc->camera.width > 0;
     if (S->has_cam) camera_setup(S->cam, desc->camera);
     return S;
@@ -907,6 +924,12 @@ static EI sample_emitter_interaction(const Scene &S, V3 si_p, const std::vector<
             spec = em.radiance / ps.pdf;        // area.cpp:71-78 with cos(wi=(0,0,1)) > 0
             ei.p = ps.p; ei.n = ps.n; ei.d = normalize(ps.p - si_p); ei.pdf = ps.pdf;
         }
+    } else if (em.type == SMS_EMITTER_DIRECTIONAL) {
+        /* DirectionalEmitter::sample_direction (src/emitters/directional.cpp:105-128) */
+        ei.kind = SMS_EI_DIRECTIONAL;
+        V3 d = em.direction;
+        ei.p = si_p - d * S.bs_dist; ei.d = -d; ei.n = -d; ei.pdf = 1.f;
+        spec = em.irradiance;
     } else {
         ei.kind = SMS_EI_POINT;
         /* PointLight::sample_direction (src/emitters/point.cpp:74-98) */
@@ -1283,7 +1306,7 @@ static float geometric_term_ss(const MV &v0, const MV &v1, const MV &v2v) {
         G = std::min(G, 10.f);
         G /= fabsf(dot(wi, v0.n));
         G *= eta * eta;
-        return G;
+        return G * g_plant[PLANT_SS_G_DIR];
     }
     if (fabsf(det2(dc1_dx1)) < 1e-6f) return 0.f;
     M2 inv = inverse2(dc1_dx1);
@@ -1512,6 +1535,8 @@ static V3 estimate_ss(const Scene &S, const sms_ss_query &q, uint64_t seed, uint
             return po;
         };
         auto visible = [&](const SI &sf) {
+            /* manifold_ss.cpp:88-102: toward a directional emitter the ray leaves the solution */
+            if (ei_is_directional(ei)) return !ray_test(S, make_ray(sf.p, ei.d));
             V3 d = sf.p - ei.p;
             float dist = norm(d);
             d = d * (1.f / dist);

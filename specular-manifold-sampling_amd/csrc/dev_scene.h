@@ -29,7 +29,7 @@ struct DShape {
     int cdf_offset, pad;
 };
 
-struct DEmitter { int type, shape; float radiance[3], position[3], intensity[3]; int ss, ms; };
+struct DEmitter { int type, shape; float radiance[3], position[3], intensity[3]; int ss, ms; float direction[3], irradiance[3]; };
 
 // Level-0 slopes stored with one replicated pad column and row ((res+1)^2 float2, row stride
 // res+1): texels (x, y), (x+1, y) of a bilinear footprint are adjacent for every x <= res-1,
@@ -71,6 +71,7 @@ struct DScene {
     const float *cdf, *pmf;              // mesh area distributions
     DCamera cam; int has_cam;
     float box_c[3], box_inv;             // scene bounding box centre, 1 / largest half extent
+    float bs_dist;                       // directional emitters: 2 x the scene's bounding-sphere radius
     sms_config cfg;
 };
 
@@ -634,6 +635,12 @@ SMS_DEV EI sample_emitter_interaction(const DScene &S, V3 si_p, const int *emitt
             spec = ld3(em.radiance) / ps.pdf;
             ei.p = ps.p; ei.n = ps.n; ei.d = normalize(ps.p - si_p); ei.pdf = ps.pdf;
         }
+    } else if (em.type == SMS_EMITTER_DIRECTIONAL) {
+        // DirectionalEmitter::sample_direction (src/emitters/directional.cpp:105-128)
+        ei.kind = SMS_EI_DIRECTIONAL;
+        V3 d = ld3(em.direction);
+        ei.p = si_p - d * S.bs_dist; ei.d = -d; ei.n = -d; ei.pdf = 1.f;
+        spec = ld3(em.irradiance);
     } else {
         ei.kind = SMS_EI_POINT;
         V3 pos = ld3(em.position);

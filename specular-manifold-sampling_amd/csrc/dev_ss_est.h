@@ -23,6 +23,8 @@ SMS_DEV V3 receiver_bsdf_eval(const DScene &S, const SI &si, V3 wi_local, V3 wo_
 }
 
 SMS_DEV bool ss_visible(const DScene &S, const EI &ei, V3 p) {
+    // manifold_ss.cpp:88-102: toward a directional emitter the ray leaves the solution
+    if (ei.kind == SMS_EI_DIRECTIONAL) return !ray_test(S, make_ray(p, ei.d));
     V3 d = p - ei.p;
     float dist = norm(d);
     d = d * (1.f / dist);

@@ -215,7 +215,7 @@ struct HostScene {
     std::vector<DNode> nodes;
     int bvh_depth = 0;
     std::vector<float4> tris;
-    float box_c[3] = {0.f, 0.f, 0.f}, box_inv = 1.f;
+    float box_c[3] = {0.f, 0.f, 0.f}, box_inv = 1.f, bs_dist = 0.f;
     bool has_cam = false;
     DCamera cam;
 
@@ -305,7 +305,15 @@ sms_status prepare_scene(const sms_scene_desc *desc, HostScene &H, std::string &
         DEmitter d;
         memset(&d, 0, sizeof(d));
         d.type = e.type; d.shape = e.shape;
-        for (int a = 0; a < 3; ++a) { d.radiance[a] = e.radiance[a]; d.position[a] = e.position[a]; d.intensity[a] = e.intensity[a]; }
+        for (int a = 0; a < 3; ++a) {
+            d.radiance[a] = e.radiance[a]; d.position[a] = e.position[a]; d.intensity[a] = e.intensity[a];
+            d.direction[a] = e.direction[a]; d.irradiance[a] = e.irradiance[a];
+        }
+        if (e.type != SMS_EMITTER_AREA && e.type != SMS_EMITTER_POINT && e.type != SMS_EMITTER_DIRECTIONAL)
+            return bad(SMS_E_INVALID, "unknown emitter type");
+        if (e.type == SMS_EMITTER_DIRECTIONAL &&
+            !(e.direction[0] * e.direction[0] + e.direction[1] * e.direction[1] + e.direction[2] * e.direction[2] > 0.f))
+            return bad(SMS_E_INVALID, "directional emitter without a direction");
         d.ss = e.caustic_ss; d.ms = e.caustic_ms;
         if (e.type == SMS_EMITTER_AREA && (e.shape < 0 || e.shape >= desc->n_shapes ||
                                            desc->shapes[e.shape].type != SMS_SHAPE_SPHERE))
@@ -446,6 +454,19 @@ sms_status prepare_scene(const sms_scene_desc *desc, HostScene &H, std::string &
     float half = 0.f;
     for (int a = 0; a < 3; ++a) { H.box_c[a] = 0.5f * (lo[a] + hi[a]); half = std::max(half, 0.5f * (hi[a] - lo[a])); }
     H.box_inv = half > 0.f ? 1.f / half : 1.f;
+    // BoundingBox::bounding_sphere of the scene (radius = |max - centre|), enlarged by
+    // DirectionalEmitter::set_scene (src/emitters/directional.cpp:63-68); sample_direction places
+    // the emitter 2 radii away (:112)
+    {
+        double r2 = 0.0;
+        for (int a = 0; a < 3; ++a) {
+            double c = 0.5 * ((double) lo[a] + (double) hi[a]), d = (double) hi[a] - c;
+            r2 += d * d;
+        }
+        double r = std::sqrt(r2) * (1.0 + (double) RAY_EPS_F);
+        if (!(r > (double) RAY_EPS_F)) r = (double) RAY_EPS_F;
+        H.bs_dist = (float) (2.0 * r);
+    }
     H.has_cam = desc->camera.width > 0 && desc->camera.height > 0;
     if (H.has_cam) camera_setup(H.cam, desc->camera);
     return SMS_OK;
@@ -468,6 +489,7 @@ sms_status upload_scene(sms_ctx *ctx, HostScene &H, const sms_config *cfg) {
     ctx->bvh_depth = H.bvh_depth;
     for (int a = 0; a < 3; ++a) S.box_c[a] = H.box_c[a];
     S.box_inv = H.box_inv;
+    S.bs_dist = H.bs_dist;
     ctx->h_vuv = H.vuv; ctx->h_faces = H.faces; ctx->h_shapes = H.shapes;
     if ((st = upload(ctx, H.shapes, &S.shapes))) return st;
     S.n_shapes = (int) H.shapes.size();

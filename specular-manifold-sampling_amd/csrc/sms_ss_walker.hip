@@ -228,12 +228,16 @@ __device__ __forceinline__ void ss_walker_body(const DScene &S, const sms_ss_que
                     has_ray = true; any_hit = false;
                 } else {
                     // shadow ray of the first solution (ss_visible)
-                    const V3 ep = ei_lane.p;
-                    V3 p = L.p_first;
-                    V3 d = p - ep;
-                    float dist = norm(d);
-                    d = d * (1.f / dist);
-                    ray = make_ray(ep, d, RAY_EPS_F * (1.f + hmax_abs(ep)), dist * (1.f - RAY_EPS_F));
+                    if (ei_lane.kind == SMS_EI_DIRECTIONAL) {
+                        ray = make_ray(L.p_first, ei_lane.d);
+                    } else {
+                        const V3 ep = ei_lane.p;
+                        V3 p = L.p_first;
+                        V3 d = p - ep;
+                        float dist = norm(d);
+                        d = d * (1.f / dist);
+                        ray = make_ray(ep, d, RAY_EPS_F * (1.f + hmax_abs(ep)), dist * (1.f - RAY_EPS_F));
+                    }
                     has_ray = true; any_hit = true;
                 }
             }

@@ -17,7 +17,7 @@ SHAPE_SPHERE, SHAPE_RECTANGLE, SHAPE_MESH = 0, 1, 2
 BSDF_DIFFUSE, BSDF_DIELECTRIC, BSDF_CONDUCTOR, BSDF_NORMALMAP = 0, 1, 2, 3
 FLAG_CAUSTIC_CASTER_SS, FLAG_CAUSTIC_CASTER_MS = 0x01, 0x02
 FLAG_CAUSTIC_RECEIVER, FLAG_CAUSTIC_BOUNCER, FLAG_GLINTY = 0x04, 0x08, 0x10
-EMITTER_AREA, EMITTER_POINT = 0, 1
+EMITTER_AREA, EMITTER_POINT, EMITTER_DIRECTIONAL = 0, 1, 2
 EI_AREA, EI_POINT, EI_DIRECTIONAL = 0, 1, 2
 
 
@@ -52,7 +52,8 @@ class SmsShapeDesc(C.Structure):
 class SmsEmitterDesc(C.Structure):
     _fields_ = [("type", C.c_int32), ("shape", C.c_int32), ("radiance", C.c_float * 3),
                 ("position", C.c_float * 3), ("intensity", C.c_float * 3),
-                ("caustic_ss", C.c_int32), ("caustic_ms", C.c_int32)]
+                ("caustic_ss", C.c_int32), ("caustic_ms", C.c_int32),
+                ("direction", C.c_float * 3), ("irradiance", C.c_float * 3)]
 
 
 class SmsNormalmapDesc(C.Structure):

@@ -68,6 +68,15 @@ class Scene:
                                   intensity=intensity, caustic_ss=caustic_ss, caustic_ms=caustic_ms))
         return len(self.emitters) - 1
 
+    def add_directional_light(self, direction, irradiance, caustic_ss=True, caustic_ms=True):
+        """Directional emitter: light travelling along `direction` with `irradiance`
+        (src/emitters/directional.cpp)."""
+        d = np.asarray(direction, np.float64)
+        d = d / np.linalg.norm(d)
+        self.emitters.append(dict(type=abi.EMITTER_DIRECTIONAL, shape=-1, direction=d, irradiance=irradiance,
+                                  caustic_ss=caustic_ss, caustic_ms=caustic_ms))
+        return len(self.emitters) - 1
+
     def add_normalmap(self, rgb):
         rgb = np.ascontiguousarray(rgb, np.uint8)
         assert rgb.ndim == 3 and rgb.shape[0] == rgb.shape[1] and rgb.shape[2] == 3
@@ -106,6 +115,8 @@ class Scene:
             _vec(d.radiance, e.get("radiance", (0, 0, 0)))
             _vec(d.position, e.get("position", (0, 0, 0)))
             _vec(d.intensity, e.get("intensity", (0, 0, 0)))
+            _vec(d.direction, e.get("direction", (0, 0, 0)))
+            _vec(d.irradiance, e.get("irradiance", (0, 0, 0)))
             d.caustic_ss, d.caustic_ms = int(e["caustic_ss"]), int(e["caustic_ms"])
         nms = (abi.SmsNormalmapDesc * max(1, len(self.normalmaps)))()
         for i, (res, rgb) in enumerate(self.normalmaps):

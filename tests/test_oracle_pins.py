@@ -28,7 +28,7 @@ from test_oracle_ss import _dome_roots, _refract
 
 ETA = 1.5
 PLANT = {"ss_g": 0, "ss_transmit": 1, "ss_ad_jac": 2, "ss_hv_jac": 3, "ms_ad_jac": 4, "ms_hv_jac": 5,
-         "mnee_target": 6, "ms_g": 7}
+         "mnee_target": 6, "ms_g": 7, "ss_g_dir": 8}
 
 
 def _lib():
