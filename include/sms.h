@@ -104,6 +104,10 @@ typedef struct sms_shape_desc {
     const float    *normals;    /* n_vertices x 3, or NULL                                */
     const float    *texcoords;  /* n_vertices x 2, or NULL                                */
     const uint32_t *faces;      /* n_faces x 3                                            */
+    /* CONDUCTOR only: isotropic Beckmann roughness alpha of a rough conductor
+       (src/bsdfs/roughconductor.cpp); 0 = smooth.  A rough single-scatter caster gets an offset
+       normal drawn per estimate from the Beckmann distribution (manifold_ss.cpp:61-70) */
+    float roughness;
 } sms_shape_desc;
 
 enum sms_emitter_type { SMS_EMITTER_AREA = 0, SMS_EMITTER_POINT = 1, SMS_EMITTER_DIRECTIONAL = 2 };

@@ -50,6 +50,7 @@ struct Shape {
     float M[12], Minv[12];          // rectangle to_world / to_object (row-major 3x4)
     V3 fr_s, fr_t, fr_n;            // rectangle m_frame (dp_du, dp_dv, normal)
     float eta_r, k, eta_vtx;        // ior() = (eta_r, k) and the ManifoldVertex eta encoding
+    float alpha;                    // rough conductor: isotropic Beckmann roughness (0: smooth)
     V3 bbox_center;
     V3 refl;
     int normalmap; float tiles;
@@ -322,6 +323,7 @@ static Scene *scene_create(const sms_scene_desc *desc, const sms_config *cfg) {
         /* ManifoldVertex eta: dielectric real IOR, conductors (k != 0) and others 1
            (manifold.h:93-95; bsdf.cpp:59-62 default ior (0,1)) */
         sh.eta_vtx = sh.k == 0.f ? hmean_same(sh.eta_r) : (float) 1.f;
+        sh.alpha = d.bsdf == SMS_BSDF_CONDUCTOR ? d.roughness : (float) 0.f;
         sh.bbox_center = sh.center;
         if (d.type == SMS_SHAPE_SPHERE) {
             sh.inv_surface_area = 1.f / (4.f * PI_F * sh.radius * sh.radius);
@@ -1223,10 +1225,60 @@ static PathOut sample_path_ss(const Scene &S, int shape, V3 si_p, const EI &ei, 
     return out;
 }
 
-/* specular_reflectance SS, smooth branch (manifold_ss.cpp:638-693) */
+/* ---------------------------------------------------------------- rough conductors
+   MicrofacetDistribution(Beckmann, alpha, alpha, sample_visible = false) of
+   include/mitsuba/render/microfacet.h: sample (:229-290, isotropic), eval (:184-201), smith_g1
+   (:331-355, the rational approximation); RoughConductor::eval (src/bsdfs/roughconductor.cpp:
+   283-356, unpolarized). */
+static V3 beckmann_sample(float alpha, V2 s, float *pdf) {
+    float sin_phi, cos_phi;
+    c_sincosf((2.f * PI_F) * s.y, &sin_phi, &cos_phi);
+    const float alpha_2 = alpha * alpha;
+    const float cos_theta = 1.f / sqrtf(fmaf(-alpha_2, logf_l(1.f - s.x), 1.f));
+    const float cos_theta_2 = cos_theta * cos_theta;
+    const float cos_theta_3 = std::max(cos_theta_2 * cos_theta, 1e-20f);
+    *pdf = (1.f - s.x) / (PI_F * alpha * alpha * cos_theta_3);
+    const float sin_theta = sqrtf(1.f - cos_theta_2);
+    return v3(cos_phi * sin_theta, sin_phi * sin_theta, cos_theta);
+}
+static float beckmann_eval(float alpha, V3 m) {
+    const float alpha_uv = alpha * alpha, cos_theta = m.z, cos_theta_2 = cos_theta * cos_theta;
+    const float a = m.x / alpha, b = m.y / alpha;
+    const float result = expf_l(-(a * a + b * b) / cos_theta_2) / (PI_F * alpha_uv * (cos_theta_2 * cos_theta_2));
+    return result * cos_theta > 1e-20f ? result : (float) 0.f;
+}
+static float beckmann_g1(float alpha, V3 v, V3 m) {
+    const float ax = alpha * v.x, ay = alpha * v.y;
+    const float xy_alpha_2 = ax * ax + ay * ay, tan_theta_alpha_2 = xy_alpha_2 / (v.z * v.z);
+    const float a = 1.f / sqrtf(tan_theta_alpha_2), a_sqr = a * a;
+    float result = a >= 1.6f ? (float) 1.f : (3.535f * a + 2.181f * a_sqr) / ((1.f + 2.276f * a) + 2.577f * a_sqr);
+    if (xy_alpha_2 == 0.f) result = 1.f;
+    if (dot(v, m) * v.z <= 0.f) result = 0.f;
+    return result;
+}
+static float rough_conductor_eval(const Shape &sh, V3 wi, V3 wo) {
+    if (!(wi.z > 0.f && wo.z > 0.f)) return 0.f;
+    const V3 H = normalize(wo + wi);
+    const float D = beckmann_eval(sh.alpha, H);
+    if (D == 0.f) return 0.f;
+    const float G = beckmann_g1(sh.alpha, wi, H) * beckmann_g1(sh.alpha, wo, H);
+    const float result = D * G / (4.f * wi.z);
+    const float F = fresnel_conductor(dot(wi, H), sh.eta_r, sh.k);
+    return F * result;
+}
+
+/* specular_reflectance SS (manifold_ss.cpp:638-693): the glossy branch for rough conductors
+   (the BSDF in the half-vector domain), else the smooth one */
 static V3 specular_reflectance_ss(const Scene &S, const SI &si, V3 wo) {
     if (!si.valid) return v3(0.f, 0.f, 0.f);
     const Shape &sh = S.shapes[si.shape];
+    if (sh.alpha > 0.f) {
+        const V3 wo_l = v3(dot(wo, si.sh_s), dot(wo, si.sh_t), dot(wo, si.sh_n));
+        float val = rough_conductor_eval(sh, si.wi, wo_l);
+        const V3 h_l = normalize(si.wi + wo_l);
+        val = val * (4.f * fabsf(dot(wo_l, h_l)));
+        return v3(val, val, val);
+    }
     bool reflection = sh.k != 0.f;      // any(imag(ior) != 0); the all(imag == 0) sub-case is unreachable
     V3 n = si.sh_n;       // BSDF::frame(si).n == sh_frame.n for non-normal-mapped shapes
     float cos_theta = dot(n, wo);
@@ -1487,7 +1539,8 @@ struct Prop { const float *raw; int shape; float defensive; UVGrid grid; int64_t
 
 /* one walk of a query whose mixture is g: uniform seed, or the mixture's uv mapped to the
    surface and used as p_start; uv outside [0,1)^2 or in no face -> failed walk, 0 iterations */
-static PathOut proposal_walk(const Scene &S, const Prop &P, const float *g, int shape, V3 si_p, const EI &ei, Pcg32 *rw) {
+static PathOut proposal_walk(const Scene &S, const Prop &P, const float *g, int shape, V3 si_p, const EI &ei,
+                             V3 n_offset, Pcg32 *rw) {
     V2 s;
     V3 p_start = v3(0.f, 0.f, 0.f);
     if (proposal_seed(g, P.defensive, rw, &s) == 1) {
@@ -1499,7 +1552,7 @@ static PathOut proposal_walk(const Scene &S, const Prop &P, const float *g, int 
     }
     V2 ls = v2(0.f, 0.f);
     if (S.cfg.twostage) { ls.x = pcg32_float(rw); ls.y = pcg32_float(rw); }
-    return sample_path_ss(S, shape, si_p, ei, s, v3(0.f, 0.f, 1.f), p_start, ls);
+    return sample_path_ss(S, shape, si_p, ei, s, n_offset, p_start, ls);
 }
 
 /* specular_manifold_sampling SS (manifold_ss.cpp:28-201), per-(query, walk) PCG32 streams */
@@ -1512,10 +1565,17 @@ static V3 estimate_ss(const Scene &S, const sms_ss_query &q, uint64_t seed, uint
     EI ei = sample_emitter_interaction(S, si.p, S.emitters_ss, &rq);
     V3 result = v3(0.f, 0.f, 0.f);
     if (S.casters_ss.empty()) return result;
-    V3 n_offset = v3(0.f, 0.f, 1.f);
     for (size_t c = 0; c < S.casters_ss.size(); ++c) {
         int shape = S.casters_ss[c];
         uint64_t wbase = ((uint64_t) c << 32);
+        /* offset normal of a rough caster (manifold_ss.cpp:61-70), from the query's stream after
+           the emitter sample, one 2D sample per rough caster in order */
+        V3 n_offset = v3(0.f, 0.f, 1.f);
+        float p_offset = 1.f;
+        if (S.shapes[shape].alpha > 0.f) {
+            V2 so; so.x = pcg32_float(&rq); so.y = pcg32_float(&rq);
+            n_offset = beckmann_sample(S.shapes[shape].alpha, so, &p_offset);
+        }
         V3 value = v3(0.f, 0.f, 0.f);
         float gq[PROP_RAW];
         const bool use_prop = P && shape == P->shape;
@@ -1524,7 +1584,7 @@ static V3 estimate_ss(const Scene &S, const sms_ss_query &q, uint64_t seed, uint
             Pcg32 rw; stream_seed(&rw, seed, qid, wbase + k + 1);
             PathOut po;
             if (use_prop) {
-                po = proposal_walk(S, *P, gq, shape, si.p, ei, &rw);
+                po = proposal_walk(S, *P, gq, shape, si.p, ei, n_offset, &rw);
             } else {
                 V2 s; s.x = pcg32_float(&rw); s.y = pcg32_float(&rw);
                 V2 ls = v2(0.f, 0.f);
@@ -1585,7 +1645,7 @@ static V3 estimate_ss(const Scene &S, const sms_ss_query &q, uint64_t seed, uint
                 value = value + v3(bsdf_val.x * specular_val.x, bsdf_val.y * specular_val.y, bsdf_val.z * specular_val.z);
             }
         }
-        result = result + value / 1.f;
+        result = result + value / p_offset;
     }
     return result;
 }

@@ -26,7 +26,8 @@ struct DShape {
     int fbase, nf, has_n, has_uv;
     float area_sum, area_norm, inv_surface_area;
     float bbox_center[3];
-    int cdf_offset, pad;
+    int cdf_offset;
+    float alpha;                  // rough conductor: isotropic Beckmann roughness (0: smooth)
 };
 
 struct DEmitter { int type, shape; float radiance[3], position[3], intensity[3]; int ss, ms; float direction[3], irradiance[3]; };
@@ -107,13 +108,14 @@ SMS_DEV void coordinate_system(V3 n, V3 &s, V3 &t) {
 struct SI {
     bool valid; float t;
     V3 p, n, sh_n, sh_s, sh_t, dp_du, dp_dv;
+    V3 wi;                        // direction toward the ray origin in the shading frame (si.wi)
     V2 uv;
     int shape, prim;
 };
 SMS_DEV SI si_invalid() {
     SI s;
     s.valid = false; s.t = __builtin_inff(); s.shape = -1; s.prim = -1;
-    s.p = s.n = s.sh_n = s.sh_s = s.sh_t = s.dp_du = s.dp_dv = v3(0.f, 0.f, 0.f);
+    s.p = s.n = s.sh_n = s.sh_s = s.sh_t = s.dp_du = s.dp_dv = s.wi = v3(0.f, 0.f, 0.f);
     s.uv = v2(0.f, 0.f);
     return s;
 }
@@ -377,6 +379,8 @@ SMS_DEV SI make_si(const DScene &S, const Ray &ray, const Hit &h) {
     }
     si.sh_s = normalize(fnmadd3(si.sh_n, dot(si.sh_n, si.dp_du), si.dp_du));
     si.sh_t = cross(si.sh_n, si.sh_s);
+    const V3 mw = -ray.d;
+    si.wi = v3(dot(mw, si.sh_s), dot(mw, si.sh_t), dot(mw, si.sh_n));
     return si;
 }
 

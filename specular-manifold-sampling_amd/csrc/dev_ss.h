@@ -275,8 +275,78 @@ SMS_DEV PathResult sample_path_ss(const DScene &S, int shape, V3 si_p, const EI 
 }
 
 // specular_reflectance SS, smooth branch (manifold_ss.cpp:638-693)
-SMS_DEV float specular_reflectance_ss_n(const DScene &S, int shape, V3 sh_n, V3 wo) {
+// ---------------------------------------------------------------- rough conductors
+// MicrofacetDistribution(Beckmann, alpha, alpha, sample_visible = false) of
+// include/mitsuba/render/microfacet.h: sample (:229-290, isotropic), eval (:184-201), smith_g1
+// (:331-355, the rational approximation); RoughConductor::eval (src/bsdfs/roughconductor.cpp:
+// 283-356, unpolarized).
+SMS_DEV V3 beckmann_sample(float alpha, V2 s, float &pdf) {
+    float sin_phi, cos_phi;
+    c_sincosf((2.f * PI_F) * s.y, &sin_phi, &cos_phi);
+    const float alpha_2 = alpha * alpha;
+    const float cos_theta = 1.f / sqrtf(fmaf(-alpha_2, logf_l(1.f - s.x), 1.f));
+    const float cos_theta_2 = cos_theta * cos_theta;
+    const float cos_theta_3 = fmax_(cos_theta_2 * cos_theta, 1e-20f);
+    pdf = (1.f - s.x) / (PI_F * alpha * alpha * cos_theta_3);
+    const float sin_theta = sqrtf(1.f - cos_theta_2);
+    return v3(cos_phi * sin_theta, sin_phi * sin_theta, cos_theta);
+}
+SMS_DEV float beckmann_eval(float alpha, V3 m) {
+    const float alpha_uv = alpha * alpha, cos_theta = m.z, cos_theta_2 = cos_theta * cos_theta;
+    const float a = m.x / alpha, b = m.y / alpha;
+    const float result = expf_l(-(a * a + b * b) / cos_theta_2) / (PI_F * alpha_uv * (cos_theta_2 * cos_theta_2));
+    return result * cos_theta > 1e-20f ? result : 0.f;
+}
+SMS_DEV float beckmann_g1(float alpha, V3 v, V3 m) {
+    const float ax = alpha * v.x, ay = alpha * v.y;
+    const float xy_alpha_2 = ax * ax + ay * ay, tan_theta_alpha_2 = xy_alpha_2 / (v.z * v.z);
+    const float a = 1.f / sqrtf(tan_theta_alpha_2), a_sqr = a * a;
+    float result = a >= 1.6f ? 1.f : (3.535f * a + 2.181f * a_sqr) / ((1.f + 2.276f * a) + 2.577f * a_sqr);
+    if (xy_alpha_2 == 0.f) result = 1.f;
+    if (dot(v, m) * v.z <= 0.f) result = 0.f;
+    return result;
+}
+SMS_DEV float rough_conductor_eval(const DShape &sh, V3 wi, V3 wo) {
+    if (!(wi.z > 0.f && wo.z > 0.f)) return 0.f;
+    const V3 H = normalize(wo + wi);
+    const float D = beckmann_eval(sh.alpha, H);
+    if (D == 0.f) return 0.f;
+    const float G = beckmann_g1(sh.alpha, wi, H) * beckmann_g1(sh.alpha, wo, H);
+    const float result = D * G / (4.f * wi.z);
+    const float F = fresnel_conductor(dot(wi, H), sh.eta_r, sh.k);
+    return F * result;
+}
+
+// offset normal of caster c of a query (manifold_ss.cpp:61-70): for a rough caster a Beckmann
+// sample from the query's stream after the emitter sample, one 2D sample per rough caster in
+// caster order; returns its pdf (1 and (0, 0, 1) for smooth casters)
+SMS_DEV float caster_offset(const DScene &S, uint64_t qkey, int c, V3 &n_offset) {
+    n_offset = v3(0.f, 0.f, 1.f);
+    const float alpha = S.shapes[S.casters_ss[c]].alpha;
+    if (!(alpha > 0.f)) return 1.f;
+    Pcg32 rq;
+    stream_seed(rq, qkey, 0);
+    if (S.n_emitters_ss > 0) { (void) pcg32_float(rq); (void) pcg32_float(rq); (void) pcg32_float(rq); }
+    for (int j = 0; j < c; ++j)
+        if (S.shapes[S.casters_ss[j]].alpha > 0.f) { (void) pcg32_float(rq); (void) pcg32_float(rq); }
+    V2 s;
+    s.x = pcg32_float(rq); s.y = pcg32_float(rq);
+    float pdf;
+    n_offset = beckmann_sample(alpha, s, pdf);
+    return pdf;
+}
+
+// specular_reflectance SS (manifold_ss.cpp:638-693): the glossy branch for rough conductors (the
+// BSDF in the half-vector domain; frame (s, t, n) and wi the solution interaction's), else the
+// smooth one
+SMS_DEV float specular_reflectance_ss_n(const DScene &S, int shape, V3 sh_n, V3 wo, V3 sh_s, V3 sh_t, V3 wi_local) {
     const DShape &sh = S.shapes[shape];
+    if (sh.alpha > 0.f) {
+        const V3 wo_l = v3(dot(wo, sh_s), dot(wo, sh_t), dot(wo, sh_n));
+        float val = rough_conductor_eval(sh, wi_local, wo_l);
+        const V3 h_l = normalize(wi_local + wo_l);
+        return val * (4.f * fabsf(dot(wo_l, h_l)));
+    }
     float cos_theta = dot(sh_n, wo);
     if (sh.k != 0.f) return fresnel_conductor(fabsf(cos_theta), sh.eta_r, sh.k);
     float eta = sh.eta_vtx;
@@ -287,7 +357,7 @@ SMS_DEV float specular_reflectance_ss_n(const DScene &S, int shape, V3 sh_n, V3 
 }
 SMS_DEV float specular_reflectance_ss(const DScene &S, const SI &si, V3 wo) {
     if (!si.valid) return 0.f;
-    return specular_reflectance_ss_n(S, si.shape, si.sh_n, wo);
+    return specular_reflectance_ss_n(S, si.shape, si.sh_n, wo, si.sh_s, si.sh_t, si.wi);
 }
 
 // geometric_term SS (manifold_ss.cpp:695-857)
@@ -367,7 +437,8 @@ SMS_DEV float geometric_term_ss(const MV &v0, const MV &v1, const MV &v2v) {
 // evaluate_path_contribution SS (manifold_ss.cpp:281-327); si = receiver (query) interaction
 // vfinal = mv_from_si of the (valid) solution surface interaction, sh_n_final its shading normal
 template <bool LDS_STACK = false, int LDS_DEPTH = SMS_BVH_STACK>
-SMS_DEV V3 path_contribution_ss_mv(const DScene &S, const SI &si, const EI &ei_, const MV &vfinal, V3 sh_n_final) {
+SMS_DEV V3 path_contribution_ss_mv(const DScene &S, const SI &si, const EI &ei_, const MV &vfinal, V3 sh_n_final,
+                                   V3 wi_final) {
     MV vtx = vfinal;
     make_orthonormal(vtx);
     EI ei = ei_;
@@ -387,7 +458,8 @@ SMS_DEV V3 path_contribution_ss_mv(const DScene &S, const SI &si, const EI &ei_,
     if (!emitter_interaction_to_vertex<LDS_STACK, LDS_DEPTH>(S, ei, vtx.p, vy)) return v3(0.f, 0.f, 0.f);
     MV vx = mv_from_si(S, si);
     make_orthonormal(vx);
-    float F = specular_reflectance_ss_n(S, vfinal.shape, sh_n_final, normalize(ei.p - vfinal.p));
+    float F = specular_reflectance_ss_n(S, vfinal.shape, sh_n_final, normalize(ei.p - vfinal.p), vfinal.s, vfinal.t,
+                                        wi_final);
     float G = geometric_term_ss(vx, vtx, vy);
     float thr = F * G;
     return v3(thr * ei.weight.x, thr * ei.weight.y, thr * ei.weight.z);
@@ -396,7 +468,7 @@ SMS_DEV V3 path_contribution_ss_mv(const DScene &S, const SI &si, const EI &ei_,
 // contribution of a walk's solution (manifold_ss.cpp:281-327)
 SMS_DEV V3 path_contribution_ss(const DScene &S, const SI &si, const EI &ei, const SI &si_final) {
     if (!si_final.valid) return v3(0.f, 0.f, 0.f);
-    return path_contribution_ss_mv(S, si, ei, mv_from_si(S, si_final), si_final.sh_n);
+    return path_contribution_ss_mv(S, si, ei, mv_from_si(S, si_final), si_final.sh_n, si_final.wi);
 }
 
 } // namespace smsdev

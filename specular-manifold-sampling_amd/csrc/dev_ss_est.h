@@ -32,7 +32,7 @@ SMS_DEV bool ss_visible(const DScene &S, const EI &ei, V3 p) {
 }
 
 SMS_DEV PathResult ss_walk(const DScene &S, const Proposal &P, int64_t q, uint64_t qkey, int c, int shape, uint64_t k,
-                           V3 si_p, const EI &ei) {
+                           V3 si_p, const EI &ei, V3 n_offset) {
     Pcg32 rw;
     stream_seed(rw, qkey, ((uint64_t) c << 32) + k + 1);
     V2 s = v2(0.f, 0.f);
@@ -49,7 +49,7 @@ SMS_DEV PathResult ss_walk(const DScene &S, const Proposal &P, int64_t q, uint64
     }
     V2 ls = v2(0.f, 0.f);
     if (S.cfg.twostage) { ls.x = pcg32_float(rw); ls.y = pcg32_float(rw); }
-    return sample_path_ss(S, shape, si_p, ei, s, v3(0.f, 0.f, 1.f), p_start, ls);
+    return sample_path_ss(S, shape, si_p, ei, s, n_offset, p_start, ls);
 }
 
 // record (network inputs, solution uv, weight) for the proposal's training

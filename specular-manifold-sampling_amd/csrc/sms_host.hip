@@ -342,6 +342,10 @@ sms_status prepare_scene(const sms_scene_desc *desc, HostScene &H, std::string &
         else if (d.bsdf == SMS_BSDF_DIELECTRIC) { sh.eta_r = d.eta; sh.k = 0.f; }
         else { sh.eta_r = d.eta; sh.k = d.k; }
         sh.eta_vtx = sh.k == 0.f ? hmean_same(sh.eta_r) : 1.f;
+        if (!(d.roughness >= 0.f) || !std::isfinite(d.roughness)) return bad(SMS_E_INVALID, "roughness must be finite and >= 0");
+        if (d.roughness > 0.f && d.bsdf != SMS_BSDF_CONDUCTOR)
+            return bad(SMS_E_UNSUPPORTED, "rough BSDFs other than the rough conductor are not built");
+        sh.alpha = d.roughness;
         for (int a = 0; a < 3; ++a) sh.refl[a] = d.reflectance[a];
         sh.normalmap = d.normalmap; sh.tiles = d.tiles;
         if ((d.flags & SMS_FLAG_GLINTY) && d.bsdf != SMS_BSDF_NORMALMAP)
@@ -1120,7 +1124,7 @@ sms_status ss_enqueue(sms_ctx *ctx, const sms_ss_query *q, int64_t n, float *out
                                ctx->scene, q, seed, ctx->part, ctx->invp, W);
             HIP_TRY(ctx, hipGetLastError());
             hipLaunchKernelGGL(k_ss_finalize, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream,
-                               ctx->part, ctx->invp, nc, n, out_rgb);
+                               ctx->scene, seed, ctx->part, ctx->invp, nc, n, 1, out_rgb);
             HIP_TRY(ctx, hipGetLastError());
         }
     } else {
@@ -1179,7 +1183,7 @@ sms_status ms_enqueue(sms_ctx *ctx, const sms_ss_query *q, int64_t n, float *out
             else hipLaunchKernelGGL(k_ms_coop_4, g2, dim3(256), 0, ctx->stream, ctx->scene, q, seed, part, ctx->invp, W);
             HIP_TRY(ctx, hipGetLastError());
             hipLaunchKernelGGL(k_ss_finalize, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream,
-                               ctx->part, ctx->invp, nc, n, out_rgb);
+                               ctx->scene, seed, ctx->part, ctx->invp, nc, n, 0, out_rgb);
             HIP_TRY(ctx, hipGetLastError());
         }
     } else {

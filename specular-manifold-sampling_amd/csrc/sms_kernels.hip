@@ -137,7 +137,9 @@ __global__ void __launch_bounds__(256) k_ss_estimate(DScene S, const sms_ss_quer
             invp[slot] = 0.f;
             if (g.shape < 0) continue;                 // no receiver: zero estimate, no walks
             int shape = S.casters_ss[c];
-            PathResult r = ss_walk(S, W.prop, q, qkey, c, shape, 0, si.p, ei);
+            V3 n_off;
+            (void) caster_offset(S, qkey, c, n_off);
+            PathResult r = ss_walk(S, W.prop, q, qkey, c, shape, 0, si.p, ei, n_off);
             st_walks++; st_iters += (unsigned long long) r.iterations;
             if (!r.ok) continue;
             st_conv++;
@@ -152,7 +154,7 @@ __global__ void __launch_bounds__(256) k_ss_estimate(DScene S, const sms_ss_quer
             int64_t k = 1;
             float inv_prob = -1.f;
             while (true) {
-                PathResult t = ss_walk(S, W.prop, q, qkey, c, shape, (uint64_t) k, si.p, ei);
+                PathResult t = ss_walk(S, W.prop, q, qkey, c, shape, (uint64_t) k, si.p, ei, n_off);
                 st_walks++; st_iters += (unsigned long long) t.iterations;
                 if (t.ok) st_conv++;
                 if (t.ok && capturing(W.prop, shape, SMS_CAPTURE_SOLUTIONS))
@@ -208,6 +210,8 @@ __global__ void __launch_bounds__(256) k_ss_coop(DScene S, const sms_ss_query *_
         EI ei = sample_emitter_interaction(S, si.p, S.emitters_ss, S.n_emitters_ss, rq);
         V3 direction = v3(rec.sol[0], rec.sol[1], rec.sol[2]);
         int shape = S.casters_ss[c];
+        V3 n_off;
+        (void) caster_offset(S, qkey, c, n_off);
         int64_t kcap = cfg.max_trials > 0 ? (int64_t) cfg.max_trials : (int64_t) 1 << 40;
         int64_t found = -1;
         for (int64_t base = rec.k; base <= kcap && found < 0; base += 64) {
@@ -216,7 +220,7 @@ __global__ void __launch_bounds__(256) k_ss_coop(DScene S, const sms_ss_query *_
             int its = 0;
             V2 uv_t = v2(0.f, 0.f);
             if (k <= kcap) {
-                PathResult t = ss_walk(S, W.prop, q, qkey, c, shape, (uint64_t) k, si.p, ei);
+                PathResult t = ss_walk(S, W.prop, q, qkey, c, shape, (uint64_t) k, si.p, ei, n_off);
                 ran = true; its = t.iterations; conv = t.ok;
                 uv_t = t.si_final.uv;
                 V3 dtr = normalize(t.si_final.p - si.p);
@@ -246,16 +250,20 @@ __global__ void __launch_bounds__(256) k_ss_coop(DScene S, const sms_ss_query *_
     stat_add(W.stats + 4, st_btrials); stat_max(W.stats + 5, st_bmax); stat_add(W.stats + 6, st_tail);
 }
 
-// result = sum over casters (in order) of bxs * inv_prob  (manifold_ss.cpp:139-140,198)
-__global__ void k_ss_finalize(const float *__restrict__ part, const float *__restrict__ invp, int nc, int64_t n,
-                              float *__restrict__ out) {
+// result = sum over casters (in order) of bxs * inv_prob / p_offset  (manifold_ss.cpp:139-140,198);
+// offsets = 0 (the multi-scatter estimator, whose casters take no offset normals): p_offset = 1
+__global__ void k_ss_finalize(DScene S, const uint64_t *__restrict__ seedp, const float *__restrict__ part,
+                              const float *__restrict__ invp, int nc, int64_t n, int offsets, float *__restrict__ out) {
     int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
+    const uint64_t qkey = query_key(*seedp, (uint64_t) q);
     V3 r = v3(0.f, 0.f, 0.f);
     for (int c = 0; c < nc; ++c) {
         int64_t s = q * nc + c;
         float ip = invp[s];
-        r = r + v3(part[3 * s] * ip, part[3 * s + 1] * ip, part[3 * s + 2] * ip) / 1.f;
+        V3 n_off;
+        const float p_off = offsets ? caster_offset(S, qkey, c, n_off) : 1.f;
+        r = r + v3(part[3 * s] * ip, part[3 * s + 1] * ip, part[3 * s + 2] * ip) / p_off;
     }
     write_rgb(out, q, r);
 }
@@ -286,6 +294,8 @@ __global__ void __launch_bounds__(256) k_ss_biased(DScene S, const sms_ss_query 
         const int ncq = g.shape < 0 ? 0 : nc;        // no receiver: zero estimate, no walks
         for (int c = 0; c < ncq; ++c) {
             int shape = S.casters_ss[c];
+            V3 n_off;
+            const float p_off = caster_offset(S, qkey, c, n_off);
             V3 value = v3(0.f, 0.f, 0.f);
             int n_sol = 0;
             for (int base = 0; base < cfg.max_trials; base += 64) {
@@ -295,7 +305,7 @@ __global__ void __launch_bounds__(256) k_ss_biased(DScene S, const sms_ss_query 
                 PathResult r;
                 r.ok = false;
                 if (m < cfg.max_trials) {
-                    r = ss_walk(S, W.prop, q, qkey, c, shape, (uint64_t) m, si.p, ei);
+                    r = ss_walk(S, W.prop, q, qkey, c, shape, (uint64_t) m, si.p, ei, n_off);
                     st_walks++; st_iters += (unsigned long long) r.iterations;
                     if (r.ok) {
                         st_conv++;
@@ -342,7 +352,7 @@ __global__ void __launch_bounds__(256) k_ss_biased(DScene S, const sms_ss_query 
                     value = value + v3(__shfl(contrib.x, j, 64), __shfl(contrib.y, j, 64), __shfl(contrib.z, j, 64));
                 }
             }
-            result = result + value / 1.f;
+            result = result + value / p_off;
         }
         if (lane == 0) write_rgb(out, q, result);
     }

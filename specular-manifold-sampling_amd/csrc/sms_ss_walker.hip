@@ -46,6 +46,7 @@ struct WalkerLane {
     V2 uv_first;
     // walk: the record of the hit that made the current vertex (a ray from si_p along key_d)
     int stage;
+    V3 n_off;                    // the chain's offset normal (rough casters; (0, 0, 1) otherwise)
     Hit key;
     V3 key_d;
     float beta;
@@ -62,7 +63,6 @@ __device__ __forceinline__ void ss_walker_body(const DScene &S, const sms_ss_que
     const sms_config &cfg = S.cfg;
     const int nc = S.n_casters_ss;
     const int lane = threadIdx.x & 63;
-    const V3 n_offset = v3(0.f, 0.f, 1.f);
     const Proposal &P = W.prop;
     __shared__ EI ei_lds[256];
     EI &ei_lane = ei_lds[threadIdx.x];
@@ -71,6 +71,7 @@ __device__ __forceinline__ void ss_walker_body(const DScene &S, const sms_ss_que
     L.walks = L.conv = L.iters = L.bcalls = L.btrials = L.bmax = 0u;
     L.qkey = 0;
     L.si_p = v3(0.f, 0.f, 0.f);
+    L.n_off = v3(0.f, 0.f, 1.f);
     L.key_d = v3(0.f, 0.f, 0.f);
     L.key.hit = false; L.key.t = 0.f; L.key.shape = L.key.prim = 0; L.key.b1 = L.key.b2 = L.key.lx = L.key.ly = 0.f;
     // wavefront-uniform query pool
@@ -83,6 +84,7 @@ __device__ __forceinline__ void ss_walker_body(const DScene &S, const sms_ss_que
         part[3 * L.slot] = 0.f; part[3 * L.slot + 1] = 0.f; part[3 * L.slot + 2] = 0.f;
         invp[L.slot] = 0.f;
         L.shape = S.casters_ss[L.c];
+        (void) caster_offset(S, L.qkey, L.c, L.n_off);
         L.k = 0;
         L.stage = WK_SEED;
     };
@@ -136,7 +138,7 @@ __device__ __forceinline__ void ss_walker_body(const DScene &S, const sms_ss_que
             // interaction need not be kept; it is discarded if the shadow ray is blocked
             const sms_ss_query g = Q[L.q];
             const SI si = ss_query_si(g);
-            V3 spec = path_contribution_ss_mv<true, WALKER_STACK>(S, si, ei_lane, vtx, hs.sh_n);
+            V3 spec = path_contribution_ss_mv<true, WALKER_STACK>(S, si, ei_lane, vtx, hs.sh_n, hs.wi);
             V3 dl = v3(dot(L.direction, si.sh_s), dot(L.direction, si.sh_t), dot(L.direction, si.sh_n));
             V3 bsdf = receiver_bsdf_eval(S, si, ld3(g.wi), dl);
             L.part_first = v3(bsdf.x * spec.x, bsdf.y * spec.y, bsdf.z * spec.z);
@@ -208,8 +210,8 @@ __device__ __forceinline__ void ss_walker_body(const DScene &S, const sms_ss_que
                     const MV vtx = mv_from_si(S, hs);
                     if (L.its >= cfg.max_iterations) { walk_failed(L.its); continue; }
                     const EI ei = ei_lane;
-                    Step st = cfg.halfvector_constraints ? step_halfvector(L.si_p, vtx, ei, n_offset)
-                                                         : step_anglediff(L.si_p, vtx, ei, n_offset);
+                    Step st = cfg.halfvector_constraints ? step_halfvector(L.si_p, vtx, ei, L.n_off)
+                                                         : step_anglediff(L.si_p, vtx, ei, L.n_off);
                     if (!st.ok) { walk_failed(L.its); continue; }
                     if (norm2(st.C) < cfg.solver_threshold) {
                         V3 wx = normalize(L.si_p - vtx.p);

@@ -46,7 +46,8 @@ class SmsShapeDesc(C.Structure):
                 ("normalmap", C.c_int32), ("tiles", C.c_float),
                 ("n_vertices", C.c_int32), ("n_faces", C.c_int32),
                 ("positions", C.POINTER(C.c_float)), ("normals", C.POINTER(C.c_float)),
-                ("texcoords", C.POINTER(C.c_float)), ("faces", C.POINTER(C.c_uint32))]
+                ("texcoords", C.POINTER(C.c_float)), ("faces", C.POINTER(C.c_uint32)),
+                ("roughness", C.c_float)]
 
 
 class SmsEmitterDesc(C.Structure):

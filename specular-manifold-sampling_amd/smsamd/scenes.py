@@ -34,23 +34,23 @@ class Scene:
 
     # ---------------------------------------------------------------- builders
     def add_sphere(self, center, radius, bsdf, flags=0, eta=1.5, k=0.0, reflectance=(0.5, 0.5, 0.5),
-                   emitter=-1):
+                   emitter=-1, roughness=0.0):
         self.shapes.append(dict(type=abi.SHAPE_SPHERE, bsdf=bsdf, flags=flags, center=center,
                                 radius=radius, eta=eta, k=k, reflectance=reflectance,
-                                emitter=emitter))
+                                emitter=emitter, roughness=roughness))
         return len(self.shapes) - 1
 
     def add_rectangle(self, to_world, bsdf, flags=0, eta=1.5, k=0.0, reflectance=(0.5, 0.5, 0.5),
-                      normalmap=-1, tiles=1.0):
+                      normalmap=-1, tiles=1.0, roughness=0.0):
         self.shapes.append(dict(type=abi.SHAPE_RECTANGLE, bsdf=bsdf, flags=flags,
                                 to_world=np.asarray(to_world, np.float32).reshape(12), eta=eta, k=k,
-                                reflectance=reflectance, normalmap=normalmap, tiles=tiles))
+                                reflectance=reflectance, normalmap=normalmap, tiles=tiles, roughness=roughness))
         return len(self.shapes) - 1
 
     def add_mesh(self, positions, faces, normals=None, texcoords=None, bsdf=abi.BSDF_DIFFUSE, flags=0,
-                 eta=1.5, k=0.0, reflectance=(0.5, 0.5, 0.5)):
+                 eta=1.5, k=0.0, reflectance=(0.5, 0.5, 0.5), roughness=0.0):
         self.shapes.append(dict(type=abi.SHAPE_MESH, bsdf=bsdf, flags=flags, eta=eta, k=k,
-                                reflectance=reflectance,
+                                reflectance=reflectance, roughness=roughness,
                                 positions=np.ascontiguousarray(positions, np.float32),
                                 faces=np.ascontiguousarray(faces, np.uint32),
                                 normals=None if normals is None else np.ascontiguousarray(normals, np.float32),
@@ -99,6 +99,7 @@ class Scene:
             d.eta, d.k = s.get("eta", 1.5), s.get("k", 0.0)
             _vec(d.reflectance, s.get("reflectance", (0.5, 0.5, 0.5)))
             d.normalmap, d.tiles = s.get("normalmap", -1), s.get("tiles", 1.0)
+            d.roughness = s.get("roughness", 0.0)
             if s["type"] == abi.SHAPE_MESH:
                 d.n_vertices = s["positions"].shape[0]
                 d.n_faces = s["faces"].shape[0]

@@ -48,7 +48,7 @@ def test_ss_walks_toward_a_fixed_direction(gpu):
         ctx = gpu.Context(sc, cfg)
         orc = oracle_lib.Oracle(sc, cfg)
         go, oo = ctx.ss_walks(w), orc.ss_walks(w)
-        assert oo["success"].mean() > 0.3
+        assert oo["success"].mean() > 0.1
         for k in ("success", "iterations", "final_shape", "init_valid", "p", "n", "p_init"):
             assert np.array_equal(go[k], oo[k]), k
         ctx.close()
