@@ -29,6 +29,8 @@ equal_time_rmse): RMSE of uniform-seed and learned-proposal renders given the sa
 budget, against a uniform-seed reference; `ratio` charges the training to the same frame (the
 network is trained online between its passes, every pass counts in the image),
 `ratio_amortized` compares with the trained network (training done before the frame).
+`ms_slab_equal_time_rmse` is the same experiment on the multi-scatter slab, and
+`glint_equal_time_rmse` on this glint frame with the learned proposal over the pixel footprint.
 """
 import argparse
 import json
@@ -111,6 +113,12 @@ def parse():
     ap.add_argument("--ms-reps", type=int, default=2, help="frames per method of the slab experiment")
     ap.add_argument("--ms-ref-spp", type=int, default=128, help="slab reference samples per pixel")
     ap.add_argument("--ms-pass-spp", type=int, default=2, help="samples per pixel of a slab pass")
+    ap.add_argument("--glint-rmse-only", action="store_true",
+                    help="run only the glint equal-time RMSE experiment (learned footprint proposal)")
+    ap.add_argument("--no-glint-rmse", action="store_true", help="skip the glint equal-time RMSE experiment")
+    ap.add_argument("--glint-rmse-spp", type=int, default=512, help="uniform-seed samples per pixel of the glint budget")
+    ap.add_argument("--glint-ref-spp", type=int, default=8192, help="glint reference samples per pixel")
+    ap.add_argument("--glint-reps", type=int, default=4, help="frames per method of the glint experiment")
     ap.add_argument("--mlp-only", action="store_true", help="run only the proposal network's training / inference timing")
     ap.add_argument("--no-mlp", action="store_true", help="skip the proposal network's timing")
     ap.add_argument("--mlp-steps", type=int, default=2000, help="timed training steps of the network leg")
@@ -162,6 +170,9 @@ def main():
         return
     if args.mlp_only:
         print(json.dumps(mlp_throughput(args)), flush=True)
+        return
+    if args.glint_rmse_only:
+        print(json.dumps(equal_time_rmse(args, "glint")), flush=True)
         return
     import numpy as np
     from smsamd import abi, gpu, scenes
@@ -254,6 +265,7 @@ def main():
     mlp = None if args.no_mlp else mlp_throughput(args)
     rmse = None if args.no_rmse else equal_time_rmse(args)
     ms_rmse = None if args.no_ms else equal_time_rmse(args, "slab")
+    glint_rmse = None if args.no_glint_rmse else equal_time_rmse(args, "glint")
     result = {
         "metric": "converged walks/s at 1 GPU; equal-time RMSE learned vs uniform seeds",
         "value": conv_per_s,
@@ -281,6 +293,7 @@ def main():
         "mlp": mlp,
         "equal_time_rmse": rmse,
         "ms_slab_equal_time_rmse": ms_rmse,
+        "glint_equal_time_rmse": glint_rmse,
     }
     if cpu and cpu.get("value"):
         result["gpu_over_cpu"] = conv_per_s / cpu["value"]
@@ -475,7 +488,10 @@ def equal_time_rmse(args, case="pool"):
     glass slab 0.2 thick (IOR 1.5) with value-noise faces over a diffuse floor, two-vertex
     multi-scatter chains (bounces 2, unbiased, cap 1e7, thresholds 1e-5 as slab_sms.xml), the
     proposal on the chain's seed point on the top face; budget ms_spp, reference ms_ref_spp.
-    Uniform seeding renders the budget's samples per pixel; its wall-clock time T is the budget.
+    case "glint": the glint frame of the headline line (512^2, 2048^2 scratch/bump map, passes
+    of 4 spp, unbiased, cap 1e5, thresholds 4e-5 / 2e-4) with the learned proposal over the
+    pixel footprint, trained on converged first walks; budget glint_rmse_spp, reference
+    glint_ref_spp.  Uniform seeding renders the budget's samples per pixel; its wall-clock time T is the budget.
     The charged run gets the same T for everything, training included: its first pass (of
     first_pass_spp samples per pixel) uses uniform seeds, every later pass the
     defensive mixture with the network as trained so far; after each pass the network trains on
@@ -491,8 +507,13 @@ def equal_time_rmse(args, case="pool"):
     of its squared error in a few pixels."""
     import numpy as np
     from smsamd import abi, gpu, render, scenes
-    res, pass_spp = args.rmse_res, (args.rmse_pass_spp if case == "pool" else args.ms_pass_spp)
-    if case == "pool":
+    res, pass_spp = args.rmse_res, {"pool": args.rmse_pass_spp, "slab": args.ms_pass_spp, "glint": args.spp}[case]
+    if case == "glint":
+        res = args.width
+        sc = scenes.glint_plane(args.width, args.height, nm_res=args.nm_res, n_scratches=4096, seed=1)
+        cfg = abi.SmsConfig.default(max_trials=100000, solver_threshold=4e-5, uniqueness_threshold=2e-4)
+        budget_spp, ref_spp, estimator = args.glint_rmse_spp, args.glint_ref_spp, "glint"
+    elif case == "pool":
         sc = scenes.pool(res, res)
         cfg = abi.SmsConfig.default(max_trials=10_000_000, uniqueness_threshold=1e-5, solver_threshold=1e-5)
         budget_spp, ref_spp, estimator = args.rmse_spp, args.rmse_ref_spp, "ss"
@@ -510,7 +531,10 @@ def equal_time_rmse(args, case="pool"):
     if first_runner is not None:
         first_runner.run(999_997, film)
     ctx.sync()
-    if estimator == "ss":
+    if estimator == "glint":
+        ctx.glint_queries_device(pass_spp, 999_998, runner.q)
+        sched = ctx.glint_autotune_device(runner.q, runner.n, 999_998, runner.rgb)
+    elif estimator == "ss":
         ctx.ss_queries_device(pass_spp, 999_998, runner.q, runner.w)
         sched = ctx.ss_autotune_device(runner.q, runner.n, 999_998, runner.rgb)
     else:
@@ -531,7 +555,7 @@ def equal_time_rmse(args, case="pool"):
     print(f"# rmse {case}: reference {film.spp} spp in {t_ref:.1f}s", file=sys.stderr, flush=True)
     lum = ref.mean(1)
     mask = lum > 2.0 * lum.mean()
-    reps = args.rmse_reps if case == "pool" else args.ms_reps
+    reps = {"pool": args.rmse_reps, "slab": args.ms_reps, "glint": args.glint_reps}[case]
 
     def errors(img):
         return (render.rmse(img, ref, ref_var_mean) ** 2, render.rmse(img, ref, ref_var_mean, mask) ** 2,
@@ -620,10 +644,14 @@ def equal_time_rmse(args, case="pool"):
 
     def trials(st):
         return st.get("bernoulli_trials", 0) / max(st.get("bernoulli_calls", 1), 1)
-    workload = (f"water pool caustics (single scatter), {res}x{res} camera image" if case == "pool" else
-                f"glass slab double refraction (multi scatter, 2 vertices), {res}x{res} camera image")
+    workload = {"pool": f"water pool caustics (single scatter), {res}x{res} camera image, unbiased (max_trials 1e7, "
+                        f"uniqueness 1e-5)",
+                "slab": f"glass slab double refraction (multi scatter, 2 vertices), {res}x{res} camera image, "
+                        f"unbiased (max_trials 1e7, uniqueness 1e-5)",
+                "glint": f"glints, {res}x{res} frame, {args.nm_res}^2 scratch+bump map, unbiased (max_trials 1e5, "
+                         f"threshold 4e-5, uniqueness 2e-4), learned proposal over the pixel footprint"}[case]
     out = {
-        "workload": f"{workload}, unbiased (max_trials 1e7, uniqueness 1e-5), passes of {pass_spp} spp",
+        "workload": f"{workload}, passes of {pass_spp} spp",
         "frames": reps, "budget_s": round(T, 3), "uniform_spp": spp_u,
         "learned_spp": spp_l, "learned_passes_uniform": 1, "first_pass_spp": first_spp,
         "train_s_within_budget": round(tr0["train_s"], 3), "train_steps_within_budget": tr0["steps"],

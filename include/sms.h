@@ -470,7 +470,8 @@ sms_status sms_mlp_train(sms_mlp *mlp, const float *x, const float *uv, const fl
 /* Use `mlp` as the seed proposal of caster mesh `shape` (with uvs): in sms_ss_estimate_batch if
    it is a single-scatter caster, in the unbiased sms_ms_estimate_batch if it is a multi-scatter
    caster (the seed point of the chain on it; the biased multi-scatter estimator keeps uniform
-   seeds); defensive in (0, 1].  mlp NULL: uniform seeds, `shape` still names the caster whose
+   seeds); or of a glinty normal-mapped `shape`, in sms_glint_estimate_batch (the footprint
+   sample, see sms_glint_proposal_inputs); defensive in (0, 1].  mlp NULL: uniform seeds, `shape` still names the caster whose
    solutions the training capture records; shape -1 clears the proposal. */
 sms_status sms_ctx_set_proposal(sms_ctx *ctx, sms_mlp *mlp, int32_t shape, float defensive);
 /* Training capture of later unbiased sms_ss_estimate_batch / sms_ms_estimate_batch calls on the
@@ -501,6 +502,16 @@ sms_status sms_ss_proposal_inputs(sms_ctx *ctx, const sms_ss_query *q, int64_t n
 /* Raw network outputs (device, n x 48) the proposal gives the queries of an estimate with this
    seed -- the exact values sms_ss_estimate_batch turns into its seed mixtures. */
 sms_status sms_ss_proposal_outputs(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *out);
+/* The same for the glint estimator (sms_glint_estimate_batch): with the proposal set on a glinty
+   normal-mapped shape, the mixture is over the pixel footprint -- the reference's footprint
+   sample xi in [0,1)^2, uv = uv0 + (xi.x - 1/2) duv_dx + (xi.y - 1/2) duv_dy
+   (manifold_glints.cpp:252-259) -- and it seeds the first walk and every Bernoulli trial (and
+   every member of a biased trial set); a mixture sample outside [0,1)^2 is a failed walk with 0
+   iterations.  Inputs: shading point, shading normal and the glint estimator's light sample.
+   The training capture (SMS_CAPTURE_SOLUTIONS) records converged first walks as footprint
+   samples xi. */
+sms_status sms_glint_proposal_inputs(sms_ctx *ctx, const sms_glint_query *q, int64_t n, uint64_t seed, float *x);
+sms_status sms_glint_proposal_outputs(sms_ctx *ctx, const sms_glint_query *q, int64_t n, uint64_t seed, float *out);
 /* The same for the multi-scatter estimator (its light samples come from the multi-scatter
    emitters). */
 sms_status sms_ms_proposal_inputs(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *x);

@@ -2054,8 +2054,11 @@ static V3 glint_bsdf_strategy(const Scene &S, const GlintSI &si, uint64_t seed, 
 /* The glint estimator (manifold_glints.cpp:28-223): SMS strategy, then the BSDF strategy,
    which only an area emitter can receive (skipped in scenes without one) */
 static V3 estimate_glint_sms(const Scene &S, const sms_glint_query &q, const GlintSI &si, V3 sensor_p, const EI &ei,
-                             V3 edir, uint64_t seed, uint64_t qid, Stats &st);
-static V3 estimate_glint(const Scene &S, const sms_glint_query &q, uint64_t seed, uint64_t qid, Stats &st) {
+                             V3 edir, uint64_t seed, uint64_t qid, Stats &st, const float *gq, float defensive);
+/* gq: the query's prepared seed mixture over the footprint (nullptr: the reference's uniform
+   footprint seeds) */
+static V3 estimate_glint(const Scene &S, const sms_glint_query &q, uint64_t seed, uint64_t qid, Stats &st,
+                         const float *gq = nullptr, float defensive = 1.f) {
     V3 result = v3(0.f, 0.f, 0.f);
     if (q.shape < 0) return result;
     GlintSI si = gsi_from_query(q);
@@ -2066,14 +2069,14 @@ static V3 estimate_glint(const Scene &S, const sms_glint_query &q, uint64_t seed
     float edist = norm(edir);
     edir = edir * (1.f / edist);
     Ray re = make_ray(si.p, edir, RAY_EPS_F * (1.f + hmax_abs(si.p)), edist * (1.f - RAY_EPS_F));
-    if (!ray_test(S, re)) result = estimate_glint_sms(S, q, si, sensor_p, ei, edir, seed, qid, st);
+    if (!ray_test(S, re)) result = estimate_glint_sms(S, q, si, sensor_p, ei, edir, seed, qid, st, gq, defensive);
     if (scene_has_area_emitter(S)) result = result + glint_bsdf_strategy(S, si, seed, qid);
     return result;
 }
 
 /* SMS strategy of the glint estimator (manifold_glints.cpp:50-181), emitter sample visible */
 static V3 estimate_glint_sms(const Scene &S, const sms_glint_query &q, const GlintSI &si, V3 sensor_p, const EI &ei,
-                             V3 edir, uint64_t seed, uint64_t qid, Stats &st) {
+                             V3 edir, uint64_t seed, uint64_t qid, Stats &st, const float *gq, float defensive) {
     const sms_config &cfg = S.cfg;
     V3 result = v3(0.f, 0.f, 0.f);
     V3 n_offset = v3(0.f, 0.f, 1.f);
@@ -2081,7 +2084,17 @@ static V3 estimate_glint_sms(const Scene &S, const sms_glint_query &q, const Gli
     V3 sms_weight = v3(0.f, 0.f, 0.f);
     auto walk = [&](uint64_t k, V2 *uvf) {
         Pcg32 rw; stream_seed(&rw, seed, qid, k + 1);
-        V2 s; s.x = pcg32_float(&rw); s.y = pcg32_float(&rw);
+        V2 s;
+        if (gq) {
+            /* learned proposal: the footprint sample from the defensive mixture; a mixture sample
+               outside the footprint is a failed walk with 0 iterations */
+            if (proposal_seed(gq, defensive, &rw, &s) == 1 && !(s.x >= 0.f && s.x < 1.f && s.y >= 0.f && s.y < 1.f)) {
+                st.walks++;
+                return false;
+            }
+        } else {
+            s.x = pcg32_float(&rw); s.y = pcg32_float(&rw);
+        }
         V2 uvi; int it;
         bool ok = sample_glint(S, sensor_p, ei.p, si, s, n_offset, v2(1.f, 1.f), uvf, &uvi, &it);
         st.walks++; st.iters += it; if (ok) st.conv++;
@@ -3089,6 +3102,40 @@ int oracle_glint_estimate_batch(void *h, const sms_glint_query *q, int64_t n, ui
         tot.btrials += s.btrials; if (s.bmax > tot.bmax) tot.bmax = s.bmax; }
     stats_out(tot, stats);
     return 0;
+}
+
+/* The glint estimator with the learned proposal: raw network outputs (n x 48) give each query's
+   defensive mixture over its pixel footprint (defensive weight `defensive`). */
+int oracle_glint_estimate_batch_prop(void *h, const sms_glint_query *q, int64_t n, uint64_t seed, const float *raw,
+                                     float defensive, float *out_rgb, sms_stats *stats, int threads) {
+    const Scene &S = *(Scene *) h;
+    std::vector<Stats> per((size_t) n);
+    parallel_for(n, threads, [&](int64_t i) {
+        Stats st; memset(&st, 0, sizeof(st));
+        float gq[PROP_RAW];
+        gmm_prepare(raw + i * PROP_RAW, gq);
+        V3 r = estimate_glint(S, q[i], seed, (uint64_t) i, st, gq, defensive);
+        out_rgb[3 * i] = r.x; out_rgb[3 * i + 1] = r.y; out_rgb[3 * i + 2] = r.z;
+        per[(size_t) i] = st;
+    });
+    Stats tot; memset(&tot, 0, sizeof(tot));
+    for (auto &s : per) { tot.walks += s.walks; tot.conv += s.conv; tot.iters += s.iters; tot.bcalls += s.bcalls;
+        tot.btrials += s.btrials; if (s.bmax > tot.bmax) tot.bmax = s.bmax; }
+    stats_out(tot, stats);
+    return 0;
+}
+
+/* Network inputs of glint queries: shading point, shading normal and the glint estimator's light
+   sample (its emitters are all emitters). */
+void oracle_glint_prop_inputs(void *h, const sms_glint_query *q, int64_t n, uint64_t seed, float *x) {
+    const Scene &S = *(Scene *) h;
+    for (int64_t i = 0; i < n; ++i) {
+        if (q[i].shape < 0) { for (int k = 0; k < 9; ++k) x[9 * i + k] = 0.f; continue; }
+        GlintSI si = gsi_from_query(q[i]);
+        Pcg32 rq; stream_seed(&rq, seed, (uint64_t) i, 0);
+        EI ei = sample_emitter_interaction(S, si.p, S.emitters_all, &rq);
+        proposal_inputs(S, si.p, si.n, ei.p, x + 9 * i);
+    }
 }
 
 int oracle_ms_walk_batch(void *h, const sms_ms_walk_in *in, int64_t n, sms_ms_walk_out *out, int threads) {

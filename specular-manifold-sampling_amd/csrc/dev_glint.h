@@ -33,6 +33,8 @@ struct GlintFootprint {
     TriTest t0, t1;              // (a,b,c) and (a,c,d) of the footprint parallelogram
     const float2 *slope;         // padded level-0 slopes
     int res, stride; float fres, tiles;
+    const float *gmm;            // learned proposal: the query's seed mixture over the footprint (or null)
+    float defensive;             // its weight of the reference's uniform footprint seed
 };
 
 // texel source of a walk: the padded slope array in global memory

@@ -20,7 +20,7 @@ struct DUVGrid { int g; const int *start; const int *faces; };
 
 struct Proposal {
     const float *gmm;                 // per query: cdf[8], mu_u[8], mu_v[8], sx[8], sy[8], rho[8]
-    int shape;                        // mesh caster the proposal seeds (-1: none)
+    int shape;                        // mesh caster or glinty shape the proposal seeds (-1: none)
     float defensive;                  // weight of the reference's uniform seed
     DUVGrid grid;
     // training capture: records (inputs[9], uv[2], weight)
@@ -93,6 +93,26 @@ SMS_DEV bool uv_to_point(const DScene &S, const DUVGrid &G, V2 uv, V3 &p) {
 // it a Box-Muller sample, whose surface point is returned in `p_start`.  False: the mixture's
 // uv is outside [0,1)^2 or in no face (a failed walk with 0 iterations).  The caller runs one
 // sample_path_ss for both kinds, so the lanes of a wavefront share one Newton solver.
+// The draw of a defensive-mixture seed: 0 = the uniform component (its 2D sample in `out`),
+// 1 = a Gaussian of the mixture (its point in `out`, not range-checked).
+SMS_DEV int proposal_draw(const float *g, float defensive, Pcg32 &rw, V2 &out) {
+    const float t0 = pcg32_float(rw);
+    if (t0 < defensive) {
+        out.x = pcg32_float(rw); out.y = pcg32_float(rw);
+        return 0;
+    }
+    const float t1 = pcg32_float(rw), t2 = pcg32_float(rw), t3 = pcg32_float(rw);
+    int c = PROP_COMP - 1;
+    for (int j = 0; j < PROP_COMP; ++j) if (t1 < g[j]) { c = j; break; }
+    const float r = sqrtf(-2.f * logf_l(1.f - t2));
+    float sn, cs;
+    c_sincosf((2.f * PI_F) * t3, &sn, &cs);
+    const float z1 = r * cs, z2 = r * sn;
+    const float rho = g[40 + c], q = sqrtf(1.f - rho * rho);
+    out = v2(g[8 + c] + g[24 + c] * z1, g[16 + c] + g[32 + c] * (rho * z1 + q * z2));
+    return 1;
+}
+
 SMS_DEV bool proposal_seed(const DScene &S, const Proposal &P, const float *g, Pcg32 &rw, V2 &sample, V3 &p_start) {
     const float t0 = pcg32_float(rw);
     if (t0 < P.defensive) {

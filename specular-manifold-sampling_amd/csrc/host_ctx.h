@@ -135,6 +135,7 @@ sms_status noalloc_init_device(sms_ctx *ctx);
 Proposal proposal_of(sms_ctx *ctx);
 sms_status proposal_alloc(sms_ctx *ctx, int64_t n);
 sms_status proposal_enqueue(sms_ctx *ctx, const sms_ss_query *q, int64_t n, int ms);
+sms_status proposal_enqueue_glint(sms_ctx *ctx, const sms_glint_query *q, int64_t n);
 void proposal_release(sms_ctx *ctx);
 
 // launch sequences shared by the estimator calls and the frame graphs (sms_host.hip).  The

@@ -157,6 +157,7 @@ constexpr int WALKER_STACK = 32;
 __global__ void k_ss_walker(DScene S, const sms_ss_query *Q, int64_t n, const uint64_t *seedp, float *part, float *invp,
                             EstimatorWork W);
 __global__ void k_ss_coop(DScene S, const sms_ss_query *Q, const uint64_t *seedp, const float *part, float *invp, EstimatorWork W);
+__global__ void k_glint_prop_inputs(DScene S, const sms_glint_query *Q, int64_t n, const uint64_t *seedp, float *x);
 __global__ void k_ss_finalize(DScene S, const uint64_t *seedp, const float *part, const float *invp, int nc, int64_t n,
                               int offsets, float *out);
 __global__ void k_ss_queries(DScene S, int spp, const uint64_t *seedp, sms_ss_query *q, float *w, int64_t n);

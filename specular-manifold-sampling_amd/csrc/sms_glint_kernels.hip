@@ -24,6 +24,7 @@
 // of query q draws from PCG32 stream (q, k + 1), so the result of every query equals the
 // sequential trial loop whatever lane, group, wavefront or pass runs which trial.
 #include "kernels.h"
+#include "dev_ss_est.h"
 
 namespace smsdev {
 
@@ -39,6 +40,26 @@ SMS_DEV void footprint_texels(const DScene &S, int shape, GlintFootprint &fp) {
     const DShape &gs = S.shapes[shape];
     const DNormalmap nm = S.nmaps[gs.normalmap];
     fp.slope = nm.slope; fp.res = nm.res; fp.stride = nm.stride; fp.fres = (float) nm.res; fp.tiles = gs.tiles;
+    fp.gmm = nullptr; fp.defensive = 1.f;
+}
+
+// the learned proposal of query q on this footprint's shape, if one is set for it
+SMS_DEV void footprint_proposal(const Proposal &P, int64_t q, int shape, GlintFootprint &fp) {
+    if (P.gmm && shape == P.shape) { fp.gmm = P.gmm + q * PROP_RAW; fp.defensive = P.defensive; }
+}
+
+// The footprint sample xi of a glint walk (manifold_glints.cpp:252-259): two draws of the walk's
+// stream, or with a learned proposal a draw of the query's defensive mixture over the footprint
+// (the same mixture for the first walk and every Bernoulli trial); a mixture sample outside the
+// footprint is a failed walk with 0 iterations (false).
+SMS_DEV bool glint_seed(const GlintFootprint &fp, Pcg32 &rw, V2 &xi) {
+    if (!fp.gmm) {
+        xi.x = pcg32_float(rw);
+        xi.y = pcg32_float(rw);
+        return true;
+    }
+    return proposal_draw(fp.gmm, fp.defensive, rw, xi) == 0 ||
+           (xi.x >= 0.f && xi.x < 1.f && xi.y >= 0.f && xi.y < 1.f);
 }
 
 SMS_DEV unsigned long long wave_sum_u64(unsigned long long v) {
@@ -106,19 +127,33 @@ SMS_DEV void glint_first_body(const DScene &S, const sms_glint_query *__restrict
         GlintFootprint fp;
         footprint_texels(S, r.shape, fp);
         footprint_from_rec(r, fp);
+        footprint_proposal(W.prop, r.q, r.shape, fp);
         Pcg32 rw;
         stream_seed(rw, query_key(seed, (uint64_t) r.q), 1);
         V2 xi;
-        xi.x = pcg32_float(rw);
-        xi.y = pcg32_float(rw);
+        const bool seeded = glint_seed(fp, rw, xi);
         GlintWalk w;
         glint_walk_start(fp, xi, w);
-        int st;
-        while ((st = glint_walk_iter(fp, S.cfg, w)) == WALK_RUNNING) {}
+        int st = WALK_FAILED;
+        if (seeded)
+            while ((st = glint_walk_iter(fp, S.cfg, w)) == WALK_RUNNING) {}
         st_walks += 1;
         st_iters += (unsigned long long) w.it;
         conv = st == WALK_CONVERGED;
         uv_sol = w.uv;
+        if (conv && capturing(W.prop, r.shape, SMS_CAPTURE_SOLUTIONS)) {
+            // training record: the solution as a footprint sample xi (uv = uv0 + (xi - 1/2) [duv_dx duv_dy])
+            const sms_glint_query g = Q[r.q];
+            const SI si = glint_query_si(g);
+            Pcg32 rq;
+            stream_seed(rq, query_key(seed, (uint64_t) r.q), 0);
+            const EI ei = sample_emitter_interaction(S, si.p, S.emitters_all, S.n_emitters_all, rq);
+            const V2 d = uv_sol - fp.uv;
+            const float det = fp.duv_dx.x * fp.duv_dy.y - fp.duv_dy.x * fp.duv_dx.y;
+            const V2 xi_sol = v2((d.x * fp.duv_dy.y - fp.duv_dy.x * d.y) / det + 0.5f,
+                                 (fp.duv_dx.x * d.y - d.x * fp.duv_dx.y) / det + 0.5f);
+            if (det != 0.f) capture_sample(S, W.prop, si, ei, xi_sol, 1.f);
+        }
     }
     if (conv) st_conv = 1;
     // append converged first walks as trial items: one atomic per block, on the block's own
@@ -175,9 +210,9 @@ SMS_DEV int glint_trial(const GlintFootprint &fp, const sms_config &cfg, uint64_
     Pcg32 rw;
     stream_seed(rw, qkey, (uint64_t) k + 1);
     V2 xi;
-    xi.x = pcg32_float(rw);
-    xi.y = pcg32_float(rw);
+    const bool seeded = glint_seed(fp, rw, xi);
     glint_walk_start(fp, xi, w);
+    if (!seeded) return WALK_FAILED;
     int s;
     while ((s = glint_walk_iter_f(fp, cfg, w, fetch)) == WALK_RUNNING) {}
     return s;
@@ -223,6 +258,7 @@ SMS_DEV void glint_trials_body(const DScene &S, uint64_t seed, EstimatorWork W) 
         GlintFootprint fp;
         footprint_texels(S, t.f.shape, fp);
         footprint_from_rec(t.f, fp);
+        footprint_proposal(W.prop, t.f.q, t.f.shape, fp);
         GlobalTexels tex; tex.slope = fp.slope; tex.res = fp.res; tex.stride = fp.stride;
         const uint64_t qkey = query_key(seed, (uint64_t) t.f.q);
         const V2 uv_sol = v2(t.uv_sol[0], t.uv_sol[1]);
@@ -330,6 +366,7 @@ SMS_DEV void glint_coop_item(const DScene &S, uint64_t seed, EstimatorWork W, co
     GlintFootprint fp;
     footprint_texels(S, tr.f.shape, fp);
     footprint_from_rec(tr.f, fp);
+    footprint_proposal(W.prop, tr.f.q, tr.f.shape, fp);
     LdsTexels<COOP_TILE> lds;
     bool staged = stage_window(fp, S.shapes[tr.f.shape].tiles, tile, lane, lds);
     GlobalTexels tex; tex.slope = fp.slope; tex.res = fp.res; tex.stride = fp.stride;
@@ -441,6 +478,7 @@ __global__ void __launch_bounds__(256) k_glint_multiwave(DScene S, const uint64_
             const TrialRec tr = W.trials.items[ov.q];
             footprint_texels(S, tr.f.shape, fp);
             footprint_from_rec(tr.f, fp);
+            footprint_proposal(W.prop, tr.f.q, tr.f.shape, fp);
             staged = stage_window(fp, S.shapes[tr.f.shape].tiles, tile, lane, lds);
             tex.slope = fp.slope; tex.res = fp.res; tex.stride = fp.stride;
             qkey = query_key(seed, (uint64_t) tr.f.q);
@@ -602,6 +640,7 @@ SMS_DEV void glint_biased_body(const DScene &S, const sms_glint_query *__restric
                     fp.duv_dx = v2(g.duv_dx[0], g.duv_dx[1]);
                     fp.duv_dy = v2(g.duv_dy[0], g.duv_dy[1]);
                     footprint_setup(fp);
+                    footprint_proposal(W.prop, q, g.shape, fp);
                 }
                 float sum = 0.f;
                 int n_sol = 0;
@@ -615,12 +654,12 @@ SMS_DEV void glint_biased_body(const DScene &S, const sms_glint_query *__restric
                             Pcg32 rw;
                             stream_seed(rw, qkey, (uint64_t) m + 1);
                             V2 xi;
-                            xi.x = pcg32_float(rw);
-                            xi.y = pcg32_float(rw);
+                            const bool seeded = glint_seed(fp, rw, xi);
                             GlintWalk w;
                             glint_walk_start(fp, xi, w);
-                            int stw;
-                            while ((stw = glint_walk_iter(fp, cfg, w)) == WALK_RUNNING) {}
+                            int stw = WALK_FAILED;
+                            if (seeded)
+                                while ((stw = glint_walk_iter(fp, cfg, w)) == WALK_RUNNING) {}
                             st_iters += (unsigned long long) w.it;
                             if (stw == WALK_CONVERGED) {
                                 st_conv++;
@@ -688,6 +727,25 @@ SMS_DEV void glint_biased_body(const DScene &S, const sms_glint_query *__restric
     }
 SMS_GLINT_BIASED_KERNEL(k_glint_biased_bvh, true)
 SMS_GLINT_BIASED_KERNEL(k_glint_biased_nobvh, false)
+
+// Inputs of the learned proposal for glint queries: shading point, shading normal and the glint
+// estimator's light sample (stream (q, 0) over all emitters); zeros for queries without a glinty hit.
+__global__ void k_glint_prop_inputs(DScene S, const sms_glint_query *__restrict__ Q, int64_t n,
+                                    const uint64_t *__restrict__ seedp, float *__restrict__ x) {
+    const int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const sms_glint_query g = Q[q];
+    float *r = x + 9 * q;
+    if (g.shape < 0) {
+        for (int k = 0; k < 9; ++k) r[k] = 0.f;
+        return;
+    }
+    const SI si = glint_query_si(g);
+    Pcg32 rq;
+    stream_seed(rq, query_key(*seedp, (uint64_t) q), 0);
+    const EI ei = sample_emitter_interaction(S, si.p, S.emitters_all, S.n_emitters_all, rq);
+    proposal_inputs(S, si.p, si.sh_n, ei.p, r);
+}
 
 // ---------------------------------------------------------------- BSDF-sampling strategy
 // manifold_glints.cpp:183-220, one lane per query, added to the SMS strategy's result in out:

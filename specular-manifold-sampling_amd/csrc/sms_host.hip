@@ -1013,8 +1013,10 @@ sms_status glint_prepare(sms_ctx *ctx, const sms_glint_query *q, int64_t n, cons
     if (ctx->scene.cfg.biased && ctx->scene.cfg.max_trials > 128)
         return fail(ctx, SMS_E_UNSUPPORTED, "biased trial sets above 128 are not built");
     if (!ctx->scene.nmaps) return fail(ctx, SMS_E_INVALID, "scene has no normal map");
+    sms_status st = proposal_alloc(ctx, n);
+    if (st) return st;
     if (ctx->scene.cfg.biased) return ensure_work(ctx, 0, 0);
-    sms_status st = ensure_work(ctx, (size_t) std::max<int64_t>(n, 1), 0);
+    st = ensure_work(ctx, (size_t) std::max<int64_t>(n, 1), 0);
     if (!st) st = ensure_glint_work(ctx, (size_t) std::max<int64_t>(n, 1));
     return st;
 }
@@ -1024,6 +1026,10 @@ sms_status glint_prepare(sms_ctx *ctx, const sms_glint_query *q, int64_t n, cons
 sms_status glint_enqueue(sms_ctx *ctx, const sms_glint_query *q, int64_t n, float *out_rgb) {
     const uint64_t *seed = ctx->seeds + SEED_EST;
     EV_RECORD(ctx, ctx->ev0);
+    {
+        sms_status st = proposal_enqueue_glint(ctx, q, n);
+        if (st) return st;
+    }
     if (n > 0 && ctx->scene.cfg.biased) {
         EstimatorWork W = work_of(ctx);
         const dim3 grid(grid_for(ctx, ctx->occ_glint_biased));

@@ -206,6 +206,29 @@ sms_status proposal_alloc(sms_ctx *ctx, int64_t n) {
 
 // network inputs -> raw outputs -> mixtures of the n queries of an estimate (prop_g), with the
 // estimator's seed slot (ms: the multi-scatter estimator's light samples)
+sms_status launch_glint_inputs(sms_ctx *ctx, const sms_glint_query *q, int64_t n, const uint64_t *seed, float *x) {
+    if (n == 0) return SMS_OK;
+    hipLaunchKernelGGL(k_glint_prop_inputs, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream, ctx->scene, q,
+                       n, seed, x);
+    HIP_TRY(ctx, hipGetLastError());
+    return SMS_OK;
+}
+
+// the glint estimator's mixtures: inputs, network, mixture preparation, when the proposal's
+// shape is glinty
+sms_status proposal_enqueue_glint(sms_ctx *ctx, const sms_glint_query *q, int64_t n) {
+    if (!ctx->prop_mlp || ctx->prop_shape < 0 || n == 0) return SMS_OK;
+    if (!(ctx->h_shapes[ctx->prop_shape].flags & SMS_FLAG_GLINTY)) return SMS_OK;
+    if ((size_t) n > ctx->prop_cap) return fail(ctx, SMS_E_INVALID, "proposal buffers not prepared");
+    sms_status st = launch_glint_inputs(ctx, q, n, ctx->seeds + SEED_EST, ctx->prop_x);
+    if (!st) st = launch_infer(ctx->prop_mlp, ctx->prop_x, n, ctx->prop_raw);
+    if (st) return st;
+    hipLaunchKernelGGL(k_gmm_prepare, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream, ctx->prop_raw, n,
+                       ctx->prop_g);
+    HIP_TRY(ctx, hipGetLastError());
+    return SMS_OK;
+}
+
 sms_status proposal_enqueue(sms_ctx *ctx, const sms_ss_query *q, int64_t n, int ms) {
     if (!ctx->prop_mlp || ctx->prop_shape < 0 || n == 0) return SMS_OK;
     // only an estimator whose casters include the proposal's shape uses the mixtures
@@ -385,12 +408,14 @@ sms_status sms_ctx_set_proposal(sms_ctx *ctx, sms_mlp *mlp, int32_t shape, float
         }
         if (shape >= (int) ctx->h_shapes.size()) return fail(ctx, SMS_E_INVALID, "sms_ctx_set_proposal: shape out of range");
         const DShape &sh = ctx->h_shapes[shape];
-        if (sh.type != SMS_SHAPE_MESH || !sh.has_uv || !(sh.flags & (SMS_FLAG_CAUSTIC_CASTER_SS | SMS_FLAG_CAUSTIC_CASTER_MS)))
-            return fail(ctx, SMS_E_INVALID, "sms_ctx_set_proposal: the shape must be a caster mesh with uvs");
+        const bool glinty = (sh.flags & SMS_FLAG_GLINTY) && sh.bsdf == SMS_BSDF_NORMALMAP;
+        if (!glinty &&
+            (sh.type != SMS_SHAPE_MESH || !sh.has_uv || !(sh.flags & (SMS_FLAG_CAUSTIC_CASTER_SS | SMS_FLAG_CAUSTIC_CASTER_MS))))
+            return fail(ctx, SMS_E_INVALID, "sms_ctx_set_proposal: the shape must be a caster mesh with uvs or a glinty shape");
         if (mlp && mlp->ctx != ctx) return fail(ctx, SMS_E_INVALID, "sms_ctx_set_proposal: network of another context");
         if (mlp && !(defensive > 0.f && defensive <= 1.f))
             return fail(ctx, SMS_E_INVALID, "sms_ctx_set_proposal: defensive weight must be in (0, 1]");
-        if (ctx->grid_shape != shape) {
+        if (!glinty && ctx->grid_shape != shape) {
             sms_status st = build_uv_grid(ctx, shape);
             if (st) return st;
         }
@@ -503,6 +528,27 @@ sms_status proposal_io(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t 
 }
 
 } // namespace
+
+sms_status glint_proposal_io(sms_ctx *ctx, const sms_glint_query *q, int64_t n, uint64_t seed, float *x, float *out) {
+    if (!ctx || n < 0 || (n > 0 && (!q || !(x || out)))) return fail(ctx, SMS_E_INVALID, "proposal inputs: bad arguments");
+    if (out && !ctx->prop_mlp) return fail(ctx, SMS_E_INVALID, "proposal outputs: no proposal network set");
+    sms_status st = out ? ensure_prop(ctx, (size_t) std::max<int64_t>(n, 1)) : SMS_OK;
+    if (st) return st;
+    hipLaunchKernelGGL(k_set_seed, dim3(1), dim3(64), 0, ctx->stream, ctx->seeds + SEED_API, seed);
+    HIP_TRY(ctx, hipGetLastError());
+    if (!out) return launch_glint_inputs(ctx, q, n, ctx->seeds + SEED_API, x);
+    st = launch_glint_inputs(ctx, q, n, ctx->seeds + SEED_API, ctx->prop_x);
+    if (!st) st = launch_infer(ctx->prop_mlp, ctx->prop_x, n, out);
+    return st;
+}
+
+sms_status sms_glint_proposal_inputs(sms_ctx *ctx, const sms_glint_query *q, int64_t n, uint64_t seed, float *x) {
+    return guarded(ctx, [&]() -> sms_status { return glint_proposal_io(ctx, q, n, seed, x, nullptr); });
+}
+
+sms_status sms_glint_proposal_outputs(sms_ctx *ctx, const sms_glint_query *q, int64_t n, uint64_t seed, float *out) {
+    return guarded(ctx, [&]() -> sms_status { return glint_proposal_io(ctx, q, n, seed, nullptr, out); });
+}
 
 sms_status sms_ss_proposal_inputs(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *x) {
     return guarded(ctx, [&]() -> sms_status {

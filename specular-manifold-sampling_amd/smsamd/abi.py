@@ -160,7 +160,7 @@ EXPORTS = ["sms_config_default", "sms_ctx_create", "sms_ctx_destroy", "sms_ctx_s
            "sms_ctx_set_training_capture", "sms_ctx_training_samples", "sms_ctx_clear_training",
            "sms_ss_proposal_inputs", "sms_ss_proposal_outputs", "sms_ss_make_queries",
            "sms_film_accumulate", "sms_ss_autotune", "sms_ms_autotune", "sms_scene_footprint",
-           "sms_ss_estimate_batch_noalloc"]
+           "sms_ss_estimate_batch_noalloc", "sms_glint_proposal_inputs", "sms_glint_proposal_outputs"]
 
 # learned seed proposal (include/sms.h)
 MLP_INPUTS = 9
@@ -226,6 +226,8 @@ def load_library(path=LIB_PATH):
     lib.sms_ss_proposal_outputs.argtypes = [vp, vp, i64, C.c_uint64, vp]
     lib.sms_ms_proposal_inputs.argtypes = [vp, vp, i64, C.c_uint64, vp]
     lib.sms_ms_proposal_outputs.argtypes = [vp, vp, i64, C.c_uint64, vp]
+    lib.sms_glint_proposal_inputs.argtypes = [vp, vp, i64, C.c_uint64, vp]
+    lib.sms_glint_proposal_outputs.argtypes = [vp, vp, i64, C.c_uint64, vp]
     lib.sms_ss_make_queries.argtypes = [vp, C.c_int32, C.c_uint64, vp, vp]
     lib.sms_film_accumulate.argtypes = [vp, vp, vp, i64, C.c_int32, vp, vp, vp]
     for name in EXPORTS:

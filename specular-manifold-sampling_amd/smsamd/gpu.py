@@ -189,12 +189,14 @@ class Context:
                                          imgsqbuf.ptr if imgsqbuf is not None else None,
                                          imgnbuf.ptr if imgnbuf is not None else None), self.h, "film_accumulate")
 
-    def glint_queries_device(self, spp, seed):
+    def glint_queries_device(self, spp, seed, qbuf=None):
+        """Camera pass of a glint frame into device memory (qbuf, or a new buffer): (buffer, n)."""
         cam = self.scene.camera
         n = cam["width"] * cam["height"] * spp
-        buf = DeviceBuffer(self, n * abi.GLINT_QUERY.itemsize)
+        buf = qbuf if qbuf is not None else DeviceBuffer(self, n * abi.GLINT_QUERY.itemsize)
         _check(lib().sms_glint_make_queries(self.h, spp, seed, buf.ptr), self.h, "glint_queries")
-        self.sync()
+        if qbuf is None:
+            self.sync()
         return buf, n
 
     def glint_estimate_device(self, qbuf, n, seed, outbuf):
@@ -323,6 +325,13 @@ class Context:
 
     def ms_proposal_outputs(self, q, seed):
         return self._per_query(lib().sms_ms_proposal_outputs, q, seed, abi.MLP_OUTPUTS, "sms_ms_proposal_outputs")
+
+    def glint_proposal_inputs(self, q, seed):
+        return self._per_query(lib().sms_glint_proposal_inputs, q, seed, abi.MLP_INPUTS, "sms_glint_proposal_inputs")
+
+    def glint_proposal_outputs(self, q, seed):
+        return self._per_query(lib().sms_glint_proposal_outputs, q, seed, abi.MLP_OUTPUTS,
+                               "sms_glint_proposal_outputs")
 
 
 class Graph:

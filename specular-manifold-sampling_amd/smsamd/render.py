@@ -61,7 +61,8 @@ class Film:
 
 class PassRunner:
     """Device buffers for passes of ``pass_spp`` samples per pixel over the camera's frame;
-    ``estimator``: "ss" (single scatter) or "ms" (multi scatter) at the first receiver."""
+    ``estimator``: "ss" (single scatter) or "ms" (multi scatter) at the first receiver, or
+    "glint" (the glint estimator at the camera ray's glinty hit, sample weight 1)."""
 
     def __init__(self, ctx, pass_spp, estimator="ss"):
         cam = ctx.scene.camera
@@ -69,16 +70,24 @@ class PassRunner:
         self.pixels = cam["width"] * cam["height"]
         self.pass_spp = pass_spp
         self.n = self.pixels * pass_spp
-        self.q = gpu.DeviceBuffer(ctx, self.n * abi.SS_QUERY.itemsize)
+        if estimator not in ("ss", "ms", "glint"):
+            raise ValueError(estimator)
+        self.estimator = estimator
+        qsize = abi.GLINT_QUERY.itemsize if estimator == "glint" else abi.SS_QUERY.itemsize
+        self.q = gpu.DeviceBuffer(ctx, self.n * qsize)
         self.w = gpu.DeviceBuffer(ctx, self.n * 4)
         self.rgb = gpu.DeviceBuffer(ctx, self.n * 12)
-        if estimator not in ("ss", "ms"):
-            raise ValueError(estimator)
-        self.estimate = ctx.ss_estimate_device if estimator == "ss" else ctx.ms_estimate_device
+        if estimator == "glint":
+            self.w.upload(np.ones(self.n, np.float32))
+        self.estimate = {"ss": ctx.ss_estimate_device, "ms": ctx.ms_estimate_device,
+                         "glint": ctx.glint_estimate_device}[estimator]
 
     def run(self, seed, film, stats=None):
         """One pass: camera samples of seed `seed`, estimates with the same seed, into film."""
-        self.ctx.ss_queries_device(self.pass_spp, seed, self.q, self.w)
+        if self.estimator == "glint":
+            self.ctx.glint_queries_device(self.pass_spp, seed, self.q)
+        else:
+            self.ctx.ss_queries_device(self.pass_spp, seed, self.q, self.w)
         st = self.estimate(self.q, self.n, seed, self.rgb)
         self.ctx.film_accumulate_device(self.rgb, self.w, self.pixels, self.pass_spp, film.sum, film.sq, film.n)
         film.spp += self.pass_spp
@@ -99,7 +108,7 @@ def render_online(ctx, runner, film, mlp, caster, budget_s, samples=2_000_000, b
     first `uniform_passes` passes use uniform seeds, every later pass the defensive mixture with
     the network as trained so far (the uniform passes with `uniform_runner`, a runner of fewer
     samples per pixel, when given).  Each pass captures estimate-weighted records on mesh
-    `caster`; a record's weight is its estimate, so the expected total weight at a solution is
+    `caster` (a glint runner: its converged first walks, weight 1); a record's weight is its estimate, so the expected total weight at a solution is
     the solution's contribution whatever proposal found it, and records of learned passes are
     as valid as uniform ones.  After a pass, epochs x (records // batch) Adam steps (at most
     max_steps_per_pass) on windows at random offsets of its records run before the next pass,
@@ -110,7 +119,9 @@ def render_online(ctx, runner, film, mlp, caster, budget_s, samples=2_000_000, b
     spent training)."""
     rng = np.random.default_rng(rng_seed)
     ctx.set_proposal(None, caster)
-    ctx.set_training_capture(max(runner.n, uniform_runner.n if uniform_runner else 0), abi.CAPTURE_ESTIMATES)
+    # glint estimates capture their converged first walks (weight 1) as footprint samples
+    mode = abi.CAPTURE_SOLUTIONS if runner.estimator == "glint" else abi.CAPTURE_ESTIMATES
+    ctx.set_training_capture(max(runner.n, uniform_runner.n if uniform_runner else 0), mode)
     t0 = time.perf_counter()
     out = {"passes": 0, "steps": 0, "losses": [], "used": 0, "train_s": 0.0}
     capturing, last = True, 0.0

@@ -73,6 +73,9 @@ def load(path=ORACLE_SO):
     lib.oracle_gmm_prepare.argtypes = [vp, i64, vp]
     lib.oracle_prop_seed.argtypes = [vp, C.c_float, C.c_uint64, C.c_uint64, C.c_uint64, i64, vp]
     lib.oracle_uv_to_point.argtypes = [vp, C.c_int, vp, i64, C.c_int, vp, vp]
+    lib.oracle_glint_estimate_batch_prop.argtypes = [vp, vp, i64, C.c_uint64, vp, C.c_float, vp,
+                                                     C.POINTER(abi.SmsStats), C.c_int]
+    lib.oracle_glint_prop_inputs.argtypes = [vp, vp, i64, C.c_uint64, vp]
     return lib
 
 
@@ -179,6 +182,22 @@ class Oracle:
                                                     defensive, out.ctypes.data, C.byref(st), self.threads)
         assert rc == 0, rc
         return out, st.as_dict()
+
+    def glint_estimate_prop(self, q, seed, raw, defensive):
+        """Glint estimator with each query's defensive mixture over its footprint; raw: (n, 48)."""
+        raw = np.ascontiguousarray(raw, np.float32)
+        assert raw.shape == (len(q), 48)
+        out = np.zeros((len(q), 3), np.float32)
+        st = abi.SmsStats()
+        rc = self.lib.oracle_glint_estimate_batch_prop(self.h, q.ctypes.data, len(q), seed, raw.ctypes.data, defensive,
+                                                       out.ctypes.data, C.byref(st), self.threads)
+        assert rc == 0, rc
+        return out, st.as_dict()
+
+    def glint_prop_inputs(self, q, seed):
+        x = np.zeros((len(q), 9), np.float32)
+        self.lib.oracle_glint_prop_inputs(self.h, q.ctypes.data, len(q), seed, x.ctypes.data)
+        return x
 
     def scene_box(self):
         c = np.zeros(3, np.float32)

@@ -350,3 +350,37 @@ def test_glint_bsdf_strategy_geometry():
     assert not zero_bsdf.any()
     assert np.array_equal(full, sms_only + values)
     orc.close()
+
+
+def test_glint_mixture_estimator_is_unbiased():
+    """The learned proposal on glint footprints: seeds drawn from a defensive mixture over the
+    footprint (concentrated away from the centre) instead of the uniform footprint sample; the
+    Bernoulli estimator's mean is unchanged within standard errors (the same mixture drives the
+    first walk and the trials), the walks converge at another rate."""
+    sc = scenes.glint_plane(32, 32, nm_res=256, n_scratches=64, seed=3)
+    cfg = abi.SmsConfig.default(max_trials=100000)
+    orc = oracle_lib.Oracle(sc, cfg)
+    q = orc.glint_queries(2, 5)
+    q = np.ascontiguousarray(q[q["shape"] >= 0][:400])
+    rng = np.random.default_rng(4)
+    raw = np.zeros((len(q), 48), np.float32)
+    for c in range(8):
+        raw[:, 6 * c] = rng.normal(0, 1, len(q))
+        raw[:, 6 * c + 1] = rng.normal(0.8, 0.5, len(q))
+        raw[:, 6 * c + 2] = rng.normal(-0.5, 0.5, len(q))
+        raw[:, 6 * c + 3] = raw[:, 6 * c + 4] = np.log(0.15)
+        raw[:, 6 * c + 5] = rng.normal(0, 0.5, len(q))
+    reps = 500
+    qq, rr = np.concatenate([q] * reps), np.concatenate([raw] * reps)
+    uni, su = orc.glint_estimate(qq, 3)
+    mix, sm = orc.glint_estimate_prop(qq, 3, rr, 0.2)
+    lu, lm = uni[:, 0].astype(np.float64), mix[:, 0].astype(np.float64)
+    # paired by query: the footprints' own spread cancels
+    du = lu.reshape(reps, -1).mean(0)
+    dm = lm.reshape(reps, -1).mean(0)
+    se = math.sqrt(lu.var() / len(lu) + lm.var() / len(lm))
+    print(f"uniform {lu.mean():.6g} mixture {lm.mean():.6g} se {se:.3g} conv {su['walks_converged'] / su['walks']:.4f} "
+          f"{sm['walks_converged'] / sm['walks']:.4f}")
+    assert (du > 0).sum() > 20
+    assert abs(lu.mean() - lm.mean()) < 4 * se
+    assert sm["walks"] != su["walks"]
