@@ -549,6 +549,61 @@ sms_status sms_graph_launch(sms_graph *g, uint64_t seed);
 sms_status sms_graph_stats(sms_graph *g, sms_stats *stats, int32_t reset);
 sms_status sms_graph_destroy(sms_graph *g);
 
+/* ---------------------------------------------------------------- path integrators
+   The path tracers around the single-scatter SMS term, over the scene's diffuse, smooth
+   dielectric and smooth conductor BSDFs (scenes with normal-mapped or rough shapes are
+   refused with SMS_E_UNSUPPORTED) and its area, point and directional emitters:
+     SMS_PATH_SMS_SS       path_sms_ss (src/integrators/path_sms_ss.cpp:69-251): emitter hits,
+                           emitter sampling on smooth BSDFs, BSDF sampling, and at every caustic
+                           receiver that is not a caster the SMS estimate of
+                           sms_ss_estimate_batch; light paths through a caster found by BSDF
+                           sampling are dropped -- except, with config.mnee_init and not
+                           biased_mnee, those an MNEE walk from the receiver does not re-find
+                           (:196-241)
+     SMS_PATH_FILTERED_SS  path_filtered_ss (src/integrators/path_filtered_ss.cpp:37-194): the
+                           same light paths by BSDF / emitter sampling only
+   Samples are PCG32 streams of (seed, sample index) (stream 0xFFFFFFFD: film jitter, then per
+   bounce the Russian-roulette, emitter and BSDF samples in the reference's order); the SMS
+   estimates of a pass run in batches of SMS_PATH_CHUNK samples, each batch's queries in
+   (sample, depth) order with its own estimator seed.  max_depth -1 (the reference's unbounded
+   default) ends paths at SMS_PATH_MAX_DEPTH. */
+#define SMS_PATH_SMS_SS       0
+#define SMS_PATH_FILTERED_SS  1
+#define SMS_PATH_MAX_DEPTH    16
+#define SMS_PATH_CHUNK        (1 << 20)
+typedef struct sms_path_config {
+    int32_t integrator;                 /* SMS_PATH_*                                           */
+    int32_t max_depth;                  /* "max_depth" (MonteCarloIntegrator, -1: unbounded)    */
+    int32_t rr_depth;                   /* "rr_depth" (5)                                        */
+    int32_t disable_reflected_emitters; /* "disable_reflected_emitters" (path_sms_ss.cpp:59)     */
+    int32_t biased_mnee;                /* "biased_mnee" (path_sms_ss.cpp:60)                    */
+    /* No counterpart in the reference: 1 keeps only the single-scatter caustic light paths
+       whose receiver is the path's first non-specular vertex (SMS mode: the SMS term there and
+       MNEE completions through it; filtered mode: BSDF-sampled emitter hits through one caster
+       after it) -- the term the bench's caustic images measure; the path's samples and every
+       other choice are those of the full integrator. */
+    int32_t caustics_only;
+    float   filter_stddev;              /* gaussian rfilter "stddev" (src/rfilters/gaussian.cpp) */
+    int32_t pad;
+} sms_path_config;
+
+/* Defaults of the reference's integrators and film: max_depth -1, rr_depth 5, stddev 0.5. */
+sms_status sms_path_config_default(sms_path_config *pc);
+/* One pass of `spp` samples per pixel over the camera's w x h film: per-sample radiance
+   (device, n = w*h*spp in pixel-major order, x 3 floats) and film positions (device, n x 2
+   floats, in pixels: pixel + jitter), and the summed statistics of the pass's SMS estimates
+   (path_sms_ss; zeros for path_filtered_ss). */
+sms_status sms_path_samples(sms_ctx *ctx, const sms_path_config *pc, int32_t spp, uint64_t seed, float *rgb,
+                            float *pos, sms_stats *stats);
+/* Film reconstruction of one pass (ImageBlock::put, src/librender/imageblock.cpp:80-165, with
+   the Gaussian filter of `stddev`, radius 4 stddev, discretized as rfilter.h:62-66): adds,
+   per pixel, the filter-weighted sums of the pass's samples (rgb, pos as written by
+   sms_path_samples) to image (device, w*h*3) and the summed weights to weight (device, w*h);
+   the developed pixel is image / weight.  Samples with a non-finite or negative (below -1e-5)
+   channel are dropped whole.  stddev 0: box filter (each sample into its own pixel, weight 1). */
+sms_status sms_film_splat(sms_ctx *ctx, const float *rgb, const float *pos, int32_t spp, float stddev, float *image,
+                          float *weight);
+
 /* ---------------------------------------------------------------- device memory helpers */
 sms_status sms_device_alloc(sms_ctx *ctx, size_t bytes, void **ptr);
 sms_status sms_device_free(sms_ctx *ctx, void *ptr);

@@ -2851,6 +2851,420 @@ static V3 estimate_ms(const Scene &S, const sms_ss_query &q, uint64_t seed, uint
     return result;
 }
 
+/* ================================================================ path integrators
+   path_sms_ss (src/integrators/path_sms_ss.cpp:69-251) and path_filtered_ss
+   (src/integrators/path_filtered_ss.cpp:37-194) over diffuse, smooth dielectric and smooth
+   conductor BSDFs.  A sample's numbers come from PCG32 stream PATH_STREAM of (seed, sample
+   index): the film jitter, then per bounce the Russian-roulette sample (1D), the emitter
+   sample (2D), the BSDF sample (1D + 2D) and, where path_sms_ss checks a BSDF-sampled caustic
+   against MNEE, the walk's seed sample (2D, and the LEAN sample with twostage), in the
+   reference's order.  The path's own terms are summed first, then its SMS estimates in depth
+   order, then its MNEE completions. */
+static const uint64_t PATH_STREAM = 0xFFFFFFFDull;
+
+/* warp::square_to_uniform_disk_concentric (include/mitsuba/core/warp.h:54-90) */
+static V2 square_to_uniform_disk_concentric(V2 s) {
+    float x = fmaf(2.f, s.x, -1.f), y = fmaf(2.f, s.y, -1.f);
+    bool is_zero = x == 0.f && y == 0.f, q13 = fabsf(x) < fabsf(y);
+    float r = q13 ? y : x, rp = q13 ? x : y;
+    float phi = .25f * PI_F * rp / r;
+    if (q13) phi = .5f * PI_F - phi;
+    if (is_zero) phi = 0.f;
+    float sn, cs; c_sincosf(phi, &sn, &cs);
+    return v2(r * cs, r * sn);
+}
+
+/* warp::square_to_cosine_hemisphere (warp.h:332-340) */
+static V3 square_to_cosine_hemisphere(V2 s) {
+    V2 p = square_to_uniform_disk_concentric(s);
+    return v3(p.x, p.y, safe_sqrt(1.f - (p.x * p.x + p.y * p.y)));
+}
+
+struct DirSample { V3 p, n, d; float dist, pdf; bool delta; int emitter; };
+
+static DirSample dir_sample_zero() {
+    DirSample ds;
+    ds.p = ds.n = ds.d = v3(0.f, 0.f, 0.f); ds.dist = 0.f; ds.pdf = 0.f; ds.delta = false; ds.emitter = -1;
+    return ds;
+}
+
+/* Sphere::sample_direction (src/shapes/sphere.cpp:177-252): a uniform direction in the cone
+   the sphere subtends seen from outside, else a uniform point on the sphere */
+static DirSample sphere_sample_direction(const Shape &sh, V3 ref, V2 sample) {
+    DirSample ds = dir_sample_zero();
+    V3 dc_v = sh.center - ref;
+    float dc_2 = dot(dc_v, dc_v);
+    float radius_adj = sh.radius * (1.f - RAY_EPS_F);
+    if (dc_2 > radius_adj * radius_adj) {
+        float inv_dc = 1.f / sqrtf(dc_2), sin_theta_max = sh.radius * inv_dc,
+              sin_theta_max_2 = sin_theta_max * sin_theta_max, inv_sin_theta_max = 1.f / sin_theta_max,
+              cos_theta_max = safe_sqrt(1.f - sin_theta_max_2);
+        float a = fmaf(cos_theta_max - 1.f, sample.x, 1.f);
+        float sin_theta_2 = sin_theta_max_2 > 0.00068523f ? 1.f - a * a : sin_theta_max_2 * sample.x;
+        float cos_theta = safe_sqrt(1.f - sin_theta_2);
+        float cos_alpha = sin_theta_2 * inv_sin_theta_max +
+                          cos_theta * safe_sqrt(fmaf(-sin_theta_2, inv_sin_theta_max * inv_sin_theta_max, 1.f));
+        float sin_alpha = safe_sqrt(fmaf(-cos_alpha, cos_alpha, 1.f));
+        float sin_phi, cos_phi; c_sincosf(sample.y * (2.f * PI_F), &sin_phi, &cos_phi);
+        V3 fn = dc_v * -inv_dc, fs, ft;
+        coordinate_system(fn, &fs, &ft);
+        V3 d = to_world(fs, ft, fn, v3(cos_phi * sin_alpha, sin_phi * sin_alpha, cos_alpha));
+        ds.p = fmadd3(d, sh.radius, sh.center);
+        ds.n = d;
+        ds.d = ds.p - ref;
+        float dist2 = dot(ds.d, ds.d);
+        ds.dist = sqrtf(dist2);
+        ds.d = ds.d / ds.dist;
+        ds.pdf = INV_TWOPI_F / (1.f - cos_theta_max);
+        if (ds.dist == 0.f) ds.pdf = 0.f;
+    } else {
+        V3 d = square_to_uniform_sphere(sample);
+        ds.p = fmadd3(d, sh.radius, sh.center);
+        ds.n = d;
+        ds.d = ds.p - ref;
+        float dist2 = dot(ds.d, ds.d);
+        ds.dist = sqrtf(dist2);
+        ds.d = ds.d / ds.dist;
+        ds.pdf = sh.inv_surface_area * dist2 / fabsf(dot(ds.d, ds.n));
+    }
+    ds.delta = sh.radius == 0.f;
+    return ds;
+}
+
+/* Scene::sample_emitter_direction (src/librender/scene.cpp:156-203) with the visibility test,
+   over AreaLight (area.cpp:104-117), PointLight (point.cpp:74-99) and DirectionalEmitter
+   (directional.cpp:105-128) */
+static DirSample scene_sample_emitter_direction(const Scene &S, V3 ref, V2 sample, bool test_visibility, V3 *spec_out) {
+    DirSample ds = dir_sample_zero();
+    V3 spec = v3(0.f, 0.f, 0.f);
+    const int n = (int) S.emitters.size();
+    *spec_out = spec;
+    if (n == 0) return ds;
+    int index = 0;
+    float emitter_pdf = 1.f;
+    if (n > 1) {
+        emitter_pdf = 1.f / (float) n;
+        uint32_t idx = (uint32_t) (sample.x * (float) n);
+        index = (int) std::min(idx, (uint32_t) (n - 1));
+        sample.x = (sample.x - (float) index * emitter_pdf) * (float) n;
+    }
+    const Emitter &em = S.emitters[(size_t) index];
+    if (em.type == SMS_EMITTER_AREA) {
+        ds = sphere_sample_direction(S.shapes[(size_t) em.shape], ref, sample);
+        bool ok = dot(ds.d, ds.n) < 0.f && ds.pdf != 0.f;
+        if (ok) spec = v3(em.radiance.x / ds.pdf, em.radiance.y / ds.pdf, em.radiance.z / ds.pdf);
+    } else if (em.type == SMS_EMITTER_POINT) {
+        ds.p = em.position; ds.pdf = 1.f; ds.delta = true;
+        ds.d = ds.p - ref;
+        ds.dist = norm(ds.d);
+        float inv_dist = 1.f / ds.dist;
+        ds.d = ds.d * inv_dist;
+        spec = em.intensity * (inv_dist * inv_dist);
+    } else {
+        V3 d = em.direction;
+        float dist = S.bs_dist;
+        ds.p = ref - d * dist; ds.n = d; ds.pdf = 1.f; ds.delta = true; ds.d = d * -1.f; ds.dist = dist;
+        spec = em.irradiance;
+    }
+    ds.emitter = index;
+    if (n > 1) {
+        ds.pdf = ds.pdf * emitter_pdf;
+        spec = spec * (1.f / emitter_pdf);
+    }
+    if (test_visibility && ds.pdf != 0.f) {
+        Ray r = make_ray(ref, ds.d, RAY_EPS_F * (1.f + hmax_abs(ref)), ds.dist * (1.f - SHADOW_EPS_F));
+        if (ray_test(S, r)) spec = v3(0.f, 0.f, 0.f);
+    }
+    *spec_out = spec;
+    return ds;
+}
+
+/* Scene::pdf_emitter_direction (scene.cpp:205-219) of a BSDF-sampled hit on an area emitter:
+   AreaLight::pdf_direction (area.cpp:119-125) and Sphere::pdf_direction (sphere.cpp:254-267)
+   with DirectionSample3f(si_hit, ref) (records.h:168-174) */
+static float pdf_emitter_hit(const Scene &S, int emitter, const SI &hit, V3 ref) {
+    const Emitter &em = S.emitters[(size_t) emitter];
+    if (em.type != SMS_EMITTER_AREA) return 0.f;
+    const Shape &sh = S.shapes[(size_t) em.shape];
+    V3 d = hit.p - ref;
+    float dist = norm(d);
+    d = d / dist;
+    if (!(dot(d, hit.sh_n) < 0.f)) return 0.f;
+    float sin_alpha = sh.radius * (1.f / norm(sh.center - ref));
+    float cos_alpha = safe_sqrt(1.f - sin_alpha * sin_alpha);
+    float pdf = sin_alpha < 0.99999994039535522461f ? INV_TWOPI_F / (1.f - cos_alpha)
+                                                    : sh.inv_surface_area * (dist * dist) / fabsf(dot(d, hit.sh_n));
+    if (S.emitters.size() > 1) pdf = pdf * (1.f / (float) S.emitters.size());
+    return pdf;
+}
+
+/* AreaLight::eval (area.cpp:71-79) at a surface interaction on its shape */
+static V3 emitter_eval_hit(const Scene &S, int emitter, const SI &hit) {
+    const Emitter &em = S.emitters[(size_t) emitter];
+    if (em.type != SMS_EMITTER_AREA || !(hit.wi.z > 0.f)) return v3(0.f, 0.f, 0.f);
+    return em.radiance;
+}
+
+struct BsdfSample { V3 wo, weight; float pdf, eta; bool delta, reflection, transmission; };
+
+/* BSDF::sample of SmoothDiffuse (diffuse.cpp:77-101), SmoothDielectric (dielectric.cpp:201-313)
+   and SmoothConductor (conductor.cpp:217-275) in the local shading frame */
+static BsdfSample bsdf_sample(const Shape &sh, V3 wi, float s1, V2 s2) {
+    BsdfSample bs;
+    bs.wo = v3(0.f, 0.f, 0.f); bs.weight = v3(0.f, 0.f, 0.f); bs.pdf = 0.f; bs.eta = 1.f;
+    bs.delta = sh.bsdf != SMS_BSDF_DIFFUSE; bs.reflection = true; bs.transmission = false;
+    const float cos_i = wi.z;
+    if (sh.bsdf == SMS_BSDF_DIFFUSE) {
+        if (!(cos_i > 0.f)) { bs.pdf = 0.f; return bs; }
+        bs.wo = square_to_cosine_hemisphere(s2);
+        bs.pdf = INV_PI_F * bs.wo.z;
+        if (bs.pdf > 0.f) bs.weight = sh.refl;
+    } else if (sh.bsdf == SMS_BSDF_DIELECTRIC) {
+        const float eta = sh.eta_r;
+        const bool outside = cos_i >= 0.f;
+        const float rcp_eta = 1.f / eta, eta_it = outside ? eta : rcp_eta, eta_ti = outside ? rcp_eta : eta;
+        const float cos_t_sqr = fmaf(-fmaf(-cos_i, cos_i, 1.f), eta_ti * eta_ti, 1.f);
+        const float cos_t_abs = safe_sqrt(cos_t_sqr);
+        const float F = fresnel(cos_i, eta);
+        if (s1 <= F) {
+            bs.wo = v3(-wi.x, -wi.y, wi.z); bs.pdf = F; bs.weight = v3(1.f, 1.f, 1.f);
+        } else {
+            const float cos_t = outside ? -cos_t_abs : cos_t_abs;
+            bs.wo = v3(-eta_ti * wi.x, -eta_ti * wi.y, cos_t);
+            bs.pdf = 1.f - F; bs.eta = eta_it; bs.reflection = false; bs.transmission = true;
+            const float f = eta_ti * eta_ti;
+            bs.weight = v3(f, f, f);
+        }
+    } else {
+        if (!(cos_i > 0.f)) return bs;
+        bs.wo = v3(-wi.x, -wi.y, wi.z); bs.pdf = 1.f;
+        const float F = fresnel_conductor(cos_i, sh.eta_r, sh.k);
+        bs.weight = v3(F, F, F);
+    }
+    return bs;
+}
+
+/* SmoothDiffuse::eval / pdf (diffuse.cpp:103-133); the delta BSDFs evaluate to zero */
+static V3 bsdf_eval(const Shape &sh, V3 wi, V3 wo) {
+    if (sh.bsdf != SMS_BSDF_DIFFUSE || !(wi.z > 0.f && wo.z > 0.f)) return v3(0.f, 0.f, 0.f);
+    return v3((sh.refl.x * INV_PI_F) * wo.z, (sh.refl.y * INV_PI_F) * wo.z, (sh.refl.z * INV_PI_F) * wo.z);
+}
+static float bsdf_pdf(const Shape &sh, V3 wi, V3 wo) {
+    if (sh.bsdf != SMS_BSDF_DIFFUSE || !(wi.z > 0.f && wo.z > 0.f)) return 0.f;
+    return INV_PI_F * wo.z;
+}
+
+static inline V3 mul3(V3 a, V3 b) { return v3(a.x * b.x, a.y * b.y, a.z * b.z); }
+
+/* the film position and primary ray of sample idx (pixel idx / spp, jitter from the path stream) */
+static Ray path_camera_ray(const Scene &S, int64_t idx, int spp, Pcg32 *r, float *pos) {
+    const Camera &c = S.cam;
+    int64_t pixel = idx / spp;
+    int px = (int) (pixel % c.w), py = (int) (pixel / c.w);
+    float sx = pcg32_float(r), sy = pcg32_float(r);
+    pos[0] = (float) px + sx; pos[1] = (float) py + sy;
+    V2 p = v2(((float) px + sx) / (float) c.w, ((float) py + sy) / (float) c.h);
+    V3 near_p = v3((1.f - 2.f * p.x) * c.tan_half, (1.f - 2.f * p.y) * c.rect_y, 1.f);
+    V3 dloc = normalize(near_p);
+    V3 d = to_world(c.left, c.up, c.dir, dloc);
+    return make_ray(c.o, d, 1e-2f * (1.f / dloc.z), 1e4f * (1.f / dloc.z));
+}
+
+struct PathSms { sms_ss_query q; V3 thr; };
+struct PathMnee { sms_ss_walk_in w; V3 value, dir_pt, ref_p; };
+struct PathSample { V3 L; float pos[2]; std::vector<PathSms> sms; std::vector<PathMnee> mnee; };
+
+static void sms_query_from_si(const SI &si, sms_ss_query *q) {
+    memset(q, 0, sizeof(*q));
+    q->shape = si.shape;
+    for (int a = 0; a < 3; ++a) {
+        q->p[a] = (&si.p.x)[a]; q->n[a] = (&si.n.x)[a]; q->sh_n[a] = (&si.sh_n.x)[a];
+        q->sh_s[a] = (&si.sh_s.x)[a]; q->sh_t[a] = (&si.sh_t.x)[a]; q->wi[a] = (&si.wi.x)[a];
+    }
+}
+
+/* one sample of path_sms_ss / path_filtered_ss (SamplingIntegrator::sample) */
+static void path_sample(const Scene &S, const sms_path_config &pc, uint64_t seed, int64_t idx, int spp, PathSample &o) {
+    const bool sms = pc.integrator == SMS_PATH_SMS_SS, only = pc.caustics_only != 0;
+    const int depth_cap = pc.max_depth < 0 ? SMS_PATH_MAX_DEPTH : pc.max_depth;
+    Pcg32 r; stream_seed(&r, seed, (uint64_t) idx, PATH_STREAM);
+    o.sms.clear(); o.mnee.clear();
+    Ray ray = path_camera_ray(S, idx, spp, &r, o.pos);
+    V3 L = v3(0.f, 0.f, 0.f), thr = v3(1.f, 1.f, 1.f);
+    float eta = 1.f;
+    SI si = ray_intersect(S, ray), si_prev = si_invalid();
+    int first_diffuse = -1;
+    if (si.valid && !only) {
+        int e = S.shapes[(size_t) si.shape].emitter;
+        if (e >= 0) L = L + emitter_eval_hit(S, e, si);
+    }
+    for (int depth = 1;; ++depth) {
+        if (!si.valid) break;
+        if (depth > pc.rr_depth) {
+            float m = thr.x < thr.y ? thr.y : thr.x;          /* std::max / std::min order */
+            m = m < thr.z ? thr.z : m;
+            float q = m * (eta * eta);
+            q = .95f < q ? (float) .95f : q;
+            if (pcg32_float(&r) > q) break;
+            thr = thr * (1.f / q);
+        }
+        if (depth >= depth_cap) break;
+        if (only && first_diffuse >= 0 && depth > first_diffuse + 1) break;     /* nothing left to count */
+        const Shape &sh = S.shapes[(size_t) si.shape];
+        const bool smooth = sh.bsdf == SMS_BSDF_DIFFUSE;
+        if (first_diffuse < 0 && smooth) first_diffuse = depth;
+        const bool receiver = (sh.flags & SMS_FLAG_CAUSTIC_RECEIVER) != 0;
+        const bool from_receiver = si_prev.valid && (S.shapes[(size_t) si_prev.shape].flags & SMS_FLAG_CAUSTIC_RECEIVER);
+        const bool glint_case = depth == 1 && !pc.disable_reflected_emitters;
+        bool on_caster;
+        float caustic_eta = 1.f;
+        if (sms) {
+            on_caster = (sh.flags & SMS_FLAG_CAUSTIC_CASTER_SS) != 0;
+            /* SMS at a caustic receiver (path_sms_ss.cpp:118-124) */
+            if (receiver && !on_caster && (pc.max_depth < 0 || depth + 1 < pc.max_depth) &&
+                (!only || depth == first_diffuse)) {
+                PathSms ps;
+                sms_query_from_si(si, &ps.q);
+                ps.thr = thr;
+                o.sms.push_back(ps);
+            }
+        } else {
+            on_caster = (sh.flags & (SMS_FLAG_CAUSTIC_CASTER_SS | SMS_FLAG_CAUSTIC_CASTER_MS)) != 0;
+            /* caustic_eta from BSDF::ior (path_filtered_ss.cpp:85-87): a dielectric's (eta, 0);
+               conductors (eta, k) and the default (0, 1) give 1 unless k = 0 */
+            if (sh.bsdf == SMS_BSDF_DIELECTRIC) caustic_eta = sh.eta_r;
+            else if (sh.bsdf == SMS_BSDF_CONDUCTOR && sh.k == 0.f) caustic_eta = sh.eta_r;
+        }
+        /* emitter sampling (path_sms_ss.cpp:142-161, path_filtered_ss.cpp:107-142) */
+        bool nee;
+        bool valid_connection = false;
+        if (sms) nee = (smooth && !on_caster) || glint_case;
+        else {
+            valid_connection = on_caster && (from_receiver || glint_case);
+            nee = smooth && (!on_caster || valid_connection);
+        }
+        if (nee) {
+            V2 s2; s2.x = pcg32_float(&r); s2.y = pcg32_float(&r);
+            V3 ew;
+            DirSample ds;
+            V3 wo = v3(0.f, 0.f, 0.f), val = v3(0.f, 0.f, 0.f);
+            /* a delta BSDF evaluates to zero whatever the visibility: no shadow ray */
+            ds = scene_sample_emitter_direction(S, si.p, s2, smooth, &ew);
+            if (ds.pdf != 0.f) {
+                wo = v3(dot(ds.d, si.sh_s), dot(ds.d, si.sh_t), dot(ds.d, si.sh_n));
+                val = bsdf_eval(sh, si.wi, wo);
+                float bpdf = bsdf_pdf(sh, si.wi, wo);
+                float mis = ds.delta ? (float) 1.f : mis_power(ds.pdf, bpdf);
+                if (!sms && valid_connection && depth > 1) {
+                    bool refraction = wo.z * si.wi.z < 0.f, reflection = !refraction;
+                    if ((caustic_eta == 1.f && !reflection) || (caustic_eta != 1.f && !refraction)) mis = 0.f;
+                }
+                if (!only) L = L + mul3(mul3(thr * mis, val), ew);
+            }
+        }
+        /* BSDF sampling */
+        float s1 = pcg32_float(&r);
+        V2 s2; s2.x = pcg32_float(&r); s2.y = pcg32_float(&r);
+        BsdfSample bs = bsdf_sample(sh, si.wi, s1, s2);
+        thr = mul3(thr, bs.weight);
+        eta = eta * bs.eta;
+        if (thr.x == 0.f && thr.y == 0.f && thr.z == 0.f) break;
+        ray = spawn_ray(si.p, to_world(si.sh_s, si.sh_t, si.sh_n, bs.wo));
+        SI hit = ray_intersect(S, ray);
+        int e = hit.valid ? S.shapes[(size_t) hit.shape].emitter : -1;
+        if (e >= 0) {
+            if (sms) {
+                if (!on_caster || glint_case) {
+                    if (!only) {
+                        V3 ev = emitter_eval_hit(S, e, hit);
+                        float epdf = bs.delta ? (float) 0.f : pdf_emitter_hit(S, e, hit, si.p);
+                        float mis = mis_power(bs.pdf, epdf);
+                        L = L + mul3(thr * mis, ev);
+                    }
+                } else if (S.cfg.mnee_init && !pc.biased_mnee && from_receiver && S.emitters[(size_t) e].ss) {
+                    /* MNEE completion (path_sms_ss.cpp:196-241): the walk from the receiver */
+                    PathMnee m;
+                    memset(&m.w, 0, sizeof(m.w));
+                    m.w.sample[0] = pcg32_float(&r); m.w.sample[1] = pcg32_float(&r);
+                    if (S.cfg.twostage) { m.w.lean_sample[0] = pcg32_float(&r); m.w.lean_sample[1] = pcg32_float(&r); }
+                    if (!only || depth - 1 == first_diffuse) {
+                        for (int a = 0; a < 3; ++a) m.w.si_p[a] = (&si_prev.p.x)[a];
+                        m.w.shape = si.shape;
+                        V3 ed = normalize(hit.p - si_prev.p);
+                        for (int a = 0; a < 3; ++a) {
+                            m.w.ei.p[a] = (&hit.p.x)[a]; m.w.ei.n[a] = (&hit.n.x)[a]; m.w.ei.d[a] = (&ed.x)[a];
+                        }
+                        m.w.ei.kind = SMS_EI_AREA; m.w.ei.emitter = e;
+                        m.w.n_offset[2] = 1.f;
+                        m.value = mul3(thr, emitter_eval_hit(S, e, hit));
+                        m.dir_pt = normalize(si.p - si_prev.p);
+                        m.ref_p = si_prev.p;
+                        o.mnee.push_back(m);
+                    }
+                }
+            } else {
+                bool r_t_valid = (caustic_eta == 1.f && bs.reflection) || (caustic_eta != 1.f && bs.transmission);
+                bool vc = on_caster && ((from_receiver && r_t_valid) || glint_case);
+                if (!on_caster || vc) {
+                    bool take = !only || (on_caster && from_receiver && r_t_valid && depth - 1 == first_diffuse);
+                    if (take) {
+                        V3 ev = emitter_eval_hit(S, e, hit);
+                        float epdf = bs.delta ? (float) 0.f : pdf_emitter_hit(S, e, hit, si.p);
+                        float mis = mis_power(bs.pdf, epdf);
+                        L = L + mul3(thr * mis, ev);
+                    }
+                }
+            }
+        }
+        si_prev = si;
+        si = hit;
+    }
+    o.L = L;
+}
+
+/* estimator seed of batch c of a pass (SMS_PATH_CHUNK samples each) */
+static uint64_t path_chunk_seed(uint64_t seed, uint64_t c) { return mix64(seed ^ mix64(c + 0x632BE59BD9B4E019ULL)); }
+
+/* ImageBlock::put (src/librender/imageblock.cpp:80-165) of one sample into a w x h film with a
+   border of 2 pixels (radius > 1: the Gaussian of `tab`, rfilter.h:62-66) or the box branch
+   (tab = nullptr), gathered per target pixel: sample j of the pass adds weight * rgb and
+   weight to pixel (x, y) if (x, y) is one of its taps */
+struct FilterTab { float values[32]; float radius, scale; int n; };
+static void filter_table(float stddev, FilterTab *t) {
+    /* GaussianFilter (src/rfilters/gaussian.cpp:34-48) and init_discretization (src/libcore/rfilter.cpp:9-20) */
+    float radius = 4.f * stddev, alpha = -1.f / (2.f * stddev * stddev), bias = expf(alpha * (radius * radius));
+    for (int i = 0; i < 31; ++i) {
+        float x = (radius * (float) i) / 31.f;
+        t->values[i] = std::max(0.f, expf(alpha * (x * x)) - bias);
+    }
+    t->values[31] = 0.f;
+    t->radius = radius; t->scale = 31.f / radius;
+    t->n = (int) ceilf((radius - 2.f * RAY_EPS_F) * 2.f);
+}
+static inline float filter_disc(const FilterTab &t, float x) {
+    int index = (int) fabsf(x * t.scale);
+    return t.values[index < 31 ? index : 31];
+}
+/* the weight of sample position P (film pixels) at image pixel (x, y): 0 when not a tap */
+static float splat_weight(const FilterTab *t, int w, int h, float Px, float Py, int x, int y) {
+    const float bx = Px + 1.5f, by = Py + 1.5f;          /* pos - (offset - border + .5) */
+    if (!t) {
+        int lx = (int) ceilf(bx - .5f) - 2, ly = (int) ceilf(by - .5f) - 2;
+        return (lx == x && ly == y) ? 1.f : 0.f;
+    }
+    int lox = std::max((int) ceilf(bx - t->radius), 0), loy = std::max((int) ceilf(by - t->radius), 0);
+    int hix = std::min((int) floorf(bx + t->radius), w + 3), hiy = std::min((int) floorf(by + t->radius), h + 3);
+    int xr = x + 2 - lox, yr = y + 2 - loy;
+    if (xr < 0 || yr < 0 || xr >= t->n || yr >= t->n || x + 2 > hix || y + 2 > hiy) return 0.f;
+    float wx = filter_disc(*t, ((float) lox - bx) + (float) xr);
+    float wy = filter_disc(*t, ((float) loy - by) + (float) yr);
+    return wy * wx;
+}
+static bool sample_valid(const float *v) {
+    for (int k = 0; k < 3; ++k) if (!(isfinite(v[k]) && v[k] >= -1e-5f)) return false;
+    return true;
+}
+
 } // namespace
 
 /* ================================================================ C entry points */
@@ -2922,6 +3336,95 @@ int oracle_ss_make_queries(void *h, int spp, uint64_t seed, sms_ss_query *q, flo
     if (!S.has_cam) return -1;
     int64_t n = (int64_t) S.cam.w * S.cam.h * spp;
     parallel_for(n, threads, [&](int64_t i) { make_ss_query(S, i, spp, seed, q + i, w + i); });
+    return 0;
+}
+
+/* path integrators (path_sms_ss / path_filtered_ss): per-sample radiance and film positions of a
+   pass of n = w*h*spp samples, the SMS estimates in batches of SMS_PATH_CHUNK samples */
+int oracle_path_samples(void *h, const sms_path_config *pc, int spp, uint64_t seed, float *rgb, float *pos,
+                        sms_stats *stats, int threads) {
+    const Scene &S = *(Scene *) h;
+    if (!S.has_cam || spp <= 0) return -1;
+    const int64_t n = (int64_t) S.cam.w * S.cam.h * spp;
+    Stats tot; memset(&tot, 0, sizeof(tot));
+    for (int64_t c0 = 0; c0 < n; c0 += SMS_PATH_CHUNK) {
+        const int64_t c1 = std::min<int64_t>(n, c0 + SMS_PATH_CHUNK);
+        std::vector<PathSample> ps((size_t) (c1 - c0));
+        parallel_for(c1 - c0, threads, [&](int64_t i) { path_sample(S, *pc, seed, c0 + i, spp, ps[(size_t) i]); });
+        std::vector<sms_ss_query> Q;
+        std::vector<const PathMnee *> M;
+        for (size_t i = 0; i < ps.size(); ++i) {
+            for (const PathSms &x : ps[i].sms) Q.push_back(x.q);
+            for (const PathMnee &m : ps[i].mnee) M.push_back(&m);
+        }
+        const uint64_t cs = path_chunk_seed(seed, (uint64_t) (c0 / SMS_PATH_CHUNK));
+        std::vector<V3> est(Q.size());
+        std::vector<Stats> per(Q.size());
+        parallel_for((int64_t) Q.size(), threads, [&](int64_t k) {
+            memset(&per[(size_t) k], 0, sizeof(Stats));
+            est[(size_t) k] = estimate_ss(S, Q[(size_t) k], cs, (uint64_t) k, per[(size_t) k]);
+        });
+        for (const Stats &x : per) {
+            tot.walks += x.walks; tot.conv += x.conv; tot.iters += x.iters; tot.bcalls += x.bcalls;
+            tot.btrials += x.btrials; if (x.bmax > tot.bmax) tot.bmax = x.bmax;
+        }
+        std::vector<char> add(M.size(), 0);
+        parallel_for((int64_t) M.size(), threads, [&](int64_t k) {
+            const PathMnee &m = *M[(size_t) k];
+            const sms_ss_walk_in &w = m.w;
+            PathOut po = sample_path_ss(S, w.shape, v3(w.si_p[0], w.si_p[1], w.si_p[2]), ei_from_c(w.ei),
+                                        v2(w.sample[0], w.sample[1]), v3(w.n_offset[0], w.n_offset[1], w.n_offset[2]),
+                                        v3(w.p_start[0], w.p_start[1], w.p_start[2]),
+                                        v2(w.lean_sample[0], w.lean_sample[1]));
+            bool ok = po.ok;
+            if (ok) {
+                V3 dm = normalize(po.si_final.p - m.ref_p);
+                if (fabsf(dot(m.dir_pt, dm) - 1.f) >= S.cfg.uniqueness_threshold) ok = false;
+            }
+            add[(size_t) k] = !ok;
+        });
+        size_t kq = 0, km = 0;
+        for (size_t i = 0; i < ps.size(); ++i) {
+            V3 L = ps[i].L;
+            for (const PathSms &x : ps[i].sms) L = L + mul3(x.thr, est[kq++]);
+            for (const PathMnee &m : ps[i].mnee) { if (add[km]) L = L + m.value; ++km; }
+            const int64_t j = c0 + (int64_t) i;
+            rgb[3 * j] = L.x; rgb[3 * j + 1] = L.y; rgb[3 * j + 2] = L.z;
+            pos[2 * j] = ps[i].pos[0]; pos[2 * j + 1] = ps[i].pos[1];
+        }
+    }
+    stats_out(tot, stats);
+    return 0;
+}
+
+/* film reconstruction of a pass (sms_film_splat): image += filter-weighted sums, weight += weights */
+int oracle_film_splat(void *h, const float *rgb, const float *pos, int spp, float stddev, float *image, float *weight,
+                      int threads) {
+    const Scene &S = *(Scene *) h;
+    if (!S.has_cam || spp <= 0) return -1;
+    const int w = S.cam.w, hh = S.cam.h;
+    FilterTab tab;
+    const FilterTab *t = nullptr;
+    if (stddev > 0.f && 4.f * stddev > 1.f) { filter_table(stddev, &tab); t = &tab; }   /* radius <= 1: box branch */
+    parallel_for((int64_t) w * hh, threads, [&](int64_t p) {
+        const int x = (int) (p % w), y = (int) (p / w);
+        float acc[3] = {0.f, 0.f, 0.f}, ws = 0.f;
+        for (int ny = y - 2; ny <= y + 2; ++ny)
+            for (int nx = x - 2; nx <= x + 2; ++nx) {
+                if (nx < 0 || ny < 0 || nx >= w || ny >= hh) continue;
+                const int64_t base = ((int64_t) ny * w + nx) * spp;
+                for (int k = 0; k < spp; ++k) {
+                    const float *v = rgb + 3 * (base + k);
+                    if (!sample_valid(v)) continue;
+                    const float wt = splat_weight(t, w, hh, pos[2 * (base + k)], pos[2 * (base + k) + 1], x, y);
+                    if (wt == 0.f) continue;
+                    for (int a = 0; a < 3; ++a) acc[a] = acc[a] + v[a] * wt;
+                    ws = ws + wt;
+                }
+            }
+        for (int a = 0; a < 3; ++a) image[3 * p + a] = image[3 * p + a] + acc[a];
+        weight[p] = weight[p] + ws;
+    });
     return 0;
 }
 

@@ -67,6 +67,8 @@ struct sms_ctx {
     bool capturing = false;                     // launch sequence being captured: no event records
     // device address of the allocation-free estimator's scratch array on this context's device
     void *scratch = nullptr;
+    // path integrators' per-batch buffers (sms_path_host.hip)
+    struct PathBuffers *path = nullptr;
 };
 
 // seed slots: estimator, camera pass, the proposal's input / output queries
@@ -154,3 +156,7 @@ sms_status ms_enqueue(sms_ctx *ctx, const sms_ss_query *q, int64_t n, float *out
 sms_status ss_queries_enqueue(sms_ctx *ctx, int32_t spp, sms_ss_query *q, float *weight);
 sms_status film_enqueue(sms_ctx *ctx, const float *rgb, const float *weight, int64_t pixels, int32_t spp, float *image,
                         float *image_sq, float *image_n);
+// statistics of the last estimator call (waits for the stream)
+sms_status estimator_stats(sms_ctx *ctx, sms_stats *stats);
+// path integrators' buffers (sms_path_host.hip)
+void path_release(sms_ctx *ctx);

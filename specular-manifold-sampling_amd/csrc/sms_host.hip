@@ -581,6 +581,8 @@ __global__ void k_stats_fold(const unsigned long long *counters, unsigned long l
         for (int j = 0; j < 9; ++j) total[j] = j == 5 ? (v[j] > total[j] ? v[j] : total[j]) : total[j] + v[j];
 }
 
+sms_status estimator_stats(sms_ctx *ctx, sms_stats *stats);
+
 sms_status frame_begin(sms_ctx *ctx, uint64_t seed) {
     hipLaunchKernelGGL(k_frame_begin, dim3((N_COUNTERS + 255) / 256), dim3(256), 0, ctx->stream, ctx->counters,
                        N_COUNTERS, ctx->seeds, seed);
@@ -735,6 +737,13 @@ sms_status read_stats(sms_ctx *ctx, sms_stats *stats) {
     return SMS_OK;
 }
 
+} // namespace
+
+// statistics of the last estimator call (waits for the stream)
+sms_status estimator_stats(sms_ctx *ctx, sms_stats *stats) { return read_stats(ctx, stats); }
+
+namespace {
+
 // Persistent kernels take their work from device counters, so blocks beyond the resident
 // limit simply start later and find the queue drained; the default is the occupancy limit,
 // at least 8 blocks per CU.
@@ -877,6 +886,7 @@ sms_status sms_ctx_destroy(sms_ctx *ctx) {
         if (ctx->res) (void) hipFree(ctx->res);
         if (ctx->seeds) (void) hipFree(ctx->seeds);
         proposal_release(ctx);
+        path_release(ctx);
         if (ctx->ev3) (void) hipEventDestroy(ctx->ev3);
         if (ctx->ev0) (void) hipEventDestroy(ctx->ev0);
         if (ctx->ev1) (void) hipEventDestroy(ctx->ev1);

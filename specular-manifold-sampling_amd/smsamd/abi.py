@@ -102,6 +102,26 @@ class SmsSchedule(C.Structure):
         return s
 
 
+class SmsPathConfig(C.Structure):
+    """sms_path_config: the path integrators' properties (path_sms_ss / path_filtered_ss)."""
+    _fields_ = [("integrator", C.c_int32), ("max_depth", C.c_int32), ("rr_depth", C.c_int32),
+                ("disable_reflected_emitters", C.c_int32), ("biased_mnee", C.c_int32),
+                ("caustics_only", C.c_int32), ("filter_stddev", C.c_float), ("pad", C.c_int32)]
+
+    @classmethod
+    def default(cls, **kw):
+        c = cls(integrator=PATH_SMS_SS, max_depth=-1, rr_depth=5, filter_stddev=0.5)
+        for k, v in kw.items():
+            if not hasattr(c, k):
+                raise ValueError(k)
+            setattr(c, k, v)
+        return c
+
+
+PATH_SMS_SS, PATH_FILTERED_SS = 0, 1
+PATH_MAX_DEPTH = 16
+PATH_CHUNK = 1 << 20
+
 F3 = (np.float32, 3)
 F2 = (np.float32, 2)
 EI_DTYPE = np.dtype([("p", F3), ("n", F3), ("d", F3), ("weight", F3), ("pdf", np.float32),
@@ -131,7 +151,7 @@ GLINT_QUERY = np.dtype([("sensor_p", F3), ("p", F3), ("n", F3), ("sh_n", F3), ("
 # sizes the C header implies (checked against the compiled library in the tests)
 C_SIZES = {"sms_emitter_interaction": 60, "sms_ss_walk_in": 116, "sms_ss_walk_out": 52,
            "sms_glint_walk_in": 128, "sms_glint_walk_out": 24, "sms_ss_query": 80,
-           "sms_glint_query": 140, "sms_config": 48, "sms_stats": 64, "sms_ms_walk_in": 116,
+           "sms_glint_query": 140, "sms_config": 48, "sms_path_config": 32, "sms_stats": 64, "sms_ms_walk_in": 116,
            "sms_ms_walk_out": 80}
 DTYPES = {"sms_emitter_interaction": EI_DTYPE, "sms_ss_walk_in": SS_WALK_IN,
           "sms_ms_walk_in": MS_WALK_IN, "sms_ms_walk_out": MS_WALK_OUT,
@@ -160,7 +180,8 @@ EXPORTS = ["sms_config_default", "sms_ctx_create", "sms_ctx_destroy", "sms_ctx_s
            "sms_ctx_set_training_capture", "sms_ctx_training_samples", "sms_ctx_clear_training",
            "sms_ss_proposal_inputs", "sms_ss_proposal_outputs", "sms_ss_make_queries",
            "sms_film_accumulate", "sms_ss_autotune", "sms_ms_autotune", "sms_scene_footprint",
-           "sms_ss_estimate_batch_noalloc", "sms_glint_proposal_inputs", "sms_glint_proposal_outputs"]
+           "sms_ss_estimate_batch_noalloc", "sms_glint_proposal_inputs", "sms_glint_proposal_outputs",
+           "sms_path_config_default", "sms_path_samples", "sms_film_splat"]
 
 # learned seed proposal (include/sms.h)
 MLP_INPUTS = 9
@@ -230,6 +251,9 @@ def load_library(path=LIB_PATH):
     lib.sms_glint_proposal_outputs.argtypes = [vp, vp, i64, C.c_uint64, vp]
     lib.sms_ss_make_queries.argtypes = [vp, C.c_int32, C.c_uint64, vp, vp]
     lib.sms_film_accumulate.argtypes = [vp, vp, vp, i64, C.c_int32, vp, vp, vp]
+    lib.sms_path_config_default.argtypes = [C.POINTER(SmsPathConfig)]
+    lib.sms_path_samples.argtypes = [vp, C.POINTER(SmsPathConfig), C.c_int32, C.c_uint64, vp, vp, C.POINTER(SmsStats)]
+    lib.sms_film_splat.argtypes = [vp, vp, vp, C.c_int32, C.c_float, vp, vp]
     for name in EXPORTS:
         fn = getattr(lib, name)
         if fn.restype is C.c_int and name not in ("sms_ctx_last_error", "sms_config_default"):

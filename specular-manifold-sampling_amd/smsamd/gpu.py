@@ -189,6 +189,56 @@ class Context:
                                          imgsqbuf.ptr if imgsqbuf is not None else None,
                                          imgnbuf.ptr if imgnbuf is not None else None), self.h, "film_accumulate")
 
+    # ---------------------------------------------------------------- path integrators
+    def path_samples_device(self, pc, spp, seed, rgbbuf, posbuf):
+        """One pass of the path integrator pc into device buffers (n x 3 radiance, n x 2 film
+        positions, n = w*h*spp); returns the pass's SMS statistics."""
+        st = abi.SmsStats()
+        _check(lib().sms_path_samples(self.h, C.byref(pc), spp, seed, rgbbuf.ptr, posbuf.ptr, C.byref(st)), self.h,
+               "path_samples")
+        return st.as_dict()
+
+    def path_samples(self, pc, spp, seed):
+        cam = self.scene.camera
+        n = cam["width"] * cam["height"] * spp
+        rgb = np.zeros((n, 3), np.float32)
+        pos = np.zeros((n, 2), np.float32)
+        bufs = [DeviceBuffer(self, rgb.nbytes), DeviceBuffer(self, pos.nbytes)]
+        try:
+            st = self.path_samples_device(pc, spp, seed, bufs[0], bufs[1])
+            bufs[0].download(rgb)
+            bufs[1].download(pos)
+        finally:
+            for b in bufs:
+                b.free()
+        return rgb, pos, st
+
+    def film_splat_device(self, rgbbuf, posbuf, spp, stddev, imgbuf, wbuf):
+        _check(lib().sms_film_splat(self.h, rgbbuf.ptr, posbuf.ptr, spp, stddev, imgbuf.ptr, wbuf.ptr), self.h,
+               "film_splat")
+
+    def film_splat(self, rgb, pos, spp, stddev):
+        cam = self.scene.camera
+        px = cam["width"] * cam["height"]
+        image = np.zeros((px, 3), np.float32)
+        weight = np.zeros(px, np.float32)
+        rgb = np.ascontiguousarray(rgb, np.float32)
+        pos = np.ascontiguousarray(pos, np.float32)
+        bufs = [DeviceBuffer(self, rgb.nbytes), DeviceBuffer(self, pos.nbytes), DeviceBuffer(self, image.nbytes),
+                DeviceBuffer(self, weight.nbytes)]
+        try:
+            bufs[0].upload(rgb)
+            bufs[1].upload(pos)
+            bufs[2].upload(image)
+            bufs[3].upload(weight)
+            self.film_splat_device(bufs[0], bufs[1], spp, stddev, bufs[2], bufs[3])
+            bufs[2].download(image)
+            bufs[3].download(weight)
+        finally:
+            for b in bufs:
+                b.free()
+        return image, weight
+
     def glint_queries_device(self, spp, seed, qbuf=None):
         """Camera pass of a glint frame into device memory (qbuf, or a new buffer): (buffer, n)."""
         cam = self.scene.camera

@@ -166,7 +166,7 @@ def mirror_wall(normalmap=None, nm_res=64, n_scratches=16, seed=1, tiles=1.0):
 
 
 # ==================================================================== sphere caustic
-def sphere_caustic(width=256, height=256, center_y=1.5):
+def sphere_caustic(width=256, height=256, center_y=1.5, light_radius=0.1, light_radiance=100.0):
     """Glass sphere (r=1, IOR 1.5) at (0,center_y,0) over a diffuse plane (albedo 0.5), sphere
     area light r=0.1 with radiance 100 at (0,5,0); camera at (0,3,6) looking at (0,0.5,0), 45 deg.
 
@@ -180,15 +180,15 @@ def sphere_caustic(width=256, height=256, center_y=1.5):
     sc.add_sphere((0.0, center_y, 0.0), 1.0, abi.BSDF_DIELECTRIC, flags=abi.FLAG_CAUSTIC_CASTER_SS, eta=1.5)
     sc.add_rectangle(plane_to_world((0, 0, 0), 10.0), abi.BSDF_DIFFUSE,
                      flags=abi.FLAG_CAUSTIC_RECEIVER, reflectance=(0.5, 0.5, 0.5))
-    light = sc.add_sphere((0.0, 5.0, 0.0), 0.1, abi.BSDF_DIFFUSE, reflectance=(0.0, 0.0, 0.0))
-    sc.add_area_light(light, (100.0, 100.0, 100.0))
+    light = sc.add_sphere((0.0, 5.0, 0.0), light_radius, abi.BSDF_DIFFUSE, reflectance=(0.0, 0.0, 0.0))
+    sc.add_area_light(light, (light_radiance,) * 3)
     sc.set_camera((0, 3, 6), (0, 0.5, 0), (0, 1, 0), 45.0, width, height)
     return sc
 
 
-def sphere_dome(width=256, height=256):
+def sphere_dome(width=256, height=256, light_radius=0.1, light_radiance=100.0):
     """The sphere-caustic scene with the sphere centre on the receiver plane (a glass dome)."""
-    return sphere_caustic(width, height, center_y=0.0)
+    return sphere_caustic(width, height, center_y=0.0, light_radius=light_radius, light_radiance=light_radiance)
 
 
 # ==================================================================== water pool
@@ -234,7 +234,7 @@ def gerstner_grid(n=512, extent=1.0, n_waves=64, seed=3, amp_range=(1e-3, 2e-2),
             uv.reshape(-1, 2).astype(np.float32), faces.astype(np.uint32))
 
 
-def pool(width=1024, height=1024, n=512, n_waves=64, seed=3):
+def pool(width=1024, height=1024, n=512, n_waves=64, seed=3, light_radius=0.05, light_radiance=2000.0):
     """Water surface (IOR 1.33, caustic caster) over a diffuse floor 1 unit below (receiver),
     sphere light of radius 0.05 above; single-scatter refractive caustics on the floor."""
     sc = Scene()
@@ -242,8 +242,8 @@ def pool(width=1024, height=1024, n=512, n_waves=64, seed=3):
     sc.add_mesh(P, F, normals=N, texcoords=UV, bsdf=abi.BSDF_DIELECTRIC, flags=abi.FLAG_CAUSTIC_CASTER_SS, eta=1.33)
     sc.add_rectangle(plane_to_world((0, -1.0, 0), 1.0), abi.BSDF_DIFFUSE, flags=abi.FLAG_CAUSTIC_RECEIVER,
                      reflectance=(0.5, 0.5, 0.5))
-    light = sc.add_sphere((0.25, 2.5, 0.1), 0.05, abi.BSDF_DIFFUSE, reflectance=(0.0, 0.0, 0.0))
-    sc.add_area_light(light, (2000.0, 2000.0, 2000.0))
+    light = sc.add_sphere((0.25, 2.5, 0.1), light_radius, abi.BSDF_DIFFUSE, reflectance=(0.0, 0.0, 0.0))
+    sc.add_area_light(light, (light_radiance,) * 3)
     sc.set_camera((0.0, 2.2, 2.6), (0.0, -1.0, 0.0), (0, 1, 0), 40.0, width, height)
     return sc
 

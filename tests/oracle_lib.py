@@ -76,6 +76,9 @@ def load(path=ORACLE_SO):
     lib.oracle_glint_estimate_batch_prop.argtypes = [vp, vp, i64, C.c_uint64, vp, C.c_float, vp,
                                                      C.POINTER(abi.SmsStats), C.c_int]
     lib.oracle_glint_prop_inputs.argtypes = [vp, vp, i64, C.c_uint64, vp]
+    lib.oracle_path_samples.argtypes = [vp, C.POINTER(abi.SmsPathConfig), C.c_int, C.c_uint64, vp, vp,
+                                        C.POINTER(abi.SmsStats), C.c_int]
+    lib.oracle_film_splat.argtypes = [vp, vp, vp, C.c_int, C.c_float, vp, vp, C.c_int]
     return lib
 
 
@@ -198,6 +201,30 @@ class Oracle:
         x = np.zeros((len(q), 9), np.float32)
         self.lib.oracle_glint_prop_inputs(self.h, q.ctypes.data, len(q), seed, x.ctypes.data)
         return x
+
+    def path_samples(self, pc, spp, seed):
+        """(rgb (n, 3), film positions (n, 2), stats) of one pass of the path integrator pc."""
+        cam = self.scene.camera
+        n = cam["width"] * cam["height"] * spp
+        rgb = np.zeros((n, 3), np.float32)
+        pos = np.zeros((n, 2), np.float32)
+        st = abi.SmsStats()
+        rc = self.lib.oracle_path_samples(self.h, C.byref(pc), spp, seed, rgb.ctypes.data, pos.ctypes.data,
+                                          C.byref(st), self.threads)
+        assert rc == 0, rc
+        return rgb, pos, st.as_dict()
+
+    def film_splat(self, rgb, pos, spp, stddev, image=None, weight=None):
+        cam = self.scene.camera
+        px = cam["width"] * cam["height"]
+        image = np.zeros((px, 3), np.float32) if image is None else image
+        weight = np.zeros(px, np.float32) if weight is None else weight
+        rgb = np.ascontiguousarray(rgb, np.float32)
+        pos = np.ascontiguousarray(pos, np.float32)
+        rc = self.lib.oracle_film_splat(self.h, rgb.ctypes.data, pos.ctypes.data, spp, stddev, image.ctypes.data,
+                                        weight.ctypes.data, self.threads)
+        assert rc == 0, rc
+        return image, weight
 
     def scene_box(self):
         c = np.zeros(3, np.float32)

@@ -84,7 +84,7 @@ def test_struct_layouts_match_c():
 CTYPES_STRUCTS = {"sms_config": abi.SmsConfig, "sms_stats": abi.SmsStats, "sms_schedule": abi.SmsSchedule,
                   "sms_shape_desc": abi.SmsShapeDesc, "sms_emitter_desc": abi.SmsEmitterDesc,
                   "sms_normalmap_desc": abi.SmsNormalmapDesc, "sms_camera_desc": abi.SmsCameraDesc,
-                  "sms_scene_desc": abi.SmsSceneDesc}
+                  "sms_scene_desc": abi.SmsSceneDesc, "sms_path_config": abi.SmsPathConfig}
 
 
 def test_ctypes_structs_match_c():
@@ -106,3 +106,14 @@ def test_ctypes_structs_match_c():
         assert int(got[n]) == C.sizeof(cls), n
         for f, _ in cls._fields_:
             assert int(got[f"{n}.{f}"]) == getattr(cls, f).offset, (n, f)
+
+
+def test_library_defines_its_kernels():
+    """Every kernel and device-side symbol the host code launches is defined in the library
+    (a kernel declared in one namespace and defined in another links as an undefined symbol)."""
+    if not os.path.exists(abi.LIB_PATH):
+        pytest.skip("library not built")
+    out = subprocess.run(["nm", "-D", "--undefined-only", abi.LIB_PATH], check=True, capture_output=True,
+                         text=True).stdout
+    missing = [line.split()[-1] for line in out.splitlines() if "smsdev" in line]
+    assert not missing, missing

@@ -1,0 +1,107 @@
+"""Path integrators and film on the GPU, through the C ABI: per-sample radiance, film positions
+and SMS statistics bit-equal to the oracle's (path_sms_ss and path_filtered_ss, caustic term and
+whole image, unbiased and MNEE configurations, passes of more than one estimator batch); the
+film's weighted sums bit-equal; refused scenes and arguments."""
+import numpy as np
+import pytest
+
+from smsamd import abi, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    from smsamd import gpu as g
+    g.lib()
+    return g
+
+
+@pytest.fixture(scope="module")
+def oracle_mod():
+    import oracle_lib
+    oracle_lib.build()
+    return oracle_lib
+
+
+def _compare(ctx, orc, pc, spp, seed):
+    rg, pg, sg = ctx.path_samples(pc, spp, seed)
+    ro, po, so = orc.path_samples(pc, spp, seed)
+    print(f"integrator {pc.integrator} caustics_only {pc.caustics_only} spp {spp}: mean {ro.mean():.6g}, "
+          f"nonzero {(ro[:, 0] != 0).mean():.3f}, stats {so}", flush=True)
+    assert np.array_equal(pg, po)
+    assert np.array_equal(rg, ro, equal_nan=True)
+    # the estimator's counters (overflow and tail counts describe the GPU's schedule)
+    for k in ("walks", "walks_converged", "newton_iterations", "bernoulli_calls", "bernoulli_trials", "bernoulli_max"):
+        assert sg[k] == so[k], k
+    return ro
+
+
+@pytest.mark.parametrize("integrator", [abi.PATH_SMS_SS, abi.PATH_FILTERED_SS])
+@pytest.mark.parametrize("caustics_only", [0, 1])
+def test_path_samples_dome(gpu, oracle_mod, integrator, caustics_only):
+    sc = scenes.sphere_dome(32, 24, light_radius=0.5, light_radiance=4.0)
+    cfg = abi.SmsConfig.default(max_trials=100000)
+    ctx = gpu.Context(sc, cfg)
+    orc = oracle_mod.Oracle(sc, cfg)
+    for max_depth in (6, -1):
+        pc = abi.SmsPathConfig.default(integrator=integrator, max_depth=max_depth, caustics_only=caustics_only)
+        r = _compare(ctx, orc, pc, 16, 3)
+        assert (r[:, 0] > 0).mean() > (0.0 if caustics_only else 0.5)
+    ctx.close()
+    orc.close()
+
+
+def test_path_samples_pool_mnee_and_batches(gpu, oracle_mod):
+    """MNEE with its completions (mnee_init, biased, one trial, path_sms_ss.cpp:196-241), biased
+    MNEE, plain SMS; and a pass of 1.5 estimator batches."""
+    sc = scenes.pool(48, 40, n=64, light_radius=0.3, light_radiance=60.0)
+    cfg_mnee = abi.SmsConfig.default(mnee_init=1, biased=1, max_trials=1, halfvector_constraints=1)
+    ctx = gpu.Context(sc, cfg_mnee)
+    orc = oracle_mod.Oracle(sc, cfg_mnee)
+    for biased_mnee in (0, 1):
+        _compare(ctx, orc, abi.SmsPathConfig.default(max_depth=7, biased_mnee=biased_mnee), 8, 5)
+    cfg = abi.SmsConfig.default(max_trials=100000, uniqueness_threshold=1e-5, solver_threshold=1e-5)
+    ctx.set_config(cfg)
+    orc.set_config(cfg)
+    _compare(ctx, orc, abi.SmsPathConfig.default(max_depth=7), 8, 6)
+    # 48 x 40 x 820 samples = 1.5 batches of SMS_PATH_CHUNK: the second batch's estimator seed
+    spp = (3 * abi.PATH_CHUNK // 2) // (48 * 40) + 1
+    _compare(ctx, orc, abi.SmsPathConfig.default(max_depth=7, caustics_only=1), spp, 7)
+    ctx.close()
+    orc.close()
+
+
+@pytest.mark.parametrize("stddev", [0.5, 0.3, 0.2, 0.0])
+def test_film_splat(gpu, oracle_mod, stddev):
+    sc = scenes.sphere_dome(40, 30, light_radius=0.5, light_radiance=4.0)
+    cfg = abi.SmsConfig.default(max_trials=100000)
+    ctx = gpu.Context(sc, cfg)
+    orc = oracle_mod.Oracle(sc, cfg)
+    pc = abi.SmsPathConfig.default(max_depth=6)
+    rgb, pos, _ = orc.path_samples(pc, 8, 9)
+    rgb[5] = (np.nan, 0.0, 0.0)                    # dropped whole
+    rgb[77] = (-1.0, 0.0, 0.0)
+    ig, wg = ctx.film_splat(rgb, pos, 8, stddev)
+    io, wo = orc.film_splat(rgb, pos, 8, stddev)
+    assert np.array_equal(wg, wo)
+    assert np.array_equal(ig, io)
+    assert (wo > 0).all() and np.isfinite(io).all()
+    ctx.close()
+    orc.close()
+
+
+def test_refused(gpu):
+    ctx = gpu.Context(scenes.glint_plane(16, 16, nm_res=64, n_scratches=4), abi.SmsConfig.default())
+    with pytest.raises(gpu.SmsError, match="normal-mapped"):
+        ctx.path_samples(abi.SmsPathConfig.default(), 1, 1)
+    ctx.close()
+    sc = scenes.sphere_dome(16, 16)
+    ctx = gpu.Context(sc, abi.SmsConfig.default())
+    for bad in (dict(max_depth=17), dict(max_depth=-2), dict(integrator=7)):
+        with pytest.raises(gpu.SmsError):
+            ctx.path_samples(abi.SmsPathConfig.default(**bad), 1, 1)
+    rgb, pos, _ = ctx.path_samples(abi.SmsPathConfig.default(), 1, 1)
+    with pytest.raises(gpu.SmsError, match="radius"):
+        ctx.film_splat(rgb, pos, 1, 0.6)
+    ctx.close()
