@@ -31,6 +31,8 @@ network is trained online between its passes, every pass counts in the image),
 `ratio_amortized` compares with the trained network (training done before the frame).
 `ms_slab_equal_time_rmse` is the same experiment on the multi-scatter slab, and
 `glint_equal_time_rmse` on this glint frame with the learned proposal over the pixel footprint.
+`image_parity` renders the dome and the pool with SMS (path_sms_ss) and with the filtered path
+tracer (path_filtered_ss) and reports their relative RMSE, caustic term and whole image.
 """
 import argparse
 import json
@@ -119,6 +121,14 @@ def parse():
     ap.add_argument("--glint-rmse-spp", type=int, default=512, help="uniform-seed samples per pixel of the glint budget")
     ap.add_argument("--glint-ref-spp", type=int, default=8192, help="glint reference samples per pixel")
     ap.add_argument("--glint-reps", type=int, default=4, help="frames per method of the glint experiment")
+    ap.add_argument("--parity-only", action="store_true",
+                    help="run only the image parity of SMS against the filtered path tracer")
+    ap.add_argument("--no-parity", action="store_true", help="skip the image parity experiment")
+    ap.add_argument("--parity-res", type=int, default=128, help="image width = height of the parity renders")
+    ap.add_argument("--parity-sms-seconds", type=float, default=5.0,
+                    help="time per scene and term of the SMS renders of the parity experiment")
+    ap.add_argument("--parity-ref-seconds", type=float, default=15.0,
+                    help="time per scene and term of the filtered path tracer's reference")
     ap.add_argument("--mlp-only", action="store_true", help="run only the proposal network's training / inference timing")
     ap.add_argument("--no-mlp", action="store_true", help="skip the proposal network's timing")
     ap.add_argument("--mlp-steps", type=int, default=2000, help="timed training steps of the network leg")
@@ -173,6 +183,9 @@ def main():
         return
     if args.glint_rmse_only:
         print(json.dumps(equal_time_rmse(args, "glint")), flush=True)
+        return
+    if args.parity_only:
+        print(json.dumps(image_parity(args)), flush=True)
         return
     import numpy as np
     from smsamd import abi, gpu, scenes
@@ -266,6 +279,7 @@ def main():
     rmse = None if args.no_rmse else equal_time_rmse(args)
     ms_rmse = None if args.no_ms else equal_time_rmse(args, "slab")
     glint_rmse = None if args.no_glint_rmse else equal_time_rmse(args, "glint")
+    parity = None if args.no_parity else image_parity(args)
     result = {
         "metric": "converged walks/s at 1 GPU; equal-time RMSE learned vs uniform seeds",
         "value": conv_per_s,
@@ -294,6 +308,7 @@ def main():
         "equal_time_rmse": rmse,
         "ms_slab_equal_time_rmse": ms_rmse,
         "glint_equal_time_rmse": glint_rmse,
+        "image_parity": parity,
     }
     if cpu and cpu.get("value"):
         result["gpu_over_cpu"] = conv_per_s / cpu["value"]
@@ -690,6 +705,100 @@ def equal_time_rmse(args, case="pool"):
         first_runner.free()
     film.free()
     ctx.close()
+    return out
+
+
+def image_parity(args):
+    """SMS against the filtered path tracer (path_sms_ss vs path_filtered_ss, the reference's own
+    comparison pair): the caustic term -- the single-scatter caustic paths through the first
+    non-specular vertex -- and the whole image of the glass dome (C1 scene: light r 0.1) and the
+    water pool (C3 mesh: light r 0.05), parity_res^2 with the Gaussian film (stddev 0.5),
+    max_depth 6, unbiased SMS (cap 1e7, thresholds 1e-5).  SMS renders passes of 64 spp for
+    parity_sms_seconds, the path tracer passes of 256 for parity_ref_seconds.  Relative RMSE =
+    sqrt(mean over pixels of (SMS - PT)^2 minus both images' own variance of the mean, from
+    their pass-to-pass spread) / mean PT pixel, per pixel and over 4 x 4-pixel blocks; the raw
+    RMSE and both noise floors are reported beside it."""
+    import numpy as np
+    from smsamd import abi, gpu, scenes
+    res = args.parity_res
+    out = {"resolution": res, "film": "gaussian stddev 0.5", "max_depth": 6, "sms_seconds_per_term": args.parity_sms_seconds,
+           "reference_seconds_per_term": args.parity_ref_seconds,
+           "rel_rmse": "sqrt(mean((SMS - PT)^2) - Var[SMS mean] - Var[PT mean]) / mean(PT)"}
+    for name, sc in (("dome", scenes.sphere_dome(res, res)), ("pool", scenes.pool(res, res))):
+        cfg = abi.SmsConfig.default(max_trials=10_000_000, uniqueness_threshold=1e-5, solver_threshold=1e-5)
+        ctx = gpu.Context(sc, cfg)
+        px = res * res
+        n_max = px * 256
+        rgb, pos = gpu.DeviceBuffer(ctx, n_max * 12), gpu.DeviceBuffer(ctx, n_max * 8)
+        img, wt = gpu.DeviceBuffer(ctx, px * 12), gpu.DeviceBuffer(ctx, px * 4)
+        zero_i, zero_w = np.zeros((px, 3), np.float32), np.zeros(px, np.float32)
+        leg = {}
+
+        def render(pc, pass_spp, passes=None, seconds=None, seed0=1):
+            """per-pass developed images: mean image, variance of the mean, passes, seconds, stats"""
+            s1, s2, k, st_sum = np.zeros((px, 3)), np.zeros((px, 3)), 0, {}
+            t0 = time.perf_counter()
+            while (passes is not None and k < passes) or (seconds is not None and time.perf_counter() - t0 < seconds):
+                img.upload(zero_i)
+                wt.upload(zero_w)
+                st = ctx.path_samples_device(pc, pass_spp, seed0 + k, rgb, pos)
+                ctx.film_splat_device(rgb, pos, pass_spp, 0.5, img, wt)
+                a, b = np.zeros((px, 3), np.float32), np.zeros(px, np.float32)
+                img.download(a)
+                wt.download(b)
+                d = a.astype(np.float64) / np.maximum(b.astype(np.float64), 1e-30)[:, None]
+                s1 += d
+                s2 += d * d
+                k += 1
+                for key, v in st.items():
+                    st_sum[key] = max(st_sum.get(key, 0), v) if key == "bernoulli_max" else st_sum.get(key, 0) + v
+            secs = time.perf_counter() - t0
+            mean = s1 / k
+            var_mean = np.maximum(s2 / k - mean * mean, 0.0) / max(k - 1, 1)
+            return mean, var_mean, k, secs, st_sum
+
+        def blocks(a, b=4):
+            return a.reshape(res // b, b, res // b, b, 3).mean((1, 3)).reshape(-1, 3)
+
+        for term, only in (("caustic", 1), ("image", 0)):
+            sms = abi.SmsPathConfig.default(max_depth=6, caustics_only=only)
+            flt = abi.SmsPathConfig.default(integrator=abi.PATH_FILTERED_SS, max_depth=6, caustics_only=only)
+            render(sms, 64, passes=1, seed0=10**6)                  # first launches
+            ms, vs, ks, ts, st = render(sms, 64, seconds=args.parity_sms_seconds, seed0=1)
+            mf, vf, kf, tf, _ = render(flt, 256, seconds=args.parity_ref_seconds, seed0=10**7)
+            ref_mean = float(mf.mean())
+            rec = {"mean_sms": float(ms.mean()), "mean_pt": ref_mean, "mean_ratio": float(ms.mean()) / ref_mean, "sms_spp": ks * 64, "sms_s": round(ts, 3),
+                   "pt_spp": kf * 256, "pt_s": round(tf, 2), "sms_samples_per_s": ks * 64 * px / ts,
+                   "pt_samples_per_s": kf * 256 * px / tf, "sms_walks": st.get("walks", 0),
+                   "sms_walks_converged": st.get("walks_converged", 0)}
+            for tag, a, b, va, vb in (("pixel", ms, mf, vs, vf),
+                                      ("block4", blocks(ms), blocks(mf), blocks(vs) / 16, blocks(vf) / 16)):
+                d2 = ((a - b) ** 2).mean()
+                noise = (va + vb).mean()
+                rec[tag] = {"rel_rmse": float(np.sqrt(max(d2 - noise, 0.0)) / ref_mean),
+                            "rel_rmse_raw": float(np.sqrt(d2) / ref_mean),
+                            "rel_noise_sms": float(np.sqrt(va.mean()) / ref_mean),
+                            "rel_noise_pt": float(np.sqrt(vb.mean()) / ref_mean)}
+            leg[term] = rec
+            print(f"# parity {name} {term}: {json.dumps(rec)}", file=sys.stderr, flush=True)
+            if name == "pool" and only:
+                # the estimator's uniqueness test merges solutions whose directions differ by less
+                # than the threshold (near caustic folds): a tighter threshold against the same PT
+                ctx.set_config(abi.SmsConfig.default(max_trials=10_000_000, uniqueness_threshold=1e-6,
+                                                     solver_threshold=1e-5))
+                mu, vu, ku, tu, _ = render(sms, 64, seconds=args.parity_sms_seconds, seed0=1)
+                ctx.set_config(cfg)
+                d2 = ((blocks(mu) - blocks(mf)) ** 2).mean()
+                noise = (blocks(vu) / 16 + blocks(vf) / 16).mean()
+                leg["caustic_uniqueness_1e-6"] = {
+                    "mean_ratio": float(mu.mean()) / ref_mean, "sms_spp": ku * 64,
+                    "block4_rel_rmse": float(np.sqrt(max(d2 - noise, 0.0)) / ref_mean)}
+                print(f"# parity pool caustic, uniqueness 1e-6: {json.dumps(leg['caustic_uniqueness_1e-6'])}",
+                      file=sys.stderr, flush=True)
+        out[name] = leg
+        for b in (rgb, pos, img, wt):
+            b.free()
+        ctx.close()
     return out
 
 
