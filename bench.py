@@ -1,6 +1,6 @@
 """Benchmark: converged glint walks/s of unbiased glint SMS on one MI355X.
 
-Workload (one step = the SMS estimate of one frame, queries resident in HBM):
+Workload (one step = one whole frame: the camera footprint queries and their SMS estimate):
   1x1 mirror plane with a seed-generated 2048^2 RGB8 normal map (4096 line scratches of
   width 1-3 texels + Gaussian bumps, slope std 0.3) under a point light, 512x512 pixels at
   4 spp (1,048,576 footprint queries), pixel-footprint UV seeding, unbiased Bernoulli
@@ -10,19 +10,23 @@ Workload (one step = the SMS estimate of one frame, queries resident in HBM):
 
 Prints ONE JSON line.  `value` = converged walks/s (walks that returned success) over the
 whole timed region, in which each frame is one replay of the frame's captured HIP graph
-(sms_glint_graph_create) with that frame's seed; the same frames are first run as eager
-estimator calls, whose HIP events time each kernel and whose statistics the replays must
-reproduce.  `roofline` refers to the dominant kernel (k_glint_first: emitter sample,
-visibility, footprint target and the first Newton walk of every query) timed with HIP events
-on the stream it runs on; `cpu_baseline` times the CPU oracle on a bounded sample
-of the same queries on this host's cores.
+(sms_glint_frame_graph_create) with that frame's seed: the camera queries are generated in
+the kernels that read them, so the frame includes query generation and reads no query
+records.  The same frames are first run as two-step frames (sms_glint_make_queries +
+sms_glint_estimate_batch) and as eager whole-frame calls, whose HIP events time each kernel;
+the replays must reproduce their statistics.  `roofline` refers to the dominant kernel
+(k_glint_first: camera query, emitter sample, visibility, footprint target and the first
+Newton walk of every query) timed with HIP events on the stream it runs on; `cpu_baseline`
+times the CPU oracle on whole frames of the same workload on this host's cores.
+`query_buffer_frame` is the frame over camera queries made once and kept in HBM (140-byte
+records), as a caller holding its own queries runs it.
 
 Beside it: `dome_ss_biased` (the plain single-scatter caustic config: glass dome, 256^2 x 16
 spp, biased M = 64) and `pool_ss` (the water pool, 1024^2 x 8 spp pass, unbiased), each with an
 FP32 roofline of its walk kernel (FLOP per Newton iteration frozen by the op-counting oracle,
 tests/golden/flop_counts.json) and a CPU-oracle baseline on 16 threads and on one core.  The
-glint CPU baseline runs whole frames with the GPU steps' seeds and must reproduce their
-statistics (and the first frames' RGB, bit for bit), or the benchmark fails.
+glint CPU baseline runs whole frames with the GPU frames' seeds and must reproduce their
+queries, statistics and the first frames' RGB, bit for bit, or the benchmark fails.
 
 `equal_time_rmse` is the second half of the metric, on the water-pool config (see
 equal_time_rmse): RMSE of uniform-seed and learned-proposal renders given the same wall-clock
@@ -48,9 +52,12 @@ FP32_PEAK_TFLOPS = 157.3       # MI355X FP32 vector
 BYTES_PER_ITER = 12            # algorithmic: 4 RGB8 texels per Newton iteration (the map's source format)
 # HBM bytes per launch of k_glint_first on this workload (default sizes and schedule), from
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs scaled by a plain 256 MiB copy's counters
-# (FETCH_SIZE reads half the copied bytes: factor 2; WRITE_SIZE factor 1); see
-# profiles/glint_first_512x512x4spp_pmc.md.
-TRAFFIC_FIRST_PASS = {"fetch_kib": 115055.6, "write_kib": 13509.6, "fetch_factor": 2.0, "write_factor": 1.0}
+# (FETCH_SIZE reads half the copied bytes: factor 2; WRITE_SIZE factor 1): the whole frame, whose
+# camera queries are generated in the kernel, and the frame over a buffer of 140-byte query
+# records; see profiles/glint_whole_frame_512x512x4spp_pmc.md.
+TRAFFIC_FIRST_PASS = {"frame": {"fetch_kib": 16013.5, "write_kib": 13485.8},
+                      "query_buffer": {"fetch_kib": 87663.0, "write_kib": 13504.8}}
+FETCH_FACTOR, WRITE_FACTOR = 2.0, 1.0
 # FP32 FLOP per Newton iteration of each estimator (its whole work divided by its iterations),
 # frozen by the op-counting build of the CPU oracle (tests/golden/make_flop_counts.py)
 FLOP_COUNTS = json.load(open(os.path.join(ROOT, "tests", "golden", "flop_counts.json")))
@@ -121,6 +128,8 @@ def parse():
     ap.add_argument("--glint-rmse-spp", type=int, default=512, help="uniform-seed samples per pixel of the glint budget")
     ap.add_argument("--glint-ref-spp", type=int, default=8192, help="glint reference samples per pixel")
     ap.add_argument("--glint-reps", type=int, default=4, help="frames per method of the glint experiment")
+    ap.add_argument("--whole-frame-only", action="store_true",
+                    help="run only the C2 whole-frame measurement (camera queries generated in the kernels)")
     ap.add_argument("--parity-only", action="store_true",
                     help="run only the image parity of SMS against the filtered path tracer")
     ap.add_argument("--no-parity", action="store_true", help="skip the image parity experiment")
@@ -208,6 +217,12 @@ def main():
     ctx.set_schedule(**sched)
     print(f"# setup {time.time() - t0:.1f}s (autotune {t_tune:.2f}s) device={dev} cus={ncu} queries={n} "
           f"schedule={sched}", file=sys.stderr, flush=True)
+    if args.whole_frame_only:
+        print(json.dumps(glint_whole_frame(args, ctx, qbuf, n, out)[0]), flush=True)
+        out.free()
+        qbuf.free()
+        ctx.close()
+        return
 
     for w in range(args.warmup):
         ctx.glint_estimate_device(qbuf, n, 1000 + w, out)
@@ -215,7 +230,6 @@ def main():
     tot = {}
     k1_ms = k2_ms = 0.0
     iters_pass1 = 0
-    step_stats = []
     t_start = time.perf_counter()
     for s in range(args.steps):
         st = ctx.glint_estimate_device(qbuf, n, 1 + s, out)
@@ -223,7 +237,6 @@ def main():
         k1_ms += a
         k2_ms += b
         iters_pass1 += st["newton_iterations"] - st["tail_iterations"]
-        step_stats.append(st)
         for k, v in st.items():
             tot[k] = tot.get(k, 0) + v
     ctx.sync()
@@ -251,25 +264,23 @@ def main():
           f"walks/s={walks_per_s:.4g} conv/s={conv_per_s:.4g} first={k1_ms / args.steps:.3f}ms "
           f"rest={k2_ms / args.steps:.3f}ms stats={tot}", file=sys.stderr, flush=True)
 
-    k1_s = k1_ms / 1e3 / args.steps
-    achieved_gbs = BYTES_PER_ITER * (iters_pass1 / args.steps) / k1_s / 1e9
-    achieved_tflops = FLOPS_PER_ITER * (iters_pass1 / args.steps) / k1_s / 1e12
     default_workload = (args.width, args.height, args.spp, args.nm_res) == (512, 512, 4, 2048)
-    traffic = None
-    if default_workload:
-        t = TRAFFIC_FIRST_PASS
-        traffic = round(1024.0 * (t["fetch_kib"] * t["fetch_factor"] + t["write_kib"] * t["write_factor"]))
-    roofline = {"bound": "hbm", "achieved": round(achieved_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved_gbs / HBM_PEAK_GBS, 5), "traffic": traffic,
-                "bytes_per_iter": BYTES_PER_ITER, "iters_per_launch": round(iters_pass1 / args.steps),
-                "kernel": "k_glint_first", "kernel_ms": round(k1_s * 1e3, 3),
-                "valu": {"achieved": round(achieved_tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                         "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 5),
-                         "flops_per_iter": FLOPS_PER_ITER}}
+    qb_roofline = first_pass_roofline(iters_pass1 / args.steps, k1_ms / args.steps,
+                                      "query_buffer" if default_workload else None)
+    query_buffer_frame = {
+        "ms_per_frame": round(ms_per_step, 4), "converged_walks_per_s": conv_per_s, "walks_per_s": walks_per_s,
+        "eager_ms_per_frame": round(1e3 * eager_s / args.steps, 4),
+        "trials_tail_finalize_ms": round(k2_ms / args.steps, 3), "roofline": qb_roofline,
+        "note": "sms_glint_make_queries once (seed 1, outside the timed region), then graph replays of "
+                "sms_glint_estimate_batch over the 140-byte query records with estimator seeds 1..steps"}
+
+    whole_frame, wf = glint_whole_frame(args, ctx, qbuf, n, out)
+    roofline = first_pass_roofline(wf["iters_per_launch"], wf["first_pass_ms"],
+                                   "frame" if default_workload else None)
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(sc, cfg, ctx, qbuf, n, out, step_stats, args)
+        cpu = cpu_baseline(sc, cfg, ctx, qbuf, n, out, wf["step_stats"], args)
     out.free()
     qbuf.free()
     ctx.close()
@@ -282,25 +293,27 @@ def main():
     parity = None if args.no_parity else image_parity(args)
     result = {
         "metric": "converged walks/s at 1 GPU; equal-time RMSE learned vs uniform seeds",
-        "value": conv_per_s,
+        "value": wf["converged_per_s"],
         "unit": "walks/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": wf["ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"glint SMS, {args.width}x{args.height}x{args.spp}spp footprint queries, "
-                               f"{args.nm_res}^2 RGB8 scratch+bump normal map, unbiased (max_trials 1e5)",
-                   "queries_per_step": n, "walks_per_step": tot["walks"] / args.steps,
-                   "walks_per_s": walks_per_s, "device": dev, "schedule": sched,
-                   "launch": "hipGraph replay per frame",
-                   "eager_ms_per_step": round(1e3 * eager_s / args.steps, 4),
-                   "trials_tail_finalize_ms": round(k2_ms / args.steps, 3)},
+        "config": {"workload": f"glint SMS, {args.width}x{args.height}x{args.spp}spp camera footprint queries "
+                               f"generated per frame, {args.nm_res}^2 RGB8 scratch+bump normal map, unbiased "
+                               f"(max_trials 1e5)",
+                   "queries_per_step": n, "walks_per_step": wf["walks"] / args.steps,
+                   "walks_per_s": whole_frame["walks_per_s"], "device": dev, "schedule": sched,
+                   "launch": "hipGraph replay per frame (sms_glint_frame_graph_create), camera seed = estimator seed",
+                   "eager_ms_per_step": whole_frame["eager_ms_per_frame"]},
         "roofline": roofline,
+        "whole_frame": whole_frame,
+        "query_buffer_frame": query_buffer_frame,
         "cpu_baseline": cpu,
         "dome_ss_biased": dome,
         "pool_ss": pool,
@@ -311,7 +324,7 @@ def main():
         "image_parity": parity,
     }
     if cpu and cpu.get("value"):
-        result["gpu_over_cpu"] = conv_per_s / cpu["value"]
+        result["gpu_over_cpu"] = wf["converged_per_s"] / cpu["value"]
     print(json.dumps(result), flush=True)
 
 
@@ -708,6 +721,89 @@ def equal_time_rmse(args, case="pool"):
     return out
 
 
+def first_pass_roofline(iters_per_launch, kernel_ms, traffic_key):
+    """Roofline of k_glint_first: algorithmic bytes (BYTES_PER_ITER per first-pass Newton
+    iteration) and FP32 FLOP per launch over its average launch time (HIP events on the library
+    stream); `traffic` = the calibrated PMC bytes of that launch on the default workload."""
+    k1_s = kernel_ms / 1e3
+    achieved_gbs = BYTES_PER_ITER * iters_per_launch / k1_s / 1e9
+    achieved_tflops = FLOPS_PER_ITER * iters_per_launch / k1_s / 1e12
+    traffic = None
+    if traffic_key is not None:
+        t = TRAFFIC_FIRST_PASS[traffic_key]
+        traffic = round(1024.0 * (t["fetch_kib"] * FETCH_FACTOR + t["write_kib"] * WRITE_FACTOR))
+    return {"bound": "hbm", "achieved": round(achieved_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved_gbs / HBM_PEAK_GBS, 5), "traffic": traffic,
+            "algorithmic_bytes": round(BYTES_PER_ITER * iters_per_launch),
+            "bytes_per_iter": BYTES_PER_ITER, "iters_per_launch": round(iters_per_launch),
+            "kernel": "k_glint_first", "kernel_ms": round(kernel_ms, 4),
+            "valu": {"achieved": round(achieved_tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 5), "flops_per_iter": FLOPS_PER_ITER}}
+
+
+def glint_whole_frame(args, ctx, qbuf, n, out):
+    """The C2 frame with its camera queries generated by the kernels that read them
+    (sms_glint_estimate_frame / sms_glint_frame_graph_create): first the two-step frames of seeds
+    1..steps (sms_glint_make_queries + sms_glint_estimate_batch, camera seed = estimator seed),
+    whose statistics the whole frames, eager and as graph replays, must reproduce; then the
+    query-buffer line's queries (seed 1) are restored in qbuf.  Returns the report and the raw
+    figures of the replays and of the eager frames' first passes."""
+    for w in range(args.warmup):
+        ctx.glint_estimate_frame_device(args.spp, 1000 + w, out)
+    ttot = {}
+    ctx.sync()
+    t_start = time.perf_counter()
+    for s in range(args.steps):
+        ctx.glint_queries_device(args.spp, 1 + s, qbuf)
+        st = ctx.glint_estimate_device(qbuf, n, 1 + s, out)
+        for k, v in st.items():
+            ttot[k] = ttot.get(k, 0) + v
+    ctx.sync()
+    two_step_s = time.perf_counter() - t_start
+    wtot, k1f_ms, iters_pass1, step_stats = {}, 0.0, 0, []
+    t_start = time.perf_counter()
+    for s in range(args.steps):
+        st = ctx.glint_estimate_frame_device(args.spp, 1 + s, out)
+        k1f_ms += ctx.last_kernel_ms()[0]
+        iters_pass1 += st["newton_iterations"] - st["tail_iterations"]
+        step_stats.append(st)
+        for k, v in st.items():
+            wtot[k] = wtot.get(k, 0) + v
+    ctx.sync()
+    eager_s = time.perf_counter() - t_start
+    fgraph = ctx.glint_frame_graph(args.spp, out)
+    fgraph.launch(1000)
+    fgraph.stats(reset=True)
+    ctx.sync()
+    t_start = time.perf_counter()
+    for s in range(args.steps):
+        fgraph.launch(1 + s)
+    ctx.sync()
+    f_elapsed = time.perf_counter() - t_start
+    ftot = fgraph.stats(reset=True)
+    fgraph.close()
+    for k in ("walks", "walks_converged", "newton_iterations"):
+        if ftot[k] != ttot[k] or wtot[k] != ttot[k]:
+            raise RuntimeError(f"whole frames differ from the two-step frames in {k}: {ftot[k]} / {wtot[k]} != {ttot[k]}")
+    ctx.glint_queries_device(args.spp, 1, qbuf)       # the query-buffer line's queries again
+    ctx.sync()
+    raw = {"converged_per_s": ftot["walks_converged"] / f_elapsed, "walks": ftot["walks"],
+           "iters_per_launch": iters_pass1 / args.steps, "first_pass_ms": k1f_ms / args.steps,
+           "ms": 1e3 * f_elapsed / args.steps, "step_stats": step_stats}
+    res = {"ms_per_frame": round(raw["ms"], 4),
+           "converged_walks_per_s": raw["converged_per_s"],
+           "walks_per_s": ftot["walks"] / f_elapsed,
+           "first_pass_ms": round(raw["first_pass_ms"], 4),
+           "eager_ms_per_frame": round(1e3 * eager_s / args.steps, 4),
+           "two_step_eager_ms_per_frame": round(1e3 * two_step_s / args.steps, 4),
+           "note": "sms_glint_frame_graph_create replays of frames with seeds 1..steps (camera seed = "
+                   "estimator seed): camera queries generated in the kernels that read them, no "
+                   "140-byte query records; statistics equal to sms_glint_make_queries + "
+                   "sms_glint_estimate_batch of the same seeds (two_step, eager calls)"}
+    print(f"# whole frame: {json.dumps(res)}", file=sys.stderr, flush=True)
+    return res, raw
+
+
 def image_parity(args):
     """SMS against the filtered path tracer (path_sms_ss vs path_filtered_ss, the reference's own
     comparison pair): the caustic term -- the single-scatter caustic paths through the first
@@ -803,22 +899,26 @@ def image_parity(args):
 
 
 def cpu_baseline(sc, cfg, ctx, qbuf, n, out, step_stats, args):
-    """CPU oracle on whole frames of the same queries and per-frame seeds as the GPU steps, on
-    CPU_THREADS host threads for ~cpu_seconds, then on one thread.  Every frame both ran must
-    agree: the oracle's statistics equal the GPU step's, and for the first frames the oracle's
-    RGB equals the GPU's bit for bit (raises otherwise)."""
+    """CPU oracle on whole frames of the same workload as the GPU's: camera queries of seed
+    1 + rep (oracle_glint_queries) estimated with seed 1 + rep, on CPU_THREADS host threads for
+    ~cpu_seconds, then on one thread.  Every frame both ran must agree: the first frame's queries
+    equal the GPU's byte for byte, the oracle's statistics equal the GPU whole frame's, and for
+    the first frames the oracle's RGB equals the GPU's bit for bit (raises otherwise)."""
     import numpy as np
     from smsamd import abi
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
-    q = np.zeros(n, abi.GLINT_QUERY)
-    qbuf.download(q)
+    q1 = np.zeros(n, abi.GLINT_QUERY)
+    qbuf.download(q1)                                 # holds the camera queries of seed 1
     checked_rgb = 0
     orc = oracle_lib.Oracle(sc, cfg, threads=CPU_THREADS)
     keys = ("walks", "walks_converged", "newton_iterations", "bernoulli_calls", "bernoulli_trials", "bernoulli_max")
 
     def frame(rep):
         nonlocal checked_rgb
+        q = orc.glint_queries(args.spp, 1 + rep)
+        if rep == 0 and q.tobytes() != q1.tobytes():
+            raise RuntimeError("CPU oracle and GPU camera queries of seed 1 differ")
         rgb, st = orc.glint_estimate(q, 1 + rep)
         if rep < len(step_stats):
             for k in keys:
@@ -826,7 +926,7 @@ def cpu_baseline(sc, cfg, ctx, qbuf, n, out, step_stats, args):
                     raise RuntimeError(f"CPU oracle and GPU differ on frame seed {1 + rep} in {k}: "
                                        f"{st[k]} != {step_stats[rep][k]}")
         if rep < 3:
-            ctx.glint_estimate_device(qbuf, n, 1 + rep, out)
+            ctx.glint_estimate_frame_device(args.spp, 1 + rep, out)
             g = out.download(np.zeros((n, 3), np.float32))
             if not np.array_equal(g, rgb, equal_nan=True):
                 raise RuntimeError(f"CPU oracle and GPU RGB differ on frame seed {1 + rep}: "
@@ -837,13 +937,17 @@ def cpu_baseline(sc, cfg, ctx, qbuf, n, out, step_stats, args):
     r16 = timed_cpu(frame, args.cpu_seconds)
     orc.close()
     orc1 = oracle_lib.Oracle(sc, cfg, threads=1)
-    r1 = timed_cpu(lambda rep: (lambda st: (st["walks"], st["walks_converged"]))(orc1.glint_estimate(q, 1 + rep)[1]),
-                   args.cpu_seconds / 3)
+
+    def frame1(rep):
+        st = orc1.glint_estimate(orc1.glint_queries(args.spp, 1 + rep), 1 + rep)[1]
+        return st["walks"], st["walks_converged"]
+    r1 = timed_cpu(frame1, args.cpu_seconds / 3)
     orc1.close()
     return {"value": r16["converged_per_s"], "unit": "walks/s", "cores": CPU_THREADS, "kind": "port",
-            "sample": f"{r16['reps']} whole frames of the same workload ({n} queries each, {r16['walks']} walks, "
-                      f"{r16['seconds']}s on {CPU_THREADS} threads); frames checked equal to the GPU's: "
-                      f"{min(r16['reps'], len(step_stats))} by statistics, {checked_rgb} bit for bit in RGB",
+            "sample": f"{r16['reps']} whole frames of the same workload (camera queries generated, {n} each, "
+                      f"{r16['walks']} walks, {r16['seconds']}s on {CPU_THREADS} threads); frames checked equal "
+                      f"to the GPU's: {min(r16['reps'], len(step_stats))} by statistics, {checked_rgb} bit for "
+                      f"bit in RGB",
             "walks_per_s": r16["walks_per_s"], "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
             "one_core": {"value": r1["converged_per_s"], "walks_per_s": r1["walks_per_s"], "cores": 1,
                          "sample": f"{r1['reps']} whole frames, {r1['seconds']}s"}}

@@ -358,6 +358,11 @@ sms_status sms_ss_estimate_batch_noalloc(sms_ctx *ctx, const sms_ss_query *q, in
    the BSDF-sampling strategy at a footprint offset (:183-220), MIS-weighted as the reference. */
 sms_status sms_glint_estimate_batch(sms_ctx *ctx, const sms_glint_query *q, int64_t n,
                                     uint64_t seed, float *out_rgb, sms_stats *stats);
+/* The glint estimate of a whole camera frame without a query buffer: the same results as
+   sms_glint_make_queries(ctx, spp, seed, q) followed by sms_glint_estimate_batch(ctx, q,
+   w*h*spp, seed, out_rgb, stats), each query generated by the kernel that reads it
+   (path_sms_glints.cpp:119-127 per pixel sample).  out_rgb: device, w*h*spp x 3. */
+sms_status sms_glint_estimate_frame(sms_ctx *ctx, int32_t spp, uint64_t seed, float *out_rgb, sms_stats *stats);
 /* Autotuner (no counterpart in the reference): times sms_glint_estimate_batch on the given
    device-resident queries for every candidate schedule (trial_group x lane_trials x
    tail_rounds x blocks_per_cu), keeps the fastest in the context and returns it in *best.
@@ -542,6 +547,8 @@ sms_status sms_ms_graph_create(sms_ctx *ctx, const sms_ss_query *q, int64_t n, f
    weight, w*h, spp, image, image_sq, image_n) (image_sq and image_n may be NULL). */
 sms_status sms_ss_pass_graph_create(sms_ctx *ctx, int32_t spp, sms_ss_query *q, float *weight, float *out_rgb,
                                     float *image, float *image_sq, float *image_n, sms_graph **out);
+/* Graph of sms_glint_estimate_frame(ctx, spp, seed, out_rgb): the whole frame, queries included. */
+sms_status sms_glint_frame_graph_create(sms_ctx *ctx, int32_t spp, float *out_rgb, sms_graph **out);
 /* Enqueue one replay with `seed` on the context stream. */
 sms_status sms_graph_launch(sms_graph *g, uint64_t seed);
 /* Waits for the stream; *stats (if not NULL) = the statistics summed over the launches since

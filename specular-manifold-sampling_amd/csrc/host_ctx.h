@@ -137,7 +137,7 @@ sms_status noalloc_init_device(sms_ctx *ctx);
 Proposal proposal_of(sms_ctx *ctx);
 sms_status proposal_alloc(sms_ctx *ctx, int64_t n);
 sms_status proposal_enqueue(sms_ctx *ctx, const sms_ss_query *q, int64_t n, int ms);
-sms_status proposal_enqueue_glint(sms_ctx *ctx, const sms_glint_query *q, int64_t n);
+sms_status proposal_enqueue_glint(sms_ctx *ctx, const GlintQuerySource &q, int64_t n);
 void proposal_release(sms_ctx *ctx);
 
 // launch sequences shared by the estimator calls and the frame graphs (sms_host.hip).  The
@@ -147,8 +147,14 @@ __global__ void k_frame_begin(unsigned long long *counters, int n, uint64_t *see
 __global__ void k_set_seed(uint64_t *slot, uint64_t seed);
 __global__ void k_stats_fold(const unsigned long long *counters, unsigned long long *total);
 sms_status frame_begin(sms_ctx *ctx, uint64_t seed);
-sms_status glint_prepare(sms_ctx *ctx, const sms_glint_query *q, int64_t n, const float *out_rgb);
-sms_status glint_enqueue(sms_ctx *ctx, const sms_glint_query *q, int64_t n, float *out_rgb);
+// glint queries from an array, or generated from the camera where they are read (the camera
+// seed is slot SEED_CAM, which frame_begin sets with the estimator's)
+inline GlintQuerySource glint_queries_of(const sms_glint_query *q) { return GlintQuerySource{q, nullptr, 0}; }
+inline GlintQuerySource glint_camera_queries(sms_ctx *ctx, int32_t spp) {
+    return GlintQuerySource{nullptr, ctx->seeds + SEED_CAM, (int) spp};
+}
+sms_status glint_prepare(sms_ctx *ctx, const sms_glint_query *q, int64_t n, const float *out_rgb, bool from_camera);
+sms_status glint_enqueue(sms_ctx *ctx, const GlintQuerySource &q, int64_t n, float *out_rgb);
 sms_status ss_prepare(sms_ctx *ctx, const sms_ss_query *q, int64_t n, const float *out_rgb);
 sms_status ss_enqueue(sms_ctx *ctx, const sms_ss_query *q, int64_t n, float *out_rgb);
 sms_status ms_prepare(sms_ctx *ctx, const sms_ss_query *q, int64_t n, const float *out_rgb);

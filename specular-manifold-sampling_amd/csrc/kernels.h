@@ -126,6 +126,66 @@ SMS_DEV SI glint_query_si(const sms_glint_query &g) {
     return si;
 }
 
+// Camera query of sample idx of a w x h x spp glint frame (integrator.cpp:243-262,
+// perspective.cpp:191-221, SurfaceInteraction::compute_partials interaction.h:216-248): the
+// pixel-sample ray differential, its hit and, on a glinty shape, the footprint; shape -1 when the
+// first hit is not glinty.  BVH false skips the triangle tree (scenes without meshes).
+template <bool BVH>
+SMS_DEV sms_glint_query make_glint_query(const DScene &S, uint64_t seed, int spp, int64_t idx) {
+    const DCamera &c = S.cam;
+    sms_glint_query o;
+    float *z = (float *) &o;
+    for (int k = 0; k < (int) (sizeof(o) / 4); ++k) z[k] = 0.f;
+    o.shape = -1;
+    const int64_t pixel = idx / spp;
+    const int px = (int) (pixel % c.w), py = (int) (pixel / c.w);
+    o.pixel = (int32_t) pixel;
+    Pcg32 r;
+    stream_seed(r, query_key(seed, (uint64_t) idx), 0xFFFFFFFFull);
+    const float sx = pcg32_float(r), sy = pcg32_float(r);
+    const V2 pos = v2(((float) px + sx) / (float) c.w, ((float) py + sy) / (float) c.h);
+    const V3 near_p = v3((1.f - 2.f * pos.x) * c.tan_half, (1.f - 2.f * pos.y) * c.rect_y, 1.f);
+    const V3 dloc = normalize(near_p);
+    const V3 L = ld3(c.left), U = ld3(c.up), D = ld3(c.dir), O = ld3(c.o);
+    const V3 d = to_world(L, U, D, dloc);
+    V3 d_x = to_world(L, U, D, normalize(near_p + ld3(c.dx))), d_y = to_world(L, U, D, normalize(near_p + ld3(c.dy)));
+    const float scale = 1.f / sqrtf((float) spp);
+    d_x = fmadd3(d_x - d, scale, d); d_y = fmadd3(d_y - d, scale, d);
+    const Ray ray = make_ray(O, d, 1e-2f * (1.f / dloc.z), 1e4f * (1.f / dloc.z));
+    const SI si = make_si(S, ray, scene_hit_t<BVH>(S, ray, false));
+    st3(o.sensor_p, O);
+    if (si.valid && (S.shapes[si.shape].flags & SMS_FLAG_GLINTY)) {
+        const float dd = dot(si.n, si.p);
+        const float t_x = (dd - dot(si.n, O)) / dot(si.n, d_x), t_y = (dd - dot(si.n, O)) / dot(si.n, d_y);
+        const V3 dp_dx = fmadd3(d_x, t_x, O) - si.p, dp_dy = fmadd3(d_y, t_y, O) - si.p;
+        const float a00 = dot(si.dp_du, si.dp_du), a01 = dot(si.dp_du, si.dp_dv), a11 = dot(si.dp_dv, si.dp_dv);
+        float inv_det = 1.f / (a00 * a11 - a01 * a01);
+        const float b0x = dot(si.dp_du, dp_dx), b1x = dot(si.dp_dv, dp_dx), b0y = dot(si.dp_du, dp_dy),
+                    b1y = dot(si.dp_dv, dp_dy);
+        if (!isfinite(inv_det)) inv_det = 0.f;
+        o.duv_dx[0] = fmaf(a11, b0x, -(a01 * b1x)) * inv_det; o.duv_dx[1] = fmaf(a00, b1x, -(a01 * b0x)) * inv_det;
+        o.duv_dy[0] = fmaf(a11, b0y, -(a01 * b1y)) * inv_det; o.duv_dy[1] = fmaf(a00, b1y, -(a01 * b0y)) * inv_det;
+        o.shape = si.shape;
+        st3(o.p, si.p); st3(o.n, si.n); st3(o.sh_n, si.sh_n); st3(o.sh_s, si.sh_s); st3(o.sh_t, si.sh_t);
+        st3(o.dp_du, si.dp_du); st3(o.dp_dv, si.dp_dv);
+        const V3 mw = -ray.d;
+        st3(o.wi, v3(dot(mw, si.sh_s), dot(mw, si.sh_t), dot(mw, si.sh_n)));
+        o.uv[0] = si.uv.x; o.uv[1] = si.uv.y;
+    }
+    return o;
+}
+
+// Where the glint estimator's queries come from: an array (Q), or the camera -- query q is
+// make_glint_query of the frame's sample q, generated where it is read, with the camera seed
+// at cam_seed (the estimate of a whole frame without a query buffer).
+struct GlintQuerySource { const sms_glint_query *Q; const uint64_t *cam_seed; int spp; };
+
+template <bool BVH = true>
+SMS_DEV sms_glint_query glint_query_fetch(const DScene &S, const GlintQuerySource &src, int64_t q) {
+    if (src.Q) return src.Q[q];
+    return make_glint_query<BVH>(S, *src.cam_seed, src.spp, q);
+}
+
 // SMS-strategy result of a resolved glint query: (f * (val * inv_prob)) * ew per channel
 // (manifold_glints.cpp:108,179)
 SMS_DEV V3 glint_result(float val, float inv_prob, float f, V3 ew) {
@@ -136,11 +196,11 @@ SMS_DEV V3 glint_result(float val, float inv_prob, float f, V3 ew) {
 __global__ void k_ss_walk(DScene S, const sms_ss_walk_in *in, int64_t n, sms_ss_walk_out *out);
 __global__ void k_glint_walk(DScene S, const sms_glint_walk_in *in, int64_t n, sms_glint_walk_out *out);
 __global__ void k_glint_queries(DScene S, int spp, const uint64_t *seedp, sms_glint_query *q, int64_t n);
-__global__ void k_glint_first_bvh(DScene S, const sms_glint_query *Q, int64_t n, const uint64_t *seedp, float *out, EstimatorWork W);
-__global__ void k_glint_biased_bvh(DScene S, const sms_glint_query *Q, int64_t n, const uint64_t *seedp, float *out, EstimatorWork W);
-__global__ void k_glint_biased_nobvh(DScene S, const sms_glint_query *Q, int64_t n, const uint64_t *seedp, float *out, EstimatorWork W);
-__global__ void k_glint_bsdf(DScene S, const sms_glint_query *Q, int64_t n, const uint64_t *seedp, float *out);
-__global__ void k_glint_first_nobvh(DScene S, const sms_glint_query *Q, int64_t n, const uint64_t *seedp, float *out, EstimatorWork W);
+__global__ void k_glint_first_bvh(DScene S, GlintQuerySource Q, int64_t n, const uint64_t *seedp, float *out, EstimatorWork W);
+__global__ void k_glint_biased_bvh(DScene S, GlintQuerySource Q, int64_t n, const uint64_t *seedp, float *out, EstimatorWork W);
+__global__ void k_glint_biased_nobvh(DScene S, GlintQuerySource Q, int64_t n, const uint64_t *seedp, float *out, EstimatorWork W);
+__global__ void k_glint_bsdf(DScene S, GlintQuerySource Q, int64_t n, const uint64_t *seedp, float *out);
+__global__ void k_glint_first_nobvh(DScene S, GlintQuerySource Q, int64_t n, const uint64_t *seedp, float *out, EstimatorWork W);
 __global__ void k_glint_trials_g1(DScene S, const uint64_t *seedp, EstimatorWork W);
 __global__ void k_glint_trials_g2(DScene S, const uint64_t *seedp, EstimatorWork W);
 __global__ void k_glint_trials_g4(DScene S, const uint64_t *seedp, EstimatorWork W);
@@ -150,7 +210,7 @@ __global__ void k_glint_coop_rounds(DScene S, const uint64_t *seedp, EstimatorWo
 __global__ void k_glint_multiwave(DScene S, const uint64_t *seedp, EstimatorWork W);
 __global__ void k_glint_multiwave_resolve(DScene S, EstimatorWork W);
 __global__ void k_glint_coop_seq(DScene S, const uint64_t *seedp, EstimatorWork W);
-__global__ void k_glint_finalize(DScene S, const sms_glint_query *Q, const uint64_t *seedp, float *out, EstimatorWork W);
+__global__ void k_glint_finalize(DScene S, GlintQuerySource Q, const uint64_t *seedp, float *out, EstimatorWork W);
 __global__ void k_ss_estimate(DScene S, const sms_ss_query *Q, int64_t n, const uint64_t *seedp, float *part, float *invp, EstimatorWork W);
 // LDS traversal-stack entries per lane of the persistent walker; scenes whose BVH is deeper run
 // the lane-per-query first pass instead
@@ -158,7 +218,7 @@ constexpr int WALKER_STACK = 32;
 __global__ void k_ss_walker(DScene S, const sms_ss_query *Q, int64_t n, const uint64_t *seedp, float *part, float *invp,
                             EstimatorWork W);
 __global__ void k_ss_coop(DScene S, const sms_ss_query *Q, const uint64_t *seedp, const float *part, float *invp, EstimatorWork W);
-__global__ void k_glint_prop_inputs(DScene S, const sms_glint_query *Q, int64_t n, const uint64_t *seedp, float *x);
+__global__ void k_glint_prop_inputs(DScene S, GlintQuerySource Q, int64_t n, const uint64_t *seedp, float *x);
 __global__ void k_ss_finalize(DScene S, const uint64_t *seedp, const float *part, const float *invp, int nc, int64_t n,
                               int offsets, float *out);
 __global__ void k_ss_queries(DScene S, int spp, const uint64_t *seedp, sms_ss_query *q, float *w, int64_t n);

@@ -70,7 +70,7 @@ SMS_DEV unsigned long long wave_sum_u64(unsigned long long v) {
 
 // ---------------------------------------------------------------- pass 1: setup + first walks
 template <bool BVH>
-SMS_DEV void glint_first_body(const DScene &S, const sms_glint_query *__restrict__ Q, int64_t n, uint64_t seed,
+SMS_DEV void glint_first_body(const DScene &S, const GlintQuerySource &Q, int64_t n, uint64_t seed,
                               float *__restrict__ out, EstimatorWork W, GlintRec *lrec, int *wcount,
                               unsigned long long *lbase, unsigned long long (*lstats)[4]) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -79,7 +79,7 @@ SMS_DEV void glint_first_body(const DScene &S, const sms_glint_query *__restrict
     unsigned long long st_walks = 0, st_conv = 0, st_iters = 0;
     GlintRec rec;
     if (q < n) {
-        const sms_glint_query g = Q[q];
+        const sms_glint_query g = glint_query_fetch<BVH>(S, Q, q);
         write_rgb(out, q, v3(0.f, 0.f, 0.f));
         if (g.shape >= 0 && g.shape < S.n_shapes && S.shapes[g.shape].bsdf == SMS_BSDF_NORMALMAP) {
             SI si = glint_query_si(g);
@@ -143,7 +143,7 @@ SMS_DEV void glint_first_body(const DScene &S, const sms_glint_query *__restrict
         uv_sol = w.uv;
         if (conv && capturing(W.prop, r.shape, SMS_CAPTURE_SOLUTIONS)) {
             // training record: the solution as a footprint sample xi (uv = uv0 + (xi - 1/2) [duv_dx duv_dy])
-            const sms_glint_query g = Q[r.q];
+            const sms_glint_query g = glint_query_fetch<BVH>(S, Q, r.q);
             const SI si = glint_query_si(g);
             Pcg32 rq;
             stream_seed(rq, query_key(seed, (uint64_t) r.q), 0);
@@ -190,7 +190,7 @@ SMS_DEV void glint_first_body(const DScene &S, const sms_glint_query *__restrict
 }
 
 #define SMS_GLINT_FIRST_KERNEL(NAME, BVH)                                                                    \
-    __global__ void __launch_bounds__(256) NAME(DScene S, const sms_glint_query *__restrict__ Q, int64_t n,  \
+    __global__ void __launch_bounds__(256) NAME(DScene S, GlintQuerySource Q, int64_t n,                     \
                                                 const uint64_t *__restrict__ seedp, float *__restrict__ out, \
                                                 EstimatorWork W) {                                           \
         __shared__ GlintRec lrec[256];                                                                       \
@@ -566,7 +566,7 @@ __global__ void __launch_bounds__(256) k_glint_multiwave_resolve(DScene S, Estim
 
 
 // ---------------------------------------------------------------- contributions
-__global__ void __launch_bounds__(256) k_glint_finalize(DScene S, const sms_glint_query *__restrict__ Q, const uint64_t *__restrict__ seedp,
+__global__ void __launch_bounds__(256) k_glint_finalize(DScene S, GlintQuerySource Q, const uint64_t *__restrict__ seedp,
                                                         float *__restrict__ out, EstimatorWork W) {
     const uint64_t seed = *seedp;
     const sms_config &cfg = S.cfg;
@@ -577,7 +577,7 @@ __global__ void __launch_bounds__(256) k_glint_finalize(DScene S, const sms_glin
         const GlintRes res = W.res[slot];
         const TrialRec tr = W.trials.items[slot];
         const int64_t q = tr.f.q;
-        const sms_glint_query g = Q[q];
+        const sms_glint_query g = glint_query_fetch(S, Q, q);
         SI si = glint_query_si(g);
         Pcg32 rq;
         stream_seed(rq, query_key(seed, (uint64_t) q), 0);
@@ -609,14 +609,14 @@ __global__ void __launch_bounds__(256) k_glint_finalize(DScene S, const sms_glin
 // unbiased finalize pass.
 constexpr int GLINT_BIASED_MAX_SOL = 128;
 template <bool BVH>
-SMS_DEV void glint_biased_body(const DScene &S, const sms_glint_query *__restrict__ Q, int64_t n, uint64_t seed,
+SMS_DEV void glint_biased_body(const DScene &S, const GlintQuerySource &Q, int64_t n, uint64_t seed,
                                float *__restrict__ out, EstimatorWork W, float2 (*sol_lds)[GLINT_BIASED_MAX_SOL]) {
     const sms_config &cfg = S.cfg;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     unsigned long long st_walks = 0, st_conv = 0, st_iters = 0;
     const int64_t n_waves = ((int64_t) gridDim.x * blockDim.x) >> 6;
     for (int64_t q = ((int64_t) blockIdx.x * blockDim.x + threadIdx.x) >> 6; q < n; q += n_waves) {
-        const sms_glint_query g = Q[q];
+        const sms_glint_query g = glint_query_fetch<BVH>(S, Q, q);
         V3 result = v3(0.f, 0.f, 0.f);
         if (g.shape >= 0 && g.shape < S.n_shapes && S.shapes[g.shape].bsdf == SMS_BSDF_NORMALMAP) {
             SI si = glint_query_si(g);
@@ -719,7 +719,7 @@ SMS_DEV void glint_biased_body(const DScene &S, const sms_glint_query *__restric
 }
 
 #define SMS_GLINT_BIASED_KERNEL(NAME, BVH)                                                                   \
-    __global__ void __launch_bounds__(256) NAME(DScene S, const sms_glint_query *__restrict__ Q, int64_t n,  \
+    __global__ void __launch_bounds__(256) NAME(DScene S, GlintQuerySource Q, int64_t n,                     \
                                                 const uint64_t *__restrict__ seedp, float *__restrict__ out, \
                                                 EstimatorWork W) {                                           \
         __shared__ float2 sol_lds[4][GLINT_BIASED_MAX_SOL];                                                  \
@@ -730,11 +730,11 @@ SMS_GLINT_BIASED_KERNEL(k_glint_biased_nobvh, false)
 
 // Inputs of the learned proposal for glint queries: shading point, shading normal and the glint
 // estimator's light sample (stream (q, 0) over all emitters); zeros for queries without a glinty hit.
-__global__ void k_glint_prop_inputs(DScene S, const sms_glint_query *__restrict__ Q, int64_t n,
+__global__ void k_glint_prop_inputs(DScene S, GlintQuerySource Q, int64_t n,
                                     const uint64_t *__restrict__ seedp, float *__restrict__ x) {
     const int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
-    const sms_glint_query g = Q[q];
+    const sms_glint_query g = glint_query_fetch(S, Q, q);
     float *r = x + 9 * q;
     if (g.shape < 0) {
         for (int k = 0; k < 9; ++k) r[k] = 0.f;
@@ -756,11 +756,11 @@ __global__ void k_glint_prop_inputs(DScene S, const sms_glint_query *__restrict_
 // emitter adds mis(p_bsdf, p_sms) * F * L (area.cpp:71-79).  Only an area emitter can receive
 // this strategy, so the host launches it for scenes that have one.
 constexpr uint64_t GLINT_BSDF_STREAM = 0xFFFFFFFEull;
-__global__ void __launch_bounds__(256) k_glint_bsdf(DScene S, const sms_glint_query *__restrict__ Q, int64_t n,
+__global__ void __launch_bounds__(256) k_glint_bsdf(DScene S, GlintQuerySource Q, int64_t n,
                                                     const uint64_t *__restrict__ seedp, float *__restrict__ out) {
     const int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
-    const sms_glint_query g = Q[q];
+    const sms_glint_query g = glint_query_fetch(S, Q, q);
     if (g.shape < 0 || g.shape >= S.n_shapes || S.shapes[g.shape].bsdf != SMS_BSDF_NORMALMAP) return;
     Pcg32 r;
     stream_seed(r, query_key(*seedp, (uint64_t) q), GLINT_BSDF_STREAM);

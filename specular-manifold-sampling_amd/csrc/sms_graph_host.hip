@@ -104,9 +104,19 @@ extern "C" {
 sms_status sms_glint_graph_create(sms_ctx *ctx, const sms_glint_query *q, int64_t n, float *out_rgb, sms_graph **out) {
     return guarded(ctx, [&]() -> sms_status {
         if (!ctx || !out) return fail(ctx, SMS_E_INVALID, "sms_glint_graph_create: null argument");
-        sms_status st = glint_prepare(ctx, q, n, out_rgb);
+        sms_status st = glint_prepare(ctx, q, n, out_rgb, false);
         if (st) return st;
-        return capture(ctx, [&] { return glint_enqueue(ctx, q, n, out_rgb); }, out);
+        return capture(ctx, [&] { return glint_enqueue(ctx, glint_queries_of(q), n, out_rgb); }, out);
+    });
+}
+
+sms_status sms_glint_frame_graph_create(sms_ctx *ctx, int32_t spp, float *out_rgb, sms_graph **out) {
+    return guarded(ctx, [&]() -> sms_status {
+        if (!ctx || !out || spp <= 0) return fail(ctx, SMS_E_INVALID, "sms_glint_frame_graph_create: bad arguments");
+        const int64_t n = (int64_t) ctx->scene.cam.w * ctx->scene.cam.h * spp;
+        sms_status st = glint_prepare(ctx, nullptr, n, out_rgb, true);
+        if (st) return st;
+        return capture(ctx, [&] { return glint_enqueue(ctx, glint_camera_queries(ctx, spp), n, out_rgb); }, out);
     });
 }
 

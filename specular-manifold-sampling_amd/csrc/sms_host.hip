@@ -1018,8 +1018,10 @@ sms_status sms_glint_make_queries(sms_ctx *ctx, int32_t spp, uint64_t seed, sms_
 
 } // extern "C"
 
-sms_status glint_prepare(sms_ctx *ctx, const sms_glint_query *q, int64_t n, const float *out_rgb) {
-    if (!ctx || n < 0 || (n > 0 && (!q || !out_rgb))) return fail(ctx, SMS_E_INVALID, "bad estimator arguments");
+sms_status glint_prepare(sms_ctx *ctx, const sms_glint_query *q, int64_t n, const float *out_rgb, bool from_camera) {
+    if (!ctx || n < 0 || (n > 0 && ((!q && !from_camera) || !out_rgb)))
+        return fail(ctx, SMS_E_INVALID, "bad estimator arguments");
+    if (from_camera && !ctx->scene.has_cam) return fail(ctx, SMS_E_INVALID, "scene has no camera");
     if (ctx->scene.cfg.biased && ctx->scene.cfg.max_trials > 128)
         return fail(ctx, SMS_E_UNSUPPORTED, "biased trial sets above 128 are not built");
     if (!ctx->scene.nmaps) return fail(ctx, SMS_E_INVALID, "scene has no normal map");
@@ -1033,7 +1035,7 @@ sms_status glint_prepare(sms_ctx *ctx, const sms_glint_query *q, int64_t n, cons
 
 // The glint pass sequence on the context stream (no host synchronisation; the frame's seed and
 // zeroed counters come first); events ev0 / ev1 / ev2 bracket the first-walk pass and the rest.
-sms_status glint_enqueue(sms_ctx *ctx, const sms_glint_query *q, int64_t n, float *out_rgb) {
+sms_status glint_enqueue(sms_ctx *ctx, const GlintQuerySource &q, int64_t n, float *out_rgb) {
     const uint64_t *seed = ctx->seeds + SEED_EST;
     EV_RECORD(ctx, ctx->ev0);
     {
@@ -1244,9 +1246,20 @@ extern "C" {
 sms_status sms_glint_estimate_batch(sms_ctx *ctx, const sms_glint_query *q, int64_t n, uint64_t seed, float *out_rgb,
                                     sms_stats *stats) {
     return guarded(ctx, [&]() -> sms_status {
-        sms_status st = glint_prepare(ctx, q, n, out_rgb);
+        sms_status st = glint_prepare(ctx, q, n, out_rgb, false);
         if (!st) st = frame_begin(ctx, seed);
-        if (!st) st = glint_enqueue(ctx, q, n, out_rgb);
+        if (!st) st = glint_enqueue(ctx, glint_queries_of(q), n, out_rgb);
+        return st ? st : estimator_done(ctx, stats);
+    });
+}
+
+sms_status sms_glint_estimate_frame(sms_ctx *ctx, int32_t spp, uint64_t seed, float *out_rgb, sms_stats *stats) {
+    return guarded(ctx, [&]() -> sms_status {
+        if (!ctx || spp <= 0) return fail(ctx, SMS_E_INVALID, "sms_glint_estimate_frame: bad arguments");
+        const int64_t n = (int64_t) ctx->scene.cam.w * ctx->scene.cam.h * spp;
+        sms_status st = glint_prepare(ctx, nullptr, n, out_rgb, true);
+        if (!st) st = frame_begin(ctx, seed);
+        if (!st) st = glint_enqueue(ctx, glint_camera_queries(ctx, spp), n, out_rgb);
         return st ? st : estimator_done(ctx, stats);
     });
 }
@@ -1255,7 +1268,7 @@ sms_status sms_glint_autotune(sms_ctx *ctx, const sms_glint_query *q, int64_t n,
                               sms_schedule *best) {
     return guarded(ctx, [&]() -> sms_status {
         if (!best) return fail(ctx, SMS_E_INVALID, "null schedule");
-        sms_status st = glint_prepare(ctx, q, n, out_rgb);
+        sms_status st = glint_prepare(ctx, q, n, out_rgb, false);
         if (st) return st;
         const sms_schedule keep = schedule_of(ctx);
         sms_schedule win = keep;
@@ -1268,7 +1281,7 @@ sms_status sms_glint_autotune(sms_ctx *ctx, const sms_glint_query *q, int64_t n,
             float ms = INFINITY;
             for (int rep = 0; rep < 4; ++rep) {
                 st = frame_begin(ctx, seed + (uint64_t) rep);
-                if (!st) st = glint_enqueue(ctx, q, n, out_rgb);
+                if (!st) st = glint_enqueue(ctx, glint_queries_of(q), n, out_rgb);
                 if (st) break;
                 HIP_TRY(ctx, hipEventSynchronize(ctx->ev2));
                 float t = 0.f;

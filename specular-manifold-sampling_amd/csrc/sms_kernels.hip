@@ -64,50 +64,9 @@ __global__ void k_glint_walk(DScene S, const sms_glint_walk_in *__restrict__ in,
 
 // ================================================================ glint queries (camera)
 __global__ void k_glint_queries(DScene S, int spp, const uint64_t *__restrict__ seedp, sms_glint_query *__restrict__ q, int64_t n) {
-    const uint64_t seed = *seedp;
-    int64_t idx = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t idx = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n) return;
-    const DCamera &c = S.cam;
-    sms_glint_query o;
-    float *z = (float *) &o;
-    for (int k = 0; k < (int) (sizeof(o) / 4); ++k) z[k] = 0.f;
-    o.shape = -1;
-    int64_t pixel = idx / spp;
-    int px = (int) (pixel % c.w), py = (int) (pixel / c.w);
-    o.pixel = (int32_t) pixel;
-    Pcg32 r;
-    stream_seed(r, query_key(seed, (uint64_t) idx), 0xFFFFFFFFull);
-    float sx = pcg32_float(r), sy = pcg32_float(r);
-    V2 pos = v2(((float) px + sx) / (float) c.w, ((float) py + sy) / (float) c.h);
-    V3 near_p = v3((1.f - 2.f * pos.x) * c.tan_half, (1.f - 2.f * pos.y) * c.rect_y, 1.f);
-    V3 dloc = normalize(near_p);
-    V3 L = ld3(c.left), U = ld3(c.up), D = ld3(c.dir), O = ld3(c.o);
-    V3 d = to_world(L, U, D, dloc);
-    V3 d_x = to_world(L, U, D, normalize(near_p + ld3(c.dx))), d_y = to_world(L, U, D, normalize(near_p + ld3(c.dy)));
-    float scale = 1.f / sqrtf((float) spp);
-    d_x = fmadd3(d_x - d, scale, d); d_y = fmadd3(d_y - d, scale, d);
-    Ray ray = make_ray(O, d, 1e-2f * (1.f / dloc.z), 1e4f * (1.f / dloc.z));
-    SI si = ray_intersect(S, ray);
-    st3(o.sensor_p, O);
-    if (si.valid && (S.shapes[si.shape].flags & SMS_FLAG_GLINTY)) {
-        // SurfaceInteraction::compute_partials (interaction.h:216-248)
-        float dd = dot(si.n, si.p);
-        float t_x = (dd - dot(si.n, O)) / dot(si.n, d_x), t_y = (dd - dot(si.n, O)) / dot(si.n, d_y);
-        V3 dp_dx = fmadd3(d_x, t_x, O) - si.p, dp_dy = fmadd3(d_y, t_y, O) - si.p;
-        float a00 = dot(si.dp_du, si.dp_du), a01 = dot(si.dp_du, si.dp_dv), a11 = dot(si.dp_dv, si.dp_dv),
-              inv_det = 1.f / (a00 * a11 - a01 * a01);
-        float b0x = dot(si.dp_du, dp_dx), b1x = dot(si.dp_dv, dp_dx), b0y = dot(si.dp_du, dp_dy), b1y = dot(si.dp_dv, dp_dy);
-        if (!isfinite(inv_det)) inv_det = 0.f;
-        o.duv_dx[0] = fmaf(a11, b0x, -(a01 * b1x)) * inv_det; o.duv_dx[1] = fmaf(a00, b1x, -(a01 * b0x)) * inv_det;
-        o.duv_dy[0] = fmaf(a11, b0y, -(a01 * b1y)) * inv_det; o.duv_dy[1] = fmaf(a00, b1y, -(a01 * b0y)) * inv_det;
-        o.shape = si.shape;
-        st3(o.p, si.p); st3(o.n, si.n); st3(o.sh_n, si.sh_n); st3(o.sh_s, si.sh_s); st3(o.sh_t, si.sh_t);
-        st3(o.dp_du, si.dp_du); st3(o.dp_dv, si.dp_dv);
-        V3 mw = -ray.d;
-        st3(o.wi, v3(dot(mw, si.sh_s), dot(mw, si.sh_t), dot(mw, si.sh_n)));
-        o.uv[0] = si.uv.x; o.uv[1] = si.uv.y;
-    }
-    q[idx] = o;
+    q[idx] = make_glint_query<true>(S, *seedp, spp, idx);
 }
 
 // ================================================================ single scatter, unbiased

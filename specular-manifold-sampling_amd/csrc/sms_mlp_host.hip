@@ -206,7 +206,7 @@ sms_status proposal_alloc(sms_ctx *ctx, int64_t n) {
 
 // network inputs -> raw outputs -> mixtures of the n queries of an estimate (prop_g), with the
 // estimator's seed slot (ms: the multi-scatter estimator's light samples)
-sms_status launch_glint_inputs(sms_ctx *ctx, const sms_glint_query *q, int64_t n, const uint64_t *seed, float *x) {
+sms_status launch_glint_inputs(sms_ctx *ctx, const GlintQuerySource &q, int64_t n, const uint64_t *seed, float *x) {
     if (n == 0) return SMS_OK;
     hipLaunchKernelGGL(k_glint_prop_inputs, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream, ctx->scene, q,
                        n, seed, x);
@@ -216,7 +216,7 @@ sms_status launch_glint_inputs(sms_ctx *ctx, const sms_glint_query *q, int64_t n
 
 // the glint estimator's mixtures: inputs, network, mixture preparation, when the proposal's
 // shape is glinty
-sms_status proposal_enqueue_glint(sms_ctx *ctx, const sms_glint_query *q, int64_t n) {
+sms_status proposal_enqueue_glint(sms_ctx *ctx, const GlintQuerySource &q, int64_t n) {
     if (!ctx->prop_mlp || ctx->prop_shape < 0 || n == 0) return SMS_OK;
     if (!(ctx->h_shapes[ctx->prop_shape].flags & SMS_FLAG_GLINTY)) return SMS_OK;
     if ((size_t) n > ctx->prop_cap) return fail(ctx, SMS_E_INVALID, "proposal buffers not prepared");
@@ -536,8 +536,8 @@ sms_status glint_proposal_io(sms_ctx *ctx, const sms_glint_query *q, int64_t n, 
     if (st) return st;
     hipLaunchKernelGGL(k_set_seed, dim3(1), dim3(64), 0, ctx->stream, ctx->seeds + SEED_API, seed);
     HIP_TRY(ctx, hipGetLastError());
-    if (!out) return launch_glint_inputs(ctx, q, n, ctx->seeds + SEED_API, x);
-    st = launch_glint_inputs(ctx, q, n, ctx->seeds + SEED_API, ctx->prop_x);
+    if (!out) return launch_glint_inputs(ctx, glint_queries_of(q), n, ctx->seeds + SEED_API, x);
+    st = launch_glint_inputs(ctx, glint_queries_of(q), n, ctx->seeds + SEED_API, ctx->prop_x);
     if (!st) st = launch_infer(ctx->prop_mlp, ctx->prop_x, n, out);
     return st;
 }

@@ -181,7 +181,8 @@ EXPORTS = ["sms_config_default", "sms_ctx_create", "sms_ctx_destroy", "sms_ctx_s
            "sms_ss_proposal_inputs", "sms_ss_proposal_outputs", "sms_ss_make_queries",
            "sms_film_accumulate", "sms_ss_autotune", "sms_ms_autotune", "sms_scene_footprint",
            "sms_ss_estimate_batch_noalloc", "sms_glint_proposal_inputs", "sms_glint_proposal_outputs",
-           "sms_path_config_default", "sms_path_samples", "sms_film_splat"]
+           "sms_path_config_default", "sms_path_samples", "sms_film_splat", "sms_glint_estimate_frame",
+           "sms_glint_frame_graph_create"]
 
 # learned seed proposal (include/sms.h)
 MLP_INPUTS = 9
@@ -252,6 +253,8 @@ def load_library(path=LIB_PATH):
     lib.sms_ss_make_queries.argtypes = [vp, C.c_int32, C.c_uint64, vp, vp]
     lib.sms_film_accumulate.argtypes = [vp, vp, vp, i64, C.c_int32, vp, vp, vp]
     lib.sms_path_config_default.argtypes = [C.POINTER(SmsPathConfig)]
+    lib.sms_glint_estimate_frame.argtypes = [vp, C.c_int32, C.c_uint64, vp, C.POINTER(SmsStats)]
+    lib.sms_glint_frame_graph_create.argtypes = [vp, C.c_int32, vp, C.POINTER(vp)]
     lib.sms_path_samples.argtypes = [vp, C.POINTER(SmsPathConfig), C.c_int32, C.c_uint64, vp, vp, C.POINTER(SmsStats)]
     lib.sms_film_splat.argtypes = [vp, vp, vp, C.c_int32, C.c_float, vp, vp]
     for name in EXPORTS:

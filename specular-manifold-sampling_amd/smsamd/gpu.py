@@ -353,6 +353,28 @@ class Context:
     def glint_graph(self, qbuf, n, outbuf):
         return self._graph(lib().sms_glint_graph_create, qbuf.ptr, n, outbuf.ptr)
 
+    def glint_frame_graph(self, spp, outbuf):
+        """Graph of the whole glint frame (camera queries generated where they are read)."""
+        return self._graph(lib().sms_glint_frame_graph_create, spp, outbuf.ptr)
+
+    def glint_estimate_frame_device(self, spp, seed, outbuf):
+        st = abi.SmsStats()
+        _check(lib().sms_glint_estimate_frame(self.h, spp, seed, outbuf.ptr, C.byref(st)), self.h,
+               "glint_estimate_frame")
+        return st.as_dict()
+
+    def glint_estimate_frame(self, spp, seed):
+        cam = self.scene.camera
+        n = cam["width"] * cam["height"] * spp
+        out = np.zeros((n, 3), np.float32)
+        buf = DeviceBuffer(self, out.nbytes)
+        try:
+            st = self.glint_estimate_frame_device(spp, seed, buf)
+            buf.download(out)
+        finally:
+            buf.free()
+        return out, st
+
     def ss_graph(self, qbuf, n, outbuf):
         return self._graph(lib().sms_ss_graph_create, qbuf.ptr, n, outbuf.ptr)
 
