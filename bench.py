@@ -118,8 +118,8 @@ def parse():
     ap.add_argument("--ms-only", action="store_true", help="run only the multi-scatter slab experiment")
     ap.add_argument("--no-ms", action="store_true", help="skip the multi-scatter slab experiment")
     ap.add_argument("--ms-spp", type=int, default=32, help="uniform-seed samples per pixel of the slab budget")
-    ap.add_argument("--rmse-reps", type=int, default=8, help="frames per method of the pool experiment")
-    ap.add_argument("--ms-reps", type=int, default=2, help="frames per method of the slab experiment")
+    ap.add_argument("--rmse-reps", type=int, default=16, help="frames per method of the pool experiment")
+    ap.add_argument("--ms-reps", type=int, default=8, help="frames per method of the slab experiment")
     ap.add_argument("--ms-ref-spp", type=int, default=128, help="slab reference samples per pixel")
     ap.add_argument("--ms-pass-spp", type=int, default=2, help="samples per pixel of a slab pass")
     ap.add_argument("--glint-rmse-only", action="store_true",
@@ -663,6 +663,15 @@ def equal_time_rmse(args, case="pool"):
     def ratio(eu, el, j):
         return float(np.sqrt(np.mean([e[j] for e in eu]) / np.mean([e[j] for e in el])))
 
+    def bootstrap(eu, el, j=0, n=4000):
+        """95 % percentile interval of the ratio over frames resampled with replacement (the
+        uniform and the learned frames independently: they share no samples)."""
+        rng = np.random.default_rng(12345)
+        u = np.array([e[j] for e in eu])
+        l_ = np.array([e[j] for e in el])
+        r = np.sqrt(u[rng.integers(0, len(u), (n, len(u)))].mean(1) / l_[rng.integers(0, len(l_), (n, len(l_)))].mean(1))
+        return [round(float(np.percentile(r, 2.5)), 3), round(float(np.percentile(r, 97.5)), 3)]
+
     def per_frame(el):
         return [round(float(np.sqrt(err_u[i][0] / el[i][0])), 3) for i in range(len(el))]
     mean_ref = float(ref.mean())
@@ -693,6 +702,8 @@ def equal_time_rmse(args, case="pool"):
         "rmse_learned": float(np.sqrt(np.mean([e[0] for e in err_l]))),
         "rmse_amortized": float(np.sqrt(np.mean([e[0] for e in err_a]))),
         "ratio": ratio(err_u, err_l, 0), "ratio_amortized": ratio(err_u, err_a, 0),
+        "ratio_ci95": bootstrap(err_u, err_l), "ratio_amortized_ci95": bootstrap(err_u, err_a),
+        "ci95": "bootstrap over frames (4000 resamples, uniform and learned frames resampled independently)",
         "ratio_per_frame": per_frame(err_l), "ratio_amortized_per_frame": per_frame(err_a),
         "max_trials_per_frame": {"uniform": max_trials_u,
                                  "learned": [s.get("bernoulli_max", 0) for s in frame_stats_l],

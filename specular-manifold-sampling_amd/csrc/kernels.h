@@ -232,14 +232,14 @@ __global__ void k_capture_scatter(const float *rec, const uint32_t *idx, int64_t
 __global__ void k_capture_gather(const float *cap, const unsigned long long *counts, unsigned long long cap_sub,
                                  float *x, float *uv, float *w);
 __global__ void k_mlp_infer(const float *p, const uint16_t *wb, const float *x9, int64_t n, float *out);
-__global__ void k_mlp_grad(const float *p, const uint16_t *wb, const float *x9, const float *uv, const float *w, int64_t n,
-                           float *partial, float *aux);
+__global__ void k_mlp_grad(const float *p, const uint16_t *wb, const uint16_t *wt, const float *x9, const float *uv,
+                           const float *w, int64_t n, float *partial, float *aux);
 __global__ void k_mlp_reduce(const float *partial, const float *aux, int nb, float *g, float *loss);
 __global__ void k_mlp_reduce_adam(const float *partial, const float *aux, int nb, float *p, float *m, float *v,
-                                  uint16_t *wb, float lr, float b1, float b2, float eps, float bc1, float bc2,
+                                  uint16_t *wb, uint16_t *wt, float lr, float b1, float b2, float eps, float bc1, float bc2,
                                   float *loss);
-__global__ void k_mlp_adam(float *p, float *g, float *m, float *v, uint16_t *wb, int n, float lr, float b1, float b2,
-                           float eps, float bc1, float bc2);
+__global__ void k_mlp_adam(float *p, float *g, float *m, float *v, uint16_t *wb, uint16_t *wt, int n, float lr, float b1,
+                           float b2, float eps, float bc1, float bc2);
 __global__ void k_ss_biased(DScene S, const sms_ss_query *Q, int64_t n, const uint64_t *seedp, float *out, EstimatorWork W);
 #define SMS_MS_KERNEL_DECLS(B)                                                                                        \
     __global__ void k_ms_walk_##B(DScene S, const sms_ms_walk_in *in, int64_t n, sms_ms_walk_out *out);               \

@@ -22,6 +22,7 @@ struct sms_mlp {
     sms_ctx *ctx = nullptr;
     float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr, *loss = nullptr;   // device, FP32
     uint16_t *wb = nullptr;                                                          // device, BF16 copy of p
+    uint16_t *wt = nullptr;               // device, BF16 W^T of layers 2-5 (the input gradients' operands)
     float *partial = nullptr, *aux = nullptr;   // device: per-block gradient rows, (sum w, sum w nll) per row
     float *loss_hist = nullptr;                 // device: per-step losses of sms_mlp_train
     int64_t loss_cap = 0;
@@ -77,17 +78,22 @@ std::vector<float> init_params(uint64_t seed) {
 
 sms_status upload_params(sms_mlp *mlp, const float *h) {
     sms_ctx *ctx = mlp->ctx;
-    std::vector<uint16_t> b(MLP_PARAMS);
+    std::vector<uint16_t> b(MLP_PARAMS), t(MLP_TPARAMS);
     for (int i = 0; i < MLP_PARAMS; ++i) b[i] = host_bf16(h[i]);
+    const int layers[4][3] = {{MLP_W2, MLP_T2, MLP_H}, {MLP_W3, MLP_T3, MLP_H}, {MLP_W4, MLP_T4, MLP_H}, {MLP_W5, MLP_T5, MLP_OUT}};
+    for (const auto &l : layers)
+        for (int n = 0; n < l[2]; ++n)
+            for (int k = 0; k < MLP_H; ++k) t[(size_t) l[1] + (size_t) k * l[2] + n] = b[(size_t) l[0] + (size_t) n * MLP_H + k];
     HIP_TRY(ctx, hipMemcpyAsync(mlp->p, h, MLP_PARAMS * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(ctx, hipMemcpyAsync(mlp->wb, b.data(), MLP_PARAMS * sizeof(uint16_t), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(mlp->wt, t.data(), MLP_TPARAMS * sizeof(uint16_t), hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return SMS_OK;
 }
 
 void free_mlp(sms_mlp *mlp) {
     for (void *q : {(void *) mlp->p, (void *) mlp->g, (void *) mlp->m, (void *) mlp->v, (void *) mlp->loss,
-                    (void *) mlp->wb, (void *) mlp->partial, (void *) mlp->aux, (void *) mlp->loss_hist})
+                    (void *) mlp->wb, (void *) mlp->wt, (void *) mlp->partial, (void *) mlp->aux, (void *) mlp->loss_hist})
         if (q) (void) hipFree(q);
     delete mlp;
 }
@@ -96,7 +102,7 @@ sms_status launch_infer(sms_mlp *mlp, const float *x, int64_t n, float *out) {
     sms_ctx *ctx = mlp->ctx;
     if (n == 0) return SMS_OK;
     const int64_t tiles = (n + MLP_TILE - 1) / MLP_TILE;
-    const unsigned grid = (unsigned) std::min<int64_t>(tiles, (int64_t) ctx->n_cu * 2);
+    const unsigned grid = (unsigned) std::min<int64_t>(tiles, (int64_t) ctx->n_cu * 3);   // three resident blocks per CU
     hipLaunchKernelGGL(k_mlp_infer, dim3(grid), dim3(256), 0, ctx->stream, mlp->p, mlp->wb, x, n, out);
     HIP_TRY(ctx, hipGetLastError());
     return SMS_OK;
@@ -269,6 +275,7 @@ sms_status sms_mlp_create(sms_ctx *ctx, uint64_t seed, sms_mlp **out) {
         if (e == hipSuccess) e = hipMalloc(&mlp->partial, (size_t) MLP_GRAD_ROWS * fb);
         if (e == hipSuccess) e = hipMalloc(&mlp->aux, 2 * MLP_GRAD_ROWS * sizeof(float));
         if (e == hipSuccess) e = hipMalloc(&mlp->wb, MLP_PARAMS * sizeof(uint16_t));
+        if (e == hipSuccess) e = hipMalloc(&mlp->wt, MLP_TPARAMS * sizeof(uint16_t));
         if (e == hipSuccess) e = hipMemsetAsync(mlp->g, 0, fb, ctx->stream);
         if (e == hipSuccess) e = hipMemsetAsync(mlp->m, 0, fb, ctx->stream);
         if (e == hipSuccess) e = hipMemsetAsync(mlp->v, 0, fb, ctx->stream);
@@ -330,8 +337,8 @@ sms_status sms_mlp_grad(sms_mlp *mlp, const float *x, const float *uv, const flo
         sms_ctx *ctx = mlp->ctx;
         if (n <= 0 || !x || !uv) return fail(ctx, SMS_E_INVALID, "sms_mlp_grad: bad arguments");
         const int rows = grad_rows(n);
-        hipLaunchKernelGGL(k_mlp_grad, dim3(rows), dim3(256), 0, ctx->stream, mlp->p, mlp->wb, x, uv, w, n, mlp->partial,
-                           mlp->aux);
+        hipLaunchKernelGGL(k_mlp_grad, dim3(rows), dim3(MLP_GRAD_THREADS), 0, ctx->stream, mlp->p, mlp->wb, mlp->wt, x, uv, w, n,
+                           mlp->partial, mlp->aux);
         HIP_TRY(ctx, hipGetLastError());
         hipLaunchKernelGGL(k_mlp_reduce, dim3((MLP_PARAMS + 63) / 64), dim3(256), 0, ctx->stream,
                            (const float *) mlp->partial, (const float *) mlp->aux, rows, mlp->g, mlp->loss);
@@ -352,7 +359,7 @@ sms_status sms_mlp_adam(sms_mlp *mlp, float lr) {
         float bc1, bc2;
         adam_corrections(mlp, &bc1, &bc2);
         hipLaunchKernelGGL(k_mlp_adam, dim3((MLP_PARAMS + 255) / 256), dim3(256), 0, ctx->stream, mlp->p, mlp->g, mlp->m,
-                           mlp->v, mlp->wb, MLP_PARAMS, lr, ADAM_B1, ADAM_B2, ADAM_EPS, bc1, bc2);
+                           mlp->v, mlp->wb, mlp->wt, MLP_PARAMS, lr, ADAM_B1, ADAM_B2, ADAM_EPS, bc1, bc2);
         HIP_TRY(ctx, hipGetLastError());
         return SMS_OK;
     });
@@ -378,14 +385,14 @@ sms_status sms_mlp_train(sms_mlp *mlp, const float *x, const float *uv, const fl
         const int rows = grad_rows(batch);
         for (int32_t i = 0; i < n_steps; ++i) {
             const int64_t o = offsets[i];
-            hipLaunchKernelGGL(k_mlp_grad, dim3(rows), dim3(256), 0, ctx->stream, mlp->p, mlp->wb, x + MLP_IN * o,
+            hipLaunchKernelGGL(k_mlp_grad, dim3(rows), dim3(MLP_GRAD_THREADS), 0, ctx->stream, mlp->p, mlp->wb, mlp->wt, x + MLP_IN * o,
                                uv + 2 * o, w ? w + o : nullptr, (int64_t) batch, mlp->partial, mlp->aux);
             HIP_TRY(ctx, hipGetLastError());
             float bc1, bc2;
             adam_corrections(mlp, &bc1, &bc2);
             hipLaunchKernelGGL(k_mlp_reduce_adam, dim3((MLP_PARAMS + 63) / 64), dim3(256), 0, ctx->stream,
                                (const float *) mlp->partial, (const float *) mlp->aux, rows, mlp->p, mlp->m, mlp->v,
-                               mlp->wb, lr, ADAM_B1, ADAM_B2, ADAM_EPS, bc1, bc2, losses ? mlp->loss_hist + i : nullptr);
+                               mlp->wb, mlp->wt, lr, ADAM_B1, ADAM_B2, ADAM_EPS, bc1, bc2, losses ? mlp->loss_hist + i : nullptr);
             HIP_TRY(ctx, hipGetLastError());
         }
         if (losses && n_steps > 0) {
