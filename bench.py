@@ -150,6 +150,8 @@ def parse():
     ap.add_argument("--train-samples", type=int, default=2_000_000, help="records the online training uses")
     ap.add_argument("--train-epochs", type=int, default=8, help="Adam steps per 8192 captured records")
     ap.add_argument("--defensive", type=float, default=0.2)
+    ap.add_argument("--anchor-radius", type=float, default=0.125,
+                    help="pool: half side of the anchored mixtures' uv box (0: mixtures over the whole uv square)")
     return ap.parse_args()
 
 
@@ -551,6 +553,8 @@ def equal_time_rmse(args, case="pool"):
                                     solver_threshold=1e-5)
         budget_spp, ref_spp, estimator = args.ms_spp, args.ms_ref_spp, "ms"
     ctx = gpu.Context(sc, cfg)
+    if case == "pool":
+        ctx.set_proposal_anchor(args.anchor_radius)
     runner = render.PassRunner(ctx, pass_spp, estimator)
     first_spp = args.first_pass_spp or pass_spp
     first_runner = render.PassRunner(ctx, first_spp, estimator) if first_spp != pass_spp else None
@@ -590,6 +594,8 @@ def equal_time_rmse(args, case="pool"):
                 render.rmse_trimmed(img, ref, ref_var_mean) ** 2)
 
     def until_budget(seed0, st):
+        """Passes until the next would end after T; then, while time is left, passes of the
+        smaller runner (every method closes its frame the same way)."""
         t0 = time.perf_counter()
         times, p = [], 0
         while True:
@@ -601,6 +607,14 @@ def equal_time_rmse(args, case="pool"):
             ctx.sync()
             times.append(time.perf_counter() - t)
             p += 1
+        if first_runner is not None and times:
+            est_small = min(times) * first_runner.pass_spp / runner.pass_spp
+            while time.perf_counter() - t0 + est_small <= T:
+                t = time.perf_counter()
+                first_runner.run(seed0 + p, film, st)
+                ctx.sync()
+                est_small = time.perf_counter() - t
+                p += 1
         return time.perf_counter() - t0
 
     # uniform seeds: the first frame's time is the budget T

@@ -479,6 +479,19 @@ sms_status sms_mlp_train(sms_mlp *mlp, const float *x, const float *uv, const fl
    sample, see sms_glint_proposal_inputs); defensive in (0, 1].  mlp NULL: uniform seeds, `shape` still names the caster whose
    solutions the training capture records; shape -1 clears the proposal. */
 sms_status sms_ctx_set_proposal(sms_ctx *ctx, sms_mlp *mlp, int32_t shape, float defensive);
+/* Anchored single-scatter mixtures (no counterpart in the reference, whose seeds are uniform):
+   with radius r > 0 and a single-scatter caster mesh as the proposal's shape, each query's
+   mixture lives in the uv box of half side r around its anchor a -- the uv where the ray from
+   the shading point toward the estimator's light sample first meets a surface, if that surface
+   is the proposal's mesh (the segment manifold next-event estimation starts from,
+   manifold_ss.cpp:247-250) --, uv = a + 2 r (xi - 1/2) for a mixture sample xi, and the
+   training capture records xi = (uv - a) / (2 r) + 1/2.  A query without an anchor seeds every
+   walk uniformly and records nothing.  r = 0 (the default): mixtures over the mesh's uv square.
+   Multi-scatter and glint proposals are not affected. */
+sms_status sms_ctx_set_proposal_anchor(sms_ctx *ctx, float radius);
+/* Anchors (device, n x 2 floats, NaN when none) of the queries of an estimate with this seed on
+   the proposal's shape, as the anchored mixtures use them. */
+sms_status sms_ss_proposal_anchors(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *anchor);
 /* Training capture of later unbiased sms_ss_estimate_batch / sms_ms_estimate_batch calls on the
    proposal shape, as records (network inputs, solution uv -- for a multi-scatter chain the uv
    of its vertex on that shape --, weight):

@@ -15,6 +15,7 @@
 #include "../include/sms.h"
 
 #include <vector>
+#include <limits>
 #include <algorithm>
 #include <thread>
 #include <atomic>
@@ -1534,18 +1535,36 @@ static int proposal_seed(const float *g, float defensive, Pcg32 *rw, V2 *out) {
     return 1;
 }
 
-/* raw: network outputs, row (qid - raw_base) for query qid */
-struct Prop { const float *raw; int shape; float defensive; UVGrid grid; int64_t raw_base = 0; };
+/* Anchor of a single-scatter query on mesh `shape`: the uv of the first surface hit by the ray
+   from the shading point toward the light sample, when that surface is `shape` (the segment
+   that manifold next-event estimation starts its walk from, manifold_ss.cpp:247-250), else
+   NaN.  Anchored mixtures place the mixture's [0,1]^2 in the uv box a + 2 r (xi - 1/2). */
+static V2 proposal_anchor(const Scene &S, int shape, V3 si_p, V3 light_p) {
+    const SI h = ray_intersect(S, make_ray(si_p, normalize(light_p - si_p)));
+    if (h.valid && h.shape == shape) return h.uv;
+    return v2(std::numeric_limits<float>::quiet_NaN(), std::numeric_limits<float>::quiet_NaN());
+}
+
+/* raw: network outputs, row (qid - raw_base) for query qid; radius > 0: anchored mixtures */
+struct Prop { const float *raw; int shape; float defensive; UVGrid grid; int64_t raw_base = 0; float radius = 0.f; };
 
 /* one walk of a query whose mixture is g: uniform seed, or the mixture's uv mapped to the
    surface and used as p_start; uv outside [0,1)^2 or in no face -> failed walk, 0 iterations */
 static PathOut proposal_walk(const Scene &S, const Prop &P, const float *g, int shape, V3 si_p, const EI &ei,
-                             V3 n_offset, Pcg32 *rw) {
+                             V3 n_offset, Pcg32 *rw, const V2 *anchor = nullptr) {
     V2 s;
     V3 p_start = v3(0.f, 0.f, 0.f);
-    if (proposal_seed(g, P.defensive, rw, &s) == 1) {
+    /* an anchored query without an anchor seeds uniformly: the defensive draw with weight 1 */
+    const bool no_anchor = anchor && !(anchor->x == anchor->x);
+    float defensive = P.defensive;
+    if (no_anchor) defensive = 1.f;
+    if (proposal_seed(g, defensive, rw, &s) == 1) {
         PathOut po;
         po.ok = false; po.iterations = 0; po.si_final = si_invalid(); po.si_init = si_invalid();
+        if (anchor) {
+            const float side = 2.f * P.radius;
+            s = v2(anchor->x + side * (s.x - 0.5f), anchor->y + side * (s.y - 0.5f));
+        }
         if (!(s.x >= 0.f && s.x < 1.f && s.y >= 0.f && s.y < 1.f)) return po;
         if (!uv_to_point(S, P.grid, s, &p_start)) return po;
         s = v2(0.f, 0.f);
@@ -1580,11 +1599,12 @@ static V3 estimate_ss(const Scene &S, const sms_ss_query &q, uint64_t seed, uint
         float gq[PROP_RAW];
         const bool use_prop = P && shape == P->shape;
         if (use_prop) gmm_prepare(P->raw + ((int64_t) qid - P->raw_base) * PROP_RAW, gq);
+        const V2 anchor = use_prop && P->radius > 0.f ? proposal_anchor(S, shape, si.p, ei.p) : v2(0.f, 0.f);
         auto run_walk = [&](uint64_t k) {
             Pcg32 rw; stream_seed(&rw, seed, qid, wbase + k + 1);
             PathOut po;
             if (use_prop) {
-                po = proposal_walk(S, *P, gq, shape, si.p, ei, n_offset, &rw);
+                po = proposal_walk(S, *P, gq, shape, si.p, ei, n_offset, &rw, P->radius > 0.f ? &anchor : nullptr);
             } else {
                 V2 s; s.x = pcg32_float(&rw); s.y = pcg32_float(&rw);
                 V2 ls = v2(0.f, 0.f);
@@ -3428,13 +3448,15 @@ int oracle_film_splat(void *h, const float *rgb, const float *pos, int spp, floa
     return 0;
 }
 
-/* estimator with the learned proposal on caster `shape`: raw = n x 48 raw network outputs */
-int oracle_ss_estimate_batch_prop(void *h, const sms_ss_query *q, int64_t n, uint64_t seed, const float *raw, int shape,
-                                  float defensive, float *out_rgb, sms_stats *stats, int threads) {
+/* estimator with the learned proposal on caster `shape`: raw = n x 48 raw network outputs;
+   radius > 0: anchored mixtures */
+int oracle_ss_estimate_batch_prop_anchored(void *h, const sms_ss_query *q, int64_t n, uint64_t seed, const float *raw,
+                                           int shape, float defensive, float radius, float *out_rgb, sms_stats *stats,
+                                           int threads) {
     const Scene &S = *(Scene *) h;
     if (shape < 0 || shape >= (int) S.shapes.size() || S.shapes[shape].type != SMS_SHAPE_MESH) return -1;
     Prop P;
-    P.raw = raw; P.shape = shape; P.defensive = defensive;
+    P.raw = raw; P.shape = shape; P.defensive = defensive; P.radius = radius;
     build_uv_grid(S, shape, P.grid);
     std::vector<Stats> per((size_t) n);
     parallel_for(n, threads, [&](int64_t i) {
@@ -3448,6 +3470,26 @@ int oracle_ss_estimate_batch_prop(void *h, const sms_ss_query *q, int64_t n, uin
         tot.btrials += s.btrials; if (s.bmax > tot.bmax) tot.bmax = s.bmax; }
     stats_out(tot, stats);
     return 0;
+}
+
+int oracle_ss_estimate_batch_prop(void *h, const sms_ss_query *q, int64_t n, uint64_t seed, const float *raw, int shape,
+                                  float defensive, float *out_rgb, sms_stats *stats, int threads) {
+    return oracle_ss_estimate_batch_prop_anchored(h, q, n, seed, raw, shape, defensive, 0.f, out_rgb, stats, threads);
+}
+
+/* anchors (n x 2, NaN: none) of the queries of an estimate on mesh `shape` */
+void oracle_prop_anchors(void *h, const sms_ss_query *q, int64_t n, uint64_t seed, int shape, float *anchor) {
+    const Scene &S = *(Scene *) h;
+    for (int64_t i = 0; i < n; ++i) {
+        V2 a = v2(std::numeric_limits<float>::quiet_NaN(), std::numeric_limits<float>::quiet_NaN());
+        if (q[i].shape >= 0) {
+            SI si = si_from_query(q[i]);
+            Pcg32 rq; stream_seed(&rq, seed, (uint64_t) i, 0);
+            EI ei = sample_emitter_interaction(S, si.p, S.emitters_ss, &rq);
+            a = proposal_anchor(S, shape, si.p, ei.p);
+        }
+        anchor[2 * i] = a.x; anchor[2 * i + 1] = a.y;
+    }
 }
 
 /* network inputs of the queries of an estimate (same light sample as the estimator) */

@@ -22,6 +22,12 @@ struct Proposal {
     const float *gmm;                 // per query: cdf[8], mu_u[8], mu_v[8], sx[8], sy[8], rho[8]
     int shape;                        // mesh caster or glinty shape the proposal seeds (-1: none)
     float defensive;                  // weight of the reference's uniform seed
+    // anchored mixtures (single scatter): per query the uv (2 floats) where the segment from the
+    // shading point to the light sample first meets the proposal's caster (NaN: none), and the
+    // half side of the uv box around it that the mixture's [0,1]^2 coordinates span
+    // (anchor null or radius 0: the mixture lives in the caster's uv square itself)
+    const float *anchor;
+    float radius;
     DUVGrid grid;
     // training capture: records (inputs[9], uv[2], weight)
     float *cap;                       // SPREAD sub-lists of cap_sub records of 12 floats
@@ -113,9 +119,24 @@ SMS_DEV int proposal_draw(const float *g, float defensive, Pcg32 &rw, V2 &out) {
     return 1;
 }
 
-SMS_DEV bool proposal_seed(const DScene &S, const Proposal &P, const float *g, Pcg32 &rw, V2 &sample, V3 &p_start) {
+// The anchor of query q (null: the mixture is absolute).
+SMS_DEV const float *proposal_anchor(const Proposal &P, int64_t q) {
+    return P.anchor && P.radius > 0.f ? P.anchor + 2 * q : nullptr;
+}
+
+// mixture coordinates -> uv: absolute, or the anchor's box a + 2 r (xi - 1/2)
+SMS_DEV V2 proposal_uv(const float *anc, float radius, V2 xi) {
+    if (!anc) return xi;
+    const float side = 2.f * radius;
+    return v2(anc[0] + side * (xi.x - 0.5f), anc[1] + side * (xi.y - 0.5f));
+}
+
+// Seed of a walk on the proposal's mesh: `anc` = the query's anchor (null: absolute mixture).
+// A query whose anchor is missing (NaN) takes the reference's uniform seed on every walk.
+SMS_DEV bool proposal_seed(const DScene &S, const Proposal &P, const float *g, const float *anc, Pcg32 &rw, V2 &sample,
+                           V3 &p_start) {
     const float t0 = pcg32_float(rw);
-    if (t0 < P.defensive) {
+    if (t0 < P.defensive || (anc && !(anc[0] == anc[0]))) {
         sample.x = pcg32_float(rw); sample.y = pcg32_float(rw);
         return true;
     }
@@ -127,9 +148,20 @@ SMS_DEV bool proposal_seed(const DScene &S, const Proposal &P, const float *g, P
     c_sincosf((2.f * PI_F) * t3, &sn, &cs);
     const float z1 = r * cs, z2 = r * sn;
     const float rho = g[40 + c], q = sqrtf(1.f - rho * rho);
-    V2 uv = v2(g[8 + c] + g[24 + c] * z1, g[16 + c] + g[32 + c] * (rho * z1 + q * z2));
+    const V2 xi = v2(g[8 + c] + g[24 + c] * z1, g[16 + c] + g[32 + c] * (rho * z1 + q * z2));
+    const V2 uv = proposal_uv(anc, P.radius, xi);
     if (!(uv.x >= 0.f && uv.x < 1.f && uv.y >= 0.f && uv.y < 1.f)) return false;
     return uv_to_point(S, P.grid, uv, p_start);
+}
+
+// Training target of a solution at `uv`: the uv itself, or its anchor-box coordinates
+// (uv - a) / (2 r) + 1/2; false when the query has no anchor.
+SMS_DEV bool proposal_target(const float *anc, float radius, V2 uv, V2 &t) {
+    if (!anc) { t = uv; return true; }
+    if (!(anc[0] == anc[0])) return false;
+    const float side = 2.f * radius;
+    t = v2((uv.x - anc[0]) / side + 0.5f, (uv.y - anc[1]) / side + 0.5f);
+    return true;
 }
 
 } // namespace smsdev

@@ -38,7 +38,7 @@ SMS_DEV PathResult ss_walk(const DScene &S, const Proposal &P, int64_t q, uint64
     V2 s = v2(0.f, 0.f);
     V3 p_start = v3(0.f, 0.f, 0.f);
     if (P.gmm && shape == P.shape) {
-        if (!proposal_seed(S, P, P.gmm + q * PROP_RAW, rw, s, p_start)) {
+        if (!proposal_seed(S, P, P.gmm + q * PROP_RAW, proposal_anchor(P, q), rw, s, p_start)) {
             PathResult res;
             res.ok = false; res.init_valid = false; res.iterations = 0;
             res.si_final = si_invalid(); res.p_init = v3(0.f, 0.f, 0.f);
@@ -53,7 +53,9 @@ SMS_DEV PathResult ss_walk(const DScene &S, const Proposal &P, int64_t q, uint64
 }
 
 // record (network inputs, solution uv, weight) for the proposal's training
-SMS_DEV void capture_sample(const DScene &S, const Proposal &P, const SI &si, const EI &ei, V2 uv, float w) {
+SMS_DEV void capture_sample(const DScene &S, const Proposal &P, const SI &si, const EI &ei, V2 uv, float w,
+                            const float *anc = nullptr) {
+    if (!proposal_target(anc, P.radius, uv, uv)) return;
     const int sub = (int) (blockIdx.x % SPREAD);
     unsigned long long pos = atomicAdd(P.cap_counts + sub * SPREAD_STRIDE, 1ull);
     if (pos >= P.cap_sub) return;

@@ -57,6 +57,10 @@ struct sms_ctx {
     int grid_g = 0, grid_shape = -1;
     int *grid_start = nullptr, *grid_faces = nullptr;
     float *prop_x = nullptr, *prop_raw = nullptr, *prop_g = nullptr; size_t prop_cap = 0;
+    // anchored single-scatter mixtures: half side of the anchor's uv box (0: absolute mixtures)
+    // and the per-query anchors (prop_cap x 2 floats)
+    float prop_radius = 0.f;
+    float *prop_anchor = nullptr;
     // capture of (inputs, solution uv) of converged first walks
     float *cap = nullptr; unsigned long long *cap_counts = nullptr; size_t cap_sub = 0; int cap_mode = 0;
     // seeds the kernels read from device memory (slots SEED_*), so that a captured frame graph

@@ -225,6 +225,7 @@ __global__ void k_ss_queries(DScene S, int spp, const uint64_t *seedp, sms_ss_qu
 __global__ void k_film_accumulate(const float *rgb, const float *w, int64_t pixels, int spp, float *image, float *image_sq,
                                   float *image_n);
 __global__ void k_prop_inputs(DScene S, const sms_ss_query *Q, int64_t n, const uint64_t *seedp, int ms, float *x);
+__global__ void k_prop_anchor(DScene S, const sms_ss_query *Q, int64_t n, const uint64_t *seedp, int shape, float *anchor);
 __global__ void k_gmm_prepare(const float *raw, int64_t n, float *g);
 __global__ void k_capture_keys(const float *cap, const unsigned long long *counts, unsigned long long cap_sub, float *rec,
                                uint64_t *keys, uint32_t *idx);

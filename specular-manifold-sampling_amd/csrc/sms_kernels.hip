@@ -103,7 +103,8 @@ __global__ void __launch_bounds__(256) k_ss_estimate(DScene S, const sms_ss_quer
             if (!r.ok) continue;
             st_conv++;
             V3 direction = normalize(r.si_final.p - si.p);
-            if (capturing(W.prop, shape, SMS_CAPTURE_SOLUTIONS)) capture_sample(S, W.prop, si, ei, r.si_final.uv, 1.f);
+            if (capturing(W.prop, shape, SMS_CAPTURE_SOLUTIONS))
+                capture_sample(S, W.prop, si, ei, r.si_final.uv, 1.f, proposal_anchor(W.prop, q));
             if (!ss_visible(S, ei, r.si_final.p)) continue;
             V3 spec = path_contribution_ss(S, si, ei, r.si_final);
             V3 dl = v3(dot(direction, si.sh_s), dot(direction, si.sh_t), dot(direction, si.sh_n));
@@ -117,7 +118,7 @@ __global__ void __launch_bounds__(256) k_ss_estimate(DScene S, const sms_ss_quer
                 st_walks++; st_iters += (unsigned long long) t.iterations;
                 if (t.ok) st_conv++;
                 if (t.ok && capturing(W.prop, shape, SMS_CAPTURE_SOLUTIONS))
-                    capture_sample(S, W.prop, si, ei, t.si_final.uv, 1.f);
+                    capture_sample(S, W.prop, si, ei, t.si_final.uv, 1.f, proposal_anchor(W.prop, q));
                 V3 dtr = normalize(t.si_final.p - si.p);
                 if (t.ok && fabsf(dot(direction, dtr) - 1.f) < cfg.uniqueness_threshold) { inv_prob = (float) k; break; }
                 k++;
@@ -130,7 +131,7 @@ __global__ void __launch_bounds__(256) k_ss_estimate(DScene S, const sms_ss_quer
                 invp[slot] = inv_prob;
                 const float wgt = mean3(&part[3 * slot]) * inv_prob;
                 if (wgt > 0.f && capturing(W.prop, shape, SMS_CAPTURE_ESTIMATES))
-                    capture_sample(S, W.prop, si, ei, r.si_final.uv, wgt);
+                    capture_sample(S, W.prop, si, ei, r.si_final.uv, wgt, proposal_anchor(W.prop, q));
             } else {
                 unsigned long long o = spread_append(W.over, sub, 1ull);
                 OverflowRec rec;
@@ -192,7 +193,8 @@ __global__ void __launch_bounds__(256) k_ss_coop(DScene S, const sms_ss_query *_
                 st_walks++; st_iters += (unsigned long long) its; st_tail += (unsigned long long) its;
                 if (conv) {
                     st_conv++;
-                    if (capturing(W.prop, shape, SMS_CAPTURE_SOLUTIONS)) capture_sample(S, W.prop, si, ei, uv_t, 1.f);
+                    if (capturing(W.prop, shape, SMS_CAPTURE_SOLUTIONS))
+                        capture_sample(S, W.prop, si, ei, uv_t, 1.f, proposal_anchor(W.prop, q));
                 }
             }
         }
@@ -202,7 +204,8 @@ __global__ void __launch_bounds__(256) k_ss_coop(DScene S, const sms_ss_query *_
             st_btrials += tr; st_bmax = st_bmax > tr ? st_bmax : tr;
             const float wgt = found > 0 ? mean3(&part[3 * rec.q]) * (float) found : 0.f;
             if (wgt > 0.f && capturing(W.prop, shape, SMS_CAPTURE_ESTIMATES))
-                capture_sample(S, W.prop, si, ei, v2(__int_as_float(rec.shapes[0]), __int_as_float(rec.shapes[1])), wgt);
+                capture_sample(S, W.prop, si, ei, v2(__int_as_float(rec.shapes[0]), __int_as_float(rec.shapes[1])), wgt,
+                               proposal_anchor(W.prop, q));
         }
     }
     stat_add(W.stats + 0, st_walks); stat_add(W.stats + 1, st_conv); stat_add(W.stats + 2, st_iters);
@@ -434,6 +437,28 @@ __global__ void k_prop_inputs(DScene S, const sms_ss_query *__restrict__ Q, int6
     float r[9];
     proposal_inputs(S, si.p, si.sh_n, ei.p, r);
     for (int j = 0; j < 9; ++j) x[q * 9 + j] = r[j];
+}
+
+// Anchor of each single-scatter query on mesh `shape`: the uv of the first surface the ray from
+// the shading point toward the estimator's light sample meets, if that surface is `shape`
+// (the segment manifold next-event estimation starts its walk from), else NaN.
+__global__ void k_prop_anchor(DScene S, const sms_ss_query *__restrict__ Q, int64_t n, const uint64_t *__restrict__ seedp,
+                              int shape, float *__restrict__ anchor) {
+    const uint64_t seed = *seedp;
+    const int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const sms_ss_query g = Q[q];
+    float a0 = __int_as_float(0x7fc00000), a1 = a0;
+    if (g.shape >= 0) {
+        const SI si = ss_query_si(g);
+        Pcg32 rq;
+        stream_seed(rq, query_key(seed, (uint64_t) q), 0);
+        const EI ei = sample_emitter_interaction(S, si.p, S.emitters_ss, S.n_emitters_ss, rq);
+        const SI h = ray_intersect(S, make_ray(si.p, normalize(ei.p - si.p)));
+        if (h.valid && h.shape == shape) { a0 = h.uv.x; a1 = h.uv.y; }
+    }
+    anchor[2 * q] = a0;
+    anchor[2 * q + 1] = a1;
 }
 
 __global__ void k_gmm_prepare(const float *__restrict__ raw, int64_t n, float *__restrict__ g) {

@@ -126,9 +126,10 @@ sms_status ensure_prop(sms_ctx *ctx, size_t n) {
     if (n <= ctx->prop_cap) return SMS_OK;
     ctx->gen++;
     const size_t cap = std::max<size_t>(n, 2 * ctx->prop_cap);
-    for (float **b : {&ctx->prop_x, &ctx->prop_raw, &ctx->prop_g})
+    for (float **b : {&ctx->prop_x, &ctx->prop_raw, &ctx->prop_g, &ctx->prop_anchor})
         if (*b) { (void) hipFree(*b); *b = nullptr; }
     ctx->prop_cap = 0;
+    HIP_TRY(ctx, hipMalloc(&ctx->prop_anchor, cap * 2 * sizeof(float)));
     HIP_TRY(ctx, hipMalloc(&ctx->prop_x, cap * MLP_IN * sizeof(float)));
     HIP_TRY(ctx, hipMalloc(&ctx->prop_raw, cap * MLP_OUT * sizeof(float)));
     HIP_TRY(ctx, hipMalloc(&ctx->prop_g, cap * MLP_OUT * sizeof(float)));
@@ -178,6 +179,15 @@ sms_status build_uv_grid(sms_ctx *ctx, int shape) {
     return SMS_OK;
 }
 
+// anchors of n single-scatter queries with the seed of device slot `seed` on the proposal's mesh
+sms_status launch_anchors(sms_ctx *ctx, const sms_ss_query *q, int64_t n, const uint64_t *seed, float *anchor) {
+    if (n == 0) return SMS_OK;
+    hipLaunchKernelGGL(k_prop_anchor, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream, ctx->scene, q, n,
+                       seed, ctx->prop_shape, anchor);
+    HIP_TRY(ctx, hipGetLastError());
+    return SMS_OK;
+}
+
 // network inputs of n queries with the seed of device slot `seed` (ms: multi-scatter emitters)
 sms_status launch_inputs(sms_ctx *ctx, const sms_ss_query *q, int64_t n, const uint64_t *seed, int ms, float *x) {
     if (n == 0) return SMS_OK;
@@ -189,6 +199,14 @@ sms_status launch_inputs(sms_ctx *ctx, const sms_ss_query *q, int64_t n, const u
 
 } // namespace
 
+// anchored single-scatter mixtures in use: a radius, a single-scatter caster mesh as the
+// proposal's shape, and a network or a training capture that reads the anchors
+bool anchored(const sms_ctx *ctx) {
+    if (!(ctx->prop_radius > 0.f) || ctx->prop_shape < 0 || !(ctx->prop_mlp || ctx->cap_sub)) return false;
+    const DShape &sh = ctx->h_shapes[ctx->prop_shape];
+    return sh.type == SMS_SHAPE_MESH && (sh.flags & SMS_FLAG_CAUSTIC_CASTER_SS);
+}
+
 // shared with sms_host.hip
 Proposal proposal_of(sms_ctx *ctx) {
     Proposal P;
@@ -196,6 +214,8 @@ Proposal proposal_of(sms_ctx *ctx) {
     P.gmm = ctx->prop_mlp && ctx->prop_shape >= 0 ? ctx->prop_g : nullptr;
     P.shape = ctx->prop_shape;
     P.defensive = ctx->prop_defensive;
+    P.anchor = anchored(ctx) ? ctx->prop_anchor : nullptr;
+    P.radius = P.anchor ? ctx->prop_radius : 0.f;
     P.grid.g = ctx->grid_g; P.grid.start = ctx->grid_start; P.grid.faces = ctx->grid_faces;
     P.cap = ctx->cap_sub ? ctx->cap : nullptr;
     P.cap_counts = ctx->cap_counts;
@@ -204,9 +224,9 @@ Proposal proposal_of(sms_ctx *ctx) {
     return P;
 }
 
-// buffers of the proposal's per-query mixtures for estimates of n queries
+// buffers of the proposal's per-query mixtures (and anchors) for estimates of n queries
 sms_status proposal_alloc(sms_ctx *ctx, int64_t n) {
-    if (!ctx->prop_mlp || ctx->prop_shape < 0 || n == 0) return SMS_OK;
+    if (!(ctx->prop_mlp || anchored(ctx)) || ctx->prop_shape < 0 || n == 0) return SMS_OK;
     return ensure_prop(ctx, (size_t) n);
 }
 
@@ -236,6 +256,11 @@ sms_status proposal_enqueue_glint(sms_ctx *ctx, const GlintQuerySource &q, int64
 }
 
 sms_status proposal_enqueue(sms_ctx *ctx, const sms_ss_query *q, int64_t n, int ms) {
+    if (!ms && n > 0 && anchored(ctx)) {
+        if ((size_t) n > ctx->prop_cap) return fail(ctx, SMS_E_INVALID, "proposal buffers not prepared");
+        sms_status st = launch_anchors(ctx, q, n, ctx->seeds + SEED_EST, ctx->prop_anchor);
+        if (st) return st;
+    }
     if (!ctx->prop_mlp || ctx->prop_shape < 0 || n == 0) return SMS_OK;
     // only an estimator whose casters include the proposal's shape uses the mixtures
     if (!(ctx->h_shapes[ctx->prop_shape].flags & (ms ? SMS_FLAG_CAUSTIC_CASTER_MS : SMS_FLAG_CAUSTIC_CASTER_SS)))
@@ -252,7 +277,7 @@ sms_status proposal_enqueue(sms_ctx *ctx, const sms_ss_query *q, int64_t n, int 
 
 void proposal_release(sms_ctx *ctx) {
     for (void *q : {(void *) ctx->grid_start, (void *) ctx->grid_faces, (void *) ctx->prop_x, (void *) ctx->prop_raw,
-                    (void *) ctx->prop_g, (void *) ctx->cap, (void *) ctx->cap_counts})
+                    (void *) ctx->prop_g, (void *) ctx->prop_anchor, (void *) ctx->cap, (void *) ctx->cap_counts})
         if (q) (void) hipFree(q);
     if (ctx->prop_mlp) ctx->prop_mlp = nullptr;
 }
@@ -431,6 +456,26 @@ sms_status sms_ctx_set_proposal(sms_ctx *ctx, sms_mlp *mlp, int32_t shape, float
         ctx->prop_defensive = mlp ? defensive : 1.f;
         ctx->gen++;
         return SMS_OK;
+    });
+}
+
+sms_status sms_ctx_set_proposal_anchor(sms_ctx *ctx, float radius) {
+    return guarded(ctx, [&]() -> sms_status {
+        if (!ctx || !(radius >= 0.f && radius <= 0.5f))
+            return fail(ctx, SMS_E_INVALID, "sms_ctx_set_proposal_anchor: radius must be in [0, 1/2]");
+        ctx->prop_radius = radius;
+        ctx->gen++;
+        return SMS_OK;
+    });
+}
+
+sms_status sms_ss_proposal_anchors(sms_ctx *ctx, const sms_ss_query *q, int64_t n, uint64_t seed, float *anchor) {
+    return guarded(ctx, [&]() -> sms_status {
+        if (!ctx || n < 0 || (n > 0 && (!q || !anchor))) return fail(ctx, SMS_E_INVALID, "proposal anchors: bad arguments");
+        if (ctx->prop_shape < 0) return fail(ctx, SMS_E_INVALID, "proposal anchors: no proposal shape set");
+        hipLaunchKernelGGL(k_set_seed, dim3(1), dim3(64), 0, ctx->stream, ctx->seeds + SEED_API, seed);
+        HIP_TRY(ctx, hipGetLastError());
+        return launch_anchors(ctx, q, n, ctx->seeds + SEED_API, anchor);
     });
 }
 

@@ -58,7 +58,7 @@ SMS_DEV void ms_walk_k(const DScene &S, const Proposal &P, const float *g, uint6
     V2 s = v2(0.f, 0.f);
     V3 p_start = v3(0.f, 0.f, 0.f);
     if (g) {
-        if (!proposal_seed(S, P, g, rw, s, p_start)) {
+        if (!proposal_seed(S, P, g, nullptr, rw, s, p_start)) {
             w.ok = false; w.seed_ok = false; w.iterations = 0; w.path.n = 0;
             for (int i = 0; i < MAXB; ++i) w.seed_shapes[i] = shapes ? shapes[i] : -1;
             return;

@@ -99,7 +99,7 @@ __device__ __forceinline__ void ss_walker_body(const DScene &S, const sms_ss_que
         invp[L.slot] = inv_prob;
         const float wgt = mean3(&part[3 * L.slot]) * inv_prob;
         if (wgt > 0.f && capturing(P, L.shape, SMS_CAPTURE_ESTIMATES))
-            capture_sample(S, P, ss_query_si(Q[L.q]), ei_lane, L.uv_first, wgt);
+            capture_sample(S, P, ss_query_si(Q[L.q]), ei_lane, L.uv_first, wgt, proposal_anchor(P, L.q));
         end_chain();
     };
     // the chain's trial k did not re-find its solution: the next trial, or the cap, or the
@@ -131,7 +131,8 @@ __device__ __forceinline__ void ss_walker_body(const DScene &S, const sms_ss_que
     auto walk_converged = [&](int iterations, const MV &vtx, const SI &hs) {
         L.walks++; L.iters += (unsigned) iterations;
         L.conv++;
-        if (capturing(P, L.shape, SMS_CAPTURE_SOLUTIONS)) capture_sample(S, P, ss_query_si(Q[L.q]), ei_lane, hs.uv, 1.f);
+        if (capturing(P, L.shape, SMS_CAPTURE_SOLUTIONS))
+            capture_sample(S, P, ss_query_si(Q[L.q]), ei_lane, hs.uv, 1.f, proposal_anchor(P, L.q));
         if (L.k == 0) {
             L.direction = normalize(vtx.p - L.si_p);
             // the contribution is evaluated now (before the shadow ray) so the solution's
@@ -186,7 +187,7 @@ __device__ __forceinline__ void ss_walker_body(const DScene &S, const sms_ss_que
                     V2 s = v2(0.f, 0.f);
                     V3 p_start = v3(0.f, 0.f, 0.f);
                     if (P.gmm && L.shape == P.shape) {
-                        if (!proposal_seed(S, P, P.gmm + L.q * PROP_RAW, rw, s, p_start)) {
+                        if (!proposal_seed(S, P, P.gmm + L.q * PROP_RAW, proposal_anchor(P, L.q), rw, s, p_start)) {
                             walk_failed(0);
                             continue;
                         }

@@ -182,7 +182,7 @@ EXPORTS = ["sms_config_default", "sms_ctx_create", "sms_ctx_destroy", "sms_ctx_s
            "sms_film_accumulate", "sms_ss_autotune", "sms_ms_autotune", "sms_scene_footprint",
            "sms_ss_estimate_batch_noalloc", "sms_glint_proposal_inputs", "sms_glint_proposal_outputs",
            "sms_path_config_default", "sms_path_samples", "sms_film_splat", "sms_glint_estimate_frame",
-           "sms_glint_frame_graph_create"]
+           "sms_glint_frame_graph_create", "sms_ctx_set_proposal_anchor", "sms_ss_proposal_anchors"]
 
 # learned seed proposal (include/sms.h)
 MLP_INPUTS = 9
@@ -241,6 +241,8 @@ def load_library(path=LIB_PATH):
     lib.sms_graph_stats.argtypes = [vp, C.POINTER(SmsStats), C.c_int32]
     lib.sms_graph_destroy.argtypes = [vp]
     lib.sms_ctx_set_proposal.argtypes = [vp, vp, C.c_int32, C.c_float]
+    lib.sms_ctx_set_proposal_anchor.argtypes = [vp, C.c_float]
+    lib.sms_ss_proposal_anchors.argtypes = [vp, vp, i64, C.c_uint64, vp]
     lib.sms_ctx_set_training_capture.argtypes = [vp, i64, C.c_int32]
     lib.sms_ctx_training_samples.argtypes = [vp, C.POINTER(i64), vp, vp, vp]
     lib.sms_ctx_clear_training.argtypes = [vp]

@@ -291,6 +291,28 @@ class Context:
         _check(lib().sms_ctx_set_proposal(self.h, mlp.h if mlp is not None else None, shape, defensive), self.h,
                "sms_ctx_set_proposal")
 
+    def set_proposal_anchor(self, radius):
+        """Anchored single-scatter mixtures (uv box of half side `radius` around each query's
+        anchor); 0: mixtures over the caster's uv square."""
+        _check(lib().sms_ctx_set_proposal_anchor(self.h, radius), self.h, "sms_ctx_set_proposal_anchor")
+
+    def ss_proposal_anchors(self, queries, seed):
+        """Anchors (n x 2, NaN: none) of single-scatter queries (structured array) for an
+        estimate with `seed`."""
+        n = len(queries)
+        qb = DeviceBuffer(self, max(queries.nbytes, 4))
+        ab = DeviceBuffer(self, max(n * 8, 4))
+        try:
+            qb.upload(queries)
+            _check(lib().sms_ss_proposal_anchors(self.h, qb.ptr, n, seed, ab.ptr), self.h, "sms_ss_proposal_anchors")
+            a = np.zeros((n, 2), np.float32)
+            if n:
+                ab.download(a)
+        finally:
+            qb.free()
+            ab.free()
+        return a
+
     def set_training_capture(self, capacity, mode=abi.CAPTURE_SOLUTIONS):
         _check(lib().sms_ctx_set_training_capture(self.h, capacity, mode), self.h, "sms_ctx_set_training_capture")
 

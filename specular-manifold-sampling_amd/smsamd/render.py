@@ -114,7 +114,8 @@ def render_online(ctx, runner, film, mlp, caster, budget_s, samples=2_000_000, b
     max_steps_per_pass) on windows at random offsets of its records run before the next pass,
     until fewer than `batch` of `samples` records are left (stop_when_trained: the frame ends
     then).  Every pass
-    counts in `film`; passes stop when the next one would end after the budget.  Returns a dict
+    counts in `film`; full passes stop when the next one would end after the budget, then passes
+    of `uniform_runner` (with the proposal as trained) fill the time left.  Returns a dict
     of passes, steps, losses (every 32nd step), used (records trained on) and train_s (seconds
     spent training)."""
     rng = np.random.default_rng(rng_seed)
@@ -124,13 +125,26 @@ def render_online(ctx, runner, film, mlp, caster, budget_s, samples=2_000_000, b
     ctx.set_training_capture(max(runner.n, uniform_runner.n if uniform_runner else 0), mode)
     t0 = time.perf_counter()
     out = {"passes": 0, "steps": 0, "losses": [], "used": 0, "train_s": 0.0}
-    capturing, last = True, 0.0
+    capturing, last, last_small = True, 0.0, None
     try:
         while True:
             t = time.perf_counter()
-            if out["passes"] and (t - t0 + last > budget_s or (stop_when_trained and not capturing)):
-                break
             first = uniform_runner is not None and out["passes"] < uniform_passes
+            if out["passes"] and (t - t0 + last > budget_s or (stop_when_trained and not capturing)):
+                # close the frame with passes of the smaller runner while they fit
+                if stop_when_trained or uniform_runner is None or last_small is None:
+                    break
+                est = min(last_small, last * uniform_runner.pass_spp / runner.pass_spp)
+                if t - t0 + est > budget_s:
+                    break
+                if not first:
+                    ctx.set_proposal(mlp, caster, defensive)
+                uniform_runner.run(seed0 + out["passes"], film, stats)
+                out["passes"] += 1
+                ctx.sync()
+                last_small = time.perf_counter() - t
+                last = float("inf")
+                continue
             (uniform_runner if first else runner).run(seed0 + out["passes"], film, stats)
             out["passes"] += 1
             if capturing:
@@ -159,7 +173,10 @@ def render_online(ctx, runner, film, mlp, caster, budget_s, samples=2_000_000, b
                 if out["passes"] >= uniform_passes:
                     ctx.set_proposal(mlp, caster, defensive)
             ctx.sync()
-            last = time.perf_counter() - t
+            if first:
+                last_small = time.perf_counter() - t
+            else:
+                last = time.perf_counter() - t
     finally:
         ctx.set_training_capture(0)
     return out

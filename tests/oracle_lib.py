@@ -64,6 +64,9 @@ def load(path=ORACLE_SO):
     lib.oracle_sample_emitter_interaction.argtypes = [vp, vp, C.c_uint64, C.c_uint64, vp]
     lib.oracle_ss_estimate_batch_prop.argtypes = [vp, vp, i64, C.c_uint64, vp, C.c_int, C.c_float, vp,
                                                   C.POINTER(abi.SmsStats), C.c_int]
+    lib.oracle_ss_estimate_batch_prop_anchored.argtypes = [vp, vp, i64, C.c_uint64, vp, C.c_int, C.c_float, C.c_float,
+                                                          vp, C.POINTER(abi.SmsStats), C.c_int]
+    lib.oracle_prop_anchors.argtypes = [vp, vp, i64, C.c_uint64, C.c_int, vp]
     lib.oracle_prop_inputs.argtypes = [vp, vp, i64, C.c_uint64, vp]
     lib.oracle_prop_inputs_ms.argtypes = [vp, vp, i64, C.c_uint64, vp]
     lib.oracle_ms_estimate_batch_prop.argtypes = [vp, vp, i64, C.c_uint64, vp, C.c_int, C.c_float, vp,
@@ -154,16 +157,24 @@ class Oracle:
         return q, w
 
     # ---------------------------------------------------------------- learned seed proposal
-    def ss_estimate_prop(self, q, seed, raw, shape, defensive):
-        """Estimator with the defensive-mixture seed of caster ``shape``; raw: (n, 48) outputs."""
+    def ss_estimate_prop(self, q, seed, raw, shape, defensive, radius=0.0):
+        """Estimator with the defensive-mixture seed of caster ``shape``; raw: (n, 48) outputs;
+        radius > 0: anchored mixtures."""
         raw = np.ascontiguousarray(raw, np.float32)
         assert raw.shape == (len(q), 48)
         out = np.zeros((len(q), 3), np.float32)
         st = abi.SmsStats()
-        rc = self.lib.oracle_ss_estimate_batch_prop(self.h, q.ctypes.data, len(q), seed, raw.ctypes.data, shape,
-                                                    defensive, out.ctypes.data, C.byref(st), self.threads)
+        rc = self.lib.oracle_ss_estimate_batch_prop_anchored(self.h, q.ctypes.data, len(q), seed, raw.ctypes.data,
+                                                             shape, defensive, radius, out.ctypes.data, C.byref(st),
+                                                             self.threads)
         assert rc == 0, rc
         return out, st.as_dict()
+
+    def prop_anchors(self, q, seed, shape):
+        """Anchors (n, 2; NaN: none) of the queries of an estimate on mesh ``shape``."""
+        a = np.zeros((len(q), 2), np.float32)
+        self.lib.oracle_prop_anchors(self.h, q.ctypes.data, len(q), seed, shape, a.ctypes.data)
+        return a
 
     def prop_inputs(self, q, seed):
         x = np.zeros((len(q), 9), np.float32)

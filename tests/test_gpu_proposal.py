@@ -254,8 +254,8 @@ def _constant_mixture(mlp, centre=(0.5, 0.5), sigma=0.08):
     mlp.set_params(p)
 
 
-@pytest.mark.parametrize("kind", ["random", "constant"])
-def test_estimator_with_proposal_matches_oracle(gpu, oracle_mod, pool_scene, kind):
+@pytest.mark.parametrize("kind,radius", [("random", 0.0), ("constant", 0.0), ("random", 0.125), ("constant", 0.125)])
+def test_estimator_with_proposal_matches_oracle(gpu, oracle_mod, pool_scene, kind, radius):
     cfg = abi.SmsConfig.default(max_trials=100000)
     ctx = gpu.Context(pool_scene, cfg)
     orc = oracle_mod.Oracle(pool_scene, cfg)
@@ -263,9 +263,10 @@ def test_estimator_with_proposal_matches_oracle(gpu, oracle_mod, pool_scene, kin
     if kind == "constant":
         _constant_mixture(mlp)
     ctx.set_proposal(mlp, 0, 0.2)
+    ctx.set_proposal_anchor(radius)
     q = _floor_queries(pool_scene, 3000, 4, extent=0.7)
     raw = ctx.proposal_outputs(q, 9)
-    ro, so = orc.ss_estimate_prop(q, 9, raw, 0, 0.2)
+    ro, so = orc.ss_estimate_prop(q, 9, raw, 0, 0.2, radius=radius)
     for first_pass in (0, 1):
         ctx.set_schedule(ss_first_pass=first_pass)
         rg, sg = ctx.ss_estimate(q, 9)
@@ -280,12 +281,62 @@ def test_estimator_with_proposal_matches_oracle(gpu, oracle_mod, pool_scene, kin
     ctx.set_config(cfg_b)
     orc.set_config(cfg_b)
     rg, sg = ctx.ss_estimate(q[:1000], 9)
-    ro, so = orc.ss_estimate_prop(q[:1000], 9, raw[:1000], 0, 0.2)
+    ro, so = orc.ss_estimate_prop(q[:1000], 9, raw[:1000], 0, 0.2, radius=radius)
     assert sg["walks"] == so["walks"] and sg["walks_converged"] == so["walks_converged"]
     assert np.array_equal(rg, ro, equal_nan=True)
     mlp.close()
     ctx.close()
     orc.close()
+
+
+def test_anchors_match_oracle(gpu, oracle_mod, pool_scene):
+    """Anchors of camera-pass queries (every pixel, receivers or not) bit-equal to the oracle's."""
+    cfg = abi.SmsConfig.default(max_trials=100000)
+    ctx = gpu.Context(pool_scene, cfg)
+    orc = oracle_mod.Oracle(pool_scene, cfg)
+    ctx.set_proposal(None, 0)
+    q, _ = orc.ss_queries(1, 21)
+    q = np.concatenate([q, _floor_queries(pool_scene, 4000, 6, extent=1.0)])
+    ag = ctx.ss_proposal_anchors(q, 13)
+    ao = orc.prop_anchors(q, 13, 0)
+    have = np.isfinite(ao[:, 0])
+    print(f"{have.sum()} of {len(q)} queries have anchors", flush=True)
+    assert have.sum() > len(q) // 4 and (~have).sum() > 0
+    assert np.array_equal(ag, ao, equal_nan=True)
+    ctx.close()
+    orc.close()
+
+
+def test_anchored_capture(gpu, oracle_mod, pool_scene):
+    """Anchored training records: the same estimates as absolute records, targets mapped to the
+    anchor's box xi = (uv - a) / (2 r) + 1/2; queries without anchors record nothing."""
+    cfg = abi.SmsConfig.default(max_trials=100000)
+    ctx = gpu.Context(pool_scene, cfg)
+    ctx.set_proposal(None, 0)
+    q = _floor_queries(pool_scene, 4000, 5, extent=1.0)
+    recs = {}
+    for radius in (0.0, 0.125):
+        ctx.set_proposal_anchor(radius)
+        ctx.set_training_capture(1 << 16, abi.CAPTURE_ESTIMATES)
+        rg, _ = ctx.ss_estimate(q, 3)
+        recs[radius] = ctx.training_samples()
+        ctx.set_training_capture(0)
+    x0, uv0, w0 = recs[0.0]
+    x1, xi1, w1 = recs[0.125]
+    xin = ctx.proposal_inputs(q, 3)
+    rows = {r.tobytes(): i for i, r in enumerate(xin)}
+    i0 = np.array([rows[r.tobytes()] for r in x0])
+    i1 = np.array([rows[r.tobytes()] for r in x1])
+    a = ctx.ss_proposal_anchors(q, 3)
+    have = np.isfinite(a[:, 0])
+    assert sorted(i1.tolist()) == sorted(i for i in i0.tolist() if have[i])
+    assert len(i1) < len(i0) or have[i0].all()
+    m0 = dict(zip(i0.tolist(), zip(uv0, w0)))
+    for j, i in enumerate(i1.tolist()):
+        uv, w = m0[i]
+        assert w1[j] == w
+        np.testing.assert_allclose(a[i] + 0.25 * (xi1[j] - 0.5), uv, atol=2e-6)
+    ctx.close()
 
 
 def test_capture(gpu, oracle_mod, pool_scene):

@@ -290,6 +290,67 @@ def test_mixture_estimator_is_unbiased(small_pool):
     orc.close()
 
 
+def test_anchor_closed_form_on_flat_water():
+    """Flat water at y = 0 over the floor at y = -1: a query's anchor is the uv where the segment
+    from its floor point to its light sample crosses the plane (u = (x + 1) / 2, v = (z + 1) / 2
+    on the grid mesh), and NaN where the segment passes beside the water."""
+    sc = scenes.pool(width=32, height=32, n=32, n_waves=0, seed=3)
+    orc = oracle_lib.Oracle(sc, abi.SmsConfig.default())
+    q = workloads.plane_queries(sc, 600, 2, extent=2.2, shape=1)
+    q["p"][:, 1] = -1.0
+    a = orc.prop_anchors(q, 7, 0)
+    c, inv = orc.scene_box()
+    lp = orc.prop_inputs(q, 7)[:, 6:9].astype(np.float64) / inv + c
+    p = q["p"].astype(np.float64)
+    t = (0.0 - p[:, 1]) / (lp[:, 1] - p[:, 1])
+    hit = p + t[:, None] * (lp - p)
+    inside = (np.abs(hit[:, 0]) < 0.999) & (np.abs(hit[:, 2]) < 0.999)
+    outside = (np.abs(hit[:, 0]) > 1.001) | (np.abs(hit[:, 2]) > 1.001)
+    assert inside.sum() > 100 and outside.sum() > 50
+    assert np.isfinite(a[inside]).all() and np.isnan(a[outside]).all()
+    np.testing.assert_allclose(a[inside, 0], (hit[inside, 0] + 1) / 2, atol=2e-5)
+    np.testing.assert_allclose(a[inside, 1], (hit[inside, 2] + 1) / 2, atol=2e-5)
+    # a query without a receiver has no anchor
+    q2 = q[:4].copy()
+    q2["shape"] = -1
+    assert np.isnan(orc.prop_anchors(q2, 7, 0)).all()
+    orc.close()
+
+
+def test_anchored_mixture_estimator_is_unbiased(small_pool):
+    """Anchored mixtures (uv box of half side 0.125 around each query's anchor) centred on the
+    anchor: the estimate's mean equals the uniform one within the standard errors, walks
+    converge more often than uniform ones, and a mixture far off the solutions (a box of half
+    side 0.5 centred a quarter of the uv square away) still leaves the mean unbiased."""
+    cfg = abi.SmsConfig.default(max_trials=100000)
+    orc = oracle_lib.Oracle(small_pool, cfg)
+    q = workloads.plane_queries(small_pool, 24, 5, extent=0.6, shape=1)
+    q["p"][:, 1] = -1.0
+    raw = np.zeros((len(q), 48), np.float32)
+    for c in range(8):
+        raw[:, 6 * c + 1] = 0.05 * (c - 3.5)
+        raw[:, 6 * c + 2] = -0.05 * (c - 3.5)
+        raw[:, 6 * c + 3] = raw[:, 6 * c + 4] = math.log(0.15)
+    reps = 60
+    qq = np.concatenate([q] * reps)
+    rr = np.concatenate([raw] * reps)
+    uni, su = orc.ss_estimate(qq, 3)
+    mix, sm = orc.ss_estimate_prop(qq, 3, rr, 0, 0.2, radius=0.125)
+    off = rr.copy()
+    off[:, 1::6] = 2.0                                   # mixture means at xi = 0.88: uv = anchor + 0.38
+    far, sf = orc.ss_estimate_prop(qq, 3, off, 0, 0.2, radius=0.5)
+    lu, lm, lf = (r[:, 0].astype(np.float64) for r in (uni, mix, far))
+    se = math.sqrt(lu.var() / len(lu) + lm.var() / len(lm))
+    se_f = math.sqrt(lu.var() / len(lu) + lf.var() / len(lf))
+    print(f"uniform {lu.mean():.5f} anchored {lm.mean():.5f} se {se:.5f} far {lf.mean():.5f} se {se_f:.5f}")
+    assert lu.mean() > 0
+    assert abs(lu.mean() - lm.mean()) < 4 * se
+    assert abs(lu.mean() - lf.mean()) < 4 * se_f
+    assert sm["walks_converged"] / sm["walks"] > su["walks_converged"] / su["walks"]
+    assert sf["walks_converged"] / sf["walks"] < sm["walks_converged"] / sm["walks"]
+    orc.close()
+
+
 def test_weighted_gradient_matches_torch_and_reduces_to_unweighted():
     """Weighted NLL: unit weights give the unweighted gradient; integer weights equal repeating
     samples; the weighted gradient matches torch autograd of the weighted loss."""
