@@ -127,6 +127,8 @@ typedef struct sms_emitter_desc {
 typedef struct sms_normalmap_desc {
     int32_t res;               /* square power-of-two resolution                           */
     const uint8_t *rgb;        /* res*res*3 bytes, row-major, byte/255 (no linearisation)  */
+    const float *rgb32f;       /* or, when rgb is NULL: res*res*3 floats, row-major, as an
+                                  OpenEXR file stores them (src/librender/normalmap.cpp:20-22) */
 } sms_normalmap_desc;
 
 typedef struct sms_camera_desc {
@@ -146,6 +148,84 @@ typedef struct sms_scene_desc {
     const sms_normalmap_desc *normalmaps;
     sms_camera_desc camera;
 } sms_scene_desc;
+
+/* ---------------------------------------------------------------- scene and asset files
+   Host-side readers (no device work) of the reference's scene inputs into the arrays above.
+     sms_obj_load      Wavefront OBJ (src/shapes/obj.cpp:95-345): positions (and normals, with
+                       the inverse transpose) through `to_world` (4x4 row-major, NULL: identity),
+                       corners de-duplicated by their (v, vt, vn) triple in order of first use,
+                       polygons fanned, texture v flipped under SMS_OBJ_FLIP_TEXCOORDS (the
+                       reference's default), missing normals recomputed angle-weighted
+                       (src/librender/mesh.cpp:420-470) unless SMS_OBJ_FACE_NORMALS.
+     sms_image_load    OpenEXR scanline files (NONE, RLE, ZIPS, ZIP; HALF, FLOAT, UINT channels;
+                       R, G, B or Y) and PNG (1-16 bit grey, grey + alpha, RGB, RGBA, palette;
+                       not interlaced) as float RGB -- EXR values as stored, PNG code values /
+                       (2^depth - 1), not linearised (src/librender/normalmap.cpp:14-22).
+     sms_scene_load    A Mitsuba 2 scene file (src/libcore/xml.cpp): <default> parameters and
+                       `defines` ("name=value", overriding them), $name substitution, nested and
+                       referenced (id / <ref>) plugins, transforms composed as the reference does
+                       (each op multiplied on the left); shapes sphere / rectangle / obj with
+                       the caustic_* and glinty flags; bsdfs diffuse, dielectric, conductor,
+                       roughconductor (beckmann), normalmap (over a dielectric or conductor;
+                       the file read by sms_image_load), twosided; emitters area (in a shape),
+                       point, directional with caustic_emitter_single / _multi; a perspective
+                       sensor (fov and fov_axis, to_world), hdrfilm size, gaussian or box
+                       rfilter, independent sampler; the integrator's type and properties are
+                       returned as they are (the SMS and path integrators read them).  Anything
+                       else is SMS_E_UNSUPPORTED, or with SMS_LOAD_SUBSTITUTE replaced --
+                       textures by their image's mean colour (or 0.5 grey), other bsdfs by a
+                       diffuse bsdf of their diffuse colour, named conductor materials by a
+                       perfect mirror -- or dropped (other shapes and emitters), each recorded
+                       as a note.  Ignored properties are noted too.
+   Errors are reported by status and, when err is not NULL, a message in err[err_len]. */
+typedef struct sms_mesh_data {
+    int64_t n_vertices, n_faces;
+    float *positions;          /* n_vertices x 3                                           */
+    float *normals;            /* n_vertices x 3, or NULL                                  */
+    float *texcoords;          /* n_vertices x 2, or NULL                                  */
+    uint32_t *faces;           /* n_faces x 3                                              */
+} sms_mesh_data;
+#define SMS_OBJ_FLIP_TEXCOORDS 0x1
+#define SMS_OBJ_FACE_NORMALS   0x2
+sms_status sms_obj_load(const char *path, const float *to_world, int32_t flags, sms_mesh_data **out, char *err,
+                        int32_t err_len);
+void sms_mesh_data_free(sms_mesh_data *m);
+
+typedef struct sms_image {
+    int32_t width, height, channels;   /* channels = 3 (RGB)                               */
+    float *pixels;                     /* row-major, top row first                          */
+} sms_image;
+sms_status sms_image_load(const char *path, sms_image **out, char *err, int32_t err_len);
+void sms_image_free(sms_image *img);
+
+typedef struct sms_scene_file sms_scene_file;
+#define SMS_LOAD_SUBSTITUTE 0x1
+#define SMS_PROP_BOOL   0
+#define SMS_PROP_INT    1
+#define SMS_PROP_FLOAT  2
+#define SMS_PROP_STRING 3
+typedef struct sms_property {
+    char name[64];
+    int32_t type;              /* SMS_PROP_*                                                */
+    double value;              /* BOOL (0 / 1), INT, FLOAT                                  */
+    char text[128];            /* STRING                                                    */
+} sms_property;
+typedef struct sms_scene_info {
+    char integrator[32];       /* the integrator plugin's type ("path" when none is given)  */
+    int32_t n_properties;      /* its properties (sms_scene_file_property)                  */
+    int32_t spp;               /* the sampler's sample_count (independent.cpp default 4)    */
+    int32_t filter_gaussian;   /* 1: gaussian rfilter of filter_stddev, 0: box              */
+    float filter_stddev;
+    int32_t n_notes;           /* substitutions and ignored entries (sms_scene_file_note)   */
+} sms_scene_info;
+sms_status sms_scene_load(const char *path, const char *const *defines, int32_t n_defines, int32_t flags,
+                          sms_scene_file **out, char *err, int32_t err_len);
+/* The scene as a description for sms_ctx_create, valid until sms_scene_file_free. */
+const sms_scene_desc *sms_scene_file_desc(const sms_scene_file *f);
+sms_status sms_scene_file_info(const sms_scene_file *f, sms_scene_info *info);
+sms_status sms_scene_file_property(const sms_scene_file *f, int32_t i, sms_property *p);
+const char *sms_scene_file_note(const sms_scene_file *f, int32_t i);
+void sms_scene_file_free(sms_scene_file *f);
 
 /* ---------------------------------------------------------------- emitter interaction */
 /* EmitterInteraction (include/mitsuba/render/manifold.h:145-190), flattened. */

@@ -284,8 +284,10 @@ static Scene *scene_create(const sms_scene_desc *desc, const sms_config *cfg) {
         nm.slope.resize(2 * N);
         /* normalmap.cpp:46-66: rgb = byte/255, n = normalize(2 rgb - 1), slope = -n.xy/n.z */
         for (size_t s = 0; s < N; ++s) {
-            float r = (float) nd.rgb[3 * s + 0] / 255.f, g = (float) nd.rgb[3 * s + 1] / 255.f,
-                  b = (float) nd.rgb[3 * s + 2] / 255.f;
+            /* an OpenEXR map (rgb32f) is used as stored (normalmap.cpp:20-22) */
+            float r = nd.rgb ? (float) nd.rgb[3 * s + 0] / 255.f : (float) nd.rgb32f[3 * s + 0];
+            float g = nd.rgb ? (float) nd.rgb[3 * s + 1] / 255.f : (float) nd.rgb32f[3 * s + 1];
+            float b = nd.rgb ? (float) nd.rgb[3 * s + 2] / 255.f : (float) nd.rgb32f[3 * s + 2];
             V3 n = normalize(v3(2.f * r - 1.f, 2.f * g - 1.f, 2.f * b - 1.f));
             nm.slope[2 * s + 0] = -n.x / n.z;
             nm.slope[2 * s + 1] = -n.y / n.z;
@@ -2970,7 +2972,21 @@ static DirSample scene_sample_emitter_direction(const Scene &S, V3 ref, V2 sampl
     }
     const Emitter &em = S.emitters[(size_t) index];
     if (em.type == SMS_EMITTER_AREA) {
-        ds = sphere_sample_direction(S.shapes[(size_t) em.shape], ref, sample);
+        const Shape &es = S.shapes[(size_t) em.shape];
+        if (es.type == SMS_SHAPE_SPHERE) {
+            ds = sphere_sample_direction(es, ref, sample);
+        } else {
+            /* Shape::sample_direction (shape.cpp:259-276): area sample, solid-angle conversion */
+            PS ps = sample_position(S, em.shape, sample);
+            ds.p = ps.p; ds.n = ps.n; ds.pdf = ps.pdf;
+            ds.d = ds.p - ref;
+            float dist2 = dot(ds.d, ds.d);
+            ds.dist = sqrtf(dist2);
+            ds.d = ds.d / ds.dist;
+            float dp = fabsf(dot(ds.d, ds.n));
+            if (dp != 0.f) ds.pdf = ds.pdf * (dist2 / dp);
+            else ds.pdf = 0.f;
+        }
         bool ok = dot(ds.d, ds.n) < 0.f && ds.pdf != 0.f;
         if (ok) spec = v3(em.radiance.x / ds.pdf, em.radiance.y / ds.pdf, em.radiance.z / ds.pdf);
     } else if (em.type == SMS_EMITTER_POINT) {
@@ -3010,6 +3026,17 @@ static float pdf_emitter_hit(const Scene &S, int emitter, const SI &hit, V3 ref)
     float dist = norm(d);
     d = d / dist;
     if (!(dot(d, hit.sh_n) < 0.f)) return 0.f;
+    if (sh.type != SMS_SHAPE_SPHERE) {
+        /* Shape::pdf_direction (shape.cpp:278-289): pdf_position (rectangle: 1 / area,
+           rectangle.cpp:130-133; mesh: the area distribution's normalisation, mesh.cpp:588-591)
+           times dist^2 / |cos| */
+        float dp = fabsf(dot(d, hit.sh_n));
+        float pdf_pos = sh.type == SMS_SHAPE_RECTANGLE ? sh.inv_surface_area : sh.area_norm;
+        float pdf = 0.f;
+        if (dp != 0.f) pdf = pdf_pos * ((dist * dist) / dp);
+        if (S.emitters.size() > 1) pdf = pdf * (1.f / (float) S.emitters.size());
+        return pdf;
+    }
     float sin_alpha = sh.radius * (1.f / norm(sh.center - ref));
     float cos_alpha = safe_sqrt(1.f - sin_alpha * sin_alpha);
     float pdf = sin_alpha < 0.99999994039535522461f ? INV_TWOPI_F / (1.f - cos_alpha)

@@ -91,6 +91,21 @@ SMS_DEV DirSample sphere_sample_direction(const DShape &sh, V3 ref, V2 sample) {
     return ds;
 }
 
+// Shape::sample_direction (src/librender/shape.cpp:259-276) of shapes without their own
+// (rectangles, meshes): a point of the shape's area sampling and the solid-angle conversion
+SMS_DEV DirSample shape_sample_direction(const DScene &S, int shape, V3 ref, V2 sample) {
+    DirSample ds = dir_sample_zero();
+    const PS ps = sample_position(S, shape, sample);
+    ds.p = ps.p; ds.n = ps.n; ds.pdf = ps.pdf;
+    ds.d = ds.p - ref;
+    const float dist2 = dot(ds.d, ds.d);
+    ds.dist = sqrtf(dist2);
+    ds.d = ds.d / ds.dist;
+    const float dp = fabsf(dot(ds.d, ds.n));
+    ds.pdf = dp != 0.f ? ds.pdf * (dist2 / dp) : 0.f;
+    return ds;
+}
+
 // Scene::sample_emitter_direction (src/librender/scene.cpp:156-203) over AreaLight
 // (area.cpp:104-117), PointLight (point.cpp:74-99) and DirectionalEmitter (directional.cpp:
 // 105-128); the visibility ray only when asked (a delta BSDF evaluates to zero whatever it sees)
@@ -109,7 +124,9 @@ SMS_DEV DirSample scene_sample_emitter_direction(const DScene &S, V3 ref, V2 sam
     }
     const DEmitter &em = S.emitters[index];
     if (em.type == SMS_EMITTER_AREA) {
-        ds = sphere_sample_direction(S.shapes[em.shape], ref, sample);
+        const DShape &es = S.shapes[em.shape];
+        ds = es.type == SMS_SHAPE_SPHERE ? sphere_sample_direction(es, ref, sample)
+                                         : shape_sample_direction(S, em.shape, ref, sample);
         if (dot(ds.d, ds.n) < 0.f && ds.pdf != 0.f)
             spec = v3(em.radiance[0] / ds.pdf, em.radiance[1] / ds.pdf, em.radiance[2] / ds.pdf);
     } else if (em.type == SMS_EMITTER_POINT) {
@@ -138,8 +155,9 @@ SMS_DEV DirSample scene_sample_emitter_direction(const DScene &S, V3 ref, V2 sam
 }
 
 // Scene::pdf_emitter_direction (scene.cpp:205-219) of a BSDF-sampled hit on an area emitter:
-// AreaLight::pdf_direction (area.cpp:119-125), Sphere::pdf_direction (sphere.cpp:254-267), with
-// DirectionSample3f(hit, ref) (records.h:168-174)
+// AreaLight::pdf_direction (area.cpp:119-125), Sphere::pdf_direction (sphere.cpp:254-267) or
+// Shape::pdf_direction (shape.cpp:278-289: pdf_position, rectangle.cpp:130-133 and
+// mesh.cpp:588-591, times dist^2 / |cos|), with DirectionSample3f(hit, ref) (records.h:168-174)
 SMS_DEV float pdf_emitter_hit(const DScene &S, int emitter, const SI &hit, V3 ref) {
     const DEmitter &em = S.emitters[emitter];
     if (em.type != SMS_EMITTER_AREA) return 0.f;
@@ -148,6 +166,13 @@ SMS_DEV float pdf_emitter_hit(const DScene &S, int emitter, const SI &hit, V3 re
     const float dist = norm(d);
     d = d / dist;
     if (!(dot(d, hit.sh_n) < 0.f)) return 0.f;
+    if (sh.type != SMS_SHAPE_SPHERE) {
+        const float dp = fabsf(dot(d, hit.sh_n));
+        const float pdf_pos = sh.type == SMS_SHAPE_RECTANGLE ? sh.inv_surface_area : sh.area_norm;
+        float pdf = dp != 0.f ? pdf_pos * ((dist * dist) / dp) : 0.f;
+        if (S.n_emitters > 1) pdf = pdf * (1.f / (float) S.n_emitters);
+        return pdf;
+    }
     const float sin_alpha = sh.radius * (1.f / norm(v3(sh.cx, sh.cy, sh.cz) - ref));
     const float cos_alpha = safe_sqrt(1.f - sin_alpha * sin_alpha);
     float pdf = sin_alpha < 0.99999994039535522461f ? INV_TWOPI_F / (1.f - cos_alpha)

@@ -243,7 +243,7 @@ sms_status prepare_scene(const sms_scene_desc *desc, HostScene &H, std::string &
     // normal maps: level-0 slopes (src/librender/normalmap.cpp:46-66)
     for (int i = 0; i < desc->n_normalmaps; ++i) {
         const sms_normalmap_desc &nd = desc->normalmaps[i];
-        if (nd.res <= 0 || (nd.res & (nd.res - 1)) != 0 || !nd.rgb)
+        if (nd.res <= 0 || (nd.res & (nd.res - 1)) != 0 || !(nd.rgb || nd.rgb32f))
             return bad(SMS_E_INVALID, "normal map must be square power-of-two");
         const size_t res = (size_t) nd.res, stride = res + 1;
         // padded layout: column res and row res replicate column/row res-1 (edge clamp)
@@ -251,8 +251,12 @@ sms_status prepare_scene(const sms_scene_desc *desc, HostScene &H, std::string &
         for (size_t y = 0; y < stride; ++y) {
             for (size_t x = 0; x < stride; ++x) {
                 size_t s = std::min(y, res - 1) * res + std::min(x, res - 1);
-                float r = (float) nd.rgb[3 * s] / 255.f, g = (float) nd.rgb[3 * s + 1] / 255.f,
-                      b = (float) nd.rgb[3 * s + 2] / 255.f;
+                float r, g, b;
+                if (nd.rgb) {
+                    r = (float) nd.rgb[3 * s] / 255.f; g = (float) nd.rgb[3 * s + 1] / 255.f; b = (float) nd.rgb[3 * s + 2] / 255.f;
+                } else {
+                    r = nd.rgb32f[3 * s]; g = nd.rgb32f[3 * s + 1]; b = nd.rgb32f[3 * s + 2];
+                }
                 float vx = 2.f * r - 1.f, vy = 2.f * g - 1.f, vz = 2.f * b - 1.f;
                 float inv = 1.f / std::sqrt((vx * vx + vy * vy) + vz * vz);
                 float nx = vx * inv, ny = vy * inv, nz = vz * inv;
@@ -315,9 +319,8 @@ sms_status prepare_scene(const sms_scene_desc *desc, HostScene &H, std::string &
             !(e.direction[0] * e.direction[0] + e.direction[1] * e.direction[1] + e.direction[2] * e.direction[2] > 0.f))
             return bad(SMS_E_INVALID, "directional emitter without a direction");
         d.ss = e.caustic_ss; d.ms = e.caustic_ms;
-        if (e.type == SMS_EMITTER_AREA && (e.shape < 0 || e.shape >= desc->n_shapes ||
-                                           desc->shapes[e.shape].type != SMS_SHAPE_SPHERE))
-            return bad(SMS_E_INVALID, "area emitters must sit on a sphere shape");
+        if (e.type == SMS_EMITTER_AREA && (e.shape < 0 || e.shape >= desc->n_shapes))
+            return bad(SMS_E_INVALID, "area emitter without a shape");
         emitters.push_back(d);
         if (e.type == SMS_EMITTER_AREA) H.area_emitters = true;
         em_all.push_back(i);

@@ -58,7 +58,7 @@ class SmsEmitterDesc(C.Structure):
 
 
 class SmsNormalmapDesc(C.Structure):
-    _fields_ = [("res", C.c_int32), ("rgb", C.POINTER(C.c_uint8))]
+    _fields_ = [("res", C.c_int32), ("rgb", C.POINTER(C.c_uint8)), ("rgb32f", C.POINTER(C.c_float))]
 
 
 class SmsCameraDesc(C.Structure):
@@ -182,11 +182,40 @@ EXPORTS = ["sms_config_default", "sms_ctx_create", "sms_ctx_destroy", "sms_ctx_s
            "sms_film_accumulate", "sms_ss_autotune", "sms_ms_autotune", "sms_scene_footprint",
            "sms_ss_estimate_batch_noalloc", "sms_glint_proposal_inputs", "sms_glint_proposal_outputs",
            "sms_path_config_default", "sms_path_samples", "sms_film_splat", "sms_glint_estimate_frame",
-           "sms_glint_frame_graph_create", "sms_ctx_set_proposal_anchor", "sms_ss_proposal_anchors"]
+           "sms_glint_frame_graph_create", "sms_ctx_set_proposal_anchor", "sms_ss_proposal_anchors",
+           "sms_obj_load", "sms_mesh_data_free", "sms_image_load", "sms_image_free", "sms_scene_load",
+           "sms_scene_file_desc", "sms_scene_file_info", "sms_scene_file_property", "sms_scene_file_note",
+           "sms_scene_file_free"]
 
 # learned seed proposal (include/sms.h)
 MLP_INPUTS = 9
 MLP_OUTPUTS = 48
+# scene and asset files
+OBJ_FLIP_TEXCOORDS, OBJ_FACE_NORMALS = 0x1, 0x2
+LOAD_SUBSTITUTE = 0x1
+PROP_BOOL, PROP_INT, PROP_FLOAT, PROP_STRING = 0, 1, 2, 3
+
+
+class SmsMeshData(C.Structure):
+    _fields_ = [("n_vertices", C.c_int64), ("n_faces", C.c_int64), ("positions", C.POINTER(C.c_float)),
+                ("normals", C.POINTER(C.c_float)), ("texcoords", C.POINTER(C.c_float)),
+                ("faces", C.POINTER(C.c_uint32))]
+
+
+class SmsImage(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("channels", C.c_int32),
+                ("pixels", C.POINTER(C.c_float))]
+
+
+class SmsProperty(C.Structure):
+    _fields_ = [("name", C.c_char * 64), ("type", C.c_int32), ("value", C.c_double), ("text", C.c_char * 128)]
+
+
+class SmsSceneInfo(C.Structure):
+    _fields_ = [("integrator", C.c_char * 32), ("n_properties", C.c_int32), ("spp", C.c_int32),
+                ("filter_gaussian", C.c_int32), ("filter_stddev", C.c_float), ("n_notes", C.c_int32)]
+
+
 MLP_PARAMS = 25904
 CAPTURE_SOLUTIONS, CAPTURE_ESTIMATES = 0, 1
 
@@ -259,6 +288,22 @@ def load_library(path=LIB_PATH):
     lib.sms_glint_frame_graph_create.argtypes = [vp, C.c_int32, vp, C.POINTER(vp)]
     lib.sms_path_samples.argtypes = [vp, C.POINTER(SmsPathConfig), C.c_int32, C.c_uint64, vp, vp, C.POINTER(SmsStats)]
     lib.sms_film_splat.argtypes = [vp, vp, vp, C.c_int32, C.c_float, vp, vp]
+    lib.sms_obj_load.argtypes = [C.c_char_p, vp, C.c_int32, C.POINTER(C.POINTER(SmsMeshData)), C.c_char_p, C.c_int32]
+    lib.sms_mesh_data_free.argtypes = [C.POINTER(SmsMeshData)]
+    lib.sms_mesh_data_free.restype = None
+    lib.sms_image_load.argtypes = [C.c_char_p, C.POINTER(C.POINTER(SmsImage)), C.c_char_p, C.c_int32]
+    lib.sms_image_free.argtypes = [C.POINTER(SmsImage)]
+    lib.sms_image_free.restype = None
+    lib.sms_scene_load.argtypes = [C.c_char_p, C.POINTER(C.c_char_p), C.c_int32, C.c_int32, C.POINTER(vp),
+                                   C.c_char_p, C.c_int32]
+    lib.sms_scene_file_desc.argtypes = [vp]
+    lib.sms_scene_file_desc.restype = C.POINTER(SmsSceneDesc)
+    lib.sms_scene_file_info.argtypes = [vp, C.POINTER(SmsSceneInfo)]
+    lib.sms_scene_file_property.argtypes = [vp, C.c_int32, C.POINTER(SmsProperty)]
+    lib.sms_scene_file_note.argtypes = [vp, C.c_int32]
+    lib.sms_scene_file_note.restype = C.c_char_p
+    lib.sms_scene_file_free.argtypes = [vp]
+    lib.sms_scene_file_free.restype = None
     for name in EXPORTS:
         fn = getattr(lib, name)
         if fn.restype is C.c_int and name not in ("sms_ctx_last_error", "sms_config_default"):

@@ -78,7 +78,9 @@ class Scene:
         return len(self.emitters) - 1
 
     def add_normalmap(self, rgb):
-        rgb = np.ascontiguousarray(rgb, np.uint8)
+        """RGB normal map: uint8 (code values / 255) or float32 (values as an OpenEXR file
+        stores them), square power-of-two."""
+        rgb = np.ascontiguousarray(rgb, np.float32 if np.asarray(rgb).dtype.kind == "f" else np.uint8)
         assert rgb.ndim == 3 and rgb.shape[0] == rgb.shape[1] and rgb.shape[2] == 3
         self.normalmaps.append((rgb.shape[0], rgb))
         return len(self.normalmaps) - 1
@@ -122,7 +124,10 @@ class Scene:
         nms = (abi.SmsNormalmapDesc * max(1, len(self.normalmaps)))()
         for i, (res, rgb) in enumerate(self.normalmaps):
             nms[i].res = res
-            nms[i].rgb = rgb.ctypes.data_as(C.POINTER(C.c_uint8))
+            if rgb.dtype == np.float32:
+                nms[i].rgb32f = rgb.ctypes.data_as(C.POINTER(C.c_float))
+            else:
+                nms[i].rgb = rgb.ctypes.data_as(C.POINTER(C.c_uint8))
         desc = abi.SmsSceneDesc()
         desc.n_shapes, desc.shapes = len(self.shapes), shapes
         desc.n_emitters, desc.emitters = len(self.emitters), emitters

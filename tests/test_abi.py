@@ -17,7 +17,7 @@ HEADER = os.path.join(ROOT, "include", "sms.h")
 
 def declared_functions():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"^(?:sms_status|void|const char \*)\s*(sms_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^(?:sms_status|void|const char \*|const sms_scene_desc \*)\s*(sms_\w+)\s*\(", text, re.M)))
 
 
 def test_header_and_python_export_lists_agree():

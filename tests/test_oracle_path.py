@@ -211,3 +211,57 @@ def test_film_filter_shape(film_scene):
     img2, wt2 = orc.film_splat(rgb2, pos, spp, 0.5)
     img1, wt1 = orc.film_splat(np.ones((w * h, 3), np.float32), pos, spp, 0.5)
     assert wt2[c] < wt1[c] and np.isfinite(img2).all()
+
+
+# ------------------------------------------------------------------ area lights on rectangles and meshes
+def _polygon_irradiance(x, n, verts, L):
+    """Lambert's formula: irradiance at x (normal n) from a uniform polygon of radiance L,
+    E = L / 2 * |sum_i theta_i (n . u_i)|, theta_i the angle subtended by edge i and u_i the
+    unit normal of the plane through x and the edge."""
+    v = [np.asarray(p, np.float64) - x for p in verts]
+    s = 0.0
+    for i in range(len(v)):
+        a, b = v[i], v[(i + 1) % len(v)]
+        c = np.cross(a, b)
+        th = math.atan2(np.linalg.norm(c), np.dot(a, b))
+        s += th * np.dot(n, c / np.linalg.norm(c))
+    return L * 0.5 * abs(s)
+
+
+def _area_light_scene(light, x0, z0):
+    """Diffuse floor (y = 0, albedo 0.5) under a light facing down 2 above it: a rectangle of
+    half side 0.5 centred at (0.3, 2, -0.2), or the same square as a two-triangle mesh; a
+    narrow camera looking straight down at (x0, 0, z0)."""
+    sc = scenes.Scene()
+    sc.add_rectangle(scenes.plane_to_world((0, 0, 0), 4.0), abi.BSDF_DIFFUSE, reflectance=(0.5, 0.5, 0.5))
+    c, a = np.array([0.3, 2.0, -0.2]), 0.5
+    corners = [c + [-a, 0, -a], c + [a, 0, -a], c + [a, 0, a], c + [-a, 0, a]]
+    if light == "rectangle":
+        to_world = np.array([[a, 0, 0, c[0]], [0, 0, -1, c[1]], [0, a, 0, c[2]]], np.float32)
+        s = sc.add_rectangle(to_world, abi.BSDF_DIFFUSE, reflectance=(0, 0, 0))
+    else:
+        P = np.array(corners, np.float32)
+        F = np.array([[0, 1, 2], [0, 2, 3]], np.uint32)          # wound so the face normal points down
+        s = sc.add_mesh(P, F, bsdf=abi.BSDF_DIFFUSE, reflectance=(0, 0, 0))
+    sc.add_area_light(s, (10.0, 10.0, 10.0))
+    sc.set_camera((x0, 1.0, z0 + 1e-4), (x0, 0.0, z0), (0, 0, -1), 0.05, 4, 4)
+    return sc, corners
+
+
+@pytest.mark.parametrize("light", ["rectangle", "mesh"])
+@pytest.mark.parametrize("x0,z0", [(0.3, -0.2), (-0.9, 0.6)])
+def test_area_light_direct_lighting_matches_polygon_formula(light, x0, z0):
+    """Direct lighting of a diffuse floor by a rectangle or mesh area light (emitter sampling
+    and BSDF sampling combined by MIS) equals albedo / pi x Lambert's polygon irradiance."""
+    sc, corners = _area_light_scene(light, x0, z0)
+    orc = oracle_lib.Oracle(sc, abi.SmsConfig.default())
+    pc = abi.SmsPathConfig.default(integrator=abi.PATH_FILTERED_SS, max_depth=2)
+    vals = []
+    for s in range(6):
+        rgb, _, _ = orc.path_samples(pc, 4096, 1 + s)
+        vals.append(rgb[:, 0])
+    m, se = _mean_se(np.concatenate(vals))
+    ref = 0.5 / math.pi * _polygon_irradiance(np.array([x0, 0.0, z0]), np.array([0, 1.0, 0]), corners, 10.0)
+    print(f"{light} at ({x0}, {z0}): {m:.5f} +- {se:.5f}, closed form {ref:.5f}")
+    assert abs(m - ref) < 4 * se + 1e-3 * ref
+    orc.close()
