@@ -662,6 +662,15 @@ sms_status sms_graph_destroy(sms_graph *g);
                            (:196-241)
      SMS_PATH_FILTERED_SS  path_filtered_ss (src/integrators/path_filtered_ss.cpp:37-194): the
                            same light paths by BSDF / emitter sampling only
+     SMS_PATH_SMS_MS       path_sms_ms (src/integrators/path_sms_ms.cpp:175-380): chains of
+                           config.bounces vertices on multi-scatter casters and bouncers, the SMS
+                           estimate of sms_ms_estimate_batch at every receiver; no emitter sampling
+                           on casters; emitter hits through them only on purely specular camera
+                           paths -- and, with config.mnee_init and not biased_mnee, those an MNEE
+                           walk from the chain's receiver does not re-find vertex for vertex
+                           (MNEEHelper, :54-143; :290-329)
+     SMS_PATH_FILTERED_MS  path_filtered_ms (src/integrators/path_filtered_ms.cpp:57-186): the
+                           same chains by BSDF / emitter sampling, tracked by PathRecord (:237-300)
    Samples are PCG32 streams of (seed, sample index) (stream 0xFFFFFFFD: film jitter, then per
    bounce the Russian-roulette, emitter and BSDF samples in the reference's order); the SMS
    estimates of a pass run in batches of SMS_PATH_CHUNK samples, each batch's queries in
@@ -669,6 +678,8 @@ sms_status sms_graph_destroy(sms_graph *g);
    default) ends paths at SMS_PATH_MAX_DEPTH. */
 #define SMS_PATH_SMS_SS       0
 #define SMS_PATH_FILTERED_SS  1
+#define SMS_PATH_SMS_MS       2
+#define SMS_PATH_FILTERED_MS  3
 #define SMS_PATH_MAX_DEPTH    16
 #define SMS_PATH_CHUNK        (1 << 20)
 typedef struct sms_path_config {

@@ -3119,7 +3119,76 @@ static Ray path_camera_ray(const Scene &S, int64_t idx, int spp, Pcg32 *r, float
 
 struct PathSms { sms_ss_query q; V3 thr; };
 struct PathMnee { sms_ss_walk_in w; V3 value, dir_pt, ref_p; };
-struct PathSample { V3 L; float pos[2]; std::vector<PathSms> sms; std::vector<PathMnee> mnee; };
+/* a multi-scatter MNEE check: the walk from the receiver and the chain the path tracer found */
+struct PathMneeMs { sms_ms_walk_in w; V3 value; V3 pos[MS_MAXB]; int npos; };
+struct PathSample {
+    V3 L; float pos[2]; std::vector<PathSms> sms; std::vector<PathMnee> mnee; std::vector<PathMneeMs> mnee_ms;
+};
+
+/* The multi-scatter integrators' record of the specular chain the path is on: MNEEHelper
+   (path_sms_ms.cpp:54-143: receiver, first caster and the chain's positions) and PathRecord
+   (path_filtered_ms.cpp:237-300: the chain's scattering types); states Default, HitReceiver,
+   HitCaster, HitEmitter.  depth_receiver: the path depth of the chain's receiver. */
+struct ChainRecord {
+    int state = 0, bounce = -1, shape0 = -1, npos = 0, depth_receiver = -1;
+    V3 endpoint = v3(0.f, 0.f, 0.f), pos[MS_MAXB];
+    void reset() { state = 0; bounce = -1; shape0 = -1; npos = 0; }
+};
+
+/* caustic_eta of BSDF::ior (path_filtered_ss.cpp:85-87, path_filtered_ms.cpp:268-270): a
+   dielectric's (eta, 0); conductors (eta, k) and the default (0, 1) give 1 unless k = 0 */
+static float caustic_eta_of(const Shape &sh) {
+    if (sh.bsdf == SMS_BSDF_DIELECTRIC) return sh.eta_r;
+    if (sh.bsdf == SMS_BSDF_CONDUCTOR && sh.k == 0.f) return sh.eta_r;
+    return 1.f;
+}
+
+/* MNEEHelper::state_transition (path_sms_ms.cpp:85-138) to the interaction `next` at `depth` */
+static void mnee_transition(const Scene &S, ChainRecord &c, const SI &next, int depth) {
+    const int bounces = S.cfg.bounces;
+    const int flags = next.valid ? S.shapes[(size_t) next.shape].flags : 0;
+    if (next.valid && (flags & SMS_FLAG_CAUSTIC_RECEIVER)) {
+        c.reset(); c.state = 1; c.endpoint = next.p; c.depth_receiver = depth;
+        return;
+    }
+    const int e = next.valid ? S.shapes[(size_t) next.shape].emitter : -1;
+    if (c.state == 1) {
+        if (next.valid && (flags & SMS_FLAG_CAUSTIC_CASTER_MS)) {
+            c.shape0 = next.shape; c.pos[0] = next.p; c.npos = 1; c.bounce = 1; c.state = 2;
+        } else c.reset();
+    } else if (c.state == 2) {
+        if (c.bounce < bounces && next.valid && (flags & (SMS_FLAG_CAUSTIC_CASTER_MS | SMS_FLAG_CAUSTIC_BOUNCER))) {
+            c.pos[c.npos++] = next.p; c.bounce++;
+        } else if (c.bounce == bounces && e >= 0 && S.emitters[(size_t) e].ms) {
+            c.state = 3;
+        } else c.reset();
+    } else c.reset();
+}
+
+/* PathRecord::state_transition (path_filtered_ms.cpp:248-299) from `cur` (BSDF-sampled with
+   reflection / transmission flags refl / trans) to `next` at `depth` */
+static void record_transition(const Scene &S, ChainRecord &c, const SI &cur, const SI &next, bool refl, bool trans,
+                              int depth) {
+    const int bounces = S.cfg.bounces;
+    const int flags = next.valid ? S.shapes[(size_t) next.shape].flags : 0;
+    if (next.valid && (flags & SMS_FLAG_CAUSTIC_RECEIVER)) {
+        c.reset(); c.state = 1; c.depth_receiver = depth;
+        return;
+    }
+    const int e = next.valid ? S.shapes[(size_t) next.shape].emitter : -1;
+    if (c.state == 1) {
+        if (next.valid && (flags & SMS_FLAG_CAUSTIC_CASTER_MS)) { c.bounce = 1; c.state = 2; }
+        else c.reset();
+    } else if (c.state == 2) {
+        if (c.bounce < bounces && next.valid && (flags & (SMS_FLAG_CAUSTIC_CASTER_MS | SMS_FLAG_CAUSTIC_BOUNCER))) {
+            const float eta = caustic_eta_of(S.shapes[(size_t) cur.shape]);
+            if ((eta == 1.f && trans) || (eta != 1.f && refl)) c.reset();
+            else c.bounce++;
+        } else if (c.bounce == bounces && e >= 0 && S.emitters[(size_t) e].ms) {
+            c.state = 3;
+        } else c.reset();
+    } else c.reset();
+}
 
 static void sms_query_from_si(const SI &si, sms_ss_query *q) {
     memset(q, 0, sizeof(*q));
@@ -3130,17 +3199,26 @@ static void sms_query_from_si(const SI &si, sms_ss_query *q) {
     }
 }
 
-/* one sample of path_sms_ss / path_filtered_ss (SamplingIntegrator::sample) */
+/* one sample of path_sms_ss / path_filtered_ss / path_sms_ms / path_filtered_ms
+   (SamplingIntegrator::sample) */
 static void path_sample(const Scene &S, const sms_path_config &pc, uint64_t seed, int64_t idx, int spp, PathSample &o) {
-    const bool sms = pc.integrator == SMS_PATH_SMS_SS, only = pc.caustics_only != 0;
+    const bool ms = pc.integrator == SMS_PATH_SMS_MS || pc.integrator == SMS_PATH_FILTERED_MS;
+    const bool sms = pc.integrator == SMS_PATH_SMS_SS || pc.integrator == SMS_PATH_SMS_MS, only = pc.caustics_only != 0;
     const int depth_cap = pc.max_depth < 0 ? SMS_PATH_MAX_DEPTH : pc.max_depth;
+    const int chain = ms ? S.cfg.bounces : 1;           /* specular vertices between receiver and light */
     Pcg32 r; stream_seed(&r, seed, (uint64_t) idx, PATH_STREAM);
-    o.sms.clear(); o.mnee.clear();
+    o.sms.clear(); o.mnee.clear(); o.mnee_ms.clear();
     Ray ray = path_camera_ray(S, idx, spp, &r, o.pos);
     V3 L = v3(0.f, 0.f, 0.f), thr = v3(1.f, 1.f, 1.f);
     float eta = 1.f;
     SI si = ray_intersect(S, ray), si_prev = si_invalid();
     int first_diffuse = -1;
+    bool specular_camera_path = true;
+    ChainRecord rec;
+    if (ms) {
+        if (sms) mnee_transition(S, rec, si, 1);
+        else record_transition(S, rec, si, si, false, false, 1);
+    }
     if (si.valid && !only) {
         int e = S.shapes[(size_t) si.shape].emitter;
         if (e >= 0) L = L + emitter_eval_hit(S, e, si);
@@ -3156,38 +3234,40 @@ static void path_sample(const Scene &S, const sms_path_config &pc, uint64_t seed
             thr = thr * (1.f / q);
         }
         if (depth >= depth_cap) break;
-        if (only && first_diffuse >= 0 && depth > first_diffuse + 1) break;     /* nothing left to count */
+        if (only && first_diffuse >= 0 && depth > first_diffuse + chain) break;     /* nothing left to count */
         const Shape &sh = S.shapes[(size_t) si.shape];
         const bool smooth = sh.bsdf == SMS_BSDF_DIFFUSE;
         if (first_diffuse < 0 && smooth) first_diffuse = depth;
         const bool receiver = (sh.flags & SMS_FLAG_CAUSTIC_RECEIVER) != 0;
         const bool from_receiver = si_prev.valid && (S.shapes[(size_t) si_prev.shape].flags & SMS_FLAG_CAUSTIC_RECEIVER);
-        const bool glint_case = depth == 1 && !pc.disable_reflected_emitters;
+        const bool glint_case = !ms && depth == 1 && !pc.disable_reflected_emitters;
         bool on_caster;
         float caustic_eta = 1.f;
-        if (sms) {
+        if (ms) {
+            on_caster = (sh.flags & (SMS_FLAG_CAUSTIC_CASTER_MS | SMS_FLAG_CAUSTIC_BOUNCER)) != 0;
+            caustic_eta = caustic_eta_of(sh);
+        } else if (sms) {
             on_caster = (sh.flags & SMS_FLAG_CAUSTIC_CASTER_SS) != 0;
-            /* SMS at a caustic receiver (path_sms_ss.cpp:118-124) */
-            if (receiver && !on_caster && (pc.max_depth < 0 || depth + 1 < pc.max_depth) &&
-                (!only || depth == first_diffuse)) {
-                PathSms ps;
-                sms_query_from_si(si, &ps.q);
-                ps.thr = thr;
-                o.sms.push_back(ps);
-            }
         } else {
             on_caster = (sh.flags & (SMS_FLAG_CAUSTIC_CASTER_SS | SMS_FLAG_CAUSTIC_CASTER_MS)) != 0;
-            /* caustic_eta from BSDF::ior (path_filtered_ss.cpp:85-87): a dielectric's (eta, 0);
-               conductors (eta, k) and the default (0, 1) give 1 unless k = 0 */
-            if (sh.bsdf == SMS_BSDF_DIELECTRIC) caustic_eta = sh.eta_r;
-            else if (sh.bsdf == SMS_BSDF_CONDUCTOR && sh.k == 0.f) caustic_eta = sh.eta_r;
+            caustic_eta = caustic_eta_of(sh);
         }
-        /* emitter sampling (path_sms_ss.cpp:142-161, path_filtered_ss.cpp:107-142) */
+        /* SMS at a caustic receiver (path_sms_ss.cpp:118-124, path_sms_ms.cpp:228-234) */
+        if (sms && receiver && !on_caster && (pc.max_depth < 0 || depth + chain < pc.max_depth) &&
+            (!only || depth == first_diffuse)) {
+            PathSms ps;
+            sms_query_from_si(si, &ps.q);
+            ps.thr = thr;
+            o.sms.push_back(ps);
+        }
+        /* emitter sampling (path_sms_ss.cpp:142-161, path_filtered_ss.cpp:107-142,
+           path_sms_ms.cpp:240-270, path_filtered_ms.cpp:97-140) */
         bool nee;
         bool valid_connection = false;
         if (sms) nee = (smooth && !on_caster) || glint_case;
         else {
-            valid_connection = on_caster && (from_receiver || glint_case);
+            valid_connection = ms ? (rec.state == 2 && rec.bounce == S.cfg.bounces)
+                                  : on_caster && (from_receiver || glint_case);
             nee = smooth && (!on_caster || valid_connection);
         }
         if (nee) {
@@ -3202,7 +3282,7 @@ static void path_sample(const Scene &S, const sms_path_config &pc, uint64_t seed
                 val = bsdf_eval(sh, si.wi, wo);
                 float bpdf = bsdf_pdf(sh, si.wi, wo);
                 float mis = ds.delta ? (float) 1.f : mis_power(ds.pdf, bpdf);
-                if (!sms && valid_connection && depth > 1) {
+                if (!sms && valid_connection && (ms || depth > 1)) {
                     bool refraction = wo.z * si.wi.z < 0.f, reflection = !refraction;
                     if ((caustic_eta == 1.f && !reflection) || (caustic_eta != 1.f && !refraction)) mis = 0.f;
                 }
@@ -3215,12 +3295,57 @@ static void path_sample(const Scene &S, const sms_path_config &pc, uint64_t seed
         BsdfSample bs = bsdf_sample(sh, si.wi, s1, s2);
         thr = mul3(thr, bs.weight);
         eta = eta * bs.eta;
+        if (!bs.delta) specular_camera_path = false;
         if (thr.x == 0.f && thr.y == 0.f && thr.z == 0.f) break;
         ray = spawn_ray(si.p, to_world(si.sh_s, si.sh_t, si.sh_n, bs.wo));
         SI hit = ray_intersect(S, ray);
+        if (ms) {
+            if (sms) mnee_transition(S, rec, hit, depth + 1);
+            else record_transition(S, rec, si, hit, bs.reflection, bs.transmission, depth + 1);
+        }
         int e = hit.valid ? S.shapes[(size_t) hit.shape].emitter : -1;
         if (e >= 0) {
-            if (sms) {
+            if (ms && sms) {
+                if (!on_caster || specular_camera_path) {
+                    if (!only) {
+                        V3 ev = emitter_eval_hit(S, e, hit);
+                        float epdf = bs.delta ? (float) 0.f : pdf_emitter_hit(S, e, hit, si.p);
+                        float mis = mis_power(bs.pdf, epdf);
+                        L = L + mul3(thr * mis, ev);
+                    }
+                } else if (S.cfg.mnee_init && !pc.biased_mnee && rec.state == 3 && rec.bounce == S.cfg.bounces) {
+                    /* MNEE check (path_sms_ms.cpp:290-329): the walk from the chain's receiver through
+                       its first caster, compared vertex by vertex with the chain found */
+                    PathMneeMs m;
+                    memset(&m.w, 0, sizeof(m.w));
+                    m.w.sample[0] = pcg32_float(&r); m.w.sample[1] = pcg32_float(&r);
+                    if (!only || rec.depth_receiver == first_diffuse) {
+                        for (int a = 0; a < 3; ++a) m.w.si_p[a] = (&rec.endpoint.x)[a];
+                        m.w.shape = rec.shape0;
+                        V3 ed = normalize(hit.p - rec.endpoint);
+                        for (int a = 0; a < 3; ++a) {
+                            m.w.ei.p[a] = (&hit.p.x)[a]; m.w.ei.n[a] = (&hit.n.x)[a]; m.w.ei.d[a] = (&ed.x)[a];
+                        }
+                        m.w.ei.kind = SMS_EI_AREA; m.w.ei.emitter = e;
+                        m.w.first_path = 1;
+                        m.value = mul3(thr, emitter_eval_hit(S, e, hit));
+                        m.npos = rec.npos;
+                        for (int k = 0; k < rec.npos; ++k) m.pos[k] = rec.pos[k];
+                        o.mnee_ms.push_back(m);
+                    }
+                }
+            } else if (ms) {
+                const bool light_hit_valid = rec.state == 3 && rec.bounce == S.cfg.bounces;
+                if (!on_caster || light_hit_valid || specular_camera_path) {
+                    bool take = !only || (light_hit_valid && rec.depth_receiver == first_diffuse);
+                    if (take) {
+                        V3 ev = emitter_eval_hit(S, e, hit);
+                        float epdf = bs.delta ? (float) 0.f : pdf_emitter_hit(S, e, hit, si.p);
+                        float mis = mis_power(bs.pdf, epdf);
+                        L = L + mul3(thr * mis, ev);
+                    }
+                }
+            } else if (sms) {
                 if (!on_caster || glint_case) {
                     if (!only) {
                         V3 ev = emitter_eval_hit(S, e, hit);
@@ -3400,16 +3525,20 @@ int oracle_path_samples(void *h, const sms_path_config *pc, int spp, uint64_t se
         parallel_for(c1 - c0, threads, [&](int64_t i) { path_sample(S, *pc, seed, c0 + i, spp, ps[(size_t) i]); });
         std::vector<sms_ss_query> Q;
         std::vector<const PathMnee *> M;
+        std::vector<const PathMneeMs *> MM;
         for (size_t i = 0; i < ps.size(); ++i) {
             for (const PathSms &x : ps[i].sms) Q.push_back(x.q);
             for (const PathMnee &m : ps[i].mnee) M.push_back(&m);
+            for (const PathMneeMs &m : ps[i].mnee_ms) MM.push_back(&m);
         }
         const uint64_t cs = path_chunk_seed(seed, (uint64_t) (c0 / SMS_PATH_CHUNK));
+        const bool ms = pc->integrator == SMS_PATH_SMS_MS;
         std::vector<V3> est(Q.size());
         std::vector<Stats> per(Q.size());
         parallel_for((int64_t) Q.size(), threads, [&](int64_t k) {
             memset(&per[(size_t) k], 0, sizeof(Stats));
-            est[(size_t) k] = estimate_ss(S, Q[(size_t) k], cs, (uint64_t) k, per[(size_t) k]);
+            est[(size_t) k] = ms ? estimate_ms(S, Q[(size_t) k], cs, (uint64_t) k, per[(size_t) k])
+                                 : estimate_ss(S, Q[(size_t) k], cs, (uint64_t) k, per[(size_t) k]);
         });
         for (const Stats &x : per) {
             tot.walks += x.walks; tot.conv += x.conv; tot.iters += x.iters; tot.bcalls += x.bcalls;
@@ -3430,11 +3559,23 @@ int oracle_path_samples(void *h, const sms_path_config *pc, int spp, uint64_t se
             }
             add[(size_t) k] = !ok;
         });
-        size_t kq = 0, km = 0;
+        std::vector<char> add_ms(MM.size(), 0);
+        parallel_for((int64_t) MM.size(), threads, [&](int64_t k) {
+            const PathMneeMs &m = *MM[(size_t) k];
+            const sms_ms_walk_in &w = m.w;
+            MSWalk wk = ms_sample_path(S, w.shape, v3(w.si_p[0], w.si_p[1], w.si_p[2]), ei_from_c(w.ei),
+                                       v2(w.sample[0], w.sample[1]), true, nullptr, v3(0.f, 0.f, 0.f));
+            bool ok = wk.ok;
+            for (int v = 0; ok && v < wk.path.n; ++v)
+                if (!(v < m.npos) || !(norm(m.pos[v] - wk.path.v[v].v.p) < 1e-5f)) ok = false;
+            add_ms[(size_t) k] = !ok;
+        });
+        size_t kq = 0, km = 0, kmm = 0;
         for (size_t i = 0; i < ps.size(); ++i) {
             V3 L = ps[i].L;
             for (const PathSms &x : ps[i].sms) L = L + mul3(x.thr, est[kq++]);
             for (const PathMnee &m : ps[i].mnee) { if (add[km]) L = L + m.value; ++km; }
+            for (const PathMneeMs &m : ps[i].mnee_ms) { if (add_ms[kmm]) L = L + m.value; ++kmm; }
             const int64_t j = c0 + (int64_t) i;
             rgb[3 * j] = L.x; rgb[3 * j + 1] = L.y; rgb[3 * j + 2] = L.z;
             pos[2 * j] = ps[i].pos[0]; pos[2 * j + 1] = ps[i].pos[1];

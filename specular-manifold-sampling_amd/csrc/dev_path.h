@@ -264,6 +264,9 @@ struct PathSlots {
     float *m_dir;                    // x 3: direction receiver -> caster vertex of the path
     float *m_ref;                    // x 3: receiver position
     int *nm;                         // MNEE checks per sample
+    sms_ms_walk_in *mm;              // multi-scatter MNEE checks (values in m_val, counts in nm)
+    float *mm_pos;                   // x 3 SMS_MS_MAX_BOUNCES: the chain the path tracer found
+    int *mm_npos;
     float *L;                        // x 3: the path's own terms
     float *pos;                      // x 2: film position
 };
@@ -274,12 +277,80 @@ SMS_DEV void sms_query_from_si(const SI &si, sms_ss_query &q) {
     st3(q.wi, si.wi);
 }
 
-// One sample of path_sms_ss / path_filtered_ss (SamplingIntegrator::sample); j = index of the
-// sample in the batch (slot row), idx = index in the pass (stream)
+// The multi-scatter integrators' record of the specular chain a path is on: MNEEHelper
+// (path_sms_ms.cpp:54-143: receiver, first caster, the chain's positions) and PathRecord
+// (path_filtered_ms.cpp:237-300: the chain's scattering types); states 0 Default, 1 HitReceiver,
+// 2 HitCaster, 3 HitEmitter; depth_receiver = the path depth of the chain's receiver.
+struct ChainRecord {
+    int state, bounce, shape0, npos, depth_receiver;
+    V3 endpoint, pos[SMS_MS_MAX_BOUNCES];
+};
+SMS_DEV void chain_reset(ChainRecord &c) { c.state = 0; c.bounce = -1; c.shape0 = -1; c.npos = 0; }
+
+// caustic_eta of BSDF::ior (path_filtered_ss.cpp:85-87, path_filtered_ms.cpp:268-270)
+SMS_DEV float caustic_eta_of(const DShape &sh) {
+    if (sh.bsdf == SMS_BSDF_DIELECTRIC) return sh.eta_r;
+    if (sh.bsdf == SMS_BSDF_CONDUCTOR && sh.k == 0.f) return sh.eta_r;
+    return 1.f;
+}
+
+// MNEEHelper::state_transition (path_sms_ms.cpp:85-138) to `next` at path depth `depth`
+SMS_DEV void mnee_transition(const DScene &S, ChainRecord &c, const SI &next, int depth) {
+    const int bounces = S.cfg.bounces;
+    const int flags = next.valid ? S.shapes[next.shape].flags : 0;
+    if (next.valid && (flags & SMS_FLAG_CAUSTIC_RECEIVER)) {
+        chain_reset(c); c.state = 1; c.endpoint = next.p; c.depth_receiver = depth;
+        return;
+    }
+    const int e = next.valid ? S.shapes[next.shape].emitter : -1;
+    if (c.state == 1) {
+        if (next.valid && (flags & SMS_FLAG_CAUSTIC_CASTER_MS)) {
+            c.shape0 = next.shape; c.pos[0] = next.p; c.npos = 1; c.bounce = 1; c.state = 2;
+        } else chain_reset(c);
+    } else if (c.state == 2) {
+        if (c.bounce < bounces && next.valid && (flags & (SMS_FLAG_CAUSTIC_CASTER_MS | SMS_FLAG_CAUSTIC_BOUNCER))) {
+            c.pos[c.npos < SMS_MS_MAX_BOUNCES ? c.npos : SMS_MS_MAX_BOUNCES - 1] = next.p;
+            c.npos++; c.bounce++;
+        } else if (c.bounce == bounces && e >= 0 && S.emitters[e].ms) {
+            c.state = 3;
+        } else chain_reset(c);
+    } else chain_reset(c);
+}
+
+// PathRecord::state_transition (path_filtered_ms.cpp:248-299) from `cur` (BSDF-sampled as a
+// reflection / transmission) to `next` at path depth `depth`
+SMS_DEV void record_transition(const DScene &S, ChainRecord &c, const SI &cur, const SI &next, bool refl, bool trans,
+                               int depth) {
+    const int bounces = S.cfg.bounces;
+    const int flags = next.valid ? S.shapes[next.shape].flags : 0;
+    if (next.valid && (flags & SMS_FLAG_CAUSTIC_RECEIVER)) {
+        chain_reset(c); c.state = 1; c.depth_receiver = depth;
+        return;
+    }
+    const int e = next.valid ? S.shapes[next.shape].emitter : -1;
+    if (c.state == 1) {
+        if (next.valid && (flags & SMS_FLAG_CAUSTIC_CASTER_MS)) { c.bounce = 1; c.state = 2; }
+        else chain_reset(c);
+    } else if (c.state == 2) {
+        if (c.bounce < bounces && next.valid && (flags & (SMS_FLAG_CAUSTIC_CASTER_MS | SMS_FLAG_CAUSTIC_BOUNCER))) {
+            const float eta = caustic_eta_of(S.shapes[cur.shape]);
+            if ((eta == 1.f && trans) || (eta != 1.f && refl)) chain_reset(c);
+            else c.bounce++;
+        } else if (c.bounce == bounces && e >= 0 && S.emitters[e].ms) {
+            c.state = 3;
+        } else chain_reset(c);
+    } else chain_reset(c);
+}
+
+// One sample of path_sms_ss / path_filtered_ss / path_sms_ms / path_filtered_ms
+// (SamplingIntegrator::sample); j = index of the sample in the batch (slot row), idx = index in
+// the pass (stream)
 SMS_DEV void path_sample(const DScene &S, const sms_path_config &pc, uint64_t seed, int64_t idx, int64_t j, int spp,
                          const PathSlots &P) {
-    const bool sms = pc.integrator == SMS_PATH_SMS_SS, only = pc.caustics_only != 0;
+    const bool ms = pc.integrator == SMS_PATH_SMS_MS || pc.integrator == SMS_PATH_FILTERED_MS;
+    const bool sms = pc.integrator == SMS_PATH_SMS_SS || pc.integrator == SMS_PATH_SMS_MS, only = pc.caustics_only != 0;
     const int depth_cap = pc.max_depth < 0 ? SMS_PATH_MAX_DEPTH : pc.max_depth;
+    const int chain = ms ? S.cfg.bounces : 1;           // specular vertices between receiver and light
     Pcg32 r;
     stream_seed(r, query_key(seed, (uint64_t) idx), PATH_STREAM);
     float pos_x, pos_y;
@@ -290,6 +361,14 @@ SMS_DEV void path_sample(const DScene &S, const sms_path_config &pc, uint64_t se
     SI si = ray_intersect(S, ray);
     SI si_prev = si_invalid();
     int first_diffuse = -1;
+    bool specular_camera_path = true;
+    ChainRecord rec;
+    chain_reset(rec);
+    rec.depth_receiver = -1; rec.endpoint = v3(0.f, 0.f, 0.f);
+    if (ms) {
+        if (sms) mnee_transition(S, rec, si, 1);
+        else record_transition(S, rec, si, si, false, false, 1);
+    }
     if (si.valid && !only) {
         const int e = S.shapes[si.shape].emitter;
         if (e >= 0) L = L + emitter_eval_hit(S, e, si);
@@ -305,41 +384,44 @@ SMS_DEV void path_sample(const DScene &S, const sms_path_config &pc, uint64_t se
             thr = thr * (1.f / q);
         }
         if (depth >= depth_cap) break;
-        if (only && first_diffuse >= 0 && depth > first_diffuse + 1) break;     // nothing left to count
+        if (only && first_diffuse >= 0 && depth > first_diffuse + chain) break;     // nothing left to count
         const DShape &sh = S.shapes[si.shape];
         const bool smooth = sh.bsdf == SMS_BSDF_DIFFUSE;
         if (first_diffuse < 0 && smooth) first_diffuse = depth;
         const bool receiver = (sh.flags & SMS_FLAG_CAUSTIC_RECEIVER) != 0;
         const bool from_receiver = si_prev.valid && (S.shapes[si_prev.shape].flags & SMS_FLAG_CAUSTIC_RECEIVER);
-        const bool glint_case = depth == 1 && !pc.disable_reflected_emitters;
+        const bool glint_case = !ms && depth == 1 && !pc.disable_reflected_emitters;
         bool on_caster;
         float caustic_eta = 1.f;
-        if (sms) {
+        if (ms) {
+            on_caster = (sh.flags & (SMS_FLAG_CAUSTIC_CASTER_MS | SMS_FLAG_CAUSTIC_BOUNCER)) != 0;
+            caustic_eta = caustic_eta_of(sh);
+        } else if (sms) {
             on_caster = (sh.flags & SMS_FLAG_CAUSTIC_CASTER_SS) != 0;
-            // SMS at a caustic receiver (path_sms_ss.cpp:118-124)
-            if (receiver && !on_caster && (pc.max_depth < 0 || depth + 1 < pc.max_depth) &&
-                (!only || depth == first_diffuse) && nq < P.depth_slots) {
-                const int64_t k = j * P.depth_slots + nq;
-                sms_ss_query q;
-                float *z = (float *) &q;
-                for (int a = 0; a < (int) (sizeof(q) / 4); ++a) z[a] = 0.f;
-                sms_query_from_si(si, q);
-                P.q[k] = q;
-                st3(P.q_thr + 3 * k, thr);
-                ++nq;
-            }
         } else {
             on_caster = (sh.flags & (SMS_FLAG_CAUSTIC_CASTER_SS | SMS_FLAG_CAUSTIC_CASTER_MS)) != 0;
-            // caustic_eta from BSDF::ior (path_filtered_ss.cpp:85-87)
-            if (sh.bsdf == SMS_BSDF_DIELECTRIC) caustic_eta = sh.eta_r;
-            else if (sh.bsdf == SMS_BSDF_CONDUCTOR && sh.k == 0.f) caustic_eta = sh.eta_r;
+            caustic_eta = caustic_eta_of(sh);
         }
-        // emitter sampling (path_sms_ss.cpp:142-161, path_filtered_ss.cpp:107-142)
+        // SMS at a caustic receiver (path_sms_ss.cpp:118-124, path_sms_ms.cpp:228-234)
+        if (sms && receiver && !on_caster && (pc.max_depth < 0 || depth + chain < pc.max_depth) &&
+            (!only || depth == first_diffuse) && nq < P.depth_slots) {
+            const int64_t k = j * P.depth_slots + nq;
+            sms_ss_query q;
+            float *z = (float *) &q;
+            for (int a = 0; a < (int) (sizeof(q) / 4); ++a) z[a] = 0.f;
+            sms_query_from_si(si, q);
+            P.q[k] = q;
+            st3(P.q_thr + 3 * k, thr);
+            ++nq;
+        }
+        // emitter sampling (path_sms_ss.cpp:142-161, path_filtered_ss.cpp:107-142,
+        // path_sms_ms.cpp:240-270, path_filtered_ms.cpp:97-140)
         bool nee, valid_connection = false;
         if (sms) {
             nee = (smooth && !on_caster) || glint_case;
         } else {
-            valid_connection = on_caster && (from_receiver || glint_case);
+            valid_connection = ms ? (rec.state == 2 && rec.bounce == S.cfg.bounces)
+                                  : on_caster && (from_receiver || glint_case);
             nee = smooth && (!on_caster || valid_connection);
         }
         if (nee) {
@@ -352,7 +434,7 @@ SMS_DEV void path_sample(const DScene &S, const sms_path_config &pc, uint64_t se
                 const V3 val = bsdf_eval(sh, si.wi, wo);
                 const float bpdf = bsdf_pdf(sh, si.wi, wo);
                 float mis = ds.delta ? 1.f : mis_power(ds.pdf, bpdf);
-                if (!sms && valid_connection && depth > 1) {
+                if (!sms && valid_connection && (ms || depth > 1)) {
                     const bool refraction = wo.z * si.wi.z < 0.f, reflection = !refraction;
                     if ((caustic_eta == 1.f && !reflection) || (caustic_eta != 1.f && !refraction)) mis = 0.f;
                 }
@@ -366,12 +448,59 @@ SMS_DEV void path_sample(const DScene &S, const sms_path_config &pc, uint64_t se
         const BsdfSample bs = bsdf_sample(sh, si.wi, s1, s2);
         thr = mul3(thr, bs.weight);
         eta = eta * bs.eta;
+        if (!bs.delta) specular_camera_path = false;
         if (thr.x == 0.f && thr.y == 0.f && thr.z == 0.f) break;
         ray = spawn_ray(si.p, to_world(si.sh_s, si.sh_t, si.sh_n, bs.wo));
         const SI hit = ray_intersect(S, ray);
+        if (ms) {
+            if (sms) mnee_transition(S, rec, hit, depth + 1);
+            else record_transition(S, rec, si, hit, bs.reflection, bs.transmission, depth + 1);
+        }
         const int e = hit.valid ? S.shapes[hit.shape].emitter : -1;
         if (e >= 0) {
-            if (sms) {
+            if (ms && sms) {
+                if (!on_caster || specular_camera_path) {
+                    if (!only) {
+                        const V3 ev = emitter_eval_hit(S, e, hit);
+                        const float epdf = bs.delta ? 0.f : pdf_emitter_hit(S, e, hit, si.p);
+                        const float mis = mis_power(bs.pdf, epdf);
+                        L = L + mul3(thr * mis, ev);
+                    }
+                } else if (S.cfg.mnee_init && !pc.biased_mnee && rec.state == 3 && rec.bounce == S.cfg.bounces) {
+                    // MNEE check (path_sms_ms.cpp:290-329): the walk from the chain's receiver
+                    // through its first caster, compared vertex by vertex with the chain found
+                    sms_ms_walk_in w;
+                    float *z = (float *) &w;
+                    for (int a = 0; a < (int) (sizeof(w) / 4); ++a) z[a] = 0.f;
+                    w.sample[0] = pcg32_float(r); w.sample[1] = pcg32_float(r);
+                    if ((!only || rec.depth_receiver == first_diffuse) && P.mm && nm < P.depth_slots) {
+                        const int64_t k = j * P.depth_slots + nm;
+                        st3(w.si_p, rec.endpoint);
+                        w.shape = rec.shape0;
+                        const V3 ed = normalize(hit.p - rec.endpoint);
+                        st3(w.ei.p, hit.p); st3(w.ei.n, hit.n); st3(w.ei.d, ed);
+                        w.ei.kind = SMS_EI_AREA; w.ei.emitter = e;
+                        w.first_path = 1;
+                        P.mm[k] = w;
+                        st3(P.m_val + 3 * k, mul3(thr, emitter_eval_hit(S, e, hit)));
+                        const int np = rec.npos < SMS_MS_MAX_BOUNCES ? rec.npos : SMS_MS_MAX_BOUNCES;
+                        for (int v = 0; v < np; ++v) st3(P.mm_pos + 3 * (SMS_MS_MAX_BOUNCES * k + v), rec.pos[v]);
+                        P.mm_npos[k] = np;
+                        ++nm;
+                    }
+                }
+            } else if (ms) {
+                const bool light_hit_valid = rec.state == 3 && rec.bounce == S.cfg.bounces;
+                if (!on_caster || light_hit_valid || specular_camera_path) {
+                    const bool take = !only || (light_hit_valid && rec.depth_receiver == first_diffuse);
+                    if (take) {
+                        const V3 ev = emitter_eval_hit(S, e, hit);
+                        const float epdf = bs.delta ? 0.f : pdf_emitter_hit(S, e, hit, si.p);
+                        const float mis = mis_power(bs.pdf, epdf);
+                        L = L + mul3(thr * mis, ev);
+                    }
+                }
+            } else if (sms) {
                 if (!on_caster || glint_case) {
                     if (!only) {
                         const V3 ev = emitter_eval_hit(S, e, hit);

@@ -258,9 +258,9 @@ SMS_MS_KERNEL_DECLS(4)
 // path integrators and film reconstruction (sms_path_kernels.hip)
 __global__ void k_path_trace(DScene S, sms_path_config pc, uint64_t seed, int64_t begin, int64_t n, int spp, PathSlots P);
 __global__ void k_path_compact(PathSlots P, int64_t n, const int *q_off, sms_ss_query *dq, const int *m_off,
-                               sms_ss_walk_in *dm);
+                               sms_ss_walk_in *dm, sms_ms_walk_in *dmm);
 __global__ void k_path_gather(PathSlots P, int64_t n, const int *q_off, const float *est, const int *m_off,
-                              const sms_ss_walk_out *mout, float uniqueness_threshold);
+                              const sms_ss_walk_out *mout, const sms_ms_walk_out *mmout, float uniqueness_threshold);
 __global__ void k_film_splat(FilmFilter F, int w, int h, int spp, const float *rgb, const float *pos, float *image,
                              float *weight);
 

@@ -21,6 +21,9 @@ struct PathBuffers {
     int *nq = nullptr, *nm = nullptr, *q_off = nullptr, *m_off = nullptr;
     sms_ss_query *dq = nullptr; float *dq_rgb = nullptr; size_t dq_cap = 0;
     sms_ss_walk_in *dm = nullptr; sms_ss_walk_out *dm_out = nullptr; size_t dm_cap = 0;
+    // multi-scatter MNEE checks: slots (walks, the path's chain, its length) and dense lists
+    sms_ms_walk_in *mm = nullptr; float *mm_pos = nullptr; int *mm_npos = nullptr; size_t mm_rows = 0;
+    sms_ms_walk_in *dmm = nullptr; sms_ms_walk_out *dmm_out = nullptr; size_t dmm_cap = 0;
     void *scan_tmp = nullptr; size_t scan_bytes = 0;
     int *totals = nullptr;           // pinned host: [0] queries, [1] MNEE walks
 };
@@ -36,7 +39,8 @@ sms_status grow(sms_ctx *ctx, T **p, size_t count) {
 }
 
 // slot and per-sample arrays for `rows` samples of `depth_slots` slots; MNEE slots when asked
-sms_status path_alloc(sms_ctx *ctx, size_t rows, int depth_slots, bool mnee) {
+// (multi-scatter ones with ms)
+sms_status path_alloc(sms_ctx *ctx, size_t rows, int depth_slots, bool mnee, bool ms) {
     if (!ctx->path) ctx->path = new PathBuffers();
     PathBuffers &B = *ctx->path;
     const size_t slot_rows = rows * (size_t) std::max(depth_slots, 1);
@@ -46,12 +50,19 @@ sms_status path_alloc(sms_ctx *ctx, size_t rows, int depth_slots, bool mnee) {
         if (!st) st = grow(ctx, &B.q_thr, 3 * slot_rows);
         if (!st) B.slot_rows = slot_rows;
     }
-    if (!st && mnee && slot_rows > B.m_rows) {
+    if (!st && mnee && ms && slot_rows > B.mm_rows) {
+        if (!st) st = grow(ctx, &B.mm, slot_rows);
+        if (!st) st = grow(ctx, &B.mm_pos, 3 * SMS_MS_MAX_BOUNCES * slot_rows);
+        if (!st) st = grow(ctx, &B.mm_npos, slot_rows);
+        if (!st) st = grow(ctx, &B.m_val, 3 * slot_rows);
+        if (!st) { B.mm_rows = slot_rows; B.m_rows = 0; }
+    }
+    if (!st && mnee && !ms && slot_rows > B.m_rows) {
         if (!st) st = grow(ctx, &B.m, slot_rows);
         if (!st) st = grow(ctx, &B.m_val, 3 * slot_rows);
         if (!st) st = grow(ctx, &B.m_dir, 3 * slot_rows);
         if (!st) st = grow(ctx, &B.m_ref, 3 * slot_rows);
-        if (!st) B.m_rows = slot_rows;
+        if (!st) { B.m_rows = slot_rows; B.mm_rows = 0; }
     }
     if (!st && rows > B.rows) {
         // counts and offsets hold one entry past the batch: the scan's last offset is the total
@@ -83,8 +94,11 @@ sms_status scan(sms_ctx *ctx, const int *count, int *off, int64_t rows) {
 // the scene features the path integrators evaluate
 sms_status check_path_scene(sms_ctx *ctx, const sms_path_config *pc) {
     if (!pc) return fail(ctx, SMS_E_INVALID, "path integrator: config is NULL");
-    if (pc->integrator != SMS_PATH_SMS_SS && pc->integrator != SMS_PATH_FILTERED_SS)
+    if (pc->integrator < SMS_PATH_SMS_SS || pc->integrator > SMS_PATH_FILTERED_MS)
         return fail(ctx, SMS_E_INVALID, "path integrator: unknown integrator");
+    if ((pc->integrator == SMS_PATH_SMS_MS || pc->integrator == SMS_PATH_FILTERED_MS) &&
+        (ctx->scene.cfg.bounces < 1 || ctx->scene.cfg.bounces > SMS_MS_MAX_BOUNCES))
+        return fail(ctx, SMS_E_UNSUPPORTED, "path integrator: multi-scatter chains of 1..4 vertices are built");
     if (pc->max_depth < -1 || pc->max_depth > SMS_PATH_MAX_DEPTH)
         return fail(ctx, SMS_E_INVALID, "path integrator: max_depth must be -1 or in [0, SMS_PATH_MAX_DEPTH]");
     if (!ctx->scene.has_cam) return fail(ctx, SMS_E_INVALID, "path integrator: the scene has no camera");
@@ -120,14 +134,15 @@ sms_status path_pass(sms_ctx *ctx, const sms_path_config *pc, int32_t spp, uint6
     if (spp <= 0 || !rgb || !pos) return fail(ctx, SMS_E_INVALID, "path integrator: bad arguments");
     if (ctx->capturing) return fail(ctx, SMS_E_INVALID, "path integrator: the context is capturing a graph");
     const sms_config &cfg = ctx->scene.cfg;
-    const bool sms = pc->integrator == SMS_PATH_SMS_SS;
+    const bool ms = pc->integrator == SMS_PATH_SMS_MS || pc->integrator == SMS_PATH_FILTERED_MS;
+    const bool sms = pc->integrator == SMS_PATH_SMS_SS || pc->integrator == SMS_PATH_SMS_MS;
     const bool mnee = sms && cfg.mnee_init && !pc->biased_mnee;
     const int depth_slots = pc->max_depth < 0 ? SMS_PATH_MAX_DEPTH : pc->max_depth;
     const int64_t n = (int64_t) ctx->scene.cam.w * ctx->scene.cam.h * spp;
     const int64_t batch = std::min<int64_t>(n, SMS_PATH_CHUNK);
     sms_stats tot;
     memset(&tot, 0, sizeof(tot));
-    st = path_alloc(ctx, (size_t) batch, depth_slots, mnee);
+    st = path_alloc(ctx, (size_t) batch, depth_slots, mnee, ms);
     if (st) return st;
     PathBuffers &B = *ctx->path;
     for (int64_t c0 = 0; c0 < n; c0 += SMS_PATH_CHUNK) {
@@ -135,7 +150,8 @@ sms_status path_pass(sms_ctx *ctx, const sms_path_config *pc, int32_t spp, uint6
         PathSlots P;
         P.depth_slots = depth_slots;
         P.q = B.q; P.q_thr = B.q_thr; P.nq = B.nq;
-        P.m = mnee ? B.m : nullptr; P.m_val = B.m_val; P.m_dir = B.m_dir; P.m_ref = B.m_ref;
+        P.m = mnee && !ms ? B.m : nullptr; P.m_val = B.m_val; P.m_dir = B.m_dir; P.m_ref = B.m_ref;
+        P.mm = mnee && ms ? B.mm : nullptr; P.mm_pos = B.mm_pos; P.mm_npos = B.mm_npos;
         P.nm = mnee ? B.nm : nullptr;
         P.L = rgb + 3 * c0; P.pos = pos + 2 * c0;
         // the scan's extra entry is zero
@@ -158,36 +174,50 @@ sms_status path_pass(sms_ctx *ctx, const sms_path_config *pc, int32_t spp, uint6
             if (st) return st;
             B.dq_cap = (size_t) nq;
         }
-        if ((size_t) nm > B.dm_cap) {
+        if (!ms && (size_t) nm > B.dm_cap) {
             st = grow(ctx, &B.dm, (size_t) nm);
             if (!st) st = grow(ctx, &B.dm_out, (size_t) nm);
             if (st) return st;
             B.dm_cap = (size_t) nm;
         }
+        if (ms && (size_t) nm > B.dmm_cap) {
+            st = grow(ctx, &B.dmm, (size_t) nm);
+            if (!st) st = grow(ctx, &B.dmm_out, (size_t) nm);
+            if (st) return st;
+            B.dmm_cap = (size_t) nm;
+        }
         if (nq > 0 || nm > 0) {
             hipLaunchKernelGGL(k_path_compact, dim3(blocks), dim3(256), 0, ctx->stream, P, nb, B.q_off,
                                nq > 0 ? B.dq : (sms_ss_query *) nullptr, B.m_off,
-                               nm > 0 ? B.dm : (sms_ss_walk_in *) nullptr);
+                               nm > 0 && !ms ? B.dm : (sms_ss_walk_in *) nullptr,
+                               nm > 0 && ms ? B.dmm : (sms_ms_walk_in *) nullptr);
             HIP_TRY(ctx, hipGetLastError());
         }
         if (nq > 0) {
-            // the batch's SMS estimates: the single-scatter estimator with the batch's seed
-            st = ss_prepare(ctx, B.dq, nq, B.dq_rgb);
+            // the batch's SMS estimates: the single- or multi-scatter estimator with the batch's seed
+            st = ms ? ms_prepare(ctx, B.dq, nq, B.dq_rgb) : ss_prepare(ctx, B.dq, nq, B.dq_rgb);
             if (!st) st = frame_begin(ctx, path_chunk_seed(seed, (uint64_t) (c0 / SMS_PATH_CHUNK)));
-            if (!st) st = ss_enqueue(ctx, B.dq, nq, B.dq_rgb);
+            if (!st) st = ms ? ms_enqueue(ctx, B.dq, nq, B.dq_rgb) : ss_enqueue(ctx, B.dq, nq, B.dq_rgb);
             sms_stats s;
             if (!st) st = estimator_stats(ctx, &s);
             if (st) return st;
             add_stats(tot, s);
         }
-        if (nm > 0) {
+        if (nm > 0 && !ms) {
             hipLaunchKernelGGL(k_ss_walk, dim3((unsigned) ((nm + 255) / 256)), dim3(256), 0, ctx->stream, ctx->scene,
                                B.dm, nm, B.dm_out);
             HIP_TRY(ctx, hipGetLastError());
         }
+        if (nm > 0 && ms) {
+            const dim3 g((unsigned) ((nm + 255) / 256));
+            if (cfg.bounces <= 2) hipLaunchKernelGGL(k_ms_walk_2, g, dim3(256), 0, ctx->stream, ctx->scene, B.dmm, nm, B.dmm_out);
+            else hipLaunchKernelGGL(k_ms_walk_4, g, dim3(256), 0, ctx->stream, ctx->scene, B.dmm, nm, B.dmm_out);
+            HIP_TRY(ctx, hipGetLastError());
+        }
         hipLaunchKernelGGL(k_path_gather, dim3(blocks), dim3(256), 0, ctx->stream, P, nb, B.q_off,
                            nq > 0 ? B.dq_rgb : (const float *) nullptr, B.m_off,
-                           nm > 0 ? B.dm_out : (const sms_ss_walk_out *) nullptr, cfg.uniqueness_threshold);
+                           nm > 0 && !ms ? B.dm_out : (const sms_ss_walk_out *) nullptr,
+                           nm > 0 && ms ? B.dmm_out : (const sms_ms_walk_out *) nullptr, cfg.uniqueness_threshold);
         HIP_TRY(ctx, hipGetLastError());
     }
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -220,7 +250,7 @@ void path_release(sms_ctx *ctx) {
     if (!ctx || !ctx->path) return;
     PathBuffers &B = *ctx->path;
     void *dev[] = {B.q, B.q_thr, B.m, B.m_val, B.m_dir, B.m_ref, B.nq, B.nm, B.q_off, B.m_off,
-                   B.dq, B.dq_rgb, B.dm, B.dm_out, B.scan_tmp};
+                   B.dq, B.dq_rgb, B.dm, B.dm_out, B.scan_tmp, B.mm, B.mm_pos, B.mm_npos, B.dmm, B.dmm_out};
     for (void *p : dev)
         if (p) (void) hipFree(p);
     if (B.totals) (void) hipHostFree(B.totals);

@@ -1,11 +1,11 @@
-// sms_path_kernels.hip -- kernels of the path integrators (path_sms_ss / path_filtered_ss) and
-// of the film reconstruction.
+// sms_path_kernels.hip -- kernels of the path integrators (path_sms_ss / path_filtered_ss /
+// path_sms_ms / path_filtered_ms) and of the film reconstruction.
 //
 // A pass batch runs: k_path_trace (one lane per camera sample: the whole path, its own terms,
 // the SMS queries of its caustic receivers and its MNEE checks into per-sample slots), an
 // exclusive scan of the slot counts, k_path_compact (the slots into dense (sample, depth)-ordered
-// lists), the single-scatter estimator over the dense queries and k_ss_walk over the dense MNEE
-// walks, then k_path_gather (per sample, in depth order: the path terms, + throughput x SMS
+// lists), the single- or multi-scatter estimator over the dense queries and k_ss_walk /
+// k_ms_walk over the dense MNEE walks, then k_path_gather (per sample, in depth order: the path terms, + throughput x SMS
 // estimate, + the MNEE completions whose walk did not re-find the path).
 #include "kernels.h"
 #include "dev_path.h"
@@ -28,17 +28,22 @@ __device__ void compact_rows(const T *__restrict__ slots, const int *__restrict_
 }
 
 __global__ void k_path_compact(PathSlots P, int64_t n, const int *__restrict__ q_off, sms_ss_query *__restrict__ dq,
-                               const int *__restrict__ m_off, sms_ss_walk_in *__restrict__ dm) {
+                               const int *__restrict__ m_off, sms_ss_walk_in *__restrict__ dm,
+                               sms_ms_walk_in *__restrict__ dmm) {
     const int64_t j = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     if (dq) compact_rows(P.q, P.nq, q_off, P.depth_slots, j, dq);
     if (dm) compact_rows(P.m, P.nm, m_off, P.depth_slots, j, dm);
+    if (dmm) compact_rows(P.mm, P.nm, m_off, P.depth_slots, j, dmm);
 }
 
-// per sample: L (already in P.L) + sum_k thr_k * est_k + the MNEE completions; written in place
+// per sample: L (already in P.L) + sum_k thr_k * est_k + the MNEE completions; written in place.
+// A single-scatter MNEE walk re-finds the path when its solution direction matches within the
+// uniqueness threshold (path_sms_ss.cpp:213-221); a multi-scatter one when every vertex of its
+// chain lies within 1e-5 of the path's (path_sms_ms.cpp:302-311).
 __global__ void k_path_gather(PathSlots P, int64_t n, const int *__restrict__ q_off, const float *__restrict__ est,
                               const int *__restrict__ m_off, const sms_ss_walk_out *__restrict__ mout,
-                              float uniqueness_threshold) {
+                              const sms_ms_walk_out *__restrict__ mmout, float uniqueness_threshold) {
     const int64_t j = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     V3 L = ld3(P.L + 3 * j);
@@ -48,7 +53,7 @@ __global__ void k_path_gather(PathSlots P, int64_t n, const int *__restrict__ q_
         const int64_t d = q_off[j] + k;
         L = L + mul3(ld3(P.q_thr + 3 * (j * D + k)), ld3(est + 3 * d));
     }
-    if (P.nm) {
+    if (P.nm && mout) {
         const int nm = P.nm[j];
         for (int k = 0; k < nm; ++k) {
             const int64_t s = j * D + k;
@@ -58,6 +63,18 @@ __global__ void k_path_gather(PathSlots P, int64_t n, const int *__restrict__ q_
                 const V3 dm = normalize(ld3(o.p) - ld3(P.m_ref + 3 * s));
                 if (fabsf(dot(ld3(P.m_dir + 3 * s), dm) - 1.f) >= uniqueness_threshold) ok = false;
             }
+            if (!ok) L = L + ld3(P.m_val + 3 * s);
+        }
+    } else if (P.nm && mmout) {
+        const int nm = P.nm[j];
+        for (int k = 0; k < nm; ++k) {
+            const int64_t s = j * D + k;
+            const sms_ms_walk_out &o = mmout[m_off[j] + k];
+            bool ok = o.success != 0;
+            const int npos = P.mm_npos[s];
+            for (int v = 0; ok && v < o.n_vertices && v < SMS_MS_MAX_BOUNCES; ++v)
+                if (!(v < npos) || !(norm(ld3(P.mm_pos + 3 * (SMS_MS_MAX_BOUNCES * s + v)) - ld3(o.p[v])) < 1e-5f))
+                    ok = false;
             if (!ok) L = L + ld3(P.m_val + 3 * s);
         }
     }

@@ -118,7 +118,7 @@ class SmsPathConfig(C.Structure):
         return c
 
 
-PATH_SMS_SS, PATH_FILTERED_SS = 0, 1
+PATH_SMS_SS, PATH_FILTERED_SS, PATH_SMS_MS, PATH_FILTERED_MS = 0, 1, 2, 3
 PATH_MAX_DEPTH = 16
 PATH_CHUNK = 1 << 20
 

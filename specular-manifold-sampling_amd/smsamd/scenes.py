@@ -298,17 +298,19 @@ def height_mesh(h, extent, y0, amplitude, flip):
             np.stack([uu, vv], -1).reshape(-1, 2).astype(np.float32), faces.astype(np.uint32))
 
 
-def slab(width=1024, height=1024, n=256, thickness=0.2, amplitude=0.01, seed=5):
-    """Glass slab (IOR 1.5) whose top (multi-scatter caster) and bottom (bouncer) faces carry
-    independent value-noise heightfields; diffuse floor below, sphere light above.  Chains
-    floor -> bottom face -> top face -> light have two specular vertices."""
+def slab(width=1024, height=1024, n=256, thickness=0.2, amplitude=0.01, seed=5, caster="top"):
+    """Glass slab (IOR 1.5) whose top and bottom faces carry independent value-noise
+    heightfields, one the multi-scatter caster (`caster`), the other a bouncer; diffuse floor
+    below, sphere light above.  Chains floor -> bottom face -> top face -> light have two
+    specular vertices.  The path integrators' chain records (path_sms_ms.cpp:85-138,
+    path_filtered_ms.cpp:248-299) expect the caster next to the receiver: caster="bottom"."""
     sc = Scene()
     top = height_mesh(value_noise(n, seed), 0.5, 0.5 + thickness / 2, amplitude, flip=False)
     bot = height_mesh(value_noise(n, seed + 1), 0.5, 0.5 - thickness / 2, amplitude, flip=True)
-    sc.add_mesh(top[0], top[3], normals=top[1], texcoords=top[2], bsdf=abi.BSDF_DIELECTRIC,
-                flags=abi.FLAG_CAUSTIC_CASTER_MS, eta=1.5)
-    sc.add_mesh(bot[0], bot[3], normals=bot[1], texcoords=bot[2], bsdf=abi.BSDF_DIELECTRIC,
-                flags=abi.FLAG_CAUSTIC_BOUNCER, eta=1.5)
+    ft, fb = ((abi.FLAG_CAUSTIC_CASTER_MS, abi.FLAG_CAUSTIC_BOUNCER) if caster == "top"
+              else (abi.FLAG_CAUSTIC_BOUNCER, abi.FLAG_CAUSTIC_CASTER_MS))
+    sc.add_mesh(top[0], top[3], normals=top[1], texcoords=top[2], bsdf=abi.BSDF_DIELECTRIC, flags=ft, eta=1.5)
+    sc.add_mesh(bot[0], bot[3], normals=bot[1], texcoords=bot[2], bsdf=abi.BSDF_DIELECTRIC, flags=fb, eta=1.5)
     sc.add_rectangle(plane_to_world((0, 0, 0), 2.0), abi.BSDF_DIFFUSE, flags=abi.FLAG_CAUSTIC_RECEIVER,
                      reflectance=(0.5, 0.5, 0.5))
     light = sc.add_sphere((0.1, 2.0, -0.1), 0.05, abi.BSDF_DIFFUSE, reflectance=(0.0, 0.0, 0.0))

@@ -72,6 +72,29 @@ def test_path_samples_pool_mnee_and_batches(gpu, oracle_mod):
     orc.close()
 
 
+def test_path_samples_slab_multi_scatter(gpu, oracle_mod):
+    """path_sms_ms (unbiased SMS; MNEE with and without its completions) and path_filtered_ms on
+    the slab, caustic term and whole image."""
+    sc = scenes.slab(24, 20, n=32, amplitude=0.01, caster="bottom")
+    sc.shapes[3]["radius"] = 0.3
+    sc.emitters[0]["radiance"] = (50.0, 50.0, 50.0)
+    cfg = abi.SmsConfig.default(bounces=2, max_trials=100000)
+    cfg_mnee = abi.SmsConfig.default(bounces=2, mnee_init=1, biased=1, max_trials=1, halfvector_constraints=1)
+    ctx = gpu.Context(sc, cfg)
+    orc = oracle_mod.Oracle(sc, cfg)
+    for only in (1, 0):
+        for integrator in (abi.PATH_SMS_MS, abi.PATH_FILTERED_MS):
+            r = _compare(ctx, orc, abi.SmsPathConfig.default(integrator=integrator, max_depth=8, caustics_only=only), 16, 9)
+            assert (r[:, 0] > 0).mean() > 0.001
+    ctx.set_config(cfg_mnee)
+    orc.set_config(cfg_mnee)
+    for biased_mnee in (0, 1):
+        _compare(ctx, orc, abi.SmsPathConfig.default(integrator=abi.PATH_SMS_MS, max_depth=8,
+                                                     biased_mnee=biased_mnee), 16, 10)
+    ctx.close()
+    orc.close()
+
+
 @pytest.mark.parametrize("stddev", [0.5, 0.3, 0.2, 0.0])
 def test_film_splat(gpu, oracle_mod, stddev):
     sc = scenes.sphere_dome(40, 30, light_radius=0.5, light_radiance=4.0)

@@ -142,6 +142,56 @@ def test_mnee_completion_is_unbiased():
     assert mb < mf - 4 * math.hypot(sb, sf)
 
 
+# ---------------------------------------------------------------- multi-scatter integrators
+@pytest.fixture(scope="module")
+def slab():
+    """The double-refraction slab with gentle faces (two-vertex chains: floor -> bottom face,
+    the caster next to the receiver -> top face, a bouncer -> light) under a larger light."""
+    sc = scenes.slab(16, 16, n=16, amplitude=0.005, caster="bottom")
+    sc.shapes[3]["radius"] = 0.4
+    sc.emitters[0]["radiance"] = (30.0, 30.0, 30.0)
+    return sc
+
+
+def test_ms_sms_caustic_matches_filtered_path_tracer(slab):
+    """path_sms_ms's caustic term (the multi-scatter estimator at the receiver) and
+    path_filtered_ms's (BSDF-sampled chains through the slab that PathRecord accepts) agree; so
+    do the whole images (emitter hits through specular camera paths, emitter sampling on the
+    floor, the chain's terms)."""
+    orc = oracle_lib.Oracle(slab, abi.SmsConfig.default(bounces=2, max_trials=100000), threads=8)
+    res = {}
+    for name, integ, only, passes in [("sms", abi.PATH_SMS_MS, 1, 8), ("filtered", abi.PATH_FILTERED_MS, 1, 32),
+                                      ("sms_full", abi.PATH_SMS_MS, 0, 8), ("filtered_full", abi.PATH_FILTERED_MS, 0, 32)]:
+        pc = abi.SmsPathConfig.default(integrator=integ, max_depth=8, caustics_only=only)
+        res[name] = _mean_se(_caustic(orc, pc, passes, seed0=100 if integ == abi.PATH_SMS_MS else 1000))
+    print({k: (round(m, 6), round(s, 6)) for k, (m, s) in res.items()})
+    for a, b in (("sms", "filtered"), ("sms_full", "filtered_full")):
+        (ma, sa), (mb, sb) = res[a], res[b]
+        assert ma > 0 and abs(ma - mb) < 4 * math.hypot(sa, sb), (a, b)
+    # the caustic is part of the whole image, which holds more (floor lit directly)
+    assert res["filtered_full"][0] > res["filtered"][0] + 4 * res["filtered_full"][1]
+    orc.close()
+
+
+def test_ms_mnee_completion_is_unbiased(slab):
+    """Multi-scatter MNEE (one deterministic walk per receiver, the half-vector constraints)
+    plus path_sms_ms's completions (the BSDF-sampled chains the MNEE walk does not re-find vertex
+    for vertex) agrees with the filtered path tracer; biased MNEE, which drops them, is lower."""
+    cfg = abi.SmsConfig.default(bounces=2, mnee_init=1, biased=1, max_trials=1, halfvector_constraints=1)
+    orc = oracle_lib.Oracle(slab, cfg, threads=8)
+    res = {}
+    for bm in (0, 1):
+        pc = abi.SmsPathConfig.default(integrator=abi.PATH_SMS_MS, max_depth=8, caustics_only=1, biased_mnee=bm)
+        res[bm] = _mean_se(_caustic(orc, pc, 24, seed0=300))
+    flt = _mean_se(_caustic(orc, abi.SmsPathConfig.default(integrator=abi.PATH_FILTERED_MS, max_depth=8,
+                                                           caustics_only=1), 48, seed0=2000))
+    print(res, flt)
+    (mu, su), (mb, sb), (mf, sf) = res[0], res[1], flt
+    assert abs(mu - mf) < 4 * math.hypot(su, sf)
+    assert mb < mu - 4 * math.hypot(sb, su)
+    orc.close()
+
+
 # ---------------------------------------------------------------- film
 @pytest.fixture(scope="module")
 def film_scene():
