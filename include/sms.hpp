@@ -31,6 +31,7 @@
 #include <cstdint>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <set>
 #include <stdexcept>
 #include <string>
@@ -155,8 +156,9 @@ inline sms_path_config path_config(const Properties &p, int32_t integrator, floa
     c.integrator = integrator;
     c.max_depth = (int32_t) p.int_("max_depth", -1);
     c.rr_depth = (int32_t) p.int_("rr_depth", 5);
-    c.disable_reflected_emitters = p.bool_("disable_reflected_emitters", false);
-    if (integrator == SMS_PATH_SMS_SS) c.biased_mnee = p.bool_("biased_mnee", false);
+    if (integrator == SMS_PATH_SMS_SS || integrator == SMS_PATH_FILTERED_SS)
+        c.disable_reflected_emitters = p.bool_("disable_reflected_emitters", false);
+    if (integrator == SMS_PATH_SMS_SS || integrator == SMS_PATH_SMS_MS) c.biased_mnee = p.bool_("biased_mnee", false);
     c.filter_stddev = filter_stddev;
     return c;
 }
@@ -332,6 +334,9 @@ class PathIntegrator {
 public:
     PathIntegrator(const Properties &props, int32_t integrator, float filter_stddev)
         : m_cfg(ss_config(props)), m_pc(path_config(props, integrator, filter_stddev)) {}
+    PathIntegrator(const sms_config &cfg, const Properties &props, int32_t integrator, float filter_stddev)
+        : m_cfg(cfg), m_pc(path_config(props, integrator, filter_stddev)) {}
+    virtual ~PathIntegrator() = default;
     const sms_config &config() const { return m_cfg; }
     const sms_path_config &path() const { return m_pc; }
     /* keep only the single-scatter caustic paths through the first non-specular vertex */
@@ -384,6 +389,96 @@ class SingleScatterFilteredPathIntegrator : public PathIntegrator {
 public:
     explicit SingleScatterFilteredPathIntegrator(const Properties &props, float filter_stddev = 0.5f)
         : PathIntegrator(props, SMS_PATH_FILTERED_SS, filter_stddev) {}
+};
+
+/* MultiScatterSMSPathIntegrator (path_sms_ms.cpp:39-164): the shared SMS keys and "bounces" (2),
+   "biased_mnee" */
+class MultiScatterSMSPathIntegrator : public PathIntegrator {
+public:
+    explicit MultiScatterSMSPathIntegrator(const Properties &props, float filter_stddev = 0.5f)
+        : PathIntegrator(ms_config(props), props, SMS_PATH_SMS_MS, filter_stddev) {}
+};
+
+/* MultiScatterFilteredPathIntegrator (path_filtered_ms.cpp:27-36): "bounces" (2), the length of
+   the chains it keeps */
+class MultiScatterFilteredPathIntegrator : public PathIntegrator {
+public:
+    explicit MultiScatterFilteredPathIntegrator(const Properties &props, float filter_stddev = 0.5f)
+        : PathIntegrator(filtered_ms_config(props), props, SMS_PATH_FILTERED_MS, filter_stddev) {}
+
+private:
+    static sms_config filtered_ms_config(const Properties &p) {
+        sms_config c = default_config();
+        c.bounces = (int32_t) p.int_("bounces", 2);
+        return c;
+    }
+};
+
+/* ---------------------------------------------------------------- scene files
+   A Mitsuba 2 scene file read by sms_scene_load: its description (for Context), the
+   integrator's type and properties, the sampler's sample count and the film's filter; and the
+   path integrator it names, created from those properties as the reference's plugin manager
+   would (src/libcore/plugin.cpp:27-59). */
+class SceneFile {
+public:
+    explicit SceneFile(const std::string &path, const std::map<std::string, std::string> &defines = {},
+                       bool substitute = false) {
+        std::vector<std::string> defs;
+        for (const auto &kv : defines) defs.push_back(kv.first + "=" + kv.second);
+        std::vector<const char *> ptrs;
+        for (const auto &d : defs) ptrs.push_back(d.c_str());
+        char err[1024] = {0};
+        const sms_status st = sms_scene_load(path.c_str(), ptrs.empty() ? nullptr : ptrs.data(), (int32_t) ptrs.size(),
+                                             substitute ? SMS_LOAD_SUBSTITUTE : 0, &m_file, err, (int32_t) sizeof(err));
+        if (st) throw Error(st, std::string("sms_scene_load: ") + err);
+        sms_scene_info info;
+        check(sms_scene_file_info(m_file, &info), nullptr, "sms_scene_file_info");
+        m_integrator = info.integrator;
+        m_spp = info.spp;
+        m_stddev = info.filter_gaussian ? info.filter_stddev : 0.f;
+        for (int32_t i = 0; i < info.n_properties; ++i) {
+            sms_property p;
+            check(sms_scene_file_property(m_file, i, &p), nullptr, "sms_scene_file_property");
+            if (p.type == SMS_PROP_BOOL) m_props.set_bool(p.name, p.value != 0.0);
+            else if (p.type == SMS_PROP_INT) m_props.set_int(p.name, (int64_t) p.value);
+            else if (p.type == SMS_PROP_FLOAT) m_props.set_float(p.name, p.value);
+        }
+        for (int32_t i = 0; i < info.n_notes; ++i) m_notes.push_back(sms_scene_file_note(m_file, i));
+    }
+    ~SceneFile() {
+        if (m_file) sms_scene_file_free(m_file);
+    }
+    SceneFile(const SceneFile &) = delete;
+    SceneFile &operator=(const SceneFile &) = delete;
+
+    const sms_scene_desc &desc() const { return *sms_scene_file_desc(m_file); }
+    const std::string &integrator() const { return m_integrator; }
+    const Properties &properties() const { return m_props; }
+    int spp() const { return m_spp; }
+    float filter_stddev() const { return m_stddev; }
+    const std::vector<std::string> &notes() const { return m_notes; }
+
+    /* the path integrator the file names (path_sms_ss, path_filtered_ss, path_sms_ms,
+       path_filtered_ms); others are SMS_E_UNSUPPORTED */
+    std::unique_ptr<PathIntegrator> make_path_integrator() const {
+        if (m_integrator == "path_sms_ss")
+            return std::unique_ptr<PathIntegrator>(new SingleScatterSMSPathIntegrator(m_props, m_stddev));
+        if (m_integrator == "path_filtered_ss")
+            return std::unique_ptr<PathIntegrator>(new SingleScatterFilteredPathIntegrator(m_props, m_stddev));
+        if (m_integrator == "path_sms_ms")
+            return std::unique_ptr<PathIntegrator>(new MultiScatterSMSPathIntegrator(m_props, m_stddev));
+        if (m_integrator == "path_filtered_ms")
+            return std::unique_ptr<PathIntegrator>(new MultiScatterFilteredPathIntegrator(m_props, m_stddev));
+        throw Error(SMS_E_UNSUPPORTED, "scene file: integrator \"" + m_integrator + "\" is not a built path integrator");
+    }
+
+private:
+    sms_scene_file *m_file = nullptr;
+    std::string m_integrator;
+    Properties m_props;
+    int m_spp = 4;
+    float m_stddev = 0.5f;
+    std::vector<std::string> m_notes;
 };
 
 /* The SMS term of a camera pass into the box film (sms_film_accumulate): per pixel the mean of

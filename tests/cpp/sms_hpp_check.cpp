@@ -5,10 +5,18 @@
 //   sms_hpp_check render <out>      (GPU) the glass dome through SingleScatterSMSPathIntegrator,
 //                                   SingleScatterFilteredPathIntegrator and
 //                                   SpecularManifoldSingleScatter, raw floats written to <out>
+//   sms_hpp_check scene <xml> [k=v]...       the integrator a scene file names, its mapped
+//                                   configuration and the file's notes
+//   sms_hpp_check scene-render <xml> <out> <spp> [k=v]...   (GPU) that integrator's render of the
+//                                   file (seed 7, passes of 8 spp), raw floats written to <out>
 #include "sms.hpp"
 
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
+#include <memory>
+#include <string>
 #include <vector>
 
 namespace {
@@ -144,16 +152,52 @@ int render(const char *path) {
     return 0;
 }
 
+std::map<std::string, std::string> defines(int argc, char **argv, int first) {
+    std::map<std::string, std::string> d;
+    for (int i = first; i < argc; ++i) {
+        const char *eq = strchr(argv[i], '=');
+        if (eq) d[std::string(argv[i], eq - argv[i])] = eq + 1;
+    }
+    return d;
+}
+
+int scene(const char *xml, const std::map<std::string, std::string> &defs) {
+    sms::SceneFile file(xml, defs);
+    std::unique_ptr<sms::PathIntegrator> integrator = file.make_path_integrator();
+    printf("integrator %s\nspp %d\nshapes %d\nemitters %d\n", file.integrator().c_str(), file.spp(),
+           file.desc().n_shapes, file.desc().n_emitters);
+    print_config("file", integrator->config());
+    print_path("file", integrator->path());
+    printf("unqueried %zu\nnotes %zu\n", file.properties().unqueried().size(), file.notes().size());
+    return 0;
+}
+
+int scene_render(const char *xml, const char *path, int spp, const std::map<std::string, std::string> &defs) {
+    sms::SceneFile file(xml, defs);
+    std::unique_ptr<sms::PathIntegrator> integrator = file.make_path_integrator();
+    sms::Context ctx(file.desc(), integrator->config());
+    sms::Image img = integrator->render(ctx, spp, 7, 8);
+    FILE *f = fopen(path, "wb");
+    if (!f) return 2;
+    fwrite(img.rgb.data(), sizeof(float), img.rgb.size(), f);
+    fclose(f);
+    printf("wrote %zu floats, %llu walks\n", img.rgb.size(), (unsigned long long) img.stats.walks);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
     try {
         if (argc >= 2 && !strcmp(argv[1], "config")) return config();
         if (argc >= 3 && !strcmp(argv[1], "render")) return render(argv[2]);
+        if (argc >= 3 && !strcmp(argv[1], "scene")) return scene(argv[2], defines(argc, argv, 3));
+        if (argc >= 5 && !strcmp(argv[1], "scene-render"))
+            return scene_render(argv[2], argv[3], atoi(argv[4]), defines(argc, argv, 5));
     } catch (const sms::Error &e) {
         fprintf(stderr, "sms error: %s\n", e.what());
         return 1;
     }
-    fprintf(stderr, "usage: sms_hpp_check config | render <out>\n");
+    fprintf(stderr, "usage: sms_hpp_check config | render <out> | scene <xml> [k=v].. | scene-render <xml> <out> <spp> [k=v]..\n");
     return 2;
 }
