@@ -166,8 +166,8 @@ typedef struct sms_scene_desc {
                        referenced (id / <ref>) plugins, transforms composed as the reference does
                        (each op multiplied on the left); shapes sphere / rectangle / obj with
                        the caustic_* and glinty flags; bsdfs diffuse, dielectric, conductor,
-                       roughconductor (beckmann), normalmap (over a dielectric or conductor;
-                       the file read by sms_image_load), twosided; emitters area (in a shape),
+                       roughconductor (beckmann), normalmap (over a dielectric or conductor, with
+                       use_slopes; the file read by sms_image_load), twosided; emitters area (in a shape),
                        point, directional with caustic_emitter_single / _multi; a perspective
                        sensor (fov and fov_axis, to_world), hdrfilm size, gaussian or box
                        rfilter, independent sampler; the integrator's type and properties are
@@ -650,9 +650,11 @@ sms_status sms_graph_stats(sms_graph *g, sms_stats *stats, int32_t reset);
 sms_status sms_graph_destroy(sms_graph *g);
 
 /* ---------------------------------------------------------------- path integrators
-   The path tracers around the single-scatter SMS term, over the scene's diffuse, smooth
-   dielectric and smooth conductor BSDFs (scenes with normal-mapped or rough shapes are
-   refused with SMS_E_UNSUPPORTED) and its area, point and directional emitters:
+   The path tracers around the SMS terms, over the scene's diffuse, smooth dielectric and smooth
+   conductor BSDFs, normal-mapped ones (the nested BSDF sampled in the perturbed frame,
+   src/bsdfs/normalmap.cpp:117-134, without the LEAN fallback; scenes with rough shapes are
+   refused with SMS_E_UNSUPPORTED) and area (on spheres, rectangles and meshes), point and
+   directional emitters:
      SMS_PATH_SMS_SS       path_sms_ss (src/integrators/path_sms_ss.cpp:69-251): emitter hits,
                            emitter sampling on smooth BSDFs, BSDF sampling, and at every caustic
                            receiver that is not a caster the SMS estimate of

@@ -3091,6 +3091,43 @@ static BsdfSample bsdf_sample(const Shape &sh, V3 wi, float s1, V2 s2) {
     return bs;
 }
 
+/* NormalmapBSDF::frame (src/bsdfs/normalmap.cpp:218-239) at smoothing 0 with use_slopes: the
+   shading frame perturbed by the map, as ManifoldVertex builds it */
+static void normalmap_frame(const Scene &S, const SI &si, V3 *n, V3 *s, V3 *t) {
+    const Shape &sh = S.shapes[si.shape];
+    V3 bn = si.sh_n;
+    V3 bs = normalize(fnmadd3(bn, dot(bn, si.dp_du), si.dp_du));
+    V3 bt = cross(bn, bs);
+    const Normalmap &nm = S.nmaps[sh.normalmap];
+    V2 uv = tile_uv(si.uv, sh.tiles);
+    Texel4 tx = nm_fetch(nm, uv);
+    V2 sl = nm_slope(tx);
+    V3 nl = v3(-sl.x, -sl.y, 1.f);
+    *n = normalize(to_world(bs, bt, bn, nl));
+    *s = normalize(si.dp_du - *n * dot(*n, si.dp_du));
+    *t = cross(*n, *s);
+}
+
+/* NormalmapBSDF::sample (bsdfs/normalmap.cpp:117-134, without the LEAN fallback): the nested
+   smooth dielectric (k = 0) or conductor sampled in the perturbed frame, the direction taken
+   back to the shading frame, rejected when the two frames put it on different sides */
+static BsdfSample bsdf_sample_normalmap(const Scene &S, const SI &si, float s1, V2 s2) {
+    const Shape &sh = S.shapes[si.shape];
+    Shape nested = sh;
+    nested.bsdf = sh.k == 0.f ? SMS_BSDF_DIELECTRIC : SMS_BSDF_CONDUCTOR;
+    V3 pn, ps, pt;
+    normalmap_frame(S, si, &pn, &ps, &pt);
+    V3 wi_w = to_world(si.sh_s, si.sh_t, si.sh_n, si.wi);
+    V3 wi_p = v3(dot(wi_w, ps), dot(wi_w, pt), dot(wi_w, pn));
+    BsdfSample bs = bsdf_sample(nested, wi_p, s1, s2);
+    if (bs.weight.x == 0.f && bs.weight.y == 0.f && bs.weight.z == 0.f) return bs;
+    V3 wo_w = to_world(ps, pt, pn, bs.wo);
+    V3 wo = v3(dot(wo_w, si.sh_s), dot(wo_w, si.sh_t), dot(wo_w, si.sh_n));
+    if (!(bs.wo.z * wo.z > 0.f)) bs.weight = v3(0.f, 0.f, 0.f);
+    bs.wo = wo;
+    return bs;
+}
+
 /* SmoothDiffuse::eval / pdf (diffuse.cpp:103-133); the delta BSDFs evaluate to zero */
 static V3 bsdf_eval(const Shape &sh, V3 wi, V3 wo) {
     if (sh.bsdf != SMS_BSDF_DIFFUSE || !(wi.z > 0.f && wo.z > 0.f)) return v3(0.f, 0.f, 0.f);
@@ -3139,7 +3176,8 @@ struct ChainRecord {
    dielectric's (eta, 0); conductors (eta, k) and the default (0, 1) give 1 unless k = 0 */
 static float caustic_eta_of(const Shape &sh) {
     if (sh.bsdf == SMS_BSDF_DIELECTRIC) return sh.eta_r;
-    if (sh.bsdf == SMS_BSDF_CONDUCTOR && sh.k == 0.f) return sh.eta_r;
+    /* a normal-mapped BSDF gives its nested BSDF's (bsdfs/normalmap.cpp:342-344) */
+    if ((sh.bsdf == SMS_BSDF_CONDUCTOR || sh.bsdf == SMS_BSDF_NORMALMAP) && sh.k == 0.f) return sh.eta_r;
     return 1.f;
 }
 
@@ -3292,7 +3330,8 @@ static void path_sample(const Scene &S, const sms_path_config &pc, uint64_t seed
         /* BSDF sampling */
         float s1 = pcg32_float(&r);
         V2 s2; s2.x = pcg32_float(&r); s2.y = pcg32_float(&r);
-        BsdfSample bs = bsdf_sample(sh, si.wi, s1, s2);
+        BsdfSample bs = sh.bsdf == SMS_BSDF_NORMALMAP ? bsdf_sample_normalmap(S, si, s1, s2)
+                                                      : bsdf_sample(sh, si.wi, s1, s2);
         thr = mul3(thr, bs.weight);
         eta = eta * bs.eta;
         if (!bs.delta) specular_camera_path = false;

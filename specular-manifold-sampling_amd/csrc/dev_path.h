@@ -227,6 +227,43 @@ SMS_DEV BsdfSample bsdf_sample(const DShape &sh, V3 wi, float s1, V2 s2) {
     return bs;
 }
 
+// NormalmapBSDF::frame (src/bsdfs/normalmap.cpp:218-239) at smoothing 0 with use_slopes: the
+// shading frame perturbed by the map, as ManifoldVertex builds it
+SMS_DEV void normalmap_frame(const DScene &S, const SI &si, V3 &n, V3 &s, V3 &t) {
+    const DShape &sh = S.shapes[si.shape];
+    const V3 bn = si.sh_n;
+    const V3 bs = normalize(fnmadd3(bn, dot(bn, si.dp_du), si.dp_du));
+    const V3 bt = cross(bn, bs);
+    const DNormalmap nm = S.nmaps[sh.normalmap];
+    const V2 uv = tile_uv(si.uv, sh.tiles);
+    const Texel4 tx = nm_fetch(nm.slope, nm.res, nm.stride, uv);
+    const V2 sl = nm_slope(tx);
+    const V3 nl = v3(-sl.x, -sl.y, 1.f);
+    n = normalize(to_world(bs, bt, bn, nl));
+    s = normalize(si.dp_du - n * dot(n, si.dp_du));
+    t = cross(n, s);
+}
+
+// NormalmapBSDF::sample (bsdfs/normalmap.cpp:117-134, without the LEAN fallback): the nested
+// smooth dielectric (k = 0) or conductor sampled in the perturbed frame, the direction taken
+// back to the shading frame, rejected when the two frames put it on different sides
+SMS_DEV BsdfSample bsdf_sample_normalmap(const DScene &S, const SI &si, float s1, V2 s2) {
+    const DShape &sh = S.shapes[si.shape];
+    DShape nested = sh;
+    nested.bsdf = sh.k == 0.f ? SMS_BSDF_DIELECTRIC : SMS_BSDF_CONDUCTOR;
+    V3 pn, ps, pt;
+    normalmap_frame(S, si, pn, ps, pt);
+    const V3 wi_w = to_world(si.sh_s, si.sh_t, si.sh_n, si.wi);
+    const V3 wi_p = v3(dot(wi_w, ps), dot(wi_w, pt), dot(wi_w, pn));
+    BsdfSample bs = bsdf_sample(nested, wi_p, s1, s2);
+    if (bs.weight.x == 0.f && bs.weight.y == 0.f && bs.weight.z == 0.f) return bs;
+    const V3 wo_w = to_world(ps, pt, pn, bs.wo);
+    const V3 wo = v3(dot(wo_w, si.sh_s), dot(wo_w, si.sh_t), dot(wo_w, si.sh_n));
+    if (!(bs.wo.z * wo.z > 0.f)) bs.weight = v3(0.f, 0.f, 0.f);
+    bs.wo = wo;
+    return bs;
+}
+
 // SmoothDiffuse::eval / pdf (diffuse.cpp:103-133); the delta BSDFs evaluate to zero
 SMS_DEV V3 bsdf_eval(const DShape &sh, V3 wi, V3 wo) {
     if (sh.bsdf != SMS_BSDF_DIFFUSE || !(wi.z > 0.f && wo.z > 0.f)) return v3(0.f, 0.f, 0.f);
@@ -287,10 +324,11 @@ struct ChainRecord {
 };
 SMS_DEV void chain_reset(ChainRecord &c) { c.state = 0; c.bounce = -1; c.shape0 = -1; c.npos = 0; }
 
-// caustic_eta of BSDF::ior (path_filtered_ss.cpp:85-87, path_filtered_ms.cpp:268-270)
+// caustic_eta of BSDF::ior (path_filtered_ss.cpp:85-87, path_filtered_ms.cpp:268-270); a
+// normal-mapped BSDF gives its nested BSDF's (bsdfs/normalmap.cpp:342-344)
 SMS_DEV float caustic_eta_of(const DShape &sh) {
     if (sh.bsdf == SMS_BSDF_DIELECTRIC) return sh.eta_r;
-    if (sh.bsdf == SMS_BSDF_CONDUCTOR && sh.k == 0.f) return sh.eta_r;
+    if ((sh.bsdf == SMS_BSDF_CONDUCTOR || sh.bsdf == SMS_BSDF_NORMALMAP) && sh.k == 0.f) return sh.eta_r;
     return 1.f;
 }
 
@@ -445,7 +483,8 @@ SMS_DEV void path_sample(const DScene &S, const sms_path_config &pc, uint64_t se
         const float s1 = pcg32_float(r);
         V2 s2;
         s2.x = pcg32_float(r); s2.y = pcg32_float(r);
-        const BsdfSample bs = bsdf_sample(sh, si.wi, s1, s2);
+        const BsdfSample bs = sh.bsdf == SMS_BSDF_NORMALMAP ? bsdf_sample_normalmap(S, si, s1, s2)
+                                                            : bsdf_sample(sh, si.wi, s1, s2);
         thr = mul3(thr, bs.weight);
         eta = eta * bs.eta;
         if (!bs.delta) specular_camera_path = false;

@@ -644,7 +644,13 @@ private:
             d.bsdf = SMS_BSDF_NORMALMAP;
             d.normalmap = normalmap(o, resolve(s(o, "filename", "")), out);
             d.tiles = (float) f(o, "tiles", 1.0);
-            (void) b(o, "use_slopes", false);
+            // the solver's frames come from the map's slopes (normalmap.h:62-104 with use_slopes);
+            // the reference's default interpolates the encoded normals instead
+            if (!b(o, "use_slopes", false)) {
+                if (!substitute())
+                    throw LoadError(SMS_E_UNSUPPORTED, where_obj(o) + "normal maps without use_slopes are not supported");
+                note(where_obj(o) + "normal map read with use_slopes (the file leaves it false)");
+            }
             (void) b(o, "lean_fallback", false);
             (void) f(o, "lean_fallback_alpha", 0.0);
         } else {

@@ -103,8 +103,6 @@ sms_status check_path_scene(sms_ctx *ctx, const sms_path_config *pc) {
         return fail(ctx, SMS_E_INVALID, "path integrator: max_depth must be -1 or in [0, SMS_PATH_MAX_DEPTH]");
     if (!ctx->scene.has_cam) return fail(ctx, SMS_E_INVALID, "path integrator: the scene has no camera");
     for (const DShape &sh : ctx->h_shapes) {
-        if (sh.bsdf == SMS_BSDF_NORMALMAP)
-            return fail(ctx, SMS_E_UNSUPPORTED, "path integrator: normal-mapped BSDFs are not evaluated by the path tracer");
         if (sh.alpha > 0.f)
             return fail(ctx, SMS_E_UNSUPPORTED, "path integrator: rough BSDFs are not evaluated by the path tracer");
     }

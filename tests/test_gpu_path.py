@@ -72,6 +72,30 @@ def test_path_samples_pool_mnee_and_batches(gpu, oracle_mod):
     orc.close()
 
 
+def test_path_samples_normal_mapped(gpu, oracle_mod):
+    """Normal-mapped casters in both single-scatter path tracers: a glass dome with a wavy map
+    (nested dielectric) and a mirror with a scratch map (nested conductor)."""
+    u = np.linspace(0, 2 * np.pi, 64, endpoint=False)
+    sx, sy = 0.12 * np.sin(2 * u)[None, :] * np.ones((64, 1)), 0.12 * np.cos(3 * u)[:, None] * np.ones((1, 64))
+    n = np.stack([-sx, -sy, np.ones_like(sx)], -1)
+    n /= np.linalg.norm(n, axis=2, keepdims=True)
+    dome = scenes.sphere_dome(24, 20, light_radius=0.5, light_radiance=4.0)
+    nm = dome.add_normalmap((0.5 * n + 0.5).astype(np.float32))
+    dome.shapes[0].update(bsdf=abi.BSDF_NORMALMAP, normalmap=nm, tiles=2.0, k=0.0, eta=1.5)
+    wall = scenes.mirror_wall(nm_res=64, n_scratches=16)
+    wall.shapes[2]["radius"] = 0.3
+    wall.emitters[0]["radiance"] = (5.0, 5.0, 5.0)
+    for sc in (dome, wall):
+        cfg = abi.SmsConfig.default(max_trials=100000)
+        ctx = gpu.Context(sc, cfg)
+        orc = oracle_mod.Oracle(sc, cfg)
+        for integrator in (abi.PATH_SMS_SS, abi.PATH_FILTERED_SS):
+            for only in (1, 0):
+                _compare(ctx, orc, abi.SmsPathConfig.default(integrator=integrator, max_depth=6, caustics_only=only), 8, 3)
+        ctx.close()
+        orc.close()
+
+
 def test_path_samples_slab_multi_scatter(gpu, oracle_mod):
     """path_sms_ms (unbiased SMS; MNEE with and without its completions) and path_filtered_ms on
     the slab, caustic term and whole image."""
@@ -115,8 +139,10 @@ def test_film_splat(gpu, oracle_mod, stddev):
 
 
 def test_refused(gpu):
-    ctx = gpu.Context(scenes.glint_plane(16, 16, nm_res=64, n_scratches=4), abi.SmsConfig.default())
-    with pytest.raises(gpu.SmsError, match="normal-mapped"):
+    rough = scenes.sphere_dome(16, 16)
+    rough.shapes[0].update(bsdf=abi.BSDF_CONDUCTOR, eta=0.0, k=1.0, roughness=0.1)
+    ctx = gpu.Context(rough, abi.SmsConfig.default())
+    with pytest.raises(gpu.SmsError, match="rough"):
         ctx.path_samples(abi.SmsPathConfig.default(), 1, 1)
     ctx.close()
     sc = scenes.sphere_dome(16, 16)

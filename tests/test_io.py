@@ -455,7 +455,7 @@ def test_scene_file_meshes_and_normal_maps(tmp_path):
                 <bsdf type="dielectric"><float name="int_ior" value="1.33"/></bsdf></bsdf></shape>
         <shape type="obj"><string name="filename" value="meshes/m.obj"/><boolean name="face_normals" value="true"/>
             <bsdf type="normalmap"><string name="filename" value="nm.exr"/>
-                <bsdf type="conductor"/></bsdf>
+                <boolean name="use_slopes" value="true"/><bsdf type="conductor"/></bsdf>
             <emitter type="area"><rgb name="radiance" value="1, 2, 3"/></emitter></shape></scene>"""
     (tmp_path / "s.xml").write_text(xml)
     sc, info = files.load_scene(tmp_path / "s.xml")
