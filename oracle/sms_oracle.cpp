@@ -40,6 +40,7 @@ enum {
     PLANT_MNEE_TARGET,      /* scale of the light point the MNEE seed ray aims at              */
     PLANT_MS_G,             /* the multi-scatter generalized geometric term                     */
     PLANT_SS_G_DIR,         /* the single-scatter geometric term toward a directional emitter  */
+    PLANT_SS_G_CLAMP,       /* the bound (1) of |det dx1/dx2| in the single-scatter term        */
     PLANT_N = 16
 };
 static float g_plant[PLANT_N] = {1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f};
@@ -1366,7 +1367,7 @@ static float geometric_term_ss(const MV &v0, const MV &v1, const MV &v2v) {
     if (fabsf(det2(dc1_dx1)) < 1e-6f) return 0.f;
     M2 inv = inverse2(dc1_dx1);
     float dx1_dx2 = fabsf(det2(mulm2(inv, dc1_dx2)));
-    dx1_dx2 = std::min(dx1_dx2, 1.f);
+    dx1_dx2 = std::min(dx1_dx2, g_plant[PLANT_SS_G_CLAMP]);
     V3 d = v0.p - v1.p;
     float inv_r2 = 1.f / sqnorm(d);
     d = d * sqrtf(inv_r2);
