@@ -558,6 +558,16 @@ static SI make_si(const Scene &S, const Ray &ray, const Hit &h) {
         V3 local = (si.p - sh.center) * sh.inv_radius;
         float rd_2 = local.x * local.x + local.y * local.y;
         si.uv = v2(0.f, 0.f);
+        if (sh.normalmap >= 0) {
+            /* sphere.cpp:343-348 (read only through a normal map); theta by unit_angle_z of
+               Mitsuba 2's core vector.h: 2 asin(|v - sign(v.z) z| / 2), mirrored for v.z < 0 */
+            float dz = local.z - (signbit(local.z) ? -1.f : 1.f);
+            float temp = 2.f * c_asinf(.5f * sqrtf((local.x * local.x + local.y * local.y) + dz * dz));
+            float theta = local.z >= 0.f ? temp : PI_F - temp;
+            float phi = c_atan2f(local.y, local.x);
+            if (phi < 0.f) phi = phi + 2.f * PI_F;
+            si.uv = v2(phi * INV_TWOPI_F, theta * INV_PI_F);
+        }
         si.dp_du = v3(-local.y, local.x, 0.f);
         float rd = sqrtf(rd_2), inv_rd = 1.f / rd;
         float cos_phi = local.x * inv_rd, sin_phi = local.y * inv_rd;

@@ -20,6 +20,7 @@ constexpr float PI_F = 3.14159265358979323846f;
 constexpr float PIO2_F = 1.57079632679489661923f;
 constexpr float PIO4_F = 0.78539816339744830962f;
 constexpr float INV_PI_F = 0.31830988618379067154f;
+constexpr float INV_TWO_PI_F = 0.15915494309189533577f;
 constexpr float EPS_F = 5.9604644775390625e-08f;            // math::Epsilon<float>
 constexpr float RAY_EPS_F = 1500.f * 5.9604644775390625e-08f;
 constexpr float SHADOW_EPS_F = 10.f * (1500.f * 5.9604644775390625e-08f);

@@ -314,6 +314,22 @@ SMS_DEV Hit scene_hit_t(const DScene &S, const Ray &ray, bool any_hit) {
     return h;
 }
 
+// Angle between the unit vector v and +z from the half chord to the pole, which stays accurate
+// near both poles (unit_angle_z of Mitsuba 2's core vector.h).
+SMS_DEV float unit_angle_z(V3 v) {
+    const float dz = v.z - mulsign(1.f, v.z);
+    const float temp = 2.f * c_asinf(.5f * sqrtf((v.x * v.x + v.y * v.y) + dz * dz));
+    return v.z >= 0.f ? temp : PI_F - temp;
+}
+
+// sphere.cpp:343-348: (phi / 2 pi, theta / pi) of the object-space point
+SMS_DEV V2 sphere_uv(V3 local) {
+    const float theta = unit_angle_z(local);
+    float phi = c_atan2f(local.y, local.x);
+    if (phi < 0.f) phi = phi + 2.f * PI_F;
+    return v2(phi * INV_TWO_PI_F, theta * INV_PI_F);
+}
+
 // Shape::fill_surface_interaction + kd-tree shading frame (kdtree.h:2335-2364)
 SMS_DEV SI make_si(const DScene &S, const Ray &ray, const Hit &h) {
     if (!h.hit) return si_invalid();
@@ -321,13 +337,15 @@ SMS_DEV SI make_si(const DScene &S, const Ray &ray, const Hit &h) {
     si.valid = true; si.t = h.t; si.shape = h.shape; si.prim = h.prim;
     const DShape &sh = S.shapes[h.shape];
     if (sh.type == SMS_SHAPE_SPHERE) {
-        // sphere.cpp:329-374 (uv is not used by the walk and is left 0)
+        // sphere.cpp:329-374; uv is read only through a normal map, so only normal-mapped
+        // spheres compute it
         V3 c = v3(sh.cx, sh.cy, sh.cz);
         V3 n = normalize(fmadd3(ray.d, si.t, ray.o) - c);
         si.sh_n = n;
         si.p = fmadd3(n, sh.radius, c);
         V3 local = (si.p - c) * sh.inv_radius;
         float rd_2 = local.x * local.x + local.y * local.y;
+        if (sh.normalmap >= 0) si.uv = sphere_uv(local);
         si.dp_du = v3(-local.y, local.x, 0.f);
         float rd = sqrtf(rd_2), inv_rd = 1.f / rd;
         float cos_phi = local.x * inv_rd, sin_phi = local.y * inv_rd;

@@ -714,6 +714,7 @@ private:
         d.bsdf = SMS_BSDF_DIFFUSE;
         for (int c = 0; c < 3; ++c) d.reflectance[c] = 0.5f;
         const std::string &t = o.type;
+        bool sphere_rotated = false;
         const Mat4 m = xf(o, "to_world");
         if (b(o, "flip_normals", false)) throw LoadError(SMS_E_UNSUPPORTED, where_obj(o) + "flip_normals is not supported");
         if (t == "sphere") {
@@ -733,6 +734,11 @@ private:
                 throw LoadError(SMS_E_UNSUPPORTED, where_obj(o) + "non-uniformly scaled spheres are not supported");
             for (int k = 0; k < 3; ++k) d.center[k] = (float) cw[k];
             d.radius = (float) rx;
+            // the sphere's uv and tangents follow its object axes, which the solver takes to be
+            // the world axes
+            for (int k = 0; k < 3; ++k)
+                if (std::fabs(ex[k] / rx - (k == 0)) > 1e-6 || std::fabs(ey[k] / rx - (k == 1)) > 1e-6 ||
+                    std::fabs(ez[k] / rx - (k == 2)) > 1e-6) sphere_rotated = true;
         } else if (t == "rectangle") {
             d.type = SMS_SHAPE_RECTANGLE;
             for (int r = 0; r < 3; ++r) for (int c = 0; c < 4; ++c) d.to_world[4 * r + c] = (float) m.a[4 * r + c];
@@ -770,6 +776,11 @@ private:
                 bsdf(*c, d, out);
                 have_bsdf = true;
             }
+        }
+        if (sphere_rotated) {
+            if (d.bsdf == SMS_BSDF_NORMALMAP)
+                throw LoadError(SMS_E_UNSUPPORTED, where_obj(o) + "rotated normal-mapped spheres are not supported");
+            note(where_obj(o) + "sphere rotation dropped: its uv and tangents follow the world axes");
         }
         out.shapes.push_back(d);
         for (auto &c : o.children) {

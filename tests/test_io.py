@@ -497,6 +497,29 @@ def test_scene_file_unsupported_and_substitutes(tmp_path):
         assert e.value.status == -1
 
 
+def test_scene_file_rotated_spheres(tmp_path):
+    """A sphere's uv and tangents follow its object axes (sphere.cpp:343-365); the solver's
+    spheres are axis-aligned, so a rotation is dropped with a note, and refused for a
+    normal-mapped sphere, whose map it would turn."""
+    write_exr(tmp_path / "nm.exr", np.tile(np.float32([0.5, 0.5, 1.0]), (4, 4, 1)))
+    rot = '<transform name="to_world"><rotate x="1" angle="90"/><scale value="2"/></transform>'
+    plain = f'<scene version="2.0.0"><sensor type="perspective"/><shape type="sphere">{rot}</shape></scene>'
+    mapped = (f'<scene version="2.0.0"><sensor type="perspective"/><shape type="sphere">{rot}'
+              '<bsdf type="normalmap"><string name="filename" value="nm.exr"/><boolean name="use_slopes" value="true"/>'
+              '<bsdf type="dielectric"/></bsdf></shape></scene>')
+    p = tmp_path / "r.xml"
+    p.write_text(plain)
+    sc, info = files.load_scene(p)
+    assert sc.shapes[0]["radius"] == 2.0 and any("rotation" in n for n in info["notes"])
+    p.write_text(mapped)
+    with pytest.raises(files.LoadError) as e:
+        files.load_scene(p)
+    assert e.value.status == -5
+    p.write_text(mapped.replace('<rotate x="1" angle="90"/>', ""))
+    sc, info = files.load_scene(p)
+    assert sc.shapes[0]["bsdf"] == abi.BSDF_NORMALMAP and not any("rotation" in n for n in info["notes"])
+
+
 # ------------------------------------------------------------------ tests: the reference's files
 @have_ref
 def test_reference_normal_maps_decode_like_numpy():
