@@ -129,6 +129,10 @@ typedef struct sms_normalmap_desc {
     const uint8_t *rgb;        /* res*res*3 bytes, row-major, byte/255 (no linearisation)  */
     const float *rgb32f;       /* or, when rgb is NULL: res*res*3 floats, row-major, as an
                                   OpenEXR file stores them (src/librender/normalmap.cpp:20-22) */
+    int32_t encoded_normals;   /* 0: frames interpolate the texels' slopes (NormalmapBSDF's
+                                  use_slopes); 1: they interpolate the stored RGB and decode
+                                  n = 2 rgb - 1 (use_slopes false, the reference's default;
+                                  bsdfs/normalmap.cpp:226-258).  Glinty shapes need 0. */
 } sms_normalmap_desc;
 
 typedef struct sms_camera_desc {

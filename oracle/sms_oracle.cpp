@@ -70,6 +70,11 @@ struct Normalmap {
     /* LEAN mip pyramid: level l has (res >> l)^2 texels of (E[x], E[y], E[x^2], E[y^2], E[xy])
        (normalmap.cpp:36-87) */
     std::vector<std::vector<float>> lean;
+    /* maps without use_slopes (sms_normalmap_desc.encoded_normals): the stored RGB (res*res*3)
+       and the RGB of the coarsest level of its box-filter pyramid */
+    bool encoded = false;
+    std::vector<float> rgb;
+    float top_rgb[3] = {0.f, 0.f, 0.f};
 };
 
 struct Camera {
@@ -294,6 +299,29 @@ static Scene *scene_create(const sms_scene_desc *desc, const sms_config *cfg) {
             nm.slope[2 * s + 1] = -n.y / n.z;
         }
         build_lean(nm);
+        if (nd.encoded_normals) {
+            nm.encoded = true;
+            nm.rgb.resize(3 * N);
+            for (size_t s = 0; s < 3 * N; ++s) nm.rgb[s] = nd.rgb ? (float) nd.rgb[s] / 255.f : (float) nd.rgb32f[s];
+            /* the normals mipmap (normalmap.cpp:70-87): the same two-tap resample as the LEAN map */
+            std::vector<float> cur = nm.rgb;
+            for (int r = nd.res; r > 1; r /= 2) {
+                const int h = r / 2;
+                std::vector<float> tmp((size_t) h * r * 3), dst((size_t) h * h * 3);
+                for (int y = 0; y < r; ++y)
+                    for (int x = 0; x < h; ++x)
+                        for (int c = 0; c < 3; ++c)
+                            tmp[((size_t) y * h + x) * 3 + c] = cur[((size_t) y * r + 2 * x) * 3 + c] * 0.5f +
+                                                                cur[((size_t) y * r + 2 * x + 1) * 3 + c] * 0.5f;
+                for (int y = 0; y < h; ++y)
+                    for (int x = 0; x < h; ++x)
+                        for (int c = 0; c < 3; ++c)
+                            dst[((size_t) y * h + x) * 3 + c] = tmp[((size_t) (2 * y) * h + x) * 3 + c] * 0.5f +
+                                                                tmp[((size_t) (2 * y + 1) * h + x) * 3 + c] * 0.5f;
+                cur = std::move(dst);
+            }
+            nm.top_rgb[0] = cur[0]; nm.top_rgb[1] = cur[1]; nm.top_rgb[2] = cur[2];
+        }
         S->nmaps.push_back(nm);
     }
     for (int i = 0; i < desc->n_emitters; ++i) {
@@ -676,8 +704,7 @@ static void normal_derivative(const Scene &S, const SI &si, V3 *dn_du, V3 *dn_dv
 /* ---------------------------------------------------------------- normal map (level 0) */
 struct Texel4 { V2 v00, v10, v01, v11; V2 w; };
 /* Normalmap::eval_normal / eval_normal_derivatives index computation (normalmap.h:62-80,148-163) */
-static Texel4 nm_fetch(const Normalmap &nm, V2 uv) {
-    int res = nm.res;
+static void nm_index(int res, V2 uv, uint32_t idx[4], V2 *w) {
     float fres = (float) res;
     float duv = 1.f / fres;
     V2 pos = v2((uv.x - 0.5f * duv) * fres, (uv.y - 0.5f * duv) * fres);
@@ -687,12 +714,16 @@ static Texel4 nm_fetch(const Normalmap &nm, V2 uv) {
     uint32_t xmax = xmin + 1, ymax = ymin + 1;
     uint32_t lim = (uint32_t) (res - 1);
     xmin = std::min(xmin, lim); ymin = std::min(ymin, lim); xmax = std::min(xmax, lim); ymax = std::min(ymax, lim);
+    idx[0] = ymin * res + xmin; idx[1] = ymin * res + xmax; idx[2] = ymax * res + xmin; idx[3] = ymax * res + xmax;
+    *w = v2(pos.x - (float) xmin, pos.y - (float) ymin);
+}
+static Texel4 nm_fetch(const Normalmap &nm, V2 uv) {
+    uint32_t i[4];
     Texel4 t;
+    nm_index(nm.res, uv, i, &t.w);
     const float *s = nm.slope.data();
-    uint32_t i00 = ymin * res + xmin, i10 = ymin * res + xmax, i01 = ymax * res + xmin, i11 = ymax * res + xmax;
-    t.v00 = v2(s[2 * i00], s[2 * i00 + 1]); t.v10 = v2(s[2 * i10], s[2 * i10 + 1]);
-    t.v01 = v2(s[2 * i01], s[2 * i01 + 1]); t.v11 = v2(s[2 * i11], s[2 * i11 + 1]);
-    t.w = v2(pos.x - (float) xmin, pos.y - (float) ymin);
+    t.v00 = v2(s[2 * i[0]], s[2 * i[0] + 1]); t.v10 = v2(s[2 * i[1]], s[2 * i[1] + 1]);
+    t.v01 = v2(s[2 * i[2]], s[2 * i[2] + 1]); t.v11 = v2(s[2 * i[3]], s[2 * i[3] + 1]);
     return t;
 }
 /* interpolated slope (v[0], v[1]) of eval_normal with use_slopes (normalmap.h:82-92) */
@@ -707,6 +738,34 @@ static void nm_slope_deriv(const Texel4 &t, float res, V2 *tu, V2 *tv) {
     V2 tmp = (t.v01 + t.v10) - t.v11;
     *tu = ((t.v10 + t.v00 * (t.w.y - 1.f)) - tmp * t.w.y) * res;
     *tv = ((t.v01 + t.v00 * (t.w.x - 1.f)) - tmp * t.w.x) * res;
+}
+/* the map's local normal (not normalised) at level 0 and its uv derivatives: with use_slopes
+   n = (-s_x, -s_y, 1) (normalmap.h:82-92,166-179); without, n = 2 rgb - 1 of the bilinearly
+   interpolated RGB and dn = 2 drgb (normalmap.h:93-103,180-190 with NormalmapBSDF::frame and
+   frame_derivative, bsdfs/normalmap.cpp:226-258) */
+static void nm_local_normal(const Normalmap &nm, V2 uv, V3 *nl, V3 *du, V3 *dv) {
+    if (!nm.encoded) {
+        Texel4 tx = nm_fetch(nm, uv);
+        V2 sl = nm_slope(tx), tu, tv;
+        nm_slope_deriv(tx, (float) nm.res, &tu, &tv);
+        *nl = v3(-sl.x, -sl.y, 1.f);
+        *du = v3(-tu.x, -tu.y, 0.f); *dv = v3(-tv.x, -tv.y, 0.f);
+        return;
+    }
+    uint32_t i[4];
+    V2 w;
+    nm_index(nm.res, uv, i, &w);
+    const float *c = nm.rgb.data();
+    V3 v00 = v3(c[3 * i[0]], c[3 * i[0] + 1], c[3 * i[0] + 2]), v10 = v3(c[3 * i[1]], c[3 * i[1] + 1], c[3 * i[1] + 2]);
+    V3 v01 = v3(c[3 * i[2]], c[3 * i[2] + 1], c[3 * i[2] + 2]), v11 = v3(c[3 * i[3]], c[3 * i[3] + 1], c[3 * i[3] + 2]);
+    float w0x = 1.f - w.x, w0y = 1.f - w.y;
+    V3 a = v3(fmaf(w0x, v00.x, w.x * v10.x), fmaf(w0x, v00.y, w.x * v10.y), fmaf(w0x, v00.z, w.x * v10.z));
+    V3 b = v3(fmaf(w0x, v01.x, w.x * v11.x), fmaf(w0x, v01.y, w.x * v11.y), fmaf(w0x, v01.z, w.x * v11.z));
+    V3 rgb = v3(fmaf(w0y, a.x, w.y * b.x), fmaf(w0y, a.y, w.y * b.y), fmaf(w0y, a.z, w.y * b.z));
+    *nl = v3(2.f * rgb.x - 1.f, 2.f * rgb.y - 1.f, 2.f * rgb.z - 1.f);
+    V3 tmp = (v01 + v10) - v11;
+    *du = (((v10 + v00 * (w.y - 1.f)) - tmp * w.y) * (float) nm.res) * 2.f;
+    *dv = (((v01 + v00 * (w.x - 1.f)) - tmp * w.x) * (float) nm.res) * 2.f;
 }
 static inline V2 tile_uv(V2 uv, float tiles) {
     uv = uv * tiles;
@@ -758,19 +817,23 @@ static MV mv_from_si(const Scene &S, const SI &si, bool smooth = false) {
     } else {
         /* NormalmapBSDF::frame / frame_derivative at smoothing 0 (src/bsdfs/normalmap.cpp:218-290) */
         const Normalmap &nm = S.nmaps[sh.normalmap];
-        V2 sl, tu, tv;
+        V3 nl, dl_du, dl_dv;
         if (smooth) {
-            const std::vector<float> &top = nm.lean.back();
-            sl = v2(top[0], top[1]);
-            tu = v2(0.f, 0.f); tv = v2(0.f, 0.f);
+            /* one texel: bilinear weights (0, 0), the derivatives of eval_normal_derivatives zero,
+               negated in its slope branch (normalmap.h:175-179) */
+            if (nm.encoded) {
+                nl = v3(2.f * nm.top_rgb[0] - 1.f, 2.f * nm.top_rgb[1] - 1.f, 2.f * nm.top_rgb[2] - 1.f);
+                dl_du = v3(0.f, 0.f, 0.f);
+            } else {
+                const std::vector<float> &top = nm.lean.back();
+                nl = v3(-top[0], -top[1], 1.f);
+                dl_du = v3(-0.f, -0.f, 0.f);
+            }
+            dl_dv = dl_du;
         } else {
-            V2 uv = tile_uv(si.uv, sh.tiles);
-            Texel4 tx = nm_fetch(nm, uv);
-            sl = nm_slope(tx);
-            nm_slope_deriv(tx, (float) nm.res, &tu, &tv);
+            nm_local_normal(nm, tile_uv(si.uv, sh.tiles), &nl, &dl_du, &dl_dv);
         }
-        V3 nl = v3(-sl.x, -sl.y, 1.f);
-        V3 dn_du = v3(-tu.x, -tu.y, 0.f) * sh.tiles, dn_dv = v3(-tv.x, -tv.y, 0.f) * sh.tiles;
+        V3 dn_du = dl_du * sh.tiles, dn_dv = dl_dv * sh.tiles;
         V3 rn = normalize(to_world(bs, bt, bn, nl));
         v.n = rn;
         v.s = normalize(si.dp_du - rn * dot(rn, si.dp_du));
@@ -1197,6 +1260,8 @@ static void lean_coarsest(const Scene &S, const SI &si, V2 *mu, float sig[3]) {
     V2 uv = si.uv * sh.tiles;
     uv = v2(uv.x - floorf(uv.x), uv.y - floorf(uv.y));
     lean_moments(nm, uv, (int) nm.lean.size() - 1, mu, sig);
+    /* without use_slopes the mean comes from eval_normal's RGB as stored, not decoded */
+    if (nm.encoded) *mu = v2(-nm.top_rgb[0] / nm.top_rgb[2], -nm.top_rgb[1] / nm.top_rgb[2]);
 }
 
 /* sample_path SS (manifold_ss.cpp:203-279): uniform seed, optional MNEE direction, and with
@@ -3102,18 +3167,16 @@ static BsdfSample bsdf_sample(const Shape &sh, V3 wi, float s1, V2 s2) {
     return bs;
 }
 
-/* NormalmapBSDF::frame (src/bsdfs/normalmap.cpp:218-239) at smoothing 0 with use_slopes: the
-   shading frame perturbed by the map, as ManifoldVertex builds it */
+/* NormalmapBSDF::frame (src/bsdfs/normalmap.cpp:218-239) at smoothing 0: the shading frame
+   perturbed by the map, as ManifoldVertex builds it */
 static void normalmap_frame(const Scene &S, const SI &si, V3 *n, V3 *s, V3 *t) {
     const Shape &sh = S.shapes[si.shape];
     V3 bn = si.sh_n;
     V3 bs = normalize(fnmadd3(bn, dot(bn, si.dp_du), si.dp_du));
     V3 bt = cross(bn, bs);
     const Normalmap &nm = S.nmaps[sh.normalmap];
-    V2 uv = tile_uv(si.uv, sh.tiles);
-    Texel4 tx = nm_fetch(nm, uv);
-    V2 sl = nm_slope(tx);
-    V3 nl = v3(-sl.x, -sl.y, 1.f);
+    V3 nl, du, dv;
+    nm_local_normal(nm, tile_uv(si.uv, sh.tiles), &nl, &du, &dv);
     *n = normalize(to_world(bs, bt, bn, nl));
     *s = normalize(si.dp_du - *n * dot(*n, si.dp_du));
     *t = cross(*n, *s);

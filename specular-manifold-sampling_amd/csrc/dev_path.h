@@ -227,18 +227,16 @@ SMS_DEV BsdfSample bsdf_sample(const DShape &sh, V3 wi, float s1, V2 s2) {
     return bs;
 }
 
-// NormalmapBSDF::frame (src/bsdfs/normalmap.cpp:218-239) at smoothing 0 with use_slopes: the
-// shading frame perturbed by the map, as ManifoldVertex builds it
+// NormalmapBSDF::frame (src/bsdfs/normalmap.cpp:218-239) at smoothing 0: the shading frame
+// perturbed by the map, as ManifoldVertex builds it
 SMS_DEV void normalmap_frame(const DScene &S, const SI &si, V3 &n, V3 &s, V3 &t) {
     const DShape &sh = S.shapes[si.shape];
     const V3 bn = si.sh_n;
     const V3 bs = normalize(fnmadd3(bn, dot(bn, si.dp_du), si.dp_du));
     const V3 bt = cross(bn, bs);
     const DNormalmap nm = S.nmaps[sh.normalmap];
-    const V2 uv = tile_uv(si.uv, sh.tiles);
-    const Texel4 tx = nm_fetch(nm.slope, nm.res, nm.stride, uv);
-    const V2 sl = nm_slope(tx);
-    const V3 nl = v3(-sl.x, -sl.y, 1.f);
+    V3 nl, du, dv;
+    nm_local_normal(nm, tile_uv(si.uv, sh.tiles), nl, du, dv);
     n = normalize(to_world(bs, bt, bn, nl));
     s = normalize(si.dp_du - n * dot(n, si.dp_du));
     t = cross(n, s);

@@ -42,6 +42,9 @@ constexpr int SMS_MAX_LEAN_LEVELS = 16;
 struct DNormalmap {
     int res; int stride; const float2 *slope;
     const float *lean; int levels; long long lean_off[SMS_MAX_LEAN_LEVELS];
+    // encoded-normal maps: the stored RGB in the padded layout of `slope` (w unused) and the RGB
+    // of the coarsest level of the box-filtered RGB pyramid
+    int encoded; const float4 *rgbn; float top_rgb[3];
 };
 
 struct DCamera {
@@ -483,6 +486,35 @@ SMS_DEV void nm_slope_deriv(const Texel4 &t, float res, V2 &tu, V2 &tv) {
     tu = ((t.v10 + t.v00 * (t.w.y - 1.f)) - tmp * t.w.y) * res;
     tv = ((t.v01 + t.v00 * (t.w.x - 1.f)) - tmp * t.w.x) * res;
 }
+// The map's local normal (not normalised) at level 0 and its uv derivatives (normalmap.h:62-191
+// with NormalmapBSDF::frame / frame_derivative, bsdfs/normalmap.cpp:226-258): a slope map gives
+// n = (-s_x, -s_y, 1), an encoded map n = 2 rgb - 1 of the bilinearly interpolated RGB
+SMS_DEV void nm_local_normal(const DNormalmap &nm, V2 uv, V3 &nl, V3 &du, V3 &dv) {
+    if (!nm.encoded) {
+        const Texel4 tx = nm_fetch(nm.slope, nm.res, nm.stride, uv);
+        const V2 sl = nm_slope(tx);
+        V2 tu, tv;
+        nm_slope_deriv(tx, (float) nm.res, tu, tv);
+        nl = v3(-sl.x, -sl.y, 1.f);
+        du = v3(-tu.x, -tu.y, 0.f); dv = v3(-tv.x, -tv.y, 0.f);
+        return;
+    }
+    const NmCoord c = nm_coord(nm.res, uv);
+    const float4 *row = nm.rgbn + (size_t) c.y * nm.stride + c.x;
+    const float4 a0 = row[0], a1 = row[1], b0 = row[nm.stride], b1 = row[nm.stride + 1];
+    const V3 v00 = v3(a0.x, a0.y, a0.z), v10 = v3(a1.x, a1.y, a1.z), v01 = v3(b0.x, b0.y, b0.z),
+             v11 = v3(b1.x, b1.y, b1.z);
+    const V2 w1 = c.w, w0 = v2(1.f - w1.x, 1.f - w1.y);
+    const V3 r0 = v3(fmaf(w0.x, v00.x, w1.x * v10.x), fmaf(w0.x, v00.y, w1.x * v10.y), fmaf(w0.x, v00.z, w1.x * v10.z));
+    const V3 r1 = v3(fmaf(w0.x, v01.x, w1.x * v11.x), fmaf(w0.x, v01.y, w1.x * v11.y), fmaf(w0.x, v01.z, w1.x * v11.z));
+    const V3 rgb = v3(fmaf(w0.y, r0.x, w1.y * r1.x), fmaf(w0.y, r0.y, w1.y * r1.y), fmaf(w0.y, r0.z, w1.y * r1.z));
+    nl = v3(2.f * rgb.x - 1.f, 2.f * rgb.y - 1.f, 2.f * rgb.z - 1.f);
+    const float fres = (float) nm.res;
+    const V3 tmp = (v01 + v10) - v11;
+    du = (((v10 + v00 * (w1.y - 1.f)) - tmp * w1.y) * fres) * 2.f;
+    dv = (((v01 + v00 * (w1.x - 1.f)) - tmp * w1.x) * fres) * 2.f;
+}
+
 SMS_DEV V2 tile_uv(V2 uv, float tiles) {
     uv = uv * tiles;
     return v2(uv.x - floorf(uv.x), uv.y - floorf(uv.y));
@@ -535,19 +567,23 @@ SMS_DEV MV mv_from_si(const DScene &S, const SI &si, bool smooth = false) {
         v.dn_du = sdn_du; v.dn_dv = sdn_dv;
     } else {
         const DNormalmap nm = S.nmaps[sh.normalmap];
-        V2 sl, tu, tv;
+        V3 nl, dl_du, dl_dv;
         if (smooth) {
-            const float *top = nm.lean + nm.lean_off[nm.levels - 1];
-            sl = v2(top[0], top[1]);
-            tu = v2(0.f, 0.f); tv = v2(0.f, 0.f);
+            // the coarsest level is one texel: bilinear weights (0, 0), derivatives zero (their
+            // signs as the slope and RGB branches of eval_normal_derivatives form them)
+            if (nm.encoded) {
+                nl = v3(2.f * nm.top_rgb[0] - 1.f, 2.f * nm.top_rgb[1] - 1.f, 2.f * nm.top_rgb[2] - 1.f);
+                dl_du = v3(0.f, 0.f, 0.f);
+            } else {
+                const float *top = nm.lean + nm.lean_off[nm.levels - 1];
+                nl = v3(-top[0], -top[1], 1.f);
+                dl_du = v3(-0.f, -0.f, 0.f);
+            }
+            dl_dv = dl_du;
         } else {
-            V2 uv = tile_uv(si.uv, sh.tiles);
-            Texel4 tx = nm_fetch(nm.slope, nm.res, nm.stride, uv);
-            sl = nm_slope(tx);
-            nm_slope_deriv(tx, (float) nm.res, tu, tv);
+            nm_local_normal(nm, tile_uv(si.uv, sh.tiles), nl, dl_du, dl_dv);
         }
-        V3 nl = v3(-sl.x, -sl.y, 1.f);
-        V3 dn_du = v3(-tu.x, -tu.y, 0.f) * sh.tiles, dn_dv = v3(-tv.x, -tv.y, 0.f) * sh.tiles;
+        V3 dn_du = dl_du * sh.tiles, dn_dv = dl_dv * sh.tiles;
         V3 rn = normalize(to_world(bs, bt, bn, nl));
         v.n = rn;
         v.s = normalize(si.dp_du - rn * dot(rn, si.dp_du));

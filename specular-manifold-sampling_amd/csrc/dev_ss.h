@@ -197,6 +197,9 @@ SMS_DEV void lean_coarsest(const DScene &S, const SI &si, V2 &mu, float sig[3]) 
     V2 uv = si.uv * sh.tiles;
     uv = v2(uv.x - floorf(uv.x), uv.y - floorf(uv.y));
     lean_moments(nm, uv, nm.levels - 1, mu, sig);
+    // without use_slopes, NormalmapBSDF::lean takes the mean from the interpolated RGB as
+    // stored, n = rgb (not decoded): mu = (-r / b, -g / b)
+    if (nm.encoded) mu = v2(-nm.top_rgb[0] / nm.top_rgb[2], -nm.top_rgb[1] / nm.top_rgb[2]);
 }
 
 // newton_solver SS (manifold_ss.cpp:329-405) from vtx; smooth: vertices at smoothing 1

@@ -200,7 +200,7 @@ void camera_setup(DCamera &c, const sms_camera_desc &d) {
 }
 
 // Host-side form of a scene: everything sms_ctx_create uploads, built without a device.
-struct HostNormalmap { DNormalmap m; std::vector<float2> slope; std::vector<float> lean; };
+struct HostNormalmap { DNormalmap m; std::vector<float2> slope; std::vector<float> lean; std::vector<float4> rgbn; };
 struct HostScene {
     std::vector<HostNormalmap> nmaps;
     std::vector<DEmitter> emitters;
@@ -223,7 +223,8 @@ struct HostScene {
     uint64_t device_bytes() const {
         uint64_t b = 0;
         for (const HostNormalmap &m : nmaps)
-            b += m.slope.size() * sizeof(float2) + m.lean.size() * sizeof(float) + sizeof(DNormalmap);
+            b += m.slope.size() * sizeof(float2) + m.lean.size() * sizeof(float) + m.rgbn.size() * sizeof(float4) +
+                 sizeof(DNormalmap);
         b += emitters.size() * sizeof(DEmitter) + (em_ss.size() + em_ms.size() + em_all.size()) * sizeof(int);
         b += shapes.size() * sizeof(DShape) + (analytic.size() + casters.size() + casters_ms.size()) * sizeof(int);
         b += (vpos.size() + vnrm.size()) * sizeof(float4) + vuv.size() * sizeof(float2) + faces.size() * sizeof(uint4);
@@ -245,9 +246,11 @@ sms_status prepare_scene(const sms_scene_desc *desc, HostScene &H, std::string &
         const sms_normalmap_desc &nd = desc->normalmaps[i];
         if (nd.res <= 0 || (nd.res & (nd.res - 1)) != 0 || !(nd.rgb || nd.rgb32f))
             return bad(SMS_E_INVALID, "normal map must be square power-of-two");
+        if (nd.encoded_normals != 0 && nd.encoded_normals != 1) return bad(SMS_E_INVALID, "encoded_normals must be 0 or 1");
         const size_t res = (size_t) nd.res, stride = res + 1;
         // padded layout: column res and row res replicate column/row res-1 (edge clamp)
         std::vector<float2> sl(stride * stride);
+        std::vector<float4> rgbn(nd.encoded_normals ? stride * stride : 0);
         for (size_t y = 0; y < stride; ++y) {
             for (size_t x = 0; x < stride; ++x) {
                 size_t s = std::min(y, res - 1) * res + std::min(x, res - 1);
@@ -257,6 +260,7 @@ sms_status prepare_scene(const sms_scene_desc *desc, HostScene &H, std::string &
                 } else {
                     r = nd.rgb32f[3 * s]; g = nd.rgb32f[3 * s + 1]; b = nd.rgb32f[3 * s + 2];
                 }
+                if (nd.encoded_normals) rgbn[y * stride + x] = make_float4(r, g, b, 0.f);
                 float vx = 2.f * r - 1.f, vy = 2.f * g - 1.f, vz = 2.f * b - 1.f;
                 float inv = 1.f / std::sqrt((vx * vx + vy * vy) + vz * vz);
                 float nx = vx * inv, ny = vy * inv, nz = vz * inv;
@@ -300,7 +304,31 @@ sms_status prepare_scene(const sms_scene_desc *desc, HostScene &H, std::string &
             m.levels++;
             off = next;
         }
-        H.nmaps.push_back(HostNormalmap{m, std::move(sl), std::move(lean)});
+        if (nd.encoded_normals) {
+            // the RGB pyramid's coarsest texel, by the same two-tap resample (normalmap.cpp:70-87)
+            m.encoded = 1;
+            std::vector<float> cur(res * res * 3), tmp;
+            for (size_t y = 0; y < res; ++y)
+                for (size_t x = 0; x < res; ++x) {
+                    const float4 v = rgbn[y * stride + x];
+                    cur[(y * res + x) * 3] = v.x; cur[(y * res + x) * 3 + 1] = v.y; cur[(y * res + x) * 3 + 2] = v.z;
+                }
+            for (size_t r = res; r > 1; r /= 2) {
+                const size_t h = r / 2;
+                tmp.assign(h * r * 3, 0.f);
+                for (size_t y = 0; y < r; ++y)
+                    for (size_t x = 0; x < h; ++x)
+                        for (int c = 0; c < 3; ++c)
+                            tmp[(y * h + x) * 3 + c] = cur[(y * r + 2 * x) * 3 + c] * 0.5f + cur[(y * r + 2 * x + 1) * 3 + c] * 0.5f;
+                cur.assign(h * h * 3, 0.f);
+                for (size_t y = 0; y < h; ++y)
+                    for (size_t x = 0; x < h; ++x)
+                        for (int c = 0; c < 3; ++c)
+                            cur[(y * h + x) * 3 + c] = tmp[((2 * y) * h + x) * 3 + c] * 0.5f + tmp[((2 * y + 1) * h + x) * 3 + c] * 0.5f;
+            }
+            m.top_rgb[0] = cur[0]; m.top_rgb[1] = cur[1]; m.top_rgb[2] = cur[2];
+        }
+        H.nmaps.push_back(HostNormalmap{m, std::move(sl), std::move(lean), std::move(rgbn)});
     }
     std::vector<DEmitter> &emitters = H.emitters;
     std::vector<int> &em_ss = H.em_ss, &em_ms = H.em_ms, &em_all = H.em_all;
@@ -355,6 +383,8 @@ sms_status prepare_scene(const sms_scene_desc *desc, HostScene &H, std::string &
             return bad(SMS_E_INVALID, "glinty shapes need the normal-mapped BSDF");
         if (d.bsdf == SMS_BSDF_NORMALMAP && (d.normalmap < 0 || d.normalmap >= desc->n_normalmaps))
             return bad(SMS_E_INVALID, "normal-mapped shape without a normal map");
+        if ((d.flags & SMS_FLAG_GLINTY) && desc->normalmaps[d.normalmap].encoded_normals)
+            return bad(SMS_E_UNSUPPORTED, "glinty shapes need a slope normal map (encoded_normals 0)");
         sh.bbox_center[0] = sh.cx; sh.bbox_center[1] = sh.cy; sh.bbox_center[2] = sh.cz;
         if (d.type == SMS_SHAPE_SPHERE) {
             sh.inv_surface_area = 1.f / (4.f * PI_F * sh.radius * sh.radius);
@@ -490,6 +520,7 @@ sms_status upload_scene(sms_ctx *ctx, HostScene &H, const sms_config *cfg) {
         DNormalmap m = hm.m;
         if ((st = upload(ctx, hm.slope, &m.slope))) return st;
         if ((st = upload(ctx, hm.lean, &m.lean))) return st;
+        if (!hm.rgbn.empty() && (st = upload(ctx, hm.rgbn, &m.rgbn))) return st;
         nmaps.push_back(m);
     }
     ctx->area_emitters = H.area_emitters;

@@ -642,15 +642,10 @@ private:
             if (d.roughness > 0.f) throw LoadError(SMS_E_UNSUPPORTED, where_obj(o) + "normal maps over rough bsdfs are not supported");
             if (d.bsdf == SMS_BSDF_DIELECTRIC) d.k = 0.f;
             d.bsdf = SMS_BSDF_NORMALMAP;
-            d.normalmap = normalmap(o, resolve(s(o, "filename", "")), out);
+            // use_slopes false (the default) interpolates the encoded normals (normalmap.cpp:226-258)
+            const bool encoded = !b(o, "use_slopes", false);
+            d.normalmap = normalmap(o, resolve(s(o, "filename", "")), encoded, out);
             d.tiles = (float) f(o, "tiles", 1.0);
-            // the solver's frames come from the map's slopes (normalmap.h:62-104 with use_slopes);
-            // the reference's default interpolates the encoded normals instead
-            if (!b(o, "use_slopes", false)) {
-                if (!substitute())
-                    throw LoadError(SMS_E_UNSUPPORTED, where_obj(o) + "normal maps without use_slopes are not supported");
-                note(where_obj(o) + "normal map read with use_slopes (the file leaves it false)");
-            }
             (void) b(o, "lean_fallback", false);
             (void) f(o, "lean_fallback_alpha", 0.0);
         } else {
@@ -685,9 +680,10 @@ private:
         throw LoadError(SMS_E_UNSUPPORTED, where_obj(o) + "index of refraction \"" + k + "\" not understood");
     }
 
-    // normal map image -> float RGB in the description (one entry per file)
-    int normalmap(Obj &o, const std::string &file, SceneFile &out) {
-        auto it = nm_index_.find(file);
+    // normal map image -> float RGB in the description (one entry per file and interpolation)
+    int normalmap(Obj &o, const std::string &file, bool encoded, SceneFile &out) {
+        const std::string key = (encoded ? "encoded:" : "slopes:") + file;
+        auto it = nm_index_.find(key);
         if (it != nm_index_.end()) return it->second;
         Image img;
         std::string err;
@@ -700,10 +696,11 @@ private:
         std::memset(&nd, 0, sizeof(nd));
         nd.res = img.width;
         nd.rgb32f = px->data();
+        nd.encoded_normals = encoded ? 1 : 0;
         out.nm_pixels.push_back(std::move(px));
         out.normalmaps.push_back(nd);
         const int idx = (int) out.normalmaps.size() - 1;
-        nm_index_[file] = idx;
+        nm_index_[key] = idx;
         return idx;
     }
 

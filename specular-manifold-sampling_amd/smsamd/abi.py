@@ -58,7 +58,8 @@ class SmsEmitterDesc(C.Structure):
 
 
 class SmsNormalmapDesc(C.Structure):
-    _fields_ = [("res", C.c_int32), ("rgb", C.POINTER(C.c_uint8)), ("rgb32f", C.POINTER(C.c_float))]
+    _fields_ = [("res", C.c_int32), ("rgb", C.POINTER(C.c_uint8)), ("rgb32f", C.POINTER(C.c_float)),
+                ("encoded_normals", C.c_int32)]
 
 
 class SmsCameraDesc(C.Structure):

@@ -109,6 +109,8 @@ def load_scene(path, defines=None, substitute=False):
             else:
                 rgb = _arr(nm.rgb32f, r * r * 3, np.float32)
             sc.normalmaps.append((r, rgb.reshape(r, r, 3)))
+            if nm.encoded_normals:
+                sc.encoded_normalmaps.add(i)
         c = d.camera
         sc.set_camera(tuple(c.origin), tuple(c.target), tuple(c.up), c.fov_deg, c.width, c.height)
         info = abi.SmsSceneInfo()

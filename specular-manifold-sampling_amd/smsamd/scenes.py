@@ -28,7 +28,8 @@ class Scene:
     def __init__(self):
         self.shapes = []        # dicts
         self.emitters = []      # dicts
-        self.normalmaps = []    # (res, uint8 array)
+        self.normalmaps = []    # (res, uint8 or float32 array)
+        self.encoded_normalmaps = set()   # indices of maps whose frames decode interpolated RGB
         self.camera = None
         self._keep = []
 
@@ -77,12 +78,15 @@ class Scene:
                                   caustic_ss=caustic_ss, caustic_ms=caustic_ms))
         return len(self.emitters) - 1
 
-    def add_normalmap(self, rgb):
+    def add_normalmap(self, rgb, encoded=False):
         """RGB normal map: uint8 (code values / 255) or float32 (values as an OpenEXR file
-        stores them), square power-of-two."""
+        stores them), square power-of-two.  encoded: frames interpolate the RGB and decode
+        2 rgb - 1 (NormalmapBSDF without use_slopes) instead of interpolating slopes."""
         rgb = np.ascontiguousarray(rgb, np.float32 if np.asarray(rgb).dtype.kind == "f" else np.uint8)
         assert rgb.ndim == 3 and rgb.shape[0] == rgb.shape[1] and rgb.shape[2] == 3
         self.normalmaps.append((rgb.shape[0], rgb))
+        if encoded:
+            self.encoded_normalmaps.add(len(self.normalmaps) - 1)
         return len(self.normalmaps) - 1
 
     def set_camera(self, origin, target, up, fov_deg, width, height):
@@ -124,6 +128,7 @@ class Scene:
         nms = (abi.SmsNormalmapDesc * max(1, len(self.normalmaps)))()
         for i, (res, rgb) in enumerate(self.normalmaps):
             nms[i].res = res
+            nms[i].encoded_normals = int(i in self.encoded_normalmaps)
             if rgb.dtype == np.float32:
                 nms[i].rgb32f = rgb.ctypes.data_as(C.POINTER(C.c_float))
             else:

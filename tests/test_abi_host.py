@@ -49,6 +49,22 @@ def test_footprint_of_a_normal_map(lib):
     assert slopes + lean < b < slopes + lean + 4096
 
 
+def test_footprint_of_an_encoded_normal_map(lib):
+    """A map whose frames decode interpolated RGB (use_slopes false) keeps its RGB texels in the
+    padded layout beside the slopes; glinty shapes need slopes and refuse it."""
+    sc = scenes.mirror_wall(nm_res=64, n_scratches=4)
+    st, b0, err = _footprint(lib, sc)
+    assert st == 0, err
+    sc.encoded_normalmaps.add(0)
+    st, b1, err = _footprint(lib, sc)
+    assert st == 0, err
+    assert b1 - b0 == 65 * 65 * 16
+    g = scenes.glint_plane(16, 16, nm_res=64, n_scratches=4)
+    g.encoded_normalmaps.add(0)
+    st, _, err = _footprint(lib, g)
+    assert st == -5 and "slope normal map" in err
+
+
 def test_bad_scene_is_rejected_before_any_device_call(lib):
     faces = np.array([[0, 1, 7]], np.uint32)                       # vertex 7 does not exist
     sc = scenes.Scene()

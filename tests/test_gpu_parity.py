@@ -139,11 +139,14 @@ def test_glint_area_light_estimate(gpu, oracle_mod, biased):
     orc.close()
 
 
-@pytest.mark.parametrize("biased", [0, 1])
-def test_ss_twostage(gpu, oracle_mod, biased):
-    """Two-stage solver (manifold_ss.cpp:251-270) on a normal-mapped mirror wall: walks with
-    explicit LEAN samples and whole estimates (the lane-per-query pass) equal the oracle."""
+@pytest.mark.parametrize("biased,encoded", [(0, False), (1, False), (0, True)])
+def test_ss_twostage(gpu, oracle_mod, biased, encoded):
+    """Two-stage solver (manifold_ss.cpp:251-270) on a normal-mapped mirror wall (frames from
+    slopes, or from decoded RGB with the LEAN mean of the stored RGB): walks with explicit LEAN
+    samples and whole estimates (the lane-per-query pass) equal the oracle."""
     sc = scenes.mirror_wall(nm_res=64, n_scratches=16, seed=4)
+    if encoded:
+        sc.encoded_normalmaps.add(0)
     cfg = abi.SmsConfig.default(max_trials=32 if biased else 100000, biased=biased, twostage=1)
     ctx = gpu.Context(sc, cfg)
     orc = oracle_mod.Oracle(sc, cfg)

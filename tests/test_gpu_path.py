@@ -74,7 +74,8 @@ def test_path_samples_pool_mnee_and_batches(gpu, oracle_mod):
 
 def test_path_samples_normal_mapped(gpu, oracle_mod):
     """Normal-mapped casters in both single-scatter path tracers: a glass dome with a wavy map
-    (nested dielectric) and a mirror with a scratch map (nested conductor)."""
+    (nested dielectric; frames from slopes and from decoded RGB) and a mirror with a scratch map
+    (nested conductor)."""
     u = np.linspace(0, 2 * np.pi, 64, endpoint=False)
     sx, sy = 0.12 * np.sin(2 * u)[None, :] * np.ones((64, 1)), 0.12 * np.cos(3 * u)[:, None] * np.ones((1, 64))
     n = np.stack([-sx, -sy, np.ones_like(sx)], -1)
@@ -85,7 +86,10 @@ def test_path_samples_normal_mapped(gpu, oracle_mod):
     wall = scenes.mirror_wall(nm_res=64, n_scratches=16)
     wall.shapes[2]["radius"] = 0.3
     wall.emitters[0]["radiance"] = (5.0, 5.0, 5.0)
-    for sc in (dome, wall):
+    encoded = scenes.sphere_dome(24, 20, light_radius=0.5, light_radiance=4.0)
+    encoded.shapes[0].update(bsdf=abi.BSDF_NORMALMAP, normalmap=encoded.add_normalmap((0.5 * n + 0.5).astype(np.float32), encoded=True),
+                             tiles=2.0, k=0.0, eta=1.5)
+    for sc in (dome, wall, encoded):
         cfg = abi.SmsConfig.default(max_trials=100000)
         ctx = gpu.Context(sc, cfg)
         orc = oracle_mod.Oracle(sc, cfg)

@@ -123,9 +123,11 @@ def test_scene_file_renders_match_oracle(gpu, oracle_mod, tmp_path):
     orc.close()
 
 
-def test_exr_normal_mapped_caster_estimates_match_oracle(gpu, oracle_mod, tmp_path):
+@pytest.mark.parametrize("encoded", [False, True])
+def test_exr_normal_mapped_caster_estimates_match_oracle(gpu, oracle_mod, tmp_path, encoded):
     """A normal-mapped mirror whose map is read from an OpenEXR file (values as stored, no 8-bit
-    quantisation): single-scatter estimates and two-stage walks equal to the oracle's."""
+    quantisation), its frames from interpolated slopes or from decoded interpolated RGB:
+    single-scatter estimates and two-stage walks equal to the oracle's."""
     rng = np.random.default_rng(4)
     res = 64
     u = np.linspace(0, 4 * np.pi, res)
@@ -137,6 +139,8 @@ def test_exr_normal_mapped_caster_estimates_match_oracle(gpu, oracle_mod, tmp_pa
     nm = files.load_image(tmp_path / "nm.exr")
     sc = scenes.mirror_wall(normalmap=np.zeros((res, res, 3), np.uint8))
     sc.normalmaps[0] = (res, nm)
+    if encoded:
+        sc.encoded_normalmaps.add(0)
     cfg = abi.SmsConfig.default(max_trials=10000)
     q = None
     for twostage in (0, 1):

@@ -497,6 +497,20 @@ def test_scene_file_unsupported_and_substitutes(tmp_path):
         assert e.value.status == -1
 
 
+def test_scene_file_normal_map_interpolation(tmp_path):
+    """use_slopes (default false) picks the frames' interpolation: decoded RGB or slopes; one
+    map entry per file and interpolation."""
+    write_exr(tmp_path / "nm.exr", np.tile(np.float32([0.5, 0.6, 0.9]), (4, 4, 1)))
+    shape = ('<shape type="rectangle"><bsdf type="normalmap"><string name="filename" value="nm.exr"/>{}'
+             '<bsdf type="conductor"/></bsdf></shape>')
+    xml = ('<scene version="2.0.0"><sensor type="perspective"/>' + shape.format("") +
+           shape.format('<boolean name="use_slopes" value="true"/>') + shape.format("") + "</scene>")
+    (tmp_path / "s.xml").write_text(xml)
+    sc, info = files.load_scene(tmp_path / "s.xml")
+    assert [s["normalmap"] for s in sc.shapes] == [0, 1, 0]
+    assert sc.encoded_normalmaps == {0} and len(sc.normalmaps) == 2 and info["notes"] == []
+
+
 def test_scene_file_rotated_spheres(tmp_path):
     """A sphere's uv and tangents follow its object axes (sphere.cpp:343-365); the solver's
     spheres are axis-aligned, so a rotation is dropped with a note, and refused for a

@@ -183,12 +183,15 @@ def test_normal_mapped_mirror_sms_matches_filtered_path_tracer():
     b.close()
 
 
-def test_normal_mapped_glass_dome_sms_matches_filtered_path_tracer():
+@pytest.mark.parametrize("encoded", [False, True])
+def test_normal_mapped_glass_dome_sms_matches_filtered_path_tracer(encoded):
     """A glass dome whose normal map varies over the sphere's (phi / 2 pi, theta / pi) uv
     (sphere.cpp:343-348): SMS solves with the map's normals and its uv derivatives in the
     geometric term, the filtered path tracer refracts through the same normals; the caustic
-    terms agree.  A sphere whose uv were left constant would read one texel's normal in the path
-    tracer but the map's derivative in the geometric term (SMS 26 % low at this amplitude)."""
+    terms agree, for frames from interpolated slopes (use_slopes) and from decoded interpolated
+    RGB (the reference's default, normalmap.cpp:226-258).  A sphere whose uv were left constant
+    would read one texel's normal in the path tracer but the map's derivative in the geometric
+    term (SMS 26 % low at this amplitude)."""
     res, a, k = 256, 0.05, 8
     u = (np.arange(res) + 0.5) / res
     U, V = np.meshgrid(u, u, indexing="xy")
@@ -196,7 +199,8 @@ def test_normal_mapped_glass_dome_sms_matches_filtered_path_tracer():
     n = np.stack([-sx, -sy, np.ones_like(sx)], -1)
     n /= np.linalg.norm(n, axis=-1, keepdims=True)
     sc = scenes.sphere_dome(16, 16, light_radius=0.5, light_radiance=4.0)
-    sc.shapes[0].update(bsdf=abi.BSDF_NORMALMAP, k=0.0, normalmap=sc.add_normalmap((0.5 * n + 0.5).astype(np.float32)))
+    nm = sc.add_normalmap((0.5 * n + 0.5).astype(np.float32), encoded=encoded)
+    sc.shapes[0].update(bsdf=abi.BSDF_NORMALMAP, k=0.0, normalmap=nm)
     orc = oracle_lib.Oracle(sc, abi.SmsConfig.default(max_trials=100000), threads=8)
     ms, ss = _mean_se(_caustic(orc, abi.SmsPathConfig.default(max_depth=6, caustics_only=1), 16, seed0=100))
     flt = abi.SmsPathConfig.default(integrator=abi.PATH_FILTERED_SS, max_depth=6, caustics_only=1)
