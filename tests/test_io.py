@@ -593,3 +593,24 @@ def test_reference_scene_files_load(rel, integrator, casters, emitter):
     assert any(e["caustic_ss"] for e in sc.emitters) == emitter
     if not emitter:
         assert any("envmap" in n for n in info["notes"])
+
+
+@have_ref
+def test_main_comparison_sphere_scene_sms_matches_filtered_path_tracer():
+    """The paper's main-comparison sphere scene (a glass sphere with a wavy normal map read
+    without use_slopes, lit by a small rectangle light), its floor texture replaced by grey, at
+    32 x 18 pixels: the oracle's path_sms_ss caustic term agrees with path_filtered_ss's."""
+    import oracle_lib
+    sc, info = files.load_scene(os.path.join(REF, "Figure_14_15_MainComparison/sphere_sms.xml"), substitute=True)
+    assert sc.encoded_normalmaps == {0} and sc.shapes[2]["bsdf"] == abi.BSDF_NORMALMAP
+    sc.camera["width"], sc.camera["height"] = 32, 18
+    orc = oracle_lib.Oracle(sc, abi.SmsConfig.default(max_trials=100000), threads=8)
+
+    def caustic(pc, passes, spp):
+        v = np.concatenate([orc.path_samples(pc, spp, 100 + s)[0][:, 0] for s in range(passes)]).astype(np.float64)
+        return v.mean(), v.std() / math.sqrt(len(v))
+    ms, ss = caustic(abi.SmsPathConfig.default(max_depth=6, caustics_only=1), 32, 16)
+    mf, sf = caustic(abi.SmsPathConfig.default(integrator=abi.PATH_FILTERED_SS, max_depth=6, caustics_only=1), 2000, 64)
+    orc.close()
+    print(f"SMS {ms:.5g} +- {ss:.2g}  filtered {mf:.5g} +- {sf:.2g}")
+    assert ms > 0 and abs(ms - mf) < 4 * math.hypot(ss, sf)
