@@ -149,7 +149,10 @@ def parse():
                     help="samples per pixel of the charged frame's uniform first pass")
     ap.add_argument("--train-samples", type=int, default=2_000_000, help="records the online training uses")
     ap.add_argument("--train-epochs", type=int, default=8, help="Adam steps per 8192 captured records")
-    ap.add_argument("--defensive", type=float, default=0.2)
+    ap.add_argument("--defensive", type=float, default=0.1,
+                    help="pool: weight of the uniform seed in the anchored mixture")
+    ap.add_argument("--defensive-other", type=float, default=0.2,
+                    help="slab and glints: weight of the uniform seed in the mixture")
     ap.add_argument("--anchor-radius", type=float, default=0.125,
                     help="pool: half side of the anchored mixtures' uv box (0: mixtures over the whole uv square)")
     return ap.parse_args()
@@ -533,10 +536,12 @@ def equal_time_rmse(args, case="pool"):
     network); a ratio is sqrt(mean MSE uniform / mean MSE learned) over them, the per-frame
     ratios and their median are reported beside it -- the errors are heavy-tailed: a frame
     that draws one of the rare samples whose solution the mixture proposes with little mass
-    (at worst 0.2 x the uniform density, so up to 5x the uniform inverse-PDF) can hold most
+    (at worst the defensive weight x the uniform density: up to 10x the uniform inverse-PDF on
+    the pool, 5x on the slab and glints) can hold most
     of its squared error in a few pixels."""
     import numpy as np
     from smsamd import abi, gpu, render, scenes
+    defensive = args.defensive if case == "pool" else args.defensive_other
     res, pass_spp = args.rmse_res, {"pool": args.rmse_pass_spp, "slab": args.ms_pass_spp, "glint": args.spp}[case]
     if case == "glint":
         res = args.width
@@ -639,7 +644,7 @@ def equal_time_rmse(args, case="pool"):
         st_l = {}
         frame_stats_l.append(st_l)
         tr = render.render_online(ctx, runner, film, m, 0, T, samples=args.train_samples, epochs=args.train_epochs,
-                                  defensive=args.defensive, seed0=30_000_000 + 1000 * r, rng_seed=r, stats=st_l,
+                                  defensive=defensive, seed0=30_000_000 + 1000 * r, rng_seed=r, stats=st_l,
                                   uniform_runner=first_runner)
         err_l.append(errors(film.download()[0]))
         spp_l.append(film.spp)
@@ -654,13 +659,13 @@ def equal_time_rmse(args, case="pool"):
         # count for the amortized frames
         film.clear()
         tr2 = render.render_online(ctx, runner, film, mlp, 0, float("inf"), samples=args.train_samples - tr0["used"],
-                                   epochs=args.train_epochs, defensive=args.defensive, seed0=35_000_000,
+                                   epochs=args.train_epochs, defensive=defensive, seed0=35_000_000,
                                    uniform_passes=0, rng_seed=99, stop_when_trained=True)
         steps += tr2["steps"]
         losses += tr2["losses"]
         train_s += tr2["train_s"]
     # learned, network already trained
-    ctx.set_proposal(mlp, 0, args.defensive)
+    ctx.set_proposal(mlp, 0, defensive)
     err_a, spp_a, st_a, max_trials_a = [], [], {}, []
     for r in range(reps):
         film.clear()
@@ -734,7 +739,7 @@ def equal_time_rmse(args, case="pool"):
         "uniform_frame": {"walks_per_s": st_u.get("walks", 0) / T, "converged_walks_per_s": st_u.get("walks_converged", 0) / T,
                           "newton_iterations_per_s": st_u.get("newton_iterations", 0) / T,
                           "note": "whole frames: camera passes, estimates and film"},
-        "defensive": args.defensive, "schedule": sched,
+        "defensive": defensive, "schedule": sched,
     }
     print(f"# rmse {case}: {json.dumps(out)}", file=sys.stderr, flush=True)
     mlp.close()
