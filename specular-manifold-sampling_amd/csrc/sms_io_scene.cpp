@@ -774,6 +774,8 @@ private:
                 have_bsdf = true;
             }
         }
+        if ((d.flags & SMS_FLAG_GLINTY) && d.bsdf == SMS_BSDF_NORMALMAP && out.normalmaps[d.normalmap].encoded_normals)
+            throw LoadError(SMS_E_UNSUPPORTED, where_obj(o) + "glinty shapes need a normal map with use_slopes");
         if (sphere_rotated) {
             if (d.bsdf == SMS_BSDF_NORMALMAP)
                 throw LoadError(SMS_E_UNSUPPORTED, where_obj(o) + "rotated normal-mapped spheres are not supported");

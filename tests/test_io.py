@@ -509,6 +509,11 @@ def test_scene_file_normal_map_interpolation(tmp_path):
     sc, info = files.load_scene(tmp_path / "s.xml")
     assert [s["normalmap"] for s in sc.shapes] == [0, 1, 0]
     assert sc.encoded_normalmaps == {0} and len(sc.normalmaps) == 2 and info["notes"] == []
+    # the glint kernels read slopes
+    (tmp_path / "g.xml").write_text(xml.replace("<shape type=\"rectangle\">", "<shape type=\"rectangle\"><boolean name=\"glinty\" value=\"true\"/>", 1))
+    with pytest.raises(files.LoadError) as e:
+        files.load_scene(tmp_path / "g.xml")
+    assert e.value.status == -5 and "use_slopes" in str(e.value)
 
 
 def test_scene_file_rotated_spheres(tmp_path):
