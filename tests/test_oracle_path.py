@@ -210,6 +210,27 @@ def test_normal_mapped_glass_dome_sms_matches_filtered_path_tracer(encoded):
     orc.close()
 
 
+def test_two_stage_on_an_encoded_map_is_unbiased():
+    """Two-stage walks on a map without use_slopes start from the map's coarsest level, with the
+    LEAN mean of the stored (undecoded) RGB (bsdfs/normalmap.cpp:292-303): only the seeds
+    change, so the unbiased SMS caustic with and without two-stage walks agree on a scratched
+    mirror."""
+    sc = scenes.mirror_wall(nm_res=64, n_scratches=16, seed=4)
+    sc.encoded_normalmaps.add(0)
+    sc.shapes[2]["radius"] = 0.3
+    sc.emitters[0]["radiance"] = (5.0, 5.0, 5.0)
+    sc.set_camera((-1.2, 2.2, 0.0), (0.2, 0.0, 0.0), (0, 1, 0), 50.0, 16, 16)
+    sms = abi.SmsPathConfig.default(max_depth=6, caustics_only=1)
+    est = []
+    for twostage in (0, 1):
+        orc = oracle_lib.Oracle(sc, abi.SmsConfig.default(max_trials=1000000, twostage=twostage), threads=8)
+        est.append(_mean_se(_caustic(orc, sms, 8, seed0=100 + 50 * twostage)))
+        orc.close()
+    print("single-stage, two-stage:", est)
+    (m0, s0), (m1, s1) = est
+    assert m0 > 0 and abs(m0 - m1) < 4 * math.hypot(s0, s1)
+
+
 # ---------------------------------------------------------------- multi-scatter integrators
 @pytest.fixture(scope="module")
 def slab():
